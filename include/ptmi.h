@@ -1,0 +1,218 @@
+/*
+ * ptmi.h — C ABI of the MI355X transport-loop backend (libptmi.so).
+ *
+ * This is the drop-in for the reference's executor boundary: the Engine loads a
+ * CompiledRenderer, registers scene data, renders frames (1 sample per pixel each) and
+ * reads exports back (src/engine/Engine.ts:83-474, src/engine/RenderExecutor.ts:159-258).
+ * The GLSL program the reference generator would emit is replaced by a pt_program record
+ * (the Plan stage's decisions as plain data) that selects template instantiations of
+ * the HIP kernels; scene data arrives as plain arrays instead of data textures.
+ *
+ * Conventions: every entry point returns pt_status; on failure pt_last_error() holds a
+ * message (thread-local). Host arrays are caller-owned and copied during the call;
+ * device memory is owned by the library. A pt_ctx owns one HIP stream and must be used
+ * from one host thread at a time. No torch types cross this boundary.
+ */
+#ifndef PTMI_H
+#define PTMI_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PT_ABI_VERSION 1u
+
+typedef enum pt_status {
+    PT_OK = 0,
+    PT_E_INVALID = 1,        /* bad argument / inconsistent description */
+    PT_E_MISSING_INPUT = 2,  /* a required scene channel or table was not uploaded */
+    PT_E_DEVICE = 3,         /* HIP runtime error */
+    PT_E_OOM = 4,            /* device allocation failed */
+    PT_E_UNSUPPORTED = 5     /* program asks for a feature this build does not provide */
+} pt_status;
+
+/* ---- scene description (the data rail: src/components/data/channels.ts:11-20) ---- */
+
+enum { PT_OBJ_MESH = 0, PT_OBJ_QUAD = 1, PT_OBJ_SPHERE = 2 };
+
+/* One scene object = one region (src/compiler/plan: objects are indexed by region id).
+ * Meshes are triangle soups in WORLD space (identity placement) over a range of the
+ * triangle arrays. closed != 0 claims an interior (watertight, consistent outward
+ * winding; src/compiler/generate/features/intersection.ts:1196-1211); quads are thin. */
+typedef struct pt_object {
+    int32_t kind;
+    int32_t material;
+    int32_t closed;
+    int32_t first_tri;       /* meshes */
+    int32_t tri_count;       /* meshes */
+    int32_t pad0, pad1, pad2;
+    float   p[12];           /* quad: corner, edge1, edge2, normal; sphere: center, radius */
+    float   box_min[4];      /* meshes: vertex AABB (containment quick reject) */
+    float   box_max[4];
+} pt_object;
+
+enum { PT_MAT_LAMBERT = 0, PT_MAT_DIELECTRIC = 1, PT_MAT_GGX = 2 };
+
+/* Material record: the MaterialProperties the generator bakes per material id
+ * (src/compiler/generate/features/materials.ts, components/materials/{lambert,dielectric,ggx}).
+ * Interior medium (homogeneous) when has_medium != 0. */
+typedef struct pt_material {
+    int32_t model;
+    int32_t emissive;        /* material_is_emissive */
+    int32_t has_medium;
+    int32_t pad0;
+    float   albedo[4];       /* lambert */
+    float   emission[4];
+    float   f0[4];           /* ggx */
+    float   transmittance[4];/* dielectric */
+    float   alpha;           /* ggx: max(1e-3, roughness^2), host-derived */
+    float   ior;             /* dielectric */
+    float   hg_g;            /* medium */
+    float   pad1;
+    float   sigma_s[4];      /* medium */
+    float   sigma_a[4];      /* medium */
+} pt_material;
+
+enum { PT_LIGHT_QUAD = 0, PT_LIGHT_POINT = 1 };
+
+/* Finite light with its host-computed selection pdf and CDF bound
+ * (src/compiler/generate/features/lighting.ts:537-620). */
+typedef struct pt_light {
+    int32_t kind;
+    int32_t region;          /* object index of a hittable light (quad), -1 for delta */
+    float   select_pdf;
+    float   cdf;             /* upper CDF bound of this light in the selection chain */
+    float   corner[4], edge1[4], edge2[4], normal[4];
+    float   radiance[4];     /* quad: Le; point: intensity */
+    float   area;
+    float   pad[3];
+} pt_light;
+
+enum { PT_ENV_NONE = 0, PT_ENV_SKY = 1 };
+
+/* Environment. PT_ENV_SKY is the procedural sky of the reference's proc-sky scenes
+ * (tests/compiler/__snapshots__/generated-glsl.snapshot.test.ts.snap:63758), sampled
+ * through an equirect importance table the library bakes itself
+ * (src/compiler/EnvironmentBake.ts:29, src/components/env/importance.ts:16). */
+typedef struct pt_env {
+    int32_t type;
+    int32_t table_w, table_h;   /* 512 x 256 default */
+    int32_t compensated;        /* MIS compensation of the table (0 = plain) */
+    float   intensity;
+    float   rotation;           /* u_envRotation (radians) */
+    float   select_prob;        /* two-stage env-vs-finite draw; ignored without finite lights */
+    float   pad;
+} pt_env;
+
+typedef struct pt_scene_desc {
+    uint32_t abi_version;
+    int32_t  n_tris;
+    const float*   tri_vertices;   /* [n_tris][9]: a, b, c (world space) */
+    const int32_t* tri_object;     /* [n_tris]: owning object (region) index */
+    int32_t  n_objects;
+    const pt_object* objects;
+    int32_t  n_materials;
+    const pt_material* materials;
+    int32_t  n_lights;
+    const pt_light* lights;
+    pt_env   env;
+} pt_scene_desc;
+
+/* ---- program (the Plan stage's decisions) ---- */
+
+enum { PT_EST_PT = 0, PT_EST_NEE = 1, PT_EST_MIS = 2 };
+enum { PT_COLOR_RGB = 0 };
+
+typedef struct pt_program {
+    uint32_t abi_version;
+    int32_t  estimator;       /* directLighting: none / nee / mis */
+    int32_t  max_bounces;     /* measurement.maxBounces */
+    int32_t  rr_start;        /* russianRoulette.startDepth, < 0 = off */
+    float    rr_max_survival; /* RR_MAX_SURVIVAL (0.95 default) */
+    int32_t  color;
+    int32_t  width, height;   /* film size */
+    int32_t  frames_in_flight;/* samples per pixel traced per wave of paths (>= 1) */
+    int32_t  pad[3];
+} pt_program;
+
+/* Live per-frame values (the engine builtins + camera uniforms,
+ * src/engine/Engine.ts:231-254, src/components/camera/basis.ts:50-56). */
+typedef struct pt_params {
+    float    cam_position[4];
+    float    cam_forward[4];
+    float    cam_right[4];
+    float    cam_up[4];
+    float    tan_fov;         /* tan(fov/2), host f64 -> f32 */
+    float    aspect;          /* width / height */
+    uint32_t reset_salt;
+    uint32_t pad0;
+    int32_t  pixel_offset[2];
+    int32_t  pad1[2];
+} pt_params;
+
+/* ---- rays for the traversal entry points (SoA) ---- */
+typedef struct pt_rays_soa {
+    const float* ox; const float* oy; const float* oz;
+    const float* dx; const float* dy; const float* dz;
+    const float* tmax;        /* closest: initial hit.t (MAX_DIST); any: maxDist */
+} pt_rays_soa;
+
+typedef struct pt_ctx pt_ctx;
+typedef struct pt_renderer pt_renderer;
+
+/* Thread-local message describing the last failure on this thread. */
+const char* pt_last_error(void);
+
+/* Engine constructor/destructor (src/engine/Engine.ts:83-152). */
+pt_status pt_ctx_create(int device_index, pt_ctx** out);
+void      pt_ctx_destroy(pt_ctx* ctx);
+
+/* Upload and build the scene: triangle wide BVH, containment structures, env table.
+ * Replaces registerDataTexture[U32]/registerEnvironmentTable (Engine.ts:438-460) plus
+ * the App-side packing (app/App.ts:301-345). */
+pt_status pt_scene_upload(pt_ctx* ctx, const pt_scene_desc* scene);
+
+/* Engine.loadRenderer (Engine.ts:153-164): select the kernel instantiation for program. */
+pt_status pt_renderer_load(pt_ctx* ctx, const pt_program* program, pt_renderer** out);
+void      pt_renderer_unload(pt_renderer* r);
+
+/* Engine.renderFrame x nframes (Engine.ts:190-204): frame k traces 1 spp per pixel with
+ * sampleCount = current count + k and averages into the film. Asynchronous. */
+pt_status pt_render_frames(pt_renderer* r, const pt_params* params, uint32_t nframes);
+
+/* Engine.readExport('radiance') (Engine.ts:351-398): RGBA32F film, row 0 = bottom
+ * (gl_FragCoord convention), width*height*4 floats. Synchronises the stream. */
+pt_status pt_read_export(pt_renderer* r, const char* name, float* out, size_t out_floats);
+
+/* Engine.clearAccumulation / pinResetSalt (Engine.ts:400-436). */
+pt_status pt_clear_accumulation(pt_renderer* r);
+uint32_t  pt_sample_count(const pt_renderer* r);
+
+/* Parity entry points: closest-hit (t, global primitive id; -1 = miss) and any-hit over
+ * the uploaded scene, for n rays in host memory. Primitive ids: triangles 0..n_tris-1,
+ * analytic object j at n_tris + j. Equal-t ties resolve to the smaller id. */
+pt_status pt_trace_closest(pt_ctx* ctx, const pt_rays_soa* rays, uint32_t n, float* t_out, int32_t* prim_out);
+pt_status pt_trace_any(pt_ctx* ctx, const pt_rays_soa* rays, uint32_t n, uint8_t* hit_out);
+
+/* Device-resident variants for timing: pointers are device memory already holding the
+ * ray SoA (7 arrays of n floats, packed back to back) and receiving the results. */
+pt_status pt_trace_closest_device(pt_ctx* ctx, const float* d_rays, uint32_t n, float* d_t, int32_t* d_prim);
+pt_status pt_trace_any_device(pt_ctx* ctx, const float* d_rays, uint32_t n, uint8_t* d_hit);
+
+/* Kernel timing of the last pt_render_frames / trace call, measured with HIP events on
+ * the context stream: fills up to n_slots (name, total ms, launches) entries. */
+typedef struct pt_kernel_timing { char name[32]; float total_ms; uint32_t launches; uint64_t items; } pt_kernel_timing;
+pt_status pt_set_timing(pt_ctx* ctx, int enabled);
+int       pt_get_timings(pt_ctx* ctx, pt_kernel_timing* out, int n_slots);
+
+/* Statistics of the last render: rays traced by kind (closest, any). */
+pt_status pt_last_ray_counts(pt_renderer* r, uint64_t* closest, uint64_t* any);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PTMI_H */

@@ -1,0 +1,1266 @@
+/*
+ * pt_oracle.c — CPU restatement of the reference's generated transport loop.
+ *
+ * TEST INFRASTRUCTURE ONLY. This file is the checker: only tests/, the smoke check and the
+ * CPU-baseline leg of bench.py load it (liboracle.so via ctypes). The product (libptmi.so)
+ * never links, calls or includes it.
+ *
+ * It follows the GLSL the reference emits for a mesh scene + quad/env lights + pt/pt-nee/
+ * pt-mis, statement by statement, in IEEE fp32 with no FMA contraction and with the
+ * transcendentals of include/ptmi_math.h (the declared arithmetic). Each function cites the
+ * reference source it restates. Declared deviations (also listed in the header of each):
+ *   - closest-hit ties at bit-equal t resolve to the smaller global primitive id (the
+ *     reference keeps the first one its BVH order visits, intersection/mesh/mesh.glsl:94),
+ *     and the node prune therefore admits tenter == tmax (accel/bvh/bvh.glsl:28-36 uses <);
+ *   - analytic primitives are tested after the meshes, with the same (t, id) rule.
+ * Host-side builders (binned-SAH BVH over f64 boxes, env CDF) follow the TS sources.
+ */
+#define _GNU_SOURCE
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+#include <float.h>
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+#include "../include/ptmi.h"
+#include "../include/ptmi_math.h"
+
+#ifdef ORACLE_LIBM
+/* libm variant: GLSL builtins mapped to the C library instead of the declared arithmetic.
+ * Used only to show the declared arithmetic does not move the estimate. */
+#define SIN sinf
+#define COS cosf
+#define EXP expf
+#define LOG logf
+#define ATAN2 atan2f
+#define POW powf
+#define TAN tanf
+#else
+#define SIN ptm_sin
+#define COS ptm_cos
+#define EXP ptm_exp
+#define LOG ptm_log
+#define ATAN2 ptm_atan2
+#define POW ptm_pow
+#define TAN ptm_tan
+#endif
+
+/* ---------------- vectors (GLSL semantics, left-to-right evaluation) ---------------- */
+typedef struct { float x, y, z; } v3;
+static inline v3 V(float x, float y, float z) { v3 r = {x, y, z}; return r; }
+static inline v3 add(v3 a, v3 b) { return V(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline v3 sub(v3 a, v3 b) { return V(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline v3 mul(v3 a, v3 b) { return V(a.x * b.x, a.y * b.y, a.z * b.z); }
+static inline v3 muls(v3 a, float s) { return V(a.x * s, a.y * s, a.z * s); }
+static inline v3 divs(v3 a, float s) { return V(a.x / s, a.y / s, a.z / s); }
+static inline v3 neg(v3 a) { return V(-a.x, -a.y, -a.z); }
+static inline float dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline v3 cross(v3 a, v3 b) { return V(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+static inline float length3(v3 a) { return sqrtf(dot(a, a)); }
+static inline v3 normalize(v3 a) { return muls(a, 1.0f / sqrtf(dot(a, a))); }
+static inline float fmaxf_(float a, float b) { return a > b ? a : b; }   /* GLSL max(a,b) */
+static inline float fminf_(float a, float b) { return a < b ? a : b; }
+static inline float clampf(float x, float lo, float hi) { return fminf_(fmaxf_(x, lo), hi); }
+static inline v3 ld3(const float* p) { return V(p[0], p[1], p[2]); }
+static inline float signf(float x) { return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : 0.0f); }
+
+#define MAX_DIST 1000.0f
+#define EPS_INTERFACE 0.001f
+#define MARCH_CLEARANCE 0.001f
+#define SHADOW_BACKOFF 0.002f
+#define BVH_TFAR_PAD 1.00000024f
+#define INV_PI (1.0f / PTM_PI)
+
+/* ---------------- RNG: pcg4d (components/sampler/pcg4d/pcg4d.glsl:12-40) ---------------- */
+typedef struct { uint32_t base, dim; } rng_t;
+static inline void pcg4d(uint32_t v[4]) {
+    for (int i = 0; i < 4; i++) v[i] = v[i] * 1664525u + 1013904223u;
+    v[0] += v[1] * v[3]; v[1] += v[2] * v[0]; v[2] += v[0] * v[1]; v[3] += v[1] * v[2];
+    for (int i = 0; i < 4; i++) v[i] ^= v[i] >> 16u;
+    v[0] += v[1] * v[3]; v[1] += v[2] * v[0]; v[2] += v[0] * v[1]; v[3] += v[1] * v[2];
+}
+static inline void rng_init(rng_t* r, uint32_t px, uint32_t py, uint32_t sample, uint32_t salt) {
+    uint32_t v[4] = {px, py, sample, salt};
+    pcg4d(v);
+    r->base = v[0];
+    r->dim = 0;
+}
+static inline uint32_t rng_u32(rng_t* r) {
+    uint32_t v[4] = {r->base, r->dim++, 0u, 0u};
+    pcg4d(v);
+    return v[0];
+}
+static inline float rnd(rng_t* r) { return (float)(rng_u32(r) >> 8) * (1.0f / 16777216.0f); }
+
+void or_pcg4d(const uint32_t* in, uint32_t* out, int n) {
+    for (int i = 0; i < n; i++) {
+        uint32_t v[4] = {in[4 * i], in[4 * i + 1], in[4 * i + 2], in[4 * i + 3]};
+        pcg4d(v);
+        memcpy(out + 4 * i, v, 16);
+    }
+}
+void or_random_stream(uint32_t px, uint32_t py, uint32_t sample, uint32_t salt, float* out, int n) {
+    rng_t r; rng_init(&r, px, py, sample, salt);
+    for (int i = 0; i < n; i++) out[i] = rnd(&r);
+}
+
+/* ---------------- binned-SAH binary BVH (accel/bvh/bvh.ts:132-334) ----------------
+ * f64 boxes, f32 centroids and f32 SAH areas, 12 bins over all three axes, leaf <= 2,
+ * strict < cost ties, median fallback. Node = (min.xyz, A, max.xyz, B):
+ * A >= 0 leaf (count A at offset B of order[]), A < 0 internal (axis -A-1, right B). */
+#define BVH_BINS 12
+typedef struct { float* nodes; int n_nodes; int cap; uint32_t* order; int n; } bvh_t;
+
+typedef struct {
+    const double* boxes; float *cx, *cy, *cz; uint32_t* order; bvh_t* out; int leaf_size;
+} bvh_build_t;
+
+static int bvh_push_node(bvh_t* b) {
+    if (b->n_nodes == b->cap) { b->cap = b->cap ? 2 * b->cap : 64; b->nodes = (float*)realloc(b->nodes, sizeof(float) * 8 * (size_t)b->cap); }
+    return b->n_nodes++;
+}
+static double box_area(const double* b) {
+    double dx = b[3] - b[0], dy = b[4] - b[1], dz = b[5] - b[2];
+    if (dx < 0 || dy < 0 || dz < 0) return 0;
+    return 2 * (dx * dy + dy * dz + dz * dx);
+}
+static void box_reset(double* b) { b[0] = b[1] = b[2] = INFINITY; b[3] = b[4] = b[5] = -INFINITY; }
+static void box_grow(double* b, const double* s) {
+    for (int a = 0; a < 3; a++) { if (s[a] < b[a]) b[a] = s[a]; if (s[3 + a] > b[3 + a]) b[3 + a] = s[3 + a]; }
+}
+
+static int bvh_emit(bvh_build_t* B, int start, int end) {
+    bvh_t* out = B->out;
+    int node = bvh_push_node(out);
+    double bounds[6], cb[6];
+    box_reset(bounds); box_reset(cb);
+    for (int i = start; i < end; i++) {
+        uint32_t t = B->order[i]; const double* bx = B->boxes + 6 * (size_t)t;
+        for (int a = 0; a < 3; a++) { if (bx[a] < bounds[a]) bounds[a] = bx[a]; if (bx[3 + a] > bounds[3 + a]) bounds[3 + a] = bx[3 + a]; }
+        double c[3] = {B->cx[t], B->cy[t], B->cz[t]};
+        for (int a = 0; a < 3; a++) { if (c[a] < cb[a]) cb[a] = c[a]; if (c[a] > cb[3 + a]) cb[3 + a] = c[a]; }
+    }
+    int count = end - start;
+#define SET_NODE(A_, B_) do { float* nd = out->nodes + 8 * (size_t)node; \
+        nd[0] = (float)bounds[0]; nd[1] = (float)bounds[1]; nd[2] = (float)bounds[2]; nd[3] = (float)(A_); \
+        nd[4] = (float)bounds[3]; nd[5] = (float)bounds[4]; nd[6] = (float)bounds[5]; nd[7] = (float)(B_); } while (0)
+    if (count <= B->leaf_size) { SET_NODE(count, start); return node; }
+    int bestAxis = -1, bestSplit = -1; double bestCost = INFINITY;
+    int v[3] = {cb[3] - cb[0] >= 1e-12, cb[4] - cb[1] >= 1e-12, cb[5] - cb[2] >= 1e-12};
+    if (v[0] || v[1] || v[2]) {
+        double s[3], lo[3] = {cb[0], cb[1], cb[2]};
+        for (int a = 0; a < 3; a++) s[a] = v[a] ? BVH_BINS / (cb[3 + a] - cb[a]) : 0;
+        double binBox[3 * BVH_BINS][6]; int binCnt[3 * BVH_BINS];
+        for (int b = 0; b < 3 * BVH_BINS; b++) { box_reset(binBox[b]); binCnt[b] = 0; }
+        for (int i = start; i < end; i++) {
+            uint32_t t = B->order[i]; const double* bx = B->boxes + 6 * (size_t)t;
+            float c[3] = {B->cx[t], B->cy[t], B->cz[t]};
+            for (int a = 0; a < 3; a++) {
+                if (!v[a]) continue;
+                int b = (int)floor(((double)c[a] - lo[a]) * s[a]);
+                if (b < 0) b = 0; if (b >= BVH_BINS) b = BVH_BINS - 1;
+                binCnt[a * BVH_BINS + b]++;
+                box_grow(binBox[a * BVH_BINS + b], bx);
+            }
+        }
+        for (int a = 0; a < 3; a++) {
+            if (!v[a]) continue;
+            float leftArea[BVH_BINS - 1], rightArea[BVH_BINS - 1];
+            int leftCnt[BVH_BINS - 1], rightCnt[BVH_BINS - 1];
+            double acc[6]; int accCnt = 0;
+            box_reset(acc);
+            for (int b = 0; b < BVH_BINS - 1; b++) {
+                box_grow(acc, binBox[a * BVH_BINS + b]); accCnt += binCnt[a * BVH_BINS + b];
+                leftArea[b] = (float)box_area(acc); leftCnt[b] = accCnt;
+            }
+            box_reset(acc); accCnt = 0;
+            for (int b = BVH_BINS - 1; b > 0; b--) {
+                box_grow(acc, binBox[a * BVH_BINS + b]); accCnt += binCnt[a * BVH_BINS + b];
+                rightArea[b - 1] = (float)box_area(acc); rightCnt[b - 1] = accCnt;
+            }
+            for (int b = 0; b < BVH_BINS - 1; b++) {
+                if (leftCnt[b] == 0 || rightCnt[b] == 0) continue;
+                double cost = (double)leftArea[b] * (double)leftCnt[b] + (double)rightArea[b] * (double)rightCnt[b];
+                if (cost < bestCost) { bestCost = cost; bestAxis = a; bestSplit = b; }
+            }
+        }
+    }
+    double leafCost = box_area(bounds) * count;
+    if (bestAxis == -1 || bestCost >= leafCost) { SET_NODE(count, start); return node; }
+    int mid = -1;
+    {
+        double plo = cb[bestAxis], scale = BVH_BINS / (cb[3 + bestAxis] - cb[bestAxis]);
+        const float* cen = bestAxis == 0 ? B->cx : bestAxis == 1 ? B->cy : B->cz;
+        int m = start;
+        for (int i = start; i < end; i++) {
+            uint32_t t = B->order[i];
+            int b = (int)floor(((double)cen[t] - plo) * scale);
+            if (b < 0) b = 0; if (b >= BVH_BINS) b = BVH_BINS - 1;
+            if (b <= bestSplit) { uint32_t tmp = B->order[i]; B->order[i] = B->order[m]; B->order[m] = tmp; m++; }
+        }
+        if (m > start && m < end) mid = m;
+    }
+    if (mid == -1) mid = (start + end) >> 1;
+    double keep[6]; memcpy(keep, bounds, sizeof keep);
+    bvh_emit(B, start, mid);
+    int right = bvh_emit(B, mid, end);
+    memcpy(bounds, keep, sizeof keep);
+    SET_NODE(-(bestAxis + 1), right);
+#undef SET_NODE
+    return node;
+}
+
+/* boxes: n x 6 f64 (min, max). */
+static void bvh_build(bvh_t* out, const double* boxes, int n, int leaf_size) {
+    memset(out, 0, sizeof *out);
+    out->n = n;
+    out->order = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(n > 0 ? n : 1));
+    bvh_build_t B;
+    B.boxes = boxes; B.out = out; B.leaf_size = leaf_size; B.order = out->order;
+    B.cx = (float*)malloc(sizeof(float) * (size_t)(n > 0 ? n : 1));
+    B.cy = (float*)malloc(sizeof(float) * (size_t)(n > 0 ? n : 1));
+    B.cz = (float*)malloc(sizeof(float) * (size_t)(n > 0 ? n : 1));
+    for (int t = 0; t < n; t++) {
+        B.cx[t] = (float)((boxes[6 * t] + boxes[6 * t + 3]) * 0.5);
+        B.cy[t] = (float)((boxes[6 * t + 1] + boxes[6 * t + 4]) * 0.5);
+        B.cz[t] = (float)((boxes[6 * t + 2] + boxes[6 * t + 5]) * 0.5);
+        out->order[t] = (uint32_t)t;
+    }
+    if (n > 0) bvh_emit(&B, 0, n);
+    free(B.cx); free(B.cy); free(B.cz);
+}
+static void bvh_free(bvh_t* b) { free(b->nodes); free(b->order); memset(b, 0, sizeof *b); }
+
+/* bvh_aabb_hit (accel/bvh/bvh.glsl:28-36), prune admitting tenter == tmax (tie rule). */
+static inline int aabb_hit(const float* n, v3 ro, v3 inv, float tmax) {
+    v3 t0 = mul(sub(V(n[0], n[1], n[2]), ro), inv);
+    v3 t1 = mul(sub(V(n[4], n[5], n[6]), ro), inv);
+    v3 tsm = V(fminf_(t0.x, t1.x), fminf_(t0.y, t1.y), fminf_(t0.z, t1.z));
+    v3 tbg = V(fmaxf_(t0.x, t1.x), fmaxf_(t0.y, t1.y), fmaxf_(t0.z, t1.z));
+    float tn = fmaxf_(fmaxf_(tsm.x, tsm.y), tsm.z);
+    float tf = fminf_(fminf_(tbg.x, tbg.y), tbg.z) * BVH_TFAR_PAD;
+    float tenter = fmaxf_(tn, 0.0f);
+    return tf >= tenter && tenter <= tmax;
+}
+
+/* ---------------- scene ---------------- */
+typedef struct {
+    int n_tris; float* tri;   /* [n][9] */
+    int32_t* tri_object;
+    int n_objects; pt_object* objects;
+    int n_materials; pt_material* materials;
+    int n_lights; pt_light* lights;
+    pt_env env;
+    bvh_t* blas;              /* per object (meshes only) */
+    bvh_t tlas;               /* over mesh objects */
+    int* tlas_obj;            /* tlas item -> object */
+    int n_mesh_objects;
+    /* env table */
+    float* env_cond; float* env_marg; int env_w, env_h; double env_total;
+    float env_rot_cs[2];
+    int has_transmission;
+    int emissive_solid;       /* some closed/solid region is emissive */
+} or_scene;
+
+/* Möller–Trumbore (intersection/mesh/mesh.glsl:41-59). */
+static inline int tri_test(v3 ro, v3 rd, v3 a, v3 b, v3 c, float* t_out, v3* g_out) {
+    v3 e1 = sub(b, a), e2 = sub(c, a);
+    v3 g = cross(e1, e2);
+    float det = -dot(rd, g);
+    if (fabsf(det) < 1e-12f) return 0;
+    float invdet = 1.0f / det;
+    v3 ao = sub(ro, a);
+    v3 dao = cross(ao, rd);
+    float u = dot(e2, dao) * invdet;
+    float v = -dot(e1, dao) * invdet;
+    float t = dot(ao, g) * invdet;
+    float w = 1.0f - u - v;
+    *t_out = t; *g_out = g;
+    const float E = 1e-6f;
+    return u >= -E && v >= -E && w >= -E;
+}
+static inline void tri_load(const or_scene* S, int i, v3* a, v3* b, v3* c) {
+    const float* p = S->tri + 9 * (size_t)i;
+    *a = V(p[0], p[1], p[2]); *b = V(p[3], p[4], p[5]); *c = V(p[6], p[7], p[8]);
+}
+
+/* Nearest over one mesh BLAS (mesh.glsl:162-191) with the (t, id) tie rule. */
+static void blas_nearest(const or_scene* S, int obj, v3 ro, v3 rd, float* tmax, int* best) {
+    const bvh_t* B = &S->blas[obj];
+    if (B->n_nodes == 0) return;
+    int first = S->objects[obj].first_tri;
+    v3 inv = V(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
+    int stack[64]; int ptr = 0; stack[0] = 0;
+    while (ptr >= 0) {
+        int ni = stack[ptr--];
+        const float* n = B->nodes + 8 * (size_t)ni;
+        if (!aabb_hit(n, ro, inv, *tmax)) continue;
+        if (n[3] >= 0.0f) {
+            int off = (int)n[7], cnt = (int)n[3];
+            for (int i = off; i < off + cnt; i++) {
+                int tri = first + (int)B->order[i];
+                v3 a, b, c, g; float t;
+                tri_load(S, tri, &a, &b, &c);
+                if (tri_test(ro, rd, a, b, c, &t, &g) && t > 0.0f && (t < *tmax || (t == *tmax && tri < *best))) {
+                    *tmax = t; *best = tri;
+                }
+            }
+        } else {
+            int axis = (int)(-n[3] - 1.0f);
+            int L = ni + 1, R = (int)n[7];
+            float rda = axis == 0 ? rd.x : axis == 1 ? rd.y : rd.z;
+            int nearFirst = rda >= 0.0f;
+            if (ptr + 2 < 64) { stack[++ptr] = nearFirst ? R : L; stack[++ptr] = nearFirst ? L : R; }
+        }
+    }
+}
+static int blas_any(const or_scene* S, int obj, v3 ro, v3 rd, float maxDist) {
+    const bvh_t* B = &S->blas[obj];
+    if (B->n_nodes == 0) return 0;
+    int first = S->objects[obj].first_tri;
+    v3 inv = V(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
+    int stack[64]; int ptr = 0; stack[0] = 0;
+    while (ptr >= 0) {
+        int ni = stack[ptr--];
+        const float* n = B->nodes + 8 * (size_t)ni;
+        if (!aabb_hit(n, ro, inv, maxDist)) continue;
+        if (n[3] >= 0.0f) {
+            int off = (int)n[7], cnt = (int)n[3];
+            for (int i = off; i < off + cnt; i++) {
+                v3 a, b, c, g; float t;
+                tri_load(S, first + (int)B->order[i], &a, &b, &c);
+                if (tri_test(ro, rd, a, b, c, &t, &g) && t > 0.0f && t < maxDist) return 1;
+            }
+        } else if (ptr + 2 < 64) {
+            stack[++ptr] = ni + 1; stack[++ptr] = (int)n[7];
+        }
+    }
+    return 0;
+}
+
+/* Analytic primitives (geometry/sphere/sphere.glsl:28-46, geometry/quad/quad.glsl:9-22). */
+static int sphere_intersect(v3 ro, v3 rd, v3 c, float r, float* t) {
+    v3 oc = sub(ro, c);
+    float b = dot(oc, rd);
+    v3 perp = sub(oc, muls(rd, b));
+    float disc = r * r - dot(perp, perp);
+    if (disc < 0.0f) return 0;
+    float cc = dot(oc, oc) - r * r;
+    float q = (b >= 0.0f) ? -b - sqrtf(disc) : -b + sqrtf(disc);
+    if (q == 0.0f) return 0;
+    float ta = q, tb = cc / q;
+    *t = (cc < 0.0f) ? fmaxf_(ta, tb) : fminf_(ta, tb);
+    return *t > 0.0f;
+}
+static int quad_intersect(v3 ro, v3 rd, const float* p, float* t) {
+    v3 corner = ld3(p), e1 = ld3(p + 3), e2 = ld3(p + 6), n = ld3(p + 9);
+    float denom = dot(rd, n);
+    if (fabsf(denom) < 1e-8f) return 0;
+    *t = dot(sub(corner, ro), n) / denom;
+    if (*t <= 0.0f) return 0;
+    v3 local = sub(add(ro, muls(rd, *t)), corner);
+    float e11 = dot(e1, e1), e22 = dot(e2, e2), e12 = dot(e1, e2);
+    float d1 = dot(local, e1), d2 = dot(local, e2);
+    float det = e11 * e22 - e12 * e12;
+    float u = (d1 * e22 - d2 * e12) / det;
+    float v = (d2 * e11 - d1 * e12) / det;
+    return u >= 0.0f && u <= 1.0f && v >= 0.0f && v <= 1.0f;
+}
+
+/* Closest hit over the whole scene: scene TLAS over mesh objects (leaf size 1,
+ * app/sceneData.ts:88 + features/intersection.ts:1411-1468), then the analytic objects.
+ * Returns the global primitive id (-1 = none) and shrinks *tmax. */
+static int scene_closest(const or_scene* S, v3 ro, v3 rd, float* tmax) {
+    int best = -1;
+    if (S->tlas.n_nodes > 0) {
+        v3 inv = V(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
+        int stack[64]; int ptr = 0; stack[0] = 0;
+        while (ptr >= 0) {
+            int ni = stack[ptr--];
+            const float* n = S->tlas.nodes + 8 * (size_t)ni;
+            if (!aabb_hit(n, ro, inv, *tmax)) continue;
+            if (n[3] >= 0.0f) {
+                int off = (int)n[7], cnt = (int)n[3];
+                for (int j = 0; j < cnt; j++) blas_nearest(S, S->tlas_obj[S->tlas.order[off + j]], ro, rd, tmax, &best);
+            } else {
+                int axis = (int)(-n[3] - 1.0f);
+                int L = ni + 1, R = (int)n[7];
+                float rda = axis == 0 ? rd.x : axis == 1 ? rd.y : rd.z;
+                int nearFirst = rda >= 0.0f;
+                if (ptr + 2 < 64) { stack[++ptr] = nearFirst ? R : L; stack[++ptr] = nearFirst ? L : R; }
+            }
+        }
+    }
+    for (int o = 0; o < S->n_objects; o++) {
+        const pt_object* ob = &S->objects[o];
+        float t; int ok = 0;
+        if (ob->kind == PT_OBJ_QUAD) ok = quad_intersect(ro, rd, ob->p, &t);
+        else if (ob->kind == PT_OBJ_SPHERE) ok = sphere_intersect(ro, rd, ld3(ob->p), ob->p[3], &t);
+        if (ok && t < *tmax) { *tmax = t; best = S->n_tris + o; }
+    }
+    return best;
+}
+static int scene_any(const or_scene* S, v3 ro, v3 rd, float maxDist) {
+    for (int o = 0; o < S->n_objects; o++) {
+        const pt_object* ob = &S->objects[o];
+        float t;
+        if (ob->kind == PT_OBJ_QUAD && quad_intersect(ro, rd, ob->p, &t) && t < maxDist) return 1;
+        if (ob->kind == PT_OBJ_SPHERE && sphere_intersect(ro, rd, ld3(ob->p), ob->p[3], &t) && t < maxDist) return 1;
+    }
+    if (S->tlas.n_nodes == 0) return 0;
+    v3 inv = V(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
+    int stack[64]; int ptr = 0; stack[0] = 0;
+    while (ptr >= 0) {
+        int ni = stack[ptr--];
+        const float* n = S->tlas.nodes + 8 * (size_t)ni;
+        if (!aabb_hit(n, ro, inv, maxDist)) continue;
+        if (n[3] >= 0.0f) {
+            int off = (int)n[7], cnt = (int)n[3];
+            for (int j = 0; j < cnt; j++) if (blas_any(S, S->tlas_obj[S->tlas.order[off + j]], ro, rd, maxDist)) return 1;
+        } else if (ptr + 2 < 64) {
+            stack[++ptr] = ni + 1; stack[++ptr] = (int)n[7];
+        }
+    }
+    return 0;
+}
+
+/* ---------------- containment: scene_region_at (features/intersection.ts:1121-1265) ---- */
+static const v3 MESH_INSIDE_DIR = {0.5320544f, 0.7396997f, 0.4114743f};
+
+/* mesh_inside_bvh (mesh.glsl:202-225): first-hit facing along a fixed direction. */
+static int mesh_inside(const or_scene* S, int obj, v3 p) {
+    const bvh_t* B = &S->blas[obj];
+    if (B->n_nodes == 0) return 0;
+    int first = S->objects[obj].first_tri;
+    float tmax = 1.0e20f, sideDot = 0.0f; int best = -1;
+    v3 dir = MESH_INSIDE_DIR;
+    v3 inv = V(1.0f / dir.x, 1.0f / dir.y, 1.0f / dir.z);
+    int stack[64]; int ptr = 0; stack[0] = 0;
+    while (ptr >= 0) {
+        int ni = stack[ptr--];
+        const float* n = B->nodes + 8 * (size_t)ni;
+        if (!aabb_hit(n, p, inv, tmax)) continue;
+        if (n[3] >= 0.0f) {
+            int off = (int)n[7], cnt = (int)n[3];
+            for (int i = off; i < off + cnt; i++) {
+                int tri = first + (int)B->order[i];
+                v3 a, b, c, g; float t;
+                tri_load(S, tri, &a, &b, &c);
+                if (tri_test(p, dir, a, b, c, &t, &g) && t > 0.0f && (t < tmax || (t == tmax && tri < best))) {
+                    tmax = t; best = tri; sideDot = dot(dir, g);
+                }
+            }
+        } else {
+            int axis = (int)(-n[3] - 1.0f);
+            int L = ni + 1, R = (int)n[7];
+            float da = axis == 0 ? dir.x : axis == 1 ? dir.y : dir.z;
+            int nearFirst = da >= 0.0f;
+            if (ptr + 2 < 64) { stack[++ptr] = nearFirst ? R : L; stack[++ptr] = nearFirst ? L : R; }
+        }
+    }
+    return sideDot > 0.0f;
+}
+/* Ericson closest point on triangle, squared (mesh.glsl:236-262). */
+static float point_tri_dist2(v3 p, v3 a, v3 b, v3 c) {
+    v3 ab = sub(b, a), ac = sub(c, a), ap = sub(p, a);
+    float d1 = dot(ab, ap), d2 = dot(ac, ap);
+    if (d1 <= 0.0f && d2 <= 0.0f) { v3 q = sub(p, a); return dot(q, q); }
+    v3 bp = sub(p, b);
+    float d3 = dot(ab, bp), d4 = dot(ac, bp);
+    if (d3 >= 0.0f && d4 <= d3) { v3 q = sub(p, b); return dot(q, q); }
+    float vc = d1 * d4 - d3 * d2;
+    if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) { v3 q = sub(ap, muls(ab, d1 / (d1 - d3))); return dot(q, q); }
+    v3 cp = sub(p, c);
+    float d5 = dot(ab, cp), d6 = dot(ac, cp);
+    if (d6 >= 0.0f && d5 <= d6) { v3 q = sub(p, c); return dot(q, q); }
+    float vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) { v3 q = sub(ap, muls(ac, d2 / (d2 - d6))); return dot(q, q); }
+    float va = d3 * d6 - d5 * d4;
+    if (va <= 0.0f && (d4 - d3) >= 0.0f && (d5 - d6) >= 0.0f) {
+        v3 q = sub(bp, muls(sub(c, b), (d4 - d3) / ((d4 - d3) + (d5 - d6))));
+        return dot(q, q);
+    }
+    float denom = 1.0f / (va + vb + vc);
+    v3 closest = add(add(a, muls(ab, vb * denom)), muls(ac, vc * denom));
+    v3 q = sub(p, closest);
+    return dot(q, q);
+}
+static float box_dist2(const float* n, v3 p) {
+    v3 d = V(fmaxf_(fmaxf_(n[0] - p.x, 0.0f), p.x - n[4]), fmaxf_(fmaxf_(n[1] - p.y, 0.0f), p.y - n[5]),
+             fmaxf_(fmaxf_(n[2] - p.z, 0.0f), p.z - n[6]));
+    return dot(d, d);
+}
+/* mesh_closest_bvh (mesh.glsl:266-296): the minimum is order independent. */
+static float mesh_closest(const or_scene* S, int obj, v3 p) {
+    const bvh_t* B = &S->blas[obj];
+    if (B->n_nodes == 0) return 1.0e20f;
+    int first = S->objects[obj].first_tri;
+    float best2 = 1.0e30f;
+    int stack[64]; int ptr = 0; stack[0] = 0;
+    while (ptr >= 0) {
+        int ni = stack[ptr--];
+        const float* n = B->nodes + 8 * (size_t)ni;
+        if (box_dist2(n, p) >= best2) continue;
+        if (n[3] >= 0.0f) {
+            int off = (int)n[7], cnt = (int)n[3];
+            for (int i = off; i < off + cnt; i++) {
+                v3 a, b, c; tri_load(S, first + (int)B->order[i], &a, &b, &c);
+                best2 = fminf_(best2, point_tri_dist2(p, a, b, c));
+            }
+        } else {
+            int L = ni + 1, R = (int)n[7];
+            int leftNear = box_dist2(B->nodes + 8 * (size_t)L, p) <= box_dist2(B->nodes + 8 * (size_t)R, p);
+            if (ptr + 2 < 64) { stack[++ptr] = leftNear ? R : L; stack[++ptr] = leftNear ? L : R; }
+        }
+    }
+    return sqrtf(best2);
+}
+static int region_at(const or_scene* S, v3 p) {
+    int region = -1; float best = -1.0e20f;
+    for (int o = 0; o < S->n_objects; o++) {
+        const pt_object* ob = &S->objects[o];
+        if (ob->kind != PT_OBJ_SPHERE) continue;
+        float d = length3(sub(p, ld3(ob->p))) - ob->p[3];
+        if (d < 0.0f && d > best) { best = d; region = o; }
+    }
+    /* closed meshes: point descent of the scene TLAS (only leaves whose box holds p) */
+    if (S->tlas.n_nodes == 0) return region;
+    int stack[64]; int ptr = 0; stack[0] = 0;
+    while (ptr >= 0) {
+        int ni = stack[ptr--];
+        const float* n = S->tlas.nodes + 8 * (size_t)ni;
+        if (!(p.x >= n[0] && p.y >= n[1] && p.z >= n[2] && p.x <= n[4] && p.y <= n[5] && p.z <= n[6])) continue;
+        if (n[3] >= 0.0f) {
+            int off = (int)n[7], cnt = (int)n[3];
+            for (int j = 0; j < cnt; j++) {
+                int o = S->tlas_obj[S->tlas.order[off + j]];
+                const pt_object* ob = &S->objects[o];
+                if (!ob->closed) continue;
+                float d = 1.0e20f;
+                if (p.x >= ob->box_min[0] && p.y >= ob->box_min[1] && p.z >= ob->box_min[2] &&
+                    p.x <= ob->box_max[0] && p.y <= ob->box_max[1] && p.z <= ob->box_max[2] && mesh_inside(S, o, p))
+                    d = -1.0f * mesh_closest(S, o, p);
+                if (d < 0.0f && d > best) { best = d; region = o; }
+            }
+        } else if (ptr + 2 < 64) {
+            stack[++ptr] = ni + 1; stack[++ptr] = (int)n[7];
+        }
+    }
+    return region;
+}
+static int object_thin(const or_scene* S, int o) {
+    const pt_object* ob = &S->objects[o];
+    return ob->kind == PT_OBJ_QUAD || (ob->kind == PT_OBJ_MESH && !ob->closed);
+}
+
+/* ---------------- hit (structs.glsl Hit; features/intersection.ts:889-948,1665-1728) ---- */
+typedef struct { v3 p, t, b, n; } frame_t;
+typedef struct {
+    float t; v3 p; frame_t frame; v3 ng;
+    int region_from, region_to, region_owner; float eps; int prim;
+} hit_t;
+
+static inline float fp_uncertainty(v3 p) {
+    return fmaxf_(1.52587890625e-5f, 3.0517578125e-5f * fmaxf_(fabsf(p.x), fmaxf_(fabsf(p.y), fabsf(p.z))));
+}
+/* ambient_frame (components/ambient/euclidean/euclidean.glsl:13-19) */
+static inline frame_t ambient_frame(v3 p, v3 n) {
+    frame_t f;
+    v3 nn = normalize(n);
+    v3 t = fabsf(nn.x) < 0.9f ? V(1.0f, 0.0f, 0.0f) : V(0.0f, 1.0f, 0.0f);
+    t = normalize(sub(t, muls(nn, dot(t, nn))));
+    f.p = p; f.t = t; f.b = cross(nn, t); f.n = nn;
+    return f;
+}
+
+static int scene_intersect(const or_scene* S, v3 ro, v3 rd, hit_t* hit) {
+    float tmax = MAX_DIST;
+    int prim = scene_closest(S, ro, rd, &tmax);
+    if (prim < 0) return 0;
+    hit->t = tmax; hit->prim = prim;
+    hit->p = add(ro, muls(rd, tmax));
+    hit->eps = fp_uncertainty(hit->p);
+    if (prim < S->n_tris) {
+        v3 a, b, c; tri_load(S, prim, &a, &b, &c);
+        v3 g = cross(sub(b, a), sub(c, a));
+        hit->frame = ambient_frame(hit->p, normalize(g));
+        hit->ng = normalize(g);
+        hit->region_owner = S->tri_object[prim];
+    } else {
+        int o = prim - S->n_tris;
+        const pt_object* ob = &S->objects[o];
+        v3 n = ob->kind == PT_OBJ_QUAD ? ld3(ob->p + 9) : normalize(sub(hit->p, ld3(ob->p)));
+        hit->frame = ambient_frame(hit->p, n);
+        hit->ng = hit->frame.n;
+        hit->region_owner = o;
+    }
+    int outside = region_at(S, add(hit->p, muls(hit->ng, EPS_INTERFACE)));
+    if (dot(rd, hit->ng) < 0.0f) {
+        hit->region_from = outside;
+        hit->region_to = hit->region_owner;
+    } else {
+        hit->region_from = object_thin(S, hit->region_owner)
+            ? region_at(S, add(hit->p, muls(hit->ng, -EPS_INTERFACE))) : hit->region_owner;
+        hit->region_to = outside;
+        hit->frame = ambient_frame(hit->p, neg(hit->frame.n));
+        hit->ng = neg(hit->ng);
+    }
+    return 1;
+}
+
+static inline int material_of(const or_scene* S, int region) { return region < 0 ? -1 : S->objects[region].material; }
+static inline float ior_of(const or_scene* S, int region) {
+    if (region < 0) return 1.0f;
+    const pt_object* ob = &S->objects[region];
+    if (ob->kind == PT_OBJ_QUAD || (ob->kind == PT_OBJ_MESH && !ob->closed)) return 1.0f;
+    const pt_material* m = &S->materials[ob->material];
+    return m->model == PT_MAT_DIELECTRIC ? m->ior : 1.0f;
+}
+
+/* ---------------- materials ---------------- */
+enum { LOBE_REFLECTION = 1, LOBE_TRANSMISSION = 2, LOBE_DELTA = 4 };
+typedef struct { v3 wi; v3 weight; float pdf; int flags; } isample_t;
+
+static inline v3 lv(const float* a) { return V(a[0], a[1], a[2]); }
+static inline v3 to_local(frame_t f, v3 v) { return V(dot(v, f.t), dot(v, f.b), dot(v, f.n)); }
+static inline v3 from_local(frame_t f, v3 v) { return normalize(add(add(muls(f.t, v.x), muls(f.b, v.y)), muls(f.n, v.z))); }
+/* microfacet.glsl:29-54 */
+static inline float mf_D(v3 h, float a) { float t = h.z * h.z * (a * a - 1.0f) + 1.0f; return a * a / (PTM_PI * t * t); }
+static inline float mf_G1(v3 v, float a) { float c = fabsf(v.z); return 2.0f * c / (c + sqrtf(a * a + (1.0f - a * a) * c * c)); }
+static v3 mf_vndf(v3 wol, float a, float ux, float uy) {
+    v3 vh = normalize(V(a * wol.x, a * wol.y, wol.z));
+    float lensq = vh.x * vh.x + vh.y * vh.y;
+    v3 T1 = lensq > 0.0f ? muls(V(-vh.y, vh.x, 0.0f), 1.0f / sqrtf(lensq)) : V(1.0f, 0.0f, 0.0f);
+    v3 T2 = cross(vh, T1);
+    float rr = sqrtf(ux);
+    float phi = PTM_TWO_PI * uy;
+    float t1 = rr * COS(phi);
+    float t2 = rr * SIN(phi);
+    float s_ = 0.5f * (1.0f + vh.z);
+    t2 = (1.0f - s_) * sqrtf(fmaxf_(0.0f, 1.0f - t1 * t1)) + s_ * t2;
+    v3 nh = add(add(muls(T1, t1), muls(T2, t2)), muls(vh, sqrtf(fmaxf_(0.0f, 1.0f - t1 * t1 - t2 * t2))));
+    return normalize(V(a * nh.x, a * nh.y, fmaxf_(1e-6f, nh.z)));
+}
+static inline v3 schlick(v3 f0, float c) {
+    float p = POW(1.0f - c, 5.0f);
+    return V(f0.x + (1.0f - f0.x) * p, f0.y + (1.0f - f0.y) * p, f0.z + (1.0f - f0.z) * p);
+}
+/* math.glsl:95-102 */
+static inline float dielectric_fresnel(float cos_i, float eta) {
+    float sin2_t = eta * eta * (1.0f - cos_i * cos_i);
+    if (sin2_t >= 1.0f) return 1.0f;
+    float cos_t = sqrtf(1.0f - sin2_t);
+    float r_par = (cos_i - eta * cos_t) / (cos_i + eta * cos_t);
+    float r_perp = (eta * cos_i - cos_t) / (eta * cos_i + cos_t);
+    return 0.5f * (r_par * r_par + r_perp * r_perp);
+}
+
+static isample_t surface_sample(const or_scene* S, const pt_material* m, v3 wo, const hit_t* hit, float uc, float ux, float uy) {
+    isample_t s;
+    frame_t f = hit->frame;
+    if (m->model == PT_MAT_LAMBERT) {               /* materials/lambert/lambert.glsl:17-28 */
+        v3 n = dot(wo, f.n) < 0.0f ? neg(f.n) : f.n;
+        float cos_theta = sqrtf(uy);
+        float sin_theta = sqrtf(fmaxf_(0.0f, 1.0f - uy));
+        float phi = PTM_TWO_PI * ux;
+        s.wi = normalize(add(add(muls(f.t, sin_theta * COS(phi)), muls(f.b, sin_theta * SIN(phi))), muls(n, cos_theta)));
+        s.weight = lv(m->albedo);
+        s.pdf = cos_theta * INV_PI;
+        s.flags = LOBE_REFLECTION;
+    } else if (m->model == PT_MAT_DIELECTRIC) {     /* materials/dielectric/dielectric.glsl:14-52 */
+        float n_i = ior_of(S, hit->region_from);
+        float n_t = ior_of(S, hit->region_to);
+        float eta = n_i / n_t;
+        v3 n = f.n;
+        float cos_i = clampf(dot(wo, n), 1e-6f, 1.0f);
+        float F = dielectric_fresnel(cos_i, eta);
+        if (uc < F) {
+            s.wi = normalize(sub(muls(n, 2.0f * cos_i), wo));
+            s.weight = V(1.0f, 1.0f, 1.0f);
+            s.pdf = 0.0f;
+            s.flags = LOBE_REFLECTION | LOBE_DELTA;
+        } else {
+            float sin2_t = eta * eta * (1.0f - cos_i * cos_i);
+            float cos_t = sqrtf(fmaxf_(0.0f, 1.0f - sin2_t));
+            s.wi = normalize(add(muls(wo, -eta), muls(n, eta * cos_i - cos_t)));
+            float radiance_scale = (n_i * n_i) / (n_t * n_t);
+            s.weight = muls(lv(m->transmittance), radiance_scale);
+            s.pdf = 0.0f;
+            s.flags = LOBE_TRANSMISSION | LOBE_DELTA;
+        }
+    } else {                                        /* materials/ggx/ggx.glsl:27-46 */
+        v3 wol = to_local(f, wo);
+        float side = wol.z < 0.0f ? -1.0f : 1.0f;
+        wol = muls(wol, side);
+        float a = m->alpha;
+        v3 h = mf_vndf(wol, a, ux, uy);
+        v3 I = neg(wol);
+        v3 wil = sub(I, muls(h, 2.0f * dot(h, I)));
+        if (wil.z <= 0.0f) {
+            s.wi = wo; s.weight = V(0.0f, 0.0f, 0.0f); s.pdf = 0.0f; s.flags = LOBE_REFLECTION;
+            return s;
+        }
+        v3 F = schlick(lv(m->f0), fabsf(dot(wol, h)));
+        s.weight = muls(F, mf_G1(wil, a));
+        s.pdf = mf_G1(wol, a) * mf_D(h, a) / (4.0f * fabsf(wol.z));
+        s.flags = LOBE_REFLECTION;
+        s.wi = from_local(f, muls(wil, side));
+    }
+    return s;
+}
+static v3 surface_eval(const pt_material* m, v3 wi, v3 wo, const hit_t* hit) {
+    frame_t f = hit->frame;
+    if (m->model == PT_MAT_LAMBERT) {
+        if (dot(wi, f.n) * dot(wo, f.n) <= 0.0f) return V(0.0f, 0.0f, 0.0f);
+        return muls(lv(m->albedo), INV_PI);
+    } else if (m->model == PT_MAT_GGX) {
+        v3 wil = to_local(f, wi), wol = to_local(f, wo);
+        if (wil.z * wol.z <= 0.0f) return V(0.0f, 0.0f, 0.0f);
+        float a = m->alpha;
+        v3 h = normalize(add(wil, wol));
+        v3 F = schlick(lv(m->f0), fabsf(dot(wol, h)));
+        return muls(F, mf_D(h, a) * mf_G1(wil, a) * mf_G1(wol, a) / (4.0f * fabsf(wil.z) * fabsf(wol.z)));
+    }
+    return V(0.0f, 0.0f, 0.0f);
+}
+static float surface_pdf(const pt_material* m, v3 wi, v3 wo, const hit_t* hit) {
+    frame_t f = hit->frame;
+    if (m->model == PT_MAT_LAMBERT) {
+        float c = dot(wi, f.n) * signf(dot(wo, f.n));
+        return fmaxf_(0.0f, c) * INV_PI;
+    } else if (m->model == PT_MAT_GGX) {
+        v3 wil = to_local(f, wi), wol = to_local(f, wo);
+        if (wil.z * wol.z <= 0.0f) return 0.0f;
+        float a = m->alpha;
+        v3 h = normalize(add(wil, wol));
+        return mf_G1(wol, a) * mf_D(h, a) / (4.0f * fabsf(wol.z));
+    }
+    return 0.0f;
+}
+static inline int has_nondelta(const pt_material* m) { return m->model != PT_MAT_DIELECTRIC; }
+
+/* ---------------- environment (components/env, proc-sky snapshot) ---------------- */
+static v3 sky_radiance_raw(v3 dir) {
+    v3 sun = normalize(V(0.4f, 0.35f, 0.2f));
+    float s = 0.5f * (dir.y + 1.0f);
+    v3 lo = V(0.06f, 0.08f, 0.12f), hi = V(0.35f, 0.45f, 0.65f);
+    v3 base = add(muls(lo, 1.0f - s), muls(hi, s));
+    float k = ptm_smoothstep(0.9994f, 0.9999f, dot(dir, sun));
+    return add(base, muls(V(18.0f, 16.0f, 13.0f), k));
+}
+static v3 env_radiance(const or_scene* S, v3 dir) {
+    if (S->env.type != PT_ENV_SKY) return V(0.0f, 0.0f, 0.0f);
+    v3 d = normalize(dir);
+    float cx = S->env_rot_cs[0], sy = S->env_rot_cs[1];
+    d = V(cx * d.x - sy * d.z, d.y, sy * d.x + cx * d.z);
+    return muls(sky_radiance_raw(d), S->env.intensity);
+}
+/* equirect chart (env/equirect/equirect_map.glsl, equirect.glsl:19-48) */
+static void equirect_uv(const or_scene* S, v3 dir, float* u, float* v) {
+    v3 n = normalize(dir);
+    float phi = (n.x == 0.0f && n.z == 0.0f ? 0.0f : ATAN2(n.z, n.x)) + S->env.rotation;
+    float theta = ATAN2(sqrtf(n.x * n.x + n.z * n.z), n.y);
+    *u = ptm_fract(phi * (1.0f / PTM_TWO_PI) + 0.5f);
+    *v = theta * (1.0f / PTM_PI);
+}
+static v3 env_chart_dir(float rot, float u, float v) {
+    float phi = (u - 0.5f) * PTM_TWO_PI - rot;
+    float theta = v * PTM_PI;
+    float s = SIN(theta);
+    return V(COS(phi) * s, COS(theta), SIN(phi) * s);
+}
+static float env_texel_dOmega(int j, int W, int H) {
+    float theta = PTM_PI * ((float)j + 0.5f) / (float)H;
+    return (PTM_TWO_PI / (float)W) * 2.0f * SIN(theta) * SIN(0.5f * PTM_PI / (float)H);
+}
+static v3 env_chart_sample(const or_scene* S, int ci, int cj, float ux, float uy) {
+    int W = S->env_w, H = S->env_h;
+    float phi = (((float)ci + ux) / (float)W - 0.5f) * PTM_TWO_PI - S->env.rotation;
+    int north = 2 * cj < H;
+    float row = north ? (float)cj : (float)(H - cj - 1);
+    float a = SIN(0.5f * PTM_PI * row / (float)H);
+    float b = SIN(0.5f * PTM_PI * (row + 1.0f) / (float)H);
+    float tt = north ? uy : 1.0f - uy;
+    float h = a * a * (1.0f - tt) + b * b * tt;
+    float y = (north ? 1.0f : -1.0f) * (1.0f - 2.0f * h);
+    float r = 2.0f * sqrtf(h * (1.0f - h));
+    return V(r * COS(phi), y, r * SIN(phi));
+}
+static inline float env_marg(const or_scene* S, int j) { return S->env_marg[j]; }
+static inline float env_cond(const or_scene* S, int j, int i) { return S->env_cond[(size_t)j * S->env_w + i]; }
+/* sampler_cdf.glsl:13-84 */
+static float env_table_pdf(const or_scene* S, v3 dir) {
+    int W = S->env_w, H = S->env_h;
+    float u, v; equirect_uv(S, dir, &u, &v);
+    int i = (int)(u * (float)W); i = i < 0 ? 0 : (i > W - 1 ? W - 1 : i);
+    int j = (int)(v * (float)H); j = j < 0 ? 0 : (j > H - 1 ? H - 1 : j);
+    float dMarg = env_marg(S, j) - (j > 0 ? env_marg(S, j - 1) : 0.0f);
+    float dCond = env_cond(S, j, i) - (i > 0 ? env_cond(S, j, i - 1) : 0.0f);
+    return (dCond * dMarg) / env_texel_dOmega(j, W, H);
+}
+static v3 env_sample_table(const or_scene* S, float x0, float x1) {
+    int W = S->env_w, H = S->env_h;
+    int lo = 0, hi = H - 1;
+    while (lo < hi) { int mid = (lo + hi) >> 1; if (x0 >= env_marg(S, mid)) lo = mid + 1; else hi = mid; }
+    int j = lo;
+    float mLo = (j > 0) ? env_marg(S, j - 1) : 0.0f;
+    float dMarg = env_marg(S, j) - mLo;
+    float vFrac = (dMarg > 0.0f) ? clampf((x0 - mLo) / dMarg, 0.0f, 0.9999999f) : 0.5f;
+    lo = 0; hi = W - 1;
+    while (lo < hi) { int mid = (lo + hi) >> 1; if (x1 >= env_cond(S, j, mid)) lo = mid + 1; else hi = mid; }
+    int i = lo;
+    float cLo = (i > 0) ? env_cond(S, j, i - 1) : 0.0f;
+    float dCond = env_cond(S, j, i) - cLo;
+    float uFrac = (dCond > 0.0f) ? clampf((x1 - cLo) / dCond, 0.0f, 0.9999999f) : 0.5f;
+    return env_chart_sample(S, i, j, uFrac, vFrac);
+}
+static float env_uniform_fraction(const or_scene* S) {
+    if (S->env.compensated) return 0.0f;
+    return env_marg(S, S->env_h - 1) > 0.0f ? 0.0625f : 1.0f;
+}
+static float environment_pdf(const or_scene* S, v3 dir) {
+    float a = env_uniform_fraction(S);
+    return (1.0f - a) * env_table_pdf(S, dir) + a / (4.0f * PTM_PI);
+}
+
+/* Env bake (compiler/EnvironmentBake.ts:62-67) + importance table (env/importance.ts:16-140). */
+static void env_build(or_scene* S) {
+    int W = S->env.table_w > 0 ? S->env.table_w : 512, H = S->env.table_h > 0 ? S->env.table_h : 256;
+    S->env_w = W; S->env_h = H;
+    float* Y = (float*)malloc(sizeof(float) * (size_t)W * H);
+    for (int j = 0; j < H; j++)
+        for (int i = 0; i < W; i++) {
+            float u = ((float)i + 0.5f) / (float)W, v = ((float)j + 0.5f) / (float)H;
+            v3 d = env_chart_dir(0.0f, u, v);
+            v3 c = sky_radiance_raw(d);
+            Y[(size_t)j * W + i] = (float)(0.2126 * (double)c.x + 0.7152 * (double)c.y + 0.0722 * (double)c.z);
+        }
+    float* Bl = (float*)malloc(sizeof(float) * (size_t)W * H);
+    for (int j = 0; j < H; j++)
+        for (int i = 0; i < W; i++) {
+            double sum = 0, wsum = 0;
+            for (int dj = -1; dj <= 1; dj++) {
+                int jj = j + dj; jj = jj < 0 ? 0 : (jj > H - 1 ? H - 1 : jj);
+                for (int di = -1; di <= 1; di++) {
+                    int ii = (i + di + W) % W;
+                    double w = (dj == 0 ? 2 : 1) * (di == 0 ? 2 : 1);
+                    sum += w * (double)Y[(size_t)jj * W + ii];
+                    wsum += w;
+                }
+            }
+            Bl[(size_t)j * W + i] = (float)(sum / wsum);
+        }
+    if (S->env.compensated) {
+        double num = 0, den = 0;
+        for (int j = 0; j < H; j++) {
+            double dO = (2 * M_PI / W) * (M_PI / H) * sin(M_PI * (j + 0.5) / H);
+            for (int i = 0; i < W; i++) { num += (double)Bl[(size_t)j * W + i] * dO; den += dO; }
+        }
+        double mean = den > 0 ? num / den : 0;
+        for (size_t k = 0; k < (size_t)W * H; k++) { double y = (double)Bl[k] - mean; Bl[k] = (float)(y > 0 ? y : 0); }
+    }
+    S->env_cond = (float*)malloc(sizeof(float) * (size_t)W * H);
+    S->env_marg = (float*)malloc(sizeof(float) * (size_t)H);
+    double* rowSum = (double*)malloc(sizeof(double) * (size_t)H);
+    double total = 0;
+    for (int j = 0; j < H; j++) {
+        double sinT = sin(M_PI * (j + 0.5) / H);
+        double sum = 0;
+        for (int i = 0; i < W; i++) { double y = Bl[(size_t)j * W + i]; sum += (y > 0 ? y : 0) * sinT; }
+        rowSum[j] = sum;
+        double c = 0;
+        for (int i = 0; i < W; i++) {
+            double y = Bl[(size_t)j * W + i]; c += (y > 0 ? y : 0) * sinT;
+            S->env_cond[(size_t)j * W + i] = (float)(sum > 0 ? c / sum : 0);
+        }
+        total += sum;
+    }
+    double acc = 0;
+    for (int j = 0; j < H; j++) { acc += rowSum[j]; S->env_marg[j] = (float)(total > 0 ? acc / total : 0); }
+    S->env_total = total;
+    free(Y); free(Bl); free(rowSum);
+}
+
+/* ---------------- lights (lights/quad/quad.glsl:14-41, features/lighting.ts:546-673) ---- */
+typedef struct { v3 wi; float distance; v3 radiance; float pdf; int flags; int light_id; } lsample_t;
+
+static lsample_t light_sample_one(const pt_light* l, v3 p, float x0, float x1) {
+    lsample_t ls;
+    if (l->kind == PT_LIGHT_QUAD) {
+        v3 q = add(add(lv(l->corner), muls(lv(l->edge1), x0)), muls(lv(l->edge2), x1));
+        v3 d = sub(q, p);
+        float d2 = dot(d, d);
+        ls.wi = muls(d, 1.0f / sqrtf(d2));
+        ls.distance = sqrtf(d2);
+        ls.radiance = lv(l->radiance);
+        ls.flags = 0; ls.light_id = -1;
+        float cos_l = dot(lv(l->normal), neg(ls.wi));
+        if (cos_l <= 0.0f) { ls.pdf = 0.0f; return ls; }
+        ls.pdf = d2 / (l->area * cos_l);
+    } else {   /* point light (lights/point/point.glsl:11-23) */
+        v3 d = sub(lv(l->corner), p);
+        float d2 = dot(d, d);
+        ls.wi = muls(d, 1.0f / sqrtf(d2));
+        ls.distance = sqrtf(d2);
+        ls.radiance = divs(lv(l->radiance), d2);
+        ls.pdf = 1.0f; ls.flags = 1; ls.light_id = -1;
+    }
+    return ls;
+}
+static float light_pdf_one(const pt_light* l, v3 p, v3 light_p, v3 wi) {
+    if (l->kind != PT_LIGHT_QUAD) return 0.0f;
+    float cos_l = dot(lv(l->normal), neg(wi));
+    if (cos_l <= 0.0f) return 0.0f;
+    v3 d = sub(light_p, p);
+    return dot(d, d) / (l->area * cos_l);
+}
+static lsample_t lighting_sample_finite(const or_scene* S, v3 p, float x0, float x1) {
+    lsample_t ls;
+    int n = S->n_lights;
+    if (n == 1) { ls = light_sample_one(&S->lights[0], p, x0, x1); ls.light_id = 0; return ls; }
+    int id = n - 1; float sel = S->lights[n - 1].select_pdf;
+    float lo = n >= 2 ? S->lights[n - 2].cdf : 0.0f;
+    for (int i = 0; i < n - 1; i++) {
+        if (x0 < S->lights[i].cdf) { id = i; sel = S->lights[i].select_pdf; lo = i == 0 ? 0.0f : S->lights[i - 1].cdf; break; }
+    }
+    float xr = (x0 - lo) / sel;
+    ls = light_sample_one(&S->lights[id], p, clampf(xr, 0.0f, 0.9999999f), x1);
+    ls.light_id = id;
+    ls.pdf *= sel;
+    return ls;
+}
+static int env_samplable(const or_scene* S) { return S->env.type == PT_ENV_SKY; }
+static int env_select_live(const or_scene* S) { return env_samplable(S) && S->n_lights > 0; }
+static lsample_t environment_sample(const or_scene* S, float x0, float x1) {
+    lsample_t ls;
+    float a = env_uniform_fraction(S);
+    if (x0 < a) {
+        float y = 1.0f - 2.0f * (x0 / a);
+        float r = sqrtf(fmaxf_(0.0f, 1.0f - y * y));
+        float phi = PTM_TWO_PI * x1;
+        ls.wi = V(r * COS(phi), y, r * SIN(phi));
+    } else {
+        ls.wi = env_sample_table(S, (x0 - a) / (1.0f - a), x1);
+    }
+    ls.distance = MAX_DIST;
+    ls.radiance = env_radiance(S, ls.wi);
+    ls.pdf = environment_pdf(S, ls.wi);
+    ls.flags = 0; ls.light_id = -1;
+    return ls;
+}
+static lsample_t lighting_sample(const or_scene* S, v3 p, float x0, float x1) {
+    lsample_t ls;
+    if (S->n_lights == 0) {
+        if (env_samplable(S)) return environment_sample(S, x0, x1);
+        ls.pdf = 0.0f; return ls;
+    }
+    if (!env_select_live(S)) return lighting_sample_finite(S, p, x0, x1);
+    float ps = S->env.select_prob;
+    if (x0 < ps) {
+        ls = environment_sample(S, clampf(x0 / ps, 0.0f, 0.9999999f), x1);
+        ls.pdf *= ps;
+        return ls;
+    }
+    float fx = clampf((x0 - ps) / (1.0f - ps), 0.0f, 0.9999999f);
+    ls = lighting_sample_finite(S, p, fx, x1);
+    ls.pdf *= (1.0f - ps);
+    return ls;
+}
+static int light_of(const or_scene* S, int region) {
+    if (region < 0) return -1;
+    for (int i = 0; i < S->n_lights; i++) if (S->lights[i].region == region) return i;
+    return -1;
+}
+static float lighting_pdf(const or_scene* S, v3 qp, v3 wi, int lid, v3 light_p) {
+    const pt_light* l = &S->lights[lid];
+    float sel = S->n_lights == 1 ? 1.0f : l->select_pdf;
+    if (env_select_live(S)) sel = sel * (1.0f - S->env.select_prob);
+    return sel * light_pdf_one(l, qp, light_p, wi);
+}
+
+/* ---------------- the walk (integrators/pt/pt.ts:299-464; combiner.ts; kernel.glsl) ---- */
+typedef struct {
+    int estimator, max_bounces, rr_start; float rr_max;
+    int width, height;
+} or_program;
+
+static inline float power_heuristic(float pf, float pg) {
+    float f2 = pf * pf;
+    if (f2 >= 1.0e38f) return 1.0f;
+    return f2 / fmaxf_(1e-20f, f2 + pg * pg);
+}
+static inline int is_black(v3 s) { return s.x <= 0.0f && s.y <= 0.0f && s.z <= 0.0f; }
+static inline float spectrum_max(v3 s) { float m = s.z; m = fmaxf_(s.y, m); m = fmaxf_(s.x, m); return m; }
+static inline v3 ray_spawn_origin(const hit_t* hit, v3 wi) {
+    float side = dot(wi, hit->ng) >= 0.0f ? 1.0f : -1.0f;
+    return add(hit->p, muls(muls(hit->ng, side), hit->eps));
+}
+
+/* One camera path; returns radiance. */
+static v3 transport_trace(const or_scene* S, const or_program* P, v3 ro, v3 rd, rng_t* rng, int* n_closest, int* n_any) {
+    v3 T = V(1.0f, 1.0f, 1.0f), L = V(0.0f, 0.0f, 0.0f);
+    int prev_was_delta = 1; float prev_bsdf_pdf = 0.0f; v3 prev_qp = ro;
+    float eta_scale = 1.0f;
+    int nee = P->estimator != PT_EST_PT, mis = P->estimator == PT_EST_MIS;
+    int emitters = S->n_lights > 0;
+    for (int bounce = 0; bounce <= P->max_bounces; bounce++) {
+        hit_t hit;
+        (*n_closest)++;
+        if (!scene_intersect(S, ro, rd, &hit)) {
+            /* kernel_score_miss + combiner_w_env */
+            float w = 1.0f;
+            if (env_samplable(S) && nee) {
+                if (prev_was_delta) w = 1.0f;
+                else if (mis) w = power_heuristic(prev_bsdf_pdf, env_select_live(S) ? S->env.select_prob * environment_pdf(S, rd) : environment_pdf(S, rd));
+                else w = 0.0f;
+            }
+            v3 Le = env_radiance(S, rd);
+            L = add(L, mul(muls(T, w), Le));
+            break;
+        }
+        int mat = material_of(S, hit.region_owner);
+        const pt_material* M = &S->materials[mat];
+        v3 wo = neg(rd);
+        /* kernel_score_emitter_hit */
+        int mat_emit = material_of(S, hit.region_to);
+        if (mat_emit >= 0 && S->materials[mat_emit].emissive) {
+            float w = 1.0f;
+            if (nee && emitters) {
+                int lid = light_of(S, hit.region_to);
+                if (lid < 0 || prev_was_delta) w = 1.0f;
+                else if (mis) w = power_heuristic(prev_bsdf_pdf, lighting_pdf(S, prev_qp, rd, lid, hit.p));
+                else w = 0.0f;
+            }
+            L = add(L, mul(muls(T, w), lv(S->materials[mat_emit].emission)));
+        }
+        if (bounce == P->max_bounces) break;
+        /* light_sample_direct (techniques/light/light.glsl:8-28) */
+        if (nee && has_nondelta(M)) {
+            float x0 = rnd(rng), x1 = rnd(rng);
+            lsample_t ls = lighting_sample(S, hit.p, x0, x1);
+            if (ls.pdf > 0.0f) {
+                v3 light_p = add(hit.p, muls(ls.wi, ls.distance));
+                v3 so = ray_spawn_origin(&hit, ls.wi);
+                v3 sd = normalize(sub(light_p, so));
+                float maxDist = length3(sub(light_p, so)) - SHADOW_BACKOFF;
+                (*n_any)++;
+                int blocked = scene_any(S, so, sd, maxDist);
+                if (!blocked) {
+                    v3 f = surface_eval(M, ls.wi, wo, &hit);
+                    float cos_i = fabsf(dot(ls.wi, hit.frame.n));
+                    float w = 1.0f;
+                    if (mis && !(ls.flags & 1)) w = power_heuristic(ls.pdf, surface_pdf(M, ls.wi, wo, &hit));
+                    v3 c = mul(mul(T, ls.radiance), f);
+                    c = muls(c, cos_i);
+                    /* vis = 1 */
+                    c = muls(c, w);
+                    c = divs(c, ls.pdf);
+                    L = add(L, c);
+                }
+            }
+        }
+        /* kernel_sample_continuation */
+        float uc = rnd(rng); float ux = rnd(rng); float uy = rnd(rng);
+        isample_t bs = surface_sample(S, M, wo, &hit, uc, ux, uy);
+        if (is_black(bs.weight)) break;
+        T = mul(T, bs.weight);
+        prev_was_delta = (bs.flags & LOBE_DELTA) != 0; prev_bsdf_pdf = bs.pdf; prev_qp = hit.p;
+        if (S->has_transmission && (bs.flags & LOBE_TRANSMISSION)) {
+            float r = ior_of(S, hit.region_to) / ior_of(S, hit.region_from);
+            eta_scale *= r * r;
+        }
+        /* roulette (pt.ts:204-222) */
+        if (P->rr_start >= 0 && bounce >= P->rr_start) {
+            float m = S->has_transmission ? spectrum_max(T) * eta_scale : spectrum_max(T);
+            float p = fminf_(P->rr_max, m);
+            if (p <= 0.0f) break;
+            if (rnd(rng) > p) break;
+            T = divs(T, p);
+        }
+        ro = ray_spawn_origin(&hit, bs.wi);
+        rd = bs.wi;
+    }
+    return L;
+}
+
+/* ---------------- ctypes API ---------------- */
+or_scene* or_scene_create(const pt_scene_desc* d) {
+    or_scene* S = (or_scene*)calloc(1, sizeof(or_scene));
+    S->n_tris = d->n_tris;
+    S->tri = (float*)malloc(sizeof(float) * 9 * (size_t)(d->n_tris > 0 ? d->n_tris : 1));
+    memcpy(S->tri, d->tri_vertices, sizeof(float) * 9 * (size_t)d->n_tris);
+    S->tri_object = (int32_t*)malloc(sizeof(int32_t) * (size_t)(d->n_tris > 0 ? d->n_tris : 1));
+    memcpy(S->tri_object, d->tri_object, sizeof(int32_t) * (size_t)d->n_tris);
+    S->n_objects = d->n_objects;
+    S->objects = (pt_object*)malloc(sizeof(pt_object) * (size_t)(d->n_objects > 0 ? d->n_objects : 1));
+    memcpy(S->objects, d->objects, sizeof(pt_object) * (size_t)d->n_objects);
+    S->n_materials = d->n_materials;
+    S->materials = (pt_material*)malloc(sizeof(pt_material) * (size_t)(d->n_materials > 0 ? d->n_materials : 1));
+    memcpy(S->materials, d->materials, sizeof(pt_material) * (size_t)d->n_materials);
+    S->n_lights = d->n_lights;
+    S->lights = (pt_light*)malloc(sizeof(pt_light) * (size_t)(d->n_lights > 0 ? d->n_lights : 1));
+    memcpy(S->lights, d->lights, sizeof(pt_light) * (size_t)d->n_lights);
+    S->env = d->env;
+    S->env_rot_cs[0] = (float)cos((double)d->env.rotation);
+    S->env_rot_cs[1] = (float)sin((double)d->env.rotation);
+    for (int m = 0; m < S->n_materials; m++) if (S->materials[m].model == PT_MAT_DIELECTRIC) S->has_transmission = 1;
+    /* per-mesh BLAS */
+    S->blas = (bvh_t*)calloc((size_t)(S->n_objects > 0 ? S->n_objects : 1), sizeof(bvh_t));
+    int nm = 0;
+    for (int o = 0; o < S->n_objects; o++) {
+        const pt_object* ob = &S->objects[o];
+        if (ob->kind != PT_OBJ_MESH) continue;
+        nm++;
+        int n = ob->tri_count;
+        double* boxes = (double*)malloc(sizeof(double) * 6 * (size_t)(n > 0 ? n : 1));
+        for (int i = 0; i < n; i++) {
+            const float* p = S->tri + 9 * (size_t)(ob->first_tri + i);
+            for (int a = 0; a < 3; a++) {
+                double mn = fmin(fmin(p[a], p[3 + a]), p[6 + a]), mx = fmax(fmax(p[a], p[3 + a]), p[6 + a]);
+                boxes[6 * i + a] = mn; boxes[6 * i + 3 + a] = mx;
+            }
+        }
+        bvh_build(&S->blas[o], boxes, n, 2);
+        free(boxes);
+    }
+    S->n_mesh_objects = nm;
+    double* tb = (double*)malloc(sizeof(double) * 6 * (size_t)(nm > 0 ? nm : 1));
+    S->tlas_obj = (int*)malloc(sizeof(int) * (size_t)(nm > 0 ? nm : 1));
+    int k = 0;
+    for (int o = 0; o < S->n_objects; o++) {
+        if (S->objects[o].kind != PT_OBJ_MESH || S->blas[o].n_nodes == 0) continue;
+        const float* n = S->blas[o].nodes;
+        for (int a = 0; a < 3; a++) { tb[6 * k + a] = n[a]; tb[6 * k + 3 + a] = n[4 + a]; }
+        S->tlas_obj[k++] = o;
+    }
+    bvh_build(&S->tlas, tb, k, 1);
+    free(tb);
+    if (S->env.type == PT_ENV_SKY) env_build(S);
+    return S;
+}
+void or_scene_destroy(or_scene* S) {
+    if (!S) return;
+    for (int o = 0; o < S->n_objects; o++) bvh_free(&S->blas[o]);
+    free(S->blas); bvh_free(&S->tlas); free(S->tlas_obj);
+    free(S->tri); free(S->tri_object); free(S->objects); free(S->materials); free(S->lights);
+    free(S->env_cond); free(S->env_marg);
+    free(S);
+}
+void or_env_table(const or_scene* S, float* cond, float* marg) {
+    if (S->env_cond) { memcpy(cond, S->env_cond, sizeof(float) * (size_t)S->env_w * S->env_h); memcpy(marg, S->env_marg, sizeof(float) * (size_t)S->env_h); }
+}
+double or_env_total(const or_scene* S) { return S->env_total; }
+
+void or_trace_closest(const or_scene* S, const float* rays, int n, float* t_out, int32_t* prim_out) {
+    /* rays: 7 arrays of n (ox, oy, oz, dx, dy, dz, tmax) */
+    #pragma omp parallel for schedule(dynamic, 256)
+    for (int i = 0; i < n; i++) {
+        v3 ro = V(rays[i], rays[n + i], rays[2 * n + i]);
+        v3 rd = V(rays[3 * n + i], rays[4 * n + i], rays[5 * n + i]);
+        float tmax = rays[6 * n + i];
+        int p = scene_closest(S, ro, rd, &tmax);
+        t_out[i] = p >= 0 ? tmax : rays[6 * n + i];
+        prim_out[i] = p;
+    }
+}
+void or_trace_any(const or_scene* S, const float* rays, int n, uint8_t* hit_out) {
+    #pragma omp parallel for schedule(dynamic, 256)
+    for (int i = 0; i < n; i++) {
+        v3 ro = V(rays[i], rays[n + i], rays[2 * n + i]);
+        v3 rd = V(rays[3 * n + i], rays[4 * n + i], rays[5 * n + i]);
+        hit_out[i] = (uint8_t)scene_any(S, ro, rd, rays[6 * n + i]);
+    }
+}
+/* Brute-force closest hit over every triangle (pins the BVH walks). */
+void or_trace_closest_brute(const or_scene* S, const float* rays, int n, float* t_out, int32_t* prim_out) {
+    #pragma omp parallel for schedule(dynamic, 16)
+    for (int i = 0; i < n; i++) {
+        v3 ro = V(rays[i], rays[n + i], rays[2 * n + i]);
+        v3 rd = V(rays[3 * n + i], rays[4 * n + i], rays[5 * n + i]);
+        float tmax = rays[6 * n + i]; int best = -1;
+        for (int tri = 0; tri < S->n_tris; tri++) {
+            v3 a, b, c, g; float t;
+            tri_load(S, tri, &a, &b, &c);
+            if (tri_test(ro, rd, a, b, c, &t, &g) && t > 0.0f && (t < tmax || (t == tmax && tri < best))) { tmax = t; best = tri; }
+        }
+        for (int o = 0; o < S->n_objects; o++) {
+            const pt_object* ob = &S->objects[o]; float t; int ok = 0;
+            if (ob->kind == PT_OBJ_QUAD) ok = quad_intersect(ro, rd, ob->p, &t);
+            else if (ob->kind == PT_OBJ_SPHERE) ok = sphere_intersect(ro, rd, ld3(ob->p), ob->p[3], &t);
+            if (ok && t < tmax) { tmax = t; best = S->n_tris + o; }
+        }
+        t_out[i] = tmax; prim_out[i] = best;
+    }
+}
+int or_region_at(const or_scene* S, const float* p) { return region_at(S, V(p[0], p[1], p[2])); }
+
+/* Render nframes frames starting at sample_count into film (RGBA32F, row 0 = bottom),
+ * following features/accumulation.ts:14-51 and accumulator/average/average.glsl:6-10.
+ * Optional per-sample radiance output `samples` [nframes][H][W][3] (may be NULL).
+ * Rows [y0, y1) only (a bounded sample); counts of closest/any queries are returned. */
+void or_render(const or_scene* S, const pt_program* prog, const pt_params* par, uint32_t sample_count,
+               int nframes, int y0, int y1, float* film, float* samples, uint64_t* counts) {
+    or_program P;
+    P.estimator = prog->estimator; P.max_bounces = prog->max_bounces; P.rr_start = prog->rr_start;
+    P.rr_max = prog->rr_max_survival; P.width = prog->width; P.height = prog->height;
+    int W = P.width, H = P.height;
+    uint64_t nc = 0, na = 0;
+    v3 F = lv(par->cam_forward), R = lv(par->cam_right), U = lv(par->cam_up), O = lv(par->cam_position);
+    #pragma omp parallel for schedule(dynamic, 1) reduction(+:nc, na)
+    for (int y = y0; y < y1; y++) {
+        for (int x = 0; x < W; x++) {
+            int c1 = 0, c2 = 0;
+            float* px = film + 4 * ((size_t)y * W + x);
+            for (int f = 0; f < nframes; f++) {
+                uint32_t n = sample_count + (uint32_t)f;
+                float fx = (float)x + 0.5f + (float)par->pixel_offset[0];
+                float fy = (float)y + 0.5f + (float)par->pixel_offset[1];
+                rng_t rng; rng_init(&rng, (uint32_t)fx, (uint32_t)fy, n, par->reset_salt);
+                float j0 = rnd(&rng), j1 = rnd(&rng);
+                float filmx = fx + (j0 - 0.5f), filmy = fy + (j1 - 0.5f);
+                (void)rnd(&rng); (void)rnd(&rng);   /* lens sample, consumed by every camera */
+                float ndcx = (2.0f * filmx / (float)W) - 1.0f;
+                float ndcy = (2.0f * filmy / (float)H) - 1.0f;
+                ndcx *= par->aspect;
+                v3 dir = normalize(add(add(F, muls(R, ndcx * par->tan_fov)), muls(U, ndcy * par->tan_fov)));
+                v3 Lr = transport_trace(S, &P, O, dir, &rng, &c1, &c2);
+                if (samples) {
+                    float* sp = samples + 3 * (((size_t)f * H + y) * W + x);
+                    sp[0] = Lr.x; sp[1] = Lr.y; sp[2] = Lr.z;
+                }
+                if (n == 0) { px[0] = Lr.x; px[1] = Lr.y; px[2] = Lr.z; }
+                else {
+                    float w = 1.0f / ((float)n + 1.0f);
+                    px[0] = px[0] * (1.0f - w) + Lr.x * w;
+                    px[1] = px[1] * (1.0f - w) + Lr.y * w;
+                    px[2] = px[2] * (1.0f - w) + Lr.z * w;
+                }
+                px[3] = 1.0f;
+            }
+            nc += (uint64_t)c1; na += (uint64_t)c2;
+        }
+    }
+    if (counts) { counts[0] = nc; counts[1] = na; }
+}
+
+/* Declared-arithmetic exports, so tests can pin them against correctly rounded math. */
+void or_math_eval(int fn, const float* x, const float* y, float* out, int n) {
+    for (int i = 0; i < n; i++) {
+        switch (fn) {
+            case 0: out[i] = ptm_sin(x[i]); break;
+            case 1: out[i] = ptm_cos(x[i]); break;
+            case 2: out[i] = ptm_exp(x[i]); break;
+            case 3: out[i] = ptm_log(x[i]); break;
+            case 4: out[i] = ptm_atan2(y[i], x[i]); break;
+            case 5: out[i] = ptm_tan(x[i]); break;
+            case 6: out[i] = ptm_pow(x[i], y[i]); break;
+            case 7: out[i] = ptm_atan(x[i]); break;
+            case 8: out[i] = ptm_acos(x[i]); break;
+            case 9: out[i] = ptm_tanh(x[i]); break;
+            case 10: out[i] = ptm_atanh(x[i]); break;
+            default: out[i] = NAN;
+        }
+    }
+}

@@ -1,0 +1,68 @@
+"""ctypes binding of libptmi.so (the in-tree HIP build). Fails loudly when it is missing:
+there is no CPU fallback for any entry point."""
+import ctypes as C
+import os
+
+from . import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libptmi.so")
+
+# every symbol declared in include/ptmi.h
+DECLARED = ["pt_last_error", "pt_ctx_create", "pt_ctx_destroy", "pt_scene_upload", "pt_renderer_load",
+            "pt_renderer_unload", "pt_render_frames", "pt_read_export", "pt_clear_accumulation",
+            "pt_sample_count", "pt_trace_closest", "pt_trace_any", "pt_trace_closest_device",
+            "pt_trace_any_device", "pt_set_timing", "pt_get_timings", "pt_last_ray_counts"]
+
+_lib = None
+
+
+class PtError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"ptmi status {status}: {msg}")
+        self.status = status
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"libptmi.so not built ({LIB_PATH}); run __graft_entry__.build()")
+    lib = C.CDLL(LIB_PATH)
+    vp, i32, u32, st = C.c_void_p, C.c_int32, C.c_uint32, C.c_int
+    lib.pt_last_error.restype = C.c_char_p
+    lib.pt_ctx_create.argtypes = [C.c_int, C.POINTER(vp)]
+    lib.pt_ctx_destroy.argtypes = [vp]
+    lib.pt_ctx_destroy.restype = None
+    lib.pt_scene_upload.argtypes = [vp, C.POINTER(abi.PtSceneDesc)]
+    lib.pt_renderer_load.argtypes = [vp, C.POINTER(abi.PtProgram), C.POINTER(vp)]
+    lib.pt_renderer_unload.argtypes = [vp]
+    lib.pt_renderer_unload.restype = None
+    lib.pt_render_frames.argtypes = [vp, C.POINTER(abi.PtParams), u32]
+    lib.pt_read_export.argtypes = [vp, C.c_char_p, vp, C.c_size_t]
+    lib.pt_clear_accumulation.argtypes = [vp]
+    lib.pt_sample_count.argtypes = [vp]
+    lib.pt_sample_count.restype = u32
+    lib.pt_trace_closest.argtypes = [vp, C.POINTER(abi.PtRaysSoa), u32, vp, vp]
+    lib.pt_trace_any.argtypes = [vp, C.POINTER(abi.PtRaysSoa), u32, vp]
+    lib.pt_trace_closest_device.argtypes = [vp, vp, u32, vp, vp]
+    lib.pt_trace_any_device.argtypes = [vp, vp, u32, vp]
+    lib.pt_set_timing.argtypes = [vp, C.c_int]
+    lib.pt_get_timings.argtypes = [vp, vp, C.c_int]
+    lib.pt_get_timings.restype = C.c_int
+    lib.pt_last_ray_counts.argtypes = [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+    lib.pt_renderer_set_traversal_counting.argtypes = [vp, C.c_int]
+    lib.pt_last_traversal_counts.argtypes = [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+    lib.pt_trace_closest_device_counted.argtypes = [vp, vp, u32, vp, vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+    lib.pt_scene_stats.argtypes = [vp, C.POINTER(u32), C.POINTER(u32), C.POINTER(u32)]
+    lib.pt_abi_sizeof.argtypes = [C.c_int]
+    lib.pt_abi_sizeof.restype = C.c_size_t
+    _ = (i32, st)
+    _lib = lib
+    return lib
+
+
+def check(status):
+    if status != 0:
+        raise PtError(status, load().pt_last_error().decode(errors="replace"))

@@ -1,0 +1,120 @@
+"""ctypes / numpy mirrors of the C ABI records declared in include/ptmi.h.
+
+Field order and sizes must match the header exactly; tests/test_abi.py checks the sizes
+against the compiled library's own sizeof() export.
+"""
+import ctypes as C
+
+import numpy as np
+
+ABI_VERSION = 1
+
+OBJ_MESH, OBJ_QUAD, OBJ_SPHERE = 0, 1, 2
+MAT_LAMBERT, MAT_DIELECTRIC, MAT_GGX = 0, 1, 2
+LIGHT_QUAD, LIGHT_POINT = 0, 1
+ENV_NONE, ENV_SKY = 0, 1
+EST_PT, EST_NEE, EST_MIS = 0, 1, 2
+
+OBJECT_DTYPE = np.dtype([
+    ("kind", "<i4"), ("material", "<i4"), ("closed", "<i4"), ("first_tri", "<i4"),
+    ("tri_count", "<i4"), ("pad", "<i4", 3),
+    ("p", "<f4", 12), ("box_min", "<f4", 4), ("box_max", "<f4", 4),
+])
+MATERIAL_DTYPE = np.dtype([
+    ("model", "<i4"), ("emissive", "<i4"), ("has_medium", "<i4"), ("pad0", "<i4"),
+    ("albedo", "<f4", 4), ("emission", "<f4", 4), ("f0", "<f4", 4), ("transmittance", "<f4", 4),
+    ("alpha", "<f4"), ("ior", "<f4"), ("hg_g", "<f4"), ("pad1", "<f4"),
+    ("sigma_s", "<f4", 4), ("sigma_a", "<f4", 4),
+])
+LIGHT_DTYPE = np.dtype([
+    ("kind", "<i4"), ("region", "<i4"), ("select_pdf", "<f4"), ("cdf", "<f4"),
+    ("corner", "<f4", 4), ("edge1", "<f4", 4), ("edge2", "<f4", 4), ("normal", "<f4", 4),
+    ("radiance", "<f4", 4), ("area", "<f4"), ("pad", "<f4", 3),
+])
+assert OBJECT_DTYPE.itemsize == 112 and MATERIAL_DTYPE.itemsize == 128 and LIGHT_DTYPE.itemsize == 112
+
+
+class PtEnv(C.Structure):
+    _fields_ = [("type", C.c_int32), ("table_w", C.c_int32), ("table_h", C.c_int32),
+                ("compensated", C.c_int32), ("intensity", C.c_float), ("rotation", C.c_float),
+                ("select_prob", C.c_float), ("pad", C.c_float)]
+
+
+class PtSceneDesc(C.Structure):
+    _fields_ = [("abi_version", C.c_uint32), ("n_tris", C.c_int32),
+                ("tri_vertices", C.c_void_p), ("tri_object", C.c_void_p),
+                ("n_objects", C.c_int32), ("objects", C.c_void_p),
+                ("n_materials", C.c_int32), ("materials", C.c_void_p),
+                ("n_lights", C.c_int32), ("lights", C.c_void_p),
+                ("env", PtEnv)]
+
+
+class PtProgram(C.Structure):
+    _fields_ = [("abi_version", C.c_uint32), ("estimator", C.c_int32), ("max_bounces", C.c_int32),
+                ("rr_start", C.c_int32), ("rr_max_survival", C.c_float), ("color", C.c_int32),
+                ("width", C.c_int32), ("height", C.c_int32), ("frames_in_flight", C.c_int32),
+                ("pad", C.c_int32 * 3)]
+
+
+class PtParams(C.Structure):
+    _fields_ = [("cam_position", C.c_float * 4), ("cam_forward", C.c_float * 4),
+                ("cam_right", C.c_float * 4), ("cam_up", C.c_float * 4),
+                ("tan_fov", C.c_float), ("aspect", C.c_float), ("reset_salt", C.c_uint32),
+                ("pad0", C.c_uint32), ("pixel_offset", C.c_int32 * 2), ("pad1", C.c_int32 * 2)]
+
+
+class PtKernelTiming(C.Structure):
+    _fields_ = [("name", C.c_char * 32), ("total_ms", C.c_float), ("launches", C.c_uint32),
+                ("items", C.c_uint64)]
+
+
+class PtRaysSoa(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("ox", "oy", "oz", "dx", "dy", "dz", "tmax")]
+
+
+def _ptr(a):
+    return a.ctypes.data if a is not None and a.size else None
+
+
+class SceneArrays:
+    """Host-side scene in the ABI layout; keeps the numpy arrays alive for the desc."""
+
+    def __init__(self, tri_vertices, tri_object, objects, materials, lights, env=None):
+        self.tri_vertices = np.ascontiguousarray(tri_vertices, dtype=np.float32).reshape(-1, 9)
+        self.tri_object = np.ascontiguousarray(tri_object, dtype=np.int32).reshape(-1)
+        self.objects = np.ascontiguousarray(objects, dtype=OBJECT_DTYPE)
+        self.materials = np.ascontiguousarray(materials, dtype=MATERIAL_DTYPE)
+        self.lights = np.ascontiguousarray(lights, dtype=LIGHT_DTYPE)
+        self.env = env if env is not None else PtEnv(ENV_NONE, 512, 256, 0, 1.0, 0.0, 0.5, 0.0)
+        assert self.tri_object.shape[0] == self.tri_vertices.shape[0]
+
+    @property
+    def n_tris(self):
+        return int(self.tri_vertices.shape[0])
+
+    def desc(self):
+        d = PtSceneDesc()
+        d.abi_version = ABI_VERSION
+        d.n_tris = self.n_tris
+        d.tri_vertices = _ptr(self.tri_vertices)
+        d.tri_object = _ptr(self.tri_object)
+        d.n_objects = len(self.objects)
+        d.objects = _ptr(self.objects)
+        d.n_materials = len(self.materials)
+        d.materials = _ptr(self.materials)
+        d.n_lights = len(self.lights)
+        d.lights = _ptr(self.lights)
+        d.env = self.env
+        return d
+
+
+def rays_soa_array(o, d, tmax):
+    """Pack rays into the 7 x n float32 block used by the trace entry points."""
+    o = np.asarray(o, np.float32)
+    d = np.asarray(d, np.float32)
+    n = o.shape[0]
+    out = np.empty((7, n), np.float32)
+    out[0:3] = o.T
+    out[3:6] = d.T
+    out[6] = np.broadcast_to(np.asarray(tmax, np.float32), (n,))
+    return np.ascontiguousarray(out)

@@ -1,0 +1,230 @@
+"""Host compiler: SceneDescription + RenderStrategy -> CompiledRenderer for the MI355X backend.
+
+Mirrors the reference pipeline Analyze -> Validate -> Plan (src/compiler/Compiler.ts:14-72,
+src/compiler/analyze/Analyzer.ts:10, src/compiler/analyze/Validator.ts:50,
+src/compiler/plan/Planner.ts:72). The Generate stage is replaced: instead of emitting GLSL, the
+plan is lowered to a pt_program record (template selection on the device side) plus the scene
+arrays of the data rail and the host-derived constants the generator would bake as literals
+(camera basis, tan(fov/2), quad normals/areas, light selection pdfs — all f64 on the host,
+rounded once to f32, as the reference's formatFloat literals are).
+"""
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import abi
+
+ESTIMATORS = {"none": abi.EST_PT, "pt": abi.EST_PT, "nee": abi.EST_NEE, "mis": abi.EST_MIS}
+MODELS = {"lambert": abi.MAT_LAMBERT, "dielectric": abi.MAT_DIELECTRIC, "ggx": abi.MAT_GGX}
+
+
+class CompileError(ValueError):
+    """Validation failure (the reference's DiagnosticBag errors, src/errors/core/DiagnosticBag.ts)."""
+
+
+@dataclass
+class RenderStrategy:
+    """measurement/estimator knobs (src/compiler/types.ts:561-721)."""
+    direct_lighting: str = "mis"          # 'none' | 'nee' | 'mis'
+    max_bounces: int = 8
+    rr_start: int = 3                     # russianRoulette.startDepth (-1 = off)
+    rr_max_survival: float = 0.95
+    frames_in_flight: int = 1
+
+
+@dataclass
+class CompiledRenderer:
+    id: str
+    scene: "abi.SceneArrays"
+    program: "abi.PtProgram"
+    width: int
+    height: int
+    camera: dict
+    reset_salt: int = 1234
+    warnings: list = field(default_factory=list)
+
+    def params(self, reset_salt=None):
+        return camera_params(self.camera, self.width, self.height,
+                             self.reset_salt if reset_salt is None else reset_salt)
+
+
+def camera_basis(position, target):
+    """cameraBasis (src/components/camera/basis.ts:50-56), in f64."""
+    p = np.asarray(position, np.float64)
+    t = np.asarray(target, np.float64)
+    f = t - p
+    f = f / math.hypot(*f)
+    up_ref = np.array([0.0, 0.0, 1.0]) if abs(f[1]) > 0.999999 else np.array([0.0, 1.0, 0.0])
+    r = np.cross(f, up_ref)
+    r = r / math.hypot(*r)
+    u = np.cross(r, f)
+    return f, r, u
+
+
+def camera_params(camera, width, height, reset_salt=1234):
+    f, r, u = camera_basis(camera["position"], camera["target"])
+    prm = abi.PtParams()
+    for dst, src in ((prm.cam_position, camera["position"]), (prm.cam_forward, f),
+                     (prm.cam_right, r), (prm.cam_up, u)):
+        for i in range(3):
+            dst[i] = float(np.float32(src[i]))
+    prm.tan_fov = float(np.float32(math.tan(camera["fov"] / 2.0)))   # features/camera.ts:54-66
+    prm.aspect = float(np.float32(width / height))
+    prm.reset_salt = reset_salt
+    return prm
+
+
+def _analyze_validate(scene, strategy):
+    if strategy.direct_lighting not in ESTIMATORS:
+        raise CompileError(f"unknown directLighting '{strategy.direct_lighting}'")
+    if strategy.max_bounces < 0:
+        raise CompileError("maxBounces must be >= 0")
+    if not (0.0 < strategy.rr_max_survival <= 1.0):
+        raise CompileError("russianRoulette.maxSurvival must be in (0, 1]")
+    mats = scene.get("materials", {})
+    for name, m in mats.items():
+        if m.get("model") not in MODELS:
+            raise CompileError(f"material '{name}': model '{m.get('model')}' is not supported by this backend")
+    for o in scene.get("objects", []):
+        if o.get("material") not in mats:
+            raise CompileError(f"object references unknown material '{o.get('material')}'")
+        if o["type"] not in ("mesh", "sphere", "quad"):
+            raise CompileError(f"object type '{o['type']}' is not supported by this backend")
+    env = scene.get("environment", {"type": "none"})
+    if env.get("type") not in ("none", "procedural"):
+        raise CompileError(f"environment '{env.get('type')}' is not supported by this backend")
+    for l in scene.get("lights", []):
+        if l["type"] == "quad":
+            area = np.linalg.norm(np.cross(l["edge1"], l["edge2"]))
+            if area < 1e-8:
+                raise CompileError("quad edges are parallel or near-parallel — area must be >= 1e-8")
+        elif l["type"] != "point":
+            raise CompileError(f"light '{l['type']}' is not supported by this backend")
+
+
+def compile_scene(scene, strategy=None, width=256, height=256, reset_salt=1234):
+    """Compile one (scene, strategy) pair into a CompiledRenderer."""
+    strategy = strategy or RenderStrategy()
+    _analyze_validate(scene, strategy)
+
+    mat_names = list(scene.get("materials", {}).keys())
+    mat_index = {n: i for i, n in enumerate(mat_names)}
+    materials = []
+    for n in mat_names:
+        m = scene["materials"][n]
+        rec = np.zeros(1, abi.MATERIAL_DTYPE)[0]
+        rec["model"] = MODELS[m["model"]]
+        rec["albedo"][:3] = m.get("albedo", [0.8, 0.8, 0.8])
+        em = m.get("emission", [0.0, 0.0, 0.0])
+        rec["emission"][:3] = em
+        rec["emissive"] = int(any(e > 0 for e in em))
+        rec["f0"][:3] = m.get("f0", [0.9, 0.9, 0.9])
+        rough = float(m.get("roughness", 0.5))
+        rec["alpha"] = max(1e-3, rough * rough)                  # ggx.ts derived alpha
+        rec["transmittance"][:3] = m.get("transmittance", [1.0, 1.0, 1.0])
+        rec["ior"] = float(m.get("ior", 1.5))
+        materials.append(rec)
+
+    tri_chunks, tri_obj_chunks, objects = [], [], []
+    n_tris = 0
+
+    def add_object(kind, material, closed=0, tris=None, p=None):
+        nonlocal n_tris
+        rec = np.zeros(1, abi.OBJECT_DTYPE)[0]
+        rec["kind"] = kind
+        rec["material"] = material
+        rec["closed"] = int(closed)
+        if tris is not None:
+            t32 = np.asarray(tris, np.float64).reshape(-1, 9).astype(np.float32)
+            rec["first_tri"] = n_tris
+            rec["tri_count"] = t32.shape[0]
+            v = t32.reshape(-1, 3)
+            rec["box_min"][:3] = v.min(axis=0)
+            rec["box_max"][:3] = v.max(axis=0)
+            tri_chunks.append(t32)
+            tri_obj_chunks.append(np.full(t32.shape[0], len(objects), np.int32))
+            n_tris += t32.shape[0]
+        if p is not None:
+            rec["p"][:len(p)] = p
+        objects.append(rec)
+        return len(objects) - 1
+
+    for o in scene.get("objects", []):
+        mi = mat_index[o["material"]]
+        if o["type"] == "mesh":
+            add_object(abi.OBJ_MESH, mi, o.get("closed", False), tris=o["triangles"])
+        elif o["type"] == "sphere":
+            add_object(abi.OBJ_SPHERE, mi, 1, p=list(o["center"]) + [o["radius"]])
+        elif o["type"] == "quad":
+            e1, e2 = np.asarray(o["edge1"], np.float64), np.asarray(o["edge2"], np.float64)
+            cr = np.cross(e1, e2)
+            add_object(abi.OBJ_QUAD, mi, 0, p=list(o["corner"]) + list(e1) + list(e2) + list(cr / np.linalg.norm(cr)))
+
+    # Lights: an area light is a thin emissive quad region plus a light record
+    # (lights/quad/quad.ts: region + derived normal/area; lighting.ts power CDF).
+    lights = []
+    for l in scene.get("lights", []):
+        rec = np.zeros(1, abi.LIGHT_DTYPE)[0]
+        rad = np.asarray(l.get("radiance", l.get("intensity", [1, 1, 1])), np.float64)
+        rec["radiance"][:3] = rad
+        if l["type"] == "quad":
+            e1, e2 = np.asarray(l["edge1"], np.float64), np.asarray(l["edge2"], np.float64)
+            cr = np.cross(e1, e2)
+            area = float(np.linalg.norm(cr))
+            nrm = cr / area
+            em = np.zeros(1, abi.MATERIAL_DTYPE)[0]
+            em["model"] = abi.MAT_LAMBERT
+            em["emission"][:3] = rad
+            em["emissive"] = 1
+            materials.append(em)
+            region = add_object(abi.OBJ_QUAD, len(materials) - 1, 0,
+                                p=list(l["corner"]) + list(e1) + list(e2) + list(nrm))
+            rec["kind"] = abi.LIGHT_QUAD
+            rec["region"] = region
+            rec["corner"][:3] = l["corner"]
+            rec["edge1"][:3] = e1
+            rec["edge2"][:3] = e2
+            rec["normal"][:3] = nrm
+            rec["area"] = area
+            power = max(1e-8, math.pi * area * float(rad.mean()))
+        else:
+            rec["kind"] = abi.LIGHT_POINT
+            rec["region"] = -1
+            rec["corner"][:3] = l["position"]
+            power = 4.0 * math.pi * float(rad.mean())
+        lights.append((rec, power))
+    total = sum(p for _, p in lights)
+    acc = 0.0
+    light_recs = []
+    for rec, p in lights:
+        acc += p
+        rec["select_pdf"] = p / total
+        rec["cdf"] = acc / total
+        light_recs.append(rec)
+
+    env = abi.PtEnv(abi.ENV_NONE, 512, 256, 0, 1.0, 0.0, 0.5, 0.0)
+    e = scene.get("environment", {"type": "none"})
+    if e.get("type") == "procedural":
+        env = abi.PtEnv(abi.ENV_SKY, 512, 256, int(e.get("compensated", False)),
+                        float(e.get("intensity", 1.0)), float(e.get("rotation", 0.0)),
+                        float(e.get("select_prob", 0.5)), 0.0)
+
+    tv = np.concatenate(tri_chunks) if tri_chunks else np.zeros((0, 9), np.float32)
+    to = np.concatenate(tri_obj_chunks) if tri_obj_chunks else np.zeros(0, np.int32)
+    arrays = abi.SceneArrays(tv, to, np.array(objects, abi.OBJECT_DTYPE),
+                             np.array(materials, abi.MATERIAL_DTYPE),
+                             np.array(light_recs, abi.LIGHT_DTYPE), env)
+
+    prog = abi.PtProgram()
+    prog.abi_version = abi.ABI_VERSION
+    prog.estimator = ESTIMATORS[strategy.direct_lighting]
+    prog.max_bounces = strategy.max_bounces
+    prog.rr_start = strategy.rr_start
+    prog.rr_max_survival = strategy.rr_max_survival
+    prog.color = 0
+    prog.width, prog.height = width, height
+    prog.frames_in_flight = max(1, strategy.frames_in_flight)
+    return CompiledRenderer(id=f"{strategy.direct_lighting}-{scene.get('id', 'scene')}", scene=arrays,
+                            program=prog, width=width, height=height, camera=scene["camera"],
+                            reset_salt=reset_salt)

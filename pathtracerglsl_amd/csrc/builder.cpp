@@ -1,0 +1,466 @@
+// builder.cpp — wide BVH, binary BVH and env-table builders (host, C++17).
+#include "builder.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+
+#include "../../include/ptmi_math.h"
+
+namespace ptmi {
+
+namespace {
+
+struct Box {
+    float lo[3], hi[3];
+    void reset() { lo[0] = lo[1] = lo[2] = INFINITY; hi[0] = hi[1] = hi[2] = -INFINITY; }
+    void grow(const Box& b) {
+        for (int a = 0; a < 3; a++) { lo[a] = std::min(lo[a], b.lo[a]); hi[a] = std::max(hi[a], b.hi[a]); }
+    }
+    void grow(const float* p) {
+        for (int a = 0; a < 3; a++) { lo[a] = std::min(lo[a], p[a]); hi[a] = std::max(hi[a], p[a]); }
+    }
+    double area() const {
+        double dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+        if (dx < 0 || dy < 0 || dz < 0) return 0.0;
+        return 2.0 * (dx * dy + dy * dz + dz * dx);
+    }
+};
+
+// ---------------------------------------------------------------- binary SAH (leaf = 1 tri)
+struct BNode {
+    Box box;
+    int left = -1, right = -1;   // internal
+    int first = 0, count = 0;    // subtree triangle range in `order`
+};
+
+struct SahBuilder {
+    const Box* tri_box;
+    std::vector<float> cen;      // 3 per item
+    std::vector<uint32_t> order;
+    std::vector<BNode> nodes;
+
+    int build(int s, int e) {
+        int id = (int)nodes.size();
+        nodes.emplace_back();
+        Box b, cb; b.reset(); cb.reset();
+        for (int i = s; i < e; i++) {
+            b.grow(tri_box[order[i]]);
+            cb.grow(&cen[3 * (size_t)order[i]]);
+        }
+        nodes[id].box = b;
+        nodes[id].first = s;
+        nodes[id].count = e - s;
+        if (e - s <= 1) return id;
+        constexpr int NB = 16;
+        int best_axis = -1, best_split = -1;
+        double best_cost = INFINITY;
+        for (int a = 0; a < 3; a++) {
+            float ext = cb.hi[a] - cb.lo[a];
+            if (!(ext > 0.0f)) continue;
+            double scale = NB / (double)ext;
+            Box bb[NB]; int bc[NB];
+            for (int k = 0; k < NB; k++) { bb[k].reset(); bc[k] = 0; }
+            for (int i = s; i < e; i++) {
+                uint32_t t = order[i];
+                int k = (int)((cen[3 * (size_t)t + a] - cb.lo[a]) * scale);
+                k = std::min(std::max(k, 0), NB - 1);
+                bb[k].grow(tri_box[t]); bc[k]++;
+            }
+            double la[NB - 1]; int lc[NB - 1];
+            Box acc; acc.reset(); int n = 0;
+            for (int k = 0; k < NB - 1; k++) { acc.grow(bb[k]); n += bc[k]; la[k] = acc.area(); lc[k] = n; }
+            acc.reset(); n = 0;
+            for (int k = NB - 1; k > 0; k--) {
+                acc.grow(bb[k]); n += bc[k];
+                if (lc[k - 1] == 0 || n == 0) continue;
+                double cost = la[k - 1] * lc[k - 1] + acc.area() * n;
+                if (cost < best_cost) { best_cost = cost; best_axis = a; best_split = k - 1; }
+            }
+        }
+        int mid = -1;
+        if (best_axis >= 0) {
+            float lo = cb.lo[best_axis];
+            double scale = NB / (double)(cb.hi[best_axis] - lo);
+            auto it = std::partition(order.begin() + s, order.begin() + e, [&](uint32_t t) {
+                int k = (int)((cen[3 * (size_t)t + best_axis] - lo) * scale);
+                k = std::min(std::max(k, 0), NB - 1);
+                return k <= best_split;
+            });
+            mid = (int)(it - order.begin());
+            if (mid <= s || mid >= e) mid = -1;
+        }
+        if (mid < 0) mid = (s + e) / 2;
+        int l = build(s, mid);
+        int r = build(mid, e);
+        nodes[id].left = l;
+        nodes[id].right = r;
+        return id;
+    }
+};
+
+inline float decode(float p, int q, float s) { return p + (float)q * s; }
+
+// Quantize [clo, chi] on one axis against origin p and scale s; false if 255 is exceeded.
+bool quantize_axis(float p, float s, float clo, float chi, int& qlo, int& qhi) {
+    qlo = (int)std::floor((clo - p) / s);
+    qhi = (int)std::ceil((chi - p) / s);
+    qlo = std::max(qlo, 0);
+    while (qlo > 0 && decode(p, qlo, s) > clo) qlo--;
+    while (qlo < 255 && decode(p, qlo + 1, s) <= clo) qlo++;
+    if (decode(p, qlo, s) > clo) return false;
+    qhi = std::max(qhi, qlo);
+    while (qhi <= 255 && decode(p, qhi, s) < chi) qhi++;
+    while (qhi > qlo && decode(p, qhi - 1, s) >= chi) qhi--;
+    return qhi <= 255;
+}
+
+}  // namespace
+
+void build_wide_bvh(const float* tv, int n_tris, WideBVH& out) {
+    out = WideBVH();
+    if (n_tris <= 0) return;
+    std::vector<Box> tri_box((size_t)n_tris);
+    SahBuilder sb;
+    sb.tri_box = tri_box.data();
+    sb.cen.resize(3 * (size_t)n_tris);
+    sb.order.resize((size_t)n_tris);
+    for (int t = 0; t < n_tris; t++) {
+        Box b; b.reset();
+        for (int v = 0; v < 3; v++) b.grow(tv + 9 * (size_t)t + 3 * v);
+        tri_box[t] = b;
+        for (int a = 0; a < 3; a++) sb.cen[3 * (size_t)t + a] = 0.5f * (b.lo[a] + b.hi[a]);
+        sb.order[t] = (uint32_t)t;
+    }
+    sb.nodes.reserve(2 * (size_t)n_tris);
+    sb.build(0, n_tris);
+    const std::vector<BNode>& bn = sb.nodes;
+
+    // Conservative child box: the subtree box widened by 4e-6 of its largest extent, so the
+    // Moller-Trumbore edge tolerance (1e-6 barycentric) cannot place a hit outside it.
+    auto padded = [&](int b) {
+        Box x = bn[b].box;
+        float ext = std::max(x.hi[0] - x.lo[0], std::max(x.hi[1] - x.lo[1], x.hi[2] - x.lo[2]));
+        float m = 4e-6f * ext + 1e-30f;
+        for (int a = 0; a < 3; a++) {
+            x.lo[a] = x.lo[a] - std::max(m, 4e-7f * std::fabs(x.lo[a]));
+            x.hi[a] = x.hi[a] + std::max(m, 4e-7f * std::fabs(x.hi[a]));
+        }
+        return x;
+    };
+
+    struct Job { int bnode; uint32_t wide; uint32_t depth; };
+    std::vector<Job> stack;
+    out.nodes.assign(20, 0u);
+    out.n_nodes = 1;
+    stack.push_back({0, 0, 1});
+    out.tris.reserve(12 * (size_t)n_tris);
+    while (!stack.empty()) {
+        Job job = stack.back(); stack.pop_back();
+        out.max_depth = std::max(out.max_depth, job.depth);
+        // collect up to 8 children by greedy largest-area expansion
+        std::vector<int> ch;
+        if (bn[job.bnode].left < 0) ch.push_back(job.bnode);
+        else { ch.push_back(bn[job.bnode].left); ch.push_back(bn[job.bnode].right); }
+        while (ch.size() < 8) {
+            int best = -1; double ba = -1.0;
+            for (size_t i = 0; i < ch.size(); i++) {
+                const BNode& c = bn[ch[i]];
+                if (c.left < 0) continue;
+                double a = c.box.area();
+                if (a > ba) { ba = a; best = (int)i; }
+            }
+            if (best < 0) break;
+            int b = ch[best];
+            ch[best] = bn[b].left;
+            ch.push_back(bn[b].right);
+        }
+        // slot assignment by centroid octant
+        Box nb; nb.reset();
+        std::vector<Box> cbox(ch.size());
+        for (size_t i = 0; i < ch.size(); i++) { cbox[i] = padded(ch[i]); nb.grow(cbox[i]); }
+        float pc[3];
+        for (int a = 0; a < 3; a++) pc[a] = 0.5f * (nb.lo[a] + nb.hi[a]);
+        struct Cand { float score; int c, s; };
+        std::vector<Cand> cand;
+        for (size_t i = 0; i < ch.size(); i++)
+            for (int s = 0; s < 8; s++) {
+                float sc = 0.0f;
+                for (int a = 0; a < 3; a++) {
+                    float d = 0.5f * (cbox[i].lo[a] + cbox[i].hi[a]) - pc[a];
+                    sc += ((s >> a) & 1) ? -d : d;
+                }
+                cand.push_back({sc, (int)i, s});
+            }
+        std::stable_sort(cand.begin(), cand.end(), [](const Cand& x, const Cand& y) { return x.score > y.score; });
+        int slot_of[8]; int child_in_slot[8];
+        for (int i = 0; i < 8; i++) { slot_of[i] = -1; child_in_slot[i] = -1; }
+        for (const Cand& c : cand) {
+            if (slot_of[c.c] >= 0 || child_in_slot[c.s] >= 0) continue;
+            slot_of[c.c] = c.s; child_in_slot[c.s] = c.c;
+        }
+        // quantization grid
+        float p[3]; int e[3]; float sc[3];
+        for (int a = 0; a < 3; a++) {
+            p[a] = nb.lo[a];
+            float ext = nb.hi[a] - nb.lo[a];
+            int ex = -126;
+            if (ext > 0.0f) {
+                int k; std::frexp(ext / 255.0f, &k);   // ext/255 in [2^(k-1), 2^k)
+                ex = std::max(k, -126);
+            }
+            e[a] = ex;
+        }
+        uint32_t* W = &out.nodes[20 * (size_t)job.wide];
+        uint8_t meta[8] = {0}, qlo[3][8] = {{0}}, qhi[3][8] = {{0}};
+        uint32_t imask = 0;
+        // internal children first (contiguous at child_base in slot order), then leaf triangles
+        uint32_t child_base = out.n_nodes, tri_base = (uint32_t)(out.tris.size() / 12);
+        int n_internal = 0, n_leaf_tris = 0;
+        for (int s = 0; s < 8; s++) {
+            int i = child_in_slot[s];
+            if (i < 0) continue;
+            const BNode& c = bn[ch[i]];
+            const Box& b = cbox[i];
+            for (int a = 0; a < 3; a++) {
+                int lo, hi;
+                for (;;) {
+                    sc[a] = ptm_u2f((uint32_t)(e[a] + 127) << 23);
+                    if (quantize_axis(p[a], sc[a], b.lo[a], b.hi[a], lo, hi)) break;
+                    e[a]++;
+                    // restart all earlier slots of this axis at the new scale
+                    for (int s2 = 0; s2 < s; s2++) {
+                        int i2 = child_in_slot[s2];
+                        if (i2 < 0) continue;
+                        int l2, h2;
+                        sc[a] = ptm_u2f((uint32_t)(e[a] + 127) << 23);
+                        if (!quantize_axis(p[a], sc[a], cbox[i2].lo[a], cbox[i2].hi[a], l2, h2)) throw std::runtime_error("wide BVH quantization failed");
+                        qlo[a][s2] = (uint8_t)l2; qhi[a][s2] = (uint8_t)h2;
+                    }
+                }
+                qlo[a][s] = (uint8_t)lo; qhi[a][s] = (uint8_t)hi;
+            }
+            if (c.count > 3) {
+                imask |= 1u << s;
+                meta[s] = (uint8_t)(0x20 | s);
+                n_internal++;
+            } else {
+                static const uint8_t unary[4] = {0, 0x20, 0x60, 0xE0};
+                meta[s] = (uint8_t)(unary[c.count] | n_leaf_tris);
+                n_leaf_tris += c.count;
+            }
+        }
+        // reserve internal children and emit leaf triangles in slot order
+        out.nodes.resize(20 * (size_t)(out.n_nodes + n_internal), 0u);
+        W = &out.nodes[20 * (size_t)job.wide];
+        uint32_t next = child_base;
+        for (int s = 0; s < 8; s++) {
+            int i = child_in_slot[s];
+            if (i < 0) continue;
+            const BNode& c = bn[ch[i]];
+            if (imask & (1u << s)) {
+                stack.push_back({ch[i], next++, job.depth + 1});
+            } else {
+                for (int k = 0; k < c.count; k++) {
+                    uint32_t t = sb.order[c.first + k];
+                    const float* v = tv + 9 * (size_t)t;
+                    float rec[12] = {v[0], v[1], v[2], ptm_u2f(t),
+                                     v[3] - v[0], v[4] - v[1], v[5] - v[2], 0.0f,
+                                     v[6] - v[0], v[7] - v[1], v[8] - v[2], 0.0f};
+                    out.tris.insert(out.tris.end(), rec, rec + 12);
+                }
+            }
+        }
+        out.n_nodes += (uint32_t)n_internal;
+        W[0] = ptm_f2u(p[0]); W[1] = ptm_f2u(p[1]); W[2] = ptm_f2u(p[2]);
+        W[3] = (uint32_t)(e[0] + 127) | ((uint32_t)(e[1] + 127) << 8) | ((uint32_t)(e[2] + 127) << 16) | (imask << 24);
+        W[4] = child_base; W[5] = tri_base;
+        auto pack4 = [](const uint8_t* b) { return (uint32_t)b[0] | ((uint32_t)b[1] << 8) | ((uint32_t)b[2] << 16) | ((uint32_t)b[3] << 24); };
+        W[6] = pack4(meta); W[7] = pack4(meta + 4);
+        W[8] = pack4(qlo[0]); W[9] = pack4(qlo[0] + 4); W[10] = pack4(qlo[1]); W[11] = pack4(qlo[1] + 4);
+        W[12] = pack4(qlo[2]); W[13] = pack4(qlo[2] + 4); W[14] = pack4(qhi[0]); W[15] = pack4(qhi[0] + 4);
+        W[16] = pack4(qhi[1]); W[17] = pack4(qhi[1] + 4); W[18] = pack4(qhi[2]); W[19] = pack4(qhi[2] + 4);
+    }
+}
+
+// ---------------------------------------------------------------- binary BVH (containment)
+namespace {
+struct BinBuild {
+    const double* boxes;
+    std::vector<float> cx, cy, cz;
+    BinaryBVH* out;
+    int leaf;
+
+    int emit(int s, int e) {
+        int node = (int)(out->nodes.size() / 8);
+        out->nodes.resize(out->nodes.size() + 8);
+        double b[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+        double c[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+        for (int i = s; i < e; i++) {
+            uint32_t t = out->order[i];
+            for (int a = 0; a < 3; a++) {
+                b[a] = std::min(b[a], boxes[6 * (size_t)t + a]);
+                b[3 + a] = std::max(b[3 + a], boxes[6 * (size_t)t + 3 + a]);
+            }
+            double cc[3] = {cx[t], cy[t], cz[t]};
+            for (int a = 0; a < 3; a++) { c[a] = std::min(c[a], cc[a]); c[3 + a] = std::max(c[3 + a], cc[a]); }
+        }
+        auto set = [&](float A, float B) {
+            float* nd = &out->nodes[8 * (size_t)node];
+            nd[0] = (float)b[0]; nd[1] = (float)b[1]; nd[2] = (float)b[2]; nd[3] = A;
+            nd[4] = (float)b[3]; nd[5] = (float)b[4]; nd[6] = (float)b[5]; nd[7] = B;
+        };
+        int count = e - s;
+        if (count <= leaf) { set((float)count, (float)s); return node; }
+        constexpr int NB = 12;
+        int ba = -1, bs = -1; double bc = INFINITY;
+        for (int a = 0; a < 3; a++) {
+            double ext = c[3 + a] - c[a];
+            if (!(ext >= 1e-12)) continue;
+            double scale = NB / ext;
+            double bb[NB][6]; int cnt[NB];
+            for (int k = 0; k < NB; k++) { bb[k][0] = bb[k][1] = bb[k][2] = INFINITY; bb[k][3] = bb[k][4] = bb[k][5] = -INFINITY; cnt[k] = 0; }
+            const std::vector<float>& cen = a == 0 ? cx : a == 1 ? cy : cz;
+            for (int i = s; i < e; i++) {
+                uint32_t t = out->order[i];
+                int k = (int)std::floor((cen[t] - c[a]) * scale);
+                k = std::min(std::max(k, 0), NB - 1);
+                cnt[k]++;
+                for (int x = 0; x < 3; x++) {
+                    bb[k][x] = std::min(bb[k][x], boxes[6 * (size_t)t + x]);
+                    bb[k][3 + x] = std::max(bb[k][3 + x], boxes[6 * (size_t)t + 3 + x]);
+                }
+            }
+            auto area = [](const double* q) {
+                double dx = q[3] - q[0], dy = q[4] - q[1], dz = q[5] - q[2];
+                return (dx < 0 || dy < 0 || dz < 0) ? 0.0 : 2 * (dx * dy + dy * dz + dz * dx);
+            };
+            double acc[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+            float la[NB - 1]; int lc[NB - 1]; int n = 0;
+            for (int k = 0; k < NB - 1; k++) {
+                for (int x = 0; x < 3; x++) { acc[x] = std::min(acc[x], bb[k][x]); acc[3 + x] = std::max(acc[3 + x], bb[k][3 + x]); }
+                n += cnt[k]; la[k] = (float)area(acc); lc[k] = n;
+            }
+            double acr[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY}; n = 0;
+            for (int k = NB - 1; k > 0; k--) {
+                for (int x = 0; x < 3; x++) { acr[x] = std::min(acr[x], bb[k][x]); acr[3 + x] = std::max(acr[3 + x], bb[k][3 + x]); }
+                n += cnt[k];
+                if (lc[k - 1] == 0 || n == 0) continue;
+                double cost = (double)la[k - 1] * lc[k - 1] + (double)(float)area(acr) * n;
+                if (cost < bc) { bc = cost; ba = a; bs = k - 1; }
+            }
+        }
+        double bxa[6] = {b[0], b[1], b[2], b[3], b[4], b[5]};
+        double dx = bxa[3] - bxa[0], dy = bxa[4] - bxa[1], dz = bxa[5] - bxa[2];
+        double leaf_cost = 2 * (dx * dy + dy * dz + dz * dx) * count;
+        if (ba < 0 || bc >= leaf_cost) { set((float)count, (float)s); return node; }
+        const std::vector<float>& cen = ba == 0 ? cx : ba == 1 ? cy : cz;
+        double scale = NB / (c[3 + ba] - c[ba]);
+        auto it = std::partition(out->order.begin() + s, out->order.begin() + e, [&](uint32_t t) {
+            int k = (int)std::floor((cen[t] - c[ba]) * scale);
+            k = std::min(std::max(k, 0), NB - 1);
+            return k <= bs;
+        });
+        int mid = (int)(it - out->order.begin());
+        if (mid <= s || mid >= e) mid = (s + e) / 2;
+        emit(s, mid);
+        int right = emit(mid, e);
+        float* nd = &out->nodes[8 * (size_t)node];
+        nd[0] = (float)b[0]; nd[1] = (float)b[1]; nd[2] = (float)b[2]; nd[3] = (float)(-(ba + 1));
+        nd[4] = (float)b[3]; nd[5] = (float)b[4]; nd[6] = (float)b[5]; nd[7] = (float)right;
+        return node;
+    }
+};
+}  // namespace
+
+void build_binary_bvh(const double* boxes, int n, int leaf_size, BinaryBVH& out) {
+    out.nodes.clear();
+    out.order.resize((size_t)std::max(n, 0));
+    if (n <= 0) return;
+    BinBuild B;
+    B.boxes = boxes; B.out = &out; B.leaf = leaf_size;
+    B.cx.resize(n); B.cy.resize(n); B.cz.resize(n);
+    for (int t = 0; t < n; t++) {
+        B.cx[t] = (float)((boxes[6 * (size_t)t] + boxes[6 * (size_t)t + 3]) * 0.5);
+        B.cy[t] = (float)((boxes[6 * (size_t)t + 1] + boxes[6 * (size_t)t + 4]) * 0.5);
+        B.cz[t] = (float)((boxes[6 * (size_t)t + 2] + boxes[6 * (size_t)t + 5]) * 0.5);
+        out.order[t] = (uint32_t)t;
+    }
+    B.emit(0, n);
+}
+
+// ---------------------------------------------------------------- env importance table
+namespace {
+struct F3 { float x, y, z; };
+inline F3 f3n(F3 a) {
+    float d = a.x * a.x + a.y * a.y + a.z * a.z;
+    float s = 1.0f / sqrtf(d);
+    return {a.x * s, a.y * s, a.z * s};
+}
+}  // namespace
+
+void build_env_table(const pt_env& env, std::vector<float>& cond, std::vector<float>& marg, double& total) {
+    const int W = env.table_w > 0 ? env.table_w : 512, H = env.table_h > 0 ? env.table_h : 256;
+    std::vector<float> Y((size_t)W * H), Bl((size_t)W * H);
+    const F3 sun = f3n({0.4f, 0.35f, 0.2f});
+    for (int j = 0; j < H; j++) {
+        for (int i = 0; i < W; i++) {
+            float u = ((float)i + 0.5f) / (float)W, v = ((float)j + 0.5f) / (float)H;
+            // env_chart_dir with the bake's rotation 0 (EnvironmentBake.ts: const u_envRotation)
+            float phi = (u - 0.5f) * PTM_TWO_PI - 0.0f;
+            float theta = v * PTM_PI;
+            float st = ptm_sin(theta);
+            F3 d = {ptm_cos(phi) * st, ptm_cos(theta), ptm_sin(phi) * st};
+            float s = 0.5f * (d.y + 1.0f);
+            float k = ptm_smoothstep(0.9994f, 0.9999f, d.x * sun.x + d.y * sun.y + d.z * sun.z);
+            F3 c = {0.06f * (1.0f - s) + 0.35f * s + 18.0f * k,
+                    0.08f * (1.0f - s) + 0.45f * s + 16.0f * k,
+                    0.12f * (1.0f - s) + 0.65f * s + 13.0f * k};
+            Y[(size_t)j * W + i] = (float)(0.2126 * (double)c.x + 0.7152 * (double)c.y + 0.0722 * (double)c.z);
+        }
+    }
+    for (int j = 0; j < H; j++)
+        for (int i = 0; i < W; i++) {
+            double sum = 0, wsum = 0;
+            for (int dj = -1; dj <= 1; dj++) {
+                int jj = std::min(H - 1, std::max(0, j + dj));
+                for (int di = -1; di <= 1; di++) {
+                    int ii = (i + di + W) % W;
+                    double w = (dj == 0 ? 2 : 1) * (di == 0 ? 2 : 1);
+                    sum += w * (double)Y[(size_t)jj * W + ii];
+                    wsum += w;
+                }
+            }
+            Bl[(size_t)j * W + i] = (float)(sum / wsum);
+        }
+    if (env.compensated) {
+        double num = 0, den = 0;
+        for (int j = 0; j < H; j++) {
+            double dO = (2 * M_PI / W) * (M_PI / H) * std::sin(M_PI * (j + 0.5) / H);
+            for (int i = 0; i < W; i++) { num += (double)Bl[(size_t)j * W + i] * dO; den += dO; }
+        }
+        double mean = den > 0 ? num / den : 0;
+        for (float& y : Bl) { double t = (double)y - mean; y = (float)(t > 0 ? t : 0); }
+    }
+    cond.assign((size_t)W * H, 0.0f);
+    marg.assign((size_t)H, 0.0f);
+    std::vector<double> row((size_t)H);
+    total = 0.0;
+    for (int j = 0; j < H; j++) {
+        double sinT = std::sin(M_PI * (j + 0.5) / H);
+        double sum = 0;
+        for (int i = 0; i < W; i++) sum += std::max(0.0, (double)Bl[(size_t)j * W + i]) * sinT;
+        row[j] = sum;
+        double c = 0;
+        for (int i = 0; i < W; i++) {
+            c += std::max(0.0, (double)Bl[(size_t)j * W + i]) * sinT;
+            cond[(size_t)j * W + i] = (float)(sum > 0 ? c / sum : 0);
+        }
+        total += sum;
+    }
+    double acc = 0;
+    for (int j = 0; j < H; j++) { acc += row[j]; marg[j] = (float)(total > 0 ? acc / total : 0); }
+}
+
+}  // namespace ptmi

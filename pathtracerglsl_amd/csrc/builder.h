@@ -1,0 +1,52 @@
+// builder.h — host-side acceleration and table builders of libptmi.
+//
+// Wide BVH: an 8-wide compressed BVH over world-space triangles (Ylitie, Karras, Laine,
+// "Efficient Incoherent Ray Traversal on GPUs Through Compressed Wide BVHs", HPG 2017),
+// the layout the reference uses for instance batches (components/accel/cwbvh/cwbvh.ts:36-50),
+// extended here to triangle leaves. Node = 80 bytes = 5 x uint4:
+//   w0: origin p.xyz (f32), [e.x | e.y<<8 | e.z<<16 | imask<<24]   (e = biased exponent)
+//   w1: child_base, tri_base, meta[0..3], meta[4..7]
+//   w2: qlo.x[0..3], qlo.x[4..7], qlo.y[0..3], qlo.y[4..7]
+//   w3: qlo.z[0..3], qlo.z[4..7], qhi.x[0..3], qhi.x[4..7]
+//   w4: qhi.y[0..3], qhi.y[4..7], qhi.z[0..3], qhi.z[4..7]
+// Child slot i box = p + q * 2^(e-127), decoded with the same fp32 ops the kernel uses and
+// checked here to contain the child's geometry (so the kernel's slab test is never less
+// conservative than an exact-box test). meta[i]: 0 = empty; internal child: 0x20 | i with
+// imask bit i set (child node = child_base + popcount(imask below i)); leaf: unary count in
+// bits 5..7 (1/3/7 = 1/2/3 triangles), triangle offset from tri_base in bits 0..4.
+// Slots are assigned by centroid octant so a ray with direction octant o visits slot s in
+// order s ^ (o ^ 7) (front to back).
+#pragma once
+#include <cstdint>
+#include <vector>
+#include <string>
+#include "../../include/ptmi.h"
+
+namespace ptmi {
+
+struct WideBVH {
+    std::vector<uint32_t> nodes;      // 20 u32 per node
+    std::vector<float> tris;          // 12 floats per triangle in leaf order: a.xyz,id | e1.xyz,0 | e2.xyz,0
+    uint32_t n_nodes = 0;
+    uint32_t max_depth = 0;
+};
+
+// Binary BVH in the reference's node layout (accel/bvh/bvh.ts:21-25): 8 floats per node
+// (min.xyz, A, max.xyz, B), A >= 0 leaf (count A at offset B of item order), A < 0 internal
+// (axis -A-1, right child B, left child = this + 1). Used for containment queries.
+struct BinaryBVH {
+    std::vector<float> nodes;
+    std::vector<uint32_t> order;
+};
+
+// Build the wide BVH over tri_vertices [n][9] (world space).
+void build_wide_bvh(const float* tri_vertices, int n_tris, WideBVH& out);
+
+// Binned-SAH binary BVH over boxes [n][6] (f64 min.xyz, max.xyz), leaf size <= leaf_size.
+void build_binary_bvh(const double* boxes, int n, int leaf_size, BinaryBVH& out);
+
+// Env importance table for the procedural sky: fp32 bake on the equirect grid, then the
+// f64 luminance / blur / CDF build, stored f32 (EnvironmentBake.ts:62-67, env/importance.ts).
+void build_env_table(const pt_env& env, std::vector<float>& cond, std::vector<float>& marg, double& total);
+
+}  // namespace ptmi

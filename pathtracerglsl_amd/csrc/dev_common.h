@@ -1,0 +1,105 @@
+// dev_common.h — device-side data views, fp32 vector helpers and the pcg4d sampler.
+//
+// Vector helpers keep GLSL's evaluation order (dot = (x*x' + y*y') + z*z', left-to-right
+// sums, normalize(v) = v * (1/sqrt(dot(v,v)))) and the whole library is compiled with
+// -ffp-contract=off, so every expression rounds exactly like the CPU restatement.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "../../include/ptmi.h"
+#include "../../include/ptmi_math.h"
+
+#define PT_DEV __device__ __forceinline__
+
+namespace ptmi {
+
+constexpr float MAX_DIST = 1000.0f;          // glsl/core/math.glsl:11
+constexpr float EPS_INTERFACE = 0.001f;      // math.glsl:12
+constexpr float SHADOW_BACKOFF = 0.002f;     // math.glsl:39
+constexpr float BVH_TFAR_PAD = 1.00000024f;  // accel/bvh/bvh.ts:48
+constexpr float INV_PI = 1.0f / PTM_PI;
+
+struct f3 { float x, y, z; };
+PT_DEV f3 mk3(float x, float y, float z) { f3 r; r.x = x; r.y = y; r.z = z; return r; }
+PT_DEV f3 operator+(f3 a, f3 b) { return mk3(a.x + b.x, a.y + b.y, a.z + b.z); }
+PT_DEV f3 operator-(f3 a, f3 b) { return mk3(a.x - b.x, a.y - b.y, a.z - b.z); }
+PT_DEV f3 operator*(f3 a, f3 b) { return mk3(a.x * b.x, a.y * b.y, a.z * b.z); }
+PT_DEV f3 operator*(f3 a, float s) { return mk3(a.x * s, a.y * s, a.z * s); }
+PT_DEV f3 operator/(f3 a, float s) { return mk3(a.x / s, a.y / s, a.z / s); }
+PT_DEV f3 operator-(f3 a) { return mk3(-a.x, -a.y, -a.z); }
+PT_DEV float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+PT_DEV f3 cross(f3 a, f3 b) { return mk3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+PT_DEV float length(f3 a) { return __fsqrt_rn(dot(a, a)); }
+PT_DEV f3 normalize(f3 a) { return a * (1.0f / __fsqrt_rn(dot(a, a))); }
+PT_DEV float gmax(float a, float b) { return a > b ? a : b; }   // GLSL max on non-NaN input
+PT_DEV float gmin(float a, float b) { return a < b ? a : b; }
+PT_DEV float gclamp(float x, float lo, float hi) { return gmin(gmax(x, lo), hi); }
+PT_DEV float gsign(float x) { return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : 0.0f); }
+PT_DEV f3 ld3(const float* p) { return mk3(p[0], p[1], p[2]); }
+PT_DEV f3 xyz(float4 v) { return mk3(v.x, v.y, v.z); }
+PT_DEV float4 f4(f3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
+PT_DEV bool is_black(f3 s) { return s.x <= 0.0f && s.y <= 0.0f && s.z <= 0.0f; }
+PT_DEV float spectrum_max(f3 s) { float m = s.z; m = gmax(s.y, m); m = gmax(s.x, m); return m; }
+
+// pcg4d (components/sampler/pcg4d/pcg4d.glsl:12-40); only .x is consumed.
+PT_DEV uint32_t pcg4d_x(uint32_t x, uint32_t y, uint32_t z, uint32_t w) {
+    x = x * 1664525u + 1013904223u; y = y * 1664525u + 1013904223u;
+    z = z * 1664525u + 1013904223u; w = w * 1664525u + 1013904223u;
+    x += y * w; y += z * x; z += x * y; w += y * z;
+    x ^= x >> 16u; y ^= y >> 16u; z ^= z >> 16u; w ^= w >> 16u;
+    x += y * w;
+    return x;
+}
+struct Rng {
+    uint32_t base, dim;
+    PT_DEV float next() {
+        uint32_t u = pcg4d_x(base, dim++, 0u, 0u);
+        return (float)(u >> 8) * (1.0f / 16777216.0f);
+    }
+};
+
+// Scene as the kernels see it (all device pointers).
+struct DevScene {
+    const uint4* cw_nodes;      // 5 uint4 per wide node
+    const float4* cw_tris;      // 3 float4 per triangle, leaf order
+    const float4* tri_shade;    // 3 float4 per triangle, original order: a | b | c
+    const int* tri_object;
+    int n_tris;
+    int n_cw_nodes;
+    const pt_object* objects;
+    int n_objects;
+    const int* analytic;        // object ids of quads / spheres
+    int n_analytic;
+    const pt_material* materials;
+    const pt_light* lights;
+    int n_lights;
+    int env_type, env_w, env_h, env_compensated;
+    float env_intensity, env_rotation, env_rot_c, env_rot_s, env_select_prob;
+    const float* env_cond;
+    const float* env_marg;
+    // containment (closed meshes)
+    const float4* obj_nodes;    // binary nodes, 2 float4 each: (min, A) (max, B)
+    const int* obj_node_base;   // per object
+    const int* obj_order;       // global triangle ids in per-object leaf order, at first_tri
+    const float4* tlas_nodes;   // binary nodes over closed-mesh boxes (leaf size 1)
+    const int* tlas_items;      // leaf item -> object id
+    int n_tlas_nodes;
+    int has_transmission;
+    int emissive_solid;
+};
+
+// Wave-aggregated queue append: every active lane calls it (convergent), lanes with
+// pred == false get -1. One atomic per wave.
+PT_DEV int wave_append(int* counter, bool pred) {
+    uint64_t mask = __ballot(pred);
+    if (mask == 0) return -1;
+    int lane = (int)__lane_id();
+    int leader = __ffsll((unsigned long long)mask) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(counter, (int)__popcll(mask));
+    base = __shfl(base, leader);
+    int prefix = (int)__popcll(mask & ((1ull << lane) - 1ull));
+    return pred ? base + prefix : -1;
+}
+
+}  // namespace ptmi

@@ -1,0 +1,920 @@
+// ptmi.hip — libptmi: the C ABI and the wavefront path-tracing pipeline for gfx950.
+//
+// One frame wave traces frames_in_flight samples for every pixel as independent paths in
+// fixed slots of an SoA path state in HBM. Per bounce, four kernels run over compacted
+// queues of path ids:
+//   extend  — closest hit over the wide BVH (+ analytic objects) for the ray queue
+//   bookkeep— queue counters (no host round trip: grids are fixed, sizes read on device)
+//   shade   — hit fill, emitter/miss scoring, NEE sample (shadow queue), BSDF continuation,
+//             Russian roulette, spawn (next ray queue)
+//   shadow  — any-hit for the shadow queue; unoccluded contributions added to the path
+// and a final accumulate kernel folds each slot's radiance into the film in sample order
+// (average_update, components/accumulator/average/average.glsl:6-10). Radiance is added to
+// a path in exactly the order the reference walk adds it (pt.ts:299-464): emission at
+// vertex k (shade k), NEE of vertex k (shadow k), emission at vertex k+1 (shade k+1).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "builder.h"
+#include "dev_common.h"
+#include "shading.h"
+#include "traversal.h"
+
+using namespace ptmi;
+
+// ============================================================================ kernels
+
+namespace {
+
+constexpr int TBLOCK = 128;     // traversal blocks (2 waves)
+constexpr int SBLOCK = 128;     // shading blocks
+constexpr int MAX_GRID = 8192;
+
+struct Paths {
+    float4* ray_o;   // origin.xyz
+    float4* ray_d;   // direction.xyz
+    float4* thr;     // throughput.xyz, eta_scale
+    float4* rad;     // radiance.xyz
+    uint2* rng;      // rng_base, rng_dim
+    float4* prev;    // prev_query.p.xyz, prev_bsdf_pdf
+    int* flags;      // bit 0: prev_was_delta
+    float2* hit;     // t, primitive id (bits)
+    int* q[2];
+    int* counts;     // [0], [1]: ray queues; [2]: shadow queue
+    int* s_path;
+    float4* s_o;     // shadow origin.xyz, maxDist
+    float4* s_d;     // shadow direction.xyz
+    float4* s_c;     // contribution.xyz
+    unsigned long long* hist;   // [0] closest queries, [1] any-hit queries
+    unsigned long long* trav;   // [0] nodes visited, [1] triangles tested (COUNT builds)
+};
+
+struct Camera {
+    float ox, oy, oz, fx, fy, fz, rx, ry, rz, ux, uy, uz, tan_fov, aspect;
+    uint32_t salt;
+    int off_x, off_y;
+};
+
+__global__ void __launch_bounds__(SBLOCK) k_raygen(Paths P, Camera C, int W, int H, int n_paths, uint32_t sample0) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) P.counts[0] = n_paths;
+    if (i >= n_paths) return;
+    int pix = i % (W * H), frame = i / (W * H);
+    int x = pix % W, y = pix / W;
+    // main() (features/accumulation.ts:14-26): pixel = gl_FragCoord.xy + u_pixelOffset
+    float fx = (float)x + 0.5f + (float)C.off_x;
+    float fy = (float)y + 0.5f + (float)C.off_y;
+    Rng r;
+    r.base = pcg4d_x((uint32_t)fx, (uint32_t)fy, sample0 + (uint32_t)frame, C.salt);
+    r.dim = 0;
+    float j0 = r.next(), j1 = r.next();
+    float filmx = fx + (j0 - 0.5f), filmy = fy + (j1 - 0.5f);   // pixel/box/box.glsl:5-7
+    r.dim += 2;                                                   // lens sample (unused by a pinhole)
+    // camera/pinhole/pinhole.glsl:4-18
+    float ndcx = (2.0f * filmx / (float)W) - 1.0f;
+    float ndcy = (2.0f * filmy / (float)H) - 1.0f;
+    ndcx *= C.aspect;
+    f3 F = mk3(C.fx, C.fy, C.fz), R = mk3(C.rx, C.ry, C.rz), U = mk3(C.ux, C.uy, C.uz);
+    f3 d = normalize((F + R * (ndcx * C.tan_fov)) + U * (ndcy * C.tan_fov));
+    P.ray_o[i] = make_float4(C.ox, C.oy, C.oz, 0.0f);
+    P.ray_d[i] = f4(d, 0.0f);
+    P.thr[i] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+    P.rad[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    P.rng[i] = make_uint2(r.base, r.dim);
+    P.prev[i] = make_float4(C.ox, C.oy, C.oz, 0.0f);
+    P.flags[i] = 1;
+    P.q[0][i] = i;
+}
+
+template <int STACK, bool COUNT>
+__global__ void __launch_bounds__(TBLOCK) k_extend(DevScene S, Paths P, int qsel) {
+    __shared__ uint2 stk[STACK * TBLOCK];
+    const int n = P.counts[qsel];
+    const int* q = P.q[qsel];
+    TraceCounters cnt = {0, 0};
+    for (int base = blockIdx.x * TBLOCK; base < n; base += gridDim.x * TBLOCK) {
+        int k = base + threadIdx.x;
+        if (k >= n) break;
+        int i = q[k];
+        float4 o4 = P.ray_o[i], d4 = P.ray_d[i];
+        f3 o = xyz(o4), d = xyz(d4);
+        float tmax = MAX_DIST; int prim = -1;
+        trace_wide<false, STACK, TBLOCK, COUNT>(S, o, d, tmax, prim, stk, cnt);
+        trace_analytic<false>(S, o, d, tmax, prim);
+        P.hit[i] = make_float2(tmax, __int_as_float(prim));
+    }
+    if (COUNT) {
+        atomicAdd(&P.trav[0], (unsigned long long)cnt.nodes);
+        atomicAdd(&P.trav[1], (unsigned long long)cnt.tris);
+    }
+}
+
+template <int STACK>
+__global__ void __launch_bounds__(TBLOCK) k_shadow(DevScene S, Paths P) {
+    __shared__ uint2 stk[STACK * TBLOCK];
+    const int n = P.counts[2];
+    TraceCounters cnt = {0, 0};
+    for (int base = blockIdx.x * TBLOCK; base < n; base += gridDim.x * TBLOCK) {
+        int k = base + threadIdx.x;
+        if (k >= n) break;
+        float4 o4 = P.s_o[k], d4 = P.s_d[k];
+        f3 o = xyz(o4), d = xyz(d4);
+        float maxd = o4.w; int prim = -1;
+        bool blocked = trace_analytic<true>(S, o, d, maxd, prim);
+        if (!blocked) blocked = trace_wide<true, STACK, TBLOCK, false>(S, o, d, maxd, prim, stk, cnt);
+        if (!blocked) {
+            int i = P.s_path[k];
+            float4 c = P.s_c[k];
+            float4 L = P.rad[i];
+            P.rad[i] = make_float4(L.x + c.x, L.y + c.y, L.z + c.z, 0.0f);
+        }
+    }
+}
+
+__global__ void k_bookkeep(Paths P, int cur, int next, int first) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        if (!first) P.hist[1] += (unsigned long long)P.counts[2];
+        if (cur >= 0) P.hist[0] += (unsigned long long)P.counts[cur];
+        if (next >= 0) P.counts[next] = 0;
+        P.counts[2] = 0;
+    }
+}
+
+// The per-vertex body of transport_trace (integrators/pt/pt.ts:299-464) with the combiner
+// (transport/combiner.ts:23-102) and the kernel/light techniques (techniques/kernel/kernel.glsl,
+// techniques/light/light.glsl:8-28).
+template <int EST>
+__global__ void __launch_bounds__(SBLOCK) k_shade(DevScene S, Paths P, int qsel, int bounce, int max_bounces,
+                                                  int rr_start, float rr_max) {
+    const int n = P.counts[qsel];
+    const int* q = P.q[qsel];
+    int* qn = P.q[qsel ^ 1];
+    constexpr bool NEE = EST != PT_EST_PT;
+    constexpr bool MIS = EST == PT_EST_MIS;
+    for (int base = blockIdx.x * SBLOCK; base < n; base += gridDim.x * SBLOCK) {
+        const int k = base + threadIdx.x;
+        const bool valid = k < n;
+        const int i = valid ? q[k] : 0;
+        bool cont = false, shadow = false;
+        f3 s_o = mk3(0, 0, 0), s_d = mk3(0, 0, 0), s_c = mk3(0, 0, 0);
+        float s_max = 0.0f;
+        if (valid) {
+            float2 hv = P.hit[i];
+            int prim = __float_as_int(hv.y);
+            f3 ro = xyz(P.ray_o[i]), rd = xyz(P.ray_d[i]);
+            float4 th = P.thr[i];
+            f3 T = xyz(th); float eta_scale = th.w;
+            f3 L = xyz(P.rad[i]);
+            uint2 rs = P.rng[i];
+            Rng rng; rng.base = rs.x; rng.dim = rs.y;
+            float4 pv = P.prev[i];
+            f3 prev_qp = xyz(pv); float prev_pdf = pv.w;
+            bool prev_delta = (P.flags[i] & 1) != 0;
+            const bool emitters = S.n_lights > 0;
+            if (prim < 0) {
+                // kernel_score_miss + combiner_w_env
+                float w = 1.0f;
+                if (NEE && env_samplable(S)) {
+                    if (prev_delta) w = 1.0f;
+                    else if (MIS) w = power_heuristic(prev_pdf, env_select_live(S) ? S.env_select_prob * environment_pdf(S, rd) : environment_pdf(S, rd));
+                    else w = 0.0f;
+                }
+                f3 Le = env_radiance(S, rd);
+                L = L + (T * w) * Le;
+            } else {
+                Hit h; bool back;
+                fill_hit(S, ro, rd, hv.x, prim, h, back);
+                const pt_material& M = S.materials[S.objects[h.owner].material];
+                bool need = M.model == PT_MAT_DIELECTRIC || (S.emissive_solid && back);
+                classify_hit(S, h, back, need);
+                f3 wo = -rd;
+                // kernel_score_emitter_hit
+                if (h.region_to >= 0) {
+                    const pt_material& E = S.materials[S.objects[h.region_to].material];
+                    if (E.emissive) {
+                        float w = 1.0f;
+                        if (NEE && emitters) {
+                            int lid = light_of(S, h.region_to);
+                            if (lid < 0 || prev_delta) w = 1.0f;
+                            else if (MIS) w = power_heuristic(prev_pdf, lighting_pdf(S, prev_qp, rd, lid, h.p));
+                            else w = 0.0f;
+                        }
+                        L = L + (T * w) * ld3(E.emission);
+                    }
+                }
+                if (bounce < max_bounces) {
+                    // light_sample_direct
+                    if (NEE && M.model != PT_MAT_DIELECTRIC) {
+                        float x0 = rng.next(), x1 = rng.next();
+                        LSample ls = lighting_sample(S, h.p, x0, x1);
+                        if (ls.pdf > 0.0f) {
+                            f3 light_p = h.p + ls.wi * ls.distance;
+                            float side = dot(ls.wi, h.ng) >= 0.0f ? 1.0f : -1.0f;
+                            s_o = h.p + (h.ng * side) * h.eps;
+                            s_d = normalize(light_p - s_o);
+                            s_max = length(light_p - s_o) - SHADOW_BACKOFF;
+                            f3 f = surface_eval(M, ls.wi, wo, h);
+                            float cos_i = fabsf(dot(ls.wi, h.frame.n));
+                            float w = 1.0f;
+                            if (MIS && !(ls.flags & 1)) w = power_heuristic(ls.pdf, surface_pdf(M, ls.wi, wo, h));
+                            s_c = ((((T * ls.radiance) * f) * cos_i) * w) / ls.pdf;
+                            shadow = true;
+                        }
+                    }
+                    // kernel_sample_continuation
+                    float uc = rng.next(), ux = rng.next(), uy = rng.next();
+                    ISample bs = surface_sample(S, M, wo, h, uc, ux, uy);
+                    if (!is_black(bs.weight)) {
+                        T = T * bs.weight;
+                        prev_delta = (bs.flags & LOBE_DELTA) != 0;
+                        prev_pdf = bs.pdf;
+                        prev_qp = h.p;
+                        if (S.has_transmission && (bs.flags & LOBE_TRANSMISSION)) {
+                            float r = ior_of(S, h.region_to) / ior_of(S, h.region_from);
+                            eta_scale *= r * r;
+                        }
+                        cont = true;
+                        // roulette (pt.ts:204-222)
+                        if (rr_start >= 0 && bounce >= rr_start) {
+                            float m = S.has_transmission ? spectrum_max(T) * eta_scale : spectrum_max(T);
+                            float p = gmin(rr_max, m);
+                            if (p <= 0.0f) cont = false;
+                            else if (rng.next() > p) cont = false;
+                            else T = T / p;
+                        }
+                        if (cont) {
+                            float side = dot(bs.wi, h.ng) >= 0.0f ? 1.0f : -1.0f;
+                            P.ray_o[i] = f4(h.p + (h.ng * side) * h.eps, 0.0f);
+                            P.ray_d[i] = f4(bs.wi, 0.0f);
+                            P.thr[i] = f4(T, eta_scale);
+                            P.prev[i] = f4(prev_qp, prev_pdf);
+                            P.flags[i] = prev_delta ? 1 : 0;
+                            P.rng[i] = make_uint2(rng.base, rng.dim);
+                        }
+                    }
+                }
+            }
+            P.rad[i] = f4(L, 0.0f);
+        }
+        int slot = wave_append(&P.counts[qsel ^ 1], cont);
+        if (cont) qn[slot] = i;
+        int sslot = wave_append(&P.counts[2], shadow);
+        if (shadow) {
+            P.s_path[sslot] = i;
+            P.s_o[sslot] = f4(s_o, s_max);
+            P.s_d[sslot] = f4(s_d, 0.0f);
+            P.s_c[sslot] = f4(s_c, 0.0f);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(SBLOCK) k_accumulate(const Paths P, float4* film, int npix, int frames, uint32_t sample0) {
+    int pix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= npix) return;
+    float4 acc = film[pix];
+    for (int f = 0; f < frames; f++) {
+        uint32_t n = sample0 + (uint32_t)f;
+        float4 x = P.rad[(size_t)f * npix + pix];
+        if (n == 0) acc = make_float4(x.x, x.y, x.z, 1.0f);
+        else {
+            float w = 1.0f / ((float)n + 1.0f);
+            acc = make_float4(acc.x * (1.0f - w) + x.x * w, acc.y * (1.0f - w) + x.y * w, acc.z * (1.0f - w) + x.z * w, 1.0f);
+        }
+    }
+    film[pix] = acc;
+}
+
+template <int STACK, bool COUNT>
+__global__ void __launch_bounds__(TBLOCK) k_trace_closest(DevScene S, const float* rays, int n, float* t_out, int* prim_out,
+                                                          unsigned long long* trav) {
+    __shared__ uint2 stk[STACK * TBLOCK];
+    TraceCounters cnt = {0, 0};
+    for (int base = blockIdx.x * TBLOCK; base < n; base += gridDim.x * TBLOCK) {
+        int i = base + threadIdx.x;
+        if (i >= n) break;
+        f3 o = mk3(rays[i], rays[n + i], rays[2 * (size_t)n + i]);
+        f3 d = mk3(rays[3 * (size_t)n + i], rays[4 * (size_t)n + i], rays[5 * (size_t)n + i]);
+        float t0 = rays[6 * (size_t)n + i], tmax = t0; int prim = -1;
+        trace_wide<false, STACK, TBLOCK, COUNT>(S, o, d, tmax, prim, stk, cnt);
+        trace_analytic<false>(S, o, d, tmax, prim);
+        t_out[i] = tmax;
+        prim_out[i] = prim;
+    }
+    if (COUNT) {
+        atomicAdd(&trav[0], (unsigned long long)cnt.nodes);
+        atomicAdd(&trav[1], (unsigned long long)cnt.tris);
+    }
+}
+
+template <int STACK>
+__global__ void __launch_bounds__(TBLOCK) k_trace_any(DevScene S, const float* rays, int n, uint8_t* hit_out) {
+    __shared__ uint2 stk[STACK * TBLOCK];
+    TraceCounters cnt = {0, 0};
+    for (int base = blockIdx.x * TBLOCK; base < n; base += gridDim.x * TBLOCK) {
+        int i = base + threadIdx.x;
+        if (i >= n) break;
+        f3 o = mk3(rays[i], rays[n + i], rays[2 * (size_t)n + i]);
+        f3 d = mk3(rays[3 * (size_t)n + i], rays[4 * (size_t)n + i], rays[5 * (size_t)n + i]);
+        float tmax = rays[6 * (size_t)n + i]; int prim = -1;
+        bool h = trace_analytic<true>(S, o, d, tmax, prim);
+        if (!h) h = trace_wide<true, STACK, TBLOCK, false>(S, o, d, tmax, prim, stk, cnt);
+        hit_out[i] = h ? 1 : 0;
+    }
+}
+
+}  // namespace
+
+// ============================================================================ host side
+
+namespace {
+thread_local std::string g_err;
+pt_status fail(pt_status s, const std::string& msg) { g_err = msg; return s; }
+
+#define HIP_TRY(expr)                                                                       \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail(e_ == hipErrorOutOfMemory ? PT_E_OOM : PT_E_DEVICE,                  \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));                 \
+    } while (0)
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+};
+
+int grid_for(long long n, int block) {
+    long long g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    return (int)std::min<long long>(g, MAX_GRID);
+}
+}  // namespace
+
+struct TimingSlot { std::vector<std::pair<hipEvent_t, hipEvent_t>> ev; float ms = 0; uint32_t launches = 0; uint64_t items = 0; };
+
+struct pt_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<std::unique_ptr<DevBuf>> scene_bufs;
+    DevScene dev{};
+    bool has_scene = false;
+    int stack = 16;
+    uint32_t wide_depth = 0;
+    bool timing = false;
+    std::map<std::string, TimingSlot> timings;
+    std::vector<std::string> timing_order;
+    DevBuf trav;   // 2 x u64 counters for instrumented traces
+    DevBuf rays_tmp, t_tmp, p_tmp;
+
+    template <class T> pt_status upload(const T* host, size_t n, const T** out) {
+        auto b = std::make_unique<DevBuf>();
+        b->bytes = std::max<size_t>(n, 1) * sizeof(T);
+        HIP_TRY(hipMalloc(&b->p, b->bytes));
+        if (n) HIP_TRY(hipMemcpyAsync(b->p, host, n * sizeof(T), hipMemcpyHostToDevice, stream));
+        *out = (const T*)b->p;
+        scene_bufs.push_back(std::move(b));
+        return PT_OK;
+    }
+    void ev_begin(const char* name, hipEvent_t* b) {
+        if (!timing) return;
+        (void)hipEventCreate(b);
+        (void)hipEventRecord(*b, stream);
+        (void)name;
+    }
+    void ev_end(const char* name, hipEvent_t b, uint64_t items) {
+        if (!timing) return;
+        hipEvent_t e;
+        (void)hipEventCreate(&e);
+        (void)hipEventRecord(e, stream);
+        auto it = timings.find(name);
+        if (it == timings.end()) { timing_order.push_back(name); it = timings.emplace(name, TimingSlot()).first; }
+        it->second.ev.emplace_back(b, e);
+        it->second.launches++;
+        it->second.items += items;
+    }
+};
+
+struct pt_renderer {
+    pt_ctx* ctx = nullptr;
+    pt_program prog{};
+    DevBuf film;
+    uint32_t sample_count = 0;
+    long long cap = 0;    // path slots allocated
+    DevBuf b_ray_o, b_ray_d, b_thr, b_rad, b_rng, b_prev, b_flags, b_hit, b_q0, b_q1, b_counts,
+        b_spath, b_so, b_sd, b_sc, b_hist, b_trav;
+    Paths paths{};
+    bool count_traversal = false;
+};
+
+#define TIMED(ctx, name, items, launch)             \
+    do {                                            \
+        hipEvent_t ev_b_ = nullptr;                 \
+        (ctx)->ev_begin(name, &ev_b_);              \
+        launch;                                     \
+        (ctx)->ev_end(name, ev_b_, items);          \
+    } while (0)
+
+extern "C" {
+
+const char* pt_last_error(void) { return g_err.c_str(); }
+
+pt_status pt_ctx_create(int device_index, pt_ctx** out) {
+    if (!out) return fail(PT_E_INVALID, "pt_ctx_create: out is null");
+    int n = 0;
+    HIP_TRY(hipGetDeviceCount(&n));
+    if (device_index < 0 || device_index >= n) return fail(PT_E_INVALID, "pt_ctx_create: no such device");
+    HIP_TRY(hipSetDevice(device_index));
+    auto* c = new pt_ctx();
+    c->device = device_index;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return fail(PT_E_DEVICE, "stream creation failed"); }
+    c->trav.bytes = 2 * sizeof(unsigned long long);
+    if (hipMalloc(&c->trav.p, c->trav.bytes) != hipSuccess) { delete c; return fail(PT_E_OOM, "counter allocation failed"); }
+    *out = c;
+    return PT_OK;
+}
+
+void pt_ctx_destroy(pt_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto& kv : ctx->timings)
+        for (auto& pr : kv.second.ev) { (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second); }
+    ctx->scene_bufs.clear();
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+pt_status pt_scene_upload(pt_ctx* ctx, const pt_scene_desc* sd) {
+    if (!ctx || !sd) return fail(PT_E_INVALID, "pt_scene_upload: null argument");
+    if (sd->abi_version != PT_ABI_VERSION) return fail(PT_E_INVALID, "pt_scene_upload: ABI version mismatch");
+    if (sd->n_tris < 0 || sd->n_objects < 0 || sd->n_materials <= 0 || sd->n_lights < 0)
+        return fail(PT_E_INVALID, "pt_scene_upload: negative count or no materials");
+    if (sd->n_tris > 0 && (!sd->tri_vertices || !sd->tri_object)) return fail(PT_E_MISSING_INPUT, "missing channel: triangles");
+    if (sd->n_objects > 0 && !sd->objects) return fail(PT_E_MISSING_INPUT, "missing channel: objects");
+    if (!sd->materials) return fail(PT_E_MISSING_INPUT, "missing channel: materials");
+    if (sd->n_lights > 0 && !sd->lights) return fail(PT_E_MISSING_INPUT, "missing channel: lights");
+    for (int t = 0; t < sd->n_tris; t++) {
+        int o = sd->tri_object[t];
+        if (o < 0 || o >= sd->n_objects || sd->objects[o].kind != PT_OBJ_MESH)
+            return fail(PT_E_INVALID, "triangle " + std::to_string(t) + " has an invalid owning object");
+    }
+    for (int o = 0; o < sd->n_objects; o++) {
+        const pt_object& ob = sd->objects[o];
+        if (ob.material < 0 || ob.material >= sd->n_materials) return fail(PT_E_INVALID, "object " + std::to_string(o) + ": bad material");
+        if (ob.kind == PT_OBJ_MESH && (ob.first_tri < 0 || ob.tri_count < 0 || ob.first_tri + ob.tri_count > sd->n_tris))
+            return fail(PT_E_INVALID, "object " + std::to_string(o) + ": triangle range out of bounds");
+    }
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ctx->scene_bufs.clear();
+    ctx->has_scene = false;
+    DevScene& D = ctx->dev;
+    D = DevScene{};
+    // wide BVH over every mesh triangle
+    WideBVH wb;
+    try { build_wide_bvh(sd->tri_vertices, sd->n_tris, wb); }
+    catch (const std::exception& e) { return fail(PT_E_INVALID, std::string("wide BVH build: ") + e.what()); }
+    ctx->wide_depth = wb.max_depth;
+    if (wb.max_depth + 1 <= 16) ctx->stack = 16;
+    else if (wb.max_depth + 1 <= 32) ctx->stack = 32;
+    else return fail(PT_E_UNSUPPORTED, "wide BVH deeper than 31 levels");
+    pt_status st;
+    if ((st = ctx->upload((const uint4*)wb.nodes.data(), wb.nodes.size() / 4, &D.cw_nodes)) != PT_OK) return st;
+    if ((st = ctx->upload((const float4*)wb.tris.data(), wb.tris.size() / 4, &D.cw_tris)) != PT_OK) return st;
+    D.n_cw_nodes = (int)wb.n_nodes;
+    std::vector<float> shade((size_t)sd->n_tris * 12);
+    for (int t = 0; t < sd->n_tris; t++)
+        for (int v = 0; v < 3; v++) {
+            for (int a = 0; a < 3; a++) shade[12 * (size_t)t + 4 * v + a] = sd->tri_vertices[9 * (size_t)t + 3 * v + a];
+            shade[12 * (size_t)t + 4 * v + 3] = 0.0f;
+        }
+    if ((st = ctx->upload((const float4*)shade.data(), (size_t)sd->n_tris * 3, &D.tri_shade)) != PT_OK) return st;
+    if ((st = ctx->upload(sd->tri_object, (size_t)sd->n_tris, &D.tri_object)) != PT_OK) return st;
+    D.n_tris = sd->n_tris;
+    if ((st = ctx->upload(sd->objects, (size_t)sd->n_objects, &D.objects)) != PT_OK) return st;
+    D.n_objects = sd->n_objects;
+    std::vector<int> analytic;
+    for (int o = 0; o < sd->n_objects; o++) if (sd->objects[o].kind != PT_OBJ_MESH) analytic.push_back(o);
+    if ((st = ctx->upload(analytic.data(), analytic.size(), &D.analytic)) != PT_OK) return st;
+    D.n_analytic = (int)analytic.size();
+    if ((st = ctx->upload(sd->materials, (size_t)sd->n_materials, &D.materials)) != PT_OK) return st;
+    if ((st = ctx->upload(sd->lights, (size_t)sd->n_lights, &D.lights)) != PT_OK) return st;
+    D.n_lights = sd->n_lights;
+    for (int m = 0; m < sd->n_materials; m++) if (sd->materials[m].model == PT_MAT_DIELECTRIC) D.has_transmission = 1;
+    for (int o = 0; o < sd->n_objects; o++) {
+        const pt_object& ob = sd->objects[o];
+        bool solid = ob.kind == PT_OBJ_SPHERE || (ob.kind == PT_OBJ_MESH && ob.closed);
+        if (solid && sd->materials[ob.material].emissive) D.emissive_solid = 1;
+    }
+    // containment: per closed mesh binary BVH + TLAS over closed-mesh boxes
+    std::vector<float> onodes;
+    std::vector<int> obase((size_t)std::max(sd->n_objects, 1), -1), oorder((size_t)std::max(sd->n_tris, 1), 0);
+    std::vector<double> tboxes; std::vector<int> titems;
+    for (int o = 0; o < sd->n_objects; o++) {
+        const pt_object& ob = sd->objects[o];
+        if (ob.kind != PT_OBJ_MESH || !ob.closed || ob.tri_count == 0) continue;
+        std::vector<double> boxes(6 * (size_t)ob.tri_count);
+        for (int i = 0; i < ob.tri_count; i++) {
+            const float* p = sd->tri_vertices + 9 * (size_t)(ob.first_tri + i);
+            for (int a = 0; a < 3; a++) {
+                boxes[6 * i + a] = std::min(std::min(p[a], p[3 + a]), p[6 + a]);
+                boxes[6 * i + 3 + a] = std::max(std::max(p[a], p[3 + a]), p[6 + a]);
+            }
+        }
+        BinaryBVH bb;
+        build_binary_bvh(boxes.data(), ob.tri_count, 2, bb);
+        obase[o] = (int)(onodes.size() / 8);
+        onodes.insert(onodes.end(), bb.nodes.begin(), bb.nodes.end());
+        for (int i = 0; i < ob.tri_count; i++) oorder[ob.first_tri + i] = ob.first_tri + (int)bb.order[i];
+        for (int a = 0; a < 3; a++) tboxes.push_back(ob.box_min[a]);
+        for (int a = 0; a < 3; a++) tboxes.push_back(ob.box_max[a]);
+        titems.push_back(o);
+    }
+    BinaryBVH tl;
+    build_binary_bvh(tboxes.data(), (int)titems.size(), 1, tl);
+    std::vector<int> tl_items(tl.order.size());
+    for (size_t k = 0; k < tl.order.size(); k++) tl_items[k] = titems[tl.order[k]];
+    if ((st = ctx->upload((const float4*)onodes.data(), onodes.size() / 4, &D.obj_nodes)) != PT_OK) return st;
+    if ((st = ctx->upload(obase.data(), obase.size(), &D.obj_node_base)) != PT_OK) return st;
+    if ((st = ctx->upload(oorder.data(), oorder.size(), &D.obj_order)) != PT_OK) return st;
+    if ((st = ctx->upload((const float4*)tl.nodes.data(), tl.nodes.size() / 4, &D.tlas_nodes)) != PT_OK) return st;
+    if ((st = ctx->upload(tl_items.data(), tl_items.size(), &D.tlas_items)) != PT_OK) return st;
+    D.n_tlas_nodes = (int)(tl.nodes.size() / 8);
+    // environment
+    D.env_type = sd->env.type;
+    if (sd->env.type == PT_ENV_SKY) {
+        std::vector<float> cond, marg; double total;
+        build_env_table(sd->env, cond, marg, total);
+        D.env_w = sd->env.table_w > 0 ? sd->env.table_w : 512;
+        D.env_h = sd->env.table_h > 0 ? sd->env.table_h : 256;
+        D.env_compensated = sd->env.compensated;
+        D.env_intensity = sd->env.intensity;
+        D.env_rotation = sd->env.rotation;
+        D.env_rot_c = (float)std::cos((double)sd->env.rotation);
+        D.env_rot_s = (float)std::sin((double)sd->env.rotation);
+        D.env_select_prob = sd->env.select_prob;
+        if ((st = ctx->upload(cond.data(), cond.size(), &D.env_cond)) != PT_OK) return st;
+        if ((st = ctx->upload(marg.data(), marg.size(), &D.env_marg)) != PT_OK) return st;
+    } else if (sd->env.type != PT_ENV_NONE) {
+        return fail(PT_E_UNSUPPORTED, "environment type not provided by this build");
+    }
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ctx->has_scene = true;
+    return PT_OK;
+}
+
+pt_status pt_renderer_load(pt_ctx* ctx, const pt_program* prog, pt_renderer** out) {
+    if (!ctx || !prog || !out) return fail(PT_E_INVALID, "pt_renderer_load: null argument");
+    if (prog->abi_version != PT_ABI_VERSION) return fail(PT_E_INVALID, "pt_renderer_load: ABI version mismatch");
+    if (prog->estimator < PT_EST_PT || prog->estimator > PT_EST_MIS) return fail(PT_E_UNSUPPORTED, "unknown estimator");
+    if (prog->color != PT_COLOR_RGB) return fail(PT_E_UNSUPPORTED, "only RGB color is provided by this build");
+    if (prog->width <= 0 || prog->height <= 0 || prog->max_bounces < 0) return fail(PT_E_INVALID, "bad film size or bounce budget");
+    HIP_TRY(hipSetDevice(ctx->device));
+    auto* r = new pt_renderer();
+    r->ctx = ctx;
+    r->prog = *prog;
+    if (r->prog.frames_in_flight < 1) r->prog.frames_in_flight = 1;
+    r->film.bytes = (size_t)prog->width * prog->height * sizeof(float4);
+    if (hipMalloc(&r->film.p, r->film.bytes) != hipSuccess) { delete r; return fail(PT_E_OOM, "film allocation failed"); }
+    (void)hipMemsetAsync(r->film.p, 0, r->film.bytes, ctx->stream);
+    *out = r;
+    return PT_OK;
+}
+
+void pt_renderer_unload(pt_renderer* r) {
+    if (!r) return;
+    (void)hipSetDevice(r->ctx->device);
+    (void)hipStreamSynchronize(r->ctx->stream);
+    delete r;
+}
+
+}  // extern "C"
+
+static pt_status ensure_paths(pt_renderer* r, long long n) {
+    if (r->cap >= n) return PT_OK;
+    struct { DevBuf* b; size_t elem; } list[] = {
+        {&r->b_ray_o, 16}, {&r->b_ray_d, 16}, {&r->b_thr, 16}, {&r->b_rad, 16}, {&r->b_rng, 8}, {&r->b_prev, 16},
+        {&r->b_flags, 4}, {&r->b_hit, 8}, {&r->b_q0, 4}, {&r->b_q1, 4}, {&r->b_spath, 4}, {&r->b_so, 16},
+        {&r->b_sd, 16}, {&r->b_sc, 16}};
+    for (auto& e : list) {
+        if (e.b->p) { HIP_TRY(hipFree(e.b->p)); e.b->p = nullptr; }
+        e.b->bytes = e.elem * (size_t)n;
+        HIP_TRY(hipMalloc(&e.b->p, e.b->bytes));
+    }
+    if (!r->b_counts.p) { r->b_counts.bytes = 4 * sizeof(int); HIP_TRY(hipMalloc(&r->b_counts.p, r->b_counts.bytes)); }
+    if (!r->b_hist.p) { r->b_hist.bytes = 2 * sizeof(unsigned long long); HIP_TRY(hipMalloc(&r->b_hist.p, r->b_hist.bytes)); }
+    if (!r->b_trav.p) { r->b_trav.bytes = 2 * sizeof(unsigned long long); HIP_TRY(hipMalloc(&r->b_trav.p, r->b_trav.bytes)); }
+    Paths& P = r->paths;
+    P.ray_o = (float4*)r->b_ray_o.p; P.ray_d = (float4*)r->b_ray_d.p; P.thr = (float4*)r->b_thr.p;
+    P.rad = (float4*)r->b_rad.p; P.rng = (uint2*)r->b_rng.p; P.prev = (float4*)r->b_prev.p;
+    P.flags = (int*)r->b_flags.p; P.hit = (float2*)r->b_hit.p; P.q[0] = (int*)r->b_q0.p; P.q[1] = (int*)r->b_q1.p;
+    P.counts = (int*)r->b_counts.p; P.s_path = (int*)r->b_spath.p; P.s_o = (float4*)r->b_so.p;
+    P.s_d = (float4*)r->b_sd.p; P.s_c = (float4*)r->b_sc.p; P.hist = (unsigned long long*)r->b_hist.p;
+    P.trav = (unsigned long long*)r->b_trav.p;
+    r->cap = n;
+    return PT_OK;
+}
+
+template <int STACK>
+static void launch_wave(pt_renderer* r, const Camera& cam, int frames, uint32_t sample0) {
+    pt_ctx* ctx = r->ctx;
+    const DevScene& S = ctx->dev;
+    Paths& P = r->paths;
+    const pt_program& pg = r->prog;
+    const int W = pg.width, H = pg.height;
+    const long long n = (long long)W * H * frames;
+    hipStream_t st = ctx->stream;
+    TIMED(ctx, "raygen", n, hipLaunchKernelGGL(k_raygen, dim3((unsigned)((n + SBLOCK - 1) / SBLOCK)), dim3(SBLOCK), 0, st, P, cam, W, H, (int)n, sample0));
+    int cur = 0;
+    for (int b = 0; b <= pg.max_bounces; b++) {
+        if (r->count_traversal)
+            TIMED(ctx, "extend_count", 0, hipLaunchKernelGGL((k_extend<STACK, true>), dim3(grid_for(n, TBLOCK)), dim3(TBLOCK), 0, st, S, P, cur));
+        else
+            TIMED(ctx, "extend", 0, hipLaunchKernelGGL((k_extend<STACK, false>), dim3(grid_for(n, TBLOCK)), dim3(TBLOCK), 0, st, S, P, cur));
+        hipLaunchKernelGGL(k_bookkeep, dim3(1), dim3(64), 0, st, P, cur, cur ^ 1, b == 0 ? 1 : 0);
+        switch (pg.estimator) {
+            case PT_EST_PT:
+                TIMED(ctx, "shade", 0, hipLaunchKernelGGL((k_shade<PT_EST_PT>), dim3(grid_for(n, SBLOCK)), dim3(SBLOCK), 0, st, S, P, cur, b, pg.max_bounces, pg.rr_start, pg.rr_max_survival));
+                break;
+            case PT_EST_NEE:
+                TIMED(ctx, "shade", 0, hipLaunchKernelGGL((k_shade<PT_EST_NEE>), dim3(grid_for(n, SBLOCK)), dim3(SBLOCK), 0, st, S, P, cur, b, pg.max_bounces, pg.rr_start, pg.rr_max_survival));
+                break;
+            default:
+                TIMED(ctx, "shade", 0, hipLaunchKernelGGL((k_shade<PT_EST_MIS>), dim3(grid_for(n, SBLOCK)), dim3(SBLOCK), 0, st, S, P, cur, b, pg.max_bounces, pg.rr_start, pg.rr_max_survival));
+        }
+        TIMED(ctx, "shadow", 0, hipLaunchKernelGGL((k_shadow<STACK>), dim3(grid_for(n, TBLOCK)), dim3(TBLOCK), 0, st, S, P));
+        cur ^= 1;
+    }
+    hipLaunchKernelGGL(k_bookkeep, dim3(1), dim3(64), 0, st, P, -1, -1, 0);
+    const int npix = W * H;
+    TIMED(ctx, "accumulate", npix, hipLaunchKernelGGL(k_accumulate, dim3((npix + SBLOCK - 1) / SBLOCK), dim3(SBLOCK), 0, st, P, (float4*)r->film.p, npix, frames, sample0));
+}
+
+extern "C" {
+
+pt_status pt_render_frames(pt_renderer* r, const pt_params* prm, uint32_t nframes) {
+    if (!r || !prm) return fail(PT_E_INVALID, "pt_render_frames: null argument");
+    pt_ctx* ctx = r->ctx;
+    if (!ctx->has_scene) return fail(PT_E_MISSING_INPUT, "missing input: no scene uploaded (pt_scene_upload)");
+    HIP_TRY(hipSetDevice(ctx->device));
+    const pt_program& pg = r->prog;
+    const int F = std::max(1, std::min<int>(pg.frames_in_flight, (int)std::max<uint32_t>(nframes, 1)));
+    pt_status st = ensure_paths(r, (long long)pg.width * pg.height * F);
+    if (st != PT_OK) return st;
+    HIP_TRY(hipMemsetAsync(r->b_hist.p, 0, r->b_hist.bytes, ctx->stream));
+    HIP_TRY(hipMemsetAsync(r->b_trav.p, 0, r->b_trav.bytes, ctx->stream));
+    Camera cam;
+    cam.ox = prm->cam_position[0]; cam.oy = prm->cam_position[1]; cam.oz = prm->cam_position[2];
+    cam.fx = prm->cam_forward[0]; cam.fy = prm->cam_forward[1]; cam.fz = prm->cam_forward[2];
+    cam.rx = prm->cam_right[0]; cam.ry = prm->cam_right[1]; cam.rz = prm->cam_right[2];
+    cam.ux = prm->cam_up[0]; cam.uy = prm->cam_up[1]; cam.uz = prm->cam_up[2];
+    cam.tan_fov = prm->tan_fov; cam.aspect = prm->aspect; cam.salt = prm->reset_salt;
+    cam.off_x = prm->pixel_offset[0]; cam.off_y = prm->pixel_offset[1];
+    uint32_t done = 0;
+    while (done < nframes) {
+        int f = (int)std::min<uint32_t>((uint32_t)F, nframes - done);
+        if (ctx->stack == 16) launch_wave<16>(r, cam, f, r->sample_count);
+        else launch_wave<32>(r, cam, f, r->sample_count);
+        HIP_TRY(hipGetLastError());
+        r->sample_count += (uint32_t)f;
+        done += (uint32_t)f;
+    }
+    return PT_OK;
+}
+
+pt_status pt_read_export(pt_renderer* r, const char* name, float* out, size_t out_floats) {
+    if (!r || !name || !out) return fail(PT_E_INVALID, "pt_read_export: null argument");
+    if (std::strcmp(name, "radiance") != 0) return fail(PT_E_INVALID, std::string("unknown export '") + name + "'");
+    size_t need = (size_t)r->prog.width * r->prog.height * 4;
+    if (out_floats < need) return fail(PT_E_INVALID, "pt_read_export: output buffer too small");
+    HIP_TRY(hipSetDevice(r->ctx->device));
+    HIP_TRY(hipMemcpyAsync(out, r->film.p, need * sizeof(float), hipMemcpyDeviceToHost, r->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(r->ctx->stream));
+    return PT_OK;
+}
+
+pt_status pt_clear_accumulation(pt_renderer* r) {
+    if (!r) return fail(PT_E_INVALID, "pt_clear_accumulation: null renderer");
+    HIP_TRY(hipSetDevice(r->ctx->device));
+    HIP_TRY(hipMemsetAsync(r->film.p, 0, r->film.bytes, r->ctx->stream));
+    r->sample_count = 0;
+    return PT_OK;
+}
+
+uint32_t pt_sample_count(const pt_renderer* r) { return r ? r->sample_count : 0; }
+
+pt_status pt_trace_closest_device(pt_ctx* ctx, const float* d_rays, uint32_t n, float* d_t, int32_t* d_prim) {
+    if (!ctx || (!d_rays && n)) return fail(PT_E_INVALID, "pt_trace_closest_device: null argument");
+    if (!ctx->has_scene) return fail(PT_E_MISSING_INPUT, "missing input: no scene uploaded (pt_scene_upload)");
+    HIP_TRY(hipSetDevice(ctx->device));
+    if (n == 0) return PT_OK;
+    unsigned long long* trav = (unsigned long long*)ctx->trav.p;
+    if (ctx->stack == 16)
+        TIMED(ctx, "trace_closest", n, hipLaunchKernelGGL((k_trace_closest<16, false>), dim3(grid_for(n, TBLOCK)), dim3(TBLOCK), 0, ctx->stream, ctx->dev, d_rays, (int)n, d_t, d_prim, trav));
+    else
+        TIMED(ctx, "trace_closest", n, hipLaunchKernelGGL((k_trace_closest<32, false>), dim3(grid_for(n, TBLOCK)), dim3(TBLOCK), 0, ctx->stream, ctx->dev, d_rays, (int)n, d_t, d_prim, trav));
+    HIP_TRY(hipGetLastError());
+    return PT_OK;
+}
+
+pt_status pt_trace_any_device(pt_ctx* ctx, const float* d_rays, uint32_t n, uint8_t* d_hit) {
+    if (!ctx || (!d_rays && n)) return fail(PT_E_INVALID, "pt_trace_any_device: null argument");
+    if (!ctx->has_scene) return fail(PT_E_MISSING_INPUT, "missing input: no scene uploaded (pt_scene_upload)");
+    HIP_TRY(hipSetDevice(ctx->device));
+    if (n == 0) return PT_OK;
+    if (ctx->stack == 16)
+        TIMED(ctx, "trace_any", n, hipLaunchKernelGGL((k_trace_any<16>), dim3(grid_for(n, TBLOCK)), dim3(TBLOCK), 0, ctx->stream, ctx->dev, d_rays, (int)n, d_hit));
+    else
+        TIMED(ctx, "trace_any", n, hipLaunchKernelGGL((k_trace_any<32>), dim3(grid_for(n, TBLOCK)), dim3(TBLOCK), 0, ctx->stream, ctx->dev, d_rays, (int)n, d_hit));
+    HIP_TRY(hipGetLastError());
+    return PT_OK;
+}
+
+}  // extern "C"
+
+static pt_status stage_rays(pt_ctx* ctx, const pt_rays_soa* rays, uint32_t n) {
+    const float* src[7] = {rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz, rays->tmax};
+    for (int k = 0; k < 7; k++) if (!src[k]) return fail(PT_E_INVALID, "ray array " + std::to_string(k) + " is null");
+    size_t bytes = (size_t)n * 7 * sizeof(float);
+    if (ctx->rays_tmp.bytes < bytes) {
+        if (ctx->rays_tmp.p) HIP_TRY(hipFree(ctx->rays_tmp.p));
+        ctx->rays_tmp.p = nullptr;
+        HIP_TRY(hipMalloc(&ctx->rays_tmp.p, bytes));
+        ctx->rays_tmp.bytes = bytes;
+    }
+    for (int k = 0; k < 7; k++)
+        HIP_TRY(hipMemcpyAsync((float*)ctx->rays_tmp.p + (size_t)k * n, src[k], (size_t)n * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    return PT_OK;
+}
+static pt_status ensure_buf(DevBuf& b, size_t bytes) {
+    if (b.bytes >= bytes) return PT_OK;
+    if (b.p) HIP_TRY(hipFree(b.p));
+    b.p = nullptr;
+    HIP_TRY(hipMalloc(&b.p, bytes));
+    b.bytes = bytes;
+    return PT_OK;
+}
+
+extern "C" {
+
+pt_status pt_trace_closest(pt_ctx* ctx, const pt_rays_soa* rays, uint32_t n, float* t_out, int32_t* prim_out) {
+    if (!ctx || !rays || ((!t_out || !prim_out) && n)) return fail(PT_E_INVALID, "pt_trace_closest: null argument");
+    if (!ctx->has_scene) return fail(PT_E_MISSING_INPUT, "missing input: no scene uploaded (pt_scene_upload)");
+    if (n == 0) return PT_OK;
+    HIP_TRY(hipSetDevice(ctx->device));
+    pt_status st;
+    if ((st = stage_rays(ctx, rays, n)) != PT_OK) return st;
+    if ((st = ensure_buf(ctx->t_tmp, (size_t)n * 4)) != PT_OK) return st;
+    if ((st = ensure_buf(ctx->p_tmp, (size_t)n * 4)) != PT_OK) return st;
+    if ((st = pt_trace_closest_device(ctx, (const float*)ctx->rays_tmp.p, n, (float*)ctx->t_tmp.p, (int32_t*)ctx->p_tmp.p)) != PT_OK) return st;
+    HIP_TRY(hipMemcpyAsync(t_out, ctx->t_tmp.p, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(prim_out, ctx->p_tmp.p, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return PT_OK;
+}
+
+pt_status pt_trace_any(pt_ctx* ctx, const pt_rays_soa* rays, uint32_t n, uint8_t* hit_out) {
+    if (!ctx || !rays || (!hit_out && n)) return fail(PT_E_INVALID, "pt_trace_any: null argument");
+    if (!ctx->has_scene) return fail(PT_E_MISSING_INPUT, "missing input: no scene uploaded (pt_scene_upload)");
+    if (n == 0) return PT_OK;
+    HIP_TRY(hipSetDevice(ctx->device));
+    pt_status st;
+    if ((st = stage_rays(ctx, rays, n)) != PT_OK) return st;
+    if ((st = ensure_buf(ctx->p_tmp, (size_t)n * 4)) != PT_OK) return st;
+    if ((st = pt_trace_any_device(ctx, (const float*)ctx->rays_tmp.p, n, (uint8_t*)ctx->p_tmp.p)) != PT_OK) return st;
+    HIP_TRY(hipMemcpyAsync(hit_out, ctx->p_tmp.p, n, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return PT_OK;
+}
+
+pt_status pt_set_timing(pt_ctx* ctx, int enabled) {
+    if (!ctx) return fail(PT_E_INVALID, "pt_set_timing: null ctx");
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    for (auto& kv : ctx->timings)
+        for (auto& pr : kv.second.ev) { (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second); }
+    ctx->timings.clear();
+    ctx->timing_order.clear();
+    ctx->timing = enabled != 0;
+    return PT_OK;
+}
+
+int pt_get_timings(pt_ctx* ctx, pt_kernel_timing* out, int n_slots) {
+    if (!ctx) return -1;
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) return -1;
+    int k = 0;
+    for (const std::string& name : ctx->timing_order) {
+        TimingSlot& s = ctx->timings[name];
+        for (auto& pr : s.ev) {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess) s.ms += ms;
+            (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second);
+        }
+        s.ev.clear();
+        if (out && k < n_slots) {
+            std::memset(&out[k], 0, sizeof(pt_kernel_timing));
+            std::strncpy(out[k].name, name.c_str(), sizeof(out[k].name) - 1);
+            out[k].total_ms = s.ms;
+            out[k].launches = s.launches;
+            out[k].items = s.items;
+        }
+        k++;
+    }
+    return k;
+}
+
+pt_status pt_last_ray_counts(pt_renderer* r, uint64_t* closest, uint64_t* any) {
+    if (!r) return fail(PT_E_INVALID, "pt_last_ray_counts: null renderer");
+    unsigned long long h[2] = {0, 0};
+    if (r->b_hist.p) {
+        HIP_TRY(hipMemcpyAsync(h, r->b_hist.p, sizeof h, hipMemcpyDeviceToHost, r->ctx->stream));
+        HIP_TRY(hipStreamSynchronize(r->ctx->stream));
+    }
+    if (closest) *closest = h[0];
+    if (any) *any = h[1];
+    return PT_OK;
+}
+
+// ---- instrumentation / introspection used by tests and the bench (not in the reference) ----
+
+// Replay a render's extend kernels with node/triangle counters (identical rays, untimed).
+pt_status pt_renderer_set_traversal_counting(pt_renderer* r, int enabled) {
+    if (!r) return fail(PT_E_INVALID, "null renderer");
+    r->count_traversal = enabled != 0;
+    return PT_OK;
+}
+pt_status pt_last_traversal_counts(pt_renderer* r, uint64_t* nodes, uint64_t* tris) {
+    if (!r) return fail(PT_E_INVALID, "null renderer");
+    unsigned long long h[2] = {0, 0};
+    if (r->b_trav.p) {
+        HIP_TRY(hipMemcpyAsync(h, r->b_trav.p, sizeof h, hipMemcpyDeviceToHost, r->ctx->stream));
+        HIP_TRY(hipStreamSynchronize(r->ctx->stream));
+    }
+    *nodes = h[0]; *tris = h[1];
+    return PT_OK;
+}
+// Closest-hit trace with node/triangle counters, device-resident rays.
+pt_status pt_trace_closest_device_counted(pt_ctx* ctx, const float* d_rays, uint32_t n, float* d_t, int32_t* d_prim,
+                                          uint64_t* nodes, uint64_t* tris) {
+    if (!ctx || !ctx->has_scene) return fail(PT_E_MISSING_INPUT, "missing input: no scene uploaded");
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipMemsetAsync(ctx->trav.p, 0, ctx->trav.bytes, ctx->stream));
+    unsigned long long* trav = (unsigned long long*)ctx->trav.p;
+    if (ctx->stack == 16)
+        hipLaunchKernelGGL((k_trace_closest<16, true>), dim3(grid_for(n, TBLOCK)), dim3(TBLOCK), 0, ctx->stream, ctx->dev, d_rays, (int)n, d_t, d_prim, trav);
+    else
+        hipLaunchKernelGGL((k_trace_closest<32, true>), dim3(grid_for(n, TBLOCK)), dim3(TBLOCK), 0, ctx->stream, ctx->dev, d_rays, (int)n, d_t, d_prim, trav);
+    HIP_TRY(hipGetLastError());
+    unsigned long long h[2];
+    HIP_TRY(hipMemcpyAsync(h, trav, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    *nodes = h[0]; *tris = h[1];
+    return PT_OK;
+}
+pt_status pt_scene_stats(pt_ctx* ctx, uint32_t* wide_nodes, uint32_t* wide_depth, uint32_t* stack) {
+    if (!ctx || !ctx->has_scene) return fail(PT_E_MISSING_INPUT, "missing input: no scene uploaded");
+    *wide_nodes = (uint32_t)ctx->dev.n_cw_nodes;
+    *wide_depth = ctx->wide_depth;
+    *stack = (uint32_t)ctx->stack;
+    return PT_OK;
+}
+// Struct sizes as compiled, for the ABI mirror check.
+size_t pt_abi_sizeof(int which) {
+    switch (which) {
+        case 0: return sizeof(pt_object);
+        case 1: return sizeof(pt_material);
+        case 2: return sizeof(pt_light);
+        case 3: return sizeof(pt_env);
+        case 4: return sizeof(pt_scene_desc);
+        case 5: return sizeof(pt_program);
+        case 6: return sizeof(pt_params);
+        default: return 0;
+    }
+}
+
+}  // extern "C"
+
+extern "C" {
+// Host-only: build the wide BVH for tri_vertices and copy it out (nodes: 20 u32 each; tris: 12
+// floats each). Returns the node count; *n_tris_out receives the leaf triangle count. Buffers
+// may be null to query sizes. Used by the CPU-side structural checks of the builder.
+long long pt_debug_wide_bvh(const float* tri_vertices, int n_tris, uint32_t* nodes, long long node_cap,
+                            float* tris, long long tri_cap, long long* n_tris_out, uint32_t* depth) {
+    WideBVH wb;
+    try { build_wide_bvh(tri_vertices, n_tris, wb); } catch (...) { return -1; }
+    long long nn = (long long)wb.n_nodes, nt = (long long)(wb.tris.size() / 12);
+    if (nodes && node_cap >= nn) std::memcpy(nodes, wb.nodes.data(), (size_t)nn * 20 * 4);
+    if (tris && tri_cap >= nt) std::memcpy(tris, wb.tris.data(), (size_t)nt * 48);
+    if (n_tris_out) *n_tris_out = nt;
+    if (depth) *depth = wb.max_depth;
+    return nn;
+}
+}  // extern "C"

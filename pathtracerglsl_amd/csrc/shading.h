@@ -1,0 +1,507 @@
+// shading.h — device side of the walk's per-vertex work: hit fill and region classification,
+// Lambert / smooth dielectric / GGX, quad + point lights with power-CDF selection, the
+// procedural sky with its equirect importance table, and the MIS combiner.
+// Each function restates the GLSL source cited beside it in the same fp32 operation order.
+#pragma once
+#include "dev_common.h"
+#include "traversal.h"
+
+namespace ptmi {
+
+struct Frame { f3 t, b, n; };
+struct Hit {
+    float t; f3 p; Frame frame; f3 ng; float eps;
+    int owner, region_from, region_to;
+};
+
+// fp_uncertainty (glsl/core/math.glsl:52-56)
+PT_DEV float fp_uncertainty(f3 p) {
+    return gmax(1.52587890625e-5f, 3.0517578125e-5f * gmax(fabsf(p.x), gmax(fabsf(p.y), fabsf(p.z))));
+}
+// ambient_frame (components/ambient/euclidean/euclidean.glsl:13-19)
+PT_DEV Frame ambient_frame(f3 n) {
+    Frame f;
+    f3 nn = normalize(n);
+    f3 t = fabsf(nn.x) < 0.9f ? mk3(1.0f, 0.0f, 0.0f) : mk3(0.0f, 1.0f, 0.0f);
+    t = normalize(t - nn * dot(t, nn));
+    f.t = t; f.b = cross(nn, t); f.n = nn;
+    return f;
+}
+
+// ---------------------------------------------------------------- containment
+// mesh_inside_bvh / mesh_closest_bvh (intersection/mesh/mesh.glsl:202-296) over the object's
+// binary BVH; scene_region_at (features/intersection.ts:1121-1265), innermost wins.
+PT_DEV bool mesh_inside(const DevScene& S, int obj, f3 p) {
+    const int base = S.obj_node_base[obj];
+    if (base < 0) return false;
+    const f3 dir = mk3(0.5320544f, 0.7396997f, 0.4114743f);
+    const f3 inv = mk3(1.0f / dir.x, 1.0f / dir.y, 1.0f / dir.z);
+    float tmax = 1.0e20f, side = 0.0f; int best = -1;
+    int stack[64]; int sp = 0; stack[0] = 0;
+    while (sp >= 0) {
+        int ni = stack[sp--];
+        float4 n0 = S.obj_nodes[2 * (base + ni)], n1 = S.obj_nodes[2 * (base + ni) + 1];
+        f3 t0 = (xyz(n0) - p) * inv, t1 = (xyz(n1) - p) * inv;
+        float tn = gmax(gmax(gmin(t0.x, t1.x), gmin(t0.y, t1.y)), gmin(t0.z, t1.z));
+        float tf = gmin(gmin(gmax(t0.x, t1.x), gmax(t0.y, t1.y)), gmax(t0.z, t1.z)) * BVH_TFAR_PAD;
+        float te = gmax(tn, 0.0f);
+        if (!(tf >= te && te <= tmax)) continue;
+        if (n0.w >= 0.0f) {
+            int off = (int)n1.w, cnt = (int)n0.w;
+            int first = S.objects[obj].first_tri;
+            for (int i = off; i < off + cnt; i++) {
+                int tri = S.obj_order[first + i];
+                const float4* tp = S.tri_shade + 3 * (size_t)tri;
+                f3 a = xyz(tp[0]), e1 = xyz(tp[1]) - a, e2 = xyz(tp[2]) - a;
+                float t;
+                if (tri_test(p, dir, a, e1, e2, t) && t > 0.0f && (t < tmax || (t == tmax && tri < best))) {
+                    tmax = t; best = tri; side = dot(dir, cross(e1, e2));
+                }
+            }
+        } else {
+            int axis = (int)(-n0.w - 1.0f);
+            int L = ni + 1, R = (int)n1.w;
+            float da = axis == 0 ? dir.x : (axis == 1 ? dir.y : dir.z);
+            bool near_first = da >= 0.0f;
+            if (sp + 2 < 64) { stack[++sp] = near_first ? R : L; stack[++sp] = near_first ? L : R; }
+        }
+    }
+    return side > 0.0f;
+}
+PT_DEV float point_tri_dist2(f3 p, f3 a, f3 b, f3 c) {
+    f3 ab = b - a, ac = c - a, ap = p - a;
+    float d1 = dot(ab, ap), d2 = dot(ac, ap);
+    if (d1 <= 0.0f && d2 <= 0.0f) { f3 q = p - a; return dot(q, q); }
+    f3 bp = p - b;
+    float d3 = dot(ab, bp), d4 = dot(ac, bp);
+    if (d3 >= 0.0f && d4 <= d3) { f3 q = p - b; return dot(q, q); }
+    float vc = d1 * d4 - d3 * d2;
+    if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) { f3 q = ap - ab * (d1 / (d1 - d3)); return dot(q, q); }
+    f3 cp = p - c;
+    float d5 = dot(ab, cp), d6 = dot(ac, cp);
+    if (d6 >= 0.0f && d5 <= d6) { f3 q = p - c; return dot(q, q); }
+    float vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) { f3 q = ap - ac * (d2 / (d2 - d6)); return dot(q, q); }
+    float va = d3 * d6 - d5 * d4;
+    if (va <= 0.0f && (d4 - d3) >= 0.0f && (d5 - d6) >= 0.0f) {
+        f3 q = bp - (c - b) * ((d4 - d3) / ((d4 - d3) + (d5 - d6)));
+        return dot(q, q);
+    }
+    float denom = 1.0f / (va + vb + vc);
+    f3 closest = (a + ab * (vb * denom)) + ac * (vc * denom);
+    f3 q = p - closest;
+    return dot(q, q);
+}
+PT_DEV float box_dist2(float4 n0, float4 n1, f3 p) {
+    f3 d = mk3(gmax(gmax(n0.x - p.x, 0.0f), p.x - n1.x), gmax(gmax(n0.y - p.y, 0.0f), p.y - n1.y),
+               gmax(gmax(n0.z - p.z, 0.0f), p.z - n1.z));
+    return dot(d, d);
+}
+PT_DEV float mesh_closest(const DevScene& S, int obj, f3 p) {
+    const int base = S.obj_node_base[obj];
+    if (base < 0) return 1.0e20f;
+    float best2 = 1.0e30f;
+    int stack[64]; int sp = 0; stack[0] = 0;
+    while (sp >= 0) {
+        int ni = stack[sp--];
+        float4 n0 = S.obj_nodes[2 * (base + ni)], n1 = S.obj_nodes[2 * (base + ni) + 1];
+        if (box_dist2(n0, n1, p) >= best2) continue;
+        if (n0.w >= 0.0f) {
+            int off = (int)n1.w, cnt = (int)n0.w;
+            int first = S.objects[obj].first_tri;
+            for (int i = off; i < off + cnt; i++) {
+                int tri = S.obj_order[first + i];
+                const float4* tp = S.tri_shade + 3 * (size_t)tri;
+                best2 = gmin(best2, point_tri_dist2(p, xyz(tp[0]), xyz(tp[1]), xyz(tp[2])));
+            }
+        } else {
+            int L = ni + 1, R = (int)n1.w;
+            bool left_near = box_dist2(S.obj_nodes[2 * (base + L)], S.obj_nodes[2 * (base + L) + 1], p) <=
+                             box_dist2(S.obj_nodes[2 * (base + R)], S.obj_nodes[2 * (base + R) + 1], p);
+            if (sp + 2 < 64) { stack[++sp] = left_near ? R : L; stack[++sp] = left_near ? L : R; }
+        }
+    }
+    return __fsqrt_rn(best2);
+}
+PT_DEV int region_at(const DevScene& S, f3 p) {
+    int region = -1; float best = -1.0e20f;
+    for (int k = 0; k < S.n_analytic; k++) {
+        int o = S.analytic[k];
+        const pt_object& ob = S.objects[o];
+        if (ob.kind != PT_OBJ_SPHERE) continue;
+        float d = length(p - ld3(ob.p)) - ob.p[3];
+        if (d < 0.0f && d > best) { best = d; region = o; }
+    }
+    if (S.n_tlas_nodes == 0) return region;
+    int stack[64]; int sp = 0; stack[0] = 0;
+    while (sp >= 0) {
+        int ni = stack[sp--];
+        float4 n0 = S.tlas_nodes[2 * ni], n1 = S.tlas_nodes[2 * ni + 1];
+        if (!(p.x >= n0.x && p.y >= n0.y && p.z >= n0.z && p.x <= n1.x && p.y <= n1.y && p.z <= n1.z)) continue;
+        if (n0.w >= 0.0f) {
+            int off = (int)n1.w, cnt = (int)n0.w;
+            for (int j = 0; j < cnt; j++) {
+                int o = S.tlas_items[off + j];
+                const pt_object& ob = S.objects[o];
+                float d = 1.0e20f;
+                if (p.x >= ob.box_min[0] && p.y >= ob.box_min[1] && p.z >= ob.box_min[2] &&
+                    p.x <= ob.box_max[0] && p.y <= ob.box_max[1] && p.z <= ob.box_max[2] && mesh_inside(S, o, p))
+                    d = -1.0f * mesh_closest(S, o, p);
+                if (d < 0.0f && d > best) { best = d; region = o; }
+            }
+        } else if (sp + 2 < 64) {
+            stack[++sp] = ni + 1; stack[++sp] = (int)n1.w;
+        }
+    }
+    return region;
+}
+PT_DEV bool object_thin(const DevScene& S, int o) {
+    const pt_object& ob = S.objects[o];
+    return ob.kind == PT_OBJ_QUAD || (ob.kind == PT_OBJ_MESH && !ob.closed);
+}
+PT_DEV float ior_of(const DevScene& S, int region) {
+    if (region < 0) return 1.0f;
+    const pt_object& ob = S.objects[region];
+    if (ob.kind == PT_OBJ_QUAD || (ob.kind == PT_OBJ_MESH && !ob.closed)) return 1.0f;
+    const pt_material& m = S.materials[ob.material];
+    return m.model == PT_MAT_DIELECTRIC ? m.ior : 1.0f;
+}
+
+// Hit fill (features/intersection.ts:889-948 mesh wrapper, primitive arms) and the
+// dispatcher's orientation step (features/intersection.ts:1690-1706). The classification
+// probes run only when something reads region_from / region_to: a transmissive material
+// (ior_of) or an emissive solid region (emission keys on region_to).
+PT_DEV void fill_hit(const DevScene& S, f3 o, f3 d, float t, int prim, Hit& h, bool& back) {
+    h.t = t;
+    h.p = o + d * t;
+    h.eps = fp_uncertainty(h.p);
+    if (prim < S.n_tris) {
+        const float4* tp = S.tri_shade + 3 * (size_t)prim;
+        f3 a = xyz(tp[0]);
+        f3 g = cross(xyz(tp[1]) - a, xyz(tp[2]) - a);
+        h.frame = ambient_frame(normalize(g));
+        h.ng = normalize(g);
+        h.owner = S.tri_object[prim];
+    } else {
+        int o2 = prim - S.n_tris;
+        const pt_object& ob = S.objects[o2];
+        f3 n = ob.kind == PT_OBJ_QUAD ? ld3(ob.p + 9) : normalize(h.p - ld3(ob.p));
+        h.frame = ambient_frame(n);
+        h.ng = h.frame.n;
+        h.owner = o2;
+    }
+    back = !(dot(d, h.ng) < 0.0f);
+}
+PT_DEV void classify_hit(const DevScene& S, Hit& h, bool back, bool need) {
+    // h.ng is still the unflipped geometric normal here
+    if (need) {
+        int outside = region_at(S, h.p + h.ng * EPS_INTERFACE);
+        if (!back) { h.region_from = outside; h.region_to = h.owner; }
+        else {
+            h.region_from = object_thin(S, h.owner) ? region_at(S, h.p + h.ng * (-EPS_INTERFACE)) : h.owner;
+            h.region_to = outside;
+        }
+    } else {
+        h.region_from = -2;
+        h.region_to = back ? -2 : h.owner;   // -2: not read (no emissive solid can be behind)
+    }
+    if (back) {
+        h.frame = ambient_frame(-h.frame.n);
+        h.ng = -h.ng;
+    }
+}
+
+// ---------------------------------------------------------------- materials
+enum { LOBE_REFLECTION = 1, LOBE_TRANSMISSION = 2, LOBE_DELTA = 4 };
+struct ISample { f3 wi; f3 weight; float pdf; int flags; };
+
+PT_DEV f3 to_local(const Frame& f, f3 v) { return mk3(dot(v, f.t), dot(v, f.b), dot(v, f.n)); }
+PT_DEV f3 from_local(const Frame& f, f3 v) { return normalize((f.t * v.x + f.b * v.y) + f.n * v.z); }
+// microfacet.glsl:29-54
+PT_DEV float mf_D(f3 h, float a) { float t = h.z * h.z * (a * a - 1.0f) + 1.0f; return a * a / (PTM_PI * t * t); }
+PT_DEV float mf_G1(f3 v, float a) { float c = fabsf(v.z); return 2.0f * c / (c + __fsqrt_rn(a * a + (1.0f - a * a) * c * c)); }
+PT_DEV f3 mf_vndf(f3 wol, float a, float ux, float uy) {
+    f3 vh = normalize(mk3(a * wol.x, a * wol.y, wol.z));
+    float lensq = vh.x * vh.x + vh.y * vh.y;
+    f3 T1 = lensq > 0.0f ? mk3(-vh.y, vh.x, 0.0f) * (1.0f / __fsqrt_rn(lensq)) : mk3(1.0f, 0.0f, 0.0f);
+    f3 T2 = cross(vh, T1);
+    float rr = __fsqrt_rn(ux);
+    float phi = PTM_TWO_PI * uy;
+    float sn, cs; ptm_sincos(phi, &sn, &cs);
+    float t1 = rr * cs;
+    float t2 = rr * sn;
+    float s_ = 0.5f * (1.0f + vh.z);
+    t2 = (1.0f - s_) * __fsqrt_rn(gmax(0.0f, 1.0f - t1 * t1)) + s_ * t2;
+    f3 nh = (T1 * t1 + T2 * t2) + vh * __fsqrt_rn(gmax(0.0f, 1.0f - t1 * t1 - t2 * t2));
+    return normalize(mk3(a * nh.x, a * nh.y, gmax(1e-6f, nh.z)));
+}
+PT_DEV f3 schlick(f3 f0, float c) {
+    float p = ptm_pow(1.0f - c, 5.0f);
+    return mk3(f0.x + (1.0f - f0.x) * p, f0.y + (1.0f - f0.y) * p, f0.z + (1.0f - f0.z) * p);
+}
+PT_DEV float dielectric_fresnel(float cos_i, float eta) {   // math.glsl:95-102
+    float sin2_t = eta * eta * (1.0f - cos_i * cos_i);
+    if (sin2_t >= 1.0f) return 1.0f;
+    float cos_t = __fsqrt_rn(1.0f - sin2_t);
+    float r_par = (cos_i - eta * cos_t) / (cos_i + eta * cos_t);
+    float r_perp = (eta * cos_i - cos_t) / (eta * cos_i + cos_t);
+    return 0.5f * (r_par * r_par + r_perp * r_perp);
+}
+PT_DEV ISample surface_sample(const DevScene& S, const pt_material& m, f3 wo, const Hit& h, float uc, float ux, float uy) {
+    ISample s;
+    const Frame& f = h.frame;
+    if (m.model == PT_MAT_LAMBERT) {            // materials/lambert/lambert.glsl:17-28
+        f3 n = dot(wo, f.n) < 0.0f ? -f.n : f.n;
+        float cos_theta = __fsqrt_rn(uy);
+        float sin_theta = __fsqrt_rn(gmax(0.0f, 1.0f - uy));
+        float phi = PTM_TWO_PI * ux;
+        float sn, cs; ptm_sincos(phi, &sn, &cs);
+        s.wi = normalize((f.t * (sin_theta * cs) + f.b * (sin_theta * sn)) + n * cos_theta);
+        s.weight = ld3(m.albedo);
+        s.pdf = cos_theta * INV_PI;
+        s.flags = LOBE_REFLECTION;
+    } else if (m.model == PT_MAT_DIELECTRIC) {  // materials/dielectric/dielectric.glsl:14-52
+        float n_i = ior_of(S, h.region_from);
+        float n_t = ior_of(S, h.region_to);
+        float eta = n_i / n_t;
+        f3 n = f.n;
+        float cos_i = gclamp(dot(wo, n), 1e-6f, 1.0f);
+        float F = dielectric_fresnel(cos_i, eta);
+        if (uc < F) {
+            s.wi = normalize(n * (2.0f * cos_i) - wo);
+            s.weight = mk3(1.0f, 1.0f, 1.0f);
+            s.pdf = 0.0f;
+            s.flags = LOBE_REFLECTION | LOBE_DELTA;
+        } else {
+            float sin2_t = eta * eta * (1.0f - cos_i * cos_i);
+            float cos_t = __fsqrt_rn(gmax(0.0f, 1.0f - sin2_t));
+            s.wi = normalize(wo * (-eta) + n * (eta * cos_i - cos_t));
+            float radiance_scale = (n_i * n_i) / (n_t * n_t);
+            s.weight = ld3(m.transmittance) * radiance_scale;
+            s.pdf = 0.0f;
+            s.flags = LOBE_TRANSMISSION | LOBE_DELTA;
+        }
+    } else {                                    // materials/ggx/ggx.glsl:27-46
+        f3 wol = to_local(f, wo);
+        float side = wol.z < 0.0f ? -1.0f : 1.0f;
+        wol = wol * side;
+        float a = m.alpha;
+        f3 hh = mf_vndf(wol, a, ux, uy);
+        f3 I = -wol;
+        f3 wil = I - hh * (2.0f * dot(hh, I));
+        if (wil.z <= 0.0f) {
+            s.wi = wo; s.weight = mk3(0.0f, 0.0f, 0.0f); s.pdf = 0.0f; s.flags = LOBE_REFLECTION;
+            return s;
+        }
+        f3 F = schlick(ld3(m.f0), fabsf(dot(wol, hh)));
+        s.weight = F * mf_G1(wil, a);
+        s.pdf = mf_G1(wol, a) * mf_D(hh, a) / (4.0f * fabsf(wol.z));
+        s.flags = LOBE_REFLECTION;
+        s.wi = from_local(f, wil * side);
+    }
+    return s;
+}
+PT_DEV f3 surface_eval(const pt_material& m, f3 wi, f3 wo, const Hit& h) {
+    const Frame& f = h.frame;
+    if (m.model == PT_MAT_LAMBERT) {
+        if (dot(wi, f.n) * dot(wo, f.n) <= 0.0f) return mk3(0.0f, 0.0f, 0.0f);
+        return ld3(m.albedo) * INV_PI;
+    } else if (m.model == PT_MAT_GGX) {
+        f3 wil = to_local(f, wi), wol = to_local(f, wo);
+        if (wil.z * wol.z <= 0.0f) return mk3(0.0f, 0.0f, 0.0f);
+        float a = m.alpha;
+        f3 hh = normalize(wil + wol);
+        f3 F = schlick(ld3(m.f0), fabsf(dot(wol, hh)));
+        return F * (mf_D(hh, a) * mf_G1(wil, a) * mf_G1(wol, a) / (4.0f * fabsf(wil.z) * fabsf(wol.z)));
+    }
+    return mk3(0.0f, 0.0f, 0.0f);
+}
+PT_DEV float surface_pdf(const pt_material& m, f3 wi, f3 wo, const Hit& h) {
+    const Frame& f = h.frame;
+    if (m.model == PT_MAT_LAMBERT) {
+        float c = dot(wi, f.n) * gsign(dot(wo, f.n));
+        return gmax(0.0f, c) * INV_PI;
+    } else if (m.model == PT_MAT_GGX) {
+        f3 wil = to_local(f, wi), wol = to_local(f, wo);
+        if (wil.z * wol.z <= 0.0f) return 0.0f;
+        float a = m.alpha;
+        f3 hh = normalize(wil + wol);
+        return mf_G1(wol, a) * mf_D(hh, a) / (4.0f * fabsf(wol.z));
+    }
+    return 0.0f;
+}
+
+// ---------------------------------------------------------------- environment
+// Procedural sky (proc-sky snapshot, generated-glsl.snapshot.test.ts.snap:63758) and the
+// equirect chart + table sampler (components/env/sampler_cdf.glsl, equirect/*.glsl).
+PT_DEV f3 sky_raw(f3 dir) {
+    f3 sun = normalize(mk3(0.4f, 0.35f, 0.2f));
+    float s = 0.5f * (dir.y + 1.0f);
+    f3 base = mk3(0.06f, 0.08f, 0.12f) * (1.0f - s) + mk3(0.35f, 0.45f, 0.65f) * s;
+    float k = ptm_smoothstep(0.9994f, 0.9999f, dot(dir, sun));
+    return base + mk3(18.0f, 16.0f, 13.0f) * k;
+}
+PT_DEV f3 env_radiance(const DevScene& S, f3 dir) {
+    if (S.env_type != PT_ENV_SKY) return mk3(0.0f, 0.0f, 0.0f);
+    f3 d = normalize(dir);
+    float cx = S.env_rot_c, sy = S.env_rot_s;
+    d = mk3(cx * d.x - sy * d.z, d.y, sy * d.x + cx * d.z);
+    return sky_raw(d) * S.env_intensity;
+}
+PT_DEV float env_texel_dOmega(int j, int W, int H) {
+    float theta = PTM_PI * ((float)j + 0.5f) / (float)H;
+    return (PTM_TWO_PI / (float)W) * 2.0f * ptm_sin(theta) * ptm_sin(0.5f * PTM_PI / (float)H);
+}
+PT_DEV float env_table_pdf(const DevScene& S, f3 dir) {
+    const int W = S.env_w, H = S.env_h;
+    f3 n = normalize(dir);
+    float phi = (n.x == 0.0f && n.z == 0.0f ? 0.0f : ptm_atan2(n.z, n.x)) + S.env_rotation;
+    float theta = ptm_atan2(__fsqrt_rn(n.x * n.x + n.z * n.z), n.y);
+    float u = ptm_fract(phi * (1.0f / PTM_TWO_PI) + 0.5f);
+    float v = theta * (1.0f / PTM_PI);
+    int i = (int)(u * (float)W); i = i < 0 ? 0 : (i > W - 1 ? W - 1 : i);
+    int j = (int)(v * (float)H); j = j < 0 ? 0 : (j > H - 1 ? H - 1 : j);
+    float dMarg = S.env_marg[j] - (j > 0 ? S.env_marg[j - 1] : 0.0f);
+    float dCond = S.env_cond[(size_t)j * W + i] - (i > 0 ? S.env_cond[(size_t)j * W + i - 1] : 0.0f);
+    return (dCond * dMarg) / env_texel_dOmega(j, W, H);
+}
+PT_DEV float env_uniform_fraction(const DevScene& S) {
+    if (S.env_compensated) return 0.0f;
+    return S.env_marg[S.env_h - 1] > 0.0f ? 0.0625f : 1.0f;
+}
+PT_DEV float environment_pdf(const DevScene& S, f3 dir) {
+    float a = env_uniform_fraction(S);
+    return (1.0f - a) * env_table_pdf(S, dir) + a / (4.0f * PTM_PI);
+}
+PT_DEV f3 env_sample_table(const DevScene& S, float x0, float x1) {
+    const int W = S.env_w, H = S.env_h;
+    int lo = 0, hi = H - 1;
+    while (lo < hi) { int mid = (lo + hi) >> 1; if (x0 >= S.env_marg[mid]) lo = mid + 1; else hi = mid; }
+    int j = lo;
+    float mLo = (j > 0) ? S.env_marg[j - 1] : 0.0f;
+    float dMarg = S.env_marg[j] - mLo;
+    float vFrac = (dMarg > 0.0f) ? gclamp((x0 - mLo) / dMarg, 0.0f, 0.9999999f) : 0.5f;
+    const float* row = S.env_cond + (size_t)j * W;
+    lo = 0; hi = W - 1;
+    while (lo < hi) { int mid = (lo + hi) >> 1; if (x1 >= row[mid]) lo = mid + 1; else hi = mid; }
+    int i = lo;
+    float cLo = (i > 0) ? row[i - 1] : 0.0f;
+    float dCond = row[i] - cLo;
+    float uFrac = (dCond > 0.0f) ? gclamp((x1 - cLo) / dCond, 0.0f, 0.9999999f) : 0.5f;
+    // env_chart_sample (equirect.glsl:36-48)
+    float phi = (((float)i + uFrac) / (float)W - 0.5f) * PTM_TWO_PI - S.env_rotation;
+    bool north = 2 * j < H;
+    float rowf = north ? (float)j : (float)(H - j - 1);
+    float a = ptm_sin(0.5f * PTM_PI * rowf / (float)H);
+    float b = ptm_sin(0.5f * PTM_PI * (rowf + 1.0f) / (float)H);
+    float tt = north ? vFrac : 1.0f - vFrac;
+    float hh = a * a * (1.0f - tt) + b * b * tt;
+    float y = (north ? 1.0f : -1.0f) * (1.0f - 2.0f * hh);
+    float r = 2.0f * __fsqrt_rn(hh * (1.0f - hh));
+    float sn, cs; ptm_sincos(phi, &sn, &cs);
+    return mk3(r * cs, y, r * sn);
+}
+
+// ---------------------------------------------------------------- lights
+struct LSample { f3 wi; float distance; f3 radiance; float pdf; int flags; int light_id; };
+
+PT_DEV LSample light_sample_one(const pt_light& l, f3 p, float x0, float x1) {
+    LSample ls;
+    ls.light_id = -1;
+    if (l.kind == PT_LIGHT_QUAD) {              // lights/quad/quad.glsl:14-30
+        f3 q = (ld3(l.corner) + ld3(l.edge1) * x0) + ld3(l.edge2) * x1;
+        f3 d = q - p;
+        float d2 = dot(d, d);
+        ls.wi = d * (1.0f / __fsqrt_rn(d2));
+        ls.distance = __fsqrt_rn(d2);
+        ls.radiance = ld3(l.radiance);
+        ls.flags = 0;
+        float cos_l = dot(ld3(l.normal), -ls.wi);
+        if (cos_l <= 0.0f) { ls.pdf = 0.0f; return ls; }
+        ls.pdf = d2 / (l.area * cos_l);
+    } else {                                    // lights/point/point.glsl:11-23
+        f3 d = ld3(l.corner) - p;
+        float d2 = dot(d, d);
+        ls.wi = d * (1.0f / __fsqrt_rn(d2));
+        ls.distance = __fsqrt_rn(d2);
+        ls.radiance = ld3(l.radiance) / d2;
+        ls.pdf = 1.0f; ls.flags = 1;
+    }
+    return ls;
+}
+PT_DEV float light_pdf_one(const pt_light& l, f3 p, f3 light_p, f3 wi) {
+    if (l.kind != PT_LIGHT_QUAD) return 0.0f;
+    float cos_l = dot(ld3(l.normal), -wi);
+    if (cos_l <= 0.0f) return 0.0f;
+    f3 d = light_p - p;
+    return dot(d, d) / (l.area * cos_l);
+}
+PT_DEV bool env_samplable(const DevScene& S) { return S.env_type == PT_ENV_SKY; }
+PT_DEV bool env_select_live(const DevScene& S) { return S.env_type == PT_ENV_SKY && S.n_lights > 0; }
+PT_DEV LSample lighting_sample_finite(const DevScene& S, f3 p, float x0, float x1) {
+    const int n = S.n_lights;
+    if (n == 1) { LSample ls = light_sample_one(S.lights[0], p, x0, x1); ls.light_id = 0; return ls; }
+    int id = n - 1; float sel = S.lights[n - 1].select_pdf, lo = S.lights[n - 2].cdf;
+    for (int i = 0; i < n - 1; i++) {
+        if (x0 < S.lights[i].cdf) { id = i; sel = S.lights[i].select_pdf; lo = i == 0 ? 0.0f : S.lights[i - 1].cdf; break; }
+    }
+    float xr = (x0 - lo) / sel;
+    LSample ls = light_sample_one(S.lights[id], p, gclamp(xr, 0.0f, 0.9999999f), x1);
+    ls.light_id = id;
+    ls.pdf *= sel;
+    return ls;
+}
+PT_DEV LSample environment_sample(const DevScene& S, float x0, float x1) {   // sampler_cdf.glsl:61-80
+    LSample ls;
+    float a = env_uniform_fraction(S);
+    if (x0 < a) {
+        float y = 1.0f - 2.0f * (x0 / a);
+        float r = __fsqrt_rn(gmax(0.0f, 1.0f - y * y));
+        float phi = PTM_TWO_PI * x1;
+        float sn, cs; ptm_sincos(phi, &sn, &cs);
+        ls.wi = mk3(r * cs, y, r * sn);
+    } else {
+        ls.wi = env_sample_table(S, (x0 - a) / (1.0f - a), x1);
+    }
+    ls.distance = MAX_DIST;
+    ls.radiance = env_radiance(S, ls.wi);
+    ls.pdf = environment_pdf(S, ls.wi);
+    ls.flags = 0; ls.light_id = -1;
+    return ls;
+}
+// lighting_sample (features/lighting.ts:546-628): env-only, finite-only or two-stage.
+PT_DEV LSample lighting_sample(const DevScene& S, f3 p, float x0, float x1) {
+    if (S.n_lights == 0) {
+        if (env_samplable(S)) return environment_sample(S, x0, x1);
+        LSample ls; ls.pdf = 0.0f; return ls;
+    }
+    if (!env_select_live(S)) return lighting_sample_finite(S, p, x0, x1);
+    float ps = S.env_select_prob;
+    if (x0 < ps) {
+        LSample ls = environment_sample(S, gclamp(x0 / ps, 0.0f, 0.9999999f), x1);
+        ls.pdf *= ps;
+        return ls;
+    }
+    float fx = gclamp((x0 - ps) / (1.0f - ps), 0.0f, 0.9999999f);
+    LSample ls = lighting_sample_finite(S, p, fx, x1);
+    ls.pdf *= (1.0f - ps);
+    return ls;
+}
+PT_DEV int light_of(const DevScene& S, int region) {
+    if (region < 0) return -1;
+    for (int i = 0; i < S.n_lights; i++) if (S.lights[i].region == region) return i;
+    return -1;
+}
+PT_DEV float lighting_pdf(const DevScene& S, f3 qp, f3 wi, int lid, f3 light_p) {   // lighting.ts:636-673
+    const pt_light& l = S.lights[lid];
+    float sel = S.n_lights == 1 ? 1.0f : l.select_pdf;
+    if (env_select_live(S)) sel = sel * (1.0f - S.env_select_prob);
+    return sel * light_pdf_one(l, qp, light_p, wi);
+}
+PT_DEV float power_heuristic(float pf, float pg) {   // transport/math_mis.glsl:5-13
+    float f2 = pf * pf;
+    if (f2 >= 1.0e38f) return 1.0f;
+    return f2 / gmax(1e-20f, f2 + pg * pg);
+}
+
+}  // namespace ptmi

@@ -1,0 +1,183 @@
+// traversal.h — closest-hit / any-hit queries over the wide BVH + analytic primitives.
+//
+// One ray per lane. A node visit loads the 80-byte node as five 16-byte loads, tests the
+// (up to) eight quantized child boxes against the ray and yields two bit groups: internal
+// children in front-to-back octant order (bits 0..7 of `hits`, position = slot ^ (oct ^ 7))
+// and leaf triangles (24-bit mask relative to tri_base). Pending node groups live on a
+// per-lane stack in LDS (bank-conflict free: entry k of lane t at k*BLOCK + t).
+//
+// Results equal the reference traversal (intersection/mesh/mesh.glsl:41-191 walking
+// accel/bvh/bvh.glsl:28-36 boxes) bit for bit:
+//   * the triangle test is Moller-Trumbore in the reference's operation order, with
+//     e1 = b - a, e2 = c - a precomputed in fp32 (the same roundings the shader does);
+//   * child boxes are decoded with the fp32 ops the builder verified against, and they
+//     contain the exact triangle boxes, so the slab test (the reference's form, far plane
+//     padded by BVH_TFAR_PAD) never rejects a box the exact-box walk would enter;
+//   * bit-equal t ties resolve to the smaller global primitive id (order independent).
+#pragma once
+#include "dev_common.h"
+
+namespace ptmi {
+
+// Moller-Trumbore (mesh.glsl:41-59). Returns true with t when u, v, w >= -1e-6.
+PT_DEV bool tri_test(f3 ro, f3 rd, f3 a, f3 e1, f3 e2, float& t_out) {
+    f3 g = cross(e1, e2);
+    float det = -dot(rd, g);
+    if (fabsf(det) < 1e-12f) return false;
+    float invdet = 1.0f / det;
+    f3 ao = ro - a;
+    f3 dao = cross(ao, rd);
+    float u = dot(e2, dao) * invdet;
+    float v = -dot(e1, dao) * invdet;
+    float t = dot(ao, g) * invdet;
+    float w = 1.0f - u - v;
+    t_out = t;
+    const float E = 1e-6f;
+    return u >= -E && v >= -E && w >= -E;
+}
+
+// Analytic sphere / quad (geometry/sphere/sphere.glsl:28-46, geometry/quad/quad.glsl:9-22).
+PT_DEV bool sphere_intersect(f3 ro, f3 rd, f3 c, float r, float& t) {
+    f3 oc = ro - c;
+    float b = dot(oc, rd);
+    f3 perp = oc - rd * b;
+    float disc = r * r - dot(perp, perp);
+    if (disc < 0.0f) return false;
+    float cc = dot(oc, oc) - r * r;
+    float q = (b >= 0.0f) ? -b - __fsqrt_rn(disc) : -b + __fsqrt_rn(disc);
+    if (q == 0.0f) return false;
+    float ta = q, tb = cc / q;
+    t = (cc < 0.0f) ? gmax(ta, tb) : gmin(ta, tb);
+    return t > 0.0f;
+}
+PT_DEV bool quad_intersect(f3 ro, f3 rd, const float* p, float& t) {
+    f3 corner = ld3(p), e1 = ld3(p + 3), e2 = ld3(p + 6), n = ld3(p + 9);
+    float denom = dot(rd, n);
+    if (fabsf(denom) < 1e-8f) return false;
+    t = dot(corner - ro, n) / denom;
+    if (t <= 0.0f) return false;
+    f3 local = (ro + rd * t) - corner;
+    float e11 = dot(e1, e1), e22 = dot(e2, e2), e12 = dot(e1, e2);
+    float d1 = dot(local, e1), d2 = dot(local, e2);
+    float det = e11 * e22 - e12 * e12;
+    float u = (d1 * e22 - d2 * e12) / det;
+    float v = (d2 * e11 - d1 * e12) / det;
+    return u >= 0.0f && u <= 1.0f && v >= 0.0f && v <= 1.0f;
+}
+
+struct TraceCounters { uint32_t nodes, tris; };
+
+PT_DEV float byte_of(uint32_t w, int k) { return (float)((w >> (8 * k)) & 0xFFu); }
+
+// Wide-BVH walk. ANY: stop at the first hit with 0 < t < tmax. Otherwise shrink tmax and
+// record the best primitive id under the (t, id) rule. STACK entries per lane in `stk`.
+template <bool ANY, int STACK, int BLOCK, bool COUNT>
+PT_DEV bool trace_wide(const DevScene& S, f3 o, f3 d, float& tmax, int& prim, uint2* stk, TraceCounters& cnt) {
+    if (S.n_cw_nodes == 0) return false;
+    const int lane_slot = threadIdx.x;
+    // inverse direction with the reference CWBVH's zero guard (|d| < 1e-20 -> +-1e-20)
+    float dx = fabsf(d.x) < 1e-20f ? copysignf(1e-20f, d.x) : d.x;
+    float dy = fabsf(d.y) < 1e-20f ? copysignf(1e-20f, d.y) : d.y;
+    float dz = fabsf(d.z) < 1e-20f ? copysignf(1e-20f, d.z) : d.z;
+    const float ix = 1.0f / dx, iy = 1.0f / dy, iz = 1.0f / dz;
+    const uint32_t oct = (d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 4u : 0u);
+    const uint32_t oct_inv = oct ^ 7u;
+    bool found = false;
+    int sp = 0;
+    // current node group: child_base + (imask << 8 | hits)
+    uint32_t g_base = 0, g_bits = (1u << 8) | (1u << (0u ^ oct_inv));   // root as slot 0 of a virtual parent
+    for (;;) {
+        uint32_t tri_base = 0, tri_hits = 0;
+        if ((g_bits & 0xFFu) != 0u) {
+            uint32_t k = __builtin_ctz(g_bits & 0xFFu);
+            g_bits &= ~(1u << k);
+            uint32_t slot = k ^ oct_inv;
+            uint32_t imask = g_bits >> 8;
+            uint32_t node = g_base + __popc(imask & ((1u << slot) - 1u));
+            if ((g_bits & 0xFFu) != 0u) {
+                stk[sp * BLOCK + lane_slot] = make_uint2(g_base, g_bits);
+                sp++;
+            }
+            const uint4* np = S.cw_nodes + 5 * (size_t)node;
+            uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3], w4 = np[4];
+            if (COUNT) cnt.nodes++;
+            const float px = __uint_as_float(w0.x), py = __uint_as_float(w0.y), pz = __uint_as_float(w0.z);
+            const float sx = __uint_as_float((w0.w & 0xFFu) << 23);
+            const float sy = __uint_as_float(((w0.w >> 8) & 0xFFu) << 23);
+            const float sz = __uint_as_float(((w0.w >> 16) & 0xFFu) << 23);
+            const uint32_t nimask = w0.w >> 24;
+            uint32_t nhits = 0;
+#pragma unroll
+            for (int s = 0; s < 8; s++) {
+                const uint32_t mw = s < 4 ? w1.z : w1.w;
+                const uint32_t meta = (mw >> (8 * (s & 3))) & 0xFFu;
+                if (meta == 0u) continue;
+                const int b = s & 3;
+                const uint32_t lxw = s < 4 ? w2.x : w2.y, lyw = s < 4 ? w2.z : w2.w;
+                const uint32_t lzw = s < 4 ? w3.x : w3.y, hxw = s < 4 ? w3.z : w3.w;
+                const uint32_t hyw = s < 4 ? w4.x : w4.y, hzw = s < 4 ? w4.z : w4.w;
+                const float lox = px + byte_of(lxw, b) * sx, hix = px + byte_of(hxw, b) * sx;
+                const float loy = py + byte_of(lyw, b) * sy, hiy = py + byte_of(hyw, b) * sy;
+                const float loz = pz + byte_of(lzw, b) * sz, hiz = pz + byte_of(hzw, b) * sz;
+                const float t0x = (lox - o.x) * ix, t1x = (hix - o.x) * ix;
+                const float t0y = (loy - o.y) * iy, t1y = (hiy - o.y) * iy;
+                const float t0z = (loz - o.z) * iz, t1z = (hiz - o.z) * iz;
+                const float tn = gmax(gmax(gmin(t0x, t1x), gmin(t0y, t1y)), gmin(t0z, t1z));
+                const float tf = gmin(gmin(gmax(t0x, t1x), gmax(t0y, t1y)), gmax(t0z, t1z)) * BVH_TFAR_PAD;
+                const float te = gmax(tn, 0.0f);
+                if (tf >= te && te <= tmax) {
+                    if (nimask & (1u << s)) nhits |= 1u << (s ^ oct_inv);
+                    else tri_hits |= ((meta >> 5) & 7u) << (meta & 31u);
+                }
+            }
+            g_base = w1.x;
+            g_bits = (nimask << 8) | nhits;
+            tri_base = w1.y;
+        }
+        while (tri_hits != 0u) {
+            uint32_t k = __builtin_ctz(tri_hits);
+            tri_hits &= tri_hits - 1u;
+            const float4* tp = S.cw_tris + 3 * (size_t)(tri_base + k);
+            float4 ta = tp[0], te1 = tp[1], te2 = tp[2];
+            if (COUNT) cnt.tris++;
+            float t;
+            if (tri_test(o, d, xyz(ta), xyz(te1), xyz(te2), t) && t > 0.0f) {
+                int id = (int)__float_as_uint(ta.w);
+                if (ANY) {
+                    if (t < tmax) return true;
+                } else if (t < tmax || (t == tmax && id < prim)) {
+                    tmax = t; prim = id; found = true;
+                }
+            }
+        }
+        if ((g_bits & 0xFFu) == 0u) {
+            if (sp == 0) break;
+            sp--;
+            uint2 e = stk[sp * BLOCK + lane_slot];
+            g_base = e.x; g_bits = e.y;
+        }
+    }
+    (void)STACK;
+    return found;
+}
+
+// Analytic objects after the meshes; their ids (n_tris + object) exceed every triangle id,
+// so a strict < keeps the (t, id) rule.
+template <bool ANY>
+PT_DEV bool trace_analytic(const DevScene& S, f3 o, f3 d, float& tmax, int& prim) {
+    bool found = false;
+    for (int k = 0; k < S.n_analytic; k++) {
+        int oid = S.analytic[k];
+        const pt_object& ob = S.objects[oid];
+        float t; bool ok;
+        if (ob.kind == PT_OBJ_QUAD) ok = quad_intersect(o, d, ob.p, t);
+        else ok = sphere_intersect(o, d, ld3(ob.p), ob.p[3], t);
+        if (ok && t < tmax) {
+            if (ANY) return true;
+            tmax = t; prim = S.n_tris + oid; found = true;
+        }
+    }
+    return found;
+}
+
+}  // namespace ptmi

@@ -1,0 +1,179 @@
+"""Synthetic SceneDescriptions for the benchmark workloads.
+
+Each builder returns a SceneDescription dict in the shape the reference's authoring layer
+produces (objects with materials, lights, environment, camera; src/compiler/types.ts:561-721),
+with meshes already flattened to world-space triangle soups. All randomness is seeded.
+"""
+import math
+
+import numpy as np
+
+# ---------------------------------------------------------------- Cornell box (workload C0)
+
+
+def _quad_tris(c, e1, e2):
+    """Two triangles covering corner c + s e1 + t e2, wound so cross(e1, e2) is the normal."""
+    c, e1, e2 = (np.asarray(v, np.float64) for v in (c, e1, e2))
+    a, b, d, e = c, c + e1, c + e1 + e2, c + e2
+    return [[a, b, d], [a, d, e]]
+
+
+def _box_tris(lo, hi):
+    """Closed axis-aligned box, outward winding, 12 triangles."""
+    x0, y0, z0 = lo
+    x1, y1, z1 = hi
+    tris = []
+    tris += _quad_tris((x0, y0, z1), (x1 - x0, 0, 0), (0, y1 - y0, 0))      # +z
+    tris += _quad_tris((x1, y0, z0), (x0 - x1, 0, 0), (0, y1 - y0, 0))      # -z
+    tris += _quad_tris((x1, y0, z1), (0, 0, z0 - z1), (0, y1 - y0, 0))      # +x
+    tris += _quad_tris((x0, y0, z0), (0, 0, z1 - z0), (0, y1 - y0, 0))      # -x
+    tris += _quad_tris((x0, y1, z1), (x1 - x0, 0, 0), (0, 0, z0 - z1))      # +y
+    tris += _quad_tris((x0, y0, z0), (x1 - x0, 0, 0), (0, 0, z1 - z0))      # -y
+    return np.asarray(tris, np.float64)
+
+
+def cornell_box(light_radiance=(17.0, 12.0, 4.0)):
+    """Open-front box [-1,1]^3 (10 triangles) + two axis-aligned inner boxes (24 triangles),
+    Lambert white/red/green, a 0.5 x 0.5 ceiling quad light at y = 0.999 facing -y."""
+    white = _quad_tris((-1, -1, 1), (2, 0, 0), (0, 0, -2))        # floor
+    white += _quad_tris((-1, 1, -1), (2, 0, 0), (0, 0, 2))        # ceiling
+    white += _quad_tris((-1, -1, -1), (2, 0, 0), (0, 2, 0))       # back
+    red = _quad_tris((-1, -1, 1), (0, 0, -2), (0, 2, 0))          # left
+    green = _quad_tris((1, -1, -1), (0, 0, 2), (0, 2, 0))         # right
+    return {
+        "id": "cornell",
+        "materials": {
+            "white": {"model": "lambert", "albedo": [0.73, 0.73, 0.73]},
+            "red": {"model": "lambert", "albedo": [0.63, 0.065, 0.05]},
+            "green": {"model": "lambert", "albedo": [0.14, 0.45, 0.09]},
+        },
+        "objects": [
+            {"type": "mesh", "triangles": np.asarray(white), "material": "white", "closed": False},
+            {"type": "mesh", "triangles": np.asarray(red), "material": "red", "closed": False},
+            {"type": "mesh", "triangles": np.asarray(green), "material": "green", "closed": False},
+            {"type": "mesh", "triangles": _box_tris((0.05, -1.0, 0.0), (0.65, -0.4, 0.6)),
+             "material": "white", "closed": True},
+            {"type": "mesh", "triangles": _box_tris((-0.65, -1.0, -0.6), (-0.05, 0.2, 0.0)),
+             "material": "white", "closed": True},
+        ],
+        "lights": [
+            {"type": "quad", "corner": [-0.25, 0.999, -0.25], "edge1": [0.5, 0.0, 0.0],
+             "edge2": [0.0, 0.0, 0.5], "radiance": list(light_radiance)},
+        ],
+        "environment": {"type": "none"},
+        "camera": {"type": "pinhole", "position": [0.0, 0.0, 3.5], "target": [0.0, 0.0, 0.0],
+                   "fov": math.radians(40.0)},
+    }
+
+
+# ---------------------------------------------------------- icosphere cloud (workload C1)
+
+_ICO_T = (1.0 + math.sqrt(5.0)) / 2.0
+_ICO_V = np.array([[-1, _ICO_T, 0], [1, _ICO_T, 0], [-1, -_ICO_T, 0], [1, -_ICO_T, 0],
+                   [0, -1, _ICO_T], [0, 1, _ICO_T], [0, -1, -_ICO_T], [0, 1, -_ICO_T],
+                   [_ICO_T, 0, -1], [_ICO_T, 0, 1], [-_ICO_T, 0, -1], [-_ICO_T, 0, 1]], np.float64)
+_ICO_F = np.array([[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4],
+                   [11, 10, 2], [10, 7, 6], [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8],
+                   [3, 8, 9], [4, 9, 5], [2, 4, 11], [6, 2, 10], [8, 6, 7], [9, 8, 1]], np.int64)
+
+
+def unit_icosphere(detail=4):
+    """Indexed unit icosphere, three.js-style subdivision: every icosahedron face is split
+    into (detail+1)^2 triangles and projected to the sphere (20 (d+1)^2 faces). Shared
+    vertices are deduplicated so the mesh is watertight; winding is outward."""
+    n = detail + 1
+    verts, keys, faces = [], {}, []
+
+    def vid(p):
+        p = p / np.linalg.norm(p)
+        k = tuple(np.round(p, 9))
+        if k not in keys:
+            keys[k] = len(verts)
+            verts.append(p)
+        return keys[k]
+
+    for fa, fb, fc in _ICO_F:
+        a, b, c = _ICO_V[fa], _ICO_V[fb], _ICO_V[fc]
+        grid = {}
+        for i in range(n + 1):
+            for j in range(n + 1 - i):
+                grid[i, j] = vid(a + (b - a) * (i / n) + (c - a) * (j / n))
+        for i in range(n):
+            for j in range(n - i):
+                faces.append((grid[i, j], grid[i + 1, j], grid[i, j + 1]))
+                if i + j + 1 < n:
+                    faces.append((grid[i + 1, j], grid[i + 1, j + 1], grid[i, j + 1]))
+    V = np.asarray(verts)
+    F = np.asarray(faces, np.int64)
+    # outward winding check: (b-a)x(c-a) . a > 0
+    g = np.cross(V[F[:, 1]] - V[F[:, 0]], V[F[:, 2]] - V[F[:, 0]])
+    if np.mean(np.einsum("ij,ij->i", g, V[F[:, 0]])) < 0:
+        F = F[:, [0, 2, 1]]
+    return V, F
+
+
+def _value_noise(p, seed):
+    """Seeded 3D value noise in [0, 1): hashed lattice values, smoothstep trilinear blend."""
+    p = np.asarray(p, np.float64)
+    i0 = np.floor(p).astype(np.int64)
+    f = p - i0
+    w = f * f * (3.0 - 2.0 * f)
+
+    def h(ix, iy, iz):
+        x = (ix * 73856093) ^ (iy * 19349663) ^ (iz * 83492791) ^ (seed * 2654435761)
+        x = (x ^ (x >> 13)) * 1274126177
+        x = x ^ (x >> 16)
+        return (x & 0xFFFFFF).astype(np.float64) / float(1 << 24)
+
+    out = 0.0
+    for dx in (0, 1):
+        for dy in (0, 1):
+            for dz in (0, 1):
+                wx = w[..., 0] if dx else 1.0 - w[..., 0]
+                wy = w[..., 1] if dy else 1.0 - w[..., 1]
+                wz = w[..., 2] if dz else 1.0 - w[..., 2]
+                out = out + wx * wy * wz * h(i0[..., 0] + dx, i0[..., 1] + dy, i0[..., 2] + dz)
+    return out
+
+
+def icosphere_cloud(n_spheres=2000, detail=4, seed=7, box=100.0, displacement=0.15):
+    """2000 noise-displaced detail-4 icospheres (500 triangles each, 1.0 M total), centres
+    uniform in [0, box]^3, radii lognormal(0, 0.5); 60 % GGX (roughness U[0.05, 0.8]),
+    20 % smooth dielectric eta 1.5, 20 % Lambert (albedo U[0.1, 0.9]); procedural sky."""
+    rng = np.random.default_rng(seed)
+    V, F = unit_icosphere(detail)
+    centres = rng.uniform(0.0, box, size=(n_spheres, 3))
+    radii = rng.lognormal(0.0, 0.5, size=n_spheres)
+    kinds = rng.choice(3, size=n_spheres, p=[0.6, 0.2, 0.2])
+    materials, objects = {}, []
+    for s in range(n_spheres):
+        noise = _value_noise(V * 2.5 + rng.uniform(0, 64, size=3), seed + s)
+        r = radii[s] * (1.0 + displacement * (2.0 * noise - 1.0))
+        P = centres[s] + V * r[:, None]
+        name = f"m{s}"
+        if kinds[s] == 0:
+            materials[name] = {"model": "ggx", "f0": rng.uniform(0.3, 0.95, size=3).tolist(),
+                               "roughness": float(rng.uniform(0.05, 0.8))}
+        elif kinds[s] == 1:
+            materials[name] = {"model": "dielectric", "ior": 1.5}
+        else:
+            materials[name] = {"model": "lambert", "albedo": [float(rng.uniform(0.1, 0.9))] * 3}
+        objects.append({"type": "mesh", "triangles": P[F], "material": name, "closed": True})
+    return {
+        "id": f"icosphere-cloud-{n_spheres}",
+        "materials": materials,
+        "objects": objects,
+        "lights": [],
+        "environment": {"type": "procedural", "intensity": 1.0},
+        "camera": {"type": "pinhole", "position": [box * 0.5, box * 0.5, -0.05 * box],
+                   "target": [box * 0.5, box * 0.5, box * 0.5], "fov": math.radians(60.0)},
+    }
+
+
+def random_rays(n, seed=1, box=100.0):
+    """Seeded hit-parity ray set: origins uniform in the box, directions uniform on S^2."""
+    rng = np.random.default_rng(seed)
+    o = rng.uniform(0.0, box, size=(n, 3))
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return o.astype(np.float32), d.astype(np.float32)

@@ -1,0 +1,90 @@
+"""ctypes loader for the CPU oracle (oracle/_build/liboracle.so) — test infrastructure only."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from pathtracerglsl_amd import abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
+
+
+def load(variant=""):
+    path = os.path.join(ORACLE_DIR, "_build", f"liboracle{variant}.so")
+    if not os.path.exists(path):
+        build()
+    lib = C.CDLL(path)
+    vp = C.c_void_p
+    lib.or_scene_create.restype = vp
+    lib.or_scene_create.argtypes = [C.POINTER(abi.PtSceneDesc)]
+    lib.or_scene_destroy.argtypes = [vp]
+    lib.or_trace_closest.argtypes = [vp, vp, C.c_int, vp, vp]
+    lib.or_trace_closest_brute.argtypes = [vp, vp, C.c_int, vp, vp]
+    lib.or_trace_any.argtypes = [vp, vp, C.c_int, vp]
+    lib.or_render.argtypes = [vp, C.POINTER(abi.PtProgram), C.POINTER(abi.PtParams), C.c_uint32,
+                              C.c_int, C.c_int, C.c_int, vp, vp, vp]
+    lib.or_pcg4d.argtypes = [vp, vp, C.c_int]
+    lib.or_random_stream.argtypes = [C.c_uint32] * 4 + [vp, C.c_int]
+    lib.or_math_eval.argtypes = [C.c_int, vp, vp, vp, C.c_int]
+    lib.or_env_table.argtypes = [vp, vp, vp]
+    lib.or_env_total.restype = C.c_double
+    lib.or_env_total.argtypes = [vp]
+    lib.or_region_at.argtypes = [vp, vp]
+    return lib
+
+
+class OracleScene:
+    def __init__(self, lib, arrays):
+        self.lib = lib
+        self.arrays = arrays
+        self._desc = arrays.desc()
+        self.h = lib.or_scene_create(C.byref(self._desc))
+
+    def close(self):
+        if self.h:
+            self.lib.or_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def trace_closest(self, rays, brute=False):
+        n = rays.shape[1]
+        t = np.empty(n, np.float32)
+        p = np.empty(n, np.int32)
+        fn = self.lib.or_trace_closest_brute if brute else self.lib.or_trace_closest
+        fn(self.h, rays.ctypes.data, n, t.ctypes.data, p.ctypes.data)
+        return t, p
+
+    def trace_any(self, rays):
+        n = rays.shape[1]
+        h = np.empty(n, np.uint8)
+        self.lib.or_trace_any(self.h, rays.ctypes.data, n, h.ctypes.data)
+        return h
+
+    def render(self, program, params, nframes, sample_count=0, rows=None, keep_samples=False):
+        W, H = program.width, program.height
+        y0, y1 = rows if rows is not None else (0, H)
+        film = np.zeros((H, W, 4), np.float32)
+        samples = np.zeros((nframes, H, W, 3), np.float32) if keep_samples else None
+        counts = np.zeros(2, np.uint64)
+        self.lib.or_render(self.h, C.byref(program), C.byref(params), sample_count, nframes, y0, y1,
+                           film.ctypes.data, samples.ctypes.data if samples is not None else None,
+                           counts.ctypes.data)
+        return film, samples, counts
+
+    def env_table(self, w=512, h=256):
+        cond = np.zeros((h, w), np.float32)
+        marg = np.zeros(h, np.float32)
+        self.lib.or_env_table(self.h, cond.ctypes.data, marg.ctypes.data)
+        return cond, marg
+
+    def region_at(self, p):
+        p = np.asarray(p, np.float32)
+        return self.lib.or_region_at(self.h, p.ctypes.data)
