@@ -163,6 +163,10 @@ typedef struct pt_rays_soa {
 typedef struct pt_ctx pt_ctx;
 typedef struct pt_renderer pt_renderer;
 
+/* Block until all work queued on the context's stream has finished (the gl.finish the
+ * reference's GPUProfiler emulates with a 1-pixel readPixels, src/engine/GPUProfiler.ts:1-21). */
+pt_status pt_finish(pt_ctx* ctx);
+
 /* Thread-local message describing the last failure on this thread. */
 const char* pt_last_error(void);
 

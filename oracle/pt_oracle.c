@@ -12,7 +12,10 @@
  *   - closest-hit ties at bit-equal t resolve to the smaller global primitive id (the
  *     reference keeps the first one its BVH order visits, intersection/mesh/mesh.glsl:94),
  *     and the node prune therefore admits tenter == tmax (accel/bvh/bvh.glsl:28-36 uses <);
- *   - analytic primitives are tested after the meshes, with the same (t, id) rule.
+ *   - analytic primitives are tested after the meshes, with the same (t, id) rule;
+ *   - slab tests take the inverse direction with the CWBVH zero guard (see guard_inv);
+ *   - BLAS triangle boxes are widened by the Moller-Trumbore tolerance, so every walk equals
+ *     the brute-force closest hit (the reference's bvh.test.ts asserts BVH == brute force).
  * Host-side builders (binned-SAH BVH over f64 boxes, env CDF) follow the TS sources.
  */
 #define _GNU_SOURCE
@@ -233,6 +236,12 @@ static void bvh_build(bvh_t* out, const double* boxes, int n, int leaf_size) {
 }
 static void bvh_free(bvh_t* b) { free(b->nodes); free(b->order); memset(b, 0, sizeof *b); }
 
+/* Inverse ray direction for the slab tests. The BLAS walk (mesh.glsl:167) takes 1/rd, which
+ * for an exactly axis-aligned ray gives 0*inf = NaN in the slab test (undefined in GLSL);
+ * declared rule: the CWBVH walk's guard |d| < 1e-20 -> +-1e-20 (accel/cwbvh, cwbvh.ts). */
+static inline float guard_inv(float d) { return 1.0f / (fabsf(d) < 1e-20f ? copysignf(1e-20f, d) : d); }
+static inline v3 inv_dir(v3 d) { return V(guard_inv(d.x), guard_inv(d.y), guard_inv(d.z)); }
+
 /* bvh_aabb_hit (accel/bvh/bvh.glsl:28-36), prune admitting tenter == tmax (tie rule). */
 static inline int aabb_hit(const float* n, v3 ro, v3 inv, float tmax) {
     v3 t0 = mul(sub(V(n[0], n[1], n[2]), ro), inv);
@@ -291,7 +300,7 @@ static void blas_nearest(const or_scene* S, int obj, v3 ro, v3 rd, float* tmax, 
     const bvh_t* B = &S->blas[obj];
     if (B->n_nodes == 0) return;
     int first = S->objects[obj].first_tri;
-    v3 inv = V(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
+    v3 inv = inv_dir(rd);
     int stack[64]; int ptr = 0; stack[0] = 0;
     while (ptr >= 0) {
         int ni = stack[ptr--];
@@ -320,7 +329,7 @@ static int blas_any(const or_scene* S, int obj, v3 ro, v3 rd, float maxDist) {
     const bvh_t* B = &S->blas[obj];
     if (B->n_nodes == 0) return 0;
     int first = S->objects[obj].first_tri;
-    v3 inv = V(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
+    v3 inv = inv_dir(rd);
     int stack[64]; int ptr = 0; stack[0] = 0;
     while (ptr >= 0) {
         int ni = stack[ptr--];
@@ -375,7 +384,7 @@ static int quad_intersect(v3 ro, v3 rd, const float* p, float* t) {
 static int scene_closest(const or_scene* S, v3 ro, v3 rd, float* tmax) {
     int best = -1;
     if (S->tlas.n_nodes > 0) {
-        v3 inv = V(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
+        v3 inv = inv_dir(rd);
         int stack[64]; int ptr = 0; stack[0] = 0;
         while (ptr >= 0) {
             int ni = stack[ptr--];
@@ -410,7 +419,7 @@ static int scene_any(const or_scene* S, v3 ro, v3 rd, float maxDist) {
         if (ob->kind == PT_OBJ_SPHERE && sphere_intersect(ro, rd, ld3(ob->p), ob->p[3], &t) && t < maxDist) return 1;
     }
     if (S->tlas.n_nodes == 0) return 0;
-    v3 inv = V(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
+    v3 inv = inv_dir(rd);
     int stack[64]; int ptr = 0; stack[0] = 0;
     while (ptr >= 0) {
         int ni = stack[ptr--];
@@ -1117,9 +1126,19 @@ or_scene* or_scene_create(const pt_scene_desc* d) {
         double* boxes = (double*)malloc(sizeof(double) * 6 * (size_t)(n > 0 ? n : 1));
         for (int i = 0; i < n; i++) {
             const float* p = S->tri + 9 * (size_t)(ob->first_tri + i);
+            double ext = 0;
             for (int a = 0; a < 3; a++) {
                 double mn = fmin(fmin(p[a], p[3 + a]), p[6 + a]), mx = fmax(fmax(p[a], p[3 + a]), p[6 + a]);
                 boxes[6 * i + a] = mn; boxes[6 * i + 3 + a] = mx;
+                if (mx - mn > ext) ext = mx - mn;
+            }
+            /* Conservative triangle box: Moller-Trumbore accepts points up to 1e-6 (barycentric)
+             * outside the triangle, so an exact box can cull an accepted hit; widen by 4e-6 of
+             * the extent (and 4e-7 relative to the coordinate) so the walk equals brute force. */
+            for (int a = 0; a < 3; a++) {
+                double m = 4e-6 * ext + 1e-30;
+                boxes[6 * i + a] -= fmax(m, 4e-7 * fabs(boxes[6 * i + a]));
+                boxes[6 * i + 3 + a] += fmax(m, 4e-7 * fabs(boxes[6 * i + 3 + a]));
             }
         }
         bvh_build(&S->blas[o], boxes, n, 2);
@@ -1260,6 +1279,22 @@ void or_math_eval(int fn, const float* x, const float* y, float* out, int n) {
             case 8: out[i] = ptm_acos(x[i]); break;
             case 9: out[i] = ptm_tanh(x[i]); break;
             case 10: out[i] = ptm_atanh(x[i]); break;
+            default: out[i] = NAN;
+        }
+    }
+}
+
+/* Basic-op companions of or_math_eval for the device comparison. */
+void or_basic_eval(int fn, const float* x, const float* y, float* out, int n) {
+    for (int i = 0; i < n; i++) {
+        float a = x[i], b = y[i];
+        switch (fn) {
+            case 20: out[i] = a / b; break;
+            case 21: case 23: out[i] = sqrtf(a); break;
+            case 22: out[i] = 1.0f / sqrtf(a); break;
+            case 24: out[i] = 1.0f / a; break;
+            case 25: out[i] = a * b + a; break;
+            case 26: out[i] = a > b ? a : b; break;
             default: out[i] = NAN;
         }
     }

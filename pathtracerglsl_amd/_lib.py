@@ -12,7 +12,7 @@ LIB_PATH = os.path.join(HERE, "libptmi.so")
 DECLARED = ["pt_last_error", "pt_ctx_create", "pt_ctx_destroy", "pt_scene_upload", "pt_renderer_load",
             "pt_renderer_unload", "pt_render_frames", "pt_read_export", "pt_clear_accumulation",
             "pt_sample_count", "pt_trace_closest", "pt_trace_any", "pt_trace_closest_device",
-            "pt_trace_any_device", "pt_set_timing", "pt_get_timings", "pt_last_ray_counts"]
+            "pt_trace_any_device", "pt_set_timing", "pt_get_timings", "pt_last_ray_counts", "pt_finish"]
 
 _lib = None
 
@@ -35,6 +35,7 @@ def load():
     lib.pt_ctx_create.argtypes = [C.c_int, C.POINTER(vp)]
     lib.pt_ctx_destroy.argtypes = [vp]
     lib.pt_ctx_destroy.restype = None
+    lib.pt_finish.argtypes = [vp]
     lib.pt_scene_upload.argtypes = [vp, C.POINTER(abi.PtSceneDesc)]
     lib.pt_renderer_load.argtypes = [vp, C.POINTER(abi.PtProgram), C.POINTER(vp)]
     lib.pt_renderer_unload.argtypes = [vp]

@@ -29,8 +29,8 @@ PT_DEV f3 operator/(f3 a, float s) { return mk3(a.x / s, a.y / s, a.z / s); }
 PT_DEV f3 operator-(f3 a) { return mk3(-a.x, -a.y, -a.z); }
 PT_DEV float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 PT_DEV f3 cross(f3 a, f3 b) { return mk3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
-PT_DEV float length(f3 a) { return __fsqrt_rn(dot(a, a)); }
-PT_DEV f3 normalize(f3 a) { return a * (1.0f / __fsqrt_rn(dot(a, a))); }
+PT_DEV float length(f3 a) { return sqrtf(dot(a, a)); }
+PT_DEV f3 normalize(f3 a) { return a * (1.0f / sqrtf(dot(a, a))); }
 PT_DEV float gmax(float a, float b) { return a > b ? a : b; }   // GLSL max on non-NaN input
 PT_DEV float gmin(float a, float b) { return a < b ? a : b; }
 PT_DEV float gclamp(float x, float lo, float hi) { return gmin(gmax(x, lo), hi); }

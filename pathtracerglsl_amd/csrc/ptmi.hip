@@ -427,6 +427,13 @@ extern "C" {
 
 const char* pt_last_error(void) { return g_err.c_str(); }
 
+pt_status pt_finish(pt_ctx* ctx) {
+    if (!ctx) return fail(PT_E_INVALID, "pt_finish: null ctx");
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return PT_OK;
+}
+
 pt_status pt_ctx_create(int device_index, pt_ctx** out) {
     if (!out) return fail(PT_E_INVALID, "pt_ctx_create: out is null");
     int n = 0;
@@ -916,5 +923,50 @@ long long pt_debug_wide_bvh(const float* tri_vertices, int n_tris, uint32_t* nod
     if (n_tris_out) *n_tris_out = nt;
     if (depth) *depth = wb.max_depth;
     return nn;
+}
+}  // extern "C"
+
+namespace {
+__global__ void k_debug_math(int fn, const float* x, const float* y, float* out, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float a = x[i], b = y[i], r;
+    switch (fn) {
+        case 0: r = ptm_sin(a); break;
+        case 1: r = ptm_cos(a); break;
+        case 2: r = ptm_exp(a); break;
+        case 3: r = ptm_log(a); break;
+        case 4: r = ptm_atan2(b, a); break;
+        case 5: r = ptm_tan(a); break;
+        case 6: r = ptm_pow(a, b); break;
+        case 7: r = ptm_atan(a); break;
+        case 8: r = ptm_acos(a); break;
+        case 9: r = ptm_tanh(a); break;
+        case 10: r = ptm_atanh(a); break;
+        case 20: r = a / b; break;
+        case 21: r = sqrtf(a); break;
+        case 22: r = 1.0f / sqrtf(a); break;
+        case 23: r = sqrtf(a); break;
+        case 24: r = 1.0f / a; break;
+        case 25: r = a * b + a; break;
+        case 26: r = gmax(a, b); break;
+        default: r = 0.0f;
+    }
+    out[i] = r;
+}
+}  // namespace
+
+extern "C" {
+// Evaluate one declared-arithmetic function (or a basic op) on the GPU; host arrays.
+pt_status pt_debug_math(int fn, const float* x, const float* y, float* out, int n) {
+    float *dx, *dy, *dout;
+    HIP_TRY(hipMalloc(&dx, n * 4)); HIP_TRY(hipMalloc(&dy, n * 4)); HIP_TRY(hipMalloc(&dout, n * 4));
+    HIP_TRY(hipMemcpy(dx, x, n * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dy, y, n * 4, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_debug_math, dim3((n + 255) / 256), dim3(256), 0, 0, fn, dx, dy, dout, n);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpy(out, dout, n * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipFree(dx)); HIP_TRY(hipFree(dy)); HIP_TRY(hipFree(dout));
+    return PT_OK;
 }
 }  // extern "C"

@@ -121,7 +121,7 @@ PT_DEV float mesh_closest(const DevScene& S, int obj, f3 p) {
             if (sp + 2 < 64) { stack[++sp] = left_near ? R : L; stack[++sp] = left_near ? L : R; }
         }
     }
-    return __fsqrt_rn(best2);
+    return sqrtf(best2);
 }
 PT_DEV int region_at(const DevScene& S, f3 p) {
     int region = -1; float best = -1.0e20f;
@@ -219,20 +219,20 @@ PT_DEV f3 to_local(const Frame& f, f3 v) { return mk3(dot(v, f.t), dot(v, f.b), 
 PT_DEV f3 from_local(const Frame& f, f3 v) { return normalize((f.t * v.x + f.b * v.y) + f.n * v.z); }
 // microfacet.glsl:29-54
 PT_DEV float mf_D(f3 h, float a) { float t = h.z * h.z * (a * a - 1.0f) + 1.0f; return a * a / (PTM_PI * t * t); }
-PT_DEV float mf_G1(f3 v, float a) { float c = fabsf(v.z); return 2.0f * c / (c + __fsqrt_rn(a * a + (1.0f - a * a) * c * c)); }
+PT_DEV float mf_G1(f3 v, float a) { float c = fabsf(v.z); return 2.0f * c / (c + sqrtf(a * a + (1.0f - a * a) * c * c)); }
 PT_DEV f3 mf_vndf(f3 wol, float a, float ux, float uy) {
     f3 vh = normalize(mk3(a * wol.x, a * wol.y, wol.z));
     float lensq = vh.x * vh.x + vh.y * vh.y;
-    f3 T1 = lensq > 0.0f ? mk3(-vh.y, vh.x, 0.0f) * (1.0f / __fsqrt_rn(lensq)) : mk3(1.0f, 0.0f, 0.0f);
+    f3 T1 = lensq > 0.0f ? mk3(-vh.y, vh.x, 0.0f) * (1.0f / sqrtf(lensq)) : mk3(1.0f, 0.0f, 0.0f);
     f3 T2 = cross(vh, T1);
-    float rr = __fsqrt_rn(ux);
+    float rr = sqrtf(ux);
     float phi = PTM_TWO_PI * uy;
     float sn, cs; ptm_sincos(phi, &sn, &cs);
     float t1 = rr * cs;
     float t2 = rr * sn;
     float s_ = 0.5f * (1.0f + vh.z);
-    t2 = (1.0f - s_) * __fsqrt_rn(gmax(0.0f, 1.0f - t1 * t1)) + s_ * t2;
-    f3 nh = (T1 * t1 + T2 * t2) + vh * __fsqrt_rn(gmax(0.0f, 1.0f - t1 * t1 - t2 * t2));
+    t2 = (1.0f - s_) * sqrtf(gmax(0.0f, 1.0f - t1 * t1)) + s_ * t2;
+    f3 nh = (T1 * t1 + T2 * t2) + vh * sqrtf(gmax(0.0f, 1.0f - t1 * t1 - t2 * t2));
     return normalize(mk3(a * nh.x, a * nh.y, gmax(1e-6f, nh.z)));
 }
 PT_DEV f3 schlick(f3 f0, float c) {
@@ -242,7 +242,7 @@ PT_DEV f3 schlick(f3 f0, float c) {
 PT_DEV float dielectric_fresnel(float cos_i, float eta) {   // math.glsl:95-102
     float sin2_t = eta * eta * (1.0f - cos_i * cos_i);
     if (sin2_t >= 1.0f) return 1.0f;
-    float cos_t = __fsqrt_rn(1.0f - sin2_t);
+    float cos_t = sqrtf(1.0f - sin2_t);
     float r_par = (cos_i - eta * cos_t) / (cos_i + eta * cos_t);
     float r_perp = (eta * cos_i - cos_t) / (eta * cos_i + cos_t);
     return 0.5f * (r_par * r_par + r_perp * r_perp);
@@ -252,8 +252,8 @@ PT_DEV ISample surface_sample(const DevScene& S, const pt_material& m, f3 wo, co
     const Frame& f = h.frame;
     if (m.model == PT_MAT_LAMBERT) {            // materials/lambert/lambert.glsl:17-28
         f3 n = dot(wo, f.n) < 0.0f ? -f.n : f.n;
-        float cos_theta = __fsqrt_rn(uy);
-        float sin_theta = __fsqrt_rn(gmax(0.0f, 1.0f - uy));
+        float cos_theta = sqrtf(uy);
+        float sin_theta = sqrtf(gmax(0.0f, 1.0f - uy));
         float phi = PTM_TWO_PI * ux;
         float sn, cs; ptm_sincos(phi, &sn, &cs);
         s.wi = normalize((f.t * (sin_theta * cs) + f.b * (sin_theta * sn)) + n * cos_theta);
@@ -274,7 +274,7 @@ PT_DEV ISample surface_sample(const DevScene& S, const pt_material& m, f3 wo, co
             s.flags = LOBE_REFLECTION | LOBE_DELTA;
         } else {
             float sin2_t = eta * eta * (1.0f - cos_i * cos_i);
-            float cos_t = __fsqrt_rn(gmax(0.0f, 1.0f - sin2_t));
+            float cos_t = sqrtf(gmax(0.0f, 1.0f - sin2_t));
             s.wi = normalize(wo * (-eta) + n * (eta * cos_i - cos_t));
             float radiance_scale = (n_i * n_i) / (n_t * n_t);
             s.weight = ld3(m.transmittance) * radiance_scale;
@@ -356,7 +356,7 @@ PT_DEV float env_table_pdf(const DevScene& S, f3 dir) {
     const int W = S.env_w, H = S.env_h;
     f3 n = normalize(dir);
     float phi = (n.x == 0.0f && n.z == 0.0f ? 0.0f : ptm_atan2(n.z, n.x)) + S.env_rotation;
-    float theta = ptm_atan2(__fsqrt_rn(n.x * n.x + n.z * n.z), n.y);
+    float theta = ptm_atan2(sqrtf(n.x * n.x + n.z * n.z), n.y);
     float u = ptm_fract(phi * (1.0f / PTM_TWO_PI) + 0.5f);
     float v = theta * (1.0f / PTM_PI);
     int i = (int)(u * (float)W); i = i < 0 ? 0 : (i > W - 1 ? W - 1 : i);
@@ -397,7 +397,7 @@ PT_DEV f3 env_sample_table(const DevScene& S, float x0, float x1) {
     float tt = north ? vFrac : 1.0f - vFrac;
     float hh = a * a * (1.0f - tt) + b * b * tt;
     float y = (north ? 1.0f : -1.0f) * (1.0f - 2.0f * hh);
-    float r = 2.0f * __fsqrt_rn(hh * (1.0f - hh));
+    float r = 2.0f * sqrtf(hh * (1.0f - hh));
     float sn, cs; ptm_sincos(phi, &sn, &cs);
     return mk3(r * cs, y, r * sn);
 }
@@ -412,8 +412,8 @@ PT_DEV LSample light_sample_one(const pt_light& l, f3 p, float x0, float x1) {
         f3 q = (ld3(l.corner) + ld3(l.edge1) * x0) + ld3(l.edge2) * x1;
         f3 d = q - p;
         float d2 = dot(d, d);
-        ls.wi = d * (1.0f / __fsqrt_rn(d2));
-        ls.distance = __fsqrt_rn(d2);
+        ls.wi = d * (1.0f / sqrtf(d2));
+        ls.distance = sqrtf(d2);
         ls.radiance = ld3(l.radiance);
         ls.flags = 0;
         float cos_l = dot(ld3(l.normal), -ls.wi);
@@ -422,8 +422,8 @@ PT_DEV LSample light_sample_one(const pt_light& l, f3 p, float x0, float x1) {
     } else {                                    // lights/point/point.glsl:11-23
         f3 d = ld3(l.corner) - p;
         float d2 = dot(d, d);
-        ls.wi = d * (1.0f / __fsqrt_rn(d2));
-        ls.distance = __fsqrt_rn(d2);
+        ls.wi = d * (1.0f / sqrtf(d2));
+        ls.distance = sqrtf(d2);
         ls.radiance = ld3(l.radiance) / d2;
         ls.pdf = 1.0f; ls.flags = 1;
     }
@@ -456,7 +456,7 @@ PT_DEV LSample environment_sample(const DevScene& S, float x0, float x1) {   // 
     float a = env_uniform_fraction(S);
     if (x0 < a) {
         float y = 1.0f - 2.0f * (x0 / a);
-        float r = __fsqrt_rn(gmax(0.0f, 1.0f - y * y));
+        float r = sqrtf(gmax(0.0f, 1.0f - y * y));
         float phi = PTM_TWO_PI * x1;
         float sn, cs; ptm_sincos(phi, &sn, &cs);
         ls.wi = mk3(r * cs, y, r * sn);

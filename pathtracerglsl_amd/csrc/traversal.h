@@ -44,7 +44,7 @@ PT_DEV bool sphere_intersect(f3 ro, f3 rd, f3 c, float r, float& t) {
     float disc = r * r - dot(perp, perp);
     if (disc < 0.0f) return false;
     float cc = dot(oc, oc) - r * r;
-    float q = (b >= 0.0f) ? -b - __fsqrt_rn(disc) : -b + __fsqrt_rn(disc);
+    float q = (b >= 0.0f) ? -b - sqrtf(disc) : -b + sqrtf(disc);
     if (q == 0.0f) return false;
     float ta = q, tb = cc / q;
     t = (cc < 0.0f) ? gmax(ta, tb) : gmin(ta, tb);

@@ -89,6 +89,9 @@ class Engine:
         _lib.check(self.lib.pt_last_traversal_counts(h, C.byref(a), C.byref(b)))
         return int(a.value), int(b.value)
 
+    def finish(self):
+        _lib.check(self.lib.pt_finish(self.ctx))
+
     # -- timing --
     def set_timing(self, on):
         _lib.check(self.lib.pt_set_timing(self.ctx, int(bool(on))))

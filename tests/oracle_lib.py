@@ -36,6 +36,7 @@ def load(variant=""):
     lib.or_env_total.restype = C.c_double
     lib.or_env_total.argtypes = [vp]
     lib.or_region_at.argtypes = [vp, vp]
+    lib.or_basic_eval.argtypes = [C.c_int, vp, vp, vp, C.c_int]
     return lib
 
 
