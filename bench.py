@@ -57,6 +57,8 @@ def main():
     ap.add_argument("--frames-in-flight", type=int, default=4)
     ap.add_argument("--cpu-spp", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="run a plain device copy of this size first (counter calibration runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -74,6 +76,13 @@ def main():
     t0 = time.perf_counter()
     eng.load_renderer(comp)
     log(f"[bench] scene upload + wide BVH build {time.perf_counter() - t0:.1f}s {eng.scene_stats()}")
+
+    if args.calib_copy_mib:
+        import ctypes as C
+        eng.lib.pt_calib_copy.argtypes = [C.c_void_p, C.c_size_t, C.c_int]
+        from pathtracerglsl_amd import _lib
+        _lib.check(eng.lib.pt_calib_copy(eng.ctx, args.calib_copy_mib << 20, 3))
+        log(f"[bench] calibration copy {args.calib_copy_mib} MiB x3 done")
 
     for w in range(args.warmup):
         eng.clear_accumulation()
@@ -105,6 +114,11 @@ def main():
     rc_closest, _ = eng.ray_counts()
     eng.set_traversal_counting(False)
 
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "extend_traffic.json")
+    if os.path.exists(tpath) and (args.width, args.height, args.spp) == (1920, 1080, 64):
+        traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
+
     paths = args.width * args.height * args.spp * args.steps
     ms_per_step = elapsed * 1e3 / args.steps
     ext_ms, ext_launches, _ = timings.get("extend", (0.0, 0, 0))
@@ -131,7 +145,7 @@ def main():
                    "spp": args.spp, "max_bounces": 12, "rr_start": 3,
                    "frames_in_flight": args.frames_in_flight, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_extend (closest-hit wide-BVH traversal)",
                      "alg_bytes_per_launch": round(alg_bytes_per_render / max(launches_per_render, 1)),
                      "avg_launch_ms": round(ext_ms_per_render / max(launches_per_render, 1), 4),

@@ -1299,3 +1299,31 @@ void or_basic_eval(int fn, const float* x, const float* y, float* out, int n) {
         }
     }
 }
+
+/* BSDF probe: sample the material at a hit with shading normal n (frame = ambient_frame(n)),
+ * then evaluate f and pdf at the sampled direction. out[12]: wi.xyz, weight.xyz, pdf, flags,
+ * f.xyz (eval at wi), pdf(wi). Region ior: from = -1 (air), to = the material itself. */
+void or_bsdf_probe(const pt_material* m, const float* n, const float* wo, float uc, float ux, float uy, float* out) {
+    or_scene S; memset(&S, 0, sizeof S);
+    pt_object ob; memset(&ob, 0, sizeof ob); ob.kind = PT_OBJ_SPHERE; ob.material = 0; ob.closed = 1;
+    S.objects = &ob; S.n_objects = 1; S.materials = (pt_material*)m; S.n_materials = 1;
+    hit_t h; memset(&h, 0, sizeof h);
+    h.frame = ambient_frame(V(0, 0, 0), V(n[0], n[1], n[2]));
+    h.ng = h.frame.n; h.region_from = -1; h.region_to = 0;
+    v3 w = V(wo[0], wo[1], wo[2]);
+    isample_t s = surface_sample(&S, m, w, &h, uc, ux, uy);
+    v3 f = surface_eval(m, s.wi, w, &h);
+    float p = surface_pdf(m, s.wi, w, &h);
+    float r[12] = {s.wi.x, s.wi.y, s.wi.z, s.weight.x, s.weight.y, s.weight.z, s.pdf, (float)s.flags, f.x, f.y, f.z, p};
+    memcpy(out, r, sizeof r);
+}
+/* Environment probes: pdf of directions, and samples with their pdfs. */
+void or_env_pdf(const or_scene* S, const float* dirs, int n, float* out) {
+    for (int i = 0; i < n; i++) out[i] = environment_pdf(S, V(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]));
+}
+void or_env_sample(const or_scene* S, const float* xi, int n, float* dirs, float* pdf) {
+    for (int i = 0; i < n; i++) {
+        lsample_t ls = environment_sample(S, xi[2 * i], xi[2 * i + 1]);
+        dirs[3 * i] = ls.wi.x; dirs[3 * i + 1] = ls.wi.y; dirs[3 * i + 2] = ls.wi.z; pdf[i] = ls.pdf;
+    }
+}

@@ -970,3 +970,39 @@ pt_status pt_debug_math(int fn, const float* x, const float* y, float* out, int 
     return PT_OK;
 }
 }  // extern "C"
+
+namespace {
+__global__ void __launch_bounds__(256) k_calib_copy(const float4* __restrict__ src, float4* __restrict__ dst, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) dst[i] = src[i];
+}
+}  // namespace
+
+extern "C" {
+// Plain device copy of `bytes` (multiple of 16) repeated `iters` times: the reference kernel of
+// known traffic that scales the FETCH_SIZE / WRITE_SIZE counters of the profiled kernels.
+pt_status pt_calib_copy(pt_ctx* ctx, size_t bytes, int iters) {
+    if (!ctx || bytes % 16 != 0 || iters < 1) return fail(PT_E_INVALID, "pt_calib_copy: bad argument");
+    HIP_TRY(hipSetDevice(ctx->device));
+    void *a, *b;
+    HIP_TRY(hipMalloc(&a, bytes)); HIP_TRY(hipMalloc(&b, bytes));
+    HIP_TRY(hipMemsetAsync(a, 0, bytes, ctx->stream));
+    for (int i = 0; i < iters; i++)
+        hipLaunchKernelGGL(k_calib_copy, dim3(4096), dim3(256), 0, ctx->stream, (const float4*)a, (float4*)b, bytes / 16);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(hipFree(a)); HIP_TRY(hipFree(b));
+    return PT_OK;
+}
+}  // extern "C"
+
+extern "C" {
+// Host-only: the env importance table libptmi builds for `env` (cond W*H, marg H floats).
+int pt_debug_env_table(const pt_env* env, float* cond, float* marg, double* total) {
+    std::vector<float> c, m; double t = 0;
+    build_env_table(*env, c, m, t);
+    std::memcpy(cond, c.data(), c.size() * 4);
+    std::memcpy(marg, m.data(), m.size() * 4);
+    *total = t;
+    return 0;
+}
+}  // extern "C"

@@ -177,3 +177,41 @@ def random_rays(n, seed=1, box=100.0):
     d = rng.normal(size=(n, 3))
     d /= np.linalg.norm(d, axis=1, keepdims=True)
     return o.astype(np.float32), d.astype(np.float32)
+
+
+# ----------------------------------------------------------------- witness scenes (tests)
+
+def mesh_furnace():
+    """F-BOX through the triangle engine (tests/witnesses/scenes/meshWitness.ts:64-81): a
+    cube wound inward, Lambert albedo 0.5 + emission 0.2, camera at the centre. Every path
+    hits a wall at every bounce, so each pixel is exactly sum_{n<=N} 0.2 * 0.5^n -> 0.4."""
+    cube = _box_tris((-1, -1, -1), (1, 1, 1))[:, [0, 2, 1]]      # inward winding
+    return {
+        "id": "mesh-furnace",
+        "materials": {"furnace": {"model": "lambert", "albedo": [0.5, 0.5, 0.5], "emission": [0.2, 0.2, 0.2]}},
+        "objects": [{"type": "mesh", "triangles": cube, "material": "furnace", "closed": False}],
+        "lights": [],
+        "environment": {"type": "none"},
+        "camera": {"type": "pinhole", "position": [0.0, 0.0, 0.0], "target": [0.0, 0.0, -1.0], "fov": 0.8},
+    }
+
+
+def eta_witness():
+    """F-ETA (tests/witnesses/scenes/dielectricWitness.ts:15-53): an emitter under a flat water
+    surface (eta 1.33), camera in air looking down. Centre pixel = (1 - R0) / 1.33^2 = 0.5540;
+    without the eta^2 factor it would read 0.980. The water half-space is a large closed box,
+    the emitter a quad at y = -1 facing up."""
+    water = _box_tris((-60.0, -3.0, -60.0), (60.0, 0.0, 60.0))
+    return {
+        "id": "eta",
+        "materials": {"water": {"model": "dielectric", "ior": 1.33},
+                      "glow": {"model": "lambert", "albedo": [0.0, 0.0, 0.0], "emission": [1.0, 1.0, 1.0]}},
+        "objects": [
+            {"type": "mesh", "triangles": water, "material": "water", "closed": True},
+            {"type": "quad", "corner": [-50.0, -1.0, 50.0], "edge1": [100.0, 0.0, 0.0], "edge2": [0.0, 0.0, -100.0],
+             "material": "glow"},
+        ],
+        "lights": [],
+        "environment": {"type": "none"},
+        "camera": {"type": "pinhole", "position": [0.0, 1.0, 0.05], "target": [0.0, -1.0, 0.0], "fov": 0.6},
+    }

@@ -1,0 +1,44 @@
+"""The reference's exact witnesses and estimator equalities, on the oracle
+(tests/witnesses/index.ts): mesh F-BOX furnace, F-ETA eta^2, pt == nee == mis."""
+import numpy as np
+
+import oracle_lib
+from pathtracerglsl_amd import compiler, scenes
+
+
+def render(oracle, sc, strategy, w, h, spp, rows=None):
+    comp = compiler.compile_scene(sc, strategy, w, h)
+    ors = oracle_lib.OracleScene(oracle, comp.scene)
+    film, samples, counts = ors.render(comp.program, comp.params(), spp, rows=rows, keep_samples=True)
+    return film, samples
+
+
+def test_mesh_furnace_is_exactly_the_partial_sum(oracle):
+    # furnace strategy: no direct lighting, 16 bounces, no RR (witness index.ts:287-317)
+    film, samples = render(oracle, scenes.mesh_furnace(), compiler.RenderStrategy("none", 16, -1), 32, 24, 8)
+    expect = 0.2 * sum(0.5 ** n for n in range(17))
+    assert abs(film[..., :3].mean() - 0.4) < 1e-4
+    assert np.abs(samples - expect).max() < 1e-6
+
+
+def test_eta_squared_centre_pixel(oracle):
+    # F-ETA: centre region (0.47..0.53) mean = 0.5540 +- 1 %; 0.980 without eta^2
+    w = h = 50
+    film, _ = render(oracle, scenes.eta_witness(), compiler.RenderStrategy("none", 4, -1), w, h, 256)
+    c = film[23:27, 23:27, :3].mean()
+    assert abs(c - 0.554) < 0.01 * 0.554, c
+
+
+def test_estimators_agree_on_cornell(oracle):
+    means = {}
+    errs = {}
+    for est in ("none", "nee", "mis"):
+        _, samples = render(oracle, scenes.cornell_box(), compiler.RenderStrategy(est, 8, 3), 48, 48, 64)
+        per = samples.mean(axis=(1, 2))            # one mean per frame (independent streams)
+        means[est] = per.mean(axis=0)
+        errs[est] = per.std(axis=0) / np.sqrt(len(per))
+    for a, b in (("none", "mis"), ("nee", "mis")):
+        z = np.abs(means[a] - means[b]) / np.sqrt(errs[a] ** 2 + errs[b] ** 2)
+        assert (z < 4.5).all(), (a, b, means[a], means[b], z)
+    # MIS/NEE are lower variance than pure path tracing here
+    assert (errs["mis"] <= errs["none"]).all()
