@@ -152,6 +152,7 @@ def main():
                      "nodes_per_query": round(nodes / max(rc_closest, 1), 2),
                      "tris_per_query": round(tris / max(rc_closest, 1), 2)},
         "kernel_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in timings.items()},
+        "shade_ms_per_step": round(sum(v[0] for k, v in timings.items() if k.startswith("shade")) / args.steps, 3),
     }
     if not args.no_cpu:
         log("[bench] CPU baseline")

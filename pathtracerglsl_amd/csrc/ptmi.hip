@@ -38,6 +38,12 @@ constexpr int TBLOCK = 128;     // traversal blocks (2 waves)
 constexpr int SBLOCK = 128;     // shading blocks
 constexpr int MAX_GRID = 8192;
 
+// Shading categories: the extend kernel bins every path by what its hit needs, so each
+// shade launch runs one code path (no material divergence inside a wave).
+enum { CAT_MISS = 0, CAT_LAMBERT = 1, CAT_GGX = 2, CAT_DIELECTRIC = 3, N_CAT = 4 };
+// counts[] slots
+enum { C_RAYQ0 = 0, C_RAYQ1 = 1, C_SHADOW = 2, C_CAT0 = 3, C_EXT_FETCH = 7, C_SHD_FETCH = 8, N_COUNTS = 16 };
+
 struct Paths {
     float4* ray_o;   // origin.xyz
     float4* ray_d;   // direction.xyz
@@ -47,8 +53,9 @@ struct Paths {
     float4* prev;    // prev_query.p.xyz, prev_bsdf_pdf
     int* flags;      // bit 0: prev_was_delta
     float2* hit;     // t, primitive id (bits)
-    int* q[2];
-    int* counts;     // [0], [1]: ray queues; [2]: shadow queue
+    int* q[2];       // ray queues (current / next bounce)
+    int* catq[N_CAT];// shading queues
+    int* counts;
     int* s_path;
     float4* s_o;     // shadow origin.xyz, maxDist
     float4* s_d;     // shadow direction.xyz
@@ -65,7 +72,7 @@ struct Camera {
 
 __global__ void __launch_bounds__(SBLOCK) k_raygen(Paths P, Camera C, int W, int H, int n_paths, uint32_t sample0) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) P.counts[0] = n_paths;
+    if (i == 0) P.counts[C_RAYQ0] = n_paths;
     if (i >= n_paths) return;
     int pix = i % (W * H), frame = i / (W * H);
     int x = pix % W, y = pix / W;
@@ -94,22 +101,84 @@ __global__ void __launch_bounds__(SBLOCK) k_raygen(Paths P, Camera C, int W, int
     P.q[0][i] = i;
 }
 
+// Persistent-lane work distribution: each wave owns a chunk of queue entries and hands them
+// to lanes as they go idle (no per-ray atomics; one atomic per CHUNK entries per wave).
+struct LaneFeeder {
+    uint32_t next, end;
+    bool exhausted;
+    PT_DEV void init() { next = end = 0; exhausted = false; }
+    // Idle lanes get a queue index (or -1). Convergent: every lane of the wave calls it.
+    template <int CHUNK>
+    PT_DEV int feed(int* counter, uint32_t n, bool idle, int min_idle) {
+        uint64_t im = __ballot(idle);
+        int nidle = (int)__popcll(im);
+        if (nidle == 0 || exhausted) return -1;
+        uint64_t am = __ballot(!idle);
+        if (am != 0 && nidle < min_idle) return -1;
+        int lane = (int)__lane_id();
+        if (end == next) {
+            uint32_t base = 0;
+            if (lane == 0) base = (uint32_t)atomicAdd(counter, CHUNK);
+            base = __shfl(base, 0);
+            if (base >= n) { exhausted = true; return -1; }
+            next = base;
+            end = min(base + (uint32_t)CHUNK, n);
+        }
+        uint32_t avail = end - next;
+        uint32_t pre = (uint32_t)__popcll(im & ((1ull << lane) - 1ull));
+        int idx = (idle && pre < avail) ? (int)(next + pre) : -1;
+        next += min((uint32_t)nidle, avail);
+        return idx;
+    }
+};
+
+constexpr int FEED_CHUNK = 256;
+constexpr int FEED_MIN_IDLE = 16;
+
+// extend: closest hit for every ray of the current queue, then bin the path by shading category.
 template <int STACK, bool COUNT>
 __global__ void __launch_bounds__(TBLOCK) k_extend(DevScene S, Paths P, int qsel) {
     __shared__ uint2 stk[STACK * TBLOCK];
-    const int n = P.counts[qsel];
+    const uint32_t n = (uint32_t)P.counts[qsel];
     const int* q = P.q[qsel];
     TraceCounters cnt = {0, 0};
-    for (int base = blockIdx.x * TBLOCK; base < n; base += gridDim.x * TBLOCK) {
-        int k = base + threadIdx.x;
-        if (k >= n) break;
-        int i = q[k];
-        float4 o4 = P.ray_o[i], d4 = P.ray_d[i];
-        f3 o = xyz(o4), d = xyz(d4);
-        float tmax = MAX_DIST; int prim = -1;
-        trace_wide<false, STACK, TBLOCK, COUNT>(S, o, d, tmax, prim, stk, cnt);
-        trace_analytic<false>(S, o, d, tmax, prim);
-        P.hit[i] = make_float2(tmax, __int_as_float(prim));
+    LaneFeeder feed; feed.init();
+    TravState st;
+    bool active = false;
+    int path = -1;
+    const bool have_bvh = S.n_cw_nodes > 0;
+    for (;;) {
+        int k = feed.feed<FEED_CHUNK>(&P.counts[C_EXT_FETCH], n, !active, FEED_MIN_IDLE);
+        if (k >= 0) {
+            path = q[k];
+            float4 o4 = P.ray_o[path], d4 = P.ray_d[path];
+            trace_init(st, xyz(o4), xyz(d4), MAX_DIST);
+            active = true;
+        }
+        if (__ballot(active) == 0) {
+            if (feed.exhausted) break;
+            continue;
+        }
+        bool done = false;
+        if (active) done = have_bvh ? trace_step<false, TBLOCK, COUNT>(S, st, stk, cnt) : true;
+        int cat = -1;
+        if (done) {
+            float tmax = st.tmax; int prim = st.prim;
+            trace_analytic<false>(S, st.o, st.d, tmax, prim);
+            P.hit[path] = make_float2(tmax, __int_as_float(prim));
+            if (prim < 0) cat = CAT_MISS;
+            else {
+                int owner = prim < S.n_tris ? S.tri_object[prim] : prim - S.n_tris;
+                int model = S.materials[S.objects[owner].material].model;
+                cat = model == PT_MAT_LAMBERT ? CAT_LAMBERT : (model == PT_MAT_GGX ? CAT_GGX : CAT_DIELECTRIC);
+            }
+            active = false;
+        }
+#pragma unroll
+        for (int c = 0; c < N_CAT; c++) {
+            int slot = wave_append(&P.counts[C_CAT0 + c], cat == c);
+            if (cat == c) P.catq[c][slot] = path;
+        }
     }
     if (COUNT) {
         atomicAdd(&P.trav[0], (unsigned long long)cnt.nodes);
@@ -117,48 +186,76 @@ __global__ void __launch_bounds__(TBLOCK) k_extend(DevScene S, Paths P, int qsel
     }
 }
 
+// shadow: any hit for every NEE ray; unoccluded contributions are added to their path.
 template <int STACK>
 __global__ void __launch_bounds__(TBLOCK) k_shadow(DevScene S, Paths P) {
     __shared__ uint2 stk[STACK * TBLOCK];
-    const int n = P.counts[2];
+    const uint32_t n = (uint32_t)P.counts[C_SHADOW];
     TraceCounters cnt = {0, 0};
-    for (int base = blockIdx.x * TBLOCK; base < n; base += gridDim.x * TBLOCK) {
-        int k = base + threadIdx.x;
-        if (k >= n) break;
-        float4 o4 = P.s_o[k], d4 = P.s_d[k];
-        f3 o = xyz(o4), d = xyz(d4);
-        float maxd = o4.w; int prim = -1;
-        bool blocked = trace_analytic<true>(S, o, d, maxd, prim);
-        if (!blocked) blocked = trace_wide<true, STACK, TBLOCK, false>(S, o, d, maxd, prim, stk, cnt);
-        if (!blocked) {
-            int i = P.s_path[k];
-            float4 c = P.s_c[k];
-            float4 L = P.rad[i];
-            P.rad[i] = make_float4(L.x + c.x, L.y + c.y, L.z + c.z, 0.0f);
+    LaneFeeder feed; feed.init();
+    TravState st;
+    bool active = false;
+    int k = -1;
+    const bool have_bvh = S.n_cw_nodes > 0;
+    for (;;) {
+        int kk = feed.feed<FEED_CHUNK>(&P.counts[C_SHD_FETCH], n, !active, FEED_MIN_IDLE);
+        if (kk >= 0) {
+            k = kk;
+            float4 o4 = P.s_o[k], d4 = P.s_d[k];
+            f3 o = xyz(o4), d = xyz(d4);
+            float maxd = o4.w; int prim = -1;
+            bool blocked = trace_analytic<true>(S, o, d, maxd, prim);
+            if (!blocked) {
+                if (have_bvh) {
+                    trace_init(st, o, d, maxd);
+                    active = true;
+                } else {                       // no triangles: unoccluded, score now
+                    int i = P.s_path[k];
+                    float4 c = P.s_c[k], L = P.rad[i];
+                    P.rad[i] = make_float4(L.x + c.x, L.y + c.y, L.z + c.z, 0.0f);
+                }
+            }
+        }
+        if (__ballot(active) == 0) {
+            if (feed.exhausted) break;
+            continue;
+        }
+        if (active && trace_step<true, TBLOCK, false>(S, st, stk, cnt)) {
+            if (st.prim < 0) {
+                int i = P.s_path[k];
+                float4 c = P.s_c[k], L = P.rad[i];
+                P.rad[i] = make_float4(L.x + c.x, L.y + c.y, L.z + c.z, 0.0f);
+            }
+            active = false;
         }
     }
 }
 
+// Queue bookkeeping before a bounce (single thread).
 __global__ void k_bookkeep(Paths P, int cur, int next, int first) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
-        if (!first) P.hist[1] += (unsigned long long)P.counts[2];
+        if (!first) P.hist[1] += (unsigned long long)P.counts[C_SHADOW];
         if (cur >= 0) P.hist[0] += (unsigned long long)P.counts[cur];
         if (next >= 0) P.counts[next] = 0;
-        P.counts[2] = 0;
+        P.counts[C_SHADOW] = 0;
+        for (int c = 0; c < N_CAT; c++) P.counts[C_CAT0 + c] = 0;
+        P.counts[C_EXT_FETCH] = 0;
+        P.counts[C_SHD_FETCH] = 0;
     }
 }
 
 // The per-vertex body of transport_trace (integrators/pt/pt.ts:299-464) with the combiner
 // (transport/combiner.ts:23-102) and the kernel/light techniques (techniques/kernel/kernel.glsl,
-// techniques/light/light.glsl:8-28).
-template <int EST>
+// techniques/light/light.glsl:8-28), for the paths of one shading category.
+template <int EST, int CAT>
 __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene S, Paths P, int qsel, int bounce, int max_bounces,
                                                   int rr_start, float rr_max) {
-    const int n = P.counts[qsel];
-    const int* q = P.q[qsel];
+    const int n = P.counts[C_CAT0 + CAT];
+    const int* q = P.catq[CAT];
     int* qn = P.q[qsel ^ 1];
     constexpr bool NEE = EST != PT_EST_PT;
     constexpr bool MIS = EST == PT_EST_MIS;
+    constexpr int MODEL = CAT == CAT_LAMBERT ? PT_MAT_LAMBERT : (CAT == CAT_GGX ? PT_MAT_GGX : PT_MAT_DIELECTRIC);
     for (int base = blockIdx.x * SBLOCK; base < n; base += gridDim.x * SBLOCK) {
         const int k = base + threadIdx.x;
         const bool valid = k < n;
@@ -167,19 +264,14 @@ __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene S, Paths P, int qsel,
         f3 s_o = mk3(0, 0, 0), s_d = mk3(0, 0, 0), s_c = mk3(0, 0, 0);
         float s_max = 0.0f;
         if (valid) {
-            float2 hv = P.hit[i];
-            int prim = __float_as_int(hv.y);
-            f3 ro = xyz(P.ray_o[i]), rd = xyz(P.ray_d[i]);
+            f3 rd = xyz(P.ray_d[i]);
             float4 th = P.thr[i];
             f3 T = xyz(th); float eta_scale = th.w;
             f3 L = xyz(P.rad[i]);
-            uint2 rs = P.rng[i];
-            Rng rng; rng.base = rs.x; rng.dim = rs.y;
             float4 pv = P.prev[i];
             f3 prev_qp = xyz(pv); float prev_pdf = pv.w;
             bool prev_delta = (P.flags[i] & 1) != 0;
-            const bool emitters = S.n_lights > 0;
-            if (prim < 0) {
+            if (CAT == CAT_MISS) {
                 // kernel_score_miss + combiner_w_env
                 float w = 1.0f;
                 if (NEE && env_samplable(S)) {
@@ -187,13 +279,18 @@ __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene S, Paths P, int qsel,
                     else if (MIS) w = power_heuristic(prev_pdf, env_select_live(S) ? S.env_select_prob * environment_pdf(S, rd) : environment_pdf(S, rd));
                     else w = 0.0f;
                 }
-                f3 Le = env_radiance(S, rd);
-                L = L + (T * w) * Le;
+                if (S.env_type != PT_ENV_NONE) L = L + (T * w) * env_radiance(S, rd);
             } else {
+                float2 hv = P.hit[i];
+                const int prim = __float_as_int(hv.y);
+                f3 ro = xyz(P.ray_o[i]);
+                uint2 rs = P.rng[i];
+                Rng rng; rng.base = rs.x; rng.dim = rs.y;
+                const bool emitters = S.n_lights > 0;
                 Hit h; bool back;
                 fill_hit(S, ro, rd, hv.x, prim, h, back);
                 const pt_material& M = S.materials[S.objects[h.owner].material];
-                bool need = M.model == PT_MAT_DIELECTRIC || (S.emissive_solid && back);
+                bool need = CAT == CAT_DIELECTRIC || (S.emissive_solid && back);
                 classify_hit(S, h, back, need);
                 f3 wo = -rd;
                 // kernel_score_emitter_hit
@@ -211,8 +308,8 @@ __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene S, Paths P, int qsel,
                     }
                 }
                 if (bounce < max_bounces) {
-                    // light_sample_direct
-                    if (NEE && M.model != PT_MAT_DIELECTRIC) {
+                    // light_sample_direct (skipped for delta-only materials)
+                    if (NEE && CAT != CAT_DIELECTRIC) {
                         float x0 = rng.next(), x1 = rng.next();
                         LSample ls = lighting_sample(S, h.p, x0, x1);
                         if (ls.pdf > 0.0f) {
@@ -221,17 +318,17 @@ __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene S, Paths P, int qsel,
                             s_o = h.p + (h.ng * side) * h.eps;
                             s_d = normalize(light_p - s_o);
                             s_max = length(light_p - s_o) - SHADOW_BACKOFF;
-                            f3 f = surface_eval(M, ls.wi, wo, h);
+                            f3 f = surface_eval<MODEL>(M, ls.wi, wo, h);
                             float cos_i = fabsf(dot(ls.wi, h.frame.n));
                             float w = 1.0f;
-                            if (MIS && !(ls.flags & 1)) w = power_heuristic(ls.pdf, surface_pdf(M, ls.wi, wo, h));
+                            if (MIS && !(ls.flags & 1)) w = power_heuristic(ls.pdf, surface_pdf<MODEL>(M, ls.wi, wo, h));
                             s_c = ((((T * ls.radiance) * f) * cos_i) * w) / ls.pdf;
                             shadow = true;
                         }
                     }
                     // kernel_sample_continuation
                     float uc = rng.next(), ux = rng.next(), uy = rng.next();
-                    ISample bs = surface_sample(S, M, wo, h, uc, ux, uy);
+                    ISample bs = surface_sample<MODEL>(S, M, wo, h, uc, ux, uy);
                     if (!is_black(bs.weight)) {
                         T = T * bs.weight;
                         prev_delta = (bs.flags & LOBE_DELTA) != 0;
@@ -266,7 +363,7 @@ __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene S, Paths P, int qsel,
         }
         int slot = wave_append(&P.counts[qsel ^ 1], cont);
         if (cont) qn[slot] = i;
-        int sslot = wave_append(&P.counts[2], shadow);
+        int sslot = wave_append(&P.counts[C_SHADOW], shadow);
         if (shadow) {
             P.s_path[sslot] = i;
             P.s_o[sslot] = f4(s_o, s_max);
@@ -369,6 +466,9 @@ struct pt_ctx {
     bool has_scene = false;
     int stack = 16;
     uint32_t wide_depth = 0;
+    int persist_blocks = 2048;   // persistent traversal grid (set from the device's CU count)
+    int shade_blocks = 2048;
+    bool has_ggx = false;
     bool timing = false;
     std::map<std::string, TimingSlot> timings;
     std::vector<std::string> timing_order;
@@ -410,7 +510,7 @@ struct pt_renderer {
     uint32_t sample_count = 0;
     long long cap = 0;    // path slots allocated
     DevBuf b_ray_o, b_ray_d, b_thr, b_rad, b_rng, b_prev, b_flags, b_hit, b_q0, b_q1, b_counts,
-        b_spath, b_so, b_sd, b_sc, b_hist, b_trav;
+        b_spath, b_so, b_sd, b_sc, b_hist, b_trav, b_cat[N_CAT];
     Paths paths{};
     bool count_traversal = false;
 };
@@ -443,6 +543,11 @@ pt_status pt_ctx_create(int device_index, pt_ctx** out) {
     auto* c = new pt_ctx();
     c->device = device_index;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return fail(PT_E_DEVICE, "stream creation failed"); }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device_index) == hipSuccess && prop.multiProcessorCount > 0) {
+        c->persist_blocks = prop.multiProcessorCount * 8;   // 16 waves per CU of 2-wave blocks
+        c->shade_blocks = prop.multiProcessorCount * 8;
+    }
     c->trav.bytes = 2 * sizeof(unsigned long long);
     if (hipMalloc(&c->trav.p, c->trav.bytes) != hipSuccess) { delete c; return fail(PT_E_OOM, "counter allocation failed"); }
     *out = c;
@@ -516,7 +621,11 @@ pt_status pt_scene_upload(pt_ctx* ctx, const pt_scene_desc* sd) {
     if ((st = ctx->upload(sd->materials, (size_t)sd->n_materials, &D.materials)) != PT_OK) return st;
     if ((st = ctx->upload(sd->lights, (size_t)sd->n_lights, &D.lights)) != PT_OK) return st;
     D.n_lights = sd->n_lights;
-    for (int m = 0; m < sd->n_materials; m++) if (sd->materials[m].model == PT_MAT_DIELECTRIC) D.has_transmission = 1;
+    ctx->has_ggx = false;
+    for (int m = 0; m < sd->n_materials; m++) {
+        if (sd->materials[m].model == PT_MAT_DIELECTRIC) D.has_transmission = 1;
+        if (sd->materials[m].model == PT_MAT_GGX) ctx->has_ggx = true;
+    }
     for (int o = 0; o < sd->n_objects; o++) {
         const pt_object& ob = sd->objects[o];
         bool solid = ob.kind == PT_OBJ_SPHERE || (ob.kind == PT_OBJ_MESH && ob.closed);
@@ -611,13 +720,17 @@ static pt_status ensure_paths(pt_renderer* r, long long n) {
     struct { DevBuf* b; size_t elem; } list[] = {
         {&r->b_ray_o, 16}, {&r->b_ray_d, 16}, {&r->b_thr, 16}, {&r->b_rad, 16}, {&r->b_rng, 8}, {&r->b_prev, 16},
         {&r->b_flags, 4}, {&r->b_hit, 8}, {&r->b_q0, 4}, {&r->b_q1, 4}, {&r->b_spath, 4}, {&r->b_so, 16},
-        {&r->b_sd, 16}, {&r->b_sc, 16}};
+        {&r->b_sd, 16}, {&r->b_sc, 16}, {&r->b_cat[0], 4}, {&r->b_cat[1], 4}, {&r->b_cat[2], 4}, {&r->b_cat[3], 4}};
     for (auto& e : list) {
         if (e.b->p) { HIP_TRY(hipFree(e.b->p)); e.b->p = nullptr; }
         e.b->bytes = e.elem * (size_t)n;
         HIP_TRY(hipMalloc(&e.b->p, e.b->bytes));
     }
-    if (!r->b_counts.p) { r->b_counts.bytes = 4 * sizeof(int); HIP_TRY(hipMalloc(&r->b_counts.p, r->b_counts.bytes)); }
+    if (!r->b_counts.p) {
+        r->b_counts.bytes = N_COUNTS * sizeof(int);
+        HIP_TRY(hipMalloc(&r->b_counts.p, r->b_counts.bytes));
+        HIP_TRY(hipMemset(r->b_counts.p, 0, r->b_counts.bytes));
+    }
     if (!r->b_hist.p) { r->b_hist.bytes = 2 * sizeof(unsigned long long); HIP_TRY(hipMalloc(&r->b_hist.p, r->b_hist.bytes)); }
     if (!r->b_trav.p) { r->b_trav.bytes = 2 * sizeof(unsigned long long); HIP_TRY(hipMalloc(&r->b_trav.p, r->b_trav.bytes)); }
     Paths& P = r->paths;
@@ -627,8 +740,23 @@ static pt_status ensure_paths(pt_renderer* r, long long n) {
     P.counts = (int*)r->b_counts.p; P.s_path = (int*)r->b_spath.p; P.s_o = (float4*)r->b_so.p;
     P.s_d = (float4*)r->b_sd.p; P.s_c = (float4*)r->b_sc.p; P.hist = (unsigned long long*)r->b_hist.p;
     P.trav = (unsigned long long*)r->b_trav.p;
+    for (int c = 0; c < N_CAT; c++) P.catq[c] = (int*)r->b_cat[c].p;
     r->cap = n;
     return PT_OK;
+}
+
+template <int EST>
+static void launch_shade(pt_renderer* r, int cur, int b) {
+    pt_ctx* ctx = r->ctx;
+    const pt_program& pg = r->prog;
+    hipStream_t st = ctx->stream;
+    const int g = ctx->shade_blocks;
+    TIMED(ctx, "shade_miss", 0, hipLaunchKernelGGL((k_shade<EST, CAT_MISS>), dim3(g), dim3(SBLOCK), 0, st, ctx->dev, r->paths, cur, b, pg.max_bounces, pg.rr_start, pg.rr_max_survival));
+    TIMED(ctx, "shade_lambert", 0, hipLaunchKernelGGL((k_shade<EST, CAT_LAMBERT>), dim3(g), dim3(SBLOCK), 0, st, ctx->dev, r->paths, cur, b, pg.max_bounces, pg.rr_start, pg.rr_max_survival));
+    if (ctx->has_ggx)
+        TIMED(ctx, "shade_ggx", 0, hipLaunchKernelGGL((k_shade<EST, CAT_GGX>), dim3(g), dim3(SBLOCK), 0, st, ctx->dev, r->paths, cur, b, pg.max_bounces, pg.rr_start, pg.rr_max_survival));
+    if (ctx->dev.has_transmission)
+        TIMED(ctx, "shade_dielectric", 0, hipLaunchKernelGGL((k_shade<EST, CAT_DIELECTRIC>), dim3(g), dim3(SBLOCK), 0, st, ctx->dev, r->paths, cur, b, pg.max_bounces, pg.rr_start, pg.rr_max_survival));
 }
 
 template <int STACK>
@@ -642,23 +770,19 @@ static void launch_wave(pt_renderer* r, const Camera& cam, int frames, uint32_t 
     hipStream_t st = ctx->stream;
     TIMED(ctx, "raygen", n, hipLaunchKernelGGL(k_raygen, dim3((unsigned)((n + SBLOCK - 1) / SBLOCK)), dim3(SBLOCK), 0, st, P, cam, W, H, (int)n, sample0));
     int cur = 0;
+    const int tgrid = ctx->persist_blocks;
     for (int b = 0; b <= pg.max_bounces; b++) {
-        if (r->count_traversal)
-            TIMED(ctx, "extend_count", 0, hipLaunchKernelGGL((k_extend<STACK, true>), dim3(grid_for(n, TBLOCK)), dim3(TBLOCK), 0, st, S, P, cur));
-        else
-            TIMED(ctx, "extend", 0, hipLaunchKernelGGL((k_extend<STACK, false>), dim3(grid_for(n, TBLOCK)), dim3(TBLOCK), 0, st, S, P, cur));
         hipLaunchKernelGGL(k_bookkeep, dim3(1), dim3(64), 0, st, P, cur, cur ^ 1, b == 0 ? 1 : 0);
+        if (r->count_traversal)
+            TIMED(ctx, "extend_count", 0, hipLaunchKernelGGL((k_extend<STACK, true>), dim3(tgrid), dim3(TBLOCK), 0, st, S, P, cur));
+        else
+            TIMED(ctx, "extend", 0, hipLaunchKernelGGL((k_extend<STACK, false>), dim3(tgrid), dim3(TBLOCK), 0, st, S, P, cur));
         switch (pg.estimator) {
-            case PT_EST_PT:
-                TIMED(ctx, "shade", 0, hipLaunchKernelGGL((k_shade<PT_EST_PT>), dim3(grid_for(n, SBLOCK)), dim3(SBLOCK), 0, st, S, P, cur, b, pg.max_bounces, pg.rr_start, pg.rr_max_survival));
-                break;
-            case PT_EST_NEE:
-                TIMED(ctx, "shade", 0, hipLaunchKernelGGL((k_shade<PT_EST_NEE>), dim3(grid_for(n, SBLOCK)), dim3(SBLOCK), 0, st, S, P, cur, b, pg.max_bounces, pg.rr_start, pg.rr_max_survival));
-                break;
-            default:
-                TIMED(ctx, "shade", 0, hipLaunchKernelGGL((k_shade<PT_EST_MIS>), dim3(grid_for(n, SBLOCK)), dim3(SBLOCK), 0, st, S, P, cur, b, pg.max_bounces, pg.rr_start, pg.rr_max_survival));
+            case PT_EST_PT: launch_shade<PT_EST_PT>(r, cur, b); break;
+            case PT_EST_NEE: launch_shade<PT_EST_NEE>(r, cur, b); break;
+            default: launch_shade<PT_EST_MIS>(r, cur, b);
         }
-        TIMED(ctx, "shadow", 0, hipLaunchKernelGGL((k_shadow<STACK>), dim3(grid_for(n, TBLOCK)), dim3(TBLOCK), 0, st, S, P));
+        TIMED(ctx, "shadow", 0, hipLaunchKernelGGL((k_shadow<STACK>), dim3(tgrid), dim3(TBLOCK), 0, st, S, P));
         cur ^= 1;
     }
     hipLaunchKernelGGL(k_bookkeep, dim3(1), dim3(64), 0, st, P, -1, -1, 0);

@@ -247,10 +247,15 @@ PT_DEV float dielectric_fresnel(float cos_i, float eta) {   // math.glsl:95-102
     float r_perp = (eta * cos_i - cos_t) / (eta * cos_i + cos_t);
     return 0.5f * (r_par * r_par + r_perp * r_perp);
 }
+// MODEL < 0: dispatch on m.model at run time; otherwise the material is known to be MODEL
+// (category kernels shade one model per launch).
+template <int MODEL> PT_DEV bool is_model(const pt_material& m, int x) { return MODEL < 0 ? m.model == x : MODEL == x; }
+
+template <int MODEL = -1>
 PT_DEV ISample surface_sample(const DevScene& S, const pt_material& m, f3 wo, const Hit& h, float uc, float ux, float uy) {
     ISample s;
     const Frame& f = h.frame;
-    if (m.model == PT_MAT_LAMBERT) {            // materials/lambert/lambert.glsl:17-28
+    if (is_model<MODEL>(m, PT_MAT_LAMBERT)) {            // materials/lambert/lambert.glsl:17-28
         f3 n = dot(wo, f.n) < 0.0f ? -f.n : f.n;
         float cos_theta = sqrtf(uy);
         float sin_theta = sqrtf(gmax(0.0f, 1.0f - uy));
@@ -260,7 +265,7 @@ PT_DEV ISample surface_sample(const DevScene& S, const pt_material& m, f3 wo, co
         s.weight = ld3(m.albedo);
         s.pdf = cos_theta * INV_PI;
         s.flags = LOBE_REFLECTION;
-    } else if (m.model == PT_MAT_DIELECTRIC) {  // materials/dielectric/dielectric.glsl:14-52
+    } else if (is_model<MODEL>(m, PT_MAT_DIELECTRIC)) {  // materials/dielectric/dielectric.glsl:14-52
         float n_i = ior_of(S, h.region_from);
         float n_t = ior_of(S, h.region_to);
         float eta = n_i / n_t;
@@ -301,12 +306,13 @@ PT_DEV ISample surface_sample(const DevScene& S, const pt_material& m, f3 wo, co
     }
     return s;
 }
+template <int MODEL = -1>
 PT_DEV f3 surface_eval(const pt_material& m, f3 wi, f3 wo, const Hit& h) {
     const Frame& f = h.frame;
-    if (m.model == PT_MAT_LAMBERT) {
+    if (is_model<MODEL>(m, PT_MAT_LAMBERT)) {
         if (dot(wi, f.n) * dot(wo, f.n) <= 0.0f) return mk3(0.0f, 0.0f, 0.0f);
         return ld3(m.albedo) * INV_PI;
-    } else if (m.model == PT_MAT_GGX) {
+    } else if (is_model<MODEL>(m, PT_MAT_GGX)) {
         f3 wil = to_local(f, wi), wol = to_local(f, wo);
         if (wil.z * wol.z <= 0.0f) return mk3(0.0f, 0.0f, 0.0f);
         float a = m.alpha;
@@ -316,12 +322,13 @@ PT_DEV f3 surface_eval(const pt_material& m, f3 wi, f3 wo, const Hit& h) {
     }
     return mk3(0.0f, 0.0f, 0.0f);
 }
+template <int MODEL = -1>
 PT_DEV float surface_pdf(const pt_material& m, f3 wi, f3 wo, const Hit& h) {
     const Frame& f = h.frame;
-    if (m.model == PT_MAT_LAMBERT) {
+    if (is_model<MODEL>(m, PT_MAT_LAMBERT)) {
         float c = dot(wi, f.n) * gsign(dot(wo, f.n));
         return gmax(0.0f, c) * INV_PI;
-    } else if (m.model == PT_MAT_GGX) {
+    } else if (is_model<MODEL>(m, PT_MAT_GGX)) {
         f3 wil = to_local(f, wi), wol = to_local(f, wo);
         if (wil.z * wol.z <= 0.0f) return 0.0f;
         float a = m.alpha;

@@ -69,95 +69,120 @@ struct TraceCounters { uint32_t nodes, tris; };
 
 PT_DEV float byte_of(uint32_t w, int k) { return (float)((w >> (8 * k)) & 0xFFu); }
 
-// Wide-BVH walk. ANY: stop at the first hit with 0 < t < tmax. Otherwise shrink tmax and
-// record the best primitive id under the (t, id) rule. STACK entries per lane in `stk`.
-template <bool ANY, int STACK, int BLOCK, bool COUNT>
-PT_DEV bool trace_wide(const DevScene& S, f3 o, f3 d, float& tmax, int& prim, uint2* stk, TraceCounters& cnt) {
-    if (S.n_cw_nodes == 0) return false;
-    const int lane_slot = threadIdx.x;
+// Per-lane traversal state of the wide-BVH walk, so a persistent kernel can interleave
+// rays: a lane that finishes picks up the next ray while its wave-mates keep walking.
+struct TravState {
+    f3 o, d;
+    float ix, iy, iz, tmax;
+    uint32_t oct_inv, g_base, g_bits;
+    int prim, sp;
+};
+
+PT_DEV void trace_init(TravState& st, f3 o, f3 d, float tmax) {
+    st.o = o; st.d = d; st.tmax = tmax; st.prim = -1; st.sp = 0;
     // inverse direction with the reference CWBVH's zero guard (|d| < 1e-20 -> +-1e-20)
     float dx = fabsf(d.x) < 1e-20f ? copysignf(1e-20f, d.x) : d.x;
     float dy = fabsf(d.y) < 1e-20f ? copysignf(1e-20f, d.y) : d.y;
     float dz = fabsf(d.z) < 1e-20f ? copysignf(1e-20f, d.z) : d.z;
-    const float ix = 1.0f / dx, iy = 1.0f / dy, iz = 1.0f / dz;
-    const uint32_t oct = (d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 4u : 0u);
-    const uint32_t oct_inv = oct ^ 7u;
-    bool found = false;
-    int sp = 0;
-    // current node group: child_base + (imask << 8 | hits)
-    uint32_t g_base = 0, g_bits = (1u << 8) | (1u << (0u ^ oct_inv));   // root as slot 0 of a virtual parent
-    for (;;) {
-        uint32_t tri_base = 0, tri_hits = 0;
-        if ((g_bits & 0xFFu) != 0u) {
-            uint32_t k = __builtin_ctz(g_bits & 0xFFu);
-            g_bits &= ~(1u << k);
-            uint32_t slot = k ^ oct_inv;
-            uint32_t imask = g_bits >> 8;
-            uint32_t node = g_base + __popc(imask & ((1u << slot) - 1u));
-            if ((g_bits & 0xFFu) != 0u) {
-                stk[sp * BLOCK + lane_slot] = make_uint2(g_base, g_bits);
-                sp++;
-            }
-            const uint4* np = S.cw_nodes + 5 * (size_t)node;
-            uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3], w4 = np[4];
-            if (COUNT) cnt.nodes++;
-            const float px = __uint_as_float(w0.x), py = __uint_as_float(w0.y), pz = __uint_as_float(w0.z);
-            const float sx = __uint_as_float((w0.w & 0xFFu) << 23);
-            const float sy = __uint_as_float(((w0.w >> 8) & 0xFFu) << 23);
-            const float sz = __uint_as_float(((w0.w >> 16) & 0xFFu) << 23);
-            const uint32_t nimask = w0.w >> 24;
-            uint32_t nhits = 0;
+    st.ix = 1.0f / dx; st.iy = 1.0f / dy; st.iz = 1.0f / dz;
+    uint32_t oct = (d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 4u : 0u);
+    st.oct_inv = oct ^ 7u;
+    st.g_base = 0;
+    st.g_bits = (1u << 8) | (1u << (0u ^ st.oct_inv));   // root = slot 0 of a virtual parent
+}
+
+// One step: visit the nearest pending node (if any) and test the triangles it exposes.
+// Returns true when the ray is finished: the stack is empty, or (ANY) a hit with t < tmax
+// was found, in which case st.prim is set to its id.
+template <bool ANY, int BLOCK, bool COUNT>
+PT_DEV bool trace_step(const DevScene& S, TravState& st, uint2* stk, TraceCounters& cnt) {
+    const int lane_slot = threadIdx.x;
+    uint32_t tri_base = 0, tri_hits = 0;
+    if ((st.g_bits & 0xFFu) != 0u) {
+        uint32_t k = __builtin_ctz(st.g_bits & 0xFFu);
+        st.g_bits &= ~(1u << k);
+        uint32_t slot = k ^ st.oct_inv;
+        uint32_t imask = st.g_bits >> 8;
+        uint32_t node = st.g_base + __popc(imask & ((1u << slot) - 1u));
+        if ((st.g_bits & 0xFFu) != 0u) {
+            stk[st.sp * BLOCK + lane_slot] = make_uint2(st.g_base, st.g_bits);
+            st.sp++;
+        }
+        const uint4* np = S.cw_nodes + 5 * (size_t)node;
+        uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3], w4 = np[4];
+        if (COUNT) cnt.nodes++;
+        const float px = __uint_as_float(w0.x), py = __uint_as_float(w0.y), pz = __uint_as_float(w0.z);
+        const float sx = __uint_as_float((w0.w & 0xFFu) << 23);
+        const float sy = __uint_as_float(((w0.w >> 8) & 0xFFu) << 23);
+        const float sz = __uint_as_float(((w0.w >> 16) & 0xFFu) << 23);
+        const uint32_t nimask = w0.w >> 24;
+        const f3 o = st.o;
+        uint32_t nhits = 0;
 #pragma unroll
-            for (int s = 0; s < 8; s++) {
-                const uint32_t mw = s < 4 ? w1.z : w1.w;
-                const uint32_t meta = (mw >> (8 * (s & 3))) & 0xFFu;
-                if (meta == 0u) continue;
-                const int b = s & 3;
-                const uint32_t lxw = s < 4 ? w2.x : w2.y, lyw = s < 4 ? w2.z : w2.w;
-                const uint32_t lzw = s < 4 ? w3.x : w3.y, hxw = s < 4 ? w3.z : w3.w;
-                const uint32_t hyw = s < 4 ? w4.x : w4.y, hzw = s < 4 ? w4.z : w4.w;
-                const float lox = px + byte_of(lxw, b) * sx, hix = px + byte_of(hxw, b) * sx;
-                const float loy = py + byte_of(lyw, b) * sy, hiy = py + byte_of(hyw, b) * sy;
-                const float loz = pz + byte_of(lzw, b) * sz, hiz = pz + byte_of(hzw, b) * sz;
-                const float t0x = (lox - o.x) * ix, t1x = (hix - o.x) * ix;
-                const float t0y = (loy - o.y) * iy, t1y = (hiy - o.y) * iy;
-                const float t0z = (loz - o.z) * iz, t1z = (hiz - o.z) * iz;
-                const float tn = gmax(gmax(gmin(t0x, t1x), gmin(t0y, t1y)), gmin(t0z, t1z));
-                const float tf = gmin(gmin(gmax(t0x, t1x), gmax(t0y, t1y)), gmax(t0z, t1z)) * BVH_TFAR_PAD;
-                const float te = gmax(tn, 0.0f);
-                if (tf >= te && te <= tmax) {
-                    if (nimask & (1u << s)) nhits |= 1u << (s ^ oct_inv);
-                    else tri_hits |= ((meta >> 5) & 7u) << (meta & 31u);
-                }
-            }
-            g_base = w1.x;
-            g_bits = (nimask << 8) | nhits;
-            tri_base = w1.y;
-        }
-        while (tri_hits != 0u) {
-            uint32_t k = __builtin_ctz(tri_hits);
-            tri_hits &= tri_hits - 1u;
-            const float4* tp = S.cw_tris + 3 * (size_t)(tri_base + k);
-            float4 ta = tp[0], te1 = tp[1], te2 = tp[2];
-            if (COUNT) cnt.tris++;
-            float t;
-            if (tri_test(o, d, xyz(ta), xyz(te1), xyz(te2), t) && t > 0.0f) {
-                int id = (int)__float_as_uint(ta.w);
-                if (ANY) {
-                    if (t < tmax) return true;
-                } else if (t < tmax || (t == tmax && id < prim)) {
-                    tmax = t; prim = id; found = true;
-                }
+        for (int s = 0; s < 8; s++) {
+            const uint32_t mw = s < 4 ? w1.z : w1.w;
+            const uint32_t meta = (mw >> (8 * (s & 3))) & 0xFFu;
+            const int b = s & 3;
+            const uint32_t lxw = s < 4 ? w2.x : w2.y, lyw = s < 4 ? w2.z : w2.w;
+            const uint32_t lzw = s < 4 ? w3.x : w3.y, hxw = s < 4 ? w3.z : w3.w;
+            const uint32_t hyw = s < 4 ? w4.x : w4.y, hzw = s < 4 ? w4.z : w4.w;
+            // decode: p + q * 2^e. q * 2^e is exact, so one fused multiply-add rounds exactly
+            // like the builder's separate multiply and add.
+            const float lox = __builtin_fmaf(byte_of(lxw, b), sx, px), hix = __builtin_fmaf(byte_of(hxw, b), sx, px);
+            const float loy = __builtin_fmaf(byte_of(lyw, b), sy, py), hiy = __builtin_fmaf(byte_of(hyw, b), sy, py);
+            const float loz = __builtin_fmaf(byte_of(lzw, b), sz, pz), hiz = __builtin_fmaf(byte_of(hzw, b), sz, pz);
+            const float t0x = (lox - o.x) * st.ix, t1x = (hix - o.x) * st.ix;
+            const float t0y = (loy - o.y) * st.iy, t1y = (hiy - o.y) * st.iy;
+            const float t0z = (loz - o.z) * st.iz, t1z = (hiz - o.z) * st.iz;
+            const float tn = gmax(gmax(gmin(t0x, t1x), gmin(t0y, t1y)), gmin(t0z, t1z));
+            const float tf = gmin(gmin(gmax(t0x, t1x), gmax(t0y, t1y)), gmax(t0z, t1z)) * BVH_TFAR_PAD;
+            const float te = gmax(tn, 0.0f);
+            if (meta != 0u && tf >= te && te <= st.tmax) {
+                if (nimask & (1u << s)) nhits |= 1u << (s ^ st.oct_inv);
+                else tri_hits |= ((meta >> 5) & 7u) << (meta & 31u);
             }
         }
-        if ((g_bits & 0xFFu) == 0u) {
-            if (sp == 0) break;
-            sp--;
-            uint2 e = stk[sp * BLOCK + lane_slot];
-            g_base = e.x; g_bits = e.y;
+        st.g_base = w1.x;
+        st.g_bits = (nimask << 8) | nhits;
+        tri_base = w1.y;
+    }
+    while (tri_hits != 0u) {
+        uint32_t k = __builtin_ctz(tri_hits);
+        tri_hits &= tri_hits - 1u;
+        const float4* tp = S.cw_tris + 3 * (size_t)(tri_base + k);
+        float4 ta = tp[0], te1 = tp[1], te2 = tp[2];
+        if (COUNT) cnt.tris++;
+        float t;
+        if (tri_test(st.o, st.d, xyz(ta), xyz(te1), xyz(te2), t) && t > 0.0f) {
+            int id = (int)__float_as_uint(ta.w);
+            if (ANY) {
+                if (t < st.tmax) { st.prim = id; return true; }
+            } else if (t < st.tmax || (t == st.tmax && id < st.prim)) {
+                st.tmax = t; st.prim = id;
+            }
         }
     }
+    if ((st.g_bits & 0xFFu) == 0u) {
+        if (st.sp == 0) return true;
+        st.sp--;
+        uint2 e = stk[st.sp * BLOCK + lane_slot];
+        st.g_base = e.x; st.g_bits = e.y;
+    }
+    return false;
+}
+
+// Whole-ray wide-BVH walk (one ray per lane, no interleaving).
+template <bool ANY, int STACK, int BLOCK, bool COUNT>
+PT_DEV bool trace_wide(const DevScene& S, f3 o, f3 d, float& tmax, int& prim, uint2* stk, TraceCounters& cnt) {
+    if (S.n_cw_nodes == 0) return false;
+    TravState st;
+    trace_init(st, o, d, tmax);
+    st.prim = prim;
+    while (!trace_step<ANY, BLOCK, COUNT>(S, st, stk, cnt)) {}
     (void)STACK;
+    if (ANY) return st.prim >= 0 && st.prim != prim;
+    bool found = st.prim != prim;
+    tmax = st.tmax; prim = st.prim;
     return found;
 }
 
