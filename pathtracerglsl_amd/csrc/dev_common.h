@@ -81,9 +81,15 @@ struct DevScene {
     const float4* obj_nodes;    // binary nodes, 2 float4 each: (min, A) (max, B)
     const int* obj_node_base;   // per object
     const int* obj_order;       // global triangle ids in per-object leaf order, at first_tri
-    const float4* tlas_nodes;   // binary nodes over closed-mesh boxes (leaf size 1)
-    const int* tlas_items;      // leaf item -> object id
-    int n_tlas_nodes;
+    const uint4* ow_nodes;      // per closed mesh wide BVH (same format as cw_nodes), concatenated
+    const float4* ow_tris;      // their leaf triangles (ids are global triangle ids)
+    const int* ow_root;         // per object: root node in ow_nodes, -1 if none
+    // uniform grid over closed-mesh boxes: candidates of scene_region_at's containment test
+    float grid_lo[3], grid_inv[3];
+    int grid_n[3];
+    const int2* grid_cells;     // (first item, count) per cell
+    const int* grid_items;      // object ids
+    int has_grid;
     int has_transmission;
     int emissive_solid;
 };

@@ -35,7 +35,7 @@ using namespace ptmi;
 namespace {
 
 constexpr int TBLOCK = 128;     // traversal blocks (2 waves)
-constexpr int SBLOCK = 128;     // shading blocks
+constexpr int SBLOCK = 256;     // shading blocks
 constexpr int MAX_GRID = 8192;
 
 // Shading categories: the extend kernel bins every path by what its hit needs, so each
@@ -159,25 +159,11 @@ __global__ void __launch_bounds__(TBLOCK) k_extend(DevScene S, Paths P, int qsel
             if (feed.exhausted) break;
             continue;
         }
-        bool done = false;
-        if (active) done = have_bvh ? trace_step<false, TBLOCK, COUNT>(S, st, stk, cnt) : true;
-        int cat = -1;
-        if (done) {
+        if (active && (!have_bvh || trace_step<false, TBLOCK, COUNT>(S.cw_nodes, S.cw_tris, st, stk, cnt))) {
             float tmax = st.tmax; int prim = st.prim;
             trace_analytic<false>(S, st.o, st.d, tmax, prim);
             P.hit[path] = make_float2(tmax, __int_as_float(prim));
-            if (prim < 0) cat = CAT_MISS;
-            else {
-                int owner = prim < S.n_tris ? S.tri_object[prim] : prim - S.n_tris;
-                int model = S.materials[S.objects[owner].material].model;
-                cat = model == PT_MAT_LAMBERT ? CAT_LAMBERT : (model == PT_MAT_GGX ? CAT_GGX : CAT_DIELECTRIC);
-            }
             active = false;
-        }
-#pragma unroll
-        for (int c = 0; c < N_CAT; c++) {
-            int slot = wave_append(&P.counts[C_CAT0 + c], cat == c);
-            if (cat == c) P.catq[c][slot] = path;
         }
     }
     if (COUNT) {
@@ -220,7 +206,7 @@ __global__ void __launch_bounds__(TBLOCK) k_shadow(DevScene S, Paths P) {
             if (feed.exhausted) break;
             continue;
         }
-        if (active && trace_step<true, TBLOCK, false>(S, st, stk, cnt)) {
+        if (active && trace_step<true, TBLOCK, false>(S.cw_nodes, S.cw_tris, st, stk, cnt)) {
             if (st.prim < 0) {
                 int i = P.s_path[k];
                 float4 c = P.s_c[k], L = P.rad[i];
@@ -228,6 +214,43 @@ __global__ void __launch_bounds__(TBLOCK) k_shadow(DevScene S, Paths P) {
             }
             active = false;
         }
+    }
+}
+
+// Block-aggregated queue appends: LDS atomics give each lane its slot inside the block's
+// share, then one global atomic per queue per block. Every thread of the block must call it.
+template <int K>
+PT_DEV int block_append(int* const* counters, int which, int* s_cnt, int* s_base) {
+    if (threadIdx.x < K) s_cnt[threadIdx.x] = 0;
+    __syncthreads();
+    int local = which >= 0 ? atomicAdd(&s_cnt[which], 1) : -1;
+    __syncthreads();
+    if (threadIdx.x < K) s_base[threadIdx.x] = s_cnt[threadIdx.x] ? atomicAdd(counters[threadIdx.x], s_cnt[threadIdx.x]) : 0;
+    __syncthreads();
+    return which >= 0 ? s_base[which] + local : -1;
+}
+
+// bin: sort the bounce's hits into shading-category queues (no material divergence later).
+__global__ void __launch_bounds__(SBLOCK) k_bin(DevScene S, Paths P, int qsel) {
+    __shared__ int s_cnt[N_CAT], s_base[N_CAT];
+    int* ctr[N_CAT] = {&P.counts[C_CAT0], &P.counts[C_CAT0 + 1], &P.counts[C_CAT0 + 2], &P.counts[C_CAT0 + 3]};
+    const int n = P.counts[qsel];
+    const int* q = P.q[qsel];
+    for (int base = blockIdx.x * SBLOCK; base < n; base += gridDim.x * SBLOCK) {
+        int k = base + threadIdx.x;
+        int cat = -1, path = 0;
+        if (k < n) {
+            path = q[k];
+            int prim = __float_as_int(P.hit[path].y);
+            if (prim < 0) cat = CAT_MISS;
+            else {
+                int owner = prim < S.n_tris ? S.tri_object[prim] : prim - S.n_tris;
+                int model = S.materials[S.objects[owner].material].model;
+                cat = model == PT_MAT_LAMBERT ? CAT_LAMBERT : (model == PT_MAT_GGX ? CAT_GGX : CAT_DIELECTRIC);
+            }
+        }
+        int slot = block_append<N_CAT>(ctr, cat, s_cnt, s_base);
+        if (cat >= 0) P.catq[cat][slot] = path;
     }
 }
 
@@ -247,9 +270,13 @@ __global__ void k_bookkeep(Paths P, int cur, int next, int first) {
 // The per-vertex body of transport_trace (integrators/pt/pt.ts:299-464) with the combiner
 // (transport/combiner.ts:23-102) and the kernel/light techniques (techniques/kernel/kernel.glsl,
 // techniques/light/light.glsl:8-28), for the paths of one shading category.
-template <int EST, int CAT>
+template <int EST, int CAT, int STACK>
 __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene S, Paths P, int qsel, int bounce, int max_bounces,
                                                   int rr_start, float rr_max) {
+    // LDS traversal stack for the containment probe (only the dielectric category probes)
+    __shared__ uint2 stk[CAT == CAT_DIELECTRIC ? STACK * SBLOCK : 1];
+    __shared__ int s_cnt[2], s_base[2];
+    int* ctr[2] = {&P.counts[qsel ^ 1], &P.counts[C_SHADOW]};
     const int n = P.counts[C_CAT0 + CAT];
     const int* q = P.catq[CAT];
     int* qn = P.q[qsel ^ 1];
@@ -291,7 +318,7 @@ __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene S, Paths P, int qsel,
                 fill_hit(S, ro, rd, hv.x, prim, h, back);
                 const pt_material& M = S.materials[S.objects[h.owner].material];
                 bool need = CAT == CAT_DIELECTRIC || (S.emissive_solid && back);
-                classify_hit(S, h, back, need);
+                classify_hit<SBLOCK>(S, h, back, need, stk);
                 f3 wo = -rd;
                 // kernel_score_emitter_hit
                 if (h.region_to >= 0) {
@@ -361,9 +388,9 @@ __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene S, Paths P, int qsel,
             }
             P.rad[i] = f4(L, 0.0f);
         }
-        int slot = wave_append(&P.counts[qsel ^ 1], cont);
+        int slot = block_append<2>(ctr, cont ? 0 : -1, s_cnt, s_base);
         if (cont) qn[slot] = i;
-        int sslot = wave_append(&P.counts[C_SHADOW], shadow);
+        int sslot = block_append<2>(ctr, shadow ? 1 : -1, s_cnt, s_base);
         if (shadow) {
             P.s_path[sslot] = i;
             P.s_o[sslot] = f4(s_o, s_max);
@@ -631,10 +658,10 @@ pt_status pt_scene_upload(pt_ctx* ctx, const pt_scene_desc* sd) {
         bool solid = ob.kind == PT_OBJ_SPHERE || (ob.kind == PT_OBJ_MESH && ob.closed);
         if (solid && sd->materials[ob.material].emissive) D.emissive_solid = 1;
     }
-    // containment: per closed mesh binary BVH + TLAS over closed-mesh boxes
+    // containment: per closed mesh binary BVH (closest-triangle distance)
     std::vector<float> onodes;
     std::vector<int> obase((size_t)std::max(sd->n_objects, 1), -1), oorder((size_t)std::max(sd->n_tris, 1), 0);
-    std::vector<double> tboxes; std::vector<int> titems;
+    std::vector<int> titems;
     for (int o = 0; o < sd->n_objects; o++) {
         const pt_object& ob = sd->objects[o];
         if (ob.kind != PT_OBJ_MESH || !ob.closed || ob.tri_count == 0) continue;
@@ -651,20 +678,82 @@ pt_status pt_scene_upload(pt_ctx* ctx, const pt_scene_desc* sd) {
         obase[o] = (int)(onodes.size() / 8);
         onodes.insert(onodes.end(), bb.nodes.begin(), bb.nodes.end());
         for (int i = 0; i < ob.tri_count; i++) oorder[ob.first_tri + i] = ob.first_tri + (int)bb.order[i];
-        for (int a = 0; a < 3; a++) tboxes.push_back(ob.box_min[a]);
-        for (int a = 0; a < 3; a++) tboxes.push_back(ob.box_max[a]);
         titems.push_back(o);
     }
-    BinaryBVH tl;
-    build_binary_bvh(tboxes.data(), (int)titems.size(), 1, tl);
-    std::vector<int> tl_items(tl.order.size());
-    for (size_t k = 0; k < tl.order.size(); k++) tl_items[k] = titems[tl.order[k]];
     if ((st = ctx->upload((const float4*)onodes.data(), onodes.size() / 4, &D.obj_nodes)) != PT_OK) return st;
     if ((st = ctx->upload(obase.data(), obase.size(), &D.obj_node_base)) != PT_OK) return st;
     if ((st = ctx->upload(oorder.data(), oorder.size(), &D.obj_order)) != PT_OK) return st;
-    if ((st = ctx->upload((const float4*)tl.nodes.data(), tl.nodes.size() / 4, &D.tlas_nodes)) != PT_OK) return st;
-    if ((st = ctx->upload(tl_items.data(), tl_items.size(), &D.tlas_items)) != PT_OK) return st;
-    D.n_tlas_nodes = (int)(tl.nodes.size() / 8);
+    // per closed mesh wide BVH (first-hit-facing inside probe), concatenated
+    std::vector<uint32_t> own; std::vector<float> owt; std::vector<int> oroot((size_t)std::max(sd->n_objects, 1), -1);
+    uint32_t obj_depth = 0;
+    for (int o : titems) {
+        const pt_object& ob = sd->objects[o];
+        WideBVH w;
+        try { build_wide_bvh(sd->tri_vertices + 9 * (size_t)ob.first_tri, ob.tri_count, w); }
+        catch (const std::exception& e) { return fail(PT_E_INVALID, std::string("object BVH build: ") + e.what()); }
+        uint32_t noff = (uint32_t)(own.size() / 20), toff = (uint32_t)(owt.size() / 12);
+        for (uint32_t k = 0; k < w.n_nodes; k++) { w.nodes[20 * k + 4] += noff; w.nodes[20 * k + 5] += toff; }
+        for (size_t k = 0; k < w.tris.size() / 12; k++)
+            w.tris[12 * k + 3] = ptm_u2f((uint32_t)ob.first_tri + ptm_f2u(w.tris[12 * k + 3]));
+        own.insert(own.end(), w.nodes.begin(), w.nodes.end());
+        owt.insert(owt.end(), w.tris.begin(), w.tris.end());
+        oroot[o] = (int)noff;
+        obj_depth = std::max(obj_depth, w.max_depth);
+    }
+    if (obj_depth + 1 > (uint32_t)ctx->stack) {
+        if (obj_depth + 1 <= 32) ctx->stack = 32;
+        else return fail(PT_E_UNSUPPORTED, "object BVH deeper than 31 levels");
+    }
+    if ((st = ctx->upload((const uint4*)own.data(), own.size() / 4, &D.ow_nodes)) != PT_OK) return st;
+    if ((st = ctx->upload((const float4*)owt.data(), owt.size() / 4, &D.ow_tris)) != PT_OK) return st;
+    if ((st = ctx->upload(oroot.data(), oroot.size(), &D.ow_root)) != PT_OK) return st;
+    // uniform grid over the closed-mesh boxes (containment candidates)
+    D.has_grid = titems.empty() ? 0 : 1;
+    if (D.has_grid) {
+        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        std::vector<float> ext;
+        for (int o : titems) {
+            const pt_object& ob = sd->objects[o];
+            float e = 0.0f;
+            for (int a = 0; a < 3; a++) {
+                lo[a] = std::min(lo[a], ob.box_min[a]); hi[a] = std::max(hi[a], ob.box_max[a]);
+                e = std::max(e, ob.box_max[a] - ob.box_min[a]);
+            }
+            ext.push_back(e);
+        }
+        std::nth_element(ext.begin(), ext.begin() + ext.size() / 2, ext.end());
+        float cell = std::max(ext[ext.size() / 2], 1e-6f);
+        for (int a = 0; a < 3; a++) {
+            float e = hi[a] - lo[a];
+            int n = (int)std::ceil(e / cell);
+            n = std::min(std::max(n, 1), 128);
+            D.grid_lo[a] = lo[a];
+            D.grid_n[a] = n;
+            D.grid_inv[a] = e > 0.0f ? (float)n / e : 0.0f;
+        }
+        auto cell_of = [&](int a, float x) {
+            int c = (int)std::floor((x - D.grid_lo[a]) * D.grid_inv[a]);
+            return std::min(std::max(c, 0), D.grid_n[a] - 1);
+        };
+        const size_t ncell = (size_t)D.grid_n[0] * D.grid_n[1] * D.grid_n[2];
+        std::vector<int2> cells(ncell, make_int2(0, 0));
+        std::vector<std::vector<int>> lists(ncell);
+        for (int o : titems) {
+            const pt_object& ob = sd->objects[o];
+            int c0[3], c1[3];
+            for (int a = 0; a < 3; a++) { c0[a] = cell_of(a, ob.box_min[a]); c1[a] = cell_of(a, ob.box_max[a]); }
+            for (int z = c0[2]; z <= c1[2]; z++)
+                for (int y = c0[1]; y <= c1[1]; y++)
+                    for (int x = c0[0]; x <= c1[0]; x++) lists[((size_t)z * D.grid_n[1] + y) * D.grid_n[0] + x].push_back(o);
+        }
+        std::vector<int> items;
+        for (size_t c = 0; c < ncell; c++) {
+            cells[c] = make_int2((int)items.size(), (int)lists[c].size());
+            items.insert(items.end(), lists[c].begin(), lists[c].end());
+        }
+        if ((st = ctx->upload(cells.data(), cells.size(), &D.grid_cells)) != PT_OK) return st;
+        if ((st = ctx->upload(items.data(), items.size(), &D.grid_items)) != PT_OK) return st;
+    }
     // environment
     D.env_type = sd->env.type;
     if (sd->env.type == PT_ENV_SKY) {
@@ -745,18 +834,18 @@ static pt_status ensure_paths(pt_renderer* r, long long n) {
     return PT_OK;
 }
 
-template <int EST>
+template <int EST, int STACK>
 static void launch_shade(pt_renderer* r, int cur, int b) {
     pt_ctx* ctx = r->ctx;
     const pt_program& pg = r->prog;
     hipStream_t st = ctx->stream;
     const int g = ctx->shade_blocks;
-    TIMED(ctx, "shade_miss", 0, hipLaunchKernelGGL((k_shade<EST, CAT_MISS>), dim3(g), dim3(SBLOCK), 0, st, ctx->dev, r->paths, cur, b, pg.max_bounces, pg.rr_start, pg.rr_max_survival));
-    TIMED(ctx, "shade_lambert", 0, hipLaunchKernelGGL((k_shade<EST, CAT_LAMBERT>), dim3(g), dim3(SBLOCK), 0, st, ctx->dev, r->paths, cur, b, pg.max_bounces, pg.rr_start, pg.rr_max_survival));
+    TIMED(ctx, "shade_miss", 0, hipLaunchKernelGGL((k_shade<EST, CAT_MISS, STACK>), dim3(g), dim3(SBLOCK), 0, st, ctx->dev, r->paths, cur, b, pg.max_bounces, pg.rr_start, pg.rr_max_survival));
+    TIMED(ctx, "shade_lambert", 0, hipLaunchKernelGGL((k_shade<EST, CAT_LAMBERT, STACK>), dim3(g), dim3(SBLOCK), 0, st, ctx->dev, r->paths, cur, b, pg.max_bounces, pg.rr_start, pg.rr_max_survival));
     if (ctx->has_ggx)
-        TIMED(ctx, "shade_ggx", 0, hipLaunchKernelGGL((k_shade<EST, CAT_GGX>), dim3(g), dim3(SBLOCK), 0, st, ctx->dev, r->paths, cur, b, pg.max_bounces, pg.rr_start, pg.rr_max_survival));
+        TIMED(ctx, "shade_ggx", 0, hipLaunchKernelGGL((k_shade<EST, CAT_GGX, STACK>), dim3(g), dim3(SBLOCK), 0, st, ctx->dev, r->paths, cur, b, pg.max_bounces, pg.rr_start, pg.rr_max_survival));
     if (ctx->dev.has_transmission)
-        TIMED(ctx, "shade_dielectric", 0, hipLaunchKernelGGL((k_shade<EST, CAT_DIELECTRIC>), dim3(g), dim3(SBLOCK), 0, st, ctx->dev, r->paths, cur, b, pg.max_bounces, pg.rr_start, pg.rr_max_survival));
+        TIMED(ctx, "shade_dielectric", 0, hipLaunchKernelGGL((k_shade<EST, CAT_DIELECTRIC, STACK>), dim3(g), dim3(SBLOCK), 0, st, ctx->dev, r->paths, cur, b, pg.max_bounces, pg.rr_start, pg.rr_max_survival));
 }
 
 template <int STACK>
@@ -777,10 +866,11 @@ static void launch_wave(pt_renderer* r, const Camera& cam, int frames, uint32_t 
             TIMED(ctx, "extend_count", 0, hipLaunchKernelGGL((k_extend<STACK, true>), dim3(tgrid), dim3(TBLOCK), 0, st, S, P, cur));
         else
             TIMED(ctx, "extend", 0, hipLaunchKernelGGL((k_extend<STACK, false>), dim3(tgrid), dim3(TBLOCK), 0, st, S, P, cur));
+        TIMED(ctx, "bin", 0, hipLaunchKernelGGL(k_bin, dim3(ctx->shade_blocks), dim3(SBLOCK), 0, st, S, P, cur));
         switch (pg.estimator) {
-            case PT_EST_PT: launch_shade<PT_EST_PT>(r, cur, b); break;
-            case PT_EST_NEE: launch_shade<PT_EST_NEE>(r, cur, b); break;
-            default: launch_shade<PT_EST_MIS>(r, cur, b);
+            case PT_EST_PT: launch_shade<PT_EST_PT, STACK>(r, cur, b); break;
+            case PT_EST_NEE: launch_shade<PT_EST_NEE, STACK>(r, cur, b); break;
+            default: launch_shade<PT_EST_MIS, STACK>(r, cur, b);
         }
         TIMED(ctx, "shadow", 0, hipLaunchKernelGGL((k_shadow<STACK>), dim3(tgrid), dim3(TBLOCK), 0, st, S, P));
         cur ^= 1;

@@ -29,44 +29,23 @@ PT_DEV Frame ambient_frame(f3 n) {
 }
 
 // ---------------------------------------------------------------- containment
-// mesh_inside_bvh / mesh_closest_bvh (intersection/mesh/mesh.glsl:202-296) over the object's
-// binary BVH; scene_region_at (features/intersection.ts:1121-1265), innermost wins.
-PT_DEV bool mesh_inside(const DevScene& S, int obj, f3 p) {
-    const int base = S.obj_node_base[obj];
-    if (base < 0) return false;
+// mesh_inside_bvh / mesh_closest_bvh (intersection/mesh/mesh.glsl:202-296) and
+// scene_region_at (features/intersection.ts:1121-1265), innermost wins. The first-hit-facing
+// probe walks the object's own wide BVH with the LDS-stack traversal; like every walk here it
+// equals the brute-force nearest triangle under the (t, id) rule.
+template <int BLOCK>
+PT_DEV bool mesh_inside(const DevScene& S, int obj, f3 p, uint2* stk) {
+    const int root = S.ow_root[obj];
+    if (root < 0) return false;
     const f3 dir = mk3(0.5320544f, 0.7396997f, 0.4114743f);
-    const f3 inv = mk3(1.0f / dir.x, 1.0f / dir.y, 1.0f / dir.z);
-    float tmax = 1.0e20f, side = 0.0f; int best = -1;
-    int stack[64]; int sp = 0; stack[0] = 0;
-    while (sp >= 0) {
-        int ni = stack[sp--];
-        float4 n0 = S.obj_nodes[2 * (base + ni)], n1 = S.obj_nodes[2 * (base + ni) + 1];
-        f3 t0 = (xyz(n0) - p) * inv, t1 = (xyz(n1) - p) * inv;
-        float tn = gmax(gmax(gmin(t0.x, t1.x), gmin(t0.y, t1.y)), gmin(t0.z, t1.z));
-        float tf = gmin(gmin(gmax(t0.x, t1.x), gmax(t0.y, t1.y)), gmax(t0.z, t1.z)) * BVH_TFAR_PAD;
-        float te = gmax(tn, 0.0f);
-        if (!(tf >= te && te <= tmax)) continue;
-        if (n0.w >= 0.0f) {
-            int off = (int)n1.w, cnt = (int)n0.w;
-            int first = S.objects[obj].first_tri;
-            for (int i = off; i < off + cnt; i++) {
-                int tri = S.obj_order[first + i];
-                const float4* tp = S.tri_shade + 3 * (size_t)tri;
-                f3 a = xyz(tp[0]), e1 = xyz(tp[1]) - a, e2 = xyz(tp[2]) - a;
-                float t;
-                if (tri_test(p, dir, a, e1, e2, t) && t > 0.0f && (t < tmax || (t == tmax && tri < best))) {
-                    tmax = t; best = tri; side = dot(dir, cross(e1, e2));
-                }
-            }
-        } else {
-            int axis = (int)(-n0.w - 1.0f);
-            int L = ni + 1, R = (int)n1.w;
-            float da = axis == 0 ? dir.x : (axis == 1 ? dir.y : dir.z);
-            bool near_first = da >= 0.0f;
-            if (sp + 2 < 64) { stack[++sp] = near_first ? R : L; stack[++sp] = near_first ? L : R; }
-        }
-    }
-    return side > 0.0f;
+    TravState st;
+    trace_init(st, p, dir, 1.0e20f, (uint32_t)root);
+    TraceCounters cnt = {0, 0};
+    while (!trace_step<false, BLOCK, false>(S.ow_nodes, S.ow_tris, st, stk, cnt)) {}
+    if (st.prim < 0) return false;
+    const float4* tp = S.tri_shade + 3 * (size_t)st.prim;
+    f3 a = xyz(tp[0]);
+    return dot(dir, cross(xyz(tp[1]) - a, xyz(tp[2]) - a)) > 0.0f;
 }
 PT_DEV float point_tri_dist2(f3 p, f3 a, f3 b, f3 c) {
     f3 ab = b - a, ac = c - a, ap = p - a;
@@ -123,7 +102,8 @@ PT_DEV float mesh_closest(const DevScene& S, int obj, f3 p) {
     }
     return sqrtf(best2);
 }
-PT_DEV int region_at(const DevScene& S, f3 p) {
+template <int BLOCK>
+PT_DEV int region_at(const DevScene& S, f3 p, uint2* stk) {
     int region = -1; float best = -1.0e20f;
     for (int k = 0; k < S.n_analytic; k++) {
         int o = S.analytic[k];
@@ -132,26 +112,23 @@ PT_DEV int region_at(const DevScene& S, f3 p) {
         float d = length(p - ld3(ob.p)) - ob.p[3];
         if (d < 0.0f && d > best) { best = d; region = o; }
     }
-    if (S.n_tlas_nodes == 0) return region;
-    int stack[64]; int sp = 0; stack[0] = 0;
-    while (sp >= 0) {
-        int ni = stack[sp--];
-        float4 n0 = S.tlas_nodes[2 * ni], n1 = S.tlas_nodes[2 * ni + 1];
-        if (!(p.x >= n0.x && p.y >= n0.y && p.z >= n0.z && p.x <= n1.x && p.y <= n1.y && p.z <= n1.z)) continue;
-        if (n0.w >= 0.0f) {
-            int off = (int)n1.w, cnt = (int)n0.w;
-            for (int j = 0; j < cnt; j++) {
-                int o = S.tlas_items[off + j];
-                const pt_object& ob = S.objects[o];
-                float d = 1.0e20f;
-                if (p.x >= ob.box_min[0] && p.y >= ob.box_min[1] && p.z >= ob.box_min[2] &&
-                    p.x <= ob.box_max[0] && p.y <= ob.box_max[1] && p.z <= ob.box_max[2] && mesh_inside(S, o, p))
-                    d = -1.0f * mesh_closest(S, o, p);
-                if (d < 0.0f && d > best) { best = d; region = o; }
-            }
-        } else if (sp + 2 < 64) {
-            stack[++sp] = ni + 1; stack[++sp] = (int)n1.w;
-        }
+    if (!S.has_grid) return region;
+    int c[3];
+    const float pc[3] = {p.x, p.y, p.z};
+    for (int a = 0; a < 3; a++) {
+        float f = floorf((pc[a] - S.grid_lo[a]) * S.grid_inv[a]);
+        if (!(f >= 0.0f && f < (float)S.grid_n[a])) return region;   // outside every closed-mesh box
+        c[a] = (int)f;
+    }
+    int2 cell = S.grid_cells[(c[2] * S.grid_n[1] + c[1]) * S.grid_n[0] + c[0]];
+    for (int j = 0; j < cell.y; j++) {
+        int o = S.grid_items[cell.x + j];
+        const pt_object& ob = S.objects[o];
+        float d = 1.0e20f;
+        if (p.x >= ob.box_min[0] && p.y >= ob.box_min[1] && p.z >= ob.box_min[2] &&
+            p.x <= ob.box_max[0] && p.y <= ob.box_max[1] && p.z <= ob.box_max[2] && mesh_inside<BLOCK>(S, o, p, stk))
+            d = -1.0f * mesh_closest(S, o, p);
+        if (d < 0.0f && d > best) { best = d; region = o; }
     }
     return region;
 }
@@ -192,13 +169,14 @@ PT_DEV void fill_hit(const DevScene& S, f3 o, f3 d, float t, int prim, Hit& h, b
     }
     back = !(dot(d, h.ng) < 0.0f);
 }
-PT_DEV void classify_hit(const DevScene& S, Hit& h, bool back, bool need) {
+template <int BLOCK>
+PT_DEV void classify_hit(const DevScene& S, Hit& h, bool back, bool need, uint2* stk) {
     // h.ng is still the unflipped geometric normal here
     if (need) {
-        int outside = region_at(S, h.p + h.ng * EPS_INTERFACE);
+        int outside = region_at<BLOCK>(S, h.p + h.ng * EPS_INTERFACE, stk);
         if (!back) { h.region_from = outside; h.region_to = h.owner; }
         else {
-            h.region_from = object_thin(S, h.owner) ? region_at(S, h.p + h.ng * (-EPS_INTERFACE)) : h.owner;
+            h.region_from = object_thin(S, h.owner) ? region_at<BLOCK>(S, h.p + h.ng * (-EPS_INTERFACE), stk) : h.owner;
             h.region_to = outside;
         }
     } else {

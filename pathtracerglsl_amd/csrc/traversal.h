@@ -78,7 +78,7 @@ struct TravState {
     int prim, sp;
 };
 
-PT_DEV void trace_init(TravState& st, f3 o, f3 d, float tmax) {
+PT_DEV void trace_init(TravState& st, f3 o, f3 d, float tmax, uint32_t root = 0) {
     st.o = o; st.d = d; st.tmax = tmax; st.prim = -1; st.sp = 0;
     // inverse direction with the reference CWBVH's zero guard (|d| < 1e-20 -> +-1e-20)
     float dx = fabsf(d.x) < 1e-20f ? copysignf(1e-20f, d.x) : d.x;
@@ -87,7 +87,7 @@ PT_DEV void trace_init(TravState& st, f3 o, f3 d, float tmax) {
     st.ix = 1.0f / dx; st.iy = 1.0f / dy; st.iz = 1.0f / dz;
     uint32_t oct = (d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 4u : 0u);
     st.oct_inv = oct ^ 7u;
-    st.g_base = 0;
+    st.g_base = root;
     st.g_bits = (1u << 8) | (1u << (0u ^ st.oct_inv));   // root = slot 0 of a virtual parent
 }
 
@@ -95,7 +95,8 @@ PT_DEV void trace_init(TravState& st, f3 o, f3 d, float tmax) {
 // Returns true when the ray is finished: the stack is empty, or (ANY) a hit with t < tmax
 // was found, in which case st.prim is set to its id.
 template <bool ANY, int BLOCK, bool COUNT>
-PT_DEV bool trace_step(const DevScene& S, TravState& st, uint2* stk, TraceCounters& cnt) {
+PT_DEV bool trace_step(const uint4* __restrict__ nodes, const float4* __restrict__ tris, TravState& st, uint2* stk,
+                       TraceCounters& cnt) {
     const int lane_slot = threadIdx.x;
     uint32_t tri_base = 0, tri_hits = 0;
     if ((st.g_bits & 0xFFu) != 0u) {
@@ -108,7 +109,7 @@ PT_DEV bool trace_step(const DevScene& S, TravState& st, uint2* stk, TraceCounte
             stk[st.sp * BLOCK + lane_slot] = make_uint2(st.g_base, st.g_bits);
             st.sp++;
         }
-        const uint4* np = S.cw_nodes + 5 * (size_t)node;
+        const uint4* np = nodes + 5 * (size_t)node;
         uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3], w4 = np[4];
         if (COUNT) cnt.nodes++;
         const float px = __uint_as_float(w0.x), py = __uint_as_float(w0.y), pz = __uint_as_float(w0.z);
@@ -149,7 +150,7 @@ PT_DEV bool trace_step(const DevScene& S, TravState& st, uint2* stk, TraceCounte
     while (tri_hits != 0u) {
         uint32_t k = __builtin_ctz(tri_hits);
         tri_hits &= tri_hits - 1u;
-        const float4* tp = S.cw_tris + 3 * (size_t)(tri_base + k);
+        const float4* tp = tris + 3 * (size_t)(tri_base + k);
         float4 ta = tp[0], te1 = tp[1], te2 = tp[2];
         if (COUNT) cnt.tris++;
         float t;
@@ -178,7 +179,7 @@ PT_DEV bool trace_wide(const DevScene& S, f3 o, f3 d, float& tmax, int& prim, ui
     TravState st;
     trace_init(st, o, d, tmax);
     st.prim = prim;
-    while (!trace_step<ANY, BLOCK, COUNT>(S, st, stk, cnt)) {}
+    while (!trace_step<ANY, BLOCK, COUNT>(S.cw_nodes, S.cw_tris, st, stk, cnt)) {}
     (void)STACK;
     if (ANY) return st.prim >= 0 && st.prim != prim;
     bool found = st.prim != prim;

@@ -1,7 +1,7 @@
 """Session-level cache of compiled workloads shared by the GPU tests."""
 import functools
 
-from pathtracerglsl_amd import compiler, scenes
+from pathtracerglsl_amd import compiler, scenes  # noqa: F401
 
 
 @functools.lru_cache(maxsize=None)

@@ -119,3 +119,20 @@ def test_render_cloud_matches_oracle(engine, oracle, cloud_pair):
     exact, bad = _compare_film("cloud-mis", g, o)
     assert bad == 0.0
     assert engine.ray_counts() == (int(counts[0]), int(counts[1]))
+
+
+def test_render_overlapping_glass_matches_oracle(engine, oracle):
+    """Dense cloud: spheres interpenetrate, so region classification (scene_region_at) sees
+    points inside other closed meshes and dielectric interfaces between media."""
+    sc = scenes.icosphere_cloud(n_spheres=300, seed=3, box=12.0)
+    comp = scene_cache.compiler.compile_scene(sc, scene_cache.compiler.RenderStrategy("mis", 12, 3), 48, 32)
+    engine.load_renderer(comp)
+    engine.clear_accumulation()
+    _say("[gpu] render overlapping icosphere cloud 48x32x4")
+    engine.render_frames(4)
+    g = engine.read_export()
+    ors = oracle_lib.OracleScene(oracle, comp.scene)
+    o, _, counts = ors.render(comp.program, comp.params(), 4)
+    exact, bad = _compare_film("overlap-mis", g, o)
+    assert bad == 0.0
+    assert engine.ray_counts() == (int(counts[0]), int(counts[1]))
