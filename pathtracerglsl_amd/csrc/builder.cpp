@@ -8,6 +8,10 @@
 
 #include "../../include/ptmi_math.h"
 
+#ifndef PTMI_COLLAPSE_RULE
+#define PTMI_COLLAPSE_RULE 1
+#endif
+
 namespace ptmi {
 
 namespace {
@@ -163,13 +167,24 @@ void build_wide_bvh(const float* tv, int n_tris, WideBVH& out) {
         std::vector<int> ch;
         if (bn[job.bnode].left < 0) ch.push_back(job.bnode);
         else { ch.push_back(bn[job.bnode].left); ch.push_back(bn[job.bnode].right); }
+        // Fill the 8 slots: open the largest subtree that cannot be a leaf (> 3 triangles)
+        // first, so big subtrees do not cost a node visit; only then spend spare slots on
+        // splitting small leaves (tighter boxes, fewer triangle tests; slot tests are paid
+        // per node visit anyway).
         while (ch.size() < 8) {
             int best = -1; double ba = -1.0;
-            for (size_t i = 0; i < ch.size(); i++) {
-                const BNode& c = bn[ch[i]];
-                if (c.left < 0) continue;
-                double a = c.box.area();
-                if (a > ba) { ba = a; best = (int)i; }
+            for (int pass = 0; pass < 2 && best < 0; pass++) {
+                for (size_t i = 0; i < ch.size(); i++) {
+                    const BNode& c = bn[ch[i]];
+                    if (c.left < 0) continue;
+#if PTMI_COLLAPSE_RULE == 0
+                    if (pass == 1) continue;
+#else
+                    if ((pass == 0) != (c.count > 3)) continue;
+#endif
+                    double a = c.box.area();
+                    if (a > ba) { ba = a; best = (int)i; }
+                }
             }
             if (best < 0) break;
             int b = ch[best];

@@ -132,8 +132,17 @@ struct LaneFeeder {
     }
 };
 
-constexpr int FEED_CHUNK = 256;
-constexpr int FEED_MIN_IDLE = 16;
+#ifndef PTMI_FEED_CHUNK
+#define PTMI_FEED_CHUNK 64
+#endif
+#ifndef PTMI_FEED_MIN_IDLE
+#define PTMI_FEED_MIN_IDLE 8
+#endif
+#ifndef PTMI_SHADOW_PERSISTENT
+#define PTMI_SHADOW_PERSISTENT 0
+#endif
+constexpr int FEED_CHUNK = PTMI_FEED_CHUNK;        // queue entries a wave claims per atomic
+constexpr int FEED_MIN_IDLE = PTMI_FEED_MIN_IDLE;  // idle lanes before a wave refills
 
 // extend: closest hit for every ray of the current queue, then bin the path by shading category.
 template <int STACK, bool COUNT>
@@ -176,6 +185,27 @@ __global__ void __launch_bounds__(TBLOCK) k_extend(DevScene S, Paths P, int qsel
 template <int STACK>
 __global__ void __launch_bounds__(TBLOCK) k_shadow(DevScene S, Paths P) {
     __shared__ uint2 stk[STACK * TBLOCK];
+#if !PTMI_SHADOW_PERSISTENT
+    {
+        const int n = P.counts[C_SHADOW];
+        TraceCounters cnt = {0, 0};
+        for (int base = blockIdx.x * TBLOCK; base < n; base += gridDim.x * TBLOCK) {
+            int k = base + threadIdx.x;
+            if (k >= n) break;
+            float4 o4 = P.s_o[k], d4 = P.s_d[k];
+            f3 o = xyz(o4), d = xyz(d4);
+            float maxd = o4.w; int prim = -1;
+            bool blocked = trace_analytic<true>(S, o, d, maxd, prim);
+            if (!blocked) blocked = trace_wide<true, STACK, TBLOCK, false>(S, o, d, maxd, prim, stk, cnt);
+            if (!blocked) {
+                int i = P.s_path[k];
+                float4 c = P.s_c[k], L = P.rad[i];
+                P.rad[i] = make_float4(L.x + c.x, L.y + c.y, L.z + c.z, 0.0f);
+            }
+        }
+        return;
+    }
+#endif
     const uint32_t n = (uint32_t)P.counts[C_SHADOW];
     TraceCounters cnt = {0, 0};
     LaneFeeder feed; feed.init();

@@ -67,6 +67,17 @@ PT_DEV bool quad_intersect(f3 ro, f3 rd, const float* p, float& t) {
 
 struct TraceCounters { uint32_t nodes, tris; };
 
+#ifndef PTMI_SLAB_IEEE_MINMAX
+#define PTMI_SLAB_IEEE_MINMAX 1
+#endif
+#if PTMI_SLAB_IEEE_MINMAX
+#define SMIN(a, b) __builtin_fminf(a, b)
+#define SMAX(a, b) __builtin_fmaxf(a, b)
+#else
+#define SMIN(a, b) gmin(a, b)
+#define SMAX(a, b) gmax(a, b)
+#endif
+
 PT_DEV float byte_of(uint32_t w, int k) { return (float)((w >> (8 * k)) & 0xFFu); }
 
 // Per-lane traversal state of the wide-BVH walk, so a persistent kernel can interleave
@@ -135,9 +146,11 @@ PT_DEV bool trace_step(const uint4* __restrict__ nodes, const float4* __restrict
             const float t0x = (lox - o.x) * st.ix, t1x = (hix - o.x) * st.ix;
             const float t0y = (loy - o.y) * st.iy, t1y = (hiy - o.y) * st.iy;
             const float t0z = (loz - o.z) * st.iz, t1z = (hiz - o.z) * st.iz;
-            const float tn = gmax(gmax(gmin(t0x, t1x), gmin(t0y, t1y)), gmin(t0z, t1z));
-            const float tf = gmin(gmin(gmax(t0x, t1x), gmax(t0y, t1y)), gmax(t0z, t1z)) * BVH_TFAR_PAD;
-            const float te = gmax(tn, 0.0f);
+            // slab test: the t values are finite (guarded inverse), so IEEE min/max equal GLSL's
+            // and every comparison below sees the same values
+            const float tn = SMAX(SMAX(SMIN(t0x, t1x), SMIN(t0y, t1y)), SMIN(t0z, t1z));
+            const float tf = SMIN(SMIN(SMAX(t0x, t1x), SMAX(t0y, t1y)), SMAX(t0z, t1z)) * BVH_TFAR_PAD;
+            const float te = SMAX(tn, 0.0f);
             if (meta != 0u && tf >= te && te <= st.tmax) {
                 if (nimask & (1u << s)) nhits |= 1u << (s ^ st.oct_inv);
                 else tri_hits |= ((meta >> 5) & 7u) << (meta & 31u);
