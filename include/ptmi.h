@@ -54,11 +54,13 @@ typedef struct pt_object {
     float   box_max[4];
 } pt_object;
 
-enum { PT_MAT_LAMBERT = 0, PT_MAT_DIELECTRIC = 1, PT_MAT_GGX = 2 };
+enum { PT_MAT_LAMBERT = 0, PT_MAT_DIELECTRIC = 1, PT_MAT_GGX = 2, PT_MAT_NULL = 3 };
 
 /* Material record: the MaterialProperties the generator bakes per material id
  * (src/compiler/generate/features/materials.ts, components/materials/{lambert,dielectric,ggx}).
- * Interior medium (homogeneous) when has_medium != 0. */
+ * PT_MAT_NULL is the reference's model 'none': a null interface that only bounds a medium
+ * (components/transport/integrators/pt/pt.ts:378-388). Interior medium (homogeneous, HG
+ * phase) when has_medium != 0 (features/materials.ts scene_medium_properties). */
 typedef struct pt_material {
     int32_t model;
     int32_t emissive;        /* material_is_emissive */
@@ -119,6 +121,8 @@ typedef struct pt_scene_desc {
     int32_t  n_lights;
     const pt_light* lights;
     pt_env   env;
+    int32_t  ambient_material;     /* material filling region -1 (ambientMedium), -1 = none */
+    int32_t  pad0;
 } pt_scene_desc;
 
 /* ---- program (the Plan stage's decisions) ---- */
@@ -135,7 +139,8 @@ typedef struct pt_program {
     int32_t  color;
     int32_t  width, height;   /* film size */
     int32_t  frames_in_flight;/* samples per pixel traced per wave of paths (>= 1) */
-    int32_t  pad[3];
+    int32_t  max_null_crossings; /* measurement.maxNullCrossings (32 default) */
+    int32_t  pad[2];
 } pt_program;
 
 /* Live per-frame values (the engine builtins + camera uniforms,

@@ -271,6 +271,8 @@ typedef struct {
     float env_rot_cs[2];
     int has_transmission;
     int emissive_solid;       /* some closed/solid region is emissive */
+    int ambient_material;     /* material of region -1 (ambientMedium), -1 = none */
+    int has_media, has_null;
 } or_scene;
 
 /* Möller–Trumbore (intersection/mesh/mesh.glsl:41-59). */
@@ -619,7 +621,10 @@ static int scene_intersect(const or_scene* S, v3 ro, v3 rd, hit_t* hit) {
     return 1;
 }
 
-static inline int material_of(const or_scene* S, int region) { return region < 0 ? -1 : S->objects[region].material; }
+/* material_of (generated): the ambient region -1 maps to the ambientMedium material. */
+static inline int material_of(const or_scene* S, int region) { return region < 0 ? S->ambient_material : S->objects[region].material; }
+static inline int is_null_interface(const or_scene* S, int mat) { return mat >= 0 && S->materials[mat].model == PT_MAT_NULL; }
+static inline int material_has_medium(const or_scene* S, int mat) { return mat >= 0 && S->materials[mat].has_medium; }
 static inline float ior_of(const or_scene* S, int region) {
     if (region < 0) return 1.0f;
     const pt_object* ob = &S->objects[region];
@@ -989,10 +994,89 @@ static float lighting_pdf(const or_scene* S, v3 qp, v3 wi, int lid, v3 light_p) 
     return sel * light_pdf_one(l, qp, light_p, wi);
 }
 
+/* ---------------- homogeneous media (transport/volume/analytic/analytic.glsl:9-51) ---- */
+static inline float spectrum_average(v3 s) { float sum = s.x; sum += s.y; sum += s.z; return sum * (1.0f / 3.0f); }
+static inline v3 spectrum_exp(v3 s) { return V(EXP(s.x), EXP(s.y), EXP(s.z)); }
+static inline float comp(v3 s, int c) { return c == 0 ? s.x : (c == 1 ? s.y : s.z); }
+typedef struct { int scattered; float t; v3 weight; } msample_t;
+static msample_t medium_sample_analytic(const pt_material* m, float t_max, float x0, float x1) {
+    msample_t ms;
+    v3 sa = lv(m->sigma_a), ss = lv(m->sigma_s);
+    v3 st = add(sa, ss);
+    int c = (int)(x0 * 3.0f); if (c > 2) c = 2;                  /* spectrum_sample_channel */
+    float sc = comp(st, c);
+    float t = sc == 0.0f ? t_max : -LOG(1.0f - x1) / sc;
+    if (t < t_max) {
+        v3 tr = spectrum_exp(muls(neg(st), t));
+        float pdf = spectrum_average(mul(st, tr));
+        ms.scattered = 1; ms.t = t;
+        ms.weight = divs(mul(ss, tr), fmaxf_(pdf, 1e-20f));
+    } else {
+        v3 tr = spectrum_exp(muls(neg(st), t_max));
+        float pdf = spectrum_average(tr);
+        ms.scattered = 0; ms.t = t_max;
+        ms.weight = divs(tr, fmaxf_(pdf, 1e-20f));
+    }
+    return ms;
+}
+static inline v3 medium_transmittance(const pt_material* m, float len) {
+    return spectrum_exp(muls(neg(add(lv(m->sigma_a), lv(m->sigma_s))), len));
+}
+/* HG phase (volume_scattering/hg/hg.glsl:11-42), forward convention. */
+static inline float hg_value(v3 wi, v3 wo, float g) {
+    float c = dot(wi, neg(wo));
+    float d = 1.0f + g * g - 2.0f * g * c;
+    return (1.0f - g * g) / (4.0f * PTM_PI * d * sqrtf(fmaxf_(d, 1e-8f)));
+}
+static inline float hg_pdf(v3 wi, v3 wo, float g) { float h = hg_value(wi, wo, g); return spectrum_average(V(h, h, h)); }
+static inline void build_basis(v3 n, v3* t, v3* b) {   /* glsl/core/math.glsl:63-67 */
+    v3 up = fabsf(n.y) < 0.999f ? V(0.0f, 1.0f, 0.0f) : V(1.0f, 0.0f, 0.0f);
+    *t = normalize(cross(up, n));
+    *b = cross(n, *t);
+}
+static v3 hg_sample(v3 wo, float g, float x0, float x1, float* pdf) {
+    float cos_theta;
+    if (fabsf(g) < 1e-3f) cos_theta = 1.0f - 2.0f * x0;
+    else {
+        float sq = (1.0f - g * g) / (1.0f - g + 2.0f * g * x0);
+        cos_theta = (1.0f + g * g - sq * sq) / (2.0f * g);
+    }
+    float sin_theta = sqrtf(fmaxf_(0.0f, 1.0f - cos_theta * cos_theta));
+    float phi = PTM_TWO_PI * x1;
+    v3 t, b;
+    v3 nw = neg(wo);
+    build_basis(nw, &t, &b);
+    v3 wi = normalize(add(add(muls(t, sin_theta * COS(phi)), muls(b, sin_theta * SIN(phi))), muls(nw, cos_theta)));
+    *pdf = hg_pdf(wi, wo, g);
+    return wi;
+}
+/* shadow_transmittance, media walker (transport/shadow/media/media.glsl:29-62). */
+static v3 shadow_transmittance_media(const or_scene* S, v3 o, v3 d, v3 light_p, int crossings_left, int* n_any) {
+    v3 T = V(1.0f, 1.0f, 1.0f);
+    int medium = region_at(S, o);
+    for (int crossed = 0; crossed <= crossings_left; crossed++) {
+        float remaining = length3(sub(light_p, o)) - SHADOW_BACKOFF;
+        if (remaining <= 0.0f) return T;
+        hit_t h;
+        (*n_any)++;
+        int hit_boundary = scene_intersect(S, o, d, &h) && h.t < remaining;
+        float seg_len = hit_boundary ? h.t : remaining;
+        int med_mat = material_of(S, medium);
+        if (material_has_medium(S, med_mat)) T = mul(T, medium_transmittance(&S->materials[med_mat], seg_len));
+        if (!hit_boundary) return T;
+        if (!is_null_interface(S, material_of(S, h.region_owner))) return V(0.0f, 0.0f, 0.0f);
+        medium = h.region_to;
+        float side = dot(d, h.ng) >= 0.0f ? 1.0f : -1.0f;
+        o = add(h.p, muls(muls(h.ng, side), h.eps));
+        d = normalize(sub(light_p, o));
+    }
+    return V(0.0f, 0.0f, 0.0f);
+}
+
 /* ---------------- the walk (integrators/pt/pt.ts:299-464; combiner.ts; kernel.glsl) ---- */
 typedef struct {
     int estimator, max_bounces, rr_start; float rr_max;
-    int width, height;
+    int width, height, max_null;
 } or_program;
 
 static inline float power_heuristic(float pf, float pg) {
@@ -1007,17 +1091,71 @@ static inline v3 ray_spawn_origin(const hit_t* hit, v3 wi) {
     return add(hit->p, muls(muls(hit->ng, side), hit->eps));
 }
 
-/* One camera path; returns radiance. */
+/* One camera path; returns radiance. The media branch, null interfaces and the media
+ * shadow walker are the program's (f.media, f.nulls) parts of pt.ts:299-464. */
 static v3 transport_trace(const or_scene* S, const or_program* P, v3 ro, v3 rd, rng_t* rng, int* n_closest, int* n_any) {
     v3 T = V(1.0f, 1.0f, 1.0f), L = V(0.0f, 0.0f, 0.0f);
     int prev_was_delta = 1; float prev_bsdf_pdf = 0.0f; v3 prev_qp = ro;
     float eta_scale = 1.0f;
     int nee = P->estimator != PT_EST_PT, mis = P->estimator == PT_EST_MIS;
     int emitters = S->n_lights > 0;
+    const int media = S->has_media;
+    int current_medium = media ? region_at(S, ro) : -1;
+    int null_crossings = 0;
     for (int bounce = 0; bounce <= P->max_bounces; bounce++) {
         hit_t hit;
         (*n_closest)++;
-        if (!scene_intersect(S, ro, rd, &hit)) {
+        int boundary = scene_intersect(S, ro, rd, &hit);
+        if (media) {
+            int med_mat = material_of(S, current_medium);
+            if (material_has_medium(S, med_mat)) {
+                const pt_material* Mm = &S->materials[med_mat];
+                float x0 = rnd(rng), x1 = rnd(rng);
+                msample_t ms = medium_sample_analytic(Mm, boundary ? hit.t : MAX_DIST, x0, x1);
+                T = mul(T, ms.weight);
+                if (ms.scattered) {
+                    if (bounce == P->max_bounces) break;
+                    v3 p_evt = add(ro, muls(rd, ms.t));
+                    v3 wo_med = neg(rd);
+                    float g = Mm->hg_g;
+                    if (nee) {   /* light_sample_direct_medium (techniques/light/light_medium.glsl:7-18) */
+                        float y0 = rnd(rng), y1 = rnd(rng);
+                        lsample_t ls = lighting_sample(S, p_evt, y0, y1);
+                        if (ls.pdf > 0.0f) {
+                            v3 light_p = add(p_evt, muls(ls.wi, ls.distance));
+                            v3 vis = shadow_transmittance_media(S, p_evt, ls.wi, light_p,
+                                                                S->has_null ? P->max_null - null_crossings : 0, n_any);
+                            if (!is_black(vis)) {
+                                float f = hg_value(ls.wi, wo_med, g);
+                                float w = 1.0f;
+                                if (mis && !(ls.flags & 1)) w = power_heuristic(ls.pdf, hg_pdf(ls.wi, wo_med, g));
+                                v3 c = mul(mul(T, ls.radiance), V(f, f, f));
+                                c = mul(c, vis);
+                                c = muls(c, w);
+                                L = add(L, divs(c, ls.pdf));
+                            }
+                        }
+                    }
+                    /* kernel_sample_phase (techniques/kernel/kernel_phase.glsl:7-15) */
+                    float z0 = rnd(rng), z1 = rnd(rng);
+                    float ppdf;
+                    v3 wi = hg_sample(wo_med, g, z0, z1, &ppdf);
+                    prev_was_delta = 0; prev_bsdf_pdf = ppdf; prev_qp = p_evt;
+                    ro = p_evt; rd = wi;
+                    /* roulette_interior with medium_survival = sigma_s / sigma_t (pt.ts:230-283) */
+                    if (P->rr_start >= 0 && bounce >= P->rr_start) {
+                        v3 ss = lv(Mm->sigma_s), st = add(lv(Mm->sigma_s), lv(Mm->sigma_a));
+                        v3 surv = V(ss.x / fmaxf_(st.x, 1e-9f), ss.y / fmaxf_(st.y, 1e-9f), ss.z / fmaxf_(st.z, 1e-9f));
+                        float p = spectrum_max(surv);
+                        if (p <= 0.0f) break;
+                        if (rnd(rng) > p) break;
+                        T = divs(T, p);
+                    }
+                    continue;
+                }
+            }
+        }
+        if (!boundary) {
             /* kernel_score_miss + combiner_w_env */
             float w = 1.0f;
             if (env_samplable(S) && nee) {
@@ -1029,7 +1167,17 @@ static v3 transport_trace(const or_scene* S, const or_program* P, v3 ro, v3 rd, 
             L = add(L, mul(muls(T, w), Le));
             break;
         }
+        if (media && hit.region_from != current_medium) current_medium = hit.region_from;   /* self-heal */
         int mat = material_of(S, hit.region_owner);
+        if (S->has_null && is_null_interface(S, mat)) {
+            current_medium = hit.region_to;
+            float side = dot(rd, hit.ng) >= 0.0f ? 1.0f : -1.0f;
+            ro = add(hit.p, muls(muls(hit.ng, side), hit.eps));
+            null_crossings++;
+            if (null_crossings > P->max_null) break;
+            bounce--;
+            continue;
+        }
         const pt_material* M = &S->materials[mat];
         v3 wo = neg(rd);
         /* kernel_score_emitter_hit */
@@ -1053,17 +1201,21 @@ static v3 transport_trace(const or_scene* S, const or_program* P, v3 ro, v3 rd, 
                 v3 light_p = add(hit.p, muls(ls.wi, ls.distance));
                 v3 so = ray_spawn_origin(&hit, ls.wi);
                 v3 sd = normalize(sub(light_p, so));
-                float maxDist = length3(sub(light_p, so)) - SHADOW_BACKOFF;
-                (*n_any)++;
-                int blocked = scene_any(S, so, sd, maxDist);
-                if (!blocked) {
+                v3 vis;
+                if (media) vis = shadow_transmittance_media(S, so, sd, light_p, S->has_null ? P->max_null - null_crossings : 0, n_any);
+                else {
+                    float maxDist = length3(sub(light_p, so)) - SHADOW_BACKOFF;
+                    (*n_any)++;
+                    vis = scene_any(S, so, sd, maxDist) ? V(0.0f, 0.0f, 0.0f) : V(1.0f, 1.0f, 1.0f);
+                }
+                if (!is_black(vis)) {
                     v3 f = surface_eval(M, ls.wi, wo, &hit);
                     float cos_i = fabsf(dot(ls.wi, hit.frame.n));
                     float w = 1.0f;
                     if (mis && !(ls.flags & 1)) w = power_heuristic(ls.pdf, surface_pdf(M, ls.wi, wo, &hit));
                     v3 c = mul(mul(T, ls.radiance), f);
                     c = muls(c, cos_i);
-                    /* vis = 1 */
+                    c = mul(c, vis);
                     c = muls(c, w);
                     c = divs(c, ls.pdf);
                     L = add(L, c);
@@ -1076,6 +1228,7 @@ static v3 transport_trace(const or_scene* S, const or_program* P, v3 ro, v3 rd, 
         if (is_black(bs.weight)) break;
         T = mul(T, bs.weight);
         prev_was_delta = (bs.flags & LOBE_DELTA) != 0; prev_bsdf_pdf = bs.pdf; prev_qp = hit.p;
+        if (media && (bs.flags & LOBE_TRANSMISSION)) current_medium = hit.region_to;
         if (S->has_transmission && (bs.flags & LOBE_TRANSMISSION)) {
             float r = ior_of(S, hit.region_to) / ior_of(S, hit.region_from);
             eta_scale *= r * r;
@@ -1112,6 +1265,11 @@ or_scene* or_scene_create(const pt_scene_desc* d) {
     S->lights = (pt_light*)malloc(sizeof(pt_light) * (size_t)(d->n_lights > 0 ? d->n_lights : 1));
     memcpy(S->lights, d->lights, sizeof(pt_light) * (size_t)d->n_lights);
     S->env = d->env;
+    S->ambient_material = d->ambient_material;
+    for (int m = 0; m < S->n_materials; m++) {
+        if (S->materials[m].has_medium) S->has_media = 1;
+        if (S->materials[m].model == PT_MAT_NULL) S->has_null = 1;
+    }
     S->env_rot_cs[0] = (float)cos((double)d->env.rotation);
     S->env_rot_cs[1] = (float)sin((double)d->env.rotation);
     for (int m = 0; m < S->n_materials; m++) if (S->materials[m].model == PT_MAT_DIELECTRIC) S->has_transmission = 1;
@@ -1224,6 +1382,7 @@ void or_render(const or_scene* S, const pt_program* prog, const pt_params* par, 
     or_program P;
     P.estimator = prog->estimator; P.max_bounces = prog->max_bounces; P.rr_start = prog->rr_start;
     P.rr_max = prog->rr_max_survival; P.width = prog->width; P.height = prog->height;
+    P.max_null = prog->max_null_crossings;
     int W = P.width, H = P.height;
     uint64_t nc = 0, na = 0;
     v3 F = lv(par->cam_forward), R = lv(par->cam_right), U = lv(par->cam_up), O = lv(par->cam_position);

@@ -10,7 +10,7 @@ import numpy as np
 ABI_VERSION = 1
 
 OBJ_MESH, OBJ_QUAD, OBJ_SPHERE = 0, 1, 2
-MAT_LAMBERT, MAT_DIELECTRIC, MAT_GGX = 0, 1, 2
+MAT_LAMBERT, MAT_DIELECTRIC, MAT_GGX, MAT_NULL = 0, 1, 2, 3
 LIGHT_QUAD, LIGHT_POINT = 0, 1
 ENV_NONE, ENV_SKY = 0, 1
 EST_PT, EST_NEE, EST_MIS = 0, 1, 2
@@ -46,14 +46,14 @@ class PtSceneDesc(C.Structure):
                 ("n_objects", C.c_int32), ("objects", C.c_void_p),
                 ("n_materials", C.c_int32), ("materials", C.c_void_p),
                 ("n_lights", C.c_int32), ("lights", C.c_void_p),
-                ("env", PtEnv)]
+                ("env", PtEnv), ("ambient_material", C.c_int32), ("pad0", C.c_int32)]
 
 
 class PtProgram(C.Structure):
     _fields_ = [("abi_version", C.c_uint32), ("estimator", C.c_int32), ("max_bounces", C.c_int32),
                 ("rr_start", C.c_int32), ("rr_max_survival", C.c_float), ("color", C.c_int32),
                 ("width", C.c_int32), ("height", C.c_int32), ("frames_in_flight", C.c_int32),
-                ("pad", C.c_int32 * 3)]
+                ("max_null_crossings", C.c_int32), ("pad", C.c_int32 * 2)]
 
 
 class PtParams(C.Structure):
@@ -79,13 +79,14 @@ def _ptr(a):
 class SceneArrays:
     """Host-side scene in the ABI layout; keeps the numpy arrays alive for the desc."""
 
-    def __init__(self, tri_vertices, tri_object, objects, materials, lights, env=None):
+    def __init__(self, tri_vertices, tri_object, objects, materials, lights, env=None, ambient_material=-1):
         self.tri_vertices = np.ascontiguousarray(tri_vertices, dtype=np.float32).reshape(-1, 9)
         self.tri_object = np.ascontiguousarray(tri_object, dtype=np.int32).reshape(-1)
         self.objects = np.ascontiguousarray(objects, dtype=OBJECT_DTYPE)
         self.materials = np.ascontiguousarray(materials, dtype=MATERIAL_DTYPE)
         self.lights = np.ascontiguousarray(lights, dtype=LIGHT_DTYPE)
         self.env = env if env is not None else PtEnv(ENV_NONE, 512, 256, 0, 1.0, 0.0, 0.5, 0.0)
+        self.ambient_material = int(ambient_material)
         assert self.tri_object.shape[0] == self.tri_vertices.shape[0]
 
     @property
@@ -105,6 +106,7 @@ class SceneArrays:
         d.n_lights = len(self.lights)
         d.lights = _ptr(self.lights)
         d.env = self.env
+        d.ambient_material = self.ambient_material
         return d
 
 

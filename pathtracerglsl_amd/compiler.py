@@ -16,7 +16,7 @@ import numpy as np
 from . import abi
 
 ESTIMATORS = {"none": abi.EST_PT, "pt": abi.EST_PT, "nee": abi.EST_NEE, "mis": abi.EST_MIS}
-MODELS = {"lambert": abi.MAT_LAMBERT, "dielectric": abi.MAT_DIELECTRIC, "ggx": abi.MAT_GGX}
+MODELS = {"lambert": abi.MAT_LAMBERT, "dielectric": abi.MAT_DIELECTRIC, "ggx": abi.MAT_GGX, "none": abi.MAT_NULL}
 
 
 class CompileError(ValueError):
@@ -31,6 +31,7 @@ class RenderStrategy:
     rr_start: int = 3                     # russianRoulette.startDepth (-1 = off)
     rr_max_survival: float = 0.95
     frames_in_flight: int = 1
+    max_null_crossings: int = 32          # measurement.maxNullCrossings (plan/measurement.ts:12)
 
 
 @dataclass
@@ -124,6 +125,14 @@ def compile_scene(scene, strategy=None, width=256, height=256, reset_salt=1234):
         rec["alpha"] = max(1e-3, rough * rough)                  # ggx.ts derived alpha
         rec["transmittance"][:3] = m.get("transmittance", [1.0, 1.0, 1.0])
         rec["ior"] = float(m.get("ior", 1.5))
+        med = m.get("medium")
+        if med is not None:
+            def rgb(v):
+                return [float(v)] * 3 if not hasattr(v, "__len__") else [float(x) for x in v]
+            rec["has_medium"] = 1
+            rec["sigma_a"][:3] = rgb(med.get("sigma_a", 0.0))
+            rec["sigma_s"][:3] = rgb(med.get("sigma_s", 0.0))
+            rec["hg_g"] = float(med.get("phase_g", 0.0))
         materials.append(rec)
 
     tri_chunks, tri_obj_chunks, objects = [], [], []
@@ -212,9 +221,13 @@ def compile_scene(scene, strategy=None, width=256, height=256, reset_salt=1234):
 
     tv = np.concatenate(tri_chunks) if tri_chunks else np.zeros((0, 9), np.float32)
     to = np.concatenate(tri_obj_chunks) if tri_obj_chunks else np.zeros(0, np.int32)
+    amb = scene.get("ambient_medium")
+    if amb is not None and amb not in mat_index:
+        raise CompileError(f"ambientMedium references unknown material '{amb}'")
     arrays = abi.SceneArrays(tv, to, np.array(objects, abi.OBJECT_DTYPE),
                              np.array(materials, abi.MATERIAL_DTYPE),
-                             np.array(light_recs, abi.LIGHT_DTYPE), env)
+                             np.array(light_recs, abi.LIGHT_DTYPE), env,
+                             mat_index[amb] if amb is not None else -1)
 
     prog = abi.PtProgram()
     prog.abi_version = abi.ABI_VERSION
@@ -225,6 +238,7 @@ def compile_scene(scene, strategy=None, width=256, height=256, reset_salt=1234):
     prog.color = 0
     prog.width, prog.height = width, height
     prog.frames_in_flight = max(1, strategy.frames_in_flight)
+    prog.max_null_crossings = strategy.max_null_crossings
     return CompiledRenderer(id=f"{strategy.direct_lighting}-{scene.get('id', 'scene')}", scene=arrays,
                             program=prog, width=width, height=height, camera=scene["camera"],
                             reset_salt=reset_salt)

@@ -92,6 +92,8 @@ struct DevScene {
     int has_grid;
     int has_transmission;
     int emissive_solid;
+    int ambient_material;       // material of region -1 (ambientMedium), -1 = none
+    int has_media, has_null;
 };
 
 // Wave-aggregated queue append: every active lane calls it (convergent), lanes with

@@ -38,11 +38,12 @@ constexpr int TBLOCK = 128;     // traversal blocks (2 waves)
 constexpr int SBLOCK = 256;     // shading blocks
 constexpr int MAX_GRID = 8192;
 
-// Shading categories: the extend kernel bins every path by what its hit needs, so each
-// shade launch runs one code path (no material divergence inside a wave).
-enum { CAT_MISS = 0, CAT_LAMBERT = 1, CAT_GGX = 2, CAT_DIELECTRIC = 3, N_CAT = 4 };
+// Shading categories: the bin kernel sorts every path of a bounce by what its vertex needs,
+// so each shade launch runs one code path (no material divergence inside a wave).
+enum { CAT_MISS = 0, CAT_LAMBERT = 1, CAT_GGX = 2, CAT_DIELECTRIC = 3, CAT_MEDIUM = 4, CAT_NULL = 5, N_CAT = 6 };
 // counts[] slots
-enum { C_RAYQ0 = 0, C_RAYQ1 = 1, C_SHADOW = 2, C_CAT0 = 3, C_EXT_FETCH = 7, C_SHD_FETCH = 8, N_COUNTS = 16 };
+enum { C_RAYQ0 = 0, C_RAYQ1 = 1, C_SHADOW = 2, C_CAT0 = 3, C_EXT_FETCH = 9, C_SHD_FETCH = 10, C_CAM_MEDIUM = 11,
+       N_COUNTS = 16 };
 
 struct Paths {
     float4* ray_o;   // origin.xyz
@@ -51,18 +52,24 @@ struct Paths {
     float4* rad;     // radiance.xyz
     uint2* rng;      // rng_base, rng_dim
     float4* prev;    // prev_query.p.xyz, prev_bsdf_pdf
-    int* flags;      // bit 0: prev_was_delta
-    float2* hit;     // t, primitive id (bits)
-    int* q[2];       // ray queues (current / next bounce)
+    int* flags;      // bit 0 prev_was_delta, bits 8..15 null crossings, bits 16..31 bounce
+    int* medium;     // current_medium region (media scenes)
+    float2* hit;     // t (medium event t after binning), primitive id (bits)
+    int* q[2];       // ray queues (current / next iteration)
     int* catq[N_CAT];// shading queues
     int* counts;
     int* s_path;
-    float4* s_o;     // shadow origin.xyz, maxDist
-    float4* s_d;     // shadow direction.xyz
-    float4* s_c;     // contribution.xyz
+    float4* s_o;     // shadow origin.xyz, maxDist (opaque) | crossings left (media, int bits)
+    float4* s_d;     // shadow direction.xyz, light pdf
+    float4* s_c;     // partial contribution.xyz (before visibility), MIS weight
+    float4* s_l;     // light point.xyz (media walker)
     unsigned long long* hist;   // [0] closest queries, [1] any-hit queries
     unsigned long long* trav;   // [0] nodes visited, [1] triangles tested (COUNT builds)
 };
+
+PT_DEV int fl_bounce(int f) { return (f >> 16) & 0xFFFF; }
+PT_DEV int fl_nulls(int f) { return (f >> 8) & 0xFF; }
+PT_DEV int mk_flags(bool delta, int nulls, int bounce) { return (delta ? 1 : 0) | (nulls << 8) | (bounce << 16); }
 
 struct Camera {
     float ox, oy, oz, fx, fy, fz, rx, ry, rz, ux, uy, uz, tan_fov, aspect;
@@ -70,7 +77,14 @@ struct Camera {
     int off_x, off_y;
 };
 
-__global__ void __launch_bounds__(SBLOCK) k_raygen(Paths P, Camera C, int W, int H, int n_paths, uint32_t sample0) {
+// path_state_init's current_medium = scene_region_at(camera origin), once per wave.
+template <int STACK>
+__global__ void __launch_bounds__(64) k_camera_medium(DevScene S, Paths P, Camera C) {
+    __shared__ uint2 stk[STACK * 64];
+    if (threadIdx.x == 0) P.counts[C_CAM_MEDIUM] = region_at<64>(S, mk3(C.ox, C.oy, C.oz), stk);
+}
+
+__global__ void __launch_bounds__(SBLOCK) k_raygen(Paths P, Camera C, int W, int H, int n_paths, uint32_t sample0, int media) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) P.counts[C_RAYQ0] = n_paths;
     if (i >= n_paths) return;
@@ -97,7 +111,8 @@ __global__ void __launch_bounds__(SBLOCK) k_raygen(Paths P, Camera C, int W, int
     P.rad[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     P.rng[i] = make_uint2(r.base, r.dim);
     P.prev[i] = make_float4(C.ox, C.oy, C.oz, 0.0f);
-    P.flags[i] = 1;
+    P.flags[i] = mk_flags(true, 0, 0);
+    if (media) P.medium[i] = P.counts[C_CAM_MEDIUM];
     P.q[0][i] = i;
 }
 
@@ -181,68 +196,72 @@ __global__ void __launch_bounds__(TBLOCK) k_extend(DevScene S, Paths P, int qsel
     }
 }
 
-// shadow: any hit for every NEE ray; unoccluded contributions are added to their path.
+// shadow (opaque scenes): any hit for every NEE ray; unoccluded contributions are added to
+// their path as ((a * vis) * w) / pdf, the reference's operation order with vis = 1.
 template <int STACK>
 __global__ void __launch_bounds__(TBLOCK) k_shadow(DevScene S, Paths P) {
     __shared__ uint2 stk[STACK * TBLOCK];
-#if !PTMI_SHADOW_PERSISTENT
-    {
-        const int n = P.counts[C_SHADOW];
-        TraceCounters cnt = {0, 0};
-        for (int base = blockIdx.x * TBLOCK; base < n; base += gridDim.x * TBLOCK) {
-            int k = base + threadIdx.x;
-            if (k >= n) break;
-            float4 o4 = P.s_o[k], d4 = P.s_d[k];
-            f3 o = xyz(o4), d = xyz(d4);
-            float maxd = o4.w; int prim = -1;
-            bool blocked = trace_analytic<true>(S, o, d, maxd, prim);
-            if (!blocked) blocked = trace_wide<true, STACK, TBLOCK, false>(S, o, d, maxd, prim, stk, cnt);
-            if (!blocked) {
-                int i = P.s_path[k];
-                float4 c = P.s_c[k], L = P.rad[i];
-                P.rad[i] = make_float4(L.x + c.x, L.y + c.y, L.z + c.z, 0.0f);
-            }
-        }
-        return;
-    }
-#endif
-    const uint32_t n = (uint32_t)P.counts[C_SHADOW];
+    const int n = P.counts[C_SHADOW];
     TraceCounters cnt = {0, 0};
-    LaneFeeder feed; feed.init();
-    TravState st;
-    bool active = false;
-    int k = -1;
-    const bool have_bvh = S.n_cw_nodes > 0;
-    for (;;) {
-        int kk = feed.feed<FEED_CHUNK>(&P.counts[C_SHD_FETCH], n, !active, FEED_MIN_IDLE);
-        if (kk >= 0) {
-            k = kk;
-            float4 o4 = P.s_o[k], d4 = P.s_d[k];
-            f3 o = xyz(o4), d = xyz(d4);
-            float maxd = o4.w; int prim = -1;
-            bool blocked = trace_analytic<true>(S, o, d, maxd, prim);
-            if (!blocked) {
-                if (have_bvh) {
-                    trace_init(st, o, d, maxd);
-                    active = true;
-                } else {                       // no triangles: unoccluded, score now
-                    int i = P.s_path[k];
-                    float4 c = P.s_c[k], L = P.rad[i];
-                    P.rad[i] = make_float4(L.x + c.x, L.y + c.y, L.z + c.z, 0.0f);
-                }
-            }
+    for (int base = blockIdx.x * TBLOCK; base < n; base += gridDim.x * TBLOCK) {
+        int k = base + threadIdx.x;
+        if (k >= n) break;
+        float4 o4 = P.s_o[k], d4 = P.s_d[k];
+        f3 o = xyz(o4), d = xyz(d4);
+        float maxd = o4.w; int prim = -1;
+        bool blocked = trace_analytic<true>(S, o, d, maxd, prim);
+        if (!blocked) blocked = trace_wide<true, STACK, TBLOCK, false>(S, o, d, maxd, prim, stk, cnt);
+        if (!blocked) {
+            int i = P.s_path[k];
+            float4 c = P.s_c[k], L = P.rad[i];
+            f3 v = ((xyz(c) * c.w) / d4.w);
+            P.rad[i] = make_float4(L.x + v.x, L.y + v.y, L.z + v.z, 0.0f);
         }
-        if (__ballot(active) == 0) {
-            if (feed.exhausted) break;
-            continue;
+    }
+}
+
+// shadow (media scenes): the segment walker of transport/shadow/media/media.glsl:29-62 —
+// closest hits up to the light point, Beer-Lambert per segment in the current medium, pass
+// through null interfaces, anything else blocks (dielectrics are opaque to shadow rays).
+template <int STACK>
+__global__ void __launch_bounds__(TBLOCK) k_shadow_media(DevScene S, Paths P) {
+    __shared__ uint2 stk[STACK * TBLOCK];
+    const int n = P.counts[C_SHADOW];
+    TraceCounters cnt = {0, 0};
+    for (int base = blockIdx.x * TBLOCK; base < n; base += gridDim.x * TBLOCK) {
+        int k = base + threadIdx.x;
+        if (k >= n) break;
+        float4 o4 = P.s_o[k], d4 = P.s_d[k], l4 = P.s_l[k];
+        f3 o = xyz(o4), d = xyz(d4), light_p = xyz(l4);
+        const int crossings_left = __float_as_int(o4.w);
+        f3 T = mk3(1.0f, 1.0f, 1.0f);
+        bool blocked = true;
+        int medium = region_at<TBLOCK>(S, o, stk);
+        for (int crossed = 0; crossed <= crossings_left; crossed++) {
+            float remaining = length(light_p - o) - SHADOW_BACKOFF;
+            if (remaining <= 0.0f) { blocked = false; break; }
+            float tmax = MAX_DIST; int prim = -1;
+            if (S.n_cw_nodes > 0) trace_wide<false, STACK, TBLOCK, false>(S, o, d, tmax, prim, stk, cnt);
+            trace_analytic<false>(S, o, d, tmax, prim);
+            bool hit_boundary = prim >= 0 && tmax < remaining;
+            float seg_len = hit_boundary ? tmax : remaining;
+            int med_mat = material_of(S, medium);
+            if (material_has_medium(S, med_mat)) T = T * medium_transmittance(S.materials[med_mat], seg_len);
+            if (!hit_boundary) { blocked = false; break; }
+            Hit h; bool back;
+            fill_hit(S, o, d, tmax, prim, h, back);
+            if (!is_null_interface(S, material_of(S, h.owner))) break;
+            classify_hit<TBLOCK>(S, h, back, true, stk);
+            medium = h.region_to;
+            float side = dot(d, h.ng) >= 0.0f ? 1.0f : -1.0f;
+            o = h.p + (h.ng * side) * h.eps;
+            d = normalize(light_p - o);
         }
-        if (active && trace_step<true, TBLOCK, false>(S.cw_nodes, S.cw_tris, st, stk, cnt)) {
-            if (st.prim < 0) {
-                int i = P.s_path[k];
-                float4 c = P.s_c[k], L = P.rad[i];
-                P.rad[i] = make_float4(L.x + c.x, L.y + c.y, L.z + c.z, 0.0f);
-            }
-            active = false;
+        if (!blocked && !is_black(T)) {
+            int i = P.s_path[k];
+            float4 c = P.s_c[k], L = P.rad[i];
+            f3 v = (((xyz(c) * T) * c.w) / d4.w);
+            P.rad[i] = make_float4(L.x + v.x, L.y + v.y, L.z + v.z, 0.0f);
         }
     }
 }
@@ -260,10 +279,14 @@ PT_DEV int block_append(int* const* counters, int which, int* s_cnt, int* s_base
     return which >= 0 ? s_base[which] + local : -1;
 }
 
-// bin: sort the bounce's hits into shading-category queues (no material divergence later).
+// bin: per path of the iteration, the medium segment first (media scenes: the analytic
+// channel-MIS distance sample of pt.ts's media branch, which may end the segment in a
+// scattering event), then the shading category of the vertex.
+template <bool MEDIA>
 __global__ void __launch_bounds__(SBLOCK) k_bin(DevScene S, Paths P, int qsel) {
     __shared__ int s_cnt[N_CAT], s_base[N_CAT];
-    int* ctr[N_CAT] = {&P.counts[C_CAT0], &P.counts[C_CAT0 + 1], &P.counts[C_CAT0 + 2], &P.counts[C_CAT0 + 3]};
+    int* ctr[N_CAT];
+    for (int c = 0; c < N_CAT; c++) ctr[c] = &P.counts[C_CAT0 + c];
     const int n = P.counts[qsel];
     const int* q = P.q[qsel];
     for (int base = blockIdx.x * SBLOCK; base < n; base += gridDim.x * SBLOCK) {
@@ -271,12 +294,31 @@ __global__ void __launch_bounds__(SBLOCK) k_bin(DevScene S, Paths P, int qsel) {
         int cat = -1, path = 0;
         if (k < n) {
             path = q[k];
-            int prim = __float_as_int(P.hit[path].y);
-            if (prim < 0) cat = CAT_MISS;
-            else {
-                int owner = prim < S.n_tris ? S.tri_object[prim] : prim - S.n_tris;
-                int model = S.materials[S.objects[owner].material].model;
-                cat = model == PT_MAT_LAMBERT ? CAT_LAMBERT : (model == PT_MAT_GGX ? CAT_GGX : CAT_DIELECTRIC);
+            float2 hv = P.hit[path];
+            int prim = __float_as_int(hv.y);
+            if (MEDIA) {
+                int med_mat = material_of(S, P.medium[path]);
+                if (material_has_medium(S, med_mat)) {
+                    uint2 rs = P.rng[path];
+                    Rng rng; rng.base = rs.x; rng.dim = rs.y;
+                    float x0 = rng.next(), x1 = rng.next();
+                    MSample ms = medium_sample_analytic(S.materials[med_mat], prim >= 0 ? hv.x : MAX_DIST, x0, x1);
+                    float4 th = P.thr[path];
+                    f3 T = xyz(th) * ms.weight;
+                    P.thr[path] = f4(T, th.w);
+                    P.rng[path] = make_uint2(rng.base, rng.dim);
+                    if (ms.scattered) { cat = CAT_MEDIUM; P.hit[path] = make_float2(ms.t, hv.y); }
+                }
+            }
+            if (cat < 0) {
+                if (prim < 0) cat = CAT_MISS;
+                else {
+                    int owner = prim < S.n_tris ? S.tri_object[prim] : prim - S.n_tris;
+                    int model = S.materials[S.objects[owner].material].model;
+                    cat = model == PT_MAT_LAMBERT ? CAT_LAMBERT
+                        : model == PT_MAT_GGX ? CAT_GGX
+                        : model == PT_MAT_DIELECTRIC ? CAT_DIELECTRIC : CAT_NULL;
+                }
             }
         }
         int slot = block_append<N_CAT>(ctr, cat, s_cnt, s_base);
@@ -284,7 +326,7 @@ __global__ void __launch_bounds__(SBLOCK) k_bin(DevScene S, Paths P, int qsel) {
     }
 }
 
-// Queue bookkeeping before a bounce (single thread).
+// Queue bookkeeping before an iteration (single thread).
 __global__ void k_bookkeep(Paths P, int cur, int next, int first) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         if (!first) P.hist[1] += (unsigned long long)P.counts[C_SHADOW];
@@ -299,12 +341,13 @@ __global__ void k_bookkeep(Paths P, int cur, int next, int first) {
 
 // The per-vertex body of transport_trace (integrators/pt/pt.ts:299-464) with the combiner
 // (transport/combiner.ts:23-102) and the kernel/light techniques (techniques/kernel/kernel.glsl,
-// techniques/light/light.glsl:8-28), for the paths of one shading category.
-template <int EST, int CAT, int STACK>
-__global__ void __launch_bounds__(SBLOCK) k_shade(DevScene S, Paths P, int qsel, int bounce, int max_bounces,
-                                                  int rr_start, float rr_max) {
-    // LDS traversal stack for the containment probe (only the dielectric category probes)
-    __shared__ uint2 stk[CAT == CAT_DIELECTRIC ? STACK * SBLOCK : 1];
+// kernel_phase.glsl, techniques/light/light.glsl, light_medium.glsl), for the paths of one
+// category. MEDIA selects the media program (region tracking, medium NEE shadow walker).
+template <int EST, int CAT, int STACK, bool MEDIA>
+__global__ void __launch_bounds__(SBLOCK) k_shade(DevScene S, Paths P, int qsel, int max_bounces,
+                                                  int rr_start, float rr_max, int max_null) {
+    constexpr bool PROBE = CAT == CAT_DIELECTRIC || MEDIA;
+    __shared__ uint2 stk[PROBE && CAT != CAT_MISS && CAT != CAT_MEDIUM ? STACK * SBLOCK : 1];
     __shared__ int s_cnt[2], s_base[2];
     int* ctr[2] = {&P.counts[qsel ^ 1], &P.counts[C_SHADOW]};
     const int n = P.counts[C_CAT0 + CAT];
@@ -318,8 +361,9 @@ __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene S, Paths P, int qsel,
         const bool valid = k < n;
         const int i = valid ? q[k] : 0;
         bool cont = false, shadow = false;
-        f3 s_o = mk3(0, 0, 0), s_d = mk3(0, 0, 0), s_c = mk3(0, 0, 0);
-        float s_max = 0.0f;
+        f3 s_o = mk3(0, 0, 0), s_d = mk3(0, 0, 0), s_a = mk3(0, 0, 0), s_l = mk3(0, 0, 0);
+        float s_max = 0.0f, s_w = 0.0f, s_pdf = 1.0f;
+        int s_cross = 0;
         if (valid) {
             f3 rd = xyz(P.ray_d[i]);
             float4 th = P.thr[i];
@@ -327,7 +371,12 @@ __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene S, Paths P, int qsel,
             f3 L = xyz(P.rad[i]);
             float4 pv = P.prev[i];
             f3 prev_qp = xyz(pv); float prev_pdf = pv.w;
-            bool prev_delta = (P.flags[i] & 1) != 0;
+            const int fl = P.flags[i];
+            bool prev_delta = (fl & 1) != 0;
+            int nulls = fl_nulls(fl);
+            const int bounce = fl_bounce(fl);
+            int medium = MEDIA ? P.medium[i] : -1;
+            const int crossings_left = S.has_null ? max_null - nulls : 0;
             if (CAT == CAT_MISS) {
                 // kernel_score_miss + combiner_w_env
                 float w = 1.0f;
@@ -337,81 +386,148 @@ __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene S, Paths P, int qsel,
                     else w = 0.0f;
                 }
                 if (S.env_type != PT_ENV_NONE) L = L + (T * w) * env_radiance(S, rd);
+            } else if (CAT == CAT_MEDIUM) {
+                // ---- medium event (pt.ts media branch: scattered) ----
+                if (bounce < max_bounces) {
+                    const pt_material& Mm = S.materials[material_of(S, medium)];
+                    const float g = Mm.hg_g;
+                    f3 ro = xyz(P.ray_o[i]);
+                    const float t = P.hit[i].x;
+                    f3 p_evt = ro + rd * t;
+                    f3 wo_med = -rd;
+                    uint2 rs = P.rng[i];
+                    Rng rng; rng.base = rs.x; rng.dim = rs.y;
+                    if (NEE) {   // light_sample_direct_medium
+                        float y0 = rng.next(), y1 = rng.next();
+                        LSample ls = lighting_sample(S, p_evt, y0, y1);
+                        if (ls.pdf > 0.0f) {
+                            s_l = p_evt + ls.wi * ls.distance;
+                            s_o = p_evt; s_d = ls.wi; s_cross = crossings_left;
+                            float f = hg_value(ls.wi, wo_med, g);
+                            s_w = 1.0f;
+                            if (MIS && !(ls.flags & 1)) s_w = power_heuristic(ls.pdf, hg_pdf(ls.wi, wo_med, g));
+                            s_a = (T * ls.radiance) * mk3(f, f, f);
+                            s_pdf = ls.pdf;
+                            shadow = true;
+                        }
+                    }
+                    // kernel_sample_phase
+                    float z0 = rng.next(), z1 = rng.next();
+                    float ppdf;
+                    f3 wi = hg_sample(wo_med, g, z0, z1, ppdf);
+                    cont = true;
+                    // roulette_interior with medium_survival = sigma_s / max(sigma_t, 1e-9)
+                    if (rr_start >= 0 && bounce >= rr_start) {
+                        f3 ss = ld3(Mm.sigma_s), st = ld3(Mm.sigma_s) + ld3(Mm.sigma_a);
+                        f3 surv = mk3(ss.x / gmax(st.x, 1e-9f), ss.y / gmax(st.y, 1e-9f), ss.z / gmax(st.z, 1e-9f));
+                        float p = spectrum_max(surv);
+                        if (p <= 0.0f) cont = false;
+                        else if (rng.next() > p) cont = false;
+                        else T = T / p;
+                    }
+                    if (cont) {
+                        P.ray_o[i] = f4(p_evt, 0.0f);
+                        P.ray_d[i] = f4(wi, 0.0f);
+                        P.thr[i] = f4(T, eta_scale);
+                        P.prev[i] = f4(p_evt, ppdf);
+                        P.flags[i] = mk_flags(false, nulls, bounce + 1);
+                    }
+                    P.rng[i] = make_uint2(rng.base, rng.dim);
+                }
             } else {
                 float2 hv = P.hit[i];
                 const int prim = __float_as_int(hv.y);
                 f3 ro = xyz(P.ray_o[i]);
-                uint2 rs = P.rng[i];
-                Rng rng; rng.base = rs.x; rng.dim = rs.y;
                 const bool emitters = S.n_lights > 0;
                 Hit h; bool back;
                 fill_hit(S, ro, rd, hv.x, prim, h, back);
                 const pt_material& M = S.materials[S.objects[h.owner].material];
-                bool need = CAT == CAT_DIELECTRIC || (S.emissive_solid && back);
+                bool need = PROBE || (S.emissive_solid && back);
                 classify_hit<SBLOCK>(S, h, back, need, stk);
-                f3 wo = -rd;
-                // kernel_score_emitter_hit
-                if (h.region_to >= 0) {
-                    const pt_material& E = S.materials[S.objects[h.region_to].material];
-                    if (E.emissive) {
-                        float w = 1.0f;
-                        if (NEE && emitters) {
-                            int lid = light_of(S, h.region_to);
-                            if (lid < 0 || prev_delta) w = 1.0f;
-                            else if (MIS) w = power_heuristic(prev_pdf, lighting_pdf(S, prev_qp, rd, lid, h.p));
-                            else w = 0.0f;
-                        }
-                        L = L + (T * w) * ld3(E.emission);
+                if (MEDIA && h.region_from != medium) medium = h.region_from;   // self-heal
+                if (CAT == CAT_NULL) {
+                    // null interface: not an optical event; no bounce consumed
+                    medium = h.region_to;
+                    float side = dot(rd, h.ng) >= 0.0f ? 1.0f : -1.0f;
+                    nulls++;
+                    cont = nulls <= max_null;
+                    if (cont) {
+                        P.ray_o[i] = f4(h.p + (h.ng * side) * h.eps, 0.0f);
+                        P.flags[i] = mk_flags(prev_delta, nulls, bounce);
+                        if (MEDIA) P.medium[i] = medium;
                     }
-                }
-                if (bounce < max_bounces) {
-                    // light_sample_direct (skipped for delta-only materials)
-                    if (NEE && CAT != CAT_DIELECTRIC) {
-                        float x0 = rng.next(), x1 = rng.next();
-                        LSample ls = lighting_sample(S, h.p, x0, x1);
-                        if (ls.pdf > 0.0f) {
-                            f3 light_p = h.p + ls.wi * ls.distance;
-                            float side = dot(ls.wi, h.ng) >= 0.0f ? 1.0f : -1.0f;
-                            s_o = h.p + (h.ng * side) * h.eps;
-                            s_d = normalize(light_p - s_o);
-                            s_max = length(light_p - s_o) - SHADOW_BACKOFF;
-                            f3 f = surface_eval<MODEL>(M, ls.wi, wo, h);
-                            float cos_i = fabsf(dot(ls.wi, h.frame.n));
+                } else {
+                    uint2 rs = P.rng[i];
+                    Rng rng; rng.base = rs.x; rng.dim = rs.y;
+                    f3 wo = -rd;
+                    // kernel_score_emitter_hit
+                    int mat_emit = h.region_to >= -1 ? material_of(S, h.region_to) : -1;
+                    if (mat_emit >= 0) {
+                        const pt_material& E = S.materials[mat_emit];
+                        if (E.emissive) {
                             float w = 1.0f;
-                            if (MIS && !(ls.flags & 1)) w = power_heuristic(ls.pdf, surface_pdf<MODEL>(M, ls.wi, wo, h));
-                            s_c = ((((T * ls.radiance) * f) * cos_i) * w) / ls.pdf;
-                            shadow = true;
+                            if (NEE && emitters) {
+                                int lid = light_of(S, h.region_to);
+                                if (lid < 0 || prev_delta) w = 1.0f;
+                                else if (MIS) w = power_heuristic(prev_pdf, lighting_pdf(S, prev_qp, rd, lid, h.p));
+                                else w = 0.0f;
+                            }
+                            L = L + (T * w) * ld3(E.emission);
                         }
                     }
-                    // kernel_sample_continuation
-                    float uc = rng.next(), ux = rng.next(), uy = rng.next();
-                    ISample bs = surface_sample<MODEL>(S, M, wo, h, uc, ux, uy);
-                    if (!is_black(bs.weight)) {
-                        T = T * bs.weight;
-                        prev_delta = (bs.flags & LOBE_DELTA) != 0;
-                        prev_pdf = bs.pdf;
-                        prev_qp = h.p;
-                        if (S.has_transmission && (bs.flags & LOBE_TRANSMISSION)) {
-                            float r = ior_of(S, h.region_to) / ior_of(S, h.region_from);
-                            eta_scale *= r * r;
+                    if (bounce < max_bounces) {
+                        // light_sample_direct (skipped for delta-only materials)
+                        if (NEE && CAT != CAT_DIELECTRIC) {
+                            float x0 = rng.next(), x1 = rng.next();
+                            LSample ls = lighting_sample(S, h.p, x0, x1);
+                            if (ls.pdf > 0.0f) {
+                                f3 light_p = h.p + ls.wi * ls.distance;
+                                float side = dot(ls.wi, h.ng) >= 0.0f ? 1.0f : -1.0f;
+                                s_o = h.p + (h.ng * side) * h.eps;
+                                s_d = normalize(light_p - s_o);
+                                s_max = length(light_p - s_o) - SHADOW_BACKOFF;
+                                s_l = light_p; s_cross = crossings_left;
+                                f3 f = surface_eval<MODEL>(M, ls.wi, wo, h);
+                                float cos_i = fabsf(dot(ls.wi, h.frame.n));
+                                s_w = 1.0f;
+                                if (MIS && !(ls.flags & 1)) s_w = power_heuristic(ls.pdf, surface_pdf<MODEL>(M, ls.wi, wo, h));
+                                s_a = ((T * ls.radiance) * f) * cos_i;
+                                s_pdf = ls.pdf;
+                                shadow = true;
+                            }
                         }
-                        cont = true;
-                        // roulette (pt.ts:204-222)
-                        if (rr_start >= 0 && bounce >= rr_start) {
-                            float m = S.has_transmission ? spectrum_max(T) * eta_scale : spectrum_max(T);
-                            float p = gmin(rr_max, m);
-                            if (p <= 0.0f) cont = false;
-                            else if (rng.next() > p) cont = false;
-                            else T = T / p;
-                        }
-                        if (cont) {
-                            float side = dot(bs.wi, h.ng) >= 0.0f ? 1.0f : -1.0f;
-                            P.ray_o[i] = f4(h.p + (h.ng * side) * h.eps, 0.0f);
-                            P.ray_d[i] = f4(bs.wi, 0.0f);
-                            P.thr[i] = f4(T, eta_scale);
-                            P.prev[i] = f4(prev_qp, prev_pdf);
-                            P.flags[i] = prev_delta ? 1 : 0;
-                            P.rng[i] = make_uint2(rng.base, rng.dim);
+                        // kernel_sample_continuation
+                        float uc = rng.next(), ux = rng.next(), uy = rng.next();
+                        ISample bs = surface_sample<MODEL>(S, M, wo, h, uc, ux, uy);
+                        if (!is_black(bs.weight)) {
+                            T = T * bs.weight;
+                            prev_delta = (bs.flags & LOBE_DELTA) != 0;
+                            prev_pdf = bs.pdf;
+                            prev_qp = h.p;
+                            if (MEDIA && (bs.flags & LOBE_TRANSMISSION)) medium = h.region_to;
+                            if (S.has_transmission && (bs.flags & LOBE_TRANSMISSION)) {
+                                float r = ior_of(S, h.region_to) / ior_of(S, h.region_from);
+                                eta_scale *= r * r;
+                            }
+                            cont = true;
+                            // roulette (pt.ts:204-222)
+                            if (rr_start >= 0 && bounce >= rr_start) {
+                                float m = S.has_transmission ? spectrum_max(T) * eta_scale : spectrum_max(T);
+                                float p = gmin(rr_max, m);
+                                if (p <= 0.0f) cont = false;
+                                else if (rng.next() > p) cont = false;
+                                else T = T / p;
+                            }
+                            if (cont) {
+                                float side = dot(bs.wi, h.ng) >= 0.0f ? 1.0f : -1.0f;
+                                P.ray_o[i] = f4(h.p + (h.ng * side) * h.eps, 0.0f);
+                                P.ray_d[i] = f4(bs.wi, 0.0f);
+                                P.thr[i] = f4(T, eta_scale);
+                                P.prev[i] = f4(prev_qp, prev_pdf);
+                                P.flags[i] = mk_flags(prev_delta, nulls, bounce + 1);
+                                P.rng[i] = make_uint2(rng.base, rng.dim);
+                                if (MEDIA) P.medium[i] = medium;
+                            }
                         }
                     }
                 }
@@ -423,9 +539,10 @@ __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene S, Paths P, int qsel,
         int sslot = block_append<2>(ctr, shadow ? 1 : -1, s_cnt, s_base);
         if (shadow) {
             P.s_path[sslot] = i;
-            P.s_o[sslot] = f4(s_o, s_max);
-            P.s_d[sslot] = f4(s_d, 0.0f);
-            P.s_c[sslot] = f4(s_c, 0.0f);
+            P.s_o[sslot] = f4(s_o, MEDIA ? __int_as_float(s_cross) : s_max);
+            P.s_d[sslot] = f4(s_d, s_pdf);
+            P.s_c[sslot] = f4(s_a, s_w);
+            if (MEDIA) P.s_l[sslot] = f4(s_l, 0.0f);
         }
     }
 }
@@ -566,8 +683,8 @@ struct pt_renderer {
     DevBuf film;
     uint32_t sample_count = 0;
     long long cap = 0;    // path slots allocated
-    DevBuf b_ray_o, b_ray_d, b_thr, b_rad, b_rng, b_prev, b_flags, b_hit, b_q0, b_q1, b_counts,
-        b_spath, b_so, b_sd, b_sc, b_hist, b_trav, b_cat[N_CAT];
+    DevBuf b_ray_o, b_ray_d, b_thr, b_rad, b_rng, b_prev, b_flags, b_medium, b_hit, b_q0, b_q1, b_counts,
+        b_spath, b_so, b_sd, b_sc, b_sl, b_hist, b_trav, b_cat[N_CAT];
     Paths paths{};
     bool count_traversal = false;
 };
@@ -682,7 +799,11 @@ pt_status pt_scene_upload(pt_ctx* ctx, const pt_scene_desc* sd) {
     for (int m = 0; m < sd->n_materials; m++) {
         if (sd->materials[m].model == PT_MAT_DIELECTRIC) D.has_transmission = 1;
         if (sd->materials[m].model == PT_MAT_GGX) ctx->has_ggx = true;
+        if (sd->materials[m].has_medium) D.has_media = 1;
+        if (sd->materials[m].model == PT_MAT_NULL) D.has_null = 1;
     }
+    if (sd->ambient_material >= sd->n_materials) return fail(PT_E_INVALID, "pt_scene_upload: ambient_material out of range");
+    D.ambient_material = sd->ambient_material < 0 ? -1 : sd->ambient_material;
     for (int o = 0; o < sd->n_objects; o++) {
         const pt_object& ob = sd->objects[o];
         bool solid = ob.kind == PT_OBJ_SPHERE || (ob.kind == PT_OBJ_MESH && ob.closed);
@@ -839,7 +960,9 @@ static pt_status ensure_paths(pt_renderer* r, long long n) {
     struct { DevBuf* b; size_t elem; } list[] = {
         {&r->b_ray_o, 16}, {&r->b_ray_d, 16}, {&r->b_thr, 16}, {&r->b_rad, 16}, {&r->b_rng, 8}, {&r->b_prev, 16},
         {&r->b_flags, 4}, {&r->b_hit, 8}, {&r->b_q0, 4}, {&r->b_q1, 4}, {&r->b_spath, 4}, {&r->b_so, 16},
-        {&r->b_sd, 16}, {&r->b_sc, 16}, {&r->b_cat[0], 4}, {&r->b_cat[1], 4}, {&r->b_cat[2], 4}, {&r->b_cat[3], 4}};
+        {&r->b_sd, 16}, {&r->b_sc, 16}, {&r->b_sl, 16}, {&r->b_medium, 4}, {&r->b_cat[0], 4}, {&r->b_cat[1], 4},
+        {&r->b_cat[2], 4}, {&r->b_cat[3], 4}, {&r->b_cat[4], 4}, {&r->b_cat[5], 4}};
+    static_assert(N_CAT == 6, "one queue buffer per shading category");
     for (auto& e : list) {
         if (e.b->p) { HIP_TRY(hipFree(e.b->p)); e.b->p = nullptr; }
         e.b->bytes = e.elem * (size_t)n;
@@ -857,28 +980,33 @@ static pt_status ensure_paths(pt_renderer* r, long long n) {
     P.rad = (float4*)r->b_rad.p; P.rng = (uint2*)r->b_rng.p; P.prev = (float4*)r->b_prev.p;
     P.flags = (int*)r->b_flags.p; P.hit = (float2*)r->b_hit.p; P.q[0] = (int*)r->b_q0.p; P.q[1] = (int*)r->b_q1.p;
     P.counts = (int*)r->b_counts.p; P.s_path = (int*)r->b_spath.p; P.s_o = (float4*)r->b_so.p;
-    P.s_d = (float4*)r->b_sd.p; P.s_c = (float4*)r->b_sc.p; P.hist = (unsigned long long*)r->b_hist.p;
+    P.s_d = (float4*)r->b_sd.p; P.s_c = (float4*)r->b_sc.p; P.s_l = (float4*)r->b_sl.p;
+    P.medium = (int*)r->b_medium.p; P.hist = (unsigned long long*)r->b_hist.p;
     P.trav = (unsigned long long*)r->b_trav.p;
     for (int c = 0; c < N_CAT; c++) P.catq[c] = (int*)r->b_cat[c].p;
     r->cap = n;
     return PT_OK;
 }
 
-template <int EST, int STACK>
-static void launch_shade(pt_renderer* r, int cur, int b) {
+template <int EST, int STACK, bool MEDIA>
+static void launch_shade(pt_renderer* r, int cur) {
     pt_ctx* ctx = r->ctx;
     const pt_program& pg = r->prog;
+    const DevScene& S = ctx->dev;
     hipStream_t st = ctx->stream;
     const int g = ctx->shade_blocks;
-    TIMED(ctx, "shade_miss", 0, hipLaunchKernelGGL((k_shade<EST, CAT_MISS, STACK>), dim3(g), dim3(SBLOCK), 0, st, ctx->dev, r->paths, cur, b, pg.max_bounces, pg.rr_start, pg.rr_max_survival));
-    TIMED(ctx, "shade_lambert", 0, hipLaunchKernelGGL((k_shade<EST, CAT_LAMBERT, STACK>), dim3(g), dim3(SBLOCK), 0, st, ctx->dev, r->paths, cur, b, pg.max_bounces, pg.rr_start, pg.rr_max_survival));
-    if (ctx->has_ggx)
-        TIMED(ctx, "shade_ggx", 0, hipLaunchKernelGGL((k_shade<EST, CAT_GGX, STACK>), dim3(g), dim3(SBLOCK), 0, st, ctx->dev, r->paths, cur, b, pg.max_bounces, pg.rr_start, pg.rr_max_survival));
-    if (ctx->dev.has_transmission)
-        TIMED(ctx, "shade_dielectric", 0, hipLaunchKernelGGL((k_shade<EST, CAT_DIELECTRIC, STACK>), dim3(g), dim3(SBLOCK), 0, st, ctx->dev, r->paths, cur, b, pg.max_bounces, pg.rr_start, pg.rr_max_survival));
+    const int mn = pg.max_null_crossings;
+#define PT_SHADE(name, CATV) TIMED(ctx, name, 0, hipLaunchKernelGGL((k_shade<EST, CATV, STACK, MEDIA>), dim3(g), dim3(SBLOCK), 0, st, S, r->paths, cur, pg.max_bounces, pg.rr_start, pg.rr_max_survival, mn))
+    PT_SHADE("shade_miss", CAT_MISS);
+    PT_SHADE("shade_lambert", CAT_LAMBERT);
+    if (ctx->has_ggx) PT_SHADE("shade_ggx", CAT_GGX);
+    if (S.has_transmission) PT_SHADE("shade_dielectric", CAT_DIELECTRIC);
+    if (MEDIA) PT_SHADE("shade_medium", CAT_MEDIUM);
+    if (S.has_null) PT_SHADE("shade_null", CAT_NULL);
+#undef PT_SHADE
 }
 
-template <int STACK>
+template <int STACK, bool MEDIA>
 static void launch_wave(pt_renderer* r, const Camera& cam, int frames, uint32_t sample0) {
     pt_ctx* ctx = r->ctx;
     const DevScene& S = ctx->dev;
@@ -887,22 +1015,28 @@ static void launch_wave(pt_renderer* r, const Camera& cam, int frames, uint32_t 
     const int W = pg.width, H = pg.height;
     const long long n = (long long)W * H * frames;
     hipStream_t st = ctx->stream;
-    TIMED(ctx, "raygen", n, hipLaunchKernelGGL(k_raygen, dim3((unsigned)((n + SBLOCK - 1) / SBLOCK)), dim3(SBLOCK), 0, st, P, cam, W, H, (int)n, sample0));
+    if (MEDIA) hipLaunchKernelGGL((k_camera_medium<STACK>), dim3(1), dim3(64), 0, st, S, P, cam);
+    TIMED(ctx, "raygen", n, hipLaunchKernelGGL(k_raygen, dim3((unsigned)((n + SBLOCK - 1) / SBLOCK)), dim3(SBLOCK), 0, st, P, cam, W, H, (int)n, sample0, MEDIA ? 1 : 0));
     int cur = 0;
     const int tgrid = ctx->persist_blocks;
-    for (int b = 0; b <= pg.max_bounces; b++) {
-        hipLaunchKernelGGL(k_bookkeep, dim3(1), dim3(64), 0, st, P, cur, cur ^ 1, b == 0 ? 1 : 0);
+    // a null crossing consumes an iteration but no bounce (pt.ts null-interface branch)
+    const int iterations = pg.max_bounces + 1 + (S.has_null ? std::max(pg.max_null_crossings, 0) : 0);
+    for (int it = 0; it < iterations; it++) {
+        hipLaunchKernelGGL(k_bookkeep, dim3(1), dim3(64), 0, st, P, cur, cur ^ 1, it == 0 ? 1 : 0);
         if (r->count_traversal)
             TIMED(ctx, "extend_count", 0, hipLaunchKernelGGL((k_extend<STACK, true>), dim3(tgrid), dim3(TBLOCK), 0, st, S, P, cur));
         else
             TIMED(ctx, "extend", 0, hipLaunchKernelGGL((k_extend<STACK, false>), dim3(tgrid), dim3(TBLOCK), 0, st, S, P, cur));
-        TIMED(ctx, "bin", 0, hipLaunchKernelGGL(k_bin, dim3(ctx->shade_blocks), dim3(SBLOCK), 0, st, S, P, cur));
+        TIMED(ctx, "bin", 0, hipLaunchKernelGGL((k_bin<MEDIA>), dim3(ctx->shade_blocks), dim3(SBLOCK), 0, st, S, P, cur));
         switch (pg.estimator) {
-            case PT_EST_PT: launch_shade<PT_EST_PT, STACK>(r, cur, b); break;
-            case PT_EST_NEE: launch_shade<PT_EST_NEE, STACK>(r, cur, b); break;
-            default: launch_shade<PT_EST_MIS, STACK>(r, cur, b);
+            case PT_EST_PT: launch_shade<PT_EST_PT, STACK, MEDIA>(r, cur); break;
+            case PT_EST_NEE: launch_shade<PT_EST_NEE, STACK, MEDIA>(r, cur); break;
+            default: launch_shade<PT_EST_MIS, STACK, MEDIA>(r, cur);
         }
-        TIMED(ctx, "shadow", 0, hipLaunchKernelGGL((k_shadow<STACK>), dim3(tgrid), dim3(TBLOCK), 0, st, S, P));
+        if (pg.estimator != PT_EST_PT) {
+            if (MEDIA) TIMED(ctx, "shadow_media", 0, hipLaunchKernelGGL((k_shadow_media<STACK>), dim3(tgrid), dim3(TBLOCK), 0, st, S, P));
+            else TIMED(ctx, "shadow", 0, hipLaunchKernelGGL((k_shadow<STACK>), dim3(tgrid), dim3(TBLOCK), 0, st, S, P));
+        }
         cur ^= 1;
     }
     hipLaunchKernelGGL(k_bookkeep, dim3(1), dim3(64), 0, st, P, -1, -1, 0);
@@ -933,8 +1067,9 @@ pt_status pt_render_frames(pt_renderer* r, const pt_params* prm, uint32_t nframe
     uint32_t done = 0;
     while (done < nframes) {
         int f = (int)std::min<uint32_t>((uint32_t)F, nframes - done);
-        if (ctx->stack == 16) launch_wave<16>(r, cam, f, r->sample_count);
-        else launch_wave<32>(r, cam, f, r->sample_count);
+        const bool media = ctx->dev.has_media != 0;
+        if (ctx->stack == 16) { if (media) launch_wave<16, true>(r, cam, f, r->sample_count); else launch_wave<16, false>(r, cam, f, r->sample_count); }
+        else { if (media) launch_wave<32, true>(r, cam, f, r->sample_count); else launch_wave<32, false>(r, cam, f, r->sample_count); }
         HIP_TRY(hipGetLastError());
         r->sample_count += (uint32_t)f;
         done += (uint32_t)f;

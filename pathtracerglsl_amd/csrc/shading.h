@@ -483,6 +483,65 @@ PT_DEV float lighting_pdf(const DevScene& S, f3 qp, f3 wi, int lid, f3 light_p) 
     if (env_select_live(S)) sel = sel * (1.0f - S.env_select_prob);
     return sel * light_pdf_one(l, qp, light_p, wi);
 }
+// ---------------------------------------------------------------- media
+// material_of for regions (the ambient region -1 maps to the ambientMedium material),
+// the analytic channel-MIS arm (transport/volume/analytic/analytic.glsl:9-51) and HG
+// (volume_scattering/hg/hg.glsl:11-42).
+PT_DEV int material_of(const DevScene& S, int region) { return region < 0 ? S.ambient_material : S.objects[region].material; }
+PT_DEV bool is_null_interface(const DevScene& S, int mat) { return mat >= 0 && S.materials[mat].model == PT_MAT_NULL; }
+PT_DEV bool material_has_medium(const DevScene& S, int mat) { return mat >= 0 && S.materials[mat].has_medium; }
+PT_DEV float spectrum_average(f3 s) { float sum = s.x; sum += s.y; sum += s.z; return sum * (1.0f / 3.0f); }
+PT_DEV f3 spectrum_exp(f3 s) { return mk3(ptm_exp(s.x), ptm_exp(s.y), ptm_exp(s.z)); }
+struct MSample { bool scattered; float t; f3 weight; };
+PT_DEV MSample medium_sample_analytic(const pt_material& m, float t_max, float x0, float x1) {
+    MSample ms;
+    f3 ss = ld3(m.sigma_s);
+    f3 st = ld3(m.sigma_a) + ss;
+    int c = (int)(x0 * 3.0f); if (c > 2) c = 2;
+    float sc = c == 0 ? st.x : (c == 1 ? st.y : st.z);
+    float t = sc == 0.0f ? t_max : -ptm_log(1.0f - x1) / sc;
+    if (t < t_max) {
+        f3 tr = spectrum_exp((-st) * t);
+        float pdf = spectrum_average(st * tr);
+        ms.scattered = true; ms.t = t;
+        ms.weight = (ss * tr) / gmax(pdf, 1e-20f);
+    } else {
+        f3 tr = spectrum_exp((-st) * t_max);
+        float pdf = spectrum_average(tr);
+        ms.scattered = false; ms.t = t_max;
+        ms.weight = tr / gmax(pdf, 1e-20f);
+    }
+    return ms;
+}
+PT_DEV f3 medium_transmittance(const pt_material& m, float len) {
+    return spectrum_exp((-(ld3(m.sigma_a) + ld3(m.sigma_s))) * len);
+}
+PT_DEV float hg_value(f3 wi, f3 wo, float g) {
+    float c = dot(wi, -wo);
+    float d = 1.0f + g * g - 2.0f * g * c;
+    return (1.0f - g * g) / (4.0f * PTM_PI * d * sqrtf(gmax(d, 1e-8f)));
+}
+PT_DEV float hg_pdf(f3 wi, f3 wo, float g) { float h = hg_value(wi, wo, g); return spectrum_average(mk3(h, h, h)); }
+PT_DEV f3 hg_sample(f3 wo, float g, float x0, float x1, float& pdf) {
+    float cos_theta;
+    if (fabsf(g) < 1e-3f) cos_theta = 1.0f - 2.0f * x0;
+    else {
+        float sq = (1.0f - g * g) / (1.0f - g + 2.0f * g * x0);
+        cos_theta = (1.0f + g * g - sq * sq) / (2.0f * g);
+    }
+    float sin_theta = sqrtf(gmax(0.0f, 1.0f - cos_theta * cos_theta));
+    float phi = PTM_TWO_PI * x1;
+    f3 nw = -wo;
+    // build_basis (glsl/core/math.glsl:63-67)
+    f3 up = fabsf(nw.y) < 0.999f ? mk3(0.0f, 1.0f, 0.0f) : mk3(1.0f, 0.0f, 0.0f);
+    f3 t = normalize(cross(up, nw));
+    f3 b = cross(nw, t);
+    float sn, cs; ptm_sincos(phi, &sn, &cs);
+    f3 wi = normalize((t * (sin_theta * cs) + b * (sin_theta * sn)) + nw * cos_theta);
+    pdf = hg_pdf(wi, wo, g);
+    return wi;
+}
+
 PT_DEV float power_heuristic(float pf, float pg) {   // transport/math_mis.glsl:5-13
     float f2 = pf * pf;
     if (f2 >= 1.0e38f) return 1.0f;

@@ -215,3 +215,49 @@ def eta_witness():
         "environment": {"type": "none"},
         "camera": {"type": "pinhole", "position": [0.0, 1.0, 0.05], "target": [0.0, -1.0, 0.0], "fov": 0.6},
     }
+
+
+def slab_witness():
+    """F-SLAB (tests/witnesses/scenes/mediaWitness.ts:15-53): an emissive screen behind an
+    absorbing ink slab z in [-1, 0] bounded by null interfaces (model 'none'); the camera looks
+    straight through, so the centre pixel is exactly exp(-sigma_a * 1) = (0.36788, 0.13534,
+    0.01832). The plane screen is a large emissive quad facing +z."""
+    return {
+        "id": "slab",
+        "materials": {"screen": {"model": "lambert", "albedo": [0.0, 0.0, 0.0], "emission": [1.0, 1.0, 1.0]},
+                      "ink": {"model": "none", "medium": {"sigma_a": [1.0, 2.0, 4.0]}}},
+        "objects": [
+            {"type": "quad", "corner": [-50.0, -50.0, -2.0], "edge1": [100.0, 0.0, 0.0], "edge2": [0.0, 100.0, 0.0],
+             "material": "screen"},
+            {"type": "mesh", "triangles": _box_tris((-4.0, -4.0, -1.0), (4.0, 4.0, 0.0)), "material": "ink", "closed": True},
+        ],
+        "lights": [],
+        "environment": {"type": "none"},
+        "camera": {"type": "pinhole", "position": [0.0, 0.0, 2.0], "target": [0.0, 0.0, -2.0], "fov": 0.6},
+    }
+
+
+def furnace_scatter():
+    """F-BOX-M (mediaWitness.ts:54-100): the F-BOX furnace filled with a non-absorbing,
+    chromatic, anisotropic ambient haze. A non-absorbing medium keeps the equilibrium field,
+    so every channel is still exactly 0.4 in expectation."""
+    sc = mesh_furnace()
+    sc["id"] = "furnace-scatter"
+    sc["materials"]["haze"] = {"model": "none", "medium": {"sigma_a": 0.0, "sigma_s": [0.5, 1.0, 2.0], "phase_g": 0.7}}
+    sc["ambient_medium"] = "haze"
+    sc["camera"]["fov"] = 1.0
+    return sc
+
+
+def cornell_glass_medium(light_radiance=(12.0, 12.0, 12.0)):
+    """The spectral-media workload's geometry in RGB: the Cornell box with a radius-0.5 smooth
+    dielectric sphere (eta 1.5) at the centre, filled with a chromatic homogeneous medium
+    sigma_t = (0.8, 1.2, 2.0), albedo 0.9, HG g = 0.7 (analytic channel-MIS distance sampling;
+    the glass is opaque to shadow rays, so the interior is lit only through BSDF paths)."""
+    sc = cornell_box(light_radiance)
+    sc["id"] = "cornell-glass-medium"
+    st = np.array([0.8, 1.2, 2.0])
+    sc["materials"]["glass"] = {"model": "dielectric", "ior": 1.5,
+                                "medium": {"sigma_s": (0.9 * st).tolist(), "sigma_a": (0.1 * st).tolist(), "phase_g": 0.7}}
+    sc["objects"].append({"type": "sphere", "center": [0.0, 0.0, 0.0], "radius": 0.5, "material": "glass"})
+    return sc

@@ -42,3 +42,34 @@ def test_estimators_agree_on_cornell(oracle):
         assert (z < 4.5).all(), (a, b, means[a], means[b], z)
     # MIS/NEE are lower variance than pure path tracing here
     assert (errs["mis"] <= errs["none"]).all()
+
+
+def test_slab_beer_lambert(oracle):
+    # F-SLAB: the centre region converges to exp(-(1, 2, 4)); the channel-MIS estimator is noisy
+    # (most samples collide and score 0), so the gate is 4 standard errors of the samples taken.
+    # Double attenuation (exp(-2 sigma)) or a null-interface Fresnel leak misses by > 20 sigma.
+    _, samples = render(oracle, scenes.slab_witness(), compiler.RenderStrategy("none", 4, -1), 40, 40, 512)
+    x = samples[:, 18:22, 18:22, :].reshape(-1, 3)
+    expect = np.exp(-np.array([1.0, 2.0, 4.0]))
+    err = np.abs(x.mean(axis=0) - expect)
+    se = x.std(axis=0) / np.sqrt(len(x))
+    assert (err < 4 * se).all(), (x.mean(axis=0), expect, se)
+    assert (se < 0.25 * expect).all()
+
+
+def test_furnace_with_scattering_haze(oracle):
+    # F-BOX-M: chromatic non-absorbing haze keeps every channel at 0.4 (maxBounces 48, RR off)
+    film, _ = render(oracle, scenes.furnace_scatter(), compiler.RenderStrategy("none", 48, -1), 32, 32, 64)
+    m = film[..., :3].reshape(-1, 3).mean(axis=0)
+    np.testing.assert_allclose(m, [0.4, 0.4, 0.4], atol=0.006)
+
+
+def test_media_estimators_agree(oracle):
+    # the glass-medium Cornell: pt == mis within noise (medium NEE + MIS vs BSDF/phase paths)
+    means, errs = {}, {}
+    for est in ("none", "mis"):
+        _, samples = render(oracle, scenes.cornell_glass_medium(), compiler.RenderStrategy(est, 16, 3), 32, 32, 64)
+        per = samples.mean(axis=(1, 2))
+        means[est], errs[est] = per.mean(axis=0), per.std(axis=0) / np.sqrt(len(per))
+    z = np.abs(means["none"] - means["mis"]) / np.sqrt(errs["none"] ** 2 + errs["mis"] ** 2)
+    assert (z < 4.5).all(), (means, z)
