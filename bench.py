@@ -55,7 +55,7 @@ def main():
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--frames-in-flight", type=int, default=4)
-    ap.add_argument("--cpu-spp", type=int, default=2)
+    ap.add_argument("--cpu-spp", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="run a plain device copy of this size first (counter calibration runs)")
