@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define PT_ABI_VERSION 1u
+#define PT_ABI_VERSION 2u
 
 typedef enum pt_status {
     PT_OK = 0,
@@ -56,6 +56,27 @@ typedef struct pt_object {
 
 enum { PT_MAT_LAMBERT = 0, PT_MAT_DIELECTRIC = 1, PT_MAT_GGX = 2, PT_MAT_NULL = 3 };
 
+/* A spectral value in one radiometric role (src/components/descriptors.ts:43), as the
+ * spectral color model evaluates it at a path's four wavelengths
+ * (src/components/color/spectral/lookup.glsl:40-60, color/spectra/blackbody/blackbody.glsl:4-10):
+ *   PT_SPEC_ZERO       0
+ *   PT_SPEC_GREY       value                                  (exact grey: no sigmoid)
+ *   PT_SPEC_SIGMOID    clamp(0.5 + 0.5 p / sqrt(1 + p^2), 0, 1),  p = (c0 t + c1) t + c2,
+ *                      t = (lambda - 360) / 470              (Jakob-Hanika coefficients c)
+ *   PT_SPEC_BLACKBODY  scale * Planck(c0 = kelvin, c1 = normalization)
+ * GREY and SIGMOID are multiplied by `scale` when scaled != 0 (role 'coefficient':
+ * scale = 2 max(rgb)) and then by the D65 curve when d65 != 0 (role 'emission').
+ * The host computes c from the authored RGB (spectral_coefficients, lookup.glsl:16-38). */
+enum { PT_SPEC_ZERO = 0, PT_SPEC_GREY = 1, PT_SPEC_SIGMOID = 2, PT_SPEC_BLACKBODY = 3 };
+typedef struct pt_spectrum {
+    int32_t kind;
+    int16_t scaled, d65;
+    float   value;
+    float   scale;
+    float   c[3];
+    float   pad;
+} pt_spectrum;
+
 /* Material record: the MaterialProperties the generator bakes per material id
  * (src/compiler/generate/features/materials.ts, components/materials/{lambert,dielectric,ggx}).
  * PT_MAT_NULL is the reference's model 'none': a null interface that only bounds a medium
@@ -76,6 +97,8 @@ typedef struct pt_material {
     float   pad1;
     float   sigma_s[4];      /* medium */
     float   sigma_a[4];      /* medium */
+    /* spectral color model: the same properties in their roles */
+    pt_spectrum s_albedo, s_emission, s_f0, s_transmittance, s_sigma_s, s_sigma_a;
 } pt_material;
 
 enum { PT_LIGHT_QUAD = 0, PT_LIGHT_POINT = 1 };
@@ -91,6 +114,7 @@ typedef struct pt_light {
     float   radiance[4];     /* quad: Le; point: intensity */
     float   area;
     float   pad[3];
+    pt_spectrum s_radiance;  /* spectral color model, role 'emission' */
 } pt_light;
 
 enum { PT_ENV_NONE = 0, PT_ENV_SKY = 1 };
@@ -123,12 +147,23 @@ typedef struct pt_scene_desc {
     pt_env   env;
     int32_t  ambient_material;     /* material filling region -1 (ambientMedium), -1 = none */
     int32_t  pad0;
+    /* spectral color model data (the u_spectralCurves row 0 texture and the cie1931 sensor
+     * constants, src/components/color/spectral/lookup.glsl:1-7, sensor/cie1931/cie1931.ts:6-10);
+     * may be null for RGB programs */
+    const float* spectral_curves;  /* [95][4]: CIE X, Y, Z, D65 at 360 + 5 i nm */
+    float    xyz_to_rgb[9];        /* row-major XYZ -> linear sRGB */
+    float    cie_y_integral;
+    int32_t  pad1[2];
 } pt_scene_desc;
 
 /* ---- program (the Plan stage's decisions) ---- */
 
 enum { PT_EST_PT = 0, PT_EST_NEE = 1, PT_EST_MIS = 2 };
-enum { PT_COLOR_RGB = 0 };
+/* measurement.color (src/compiler/types.ts): 'rgb' or 'spectral' with four hero wavelengths;
+ * estimator.wavelengths: 'visible' (truncated sech^2) or 'uniform' on 360-830 nm
+ * (src/components/color/spectral/visible.glsl, uniform.glsl). */
+enum { PT_COLOR_RGB = 0, PT_COLOR_SPECTRAL = 1 };
+enum { PT_WL_VISIBLE = 0, PT_WL_UNIFORM = 1 };
 
 typedef struct pt_program {
     uint32_t abi_version;
@@ -140,7 +175,8 @@ typedef struct pt_program {
     int32_t  width, height;   /* film size */
     int32_t  frames_in_flight;/* samples per pixel traced per wave of paths (>= 1) */
     int32_t  max_null_crossings; /* measurement.maxNullCrossings (32 default) */
-    int32_t  pad[2];
+    int32_t  wavelengths;     /* PT_WL_* (spectral only) */
+    int32_t  pad;
 } pt_program;
 
 /* Live per-frame values (the engine builtins + camera uniforms,

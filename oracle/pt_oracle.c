@@ -273,6 +273,9 @@ typedef struct {
     int emissive_solid;       /* some closed/solid region is emissive */
     int ambient_material;     /* material of region -1 (ambientMedium), -1 = none */
     int has_media, has_null;
+    float spec_curves[95 * 4];/* CIE X, Y, Z, D65 at 360 + 5 i nm (spectral programs) */
+    float xyz_rgb[9], cie_y;
+    int has_spectral;
 } or_scene;
 
 /* Möller–Trumbore (intersection/mesh/mesh.glsl:41-59). */
@@ -633,11 +636,100 @@ static inline float ior_of(const or_scene* S, int region) {
     return m->model == PT_MAT_DIELECTRIC ? m->ior : 1.0f;
 }
 
+static inline v3 lv(const float* a) { return V(a[0], a[1], a[2]); }
+
+/* ---------------- spectra (components/color/spectrum.glsl, components/color/spectral) ----
+ * Four lanes. An RGB program uses lanes 0-2 and reduces over 3; a spectral program carries
+ * the path's four hero wavelengths in wl_t and reduces over 4. */
+typedef struct { float x, y, z, w; } sp;
+typedef struct { int spec; int uniform; float lam[4]; } wl_t;
+static inline sp SP(float x, float y, float z, float w) { sp r; r.x = x; r.y = y; r.z = z; r.w = w; return r; }
+static inline sp sgrey(float v) { return SP(v, v, v, v); }
+static inline sp sadd(sp a, sp b) { return SP(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w); }
+static inline sp smul(sp a, sp b) { return SP(a.x * b.x, a.y * b.y, a.z * b.z, a.w * b.w); }
+static inline sp smuls(sp a, float s) { return SP(a.x * s, a.y * s, a.z * s, a.w * s); }
+static inline sp sdivs(sp a, float s) { return SP(a.x / s, a.y / s, a.z / s, a.w / s); }
+static inline sp sneg(sp a) { return SP(-a.x, -a.y, -a.z, -a.w); }
+static inline sp srgb(const float* p) { return SP(p[0], p[1], p[2], 0.0f); }
+static inline sp srgbv(v3 v) { return SP(v.x, v.y, v.z, 0.0f); }
+static inline float s_lane(sp s, int c) { return c == 0 ? s.x : (c == 1 ? s.y : (c == 2 ? s.z : s.w)); }
+static inline float s_avg(const wl_t* wl, sp s) {   /* spectrum_average */
+    float sum = s.x; sum += s.y; sum += s.z;
+    if (wl->spec) { sum += s.w; return sum * 0.25f; }
+    return sum * (1.0f / 3.0f);
+}
+static inline float s_max(const wl_t* wl, sp s) {   /* spectrum_max */
+    float m = wl->spec ? s.w : s.z;
+    if (wl->spec) m = fmaxf_(s.z, m);
+    m = fmaxf_(s.y, m); m = fmaxf_(s.x, m);
+    return m;
+}
+static inline int s_black(const wl_t* wl, sp s) { return s.x <= 0.0f && s.y <= 0.0f && s.z <= 0.0f && (!wl->spec || s.w <= 0.0f); }
+static inline sp s_exp(sp s) { return SP(EXP(s.x), EXP(s.y), EXP(s.z), EXP(s.w)); }
+static inline float glsl_mix(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+/* spectral_curves (lookup.glsl:2-7) */
+static inline void spectral_curves(const or_scene* S, float lam, float out[4]) {
+    float x = (lam - 360.0f) / 5.0f;
+    int i = (int)x; if (i > 93) i = 93;
+    float f = x - (float)i;
+    for (int c = 0; c < 4; c++) out[c] = glsl_mix(S->spec_curves[4 * i + c], S->spec_curves[4 * (i + 1) + c], f);
+}
+/* spectral_bounded (lookup.glsl:40-48) for one lane */
+static inline float spectral_bounded_lane(const pt_spectrum* r, float lam) {
+    float t = (lam - 360.0f) / 470.0f;
+    float p = (r->c[0] * t + r->c[1]) * t + r->c[2];
+    return clampf(0.5f + 0.5f * p / sqrtf(1.0f + p * p), 0.0f, 1.0f);
+}
+/* blackbody_emission (color/spectra/blackbody/blackbody.glsl:4-10) */
+static inline float blackbody_lane(float kelvin, float normalization, float lam) {
+    float q = 14387768.775039338f / kelvin;
+    float ratio = 550.0f / lam;
+    float ratio2 = ratio * ratio;
+    return normalization * (EXP(q / 550.0f) - 1.0f) * ratio2 * ratio2 * ratio / (EXP(q / lam) - 1.0f);
+}
+/* color_lift_{reflectance,coefficient,emission} (lookup.glsl:50-60) or scale * blackbody */
+static sp spectrum_eval(const or_scene* S, const pt_spectrum* r, const wl_t* wl) {
+    float v[4];
+    for (int i = 0; i < 4; i++) {
+        float lam = wl->lam[i], x;
+        if (r->kind == PT_SPEC_BLACKBODY) x = r->scale * blackbody_lane(r->c[0], r->c[1], lam);
+        else {
+            x = r->kind == PT_SPEC_ZERO ? 0.0f : (r->kind == PT_SPEC_GREY ? r->value : spectral_bounded_lane(r, lam));
+            if (r->scaled) x = r->scale * x;
+            if (r->d65) { float c[4]; spectral_curves(S, lam, c); x = x * c[3]; }
+        }
+        v[i] = x;
+    }
+    return SP(v[0], v[1], v[2], v[3]);
+}
+static inline sp lift(const or_scene* S, const float* rgb, const pt_spectrum* r, const wl_t* wl) {
+    return wl->spec ? spectrum_eval(S, r, wl) : srgb(rgb);
+}
+/* wavelength_from_cdf / wavelength_pdf (color/spectral/visible.glsl:3-13, uniform.glsl:1-2) */
+static inline float wl_visible_a(void) { return ptm_tanh(0.0072f * (360.0f - 538.0f)); }
+static inline float wl_visible_b(void) { return ptm_tanh(0.0072f * (830.0f - 538.0f)); }
+static inline float wavelength_from_cdf(int uniform, float u) {
+    if (uniform) return 360.0f + 470.0f * u;
+    return 538.0f + ptm_atanh(glsl_mix(wl_visible_a(), wl_visible_b(), u)) / 0.0072f;
+}
+static inline float wavelength_pdf(int uniform, float lam) {
+    if (uniform) return 1.0f / 470.0f;
+    float c = ptm_cosh(0.0072f * (lam - 538.0f));
+    return 0.0072f / ((wl_visible_b() - wl_visible_a()) * c * c);
+}
+/* sensor_response (sensor/cie1931/cie1931.glsl:3-5) */
+static inline v3 sensor_response(const or_scene* S, float lam) {
+    float c[4]; spectral_curves(S, lam, c);
+    const float* m = S->xyz_rgb;
+    v3 r = V((m[0] * c[0] + m[1] * c[1]) + m[2] * c[2], (m[3] * c[0] + m[4] * c[1]) + m[5] * c[2],
+             (m[6] * c[0] + m[7] * c[1]) + m[8] * c[2]);
+    return divs(r, S->cie_y);
+}
+
 /* ---------------- materials ---------------- */
 enum { LOBE_REFLECTION = 1, LOBE_TRANSMISSION = 2, LOBE_DELTA = 4 };
-typedef struct { v3 wi; v3 weight; float pdf; int flags; } isample_t;
+typedef struct { v3 wi; sp weight; float pdf; int flags; } isample_t;
 
-static inline v3 lv(const float* a) { return V(a[0], a[1], a[2]); }
 static inline v3 to_local(frame_t f, v3 v) { return V(dot(v, f.t), dot(v, f.b), dot(v, f.n)); }
 static inline v3 from_local(frame_t f, v3 v) { return normalize(add(add(muls(f.t, v.x), muls(f.b, v.y)), muls(f.n, v.z))); }
 /* microfacet.glsl:29-54 */
@@ -657,9 +749,9 @@ static v3 mf_vndf(v3 wol, float a, float ux, float uy) {
     v3 nh = add(add(muls(T1, t1), muls(T2, t2)), muls(vh, sqrtf(fmaxf_(0.0f, 1.0f - t1 * t1 - t2 * t2))));
     return normalize(V(a * nh.x, a * nh.y, fmaxf_(1e-6f, nh.z)));
 }
-static inline v3 schlick(v3 f0, float c) {
+static inline sp schlick(sp f0, float c) {
     float p = POW(1.0f - c, 5.0f);
-    return V(f0.x + (1.0f - f0.x) * p, f0.y + (1.0f - f0.y) * p, f0.z + (1.0f - f0.z) * p);
+    return SP(f0.x + (1.0f - f0.x) * p, f0.y + (1.0f - f0.y) * p, f0.z + (1.0f - f0.z) * p, f0.w + (1.0f - f0.w) * p);
 }
 /* math.glsl:95-102 */
 static inline float dielectric_fresnel(float cos_i, float eta) {
@@ -671,7 +763,7 @@ static inline float dielectric_fresnel(float cos_i, float eta) {
     return 0.5f * (r_par * r_par + r_perp * r_perp);
 }
 
-static isample_t surface_sample(const or_scene* S, const pt_material* m, v3 wo, const hit_t* hit, float uc, float ux, float uy) {
+static isample_t surface_sample(const or_scene* S, const pt_material* m, v3 wo, const hit_t* hit, float uc, float ux, float uy, const wl_t* wl) {
     isample_t s;
     frame_t f = hit->frame;
     if (m->model == PT_MAT_LAMBERT) {               /* materials/lambert/lambert.glsl:17-28 */
@@ -680,7 +772,7 @@ static isample_t surface_sample(const or_scene* S, const pt_material* m, v3 wo, 
         float sin_theta = sqrtf(fmaxf_(0.0f, 1.0f - uy));
         float phi = PTM_TWO_PI * ux;
         s.wi = normalize(add(add(muls(f.t, sin_theta * COS(phi)), muls(f.b, sin_theta * SIN(phi))), muls(n, cos_theta)));
-        s.weight = lv(m->albedo);
+        s.weight = lift(S, m->albedo, &m->s_albedo, wl);
         s.pdf = cos_theta * INV_PI;
         s.flags = LOBE_REFLECTION;
     } else if (m->model == PT_MAT_DIELECTRIC) {     /* materials/dielectric/dielectric.glsl:14-52 */
@@ -692,7 +784,7 @@ static isample_t surface_sample(const or_scene* S, const pt_material* m, v3 wo, 
         float F = dielectric_fresnel(cos_i, eta);
         if (uc < F) {
             s.wi = normalize(sub(muls(n, 2.0f * cos_i), wo));
-            s.weight = V(1.0f, 1.0f, 1.0f);
+            s.weight = sgrey(1.0f);
             s.pdf = 0.0f;
             s.flags = LOBE_REFLECTION | LOBE_DELTA;
         } else {
@@ -700,7 +792,7 @@ static isample_t surface_sample(const or_scene* S, const pt_material* m, v3 wo, 
             float cos_t = sqrtf(fmaxf_(0.0f, 1.0f - sin2_t));
             s.wi = normalize(add(muls(wo, -eta), muls(n, eta * cos_i - cos_t)));
             float radiance_scale = (n_i * n_i) / (n_t * n_t);
-            s.weight = muls(lv(m->transmittance), radiance_scale);
+            s.weight = smuls(lift(S, m->transmittance, &m->s_transmittance, wl), radiance_scale);
             s.pdf = 0.0f;
             s.flags = LOBE_TRANSMISSION | LOBE_DELTA;
         }
@@ -713,31 +805,31 @@ static isample_t surface_sample(const or_scene* S, const pt_material* m, v3 wo, 
         v3 I = neg(wol);
         v3 wil = sub(I, muls(h, 2.0f * dot(h, I)));
         if (wil.z <= 0.0f) {
-            s.wi = wo; s.weight = V(0.0f, 0.0f, 0.0f); s.pdf = 0.0f; s.flags = LOBE_REFLECTION;
+            s.wi = wo; s.weight = sgrey(0.0f); s.pdf = 0.0f; s.flags = LOBE_REFLECTION;
             return s;
         }
-        v3 F = schlick(lv(m->f0), fabsf(dot(wol, h)));
-        s.weight = muls(F, mf_G1(wil, a));
+        sp F = schlick(lift(S, m->f0, &m->s_f0, wl), fabsf(dot(wol, h)));
+        s.weight = smuls(F, mf_G1(wil, a));
         s.pdf = mf_G1(wol, a) * mf_D(h, a) / (4.0f * fabsf(wol.z));
         s.flags = LOBE_REFLECTION;
         s.wi = from_local(f, muls(wil, side));
     }
     return s;
 }
-static v3 surface_eval(const pt_material* m, v3 wi, v3 wo, const hit_t* hit) {
+static sp surface_eval(const or_scene* S, const pt_material* m, v3 wi, v3 wo, const hit_t* hit, const wl_t* wl) {
     frame_t f = hit->frame;
     if (m->model == PT_MAT_LAMBERT) {
-        if (dot(wi, f.n) * dot(wo, f.n) <= 0.0f) return V(0.0f, 0.0f, 0.0f);
-        return muls(lv(m->albedo), INV_PI);
+        if (dot(wi, f.n) * dot(wo, f.n) <= 0.0f) return sgrey(0.0f);
+        return smuls(lift(S, m->albedo, &m->s_albedo, wl), INV_PI);
     } else if (m->model == PT_MAT_GGX) {
         v3 wil = to_local(f, wi), wol = to_local(f, wo);
-        if (wil.z * wol.z <= 0.0f) return V(0.0f, 0.0f, 0.0f);
+        if (wil.z * wol.z <= 0.0f) return sgrey(0.0f);
         float a = m->alpha;
         v3 h = normalize(add(wil, wol));
-        v3 F = schlick(lv(m->f0), fabsf(dot(wol, h)));
-        return muls(F, mf_D(h, a) * mf_G1(wil, a) * mf_G1(wol, a) / (4.0f * fabsf(wil.z) * fabsf(wol.z)));
+        sp F = schlick(lift(S, m->f0, &m->s_f0, wl), fabsf(dot(wol, h)));
+        return smuls(F, mf_D(h, a) * mf_G1(wil, a) * mf_G1(wol, a) / (4.0f * fabsf(wil.z) * fabsf(wol.z)));
     }
-    return V(0.0f, 0.0f, 0.0f);
+    return sgrey(0.0f);
 }
 static float surface_pdf(const pt_material* m, v3 wi, v3 wo, const hit_t* hit) {
     frame_t f = hit->frame;
@@ -764,12 +856,12 @@ static v3 sky_radiance_raw(v3 dir) {
     float k = ptm_smoothstep(0.9994f, 0.9999f, dot(dir, sun));
     return add(base, muls(V(18.0f, 16.0f, 13.0f), k));
 }
-static v3 env_radiance(const or_scene* S, v3 dir) {
-    if (S->env.type != PT_ENV_SKY) return V(0.0f, 0.0f, 0.0f);
+static sp env_radiance(const or_scene* S, v3 dir) {
+    if (S->env.type != PT_ENV_SKY) return sgrey(0.0f);
     v3 d = normalize(dir);
     float cx = S->env_rot_cs[0], sy = S->env_rot_cs[1];
     d = V(cx * d.x - sy * d.z, d.y, sy * d.x + cx * d.z);
-    return muls(sky_radiance_raw(d), S->env.intensity);
+    return srgbv(muls(sky_radiance_raw(d), S->env.intensity));
 }
 /* equirect chart (env/equirect/equirect_map.glsl, equirect.glsl:19-48) */
 static void equirect_uv(const or_scene* S, v3 dir, float* u, float* v) {
@@ -898,9 +990,9 @@ static void env_build(or_scene* S) {
 }
 
 /* ---------------- lights (lights/quad/quad.glsl:14-41, features/lighting.ts:546-673) ---- */
-typedef struct { v3 wi; float distance; v3 radiance; float pdf; int flags; int light_id; } lsample_t;
+typedef struct { v3 wi; float distance; sp radiance; float pdf; int flags; int light_id; } lsample_t;
 
-static lsample_t light_sample_one(const pt_light* l, v3 p, float x0, float x1) {
+static lsample_t light_sample_one(const or_scene* S, const pt_light* l, v3 p, float x0, float x1, const wl_t* wl) {
     lsample_t ls;
     if (l->kind == PT_LIGHT_QUAD) {
         v3 q = add(add(lv(l->corner), muls(lv(l->edge1), x0)), muls(lv(l->edge2), x1));
@@ -908,7 +1000,7 @@ static lsample_t light_sample_one(const pt_light* l, v3 p, float x0, float x1) {
         float d2 = dot(d, d);
         ls.wi = muls(d, 1.0f / sqrtf(d2));
         ls.distance = sqrtf(d2);
-        ls.radiance = lv(l->radiance);
+        ls.radiance = lift(S, l->radiance, &l->s_radiance, wl);
         ls.flags = 0; ls.light_id = -1;
         float cos_l = dot(lv(l->normal), neg(ls.wi));
         if (cos_l <= 0.0f) { ls.pdf = 0.0f; return ls; }
@@ -918,7 +1010,7 @@ static lsample_t light_sample_one(const pt_light* l, v3 p, float x0, float x1) {
         float d2 = dot(d, d);
         ls.wi = muls(d, 1.0f / sqrtf(d2));
         ls.distance = sqrtf(d2);
-        ls.radiance = divs(lv(l->radiance), d2);
+        ls.radiance = sdivs(lift(S, l->radiance, &l->s_radiance, wl), d2);
         ls.pdf = 1.0f; ls.flags = 1; ls.light_id = -1;
     }
     return ls;
@@ -930,17 +1022,17 @@ static float light_pdf_one(const pt_light* l, v3 p, v3 light_p, v3 wi) {
     v3 d = sub(light_p, p);
     return dot(d, d) / (l->area * cos_l);
 }
-static lsample_t lighting_sample_finite(const or_scene* S, v3 p, float x0, float x1) {
+static lsample_t lighting_sample_finite(const or_scene* S, v3 p, float x0, float x1, const wl_t* wl) {
     lsample_t ls;
     int n = S->n_lights;
-    if (n == 1) { ls = light_sample_one(&S->lights[0], p, x0, x1); ls.light_id = 0; return ls; }
+    if (n == 1) { ls = light_sample_one(S, &S->lights[0], p, x0, x1, wl); ls.light_id = 0; return ls; }
     int id = n - 1; float sel = S->lights[n - 1].select_pdf;
     float lo = n >= 2 ? S->lights[n - 2].cdf : 0.0f;
     for (int i = 0; i < n - 1; i++) {
         if (x0 < S->lights[i].cdf) { id = i; sel = S->lights[i].select_pdf; lo = i == 0 ? 0.0f : S->lights[i - 1].cdf; break; }
     }
     float xr = (x0 - lo) / sel;
-    ls = light_sample_one(&S->lights[id], p, clampf(xr, 0.0f, 0.9999999f), x1);
+    ls = light_sample_one(S, &S->lights[id], p, clampf(xr, 0.0f, 0.9999999f), x1, wl);
     ls.light_id = id;
     ls.pdf *= sel;
     return ls;
@@ -964,13 +1056,13 @@ static lsample_t environment_sample(const or_scene* S, float x0, float x1) {
     ls.flags = 0; ls.light_id = -1;
     return ls;
 }
-static lsample_t lighting_sample(const or_scene* S, v3 p, float x0, float x1) {
+static lsample_t lighting_sample(const or_scene* S, v3 p, float x0, float x1, const wl_t* wl) {
     lsample_t ls;
     if (S->n_lights == 0) {
         if (env_samplable(S)) return environment_sample(S, x0, x1);
         ls.pdf = 0.0f; return ls;
     }
-    if (!env_select_live(S)) return lighting_sample_finite(S, p, x0, x1);
+    if (!env_select_live(S)) return lighting_sample_finite(S, p, x0, x1, wl);
     float ps = S->env.select_prob;
     if (x0 < ps) {
         ls = environment_sample(S, clampf(x0 / ps, 0.0f, 0.9999999f), x1);
@@ -978,7 +1070,7 @@ static lsample_t lighting_sample(const or_scene* S, v3 p, float x0, float x1) {
         return ls;
     }
     float fx = clampf((x0 - ps) / (1.0f - ps), 0.0f, 0.9999999f);
-    ls = lighting_sample_finite(S, p, fx, x1);
+    ls = lighting_sample_finite(S, p, fx, x1, wl);
     ls.pdf *= (1.0f - ps);
     return ls;
 }
@@ -995,32 +1087,32 @@ static float lighting_pdf(const or_scene* S, v3 qp, v3 wi, int lid, v3 light_p) 
 }
 
 /* ---------------- homogeneous media (transport/volume/analytic/analytic.glsl:9-51) ---- */
-static inline float spectrum_average(v3 s) { float sum = s.x; sum += s.y; sum += s.z; return sum * (1.0f / 3.0f); }
-static inline v3 spectrum_exp(v3 s) { return V(EXP(s.x), EXP(s.y), EXP(s.z)); }
-static inline float comp(v3 s, int c) { return c == 0 ? s.x : (c == 1 ? s.y : s.z); }
-typedef struct { int scattered; float t; v3 weight; } msample_t;
-static msample_t medium_sample_analytic(const pt_material* m, float t_max, float x0, float x1) {
+static inline sp sigma_s_of(const or_scene* S, const pt_material* m, const wl_t* wl) { return lift(S, m->sigma_s, &m->s_sigma_s, wl); }
+static inline sp sigma_a_of(const or_scene* S, const pt_material* m, const wl_t* wl) { return lift(S, m->sigma_a, &m->s_sigma_a, wl); }
+typedef struct { int scattered; float t; sp weight; } msample_t;
+static msample_t medium_sample_analytic(const or_scene* S, const pt_material* m, float t_max, float x0, float x1, const wl_t* wl) {
     msample_t ms;
-    v3 sa = lv(m->sigma_a), ss = lv(m->sigma_s);
-    v3 st = add(sa, ss);
-    int c = (int)(x0 * 3.0f); if (c > 2) c = 2;                  /* spectrum_sample_channel */
-    float sc = comp(st, c);
+    sp sa = sigma_a_of(S, m, wl), ss = sigma_s_of(S, m, wl);
+    sp st = sadd(sa, ss);
+    const int N = wl->spec ? 4 : 3;
+    int c = (int)(x0 * (float)N); if (c > N - 1) c = N - 1;     /* spectrum_sample_channel */
+    float sc = s_lane(st, c);
     float t = sc == 0.0f ? t_max : -LOG(1.0f - x1) / sc;
     if (t < t_max) {
-        v3 tr = spectrum_exp(muls(neg(st), t));
-        float pdf = spectrum_average(mul(st, tr));
+        sp tr = s_exp(smuls(sneg(st), t));
+        float pdf = s_avg(wl, smul(st, tr));
         ms.scattered = 1; ms.t = t;
-        ms.weight = divs(mul(ss, tr), fmaxf_(pdf, 1e-20f));
+        ms.weight = sdivs(smul(ss, tr), fmaxf_(pdf, 1e-20f));
     } else {
-        v3 tr = spectrum_exp(muls(neg(st), t_max));
-        float pdf = spectrum_average(tr);
+        sp tr = s_exp(smuls(sneg(st), t_max));
+        float pdf = s_avg(wl, tr);
         ms.scattered = 0; ms.t = t_max;
-        ms.weight = divs(tr, fmaxf_(pdf, 1e-20f));
+        ms.weight = sdivs(tr, fmaxf_(pdf, 1e-20f));
     }
     return ms;
 }
-static inline v3 medium_transmittance(const pt_material* m, float len) {
-    return spectrum_exp(muls(neg(add(lv(m->sigma_a), lv(m->sigma_s))), len));
+static inline sp medium_transmittance(const or_scene* S, const pt_material* m, float len, const wl_t* wl) {
+    return s_exp(smuls(sneg(sadd(sigma_a_of(S, m, wl), sigma_s_of(S, m, wl))), len));
 }
 /* HG phase (volume_scattering/hg/hg.glsl:11-42), forward convention. */
 static inline float hg_value(v3 wi, v3 wo, float g) {
@@ -1028,13 +1120,13 @@ static inline float hg_value(v3 wi, v3 wo, float g) {
     float d = 1.0f + g * g - 2.0f * g * c;
     return (1.0f - g * g) / (4.0f * PTM_PI * d * sqrtf(fmaxf_(d, 1e-8f)));
 }
-static inline float hg_pdf(v3 wi, v3 wo, float g) { float h = hg_value(wi, wo, g); return spectrum_average(V(h, h, h)); }
+static inline float hg_pdf(v3 wi, v3 wo, float g, const wl_t* wl) { float h = hg_value(wi, wo, g); return s_avg(wl, sgrey(h)); }
 static inline void build_basis(v3 n, v3* t, v3* b) {   /* glsl/core/math.glsl:63-67 */
     v3 up = fabsf(n.y) < 0.999f ? V(0.0f, 1.0f, 0.0f) : V(1.0f, 0.0f, 0.0f);
     *t = normalize(cross(up, n));
     *b = cross(n, *t);
 }
-static v3 hg_sample(v3 wo, float g, float x0, float x1, float* pdf) {
+static v3 hg_sample(v3 wo, float g, float x0, float x1, float* pdf, const wl_t* wl) {
     float cos_theta;
     if (fabsf(g) < 1e-3f) cos_theta = 1.0f - 2.0f * x0;
     else {
@@ -1047,12 +1139,12 @@ static v3 hg_sample(v3 wo, float g, float x0, float x1, float* pdf) {
     v3 nw = neg(wo);
     build_basis(nw, &t, &b);
     v3 wi = normalize(add(add(muls(t, sin_theta * COS(phi)), muls(b, sin_theta * SIN(phi))), muls(nw, cos_theta)));
-    *pdf = hg_pdf(wi, wo, g);
+    *pdf = hg_pdf(wi, wo, g, wl);
     return wi;
 }
 /* shadow_transmittance, media walker (transport/shadow/media/media.glsl:29-62). */
-static v3 shadow_transmittance_media(const or_scene* S, v3 o, v3 d, v3 light_p, int crossings_left, int* n_any) {
-    v3 T = V(1.0f, 1.0f, 1.0f);
+static sp shadow_transmittance_media(const or_scene* S, v3 o, v3 d, v3 light_p, int crossings_left, int* n_any, const wl_t* wl) {
+    sp T = sgrey(1.0f);
     int medium = region_at(S, o);
     for (int crossed = 0; crossed <= crossings_left; crossed++) {
         float remaining = length3(sub(light_p, o)) - SHADOW_BACKOFF;
@@ -1062,21 +1154,22 @@ static v3 shadow_transmittance_media(const or_scene* S, v3 o, v3 d, v3 light_p, 
         int hit_boundary = scene_intersect(S, o, d, &h) && h.t < remaining;
         float seg_len = hit_boundary ? h.t : remaining;
         int med_mat = material_of(S, medium);
-        if (material_has_medium(S, med_mat)) T = mul(T, medium_transmittance(&S->materials[med_mat], seg_len));
+        if (material_has_medium(S, med_mat)) T = smul(T, medium_transmittance(S, &S->materials[med_mat], seg_len, wl));
         if (!hit_boundary) return T;
-        if (!is_null_interface(S, material_of(S, h.region_owner))) return V(0.0f, 0.0f, 0.0f);
+        if (!is_null_interface(S, material_of(S, h.region_owner))) return sgrey(0.0f);
         medium = h.region_to;
         float side = dot(d, h.ng) >= 0.0f ? 1.0f : -1.0f;
         o = add(h.p, muls(muls(h.ng, side), h.eps));
         d = normalize(sub(light_p, o));
     }
-    return V(0.0f, 0.0f, 0.0f);
+    return sgrey(0.0f);
 }
 
 /* ---------------- the walk (integrators/pt/pt.ts:299-464; combiner.ts; kernel.glsl) ---- */
 typedef struct {
     int estimator, max_bounces, rr_start; float rr_max;
     int width, height, max_null;
+    int spectral, wl_uniform;
 } or_program;
 
 static inline float power_heuristic(float pf, float pg) {
@@ -1084,8 +1177,6 @@ static inline float power_heuristic(float pf, float pg) {
     if (f2 >= 1.0e38f) return 1.0f;
     return f2 / fmaxf_(1e-20f, f2 + pg * pg);
 }
-static inline int is_black(v3 s) { return s.x <= 0.0f && s.y <= 0.0f && s.z <= 0.0f; }
-static inline float spectrum_max(v3 s) { float m = s.z; m = fmaxf_(s.y, m); m = fmaxf_(s.x, m); return m; }
 static inline v3 ray_spawn_origin(const hit_t* hit, v3 wi) {
     float side = dot(wi, hit->ng) >= 0.0f ? 1.0f : -1.0f;
     return add(hit->p, muls(muls(hit->ng, side), hit->eps));
@@ -1093,8 +1184,9 @@ static inline v3 ray_spawn_origin(const hit_t* hit, v3 wi) {
 
 /* One camera path; returns radiance. The media branch, null interfaces and the media
  * shadow walker are the program's (f.media, f.nulls) parts of pt.ts:299-464. */
-static v3 transport_trace(const or_scene* S, const or_program* P, v3 ro, v3 rd, rng_t* rng, int* n_closest, int* n_any) {
-    v3 T = V(1.0f, 1.0f, 1.0f), L = V(0.0f, 0.0f, 0.0f);
+static sp transport_trace(const or_scene* S, const or_program* P, v3 ro, v3 rd, rng_t* rng, int* n_closest, int* n_any,
+                          const wl_t* wl) {
+    sp T = sgrey(1.0f), L = sgrey(0.0f);
     int prev_was_delta = 1; float prev_bsdf_pdf = 0.0f; v3 prev_qp = ro;
     float eta_scale = 1.0f;
     int nee = P->estimator != PT_EST_PT, mis = P->estimator == PT_EST_MIS;
@@ -1111,8 +1203,8 @@ static v3 transport_trace(const or_scene* S, const or_program* P, v3 ro, v3 rd, 
             if (material_has_medium(S, med_mat)) {
                 const pt_material* Mm = &S->materials[med_mat];
                 float x0 = rnd(rng), x1 = rnd(rng);
-                msample_t ms = medium_sample_analytic(Mm, boundary ? hit.t : MAX_DIST, x0, x1);
-                T = mul(T, ms.weight);
+                msample_t ms = medium_sample_analytic(S, Mm, boundary ? hit.t : MAX_DIST, x0, x1, wl);
+                T = smul(T, ms.weight);
                 if (ms.scattered) {
                     if (bounce == P->max_bounces) break;
                     v3 p_evt = add(ro, muls(rd, ms.t));
@@ -1120,36 +1212,37 @@ static v3 transport_trace(const or_scene* S, const or_program* P, v3 ro, v3 rd, 
                     float g = Mm->hg_g;
                     if (nee) {   /* light_sample_direct_medium (techniques/light/light_medium.glsl:7-18) */
                         float y0 = rnd(rng), y1 = rnd(rng);
-                        lsample_t ls = lighting_sample(S, p_evt, y0, y1);
+                        lsample_t ls = lighting_sample(S, p_evt, y0, y1, wl);
                         if (ls.pdf > 0.0f) {
                             v3 light_p = add(p_evt, muls(ls.wi, ls.distance));
-                            v3 vis = shadow_transmittance_media(S, p_evt, ls.wi, light_p,
-                                                                S->has_null ? P->max_null - null_crossings : 0, n_any);
-                            if (!is_black(vis)) {
+                            sp vis = shadow_transmittance_media(S, p_evt, ls.wi, light_p,
+                                                                S->has_null ? P->max_null - null_crossings : 0, n_any, wl);
+                            if (!s_black(wl, vis)) {
                                 float f = hg_value(ls.wi, wo_med, g);
                                 float w = 1.0f;
-                                if (mis && !(ls.flags & 1)) w = power_heuristic(ls.pdf, hg_pdf(ls.wi, wo_med, g));
-                                v3 c = mul(mul(T, ls.radiance), V(f, f, f));
-                                c = mul(c, vis);
-                                c = muls(c, w);
-                                L = add(L, divs(c, ls.pdf));
+                                if (mis && !(ls.flags & 1)) w = power_heuristic(ls.pdf, hg_pdf(ls.wi, wo_med, g, wl));
+                                sp c = smul(smul(T, ls.radiance), sgrey(f));
+                                c = smul(c, vis);
+                                c = smuls(c, w);
+                                L = sadd(L, sdivs(c, ls.pdf));
                             }
                         }
                     }
                     /* kernel_sample_phase (techniques/kernel/kernel_phase.glsl:7-15) */
                     float z0 = rnd(rng), z1 = rnd(rng);
                     float ppdf;
-                    v3 wi = hg_sample(wo_med, g, z0, z1, &ppdf);
+                    v3 wi = hg_sample(wo_med, g, z0, z1, &ppdf, wl);
                     prev_was_delta = 0; prev_bsdf_pdf = ppdf; prev_qp = p_evt;
                     ro = p_evt; rd = wi;
                     /* roulette_interior with medium_survival = sigma_s / sigma_t (pt.ts:230-283) */
                     if (P->rr_start >= 0 && bounce >= P->rr_start) {
-                        v3 ss = lv(Mm->sigma_s), st = add(lv(Mm->sigma_s), lv(Mm->sigma_a));
-                        v3 surv = V(ss.x / fmaxf_(st.x, 1e-9f), ss.y / fmaxf_(st.y, 1e-9f), ss.z / fmaxf_(st.z, 1e-9f));
-                        float p = spectrum_max(surv);
+                        sp ss = sigma_s_of(S, Mm, wl), st = sadd(ss, sigma_a_of(S, Mm, wl));
+                        sp surv = SP(ss.x / fmaxf_(st.x, 1e-9f), ss.y / fmaxf_(st.y, 1e-9f), ss.z / fmaxf_(st.z, 1e-9f),
+                                     ss.w / fmaxf_(st.w, 1e-9f));
+                        float p = s_max(wl, surv);
                         if (p <= 0.0f) break;
                         if (rnd(rng) > p) break;
-                        T = divs(T, p);
+                        T = sdivs(T, p);
                     }
                     continue;
                 }
@@ -1163,8 +1256,8 @@ static v3 transport_trace(const or_scene* S, const or_program* P, v3 ro, v3 rd, 
                 else if (mis) w = power_heuristic(prev_bsdf_pdf, env_select_live(S) ? S->env.select_prob * environment_pdf(S, rd) : environment_pdf(S, rd));
                 else w = 0.0f;
             }
-            v3 Le = env_radiance(S, rd);
-            L = add(L, mul(muls(T, w), Le));
+            sp Le = env_radiance(S, rd);
+            L = sadd(L, smul(smuls(T, w), Le));
             break;
         }
         if (media && hit.region_from != current_medium) current_medium = hit.region_from;   /* self-heal */
@@ -1190,43 +1283,44 @@ static v3 transport_trace(const or_scene* S, const or_program* P, v3 ro, v3 rd, 
                 else if (mis) w = power_heuristic(prev_bsdf_pdf, lighting_pdf(S, prev_qp, rd, lid, hit.p));
                 else w = 0.0f;
             }
-            L = add(L, mul(muls(T, w), lv(S->materials[mat_emit].emission)));
+            const pt_material* E = &S->materials[mat_emit];
+            L = sadd(L, smul(smuls(T, w), lift(S, E->emission, &E->s_emission, wl)));
         }
         if (bounce == P->max_bounces) break;
         /* light_sample_direct (techniques/light/light.glsl:8-28) */
         if (nee && has_nondelta(M)) {
             float x0 = rnd(rng), x1 = rnd(rng);
-            lsample_t ls = lighting_sample(S, hit.p, x0, x1);
+            lsample_t ls = lighting_sample(S, hit.p, x0, x1, wl);
             if (ls.pdf > 0.0f) {
                 v3 light_p = add(hit.p, muls(ls.wi, ls.distance));
                 v3 so = ray_spawn_origin(&hit, ls.wi);
                 v3 sd = normalize(sub(light_p, so));
-                v3 vis;
-                if (media) vis = shadow_transmittance_media(S, so, sd, light_p, S->has_null ? P->max_null - null_crossings : 0, n_any);
+                sp vis;
+                if (media) vis = shadow_transmittance_media(S, so, sd, light_p, S->has_null ? P->max_null - null_crossings : 0, n_any, wl);
                 else {
                     float maxDist = length3(sub(light_p, so)) - SHADOW_BACKOFF;
                     (*n_any)++;
-                    vis = scene_any(S, so, sd, maxDist) ? V(0.0f, 0.0f, 0.0f) : V(1.0f, 1.0f, 1.0f);
+                    vis = scene_any(S, so, sd, maxDist) ? sgrey(0.0f) : sgrey(1.0f);
                 }
-                if (!is_black(vis)) {
-                    v3 f = surface_eval(M, ls.wi, wo, &hit);
+                if (!s_black(wl, vis)) {
+                    sp f = surface_eval(S, M, ls.wi, wo, &hit, wl);
                     float cos_i = fabsf(dot(ls.wi, hit.frame.n));
                     float w = 1.0f;
                     if (mis && !(ls.flags & 1)) w = power_heuristic(ls.pdf, surface_pdf(M, ls.wi, wo, &hit));
-                    v3 c = mul(mul(T, ls.radiance), f);
-                    c = muls(c, cos_i);
-                    c = mul(c, vis);
-                    c = muls(c, w);
-                    c = divs(c, ls.pdf);
-                    L = add(L, c);
+                    sp c = smul(smul(T, ls.radiance), f);
+                    c = smuls(c, cos_i);
+                    c = smul(c, vis);
+                    c = smuls(c, w);
+                    c = sdivs(c, ls.pdf);
+                    L = sadd(L, c);
                 }
             }
         }
         /* kernel_sample_continuation */
         float uc = rnd(rng); float ux = rnd(rng); float uy = rnd(rng);
-        isample_t bs = surface_sample(S, M, wo, &hit, uc, ux, uy);
-        if (is_black(bs.weight)) break;
-        T = mul(T, bs.weight);
+        isample_t bs = surface_sample(S, M, wo, &hit, uc, ux, uy, wl);
+        if (s_black(wl, bs.weight)) break;
+        T = smul(T, bs.weight);
         prev_was_delta = (bs.flags & LOBE_DELTA) != 0; prev_bsdf_pdf = bs.pdf; prev_qp = hit.p;
         if (media && (bs.flags & LOBE_TRANSMISSION)) current_medium = hit.region_to;
         if (S->has_transmission && (bs.flags & LOBE_TRANSMISSION)) {
@@ -1235,11 +1329,11 @@ static v3 transport_trace(const or_scene* S, const or_program* P, v3 ro, v3 rd, 
         }
         /* roulette (pt.ts:204-222) */
         if (P->rr_start >= 0 && bounce >= P->rr_start) {
-            float m = S->has_transmission ? spectrum_max(T) * eta_scale : spectrum_max(T);
+            float m = S->has_transmission ? s_max(wl, T) * eta_scale : s_max(wl, T);
             float p = fminf_(P->rr_max, m);
             if (p <= 0.0f) break;
             if (rnd(rng) > p) break;
-            T = divs(T, p);
+            T = sdivs(T, p);
         }
         ro = ray_spawn_origin(&hit, bs.wi);
         rd = bs.wi;
@@ -1266,6 +1360,12 @@ or_scene* or_scene_create(const pt_scene_desc* d) {
     memcpy(S->lights, d->lights, sizeof(pt_light) * (size_t)d->n_lights);
     S->env = d->env;
     S->ambient_material = d->ambient_material;
+    if (d->spectral_curves) {
+        memcpy(S->spec_curves, d->spectral_curves, sizeof S->spec_curves);
+        memcpy(S->xyz_rgb, d->xyz_to_rgb, sizeof S->xyz_rgb);
+        S->cie_y = d->cie_y_integral;
+        S->has_spectral = 1;
+    }
     for (int m = 0; m < S->n_materials; m++) {
         if (S->materials[m].has_medium) S->has_media = 1;
         if (S->materials[m].model == PT_MAT_NULL) S->has_null = 1;
@@ -1383,6 +1483,8 @@ void or_render(const or_scene* S, const pt_program* prog, const pt_params* par, 
     P.estimator = prog->estimator; P.max_bounces = prog->max_bounces; P.rr_start = prog->rr_start;
     P.rr_max = prog->rr_max_survival; P.width = prog->width; P.height = prog->height;
     P.max_null = prog->max_null_crossings;
+    P.spectral = prog->color == PT_COLOR_SPECTRAL; P.wl_uniform = prog->wavelengths == PT_WL_UNIFORM;
+    if (P.spectral && !S->has_spectral) { if (counts) { counts[0] = counts[1] = 0; } return; }
     int W = P.width, H = P.height;
     uint64_t nc = 0, na = 0;
     v3 F = lv(par->cam_forward), R = lv(par->cam_right), U = lv(par->cam_up), O = lv(par->cam_position);
@@ -1399,11 +1501,24 @@ void or_render(const or_scene* S, const pt_program* prog, const pt_params* par, 
                 float j0 = rnd(&rng), j1 = rnd(&rng);
                 float filmx = fx + (j0 - 0.5f), filmy = fy + (j1 - 0.5f);
                 (void)rnd(&rng); (void)rnd(&rng);   /* lens sample, consumed by every camera */
+                wl_t wl; wl.spec = P.spectral; wl.uniform = P.wl_uniform;
+                wl.lam[0] = wl.lam[1] = wl.lam[2] = wl.lam[3] = 0.0f;
+                if (P.spectral) {                   /* wavelengths_sample (color/spectral/spectral.glsl:1-10) */
+                    float u = rnd(&rng);
+                    for (int k = 0; k < 4; k++) wl.lam[k] = wavelength_from_cdf(P.wl_uniform, ptm_fract(u + 0.25f * (float)k));
+                }
                 float ndcx = (2.0f * filmx / (float)W) - 1.0f;
                 float ndcy = (2.0f * filmy / (float)H) - 1.0f;
                 ndcx *= par->aspect;
                 v3 dir = normalize(add(add(F, muls(R, ndcx * par->tan_fov)), muls(U, ndcy * par->tan_fov)));
-                v3 Lr = transport_trace(S, &P, O, dir, &rng, &c1, &c2);
+                sp Ls = transport_trace(S, &P, O, dir, &rng, &c1, &c2, &wl);
+                v3 Lr = V(Ls.x, Ls.y, Ls.z);
+                if (P.spectral) {                   /* color_to_pixel (color/spectral/spectral.glsl:27-33) */
+                    v3 pix = V(0.0f, 0.0f, 0.0f);
+                    for (int k = 0; k < 4; k++)
+                        pix = add(pix, muls(sensor_response(S, wl.lam[k]), s_lane(Ls, k) / wavelength_pdf(P.wl_uniform, wl.lam[k])));
+                    Lr = muls(pix, 0.25f);
+                }
                 if (samples) {
                     float* sp = samples + 3 * (((size_t)f * H + y) * W + x);
                     sp[0] = Lr.x; sp[1] = Lr.y; sp[2] = Lr.z;
@@ -1470,8 +1585,9 @@ void or_bsdf_probe(const pt_material* m, const float* n, const float* wo, float 
     h.frame = ambient_frame(V(0, 0, 0), V(n[0], n[1], n[2]));
     h.ng = h.frame.n; h.region_from = -1; h.region_to = 0;
     v3 w = V(wo[0], wo[1], wo[2]);
-    isample_t s = surface_sample(&S, m, w, &h, uc, ux, uy);
-    v3 f = surface_eval(m, s.wi, w, &h);
+    wl_t wl; memset(&wl, 0, sizeof wl);
+    isample_t s = surface_sample(&S, m, w, &h, uc, ux, uy, &wl);
+    sp f = surface_eval(&S, m, s.wi, w, &h, &wl);
     float p = surface_pdf(m, s.wi, w, &h);
     float r[12] = {s.wi.x, s.wi.y, s.wi.z, s.weight.x, s.weight.y, s.weight.z, s.pdf, (float)s.flags, f.x, f.y, f.z, p};
     memcpy(out, r, sizeof r);

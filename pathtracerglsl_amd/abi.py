@@ -7,13 +7,21 @@ import ctypes as C
 
 import numpy as np
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 OBJ_MESH, OBJ_QUAD, OBJ_SPHERE = 0, 1, 2
 MAT_LAMBERT, MAT_DIELECTRIC, MAT_GGX, MAT_NULL = 0, 1, 2, 3
 LIGHT_QUAD, LIGHT_POINT = 0, 1
 ENV_NONE, ENV_SKY = 0, 1
 EST_PT, EST_NEE, EST_MIS = 0, 1, 2
+COLOR_RGB, COLOR_SPECTRAL = 0, 1
+WL_VISIBLE, WL_UNIFORM = 0, 1
+SPEC_ZERO, SPEC_GREY, SPEC_SIGMOID, SPEC_BLACKBODY = 0, 1, 2, 3
+
+SPECTRUM_DTYPE = np.dtype([
+    ("kind", "<i4"), ("scaled", "<i2"), ("d65", "<i2"), ("value", "<f4"), ("scale", "<f4"),
+    ("c", "<f4", 3), ("pad", "<f4"),
+])
 
 OBJECT_DTYPE = np.dtype([
     ("kind", "<i4"), ("material", "<i4"), ("closed", "<i4"), ("first_tri", "<i4"),
@@ -25,13 +33,16 @@ MATERIAL_DTYPE = np.dtype([
     ("albedo", "<f4", 4), ("emission", "<f4", 4), ("f0", "<f4", 4), ("transmittance", "<f4", 4),
     ("alpha", "<f4"), ("ior", "<f4"), ("hg_g", "<f4"), ("pad1", "<f4"),
     ("sigma_s", "<f4", 4), ("sigma_a", "<f4", 4),
+    ("s_albedo", SPECTRUM_DTYPE), ("s_emission", SPECTRUM_DTYPE), ("s_f0", SPECTRUM_DTYPE),
+    ("s_transmittance", SPECTRUM_DTYPE), ("s_sigma_s", SPECTRUM_DTYPE), ("s_sigma_a", SPECTRUM_DTYPE),
 ])
 LIGHT_DTYPE = np.dtype([
     ("kind", "<i4"), ("region", "<i4"), ("select_pdf", "<f4"), ("cdf", "<f4"),
     ("corner", "<f4", 4), ("edge1", "<f4", 4), ("edge2", "<f4", 4), ("normal", "<f4", 4),
-    ("radiance", "<f4", 4), ("area", "<f4"), ("pad", "<f4", 3),
+    ("radiance", "<f4", 4), ("area", "<f4"), ("pad", "<f4", 3), ("s_radiance", SPECTRUM_DTYPE),
 ])
-assert OBJECT_DTYPE.itemsize == 112 and MATERIAL_DTYPE.itemsize == 128 and LIGHT_DTYPE.itemsize == 112
+assert SPECTRUM_DTYPE.itemsize == 32
+assert OBJECT_DTYPE.itemsize == 112 and MATERIAL_DTYPE.itemsize == 320 and LIGHT_DTYPE.itemsize == 144
 
 
 class PtEnv(C.Structure):
@@ -46,14 +57,16 @@ class PtSceneDesc(C.Structure):
                 ("n_objects", C.c_int32), ("objects", C.c_void_p),
                 ("n_materials", C.c_int32), ("materials", C.c_void_p),
                 ("n_lights", C.c_int32), ("lights", C.c_void_p),
-                ("env", PtEnv), ("ambient_material", C.c_int32), ("pad0", C.c_int32)]
+                ("env", PtEnv), ("ambient_material", C.c_int32), ("pad0", C.c_int32),
+                ("spectral_curves", C.c_void_p), ("xyz_to_rgb", C.c_float * 9),
+                ("cie_y_integral", C.c_float), ("pad1", C.c_int32 * 2)]
 
 
 class PtProgram(C.Structure):
     _fields_ = [("abi_version", C.c_uint32), ("estimator", C.c_int32), ("max_bounces", C.c_int32),
                 ("rr_start", C.c_int32), ("rr_max_survival", C.c_float), ("color", C.c_int32),
                 ("width", C.c_int32), ("height", C.c_int32), ("frames_in_flight", C.c_int32),
-                ("max_null_crossings", C.c_int32), ("pad", C.c_int32 * 2)]
+                ("max_null_crossings", C.c_int32), ("wavelengths", C.c_int32), ("pad", C.c_int32)]
 
 
 class PtParams(C.Structure):
@@ -79,7 +92,8 @@ def _ptr(a):
 class SceneArrays:
     """Host-side scene in the ABI layout; keeps the numpy arrays alive for the desc."""
 
-    def __init__(self, tri_vertices, tri_object, objects, materials, lights, env=None, ambient_material=-1):
+    def __init__(self, tri_vertices, tri_object, objects, materials, lights, env=None, ambient_material=-1,
+                 spectral_curves=None, xyz_to_rgb=None, cie_y_integral=0.0):
         self.tri_vertices = np.ascontiguousarray(tri_vertices, dtype=np.float32).reshape(-1, 9)
         self.tri_object = np.ascontiguousarray(tri_object, dtype=np.int32).reshape(-1)
         self.objects = np.ascontiguousarray(objects, dtype=OBJECT_DTYPE)
@@ -87,6 +101,10 @@ class SceneArrays:
         self.lights = np.ascontiguousarray(lights, dtype=LIGHT_DTYPE)
         self.env = env if env is not None else PtEnv(ENV_NONE, 512, 256, 0, 1.0, 0.0, 0.5, 0.0)
         self.ambient_material = int(ambient_material)
+        self.spectral_curves = (None if spectral_curves is None
+                                else np.ascontiguousarray(spectral_curves, dtype=np.float32).reshape(95, 4))
+        self.xyz_to_rgb = np.zeros(9, np.float32) if xyz_to_rgb is None else np.asarray(xyz_to_rgb, np.float32).reshape(9)
+        self.cie_y_integral = float(cie_y_integral)
         assert self.tri_object.shape[0] == self.tri_vertices.shape[0]
 
     @property
@@ -107,6 +125,10 @@ class SceneArrays:
         d.lights = _ptr(self.lights)
         d.env = self.env
         d.ambient_material = self.ambient_material
+        d.spectral_curves = _ptr(self.spectral_curves)
+        for i in range(9):
+            d.xyz_to_rgb[i] = float(self.xyz_to_rgb[i])
+        d.cie_y_integral = self.cie_y_integral
         return d
 
 

@@ -13,7 +13,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import abi
+from . import abi, spectral
 
 ESTIMATORS = {"none": abi.EST_PT, "pt": abi.EST_PT, "nee": abi.EST_NEE, "mis": abi.EST_MIS}
 MODELS = {"lambert": abi.MAT_LAMBERT, "dielectric": abi.MAT_DIELECTRIC, "ggx": abi.MAT_GGX, "none": abi.MAT_NULL}
@@ -32,6 +32,8 @@ class RenderStrategy:
     rr_max_survival: float = 0.95
     frames_in_flight: int = 1
     max_null_crossings: int = 32          # measurement.maxNullCrossings (plan/measurement.ts:12)
+    color: str = "rgb"                    # measurement.color: 'rgb' | 'spectral'
+    wavelengths: str = "visible"          # estimator.wavelengths (spectral): 'visible' | 'uniform'
 
 
 @dataclass
@@ -83,6 +85,12 @@ def _analyze_validate(scene, strategy):
         raise CompileError("maxBounces must be >= 0")
     if not (0.0 < strategy.rr_max_survival <= 1.0):
         raise CompileError("russianRoulette.maxSurvival must be in (0, 1]")
+    if strategy.color not in ("rgb", "spectral"):
+        raise CompileError(f"unknown color model '{strategy.color}'")
+    if strategy.wavelengths not in ("visible", "uniform"):
+        raise CompileError(f"unknown wavelength sampler '{strategy.wavelengths}'")
+    if strategy.color == "spectral" and scene.get("environment", {"type": "none"}).get("type", "none") != "none":
+        raise CompileError("spectral programs with an environment light are not supported by this backend")
     mats = scene.get("materials", {})
     for name, m in mats.items():
         if m.get("model") not in MODELS:
@@ -109,6 +117,21 @@ def compile_scene(scene, strategy=None, width=256, height=256, reset_salt=1234):
     strategy = strategy or RenderStrategy()
     _analyze_validate(scene, strategy)
 
+    spec = strategy.color == "spectral"
+
+    def srec(value, role):
+        if not spec:
+            return np.zeros(1, abi.SPECTRUM_DTYPE)[0]
+        try:
+            return spectral.spectrum_record(value, role)
+        except spectral.SpectralDataError as e:
+            raise CompileError(str(e)) from None
+
+    def emits(value):
+        if isinstance(value, dict) and "blackbody" in value:
+            return float(value["blackbody"].get("scale", 1.0)) > 0.0
+        return any(e > 0 for e in spectral.rgb_of(value))
+
     mat_names = list(scene.get("materials", {}).keys())
     mat_index = {n: i for i, n in enumerate(mat_names)}
     materials = []
@@ -116,14 +139,21 @@ def compile_scene(scene, strategy=None, width=256, height=256, reset_salt=1234):
         m = scene["materials"][n]
         rec = np.zeros(1, abi.MATERIAL_DTYPE)[0]
         rec["model"] = MODELS[m["model"]]
-        rec["albedo"][:3] = m.get("albedo", [0.8, 0.8, 0.8])
+        albedo = m.get("albedo", [0.8, 0.8, 0.8])
+        rec["albedo"][:3] = spectral.rgb_of(albedo)
+        rec["s_albedo"] = srec(albedo, spectral.ROLE_REFLECTANCE)
         em = m.get("emission", [0.0, 0.0, 0.0])
-        rec["emission"][:3] = em
-        rec["emissive"] = int(any(e > 0 for e in em))
-        rec["f0"][:3] = m.get("f0", [0.9, 0.9, 0.9])
+        rec["emission"][:3] = spectral.rgb_of(em)
+        rec["s_emission"] = srec(em, spectral.ROLE_EMISSION)
+        rec["emissive"] = int(emits(em))
+        f0 = m.get("f0", [0.9, 0.9, 0.9])
+        rec["f0"][:3] = spectral.rgb_of(f0)
+        rec["s_f0"] = srec(f0, spectral.ROLE_REFLECTANCE)
         rough = float(m.get("roughness", 0.5))
         rec["alpha"] = max(1e-3, rough * rough)                  # ggx.ts derived alpha
-        rec["transmittance"][:3] = m.get("transmittance", [1.0, 1.0, 1.0])
+        tr = m.get("transmittance", [1.0, 1.0, 1.0])
+        rec["transmittance"][:3] = spectral.rgb_of(tr)
+        rec["s_transmittance"] = srec(tr, spectral.ROLE_REFLECTANCE)
         rec["ior"] = float(m.get("ior", 1.5))
         med = m.get("medium")
         if med is not None:
@@ -132,6 +162,8 @@ def compile_scene(scene, strategy=None, width=256, height=256, reset_salt=1234):
             rec["has_medium"] = 1
             rec["sigma_a"][:3] = rgb(med.get("sigma_a", 0.0))
             rec["sigma_s"][:3] = rgb(med.get("sigma_s", 0.0))
+            rec["s_sigma_a"] = srec(rgb(med.get("sigma_a", 0.0)), spectral.ROLE_COEFFICIENT)
+            rec["s_sigma_s"] = srec(rgb(med.get("sigma_s", 0.0)), spectral.ROLE_COEFFICIENT)
             rec["hg_g"] = float(med.get("phase_g", 0.0))
         materials.append(rec)
 
@@ -175,8 +207,10 @@ def compile_scene(scene, strategy=None, width=256, height=256, reset_salt=1234):
     lights = []
     for l in scene.get("lights", []):
         rec = np.zeros(1, abi.LIGHT_DTYPE)[0]
-        rad = np.asarray(l.get("radiance", l.get("intensity", [1, 1, 1])), np.float64)
+        rad_value = l.get("radiance", l.get("intensity", [1, 1, 1]))
+        rad = np.asarray(spectral.rgb_of(rad_value), np.float64)
         rec["radiance"][:3] = rad
+        rec["s_radiance"] = srec(rad_value, spectral.ROLE_EMISSION)
         if l["type"] == "quad":
             e1, e2 = np.asarray(l["edge1"], np.float64), np.asarray(l["edge2"], np.float64)
             cr = np.cross(e1, e2)
@@ -185,6 +219,7 @@ def compile_scene(scene, strategy=None, width=256, height=256, reset_salt=1234):
             em = np.zeros(1, abi.MATERIAL_DTYPE)[0]
             em["model"] = abi.MAT_LAMBERT
             em["emission"][:3] = rad
+            em["s_emission"] = srec(rad_value, spectral.ROLE_EMISSION)
             em["emissive"] = 1
             materials.append(em)
             region = add_object(abi.OBJ_QUAD, len(materials) - 1, 0,
@@ -227,7 +262,8 @@ def compile_scene(scene, strategy=None, width=256, height=256, reset_salt=1234):
     arrays = abi.SceneArrays(tv, to, np.array(objects, abi.OBJECT_DTYPE),
                              np.array(materials, abi.MATERIAL_DTYPE),
                              np.array(light_recs, abi.LIGHT_DTYPE), env,
-                             mat_index[amb] if amb is not None else -1)
+                             mat_index[amb] if amb is not None else -1,
+                             spectral.curve_rows(), spectral.XYZ_TO_RGB.reshape(9), spectral.CIE_Y_INTEGRAL)
 
     prog = abi.PtProgram()
     prog.abi_version = abi.ABI_VERSION
@@ -235,10 +271,11 @@ def compile_scene(scene, strategy=None, width=256, height=256, reset_salt=1234):
     prog.max_bounces = strategy.max_bounces
     prog.rr_start = strategy.rr_start
     prog.rr_max_survival = strategy.rr_max_survival
-    prog.color = 0
+    prog.color = abi.COLOR_SPECTRAL if spec else abi.COLOR_RGB
+    prog.wavelengths = abi.WL_UNIFORM if strategy.wavelengths == "uniform" else abi.WL_VISIBLE
     prog.width, prog.height = width, height
     prog.frames_in_flight = max(1, strategy.frames_in_flight)
     prog.max_null_crossings = strategy.max_null_crossings
-    return CompiledRenderer(id=f"{strategy.direct_lighting}-{scene.get('id', 'scene')}", scene=arrays,
+    return CompiledRenderer(id=f"{strategy.color}-{strategy.direct_lighting}-{scene.get('id', 'scene')}", scene=arrays,
                             program=prog, width=width, height=height, camera=scene["camera"],
                             reset_salt=reset_salt)

@@ -38,8 +38,35 @@ PT_DEV float gsign(float x) { return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : 0.0f)
 PT_DEV f3 ld3(const float* p) { return mk3(p[0], p[1], p[2]); }
 PT_DEV f3 xyz(float4 v) { return mk3(v.x, v.y, v.z); }
 PT_DEV float4 f4(f3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
-PT_DEV bool is_black(f3 s) { return s.x <= 0.0f && s.y <= 0.0f && s.z <= 0.0f; }
-PT_DEV float spectrum_max(f3 s) { float m = s.z; m = gmax(s.y, m); m = gmax(s.x, m); return m; }
+
+// Spectrum: four lanes. RGB programs use lanes 0-2 (lane 3 is never stored or reduced, so
+// it costs nothing after dead-code elimination); spectral programs use all four hero
+// wavelengths. Reductions take the lane count from SPEC (components/color/spectrum.glsl:5-24).
+struct sp4 { float x, y, z, w; };
+PT_DEV sp4 mk4(float x, float y, float z, float w) { sp4 r; r.x = x; r.y = y; r.z = z; r.w = w; return r; }
+PT_DEV sp4 sgrey(float v) { return mk4(v, v, v, v); }
+PT_DEV sp4 operator+(sp4 a, sp4 b) { return mk4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w); }
+PT_DEV sp4 operator*(sp4 a, sp4 b) { return mk4(a.x * b.x, a.y * b.y, a.z * b.z, a.w * b.w); }
+PT_DEV sp4 operator*(sp4 a, float s) { return mk4(a.x * s, a.y * s, a.z * s, a.w * s); }
+PT_DEV sp4 operator/(sp4 a, float s) { return mk4(a.x / s, a.y / s, a.z / s, a.w / s); }
+PT_DEV sp4 operator-(sp4 a) { return mk4(-a.x, -a.y, -a.z, -a.w); }
+PT_DEV sp4 ld4(float4 v) { return mk4(v.x, v.y, v.z, v.w); }
+PT_DEV float4 st4(sp4 v) { return make_float4(v.x, v.y, v.z, v.w); }
+PT_DEV sp4 rgb4(const float* p) { return mk4(p[0], p[1], p[2], 0.0f); }
+PT_DEV sp4 rgb4(f3 v) { return mk4(v.x, v.y, v.z, 0.0f); }
+template <bool SPEC> PT_DEV bool is_black(sp4 s) { return s.x <= 0.0f && s.y <= 0.0f && s.z <= 0.0f && (!SPEC || s.w <= 0.0f); }
+template <bool SPEC> PT_DEV float spectrum_max(sp4 s) {
+    float m = SPEC ? s.w : s.z;
+    if (SPEC) m = gmax(s.z, m);
+    m = gmax(s.y, m); m = gmax(s.x, m);
+    return m;
+}
+template <bool SPEC> PT_DEV float spectrum_average(sp4 s) {
+    float sum = s.x; sum += s.y; sum += s.z;
+    if (SPEC) { sum += s.w; return sum * 0.25f; }
+    return sum * (1.0f / 3.0f);
+}
+PT_DEV float spectrum_comp(sp4 s, int c) { return c == 0 ? s.x : (c == 1 ? s.y : (c == 2 ? s.z : s.w)); }
 
 // pcg4d (components/sampler/pcg4d/pcg4d.glsl:12-40); only .x is consumed.
 PT_DEV uint32_t pcg4d_x(uint32_t x, uint32_t y, uint32_t z, uint32_t w) {
@@ -94,6 +121,10 @@ struct DevScene {
     int emissive_solid;
     int ambient_material;       // material of region -1 (ambientMedium), -1 = none
     int has_media, has_null;
+    // spectral color model: CIE X, Y, Z, D65 at 360 + 5 i nm (95 rows) and the cie1931 sensor
+    const float4* spec_curves;
+    float xyz_rgb[9];
+    float cie_y;
 };
 
 // Wave-aggregated queue append: every active lane calls it (convergent), lanes with

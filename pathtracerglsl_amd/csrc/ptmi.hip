@@ -48,8 +48,8 @@ enum { C_RAYQ0 = 0, C_RAYQ1 = 1, C_SHADOW = 2, C_CAT0 = 3, C_EXT_FETCH = 9, C_SH
 struct Paths {
     float4* ray_o;   // origin.xyz
     float4* ray_d;   // direction.xyz
-    float4* thr;     // throughput.xyz, eta_scale
-    float4* rad;     // radiance.xyz
+    float4* thr;     // throughput.xyz, eta_scale (RGB) | throughput, 4 lanes (spectral)
+    float4* rad;     // radiance.xyz (RGB) | 4 lanes (spectral)
     uint2* rng;      // rng_base, rng_dim
     float4* prev;    // prev_query.p.xyz, prev_bsdf_pdf
     int* flags;      // bit 0 prev_was_delta, bits 8..15 null crossings, bits 16..31 bounce
@@ -62,7 +62,9 @@ struct Paths {
     float4* s_o;     // shadow origin.xyz, maxDist (opaque) | crossings left (media, int bits)
     float4* s_d;     // shadow direction.xyz, light pdf
     float4* s_c;     // partial contribution.xyz (before visibility), MIS weight
-    float4* s_l;     // light point.xyz (media walker)
+    float4* s_l;     // light point.xyz (media walker), MIS weight (spectral)
+    float4* wl;      // hero wavelengths (spectral)
+    float* eta;      // eta_scale (spectral; RGB keeps it in thr.w)
     unsigned long long* hist;   // [0] closest queries, [1] any-hit queries
     unsigned long long* trav;   // [0] nodes visited, [1] triangles tested (COUNT builds)
 };
@@ -70,6 +72,26 @@ struct Paths {
 PT_DEV int fl_bounce(int f) { return (f >> 16) & 0xFFFF; }
 PT_DEV int fl_nulls(int f) { return (f >> 8) & 0xFF; }
 PT_DEV int mk_flags(bool delta, int nulls, int bounce) { return (delta ? 1 : 0) | (nulls << 8) | (bounce << 16); }
+
+template <bool SPEC> PT_DEV void load_thr(const Paths& P, int i, sp4& T, float& eta) {
+    float4 t = P.thr[i];
+    if (SPEC) { T = ld4(t); eta = P.eta[i]; } else { T = mk4(t.x, t.y, t.z, 0.0f); eta = t.w; }
+}
+template <bool SPEC> PT_DEV void store_thr(const Paths& P, int i, sp4 T, float eta) {
+    if (SPEC) { P.thr[i] = st4(T); P.eta[i] = eta; } else P.thr[i] = make_float4(T.x, T.y, T.z, eta);
+}
+template <bool SPEC> PT_DEV sp4 load_rad(const Paths& P, int i) {
+    float4 v = P.rad[i];
+    return SPEC ? ld4(v) : mk4(v.x, v.y, v.z, 0.0f);
+}
+template <bool SPEC> PT_DEV void store_rad(const Paths& P, int i, sp4 L) {
+    P.rad[i] = SPEC ? st4(L) : make_float4(L.x, L.y, L.z, 0.0f);
+}
+template <bool SPEC> PT_DEV Wl load_wl(const Paths& P, int i) {
+    Wl w;
+    w.lam = SPEC ? P.wl[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    return w;
+}
 
 struct Camera {
     float ox, oy, oz, fx, fy, fz, rx, ry, rz, ux, uy, uz, tan_fov, aspect;
@@ -84,7 +106,8 @@ __global__ void __launch_bounds__(64) k_camera_medium(DevScene S, Paths P, Camer
     if (threadIdx.x == 0) P.counts[C_CAM_MEDIUM] = region_at<64>(S, mk3(C.ox, C.oy, C.oz), stk);
 }
 
-__global__ void __launch_bounds__(SBLOCK) k_raygen(Paths P, Camera C, int W, int H, int n_paths, uint32_t sample0, int media) {
+template <bool SPEC>
+__global__ void __launch_bounds__(SBLOCK) k_raygen(Paths P, Camera C, int W, int H, int n_paths, uint32_t sample0, int media, int wl_uniform) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) P.counts[C_RAYQ0] = n_paths;
     if (i >= n_paths) return;
@@ -99,6 +122,13 @@ __global__ void __launch_bounds__(SBLOCK) k_raygen(Paths P, Camera C, int W, int
     float j0 = r.next(), j1 = r.next();
     float filmx = fx + (j0 - 0.5f), filmy = fy + (j1 - 0.5f);   // pixel/box/box.glsl:5-7
     r.dim += 2;                                                   // lens sample (unused by a pinhole)
+    if (SPEC) {                                                   // wavelengths_sample (spectral.glsl:1-10)
+        float u = r.next();
+        float l[4];
+        for (int k = 0; k < 4; k++) l[k] = wavelength_from_cdf(wl_uniform != 0, ptm_fract(u + 0.25f * (float)k));
+        P.wl[i] = make_float4(l[0], l[1], l[2], l[3]);
+        P.eta[i] = 1.0f;
+    }
     // camera/pinhole/pinhole.glsl:4-18
     float ndcx = (2.0f * filmx / (float)W) - 1.0f;
     float ndcy = (2.0f * filmy / (float)H) - 1.0f;
@@ -198,7 +228,7 @@ __global__ void __launch_bounds__(TBLOCK) k_extend(DevScene S, Paths P, int qsel
 
 // shadow (opaque scenes): any hit for every NEE ray; unoccluded contributions are added to
 // their path as ((a * vis) * w) / pdf, the reference's operation order with vis = 1.
-template <int STACK>
+template <int STACK, bool SPEC>
 __global__ void __launch_bounds__(TBLOCK) k_shadow(DevScene S, Paths P) {
     __shared__ uint2 stk[STACK * TBLOCK];
     const int n = P.counts[C_SHADOW];
@@ -213,9 +243,10 @@ __global__ void __launch_bounds__(TBLOCK) k_shadow(DevScene S, Paths P) {
         if (!blocked) blocked = trace_wide<true, STACK, TBLOCK, false>(S, o, d, maxd, prim, stk, cnt);
         if (!blocked) {
             int i = P.s_path[k];
-            float4 c = P.s_c[k], L = P.rad[i];
-            f3 v = ((xyz(c) * c.w) / d4.w);
-            P.rad[i] = make_float4(L.x + v.x, L.y + v.y, L.z + v.z, 0.0f);
+            float4 c = P.s_c[k];
+            const float w = SPEC ? P.s_l[k].w : c.w;
+            sp4 a = SPEC ? ld4(c) : mk4(c.x, c.y, c.z, 0.0f);
+            store_rad<SPEC>(P, i, load_rad<SPEC>(P, i) + (a * w) / d4.w);
         }
     }
 }
@@ -223,7 +254,7 @@ __global__ void __launch_bounds__(TBLOCK) k_shadow(DevScene S, Paths P) {
 // shadow (media scenes): the segment walker of transport/shadow/media/media.glsl:29-62 —
 // closest hits up to the light point, Beer-Lambert per segment in the current medium, pass
 // through null interfaces, anything else blocks (dielectrics are opaque to shadow rays).
-template <int STACK>
+template <int STACK, bool SPEC>
 __global__ void __launch_bounds__(TBLOCK) k_shadow_media(DevScene S, Paths P) {
     __shared__ uint2 stk[STACK * TBLOCK];
     const int n = P.counts[C_SHADOW];
@@ -234,7 +265,8 @@ __global__ void __launch_bounds__(TBLOCK) k_shadow_media(DevScene S, Paths P) {
         float4 o4 = P.s_o[k], d4 = P.s_d[k], l4 = P.s_l[k];
         f3 o = xyz(o4), d = xyz(d4), light_p = xyz(l4);
         const int crossings_left = __float_as_int(o4.w);
-        f3 T = mk3(1.0f, 1.0f, 1.0f);
+        const Wl wl = load_wl<SPEC>(P, P.s_path[k]);
+        sp4 T = sgrey(1.0f);
         bool blocked = true;
         int medium = region_at<TBLOCK>(S, o, stk);
         for (int crossed = 0; crossed <= crossings_left; crossed++) {
@@ -246,7 +278,7 @@ __global__ void __launch_bounds__(TBLOCK) k_shadow_media(DevScene S, Paths P) {
             bool hit_boundary = prim >= 0 && tmax < remaining;
             float seg_len = hit_boundary ? tmax : remaining;
             int med_mat = material_of(S, medium);
-            if (material_has_medium(S, med_mat)) T = T * medium_transmittance(S.materials[med_mat], seg_len);
+            if (material_has_medium(S, med_mat)) T = T * medium_transmittance<SPEC>(S, S.materials[med_mat], seg_len, wl);
             if (!hit_boundary) { blocked = false; break; }
             Hit h; bool back;
             fill_hit(S, o, d, tmax, prim, h, back);
@@ -257,11 +289,12 @@ __global__ void __launch_bounds__(TBLOCK) k_shadow_media(DevScene S, Paths P) {
             o = h.p + (h.ng * side) * h.eps;
             d = normalize(light_p - o);
         }
-        if (!blocked && !is_black(T)) {
+        if (!blocked && !is_black<SPEC>(T)) {
             int i = P.s_path[k];
-            float4 c = P.s_c[k], L = P.rad[i];
-            f3 v = (((xyz(c) * T) * c.w) / d4.w);
-            P.rad[i] = make_float4(L.x + v.x, L.y + v.y, L.z + v.z, 0.0f);
+            float4 c = P.s_c[k];
+            const float w = SPEC ? l4.w : c.w;
+            sp4 a = SPEC ? ld4(c) : mk4(c.x, c.y, c.z, 0.0f);
+            store_rad<SPEC>(P, i, load_rad<SPEC>(P, i) + ((a * T) * w) / d4.w);
         }
     }
 }
@@ -282,7 +315,7 @@ PT_DEV int block_append(int* const* counters, int which, int* s_cnt, int* s_base
 // bin: per path of the iteration, the medium segment first (media scenes: the analytic
 // channel-MIS distance sample of pt.ts's media branch, which may end the segment in a
 // scattering event), then the shading category of the vertex.
-template <bool MEDIA>
+template <bool MEDIA, bool SPEC>
 __global__ void __launch_bounds__(SBLOCK) k_bin(DevScene S, Paths P, int qsel) {
     __shared__ int s_cnt[N_CAT], s_base[N_CAT];
     int* ctr[N_CAT];
@@ -302,10 +335,11 @@ __global__ void __launch_bounds__(SBLOCK) k_bin(DevScene S, Paths P, int qsel) {
                     uint2 rs = P.rng[path];
                     Rng rng; rng.base = rs.x; rng.dim = rs.y;
                     float x0 = rng.next(), x1 = rng.next();
-                    MSample ms = medium_sample_analytic(S.materials[med_mat], prim >= 0 ? hv.x : MAX_DIST, x0, x1);
-                    float4 th = P.thr[path];
-                    f3 T = xyz(th) * ms.weight;
-                    P.thr[path] = f4(T, th.w);
+                    MSample ms = medium_sample_analytic<SPEC>(S, S.materials[med_mat], prim >= 0 ? hv.x : MAX_DIST, x0, x1,
+                                                              load_wl<SPEC>(P, path));
+                    sp4 T; float eta;
+                    load_thr<SPEC>(P, path, T, eta);
+                    store_thr<SPEC>(P, path, T * ms.weight, eta);
                     P.rng[path] = make_uint2(rng.base, rng.dim);
                     if (ms.scattered) { cat = CAT_MEDIUM; P.hit[path] = make_float2(ms.t, hv.y); }
                 }
@@ -343,7 +377,7 @@ __global__ void k_bookkeep(Paths P, int cur, int next, int first) {
 // (transport/combiner.ts:23-102) and the kernel/light techniques (techniques/kernel/kernel.glsl,
 // kernel_phase.glsl, techniques/light/light.glsl, light_medium.glsl), for the paths of one
 // category. MEDIA selects the media program (region tracking, medium NEE shadow walker).
-template <int EST, int CAT, int STACK, bool MEDIA>
+template <int EST, int CAT, int STACK, bool MEDIA, bool SPEC>
 __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene S, Paths P, int qsel, int max_bounces,
                                                   int rr_start, float rr_max, int max_null) {
     constexpr bool PROBE = CAT == CAT_DIELECTRIC || MEDIA;
@@ -361,14 +395,16 @@ __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene S, Paths P, int qsel,
         const bool valid = k < n;
         const int i = valid ? q[k] : 0;
         bool cont = false, shadow = false;
-        f3 s_o = mk3(0, 0, 0), s_d = mk3(0, 0, 0), s_a = mk3(0, 0, 0), s_l = mk3(0, 0, 0);
+        f3 s_o = mk3(0, 0, 0), s_d = mk3(0, 0, 0), s_l = mk3(0, 0, 0);
+        sp4 s_a = sgrey(0.0f);
         float s_max = 0.0f, s_w = 0.0f, s_pdf = 1.0f;
         int s_cross = 0;
         if (valid) {
             f3 rd = xyz(P.ray_d[i]);
-            float4 th = P.thr[i];
-            f3 T = xyz(th); float eta_scale = th.w;
-            f3 L = xyz(P.rad[i]);
+            sp4 T; float eta_scale;
+            load_thr<SPEC>(P, i, T, eta_scale);
+            sp4 L = load_rad<SPEC>(P, i);
+            const Wl wl = load_wl<SPEC>(P, i);
             float4 pv = P.prev[i];
             f3 prev_qp = xyz(pv); float prev_pdf = pv.w;
             const int fl = P.flags[i];
@@ -399,14 +435,14 @@ __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene S, Paths P, int qsel,
                     Rng rng; rng.base = rs.x; rng.dim = rs.y;
                     if (NEE) {   // light_sample_direct_medium
                         float y0 = rng.next(), y1 = rng.next();
-                        LSample ls = lighting_sample(S, p_evt, y0, y1);
+                        LSample ls = lighting_sample<SPEC>(S, p_evt, y0, y1, wl);
                         if (ls.pdf > 0.0f) {
                             s_l = p_evt + ls.wi * ls.distance;
                             s_o = p_evt; s_d = ls.wi; s_cross = crossings_left;
                             float f = hg_value(ls.wi, wo_med, g);
                             s_w = 1.0f;
-                            if (MIS && !(ls.flags & 1)) s_w = power_heuristic(ls.pdf, hg_pdf(ls.wi, wo_med, g));
-                            s_a = (T * ls.radiance) * mk3(f, f, f);
+                            if (MIS && !(ls.flags & 1)) s_w = power_heuristic(ls.pdf, hg_pdf<SPEC>(ls.wi, wo_med, g));
+                            s_a = (T * ls.radiance) * sgrey(f);
                             s_pdf = ls.pdf;
                             shadow = true;
                         }
@@ -414,13 +450,13 @@ __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene S, Paths P, int qsel,
                     // kernel_sample_phase
                     float z0 = rng.next(), z1 = rng.next();
                     float ppdf;
-                    f3 wi = hg_sample(wo_med, g, z0, z1, ppdf);
+                    f3 wi = hg_sample<SPEC>(wo_med, g, z0, z1, ppdf);
                     cont = true;
                     // roulette_interior with medium_survival = sigma_s / max(sigma_t, 1e-9)
                     if (rr_start >= 0 && bounce >= rr_start) {
-                        f3 ss = ld3(Mm.sigma_s), st = ld3(Mm.sigma_s) + ld3(Mm.sigma_a);
-                        f3 surv = mk3(ss.x / gmax(st.x, 1e-9f), ss.y / gmax(st.y, 1e-9f), ss.z / gmax(st.z, 1e-9f));
-                        float p = spectrum_max(surv);
+                        sp4 ss = sigma_s_of<SPEC>(S, Mm, wl), st = ss + sigma_a_of<SPEC>(S, Mm, wl);
+                        sp4 surv = mk4(ss.x / gmax(st.x, 1e-9f), ss.y / gmax(st.y, 1e-9f), ss.z / gmax(st.z, 1e-9f), ss.w / gmax(st.w, 1e-9f));
+                        float p = spectrum_max<SPEC>(surv);
                         if (p <= 0.0f) cont = false;
                         else if (rng.next() > p) cont = false;
                         else T = T / p;
@@ -428,7 +464,7 @@ __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene S, Paths P, int qsel,
                     if (cont) {
                         P.ray_o[i] = f4(p_evt, 0.0f);
                         P.ray_d[i] = f4(wi, 0.0f);
-                        P.thr[i] = f4(T, eta_scale);
+                        store_thr<SPEC>(P, i, T, eta_scale);
                         P.prev[i] = f4(p_evt, ppdf);
                         P.flags[i] = mk_flags(false, nulls, bounce + 1);
                     }
@@ -472,14 +508,14 @@ __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene S, Paths P, int qsel,
                                 else if (MIS) w = power_heuristic(prev_pdf, lighting_pdf(S, prev_qp, rd, lid, h.p));
                                 else w = 0.0f;
                             }
-                            L = L + (T * w) * ld3(E.emission);
+                            L = L + (T * w) * lift<SPEC>(S, E.emission, E.s_emission, wl);
                         }
                     }
                     if (bounce < max_bounces) {
                         // light_sample_direct (skipped for delta-only materials)
                         if (NEE && CAT != CAT_DIELECTRIC) {
                             float x0 = rng.next(), x1 = rng.next();
-                            LSample ls = lighting_sample(S, h.p, x0, x1);
+                            LSample ls = lighting_sample<SPEC>(S, h.p, x0, x1, wl);
                             if (ls.pdf > 0.0f) {
                                 f3 light_p = h.p + ls.wi * ls.distance;
                                 float side = dot(ls.wi, h.ng) >= 0.0f ? 1.0f : -1.0f;
@@ -487,7 +523,7 @@ __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene S, Paths P, int qsel,
                                 s_d = normalize(light_p - s_o);
                                 s_max = length(light_p - s_o) - SHADOW_BACKOFF;
                                 s_l = light_p; s_cross = crossings_left;
-                                f3 f = surface_eval<MODEL>(M, ls.wi, wo, h);
+                                sp4 f = surface_eval<MODEL, SPEC>(S, M, ls.wi, wo, h, wl);
                                 float cos_i = fabsf(dot(ls.wi, h.frame.n));
                                 s_w = 1.0f;
                                 if (MIS && !(ls.flags & 1)) s_w = power_heuristic(ls.pdf, surface_pdf<MODEL>(M, ls.wi, wo, h));
@@ -498,8 +534,8 @@ __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene S, Paths P, int qsel,
                         }
                         // kernel_sample_continuation
                         float uc = rng.next(), ux = rng.next(), uy = rng.next();
-                        ISample bs = surface_sample<MODEL>(S, M, wo, h, uc, ux, uy);
-                        if (!is_black(bs.weight)) {
+                        ISample bs = surface_sample<MODEL, SPEC>(S, M, wo, h, uc, ux, uy, wl);
+                        if (!is_black<SPEC>(bs.weight)) {
                             T = T * bs.weight;
                             prev_delta = (bs.flags & LOBE_DELTA) != 0;
                             prev_pdf = bs.pdf;
@@ -512,7 +548,7 @@ __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene S, Paths P, int qsel,
                             cont = true;
                             // roulette (pt.ts:204-222)
                             if (rr_start >= 0 && bounce >= rr_start) {
-                                float m = S.has_transmission ? spectrum_max(T) * eta_scale : spectrum_max(T);
+                                float m = S.has_transmission ? spectrum_max<SPEC>(T) * eta_scale : spectrum_max<SPEC>(T);
                                 float p = gmin(rr_max, m);
                                 if (p <= 0.0f) cont = false;
                                 else if (rng.next() > p) cont = false;
@@ -522,7 +558,7 @@ __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene S, Paths P, int qsel,
                                 float side = dot(bs.wi, h.ng) >= 0.0f ? 1.0f : -1.0f;
                                 P.ray_o[i] = f4(h.p + (h.ng * side) * h.eps, 0.0f);
                                 P.ray_d[i] = f4(bs.wi, 0.0f);
-                                P.thr[i] = f4(T, eta_scale);
+                                store_thr<SPEC>(P, i, T, eta_scale);
                                 P.prev[i] = f4(prev_qp, prev_pdf);
                                 P.flags[i] = mk_flags(prev_delta, nulls, bounce + 1);
                                 P.rng[i] = make_uint2(rng.base, rng.dim);
@@ -532,7 +568,7 @@ __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene S, Paths P, int qsel,
                     }
                 }
             }
-            P.rad[i] = f4(L, 0.0f);
+            store_rad<SPEC>(P, i, L);
         }
         int slot = block_append<2>(ctr, cont ? 0 : -1, s_cnt, s_base);
         if (cont) qn[slot] = i;
@@ -541,19 +577,35 @@ __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene S, Paths P, int qsel,
             P.s_path[sslot] = i;
             P.s_o[sslot] = f4(s_o, MEDIA ? __int_as_float(s_cross) : s_max);
             P.s_d[sslot] = f4(s_d, s_pdf);
-            P.s_c[sslot] = f4(s_a, s_w);
-            if (MEDIA) P.s_l[sslot] = f4(s_l, 0.0f);
+            P.s_c[sslot] = SPEC ? st4(s_a) : make_float4(s_a.x, s_a.y, s_a.z, s_w);
+            if (MEDIA || SPEC) P.s_l[sslot] = f4(s_l, s_w);
         }
     }
 }
 
-__global__ void __launch_bounds__(SBLOCK) k_accumulate(const Paths P, float4* film, int npix, int frames, uint32_t sample0) {
+// film: average_update (components/accumulator/average/average.glsl:6-10) of each frame's
+// sample in sample order; spectral samples go through color_to_pixel
+// (color/spectral/spectral.glsl:27-33) with the cie1931 sensor first.
+template <bool SPEC>
+__global__ void __launch_bounds__(SBLOCK) k_accumulate(const DevScene S, const Paths P, float4* film, int npix, int frames,
+                                                       uint32_t sample0, int wl_uniform) {
     int pix = blockIdx.x * blockDim.x + threadIdx.x;
     if (pix >= npix) return;
     float4 acc = film[pix];
     for (int f = 0; f < frames; f++) {
         uint32_t n = sample0 + (uint32_t)f;
-        float4 x = P.rad[(size_t)f * npix + pix];
+        const size_t i = (size_t)f * npix + pix;
+        float4 x = P.rad[i];
+        if (SPEC) {
+            float4 lam = P.wl[i];
+            f3 px = mk3(0.0f, 0.0f, 0.0f);
+            for (int k = 0; k < 4; k++) {
+                float l = lane(lam, k);
+                px = px + sensor_response(S, l) * (lane(x, k) / wavelength_pdf(wl_uniform != 0, l));
+            }
+            px = px * 0.25f;
+            x = make_float4(px.x, px.y, px.z, 0.0f);
+        }
         if (n == 0) acc = make_float4(x.x, x.y, x.z, 1.0f);
         else {
             float w = 1.0f / ((float)n + 1.0f);
@@ -684,7 +736,7 @@ struct pt_renderer {
     uint32_t sample_count = 0;
     long long cap = 0;    // path slots allocated
     DevBuf b_ray_o, b_ray_d, b_thr, b_rad, b_rng, b_prev, b_flags, b_medium, b_hit, b_q0, b_q1, b_counts,
-        b_spath, b_so, b_sd, b_sc, b_sl, b_hist, b_trav, b_cat[N_CAT];
+        b_spath, b_so, b_sd, b_sc, b_sl, b_wl, b_eta, b_hist, b_trav, b_cat[N_CAT];
     Paths paths{};
     bool count_traversal = false;
 };
@@ -804,6 +856,11 @@ pt_status pt_scene_upload(pt_ctx* ctx, const pt_scene_desc* sd) {
     }
     if (sd->ambient_material >= sd->n_materials) return fail(PT_E_INVALID, "pt_scene_upload: ambient_material out of range");
     D.ambient_material = sd->ambient_material < 0 ? -1 : sd->ambient_material;
+    if (sd->spectral_curves) {
+        if ((st = ctx->upload((const float4*)sd->spectral_curves, 95, &D.spec_curves)) != PT_OK) return st;
+        for (int k = 0; k < 9; k++) D.xyz_rgb[k] = sd->xyz_to_rgb[k];
+        D.cie_y = sd->cie_y_integral;
+    }
     for (int o = 0; o < sd->n_objects; o++) {
         const pt_object& ob = sd->objects[o];
         bool solid = ob.kind == PT_OBJ_SPHERE || (ob.kind == PT_OBJ_MESH && ob.closed);
@@ -932,7 +989,10 @@ pt_status pt_renderer_load(pt_ctx* ctx, const pt_program* prog, pt_renderer** ou
     if (!ctx || !prog || !out) return fail(PT_E_INVALID, "pt_renderer_load: null argument");
     if (prog->abi_version != PT_ABI_VERSION) return fail(PT_E_INVALID, "pt_renderer_load: ABI version mismatch");
     if (prog->estimator < PT_EST_PT || prog->estimator > PT_EST_MIS) return fail(PT_E_UNSUPPORTED, "unknown estimator");
-    if (prog->color != PT_COLOR_RGB) return fail(PT_E_UNSUPPORTED, "only RGB color is provided by this build");
+    if (prog->color != PT_COLOR_RGB && prog->color != PT_COLOR_SPECTRAL) return fail(PT_E_UNSUPPORTED, "unknown color model");
+    if (prog->wavelengths != PT_WL_VISIBLE && prog->wavelengths != PT_WL_UNIFORM) return fail(PT_E_UNSUPPORTED, "unknown wavelength sampler");
+    if (prog->color == PT_COLOR_SPECTRAL && ctx->has_scene && ctx->dev.env_type != PT_ENV_NONE)
+        return fail(PT_E_UNSUPPORTED, "spectral programs with an environment light are not provided by this build");
     if (prog->width <= 0 || prog->height <= 0 || prog->max_bounces < 0) return fail(PT_E_INVALID, "bad film size or bounce budget");
     HIP_TRY(hipSetDevice(ctx->device));
     auto* r = new pt_renderer();
@@ -960,7 +1020,7 @@ static pt_status ensure_paths(pt_renderer* r, long long n) {
     struct { DevBuf* b; size_t elem; } list[] = {
         {&r->b_ray_o, 16}, {&r->b_ray_d, 16}, {&r->b_thr, 16}, {&r->b_rad, 16}, {&r->b_rng, 8}, {&r->b_prev, 16},
         {&r->b_flags, 4}, {&r->b_hit, 8}, {&r->b_q0, 4}, {&r->b_q1, 4}, {&r->b_spath, 4}, {&r->b_so, 16},
-        {&r->b_sd, 16}, {&r->b_sc, 16}, {&r->b_sl, 16}, {&r->b_medium, 4}, {&r->b_cat[0], 4}, {&r->b_cat[1], 4},
+        {&r->b_sd, 16}, {&r->b_sc, 16}, {&r->b_sl, 16}, {&r->b_medium, 4}, {&r->b_wl, 16}, {&r->b_eta, 4}, {&r->b_cat[0], 4}, {&r->b_cat[1], 4},
         {&r->b_cat[2], 4}, {&r->b_cat[3], 4}, {&r->b_cat[4], 4}, {&r->b_cat[5], 4}};
     static_assert(N_CAT == 6, "one queue buffer per shading category");
     for (auto& e : list) {
@@ -981,14 +1041,14 @@ static pt_status ensure_paths(pt_renderer* r, long long n) {
     P.flags = (int*)r->b_flags.p; P.hit = (float2*)r->b_hit.p; P.q[0] = (int*)r->b_q0.p; P.q[1] = (int*)r->b_q1.p;
     P.counts = (int*)r->b_counts.p; P.s_path = (int*)r->b_spath.p; P.s_o = (float4*)r->b_so.p;
     P.s_d = (float4*)r->b_sd.p; P.s_c = (float4*)r->b_sc.p; P.s_l = (float4*)r->b_sl.p;
-    P.medium = (int*)r->b_medium.p; P.hist = (unsigned long long*)r->b_hist.p;
+    P.medium = (int*)r->b_medium.p; P.wl = (float4*)r->b_wl.p; P.eta = (float*)r->b_eta.p; P.hist = (unsigned long long*)r->b_hist.p;
     P.trav = (unsigned long long*)r->b_trav.p;
     for (int c = 0; c < N_CAT; c++) P.catq[c] = (int*)r->b_cat[c].p;
     r->cap = n;
     return PT_OK;
 }
 
-template <int EST, int STACK, bool MEDIA>
+template <int EST, int STACK, bool MEDIA, bool SPEC>
 static void launch_shade(pt_renderer* r, int cur) {
     pt_ctx* ctx = r->ctx;
     const pt_program& pg = r->prog;
@@ -996,7 +1056,7 @@ static void launch_shade(pt_renderer* r, int cur) {
     hipStream_t st = ctx->stream;
     const int g = ctx->shade_blocks;
     const int mn = pg.max_null_crossings;
-#define PT_SHADE(name, CATV) TIMED(ctx, name, 0, hipLaunchKernelGGL((k_shade<EST, CATV, STACK, MEDIA>), dim3(g), dim3(SBLOCK), 0, st, S, r->paths, cur, pg.max_bounces, pg.rr_start, pg.rr_max_survival, mn))
+#define PT_SHADE(name, CATV) TIMED(ctx, name, 0, hipLaunchKernelGGL((k_shade<EST, CATV, STACK, MEDIA, SPEC>), dim3(g), dim3(SBLOCK), 0, st, S, r->paths, cur, pg.max_bounces, pg.rr_start, pg.rr_max_survival, mn))
     PT_SHADE("shade_miss", CAT_MISS);
     PT_SHADE("shade_lambert", CAT_LAMBERT);
     if (ctx->has_ggx) PT_SHADE("shade_ggx", CAT_GGX);
@@ -1006,7 +1066,7 @@ static void launch_shade(pt_renderer* r, int cur) {
 #undef PT_SHADE
 }
 
-template <int STACK, bool MEDIA>
+template <int STACK, bool MEDIA, bool SPEC>
 static void launch_wave(pt_renderer* r, const Camera& cam, int frames, uint32_t sample0) {
     pt_ctx* ctx = r->ctx;
     const DevScene& S = ctx->dev;
@@ -1016,7 +1076,7 @@ static void launch_wave(pt_renderer* r, const Camera& cam, int frames, uint32_t 
     const long long n = (long long)W * H * frames;
     hipStream_t st = ctx->stream;
     if (MEDIA) hipLaunchKernelGGL((k_camera_medium<STACK>), dim3(1), dim3(64), 0, st, S, P, cam);
-    TIMED(ctx, "raygen", n, hipLaunchKernelGGL(k_raygen, dim3((unsigned)((n + SBLOCK - 1) / SBLOCK)), dim3(SBLOCK), 0, st, P, cam, W, H, (int)n, sample0, MEDIA ? 1 : 0));
+    TIMED(ctx, "raygen", n, hipLaunchKernelGGL((k_raygen<SPEC>), dim3((unsigned)((n + SBLOCK - 1) / SBLOCK)), dim3(SBLOCK), 0, st, P, cam, W, H, (int)n, sample0, MEDIA ? 1 : 0, pg.wavelengths));
     int cur = 0;
     const int tgrid = ctx->persist_blocks;
     // a null crossing consumes an iteration but no bounce (pt.ts null-interface branch)
@@ -1027,21 +1087,21 @@ static void launch_wave(pt_renderer* r, const Camera& cam, int frames, uint32_t 
             TIMED(ctx, "extend_count", 0, hipLaunchKernelGGL((k_extend<STACK, true>), dim3(tgrid), dim3(TBLOCK), 0, st, S, P, cur));
         else
             TIMED(ctx, "extend", 0, hipLaunchKernelGGL((k_extend<STACK, false>), dim3(tgrid), dim3(TBLOCK), 0, st, S, P, cur));
-        TIMED(ctx, "bin", 0, hipLaunchKernelGGL((k_bin<MEDIA>), dim3(ctx->shade_blocks), dim3(SBLOCK), 0, st, S, P, cur));
+        TIMED(ctx, "bin", 0,  hipLaunchKernelGGL((k_bin<MEDIA, SPEC>), dim3(ctx->shade_blocks), dim3(SBLOCK), 0, st, S, P, cur));
         switch (pg.estimator) {
-            case PT_EST_PT: launch_shade<PT_EST_PT, STACK, MEDIA>(r, cur); break;
-            case PT_EST_NEE: launch_shade<PT_EST_NEE, STACK, MEDIA>(r, cur); break;
-            default: launch_shade<PT_EST_MIS, STACK, MEDIA>(r, cur);
+            case PT_EST_PT: launch_shade<PT_EST_PT, STACK, MEDIA, SPEC>(r, cur); break;
+            case PT_EST_NEE: launch_shade<PT_EST_NEE, STACK, MEDIA, SPEC>(r, cur); break;
+            default: launch_shade<PT_EST_MIS, STACK, MEDIA, SPEC>(r, cur);
         }
         if (pg.estimator != PT_EST_PT) {
-            if (MEDIA) TIMED(ctx, "shadow_media", 0, hipLaunchKernelGGL((k_shadow_media<STACK>), dim3(tgrid), dim3(TBLOCK), 0, st, S, P));
-            else TIMED(ctx, "shadow", 0, hipLaunchKernelGGL((k_shadow<STACK>), dim3(tgrid), dim3(TBLOCK), 0, st, S, P));
+            if (MEDIA) TIMED(ctx, "shadow_media", 0, hipLaunchKernelGGL((k_shadow_media<STACK, SPEC>), dim3(tgrid), dim3(TBLOCK), 0, st, S, P));
+            else TIMED(ctx, "shadow", 0, hipLaunchKernelGGL((k_shadow<STACK, SPEC>), dim3(tgrid), dim3(TBLOCK), 0, st, S, P));
         }
         cur ^= 1;
     }
     hipLaunchKernelGGL(k_bookkeep, dim3(1), dim3(64), 0, st, P, -1, -1, 0);
     const int npix = W * H;
-    TIMED(ctx, "accumulate", npix, hipLaunchKernelGGL(k_accumulate, dim3((npix + SBLOCK - 1) / SBLOCK), dim3(SBLOCK), 0, st, P, (float4*)r->film.p, npix, frames, sample0));
+    TIMED(ctx, "accumulate", npix, hipLaunchKernelGGL((k_accumulate<SPEC>), dim3((npix + SBLOCK - 1) / SBLOCK), dim3(SBLOCK), 0, st, S, P, (float4*)r->film.p, npix, frames, sample0, pg.wavelengths));
 }
 
 extern "C" {
@@ -1053,6 +1113,9 @@ pt_status pt_render_frames(pt_renderer* r, const pt_params* prm, uint32_t nframe
     HIP_TRY(hipSetDevice(ctx->device));
     const pt_program& pg = r->prog;
     const int F = std::max(1, std::min<int>(pg.frames_in_flight, (int)std::max<uint32_t>(nframes, 1)));
+    const bool spectral = pg.color == PT_COLOR_SPECTRAL;
+    if (spectral && !ctx->dev.spec_curves)
+        return fail(PT_E_MISSING_INPUT, "missing input: spectral_curves (the spectral program needs the CIE/D65 curves in pt_scene_desc)");
     pt_status st = ensure_paths(r, (long long)pg.width * pg.height * F);
     if (st != PT_OK) return st;
     HIP_TRY(hipMemsetAsync(r->b_hist.p, 0, r->b_hist.bytes, ctx->stream));
@@ -1067,9 +1130,17 @@ pt_status pt_render_frames(pt_renderer* r, const pt_params* prm, uint32_t nframe
     uint32_t done = 0;
     while (done < nframes) {
         int f = (int)std::min<uint32_t>((uint32_t)F, nframes - done);
-        const bool media = ctx->dev.has_media != 0;
-        if (ctx->stack == 16) { if (media) launch_wave<16, true>(r, cam, f, r->sample_count); else launch_wave<16, false>(r, cam, f, r->sample_count); }
-        else { if (media) launch_wave<32, true>(r, cam, f, r->sample_count); else launch_wave<32, false>(r, cam, f, r->sample_count); }
+        const int v = (ctx->stack == 16 ? 0 : 4) | (ctx->dev.has_media ? 2 : 0) | (spectral ? 1 : 0);
+        switch (v) {
+            case 0: launch_wave<16, false, false>(r, cam, f, r->sample_count); break;
+            case 1: launch_wave<16, false, true>(r, cam, f, r->sample_count); break;
+            case 2: launch_wave<16, true, false>(r, cam, f, r->sample_count); break;
+            case 3: launch_wave<16, true, true>(r, cam, f, r->sample_count); break;
+            case 4: launch_wave<32, false, false>(r, cam, f, r->sample_count); break;
+            case 5: launch_wave<32, false, true>(r, cam, f, r->sample_count); break;
+            case 6: launch_wave<32, true, false>(r, cam, f, r->sample_count); break;
+            default: launch_wave<32, true, true>(r, cam, f, r->sample_count);
+        }
         HIP_TRY(hipGetLastError());
         r->sample_count += (uint32_t)f;
         done += (uint32_t)f;

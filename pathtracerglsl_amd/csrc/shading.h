@@ -189,9 +189,82 @@ PT_DEV void classify_hit(const DevScene& S, Hit& h, bool back, bool need, uint2*
     }
 }
 
+// ---------------------------------------------------------------- spectra
+// Wavelengths of a path (spectral programs): four hero lanes.
+struct Wl { float4 lam; };
+PT_DEV float glsl_mix(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+// spectral_curves (components/color/spectral/lookup.glsl:2-7): linear interpolation of the
+// 5 nm rows, int() truncation of the row coordinate.
+PT_DEV float4 spectral_curves(const DevScene& S, float lam) {
+    float x = (lam - 360.0f) / 5.0f;
+    int i = (int)x; i = i < 93 ? i : 93;
+    float f = x - (float)i;
+    float4 a = S.spec_curves[i], b = S.spec_curves[i + 1];
+    return make_float4(glsl_mix(a.x, b.x, f), glsl_mix(a.y, b.y, f), glsl_mix(a.z, b.z, f), glsl_mix(a.w, b.w, f));
+}
+PT_DEV float lane(const float4& v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : (i == 2 ? v.z : v.w)); }
+PT_DEV float spectral_bounded_lane(const pt_spectrum& r, float lam) {   // lookup.glsl:40-48
+    float t = (lam - 360.0f) / 470.0f;
+    float p = (r.c[0] * t + r.c[1]) * t + r.c[2];
+    return gclamp(0.5f + 0.5f * p / sqrtf(1.0f + p * p), 0.0f, 1.0f);
+}
+// blackbody_emission (color/spectra/blackbody/blackbody.glsl:4-10); 14387768.775039338 is
+// the GLSL literal, rounded to fp32.
+PT_DEV float blackbody_lane(float kelvin, float normalization, float lam) {
+    float q = 14387768.775039338f / kelvin;
+    float ratio = 550.0f / lam;
+    float ratio2 = ratio * ratio;
+    return normalization * (ptm_exp(q / 550.0f) - 1.0f) * ratio2 * ratio2 * ratio / (ptm_exp(q / lam) - 1.0f);
+}
+// A pt_spectrum at the path's four wavelengths: color_lift_{reflectance,coefficient,emission}
+// (lookup.glsl:50-60) for an authored RGB, or scale * blackbody_emission.
+PT_DEV sp4 spectrum_eval(const DevScene& S, const pt_spectrum& r, const Wl& wl) {
+    float v[4];
+    for (int i = 0; i < 4; i++) {
+        const float lam = lane(wl.lam, i);
+        float x;
+        if (r.kind == PT_SPEC_BLACKBODY) {
+            x = r.scale * blackbody_lane(r.c[0], r.c[1], lam);
+        } else {
+            x = r.kind == PT_SPEC_ZERO ? 0.0f : (r.kind == PT_SPEC_GREY ? r.value : spectral_bounded_lane(r, lam));
+            if (r.scaled) x = r.scale * x;
+            if (r.d65) x = x * spectral_curves(S, lam).w;
+        }
+        v[i] = x;
+    }
+    return mk4(v[0], v[1], v[2], v[3]);
+}
+// The color model's lift of a material / light property: the authored RGB in an RGB program,
+// the lifted spectrum in a spectral one.
+template <bool SPEC>
+PT_DEV sp4 lift(const DevScene& S, const float* rgb, const pt_spectrum& r, const Wl& wl) {
+    if (!SPEC) return rgb4(rgb);
+    return spectrum_eval(S, r, wl);
+}
+// wavelength_from_cdf / wavelength_pdf (color/spectral/visible.glsl:3-13, uniform.glsl:1-2)
+PT_DEV float wl_visible_a() { return ptm_tanh(0.0072f * (360.0f - 538.0f)); }
+PT_DEV float wl_visible_b() { return ptm_tanh(0.0072f * (830.0f - 538.0f)); }
+PT_DEV float wavelength_from_cdf(bool uniform, float u) {
+    if (uniform) return 360.0f + 470.0f * u;
+    return 538.0f + ptm_atanh(glsl_mix(wl_visible_a(), wl_visible_b(), u)) / 0.0072f;
+}
+PT_DEV float wavelength_pdf(bool uniform, float lam) {
+    if (uniform) return 1.0f / 470.0f;
+    float c = ptm_cosh(0.0072f * (lam - 538.0f));
+    return 0.0072f / ((wl_visible_b() - wl_visible_a()) * c * c);
+}
+// sensor_response (sensor/cie1931/cie1931.glsl:3-5): CIE_XYZ_TO_RGB * xyz / CIE_Y_INTEGRAL
+PT_DEV f3 sensor_response(const DevScene& S, float lam) {
+    float4 c = spectral_curves(S, lam);
+    const float* m = S.xyz_rgb;
+    f3 r = mk3((m[0] * c.x + m[1] * c.y) + m[2] * c.z, (m[3] * c.x + m[4] * c.y) + m[5] * c.z,
+               (m[6] * c.x + m[7] * c.y) + m[8] * c.z);
+    return r / S.cie_y;
+}
+
 // ---------------------------------------------------------------- materials
 enum { LOBE_REFLECTION = 1, LOBE_TRANSMISSION = 2, LOBE_DELTA = 4 };
-struct ISample { f3 wi; f3 weight; float pdf; int flags; };
+struct ISample { f3 wi; sp4 weight; float pdf; int flags; };
 
 PT_DEV f3 to_local(const Frame& f, f3 v) { return mk3(dot(v, f.t), dot(v, f.b), dot(v, f.n)); }
 PT_DEV f3 from_local(const Frame& f, f3 v) { return normalize((f.t * v.x + f.b * v.y) + f.n * v.z); }
@@ -213,9 +286,9 @@ PT_DEV f3 mf_vndf(f3 wol, float a, float ux, float uy) {
     f3 nh = (T1 * t1 + T2 * t2) + vh * sqrtf(gmax(0.0f, 1.0f - t1 * t1 - t2 * t2));
     return normalize(mk3(a * nh.x, a * nh.y, gmax(1e-6f, nh.z)));
 }
-PT_DEV f3 schlick(f3 f0, float c) {
+PT_DEV sp4 schlick(sp4 f0, float c) {
     float p = ptm_pow(1.0f - c, 5.0f);
-    return mk3(f0.x + (1.0f - f0.x) * p, f0.y + (1.0f - f0.y) * p, f0.z + (1.0f - f0.z) * p);
+    return mk4(f0.x + (1.0f - f0.x) * p, f0.y + (1.0f - f0.y) * p, f0.z + (1.0f - f0.z) * p, f0.w + (1.0f - f0.w) * p);
 }
 PT_DEV float dielectric_fresnel(float cos_i, float eta) {   // math.glsl:95-102
     float sin2_t = eta * eta * (1.0f - cos_i * cos_i);
@@ -229,8 +302,8 @@ PT_DEV float dielectric_fresnel(float cos_i, float eta) {   // math.glsl:95-102
 // (category kernels shade one model per launch).
 template <int MODEL> PT_DEV bool is_model(const pt_material& m, int x) { return MODEL < 0 ? m.model == x : MODEL == x; }
 
-template <int MODEL = -1>
-PT_DEV ISample surface_sample(const DevScene& S, const pt_material& m, f3 wo, const Hit& h, float uc, float ux, float uy) {
+template <int MODEL, bool SPEC>
+PT_DEV ISample surface_sample(const DevScene& S, const pt_material& m, f3 wo, const Hit& h, float uc, float ux, float uy, const Wl& wl) {
     ISample s;
     const Frame& f = h.frame;
     if (is_model<MODEL>(m, PT_MAT_LAMBERT)) {            // materials/lambert/lambert.glsl:17-28
@@ -240,7 +313,7 @@ PT_DEV ISample surface_sample(const DevScene& S, const pt_material& m, f3 wo, co
         float phi = PTM_TWO_PI * ux;
         float sn, cs; ptm_sincos(phi, &sn, &cs);
         s.wi = normalize((f.t * (sin_theta * cs) + f.b * (sin_theta * sn)) + n * cos_theta);
-        s.weight = ld3(m.albedo);
+        s.weight = lift<SPEC>(S, m.albedo, m.s_albedo, wl);
         s.pdf = cos_theta * INV_PI;
         s.flags = LOBE_REFLECTION;
     } else if (is_model<MODEL>(m, PT_MAT_DIELECTRIC)) {  // materials/dielectric/dielectric.glsl:14-52
@@ -252,7 +325,7 @@ PT_DEV ISample surface_sample(const DevScene& S, const pt_material& m, f3 wo, co
         float F = dielectric_fresnel(cos_i, eta);
         if (uc < F) {
             s.wi = normalize(n * (2.0f * cos_i) - wo);
-            s.weight = mk3(1.0f, 1.0f, 1.0f);
+            s.weight = sgrey(1.0f);
             s.pdf = 0.0f;
             s.flags = LOBE_REFLECTION | LOBE_DELTA;
         } else {
@@ -260,7 +333,7 @@ PT_DEV ISample surface_sample(const DevScene& S, const pt_material& m, f3 wo, co
             float cos_t = sqrtf(gmax(0.0f, 1.0f - sin2_t));
             s.wi = normalize(wo * (-eta) + n * (eta * cos_i - cos_t));
             float radiance_scale = (n_i * n_i) / (n_t * n_t);
-            s.weight = ld3(m.transmittance) * radiance_scale;
+            s.weight = lift<SPEC>(S, m.transmittance, m.s_transmittance, wl) * radiance_scale;
             s.pdf = 0.0f;
             s.flags = LOBE_TRANSMISSION | LOBE_DELTA;
         }
@@ -273,10 +346,10 @@ PT_DEV ISample surface_sample(const DevScene& S, const pt_material& m, f3 wo, co
         f3 I = -wol;
         f3 wil = I - hh * (2.0f * dot(hh, I));
         if (wil.z <= 0.0f) {
-            s.wi = wo; s.weight = mk3(0.0f, 0.0f, 0.0f); s.pdf = 0.0f; s.flags = LOBE_REFLECTION;
+            s.wi = wo; s.weight = sgrey(0.0f); s.pdf = 0.0f; s.flags = LOBE_REFLECTION;
             return s;
         }
-        f3 F = schlick(ld3(m.f0), fabsf(dot(wol, hh)));
+        sp4 F = schlick(lift<SPEC>(S, m.f0, m.s_f0, wl), fabsf(dot(wol, hh)));
         s.weight = F * mf_G1(wil, a);
         s.pdf = mf_G1(wol, a) * mf_D(hh, a) / (4.0f * fabsf(wol.z));
         s.flags = LOBE_REFLECTION;
@@ -284,21 +357,21 @@ PT_DEV ISample surface_sample(const DevScene& S, const pt_material& m, f3 wo, co
     }
     return s;
 }
-template <int MODEL = -1>
-PT_DEV f3 surface_eval(const pt_material& m, f3 wi, f3 wo, const Hit& h) {
+template <int MODEL, bool SPEC>
+PT_DEV sp4 surface_eval(const DevScene& S, const pt_material& m, f3 wi, f3 wo, const Hit& h, const Wl& wl) {
     const Frame& f = h.frame;
     if (is_model<MODEL>(m, PT_MAT_LAMBERT)) {
-        if (dot(wi, f.n) * dot(wo, f.n) <= 0.0f) return mk3(0.0f, 0.0f, 0.0f);
-        return ld3(m.albedo) * INV_PI;
+        if (dot(wi, f.n) * dot(wo, f.n) <= 0.0f) return sgrey(0.0f);
+        return lift<SPEC>(S, m.albedo, m.s_albedo, wl) * INV_PI;
     } else if (is_model<MODEL>(m, PT_MAT_GGX)) {
         f3 wil = to_local(f, wi), wol = to_local(f, wo);
-        if (wil.z * wol.z <= 0.0f) return mk3(0.0f, 0.0f, 0.0f);
+        if (wil.z * wol.z <= 0.0f) return sgrey(0.0f);
         float a = m.alpha;
         f3 hh = normalize(wil + wol);
-        f3 F = schlick(ld3(m.f0), fabsf(dot(wol, hh)));
+        sp4 F = schlick(lift<SPEC>(S, m.f0, m.s_f0, wl), fabsf(dot(wol, hh)));
         return F * (mf_D(hh, a) * mf_G1(wil, a) * mf_G1(wol, a) / (4.0f * fabsf(wil.z) * fabsf(wol.z)));
     }
-    return mk3(0.0f, 0.0f, 0.0f);
+    return sgrey(0.0f);
 }
 template <int MODEL = -1>
 PT_DEV float surface_pdf(const pt_material& m, f3 wi, f3 wo, const Hit& h) {
@@ -315,7 +388,6 @@ PT_DEV float surface_pdf(const pt_material& m, f3 wi, f3 wo, const Hit& h) {
     }
     return 0.0f;
 }
-
 // ---------------------------------------------------------------- environment
 // Procedural sky (proc-sky snapshot, generated-glsl.snapshot.test.ts.snap:63758) and the
 // equirect chart + table sampler (components/env/sampler_cdf.glsl, equirect/*.glsl).
@@ -326,12 +398,13 @@ PT_DEV f3 sky_raw(f3 dir) {
     float k = ptm_smoothstep(0.9994f, 0.9999f, dot(dir, sun));
     return base + mk3(18.0f, 16.0f, 13.0f) * k;
 }
-PT_DEV f3 env_radiance(const DevScene& S, f3 dir) {
-    if (S.env_type != PT_ENV_SKY) return mk3(0.0f, 0.0f, 0.0f);
+// Environment radiance: RGB programs only (the spectral compile rejects environments).
+PT_DEV sp4 env_radiance(const DevScene& S, f3 dir) {
+    if (S.env_type != PT_ENV_SKY) return sgrey(0.0f);
     f3 d = normalize(dir);
     float cx = S.env_rot_c, sy = S.env_rot_s;
     d = mk3(cx * d.x - sy * d.z, d.y, sy * d.x + cx * d.z);
-    return sky_raw(d) * S.env_intensity;
+    return rgb4(sky_raw(d) * S.env_intensity);
 }
 PT_DEV float env_texel_dOmega(int j, int W, int H) {
     float theta = PTM_PI * ((float)j + 0.5f) / (float)H;
@@ -388,9 +461,10 @@ PT_DEV f3 env_sample_table(const DevScene& S, float x0, float x1) {
 }
 
 // ---------------------------------------------------------------- lights
-struct LSample { f3 wi; float distance; f3 radiance; float pdf; int flags; int light_id; };
+struct LSample { f3 wi; float distance; sp4 radiance; float pdf; int flags; int light_id; };
 
-PT_DEV LSample light_sample_one(const pt_light& l, f3 p, float x0, float x1) {
+template <bool SPEC>
+PT_DEV LSample light_sample_one(const DevScene& S, const pt_light& l, f3 p, float x0, float x1, const Wl& wl) {
     LSample ls;
     ls.light_id = -1;
     if (l.kind == PT_LIGHT_QUAD) {              // lights/quad/quad.glsl:14-30
@@ -399,7 +473,7 @@ PT_DEV LSample light_sample_one(const pt_light& l, f3 p, float x0, float x1) {
         float d2 = dot(d, d);
         ls.wi = d * (1.0f / sqrtf(d2));
         ls.distance = sqrtf(d2);
-        ls.radiance = ld3(l.radiance);
+        ls.radiance = lift<SPEC>(S, l.radiance, l.s_radiance, wl);
         ls.flags = 0;
         float cos_l = dot(ld3(l.normal), -ls.wi);
         if (cos_l <= 0.0f) { ls.pdf = 0.0f; return ls; }
@@ -409,7 +483,7 @@ PT_DEV LSample light_sample_one(const pt_light& l, f3 p, float x0, float x1) {
         float d2 = dot(d, d);
         ls.wi = d * (1.0f / sqrtf(d2));
         ls.distance = sqrtf(d2);
-        ls.radiance = ld3(l.radiance) / d2;
+        ls.radiance = lift<SPEC>(S, l.radiance, l.s_radiance, wl) / d2;
         ls.pdf = 1.0f; ls.flags = 1;
     }
     return ls;
@@ -423,15 +497,16 @@ PT_DEV float light_pdf_one(const pt_light& l, f3 p, f3 light_p, f3 wi) {
 }
 PT_DEV bool env_samplable(const DevScene& S) { return S.env_type == PT_ENV_SKY; }
 PT_DEV bool env_select_live(const DevScene& S) { return S.env_type == PT_ENV_SKY && S.n_lights > 0; }
-PT_DEV LSample lighting_sample_finite(const DevScene& S, f3 p, float x0, float x1) {
+template <bool SPEC>
+PT_DEV LSample lighting_sample_finite(const DevScene& S, f3 p, float x0, float x1, const Wl& wl) {
     const int n = S.n_lights;
-    if (n == 1) { LSample ls = light_sample_one(S.lights[0], p, x0, x1); ls.light_id = 0; return ls; }
+    if (n == 1) { LSample ls = light_sample_one<SPEC>(S, S.lights[0], p, x0, x1, wl); ls.light_id = 0; return ls; }
     int id = n - 1; float sel = S.lights[n - 1].select_pdf, lo = S.lights[n - 2].cdf;
     for (int i = 0; i < n - 1; i++) {
         if (x0 < S.lights[i].cdf) { id = i; sel = S.lights[i].select_pdf; lo = i == 0 ? 0.0f : S.lights[i - 1].cdf; break; }
     }
     float xr = (x0 - lo) / sel;
-    LSample ls = light_sample_one(S.lights[id], p, gclamp(xr, 0.0f, 0.9999999f), x1);
+    LSample ls = light_sample_one<SPEC>(S, S.lights[id], p, gclamp(xr, 0.0f, 0.9999999f), x1, wl);
     ls.light_id = id;
     ls.pdf *= sel;
     return ls;
@@ -455,12 +530,13 @@ PT_DEV LSample environment_sample(const DevScene& S, float x0, float x1) {   // 
     return ls;
 }
 // lighting_sample (features/lighting.ts:546-628): env-only, finite-only or two-stage.
-PT_DEV LSample lighting_sample(const DevScene& S, f3 p, float x0, float x1) {
+template <bool SPEC>
+PT_DEV LSample lighting_sample(const DevScene& S, f3 p, float x0, float x1, const Wl& wl) {
     if (S.n_lights == 0) {
         if (env_samplable(S)) return environment_sample(S, x0, x1);
         LSample ls; ls.pdf = 0.0f; return ls;
     }
-    if (!env_select_live(S)) return lighting_sample_finite(S, p, x0, x1);
+    if (!env_select_live(S)) return lighting_sample_finite<SPEC>(S, p, x0, x1, wl);
     float ps = S.env_select_prob;
     if (x0 < ps) {
         LSample ls = environment_sample(S, gclamp(x0 / ps, 0.0f, 0.9999999f), x1);
@@ -468,7 +544,7 @@ PT_DEV LSample lighting_sample(const DevScene& S, f3 p, float x0, float x1) {
         return ls;
     }
     float fx = gclamp((x0 - ps) / (1.0f - ps), 0.0f, 0.9999999f);
-    LSample ls = lighting_sample_finite(S, p, fx, x1);
+    LSample ls = lighting_sample_finite<SPEC>(S, p, fx, x1, wl);
     ls.pdf *= (1.0f - ps);
     return ls;
 }
@@ -490,38 +566,44 @@ PT_DEV float lighting_pdf(const DevScene& S, f3 qp, f3 wi, int lid, f3 light_p) 
 PT_DEV int material_of(const DevScene& S, int region) { return region < 0 ? S.ambient_material : S.objects[region].material; }
 PT_DEV bool is_null_interface(const DevScene& S, int mat) { return mat >= 0 && S.materials[mat].model == PT_MAT_NULL; }
 PT_DEV bool material_has_medium(const DevScene& S, int mat) { return mat >= 0 && S.materials[mat].has_medium; }
-PT_DEV float spectrum_average(f3 s) { float sum = s.x; sum += s.y; sum += s.z; return sum * (1.0f / 3.0f); }
-PT_DEV f3 spectrum_exp(f3 s) { return mk3(ptm_exp(s.x), ptm_exp(s.y), ptm_exp(s.z)); }
-struct MSample { bool scattered; float t; f3 weight; };
-PT_DEV MSample medium_sample_analytic(const pt_material& m, float t_max, float x0, float x1) {
+PT_DEV sp4 spectrum_exp(sp4 s) { return mk4(ptm_exp(s.x), ptm_exp(s.y), ptm_exp(s.z), ptm_exp(s.w)); }
+template <bool SPEC> PT_DEV sp4 sigma_s_of(const DevScene& S, const pt_material& m, const Wl& wl) { return lift<SPEC>(S, m.sigma_s, m.s_sigma_s, wl); }
+template <bool SPEC> PT_DEV sp4 sigma_a_of(const DevScene& S, const pt_material& m, const Wl& wl) { return lift<SPEC>(S, m.sigma_a, m.s_sigma_a, wl); }
+struct MSample { bool scattered; float t; sp4 weight; };
+template <bool SPEC>
+PT_DEV MSample medium_sample_analytic(const DevScene& S, const pt_material& m, float t_max, float x0, float x1, const Wl& wl) {
     MSample ms;
-    f3 ss = ld3(m.sigma_s);
-    f3 st = ld3(m.sigma_a) + ss;
-    int c = (int)(x0 * 3.0f); if (c > 2) c = 2;
-    float sc = c == 0 ? st.x : (c == 1 ? st.y : st.z);
+    sp4 ss = sigma_s_of<SPEC>(S, m, wl);
+    sp4 st = sigma_a_of<SPEC>(S, m, wl) + ss;
+    constexpr int N = SPEC ? 4 : 3;
+    int c = (int)(x0 * (float)N); if (c > N - 1) c = N - 1;     // spectrum_sample_channel
+    float sc = spectrum_comp(st, c);
     float t = sc == 0.0f ? t_max : -ptm_log(1.0f - x1) / sc;
     if (t < t_max) {
-        f3 tr = spectrum_exp((-st) * t);
-        float pdf = spectrum_average(st * tr);
+        sp4 tr = spectrum_exp((-st) * t);
+        float pdf = spectrum_average<SPEC>(st * tr);
         ms.scattered = true; ms.t = t;
         ms.weight = (ss * tr) / gmax(pdf, 1e-20f);
     } else {
-        f3 tr = spectrum_exp((-st) * t_max);
-        float pdf = spectrum_average(tr);
+        sp4 tr = spectrum_exp((-st) * t_max);
+        float pdf = spectrum_average<SPEC>(tr);
         ms.scattered = false; ms.t = t_max;
         ms.weight = tr / gmax(pdf, 1e-20f);
     }
     return ms;
 }
-PT_DEV f3 medium_transmittance(const pt_material& m, float len) {
-    return spectrum_exp((-(ld3(m.sigma_a) + ld3(m.sigma_s))) * len);
+template <bool SPEC>
+PT_DEV sp4 medium_transmittance(const DevScene& S, const pt_material& m, float len, const Wl& wl) {
+    return spectrum_exp((-(sigma_a_of<SPEC>(S, m, wl) + sigma_s_of<SPEC>(S, m, wl))) * len);
 }
 PT_DEV float hg_value(f3 wi, f3 wo, float g) {
     float c = dot(wi, -wo);
     float d = 1.0f + g * g - 2.0f * g * c;
     return (1.0f - g * g) / (4.0f * PTM_PI * d * sqrtf(gmax(d, 1e-8f)));
 }
-PT_DEV float hg_pdf(f3 wi, f3 wo, float g) { float h = hg_value(wi, wo, g); return spectrum_average(mk3(h, h, h)); }
+template <bool SPEC>
+PT_DEV float hg_pdf(f3 wi, f3 wo, float g) { float h = hg_value(wi, wo, g); return spectrum_average<SPEC>(sgrey(h)); }
+template <bool SPEC>
 PT_DEV f3 hg_sample(f3 wo, float g, float x0, float x1, float& pdf) {
     float cos_theta;
     if (fabsf(g) < 1e-3f) cos_theta = 1.0f - 2.0f * x0;
@@ -538,7 +620,7 @@ PT_DEV f3 hg_sample(f3 wo, float g, float x0, float x1, float& pdf) {
     f3 b = cross(nw, t);
     float sn, cs; ptm_sincos(phi, &sn, &cs);
     f3 wi = normalize((t * (sin_theta * cs) + b * (sin_theta * sn)) + nw * cos_theta);
-    pdf = hg_pdf(wi, wo, g);
+    pdf = hg_pdf<SPEC>(wi, wo, g);
     return wi;
 }
 

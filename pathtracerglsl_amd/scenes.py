@@ -261,3 +261,49 @@ def cornell_glass_medium(light_radiance=(12.0, 12.0, 12.0)):
                                 "medium": {"sigma_s": (0.9 * st).tolist(), "sigma_a": (0.1 * st).tolist(), "phase_g": 0.7}}
     sc["objects"].append({"type": "sphere", "center": [0.0, 0.0, 0.0], "radius": 0.5, "material": "glass"})
     return sc
+
+
+def spectral_direct():
+    """spectral-direct (tests/witnesses/scenes/spectralFoundation.ts:21-24,33-38): the camera sees
+    only a unit grey emitter (D65 white under the spectral lift); with maxBounces 0 every pixel
+    projects to (1, 1, 1) through the CIE sensor. The plane is a large quad at z = -1."""
+    return {
+        "id": "spectral-direct",
+        "materials": {"plane": {"model": "lambert", "albedo": 0.0, "emission": 1.0}},
+        "objects": [{"type": "quad", "corner": [-50.0, -50.0, -1.0], "edge1": [100.0, 0.0, 0.0],
+                     "edge2": [0.0, 100.0, 0.0], "material": "plane"}],
+        "lights": [],
+        "environment": {"type": "none"},
+        "camera": {"type": "pinhole", "position": [0.0, 0.0, 0.0], "target": [0.0, 0.0, -1.0], "fov": 0.8},
+    }
+
+
+def spectral_furnace(albedo=(0.5, 0.5, 0.5)):
+    """spectral-furnace / spectral-colored-furnace (spectralFoundation.ts:15-19,39-48): the
+    mesh furnace under the spectral color model; grey albedo 0.5 gives 0.4 (1 - 0.5^7)
+    through bounce six, albedo (0.2, 0.5, 0.8) the projected finite series
+    (0.22947, 0.39324, 0.76522)."""
+    sc = mesh_furnace()
+    sc["id"] = "spectral-furnace"
+    sc["materials"]["furnace"]["albedo"] = list(albedo)
+    return sc
+
+
+def spectral_slab():
+    """spectral-slab (tests/witnesses/scenes/spectralMedia.ts:6-9,41-43): F-SLAB with
+    sigma_a = (0.3, 0.7, 1.2) lifted as a coefficient spectrum; the D65 screen seen through
+    one unit of ink projects to (0.72520, 0.49546, 0.29589)."""
+    sc = slab_witness()
+    sc["id"] = "spectral-slab"
+    sc["materials"]["ink"] = {"model": "none", "medium": {"sigma_a": [0.3, 0.7, 1.2]}}
+    return sc
+
+
+def cornell_spectral_medium(kelvin=6500.0, scale=12.0):
+    """The spectral homogeneous-media workload: the glass-medium Cornell box lit by a
+    blackbody quad (kelvin, scale) under the spectral color model (four hero wavelengths)."""
+    sc = cornell_glass_medium()
+    sc["id"] = "cornell-spectral-medium"
+    for l in sc["lights"]:
+        l["radiance"] = {"blackbody": {"kelvin": kelvin, "scale": scale}}
+    return sc

@@ -1,0 +1,257 @@
+"""Spectral color model, host side: the data the four-wavelength transport reads and the lift of
+authored RGB values into per-material spectra.
+
+The device evaluates a spectrum at a path's wavelengths from a pt_spectrum record
+(include/ptmi.h); this module builds those records the way the reference's GLSL would evaluate
+its lifts for a constant authored value:
+
+- reflectance role: spectral_bounded(rgb) (src/components/color/spectral/lookup.glsl:40-48);
+- coefficient role: 2 max(rgb) * spectral_bounded(rgb / (2 max(rgb))) (lookup.glsl:52-56);
+- emission role: the coefficient lift times the D65 curve (lookup.glsl:57-60);
+- blackbody emission: scale * Planck relative to 550 nm with a host CIE-Y normalisation
+  (src/components/color/spectra/blackbody/blackbody.ts:13-41, blackbody.glsl:4-10).
+
+spectral_bounded's sigmoid-quadratic coefficients come from the Jakob-Hanika coefficient atlas
+through the trilinear lookup of lookup.glsl:16-38 (spectral_coefficients), restated here in
+float32 operations in the same order. The atlas (3 x 64^3 RGBA32F texels, 12.6 MB) is the
+reference's shipped asset `coefficients.bin`; it is read from the path in PTMI_SPECTRAL_ATLAS
+when set. Coefficients of the colours used by the bundled scenes are kept in a small table
+(data/spectral_lift_cache.json) built from that atlas by data/make_spectral_data.py, so those
+scenes compile without the atlas. The CIE 1931 / D65 curves (curves.bin, 4 KB) ship as
+data/spectral_curves.f32.
+"""
+import hashlib
+import json
+import math
+import os
+
+import numpy as np
+
+from . import abi
+
+DATA_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
+CURVES_PATH = os.path.join(DATA_DIR, "spectral_curves.f32")
+CACHE_PATH = os.path.join(DATA_DIR, "spectral_lift_cache.json")
+CURVES_SHA256 = "86f1739e08f078a4665a7bc64a8b9fb0bf261c592f2696dd73af0a32ee52650e"
+ATLAS_SHA256 = "a753491b81547d3f033a5fc91c523a21079ac43a7d2360c9258eb0912823748b"
+
+# src/components/color/spectral/data.ts:15-45 (generated tables, jh-cie1931-d65-v1)
+CIE_Y_INTEGRAL = 106.85702892807676
+XYZ_TO_RGB = np.array([
+    [3.2401742558155275, -1.5370057367330066, -0.4984883470485426],
+    [-0.9692738386733962, 1.8760259581276126, 0.04155635229538314],
+    [0.055653158206683866, -0.20406158009117398, 1.0574100059269926]])
+RGB_TO_XYZ = np.array([
+    [0.4124920696286926, 0.35757319712237773, 0.1804059457916708],
+    [0.21269122340229463, 0.7151463942447555, 0.07216237831666832],
+    [0.019335565763844945, 0.11919106570745922, 0.9501379811694662]])
+PLANCK_C2 = 6.62607015e-34 * 299792458 / 1.380649e-23 * 1e9    # blackbody.ts:11, nm K
+
+ROLE_REFLECTANCE, ROLE_COEFFICIENT, ROLE_EMISSION = "reflectance", "coefficient", "emission"
+F32 = np.float32
+
+
+class SpectralDataError(ValueError):
+    pass
+
+
+_curves = None
+
+
+def curves_texture():
+    """The 128 x 2 RGBA32F curves texture: row 0 = (X, Y, Z, D65) at 360 + 5 i nm for
+    i < 95, row 1 .x = the 64 z nodes of the coefficient atlas."""
+    global _curves
+    if _curves is None:
+        raw = open(CURVES_PATH, "rb").read()
+        if hashlib.sha256(raw).hexdigest() != CURVES_SHA256:
+            raise SpectralDataError(f"{CURVES_PATH}: SHA-256 does not match the shipped curves")
+        _curves = np.frombuffer(raw, np.float32).reshape(2, 128, 4)
+    return _curves
+
+
+def curve_rows():
+    """[95][4] float32: CIE X, Y, Z and D65 (what pt_scene_desc.spectral_curves carries)."""
+    return np.ascontiguousarray(curves_texture()[0, :95, :])
+
+
+def z_nodes():
+    return curves_texture()[1, :64, 0]
+
+
+def load_atlas(path):
+    raw = np.fromfile(path, np.float32)
+    if raw.size != 3 * 64 ** 3 * 4:
+        raise SpectralDataError(f"{path}: expected 3 x 64^3 RGBA32F texels")
+    h = hashlib.sha256(raw.tobytes()).hexdigest()
+    if h != ATLAS_SHA256:
+        raise SpectralDataError(f"{path}: SHA-256 {h} does not match the shipped coefficient atlas")
+    return raw.reshape(-1, 4)
+
+
+def _mix(a, b, t):
+    """GLSL mix(a, b, t) = a (1 - t) + b t, float32."""
+    return F32(F32(a * F32(F32(1.0) - t)) + F32(b * t))
+
+
+def spectral_coefficients(rgb, atlas):
+    """spectral_coefficients (lookup.glsl:16-38) in float32 operations."""
+    r = [F32(v) for v in rgb]
+    ch = 0 if (r[0] >= r[1] and r[0] >= r[2]) else (1 if r[1] >= r[2] else 2)
+    z = r[ch]
+    x = F32(F32(r[(ch + 1) % 3] / z) * F32(63.0))
+    y = F32(F32(r[(ch + 2) % 3] / z) * F32(63.0))
+    ix, iy = min(int(x), 62), min(int(y), 62)
+    fx, fy = F32(x - F32(ix)), F32(y - F32(iy))
+    zn = z_nodes()
+    lo, hi = 0, 63
+    for _ in range(6):
+        mid = (lo + hi) // 2
+        if zn[mid] <= z:
+            lo = mid
+        else:
+            hi = mid
+    z0, z1 = zn[lo], zn[lo + 1]
+    fz = F32(F32(z - z0) / F32(z1 - z0))
+
+    def node(xx, yy, zz):
+        return atlas[((ch * 64 + zz) * 64 + yy) * 64 + xx, :3]
+
+    out = []
+    for c in range(3):
+        def plane(zz):
+            return _mix(_mix(node(ix, iy, zz)[c], node(ix + 1, iy, zz)[c], fx),
+                        _mix(node(ix, iy + 1, zz)[c], node(ix + 1, iy + 1, zz)[c], fx), fy)
+        out.append(_mix(plane(lo), plane(lo + 1), fz))
+    return out
+
+
+def _key(rgb):
+    return ",".join(f"{np.float32(v).view(np.uint32):08x}" for v in rgb)
+
+
+_cache = None
+_atlas = None
+
+
+def _lookup(bounded):
+    """Coefficients of a non-grey bounded rgb: the lift table first, then the atlas."""
+    global _cache, _atlas
+    if _cache is None:
+        _cache = json.load(open(CACHE_PATH)) if os.path.exists(CACHE_PATH) else {}
+    k = _key(bounded)
+    if k in _cache:
+        return [np.uint32(int(h, 16)).view(np.float32) for h in _cache[k]]
+    path = os.environ.get("PTMI_SPECTRAL_ATLAS")
+    if not path:
+        raise SpectralDataError(
+            f"spectral lift of rgb {tuple(float(v) for v in bounded)} needs the Jakob-Hanika coefficient "
+            "atlas: set PTMI_SPECTRAL_ATLAS to the path of coefficients.bin")
+    if _atlas is None:
+        _atlas = load_atlas(path)
+    return spectral_coefficients(bounded, _atlas)
+
+
+def bounded_input(rgb, role):
+    """(scale, bounded rgb) as the GLSL computes them for the role (float32)."""
+    r = [F32(v) for v in rgb]
+    if role == ROLE_REFLECTANCE:
+        return None, r
+    scale = F32(F32(2.0) * max(r[0], max(r[1], r[2])))
+    if scale == 0.0:
+        return scale, None
+    return scale, [F32(v / scale) for v in r]
+
+
+def spectrum_record(value, role):
+    """pt_spectrum for an authored value (rgb triple, scalar, or {'blackbody': {...}})."""
+    rec = np.zeros(1, abi.SPECTRUM_DTYPE)[0]
+    if isinstance(value, dict) and "blackbody" in value:
+        if role != ROLE_EMISSION:
+            raise SpectralDataError("blackbody values are emission only")
+        bb = value["blackbody"]
+        kelvin = float(bb["kelvin"])
+        if not (1000.0 <= kelvin <= 40000.0):
+            raise SpectralDataError("blackbody kelvin must be in [1000, 40000]")
+        rec["kind"] = abi.SPEC_BLACKBODY
+        rec["scale"] = float(bb.get("scale", 1.0))
+        rec["c"][:2] = [kelvin, blackbody_color(kelvin)[1]]
+        return rec
+    rgb = [float(value)] * 3 if not hasattr(value, "__len__") else [float(v) for v in value]
+    scale, b = bounded_input(rgb, role)
+    rec["scaled"] = int(role != ROLE_REFLECTANCE)
+    rec["d65"] = int(role == ROLE_EMISSION)
+    if role != ROLE_REFLECTANCE:
+        rec["scale"] = scale
+    if b is None:
+        rec["kind"] = abi.SPEC_ZERO
+    elif b[0] == b[1] and b[1] == b[2]:
+        rec["kind"] = abi.SPEC_GREY
+        rec["value"] = b[0]
+    else:
+        rec["kind"] = abi.SPEC_SIGMOID
+        rec["c"][:3] = _lookup(b)
+    return rec
+
+
+def rgb_of(value):
+    """RGB stand-in of an authored value in an RGB program (blackbody: blackbodyRGB)."""
+    if isinstance(value, dict) and "blackbody" in value:
+        bb = value["blackbody"]
+        rgb, _ = blackbody_color(float(bb["kelvin"]))
+        return [v * float(bb.get("scale", 1.0)) for v in rgb]
+    return [float(value)] * 3 if not hasattr(value, "__len__") else [float(v) for v in value]
+
+
+def planck_shape(lam, kelvin):
+    """Planck radiance relative to 550 nm (blackbody.ts:13-17), float64."""
+    return (550.0 / lam) ** 5 * math.expm1(PLANCK_C2 / (550.0 * kelvin)) / math.expm1(PLANCK_C2 / (lam * kelvin))
+
+
+def blackbody_color(kelvin):
+    """blackbodyColor (blackbody.ts:21-41): (rgb at unit CIE Y, normalization), float64,
+    Simpson over each 5 nm CMF interval with 0.5 nm steps on the float32 CMFs."""
+    cmf = curve_rows()[:, :3].astype(np.float64)
+    xyz = [0.0, 0.0, 0.0]
+    for interval in range(cmf.shape[0] - 1):
+        for sample in range(11):
+            fraction = sample / 10
+            lam = 360 + 5 * (interval + fraction)
+            weight = 0.5 / 3 * (1 if sample in (0, 10) else (4 if sample % 2 else 2))
+            value = weight * planck_shape(lam, kelvin) / CIE_Y_INTEGRAL
+            for c in range(3):
+                response = cmf[interval][c] * (1 - fraction) + cmf[interval + 1][c] * fraction
+                xyz[c] += value * response
+    rgb = [sum(XYZ_TO_RGB[r][i] * xyz[i] for i in range(3)) for r in range(3)]
+    normalization = 1 / xyz[1]
+    return [v * normalization for v in rgb], normalization
+
+
+# ---- float64 references for tests: quadrature of lifted spectra against the observer ----
+
+def curves_at(lam):
+    """spectralCurves (lookup.ts:33-37): linear interpolation of the float32 samples."""
+    c = curve_rows().astype(np.float64)
+    x = (lam - 360.0) / 5.0
+    i = min(int(math.floor(x)), 93)
+    f = x - i
+    return c[i] * (1 - f) + c[i + 1] * f
+
+
+def sigmoid(c, lam):
+    """evaluatePolynomial (lookup.ts:26-30), float64."""
+    t = (lam - 360.0) / 470.0
+    p = (c[0] * t + c[1]) * t + c[2]
+    return 0.5 + 0.5 * p / math.sqrt(1 + p * p)
+
+
+def project_rgb(spectrum, step=0.25):
+    """Signed linear sRGB of a spectrum under the CIE 1931 observer (Simpson, 360-830 nm)."""
+    n = int(round(470 / step))
+    xyz = np.zeros(3)
+    for i in range(n + 1):
+        lam = 360 + step * i
+        w = step / 3 * (1 if i in (0, n) else (4 if i % 2 else 2))
+        xyz += w * spectrum(lam) * curves_at(lam)[:3]
+    xyz /= CIE_Y_INTEGRAL
+    return XYZ_TO_RGB @ xyz

@@ -125,6 +125,7 @@ def test_render_overlapping_glass_matches_oracle(engine, oracle):
     """Dense cloud: spheres interpenetrate, so region classification (scene_region_at) sees
     points inside other closed meshes and dielectric interfaces between media."""
     sc = scenes.icosphere_cloud(n_spheres=300, seed=3, box=12.0)
+    sc["camera"]["position"] = [6.0, 6.0, -10.0]       # outside every sphere, so sky light reaches the film
     comp = scene_cache.compiler.compile_scene(sc, scene_cache.compiler.RenderStrategy("mis", 12, 3), 48, 32)
     engine.load_renderer(comp)
     engine.clear_accumulation()
@@ -134,5 +135,6 @@ def test_render_overlapping_glass_matches_oracle(engine, oracle):
     ors = oracle_lib.OracleScene(oracle, comp.scene)
     o, _, counts = ors.render(comp.program, comp.params(), 4)
     exact, bad = _compare_film("overlap-mis", g, o)
+    assert o[..., :3].mean() > 0.01
     assert bad == 0.0
     assert engine.ray_counts() == (int(counts[0]), int(counts[1]))
