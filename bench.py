@@ -8,6 +8,12 @@ bounce 3, through the 8-wide compressed BVH. One step = one full 64-spp render o
 (pt_render_frames(64) + pt_finish); inputs (scene, BVH, env table) are resident in HBM
 before the timed region.
 
+Adjoint leg (the differentiable media inverse render): the glass-medium Cornell box with 16 grey
+Lambert patches at 512x512, 64 spp, depth 32, pt-mis; the adjoint image is the L2 loss gradient
+between the perturbed-init render and the hidden-target render. Reported: pt_forward and
+pt_backward wall times (their ratio is the adjoint/forward cost) and the gradient's relative
+error against the FP64 CPU replay of the same paths.
+
 Also reported: the roofline of the dominant kernel (the extend / closest-hit traversal),
 whose algorithmic bytes per query are 32 (ray) + 16 (hit) + 80 per wide node visited + 36 per
 triangle tested, counted by an instrumented, untimed replay of the identical render; and a
@@ -46,6 +52,51 @@ def cpu_baseline(comp, spp):
             "seconds": round(dt, 2), "mrays_per_s": round(float(counts.sum()) / dt / 1e6, 4)}
 
 
+def adjoint_leg(Engine, compiler, scenes, abi, fp64=True, res=512, spp=64):
+    """Forward / backward cost of the path-replay adjoint and its error vs the FP64 replay."""
+    import numpy as np
+    target, init = scenes.inverse_media_params()
+    strategy = compiler.RenderStrategy("mis", 32, 3, 0.95, 4)
+    eng = Engine(0)
+    ct = compiler.compile_scene(scenes.inverse_media_scene(target), strategy, res, res)
+    eng.load_renderer(ct)
+    eng.forward(spp, ct.params(101))
+    tgt = eng.read_export()
+    comp = compiler.compile_scene(scenes.inverse_media_scene(init), strategy, res, res)
+    med, alb = scenes.inverse_adjoint_materials(comp)
+    adj = abi.PtAdjointParams.make(med, alb, abi.REDUCE_DETERMINISTIC)
+    eng.load_renderer(comp)
+    eng.forward(spp, comp.params(102))
+    img = eng.read_export()
+    dL = np.zeros(img.shape, np.float32)
+    dL[..., :3] = 2.0 * (img[..., :3] - tgt[..., :3]) / (res * res)
+    eng.backward(spp, adj, dL, comp.params(103))          # warm
+    tf, tb = [], []
+    for _ in range(3):
+        eng.finish()
+        t0 = time.perf_counter()
+        eng.forward(spp, comp.params(103))
+        eng.finish()
+        tf.append(time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        g = eng.backward(spp, adj, dL, comp.params(103))
+        tb.append(time.perf_counter() - t0)
+    out = {"workload": f"inverse-media cornell 16 albedos + medium {res}x{res} {spp}spp depth32 pt-mis",
+           "paths": res * res * spp, "forward_ms": round(1e3 * min(tf), 2), "backward_ms": round(1e3 * min(tb), 2),
+           "adjoint_forward_ratio": round(min(tb) / min(tf), 3), "n_params": abi.ADJ_N_PARAMS,
+           "reduce_mode": "deterministic"}
+    if fp64:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib
+        ors = oracle_lib.OracleScene(oracle_lib.load(), comp.scene)
+        t0 = time.perf_counter()
+        ref = ors.adjoint(comp.program, comp.params(103), spp, adj, dL)
+        out["fp64_replay_seconds"] = round(time.perf_counter() - t0, 2)
+        out["grad_rel_err_vs_fp64"] = float(np.linalg.norm(g.astype(np.float64) - ref) / max(np.linalg.norm(ref), 1e-30))
+    eng.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -57,6 +108,7 @@ def main():
     ap.add_argument("--frames-in-flight", type=int, default=4)
     ap.add_argument("--cpu-spp", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="run a plain device copy of this size first (counter calibration runs)")
     args = ap.parse_args()
@@ -158,6 +210,11 @@ def main():
         log("[bench] CPU baseline")
         result["cpu_baseline"] = cpu_baseline(comp, args.cpu_spp)
     eng.close()
+    if not args.no_adjoint:
+        log("[bench] adjoint leg")
+        from pathtracerglsl_amd import abi
+        result["adjoint"] = adjoint_leg(Engine, compiler, scenes, abi, fp64=not args.no_cpu)
+        log(f"[bench] adjoint {result['adjoint']}")
     print(json.dumps(result), flush=True)
 
 

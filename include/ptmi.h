@@ -256,6 +256,42 @@ int       pt_get_timings(pt_ctx* ctx, pt_kernel_timing* out, int n_slots);
 /* Statistics of the last render: rays traced by kind (closest, any). */
 pt_status pt_last_ray_counts(pt_renderer* r, uint64_t* closest, uint64_t* any);
 
+/* Live material values (the reference's setParameter for material uniforms,
+ * src/engine/Engine.ts:206-236): replace the n material records of the uploaded scene
+ * without rebuilding geometry. model, has_medium and emissive must not change. */
+pt_status pt_scene_update_materials(pt_ctx* ctx, const pt_material* materials, int32_t n);
+
+/* ---- differentiation: path-replay adjoint (RGB programs) ----
+ * Parameters: the homogeneous medium of material `medium_material` (sigma_s RGB, sigma_a RGB,
+ * HG g) and the grey Lambert albedos of up to 16 materials. Gradient layout:
+ * [0..2] d/d sigma_s, [3..5] d/d sigma_a, [6] d/d g, [7 + k] d/d albedo of albedo_material[k].
+ * The backward pass replays the forward pass's paths (same sampleCount / resetSalt streams)
+ * and accumulates, per path, the derivative of the log of the measurement contribution with
+ * the sampling decisions held fixed (detached sampling): every contribution c scatters
+ * sum_ch dL/dI_ch * c_ch * dlog f_ch/dtheta. Sampling densities, MIS weights and Russian
+ * roulette are estimator choices and carry no derivative. A channel with sigma_s = 0 scatters
+ * nothing, so its d/dsigma_s holds the extinction term only. */
+enum { PT_REDUCE_ATOMIC = 0, PT_REDUCE_DETERMINISTIC = 1 };
+enum { PT_ADJ_MAX_ALBEDO = 16, PT_ADJ_N_PARAMS = 7 + PT_ADJ_MAX_ALBEDO };
+typedef struct pt_adjoint_params {
+    int32_t medium_material;                       /* -1: no medium parameters */
+    int32_t n_albedo;                              /* 0..16 */
+    int32_t albedo_material[PT_ADJ_MAX_ALBEDO];    /* Lambert materials with grey albedo */
+    int32_t reduce_mode;                           /* PT_REDUCE_* */
+    int32_t pad;
+} pt_adjoint_params;
+
+/* Forward pass: clear the film and render nframes from sampleCount 0 (the primal image of
+ * the adjoint pair; read it with pt_read_export). */
+pt_status pt_forward(pt_renderer* r, const pt_params* params, uint32_t nframes);
+/* Backward pass: replay the same nframes and write dL/dparams (PT_ADJ_N_PARAMS floats) for
+ * the adjoint image dL_dimage (width*height*4 floats, host memory, row 0 = bottom; the film
+ * average's 1/nframes is applied inside). PT_REDUCE_DETERMINISTIC sums per-pixel partials in
+ * a fixed order (bitwise reproducible); PT_REDUCE_ATOMIC scatters block partials with fp32
+ * atomics. Synchronises. */
+pt_status pt_backward(pt_renderer* r, const pt_params* params, uint32_t nframes, const pt_adjoint_params* adj,
+                      const float* dL_dimage, float* dparams);
+
 #ifdef __cplusplus
 }
 #endif

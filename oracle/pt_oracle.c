@@ -1143,7 +1143,8 @@ static v3 hg_sample(v3 wo, float g, float x0, float x1, float* pdf, const wl_t* 
     return wi;
 }
 /* shadow_transmittance, media walker (transport/shadow/media/media.glsl:29-62). */
-static sp shadow_transmittance_media(const or_scene* S, v3 o, v3 d, v3 light_p, int crossings_left, int* n_any, const wl_t* wl) {
+static sp shadow_transmittance_media(const or_scene* S, v3 o, v3 d, v3 light_p, int crossings_left, int* n_any, const wl_t* wl,
+                                     int len_mat, double* len_in) {
     sp T = sgrey(1.0f);
     int medium = region_at(S, o);
     for (int crossed = 0; crossed <= crossings_left; crossed++) {
@@ -1155,6 +1156,7 @@ static sp shadow_transmittance_media(const or_scene* S, v3 o, v3 d, v3 light_p, 
         float seg_len = hit_boundary ? h.t : remaining;
         int med_mat = material_of(S, medium);
         if (material_has_medium(S, med_mat)) T = smul(T, medium_transmittance(S, &S->materials[med_mat], seg_len, wl));
+        if (len_in && med_mat == len_mat && material_has_medium(S, med_mat)) *len_in += (double)seg_len;
         if (!hit_boundary) return T;
         if (!is_null_interface(S, material_of(S, h.region_owner))) return sgrey(0.0f);
         medium = h.region_to;
@@ -1184,8 +1186,29 @@ static inline v3 ray_spawn_origin(const hit_t* hit, v3 wi) {
 
 /* One camera path; returns radiance. The media branch, null interfaces and the media
  * shadow walker are the program's (f.media, f.nulls) parts of pt.ts:299-464. */
+/* Path-replay adjoint state (FP64): D = dlog f/dtheta of the path so far with sampling
+ * decisions detached, G = the path's gradient sum_contributions dL/dI . c . D.
+ * Parameter layout: [0..2] sigma_s, [3..5] sigma_a (one channel each), [6] g, [7+k] albedo k. */
+#define ADJ_N (7 + PT_ADJ_MAX_ALBEDO)
+typedef struct { const pt_adjoint_params* p; double delta[3]; double D[ADJ_N]; double G[ADJ_N]; } adj_t;
+static inline int adj_albedo_index(const adj_t* A, int mat) {
+    for (int k = 0; k < A->p->n_albedo; k++) if (A->p->albedo_material[k] == mat) return k;
+    return -1;
+}
+/* d log hg / dg at c = dot(wi, -wo) (volume_scattering/hg/hg.glsl:11-19) */
+static inline double dlog_hg(double g, double c) {
+    double d = 1.0 + g * g - 2.0 * g * c;
+    return -2.0 * g / (1.0 - g * g) - 3.0 * (g - c) / d;
+}
+static inline void adj_scatter(adj_t* A, sp c, const double* D) {
+    double e[3] = {A->delta[0] * (double)c.x, A->delta[1] * (double)c.y, A->delta[2] * (double)c.z};
+    for (int ch = 0; ch < 3; ch++) { A->G[ch] += e[ch] * D[ch]; A->G[3 + ch] += e[ch] * D[3 + ch]; }
+    double es = e[0] + e[1] + e[2];
+    for (int k = 6; k < ADJ_N; k++) A->G[k] += es * D[k];
+}
+
 static sp transport_trace(const or_scene* S, const or_program* P, v3 ro, v3 rd, rng_t* rng, int* n_closest, int* n_any,
-                          const wl_t* wl) {
+                          const wl_t* wl, adj_t* A) {
     sp T = sgrey(1.0f), L = sgrey(0.0f);
     int prev_was_delta = 1; float prev_bsdf_pdf = 0.0f; v3 prev_qp = ro;
     float eta_scale = 1.0f;
@@ -1205,6 +1228,14 @@ static sp transport_trace(const or_scene* S, const or_program* P, v3 ro, v3 rd, 
                 float x0 = rnd(rng), x1 = rnd(rng);
                 msample_t ms = medium_sample_analytic(S, Mm, boundary ? hit.t : MAX_DIST, x0, x1, wl);
                 T = smul(T, ms.weight);
+                if (A && med_mat == A->p->medium_material) {   /* f = sigma_s e^{-sigma_t t} | e^{-sigma_t t_max} */
+                    const float* ssv = Mm->sigma_s;
+                    for (int ch = 0; ch < 3; ch++) {
+                        /* a channel with sigma_s = 0 scatters nothing (f = 0): no finite dlog f */
+                        A->D[ch] += (ms.scattered && ssv[ch] > 0.0f ? 1.0 / (double)ssv[ch] : 0.0) - (double)ms.t;
+                        A->D[3 + ch] -= (double)ms.t;
+                    }
+                }
                 if (ms.scattered) {
                     if (bounce == P->max_bounces) break;
                     v3 p_evt = add(ro, muls(rd, ms.t));
@@ -1215,8 +1246,10 @@ static sp transport_trace(const or_scene* S, const or_program* P, v3 ro, v3 rd, 
                         lsample_t ls = lighting_sample(S, p_evt, y0, y1, wl);
                         if (ls.pdf > 0.0f) {
                             v3 light_p = add(p_evt, muls(ls.wi, ls.distance));
+                            double sh_len = 0.0;
                             sp vis = shadow_transmittance_media(S, p_evt, ls.wi, light_p,
-                                                                S->has_null ? P->max_null - null_crossings : 0, n_any, wl);
+                                                                S->has_null ? P->max_null - null_crossings : 0, n_any, wl,
+                                                                A ? A->p->medium_material : -2, A ? &sh_len : NULL);
                             if (!s_black(wl, vis)) {
                                 float f = hg_value(ls.wi, wo_med, g);
                                 float w = 1.0f;
@@ -1225,6 +1258,13 @@ static sp transport_trace(const or_scene* S, const or_program* P, v3 ro, v3 rd, 
                                 c = smul(c, vis);
                                 c = smuls(c, w);
                                 L = sadd(L, sdivs(c, ls.pdf));
+                                if (A) {
+                                    double Dl[ADJ_N];
+                                    memcpy(Dl, A->D, sizeof Dl);
+                                    if (med_mat == A->p->medium_material) Dl[6] += dlog_hg(g, dot(ls.wi, neg(wo_med)));
+                                    for (int ch = 0; ch < 6; ch++) Dl[ch] -= sh_len;
+                                    adj_scatter(A, sdivs(c, ls.pdf), Dl);
+                                }
                             }
                         }
                     }
@@ -1232,6 +1272,7 @@ static sp transport_trace(const or_scene* S, const or_program* P, v3 ro, v3 rd, 
                     float z0 = rnd(rng), z1 = rnd(rng);
                     float ppdf;
                     v3 wi = hg_sample(wo_med, g, z0, z1, &ppdf, wl);
+                    if (A && med_mat == A->p->medium_material) A->D[6] += dlog_hg(g, dot(wi, neg(wo_med)));
                     prev_was_delta = 0; prev_bsdf_pdf = ppdf; prev_qp = p_evt;
                     ro = p_evt; rd = wi;
                     /* roulette_interior with medium_survival = sigma_s / sigma_t (pt.ts:230-283) */
@@ -1258,6 +1299,7 @@ static sp transport_trace(const or_scene* S, const or_program* P, v3 ro, v3 rd, 
             }
             sp Le = env_radiance(S, rd);
             L = sadd(L, smul(smuls(T, w), Le));
+            if (A) adj_scatter(A, smul(smuls(T, w), Le), A->D);
             break;
         }
         if (media && hit.region_from != current_medium) current_medium = hit.region_from;   /* self-heal */
@@ -1285,6 +1327,7 @@ static sp transport_trace(const or_scene* S, const or_program* P, v3 ro, v3 rd, 
             }
             const pt_material* E = &S->materials[mat_emit];
             L = sadd(L, smul(smuls(T, w), lift(S, E->emission, &E->s_emission, wl)));
+            if (A) adj_scatter(A, smul(smuls(T, w), lift(S, E->emission, &E->s_emission, wl)), A->D);
         }
         if (bounce == P->max_bounces) break;
         /* light_sample_direct (techniques/light/light.glsl:8-28) */
@@ -1296,7 +1339,9 @@ static sp transport_trace(const or_scene* S, const or_program* P, v3 ro, v3 rd, 
                 v3 so = ray_spawn_origin(&hit, ls.wi);
                 v3 sd = normalize(sub(light_p, so));
                 sp vis;
-                if (media) vis = shadow_transmittance_media(S, so, sd, light_p, S->has_null ? P->max_null - null_crossings : 0, n_any, wl);
+                double sh_len = 0.0;
+                if (media) vis = shadow_transmittance_media(S, so, sd, light_p, S->has_null ? P->max_null - null_crossings : 0, n_any, wl,
+                                                            A ? A->p->medium_material : -2, A ? &sh_len : NULL);
                 else {
                     float maxDist = length3(sub(light_p, so)) - SHADOW_BACKOFF;
                     (*n_any)++;
@@ -1313,6 +1358,14 @@ static sp transport_trace(const or_scene* S, const or_program* P, v3 ro, v3 rd, 
                     c = smuls(c, w);
                     c = sdivs(c, ls.pdf);
                     L = sadd(L, c);
+                    if (A) {
+                        double Dl[ADJ_N];
+                        memcpy(Dl, A->D, sizeof Dl);
+                        int ak = M->model == PT_MAT_LAMBERT ? adj_albedo_index(A, mat) : -1;
+                        if (ak >= 0) Dl[7 + ak] += 1.0 / (double)M->albedo[0];
+                        for (int ch = 0; ch < 6; ch++) Dl[ch] -= sh_len;
+                        adj_scatter(A, c, Dl);
+                    }
                 }
             }
         }
@@ -1321,6 +1374,10 @@ static sp transport_trace(const or_scene* S, const or_program* P, v3 ro, v3 rd, 
         isample_t bs = surface_sample(S, M, wo, &hit, uc, ux, uy, wl);
         if (s_black(wl, bs.weight)) break;
         T = smul(T, bs.weight);
+        if (A && M->model == PT_MAT_LAMBERT) {
+            int ak = adj_albedo_index(A, mat);
+            if (ak >= 0) A->D[7 + ak] += 1.0 / (double)M->albedo[0];
+        }
         prev_was_delta = (bs.flags & LOBE_DELTA) != 0; prev_bsdf_pdf = bs.pdf; prev_qp = hit.p;
         if (media && (bs.flags & LOBE_TRANSMISSION)) current_medium = hit.region_to;
         if (S->has_transmission && (bs.flags & LOBE_TRANSMISSION)) {
@@ -1511,7 +1568,7 @@ void or_render(const or_scene* S, const pt_program* prog, const pt_params* par, 
                 float ndcy = (2.0f * filmy / (float)H) - 1.0f;
                 ndcx *= par->aspect;
                 v3 dir = normalize(add(add(F, muls(R, ndcx * par->tan_fov)), muls(U, ndcy * par->tan_fov)));
-                sp Ls = transport_trace(S, &P, O, dir, &rng, &c1, &c2, &wl);
+                sp Ls = transport_trace(S, &P, O, dir, &rng, &c1, &c2, &wl, NULL);
                 v3 Lr = V(Ls.x, Ls.y, Ls.z);
                 if (P.spectral) {                   /* color_to_pixel (color/spectral/spectral.glsl:27-33) */
                     v3 pix = V(0.0f, 0.0f, 0.0f);
@@ -1601,4 +1658,47 @@ void or_env_sample(const or_scene* S, const float* xi, int n, float* dirs, float
         lsample_t ls = environment_sample(S, xi[2 * i], xi[2 * i + 1]);
         dirs[3 * i] = ls.wi.x; dirs[3 * i + 1] = ls.wi.y; dirs[3 * i + 2] = ls.wi.z; pdf[i] = ls.pdf;
     }
+}
+
+/* FP64 replay of the path-replay adjoint: the forward paths of or_render (identical fp32 walk,
+ * sampleCount 0..nframes-1), with D and G accumulated in double. dL [H][W][4]; grad
+ * [PT_ADJ_N_PARAMS]. Rows are summed in order, so the result is deterministic. */
+void or_adjoint(const or_scene* S, const pt_program* prog, const pt_params* par, int nframes,
+                const pt_adjoint_params* adj, const float* dL, double* grad) {
+    or_program P;
+    P.estimator = prog->estimator; P.max_bounces = prog->max_bounces; P.rr_start = prog->rr_start;
+    P.rr_max = prog->rr_max_survival; P.width = prog->width; P.height = prog->height;
+    P.max_null = prog->max_null_crossings; P.spectral = 0; P.wl_uniform = 0;
+    const int W = P.width, H = P.height;
+    double* rows = (double*)calloc((size_t)H * ADJ_N, sizeof(double));
+    v3 F = lv(par->cam_forward), R = lv(par->cam_right), U = lv(par->cam_up), O = lv(par->cam_position);
+    #pragma omp parallel for schedule(dynamic, 1)
+    for (int y = 0; y < H; y++) {
+        double* gr = rows + (size_t)y * ADJ_N;
+        for (int x = 0; x < W; x++) {
+            adj_t A; memset(&A, 0, sizeof A); A.p = adj;
+            const float* d = dL + 4 * ((size_t)y * W + x);
+            for (int c = 0; c < 3; c++) A.delta[c] = (double)d[c] / (double)nframes;
+            for (int f = 0; f < nframes; f++) {
+                int c1 = 0, c2 = 0;
+                float fx = (float)x + 0.5f + (float)par->pixel_offset[0];
+                float fy = (float)y + 0.5f + (float)par->pixel_offset[1];
+                rng_t rng; rng_init(&rng, (uint32_t)fx, (uint32_t)fy, (uint32_t)f, par->reset_salt);
+                float j0 = rnd(&rng), j1 = rnd(&rng);
+                float filmx = fx + (j0 - 0.5f), filmy = fy + (j1 - 0.5f);
+                (void)rnd(&rng); (void)rnd(&rng);
+                float ndcx = (2.0f * filmx / (float)W) - 1.0f;
+                float ndcy = (2.0f * filmy / (float)H) - 1.0f;
+                ndcx *= par->aspect;
+                v3 dir = normalize(add(add(F, muls(R, ndcx * par->tan_fov)), muls(U, ndcy * par->tan_fov)));
+                wl_t wl; memset(&wl, 0, sizeof wl);
+                memset(A.D, 0, sizeof A.D);
+                (void)transport_trace(S, &P, O, dir, &rng, &c1, &c2, &wl, &A);
+            }
+            for (int k = 0; k < ADJ_N; k++) gr[k] += A.G[k];
+        }
+    }
+    for (int k = 0; k < ADJ_N; k++) grad[k] = 0.0;
+    for (int y = 0; y < H; y++) for (int k = 0; k < ADJ_N; k++) grad[k] += rows[(size_t)y * ADJ_N + k];
+    free(rows);
 }

@@ -12,7 +12,8 @@ LIB_PATH = os.path.join(HERE, "libptmi.so")
 DECLARED = ["pt_last_error", "pt_ctx_create", "pt_ctx_destroy", "pt_scene_upload", "pt_renderer_load",
             "pt_renderer_unload", "pt_render_frames", "pt_read_export", "pt_clear_accumulation",
             "pt_sample_count", "pt_trace_closest", "pt_trace_any", "pt_trace_closest_device",
-            "pt_trace_any_device", "pt_set_timing", "pt_get_timings", "pt_last_ray_counts", "pt_finish"]
+            "pt_trace_any_device", "pt_set_timing", "pt_get_timings", "pt_last_ray_counts", "pt_finish",
+            "pt_scene_update_materials", "pt_forward", "pt_backward"]
 
 _lib = None
 
@@ -57,6 +58,9 @@ def load():
     lib.pt_last_traversal_counts.argtypes = [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
     lib.pt_trace_closest_device_counted.argtypes = [vp, vp, u32, vp, vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
     lib.pt_scene_stats.argtypes = [vp, C.POINTER(u32), C.POINTER(u32), C.POINTER(u32)]
+    lib.pt_scene_update_materials.argtypes = [vp, vp, i32]
+    lib.pt_forward.argtypes = [vp, C.POINTER(abi.PtParams), u32]
+    lib.pt_backward.argtypes = [vp, C.POINTER(abi.PtParams), u32, C.POINTER(abi.PtAdjointParams), vp, vp]
     lib.pt_abi_sizeof.argtypes = [C.c_int]
     lib.pt_abi_sizeof.restype = C.c_size_t
     _ = (i32, st)

@@ -76,6 +76,26 @@ class PtParams(C.Structure):
                 ("pad0", C.c_uint32), ("pixel_offset", C.c_int32 * 2), ("pad1", C.c_int32 * 2)]
 
 
+ADJ_MAX_ALBEDO = 16
+ADJ_N_PARAMS = 7 + ADJ_MAX_ALBEDO
+REDUCE_ATOMIC, REDUCE_DETERMINISTIC = 0, 1
+
+
+class PtAdjointParams(C.Structure):
+    _fields_ = [("medium_material", C.c_int32), ("n_albedo", C.c_int32),
+                ("albedo_material", C.c_int32 * ADJ_MAX_ALBEDO), ("reduce_mode", C.c_int32), ("pad", C.c_int32)]
+
+    @classmethod
+    def make(cls, medium_material, albedo_materials, reduce_mode=REDUCE_DETERMINISTIC):
+        a = cls()
+        a.medium_material = int(medium_material)
+        a.n_albedo = len(albedo_materials)
+        for k, m in enumerate(albedo_materials):
+            a.albedo_material[k] = int(m)
+        a.reduce_mode = int(reduce_mode)
+        return a
+
+
 class PtKernelTiming(C.Structure):
     _fields_ = [("name", C.c_char * 32), ("total_ms", C.c_float), ("launches", C.c_uint32),
                 ("items", C.c_uint64)]

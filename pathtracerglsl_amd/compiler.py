@@ -46,6 +46,7 @@ class CompiledRenderer:
     camera: dict
     reset_salt: int = 1234
     warnings: list = field(default_factory=list)
+    material_names: list = field(default_factory=list)   # authored material of each record index
 
     def params(self, reset_salt=None):
         return camera_params(self.camera, self.width, self.height,
@@ -278,4 +279,4 @@ def compile_scene(scene, strategy=None, width=256, height=256, reset_salt=1234):
     prog.max_null_crossings = strategy.max_null_crossings
     return CompiledRenderer(id=f"{strategy.color}-{strategy.direct_lighting}-{scene.get('id', 'scene')}", scene=arrays,
                             program=prog, width=width, height=height, camera=scene["camera"],
-                            reset_salt=reset_salt)
+                            reset_salt=reset_salt, material_names=list(mat_names))

@@ -67,7 +67,69 @@ struct Paths {
     float* eta;      // eta_scale (spectral; RGB keeps it in thr.w)
     unsigned long long* hist;   // [0] closest queries, [1] any-hit queries
     unsigned long long* trav;   // [0] nodes visited, [1] triangles tested (COUNT builds)
+    // path-replay adjoint (ADJ kernels): SoA per parameter, [k * cap + path]
+    float* adj_D;               // dlog f / dtheta of the path so far (sampling detached)
+    float* adj_G;               // the path's gradient
+    float* adj_sD;              // [k * cap + shadow entry]: D at the NEE vertex incl. local terms
+    const float4* adj_delta;    // dL/dI per pixel (times 1/nframes)
+    int cap, npix;
+    int adj_medium;             // material whose medium sigma_s / sigma_a / g are parameters
+    int adj_n_albedo;
+    int adj_albedo[PT_ADJ_MAX_ALBEDO];
 };
+constexpr int ADJ_N = PT_ADJ_N_PARAMS;
+
+// Adjoint helpers. Parameter layout: [0..2] sigma_s (channel c), [3..5] sigma_a (channel c),
+// [6] g, [7 + k] albedo k; a contribution c scatters e = dL/dI * c into G with the channel
+// structure of D (sigma terms act on one channel, g and albedos on all three).
+// D is loaded once per vertex and written back only when a parameter factor changed it; G is
+// read-modified-written only where a contribution lands (most vertices have none).
+struct AdjState { float D[ADJ_N]; bool dirty; };
+PT_DEV void adj_load(const Paths& P, int i, AdjState& A) {
+#pragma unroll
+    for (int k = 0; k < ADJ_N; k++) A.D[k] = P.adj_D[(size_t)k * P.cap + i];
+    A.dirty = false;
+}
+PT_DEV void adj_store(const Paths& P, int i, const AdjState& A) {
+    if (!A.dirty) return;
+#pragma unroll
+    for (int k = 0; k < ADJ_N; k++) P.adj_D[(size_t)k * P.cap + i] = A.D[k];
+}
+PT_DEV void adj_scatter(float* G, const float* D, f3 e);
+PT_DEV void adj_add_G(const Paths& P, int i, const float* D, f3 e) {
+    float G[ADJ_N];
+#pragma unroll
+    for (int k = 0; k < ADJ_N; k++) G[k] = P.adj_G[(size_t)k * P.cap + i];
+    adj_scatter(G, D, e);
+#pragma unroll
+    for (int k = 0; k < ADJ_N; k++) P.adj_G[(size_t)k * P.cap + i] = G[k];
+}
+PT_DEV void adj_scatter(float* G, const float* D, f3 e) {
+    G[0] += e.x * D[0]; G[1] += e.y * D[1]; G[2] += e.z * D[2];
+    G[3] += e.x * D[3]; G[4] += e.y * D[4]; G[5] += e.z * D[5];
+    const float es = (e.x + e.y) + e.z;
+#pragma unroll
+    for (int k = 6; k < ADJ_N; k++) G[k] += es * D[k];
+}
+PT_DEV f3 adj_e(const Paths& P, int path, sp4 c) {
+    float4 d = P.adj_delta[path % P.npix];
+    return mk3(d.x * c.x, d.y * c.y, d.z * c.z);
+}
+PT_DEV int adj_albedo_index(const Paths& P, int mat) {
+    int r = -1;
+    for (int k = 0; k < P.adj_n_albedo; k++) if (P.adj_albedo[k] == mat) r = k;
+    return r;
+}
+// add v to D[7 + k] without a run-time register index
+PT_DEV void adj_add_albedo(float* D, int k, float v) {
+#pragma unroll
+    for (int kk = 0; kk < PT_ADJ_MAX_ALBEDO; kk++) if (kk == k) D[7 + kk] += v;
+}
+// d log hg / dg at c = dot(wi, -wo) (volume_scattering/hg/hg.glsl:11-19)
+PT_DEV float dlog_hg(float g, float c) {
+    float d = 1.0f + g * g - 2.0f * g * c;
+    return -2.0f * g / (1.0f - g * g) - 3.0f * (g - c) / d;
+}
 
 PT_DEV int fl_bounce(int f) { return (f >> 16) & 0xFFFF; }
 PT_DEV int fl_nulls(int f) { return (f >> 8) & 0xFF; }
@@ -106,7 +168,7 @@ __global__ void __launch_bounds__(64) k_camera_medium(DevScene S, Paths P, Camer
     if (threadIdx.x == 0) P.counts[C_CAM_MEDIUM] = region_at<64>(S, mk3(C.ox, C.oy, C.oz), stk);
 }
 
-template <bool SPEC>
+template <bool SPEC, bool ADJ>
 __global__ void __launch_bounds__(SBLOCK) k_raygen(Paths P, Camera C, int W, int H, int n_paths, uint32_t sample0, int media, int wl_uniform) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) P.counts[C_RAYQ0] = n_paths;
@@ -144,6 +206,10 @@ __global__ void __launch_bounds__(SBLOCK) k_raygen(Paths P, Camera C, int W, int
     P.flags[i] = mk_flags(true, 0, 0);
     if (media) P.medium[i] = P.counts[C_CAM_MEDIUM];
     P.q[0][i] = i;
+    if (ADJ) {
+#pragma unroll
+        for (int k = 0; k < ADJ_N; k++) { P.adj_D[(size_t)k * P.cap + i] = 0.0f; P.adj_G[(size_t)k * P.cap + i] = 0.0f; }
+    }
 }
 
 // Persistent-lane work distribution: each wave owns a chunk of queue entries and hands them
@@ -228,7 +294,7 @@ __global__ void __launch_bounds__(TBLOCK) k_extend(DevScene S, Paths P, int qsel
 
 // shadow (opaque scenes): any hit for every NEE ray; unoccluded contributions are added to
 // their path as ((a * vis) * w) / pdf, the reference's operation order with vis = 1.
-template <int STACK, bool SPEC>
+template <int STACK, bool SPEC, bool ADJ>
 __global__ void __launch_bounds__(TBLOCK) k_shadow(DevScene S, Paths P) {
     __shared__ uint2 stk[STACK * TBLOCK];
     const int n = P.counts[C_SHADOW];
@@ -246,7 +312,14 @@ __global__ void __launch_bounds__(TBLOCK) k_shadow(DevScene S, Paths P) {
             float4 c = P.s_c[k];
             const float w = SPEC ? P.s_l[k].w : c.w;
             sp4 a = SPEC ? ld4(c) : mk4(c.x, c.y, c.z, 0.0f);
-            store_rad<SPEC>(P, i, load_rad<SPEC>(P, i) + (a * w) / d4.w);
+            const sp4 v = (a * w) / d4.w;
+            store_rad<SPEC>(P, i, load_rad<SPEC>(P, i) + v);
+            if (ADJ) {
+                float D[ADJ_N];
+#pragma unroll
+                for (int kk = 0; kk < ADJ_N; kk++) D[kk] = P.adj_sD[(size_t)kk * P.cap + k];
+                adj_add_G(P, i, D, adj_e(P, i, v));
+            }
         }
     }
 }
@@ -254,7 +327,7 @@ __global__ void __launch_bounds__(TBLOCK) k_shadow(DevScene S, Paths P) {
 // shadow (media scenes): the segment walker of transport/shadow/media/media.glsl:29-62 —
 // closest hits up to the light point, Beer-Lambert per segment in the current medium, pass
 // through null interfaces, anything else blocks (dielectrics are opaque to shadow rays).
-template <int STACK, bool SPEC>
+template <int STACK, bool SPEC, bool ADJ>
 __global__ void __launch_bounds__(TBLOCK) k_shadow_media(DevScene S, Paths P) {
     __shared__ uint2 stk[STACK * TBLOCK];
     const int n = P.counts[C_SHADOW];
@@ -267,6 +340,7 @@ __global__ void __launch_bounds__(TBLOCK) k_shadow_media(DevScene S, Paths P) {
         const int crossings_left = __float_as_int(o4.w);
         const Wl wl = load_wl<SPEC>(P, P.s_path[k]);
         sp4 T = sgrey(1.0f);
+        float len_param = 0.0f;     // length travelled inside the parameter medium
         bool blocked = true;
         int medium = region_at<TBLOCK>(S, o, stk);
         for (int crossed = 0; crossed <= crossings_left; crossed++) {
@@ -279,6 +353,7 @@ __global__ void __launch_bounds__(TBLOCK) k_shadow_media(DevScene S, Paths P) {
             float seg_len = hit_boundary ? tmax : remaining;
             int med_mat = material_of(S, medium);
             if (material_has_medium(S, med_mat)) T = T * medium_transmittance<SPEC>(S, S.materials[med_mat], seg_len, wl);
+            if (ADJ && med_mat == P.adj_medium && material_has_medium(S, med_mat)) len_param += seg_len;
             if (!hit_boundary) { blocked = false; break; }
             Hit h; bool back;
             fill_hit(S, o, d, tmax, prim, h, back);
@@ -294,7 +369,16 @@ __global__ void __launch_bounds__(TBLOCK) k_shadow_media(DevScene S, Paths P) {
             float4 c = P.s_c[k];
             const float w = SPEC ? l4.w : c.w;
             sp4 a = SPEC ? ld4(c) : mk4(c.x, c.y, c.z, 0.0f);
-            store_rad<SPEC>(P, i, load_rad<SPEC>(P, i) + ((a * T) * w) / d4.w);
+            const sp4 v = ((a * T) * w) / d4.w;
+            store_rad<SPEC>(P, i, load_rad<SPEC>(P, i) + v);
+            if (ADJ) {
+                float D[ADJ_N];
+#pragma unroll
+                for (int kk = 0; kk < ADJ_N; kk++) D[kk] = P.adj_sD[(size_t)kk * P.cap + k];
+#pragma unroll
+                for (int kk = 0; kk < 6; kk++) D[kk] -= len_param;     // e^{-sigma_t len} along the shadow ray
+                adj_add_G(P, i, D, adj_e(P, i, v));
+            }
         }
     }
 }
@@ -315,7 +399,7 @@ PT_DEV int block_append(int* const* counters, int which, int* s_cnt, int* s_base
 // bin: per path of the iteration, the medium segment first (media scenes: the analytic
 // channel-MIS distance sample of pt.ts's media branch, which may end the segment in a
 // scattering event), then the shading category of the vertex.
-template <bool MEDIA, bool SPEC>
+template <bool MEDIA, bool SPEC, bool ADJ>
 __global__ void __launch_bounds__(SBLOCK) k_bin(DevScene S, Paths P, int qsel) {
     __shared__ int s_cnt[N_CAT], s_base[N_CAT];
     int* ctr[N_CAT];
@@ -340,6 +424,15 @@ __global__ void __launch_bounds__(SBLOCK) k_bin(DevScene S, Paths P, int qsel) {
                     sp4 T; float eta;
                     load_thr<SPEC>(P, path, T, eta);
                     store_thr<SPEC>(P, path, T * ms.weight, eta);
+                    if (ADJ && med_mat == P.adj_medium) {   // f = sigma_s e^{-sigma_t t} | e^{-sigma_t t_max}
+                        const float* ssv = S.materials[med_mat].sigma_s;
+#pragma unroll
+                        for (int ch = 0; ch < 3; ch++) {
+                            const size_t ks = (size_t)ch * P.cap + path, ka = (size_t)(3 + ch) * P.cap + path;
+                            P.adj_D[ks] += (ms.scattered && ssv[ch] > 0.0f ? 1.0f / ssv[ch] : 0.0f) - ms.t;
+                            P.adj_D[ka] -= ms.t;
+                        }
+                    }
                     P.rng[path] = make_uint2(rng.base, rng.dim);
                     if (ms.scattered) { cat = CAT_MEDIUM; P.hit[path] = make_float2(ms.t, hv.y); }
                 }
@@ -377,7 +470,7 @@ __global__ void k_bookkeep(Paths P, int cur, int next, int first) {
 // (transport/combiner.ts:23-102) and the kernel/light techniques (techniques/kernel/kernel.glsl,
 // kernel_phase.glsl, techniques/light/light.glsl, light_medium.glsl), for the paths of one
 // category. MEDIA selects the media program (region tracking, medium NEE shadow walker).
-template <int EST, int CAT, int STACK, bool MEDIA, bool SPEC>
+template <int EST, int CAT, int STACK, bool MEDIA, bool SPEC, bool ADJ>
 __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene S, Paths P, int qsel, int max_bounces,
                                                   int rr_start, float rr_max, int max_null) {
     constexpr bool PROBE = CAT == CAT_DIELECTRIC || MEDIA;
@@ -399,7 +492,11 @@ __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene S, Paths P, int qsel,
         sp4 s_a = sgrey(0.0f);
         float s_max = 0.0f, s_w = 0.0f, s_pdf = 1.0f;
         int s_cross = 0;
+        AdjState A;
+        float sD[ADJ_N];
         if (valid) {
+            if (ADJ && (CAT != CAT_MISS || S.env_type != PT_ENV_NONE)) adj_load(P, i, A);
+            else A.dirty = false;
             f3 rd = xyz(P.ray_d[i]);
             sp4 T; float eta_scale;
             load_thr<SPEC>(P, i, T, eta_scale);
@@ -421,12 +518,18 @@ __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene S, Paths P, int qsel,
                     else if (MIS) w = power_heuristic(prev_pdf, env_select_live(S) ? S.env_select_prob * environment_pdf(S, rd) : environment_pdf(S, rd));
                     else w = 0.0f;
                 }
-                if (S.env_type != PT_ENV_NONE) L = L + (T * w) * env_radiance(S, rd);
+                if (S.env_type != PT_ENV_NONE) {
+                    const sp4 c = (T * w) * env_radiance(S, rd);
+                    L = L + c;
+                    if (ADJ) adj_add_G(P, i, A.D, adj_e(P, i, c));
+                }
             } else if (CAT == CAT_MEDIUM) {
                 // ---- medium event (pt.ts media branch: scattered) ----
                 if (bounce < max_bounces) {
-                    const pt_material& Mm = S.materials[material_of(S, medium)];
+                    const int mmat = material_of(S, medium);
+                    const pt_material& Mm = S.materials[mmat];
                     const float g = Mm.hg_g;
+                    const bool param_medium = ADJ && mmat == P.adj_medium;
                     f3 ro = xyz(P.ray_o[i]);
                     const float t = P.hit[i].x;
                     f3 p_evt = ro + rd * t;
@@ -445,12 +548,18 @@ __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene S, Paths P, int qsel,
                             s_a = (T * ls.radiance) * sgrey(f);
                             s_pdf = ls.pdf;
                             shadow = true;
+                            if (ADJ) {
+#pragma unroll
+                                for (int kk = 0; kk < ADJ_N; kk++) sD[kk] = A.D[kk];
+                                if (param_medium) sD[6] += dlog_hg(g, dot(ls.wi, -wo_med));
+                            }
                         }
                     }
                     // kernel_sample_phase
                     float z0 = rng.next(), z1 = rng.next();
                     float ppdf;
                     f3 wi = hg_sample<SPEC>(wo_med, g, z0, z1, ppdf);
+                    if (param_medium) { A.D[6] += dlog_hg(g, dot(wi, -wo_med)); A.dirty = true; }
                     cont = true;
                     // roulette_interior with medium_survival = sigma_s / max(sigma_t, 1e-9)
                     if (rr_start >= 0 && bounce >= rr_start) {
@@ -508,7 +617,9 @@ __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene S, Paths P, int qsel,
                                 else if (MIS) w = power_heuristic(prev_pdf, lighting_pdf(S, prev_qp, rd, lid, h.p));
                                 else w = 0.0f;
                             }
-                            L = L + (T * w) * lift<SPEC>(S, E.emission, E.s_emission, wl);
+                            const sp4 c = (T * w) * lift<SPEC>(S, E.emission, E.s_emission, wl);
+                            L = L + c;
+                            if (ADJ) adj_add_G(P, i, A.D, adj_e(P, i, c));
                         }
                     }
                     if (bounce < max_bounces) {
@@ -530,6 +641,14 @@ __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene S, Paths P, int qsel,
                                 s_a = ((T * ls.radiance) * f) * cos_i;
                                 s_pdf = ls.pdf;
                                 shadow = true;
+                                if (ADJ) {
+#pragma unroll
+                                    for (int kk = 0; kk < ADJ_N; kk++) sD[kk] = A.D[kk];
+                                    if (CAT == CAT_LAMBERT) {
+                                        const int ak = adj_albedo_index(P, S.objects[h.owner].material);
+                                        if (ak >= 0) adj_add_albedo(sD, ak, 1.0f / M.albedo[0]);
+                                    }
+                                }
                             }
                         }
                         // kernel_sample_continuation
@@ -537,6 +656,10 @@ __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene S, Paths P, int qsel,
                         ISample bs = surface_sample<MODEL, SPEC>(S, M, wo, h, uc, ux, uy, wl);
                         if (!is_black<SPEC>(bs.weight)) {
                             T = T * bs.weight;
+                            if (ADJ && CAT == CAT_LAMBERT) {
+                                const int ak = adj_albedo_index(P, S.objects[h.owner].material);
+                                if (ak >= 0) { adj_add_albedo(A.D, ak, 1.0f / M.albedo[0]); A.dirty = true; }
+                            }
                             prev_delta = (bs.flags & LOBE_DELTA) != 0;
                             prev_pdf = bs.pdf;
                             prev_qp = h.p;
@@ -569,6 +692,7 @@ __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene S, Paths P, int qsel,
                 }
             }
             store_rad<SPEC>(P, i, L);
+            if (ADJ) adj_store(P, i, A);
         }
         int slot = block_append<2>(ctr, cont ? 0 : -1, s_cnt, s_base);
         if (cont) qn[slot] = i;
@@ -579,7 +703,52 @@ __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene S, Paths P, int qsel,
             P.s_d[sslot] = f4(s_d, s_pdf);
             P.s_c[sslot] = SPEC ? st4(s_a) : make_float4(s_a.x, s_a.y, s_a.z, s_w);
             if (MEDIA || SPEC) P.s_l[sslot] = f4(s_l, s_w);
+            if (ADJ) {
+#pragma unroll
+                for (int kk = 0; kk < ADJ_N; kk++) P.adj_sD[(size_t)kk * P.cap + sslot] = sD[kk];
+            }
         }
+    }
+}
+
+// Adjoint reductions. Deterministic: per pixel, the wave's paths are added in sample order
+// into a per-pixel partial; after the last wave one block per parameter sums the partials
+// in a fixed order. Atomic: block partials (fixed-order LDS tree) are added to the gradient
+// with fp32 atomics.
+__global__ void __launch_bounds__(SBLOCK) k_adj_pixel(const Paths P, float* pix_grad, int frames) {
+    const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= P.npix) return;
+    for (int k = 0; k < ADJ_N; k++) {
+        float acc = pix_grad[(size_t)k * P.npix + pix];
+        for (int f = 0; f < frames; f++) acc += P.adj_G[(size_t)k * P.cap + (size_t)f * P.npix + pix];
+        pix_grad[(size_t)k * P.npix + pix] = acc;
+    }
+}
+__global__ void __launch_bounds__(SBLOCK) k_adj_final(const float* pix_grad, int npix, float* grad) {
+    __shared__ float red[SBLOCK];
+    const int k = blockIdx.x;
+    float acc = 0.0f;
+    for (int p = threadIdx.x; p < npix; p += SBLOCK) acc += pix_grad[(size_t)k * npix + p];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = SBLOCK / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) grad[k] = red[0];
+}
+__global__ void __launch_bounds__(SBLOCK) k_adj_atomic(const Paths P, int n_paths, float* grad) {
+    __shared__ float red[SBLOCK];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    for (int k = 0; k < ADJ_N; k++) {
+        red[threadIdx.x] = i < n_paths ? P.adj_G[(size_t)k * P.cap + i] : 0.0f;
+        __syncthreads();
+        for (int w = SBLOCK / 2; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) atomicAdd(&grad[k], red[0]);
+        __syncthreads();
     }
 }
 
@@ -695,6 +864,7 @@ struct pt_ctx {
     int persist_blocks = 2048;   // persistent traversal grid (set from the device's CU count)
     int shade_blocks = 2048;
     bool has_ggx = false;
+    std::vector<pt_material> host_materials;   // the uploaded records (adjoint checks, updates)
     bool timing = false;
     std::map<std::string, TimingSlot> timings;
     std::vector<std::string> timing_order;
@@ -737,6 +907,10 @@ struct pt_renderer {
     long long cap = 0;    // path slots allocated
     DevBuf b_ray_o, b_ray_d, b_thr, b_rad, b_rng, b_prev, b_flags, b_medium, b_hit, b_q0, b_q1, b_counts,
         b_spath, b_so, b_sd, b_sc, b_sl, b_wl, b_eta, b_hist, b_trav, b_cat[N_CAT];
+    // adjoint buffers (allocated by the first pt_backward)
+    DevBuf b_adjD, b_adjG, b_adjsD, b_delta, b_pixgrad, b_grad;
+    long long adj_cap = 0;
+    int adj_mode = PT_REDUCE_DETERMINISTIC;
     Paths paths{};
     bool count_traversal = false;
 };
@@ -845,6 +1019,7 @@ pt_status pt_scene_upload(pt_ctx* ctx, const pt_scene_desc* sd) {
     if ((st = ctx->upload(analytic.data(), analytic.size(), &D.analytic)) != PT_OK) return st;
     D.n_analytic = (int)analytic.size();
     if ((st = ctx->upload(sd->materials, (size_t)sd->n_materials, &D.materials)) != PT_OK) return st;
+    ctx->host_materials.assign(sd->materials, sd->materials + sd->n_materials);
     if ((st = ctx->upload(sd->lights, (size_t)sd->n_lights, &D.lights)) != PT_OK) return st;
     D.n_lights = sd->n_lights;
     ctx->has_ggx = false;
@@ -1044,11 +1219,13 @@ static pt_status ensure_paths(pt_renderer* r, long long n) {
     P.medium = (int*)r->b_medium.p; P.wl = (float4*)r->b_wl.p; P.eta = (float*)r->b_eta.p; P.hist = (unsigned long long*)r->b_hist.p;
     P.trav = (unsigned long long*)r->b_trav.p;
     for (int c = 0; c < N_CAT; c++) P.catq[c] = (int*)r->b_cat[c].p;
+    P.cap = (int)n;
+    P.npix = r->prog.width * r->prog.height;
     r->cap = n;
     return PT_OK;
 }
 
-template <int EST, int STACK, bool MEDIA, bool SPEC>
+template <int EST, int STACK, bool MEDIA, bool SPEC, bool ADJ>
 static void launch_shade(pt_renderer* r, int cur) {
     pt_ctx* ctx = r->ctx;
     const pt_program& pg = r->prog;
@@ -1056,7 +1233,7 @@ static void launch_shade(pt_renderer* r, int cur) {
     hipStream_t st = ctx->stream;
     const int g = ctx->shade_blocks;
     const int mn = pg.max_null_crossings;
-#define PT_SHADE(name, CATV) TIMED(ctx, name, 0, hipLaunchKernelGGL((k_shade<EST, CATV, STACK, MEDIA, SPEC>), dim3(g), dim3(SBLOCK), 0, st, S, r->paths, cur, pg.max_bounces, pg.rr_start, pg.rr_max_survival, mn))
+#define PT_SHADE(name, CATV) TIMED(ctx, name, 0, hipLaunchKernelGGL((k_shade<EST, CATV, STACK, MEDIA, SPEC, ADJ>), dim3(g), dim3(SBLOCK), 0, st, S, r->paths, cur, pg.max_bounces, pg.rr_start, pg.rr_max_survival, mn))
     PT_SHADE("shade_miss", CAT_MISS);
     PT_SHADE("shade_lambert", CAT_LAMBERT);
     if (ctx->has_ggx) PT_SHADE("shade_ggx", CAT_GGX);
@@ -1066,8 +1243,9 @@ static void launch_shade(pt_renderer* r, int cur) {
 #undef PT_SHADE
 }
 
-template <int STACK, bool MEDIA, bool SPEC>
+template <int STACK, bool MEDIA, bool SPEC, bool ADJ>
 static void launch_wave(pt_renderer* r, const Camera& cam, int frames, uint32_t sample0) {
+    static_assert(!(SPEC && ADJ), "the adjoint runs RGB programs");
     pt_ctx* ctx = r->ctx;
     const DevScene& S = ctx->dev;
     Paths& P = r->paths;
@@ -1076,7 +1254,7 @@ static void launch_wave(pt_renderer* r, const Camera& cam, int frames, uint32_t 
     const long long n = (long long)W * H * frames;
     hipStream_t st = ctx->stream;
     if (MEDIA) hipLaunchKernelGGL((k_camera_medium<STACK>), dim3(1), dim3(64), 0, st, S, P, cam);
-    TIMED(ctx, "raygen", n, hipLaunchKernelGGL((k_raygen<SPEC>), dim3((unsigned)((n + SBLOCK - 1) / SBLOCK)), dim3(SBLOCK), 0, st, P, cam, W, H, (int)n, sample0, MEDIA ? 1 : 0, pg.wavelengths));
+    TIMED(ctx, "raygen", n,  hipLaunchKernelGGL((k_raygen<SPEC, ADJ>), dim3((unsigned)((n + SBLOCK - 1) / SBLOCK)), dim3(SBLOCK), 0, st, P, cam, W, H, (int)n, sample0, MEDIA ? 1 : 0, pg.wavelengths));
     int cur = 0;
     const int tgrid = ctx->persist_blocks;
     // a null crossing consumes an iteration but no bounce (pt.ts null-interface branch)
@@ -1087,21 +1265,87 @@ static void launch_wave(pt_renderer* r, const Camera& cam, int frames, uint32_t 
             TIMED(ctx, "extend_count", 0, hipLaunchKernelGGL((k_extend<STACK, true>), dim3(tgrid), dim3(TBLOCK), 0, st, S, P, cur));
         else
             TIMED(ctx, "extend", 0, hipLaunchKernelGGL((k_extend<STACK, false>), dim3(tgrid), dim3(TBLOCK), 0, st, S, P, cur));
-        TIMED(ctx, "bin", 0,  hipLaunchKernelGGL((k_bin<MEDIA, SPEC>), dim3(ctx->shade_blocks), dim3(SBLOCK), 0, st, S, P, cur));
+        TIMED(ctx, "bin", 0,   hipLaunchKernelGGL((k_bin<MEDIA, SPEC, ADJ>), dim3(ctx->shade_blocks), dim3(SBLOCK), 0, st, S, P, cur));
         switch (pg.estimator) {
-            case PT_EST_PT: launch_shade<PT_EST_PT, STACK, MEDIA, SPEC>(r, cur); break;
-            case PT_EST_NEE: launch_shade<PT_EST_NEE, STACK, MEDIA, SPEC>(r, cur); break;
-            default: launch_shade<PT_EST_MIS, STACK, MEDIA, SPEC>(r, cur);
+            case PT_EST_PT: launch_shade<PT_EST_PT, STACK, MEDIA, SPEC, ADJ>(r, cur); break;
+            case PT_EST_NEE: launch_shade<PT_EST_NEE, STACK, MEDIA, SPEC, ADJ>(r, cur); break;
+            default: launch_shade<PT_EST_MIS, STACK, MEDIA, SPEC, ADJ>(r, cur);
         }
         if (pg.estimator != PT_EST_PT) {
-            if (MEDIA) TIMED(ctx, "shadow_media", 0, hipLaunchKernelGGL((k_shadow_media<STACK, SPEC>), dim3(tgrid), dim3(TBLOCK), 0, st, S, P));
-            else TIMED(ctx, "shadow", 0, hipLaunchKernelGGL((k_shadow<STACK, SPEC>), dim3(tgrid), dim3(TBLOCK), 0, st, S, P));
+            if (MEDIA) TIMED(ctx, "shadow_media", 0, hipLaunchKernelGGL((k_shadow_media<STACK, SPEC, ADJ>), dim3(tgrid), dim3(TBLOCK), 0, st, S, P));
+            else TIMED(ctx, "shadow", 0, hipLaunchKernelGGL((k_shadow<STACK, SPEC, ADJ>), dim3(tgrid), dim3(TBLOCK), 0, st, S, P));
         }
         cur ^= 1;
     }
     hipLaunchKernelGGL(k_bookkeep, dim3(1), dim3(64), 0, st, P, -1, -1, 0);
     const int npix = W * H;
+    if (ADJ) {
+        if (r->adj_mode == PT_REDUCE_DETERMINISTIC)
+            TIMED(ctx, "adj_reduce", npix, hipLaunchKernelGGL(k_adj_pixel, dim3((npix + SBLOCK - 1) / SBLOCK), dim3(SBLOCK), 0, st, P, (float*)r->b_pixgrad.p, frames));
+        else
+            TIMED(ctx, "adj_reduce", n, hipLaunchKernelGGL(k_adj_atomic, dim3((unsigned)((n + SBLOCK - 1) / SBLOCK)), dim3(SBLOCK), 0, st, P, (int)n, (float*)r->b_grad.p));
+        return;
+    }
     TIMED(ctx, "accumulate", npix, hipLaunchKernelGGL((k_accumulate<SPEC>), dim3((npix + SBLOCK - 1) / SBLOCK), dim3(SBLOCK), 0, st, S, P, (float4*)r->film.p, npix, frames, sample0, pg.wavelengths));
+}
+
+extern "C" {
+
+}  // extern "C"
+
+static Camera make_camera(const pt_params* prm) {
+    Camera cam;
+    cam.ox = prm->cam_position[0]; cam.oy = prm->cam_position[1]; cam.oz = prm->cam_position[2];
+    cam.fx = prm->cam_forward[0]; cam.fy = prm->cam_forward[1]; cam.fz = prm->cam_forward[2];
+    cam.rx = prm->cam_right[0]; cam.ry = prm->cam_right[1]; cam.rz = prm->cam_right[2];
+    cam.ux = prm->cam_up[0]; cam.uy = prm->cam_up[1]; cam.uz = prm->cam_up[2];
+    cam.tan_fov = prm->tan_fov; cam.aspect = prm->aspect; cam.salt = prm->reset_salt;
+    cam.off_x = prm->pixel_offset[0]; cam.off_y = prm->pixel_offset[1];
+    return cam;
+}
+
+// Trace nframes samples (sampleCount sample0, sample0 + 1, ...) in waves of frames_in_flight.
+// The forward program accumulates into the film; the adjoint program (ADJ) replays the same
+// paths and reduces their gradients instead.
+static pt_status run_waves(pt_renderer* r, const pt_params* prm, uint32_t nframes, uint32_t sample0, bool adj) {
+    pt_ctx* ctx = r->ctx;
+    const pt_program& pg = r->prog;
+    const int F = std::max(1, std::min<int>(pg.frames_in_flight, (int)std::max<uint32_t>(nframes, 1)));
+    const bool spectral = pg.color == PT_COLOR_SPECTRAL;
+    if (spectral && !ctx->dev.spec_curves)
+        return fail(PT_E_MISSING_INPUT, "missing input: spectral_curves (the spectral program needs the CIE/D65 curves in pt_scene_desc)");
+    HIP_TRY(hipMemsetAsync(r->b_hist.p, 0, r->b_hist.bytes, ctx->stream));
+    HIP_TRY(hipMemsetAsync(r->b_trav.p, 0, r->b_trav.bytes, ctx->stream));
+    const Camera cam = make_camera(prm);
+    uint32_t done = 0;
+    while (done < nframes) {
+        int f = (int)std::min<uint32_t>((uint32_t)F, nframes - done);
+        const uint32_t s0 = sample0 + done;
+        const int v = (ctx->stack == 16 ? 0 : 4) | (ctx->dev.has_media ? 2 : 0) | (spectral ? 1 : 0);
+        if (adj) {
+            switch (v) {
+                case 0: launch_wave<16, false, false, true>(r, cam, f, s0); break;
+                case 2: launch_wave<16, true, false, true>(r, cam, f, s0); break;
+                case 4: launch_wave<32, false, false, true>(r, cam, f, s0); break;
+                case 6: launch_wave<32, true, false, true>(r, cam, f, s0); break;
+                default: return fail(PT_E_UNSUPPORTED, "the adjoint runs RGB programs only");
+            }
+        } else {
+            switch (v) {
+                case 0: launch_wave<16, false, false, false>(r, cam, f, s0); break;
+                case 1: launch_wave<16, false, true, false>(r, cam, f, s0); break;
+                case 2: launch_wave<16, true, false, false>(r, cam, f, s0); break;
+                case 3: launch_wave<16, true, true, false>(r, cam, f, s0); break;
+                case 4: launch_wave<32, false, false, false>(r, cam, f, s0); break;
+                case 5: launch_wave<32, false, true, false>(r, cam, f, s0); break;
+                case 6: launch_wave<32, true, false, false>(r, cam, f, s0); break;
+                default: launch_wave<32, true, true, false>(r, cam, f, s0);
+            }
+        }
+        HIP_TRY(hipGetLastError());
+        done += (uint32_t)f;
+    }
+    return PT_OK;
 }
 
 extern "C" {
@@ -1113,38 +1357,108 @@ pt_status pt_render_frames(pt_renderer* r, const pt_params* prm, uint32_t nframe
     HIP_TRY(hipSetDevice(ctx->device));
     const pt_program& pg = r->prog;
     const int F = std::max(1, std::min<int>(pg.frames_in_flight, (int)std::max<uint32_t>(nframes, 1)));
-    const bool spectral = pg.color == PT_COLOR_SPECTRAL;
-    if (spectral && !ctx->dev.spec_curves)
-        return fail(PT_E_MISSING_INPUT, "missing input: spectral_curves (the spectral program needs the CIE/D65 curves in pt_scene_desc)");
     pt_status st = ensure_paths(r, (long long)pg.width * pg.height * F);
     if (st != PT_OK) return st;
-    HIP_TRY(hipMemsetAsync(r->b_hist.p, 0, r->b_hist.bytes, ctx->stream));
-    HIP_TRY(hipMemsetAsync(r->b_trav.p, 0, r->b_trav.bytes, ctx->stream));
-    Camera cam;
-    cam.ox = prm->cam_position[0]; cam.oy = prm->cam_position[1]; cam.oz = prm->cam_position[2];
-    cam.fx = prm->cam_forward[0]; cam.fy = prm->cam_forward[1]; cam.fz = prm->cam_forward[2];
-    cam.rx = prm->cam_right[0]; cam.ry = prm->cam_right[1]; cam.rz = prm->cam_right[2];
-    cam.ux = prm->cam_up[0]; cam.uy = prm->cam_up[1]; cam.uz = prm->cam_up[2];
-    cam.tan_fov = prm->tan_fov; cam.aspect = prm->aspect; cam.salt = prm->reset_salt;
-    cam.off_x = prm->pixel_offset[0]; cam.off_y = prm->pixel_offset[1];
-    uint32_t done = 0;
-    while (done < nframes) {
-        int f = (int)std::min<uint32_t>((uint32_t)F, nframes - done);
-        const int v = (ctx->stack == 16 ? 0 : 4) | (ctx->dev.has_media ? 2 : 0) | (spectral ? 1 : 0);
-        switch (v) {
-            case 0: launch_wave<16, false, false>(r, cam, f, r->sample_count); break;
-            case 1: launch_wave<16, false, true>(r, cam, f, r->sample_count); break;
-            case 2: launch_wave<16, true, false>(r, cam, f, r->sample_count); break;
-            case 3: launch_wave<16, true, true>(r, cam, f, r->sample_count); break;
-            case 4: launch_wave<32, false, false>(r, cam, f, r->sample_count); break;
-            case 5: launch_wave<32, false, true>(r, cam, f, r->sample_count); break;
-            case 6: launch_wave<32, true, false>(r, cam, f, r->sample_count); break;
-            default: launch_wave<32, true, true>(r, cam, f, r->sample_count);
-        }
-        HIP_TRY(hipGetLastError());
-        r->sample_count += (uint32_t)f;
-        done += (uint32_t)f;
+    st = run_waves(r, prm, nframes, r->sample_count, false);
+    if (st != PT_OK) return st;
+    r->sample_count += nframes;
+    return PT_OK;
+}
+
+pt_status pt_forward(pt_renderer* r, const pt_params* prm, uint32_t nframes) {
+    if (!r || !prm) return fail(PT_E_INVALID, "pt_forward: null argument");
+    pt_status st = pt_clear_accumulation(r);
+    if (st != PT_OK) return st;
+    return pt_render_frames(r, prm, nframes);
+}
+
+pt_status pt_backward(pt_renderer* r, const pt_params* prm, uint32_t nframes, const pt_adjoint_params* adj,
+                      const float* dL_dimage, float* dparams) {
+    if (!r || !prm || !adj || !dL_dimage || !dparams) return fail(PT_E_INVALID, "pt_backward: null argument");
+    pt_ctx* ctx = r->ctx;
+    if (!ctx->has_scene) return fail(PT_E_MISSING_INPUT, "missing input: no scene uploaded (pt_scene_upload)");
+    const pt_program& pg = r->prog;
+    if (pg.color != PT_COLOR_RGB) return fail(PT_E_UNSUPPORTED, "pt_backward: the adjoint runs RGB programs only");
+    if (nframes == 0) return fail(PT_E_INVALID, "pt_backward: nframes must be >= 1");
+    if (adj->reduce_mode != PT_REDUCE_ATOMIC && adj->reduce_mode != PT_REDUCE_DETERMINISTIC)
+        return fail(PT_E_INVALID, "pt_backward: unknown reduce_mode");
+    if (adj->n_albedo < 0 || adj->n_albedo > PT_ADJ_MAX_ALBEDO) return fail(PT_E_INVALID, "pt_backward: n_albedo out of range");
+    const std::vector<pt_material>& mats = ctx->host_materials;
+    const int nm = (int)mats.size();
+    if (adj->medium_material >= nm) return fail(PT_E_INVALID, "pt_backward: medium_material out of range");
+    if (adj->medium_material >= 0) {
+        const pt_material& m = mats[adj->medium_material];
+        if (!m.has_medium) return fail(PT_E_INVALID, "pt_backward: medium_material has no medium");
+        // sigma_s = 0 is allowed: such a channel scatters nothing, and its d/dsigma_s holds
+        // the extinction term only (the in-scattering term has no sampled path)
+        for (int c = 0; c < 3; c++)
+            if (!(m.sigma_s[c] >= 0.0f) || !(m.sigma_a[c] >= 0.0f))
+                return fail(PT_E_INVALID, "pt_backward: medium parameters need sigma_s, sigma_a >= 0");
+        if (!(m.hg_g > -1.0f && m.hg_g < 1.0f)) return fail(PT_E_INVALID, "pt_backward: HG g must be in (-1, 1)");
     }
+    for (int k = 0; k < adj->n_albedo; k++) {
+        const int a = adj->albedo_material[k];
+        if (a < 0 || a >= nm) return fail(PT_E_INVALID, "pt_backward: albedo_material out of range");
+        const pt_material& m = mats[a];
+        if (m.model != PT_MAT_LAMBERT || !(m.albedo[0] > 0.0f) || m.albedo[0] != m.albedo[1] || m.albedo[1] != m.albedo[2])
+            return fail(PT_E_INVALID, "pt_backward: albedo parameters must be Lambert materials with a grey albedo > 0");
+    }
+    HIP_TRY(hipSetDevice(ctx->device));
+    const int F = std::max(1, std::min<int>(pg.frames_in_flight, (int)nframes));
+    pt_status st = ensure_paths(r, (long long)pg.width * pg.height * F);
+    if (st != PT_OK) return st;
+    const size_t npix = (size_t)pg.width * pg.height;
+    if (r->adj_cap != r->cap) {
+        struct { DevBuf* b; size_t bytes; } list[] = {
+            {&r->b_adjD, (size_t)ADJ_N * r->cap * 4}, {&r->b_adjG, (size_t)ADJ_N * r->cap * 4},
+            {&r->b_adjsD, (size_t)ADJ_N * r->cap * 4}, {&r->b_delta, npix * 16}, {&r->b_pixgrad, (size_t)ADJ_N * npix * 4},
+            {&r->b_grad, (size_t)ADJ_N * 4}};
+        for (auto& e : list) {
+            if (e.b->p) { HIP_TRY(hipFree(e.b->p)); e.b->p = nullptr; }
+            e.b->bytes = e.bytes;
+            HIP_TRY(hipMalloc(&e.b->p, e.b->bytes));
+        }
+        r->adj_cap = r->cap;
+    }
+    // dL/dI of the film average: each sample enters with weight 1/nframes
+    std::vector<float> delta(npix * 4);
+    const float inv_n = 1.0f / (float)nframes;
+    for (size_t i = 0; i < npix * 4; i++) delta[i] = dL_dimage[i] * inv_n;
+    HIP_TRY(hipMemcpyAsync(r->b_delta.p, delta.data(), npix * 16, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemsetAsync(r->b_pixgrad.p, 0, r->b_pixgrad.bytes, ctx->stream));
+    HIP_TRY(hipMemsetAsync(r->b_grad.p, 0, r->b_grad.bytes, ctx->stream));
+    Paths& P = r->paths;
+    P.adj_D = (float*)r->b_adjD.p; P.adj_G = (float*)r->b_adjG.p; P.adj_sD = (float*)r->b_adjsD.p;
+    P.adj_delta = (const float4*)r->b_delta.p;
+    P.adj_medium = adj->medium_material;
+    P.adj_n_albedo = adj->n_albedo;
+    for (int k = 0; k < PT_ADJ_MAX_ALBEDO; k++) P.adj_albedo[k] = k < adj->n_albedo ? adj->albedo_material[k] : -1;
+    r->adj_mode = adj->reduce_mode;
+    st = run_waves(r, prm, nframes, 0, true);
+    if (st != PT_OK) return st;
+    if (adj->reduce_mode == PT_REDUCE_DETERMINISTIC)
+        TIMED(ctx, "adj_final", npix, hipLaunchKernelGGL(k_adj_final, dim3(ADJ_N), dim3(SBLOCK), 0, ctx->stream,
+                                                          (const float*)r->b_pixgrad.p, (int)npix, (float*)r->b_grad.p));
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(dparams, r->b_grad.p, (size_t)ADJ_N * 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return PT_OK;
+}
+
+pt_status pt_scene_update_materials(pt_ctx* ctx, const pt_material* materials, int32_t n) {
+    if (!ctx || !materials) return fail(PT_E_INVALID, "pt_scene_update_materials: null argument");
+    if (!ctx->has_scene) return fail(PT_E_MISSING_INPUT, "missing input: no scene uploaded (pt_scene_upload)");
+    if (n != (int32_t)ctx->host_materials.size()) return fail(PT_E_INVALID, "pt_scene_update_materials: material count differs from the scene");
+    for (int m = 0; m < n; m++) {
+        const pt_material& a = ctx->host_materials[m];
+        const pt_material& b = materials[m];
+        if (a.model != b.model || a.has_medium != b.has_medium || a.emissive != b.emissive)
+            return fail(PT_E_INVALID, "pt_scene_update_materials: model, has_medium and emissive must not change");
+    }
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(hipMemcpy((void*)ctx->dev.materials, materials, sizeof(pt_material) * (size_t)n, hipMemcpyHostToDevice));
+    ctx->host_materials.assign(materials, materials + n);
     return PT_OK;
 }
 

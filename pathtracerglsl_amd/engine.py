@@ -58,6 +58,32 @@ class Engine:
     def render_frame(self, params=None):
         self.render_frames(1, params)
 
+    # -- differentiation (the adjoint pair) --
+    def forward(self, n, params=None):
+        """pt_forward: clear, then render n frames from sampleCount 0 (the primal image)."""
+        h, comp = self._cur()
+        prm = params if params is not None else comp.params(self._salt)
+        _lib.check(self.lib.pt_forward(h, C.byref(prm), int(n)))
+
+    def backward(self, n, adj, dL_dimage, params=None):
+        """pt_backward: dL/dparams (abi.ADJ_N_PARAMS floats) of the replayed n-frame forward pass
+        for the adjoint image dL_dimage [H][W][4] (row 0 = bottom)."""
+        h, comp = self._cur()
+        prm = params if params is not None else comp.params(self._salt)
+        dL = np.ascontiguousarray(dL_dimage, np.float32)
+        if dL.shape != (comp.height, comp.width, 4):
+            raise ValueError(f"dL_dimage must be [{comp.height}][{comp.width}][4]")
+        out = np.zeros(abi.ADJ_N_PARAMS, np.float32)
+        _lib.check(self.lib.pt_backward(h, C.byref(prm), int(n), C.byref(adj), dL.ctypes.data, out.ctypes.data))
+        return out
+
+    def update_materials(self, materials):
+        """pt_scene_update_materials: new values for the uploaded scene's material records."""
+        m = np.ascontiguousarray(materials, dtype=abi.MATERIAL_DTYPE)
+        _lib.check(self.lib.pt_scene_update_materials(self.ctx, m.ctypes.data, len(m)))
+        if self._scene is not None:
+            self._scene.materials = m
+
     @property
     def sample_count(self):
         h, _ = self._cur()

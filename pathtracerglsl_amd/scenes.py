@@ -307,3 +307,71 @@ def cornell_spectral_medium(kelvin=6500.0, scale=12.0):
     for l in sc["lights"]:
         l["radiance"] = {"blackbody": {"kelvin": kelvin, "scale": scale}}
     return sc
+
+
+# ------------------------------------------------ differentiable media inverse render (C4)
+
+INVERSE_N_ALBEDO = 16
+
+
+def inverse_media_scene(theta):
+    """The glass-medium Cornell box with its walls and inner boxes split into 16 grey Lambert
+    patches (floor 2 x 2, back wall 2 x 2, left / right / ceiling 2 each, one per inner box);
+    theta = [sigma_s RGB, sigma_a RGB, g, albedo x 16] (the adjoint's parameter layout), the
+    sphere's interior medium carrying sigma_s / sigma_a / g. RGB light (12, 12, 12)."""
+    theta = [float(v) for v in theta]
+    assert len(theta) == 7 + INVERSE_N_ALBEDO
+    patches = []
+    for i in range(2):
+        for j in range(2):
+            patches.append(_quad_tris((-1 + i, -1, 1 - j), (1, 0, 0), (0, 0, -1)))            # floor
+    for i in range(2):
+        for j in range(2):
+            patches.append(_quad_tris((-1 + i, -1 + j, -1), (1, 0, 0), (0, 1, 0)))            # back
+    for j in range(2):
+        patches.append(_quad_tris((-1, -1 + j, 1), (0, 0, -2), (0, 1, 0)))                    # left
+    for j in range(2):
+        patches.append(_quad_tris((1, -1 + j, -1), (0, 0, 2), (0, 1, 0)))                     # right
+    for i in range(2):
+        patches.append(_quad_tris((-1 + i, 1, -1), (1, 0, 0), (0, 0, 2)))                     # ceiling
+    materials, objects = {}, []
+    for k, tris in enumerate(patches):
+        materials[f"patch{k}"] = {"model": "lambert", "albedo": theta[7 + k]}
+        objects.append({"type": "mesh", "triangles": np.asarray(tris), "material": f"patch{k}", "closed": False})
+    for k, (lo, hi) in enumerate((((0.05, -1.0, 0.0), (0.65, -0.4, 0.6)), ((-0.65, -1.0, -0.6), (-0.05, 0.2, 0.0)))):
+        name = f"patch{14 + k}"
+        materials[name] = {"model": "lambert", "albedo": theta[21 + k]}
+        objects.append({"type": "mesh", "triangles": _box_tris(lo, hi), "material": name, "closed": True})
+    materials["glass"] = {"model": "dielectric", "ior": 1.5,
+                          "medium": {"sigma_s": theta[0:3], "sigma_a": theta[3:6], "phase_g": theta[6]}}
+    objects.append({"type": "sphere", "center": [0.0, 0.0, 0.0], "radius": 0.5, "material": "glass"})
+    return {
+        "id": "inverse-media",
+        "materials": materials,
+        "objects": objects,
+        "lights": [{"type": "quad", "corner": [-0.25, 0.999, -0.25], "edge1": [0.5, 0.0, 0.0],
+                    "edge2": [0.0, 0.0, 0.5], "radiance": [12.0, 12.0, 12.0]}],
+        "environment": {"type": "none"},
+        "camera": {"type": "pinhole", "position": [0.0, 0.0, 3.5], "target": [0.0, 0.0, 0.0],
+                   "fov": math.radians(40.0)},
+    }
+
+
+def inverse_media_params(seed=2024):
+    """(hidden target, perturbed init) parameter vectors drawn from a seed: sigma_t per channel
+    in [0.6, 2.2] with albedo 0.9 (sigma_s = 0.9 sigma_t), g in [0.3, 0.8], patch albedos in
+    [0.2, 0.9]; the init scales every parameter by a factor in [0.7, 1.3] (g and albedos
+    clamped to their ranges)."""
+    rng = np.random.default_rng(seed)
+    st = rng.uniform(0.6, 2.2, size=3)
+    target = np.concatenate([0.9 * st, 0.1 * st, [rng.uniform(0.3, 0.8)], rng.uniform(0.2, 0.9, size=INVERSE_N_ALBEDO)])
+    init = target * rng.uniform(0.7, 1.3, size=target.size)
+    init[6] = float(np.clip(init[6], 0.05, 0.9))
+    init[7:] = np.clip(init[7:], 0.05, 0.95)
+    return target, init
+
+
+def inverse_adjoint_materials(comp):
+    """(medium material index, albedo material indices) of a compiled inverse_media_scene."""
+    names = comp.material_names
+    return names.index("glass"), [names.index(f"patch{k}") for k in range(INVERSE_N_ALBEDO)]

@@ -40,6 +40,8 @@ def load(variant=""):
     lib.or_env_pdf.argtypes = [vp, vp, C.c_int, vp]
     lib.or_env_sample.argtypes = [vp, vp, C.c_int, vp, vp]
     lib.or_basic_eval.argtypes = [C.c_int, vp, vp, vp, C.c_int]
+    lib.or_adjoint.argtypes = [vp, C.POINTER(abi.PtProgram), C.POINTER(abi.PtParams), C.c_int,
+                               C.POINTER(abi.PtAdjointParams), vp, vp]
     return lib
 
 
@@ -82,6 +84,14 @@ class OracleScene:
                            film.ctypes.data, samples.ctypes.data if samples is not None else None,
                            counts.ctypes.data)
         return film, samples, counts
+
+    def adjoint(self, program, params, nframes, adj, dL):
+        """FP64 replay of the path-replay adjoint: d(sum dL . film)/dtheta."""
+        dL = np.ascontiguousarray(dL, np.float32)
+        assert dL.shape == (program.height, program.width, 4)
+        g = np.zeros(abi.ADJ_N_PARAMS, np.float64)
+        self.lib.or_adjoint(self.h, C.byref(program), C.byref(params), nframes, C.byref(adj), dL.ctypes.data, g.ctypes.data)
+        return g
 
     def env_table(self, w=512, h=256):
         cond = np.zeros((h, w), np.float32)
