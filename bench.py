@@ -14,6 +14,12 @@ between the perturbed-init render and the hidden-target render. Reported: pt_for
 pt_backward wall times (their ratio is the adjoint/forward cost) and the gradient's relative
 error against the FP64 CPU replay of the same paths.
 
+Batch leg (the batched persistent wavefront): 64 seeded icosphere-cloud scenes with log-uniform
+triangle count (1k-500k), resolution (128^2-1024^2) and spp (4-256), pt-mis depth 12, rendered
+as one hipGraph-captured batch after the autotuner's sweep, against 64 sequential plain
+launches (pt_render_frames per scene; uploads untimed); reported with the per-pixel identity
+of all 64 films.
+
 Also reported: the roofline of the dominant kernel (the extend / closest-hit traversal),
 whose algorithmic bytes per query are 32 (ray) + 16 (hit) + 80 per wide node visited + 36 per
 triangle tested, counted by an instrumented, untimed replay of the identical render; and a
@@ -97,6 +103,53 @@ def adjoint_leg(Engine, compiler, scenes, abi, fp64=True, res=512, spp=64):
     return out
 
 
+def batch_leg(Engine, compiler, scenes, abi):
+    """64-scene batch vs 64 sequential plain launches, identical films required."""
+    import numpy as np
+    from pathtracerglsl_amd.batch import BatchRenderer
+    items = scenes.batch_workload()
+    strategy = compiler.RenderStrategy("mis", 12, 3, 0.95, 4)
+    comps = [compiler.compile_scene(sc, strategy, w, h) for sc, w, h, _ in items]
+    spp = [n for _, _, _, n in items]
+    paths = sum(c.width * c.height * n for c, n in zip(comps, spp))
+    eng = Engine(0)
+    seq, films = 0.0, []
+    for c, n in zip(comps, spp):
+        eng.load_renderer(c)
+        eng.finish()
+        t0 = time.perf_counter()
+        eng.clear_accumulation()
+        eng.render_frames(n)
+        eng.finish()
+        seq += time.perf_counter() - t0
+        films.append(eng.read_export())
+    log(f"[bench] batch: sequential {seq:.3f}s for {paths / 1e6:.1f} Mpaths")
+    br = BatchRenderer(eng, comps)
+    best, sweep = br.autotune(spp, probe_div=8)
+    br.render(spp)                       # capture + first replay
+    eng.finish()
+    times = []
+    for _ in range(2):
+        t0 = time.perf_counter()
+        br.render(spp)
+        eng.finish()
+        times.append(time.perf_counter() - t0)
+    bt = min(times)
+    bfilms = [br.read_export(s) for s in range(len(comps))]
+    identical = all(np.array_equal(bfilms[s], films[s], equal_nan=True) for s in range(len(comps)))
+    nonfinite = int(sum(int((~np.isfinite(f)).sum()) for f in bfilms))
+    sweep.sort(key=lambda x: x[1])
+    out = {"workload": "64 icosphere-cloud scenes, log-uniform 1k-500k tris, 128^2-1024^2, 4-256 spp, pt-mis depth12",
+           "triangles": int(sum(c.scene.n_tris for c in comps)), "paths": int(paths),
+           "sequential_s": round(seq, 4), "batched_s": round(bt, 4), "speedup": round(seq / bt, 3),
+           "batched_mpaths_per_s": round(paths / bt / 1e6, 2), "sequential_mpaths_per_s": round(paths / seq / 1e6, 2),
+           "identical_films": bool(identical), "nonfinite_pixels": nonfinite, "autotuned": best.as_dict(),
+           "sweep_top": [{"config": c, "ms": round(m, 2)} for c, m in sweep[:5]], "sweep_size": len(sweep)}
+    br.close()
+    eng.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,6 +162,7 @@ def main():
     ap.add_argument("--cpu-spp", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="run a plain device copy of this size first (counter calibration runs)")
     args = ap.parse_args()
@@ -215,6 +269,11 @@ def main():
         from pathtracerglsl_amd import abi
         result["adjoint"] = adjoint_leg(Engine, compiler, scenes, abi, fp64=not args.no_cpu)
         log(f"[bench] adjoint {result['adjoint']}")
+    if not args.no_batch:
+        log("[bench] batch leg")
+        from pathtracerglsl_amd import abi
+        result["batch"] = batch_leg(Engine, compiler, scenes, abi)
+        log(f"[bench] batch {result['batch']}")
     print(json.dumps(result), flush=True)
 
 

@@ -281,6 +281,36 @@ typedef struct pt_adjoint_params {
     int32_t pad;
 } pt_adjoint_params;
 
+/* ---- batched persistent wavefront (many scenes in one launch) ----
+ * No counterpart in the reference (it renders one compiled scene per Engine): the batch
+ * holds n scenes of uneven size, each with its own compiled program (same estimator, bounce
+ * budget, RR and RGB color model), camera and film, and renders all of them as one sequence
+ * of waves whose path slots mix scenes, captured into one hipGraph. Per-pixel output equals
+ * pt_render_frames on each scene alone (same sample streams and arithmetic). */
+typedef struct pt_batch pt_batch;
+typedef struct pt_batch_config {
+    int32_t wave_paths;          /* path slots per wave (queue capacity), >= the largest film */
+    int32_t trace_block;         /* traversal block size: 64, 128 or 256 */
+    int32_t trace_blocks_per_cu; /* persistent traversal / shadow grid */
+    int32_t shade_blocks_per_cu; /* bin / shade grid */
+    int32_t stack;               /* LDS stack entries per lane: 16 or 32, >= every scene's need */
+    int32_t use_graph;           /* capture the whole batch into one hipGraph */
+    int32_t pad[2];
+} pt_batch_config;
+pt_status pt_batch_create(pt_ctx* ctx, int32_t n_scenes, const pt_scene_desc* scenes, const pt_program* programs,
+                          const pt_params* params, pt_batch** out);
+void      pt_batch_destroy(pt_batch* b);
+pt_status pt_batch_set_config(pt_batch* b, const pt_batch_config* cfg);
+pt_status pt_batch_get_config(const pt_batch* b, pt_batch_config* cfg);
+/* Clear every film and render spp[s] frames of scene s from sampleCount 0. Asynchronous. */
+pt_status pt_batch_render(pt_batch* b, const uint32_t* spp);
+pt_status pt_batch_read_export(pt_batch* b, int32_t scene, float* out, size_t out_floats);
+/* Autotuner: times the batch at spp[s] / probe_div (at least 1) over a sweep of wave size,
+ * traversal block size, grid size and stack depth; sets and returns the fastest. The sweep
+ * (configs, ms) is written to the first max_results entries of the optional arrays. */
+pt_status pt_batch_autotune(pt_batch* b, const uint32_t* spp, uint32_t probe_div, pt_batch_config* best,
+                            pt_batch_config* results, float* ms, int32_t max_results, int32_t* n_results);
+
 /* Forward pass: clear the film and render nframes from sampleCount 0 (the primal image of
  * the adjoint pair; read it with pt_read_export). */
 pt_status pt_forward(pt_renderer* r, const pt_params* params, uint32_t nframes);

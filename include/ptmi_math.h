@@ -149,10 +149,17 @@ PTM_FN float ptm_atan2(float y, float x) {
 PTM_FN float ptm_acos(float x) {
     return ptm_atan2(sqrtf((1.0f - x) * (1.0f + x)), x);
 }
-/* pow(x, y) = exp(y log x) for x > 0; pow(0, y>0) = 0; negative x is undefined in GLSL. */
+/* pow(x, y) = exp(y log x) for x > 0; pow(0, y>0) = 0. GLSL leaves negative x undefined;
+ * declared here: an integer exponent gives the real power (sign of x^y), any other NaN.
+ * Schlick's pow(1 - cos, 5) meets x = -1 ulp when a rounded |dot| exceeds 1. */
 PTM_FN float ptm_pow(float x, float y) {
     if (x == 0.0f) return y > 0.0f ? 0.0f : (y == 0.0f ? 1.0f : INFINITY);
     if (y == 0.0f) return 1.0f;
+    if (x < 0.0f) {
+        if (floorf(y) != y) return NAN;
+        float r = ptm_exp(y * ptm_log(-x));
+        return fmodf(y, 2.0f) != 0.0f ? -r : r;
+    }
     return ptm_exp(y * ptm_log(x));
 }
 PTM_FN float ptm_tanh(float x) {

@@ -13,7 +13,9 @@ DECLARED = ["pt_last_error", "pt_ctx_create", "pt_ctx_destroy", "pt_scene_upload
             "pt_renderer_unload", "pt_render_frames", "pt_read_export", "pt_clear_accumulation",
             "pt_sample_count", "pt_trace_closest", "pt_trace_any", "pt_trace_closest_device",
             "pt_trace_any_device", "pt_set_timing", "pt_get_timings", "pt_last_ray_counts", "pt_finish",
-            "pt_scene_update_materials", "pt_forward", "pt_backward"]
+            "pt_scene_update_materials", "pt_forward", "pt_backward",
+            "pt_batch_create", "pt_batch_destroy", "pt_batch_set_config", "pt_batch_get_config",
+            "pt_batch_render", "pt_batch_read_export", "pt_batch_autotune"]
 
 _lib = None
 
@@ -61,6 +63,14 @@ def load():
     lib.pt_scene_update_materials.argtypes = [vp, vp, i32]
     lib.pt_forward.argtypes = [vp, C.POINTER(abi.PtParams), u32]
     lib.pt_backward.argtypes = [vp, C.POINTER(abi.PtParams), u32, C.POINTER(abi.PtAdjointParams), vp, vp]
+    lib.pt_batch_create.argtypes = [vp, i32, vp, vp, vp, C.POINTER(vp)]
+    lib.pt_batch_destroy.argtypes = [vp]
+    lib.pt_batch_destroy.restype = None
+    lib.pt_batch_set_config.argtypes = [vp, C.POINTER(abi.PtBatchConfig)]
+    lib.pt_batch_get_config.argtypes = [vp, C.POINTER(abi.PtBatchConfig)]
+    lib.pt_batch_render.argtypes = [vp, vp]
+    lib.pt_batch_read_export.argtypes = [vp, i32, vp, C.c_size_t]
+    lib.pt_batch_autotune.argtypes = [vp, vp, u32, C.POINTER(abi.PtBatchConfig), vp, vp, i32, C.POINTER(i32)]
     lib.pt_abi_sizeof.argtypes = [C.c_int]
     lib.pt_abi_sizeof.restype = C.c_size_t
     _ = (i32, st)

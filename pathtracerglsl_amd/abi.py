@@ -96,6 +96,15 @@ class PtAdjointParams(C.Structure):
         return a
 
 
+class PtBatchConfig(C.Structure):
+    _fields_ = [("wave_paths", C.c_int32), ("trace_block", C.c_int32), ("trace_blocks_per_cu", C.c_int32),
+                ("shade_blocks_per_cu", C.c_int32), ("stack", C.c_int32), ("use_graph", C.c_int32),
+                ("pad", C.c_int32 * 2)]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_ if k != "pad"}
+
+
 class PtKernelTiming(C.Structure):
     _fields_ = [("name", C.c_char * 32), ("total_ms", C.c_float), ("launches", C.c_uint32),
                 ("items", C.c_uint64)]

@@ -76,7 +76,14 @@ struct Paths {
     int adj_medium;             // material whose medium sigma_s / sigma_a / g are parameters
     int adj_n_albedo;
     int adj_albedo[PT_ADJ_MAX_ALBEDO];
+    // batched wavefront (BATCH kernels): one DevScene per scene, the scene of each path slot
+    const DevScene* scenes;
+    int* path_scene;
 };
+// The scene a path belongs to: the kernel argument, or its entry in the batch.
+template <bool BATCH> PT_DEV const DevScene& scene_of(const DevScene& Sk, const Paths& P, int path) {
+    return BATCH ? P.scenes[P.path_scene[path]] : Sk;
+}
 constexpr int ADJ_N = PT_ADJ_N_PARAMS;
 
 // Adjoint helpers. Parameter layout: [0..2] sigma_s (channel c), [3..5] sigma_a (channel c),
@@ -168,18 +175,14 @@ __global__ void __launch_bounds__(64) k_camera_medium(DevScene S, Paths P, Camer
     if (threadIdx.x == 0) P.counts[C_CAM_MEDIUM] = region_at<64>(S, mk3(C.ox, C.oy, C.oz), stk);
 }
 
+// One camera sample of pixel (x, y) at sampleCount `sample` into path slot i.
 template <bool SPEC, bool ADJ>
-__global__ void __launch_bounds__(SBLOCK) k_raygen(Paths P, Camera C, int W, int H, int n_paths, uint32_t sample0, int media, int wl_uniform) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) P.counts[C_RAYQ0] = n_paths;
-    if (i >= n_paths) return;
-    int pix = i % (W * H), frame = i / (W * H);
-    int x = pix % W, y = pix / W;
+PT_DEV void raygen_path(const Paths& P, const Camera& C, int W, int H, int i, int x, int y, uint32_t sample, int media, int wl_uniform) {
     // main() (features/accumulation.ts:14-26): pixel = gl_FragCoord.xy + u_pixelOffset
     float fx = (float)x + 0.5f + (float)C.off_x;
     float fy = (float)y + 0.5f + (float)C.off_y;
     Rng r;
-    r.base = pcg4d_x((uint32_t)fx, (uint32_t)fy, sample0 + (uint32_t)frame, C.salt);
+    r.base = pcg4d_x((uint32_t)fx, (uint32_t)fy, sample, C.salt);
     r.dim = 0;
     float j0 = r.next(), j1 = r.next();
     float filmx = fx + (j0 - 0.5f), filmy = fy + (j1 - 0.5f);   // pixel/box/box.glsl:5-7
@@ -210,6 +213,57 @@ __global__ void __launch_bounds__(SBLOCK) k_raygen(Paths P, Camera C, int W, int
 #pragma unroll
         for (int k = 0; k < ADJ_N; k++) { P.adj_D[(size_t)k * P.cap + i] = 0.0f; P.adj_G[(size_t)k * P.cap + i] = 0.0f; }
     }
+}
+
+template <bool SPEC, bool ADJ>
+__global__ void __launch_bounds__(SBLOCK) k_raygen(Paths P, Camera C, int W, int H, int n_paths, uint32_t sample0, int media, int wl_uniform) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) P.counts[C_RAYQ0] = n_paths;
+    if (i >= n_paths) return;
+    int pix = i % (W * H), frame = i / (W * H);
+    raygen_path<SPEC, ADJ>(P, C, W, H, i, pix % W, pix / W, sample0 + (uint32_t)frame, media, wl_uniform);
+}
+
+// Batched wavefront: a wave is a list of (scene, first frame, frames) chunks laid out back to
+// back in the path slots; every scene keeps its own camera, film and sample order.
+struct BatchEntry {
+    int scene, W, H, npix;
+    int f0, nf;
+    long long base;       // first path slot of the chunk
+    long long film;       // first pixel of the scene's film
+};
+__global__ void __launch_bounds__(SBLOCK) k_raygen_batch(Paths P, const BatchEntry* E, int ne, const Camera* cams, int n_paths) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) P.counts[C_RAYQ0] = n_paths;
+    if (i >= n_paths) return;
+    int lo = 0, hi = ne - 1;                      // last entry with base <= i
+    while (lo < hi) { int mid = (lo + hi + 1) >> 1; if (E[mid].base <= (long long)i) lo = mid; else hi = mid - 1; }
+    const BatchEntry& e = E[lo];
+    const int local = (int)((long long)i - e.base);
+    const int pix = local % e.npix, frame = e.f0 + local / e.npix;
+    P.path_scene[i] = e.scene;
+    raygen_path<false, false>(P, cams[e.scene], e.W, e.H, i, pix % e.W, pix / e.W, (uint32_t)frame, 0, 0);
+}
+__global__ void __launch_bounds__(SBLOCK) k_accumulate_batch(const Paths P, const BatchEntry* E, float4* films) {
+    const BatchEntry& e = E[blockIdx.y];
+    const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= e.npix) return;
+    float4* film = films + e.film;
+    float4 acc = film[pix];
+    for (int f = 0; f < e.nf; f++) {
+        const uint32_t n = (uint32_t)(e.f0 + f);
+        float4 x = P.rad[e.base + (long long)f * e.npix + pix];
+        if (n == 0) acc = make_float4(x.x, x.y, x.z, 1.0f);
+        else {
+            float w = 1.0f / ((float)n + 1.0f);
+            acc = make_float4(acc.x * (1.0f - w) + x.x * w, acc.y * (1.0f - w) + x.y * w, acc.z * (1.0f - w) + x.z * w, 1.0f);
+        }
+    }
+    film[pix] = acc;
+}
+__global__ void __launch_bounds__(SBLOCK) k_fill_f4(float4* p, long long n) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < n; i += (long long)gridDim.x * blockDim.x) p[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
 // Persistent-lane work distribution: each wave owns a chunk of queue entries and hands them
@@ -256,9 +310,9 @@ constexpr int FEED_CHUNK = PTMI_FEED_CHUNK;        // queue entries a wave claim
 constexpr int FEED_MIN_IDLE = PTMI_FEED_MIN_IDLE;  // idle lanes before a wave refills
 
 // extend: closest hit for every ray of the current queue, then bin the path by shading category.
-template <int STACK, bool COUNT>
-__global__ void __launch_bounds__(TBLOCK) k_extend(DevScene S, Paths P, int qsel) {
-    __shared__ uint2 stk[STACK * TBLOCK];
+template <int STACK, bool COUNT, bool BATCH = false, int TB = TBLOCK>
+__global__ void __launch_bounds__(TB) k_extend(DevScene Sk, Paths P, int qsel) {
+    __shared__ uint2 stk[STACK * TB];
     const uint32_t n = (uint32_t)P.counts[qsel];
     const int* q = P.q[qsel];
     TraceCounters cnt = {0, 0};
@@ -266,11 +320,12 @@ __global__ void __launch_bounds__(TBLOCK) k_extend(DevScene S, Paths P, int qsel
     TravState st;
     bool active = false;
     int path = -1;
-    const bool have_bvh = S.n_cw_nodes > 0;
+    const DevScene* Sp = &Sk;
     for (;;) {
         int k = feed.feed<FEED_CHUNK>(&P.counts[C_EXT_FETCH], n, !active, FEED_MIN_IDLE);
         if (k >= 0) {
             path = q[k];
+            Sp = &scene_of<BATCH>(Sk, P, path);
             float4 o4 = P.ray_o[path], d4 = P.ray_d[path];
             trace_init(st, xyz(o4), xyz(d4), MAX_DIST);
             active = true;
@@ -279,9 +334,9 @@ __global__ void __launch_bounds__(TBLOCK) k_extend(DevScene S, Paths P, int qsel
             if (feed.exhausted) break;
             continue;
         }
-        if (active && (!have_bvh || trace_step<false, TBLOCK, COUNT>(S.cw_nodes, S.cw_tris, st, stk, cnt))) {
+        if (active && (Sp->n_cw_nodes == 0 || trace_step<false, TB, COUNT>(Sp->cw_nodes, Sp->cw_tris, st, stk, cnt))) {
             float tmax = st.tmax; int prim = st.prim;
-            trace_analytic<false>(S, st.o, st.d, tmax, prim);
+            trace_analytic<false>(*Sp, st.o, st.d, tmax, prim);
             P.hit[path] = make_float2(tmax, __int_as_float(prim));
             active = false;
         }
@@ -294,19 +349,20 @@ __global__ void __launch_bounds__(TBLOCK) k_extend(DevScene S, Paths P, int qsel
 
 // shadow (opaque scenes): any hit for every NEE ray; unoccluded contributions are added to
 // their path as ((a * vis) * w) / pdf, the reference's operation order with vis = 1.
-template <int STACK, bool SPEC, bool ADJ>
-__global__ void __launch_bounds__(TBLOCK) k_shadow(DevScene S, Paths P) {
-    __shared__ uint2 stk[STACK * TBLOCK];
+template <int STACK, bool SPEC, bool ADJ, bool BATCH = false, int TB = TBLOCK>
+__global__ void __launch_bounds__(TB) k_shadow(DevScene Sk, Paths P) {
+    __shared__ uint2 stk[STACK * TB];
     const int n = P.counts[C_SHADOW];
     TraceCounters cnt = {0, 0};
-    for (int base = blockIdx.x * TBLOCK; base < n; base += gridDim.x * TBLOCK) {
+    for (int base = blockIdx.x * TB; base < n; base += gridDim.x * TB) {
         int k = base + threadIdx.x;
         if (k >= n) break;
+        const DevScene& S = scene_of<BATCH>(Sk, P, P.s_path[k]);
         float4 o4 = P.s_o[k], d4 = P.s_d[k];
         f3 o = xyz(o4), d = xyz(d4);
         float maxd = o4.w; int prim = -1;
         bool blocked = trace_analytic<true>(S, o, d, maxd, prim);
-        if (!blocked) blocked = trace_wide<true, STACK, TBLOCK, false>(S, o, d, maxd, prim, stk, cnt);
+        if (!blocked) blocked = trace_wide<true, STACK, TB, false>(S, o, d, maxd, prim, stk, cnt);
         if (!blocked) {
             int i = P.s_path[k];
             float4 c = P.s_c[k];
@@ -399,8 +455,8 @@ PT_DEV int block_append(int* const* counters, int which, int* s_cnt, int* s_base
 // bin: per path of the iteration, the medium segment first (media scenes: the analytic
 // channel-MIS distance sample of pt.ts's media branch, which may end the segment in a
 // scattering event), then the shading category of the vertex.
-template <bool MEDIA, bool SPEC, bool ADJ>
-__global__ void __launch_bounds__(SBLOCK) k_bin(DevScene S, Paths P, int qsel) {
+template <bool MEDIA, bool SPEC, bool ADJ, bool BATCH = false>
+__global__ void __launch_bounds__(SBLOCK) k_bin(DevScene Sk, Paths P, int qsel) {
     __shared__ int s_cnt[N_CAT], s_base[N_CAT];
     int* ctr[N_CAT];
     for (int c = 0; c < N_CAT; c++) ctr[c] = &P.counts[C_CAT0 + c];
@@ -411,6 +467,7 @@ __global__ void __launch_bounds__(SBLOCK) k_bin(DevScene S, Paths P, int qsel) {
         int cat = -1, path = 0;
         if (k < n) {
             path = q[k];
+            const DevScene& S = scene_of<BATCH>(Sk, P, path);
             float2 hv = P.hit[path];
             int prim = __float_as_int(hv.y);
             if (MEDIA) {
@@ -470,8 +527,8 @@ __global__ void k_bookkeep(Paths P, int cur, int next, int first) {
 // (transport/combiner.ts:23-102) and the kernel/light techniques (techniques/kernel/kernel.glsl,
 // kernel_phase.glsl, techniques/light/light.glsl, light_medium.glsl), for the paths of one
 // category. MEDIA selects the media program (region tracking, medium NEE shadow walker).
-template <int EST, int CAT, int STACK, bool MEDIA, bool SPEC, bool ADJ>
-__global__ void __launch_bounds__(SBLOCK) k_shade(DevScene S, Paths P, int qsel, int max_bounces,
+template <int EST, int CAT, int STACK, bool MEDIA, bool SPEC, bool ADJ, bool BATCH = false>
+__global__ void __launch_bounds__(SBLOCK) k_shade(DevScene Sk, Paths P, int qsel, int max_bounces,
                                                   int rr_start, float rr_max, int max_null) {
     constexpr bool PROBE = CAT == CAT_DIELECTRIC || MEDIA;
     __shared__ uint2 stk[PROBE && CAT != CAT_MISS && CAT != CAT_MEDIUM ? STACK * SBLOCK : 1];
@@ -495,6 +552,7 @@ __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene S, Paths P, int qsel,
         AdjState A;
         float sD[ADJ_N];
         if (valid) {
+            const DevScene& S = scene_of<BATCH>(Sk, P, i);
             if (ADJ && (CAT != CAT_MISS || S.env_type != PT_ENV_NONE)) adj_load(P, i, A);
             else A.dirty = false;
             f3 rd = xyz(P.ray_d[i]);
@@ -853,6 +911,26 @@ int grid_for(long long n, int block) {
 
 struct TimingSlot { std::vector<std::pair<hipEvent_t, hipEvent_t>> ev; float ms = 0; uint32_t launches = 0; uint64_t items = 0; };
 
+// One uploaded scene: its device buffers and the DevScene view of them.
+struct SceneStore {
+    std::vector<std::unique_ptr<DevBuf>> bufs;
+    DevScene dev{};
+    int stack = 16;
+    uint32_t wide_depth = 0;
+    bool has_ggx = false;
+    std::vector<pt_material> host_materials;
+    hipStream_t stream = nullptr;
+    template <class T> pt_status upload(const T* host, size_t n, const T** out) {
+        auto b = std::make_unique<DevBuf>();
+        b->bytes = std::max<size_t>(n, 1) * sizeof(T);
+        HIP_TRY(hipMalloc(&b->p, b->bytes));
+        if (n) HIP_TRY(hipMemcpyAsync(b->p, host, n * sizeof(T), hipMemcpyHostToDevice, stream));
+        *out = (const T*)b->p;
+        bufs.push_back(std::move(b));
+        return PT_OK;
+    }
+};
+
 struct pt_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -862,6 +940,7 @@ struct pt_ctx {
     int stack = 16;
     uint32_t wide_depth = 0;
     int persist_blocks = 2048;   // persistent traversal grid (set from the device's CU count)
+    int cu_count = 256;
     int shade_blocks = 2048;
     bool has_ggx = false;
     std::vector<pt_material> host_materials;   // the uploaded records (adjoint checks, updates)
@@ -909,6 +988,7 @@ struct pt_renderer {
         b_spath, b_so, b_sd, b_sc, b_sl, b_wl, b_eta, b_hist, b_trav, b_cat[N_CAT];
     // adjoint buffers (allocated by the first pt_backward)
     DevBuf b_adjD, b_adjG, b_adjsD, b_delta, b_pixgrad, b_grad;
+    DevBuf b_pscene;      // scene of each path slot (batched wavefront)
     long long adj_cap = 0;
     int adj_mode = PT_REDUCE_DETERMINISTIC;
     Paths paths{};
@@ -946,6 +1026,7 @@ pt_status pt_ctx_create(int device_index, pt_ctx** out) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device_index) == hipSuccess && prop.multiProcessorCount > 0) {
         c->persist_blocks = prop.multiProcessorCount * 8;   // 16 waves per CU of 2-wave blocks
+        c->cu_count = prop.multiProcessorCount;
         c->shade_blocks = prop.multiProcessorCount * 8;
     }
     c->trav.bytes = 2 * sizeof(unsigned long long);
@@ -965,8 +1046,8 @@ void pt_ctx_destroy(pt_ctx* ctx) {
     delete ctx;
 }
 
-pt_status pt_scene_upload(pt_ctx* ctx, const pt_scene_desc* sd) {
-    if (!ctx || !sd) return fail(PT_E_INVALID, "pt_scene_upload: null argument");
+static pt_status validate_scene(const pt_scene_desc* sd) {
+    if (!sd) return fail(PT_E_INVALID, "pt_scene_upload: null argument");
     if (sd->abi_version != PT_ABI_VERSION) return fail(PT_E_INVALID, "pt_scene_upload: ABI version mismatch");
     if (sd->n_tris < 0 || sd->n_objects < 0 || sd->n_materials <= 0 || sd->n_lights < 0)
         return fail(PT_E_INVALID, "pt_scene_upload: negative count or no materials");
@@ -985,23 +1066,25 @@ pt_status pt_scene_upload(pt_ctx* ctx, const pt_scene_desc* sd) {
         if (ob.kind == PT_OBJ_MESH && (ob.first_tri < 0 || ob.tri_count < 0 || ob.first_tri + ob.tri_count > sd->n_tris))
             return fail(PT_E_INVALID, "object " + std::to_string(o) + ": triangle range out of bounds");
     }
-    HIP_TRY(hipSetDevice(ctx->device));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
-    ctx->scene_bufs.clear();
-    ctx->has_scene = false;
-    DevScene& D = ctx->dev;
+    return PT_OK;
+}
+
+// Build every device structure of one scene (wide BVH, containment, env table) into SS.
+static pt_status build_scene(const pt_scene_desc* sd, hipStream_t stream, SceneStore& SS) {
+    SS.stream = stream;
+    DevScene& D = SS.dev;
     D = DevScene{};
     // wide BVH over every mesh triangle
     WideBVH wb;
     try { build_wide_bvh(sd->tri_vertices, sd->n_tris, wb); }
     catch (const std::exception& e) { return fail(PT_E_INVALID, std::string("wide BVH build: ") + e.what()); }
-    ctx->wide_depth = wb.max_depth;
-    if (wb.max_depth + 1 <= 16) ctx->stack = 16;
-    else if (wb.max_depth + 1 <= 32) ctx->stack = 32;
+    SS.wide_depth = wb.max_depth;
+    if (wb.max_depth + 1 <= 16) SS.stack = 16;
+    else if (wb.max_depth + 1 <= 32) SS.stack = 32;
     else return fail(PT_E_UNSUPPORTED, "wide BVH deeper than 31 levels");
     pt_status st;
-    if ((st = ctx->upload((const uint4*)wb.nodes.data(), wb.nodes.size() / 4, &D.cw_nodes)) != PT_OK) return st;
-    if ((st = ctx->upload((const float4*)wb.tris.data(), wb.tris.size() / 4, &D.cw_tris)) != PT_OK) return st;
+    if ((st = SS.upload((const uint4*)wb.nodes.data(), wb.nodes.size() / 4, &D.cw_nodes)) != PT_OK) return st;
+    if ((st = SS.upload((const float4*)wb.tris.data(), wb.tris.size() / 4, &D.cw_tris)) != PT_OK) return st;
     D.n_cw_nodes = (int)wb.n_nodes;
     std::vector<float> shade((size_t)sd->n_tris * 12);
     for (int t = 0; t < sd->n_tris; t++)
@@ -1009,30 +1092,30 @@ pt_status pt_scene_upload(pt_ctx* ctx, const pt_scene_desc* sd) {
             for (int a = 0; a < 3; a++) shade[12 * (size_t)t + 4 * v + a] = sd->tri_vertices[9 * (size_t)t + 3 * v + a];
             shade[12 * (size_t)t + 4 * v + 3] = 0.0f;
         }
-    if ((st = ctx->upload((const float4*)shade.data(), (size_t)sd->n_tris * 3, &D.tri_shade)) != PT_OK) return st;
-    if ((st = ctx->upload(sd->tri_object, (size_t)sd->n_tris, &D.tri_object)) != PT_OK) return st;
+    if ((st = SS.upload((const float4*)shade.data(), (size_t)sd->n_tris * 3, &D.tri_shade)) != PT_OK) return st;
+    if ((st = SS.upload(sd->tri_object, (size_t)sd->n_tris, &D.tri_object)) != PT_OK) return st;
     D.n_tris = sd->n_tris;
-    if ((st = ctx->upload(sd->objects, (size_t)sd->n_objects, &D.objects)) != PT_OK) return st;
+    if ((st = SS.upload(sd->objects, (size_t)sd->n_objects, &D.objects)) != PT_OK) return st;
     D.n_objects = sd->n_objects;
     std::vector<int> analytic;
     for (int o = 0; o < sd->n_objects; o++) if (sd->objects[o].kind != PT_OBJ_MESH) analytic.push_back(o);
-    if ((st = ctx->upload(analytic.data(), analytic.size(), &D.analytic)) != PT_OK) return st;
+    if ((st = SS.upload(analytic.data(), analytic.size(), &D.analytic)) != PT_OK) return st;
     D.n_analytic = (int)analytic.size();
-    if ((st = ctx->upload(sd->materials, (size_t)sd->n_materials, &D.materials)) != PT_OK) return st;
-    ctx->host_materials.assign(sd->materials, sd->materials + sd->n_materials);
-    if ((st = ctx->upload(sd->lights, (size_t)sd->n_lights, &D.lights)) != PT_OK) return st;
+    if ((st = SS.upload(sd->materials, (size_t)sd->n_materials, &D.materials)) != PT_OK) return st;
+    SS.host_materials.assign(sd->materials, sd->materials + sd->n_materials);
+    if ((st = SS.upload(sd->lights, (size_t)sd->n_lights, &D.lights)) != PT_OK) return st;
     D.n_lights = sd->n_lights;
-    ctx->has_ggx = false;
+    SS.has_ggx = false;
     for (int m = 0; m < sd->n_materials; m++) {
         if (sd->materials[m].model == PT_MAT_DIELECTRIC) D.has_transmission = 1;
-        if (sd->materials[m].model == PT_MAT_GGX) ctx->has_ggx = true;
+        if (sd->materials[m].model == PT_MAT_GGX) SS.has_ggx = true;
         if (sd->materials[m].has_medium) D.has_media = 1;
         if (sd->materials[m].model == PT_MAT_NULL) D.has_null = 1;
     }
     if (sd->ambient_material >= sd->n_materials) return fail(PT_E_INVALID, "pt_scene_upload: ambient_material out of range");
     D.ambient_material = sd->ambient_material < 0 ? -1 : sd->ambient_material;
     if (sd->spectral_curves) {
-        if ((st = ctx->upload((const float4*)sd->spectral_curves, 95, &D.spec_curves)) != PT_OK) return st;
+        if ((st = SS.upload((const float4*)sd->spectral_curves, 95, &D.spec_curves)) != PT_OK) return st;
         for (int k = 0; k < 9; k++) D.xyz_rgb[k] = sd->xyz_to_rgb[k];
         D.cie_y = sd->cie_y_integral;
     }
@@ -1063,9 +1146,9 @@ pt_status pt_scene_upload(pt_ctx* ctx, const pt_scene_desc* sd) {
         for (int i = 0; i < ob.tri_count; i++) oorder[ob.first_tri + i] = ob.first_tri + (int)bb.order[i];
         titems.push_back(o);
     }
-    if ((st = ctx->upload((const float4*)onodes.data(), onodes.size() / 4, &D.obj_nodes)) != PT_OK) return st;
-    if ((st = ctx->upload(obase.data(), obase.size(), &D.obj_node_base)) != PT_OK) return st;
-    if ((st = ctx->upload(oorder.data(), oorder.size(), &D.obj_order)) != PT_OK) return st;
+    if ((st = SS.upload((const float4*)onodes.data(), onodes.size() / 4, &D.obj_nodes)) != PT_OK) return st;
+    if ((st = SS.upload(obase.data(), obase.size(), &D.obj_node_base)) != PT_OK) return st;
+    if ((st = SS.upload(oorder.data(), oorder.size(), &D.obj_order)) != PT_OK) return st;
     // per closed mesh wide BVH (first-hit-facing inside probe), concatenated
     std::vector<uint32_t> own; std::vector<float> owt; std::vector<int> oroot((size_t)std::max(sd->n_objects, 1), -1);
     uint32_t obj_depth = 0;
@@ -1083,13 +1166,13 @@ pt_status pt_scene_upload(pt_ctx* ctx, const pt_scene_desc* sd) {
         oroot[o] = (int)noff;
         obj_depth = std::max(obj_depth, w.max_depth);
     }
-    if (obj_depth + 1 > (uint32_t)ctx->stack) {
-        if (obj_depth + 1 <= 32) ctx->stack = 32;
+    if (obj_depth + 1 > (uint32_t)SS.stack) {
+        if (obj_depth + 1 <= 32) SS.stack = 32;
         else return fail(PT_E_UNSUPPORTED, "object BVH deeper than 31 levels");
     }
-    if ((st = ctx->upload((const uint4*)own.data(), own.size() / 4, &D.ow_nodes)) != PT_OK) return st;
-    if ((st = ctx->upload((const float4*)owt.data(), owt.size() / 4, &D.ow_tris)) != PT_OK) return st;
-    if ((st = ctx->upload(oroot.data(), oroot.size(), &D.ow_root)) != PT_OK) return st;
+    if ((st = SS.upload((const uint4*)own.data(), own.size() / 4, &D.ow_nodes)) != PT_OK) return st;
+    if ((st = SS.upload((const float4*)owt.data(), owt.size() / 4, &D.ow_tris)) != PT_OK) return st;
+    if ((st = SS.upload(oroot.data(), oroot.size(), &D.ow_root)) != PT_OK) return st;
     // uniform grid over the closed-mesh boxes (containment candidates)
     D.has_grid = titems.empty() ? 0 : 1;
     if (D.has_grid) {
@@ -1134,8 +1217,8 @@ pt_status pt_scene_upload(pt_ctx* ctx, const pt_scene_desc* sd) {
             cells[c] = make_int2((int)items.size(), (int)lists[c].size());
             items.insert(items.end(), lists[c].begin(), lists[c].end());
         }
-        if ((st = ctx->upload(cells.data(), cells.size(), &D.grid_cells)) != PT_OK) return st;
-        if ((st = ctx->upload(items.data(), items.size(), &D.grid_items)) != PT_OK) return st;
+        if ((st = SS.upload(cells.data(), cells.size(), &D.grid_cells)) != PT_OK) return st;
+        if ((st = SS.upload(items.data(), items.size(), &D.grid_items)) != PT_OK) return st;
     }
     // environment
     D.env_type = sd->env.type;
@@ -1150,12 +1233,32 @@ pt_status pt_scene_upload(pt_ctx* ctx, const pt_scene_desc* sd) {
         D.env_rot_c = (float)std::cos((double)sd->env.rotation);
         D.env_rot_s = (float)std::sin((double)sd->env.rotation);
         D.env_select_prob = sd->env.select_prob;
-        if ((st = ctx->upload(cond.data(), cond.size(), &D.env_cond)) != PT_OK) return st;
-        if ((st = ctx->upload(marg.data(), marg.size(), &D.env_marg)) != PT_OK) return st;
+        if ((st = SS.upload(cond.data(), cond.size(), &D.env_cond)) != PT_OK) return st;
+        if ((st = SS.upload(marg.data(), marg.size(), &D.env_marg)) != PT_OK) return st;
     } else if (sd->env.type != PT_ENV_NONE) {
         return fail(PT_E_UNSUPPORTED, "environment type not provided by this build");
     }
+    HIP_TRY(hipStreamSynchronize(SS.stream));
+    return PT_OK;
+}
+
+pt_status pt_scene_upload(pt_ctx* ctx, const pt_scene_desc* sd) {
+    if (!ctx) return fail(PT_E_INVALID, "pt_scene_upload: null argument");
+    pt_status st = validate_scene(sd);
+    if (st != PT_OK) return st;
+    HIP_TRY(hipSetDevice(ctx->device));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ctx->scene_bufs.clear();
+    ctx->has_scene = false;
+    ctx->dev = DevScene{};
+    SceneStore SS;
+    if ((st = build_scene(sd, ctx->stream, SS)) != PT_OK) return st;
+    ctx->scene_bufs = std::move(SS.bufs);
+    ctx->dev = SS.dev;
+    ctx->stack = SS.stack;
+    ctx->wide_depth = SS.wide_depth;
+    ctx->has_ggx = SS.has_ggx;
+    ctx->host_materials = std::move(SS.host_materials);
     ctx->has_scene = true;
     return PT_OK;
 }
@@ -1195,7 +1298,7 @@ static pt_status ensure_paths(pt_renderer* r, long long n) {
     struct { DevBuf* b; size_t elem; } list[] = {
         {&r->b_ray_o, 16}, {&r->b_ray_d, 16}, {&r->b_thr, 16}, {&r->b_rad, 16}, {&r->b_rng, 8}, {&r->b_prev, 16},
         {&r->b_flags, 4}, {&r->b_hit, 8}, {&r->b_q0, 4}, {&r->b_q1, 4}, {&r->b_spath, 4}, {&r->b_so, 16},
-        {&r->b_sd, 16}, {&r->b_sc, 16}, {&r->b_sl, 16}, {&r->b_medium, 4}, {&r->b_wl, 16}, {&r->b_eta, 4}, {&r->b_cat[0], 4}, {&r->b_cat[1], 4},
+        {&r->b_sd, 16}, {&r->b_sc, 16}, {&r->b_sl, 16}, {&r->b_medium, 4}, {&r->b_wl, 16}, {&r->b_eta, 4}, {&r->b_pscene, 4}, {&r->b_cat[0], 4}, {&r->b_cat[1], 4},
         {&r->b_cat[2], 4}, {&r->b_cat[3], 4}, {&r->b_cat[4], 4}, {&r->b_cat[5], 4}};
     static_assert(N_CAT == 6, "one queue buffer per shading category");
     for (auto& e : list) {
@@ -1216,7 +1319,8 @@ static pt_status ensure_paths(pt_renderer* r, long long n) {
     P.flags = (int*)r->b_flags.p; P.hit = (float2*)r->b_hit.p; P.q[0] = (int*)r->b_q0.p; P.q[1] = (int*)r->b_q1.p;
     P.counts = (int*)r->b_counts.p; P.s_path = (int*)r->b_spath.p; P.s_o = (float4*)r->b_so.p;
     P.s_d = (float4*)r->b_sd.p; P.s_c = (float4*)r->b_sc.p; P.s_l = (float4*)r->b_sl.p;
-    P.medium = (int*)r->b_medium.p; P.wl = (float4*)r->b_wl.p; P.eta = (float*)r->b_eta.p; P.hist = (unsigned long long*)r->b_hist.p;
+    P.medium = (int*)r->b_medium.p; P.wl = (float4*)r->b_wl.p; P.eta = (float*)r->b_eta.p;
+    P.path_scene = (int*)r->b_pscene.p; P.hist = (unsigned long long*)r->b_hist.p;
     P.trav = (unsigned long long*)r->b_trav.p;
     for (int c = 0; c < N_CAT; c++) P.catq[c] = (int*)r->b_cat[c].p;
     P.cap = (int)n;
@@ -1442,6 +1546,335 @@ pt_status pt_backward(pt_renderer* r, const pt_params* prm, uint32_t nframes, co
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(dparams, r->b_grad.p, (size_t)ADJ_N * 4, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return PT_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- batched wavefront
+struct pt_batch {
+    pt_ctx* ctx = nullptr;
+    std::vector<std::unique_ptr<SceneStore>> stores;
+    std::vector<pt_program> progs;
+    std::vector<Camera> cams;
+    std::vector<long long> film_off;   // first pixel of each scene's film
+    long long film_pixels = 0;
+    int max_npix = 0, min_stack = 16;
+    bool any_ggx = false, any_dielectric = false;
+    DevBuf d_scenes, d_cams, d_entries, films;
+    pt_renderer* r = nullptr;          // path slots
+    pt_batch_config cfg{};
+    std::vector<BatchEntry> entries;
+    std::vector<std::pair<int, int>> waves;   // (first entry, entry count)
+    std::vector<long long> wave_paths;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<uint32_t> graph_spp;
+    pt_batch_config graph_cfg{};
+    ~pt_batch() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        delete r;
+    }
+    void drop_graph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr; graph = nullptr; graph_spp.clear();
+    }
+};
+
+static bool same_cfg(const pt_batch_config& a, const pt_batch_config& b) {
+    return a.wave_paths == b.wave_paths && a.trace_block == b.trace_block && a.trace_blocks_per_cu == b.trace_blocks_per_cu &&
+           a.shade_blocks_per_cu == b.shade_blocks_per_cu && a.stack == b.stack && a.use_graph == b.use_graph;
+}
+
+// Waves of at most wave_paths slots; each scene contributes whole frames, in sample order.
+static pt_status batch_schedule(pt_batch* b, const uint32_t* spp) {
+    b->entries.clear(); b->waves.clear(); b->wave_paths.clear();
+    const long long cap = b->cfg.wave_paths;
+    long long used = 0;
+    int first = 0;
+    for (size_t s = 0; s < b->stores.size(); s++) {
+        const int W = b->progs[s].width, H = b->progs[s].height, npix = W * H;
+        uint32_t f = 0;
+        while (f < spp[s]) {
+            long long nf = std::min<long long>((long long)(spp[s] - f), (cap - used) / npix);
+            if (nf <= 0) {
+                b->waves.emplace_back(first, (int)b->entries.size() - first);
+                b->wave_paths.push_back(used);
+                first = (int)b->entries.size(); used = 0;
+                continue;
+            }
+            BatchEntry e;
+            e.scene = (int)s; e.W = W; e.H = H; e.npix = npix; e.f0 = (int)f; e.nf = (int)nf;
+            e.base = used; e.film = b->film_off[s];
+            b->entries.push_back(e);
+            used += nf * npix; f += (uint32_t)nf;
+        }
+    }
+    if (used > 0) { b->waves.emplace_back(first, (int)b->entries.size() - first); b->wave_paths.push_back(used); }
+    return PT_OK;
+}
+
+template <int EST, int STACK>
+static void batch_shade(pt_batch* b, int cur) {
+    pt_ctx* ctx = b->ctx;
+    const pt_program& pg = b->progs[0];
+    const DevScene& S0 = b->stores[0]->dev;
+    const int g = ctx->cu_count * b->cfg.shade_blocks_per_cu;
+    Paths& P = b->r->paths;
+    hipStream_t st = ctx->stream;
+#define PT_BSHADE(CATV) hipLaunchKernelGGL((k_shade<EST, CATV, STACK, false, false, false, true>), dim3(g), dim3(SBLOCK), 0, st, S0, P, cur, pg.max_bounces, pg.rr_start, pg.rr_max_survival, pg.max_null_crossings)
+    PT_BSHADE(CAT_MISS);
+    PT_BSHADE(CAT_LAMBERT);
+    if (b->any_ggx) PT_BSHADE(CAT_GGX);
+    if (b->any_dielectric) PT_BSHADE(CAT_DIELECTRIC);
+#undef PT_BSHADE
+}
+
+template <int STACK, int TB>
+static void batch_waves(pt_batch* b) {
+    pt_ctx* ctx = b->ctx;
+    const pt_program& pg = b->progs[0];
+    const DevScene& S0 = b->stores[0]->dev;
+    Paths& P = b->r->paths;
+    hipStream_t st = ctx->stream;
+    const int tgrid = ctx->cu_count * b->cfg.trace_blocks_per_cu;
+    const int sgrid = ctx->cu_count * b->cfg.shade_blocks_per_cu;
+    const BatchEntry* dE = (const BatchEntry*)b->d_entries.p;
+    for (size_t w = 0; w < b->waves.size(); w++) {
+        const int e0 = b->waves[w].first, ne = b->waves[w].second;
+        const long long n = b->wave_paths[w];
+        hipLaunchKernelGGL(k_raygen_batch, dim3((unsigned)((n + SBLOCK - 1) / SBLOCK)), dim3(SBLOCK), 0, st, P, dE + e0, ne,
+                           (const Camera*)b->d_cams.p, (int)n);
+        int cur = 0;
+        for (int it = 0; it <= pg.max_bounces; it++) {
+            hipLaunchKernelGGL(k_bookkeep, dim3(1), dim3(64), 0, st, P, cur, cur ^ 1, it == 0 ? 1 : 0);
+            hipLaunchKernelGGL((k_extend<STACK, false, true, TB>), dim3(tgrid), dim3(TB), 0, st, S0, P, cur);
+            hipLaunchKernelGGL((k_bin<false, false, false, true>), dim3(sgrid), dim3(SBLOCK), 0, st, S0, P, cur);
+            switch (pg.estimator) {
+                case PT_EST_PT: batch_shade<PT_EST_PT, STACK>(b, cur); break;
+                case PT_EST_NEE: batch_shade<PT_EST_NEE, STACK>(b, cur); break;
+                default: batch_shade<PT_EST_MIS, STACK>(b, cur);
+            }
+            if (pg.estimator != PT_EST_PT)
+                hipLaunchKernelGGL((k_shadow<STACK, false, false, true, TB>), dim3(tgrid), dim3(TB), 0, st, S0, P);
+            cur ^= 1;
+        }
+        hipLaunchKernelGGL(k_bookkeep, dim3(1), dim3(64), 0, st, P, -1, -1, 0);
+        int maxpix = 0;
+        for (int e = e0; e < e0 + ne; e++) maxpix = std::max(maxpix, b->entries[e].npix);
+        hipLaunchKernelGGL(k_accumulate_batch, dim3((maxpix + SBLOCK - 1) / SBLOCK, ne), dim3(SBLOCK), 0, st, P, dE + e0,
+                           (float4*)b->films.p);
+    }
+}
+
+static void batch_launch(pt_batch* b) {
+    hipLaunchKernelGGL(k_fill_f4, dim3(1024), dim3(SBLOCK), 0, b->ctx->stream, (float4*)b->films.p, b->film_pixels);
+    const int v = (b->cfg.stack == 16 ? 0 : 3) + (b->cfg.trace_block == 64 ? 0 : (b->cfg.trace_block == 128 ? 1 : 2));
+    switch (v) {
+        case 0: batch_waves<16, 64>(b); break;
+        case 1: batch_waves<16, 128>(b); break;
+        case 2: batch_waves<16, 256>(b); break;
+        case 3: batch_waves<32, 64>(b); break;
+        case 4: batch_waves<32, 128>(b); break;
+        default: batch_waves<32, 256>(b);
+    }
+}
+
+static pt_status batch_check_config(const pt_batch* b, const pt_batch_config& c) {
+    if (c.trace_block != 64 && c.trace_block != 128 && c.trace_block != 256) return fail(PT_E_INVALID, "pt_batch: trace_block must be 64, 128 or 256");
+    if (c.stack != 16 && c.stack != 32) return fail(PT_E_INVALID, "pt_batch: stack must be 16 or 32");
+    if (c.stack < b->min_stack) return fail(PT_E_INVALID, "pt_batch: stack shallower than a scene's BVH needs");
+    if (c.wave_paths < b->max_npix || c.wave_paths > (1 << 26)) return fail(PT_E_INVALID, "pt_batch: wave_paths must hold the largest film and be <= 2^26");
+    if (c.trace_blocks_per_cu < 1 || c.trace_blocks_per_cu > 32 || c.shade_blocks_per_cu < 1 || c.shade_blocks_per_cu > 32)
+        return fail(PT_E_INVALID, "pt_batch: blocks per CU must be in [1, 32]");
+    return PT_OK;
+}
+
+extern "C" {
+
+pt_status pt_batch_create(pt_ctx* ctx, int32_t n, const pt_scene_desc* scenes, const pt_program* programs,
+                          const pt_params* params, pt_batch** out) {
+    if (!ctx || !scenes || !programs || !params || !out) return fail(PT_E_INVALID, "pt_batch_create: null argument");
+    if (n <= 0) return fail(PT_E_INVALID, "pt_batch_create: no scenes");
+    const pt_program& p0 = programs[0];
+    for (int s = 0; s < n; s++) {
+        const pt_program& p = programs[s];
+        if (p.abi_version != PT_ABI_VERSION || scenes[s].abi_version != PT_ABI_VERSION) return fail(PT_E_INVALID, "pt_batch_create: ABI version mismatch");
+        if (p.estimator != p0.estimator || p.max_bounces != p0.max_bounces || p.rr_start != p0.rr_start || p.rr_max_survival != p0.rr_max_survival)
+            return fail(PT_E_UNSUPPORTED, "pt_batch_create: every program must share estimator, maxBounces and Russian roulette");
+        if (p.color != PT_COLOR_RGB) return fail(PT_E_UNSUPPORTED, "pt_batch_create: batches run RGB programs");
+        if (p.width <= 0 || p.height <= 0) return fail(PT_E_INVALID, "pt_batch_create: bad film size");
+        pt_status st = validate_scene(&scenes[s]);
+        if (st != PT_OK) return st;
+        for (int m = 0; m < scenes[s].n_materials; m++)
+            if (scenes[s].materials[m].has_medium || scenes[s].materials[m].model == PT_MAT_NULL)
+                return fail(PT_E_UNSUPPORTED, "pt_batch_create: batches run scenes without participating media");
+    }
+    HIP_TRY(hipSetDevice(ctx->device));
+    auto* b = new pt_batch();
+    b->ctx = ctx;
+    std::vector<DevScene> devs;
+    for (int s = 0; s < n; s++) {
+        auto store = std::make_unique<SceneStore>();
+        pt_status st = build_scene(&scenes[s], ctx->stream, *store);
+        if (st != PT_OK) { delete b; return st; }
+        b->min_stack = std::max(b->min_stack, store->stack);
+        b->any_ggx = b->any_ggx || store->has_ggx;
+        b->any_dielectric = b->any_dielectric || store->dev.has_transmission;
+        devs.push_back(store->dev);
+        b->stores.push_back(std::move(store));
+        b->progs.push_back(programs[s]);
+        b->cams.push_back(make_camera(&params[s]));
+        b->film_off.push_back(b->film_pixels);
+        b->film_pixels += (long long)programs[s].width * programs[s].height;
+        b->max_npix = std::max(b->max_npix, programs[s].width * programs[s].height);
+    }
+    struct { DevBuf* d; const void* h; size_t bytes; } up[] = {
+        {&b->d_scenes, devs.data(), devs.size() * sizeof(DevScene)}, {&b->d_cams, b->cams.data(), b->cams.size() * sizeof(Camera)}};
+    for (auto& u : up) {
+        u.d->bytes = u.bytes;
+        if (hipMalloc(&u.d->p, u.bytes) != hipSuccess || hipMemcpy(u.d->p, u.h, u.bytes, hipMemcpyHostToDevice) != hipSuccess) {
+            delete b; return fail(PT_E_OOM, "pt_batch_create: device allocation failed");
+        }
+    }
+    b->films.bytes = (size_t)b->film_pixels * sizeof(float4);
+    if (hipMalloc(&b->films.p, b->films.bytes) != hipSuccess) { delete b; return fail(PT_E_OOM, "pt_batch_create: film allocation failed"); }
+    b->r = new pt_renderer();
+    b->r->ctx = ctx;
+    b->r->prog = p0;
+    b->cfg.wave_paths = std::max(b->max_npix, 1 << 23);
+    b->cfg.trace_block = 128;
+    b->cfg.trace_blocks_per_cu = 8;
+    b->cfg.shade_blocks_per_cu = 8;
+    b->cfg.stack = b->min_stack;
+    b->cfg.use_graph = 1;
+    *out = b;
+    return PT_OK;
+}
+
+void pt_batch_destroy(pt_batch* b) {
+    if (!b) return;
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipStreamSynchronize(b->ctx->stream);
+    delete b;
+}
+
+pt_status pt_batch_set_config(pt_batch* b, const pt_batch_config* cfg) {
+    if (!b || !cfg) return fail(PT_E_INVALID, "pt_batch_set_config: null argument");
+    pt_status st = batch_check_config(b, *cfg);
+    if (st != PT_OK) return st;
+    b->cfg = *cfg;
+    return PT_OK;
+}
+
+pt_status pt_batch_get_config(const pt_batch* b, pt_batch_config* cfg) {
+    if (!b || !cfg) return fail(PT_E_INVALID, "pt_batch_get_config: null argument");
+    *cfg = b->cfg;
+    return PT_OK;
+}
+
+pt_status pt_batch_render(pt_batch* b, const uint32_t* spp) {
+    if (!b || !spp) return fail(PT_E_INVALID, "pt_batch_render: null argument");
+    pt_ctx* ctx = b->ctx;
+    HIP_TRY(hipSetDevice(ctx->device));
+    std::vector<uint32_t> sv(spp, spp + b->stores.size());
+    const bool reuse = b->cfg.use_graph && b->exec && sv == b->graph_spp && same_cfg(b->cfg, b->graph_cfg);
+    if (!reuse) {
+        b->drop_graph();
+        pt_status st = batch_schedule(b, spp);
+        if (st != PT_OK) return st;
+        if ((st = ensure_paths(b->r, b->cfg.wave_paths)) != PT_OK) return st;
+        b->r->paths.scenes = (const DevScene*)b->d_scenes.p;
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        if (b->d_entries.p) { HIP_TRY(hipFree(b->d_entries.p)); b->d_entries.p = nullptr; }
+        b->d_entries.bytes = std::max<size_t>(b->entries.size(), 1) * sizeof(BatchEntry);
+        HIP_TRY(hipMalloc(&b->d_entries.p, b->d_entries.bytes));
+        if (!b->entries.empty())
+            HIP_TRY(hipMemcpy(b->d_entries.p, b->entries.data(), b->entries.size() * sizeof(BatchEntry), hipMemcpyHostToDevice));
+    }
+    const bool timing = ctx->timing;
+    if (b->cfg.use_graph) {
+        if (!reuse) {
+            ctx->timing = false;   // no event records inside the captured graph
+            HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+            batch_launch(b);
+            hipError_t e = hipStreamEndCapture(ctx->stream, &b->graph);
+            ctx->timing = timing;
+            if (e != hipSuccess) return fail(PT_E_DEVICE, std::string("pt_batch_render: graph capture: ") + hipGetErrorString(e));
+            HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+            b->graph_spp = sv;
+            b->graph_cfg = b->cfg;
+        }
+        HIP_TRY(hipGraphLaunch(b->exec, ctx->stream));
+    } else {
+        batch_launch(b);
+    }
+    HIP_TRY(hipGetLastError());
+    return PT_OK;
+}
+
+pt_status pt_batch_read_export(pt_batch* b, int32_t scene, float* out, size_t out_floats) {
+    if (!b || !out) return fail(PT_E_INVALID, "pt_batch_read_export: null argument");
+    if (scene < 0 || scene >= (int32_t)b->stores.size()) return fail(PT_E_INVALID, "pt_batch_read_export: no such scene");
+    const size_t need = (size_t)b->progs[scene].width * b->progs[scene].height * 4;
+    if (out_floats < need) return fail(PT_E_INVALID, "pt_batch_read_export: output buffer too small");
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    HIP_TRY(hipStreamSynchronize(b->ctx->stream));
+    HIP_TRY(hipMemcpy(out, (const float4*)b->films.p + b->film_off[scene], need * sizeof(float), hipMemcpyDeviceToHost));
+    return PT_OK;
+}
+
+pt_status pt_batch_autotune(pt_batch* b, const uint32_t* spp, uint32_t probe_div, pt_batch_config* best,
+                            pt_batch_config* results, float* ms, int32_t max_results, int32_t* n_results) {
+    if (!b || !spp) return fail(PT_E_INVALID, "pt_batch_autotune: null argument");
+    pt_ctx* ctx = b->ctx;
+    HIP_TRY(hipSetDevice(ctx->device));
+    std::vector<uint32_t> probe(b->stores.size());
+    for (size_t s = 0; s < probe.size(); s++) probe[s] = std::max<uint32_t>(1, spp[s] / std::max<uint32_t>(probe_div, 1));
+    const pt_batch_config start = b->cfg;
+    pt_batch_config winner = start;
+    float best_ms = INFINITY;
+    int count = 0;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    const int waves[] = {1 << 21, 1 << 22, 1 << 23};
+    const int tbs[] = {64, 128, 256};
+    const int bpcs[] = {4, 8};
+    const int stacks[] = {16, 32};
+    for (int wp : waves)
+        for (int tb : tbs)
+            for (int bpc : bpcs)
+                for (int sk : stacks) {
+                    pt_batch_config c = start;
+                    c.wave_paths = std::max(wp, b->max_npix);
+                    c.trace_block = tb;
+                    c.trace_blocks_per_cu = std::max(1, bpc * 128 / tb);   // bpc x 128 lanes per CU
+                    c.stack = sk;
+                    c.use_graph = 1;
+                    if (batch_check_config(b, c) != PT_OK) continue;
+                    b->cfg = c;
+                    pt_status st = pt_batch_render(b, probe.data());       // build + warm
+                    if (st != PT_OK) { (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); return st; }
+                    HIP_TRY(hipEventRecord(e0, ctx->stream));
+                    st = pt_batch_render(b, probe.data());
+                    if (st != PT_OK) { (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); return st; }
+                    HIP_TRY(hipEventRecord(e1, ctx->stream));
+                    HIP_TRY(hipEventSynchronize(e1));
+                    float t = 0.0f;
+                    HIP_TRY(hipEventElapsedTime(&t, e0, e1));
+                    if (results && ms && count < max_results) { results[count] = c; ms[count] = t; }
+                    count++;
+                    if (t < best_ms) { best_ms = t; winner = c; }
+                }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    b->cfg = winner;
+    b->drop_graph();
+    if (best) *best = winner;
+    if (n_results) *n_results = count;
     return PT_OK;
 }
 

@@ -375,3 +375,25 @@ def inverse_adjoint_materials(comp):
     """(medium material index, albedo material indices) of a compiled inverse_media_scene."""
     names = comp.material_names
     return names.index("glass"), [names.index(f"patch{k}") for k in range(INVERSE_N_ALBEDO)]
+
+
+# --------------------------------------------------- batched wavefront workload (C3)
+
+def batch_workload(n_scenes=64, seed=99, tri_range=(1000, 500000), res_range=(128, 1024), spp_range=(4, 256)):
+    """Seeded scenes of the icosphere-cloud kind with log-uniform triangle count, square
+    resolution and spp: returns [(scene, width, height, spp)]. Triangle counts are whole
+    detail-4 icospheres (500 triangles each); the cloud's box scales with the cube root of
+    the sphere count so density stays comparable."""
+    rng = np.random.default_rng(seed)
+    out = []
+    lg = lambda lo, hi: float(np.exp(rng.uniform(np.log(lo), np.log(hi))))
+    for k in range(n_scenes):
+        tris = lg(*tri_range)
+        n_spheres = max(2, int(round(tris / 500.0)))
+        res = int(round(lg(*res_range)))
+        spp = int(round(lg(*spp_range)))
+        box = 100.0 * (n_spheres / 2000.0) ** (1.0 / 3.0)
+        sc = icosphere_cloud(n_spheres=n_spheres, seed=1000 + k, box=box)
+        sc["id"] = f"batch-{k}"
+        out.append((sc, res, res, spp))
+    return out

@@ -71,6 +71,12 @@ def test_tan_pow_tanh_atanh(oracle):
     # exp(y log x) carries the rounding of y*log(x): |y ln x| * 2^-23 relative, plus a few ulp
     bound = (np.abs(5.0 * np.log(np.maximum(b.astype(np.float64), 1e-30))) + 4.0) * 1.2e-7
     assert (np.abs(got - ref) <= bound * ref + 1e-37).all()
+    # negative base: integer exponents give the real power, others NaN (GLSL: undefined)
+    neg = run(oracle, 6, np.array([-1.1920929e-07, -0.5, -2.0, -0.5], np.float32), np.array([5.0, 2.0, 3.0, 0.5], np.float32))
+    tiny = float(np.float32(1.1920929e-07)) ** 5
+    assert neg[0] < 0.0 and abs(neg[0] + tiny) <= (abs(5.0 * np.log(1.1920929e-07)) + 4.0) * 1.2e-7 * tiny
+    np.testing.assert_allclose(neg[1:3], [0.25, -8.0], rtol=2e-6)
+    assert np.isnan(neg[3])
     t = rng.uniform(-5, 5, 50000).astype(np.float32)
     assert np.abs(run(oracle, 9, t) - np.tanh(t.astype(np.float64))).max() < 3e-7
     a = rng.uniform(-0.99, 0.99, 50000).astype(np.float32)
