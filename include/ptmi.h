@@ -164,6 +164,10 @@ enum { PT_EST_PT = 0, PT_EST_NEE = 1, PT_EST_MIS = 2 };
  * (src/components/color/spectral/visible.glsl, uniform.glsl). */
 enum { PT_COLOR_RGB = 0, PT_COLOR_SPECTRAL = 1 };
 enum { PT_WL_VISIBLE = 0, PT_WL_UNIFORM = 1 };
+/* Medium NEE placement: at the scattering vertex (techniques/light/light_medium.glsl) or once
+ * per medium segment by equiangular sampling toward a delta light (Kulla-Fajardo 2012,
+ * techniques/equiangular/equiangular.glsl; pt-nee with point lights only). */
+enum { PT_MLS_VERTEX = 0, PT_MLS_EQUIANGULAR = 1 };
 
 typedef struct pt_program {
     uint32_t abi_version;
@@ -176,6 +180,9 @@ typedef struct pt_program {
     int32_t  frames_in_flight;/* samples per pixel traced per wave of paths (>= 1) */
     int32_t  max_null_crossings; /* measurement.maxNullCrossings (32 default) */
     int32_t  wavelengths;     /* PT_WL_* (spectral only) */
+    int32_t  medium_light_sampling; /* PT_MLS_*: estimator.mediumLightSampling */
+    int32_t  variance;        /* != 0: also accumulate the variance picture (export "variance",
+                                 components/accumulator/variance/variance.glsl) */
     int32_t  pad;
 } pt_program;
 
@@ -229,7 +236,9 @@ void      pt_renderer_unload(pt_renderer* r);
 pt_status pt_render_frames(pt_renderer* r, const pt_params* params, uint32_t nframes);
 
 /* Engine.readExport('radiance') (Engine.ts:351-398): RGBA32F film, row 0 = bottom
- * (gl_FragCoord convention), width*height*4 floats. Synchronises the stream. */
+ * (gl_FragCoord convention), width*height*4 floats. Synchronises the stream. "variance"
+ * reads the Welford statistic picture (per-channel population variance of the samples) of a
+ * program compiled with variance != 0. */
 pt_status pt_read_export(pt_renderer* r, const char* name, float* out, size_t out_floats);
 
 /* Engine.clearAccumulation / pinResetSalt (Engine.ts:400-436). */

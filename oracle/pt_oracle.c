@@ -1172,6 +1172,7 @@ typedef struct {
     int estimator, max_bounces, rr_start; float rr_max;
     int width, height, max_null;
     int spectral, wl_uniform;
+    int equiangular;          /* estimator.mediumLightSampling == 'equiangular' */
 } or_program;
 
 static inline float power_heuristic(float pf, float pg) {
@@ -1207,6 +1208,46 @@ static inline void adj_scatter(adj_t* A, sp c, const double* D) {
     for (int k = 6; k < ADJ_N; k++) A->G[k] += es * D[k];
 }
 
+/* lighting_query_delta (features/lighting.ts:306-336): the CDF chain over the delta lights. */
+static float lighting_query_delta(const or_scene* S, float uc, int* id) {
+    const int n = S->n_lights;
+    *id = n - 1;
+    for (int i = 0; i < n - 1; i++) if (uc < S->lights[i].cdf) { *id = i; break; }
+    return S->lights[*id].select_pdf;
+}
+/* equiangular_sample_direct (techniques/equiangular/equiangular.glsl:18-57): one direct-light
+ * vertex per medium segment, t distributed uniformly in the angle subtended at the light. */
+static sp equiangular_sample_direct(const or_scene* S, const or_program* P, v3 o, v3 d, sp T, const pt_material* Mm,
+                                    float t_max, int crossings_left, rng_t* rng, int* n_any, const wl_t* wl) {
+    int id;
+    float select_pdf = lighting_query_delta(S, rnd(rng), &id);
+    const pt_light* l = &S->lights[id];
+    v3 pos = lv(l->corner);
+    sp intensity = lift(S, l->radiance, &l->s_radiance, wl);
+    v3 D = sub(pos, o);
+    float t_c = dot(D, d);
+    float h2 = fmaxf_(dot(D, D) - t_c * t_c, 1e-8f);
+    float h = sqrtf(h2);
+    float theta_a = ATAN2(0.0f - t_c, h);
+    float theta_b = ATAN2(t_max - t_c, h);
+    float u = rnd(rng);
+    float theta = theta_a * (1.0f - u) + theta_b * u;
+    float t = t_c + h * TAN(theta);
+    float pdf_t = h / ((theta_b - theta_a) * (h2 + (t - t_c) * (t - t_c)));
+    v3 p_evt = add(o, muls(d, t));
+    sp T_seg = s_exp(smuls(sneg(sadd(sigma_a_of(S, Mm, wl), sigma_s_of(S, Mm, wl))), t));
+    v3 to_light = sub(pos, p_evt);
+    float d2 = fmaxf_(dot(to_light, to_light), 1e-8f);
+    float dist = sqrtf(d2);
+    v3 wi = V(to_light.x / dist, to_light.y / dist, to_light.z / dist);
+    sp vis = shadow_transmittance_media(S, p_evt, wi, pos, crossings_left, n_any, wl, -2, NULL);
+    if (s_black(wl, vis)) return sgrey(0.0f);
+    float f = hg_value(wi, neg(d), Mm->hg_g);
+    sp c = smul(smul(smul(smul(T, T_seg), sigma_s_of(S, Mm, wl)), sgrey(f)), sdivs(intensity, d2));
+    c = smul(c, vis);
+    return sdivs(c, pdf_t * select_pdf);
+}
+
 static sp transport_trace(const or_scene* S, const or_program* P, v3 ro, v3 rd, rng_t* rng, int* n_closest, int* n_any,
                           const wl_t* wl, adj_t* A) {
     sp T = sgrey(1.0f), L = sgrey(0.0f);
@@ -1225,6 +1266,9 @@ static sp transport_trace(const or_scene* S, const or_program* P, v3 ro, v3 rd, 
             int med_mat = material_of(S, current_medium);
             if (material_has_medium(S, med_mat)) {
                 const pt_material* Mm = &S->materials[med_mat];
+                if (P->equiangular && bounce < P->max_bounces)   /* per-segment direct light */
+                    L = sadd(L, equiangular_sample_direct(S, P, ro, rd, T, Mm, boundary ? hit.t : MAX_DIST,
+                                                          S->has_null ? P->max_null - null_crossings : 0, rng, n_any, wl));
                 float x0 = rnd(rng), x1 = rnd(rng);
                 msample_t ms = medium_sample_analytic(S, Mm, boundary ? hit.t : MAX_DIST, x0, x1, wl);
                 T = smul(T, ms.weight);
@@ -1241,7 +1285,7 @@ static sp transport_trace(const or_scene* S, const or_program* P, v3 ro, v3 rd, 
                     v3 p_evt = add(ro, muls(rd, ms.t));
                     v3 wo_med = neg(rd);
                     float g = Mm->hg_g;
-                    if (nee) {   /* light_sample_direct_medium (techniques/light/light_medium.glsl:7-18) */
+                    if (nee && !P->equiangular) {   /* light_sample_direct_medium (techniques/light/light_medium.glsl:7-18) */
                         float y0 = rnd(rng), y1 = rnd(rng);
                         lsample_t ls = lighting_sample(S, p_evt, y0, y1, wl);
                         if (ls.pdf > 0.0f) {
@@ -1541,6 +1585,7 @@ void or_render(const or_scene* S, const pt_program* prog, const pt_params* par, 
     P.rr_max = prog->rr_max_survival; P.width = prog->width; P.height = prog->height;
     P.max_null = prog->max_null_crossings;
     P.spectral = prog->color == PT_COLOR_SPECTRAL; P.wl_uniform = prog->wavelengths == PT_WL_UNIFORM;
+    P.equiangular = prog->medium_light_sampling == PT_MLS_EQUIANGULAR;
     if (P.spectral && !S->has_spectral) { if (counts) { counts[0] = counts[1] = 0; } return; }
     int W = P.width, H = P.height;
     uint64_t nc = 0, na = 0;
@@ -1668,7 +1713,7 @@ void or_adjoint(const or_scene* S, const pt_program* prog, const pt_params* par,
     or_program P;
     P.estimator = prog->estimator; P.max_bounces = prog->max_bounces; P.rr_start = prog->rr_start;
     P.rr_max = prog->rr_max_survival; P.width = prog->width; P.height = prog->height;
-    P.max_null = prog->max_null_crossings; P.spectral = 0; P.wl_uniform = 0;
+    P.max_null = prog->max_null_crossings; P.spectral = 0; P.wl_uniform = 0; P.equiangular = 0;
     const int W = P.width, H = P.height;
     double* rows = (double*)calloc((size_t)H * ADJ_N, sizeof(double));
     v3 F = lv(par->cam_forward), R = lv(par->cam_right), U = lv(par->cam_up), O = lv(par->cam_position);

@@ -16,6 +16,7 @@ ENV_NONE, ENV_SKY = 0, 1
 EST_PT, EST_NEE, EST_MIS = 0, 1, 2
 COLOR_RGB, COLOR_SPECTRAL = 0, 1
 WL_VISIBLE, WL_UNIFORM = 0, 1
+MLS_VERTEX, MLS_EQUIANGULAR = 0, 1
 SPEC_ZERO, SPEC_GREY, SPEC_SIGMOID, SPEC_BLACKBODY = 0, 1, 2, 3
 
 SPECTRUM_DTYPE = np.dtype([
@@ -66,7 +67,8 @@ class PtProgram(C.Structure):
     _fields_ = [("abi_version", C.c_uint32), ("estimator", C.c_int32), ("max_bounces", C.c_int32),
                 ("rr_start", C.c_int32), ("rr_max_survival", C.c_float), ("color", C.c_int32),
                 ("width", C.c_int32), ("height", C.c_int32), ("frames_in_flight", C.c_int32),
-                ("max_null_crossings", C.c_int32), ("wavelengths", C.c_int32), ("pad", C.c_int32)]
+                ("max_null_crossings", C.c_int32), ("wavelengths", C.c_int32), ("medium_light_sampling", C.c_int32),
+                ("variance", C.c_int32), ("pad", C.c_int32)]
 
 
 class PtParams(C.Structure):

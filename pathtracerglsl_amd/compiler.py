@@ -34,6 +34,8 @@ class RenderStrategy:
     max_null_crossings: int = 32          # measurement.maxNullCrossings (plan/measurement.ts:12)
     color: str = "rgb"                    # measurement.color: 'rgb' | 'spectral'
     wavelengths: str = "visible"          # estimator.wavelengths (spectral): 'visible' | 'uniform'
+    medium_light_sampling: str = "vertex" # estimator.mediumLightSampling: 'vertex' | 'equiangular'
+    variance: bool = False                # accumulate the variance statistic picture
 
 
 @dataclass
@@ -90,6 +92,16 @@ def _analyze_validate(scene, strategy):
         raise CompileError(f"unknown color model '{strategy.color}'")
     if strategy.wavelengths not in ("visible", "uniform"):
         raise CompileError(f"unknown wavelength sampler '{strategy.wavelengths}'")
+    if strategy.medium_light_sampling not in ("vertex", "equiangular"):
+        raise CompileError(f"unknown mediumLightSampling '{strategy.medium_light_sampling}'")
+    if strategy.medium_light_sampling == "equiangular":
+        # Validator.ts:506-579: equiangular placement runs under pt-nee with delta lights only
+        if strategy.direct_lighting != "nee":
+            raise CompileError("mediumLightSampling 'equiangular' requires directLighting 'nee'")
+        if not scene.get("lights") or any(l["type"] != "point" for l in scene.get("lights", [])):
+            raise CompileError("mediumLightSampling 'equiangular' requires every light to be a point light")
+        if scene.get("environment", {"type": "none"}).get("type", "none") != "none":
+            raise CompileError("mediumLightSampling 'equiangular' does not combine with an environment light")
     if strategy.color == "spectral" and scene.get("environment", {"type": "none"}).get("type", "none") != "none":
         raise CompileError("spectral programs with an environment light are not supported by this backend")
     mats = scene.get("materials", {})
@@ -274,6 +286,8 @@ def compile_scene(scene, strategy=None, width=256, height=256, reset_salt=1234):
     prog.rr_max_survival = strategy.rr_max_survival
     prog.color = abi.COLOR_SPECTRAL if spec else abi.COLOR_RGB
     prog.wavelengths = abi.WL_UNIFORM if strategy.wavelengths == "uniform" else abi.WL_VISIBLE
+    prog.medium_light_sampling = abi.MLS_EQUIANGULAR if strategy.medium_light_sampling == "equiangular" else abi.MLS_VERTEX
+    prog.variance = int(bool(strategy.variance))
     prog.width, prog.height = width, height
     prog.frames_in_flight = max(1, strategy.frames_in_flight)
     prog.max_null_crossings = strategy.max_null_crossings

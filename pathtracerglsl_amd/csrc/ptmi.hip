@@ -456,15 +456,21 @@ PT_DEV int block_append(int* const* counters, int which, int* s_cnt, int* s_base
 // channel-MIS distance sample of pt.ts's media branch, which may end the segment in a
 // scattering event), then the shading category of the vertex.
 template <bool MEDIA, bool SPEC, bool ADJ, bool BATCH = false>
-__global__ void __launch_bounds__(SBLOCK) k_bin(DevScene Sk, Paths P, int qsel) {
+__global__ void __launch_bounds__(SBLOCK) k_bin(DevScene Sk, Paths P, int qsel, int equiangular, int max_bounces, int max_null) {
     __shared__ int s_cnt[N_CAT], s_base[N_CAT];
     int* ctr[N_CAT];
     for (int c = 0; c < N_CAT; c++) ctr[c] = &P.counts[C_CAT0 + c];
+    int* sctr[1] = {&P.counts[C_SHADOW]};
     const int n = P.counts[qsel];
     const int* q = P.q[qsel];
     for (int base = blockIdx.x * SBLOCK; base < n; base += gridDim.x * SBLOCK) {
         int k = base + threadIdx.x;
         int cat = -1, path = 0;
+        bool eq_shadow = false;
+        f3 eq_o = mk3(0, 0, 0), eq_wi = mk3(0, 0, 0), eq_pos = mk3(0, 0, 0);
+        sp4 eq_a = sgrey(0.0f);
+        float eq_pdf = 1.0f;
+        int eq_cross = 0;
         if (k < n) {
             path = q[k];
             const DevScene& S = scene_of<BATCH>(Sk, P, path);
@@ -475,6 +481,47 @@ __global__ void __launch_bounds__(SBLOCK) k_bin(DevScene Sk, Paths P, int qsel) 
                 if (material_has_medium(S, med_mat)) {
                     uint2 rs = P.rng[path];
                     Rng rng; rng.base = rs.x; rng.dim = rs.y;
+                    const float t_max = prim >= 0 ? hv.x : MAX_DIST;
+                    const int fl = P.flags[path];
+                    if (!ADJ && !BATCH && equiangular && fl_bounce(fl) < max_bounces) {
+                        // equiangular_sample_direct (techniques/equiangular/equiangular.glsl:18-57): one
+                        // direct-light vertex per medium segment, from segment-start throughput; the
+                        // shadow walker runs before this iteration's shading, in the reference's order.
+                        const pt_material& Mm = S.materials[med_mat];
+                        const Wl wl = load_wl<SPEC>(P, path);
+                        const float uc = rng.next();
+                        int id = S.n_lights - 1;
+                        for (int li = 0; li < S.n_lights - 1; li++) if (uc < S.lights[li].cdf) { id = li; break; }
+                        const pt_light& l = S.lights[id];
+                        const float select_pdf = l.select_pdf;
+                        const f3 pos = ld3(l.corner);
+                        const sp4 intensity = lift<SPEC>(S, l.radiance, l.s_radiance, wl);
+                        const f3 o = xyz(P.ray_o[path]), d = xyz(P.ray_d[path]);
+                        const f3 D = pos - o;
+                        const float t_c = dot(D, d);
+                        const float h2 = gmax(dot(D, D) - t_c * t_c, 1e-8f);
+                        const float h = sqrtf(h2);
+                        const float theta_a = ptm_atan2(0.0f - t_c, h);
+                        const float theta_b = ptm_atan2(t_max - t_c, h);
+                        const float u = rng.next();
+                        const float theta = glsl_mix(theta_a, theta_b, u);
+                        const float t = t_c + h * ptm_tan(theta);
+                        const float pdf_t = h / ((theta_b - theta_a) * (h2 + (t - t_c) * (t - t_c)));
+                        const f3 p_evt = o + d * t;
+                        sp4 Tp; float etap;
+                        load_thr<SPEC>(P, path, Tp, etap);
+                        const sp4 T_seg = spectrum_exp((-(sigma_a_of<SPEC>(S, Mm, wl) + sigma_s_of<SPEC>(S, Mm, wl))) * t);
+                        const f3 to_light = pos - p_evt;
+                        const float d2 = gmax(dot(to_light, to_light), 1e-8f);
+                        const float dist = sqrtf(d2);
+                        eq_wi = mk3(to_light.x / dist, to_light.y / dist, to_light.z / dist);
+                        const float f = hg_value(eq_wi, -d, Mm.hg_g);
+                        eq_a = (((Tp * T_seg) * sigma_s_of<SPEC>(S, Mm, wl)) * sgrey(f)) * (intensity / d2);
+                        eq_pdf = pdf_t * select_pdf;
+                        eq_o = p_evt; eq_pos = pos;
+                        eq_cross = S.has_null ? max_null - fl_nulls(fl) : 0;
+                        eq_shadow = true;
+                    }
                     float x0 = rng.next(), x1 = rng.next();
                     MSample ms = medium_sample_analytic<SPEC>(S, S.materials[med_mat], prim >= 0 ? hv.x : MAX_DIST, x0, x1,
                                                               load_wl<SPEC>(P, path));
@@ -507,6 +554,25 @@ __global__ void __launch_bounds__(SBLOCK) k_bin(DevScene Sk, Paths P, int qsel) 
         }
         int slot = block_append<N_CAT>(ctr, cat, s_cnt, s_base);
         if (cat >= 0) P.catq[cat][slot] = path;
+        if (MEDIA && !ADJ && !BATCH && equiangular) {
+            int ss = block_append<1>(sctr, eq_shadow ? 0 : -1, s_cnt, s_base);
+            if (eq_shadow) {   // the shadow walker adds ((a * vis) * 1) / pdf
+                P.s_path[ss] = path;
+                P.s_o[ss] = f4(eq_o, __int_as_float(eq_cross));
+                P.s_d[ss] = f4(eq_wi, eq_pdf);
+                P.s_c[ss] = SPEC ? st4(eq_a) : make_float4(eq_a.x, eq_a.y, eq_a.z, 1.0f);
+                P.s_l[ss] = f4(eq_pos, 1.0f);
+            }
+        }
+    }
+}
+
+// After the equiangular shadow rays of an iteration: count them and empty the shadow queue
+// for the surface NEE of the same iteration.
+__global__ void k_shadow_reset(Paths P) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        P.hist[1] += (unsigned long long)P.counts[C_SHADOW];
+        P.counts[C_SHADOW] = 0;
     }
 }
 
@@ -529,7 +595,7 @@ __global__ void k_bookkeep(Paths P, int cur, int next, int first) {
 // category. MEDIA selects the media program (region tracking, medium NEE shadow walker).
 template <int EST, int CAT, int STACK, bool MEDIA, bool SPEC, bool ADJ, bool BATCH = false>
 __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene Sk, Paths P, int qsel, int max_bounces,
-                                                  int rr_start, float rr_max, int max_null) {
+                                                  int rr_start, float rr_max, int max_null, int equiangular = 0) {
     constexpr bool PROBE = CAT == CAT_DIELECTRIC || MEDIA;
     __shared__ uint2 stk[PROBE && CAT != CAT_MISS && CAT != CAT_MEDIUM ? STACK * SBLOCK : 1];
     __shared__ int s_cnt[2], s_base[2];
@@ -594,7 +660,7 @@ __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene Sk, Paths P, int qsel
                     f3 wo_med = -rd;
                     uint2 rs = P.rng[i];
                     Rng rng; rng.base = rs.x; rng.dim = rs.y;
-                    if (NEE) {   // light_sample_direct_medium
+                    if (NEE && !equiangular) {   // light_sample_direct_medium
                         float y0 = rng.next(), y1 = rng.next();
                         LSample ls = lighting_sample<SPEC>(S, p_evt, y0, y1, wl);
                         if (ls.pdf > 0.0f) {
@@ -815,10 +881,11 @@ __global__ void __launch_bounds__(SBLOCK) k_adj_atomic(const Paths P, int n_path
 // (color/spectral/spectral.glsl:27-33) with the cie1931 sensor first.
 template <bool SPEC>
 __global__ void __launch_bounds__(SBLOCK) k_accumulate(const DevScene S, const Paths P, float4* film, int npix, int frames,
-                                                       uint32_t sample0, int wl_uniform) {
+                                                       uint32_t sample0, int wl_uniform, float4* var) {
     int pix = blockIdx.x * blockDim.x + threadIdx.x;
     if (pix >= npix) return;
     float4 acc = film[pix];
+    float4 v = var ? var[pix] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     for (int f = 0; f < frames; f++) {
         uint32_t n = sample0 + (uint32_t)f;
         const size_t i = (size_t)f * npix + pix;
@@ -833,13 +900,23 @@ __global__ void __launch_bounds__(SBLOCK) k_accumulate(const DevScene S, const P
             px = px * 0.25f;
             x = make_float4(px.x, px.y, px.z, 0.0f);
         }
+        const float4 prev = acc;
         if (n == 0) acc = make_float4(x.x, x.y, x.z, 1.0f);
         else {
             float w = 1.0f / ((float)n + 1.0f);
             acc = make_float4(acc.x * (1.0f - w) + x.x * w, acc.y * (1.0f - w) + x.y * w, acc.z * (1.0f - w) + x.z * w, 1.0f);
         }
+        if (var) {   // variance_update (components/accumulator/variance/variance.glsl:10-16)
+            if (n == 0) v = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+            else {
+                float w = 1.0f / ((float)n + 1.0f);
+                float dx = (x.x - prev.x) * (x.x - acc.x), dy = (x.y - prev.y) * (x.y - acc.y), dz = (x.z - prev.z) * (x.z - acc.z);
+                v = make_float4(v.x * (1.0f - w) + dx * w, v.y * (1.0f - w) + dy * w, v.z * (1.0f - w) + dz * w, 1.0f);
+            }
+        }
     }
     film[pix] = acc;
+    if (var) var[pix] = v;
 }
 
 template <int STACK, bool COUNT>
@@ -981,7 +1058,7 @@ struct pt_ctx {
 struct pt_renderer {
     pt_ctx* ctx = nullptr;
     pt_program prog{};
-    DevBuf film;
+    DevBuf film, var_film;
     uint32_t sample_count = 0;
     long long cap = 0;    // path slots allocated
     DevBuf b_ray_o, b_ray_d, b_thr, b_rad, b_rng, b_prev, b_flags, b_medium, b_hit, b_q0, b_q1, b_counts,
@@ -1269,6 +1346,10 @@ pt_status pt_renderer_load(pt_ctx* ctx, const pt_program* prog, pt_renderer** ou
     if (prog->estimator < PT_EST_PT || prog->estimator > PT_EST_MIS) return fail(PT_E_UNSUPPORTED, "unknown estimator");
     if (prog->color != PT_COLOR_RGB && prog->color != PT_COLOR_SPECTRAL) return fail(PT_E_UNSUPPORTED, "unknown color model");
     if (prog->wavelengths != PT_WL_VISIBLE && prog->wavelengths != PT_WL_UNIFORM) return fail(PT_E_UNSUPPORTED, "unknown wavelength sampler");
+    if (prog->medium_light_sampling != PT_MLS_VERTEX && prog->medium_light_sampling != PT_MLS_EQUIANGULAR)
+        return fail(PT_E_UNSUPPORTED, "unknown medium light sampling");
+    if (prog->medium_light_sampling == PT_MLS_EQUIANGULAR && prog->estimator != PT_EST_NEE)
+        return fail(PT_E_UNSUPPORTED, "equiangular medium NEE runs under the nee estimator");
     if (prog->color == PT_COLOR_SPECTRAL && ctx->has_scene && ctx->dev.env_type != PT_ENV_NONE)
         return fail(PT_E_UNSUPPORTED, "spectral programs with an environment light are not provided by this build");
     if (prog->width <= 0 || prog->height <= 0 || prog->max_bounces < 0) return fail(PT_E_INVALID, "bad film size or bounce budget");
@@ -1279,6 +1360,10 @@ pt_status pt_renderer_load(pt_ctx* ctx, const pt_program* prog, pt_renderer** ou
     if (r->prog.frames_in_flight < 1) r->prog.frames_in_flight = 1;
     r->film.bytes = (size_t)prog->width * prog->height * sizeof(float4);
     if (hipMalloc(&r->film.p, r->film.bytes) != hipSuccess) { delete r; return fail(PT_E_OOM, "film allocation failed"); }
+    if (prog->variance) {
+        r->var_film.bytes = r->film.bytes;
+        if (hipMalloc(&r->var_film.p, r->var_film.bytes) != hipSuccess) { delete r; return fail(PT_E_OOM, "film allocation failed"); }
+    }
     (void)hipMemsetAsync(r->film.p, 0, r->film.bytes, ctx->stream);
     *out = r;
     return PT_OK;
@@ -1337,7 +1422,7 @@ static void launch_shade(pt_renderer* r, int cur) {
     hipStream_t st = ctx->stream;
     const int g = ctx->shade_blocks;
     const int mn = pg.max_null_crossings;
-#define PT_SHADE(name, CATV) TIMED(ctx, name, 0, hipLaunchKernelGGL((k_shade<EST, CATV, STACK, MEDIA, SPEC, ADJ>), dim3(g), dim3(SBLOCK), 0, st, S, r->paths, cur, pg.max_bounces, pg.rr_start, pg.rr_max_survival, mn))
+#define PT_SHADE(name, CATV) TIMED(ctx, name, 0, hipLaunchKernelGGL((k_shade<EST, CATV, STACK, MEDIA, SPEC, ADJ>), dim3(g), dim3(SBLOCK), 0, st, S, r->paths, cur, pg.max_bounces, pg.rr_start, pg.rr_max_survival, mn, pg.medium_light_sampling))
     PT_SHADE("shade_miss", CAT_MISS);
     PT_SHADE("shade_lambert", CAT_LAMBERT);
     if (ctx->has_ggx) PT_SHADE("shade_ggx", CAT_GGX);
@@ -1369,7 +1454,12 @@ static void launch_wave(pt_renderer* r, const Camera& cam, int frames, uint32_t 
             TIMED(ctx, "extend_count", 0, hipLaunchKernelGGL((k_extend<STACK, true>), dim3(tgrid), dim3(TBLOCK), 0, st, S, P, cur));
         else
             TIMED(ctx, "extend", 0, hipLaunchKernelGGL((k_extend<STACK, false>), dim3(tgrid), dim3(TBLOCK), 0, st, S, P, cur));
-        TIMED(ctx, "bin", 0,   hipLaunchKernelGGL((k_bin<MEDIA, SPEC, ADJ>), dim3(ctx->shade_blocks), dim3(SBLOCK), 0, st, S, P, cur));
+        const int eq = MEDIA && !ADJ && pg.medium_light_sampling == PT_MLS_EQUIANGULAR ? 1 : 0;
+        TIMED(ctx, "bin", 0, hipLaunchKernelGGL((k_bin<MEDIA, SPEC, ADJ>), dim3(ctx->shade_blocks), dim3(SBLOCK), 0, st, S, P, cur, eq, pg.max_bounces, pg.max_null_crossings));
+        if (eq) {
+            TIMED(ctx, "shadow_media", 0, hipLaunchKernelGGL((k_shadow_media<STACK, SPEC, ADJ>), dim3(tgrid), dim3(TBLOCK), 0, st, S, P));
+            hipLaunchKernelGGL(k_shadow_reset, dim3(1), dim3(64), 0, st, P);
+        }
         switch (pg.estimator) {
             case PT_EST_PT: launch_shade<PT_EST_PT, STACK, MEDIA, SPEC, ADJ>(r, cur); break;
             case PT_EST_NEE: launch_shade<PT_EST_NEE, STACK, MEDIA, SPEC, ADJ>(r, cur); break;
@@ -1390,7 +1480,7 @@ static void launch_wave(pt_renderer* r, const Camera& cam, int frames, uint32_t 
             TIMED(ctx, "adj_reduce", n, hipLaunchKernelGGL(k_adj_atomic, dim3((unsigned)((n + SBLOCK - 1) / SBLOCK)), dim3(SBLOCK), 0, st, P, (int)n, (float*)r->b_grad.p));
         return;
     }
-    TIMED(ctx, "accumulate", npix, hipLaunchKernelGGL((k_accumulate<SPEC>), dim3((npix + SBLOCK - 1) / SBLOCK), dim3(SBLOCK), 0, st, S, P, (float4*)r->film.p, npix, frames, sample0, pg.wavelengths));
+    TIMED(ctx, "accumulate", npix, hipLaunchKernelGGL((k_accumulate<SPEC>), dim3((npix + SBLOCK - 1) / SBLOCK), dim3(SBLOCK), 0, st, S, P, (float4*)r->film.p, npix, frames, sample0, pg.wavelengths, (float4*)r->var_film.p));
 }
 
 extern "C" {
@@ -1483,6 +1573,7 @@ pt_status pt_backward(pt_renderer* r, const pt_params* prm, uint32_t nframes, co
     if (!ctx->has_scene) return fail(PT_E_MISSING_INPUT, "missing input: no scene uploaded (pt_scene_upload)");
     const pt_program& pg = r->prog;
     if (pg.color != PT_COLOR_RGB) return fail(PT_E_UNSUPPORTED, "pt_backward: the adjoint runs RGB programs only");
+    if (pg.medium_light_sampling != PT_MLS_VERTEX) return fail(PT_E_UNSUPPORTED, "pt_backward: the adjoint runs vertex medium NEE only");
     if (nframes == 0) return fail(PT_E_INVALID, "pt_backward: nframes must be >= 1");
     if (adj->reduce_mode != PT_REDUCE_ATOMIC && adj->reduce_mode != PT_REDUCE_DETERMINISTIC)
         return fail(PT_E_INVALID, "pt_backward: unknown reduce_mode");
@@ -1624,7 +1715,7 @@ static void batch_shade(pt_batch* b, int cur) {
     const int g = ctx->cu_count * b->cfg.shade_blocks_per_cu;
     Paths& P = b->r->paths;
     hipStream_t st = ctx->stream;
-#define PT_BSHADE(CATV) hipLaunchKernelGGL((k_shade<EST, CATV, STACK, false, false, false, true>), dim3(g), dim3(SBLOCK), 0, st, S0, P, cur, pg.max_bounces, pg.rr_start, pg.rr_max_survival, pg.max_null_crossings)
+#define PT_BSHADE(CATV) hipLaunchKernelGGL((k_shade<EST, CATV, STACK, false, false, false, true>), dim3(g), dim3(SBLOCK), 0, st, S0, P, cur, pg.max_bounces, pg.rr_start, pg.rr_max_survival, pg.max_null_crossings, 0)
     PT_BSHADE(CAT_MISS);
     PT_BSHADE(CAT_LAMBERT);
     if (b->any_ggx) PT_BSHADE(CAT_GGX);
@@ -1651,7 +1742,7 @@ static void batch_waves(pt_batch* b) {
         for (int it = 0; it <= pg.max_bounces; it++) {
             hipLaunchKernelGGL(k_bookkeep, dim3(1), dim3(64), 0, st, P, cur, cur ^ 1, it == 0 ? 1 : 0);
             hipLaunchKernelGGL((k_extend<STACK, false, true, TB>), dim3(tgrid), dim3(TB), 0, st, S0, P, cur);
-            hipLaunchKernelGGL((k_bin<false, false, false, true>), dim3(sgrid), dim3(SBLOCK), 0, st, S0, P, cur);
+            hipLaunchKernelGGL((k_bin<false, false, false, true>), dim3(sgrid), dim3(SBLOCK), 0, st, S0, P, cur, 0, pg.max_bounces, pg.max_null_crossings);
             switch (pg.estimator) {
                 case PT_EST_PT: batch_shade<PT_EST_PT, STACK>(b, cur); break;
                 case PT_EST_NEE: batch_shade<PT_EST_NEE, STACK>(b, cur); break;
@@ -1897,11 +1988,13 @@ pt_status pt_scene_update_materials(pt_ctx* ctx, const pt_material* materials, i
 
 pt_status pt_read_export(pt_renderer* r, const char* name, float* out, size_t out_floats) {
     if (!r || !name || !out) return fail(PT_E_INVALID, "pt_read_export: null argument");
-    if (std::strcmp(name, "radiance") != 0) return fail(PT_E_INVALID, std::string("unknown export '") + name + "'");
+    const bool var = std::strcmp(name, "variance") == 0;
+    if (std::strcmp(name, "radiance") != 0 && !var) return fail(PT_E_INVALID, std::string("unknown export '") + name + "'");
+    if (var && !r->var_film.p) return fail(PT_E_MISSING_INPUT, "missing export: 'variance' (compile the program with variance on)");
     size_t need = (size_t)r->prog.width * r->prog.height * 4;
     if (out_floats < need) return fail(PT_E_INVALID, "pt_read_export: output buffer too small");
     HIP_TRY(hipSetDevice(r->ctx->device));
-    HIP_TRY(hipMemcpyAsync(out, r->film.p, need * sizeof(float), hipMemcpyDeviceToHost, r->ctx->stream));
+    HIP_TRY(hipMemcpyAsync(out, var ? r->var_film.p : r->film.p, need * sizeof(float), hipMemcpyDeviceToHost, r->ctx->stream));
     HIP_TRY(hipStreamSynchronize(r->ctx->stream));
     return PT_OK;
 }
@@ -1910,6 +2003,7 @@ pt_status pt_clear_accumulation(pt_renderer* r) {
     if (!r) return fail(PT_E_INVALID, "pt_clear_accumulation: null renderer");
     HIP_TRY(hipSetDevice(r->ctx->device));
     HIP_TRY(hipMemsetAsync(r->film.p, 0, r->film.bytes, r->ctx->stream));
+    if (r->var_film.p) HIP_TRY(hipMemsetAsync(r->var_film.p, 0, r->var_film.bytes, r->ctx->stream));
     r->sample_count = 0;
     return PT_OK;
 }

@@ -397,3 +397,27 @@ def batch_workload(n_scenes=64, seed=99, tri_range=(1000, 500000), res_range=(12
         sc["id"] = f"batch-{k}"
         out.append((sc, res, res, spp))
     return out
+
+
+def haze():
+    """haze (tests/witnesses/scenes/mediaWitness.ts:113-147): a grey room (x, z in [-2.5, 2.5] x
+    [-2.5, 6], y in [0, 2.5]) filled with an ambient haze (sigma_a 0.02, sigma_s 0.4, g 0.7), a
+    point light (intensity 10) near the ceiling and a small emissive panel on the back wall;
+    the camera at (0, 1.2, 5) looks at (0, 1.2, -1). The vertex and equiangular medium-NEE
+    placements must converge to the same image."""
+    room = _box_tris((-2.5, 0.0, -2.5), (2.5, 2.5, 6.0))[:, [0, 2, 1]]       # inward winding
+    return {
+        "id": "haze",
+        "materials": {"gray": {"model": "lambert", "albedo": 0.5},
+                      "panel": {"model": "lambert", "albedo": 0.0, "emission": 4.0},
+                      "fog": {"model": "none", "medium": {"sigma_a": 0.02, "sigma_s": 0.4, "phase_g": 0.7}}},
+        "objects": [
+            {"type": "mesh", "triangles": room, "material": "gray", "closed": False},
+            {"type": "mesh", "triangles": _box_tris((-1.6, 0.8, -2.45), (-0.8, 1.6, -2.35)), "material": "panel",
+             "closed": True},
+        ],
+        "lights": [{"type": "point", "position": [0.8, 2.2, -1.0], "intensity": 10.0}],
+        "environment": {"type": "none"},
+        "ambient_medium": "fog",
+        "camera": {"type": "pinhole", "position": [0.0, 1.2, 5.0], "target": [0.0, 1.2, -1.0], "fov": 0.9},
+    }

@@ -71,3 +71,11 @@ def test_media_frames_in_flight_do_not_change_pixels(engine):
         engine.render_frames(5)
         films.append(engine.read_export())
     assert np.array_equal(films[0], films[1])
+
+
+@pytest.mark.parametrize("mls", ["vertex", "equiangular"])
+def test_haze_medium_nee_placements_match_oracle(engine, oracle, mls):
+    _, o, bad = _render_both(engine, oracle, f"haze-{mls}", scenes.haze(),
+                             compiler.RenderStrategy("nee", 24, 4, medium_light_sampling=mls), 32, 24, 8)
+    assert o[..., :3].max() > 0.0
+    assert bad == 0.0

@@ -138,3 +138,30 @@ def test_render_overlapping_glass_matches_oracle(engine, oracle):
     assert o[..., :3].mean() > 0.01
     assert bad == 0.0
     assert engine.ray_counts() == (int(counts[0]), int(counts[1]))
+
+
+def test_variance_picture_matches_welford_over_oracle_samples(engine, oracle):
+    """variance_update (components/accumulator/variance/variance.glsl:10-16): the GPU's
+    statistic picture equals the fp32 Welford recursion over the oracle's per-sample radiance."""
+    comp = scene_cache.compiler.compile_scene(scenes.cornell_box(), scene_cache.compiler.RenderStrategy("mis", 8, 3, variance=True),
+                                              32, 32)
+    engine.load_renderer(comp)
+    engine.clear_accumulation()
+    engine.render_frames(12)
+    gv = engine.read_export("variance")
+    ors = oracle_lib.OracleScene(oracle, comp.scene)
+    _, samples, _ = ors.render(comp.program, comp.params(), 12, keep_samples=True)
+    f32 = np.float32
+    mean = np.zeros(samples.shape[1:], f32)
+    var = np.zeros(samples.shape[1:], f32)
+    for n in range(samples.shape[0]):
+        x = samples[n]
+        prev = mean
+        if n == 0:
+            mean, var = x.copy(), np.zeros_like(x)
+        else:
+            w = f32(1.0) / (f32(n) + f32(1.0))
+            mean = mean * (f32(1.0) - w) + x * w
+            var = var * (f32(1.0) - w) + ((x - prev) * (x - mean)) * w
+    assert var.max() > 0.0
+    assert np.array_equal(gv[..., :3], var)

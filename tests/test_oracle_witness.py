@@ -73,3 +73,21 @@ def test_media_estimators_agree(oracle):
         means[est], errs[est] = per.mean(axis=0), per.std(axis=0) / np.sqrt(len(per))
     z = np.abs(means["none"] - means["mis"]) / np.sqrt(errs["none"] ** 2 + errs["mis"] ** 2)
     assert (z < 4.5).all(), (means, z)
+
+
+def test_haze_vertex_equals_equiangular(oracle):
+    """haze (tests/witnesses/index.ts:1288-1310): vertex and equiangular medium-NEE placement
+    converge to the same image (meanTol 0.02)."""
+    means = []
+    for mls in ("vertex", "equiangular"):
+        film, _ = render(oracle, scenes.haze(), compiler.RenderStrategy("nee", 24, 4, medium_light_sampling=mls), 40, 30, 192)
+        means.append(film[..., :3].mean())
+    assert abs(means[0] - means[1]) <= 0.02 * means[0], means
+
+
+def test_equiangular_validation():
+    import pytest
+    with pytest.raises(compiler.CompileError, match="nee"):
+        compiler.compile_scene(scenes.haze(), compiler.RenderStrategy("mis", 4, -1, medium_light_sampling="equiangular"), 8, 8)
+    with pytest.raises(compiler.CompileError, match="point"):
+        compiler.compile_scene(scenes.cornell_glass_medium(), compiler.RenderStrategy("nee", 4, -1, medium_light_sampling="equiangular"), 8, 8)
