@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define PT_ABI_VERSION 2u
+#define PT_ABI_VERSION 3u
 
 typedef enum pt_status {
     PT_OK = 0,
@@ -36,7 +36,18 @@ typedef enum pt_status {
 
 /* ---- scene description (the data rail: src/components/data/channels.ts:11-20) ---- */
 
-enum { PT_OBJ_MESH = 0, PT_OBJ_QUAD = 1, PT_OBJ_SPHERE = 2 };
+enum { PT_OBJ_MESH = 0, PT_OBJ_QUAD = 1, PT_OBJ_SPHERE = 2, PT_OBJ_INSTANCES = 3 };
+
+/* One placement of an instance batch (src/glsl/core/placement.glsl:1-41): q is the INVERSE
+ * rotation quaternion (world -> local, x y z w) and ts = (t_rigid, s) with t_rigid = -R^T t and
+ * s the forward uniform scale. A world ray maps to the prototype's frame as
+ * o' = (qrot(q, o) + t_rigid) / s, d' = qrot(q, d) / s (ray t preserved); normals return by
+ * the forward rotation qrot(conj(q), n). An instance batch object (PT_OBJ_INSTANCES) is ONE
+ * region for all of its placements (features/intersection.ts:981-1066 instanceLeafItem). */
+typedef struct pt_instance {
+    float q[4];
+    float ts[4];
+} pt_instance;
 
 /* One scene object = one region (src/compiler/plan: objects are indexed by region id).
  * Meshes are triangle soups in WORLD space (identity placement) over a range of the
@@ -46,11 +57,13 @@ typedef struct pt_object {
     int32_t kind;
     int32_t material;
     int32_t closed;
-    int32_t first_tri;       /* meshes */
-    int32_t tri_count;       /* meshes */
-    int32_t pad0, pad1, pad2;
+    int32_t first_tri;       /* meshes: world triangles; instance batches: prototype triangles */
+    int32_t tri_count;
+    int32_t first_instance;  /* instance batches: placements [first_instance, + instance_count) */
+    int32_t instance_count;
+    int32_t pad2;
     float   p[12];           /* quad: corner, edge1, edge2, normal; sphere: center, radius */
-    float   box_min[4];      /* meshes: vertex AABB (containment quick reject) */
+    float   box_min[4];      /* meshes: vertex AABB (containment quick reject); batches: local */
     float   box_max[4];
 } pt_object;
 
@@ -154,6 +167,12 @@ typedef struct pt_scene_desc {
     float    xyz_to_rgb[9];        /* row-major XYZ -> linear sRGB */
     float    cie_y_integral;
     int32_t  pad1[2];
+    /* instance batches: prototype triangles in their local frames [n_proto_tris][9] and the
+     * placements [n_instances] the batch objects index */
+    int32_t  n_proto_tris;
+    int32_t  n_instances;
+    const float*       proto_vertices;
+    const pt_instance* instances;
 } pt_scene_desc;
 
 /* ---- program (the Plan stage's decisions) ---- */

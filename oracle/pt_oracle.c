@@ -273,6 +273,16 @@ typedef struct {
     int emissive_solid;       /* some closed/solid region is emissive */
     int ambient_material;     /* material of region -1 (ambientMedium), -1 = none */
     int has_media, has_null;
+    /* instance batches: prototype triangles follow the world triangles in tri[]; each batch
+     * object's BLAS is over its prototype range; a TLAS over the placements' world boxes */
+    int n_proto;
+    int* obj_first;           /* absolute first triangle of every object's range in tri[] */
+    int n_inst;
+    float* inst;              /* [n_inst][8]: q xyzw, t_rigid xyz, s */
+    int* inst_batch;          /* owning batch object */
+    int* inst_base;           /* first global primitive id offset of each placement */
+    int inst_prim0;           /* n_tris + n_objects: the first instance primitive id */
+    bvh_t itlas;
     float spec_curves[95 * 4];/* CIE X, Y, Z, D65 at 360 + 5 i nm (spectral programs) */
     float xyz_rgb[9], cie_y;
     int has_spectral;
@@ -304,7 +314,7 @@ static inline void tri_load(const or_scene* S, int i, v3* a, v3* b, v3* c) {
 static void blas_nearest(const or_scene* S, int obj, v3 ro, v3 rd, float* tmax, int* best) {
     const bvh_t* B = &S->blas[obj];
     if (B->n_nodes == 0) return;
-    int first = S->objects[obj].first_tri;
+    int first = S->obj_first[obj];
     v3 inv = inv_dir(rd);
     int stack[64]; int ptr = 0; stack[0] = 0;
     while (ptr >= 0) {
@@ -333,7 +343,7 @@ static void blas_nearest(const or_scene* S, int obj, v3 ro, v3 rd, float* tmax, 
 static int blas_any(const or_scene* S, int obj, v3 ro, v3 rd, float maxDist) {
     const bvh_t* B = &S->blas[obj];
     if (B->n_nodes == 0) return 0;
-    int first = S->objects[obj].first_tri;
+    int first = S->obj_first[obj];
     v3 inv = inv_dir(rd);
     int stack[64]; int ptr = 0; stack[0] = 0;
     while (ptr >= 0) {
@@ -383,6 +393,120 @@ static int quad_intersect(v3 ro, v3 rd, const float* p, float* t) {
     return u >= 0.0f && u <= 1.0f && v >= 0.0f && v <= 1.0f;
 }
 
+/* Placements (glsl/core/placement.glsl:19-41): v + 2 q.xyz x (q.xyz x v + q.w v). */
+static inline v3 qrot(const float* q, v3 v) {
+    v3 qv = V(q[0], q[1], q[2]);
+    v3 c = add(cross(qv, v), muls(v, q[3]));
+    return add(v, muls(cross(qv, c), 2.0f));
+}
+static inline void inst_ray(const float* I, v3 ro, v3 rd, v3* lo, v3* ld) {
+    v3 r = add(qrot(I, ro), V(I[4], I[5], I[6]));
+    v3 d = qrot(I, rd);
+    *lo = V(r.x / I[7], r.y / I[7], r.z / I[7]);
+    *ld = V(d.x / I[7], d.y / I[7], d.z / I[7]);
+}
+static inline v3 inst_normal(const float* I, v3 n) {
+    float qc[4] = {-I[0], -I[1], -I[2], I[3]};
+    return qrot(qc, n);
+}
+/* The prototype BLAS in the placement's frame; candidates compete by GLOBAL id. */
+static void instance_nearest(const or_scene* S, int i, v3 ro, v3 rd, float* tmax, int* best) {
+    const float* I = S->inst + 8 * (size_t)i;
+    int obj = S->inst_batch[i];
+    const bvh_t* B = &S->blas[obj];
+    if (B->n_nodes == 0) return;
+    v3 lo, ld; inst_ray(I, ro, rd, &lo, &ld);
+    int first = S->obj_first[obj];
+    int pbase = S->inst_prim0 + S->inst_base[i];
+    v3 inv = inv_dir(ld);
+    int stack[64]; int ptr = 0; stack[0] = 0;
+    while (ptr >= 0) {
+        int ni = stack[ptr--];
+        const float* n = B->nodes + 8 * (size_t)ni;
+        if (!aabb_hit(n, lo, inv, *tmax)) continue;
+        if (n[3] >= 0.0f) {
+            int off = (int)n[7], cnt = (int)n[3];
+            for (int k = off; k < off + cnt; k++) {
+                int local = (int)B->order[k];
+                v3 a, b, c, g; float t;
+                tri_load(S, first + local, &a, &b, &c);
+                int prim = pbase + local;
+                if (tri_test(lo, ld, a, b, c, &t, &g) && t > 0.0f && (t < *tmax || (t == *tmax && prim < *best))) {
+                    *tmax = t; *best = prim;
+                }
+            }
+        } else if (ptr + 2 < 64) {
+            stack[++ptr] = ni + 1; stack[++ptr] = (int)n[7];
+        }
+    }
+}
+static int instance_any(const or_scene* S, int i, v3 ro, v3 rd, float maxDist) {
+    const float* I = S->inst + 8 * (size_t)i;
+    int obj = S->inst_batch[i];
+    const bvh_t* B = &S->blas[obj];
+    if (B->n_nodes == 0) return 0;
+    v3 lo, ld; inst_ray(I, ro, rd, &lo, &ld);
+    int first = S->obj_first[obj];
+    v3 inv = inv_dir(ld);
+    int stack[64]; int ptr = 0; stack[0] = 0;
+    while (ptr >= 0) {
+        int ni = stack[ptr--];
+        const float* n = B->nodes + 8 * (size_t)ni;
+        if (!aabb_hit(n, lo, inv, maxDist)) continue;
+        if (n[3] >= 0.0f) {
+            int off = (int)n[7], cnt = (int)n[3];
+            for (int k = off; k < off + cnt; k++) {
+                v3 a, b, c, g; float t;
+                tri_load(S, first + (int)B->order[k], &a, &b, &c);
+                if (tri_test(lo, ld, a, b, c, &t, &g) && t > 0.0f && t < maxDist) return 1;
+            }
+        } else if (ptr + 2 < 64) {
+            stack[++ptr] = ni + 1; stack[++ptr] = (int)n[7];
+        }
+    }
+    return 0;
+}
+/* Instance TLAS over the placements' world boxes (intersection/index.ts:83-91). */
+static void instances_nearest(const or_scene* S, v3 ro, v3 rd, float* tmax, int* best) {
+    if (S->itlas.n_nodes == 0) return;
+    v3 inv = inv_dir(rd);
+    int stack[64]; int ptr = 0; stack[0] = 0;
+    while (ptr >= 0) {
+        int ni = stack[ptr--];
+        const float* n = S->itlas.nodes + 8 * (size_t)ni;
+        if (!aabb_hit(n, ro, inv, *tmax)) continue;
+        if (n[3] >= 0.0f) {
+            int off = (int)n[7], cnt = (int)n[3];
+            for (int j = 0; j < cnt; j++) instance_nearest(S, (int)S->itlas.order[off + j], ro, rd, tmax, best);
+        } else if (ptr + 2 < 64) {
+            stack[++ptr] = ni + 1; stack[++ptr] = (int)n[7];
+        }
+    }
+}
+static int instances_any(const or_scene* S, v3 ro, v3 rd, float maxDist) {
+    if (S->itlas.n_nodes == 0) return 0;
+    v3 inv = inv_dir(rd);
+    int stack[64]; int ptr = 0; stack[0] = 0;
+    while (ptr >= 0) {
+        int ni = stack[ptr--];
+        const float* n = S->itlas.nodes + 8 * (size_t)ni;
+        if (!aabb_hit(n, ro, inv, maxDist)) continue;
+        if (n[3] >= 0.0f) {
+            int off = (int)n[7], cnt = (int)n[3];
+            for (int j = 0; j < cnt; j++) if (instance_any(S, (int)S->itlas.order[off + j], ro, rd, maxDist)) return 1;
+        } else if (ptr + 2 < 64) {
+            stack[++ptr] = ni + 1; stack[++ptr] = (int)n[7];
+        }
+    }
+    return 0;
+}
+/* The placement of an instance primitive id. */
+static inline int instance_of(const or_scene* S, int prim) {
+    int rel = prim - S->inst_prim0, lo = 0, hi = S->n_inst - 1;
+    while (lo < hi) { int mid = (lo + hi + 1) >> 1; if (S->inst_base[mid] <= rel) lo = mid; else hi = mid - 1; }
+    return lo;
+}
+
 /* Closest hit over the whole scene: scene TLAS over mesh objects (leaf size 1,
  * app/sceneData.ts:88 + features/intersection.ts:1411-1468), then the analytic objects.
  * Returns the global primitive id (-1 = none) and shrinks *tmax. */
@@ -414,9 +538,11 @@ static int scene_closest(const or_scene* S, v3 ro, v3 rd, float* tmax) {
         else if (ob->kind == PT_OBJ_SPHERE) ok = sphere_intersect(ro, rd, ld3(ob->p), ob->p[3], &t);
         if (ok && t < *tmax) { *tmax = t; best = S->n_tris + o; }
     }
+    instances_nearest(S, ro, rd, tmax, &best);
     return best;
 }
 static int scene_any(const or_scene* S, v3 ro, v3 rd, float maxDist) {
+    if (instances_any(S, ro, rd, maxDist)) return 1;
     for (int o = 0; o < S->n_objects; o++) {
         const pt_object* ob = &S->objects[o];
         float t;
@@ -447,7 +573,7 @@ static const v3 MESH_INSIDE_DIR = {0.5320544f, 0.7396997f, 0.4114743f};
 static int mesh_inside(const or_scene* S, int obj, v3 p) {
     const bvh_t* B = &S->blas[obj];
     if (B->n_nodes == 0) return 0;
-    int first = S->objects[obj].first_tri;
+    int first = S->obj_first[obj];
     float tmax = 1.0e20f, sideDot = 0.0f; int best = -1;
     v3 dir = MESH_INSIDE_DIR;
     v3 inv = V(1.0f / dir.x, 1.0f / dir.y, 1.0f / dir.z);
@@ -510,7 +636,7 @@ static float box_dist2(const float* n, v3 p) {
 static float mesh_closest(const or_scene* S, int obj, v3 p) {
     const bvh_t* B = &S->blas[obj];
     if (B->n_nodes == 0) return 1.0e20f;
-    int first = S->objects[obj].first_tri;
+    int first = S->obj_first[obj];
     float best2 = 1.0e30f;
     int stack[64]; int ptr = 0; stack[0] = 0;
     while (ptr >= 0) {
@@ -531,6 +657,38 @@ static float mesh_closest(const or_scene* S, int obj, v3 p) {
     }
     return sqrtf(best2);
 }
+/* Instanced batches claiming an interior (features/intersection.ts:1196-1215): point descent
+ * of the instance TLAS; per placement, the prototype's frame (÷ s), the local box, the
+ * first-hit-facing inside test, and the world distance s x closest-triangle distance. */
+static int region_instances(const or_scene* S, v3 p, int region, float best) {
+    if (S->itlas.n_nodes == 0) return region;
+    int stack[64]; int ptr = 0; stack[0] = 0;
+    while (ptr >= 0) {
+        int ni = stack[ptr--];
+        const float* n = S->itlas.nodes + 8 * (size_t)ni;
+        if (!(p.x >= n[0] && p.y >= n[1] && p.z >= n[2] && p.x <= n[4] && p.y <= n[5] && p.z <= n[6])) continue;
+        if (n[3] >= 0.0f) {
+            int off = (int)n[7], cnt = (int)n[3];
+            for (int j = 0; j < cnt; j++) {
+                int i = (int)S->itlas.order[off + j];
+                int o = S->inst_batch[i];
+                const pt_object* ob = &S->objects[o];
+                if (!ob->closed) continue;
+                const float* I = S->inst + 8 * (size_t)i;
+                v3 r = add(qrot(I, p), V(I[4], I[5], I[6]));
+                v3 lp = V(r.x / I[7], r.y / I[7], r.z / I[7]);
+                float d = 1.0e20f;
+                if (lp.x >= ob->box_min[0] && lp.y >= ob->box_min[1] && lp.z >= ob->box_min[2] &&
+                    lp.x <= ob->box_max[0] && lp.y <= ob->box_max[1] && lp.z <= ob->box_max[2] && mesh_inside(S, o, lp))
+                    d = -I[7] * mesh_closest(S, o, lp);
+                if (d < 0.0f && d > best) { best = d; region = o; }
+            }
+        } else if (ptr + 2 < 64) {
+            stack[++ptr] = ni + 1; stack[++ptr] = (int)n[7];
+        }
+    }
+    return region;
+}
 static int region_at(const or_scene* S, v3 p) {
     int region = -1; float best = -1.0e20f;
     for (int o = 0; o < S->n_objects; o++) {
@@ -540,7 +698,7 @@ static int region_at(const or_scene* S, v3 p) {
         if (d < 0.0f && d > best) { best = d; region = o; }
     }
     /* closed meshes: point descent of the scene TLAS (only leaves whose box holds p) */
-    if (S->tlas.n_nodes == 0) return region;
+    if (S->tlas.n_nodes == 0) return region_instances(S, p, region, best);
     int stack[64]; int ptr = 0; stack[0] = 0;
     while (ptr >= 0) {
         int ni = stack[ptr--];
@@ -562,11 +720,11 @@ static int region_at(const or_scene* S, v3 p) {
             stack[++ptr] = ni + 1; stack[++ptr] = (int)n[7];
         }
     }
-    return region;
+    return region_instances(S, p, region, best);
 }
 static int object_thin(const or_scene* S, int o) {
     const pt_object* ob = &S->objects[o];
-    return ob->kind == PT_OBJ_QUAD || (ob->kind == PT_OBJ_MESH && !ob->closed);
+    return ob->kind == PT_OBJ_QUAD || ((ob->kind == PT_OBJ_MESH || ob->kind == PT_OBJ_INSTANCES) && !ob->closed);
 }
 
 /* ---------------- hit (structs.glsl Hit; features/intersection.ts:889-948,1665-1728) ---- */
@@ -602,6 +760,15 @@ static int scene_intersect(const or_scene* S, v3 ro, v3 rd, hit_t* hit) {
         hit->frame = ambient_frame(hit->p, normalize(g));
         hit->ng = normalize(g);
         hit->region_owner = S->tri_object[prim];
+    } else if (prim >= S->inst_prim0) {   /* instanceLeafItem mesh arm: normals by the forward rotation */
+        int i = instance_of(S, prim);
+        int o = S->inst_batch[i];
+        v3 a, b, c; tri_load(S, S->obj_first[o] + (prim - S->inst_prim0 - S->inst_base[i]), &a, &b, &c);
+        v3 gl = cross(sub(b, a), sub(c, a));
+        v3 ng = normalize(inst_normal(S->inst + 8 * (size_t)i, gl));
+        hit->frame = ambient_frame(hit->p, ng);
+        hit->ng = ng;
+        hit->region_owner = o;
     } else {
         int o = prim - S->n_tris;
         const pt_object* ob = &S->objects[o];
@@ -631,7 +798,7 @@ static inline int material_has_medium(const or_scene* S, int mat) { return mat >
 static inline float ior_of(const or_scene* S, int region) {
     if (region < 0) return 1.0f;
     const pt_object* ob = &S->objects[region];
-    if (ob->kind == PT_OBJ_QUAD || (ob->kind == PT_OBJ_MESH && !ob->closed)) return 1.0f;
+    if (ob->kind == PT_OBJ_QUAD || ((ob->kind == PT_OBJ_MESH || ob->kind == PT_OBJ_INSTANCES) && !ob->closed)) return 1.0f;
     const pt_material* m = &S->materials[ob->material];
     return m->model == PT_MAT_DIELECTRIC ? m->ior : 1.0f;
 }
@@ -1446,8 +1613,11 @@ static sp transport_trace(const or_scene* S, const or_program* P, v3 ro, v3 rd, 
 or_scene* or_scene_create(const pt_scene_desc* d) {
     or_scene* S = (or_scene*)calloc(1, sizeof(or_scene));
     S->n_tris = d->n_tris;
-    S->tri = (float*)malloc(sizeof(float) * 9 * (size_t)(d->n_tris > 0 ? d->n_tris : 1));
+    S->n_proto = d->proto_vertices ? d->n_proto_tris : 0;
+    const size_t ntot = (size_t)d->n_tris + (size_t)S->n_proto;
+    S->tri = (float*)malloc(sizeof(float) * 9 * (ntot > 0 ? ntot : 1));
     memcpy(S->tri, d->tri_vertices, sizeof(float) * 9 * (size_t)d->n_tris);
+    if (S->n_proto) memcpy(S->tri + 9 * (size_t)d->n_tris, d->proto_vertices, sizeof(float) * 9 * (size_t)S->n_proto);
     S->tri_object = (int32_t*)malloc(sizeof(int32_t) * (size_t)(d->n_tris > 0 ? d->n_tris : 1));
     memcpy(S->tri_object, d->tri_object, sizeof(int32_t) * (size_t)d->n_tris);
     S->n_objects = d->n_objects;
@@ -1474,17 +1644,20 @@ or_scene* or_scene_create(const pt_scene_desc* d) {
     S->env_rot_cs[0] = (float)cos((double)d->env.rotation);
     S->env_rot_cs[1] = (float)sin((double)d->env.rotation);
     for (int m = 0; m < S->n_materials; m++) if (S->materials[m].model == PT_MAT_DIELECTRIC) S->has_transmission = 1;
-    /* per-mesh BLAS */
+    /* per-mesh BLAS; instance batches get theirs over the prototype range */
     S->blas = (bvh_t*)calloc((size_t)(S->n_objects > 0 ? S->n_objects : 1), sizeof(bvh_t));
+    S->obj_first = (int*)calloc((size_t)(S->n_objects > 0 ? S->n_objects : 1), sizeof(int));
+    for (int o = 0; o < S->n_objects; o++)
+        S->obj_first[o] = S->objects[o].kind == PT_OBJ_INSTANCES ? d->n_tris + S->objects[o].first_tri : S->objects[o].first_tri;
     int nm = 0;
     for (int o = 0; o < S->n_objects; o++) {
         const pt_object* ob = &S->objects[o];
-        if (ob->kind != PT_OBJ_MESH) continue;
-        nm++;
+        if (ob->kind != PT_OBJ_MESH && ob->kind != PT_OBJ_INSTANCES) continue;
+        if (ob->kind == PT_OBJ_MESH) nm++;
         int n = ob->tri_count;
         double* boxes = (double*)malloc(sizeof(double) * 6 * (size_t)(n > 0 ? n : 1));
         for (int i = 0; i < n; i++) {
-            const float* p = S->tri + 9 * (size_t)(ob->first_tri + i);
+            const float* p = S->tri + 9 * (size_t)(S->obj_first[o] + i);
             double ext = 0;
             for (int a = 0; a < 3; a++) {
                 double mn = fmin(fmin(p[a], p[3 + a]), p[6 + a]), mx = fmax(fmax(p[a], p[3 + a]), p[6 + a]);
@@ -1515,6 +1688,49 @@ or_scene* or_scene_create(const pt_scene_desc* d) {
     }
     bvh_build(&S->tlas, tb, k, 1);
     free(tb);
+    /* placements: world boxes of the prototype's local box (f64, padded), instance TLAS */
+    S->inst_prim0 = d->n_tris + d->n_objects;
+    S->n_inst = d->instances ? d->n_instances : 0;
+    if (S->n_inst > 0) {
+        S->inst = (float*)malloc(sizeof(float) * 8 * (size_t)S->n_inst);
+        S->inst_batch = (int*)malloc(sizeof(int) * (size_t)S->n_inst);
+        S->inst_base = (int*)malloc(sizeof(int) * (size_t)S->n_inst);
+        for (int i = 0; i < S->n_inst; i++) {
+            memcpy(S->inst + 8 * (size_t)i, d->instances[i].q, 4 * sizeof(float));
+            memcpy(S->inst + 8 * (size_t)i + 4, d->instances[i].ts, 4 * sizeof(float));
+            S->inst_batch[i] = -1;
+        }
+        for (int o = 0; o < S->n_objects; o++) {
+            const pt_object* ob = &S->objects[o];
+            if (ob->kind != PT_OBJ_INSTANCES) continue;
+            for (int i = ob->first_instance; i < ob->first_instance + ob->instance_count; i++) S->inst_batch[i] = o;
+        }
+        int acc = 0;
+        for (int i = 0; i < S->n_inst; i++) { S->inst_base[i] = acc; acc += S->objects[S->inst_batch[i]].tri_count; }
+        double* ib = (double*)malloc(sizeof(double) * 6 * (size_t)S->n_inst);
+        for (int i = 0; i < S->n_inst; i++) {
+            const float* I = S->inst + 8 * (size_t)i;
+            const pt_object* ob = &S->objects[S->inst_batch[i]];
+            double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+            double qx = -I[0], qy = -I[1], qz = -I[2], qw = I[3], sc = I[7];
+            for (int c = 0; c < 8; c++) {
+                double v[3];
+                for (int a = 0; a < 3; a++) v[a] = sc * ((c >> a) & 1 ? ob->box_max[a] : ob->box_min[a]) - I[4 + a];
+                /* forward rotation conj(q): v + 2 q x (q x v + w v) */
+                double c1[3] = {qy * v[2] - qz * v[1] + qw * v[0], qz * v[0] - qx * v[2] + qw * v[1], qx * v[1] - qy * v[0] + qw * v[2]};
+                double w[3] = {v[0] + 2.0 * (qy * c1[2] - qz * c1[1]), v[1] + 2.0 * (qz * c1[0] - qx * c1[2]), v[2] + 2.0 * (qx * c1[1] - qy * c1[0])};
+                for (int a = 0; a < 3; a++) { if (w[a] < lo[a]) lo[a] = w[a]; if (w[a] > hi[a]) hi[a] = w[a]; }
+            }
+            double ext = fmax(hi[0] - lo[0], fmax(hi[1] - lo[1], hi[2] - lo[2]));
+            for (int a = 0; a < 3; a++) {
+                double m = 1e-5 * ext + 1e-30;
+                ib[6 * i + a] = lo[a] - fmax(m, 1e-6 * fabs(lo[a]));
+                ib[6 * i + 3 + a] = hi[a] + fmax(m, 1e-6 * fabs(hi[a]));
+            }
+        }
+        bvh_build(&S->itlas, ib, S->n_inst, 1);
+        free(ib);
+    }
     if (S->env.type == PT_ENV_SKY) env_build(S);
     return S;
 }
@@ -1522,6 +1738,7 @@ void or_scene_destroy(or_scene* S) {
     if (!S) return;
     for (int o = 0; o < S->n_objects; o++) bvh_free(&S->blas[o]);
     free(S->blas); bvh_free(&S->tlas); free(S->tlas_obj);
+    free(S->obj_first); bvh_free(&S->itlas); free(S->inst); free(S->inst_batch); free(S->inst_base);
     free(S->tri); free(S->tri_object); free(S->objects); free(S->materials); free(S->lights);
     free(S->env_cond); free(S->env_marg);
     free(S);
@@ -1568,6 +1785,16 @@ void or_trace_closest_brute(const or_scene* S, const float* rays, int n, float* 
             if (ob->kind == PT_OBJ_QUAD) ok = quad_intersect(ro, rd, ob->p, &t);
             else if (ob->kind == PT_OBJ_SPHERE) ok = sphere_intersect(ro, rd, ld3(ob->p), ob->p[3], &t);
             if (ok && t < tmax) { tmax = t; best = S->n_tris + o; }
+        }
+        for (int k = 0; k < S->n_inst; k++) {   /* every placement, every prototype triangle */
+            v3 lo, ld; inst_ray(S->inst + 8 * (size_t)k, ro, rd, &lo, &ld);
+            const pt_object* ob = &S->objects[S->inst_batch[k]];
+            for (int j = 0; j < ob->tri_count; j++) {
+                v3 a, b, c, g; float t;
+                tri_load(S, S->obj_first[S->inst_batch[k]] + j, &a, &b, &c);
+                int prim = S->inst_prim0 + S->inst_base[k] + j;
+                if (tri_test(lo, ld, a, b, c, &t, &g) && t > 0.0f && (t < tmax || (t == tmax && prim < best))) { tmax = t; best = prim; }
+            }
         }
         t_out[i] = tmax; prim_out[i] = best;
     }

@@ -7,9 +7,9 @@ import ctypes as C
 
 import numpy as np
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 
-OBJ_MESH, OBJ_QUAD, OBJ_SPHERE = 0, 1, 2
+OBJ_MESH, OBJ_QUAD, OBJ_SPHERE, OBJ_INSTANCES = 0, 1, 2, 3
 MAT_LAMBERT, MAT_DIELECTRIC, MAT_GGX, MAT_NULL = 0, 1, 2, 3
 LIGHT_QUAD, LIGHT_POINT = 0, 1
 ENV_NONE, ENV_SKY = 0, 1
@@ -26,7 +26,7 @@ SPECTRUM_DTYPE = np.dtype([
 
 OBJECT_DTYPE = np.dtype([
     ("kind", "<i4"), ("material", "<i4"), ("closed", "<i4"), ("first_tri", "<i4"),
-    ("tri_count", "<i4"), ("pad", "<i4", 3),
+    ("tri_count", "<i4"), ("first_instance", "<i4"), ("instance_count", "<i4"), ("pad", "<i4"),
     ("p", "<f4", 12), ("box_min", "<f4", 4), ("box_max", "<f4", 4),
 ])
 MATERIAL_DTYPE = np.dtype([
@@ -42,7 +42,8 @@ LIGHT_DTYPE = np.dtype([
     ("corner", "<f4", 4), ("edge1", "<f4", 4), ("edge2", "<f4", 4), ("normal", "<f4", 4),
     ("radiance", "<f4", 4), ("area", "<f4"), ("pad", "<f4", 3), ("s_radiance", SPECTRUM_DTYPE),
 ])
-assert SPECTRUM_DTYPE.itemsize == 32
+INSTANCE_DTYPE = np.dtype([("q", "<f4", 4), ("ts", "<f4", 4)])
+assert SPECTRUM_DTYPE.itemsize == 32 and INSTANCE_DTYPE.itemsize == 32
 assert OBJECT_DTYPE.itemsize == 112 and MATERIAL_DTYPE.itemsize == 320 and LIGHT_DTYPE.itemsize == 144
 
 
@@ -60,7 +61,9 @@ class PtSceneDesc(C.Structure):
                 ("n_lights", C.c_int32), ("lights", C.c_void_p),
                 ("env", PtEnv), ("ambient_material", C.c_int32), ("pad0", C.c_int32),
                 ("spectral_curves", C.c_void_p), ("xyz_to_rgb", C.c_float * 9),
-                ("cie_y_integral", C.c_float), ("pad1", C.c_int32 * 2)]
+                ("cie_y_integral", C.c_float), ("pad1", C.c_int32 * 2),
+                ("n_proto_tris", C.c_int32), ("n_instances", C.c_int32),
+                ("proto_vertices", C.c_void_p), ("instances", C.c_void_p)]
 
 
 class PtProgram(C.Structure):
@@ -124,7 +127,7 @@ class SceneArrays:
     """Host-side scene in the ABI layout; keeps the numpy arrays alive for the desc."""
 
     def __init__(self, tri_vertices, tri_object, objects, materials, lights, env=None, ambient_material=-1,
-                 spectral_curves=None, xyz_to_rgb=None, cie_y_integral=0.0):
+                 spectral_curves=None, xyz_to_rgb=None, cie_y_integral=0.0, proto_vertices=None, instances=None):
         self.tri_vertices = np.ascontiguousarray(tri_vertices, dtype=np.float32).reshape(-1, 9)
         self.tri_object = np.ascontiguousarray(tri_object, dtype=np.int32).reshape(-1)
         self.objects = np.ascontiguousarray(objects, dtype=OBJECT_DTYPE)
@@ -136,6 +139,10 @@ class SceneArrays:
                                 else np.ascontiguousarray(spectral_curves, dtype=np.float32).reshape(95, 4))
         self.xyz_to_rgb = np.zeros(9, np.float32) if xyz_to_rgb is None else np.asarray(xyz_to_rgb, np.float32).reshape(9)
         self.cie_y_integral = float(cie_y_integral)
+        self.proto_vertices = (np.zeros((0, 9), np.float32) if proto_vertices is None
+                               else np.ascontiguousarray(proto_vertices, dtype=np.float32).reshape(-1, 9))
+        self.instances = (np.zeros(0, INSTANCE_DTYPE) if instances is None
+                          else np.ascontiguousarray(instances, dtype=INSTANCE_DTYPE))
         assert self.tri_object.shape[0] == self.tri_vertices.shape[0]
 
     @property
@@ -160,6 +167,10 @@ class SceneArrays:
         for i in range(9):
             d.xyz_to_rgb[i] = float(self.xyz_to_rgb[i])
         d.cie_y_integral = self.cie_y_integral
+        d.n_proto_tris = int(self.proto_vertices.shape[0])
+        d.n_instances = int(self.instances.shape[0])
+        d.proto_vertices = _ptr(self.proto_vertices)
+        d.instances = _ptr(self.instances)
         return d
 
 

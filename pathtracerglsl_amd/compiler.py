@@ -111,7 +111,7 @@ def _analyze_validate(scene, strategy):
     for o in scene.get("objects", []):
         if o.get("material") not in mats:
             raise CompileError(f"object references unknown material '{o.get('material')}'")
-        if o["type"] not in ("mesh", "sphere", "quad"):
+        if o["type"] not in ("mesh", "sphere", "quad", "instances"):
             raise CompileError(f"object type '{o['type']}' is not supported by this backend")
     env = scene.get("environment", {"type": "none"})
     if env.get("type") not in ("none", "procedural"):
@@ -204,8 +204,51 @@ def compile_scene(scene, strategy=None, width=256, height=256, reset_salt=1234):
         objects.append(rec)
         return len(objects) - 1
 
+    proto_chunks, inst_recs = [], []
+    n_proto = 0
+
+    def add_batch(o, material):
+        """An instance batch (features/intersection.ts:981-1066): prototype triangles in their
+        local frame and placements {rotation: unit quaternion x y z w (local -> world),
+        translation, scale}; the host precomposes each placement's inverse in f64 as the
+        reference's driven-placement contract does (glsl/core/placement.glsl:1-17)."""
+        nonlocal n_proto
+        t32 = np.asarray(o["triangles"], np.float64).reshape(-1, 9).astype(np.float32)
+        rec = np.zeros(1, abi.OBJECT_DTYPE)[0]
+        rec["kind"] = abi.OBJ_INSTANCES
+        rec["material"] = material
+        rec["closed"] = int(o.get("closed", False))
+        rec["first_tri"] = n_proto
+        rec["tri_count"] = t32.shape[0]
+        v = t32.reshape(-1, 3)
+        rec["box_min"][:3] = v.min(axis=0)
+        rec["box_max"][:3] = v.max(axis=0)
+        rec["first_instance"] = len(inst_recs)
+        rec["instance_count"] = len(o["placements"])
+        proto_chunks.append(t32)
+        n_proto += t32.shape[0]
+        for pl in o["placements"]:
+            q = np.asarray(pl.get("rotation", [0.0, 0.0, 0.0, 1.0]), np.float64)
+            q = q / math.sqrt(float(q @ q))
+            qi = np.array([-q[0], -q[1], -q[2], q[3]])                    # world -> local
+            t = np.asarray(pl.get("translation", [0.0, 0.0, 0.0]), np.float64)
+            s_ = float(pl.get("scale", 1.0))
+            if not s_ > 0.0:
+                raise CompileError("instance scale must be > 0")
+            u, w = qi[:3], qi[3]
+            rt = t + 2.0 * np.cross(u, np.cross(u, t) + w * t)             # R^-1 t
+            ir = np.zeros(1, abi.INSTANCE_DTYPE)[0]
+            ir["q"][:] = qi
+            ir["ts"][:3] = -rt
+            ir["ts"][3] = s_
+            inst_recs.append(ir)
+        objects.append(rec)
+
     for o in scene.get("objects", []):
         mi = mat_index[o["material"]]
+        if o["type"] == "instances":
+            add_batch(o, mi)
+            continue
         if o["type"] == "mesh":
             add_object(abi.OBJ_MESH, mi, o.get("closed", False), tris=o["triangles"])
         elif o["type"] == "sphere":
@@ -276,7 +319,9 @@ def compile_scene(scene, strategy=None, width=256, height=256, reset_salt=1234):
                              np.array(materials, abi.MATERIAL_DTYPE),
                              np.array(light_recs, abi.LIGHT_DTYPE), env,
                              mat_index[amb] if amb is not None else -1,
-                             spectral.curve_rows(), spectral.XYZ_TO_RGB.reshape(9), spectral.CIE_Y_INTEGRAL)
+                             spectral.curve_rows(), spectral.XYZ_TO_RGB.reshape(9), spectral.CIE_Y_INTEGRAL,
+                             np.concatenate(proto_chunks) if proto_chunks else None,
+                             np.array(inst_recs, abi.INSTANCE_DTYPE) if inst_recs else None)
 
     prog = abi.PtProgram()
     prog.abi_version = abi.ABI_VERSION

@@ -121,6 +121,18 @@ struct DevScene {
     int emissive_solid;
     int ambient_material;       // material of region -1 (ambientMedium), -1 = none
     int has_media, has_null;
+    // instance batches: binary TLAS over placement world boxes, placements (world -> local
+    // quaternion, (t_rigid, s)), owning batch, first primitive-id offset; prototype triangles
+    // sit in tri_shade after the world triangles and have per-batch wide BVHs in ow_*
+    const float4* inst_nodes;
+    int n_inst_nodes;
+    const int* inst_order;
+    const float4* inst_q;
+    const float4* inst_ts;
+    const int* inst_batch;
+    const int* inst_base;
+    int n_instances;
+    int inst_prim0;             // n_tris + n_objects: the first instance primitive id
     // spectral color model: CIE X, Y, Z, D65 at 360 + 5 i nm (95 rows) and the cie1931 sensor
     const float4* spec_curves;
     float xyz_rgb[9];

@@ -337,6 +337,7 @@ __global__ void __launch_bounds__(TB) k_extend(DevScene Sk, Paths P, int qsel) {
         if (active && (Sp->n_cw_nodes == 0 || trace_step<false, TB, COUNT>(Sp->cw_nodes, Sp->cw_tris, st, stk, cnt))) {
             float tmax = st.tmax; int prim = st.prim;
             trace_analytic<false>(*Sp, st.o, st.d, tmax, prim);
+            trace_instances<false, TB, COUNT>(*Sp, st.o, st.d, tmax, prim, stk, cnt);
             P.hit[path] = make_float2(tmax, __int_as_float(prim));
             active = false;
         }
@@ -363,6 +364,7 @@ __global__ void __launch_bounds__(TB) k_shadow(DevScene Sk, Paths P) {
         float maxd = o4.w; int prim = -1;
         bool blocked = trace_analytic<true>(S, o, d, maxd, prim);
         if (!blocked) blocked = trace_wide<true, STACK, TB, false>(S, o, d, maxd, prim, stk, cnt);
+        if (!blocked) blocked = trace_instances<true, TB, false>(S, o, d, maxd, prim, stk, cnt);
         if (!blocked) {
             int i = P.s_path[k];
             float4 c = P.s_c[k];
@@ -405,6 +407,7 @@ __global__ void __launch_bounds__(TBLOCK) k_shadow_media(DevScene S, Paths P) {
             float tmax = MAX_DIST; int prim = -1;
             if (S.n_cw_nodes > 0) trace_wide<false, STACK, TBLOCK, false>(S, o, d, tmax, prim, stk, cnt);
             trace_analytic<false>(S, o, d, tmax, prim);
+            trace_instances<false, TBLOCK, false>(S, o, d, tmax, prim, stk, cnt);
             bool hit_boundary = prim >= 0 && tmax < remaining;
             float seg_len = hit_boundary ? tmax : remaining;
             int med_mat = material_of(S, medium);
@@ -544,7 +547,7 @@ __global__ void __launch_bounds__(SBLOCK) k_bin(DevScene Sk, Paths P, int qsel, 
             if (cat < 0) {
                 if (prim < 0) cat = CAT_MISS;
                 else {
-                    int owner = prim < S.n_tris ? S.tri_object[prim] : prim - S.n_tris;
+                    int owner = prim_owner(S, prim);
                     int model = S.materials[S.objects[owner].material].model;
                     cat = model == PT_MAT_LAMBERT ? CAT_LAMBERT
                         : model == PT_MAT_GGX ? CAT_GGX
@@ -932,6 +935,7 @@ __global__ void __launch_bounds__(TBLOCK) k_trace_closest(DevScene S, const floa
         float t0 = rays[6 * (size_t)n + i], tmax = t0; int prim = -1;
         trace_wide<false, STACK, TBLOCK, COUNT>(S, o, d, tmax, prim, stk, cnt);
         trace_analytic<false>(S, o, d, tmax, prim);
+        trace_instances<false, TBLOCK, COUNT>(S, o, d, tmax, prim, stk, cnt);
         t_out[i] = tmax;
         prim_out[i] = prim;
     }
@@ -953,6 +957,7 @@ __global__ void __launch_bounds__(TBLOCK) k_trace_any(DevScene S, const float* r
         float tmax = rays[6 * (size_t)n + i]; int prim = -1;
         bool h = trace_analytic<true>(S, o, d, tmax, prim);
         if (!h) h = trace_wide<true, STACK, TBLOCK, false>(S, o, d, tmax, prim, stk, cnt);
+        if (!h) h = trace_instances<true, TBLOCK, false>(S, o, d, tmax, prim, stk, cnt);
         hit_out[i] = h ? 1 : 0;
     }
 }
@@ -1123,6 +1128,20 @@ void pt_ctx_destroy(pt_ctx* ctx) {
     delete ctx;
 }
 
+// Depth of a binary BVH (nodes of 8 floats: min.xyz, count or -1; max.xyz, offset or right
+// child); the device walks keep a 64-entry private stack.
+static int binary_depth(const std::vector<float>& nodes) {
+    if (nodes.empty()) return 0;
+    std::vector<std::pair<int, int>> todo{{0, 1}};
+    int depth = 0;
+    while (!todo.empty()) {
+        auto [n, d] = todo.back(); todo.pop_back();
+        depth = std::max(depth, d);
+        if (nodes[8 * (size_t)n + 3] < 0.0f) { todo.push_back({n + 1, d + 1}); todo.push_back({(int)nodes[8 * (size_t)n + 7], d + 1}); }
+    }
+    return depth;
+}
+
 static pt_status validate_scene(const pt_scene_desc* sd) {
     if (!sd) return fail(PT_E_INVALID, "pt_scene_upload: null argument");
     if (sd->abi_version != PT_ABI_VERSION) return fail(PT_E_INVALID, "pt_scene_upload: ABI version mismatch");
@@ -1137,8 +1156,28 @@ static pt_status validate_scene(const pt_scene_desc* sd) {
         if (o < 0 || o >= sd->n_objects || sd->objects[o].kind != PT_OBJ_MESH)
             return fail(PT_E_INVALID, "triangle " + std::to_string(t) + " has an invalid owning object");
     }
+    const int n_proto = sd->proto_vertices ? sd->n_proto_tris : 0;
+    const int n_inst = sd->instances ? sd->n_instances : 0;
+    if (n_proto < 0 || n_inst < 0) return fail(PT_E_INVALID, "pt_scene_upload: negative prototype or instance count");
+    std::vector<int> owner((size_t)std::max(n_inst, 1), -1);
     for (int o = 0; o < sd->n_objects; o++) {
         const pt_object& ob = sd->objects[o];
+        if (ob.kind != PT_OBJ_INSTANCES) continue;
+        if (ob.tri_count <= 0 || ob.first_tri < 0 || ob.first_tri + ob.tri_count > n_proto)
+            return fail(PT_E_INVALID, "object " + std::to_string(o) + ": prototype triangle range out of bounds");
+        if (ob.instance_count <= 0 || ob.first_instance < 0 || ob.first_instance + ob.instance_count > n_inst)
+            return fail(PT_E_INVALID, "object " + std::to_string(o) + ": instance range out of bounds");
+        for (int i = ob.first_instance; i < ob.first_instance + ob.instance_count; i++) {
+            if (owner[i] >= 0) return fail(PT_E_INVALID, "instance " + std::to_string(i) + " belongs to two batches");
+            owner[i] = o;
+            if (!(sd->instances[i].ts[3] > 0.0f)) return fail(PT_E_INVALID, "instance " + std::to_string(i) + ": scale must be > 0");
+        }
+    }
+    for (int i = 0; i < n_inst; i++)
+        if (owner[i] < 0) return fail(PT_E_INVALID, "instance " + std::to_string(i) + " belongs to no batch");
+    for (int o = 0; o < sd->n_objects; o++) {
+        const pt_object& ob = sd->objects[o];
+        if (ob.kind < PT_OBJ_MESH || ob.kind > PT_OBJ_INSTANCES) return fail(PT_E_INVALID, "object " + std::to_string(o) + ": unknown kind");
         if (ob.material < 0 || ob.material >= sd->n_materials) return fail(PT_E_INVALID, "object " + std::to_string(o) + ": bad material");
         if (ob.kind == PT_OBJ_MESH && (ob.first_tri < 0 || ob.tri_count < 0 || ob.first_tri + ob.tri_count > sd->n_tris))
             return fail(PT_E_INVALID, "object " + std::to_string(o) + ": triangle range out of bounds");
@@ -1163,19 +1202,24 @@ static pt_status build_scene(const pt_scene_desc* sd, hipStream_t stream, SceneS
     if ((st = SS.upload((const uint4*)wb.nodes.data(), wb.nodes.size() / 4, &D.cw_nodes)) != PT_OK) return st;
     if ((st = SS.upload((const float4*)wb.tris.data(), wb.tris.size() / 4, &D.cw_tris)) != PT_OK) return st;
     D.n_cw_nodes = (int)wb.n_nodes;
-    std::vector<float> shade((size_t)sd->n_tris * 12);
-    for (int t = 0; t < sd->n_tris; t++)
+    const int n_proto = sd->proto_vertices ? sd->n_proto_tris : 0;
+    const size_t n_shade = (size_t)sd->n_tris + (size_t)n_proto;
+    std::vector<float> shade(n_shade * 12);
+    for (size_t t = 0; t < n_shade; t++) {
+        const float* src = t < (size_t)sd->n_tris ? sd->tri_vertices + 9 * t : sd->proto_vertices + 9 * (t - sd->n_tris);
         for (int v = 0; v < 3; v++) {
-            for (int a = 0; a < 3; a++) shade[12 * (size_t)t + 4 * v + a] = sd->tri_vertices[9 * (size_t)t + 3 * v + a];
-            shade[12 * (size_t)t + 4 * v + 3] = 0.0f;
+            for (int a = 0; a < 3; a++) shade[12 * t + 4 * v + a] = src[3 * v + a];
+            shade[12 * t + 4 * v + 3] = 0.0f;
         }
-    if ((st = SS.upload((const float4*)shade.data(), (size_t)sd->n_tris * 3, &D.tri_shade)) != PT_OK) return st;
+    }
+    if ((st = SS.upload((const float4*)shade.data(), n_shade * 3, &D.tri_shade)) != PT_OK) return st;
     if ((st = SS.upload(sd->tri_object, (size_t)sd->n_tris, &D.tri_object)) != PT_OK) return st;
     D.n_tris = sd->n_tris;
     if ((st = SS.upload(sd->objects, (size_t)sd->n_objects, &D.objects)) != PT_OK) return st;
     D.n_objects = sd->n_objects;
     std::vector<int> analytic;
-    for (int o = 0; o < sd->n_objects; o++) if (sd->objects[o].kind != PT_OBJ_MESH) analytic.push_back(o);
+    for (int o = 0; o < sd->n_objects; o++)
+        if (sd->objects[o].kind == PT_OBJ_QUAD || sd->objects[o].kind == PT_OBJ_SPHERE) analytic.push_back(o);
     if ((st = SS.upload(analytic.data(), analytic.size(), &D.analytic)) != PT_OK) return st;
     D.n_analytic = (int)analytic.size();
     if ((st = SS.upload(sd->materials, (size_t)sd->n_materials, &D.materials)) != PT_OK) return st;
@@ -1198,19 +1242,30 @@ static pt_status build_scene(const pt_scene_desc* sd, hipStream_t stream, SceneS
     }
     for (int o = 0; o < sd->n_objects; o++) {
         const pt_object& ob = sd->objects[o];
-        bool solid = ob.kind == PT_OBJ_SPHERE || (ob.kind == PT_OBJ_MESH && ob.closed);
+        bool solid = ob.kind == PT_OBJ_SPHERE || ((ob.kind == PT_OBJ_MESH || ob.kind == PT_OBJ_INSTANCES) && ob.closed);
         if (solid && sd->materials[ob.material].emissive) D.emissive_solid = 1;
     }
-    // containment: per closed mesh binary BVH (closest-triangle distance)
+    // containment: per closed mesh (and closed instance batch) binary BVH, closest-triangle distance.
+    // Triangle ids are shade indices: world triangles first, prototype triangles after n_tris.
+    auto obj_tris = [&](const pt_object& ob) {
+        return ob.kind == PT_OBJ_INSTANCES ? sd->proto_vertices + 9 * (size_t)ob.first_tri
+                                           : sd->tri_vertices + 9 * (size_t)ob.first_tri;
+    };
+    auto obj_shade0 = [&](const pt_object& ob) { return ob.kind == PT_OBJ_INSTANCES ? sd->n_tris + ob.first_tri : ob.first_tri; };
     std::vector<float> onodes;
-    std::vector<int> obase((size_t)std::max(sd->n_objects, 1), -1), oorder((size_t)std::max(sd->n_tris, 1), 0);
-    std::vector<int> titems;
+    std::vector<int> obase((size_t)std::max(sd->n_objects, 1), -1), oorder(std::max(n_shade, (size_t)1), 0);
+    std::vector<int> titems, wide_items;
     for (int o = 0; o < sd->n_objects; o++) {
         const pt_object& ob = sd->objects[o];
-        if (ob.kind != PT_OBJ_MESH || !ob.closed || ob.tri_count == 0) continue;
+        const bool batch = ob.kind == PT_OBJ_INSTANCES;
+        if (!(ob.kind == PT_OBJ_MESH || batch) || ob.tri_count == 0) continue;
+        if (batch || ob.closed) wide_items.push_back(o);
+        if (!ob.closed) continue;
+        if (!batch) titems.push_back(o);
+        const float* tv = obj_tris(ob);
         std::vector<double> boxes(6 * (size_t)ob.tri_count);
         for (int i = 0; i < ob.tri_count; i++) {
-            const float* p = sd->tri_vertices + 9 * (size_t)(ob.first_tri + i);
+            const float* p = tv + 9 * (size_t)i;
             for (int a = 0; a < 3; a++) {
                 boxes[6 * i + a] = std::min(std::min(p[a], p[3 + a]), p[6 + a]);
                 boxes[6 * i + 3 + a] = std::max(std::max(p[a], p[3 + a]), p[6 + a]);
@@ -1218,26 +1273,28 @@ static pt_status build_scene(const pt_scene_desc* sd, hipStream_t stream, SceneS
         }
         BinaryBVH bb;
         build_binary_bvh(boxes.data(), ob.tri_count, 2, bb);
+        if (binary_depth(bb.nodes) >= 60) return fail(PT_E_UNSUPPORTED, "object " + std::to_string(o) + ": containment BVH too deep");
         obase[o] = (int)(onodes.size() / 8);
         onodes.insert(onodes.end(), bb.nodes.begin(), bb.nodes.end());
-        for (int i = 0; i < ob.tri_count; i++) oorder[ob.first_tri + i] = ob.first_tri + (int)bb.order[i];
-        titems.push_back(o);
+        const int s0 = obj_shade0(ob);
+        for (int i = 0; i < ob.tri_count; i++) oorder[s0 + i] = s0 + (int)bb.order[i];
     }
     if ((st = SS.upload((const float4*)onodes.data(), onodes.size() / 4, &D.obj_nodes)) != PT_OK) return st;
     if ((st = SS.upload(obase.data(), obase.size(), &D.obj_node_base)) != PT_OK) return st;
     if ((st = SS.upload(oorder.data(), oorder.size(), &D.obj_order)) != PT_OK) return st;
-    // per closed mesh wide BVH (first-hit-facing inside probe), concatenated
+    // per closed mesh / instance batch wide BVH (inside probe, prototype walk), concatenated
     std::vector<uint32_t> own; std::vector<float> owt; std::vector<int> oroot((size_t)std::max(sd->n_objects, 1), -1);
     uint32_t obj_depth = 0;
-    for (int o : titems) {
+    for (int o : wide_items) {
         const pt_object& ob = sd->objects[o];
         WideBVH w;
-        try { build_wide_bvh(sd->tri_vertices + 9 * (size_t)ob.first_tri, ob.tri_count, w); }
+        try { build_wide_bvh(obj_tris(ob), ob.tri_count, w); }
         catch (const std::exception& e) { return fail(PT_E_INVALID, std::string("object BVH build: ") + e.what()); }
         uint32_t noff = (uint32_t)(own.size() / 20), toff = (uint32_t)(owt.size() / 12);
         for (uint32_t k = 0; k < w.n_nodes; k++) { w.nodes[20 * k + 4] += noff; w.nodes[20 * k + 5] += toff; }
+        const uint32_t s0 = (uint32_t)obj_shade0(ob);
         for (size_t k = 0; k < w.tris.size() / 12; k++)
-            w.tris[12 * k + 3] = ptm_u2f((uint32_t)ob.first_tri + ptm_f2u(w.tris[12 * k + 3]));
+            w.tris[12 * k + 3] = ptm_u2f(s0 + ptm_f2u(w.tris[12 * k + 3]));
         own.insert(own.end(), w.nodes.begin(), w.nodes.end());
         owt.insert(owt.end(), w.tris.begin(), w.tris.end());
         oroot[o] = (int)noff;
@@ -1250,6 +1307,63 @@ static pt_status build_scene(const pt_scene_desc* sd, hipStream_t stream, SceneS
     if ((st = SS.upload((const uint4*)own.data(), own.size() / 4, &D.ow_nodes)) != PT_OK) return st;
     if ((st = SS.upload((const float4*)owt.data(), owt.size() / 4, &D.ow_tris)) != PT_OK) return st;
     if ((st = SS.upload(oroot.data(), oroot.size(), &D.ow_root)) != PT_OK) return st;
+    // placements (features/intersection.ts:981-1066): per instance the world->local rotation and
+    // (translation, scale), owning batch, first primitive offset; a binary TLAS over the world
+    // boxes of the prototype's local box (f64, padded so float ray transforms stay inside)
+    const int n_inst = sd->instances ? sd->n_instances : 0;
+    D.inst_prim0 = sd->n_tris + sd->n_objects;
+    D.n_instances = n_inst;
+    if (n_inst > 0) {
+        std::vector<float4> iq(n_inst), its(n_inst);
+        std::vector<int> ibatch(n_inst, -1), ibase(n_inst, 0);
+        for (int o = 0; o < sd->n_objects; o++) {
+            const pt_object& ob = sd->objects[o];
+            if (ob.kind != PT_OBJ_INSTANCES) continue;
+            for (int i = ob.first_instance; i < ob.first_instance + ob.instance_count; i++) ibatch[i] = o;
+        }
+        long long acc = 0;
+        for (int i = 0; i < n_inst; i++) {
+            const pt_instance& I = sd->instances[i];
+            iq[i] = make_float4(I.q[0], I.q[1], I.q[2], I.q[3]);
+            its[i] = make_float4(I.ts[0], I.ts[1], I.ts[2], I.ts[3]);
+            ibase[i] = (int)acc;
+            acc += sd->objects[ibatch[i]].tri_count;
+        }
+        if ((long long)D.inst_prim0 + acc >= 0x7fffffffLL) return fail(PT_E_UNSUPPORTED, "instanced primitive ids exceed 31 bits");
+        std::vector<double> ib(6 * (size_t)n_inst);
+        for (int i = 0; i < n_inst; i++) {
+            const pt_instance& I = sd->instances[i];
+            const pt_object& ob = sd->objects[ibatch[i]];
+            double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+            const double ux = -(double)I.q[0], uy = -(double)I.q[1], uz = -(double)I.q[2], w = I.q[3], sc = I.ts[3];
+            for (int c = 0; c < 8; c++) {
+                double v[3];
+                for (int k = 0; k < 3; k++) v[k] = sc * (double)((c >> k) & 1 ? ob.box_max[k] : ob.box_min[k]) - (double)I.ts[k];
+                // local -> world: rotate s * p - t by conj(q), v + 2 u x (u x v + w v)
+                const double c1[3] = {uy * v[2] - uz * v[1] + w * v[0], uz * v[0] - ux * v[2] + w * v[1], ux * v[1] - uy * v[0] + w * v[2]};
+                const double r[3] = {v[0] + 2.0 * (uy * c1[2] - uz * c1[1]), v[1] + 2.0 * (uz * c1[0] - ux * c1[2]),
+                                     v[2] + 2.0 * (ux * c1[1] - uy * c1[0])};
+                for (int k = 0; k < 3; k++) { lo[k] = std::min(lo[k], r[k]); hi[k] = std::max(hi[k], r[k]); }
+            }
+            const double ext = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
+            for (int k = 0; k < 3; k++) {
+                const double m = 1e-5 * ext + 1e-30;
+                ib[6 * i + k] = lo[k] - std::max(m, 1e-6 * std::fabs(lo[k]));
+                ib[6 * i + 3 + k] = hi[k] + std::max(m, 1e-6 * std::fabs(hi[k]));
+            }
+        }
+        BinaryBVH tl;
+        build_binary_bvh(ib.data(), n_inst, 1, tl);
+        if (binary_depth(tl.nodes) >= 60) return fail(PT_E_UNSUPPORTED, "instance TLAS too deep");
+        std::vector<int> iorder(tl.order.begin(), tl.order.end());
+        if ((st = SS.upload((const float4*)tl.nodes.data(), tl.nodes.size() / 4, &D.inst_nodes)) != PT_OK) return st;
+        D.n_inst_nodes = (int)(tl.nodes.size() / 8);
+        if ((st = SS.upload(iorder.data(), iorder.size(), &D.inst_order)) != PT_OK) return st;
+        if ((st = SS.upload(iq.data(), iq.size(), &D.inst_q)) != PT_OK) return st;
+        if ((st = SS.upload(its.data(), its.size(), &D.inst_ts)) != PT_OK) return st;
+        if ((st = SS.upload(ibatch.data(), ibatch.size(), &D.inst_batch)) != PT_OK) return st;
+        if ((st = SS.upload(ibase.data(), ibase.size(), &D.inst_base)) != PT_OK) return st;
+    }
     // uniform grid over the closed-mesh boxes (containment candidates)
     D.has_grid = titems.empty() ? 0 : 1;
     if (D.has_grid) {
@@ -1518,21 +1632,18 @@ static pt_status run_waves(pt_renderer* r, const pt_params* prm, uint32_t nframe
         const int v = (ctx->stack == 16 ? 0 : 4) | (ctx->dev.has_media ? 2 : 0) | (spectral ? 1 : 0);
         if (adj) {
             switch (v) {
-                case 0: launch_wave<16, false, false, true>(r, cam, f, s0); break;
-                case 2: launch_wave<16, true, false, true>(r, cam, f, s0); break;
-                case 4: launch_wave<32, false, false, true>(r, cam, f, s0); break;
-                case 6: launch_wave<32, true, false, true>(r, cam, f, s0); break;
+                // the media, spectral and adjoint programs run with the 32-entry stack, which
+                // holds every accepted BVH (depth <= 31); only the plain RGB path keeps two depths
+                case 0: case 4: launch_wave<32, false, false, true>(r, cam, f, s0); break;
+                case 2: case 6: launch_wave<32, true, false, true>(r, cam, f, s0); break;
                 default: return fail(PT_E_UNSUPPORTED, "the adjoint runs RGB programs only");
             }
         } else {
             switch (v) {
                 case 0: launch_wave<16, false, false, false>(r, cam, f, s0); break;
-                case 1: launch_wave<16, false, true, false>(r, cam, f, s0); break;
-                case 2: launch_wave<16, true, false, false>(r, cam, f, s0); break;
-                case 3: launch_wave<16, true, true, false>(r, cam, f, s0); break;
                 case 4: launch_wave<32, false, false, false>(r, cam, f, s0); break;
-                case 5: launch_wave<32, false, true, false>(r, cam, f, s0); break;
-                case 6: launch_wave<32, true, false, false>(r, cam, f, s0); break;
+                case 1: case 5: launch_wave<32, false, true, false>(r, cam, f, s0); break;
+                case 2: case 6: launch_wave<32, true, false, false>(r, cam, f, s0); break;
                 default: launch_wave<32, true, true, false>(r, cam, f, s0);
             }
         }

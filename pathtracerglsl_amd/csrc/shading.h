@@ -87,7 +87,8 @@ PT_DEV float mesh_closest(const DevScene& S, int obj, f3 p) {
         if (box_dist2(n0, n1, p) >= best2) continue;
         if (n0.w >= 0.0f) {
             int off = (int)n1.w, cnt = (int)n0.w;
-            int first = S.objects[obj].first_tri;
+            const pt_object& ob = S.objects[obj];
+            int first = ob.kind == PT_OBJ_INSTANCES ? S.n_tris + ob.first_tri : ob.first_tri;
             for (int i = off; i < off + cnt; i++) {
                 int tri = S.obj_order[first + i];
                 const float4* tp = S.tri_shade + 3 * (size_t)tri;
@@ -112,36 +113,83 @@ PT_DEV int region_at(const DevScene& S, f3 p, uint2* stk) {
         float d = length(p - ld3(ob.p)) - ob.p[3];
         if (d < 0.0f && d > best) { best = d; region = o; }
     }
-    if (!S.has_grid) return region;
-    int c[3];
-    const float pc[3] = {p.x, p.y, p.z};
-    for (int a = 0; a < 3; a++) {
-        float f = floorf((pc[a] - S.grid_lo[a]) * S.grid_inv[a]);
-        if (!(f >= 0.0f && f < (float)S.grid_n[a])) return region;   // outside every closed-mesh box
-        c[a] = (int)f;
+    if (S.has_grid) {
+        int c[3];
+        const float pc[3] = {p.x, p.y, p.z};
+        bool inside_grid = true;
+        for (int a = 0; a < 3; a++) {
+            float f = floorf((pc[a] - S.grid_lo[a]) * S.grid_inv[a]);
+            if (!(f >= 0.0f && f < (float)S.grid_n[a])) inside_grid = false;   // outside every closed-mesh box
+            c[a] = inside_grid ? (int)f : 0;
+        }
+        if (inside_grid) {
+            int2 cell = S.grid_cells[(c[2] * S.grid_n[1] + c[1]) * S.grid_n[0] + c[0]];
+            for (int j = 0; j < cell.y; j++) {
+                int o = S.grid_items[cell.x + j];
+                const pt_object& ob = S.objects[o];
+                float d = 1.0e20f;
+                if (p.x >= ob.box_min[0] && p.y >= ob.box_min[1] && p.z >= ob.box_min[2] &&
+                    p.x <= ob.box_max[0] && p.y <= ob.box_max[1] && p.z <= ob.box_max[2] && mesh_inside<BLOCK>(S, o, p, stk))
+                    d = -1.0f * mesh_closest(S, o, p);
+                if (d < 0.0f && d > best) { best = d; region = o; }
+            }
+        }
     }
-    int2 cell = S.grid_cells[(c[2] * S.grid_n[1] + c[1]) * S.grid_n[0] + c[0]];
-    for (int j = 0; j < cell.y; j++) {
-        int o = S.grid_items[cell.x + j];
-        const pt_object& ob = S.objects[o];
-        float d = 1.0e20f;
-        if (p.x >= ob.box_min[0] && p.y >= ob.box_min[1] && p.z >= ob.box_min[2] &&
-            p.x <= ob.box_max[0] && p.y <= ob.box_max[1] && p.z <= ob.box_max[2] && mesh_inside<BLOCK>(S, o, p, stk))
-            d = -1.0f * mesh_closest(S, o, p);
-        if (d < 0.0f && d > best) { best = d; region = o; }
+    // instanced batches claiming an interior (features/intersection.ts:1196-1215): point descent
+    // of the instance TLAS, the prototype's frame (/ s), world distance s x closest distance
+    if (S.n_inst_nodes > 0) {
+        int stack[64]; int sp = 0; stack[0] = 0;
+        while (sp >= 0) {
+            const int ni = stack[sp--];
+            const float4 n0 = S.inst_nodes[2 * ni], n1 = S.inst_nodes[2 * ni + 1];
+            if (!(p.x >= n0.x && p.y >= n0.y && p.z >= n0.z && p.x <= n1.x && p.y <= n1.y && p.z <= n1.z)) continue;
+            if (n0.w >= 0.0f) {
+                const int off = (int)n1.w, count = (int)n0.w;
+                for (int j = off; j < off + count; j++) {
+                    const int i = S.inst_order[j];
+                    const int o = S.inst_batch[i];
+                    const pt_object& ob = S.objects[o];
+                    if (!ob.closed) continue;
+                    const float4 q = S.inst_q[i], ts = S.inst_ts[i];
+                    const f3 r = qrot(q, p) + xyz(ts);
+                    const f3 lp = mk3(r.x / ts.w, r.y / ts.w, r.z / ts.w);
+                    float d = 1.0e20f;
+                    if (lp.x >= ob.box_min[0] && lp.y >= ob.box_min[1] && lp.z >= ob.box_min[2] &&
+                        lp.x <= ob.box_max[0] && lp.y <= ob.box_max[1] && lp.z <= ob.box_max[2] && mesh_inside<BLOCK>(S, o, lp, stk))
+                        d = -ts.w * mesh_closest(S, o, lp);
+                    if (d < 0.0f && d > best) { best = d; region = o; }
+                }
+            } else if (sp + 2 < 64) {
+                stack[++sp] = ni + 1; stack[++sp] = (int)n1.w;
+            }
+        }
     }
     return region;
 }
 PT_DEV bool object_thin(const DevScene& S, int o) {
     const pt_object& ob = S.objects[o];
-    return ob.kind == PT_OBJ_QUAD || (ob.kind == PT_OBJ_MESH && !ob.closed);
+    return ob.kind == PT_OBJ_QUAD || ((ob.kind == PT_OBJ_MESH || ob.kind == PT_OBJ_INSTANCES) && !ob.closed);
 }
 PT_DEV float ior_of(const DevScene& S, int region) {
     if (region < 0) return 1.0f;
     const pt_object& ob = S.objects[region];
-    if (ob.kind == PT_OBJ_QUAD || (ob.kind == PT_OBJ_MESH && !ob.closed)) return 1.0f;
+    if (ob.kind == PT_OBJ_QUAD || ((ob.kind == PT_OBJ_MESH || ob.kind == PT_OBJ_INSTANCES) && !ob.closed)) return 1.0f;
     const pt_material& m = S.materials[ob.material];
     return m.model == PT_MAT_DIELECTRIC ? m.ior : 1.0f;
+}
+
+// The placement owning an instance primitive id: the last instance whose first id <= prim.
+PT_DEV int instance_of(const DevScene& S, int prim) {
+    const int rel = prim - S.inst_prim0;
+    int lo = 0, hi = S.n_instances - 1;
+    while (lo < hi) { int mid = (lo + hi + 1) >> 1; if (S.inst_base[mid] <= rel) lo = mid; else hi = mid - 1; }
+    return lo;
+}
+// The region object of a primitive id: triangle owner, instance batch or analytic object.
+PT_DEV int prim_owner(const DevScene& S, int prim) {
+    if (prim < S.n_tris) return S.tri_object[prim];
+    if (prim >= S.inst_prim0 && S.n_instances > 0) return S.inst_batch[instance_of(S, prim)];
+    return prim - S.n_tris;
 }
 
 // Hit fill (features/intersection.ts:889-948 mesh wrapper, primitive arms) and the
@@ -159,6 +207,19 @@ PT_DEV void fill_hit(const DevScene& S, f3 o, f3 d, float t, int prim, Hit& h, b
         h.frame = ambient_frame(normalize(g));
         h.ng = normalize(g);
         h.owner = S.tri_object[prim];
+    } else if (prim >= S.inst_prim0 && S.n_instances > 0) {
+        // instanceLeafItem mesh arm: the prototype triangle, normals by the forward rotation
+        const int rel = prim - S.inst_prim0;
+        const int lo = instance_of(S, prim);
+        const int batch = S.inst_batch[lo];
+        const float4* tp = S.tri_shade + 3 * (size_t)(S.n_tris + S.objects[batch].first_tri + (rel - S.inst_base[lo]));
+        f3 a = xyz(tp[0]);
+        f3 gl = cross(xyz(tp[1]) - a, xyz(tp[2]) - a);
+        const float4 q = S.inst_q[lo];
+        const f3 ng = normalize(qrot(make_float4(-q.x, -q.y, -q.z, q.w), gl));
+        h.frame = ambient_frame(ng);
+        h.ng = ng;
+        h.owner = batch;
     } else {
         int o2 = prim - S.n_tris;
         const pt_object& ob = S.objects[o2];

@@ -200,6 +200,70 @@ PT_DEV bool trace_wide(const DevScene& S, f3 o, f3 d, float& tmax, int& prim, ui
     return found;
 }
 
+// Placements (glsl/core/placement.glsl:19-41): rotate by a unit quaternion.
+PT_DEV f3 qrot(float4 q, f3 v) {
+    const f3 qv = mk3(q.x, q.y, q.z);
+    const f3 c = cross(qv, v) + v * q.w;
+    return v + cross(qv, c) * 2.0f;
+}
+// bvh_aabb_hit (accel/bvh/bvh.glsl:28-36) on a binary node (min, A) (max, B).
+PT_DEV bool bin_aabb_hit(float4 n0, float4 n1, f3 o, f3 inv, float tmax) {
+    const float t0x = (n0.x - o.x) * inv.x, t1x = (n1.x - o.x) * inv.x;
+    const float t0y = (n0.y - o.y) * inv.y, t1y = (n1.y - o.y) * inv.y;
+    const float t0z = (n0.z - o.z) * inv.z, t1z = (n1.z - o.z) * inv.z;
+    const float tn = gmax(gmax(gmin(t0x, t1x), gmin(t0y, t1y)), gmin(t0z, t1z));
+    const float tf = gmin(gmin(gmax(t0x, t1x), gmax(t0y, t1y)), gmax(t0z, t1z)) * BVH_TFAR_PAD;
+    const float te = gmax(tn, 0.0f);
+    return tf >= te && te <= tmax;
+}
+// Instance batches after the analytic objects (intersection/index.ts:83-91 tlas walk +
+// features/intersection.ts:981-1066 instanceLeafItem): a binary TLAS over the placements'
+// world boxes; per placement the ray goes to the prototype's frame (t is preserved) and walks
+// the batch's wide BVH on the same LDS stack. Instance ids (inst_prim0 + base + local) exceed
+// every other id; ties across placements keep the (t, id) rule: a placement whose ids are all
+// below the current best accepts an equal t.
+template <bool ANY, int BLOCK, bool COUNT>
+PT_DEV bool trace_instances(const DevScene& S, f3 o, f3 d, float& tmax, int& prim, uint2* stk, TraceCounters& cnt) {
+    if (S.n_inst_nodes == 0) return false;
+    const f3 inv = mk3(1.0f / (fabsf(d.x) < 1e-20f ? copysignf(1e-20f, d.x) : d.x),
+                       1.0f / (fabsf(d.y) < 1e-20f ? copysignf(1e-20f, d.y) : d.y),
+                       1.0f / (fabsf(d.z) < 1e-20f ? copysignf(1e-20f, d.z) : d.z));
+    int stack[64]; int sp = 0; stack[0] = 0;
+    bool found = false;
+    while (sp >= 0) {
+        const int ni = stack[sp--];
+        const float4 n0 = S.inst_nodes[2 * ni], n1 = S.inst_nodes[2 * ni + 1];
+        if (!bin_aabb_hit(n0, n1, o, inv, tmax)) continue;
+        if (n0.w >= 0.0f) {
+            const int off = (int)n1.w, count = (int)n0.w;
+            for (int j = off; j < off + count; j++) {
+                const int i = S.inst_order[j];
+                const int batch = S.inst_batch[i];
+                const int root = S.ow_root[batch];
+                if (root < 0) continue;
+                const float4 q = S.inst_q[i], ts = S.inst_ts[i];
+                const f3 r = qrot(q, o) + xyz(ts), rd = qrot(q, d);
+                const f3 lo = mk3(r.x / ts.w, r.y / ts.w, r.z / ts.w), ld = mk3(rd.x / ts.w, rd.y / ts.w, rd.z / ts.w);
+                const int pbase = S.inst_prim0 + S.inst_base[i];
+                const int shade_first = S.n_tris + S.objects[batch].first_tri;
+                TravState st;
+                trace_init(st, lo, ld, tmax, (uint32_t)root);
+                const int init = (!ANY && prim >= 0 && prim > pbase) ? 0x7fffffff : -1;
+                st.prim = init;
+                while (!trace_step<ANY, BLOCK, COUNT>(S.ow_nodes, S.ow_tris, st, stk, cnt)) {}
+                if (ANY) {
+                    if (st.prim >= 0 && st.prim != init) { prim = pbase + (st.prim - shade_first); return true; }
+                } else if (st.prim != init) {
+                    tmax = st.tmax; prim = pbase + (st.prim - shade_first); found = true;
+                }
+            }
+        } else if (sp + 2 < 64) {
+            stack[++sp] = ni + 1; stack[++sp] = (int)n1.w;
+        }
+    }
+    return found;
+}
+
 // Analytic objects after the meshes; their ids (n_tris + object) exceed every triangle id,
 // so a strict < keeps the (t, id) rule.
 template <bool ANY>

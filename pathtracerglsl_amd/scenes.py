@@ -421,3 +421,82 @@ def haze():
         "ambient_medium": "fog",
         "camera": {"type": "pinhole", "position": [0.0, 1.2, 5.0], "target": [0.0, 1.2, -1.0], "fov": 0.9},
     }
+
+
+def instanced_cloud(n_instances=2000, seed=7, box=100.0, detail=4, n_materials=3):
+    """The icosphere cloud through an instance batch per material kind: one noise-displaced
+    detail-4 icosphere prototype (500 triangles) placed n times with seeded rotations,
+    lognormal(0, 0.5) scales and positions uniform in [0, box]^3 (GGX, dielectric and Lambert
+    batches, 60 / 20 / 20 %); procedural sky."""
+    rng = np.random.default_rng(seed)
+    V, F = unit_icosphere(detail)
+    noise = _value_noise(V * 2.5 + 11.0, seed)
+    P = V * (1.0 + 0.15 * (2.0 * noise - 1.0))[:, None]
+    tris = P[F]
+    kinds = rng.choice(3, size=n_instances, p=[0.6, 0.2, 0.2])
+    materials = {"ggx": {"model": "ggx", "f0": [0.9, 0.7, 0.5], "roughness": 0.3},
+                 "glass": {"model": "dielectric", "ior": 1.5},
+                 "matte": {"model": "lambert", "albedo": 0.6}}
+    names = ["ggx", "glass", "matte"]
+    placements = [[] for _ in range(3)]
+    for k in range(n_instances):
+        q = rng.normal(size=4)
+        placements[kinds[k]].append({"rotation": (q / np.linalg.norm(q)).tolist(),
+                                     "translation": rng.uniform(0.0, box, size=3).tolist(),
+                                     "scale": float(rng.lognormal(0.0, 0.5))})
+    objects = [{"type": "instances", "triangles": tris, "material": names[m], "closed": True, "placements": placements[m]}
+               for m in range(3) if placements[m]]
+    return {
+        "id": f"instanced-cloud-{n_instances}",
+        "materials": materials,
+        "objects": objects,
+        "lights": [],
+        "environment": {"type": "procedural", "intensity": 1.0},
+        "camera": {"type": "pinhole", "position": [box * 0.5, box * 0.5, -0.3 * box],
+                   "target": [box * 0.5, box * 0.5, box * 0.5], "fov": math.radians(60.0)},
+    }
+
+
+def cornell_identity_instances():
+    """The Cornell box with its two inner boxes as instance batches of one identity placement
+    each: world-space prototype triangles, rotation (0, 0, 0, 1), translation 0, scale 1. The
+    placement transform is then exact, so every hit, normal and containment query equals the
+    plain mesh scene's bit for bit."""
+    sc = cornell_box()
+    for o in sc["objects"]:
+        if o["type"] == "mesh" and o["closed"]:
+            o["type"] = "instances"
+            o["placements"] = [{"rotation": [0.0, 0.0, 0.0, 1.0], "translation": [0.0, 0.0, 0.0], "scale": 1.0}]
+    sc["id"] = "cornell-identity-instances"
+    return sc
+
+
+def cornell_glass_instances(n=3, seed=5, detail=2):
+    """Cornell box plus a glass instance batch: a detail-2 icosphere placed n times with seeded
+    rotations, scales in [0.2, 0.32] and centres inside the box, one of them cutting into the
+    tall inner box (nested closed regions and dielectric interfaces through placements)."""
+    sc = cornell_box()
+    rng = np.random.default_rng(seed)
+    V, F = unit_icosphere(detail)
+    centres = [[-0.35, 0.25, -0.3]] + [[rng.uniform(-0.6, 0.6), rng.uniform(-0.6, 0.5), rng.uniform(-0.2, 0.6)]
+                                       for _ in range(n - 1)]
+    pls = []
+    for c in centres:
+        q = rng.normal(size=4)
+        pls.append({"rotation": (q / np.linalg.norm(q)).tolist(), "translation": list(map(float, c)),
+                    "scale": float(rng.uniform(0.2, 0.32))})
+    sc["materials"]["glass"] = {"model": "dielectric", "ior": 1.5}
+    sc["objects"].append({"type": "instances", "triangles": V[F], "material": "glass", "closed": True, "placements": pls})
+    sc["id"] = f"cornell-glass-instances-{n}"
+    return sc
+
+
+def transformed_triangles(tris, rotation, translation, scale):
+    """World triangles of a placement in f64: R(q) (s p) + t, q = (x, y, z, w) local -> world."""
+    q = np.asarray(rotation, np.float64)
+    q = q / np.linalg.norm(q)
+    u, w = q[:3], q[3]
+    p = np.asarray(tris, np.float64).reshape(-1, 3) * float(scale)
+    c = np.cross(u, p) + w * p
+    r = p + 2.0 * np.cross(u, c)
+    return (r + np.asarray(translation, np.float64)).reshape(-1, 9)
