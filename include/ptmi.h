@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define PT_ABI_VERSION 3u
+#define PT_ABI_VERSION 4u
 
 typedef enum pt_status {
     PT_OK = 0,
@@ -69,6 +69,16 @@ typedef struct pt_object {
 
 enum { PT_MAT_LAMBERT = 0, PT_MAT_DIELECTRIC = 1, PT_MAT_GGX = 2, PT_MAT_NULL = 3 };
 
+/* The arm a homogeneous medium is sampled by (src/compiler/generate/features/materials.ts:675-728,
+ * routing predicates src/compiler/types.ts:351-417):
+ *   PT_MED_ANALYTIC  scattering, non-emissive: channel-MIS distance sample (analytic.glsl:9-43)
+ *   PT_MED_ABSORB    non-scattering: deterministic exp(-sigma_a t_max) and, when emissive,
+ *                    radiance eps * integral_0^t_max exp(-sigma_a s) ds (analytic/emission.glsl)
+ *   PT_MED_DELTA     scattering and emissive: delta tracking at the derived majorant
+ *                    (delta_tracking.glsl:56-120), emission collected per tentative collision
+ * The host decides the arm from the authored values, as the reference's Planner does. */
+enum { PT_MED_ANALYTIC = 0, PT_MED_ABSORB = 1, PT_MED_DELTA = 2 };
+
 /* A spectral value in one radiometric role (src/components/descriptors.ts:43), as the
  * spectral color model evaluates it at a path's four wavelengths
  * (src/components/color/spectral/lookup.glsl:40-60, color/spectra/blackbody/blackbody.glsl:4-10):
@@ -99,7 +109,7 @@ typedef struct pt_material {
     int32_t model;
     int32_t emissive;        /* material_is_emissive */
     int32_t has_medium;
-    int32_t pad0;
+    int32_t medium_arm;      /* PT_MED_* */
     float   albedo[4];       /* lambert */
     float   emission[4];
     float   f0[4];           /* ggx */
@@ -112,6 +122,14 @@ typedef struct pt_material {
     float   sigma_a[4];      /* medium */
     /* spectral color model: the same properties in their roles */
     pt_spectrum s_albedo, s_emission, s_f0, s_transmittance, s_sigma_s, s_sigma_a;
+    float   medium_emission[4];  /* medium: volume emission coefficient eps (dL/ds), RGB */
+    pt_spectrum s_medium_emission;   /* spectral, role 'emission' */
+    /* Delta-tracking majorant sigma_bar, derived by the library at upload and at every
+     * material update (features/materials.ts:774-813 derivedMajorant + the color model's
+     * mediumBound): [0] RGB = max(1e-6, max_c(sigma_a + sigma_s)); [1] spectral = the
+     * float32 bound of components/color/spectral/majorant.ts. Values passed in are ignored. */
+    float   majorant[2];
+    float   pad2[2];
 } pt_material;
 
 enum { PT_LIGHT_QUAD = 0, PT_LIGHT_POINT = 1 };
@@ -187,6 +205,7 @@ enum { PT_WL_VISIBLE = 0, PT_WL_UNIFORM = 1 };
  * per medium segment by equiangular sampling toward a delta light (Kulla-Fajardo 2012,
  * techniques/equiangular/equiangular.glsl; pt-nee with point lights only). */
 enum { PT_MLS_VERTEX = 0, PT_MLS_EQUIANGULAR = 1 };
+enum { PT_VS_ANALYTIC = 0, PT_VS_DELTA_TRACKING = 1 };
 
 typedef struct pt_program {
     uint32_t abi_version;
@@ -202,7 +221,7 @@ typedef struct pt_program {
     int32_t  medium_light_sampling; /* PT_MLS_*: estimator.mediumLightSampling */
     int32_t  variance;        /* != 0: also accumulate the variance picture (export "variance",
                                  components/accumulator/variance/variance.glsl) */
-    int32_t  pad;
+    int32_t  volume_sampling; /* PT_VS_*: estimator.volumeSampling (validation: the arms are per material) */
 } pt_program;
 
 /* Live per-frame values (the engine builtins + camera uniforms,

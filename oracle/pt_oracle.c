@@ -273,6 +273,7 @@ typedef struct {
     int emissive_solid;       /* some closed/solid region is emissive */
     int ambient_material;     /* material of region -1 (ambientMedium), -1 = none */
     int has_media, has_null;
+    int weighted_absorption;  /* some scattering medium weights absorption (Planner.ts:545-547) */
     /* instance batches: prototype triangles follow the world triangles in tri[]; each batch
      * object's BLAS is over its prototype range; a TLAS over the placements' world boxes */
     int n_proto;
@@ -1256,7 +1257,27 @@ static float lighting_pdf(const or_scene* S, v3 qp, v3 wi, int lid, v3 light_p) 
 /* ---------------- homogeneous media (transport/volume/analytic/analytic.glsl:9-51) ---- */
 static inline sp sigma_s_of(const or_scene* S, const pt_material* m, const wl_t* wl) { return lift(S, m->sigma_s, &m->s_sigma_s, wl); }
 static inline sp sigma_a_of(const or_scene* S, const pt_material* m, const wl_t* wl) { return lift(S, m->sigma_a, &m->s_sigma_a, wl); }
-typedef struct { int scattered; float t; sp weight; } msample_t;
+typedef struct { int scattered; float t; sp weight; sp radiance; } msample_t;
+static inline sp medium_emission_of(const or_scene* S, const pt_material* m, const wl_t* wl) {
+    return lift(S, m->medium_emission, &m->s_medium_emission, wl);
+}
+static inline int medium_emits(const pt_material* m) {
+    return m->medium_emission[0] != 0.0f || m->medium_emission[1] != 0.0f || m->medium_emission[2] != 0.0f;
+}
+/* derivedMajorant (features/materials.ts:774-789) in the two colour models' bounds: RGB keeps the
+ * ceiling (components/color/rgb/rgb.ts:5); spectral widens it to the float32 step above
+ * 4 c (1 + 2^-20)^5 / (1 - 2^-20)^2 (components/color/spectral/majorant.ts). */
+static void derive_majorants(pt_material* m) {
+    double c = 1e-6;
+    for (int k = 0; k < 3; k++) { double v = (double)m->sigma_a[k] + (double)m->sigma_s[k]; if (v > c) c = v; }
+    m->majorant[0] = (float)c;
+    const double ceiling = (double)(float)c;
+    const double up = 1.0 + ldexp(1.0, -20), down = 1.0 - ldexp(1.0, -20);
+    const double bound = 4.0 * ceiling * up * up * up * up * up / (down * down);
+    float r = (float)bound;
+    uint32_t u; memcpy(&u, &r, 4); u++; memcpy(&r, &u, 4);
+    m->majorant[1] = r;
+}
 static msample_t medium_sample_analytic(const or_scene* S, const pt_material* m, float t_max, float x0, float x1, const wl_t* wl) {
     msample_t ms;
     sp sa = sigma_a_of(S, m, wl), ss = sigma_s_of(S, m, wl);
@@ -1276,6 +1297,65 @@ static msample_t medium_sample_analytic(const or_scene* S, const pt_material* m,
         ms.scattered = 0; ms.t = t_max;
         ms.weight = sdivs(tr, fmaxf_(pdf, 1e-20f));
     }
+    ms.radiance = sgrey(0.0f);
+    return ms;
+}
+/* Deterministic absorbing arm (features/materials.ts:708-718) with the emission source term
+ * eps * integral_0^len exp(-sigma_a s) ds (transport/volume/analytic/emission.glsl:4-13). */
+static inline float absorption_integral1(float sigma, float len) {
+    float tau = sigma * len;
+    return tau < 0.1f ? len * (1.0f + tau * (-0.5f + tau * (1.0f / 6.0f + tau * (-1.0f / 24.0f + tau / 120.0f))))
+                      : (1.0f - EXP(-tau)) / sigma;
+}
+static msample_t medium_sample_absorb(const or_scene* S, const pt_material* m, float t_max, const wl_t* wl) {
+    msample_t ms;
+    sp sa = sigma_a_of(S, m, wl);
+    ms.scattered = 0; ms.t = t_max;
+    ms.weight = s_exp(smuls(sneg(sa), t_max));
+    ms.radiance = sgrey(0.0f);
+    if (medium_emits(m)) {
+        sp e = medium_emission_of(S, m, wl);
+        sp I = SP(absorption_integral1(sa.x, t_max), absorption_integral1(sa.y, t_max),
+                  absorption_integral1(sa.z, t_max), absorption_integral1(sa.w, t_max));
+        ms.radiance = smul(e, I);
+    }
+    return ms;
+}
+/* tracking_cap (delta_tracking.glsl:46-50): ceil(lambda + 6.5 sqrt(lambda) + 12). */
+static inline int tracking_cap(float lambda) { return (int)ceilf(lambda + 6.5f * sqrtf(lambda) + 12.0f); }
+/* Delta tracking (delta_tracking.glsl:56-120; Kutz et al. 2017 Alg. 4 with the history-aware
+ * average probabilities) on a constant medium at the scalar majorant: emission collected per
+ * tentative collision before the lottery, absorption a pure terminator. xi carries the first
+ * distance and event draws; later ones come from the stream. */
+static msample_t medium_sample_delta(const or_scene* S, const pt_material* m, float t_max, float x0, float x1,
+                                     rng_t* rng, const wl_t* wl) {
+    msample_t ms;
+    ms.scattered = 0; ms.t = t_max; ms.weight = sgrey(1.0f); ms.radiance = sgrey(0.0f);
+    const float sb = wl->spec ? m->majorant[1] : m->majorant[0];
+    const sp sa = sigma_a_of(S, m, wl), ss = sigma_s_of(S, m, wl), eps = medium_emission_of(S, m, wl);
+    const sp st = sadd(sa, ss);
+    const sp sn = SP(fmaxf_(sb - st.x, 0.0f), fmaxf_(sb - st.y, 0.0f), fmaxf_(sb - st.z, 0.0f), fmaxf_(sb - st.w, 0.0f));
+    sp w = sgrey(1.0f);
+    float t = 0.0f, xd = x0, xe = x1;
+    const int cap = tracking_cap(sb * t_max);
+    for (int i = 0; i < cap; i++) {
+        t += -LOG(1.0f - xd) / sb;
+        if (t >= t_max) { ms.weight = w; return ms; }
+        ms.radiance = sadd(ms.radiance, sdivs(smul(w, eps), sb));
+        float pa = s_avg(wl, smul(sa, w)), ps = s_avg(wl, smul(ss, w)), pn = s_avg(wl, smul(sn, w));
+        float c = pa + ps + pn;
+        if (c <= 0.0f) { ms.weight = sgrey(0.0f); return ms; }
+        pa /= c; ps /= c; pn /= c;
+        if (xe < pa) { ms.weight = sgrey(0.0f); return ms; }
+        if (xe < pa + ps) {
+            ms.scattered = 1; ms.t = t;
+            ms.weight = sdivs(smul(w, ss), sb * fmaxf_(ps, 1e-20f));
+            return ms;
+        }
+        w = smul(w, sdivs(sn, sb * fmaxf_(pn, 1e-20f)));
+        xd = rnd(rng); xe = rnd(rng);
+    }
+    ms.weight = w;
     return ms;
 }
 static inline sp medium_transmittance(const or_scene* S, const pt_material* m, float len, const wl_t* wl) {
@@ -1437,13 +1517,19 @@ static sp transport_trace(const or_scene* S, const or_program* P, v3 ro, v3 rd, 
                     L = sadd(L, equiangular_sample_direct(S, P, ro, rd, T, Mm, boundary ? hit.t : MAX_DIST,
                                                           S->has_null ? P->max_null - null_crossings : 0, rng, n_any, wl));
                 float x0 = rnd(rng), x1 = rnd(rng);
-                msample_t ms = medium_sample_analytic(S, Mm, boundary ? hit.t : MAX_DIST, x0, x1, wl);
+                const float seg = boundary ? hit.t : MAX_DIST;
+                msample_t ms = Mm->medium_arm == PT_MED_DELTA ? medium_sample_delta(S, Mm, seg, x0, x1, rng, wl)
+                             : Mm->medium_arm == PT_MED_ABSORB ? medium_sample_absorb(S, Mm, seg, wl)
+                             : medium_sample_analytic(S, Mm, seg, x0, x1, wl);
+                /* the source term is weighted by segment-start throughput (pt.ts:333-339) */
+                if (medium_emits(Mm)) L = sadd(L, smul(T, ms.radiance));
                 T = smul(T, ms.weight);
-                if (A && med_mat == A->p->medium_material) {   /* f = sigma_s e^{-sigma_t t} | e^{-sigma_t t_max} */
+                if (A && med_mat == A->p->medium_material) {   /* f = sigma_s e^{-sigma_t t} | e^{-sigma_t t_max} | e^{-sigma_a t_max} */
                     const float* ssv = Mm->sigma_s;
                     for (int ch = 0; ch < 3; ch++) {
                         /* a channel with sigma_s = 0 scatters nothing (f = 0): no finite dlog f */
-                        A->D[ch] += (ms.scattered && ssv[ch] > 0.0f ? 1.0 / (double)ssv[ch] : 0.0) - (double)ms.t;
+                        if (Mm->medium_arm != PT_MED_ABSORB)
+                            A->D[ch] += (ms.scattered && ssv[ch] > 0.0f ? 1.0 / (double)ssv[ch] : 0.0) - (double)ms.t;
                         A->D[3 + ch] -= (double)ms.t;
                     }
                 }
@@ -1486,11 +1572,15 @@ static sp transport_trace(const or_scene* S, const or_program* P, v3 ro, v3 rd, 
                     if (A && med_mat == A->p->medium_material) A->D[6] += dlog_hg(g, dot(wi, neg(wo_med)));
                     prev_was_delta = 0; prev_bsdf_pdf = ppdf; prev_qp = p_evt;
                     ro = p_evt; rd = wi;
-                    /* roulette_interior with medium_survival = sigma_s / sigma_t (pt.ts:230-283) */
-                    if (P->rr_start >= 0 && bounce >= P->rr_start) {
-                        sp ss = sigma_s_of(S, Mm, wl), st = sadd(ss, sigma_a_of(S, Mm, wl));
-                        sp surv = SP(ss.x / fmaxf_(st.x, 1e-9f), ss.y / fmaxf_(st.y, 1e-9f), ss.z / fmaxf_(st.z, 1e-9f),
-                                     ss.w / fmaxf_(st.w, 1e-9f));
+                    /* roulette_interior (pt.ts:230-283), present when some medium weights absorption;
+                     * medium_survival = sigma_s / sigma_t there, 1 on the tracking arm (materials.ts:731-772) */
+                    if (P->rr_start >= 0 && bounce >= P->rr_start && S->weighted_absorption) {
+                        sp surv = sgrey(1.0f);
+                        if (Mm->medium_arm == PT_MED_ANALYTIC) {
+                            sp ss = sigma_s_of(S, Mm, wl), st = sadd(ss, sigma_a_of(S, Mm, wl));
+                            surv = SP(ss.x / fmaxf_(st.x, 1e-9f), ss.y / fmaxf_(st.y, 1e-9f), ss.z / fmaxf_(st.z, 1e-9f),
+                                      ss.w / fmaxf_(st.w, 1e-9f));
+                        }
                         float p = s_max(wl, surv);
                         if (p <= 0.0f) break;
                         if (rnd(rng) > p) break;
@@ -1640,6 +1730,8 @@ or_scene* or_scene_create(const pt_scene_desc* d) {
     for (int m = 0; m < S->n_materials; m++) {
         if (S->materials[m].has_medium) S->has_media = 1;
         if (S->materials[m].model == PT_MAT_NULL) S->has_null = 1;
+        if (S->materials[m].has_medium && S->materials[m].medium_arm == PT_MED_ANALYTIC) S->weighted_absorption = 1;
+        derive_majorants(&S->materials[m]);
     }
     S->env_rot_cs[0] = (float)cos((double)d->env.rotation);
     S->env_rot_cs[1] = (float)sin((double)d->env.rotation);

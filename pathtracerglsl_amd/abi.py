@@ -7,10 +7,12 @@ import ctypes as C
 
 import numpy as np
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 OBJ_MESH, OBJ_QUAD, OBJ_SPHERE, OBJ_INSTANCES = 0, 1, 2, 3
 MAT_LAMBERT, MAT_DIELECTRIC, MAT_GGX, MAT_NULL = 0, 1, 2, 3
+MED_ANALYTIC, MED_ABSORB, MED_DELTA = 0, 1, 2
+VS_ANALYTIC, VS_DELTA_TRACKING = 0, 1
 LIGHT_QUAD, LIGHT_POINT = 0, 1
 ENV_NONE, ENV_SKY = 0, 1
 EST_PT, EST_NEE, EST_MIS = 0, 1, 2
@@ -30,12 +32,13 @@ OBJECT_DTYPE = np.dtype([
     ("p", "<f4", 12), ("box_min", "<f4", 4), ("box_max", "<f4", 4),
 ])
 MATERIAL_DTYPE = np.dtype([
-    ("model", "<i4"), ("emissive", "<i4"), ("has_medium", "<i4"), ("pad0", "<i4"),
+    ("model", "<i4"), ("emissive", "<i4"), ("has_medium", "<i4"), ("medium_arm", "<i4"),
     ("albedo", "<f4", 4), ("emission", "<f4", 4), ("f0", "<f4", 4), ("transmittance", "<f4", 4),
     ("alpha", "<f4"), ("ior", "<f4"), ("hg_g", "<f4"), ("pad1", "<f4"),
     ("sigma_s", "<f4", 4), ("sigma_a", "<f4", 4),
     ("s_albedo", SPECTRUM_DTYPE), ("s_emission", SPECTRUM_DTYPE), ("s_f0", SPECTRUM_DTYPE),
     ("s_transmittance", SPECTRUM_DTYPE), ("s_sigma_s", SPECTRUM_DTYPE), ("s_sigma_a", SPECTRUM_DTYPE),
+    ("medium_emission", "<f4", 4), ("s_medium_emission", SPECTRUM_DTYPE), ("majorant", "<f4", 2), ("pad2", "<f4", 2),
 ])
 LIGHT_DTYPE = np.dtype([
     ("kind", "<i4"), ("region", "<i4"), ("select_pdf", "<f4"), ("cdf", "<f4"),
@@ -44,7 +47,7 @@ LIGHT_DTYPE = np.dtype([
 ])
 INSTANCE_DTYPE = np.dtype([("q", "<f4", 4), ("ts", "<f4", 4)])
 assert SPECTRUM_DTYPE.itemsize == 32 and INSTANCE_DTYPE.itemsize == 32
-assert OBJECT_DTYPE.itemsize == 112 and MATERIAL_DTYPE.itemsize == 320 and LIGHT_DTYPE.itemsize == 144
+assert OBJECT_DTYPE.itemsize == 112 and MATERIAL_DTYPE.itemsize == 384 and LIGHT_DTYPE.itemsize == 144
 
 
 class PtEnv(C.Structure):
@@ -71,7 +74,7 @@ class PtProgram(C.Structure):
                 ("rr_start", C.c_int32), ("rr_max_survival", C.c_float), ("color", C.c_int32),
                 ("width", C.c_int32), ("height", C.c_int32), ("frames_in_flight", C.c_int32),
                 ("max_null_crossings", C.c_int32), ("wavelengths", C.c_int32), ("medium_light_sampling", C.c_int32),
-                ("variance", C.c_int32), ("pad", C.c_int32)]
+                ("variance", C.c_int32), ("volume_sampling", C.c_int32)]
 
 
 class PtParams(C.Structure):

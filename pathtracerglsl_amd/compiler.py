@@ -36,6 +36,8 @@ class RenderStrategy:
     wavelengths: str = "visible"          # estimator.wavelengths (spectral): 'visible' | 'uniform'
     medium_light_sampling: str = "vertex" # estimator.mediumLightSampling: 'vertex' | 'equiangular'
     variance: bool = False                # accumulate the variance statistic picture
+    volume_sampling: str = "analytic"     # estimator.volumeSampling: 'analytic' | 'delta-tracking'
+    scattering: str = "full"              # measurement.scattering: 'full' | 'ignored' (absorbing-only media)
 
 
 @dataclass
@@ -81,6 +83,20 @@ def camera_params(camera, width, height, reset_salt=1234):
     return prm
 
 
+def _rgb3(v):
+    return [float(v)] * 3 if not hasattr(v, "__len__") else [float(x) for x in v]
+
+
+def medium_arm(med, strategy):
+    """The arm a constant medium is sampled by (src/compiler/types.ts:351-417: mediumScatters,
+    isEmissiveMedium, mediumRoutesToTracking; features/materials.ts:675-728 dispatch)."""
+    scatters = strategy.scattering == "full" and any(x != 0.0 for x in _rgb3(med.get("sigma_s", 0.0)))
+    emissive = any(x > 0.0 for x in spectral.rgb_of(med.get("emission", 0.0)))
+    if scatters and emissive:
+        return abi.MED_DELTA
+    return abi.MED_ANALYTIC if scatters else abi.MED_ABSORB
+
+
 def _analyze_validate(scene, strategy):
     if strategy.direct_lighting not in ESTIMATORS:
         raise CompileError(f"unknown directLighting '{strategy.direct_lighting}'")
@@ -102,6 +118,24 @@ def _analyze_validate(scene, strategy):
             raise CompileError("mediumLightSampling 'equiangular' requires every light to be a point light")
         if scene.get("environment", {"type": "none"}).get("type", "none") != "none":
             raise CompileError("mediumLightSampling 'equiangular' does not combine with an environment light")
+    # volumeSampling axis (Validator.ts:497-615): a constant-emission SCATTERING medium routes to
+    # the null-collision (delta-tracking) arm; everything else stays on the analytic arms
+    if strategy.scattering not in ("full", "ignored"):
+        raise CompileError(f"unknown measurement.scattering '{strategy.scattering}'")
+    if strategy.volume_sampling not in ("analytic", "delta-tracking"):
+        raise CompileError(f"volumeSampling '{strategy.volume_sampling}' not supported — use 'analytic' (constant "
+                           "media, exact) or 'delta-tracking' (null-collision)")
+    tracked = [n for n, m in scene.get("materials", {}).items()
+               if m.get("medium") is not None and medium_arm(m["medium"], strategy) == abi.MED_DELTA]
+    if strategy.volume_sampling == "delta-tracking":
+        if strategy.direct_lighting == "mis":
+            raise CompileError("volumeSampling 'delta-tracking' with directLighting 'mis' is reserved — delta "
+                               "tracking's distance pdf is unknowable; use 'nee' or 'none'")
+    elif tracked:
+        raise CompileError("This scene has media that need the null-collision arms (an emissive scattering medium: "
+                           f"{', '.join(tracked)}) but volumeSampling is 'analytic' — set volumeSampling 'delta-tracking'")
+    if tracked and strategy.medium_light_sampling == "equiangular":
+        raise CompileError("mediumLightSampling 'equiangular' does not support media on the null-collision arms; use 'vertex'")
     if strategy.color == "spectral" and scene.get("environment", {"type": "none"}).get("type", "none") != "none":
         raise CompileError("spectral programs with an environment light are not supported by this backend")
     mats = scene.get("materials", {})
@@ -170,14 +204,18 @@ def compile_scene(scene, strategy=None, width=256, height=256, reset_salt=1234):
         rec["ior"] = float(m.get("ior", 1.5))
         med = m.get("medium")
         if med is not None:
-            def rgb(v):
-                return [float(v)] * 3 if not hasattr(v, "__len__") else [float(x) for x in v]
             rec["has_medium"] = 1
-            rec["sigma_a"][:3] = rgb(med.get("sigma_a", 0.0))
-            rec["sigma_s"][:3] = rgb(med.get("sigma_s", 0.0))
-            rec["s_sigma_a"] = srec(rgb(med.get("sigma_a", 0.0)), spectral.ROLE_COEFFICIENT)
-            rec["s_sigma_s"] = srec(rgb(med.get("sigma_s", 0.0)), spectral.ROLE_COEFFICIENT)
+            rec["medium_arm"] = medium_arm(med, strategy)
+            # measurement.scattering 'ignored': the medium renders absorbing-only, shadows included
+            sig_s = _rgb3(med.get("sigma_s", 0.0)) if strategy.scattering == "full" else [0.0, 0.0, 0.0]
+            rec["sigma_a"][:3] = _rgb3(med.get("sigma_a", 0.0))
+            rec["sigma_s"][:3] = sig_s
+            rec["s_sigma_a"] = srec(_rgb3(med.get("sigma_a", 0.0)), spectral.ROLE_COEFFICIENT)
+            rec["s_sigma_s"] = srec(sig_s, spectral.ROLE_COEFFICIENT)
             rec["hg_g"] = float(med.get("phase_g", 0.0))
+            eps = med.get("emission", 0.0)
+            rec["medium_emission"][:3] = spectral.rgb_of(eps)
+            rec["s_medium_emission"] = srec(eps, spectral.ROLE_EMISSION)
         materials.append(rec)
 
     tri_chunks, tri_obj_chunks, objects = [], [], []
@@ -332,10 +370,16 @@ def compile_scene(scene, strategy=None, width=256, height=256, reset_salt=1234):
     prog.color = abi.COLOR_SPECTRAL if spec else abi.COLOR_RGB
     prog.wavelengths = abi.WL_UNIFORM if strategy.wavelengths == "uniform" else abi.WL_VISIBLE
     prog.medium_light_sampling = abi.MLS_EQUIANGULAR if strategy.medium_light_sampling == "equiangular" else abi.MLS_VERTEX
+    prog.volume_sampling = abi.VS_DELTA_TRACKING if strategy.volume_sampling == "delta-tracking" else abi.VS_ANALYTIC
     prog.variance = int(bool(strategy.variance))
     prog.width, prog.height = width, height
     prog.frames_in_flight = max(1, strategy.frames_in_flight)
     prog.max_null_crossings = strategy.max_null_crossings
+    warnings = []
+    if strategy.volume_sampling == "delta-tracking" and not any(int(r["medium_arm"]) == abi.MED_DELTA and r["has_medium"]
+                                                                 for r in materials):
+        warnings.append("volumeSampling 'delta-tracking' controls nothing here — no medium is emissive and "
+                        "scattering; constant media stay on the analytic arms")
     return CompiledRenderer(id=f"{strategy.color}-{strategy.direct_lighting}-{scene.get('id', 'scene')}", scene=arrays,
                             program=prog, width=width, height=height, camera=scene["camera"],
-                            reset_salt=reset_salt, material_names=list(mat_names))
+                            reset_salt=reset_salt, warnings=warnings, material_names=list(mat_names))

@@ -120,6 +120,7 @@ struct DevScene {
     int has_transmission;
     int emissive_solid;
     int ambient_material;       // material of region -1 (ambientMedium), -1 = none
+    int weighted_absorption;    // some medium is on the analytic scattering arm (interior roulette runs)
     int has_media, has_null;
     // instance batches: binary TLAS over placement world boxes, placements (world -> local
     // quaternion, (t_rigid, s)), owning batch, first primitive-id offset; prototype triangles

@@ -526,17 +526,24 @@ __global__ void __launch_bounds__(SBLOCK) k_bin(DevScene Sk, Paths P, int qsel, 
                         eq_shadow = true;
                     }
                     float x0 = rng.next(), x1 = rng.next();
-                    MSample ms = medium_sample_analytic<SPEC>(S, S.materials[med_mat], prim >= 0 ? hv.x : MAX_DIST, x0, x1,
-                                                              load_wl<SPEC>(P, path));
+                    const pt_material& Mm = S.materials[med_mat];
+                    const Wl wl = load_wl<SPEC>(P, path);
+                    // medium_sample dispatch by the material's arm (features/materials.ts:675-728)
+                    MSample ms = Mm.medium_arm == PT_MED_DELTA ? medium_sample_delta<SPEC>(S, Mm, t_max, x0, x1, rng, wl)
+                               : Mm.medium_arm == PT_MED_ABSORB ? medium_sample_absorb<SPEC>(S, Mm, t_max, wl)
+                               : medium_sample_analytic<SPEC>(S, Mm, t_max, x0, x1, wl);
                     sp4 T; float eta;
                     load_thr<SPEC>(P, path, T, eta);
+                    // the source term, weighted by segment-start throughput (pt.ts:333-339)
+                    if (medium_emits(Mm)) store_rad<SPEC>(P, path, load_rad<SPEC>(P, path) + T * ms.radiance);
                     store_thr<SPEC>(P, path, T * ms.weight, eta);
                     if (ADJ && med_mat == P.adj_medium) {   // f = sigma_s e^{-sigma_t t} | e^{-sigma_t t_max}
                         const float* ssv = S.materials[med_mat].sigma_s;
 #pragma unroll
                         for (int ch = 0; ch < 3; ch++) {
                             const size_t ks = (size_t)ch * P.cap + path, ka = (size_t)(3 + ch) * P.cap + path;
-                            P.adj_D[ks] += (ms.scattered && ssv[ch] > 0.0f ? 1.0f / ssv[ch] : 0.0f) - ms.t;
+                            if (Mm.medium_arm != PT_MED_ABSORB)
+                                P.adj_D[ks] += (ms.scattered && ssv[ch] > 0.0f ? 1.0f / ssv[ch] : 0.0f) - ms.t;
                             P.adj_D[ka] -= ms.t;
                         }
                     }
@@ -688,10 +695,14 @@ __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene Sk, Paths P, int qsel
                     f3 wi = hg_sample<SPEC>(wo_med, g, z0, z1, ppdf);
                     if (param_medium) { A.D[6] += dlog_hg(g, dot(wi, -wo_med)); A.dirty = true; }
                     cont = true;
-                    // roulette_interior with medium_survival = sigma_s / max(sigma_t, 1e-9)
-                    if (rr_start >= 0 && bounce >= rr_start) {
-                        sp4 ss = sigma_s_of<SPEC>(S, Mm, wl), st = ss + sigma_a_of<SPEC>(S, Mm, wl);
-                        sp4 surv = mk4(ss.x / gmax(st.x, 1e-9f), ss.y / gmax(st.y, 1e-9f), ss.z / gmax(st.z, 1e-9f), ss.w / gmax(st.w, 1e-9f));
+                    // roulette_interior, present when some medium weights absorption; medium_survival =
+                    // sigma_s / max(sigma_t, 1e-9) there, 1 on the tracking arm (features/materials.ts:731-772)
+                    if (rr_start >= 0 && bounce >= rr_start && S.weighted_absorption) {
+                        sp4 surv = sgrey(1.0f);
+                        if (Mm.medium_arm == PT_MED_ANALYTIC) {
+                            sp4 ss = sigma_s_of<SPEC>(S, Mm, wl), st = ss + sigma_a_of<SPEC>(S, Mm, wl);
+                            surv = mk4(ss.x / gmax(st.x, 1e-9f), ss.y / gmax(st.y, 1e-9f), ss.z / gmax(st.z, 1e-9f), ss.w / gmax(st.w, 1e-9f));
+                        }
                         float p = spectrum_max<SPEC>(surv);
                         if (p <= 0.0f) cont = false;
                         else if (rng.next() > p) cont = false;
@@ -1128,6 +1139,20 @@ void pt_ctx_destroy(pt_ctx* ctx) {
     delete ctx;
 }
 
+// The delta-tracking majorant of a medium record (features/materials.ts:774-789
+// derivedMajorant: max(1e-6, max_c(sigma_a + sigma_s)) on the live values), as the RGB colour
+// model keeps it (components/color/rgb/rgb.ts:5) and as the spectral model bounds it: the float32
+// step above 4 c (1 + 2^-20)^5 / (1 - 2^-20)^2 (components/color/spectral/majorant.ts).
+static void derive_majorants(pt_material& m) {
+    double c = 1e-6;
+    for (int k = 0; k < 3; k++) c = std::max(c, (double)m.sigma_a[k] + (double)m.sigma_s[k]);
+    m.majorant[0] = (float)c;
+    const double ceiling = (double)(float)c;
+    const double up = 1.0 + std::ldexp(1.0, -20), down = 1.0 - std::ldexp(1.0, -20);
+    const double bound = 4.0 * ceiling * up * up * up * up * up / (down * down);
+    m.majorant[1] = std::nextafter((float)bound, INFINITY);
+}
+
 // Depth of a binary BVH (nodes of 8 floats: min.xyz, count or -1; max.xyz, offset or right
 // child); the device walks keep a 64-entry private stack.
 static int binary_depth(const std::vector<float>& nodes) {
@@ -1148,6 +1173,10 @@ static pt_status validate_scene(const pt_scene_desc* sd) {
     if (sd->n_tris < 0 || sd->n_objects < 0 || sd->n_materials <= 0 || sd->n_lights < 0)
         return fail(PT_E_INVALID, "pt_scene_upload: negative count or no materials");
     if (sd->n_tris > 0 && (!sd->tri_vertices || !sd->tri_object)) return fail(PT_E_MISSING_INPUT, "missing channel: triangles");
+    if (!sd->materials) return fail(PT_E_MISSING_INPUT, "missing channel: materials");
+    for (int m = 0; m < sd->n_materials; m++)
+        if (sd->materials[m].has_medium && (sd->materials[m].medium_arm < PT_MED_ANALYTIC || sd->materials[m].medium_arm > PT_MED_DELTA))
+            return fail(PT_E_INVALID, "material " + std::to_string(m) + ": unknown medium arm");
     if (sd->n_objects > 0 && !sd->objects) return fail(PT_E_MISSING_INPUT, "missing channel: objects");
     if (!sd->materials) return fail(PT_E_MISSING_INPUT, "missing channel: materials");
     if (sd->n_lights > 0 && !sd->lights) return fail(PT_E_MISSING_INPUT, "missing channel: lights");
@@ -1222,8 +1251,12 @@ static pt_status build_scene(const pt_scene_desc* sd, hipStream_t stream, SceneS
         if (sd->objects[o].kind == PT_OBJ_QUAD || sd->objects[o].kind == PT_OBJ_SPHERE) analytic.push_back(o);
     if ((st = SS.upload(analytic.data(), analytic.size(), &D.analytic)) != PT_OK) return st;
     D.n_analytic = (int)analytic.size();
-    if ((st = SS.upload(sd->materials, (size_t)sd->n_materials, &D.materials)) != PT_OK) return st;
     SS.host_materials.assign(sd->materials, sd->materials + sd->n_materials);
+    for (pt_material& m : SS.host_materials) {
+        derive_majorants(m);
+        if (m.has_medium && m.medium_arm == PT_MED_ANALYTIC) D.weighted_absorption = 1;
+    }
+    if ((st = SS.upload(SS.host_materials.data(), SS.host_materials.size(), &D.materials)) != PT_OK) return st;
     if ((st = SS.upload(sd->lights, (size_t)sd->n_lights, &D.lights)) != PT_OK) return st;
     D.n_lights = sd->n_lights;
     SS.has_ggx = false;
@@ -1462,6 +1495,8 @@ pt_status pt_renderer_load(pt_ctx* ctx, const pt_program* prog, pt_renderer** ou
     if (prog->wavelengths != PT_WL_VISIBLE && prog->wavelengths != PT_WL_UNIFORM) return fail(PT_E_UNSUPPORTED, "unknown wavelength sampler");
     if (prog->medium_light_sampling != PT_MLS_VERTEX && prog->medium_light_sampling != PT_MLS_EQUIANGULAR)
         return fail(PT_E_UNSUPPORTED, "unknown medium light sampling");
+    if (prog->volume_sampling != PT_VS_ANALYTIC && prog->volume_sampling != PT_VS_DELTA_TRACKING)
+        return fail(PT_E_UNSUPPORTED, "unknown volume sampling");
     if (prog->medium_light_sampling == PT_MLS_EQUIANGULAR && prog->estimator != PT_EST_NEE)
         return fail(PT_E_UNSUPPORTED, "equiangular medium NEE runs under the nee estimator");
     if (prog->color == PT_COLOR_SPECTRAL && ctx->has_scene && ctx->dev.env_type != PT_ENV_NONE)
@@ -1685,6 +1720,10 @@ pt_status pt_backward(pt_renderer* r, const pt_params* prm, uint32_t nframes, co
     const pt_program& pg = r->prog;
     if (pg.color != PT_COLOR_RGB) return fail(PT_E_UNSUPPORTED, "pt_backward: the adjoint runs RGB programs only");
     if (pg.medium_light_sampling != PT_MLS_VERTEX) return fail(PT_E_UNSUPPORTED, "pt_backward: the adjoint runs vertex medium NEE only");
+    for (const pt_material& m : ctx->host_materials)
+        if (m.has_medium && (m.medium_arm == PT_MED_DELTA || m.medium_emission[0] != 0.0f || m.medium_emission[1] != 0.0f ||
+                             m.medium_emission[2] != 0.0f))
+            return fail(PT_E_UNSUPPORTED, "pt_backward: emissive media (and the delta-tracking arm) are outside the adjoint");
     if (nframes == 0) return fail(PT_E_INVALID, "pt_backward: nframes must be >= 1");
     if (adj->reduce_mode != PT_REDUCE_ATOMIC && adj->reduce_mode != PT_REDUCE_DETERMINISTIC)
         return fail(PT_E_INVALID, "pt_backward: unknown reduce_mode");
@@ -2087,13 +2126,15 @@ pt_status pt_scene_update_materials(pt_ctx* ctx, const pt_material* materials, i
     for (int m = 0; m < n; m++) {
         const pt_material& a = ctx->host_materials[m];
         const pt_material& b = materials[m];
-        if (a.model != b.model || a.has_medium != b.has_medium || a.emissive != b.emissive)
-            return fail(PT_E_INVALID, "pt_scene_update_materials: model, has_medium and emissive must not change");
+        if (a.model != b.model || a.has_medium != b.has_medium || a.emissive != b.emissive || a.medium_arm != b.medium_arm)
+            return fail(PT_E_INVALID, "pt_scene_update_materials: model, has_medium, medium_arm and emissive must not change");
     }
+    std::vector<pt_material> next(materials, materials + n);
+    for (pt_material& m : next) derive_majorants(m);   // the majorant follows live coefficients
     HIP_TRY(hipSetDevice(ctx->device));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
-    HIP_TRY(hipMemcpy((void*)ctx->dev.materials, materials, sizeof(pt_material) * (size_t)n, hipMemcpyHostToDevice));
-    ctx->host_materials.assign(materials, materials + n);
+    HIP_TRY(hipMemcpy((void*)ctx->dev.materials, next.data(), sizeof(pt_material) * (size_t)n, hipMemcpyHostToDevice));
+    ctx->host_materials = std::move(next);
     return PT_OK;
 }
 

@@ -630,7 +630,7 @@ PT_DEV bool material_has_medium(const DevScene& S, int mat) { return mat >= 0 &&
 PT_DEV sp4 spectrum_exp(sp4 s) { return mk4(ptm_exp(s.x), ptm_exp(s.y), ptm_exp(s.z), ptm_exp(s.w)); }
 template <bool SPEC> PT_DEV sp4 sigma_s_of(const DevScene& S, const pt_material& m, const Wl& wl) { return lift<SPEC>(S, m.sigma_s, m.s_sigma_s, wl); }
 template <bool SPEC> PT_DEV sp4 sigma_a_of(const DevScene& S, const pt_material& m, const Wl& wl) { return lift<SPEC>(S, m.sigma_a, m.s_sigma_a, wl); }
-struct MSample { bool scattered; float t; sp4 weight; };
+struct MSample { bool scattered; float t; sp4 weight; sp4 radiance; };
 template <bool SPEC>
 PT_DEV MSample medium_sample_analytic(const DevScene& S, const pt_material& m, float t_max, float x0, float x1, const Wl& wl) {
     MSample ms;
@@ -651,6 +651,71 @@ PT_DEV MSample medium_sample_analytic(const DevScene& S, const pt_material& m, f
         ms.scattered = false; ms.t = t_max;
         ms.weight = tr / gmax(pdf, 1e-20f);
     }
+    ms.radiance = sgrey(0.0f);
+    return ms;
+}
+template <bool SPEC> PT_DEV sp4 medium_emission_of(const DevScene& S, const pt_material& m, const Wl& wl) {
+    return lift<SPEC>(S, m.medium_emission, m.s_medium_emission, wl);
+}
+PT_DEV bool medium_emits(const pt_material& m) {
+    return m.medium_emission[0] != 0.0f || m.medium_emission[1] != 0.0f || m.medium_emission[2] != 0.0f;
+}
+// integral_0^len exp(-sigma s) ds, Taylor form below optical depth 0.1 (analytic/emission.glsl:4-13)
+PT_DEV float absorption_integral1(float sigma, float len) {
+    const float tau = sigma * len;
+    return tau < 0.1f ? len * (1.0f + tau * (-0.5f + tau * (1.0f / 6.0f + tau * (-1.0f / 24.0f + tau / 120.0f))))
+                      : (1.0f - ptm_exp(-tau)) / sigma;
+}
+// Deterministic absorbing arm (features/materials.ts:708-718): no collision, weight
+// exp(-sigma_a t_max), emission eps * integral of the attenuation along the segment.
+template <bool SPEC>
+PT_DEV MSample medium_sample_absorb(const DevScene& S, const pt_material& m, float t_max, const Wl& wl) {
+    MSample ms;
+    const sp4 sa = sigma_a_of<SPEC>(S, m, wl);
+    ms.scattered = false; ms.t = t_max;
+    ms.weight = spectrum_exp((-sa) * t_max);
+    ms.radiance = sgrey(0.0f);
+    if (medium_emits(m))
+        ms.radiance = medium_emission_of<SPEC>(S, m, wl) *
+                      mk4(absorption_integral1(sa.x, t_max), absorption_integral1(sa.y, t_max),
+                          absorption_integral1(sa.z, t_max), absorption_integral1(sa.w, t_max));
+    return ms;
+}
+// Delta tracking on a constant medium at the scalar majorant (delta_tracking.glsl:46-120: Kutz
+// et al. 2017 Alg. 4, history-aware average probabilities; tracking_cap bounds the walk):
+// emission per tentative collision before the lottery, absorption terminates, null collisions
+// reweight and draw again from the path's stream.
+template <bool SPEC>
+PT_DEV MSample medium_sample_delta(const DevScene& S, const pt_material& m, float t_max, float x0, float x1, Rng& rng,
+                                   const Wl& wl) {
+    MSample ms;
+    ms.scattered = false; ms.t = t_max; ms.weight = sgrey(1.0f); ms.radiance = sgrey(0.0f);
+    const float sb = SPEC ? m.majorant[1] : m.majorant[0];
+    const sp4 sa = sigma_a_of<SPEC>(S, m, wl), ss = sigma_s_of<SPEC>(S, m, wl), eps = medium_emission_of<SPEC>(S, m, wl);
+    const sp4 st = sa + ss;
+    const sp4 sn = mk4(gmax(sb - st.x, 0.0f), gmax(sb - st.y, 0.0f), gmax(sb - st.z, 0.0f), gmax(sb - st.w, 0.0f));
+    sp4 w = sgrey(1.0f);
+    float t = 0.0f, xd = x0, xe = x1;
+    const float lam = sb * t_max;
+    const int cap = (int)ceilf(lam + 6.5f * sqrtf(lam) + 12.0f);
+    for (int i = 0; i < cap; i++) {
+        t += -ptm_log(1.0f - xd) / sb;
+        if (t >= t_max) { ms.weight = w; return ms; }
+        ms.radiance = ms.radiance + (w * eps) / sb;
+        float pa = spectrum_average<SPEC>(sa * w), ps = spectrum_average<SPEC>(ss * w), pn = spectrum_average<SPEC>(sn * w);
+        const float c = pa + ps + pn;
+        if (c <= 0.0f) { ms.weight = sgrey(0.0f); return ms; }
+        pa /= c; ps /= c; pn /= c;
+        if (xe < pa) { ms.weight = sgrey(0.0f); return ms; }
+        if (xe < pa + ps) {
+            ms.scattered = true; ms.t = t;
+            ms.weight = (w * ss) / (sb * gmax(ps, 1e-20f));
+            return ms;
+        }
+        w = w * (sn / (sb * gmax(pn, 1e-20f)));
+        xd = rng.next(); xe = rng.next();
+    }
+    ms.weight = w;
     return ms;
 }
 template <bool SPEC>

@@ -29,6 +29,8 @@ COLOURS = [
     ((0.3, 0.7, 1.2), "coefficient"), ((0.1, 0.3, 0.8), "coefficient"),
     ((0.1, 0.2, 0.4), "coefficient"), ((0.5, 1.0, 2.0), "coefficient"),
     ((1.0, 2.0, 4.0), "coefficient"), ((0.9, 0.9, 0.9), "reflectance"),
+    ((0.4, 0.8, 1.2), "emission"), ((1.0, 2.0, 4.0), "emission"),                   # medium emission witnesses
+    ((0.8, 0.4, 0.2), "emission"), ((0.02, 0.05, 0.1), "coefficient"), ((0.2, 0.4, 0.7), "coefficient"),
 ]
 
 

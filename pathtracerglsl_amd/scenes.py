@@ -500,3 +500,57 @@ def transformed_triangles(tris, rotation, translation, scale):
     c = np.cross(u, p) + w * p
     r = p + 2.0 * np.cross(u, c)
     return (r + np.asarray(translation, np.float64)).reshape(-1, 9)
+
+
+# ------------------------------------------------------------ medium emission witnesses
+
+def emit_slab(sigma_a=1.0, emission=(0.5, 1.0, 2.0)):
+    """F-EMIT (tests/witnesses/scenes/emissionWitness.ts:22-56): the F-SLAB geometry with a
+    glowing absorbing ink (volume emission coefficient eps) in front of the L = 1 screen:
+    L = eps / sigma_a (1 - e^{-sigma_a}) + e^{-sigma_a} = (0.68394, 1, 1.63212) at the centre; the
+    green channel is the equilibrium eps = sigma_a L_back, identically 1."""
+    sc = slab_witness()
+    sc["id"] = "emit"
+    sc["materials"]["ink"] = {"model": "none", "medium": {"sigma_a": sigma_a, "emission": list(emission)
+                                                          if hasattr(emission, "__len__") else emission}}
+    return sc
+
+
+def emit_sat():
+    """F-EMIT-SAT (emissionWitness.ts:78-105): the camera deep inside a uniform glowing scattering
+    ambient medium (sigma_a 2, sigma_s 1, eps (1, 2, 4), isotropic): every pixel converges to
+    eps / sigma_a = (0.5, 1, 2); at maxBounces N it is (eps / sigma_t) sum_{n<=N} 3^-n."""
+    return {
+        "id": "emit-sat",
+        "materials": {"gray": {"model": "lambert", "albedo": 0.5},
+                      "glow": {"model": "none", "medium": {"sigma_a": 2.0, "sigma_s": 1.0, "emission": [1.0, 2.0, 4.0],
+                                                           "phase_g": 0.0}}},
+        "objects": [{"type": "sphere", "center": [0.0, 1.0, 0.0], "radius": 0.5, "material": "gray"}],
+        "lights": [],
+        "environment": {"type": "none"},
+        "ambient_medium": "glow",
+        "camera": {"type": "pinhole", "position": [30.0, 1.0, 0.0], "target": [31.0, 1.0, 0.0], "fov": 0.8},
+    }
+
+
+def emit_scatter(sigma_a=(0.1, 0.1, 0.1), sigma_s=(0.6, 0.6, 0.6)):
+    """emit-scatter (emissionWitness.ts:153-188): a glowing scattering fog box (eps (0.8, 0.4, 0.2),
+    g 0.3) in a grey room (floor y = 0, ceiling y = 2.5, back wall z = -3 as 100 x 100 quads)
+    under a downward ceiling quad light (emission 12): pt-nee and pt estimate the same image
+    (volumes are never light-sampled)."""
+    return {
+        "id": "emit-scatter",
+        "materials": {"gray": {"model": "lambert", "albedo": 0.5},
+                      "glowfog": {"model": "none", "medium": {"sigma_a": list(sigma_a), "sigma_s": list(sigma_s),
+                                                              "emission": [0.8, 0.4, 0.2], "phase_g": 0.3}}},
+        "objects": [
+            {"type": "quad", "corner": [-50.0, 0.0, -50.0], "edge1": [0.0, 0.0, 100.0], "edge2": [100.0, 0.0, 0.0], "material": "gray"},
+            {"type": "quad", "corner": [-50.0, 2.5, -50.0], "edge1": [100.0, 0.0, 0.0], "edge2": [0.0, 0.0, 100.0], "material": "gray"},
+            {"type": "quad", "corner": [-50.0, -50.0, -3.0], "edge1": [100.0, 0.0, 0.0], "edge2": [0.0, 100.0, 0.0], "material": "gray"},
+            {"type": "mesh", "triangles": _box_tris((-1.0, 0.0, -1.5), (1.0, 2.0, 0.5)), "material": "glowfog", "closed": True},
+        ],
+        "lights": [{"type": "quad", "corner": [-0.5, 2.48, -1.0], "edge1": [1.0, 0.0, 0.0], "edge2": [0.0, 0.0, 1.0],
+                    "radiance": [12.0, 12.0, 12.0]}],
+        "environment": {"type": "none"},
+        "camera": {"type": "pinhole", "position": [0.0, 1.2, 4.2], "target": [0.0, 1.0, 0.0], "fov": 0.9},
+    }
