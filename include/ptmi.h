@@ -78,6 +78,7 @@ enum { PT_MAT_LAMBERT = 0, PT_MAT_DIELECTRIC = 1, PT_MAT_GGX = 2, PT_MAT_NULL = 
  *                    (delta_tracking.glsl:56-120), emission collected per tentative collision
  * The host decides the arm from the authored values, as the reference's Planner does. */
 enum { PT_MED_ANALYTIC = 0, PT_MED_ABSORB = 1, PT_MED_DELTA = 2 };
+enum { PT_IOR_CONSTANT = 0, PT_IOR_CAUCHY = 1, PT_IOR_SELLMEIER = 2 };
 
 /* A spectral value in one radiometric role (src/components/descriptors.ts:43), as the
  * spectral color model evaluates it at a path's four wavelengths
@@ -129,7 +130,14 @@ typedef struct pt_material {
      * mediumBound): [0] RGB = max(1e-6, max_c(sigma_a + sigma_s)); [1] spectral = the
      * float32 bound of components/color/spectral/majorant.ts. Values passed in are ignored. */
     float   majorant[2];
-    float   pad2[2];
+    /* Dispersive index of a dielectric (src/components/color/spectra/cauchy, sellmeier), read at
+     * the hero wavelength by spectral programs (ior_of, features/intersection.ts:1609-1649);
+     * RGB programs and PT_IOR_CONSTANT use `ior` (the index at 587.5618 nm, host-evaluated).
+     *   PT_IOR_CAUCHY     ior_coef = A, B, C (micrometres): A + B / l^2 + C / l^4
+     *   PT_IOR_SELLMEIER  ior_coef = B1 B2 B3 C1 C2 C3: n^2 = 1 + sum B_i l^2 / (l^2 - C_i) */
+    int32_t ior_kind;
+    float   pad2;
+    float   ior_coef[8];
 } pt_material;
 
 enum { PT_LIGHT_QUAD = 0, PT_LIGHT_POINT = 1 };

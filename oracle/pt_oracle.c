@@ -78,6 +78,8 @@ static inline float signf(float x) { return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f 
 
 /* ---------------- RNG: pcg4d (components/sampler/pcg4d/pcg4d.glsl:12-40) ---------------- */
 typedef struct { uint32_t base, dim; } rng_t;
+/* the path's wavelengths: lam[0] is the hero (spectral programs) */
+typedef struct { int spec; int uniform; float lam[4]; } wl_t;
 static inline void pcg4d(uint32_t v[4]) {
     for (int i = 0; i < 4; i++) v[i] = v[i] * 1664525u + 1013904223u;
     v[0] += v[1] * v[3]; v[1] += v[2] * v[0]; v[2] += v[0] * v[1]; v[3] += v[1] * v[2];
@@ -796,12 +798,40 @@ static int scene_intersect(const or_scene* S, v3 ro, v3 rd, hit_t* hit) {
 static inline int material_of(const or_scene* S, int region) { return region < 0 ? S->ambient_material : S->objects[region].material; }
 static inline int is_null_interface(const or_scene* S, int mat) { return mat >= 0 && S->materials[mat].model == PT_MAT_NULL; }
 static inline int material_has_medium(const or_scene* S, int mat) { return mat >= 0 && S->materials[mat].has_medium; }
-static inline float ior_of(const or_scene* S, int region) {
+/* cauchy_index / sellmeier_index (components/color/spectra/cauchy/cauchy.glsl,
+ * sellmeier/sellmeier.glsl): coefficients in micrometres, the carrier in nm. */
+static inline float cauchy_index(const float* c, float lam) {
+    float inv2 = 1e6f / (lam * lam);
+    return c[0] + inv2 * (c[1] + inv2 * c[2]);
+}
+static inline float sellmeier_index(const float* c, float lam) {
+    float l2 = (lam * 0.001f) * (lam * 0.001f);
+    float t0 = c[0] * l2 / (l2 - c[3]), t1 = c[1] * l2 / (l2 - c[4]), t2 = c[2] * l2 / (l2 - c[5]);
+    return sqrtf(1.0f + t0 + t1 + t2);
+}
+/* Regions with a row in the generated ior_of table (features/intersection.ts:1609-1649): every
+ * primitive object, closed meshes and closed instance batches; open meshes are thin (1.0). */
+static inline int region_has_index_row(const pt_object* ob) {
+    return !((ob->kind == PT_OBJ_MESH || ob->kind == PT_OBJ_INSTANCES) && !ob->closed);
+}
+/* ior_of: the dielectric's index, at the hero wavelength wavelengths_hero (spectral.glsl:12-15)
+ * for a dispersive formula under the spectral colour model. */
+static inline float ior_of(const or_scene* S, int region, const wl_t* wl) {
     if (region < 0) return 1.0f;
     const pt_object* ob = &S->objects[region];
-    if (ob->kind == PT_OBJ_QUAD || ((ob->kind == PT_OBJ_MESH || ob->kind == PT_OBJ_INSTANCES) && !ob->closed)) return 1.0f;
+    if (!region_has_index_row(ob)) return 1.0f;
     const pt_material* m = &S->materials[ob->material];
-    return m->model == PT_MAT_DIELECTRIC ? m->ior : 1.0f;
+    if (m->model != PT_MAT_DIELECTRIC) return 1.0f;
+    if (wl->spec && m->ior_kind == PT_IOR_CAUCHY) return cauchy_index(m->ior_coef, wl->lam[0]);
+    if (wl->spec && m->ior_kind == PT_IOR_SELLMEIER) return sellmeier_index(m->ior_coef, wl->lam[0]);
+    return m->ior;
+}
+/* A region of the program's dispersion record (Planner.ts:1038-1053 planDispersion). */
+static inline int dispersive_region(const or_scene* S, int region) {
+    if (region < 0) return 0;
+    const pt_object* ob = &S->objects[region];
+    const pt_material* m = &S->materials[ob->material];
+    return region_has_index_row(ob) && m->model == PT_MAT_DIELECTRIC && m->ior_kind != PT_IOR_CONSTANT;
 }
 
 static inline v3 lv(const float* a) { return V(a[0], a[1], a[2]); }
@@ -810,7 +840,6 @@ static inline v3 lv(const float* a) { return V(a[0], a[1], a[2]); }
  * Four lanes. An RGB program uses lanes 0-2 and reduces over 3; a spectral program carries
  * the path's four hero wavelengths in wl_t and reduces over 4. */
 typedef struct { float x, y, z, w; } sp;
-typedef struct { int spec; int uniform; float lam[4]; } wl_t;
 static inline sp SP(float x, float y, float z, float w) { sp r; r.x = x; r.y = y; r.z = z; r.w = w; return r; }
 static inline sp sgrey(float v) { return SP(v, v, v, v); }
 static inline sp sadd(sp a, sp b) { return SP(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w); }
@@ -944,8 +973,8 @@ static isample_t surface_sample(const or_scene* S, const pt_material* m, v3 wo, 
         s.pdf = cos_theta * INV_PI;
         s.flags = LOBE_REFLECTION;
     } else if (m->model == PT_MAT_DIELECTRIC) {     /* materials/dielectric/dielectric.glsl:14-52 */
-        float n_i = ior_of(S, hit->region_from);
-        float n_t = ior_of(S, hit->region_to);
+        float n_i = ior_of(S, hit->region_from, wl);
+        float n_t = ior_of(S, hit->region_to, wl);
         float eta = n_i / n_t;
         v3 n = f.n;
         float cos_i = clampf(dot(wo, n), 1e-6f, 1.0f);
@@ -1505,6 +1534,7 @@ static sp transport_trace(const or_scene* S, const or_program* P, v3 ro, v3 rd, 
     const int media = S->has_media;
     int current_medium = media ? region_at(S, ro) : -1;
     int null_crossings = 0;
+    int collapsed = 0;
     for (int bounce = 0; bounce <= P->max_bounces; bounce++) {
         hit_t hit;
         (*n_closest)++;
@@ -1631,6 +1661,13 @@ static sp transport_trace(const or_scene* S, const or_program* P, v3 ro, v3 rd, 
             if (A) adj_scatter(A, smul(smuls(T, w), lift(S, E->emission, &E->s_emission, wl)), A->D);
         }
         if (bounce == P->max_bounces) break;
+        /* collapse_wavelengths (pt.ts:440,466-475): a dielectric bounding a dispersive region sends
+         * the path along the hero wavelength; the hero lane carries all four (once per path) */
+        if (wl->spec && !collapsed && M->model == PT_MAT_DIELECTRIC &&
+            (dispersive_region(S, hit.region_from) || dispersive_region(S, hit.region_to))) {
+            T = smul(T, SP(4.0f, 0.0f, 0.0f, 0.0f));
+            collapsed = 1;
+        }
         /* light_sample_direct (techniques/light/light.glsl:8-28) */
         if (nee && has_nondelta(M)) {
             float x0 = rnd(rng), x1 = rnd(rng);
@@ -1682,7 +1719,7 @@ static sp transport_trace(const or_scene* S, const or_program* P, v3 ro, v3 rd, 
         prev_was_delta = (bs.flags & LOBE_DELTA) != 0; prev_bsdf_pdf = bs.pdf; prev_qp = hit.p;
         if (media && (bs.flags & LOBE_TRANSMISSION)) current_medium = hit.region_to;
         if (S->has_transmission && (bs.flags & LOBE_TRANSMISSION)) {
-            float r = ior_of(S, hit.region_to) / ior_of(S, hit.region_from);
+            float r = ior_of(S, hit.region_to, wl) / ior_of(S, hit.region_from, wl);
             eta_scale *= r * r;
         }
         /* roulette (pt.ts:204-222) */

@@ -13,6 +13,7 @@ OBJ_MESH, OBJ_QUAD, OBJ_SPHERE, OBJ_INSTANCES = 0, 1, 2, 3
 MAT_LAMBERT, MAT_DIELECTRIC, MAT_GGX, MAT_NULL = 0, 1, 2, 3
 MED_ANALYTIC, MED_ABSORB, MED_DELTA = 0, 1, 2
 VS_ANALYTIC, VS_DELTA_TRACKING = 0, 1
+IOR_CONSTANT, IOR_CAUCHY, IOR_SELLMEIER = 0, 1, 2
 LIGHT_QUAD, LIGHT_POINT = 0, 1
 ENV_NONE, ENV_SKY = 0, 1
 EST_PT, EST_NEE, EST_MIS = 0, 1, 2
@@ -38,7 +39,8 @@ MATERIAL_DTYPE = np.dtype([
     ("sigma_s", "<f4", 4), ("sigma_a", "<f4", 4),
     ("s_albedo", SPECTRUM_DTYPE), ("s_emission", SPECTRUM_DTYPE), ("s_f0", SPECTRUM_DTYPE),
     ("s_transmittance", SPECTRUM_DTYPE), ("s_sigma_s", SPECTRUM_DTYPE), ("s_sigma_a", SPECTRUM_DTYPE),
-    ("medium_emission", "<f4", 4), ("s_medium_emission", SPECTRUM_DTYPE), ("majorant", "<f4", 2), ("pad2", "<f4", 2),
+    ("medium_emission", "<f4", 4), ("s_medium_emission", SPECTRUM_DTYPE), ("majorant", "<f4", 2),
+    ("ior_kind", "<i4"), ("pad2", "<f4"), ("ior_coef", "<f4", 8),
 ])
 LIGHT_DTYPE = np.dtype([
     ("kind", "<i4"), ("region", "<i4"), ("select_pdf", "<f4"), ("cdf", "<f4"),
@@ -47,7 +49,7 @@ LIGHT_DTYPE = np.dtype([
 ])
 INSTANCE_DTYPE = np.dtype([("q", "<f4", 4), ("ts", "<f4", 4)])
 assert SPECTRUM_DTYPE.itemsize == 32 and INSTANCE_DTYPE.itemsize == 32
-assert OBJECT_DTYPE.itemsize == 112 and MATERIAL_DTYPE.itemsize == 384 and LIGHT_DTYPE.itemsize == 144
+assert OBJECT_DTYPE.itemsize == 112 and MATERIAL_DTYPE.itemsize == 416 and LIGHT_DTYPE.itemsize == 144
 
 
 class PtEnv(C.Structure):

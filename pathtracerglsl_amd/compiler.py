@@ -83,6 +83,59 @@ def camera_params(camera, width, height, reset_salt=1234):
     return prm
 
 
+INDEX_REFERENCE_NM = 587.5618          # components/color/spectra/spectrumKind.ts:15
+SPECTRAL_RANGE_NM = (360.0, 830.0)
+
+
+def cauchy_index(coef, lam_nm):
+    """cauchyIndex (components/color/spectra/cauchy/cauchy.ts): A + B / l^2 + C / l^4, l in um."""
+    a, b, c = (list(coef) + [0.0])[:3]
+    inv2 = (1000.0 / lam_nm) ** 2
+    return a + inv2 * (b + inv2 * c)
+
+
+def sellmeier_index(B, C, lam_nm):
+    """sellmeierIndex (components/color/spectra/sellmeier/sellmeier.ts): three-term, l in um."""
+    l2 = (lam_nm / 1000.0) ** 2
+    return math.sqrt(1.0 + sum(b * l2 / (l2 - c) for b, c in zip(B, C)))
+
+
+def index_record(value, spectral_program, label):
+    """(ior at 587.5618 nm, PT_IOR_* kind, 8 coefficients) of an index property: a number, or
+    {"cauchy": [A, B, C?]} / {"sellmeier": {"B": [..], "C": [..]}} with the kinds' validation.
+    The dispersive formula is kept only for spectral programs (RGB evaluates the reference index);
+    a formula constant in wavelength folds to its value, as the kinds' emit() does."""
+    coef = [0.0] * 8
+    if isinstance(value, dict) and "cauchy" in value:
+        c = value["cauchy"]
+        if (not isinstance(c, (list, tuple)) or not 2 <= len(c) <= 3 or any(not math.isfinite(float(x)) for x in c)
+                or not float(c[0]) > 0 or any(float(x) < 0 for x in c[1:])):
+            raise CompileError(f"{label}: Cauchy requires constant [A, B, C?], with finite A > 0 and B, C >= 0 (micrometre units)")
+        c = [float(x) for x in c] + [0.0] * (3 - len(c))
+        n = cauchy_index(c, INDEX_REFERENCE_NM)
+        dispersive = c[1] != 0.0 or c[2] != 0.0
+        coef[:3] = c
+        return n, (abi.IOR_CAUCHY if spectral_program and dispersive else abi.IOR_CONSTANT), coef
+    if isinstance(value, dict) and "sellmeier" in value:
+        B, C = [float(x) for x in value["sellmeier"]["B"]], [float(x) for x in value["sellmeier"]["C"]]
+        if len(B) != 3 or len(C) != 3 or any(x < 0 or not math.isfinite(x) for x in B + C):
+            raise CompileError(f"{label}: Sellmeier requires three non-negative B and C terms")
+        lo2, hi2 = (SPECTRAL_RANGE_NM[0] / 1000.0) ** 2, (SPECTRAL_RANGE_NM[1] / 1000.0) ** 2
+        if any(lo2 <= c <= hi2 for c in C):
+            raise CompileError(f"{label}: Sellmeier poles must lie outside 360–830 nm")
+        try:
+            ok = sellmeier_index(B, C, SPECTRAL_RANGE_NM[1]) > 0 and math.isfinite(sellmeier_index(B, C, SPECTRAL_RANGE_NM[0]))
+        except ValueError:
+            ok = False
+        if not ok:
+            raise CompileError(f"{label}: Sellmeier must give finite positive n² throughout the spectral interval")
+        n = sellmeier_index(B, C, INDEX_REFERENCE_NM)
+        dispersive = any(b != 0.0 and c != 0.0 for b, c in zip(B, C))
+        coef[:6] = B + C
+        return n, (abi.IOR_SELLMEIER if spectral_program and dispersive else abi.IOR_CONSTANT), coef
+    return float(value), abi.IOR_CONSTANT, coef
+
+
 def _rgb3(v):
     return [float(v)] * 3 if not hasattr(v, "__len__") else [float(x) for x in v]
 
@@ -201,7 +254,10 @@ def compile_scene(scene, strategy=None, width=256, height=256, reset_salt=1234):
         tr = m.get("transmittance", [1.0, 1.0, 1.0])
         rec["transmittance"][:3] = spectral.rgb_of(tr)
         rec["s_transmittance"] = srec(tr, spectral.ROLE_REFLECTANCE)
-        rec["ior"] = float(m.get("ior", 1.5))
+        n_ref, kind, coef = index_record(m.get("ior", 1.5), spec, f"material '{n}': ior")
+        rec["ior"] = n_ref
+        rec["ior_kind"] = kind
+        rec["ior_coef"][:] = coef
         med = m.get("medium")
         if med is not None:
             rec["has_medium"] = 1

@@ -140,7 +140,10 @@ PT_DEV float dlog_hg(float g, float c) {
 
 PT_DEV int fl_bounce(int f) { return (f >> 16) & 0xFFFF; }
 PT_DEV int fl_nulls(int f) { return (f >> 8) & 0xFF; }
-PT_DEV int mk_flags(bool delta, int nulls, int bounce) { return (delta ? 1 : 0) | (nulls << 8) | (bounce << 16); }
+// bit 1: the path's wavelengths collapsed to the hero (pt.ts:466-475, once per path)
+PT_DEV int mk_flags(bool delta, int nulls, int bounce, bool collapsed = false) {
+    return (delta ? 1 : 0) | (collapsed ? 2 : 0) | (nulls << 8) | (bounce << 16);
+}
 
 template <bool SPEC> PT_DEV void load_thr(const Paths& P, int i, sp4& T, float& eta) {
     float4 t = P.thr[i];
@@ -640,6 +643,7 @@ __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene Sk, Paths P, int qsel
             f3 prev_qp = xyz(pv); float prev_pdf = pv.w;
             const int fl = P.flags[i];
             bool prev_delta = (fl & 1) != 0;
+            bool collapsed = SPEC && (fl & 2) != 0;
             int nulls = fl_nulls(fl);
             const int bounce = fl_bounce(fl);
             int medium = MEDIA ? P.medium[i] : -1;
@@ -713,7 +717,7 @@ __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene Sk, Paths P, int qsel
                         P.ray_d[i] = f4(wi, 0.0f);
                         store_thr<SPEC>(P, i, T, eta_scale);
                         P.prev[i] = f4(p_evt, ppdf);
-                        P.flags[i] = mk_flags(false, nulls, bounce + 1);
+                        P.flags[i] = mk_flags(false, nulls, bounce + 1, collapsed);
                     }
                     P.rng[i] = make_uint2(rng.base, rng.dim);
                 }
@@ -736,7 +740,7 @@ __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene Sk, Paths P, int qsel
                     cont = nulls <= max_null;
                     if (cont) {
                         P.ray_o[i] = f4(h.p + (h.ng * side) * h.eps, 0.0f);
-                        P.flags[i] = mk_flags(prev_delta, nulls, bounce);
+                        P.flags[i] = mk_flags(prev_delta, nulls, bounce, collapsed);
                         if (MEDIA) P.medium[i] = medium;
                     }
                 } else {
@@ -761,6 +765,13 @@ __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene Sk, Paths P, int qsel
                         }
                     }
                     if (bounce < max_bounces) {
+                        // collapse_wavelengths (pt.ts:440,466-475): a dielectric bounding a dispersive
+                        // region sends the path along the hero wavelength, lane 0 carrying all four
+                        if (SPEC && CAT == CAT_DIELECTRIC && !collapsed &&
+                            (dispersive_region(S, h.region_from) || dispersive_region(S, h.region_to))) {
+                            T = T * mk4(4.0f, 0.0f, 0.0f, 0.0f);
+                            collapsed = true;
+                        }
                         // light_sample_direct (skipped for delta-only materials)
                         if (NEE && CAT != CAT_DIELECTRIC) {
                             float x0 = rng.next(), x1 = rng.next();
@@ -803,7 +814,7 @@ __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene Sk, Paths P, int qsel
                             prev_qp = h.p;
                             if (MEDIA && (bs.flags & LOBE_TRANSMISSION)) medium = h.region_to;
                             if (S.has_transmission && (bs.flags & LOBE_TRANSMISSION)) {
-                                float r = ior_of(S, h.region_to) / ior_of(S, h.region_from);
+                                float r = ior_of<SPEC>(S, h.region_to, wl) / ior_of<SPEC>(S, h.region_from, wl);
                                 eta_scale *= r * r;
                             }
                             cont = true;
@@ -821,7 +832,7 @@ __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene Sk, Paths P, int qsel
                                 P.ray_d[i] = f4(bs.wi, 0.0f);
                                 store_thr<SPEC>(P, i, T, eta_scale);
                                 P.prev[i] = f4(prev_qp, prev_pdf);
-                                P.flags[i] = mk_flags(prev_delta, nulls, bounce + 1);
+                                P.flags[i] = mk_flags(prev_delta, nulls, bounce + 1, collapsed);
                                 P.rng[i] = make_uint2(rng.base, rng.dim);
                                 if (MEDIA) P.medium[i] = medium;
                             }

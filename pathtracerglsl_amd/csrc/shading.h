@@ -170,12 +170,42 @@ PT_DEV bool object_thin(const DevScene& S, int o) {
     const pt_object& ob = S.objects[o];
     return ob.kind == PT_OBJ_QUAD || ((ob.kind == PT_OBJ_MESH || ob.kind == PT_OBJ_INSTANCES) && !ob.closed);
 }
-PT_DEV float ior_of(const DevScene& S, int region) {
+// The path's four wavelengths (nm); lane 0 is the hero (spectral programs only).
+struct Wl { float4 lam; };
+// Regions with a row in the generated ior_of table (features/intersection.ts:1609-1649): every
+// primitive object, closed meshes and closed instance batches; open meshes are thin (1.0).
+PT_DEV bool region_has_index_row(const pt_object& ob) {
+    return !((ob.kind == PT_OBJ_MESH || ob.kind == PT_OBJ_INSTANCES) && !ob.closed);
+}
+// ior_of: a dielectric region's index; a dispersive formula under the spectral colour model is
+// read at the hero wavelength (spectral.glsl:12-15; cauchy.glsl, sellmeier.glsl, micrometres).
+template <bool SPEC>
+PT_DEV float ior_of(const DevScene& S, int region, const Wl& wl) {
     if (region < 0) return 1.0f;
     const pt_object& ob = S.objects[region];
-    if (ob.kind == PT_OBJ_QUAD || ((ob.kind == PT_OBJ_MESH || ob.kind == PT_OBJ_INSTANCES) && !ob.closed)) return 1.0f;
+    if (!region_has_index_row(ob)) return 1.0f;
     const pt_material& m = S.materials[ob.material];
-    return m.model == PT_MAT_DIELECTRIC ? m.ior : 1.0f;
+    if (m.model != PT_MAT_DIELECTRIC) return 1.0f;
+    if (SPEC && m.ior_kind == PT_IOR_CAUCHY) {
+        const float lam = wl.lam.x;
+        const float inv2 = 1e6f / (lam * lam);
+        return m.ior_coef[0] + inv2 * (m.ior_coef[1] + inv2 * m.ior_coef[2]);
+    }
+    if (SPEC && m.ior_kind == PT_IOR_SELLMEIER) {
+        const float lam = wl.lam.x;
+        const float l2 = (lam * 0.001f) * (lam * 0.001f);
+        const float t0 = m.ior_coef[0] * l2 / (l2 - m.ior_coef[3]), t1 = m.ior_coef[1] * l2 / (l2 - m.ior_coef[4]),
+                    t2 = m.ior_coef[2] * l2 / (l2 - m.ior_coef[5]);
+        return sqrtf(1.0f + t0 + t1 + t2);
+    }
+    return m.ior;
+}
+// A region of the program's dispersion record (Planner.ts:1038-1053 planDispersion).
+PT_DEV bool dispersive_region(const DevScene& S, int region) {
+    if (region < 0) return false;
+    const pt_object& ob = S.objects[region];
+    const pt_material& m = S.materials[ob.material];
+    return region_has_index_row(ob) && m.model == PT_MAT_DIELECTRIC && m.ior_kind != PT_IOR_CONSTANT;
 }
 
 // The placement owning an instance primitive id: the last instance whose first id <= prim.
@@ -252,7 +282,6 @@ PT_DEV void classify_hit(const DevScene& S, Hit& h, bool back, bool need, uint2*
 
 // ---------------------------------------------------------------- spectra
 // Wavelengths of a path (spectral programs): four hero lanes.
-struct Wl { float4 lam; };
 PT_DEV float glsl_mix(float a, float b, float t) { return a * (1.0f - t) + b * t; }
 // spectral_curves (components/color/spectral/lookup.glsl:2-7): linear interpolation of the
 // 5 nm rows, int() truncation of the row coordinate.
@@ -378,8 +407,8 @@ PT_DEV ISample surface_sample(const DevScene& S, const pt_material& m, f3 wo, co
         s.pdf = cos_theta * INV_PI;
         s.flags = LOBE_REFLECTION;
     } else if (is_model<MODEL>(m, PT_MAT_DIELECTRIC)) {  // materials/dielectric/dielectric.glsl:14-52
-        float n_i = ior_of(S, h.region_from);
-        float n_t = ior_of(S, h.region_to);
+        float n_i = ior_of<SPEC>(S, h.region_from, wl);
+        float n_t = ior_of<SPEC>(S, h.region_to, wl);
         float eta = n_i / n_t;
         f3 n = f.n;
         float cos_i = gclamp(dot(wo, n), 1e-6f, 1.0f);

@@ -554,3 +554,50 @@ def emit_scatter(sigma_a=(0.1, 0.1, 0.1), sigma_s=(0.6, 0.6, 0.6)):
         "environment": {"type": "none"},
         "camera": {"type": "pinhole", "position": [0.0, 1.2, 4.2], "target": [0.0, 1.0, 0.0], "fov": 0.9},
     }
+
+
+# ------------------------------------------------------------------ dispersion witnesses
+
+N_BK7 = {"sellmeier": {"B": [1.03961212, 0.231792344, 1.01046945], "C": [0.00600069867, 0.0200179144, 103.560653]}}
+
+
+def dispersion_slab(ior=None):
+    """spectral-dispersion-slab (tests/witnesses/scenes/spectralDispersion.ts:5-22): a glass slab
+    z in [-0.5, 0.5] (Cauchy [1.5, 0.01] by default) in front of an emission-1 screen at z = -2,
+    seen straight on: two normal transmissions, (1 - F(lambda))^2 of normalized D65 ->
+    (0.91574, 0.91323, 0.90889). A second hero collapse would quadruple the answer."""
+    return {
+        "id": "spectral-dispersion-slab",
+        "materials": {"glass": {"model": "dielectric", "ior": ior if ior is not None else {"cauchy": [1.5, 0.01]},
+                                "transmittance": 1.0},
+                      "lamp": {"model": "lambert", "albedo": 0.0, "emission": 1.0}},
+        "objects": [
+            {"type": "mesh", "triangles": _box_tris((-100.0, -100.0, -0.5), (100.0, 100.0, 0.5)), "material": "glass", "closed": True},
+            {"type": "quad", "corner": [-200.0, -200.0, -2.0], "edge1": [400.0, 0.0, 0.0], "edge2": [0.0, 400.0, 0.0], "material": "lamp"},
+        ],
+        "lights": [],
+        "environment": {"type": "none"},
+        "camera": {"type": "pinhole", "position": [0.0, 0.0, 2.0], "target": [0.0, 0.0, -2.0], "fov": 0.3},
+    }
+
+
+def dispersion_prism(ior=None):
+    """spectral-dispersion-prism (spectralDispersion.ts:24-42): a tall closed triangular prism
+    (Cauchy [1.45, 0.02] by default) before a thin emissive stripe (emission 3) and a dark
+    surround wall; the stripe's image through the prism splits by wavelength."""
+    P = np.array([[-1, -1, -0.7], [1, -1, -0.7], [0, -1, 0.9], [-1, 1, -0.7], [1, 1, -0.7], [0, 1, 0.9]], np.float64)
+    I = np.array([[0, 1, 2], [3, 5, 4], [0, 3, 4], [0, 4, 1], [1, 4, 5], [1, 5, 2], [2, 5, 3], [2, 3, 0]])
+    return {
+        "id": "spectral-dispersion-prism",
+        "materials": {"glass": {"model": "dielectric", "ior": ior if ior is not None else {"cauchy": [1.45, 0.02]}},
+                      "lamp": {"model": "lambert", "albedo": 0.0, "emission": 3.0},
+                      "surround": {"model": "lambert", "albedo": 0.1}},
+        "objects": [
+            {"type": "mesh", "triangles": P[I], "material": "glass", "closed": True},
+            {"type": "mesh", "triangles": _box_tris((-0.15, -3.0, -3.03), (0.15, 3.0, -2.97)), "material": "lamp", "closed": True},
+            {"type": "quad", "corner": [-50.0, -50.0, -3.1], "edge1": [100.0, 0.0, 0.0], "edge2": [0.0, 100.0, 0.0], "material": "surround"},
+        ],
+        "lights": [],
+        "environment": {"type": "none"},
+        "camera": {"type": "pinhole", "position": [0.0, 0.3, 5.0], "target": [0.0, 0.0, 0.0], "fov": math.radians(38.0)},
+    }

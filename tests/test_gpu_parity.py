@@ -165,3 +165,22 @@ def test_variance_picture_matches_welford_over_oracle_samples(engine, oracle):
             var = var * (f32(1.0) - w) + ((x - prev) * (x - mean)) * w
     assert var.max() > 0.0
     assert np.array_equal(gv[..., :3], var)
+
+
+def test_dielectric_quad_pane_matches_oracle(engine, oracle):
+    """A glass quad has a row in the ior table (features/intersection.ts:1619): it refracts on
+    its front face, while its back face probes the entering side and sees eta = 1."""
+    sc = scenes.cornell_box()
+    sc["materials"]["pane"] = {"model": "dielectric", "ior": 1.5}
+    sc["objects"].append({"type": "quad", "corner": [-0.6, -0.6, 0.8], "edge1": [1.2, 0.0, 0.0], "edge2": [0.0, 1.2, 0.0],
+                          "material": "pane"})
+    comp = scene_cache.compiler.compile_scene(sc, scene_cache.compiler.RenderStrategy("mis", 10, 3), 40, 40)
+    engine.load_renderer(comp)
+    engine.clear_accumulation()
+    _say("[gpu] render cornell + glass quad pane 40x40x8")
+    engine.render_frames(8)
+    g = engine.read_export()
+    o, _, counts = oracle_lib.OracleScene(oracle, comp.scene).render(comp.program, comp.params(), 8)
+    exact, bad = _compare_film("quad-pane", g, o)
+    assert bad == 0.0
+    assert engine.ray_counts() == (int(counts[0]), int(counts[1]))

@@ -63,3 +63,19 @@ def test_spectral_coloured_furnace_matches_oracle(engine, oracle):
     bad, _ = _compare(engine, oracle, "spectral-furnace", scenes.spectral_furnace((0.2, 0.5, 0.8)),
                       compiler.RenderStrategy("none", 6, -1, color="spectral"), 24, 24, 8)
     assert bad == 0.0
+
+
+@pytest.mark.parametrize("ior", ["cauchy", "bk7"])
+def test_dispersion_slab_matches_oracle(engine, oracle, ior):
+    sc = scenes.dispersion_slab(None if ior == "cauchy" else scenes.N_BK7)
+    bad, _ = _compare(engine, oracle, f"dispersion-slab-{ior}", sc,
+                      compiler.RenderStrategy("none", 2, -1, color="spectral"), 24, 24, 16)
+    assert bad == 0.0
+
+
+@pytest.mark.parametrize("color,ior", [("spectral", "cauchy"), ("spectral", "bk7"), ("rgb", "cauchy")])
+def test_dispersive_prism_matches_oracle(engine, oracle, color, ior):
+    sc = scenes.dispersion_prism(None if ior == "cauchy" else scenes.N_BK7)
+    bad, _ = _compare(engine, oracle, f"prism-{color}-{ior}", sc,
+                      compiler.RenderStrategy("none", 12, 3, color=color), 32, 24, 8)
+    assert bad == 0.0
