@@ -61,7 +61,8 @@ typedef struct pt_object {
     int32_t tri_count;
     int32_t first_instance;  /* instance batches: placements [first_instance, + instance_count) */
     int32_t instance_count;
-    int32_t pad2;
+    int32_t smooth;          /* meshes / batches: barycentric per-vertex shading normals
+                                (components/intersection/mesh/mesh.glsl:84-109; Planner.ts:142) */
     float   p[12];           /* quad: corner, edge1, edge2, normal; sphere: center, radius */
     float   box_min[4];      /* meshes: vertex AABB (containment quick reject); batches: local */
     float   box_max[4];
@@ -199,6 +200,10 @@ typedef struct pt_scene_desc {
     int32_t  n_instances;
     const float*       proto_vertices;
     const pt_instance* instances;
+    /* per-corner shading normals of smooth meshes / prototypes, the same layout as the
+     * vertices ([n][9], object-local for prototypes); may be null when no object is smooth */
+    const float* tri_normals;
+    const float* proto_normals;
 } pt_scene_desc;
 
 /* ---- program (the Plan stage's decisions) ---- */

@@ -258,7 +258,8 @@ static inline int aabb_hit(const float* n, v3 ro, v3 inv, float tmax) {
 
 /* ---------------- scene ---------------- */
 typedef struct {
-    int n_tris; float* tri;   /* [n][9] */
+    int n_tris; float* tri;   /* [n][9]: world triangles, then prototypes */
+    float* nrm;               /* [n][9] per-corner shading normals (smooth objects), or NULL */
     int32_t* tri_object;
     int n_objects; pt_object* objects;
     int n_materials; pt_material* materials;
@@ -311,6 +312,23 @@ static inline int tri_test(v3 ro, v3 rd, v3 a, v3 b, v3 c, float* t_out, v3* g_o
 static inline void tri_load(const or_scene* S, int i, v3* a, v3* b, v3* c) {
     const float* p = S->tri + 9 * (size_t)i;
     *a = V(p[0], p[1], p[2]); *b = V(p[3], p[4], p[5]); *c = V(p[6], p[7], p[8]);
+}
+/* mesh_test_range's smooth shading normal (mesh.glsl:84-109): the hit's barycentrics from
+ * mesh_tri_test, bary.x n_a + bary.y n_b + bary.z n_c, turned to the geometric normal's ray
+ * side. ro, rd: the ray in the mesh's frame. */
+static v3 smooth_normal(const or_scene* S, int i, v3 ro, v3 rd, v3 a, v3 b, v3 c) {
+    v3 e1 = sub(b, a), e2 = sub(c, a);
+    v3 g = cross(e1, e2);
+    float invdet = 1.0f / -dot(rd, g);
+    v3 ao = sub(ro, a);
+    v3 dao = cross(ao, rd);
+    float u = dot(e2, dao) * invdet;
+    float v = -dot(e1, dao) * invdet;
+    float w = 1.0f - u - v;
+    const float* n = S->nrm + 9 * (size_t)i;
+    v3 ns = add(add(muls(V(n[0], n[1], n[2]), w), muls(V(n[3], n[4], n[5]), u)), muls(V(n[6], n[7], n[8]), v));
+    if (dot(rd, ns) * dot(rd, g) < 0.0f) ns = neg(ns);
+    return ns;
 }
 
 /* Nearest over one mesh BLAS (mesh.glsl:162-191) with the (t, id) tie rule. */
@@ -760,7 +778,7 @@ static int scene_intersect(const or_scene* S, v3 ro, v3 rd, hit_t* hit) {
     if (prim < S->n_tris) {
         v3 a, b, c; tri_load(S, prim, &a, &b, &c);
         v3 g = cross(sub(b, a), sub(c, a));
-        hit->frame = ambient_frame(hit->p, normalize(g));
+        hit->frame = ambient_frame(hit->p, normalize(S->objects[S->tri_object[prim]].smooth ? smooth_normal(S, prim, ro, rd, a, b, c) : g));
         hit->ng = normalize(g);
         hit->region_owner = S->tri_object[prim];
     } else if (prim >= S->inst_prim0) {   /* instanceLeafItem mesh arm: normals by the forward rotation */
@@ -769,7 +787,13 @@ static int scene_intersect(const or_scene* S, v3 ro, v3 rd, hit_t* hit) {
         v3 a, b, c; tri_load(S, S->obj_first[o] + (prim - S->inst_prim0 - S->inst_base[i]), &a, &b, &c);
         v3 gl = cross(sub(b, a), sub(c, a));
         v3 ng = normalize(inst_normal(S->inst + 8 * (size_t)i, gl));
-        hit->frame = ambient_frame(hit->p, ng);
+        v3 ns = ng;
+        if (S->objects[o].smooth) {
+            v3 lo, ld; inst_ray(S->inst + 8 * (size_t)i, ro, rd, &lo, &ld);
+            ns = normalize(inst_normal(S->inst + 8 * (size_t)i,
+                                       smooth_normal(S, S->obj_first[o] + (prim - S->inst_prim0 - S->inst_base[i]), lo, ld, a, b, c)));
+        }
+        hit->frame = ambient_frame(hit->p, ns);
         hit->ng = ng;
         hit->region_owner = o;
     } else {
@@ -1745,6 +1769,12 @@ or_scene* or_scene_create(const pt_scene_desc* d) {
     S->tri = (float*)malloc(sizeof(float) * 9 * (ntot > 0 ? ntot : 1));
     memcpy(S->tri, d->tri_vertices, sizeof(float) * 9 * (size_t)d->n_tris);
     if (S->n_proto) memcpy(S->tri + 9 * (size_t)d->n_tris, d->proto_vertices, sizeof(float) * 9 * (size_t)S->n_proto);
+    if (d->tri_normals || d->proto_normals) {
+        S->nrm = (float*)calloc(9 * (ntot > 0 ? ntot : 1), sizeof(float));
+        if (d->tri_normals) memcpy(S->nrm, d->tri_normals, sizeof(float) * 9 * (size_t)d->n_tris);
+        if (d->proto_normals && S->n_proto)
+            memcpy(S->nrm + 9 * (size_t)d->n_tris, d->proto_normals, sizeof(float) * 9 * (size_t)S->n_proto);
+    }
     S->tri_object = (int32_t*)malloc(sizeof(int32_t) * (size_t)(d->n_tris > 0 ? d->n_tris : 1));
     memcpy(S->tri_object, d->tri_object, sizeof(int32_t) * (size_t)d->n_tris);
     S->n_objects = d->n_objects;
@@ -1868,7 +1898,7 @@ void or_scene_destroy(or_scene* S) {
     for (int o = 0; o < S->n_objects; o++) bvh_free(&S->blas[o]);
     free(S->blas); bvh_free(&S->tlas); free(S->tlas_obj);
     free(S->obj_first); bvh_free(&S->itlas); free(S->inst); free(S->inst_batch); free(S->inst_base);
-    free(S->tri); free(S->tri_object); free(S->objects); free(S->materials); free(S->lights);
+    free(S->tri); free(S->nrm); free(S->tri_object); free(S->objects); free(S->materials); free(S->lights);
     free(S->env_cond); free(S->env_marg);
     free(S);
 }

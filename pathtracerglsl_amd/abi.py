@@ -29,7 +29,7 @@ SPECTRUM_DTYPE = np.dtype([
 
 OBJECT_DTYPE = np.dtype([
     ("kind", "<i4"), ("material", "<i4"), ("closed", "<i4"), ("first_tri", "<i4"),
-    ("tri_count", "<i4"), ("first_instance", "<i4"), ("instance_count", "<i4"), ("pad", "<i4"),
+    ("tri_count", "<i4"), ("first_instance", "<i4"), ("instance_count", "<i4"), ("smooth", "<i4"),
     ("p", "<f4", 12), ("box_min", "<f4", 4), ("box_max", "<f4", 4),
 ])
 MATERIAL_DTYPE = np.dtype([
@@ -68,7 +68,8 @@ class PtSceneDesc(C.Structure):
                 ("spectral_curves", C.c_void_p), ("xyz_to_rgb", C.c_float * 9),
                 ("cie_y_integral", C.c_float), ("pad1", C.c_int32 * 2),
                 ("n_proto_tris", C.c_int32), ("n_instances", C.c_int32),
-                ("proto_vertices", C.c_void_p), ("instances", C.c_void_p)]
+                ("proto_vertices", C.c_void_p), ("instances", C.c_void_p),
+                ("tri_normals", C.c_void_p), ("proto_normals", C.c_void_p)]
 
 
 class PtProgram(C.Structure):
@@ -132,7 +133,8 @@ class SceneArrays:
     """Host-side scene in the ABI layout; keeps the numpy arrays alive for the desc."""
 
     def __init__(self, tri_vertices, tri_object, objects, materials, lights, env=None, ambient_material=-1,
-                 spectral_curves=None, xyz_to_rgb=None, cie_y_integral=0.0, proto_vertices=None, instances=None):
+                 spectral_curves=None, xyz_to_rgb=None, cie_y_integral=0.0, proto_vertices=None, instances=None,
+                 tri_normals=None, proto_normals=None):
         self.tri_vertices = np.ascontiguousarray(tri_vertices, dtype=np.float32).reshape(-1, 9)
         self.tri_object = np.ascontiguousarray(tri_object, dtype=np.int32).reshape(-1)
         self.objects = np.ascontiguousarray(objects, dtype=OBJECT_DTYPE)
@@ -148,7 +150,11 @@ class SceneArrays:
                                else np.ascontiguousarray(proto_vertices, dtype=np.float32).reshape(-1, 9))
         self.instances = (np.zeros(0, INSTANCE_DTYPE) if instances is None
                           else np.ascontiguousarray(instances, dtype=INSTANCE_DTYPE))
+        self.tri_normals = None if tri_normals is None else np.ascontiguousarray(tri_normals, np.float32).reshape(-1, 9)
+        self.proto_normals = None if proto_normals is None else np.ascontiguousarray(proto_normals, np.float32).reshape(-1, 9)
         assert self.tri_object.shape[0] == self.tri_vertices.shape[0]
+        assert self.tri_normals is None or self.tri_normals.shape == self.tri_vertices.shape
+        assert self.proto_normals is None or self.proto_normals.shape == self.proto_vertices.shape
 
     @property
     def n_tris(self):
@@ -176,6 +182,8 @@ class SceneArrays:
         d.n_instances = int(self.instances.shape[0])
         d.proto_vertices = _ptr(self.proto_vertices)
         d.instances = _ptr(self.instances)
+        d.tri_normals = _ptr(self.tri_normals)
+        d.proto_normals = _ptr(self.proto_normals)
         return d
 
 

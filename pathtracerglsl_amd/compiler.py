@@ -274,10 +274,20 @@ def compile_scene(scene, strategy=None, width=256, height=256, reset_salt=1234):
             rec["s_medium_emission"] = srec(eps, spectral.ROLE_EMISSION)
         materials.append(rec)
 
-    tri_chunks, tri_obj_chunks, objects = [], [], []
+    tri_chunks, tri_obj_chunks, nrm_chunks, objects = [], [], [], []
     n_tris = 0
 
-    def add_object(kind, material, closed=0, tris=None, p=None):
+    def corner_normals(o, n):
+        """Per-corner shading normals of a smooth mesh ([n][3][3], like the triangles), or None
+        (flat: the geometric normal, MeshObject.normals absent, compiler/types.ts:140-141)."""
+        if o.get("normals") is None:
+            return None
+        nr = np.asarray(o["normals"], np.float64).reshape(-1, 9).astype(np.float32)
+        if nr.shape[0] != n:
+            raise CompileError("mesh normals must give one normal per triangle corner")
+        return nr
+
+    def add_object(kind, material, closed=0, tris=None, p=None, normals=None):
         nonlocal n_tris
         rec = np.zeros(1, abi.OBJECT_DTYPE)[0]
         rec["kind"] = kind
@@ -292,13 +302,15 @@ def compile_scene(scene, strategy=None, width=256, height=256, reset_salt=1234):
             rec["box_max"][:3] = v.max(axis=0)
             tri_chunks.append(t32)
             tri_obj_chunks.append(np.full(t32.shape[0], len(objects), np.int32))
+            nrm_chunks.append(normals if normals is not None else np.zeros_like(t32))
+            rec["smooth"] = int(normals is not None)
             n_tris += t32.shape[0]
         if p is not None:
             rec["p"][:len(p)] = p
         objects.append(rec)
         return len(objects) - 1
 
-    proto_chunks, inst_recs = [], []
+    proto_chunks, proto_nrm_chunks, inst_recs = [], [], []
     n_proto = 0
 
     def add_batch(o, material):
@@ -320,6 +332,9 @@ def compile_scene(scene, strategy=None, width=256, height=256, reset_salt=1234):
         rec["first_instance"] = len(inst_recs)
         rec["instance_count"] = len(o["placements"])
         proto_chunks.append(t32)
+        pn = corner_normals(o, t32.shape[0])
+        proto_nrm_chunks.append(pn if pn is not None else np.zeros_like(t32))
+        rec["smooth"] = int(pn is not None)
         n_proto += t32.shape[0]
         for pl in o["placements"]:
             q = np.asarray(pl.get("rotation", [0.0, 0.0, 0.0, 1.0]), np.float64)
@@ -344,7 +359,8 @@ def compile_scene(scene, strategy=None, width=256, height=256, reset_salt=1234):
             add_batch(o, mi)
             continue
         if o["type"] == "mesh":
-            add_object(abi.OBJ_MESH, mi, o.get("closed", False), tris=o["triangles"])
+            tris = np.asarray(o["triangles"], np.float64).reshape(-1, 9)
+            add_object(abi.OBJ_MESH, mi, o.get("closed", False), tris=tris, normals=corner_normals(o, tris.shape[0]))
         elif o["type"] == "sphere":
             add_object(abi.OBJ_SPHERE, mi, 1, p=list(o["center"]) + [o["radius"]])
         elif o["type"] == "quad":
@@ -415,7 +431,10 @@ def compile_scene(scene, strategy=None, width=256, height=256, reset_salt=1234):
                              mat_index[amb] if amb is not None else -1,
                              spectral.curve_rows(), spectral.XYZ_TO_RGB.reshape(9), spectral.CIE_Y_INTEGRAL,
                              np.concatenate(proto_chunks) if proto_chunks else None,
-                             np.array(inst_recs, abi.INSTANCE_DTYPE) if inst_recs else None)
+                             np.array(inst_recs, abi.INSTANCE_DTYPE) if inst_recs else None,
+                             np.concatenate(nrm_chunks) if any(r["smooth"] and r["kind"] == abi.OBJ_MESH for r in objects) else None,
+                             np.concatenate(proto_nrm_chunks) if any(r["smooth"] and r["kind"] == abi.OBJ_INSTANCES
+                                                                     for r in objects) else None)
 
     prog = abi.PtProgram()
     prog.abi_version = abi.ABI_VERSION

@@ -108,6 +108,7 @@ struct DevScene {
     const float4* obj_nodes;    // binary nodes, 2 float4 each: (min, A) (max, B)
     const int* obj_node_base;   // per object
     const int* obj_order;       // global triangle ids in per-object leaf order, at first_tri
+    const float* tri_nrm;       // [shade triangle][9] per-corner normals of smooth objects (null: none)
     const uint4* ow_nodes;      // per closed mesh wide BVH (same format as cw_nodes), concatenated
     const float4* ow_tris;      // their leaf triangles (ids are global triangle ids)
     const int* ow_root;         // per object: root node in ow_nodes, -1 if none

@@ -1218,6 +1218,10 @@ static pt_status validate_scene(const pt_scene_desc* sd) {
     for (int o = 0; o < sd->n_objects; o++) {
         const pt_object& ob = sd->objects[o];
         if (ob.kind < PT_OBJ_MESH || ob.kind > PT_OBJ_INSTANCES) return fail(PT_E_INVALID, "object " + std::to_string(o) + ": unknown kind");
+        if (ob.smooth && ((ob.kind == PT_OBJ_MESH && !sd->tri_normals) || (ob.kind == PT_OBJ_INSTANCES && !sd->proto_normals)))
+            return fail(PT_E_MISSING_INPUT, "missing channel: normals (object " + std::to_string(o) + " is smooth)");
+        if (ob.smooth && ob.kind != PT_OBJ_MESH && ob.kind != PT_OBJ_INSTANCES)
+            return fail(PT_E_INVALID, "object " + std::to_string(o) + ": only meshes shade smooth");
         if (ob.material < 0 || ob.material >= sd->n_materials) return fail(PT_E_INVALID, "object " + std::to_string(o) + ": bad material");
         if (ob.kind == PT_OBJ_MESH && (ob.first_tri < 0 || ob.tri_count < 0 || ob.first_tri + ob.tri_count > sd->n_tris))
             return fail(PT_E_INVALID, "object " + std::to_string(o) + ": triangle range out of bounds");
@@ -1253,6 +1257,13 @@ static pt_status build_scene(const pt_scene_desc* sd, hipStream_t stream, SceneS
         }
     }
     if ((st = SS.upload((const float4*)shade.data(), n_shade * 3, &D.tri_shade)) != PT_OK) return st;
+    if (sd->tri_normals || sd->proto_normals) {
+        std::vector<float> nrm(n_shade * 9, 0.0f);
+        if (sd->tri_normals) std::copy(sd->tri_normals, sd->tri_normals + 9 * (size_t)sd->n_tris, nrm.begin());
+        if (sd->proto_normals && n_proto)
+            std::copy(sd->proto_normals, sd->proto_normals + 9 * (size_t)n_proto, nrm.begin() + 9 * (size_t)sd->n_tris);
+        if ((st = SS.upload(nrm.data(), nrm.size(), &D.tri_nrm)) != PT_OK) return st;
+    }
     if ((st = SS.upload(sd->tri_object, (size_t)sd->n_tris, &D.tri_object)) != PT_OK) return st;
     D.n_tris = sd->n_tris;
     if ((st = SS.upload(sd->objects, (size_t)sd->n_objects, &D.objects)) != PT_OK) return st;

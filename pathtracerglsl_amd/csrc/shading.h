@@ -226,6 +226,23 @@ PT_DEV int prim_owner(const DevScene& S, int prim) {
 // dispatcher's orientation step (features/intersection.ts:1690-1706). The classification
 // probes run only when something reads region_from / region_to: a transmissive material
 // (ior_of) or an emissive solid region (emission keys on region_to).
+// mesh_test_range's smooth shading normal (mesh.glsl:84-109): the hit's barycentrics from
+// mesh_tri_test, bary.x n_a + bary.y n_b + bary.z n_c, turned to the geometric normal's ray
+// side; (o, d) is the ray in the mesh's frame, tri the shade-triangle index.
+PT_DEV f3 smooth_normal(const DevScene& S, int tri, f3 o, f3 d, f3 a, f3 b, f3 c) {
+    const f3 e1 = b - a, e2 = c - a;
+    const f3 g = cross(e1, e2);
+    const float invdet = 1.0f / -dot(d, g);
+    const f3 ao = o - a;
+    const f3 dao = cross(ao, d);
+    const float u = dot(e2, dao) * invdet;
+    const float v = -dot(e1, dao) * invdet;
+    const float w = 1.0f - u - v;
+    const float* n = S.tri_nrm + 9 * (size_t)tri;
+    f3 ns = (mk3(n[0], n[1], n[2]) * w + mk3(n[3], n[4], n[5]) * u) + mk3(n[6], n[7], n[8]) * v;
+    if (dot(d, ns) * dot(d, g) < 0.0f) ns = -ns;
+    return ns;
+}
 PT_DEV void fill_hit(const DevScene& S, f3 o, f3 d, float t, int prim, Hit& h, bool& back) {
     h.t = t;
     h.p = o + d * t;
@@ -234,20 +251,29 @@ PT_DEV void fill_hit(const DevScene& S, f3 o, f3 d, float t, int prim, Hit& h, b
         const float4* tp = S.tri_shade + 3 * (size_t)prim;
         f3 a = xyz(tp[0]);
         f3 g = cross(xyz(tp[1]) - a, xyz(tp[2]) - a);
-        h.frame = ambient_frame(normalize(g));
-        h.ng = normalize(g);
         h.owner = S.tri_object[prim];
+        h.frame = ambient_frame(normalize(S.objects[h.owner].smooth ? smooth_normal(S, prim, o, d, a, xyz(tp[1]), xyz(tp[2])) : g));
+        h.ng = normalize(g);
     } else if (prim >= S.inst_prim0 && S.n_instances > 0) {
         // instanceLeafItem mesh arm: the prototype triangle, normals by the forward rotation
         const int rel = prim - S.inst_prim0;
         const int lo = instance_of(S, prim);
         const int batch = S.inst_batch[lo];
-        const float4* tp = S.tri_shade + 3 * (size_t)(S.n_tris + S.objects[batch].first_tri + (rel - S.inst_base[lo]));
+        const int tri = S.n_tris + S.objects[batch].first_tri + (rel - S.inst_base[lo]);
+        const float4* tp = S.tri_shade + 3 * (size_t)tri;
         f3 a = xyz(tp[0]);
         f3 gl = cross(xyz(tp[1]) - a, xyz(tp[2]) - a);
         const float4 q = S.inst_q[lo];
-        const f3 ng = normalize(qrot(make_float4(-q.x, -q.y, -q.z, q.w), gl));
-        h.frame = ambient_frame(ng);
+        const float4 qc = make_float4(-q.x, -q.y, -q.z, q.w);
+        const f3 ng = normalize(qrot(qc, gl));
+        f3 ns = ng;
+        if (S.objects[batch].smooth) {   // the placement's local ray, as the prototype walk saw it
+            const float4 ts = S.inst_ts[lo];
+            const f3 r = qrot(q, o) + xyz(ts), rd = qrot(q, d);
+            const f3 lo3 = mk3(r.x / ts.w, r.y / ts.w, r.z / ts.w), ld = mk3(rd.x / ts.w, rd.y / ts.w, rd.z / ts.w);
+            ns = normalize(qrot(qc, smooth_normal(S, tri, lo3, ld, a, xyz(tp[1]), xyz(tp[2]))));
+        }
+        h.frame = ambient_frame(ns);
         h.ng = ng;
         h.owner = batch;
     } else {

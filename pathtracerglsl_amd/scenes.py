@@ -601,3 +601,46 @@ def dispersion_prism(ior=None):
         "environment": {"type": "none"},
         "camera": {"type": "pinhole", "position": [0.0, 0.3, 5.0], "target": [0.0, 0.0, 0.0], "fov": math.radians(38.0)},
     }
+
+
+def smooth_sphere(mode="smooth", detail=3):
+    """A Lambert unit sphere lit by one point light, seen from the front: mode 'smooth' is the
+    icosphere mesh with per-corner normals = the unit vertex directions (MeshObject.normals,
+    mesh.glsl:84-109), 'flat' the same mesh with geometric normals, 'analytic' the exact sphere.
+    Smooth shading must approach the analytic image far closer than flat facets do."""
+    V, F = unit_icosphere(detail)
+    obj = ({"type": "sphere", "center": [0.0, 0.0, 0.0], "radius": 1.0, "material": "m"} if mode == "analytic" else
+           {"type": "mesh", "triangles": V[F], "material": "m", "closed": True})
+    if mode == "smooth":
+        obj["normals"] = (V / np.linalg.norm(V, axis=1, keepdims=True))[F]
+    return {
+        "id": f"sphere-{mode}",
+        "materials": {"m": {"model": "lambert", "albedo": 0.8}},
+        "objects": [obj],
+        "lights": [{"type": "point", "position": [2.0, 3.0, 4.0], "intensity": 30.0}],
+        "environment": {"type": "none"},
+        "camera": {"type": "pinhole", "position": [0.0, 0.0, 4.0], "target": [0.0, 0.0, 0.0], "fov": 0.6},
+    }
+
+
+def cornell_smooth_instances(n=4, seed=11, detail=2):
+    """Cornell box with a smooth-shaded GGX instance batch (icosphere prototype with per-corner
+    normals) at seeded rotations, scales and positions, plus one smooth glass icosphere mesh."""
+    sc = cornell_box()
+    rng = np.random.default_rng(seed)
+    V, F = unit_icosphere(detail)
+    N = (V / np.linalg.norm(V, axis=1, keepdims=True))[F]
+    pls = []
+    for _ in range(n):
+        q = rng.normal(size=4)
+        pls.append({"rotation": (q / np.linalg.norm(q)).tolist(),
+                    "translation": [float(rng.uniform(-0.6, 0.6)), float(rng.uniform(-0.3, 0.6)), float(rng.uniform(-0.5, 0.5))],
+                    "scale": float(rng.uniform(0.12, 0.25))})
+    sc["materials"]["metal"] = {"model": "ggx", "f0": [0.9, 0.8, 0.6], "roughness": 0.25}
+    sc["materials"]["glass"] = {"model": "dielectric", "ior": 1.5}
+    sc["objects"].append({"type": "instances", "triangles": V[F], "normals": N, "material": "metal", "closed": True,
+                          "placements": pls})
+    sc["objects"].append({"type": "mesh", "triangles": V[F] * 0.3 + np.array([0.35, -0.7, 0.4]), "normals": N,
+                          "material": "glass", "closed": True})
+    sc["id"] = "cornell-smooth-instances"
+    return sc

@@ -184,3 +184,31 @@ def test_dielectric_quad_pane_matches_oracle(engine, oracle):
     exact, bad = _compare_film("quad-pane", g, o)
     assert bad == 0.0
     assert engine.ray_counts() == (int(counts[0]), int(counts[1]))
+
+
+@pytest.mark.parametrize("mode", ["smooth", "analytic"])
+def test_smooth_sphere_matches_oracle(engine, oracle, mode):
+    comp = scene_cache.compiler.compile_scene(scenes.smooth_sphere(mode), scene_cache.compiler.RenderStrategy("mis", 4, 3), 32, 32)
+    engine.load_renderer(comp)
+    engine.clear_accumulation()
+    engine.render_frames(8)
+    g = engine.read_export()
+    o, _, counts = oracle_lib.OracleScene(oracle, comp.scene).render(comp.program, comp.params(), 8)
+    exact, bad = _compare_film(f"sphere-{mode}", g, o)
+    assert bad == 0.0
+
+
+@pytest.mark.parametrize("est", ["mis", "none"])
+def test_smooth_instances_and_glass_mesh_match_oracle(engine, oracle, est):
+    comp = scene_cache.compiler.compile_scene(scenes.cornell_smooth_instances(), scene_cache.compiler.RenderStrategy(est, 12, 3),
+                                              40, 40)
+    engine.load_renderer(comp)
+    engine.clear_accumulation()
+    _say(f"[gpu] render cornell smooth instances {est} 40x40x8")
+    engine.render_frames(8)
+    g = engine.read_export()
+    o, _, counts = oracle_lib.OracleScene(oracle, comp.scene).render(comp.program, comp.params(), 8)
+    exact, bad = _compare_film(f"smooth-instances-{est}", g, o)
+    assert o[..., :3].mean() > 0.01
+    assert bad == 0.0
+    assert engine.ray_counts() == (int(counts[0]), int(counts[1]))
