@@ -93,6 +93,7 @@ struct DevScene {
     const int* tri_object;
     int n_tris;
     int n_cw_nodes;
+    int cw_depth;               // levels of the wide BVH (bounds the node groups a walk stacks)
     const pt_object* objects;
     int n_objects;
     const int* analytic;        // object ids of quads / spheres

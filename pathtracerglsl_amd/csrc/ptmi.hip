@@ -309,11 +309,21 @@ struct LaneFeeder {
 #ifndef PTMI_SHADOW_PERSISTENT
 #define PTMI_SHADOW_PERSISTENT 0
 #endif
+#ifndef PTMI_POSTPONE_EXT
+#define PTMI_POSTPONE_EXT 0
+#endif
+#ifndef PTMI_POSTPONE_SHD
+#define PTMI_POSTPONE_SHD 0
+#endif
+constexpr int POSTPONE_EXT = PTMI_POSTPONE_EXT;    // triangle postponing below this many lanes (0: off)
+constexpr int POSTPONE_SHD = PTMI_POSTPONE_SHD;    // the same for opaque shadow rays
 constexpr int FEED_CHUNK = PTMI_FEED_CHUNK;        // queue entries a wave claims per atomic
 constexpr int FEED_MIN_IDLE = PTMI_FEED_MIN_IDLE;  // idle lanes before a wave refills
 
 // extend: closest hit for every ray of the current queue, then bin the path by shading category.
-template <int STACK, bool COUNT, bool BATCH = false, int TB = TBLOCK>
+// INST: the scene has instance batches. The TLAS walk keeps a private stack, so only scenes
+// with placements compile it in (the plain kernels then need no scratch memory).
+template <int STACK, bool COUNT, bool BATCH = false, int TB = TBLOCK, bool INST = false>
 __global__ void __launch_bounds__(TB) k_extend(DevScene Sk, Paths P, int qsel) {
     __shared__ uint2 stk[STACK * TB];
     const uint32_t n = (uint32_t)P.counts[qsel];
@@ -337,10 +347,11 @@ __global__ void __launch_bounds__(TB) k_extend(DevScene Sk, Paths P, int qsel) {
             if (feed.exhausted) break;
             continue;
         }
-        if (active && (Sp->n_cw_nodes == 0 || trace_step<false, TB, COUNT>(Sp->cw_nodes, Sp->cw_tris, st, stk, cnt))) {
+        if (active && (Sp->n_cw_nodes == 0 ||
+                       trace_step<false, TB, COUNT, POSTPONE_EXT, STACK>(Sp->cw_nodes, Sp->cw_tris, st, stk, cnt, Sp->cw_depth))) {
             float tmax = st.tmax; int prim = st.prim;
             trace_analytic<false>(*Sp, st.o, st.d, tmax, prim);
-            trace_instances<false, TB, COUNT>(*Sp, st.o, st.d, tmax, prim, stk, cnt);
+            if (INST) trace_instances<false, TB, COUNT>(*Sp, st.o, st.d, tmax, prim, stk, cnt);
             P.hit[path] = make_float2(tmax, __int_as_float(prim));
             active = false;
         }
@@ -353,7 +364,7 @@ __global__ void __launch_bounds__(TB) k_extend(DevScene Sk, Paths P, int qsel) {
 
 // shadow (opaque scenes): any hit for every NEE ray; unoccluded contributions are added to
 // their path as ((a * vis) * w) / pdf, the reference's operation order with vis = 1.
-template <int STACK, bool SPEC, bool ADJ, bool BATCH = false, int TB = TBLOCK>
+template <int STACK, bool SPEC, bool ADJ, bool BATCH = false, int TB = TBLOCK, bool INST = false>
 __global__ void __launch_bounds__(TB) k_shadow(DevScene Sk, Paths P) {
     __shared__ uint2 stk[STACK * TB];
     const int n = P.counts[C_SHADOW];
@@ -366,8 +377,8 @@ __global__ void __launch_bounds__(TB) k_shadow(DevScene Sk, Paths P) {
         f3 o = xyz(o4), d = xyz(d4);
         float maxd = o4.w; int prim = -1;
         bool blocked = trace_analytic<true>(S, o, d, maxd, prim);
-        if (!blocked) blocked = trace_wide<true, STACK, TB, false>(S, o, d, maxd, prim, stk, cnt);
-        if (!blocked) blocked = trace_instances<true, TB, false>(S, o, d, maxd, prim, stk, cnt);
+        if (!blocked) blocked = trace_wide<true, STACK, TB, false, POSTPONE_SHD>(S, o, d, maxd, prim, stk, cnt);
+        if (INST && !blocked) blocked = trace_instances<true, TB, false>(S, o, d, maxd, prim, stk, cnt);
         if (!blocked) {
             int i = P.s_path[k];
             float4 c = P.s_c[k];
@@ -388,7 +399,7 @@ __global__ void __launch_bounds__(TB) k_shadow(DevScene Sk, Paths P) {
 // shadow (media scenes): the segment walker of transport/shadow/media/media.glsl:29-62 —
 // closest hits up to the light point, Beer-Lambert per segment in the current medium, pass
 // through null interfaces, anything else blocks (dielectrics are opaque to shadow rays).
-template <int STACK, bool SPEC, bool ADJ>
+template <int STACK, bool SPEC, bool ADJ, bool INST = false>
 __global__ void __launch_bounds__(TBLOCK) k_shadow_media(DevScene S, Paths P) {
     __shared__ uint2 stk[STACK * TBLOCK];
     const int n = P.counts[C_SHADOW];
@@ -410,7 +421,7 @@ __global__ void __launch_bounds__(TBLOCK) k_shadow_media(DevScene S, Paths P) {
             float tmax = MAX_DIST; int prim = -1;
             if (S.n_cw_nodes > 0) trace_wide<false, STACK, TBLOCK, false>(S, o, d, tmax, prim, stk, cnt);
             trace_analytic<false>(S, o, d, tmax, prim);
-            trace_instances<false, TBLOCK, false>(S, o, d, tmax, prim, stk, cnt);
+            if (INST) trace_instances<false, TBLOCK, false>(S, o, d, tmax, prim, stk, cnt);
             bool hit_boundary = prim >= 0 && tmax < remaining;
             float seg_len = hit_boundary ? tmax : remaining;
             int med_mat = material_of(S, medium);
@@ -1246,6 +1257,7 @@ static pt_status build_scene(const pt_scene_desc* sd, hipStream_t stream, SceneS
     if ((st = SS.upload((const uint4*)wb.nodes.data(), wb.nodes.size() / 4, &D.cw_nodes)) != PT_OK) return st;
     if ((st = SS.upload((const float4*)wb.tris.data(), wb.tris.size() / 4, &D.cw_tris)) != PT_OK) return st;
     D.n_cw_nodes = (int)wb.n_nodes;
+    D.cw_depth = (int)wb.max_depth;
     const int n_proto = sd->proto_vertices ? sd->n_proto_tris : 0;
     const size_t n_shade = (size_t)sd->n_tris + (size_t)n_proto;
     std::vector<float> shade(n_shade * 12);
@@ -1621,14 +1633,19 @@ static void launch_wave(pt_renderer* r, const Camera& cam, int frames, uint32_t 
     const int iterations = pg.max_bounces + 1 + (S.has_null ? std::max(pg.max_null_crossings, 0) : 0);
     for (int it = 0; it < iterations; it++) {
         hipLaunchKernelGGL(k_bookkeep, dim3(1), dim3(64), 0, st, P, cur, cur ^ 1, it == 0 ? 1 : 0);
-        if (r->count_traversal)
-            TIMED(ctx, "extend_count", 0, hipLaunchKernelGGL((k_extend<STACK, true>), dim3(tgrid), dim3(TBLOCK), 0, st, S, P, cur));
-        else
-            TIMED(ctx, "extend", 0, hipLaunchKernelGGL((k_extend<STACK, false>), dim3(tgrid), dim3(TBLOCK), 0, st, S, P, cur));
+        const bool inst = S.n_inst_nodes > 0;
+        if (r->count_traversal) {
+            if (inst) TIMED(ctx, "extend_count", 0, hipLaunchKernelGGL((k_extend<STACK, true, false, TBLOCK, true>), dim3(tgrid), dim3(TBLOCK), 0, st, S, P, cur));
+            else TIMED(ctx, "extend_count", 0, hipLaunchKernelGGL((k_extend<STACK, true>), dim3(tgrid), dim3(TBLOCK), 0, st, S, P, cur));
+        } else {
+            if (inst) TIMED(ctx, "extend", 0, hipLaunchKernelGGL((k_extend<STACK, false, false, TBLOCK, true>), dim3(tgrid), dim3(TBLOCK), 0, st, S, P, cur));
+            else TIMED(ctx, "extend", 0, hipLaunchKernelGGL((k_extend<STACK, false>), dim3(tgrid), dim3(TBLOCK), 0, st, S, P, cur));
+        }
         const int eq = MEDIA && !ADJ && pg.medium_light_sampling == PT_MLS_EQUIANGULAR ? 1 : 0;
         TIMED(ctx, "bin", 0, hipLaunchKernelGGL((k_bin<MEDIA, SPEC, ADJ>), dim3(ctx->shade_blocks), dim3(SBLOCK), 0, st, S, P, cur, eq, pg.max_bounces, pg.max_null_crossings));
         if (eq) {
-            TIMED(ctx, "shadow_media", 0, hipLaunchKernelGGL((k_shadow_media<STACK, SPEC, ADJ>), dim3(tgrid), dim3(TBLOCK), 0, st, S, P));
+            if (inst) TIMED(ctx, "shadow_media", 0, hipLaunchKernelGGL((k_shadow_media<STACK, SPEC, ADJ, true>), dim3(tgrid), dim3(TBLOCK), 0, st, S, P));
+            else TIMED(ctx, "shadow_media", 0, hipLaunchKernelGGL((k_shadow_media<STACK, SPEC, ADJ>), dim3(tgrid), dim3(TBLOCK), 0, st, S, P));
             hipLaunchKernelGGL(k_shadow_reset, dim3(1), dim3(64), 0, st, P);
         }
         switch (pg.estimator) {
@@ -1637,7 +1654,9 @@ static void launch_wave(pt_renderer* r, const Camera& cam, int frames, uint32_t 
             default: launch_shade<PT_EST_MIS, STACK, MEDIA, SPEC, ADJ>(r, cur);
         }
         if (pg.estimator != PT_EST_PT) {
-            if (MEDIA) TIMED(ctx, "shadow_media", 0, hipLaunchKernelGGL((k_shadow_media<STACK, SPEC, ADJ>), dim3(tgrid), dim3(TBLOCK), 0, st, S, P));
+            if (MEDIA && inst) TIMED(ctx, "shadow_media", 0, hipLaunchKernelGGL((k_shadow_media<STACK, SPEC, ADJ, true>), dim3(tgrid), dim3(TBLOCK), 0, st, S, P));
+            else if (MEDIA) TIMED(ctx, "shadow_media", 0, hipLaunchKernelGGL((k_shadow_media<STACK, SPEC, ADJ>), dim3(tgrid), dim3(TBLOCK), 0, st, S, P));
+            else if (inst) TIMED(ctx, "shadow", 0, hipLaunchKernelGGL((k_shadow<STACK, SPEC, ADJ, false, TBLOCK, true>), dim3(tgrid), dim3(TBLOCK), 0, st, S, P));
             else TIMED(ctx, "shadow", 0, hipLaunchKernelGGL((k_shadow<STACK, SPEC, ADJ>), dim3(tgrid), dim3(TBLOCK), 0, st, S, P));
         }
         cur ^= 1;
@@ -1974,6 +1993,8 @@ pt_status pt_batch_create(pt_ctx* ctx, int32_t n, const pt_scene_desc* scenes, c
         for (int m = 0; m < scenes[s].n_materials; m++)
             if (scenes[s].materials[m].has_medium || scenes[s].materials[m].model == PT_MAT_NULL)
                 return fail(PT_E_UNSUPPORTED, "pt_batch_create: batches run scenes without participating media");
+        if (scenes[s].instances && scenes[s].n_instances > 0)
+            return fail(PT_E_UNSUPPORTED, "pt_batch_create: batches run scenes without instance batches");
     }
     HIP_TRY(hipSetDevice(ctx->device));
     auto* b = new pt_batch();

@@ -80,12 +80,20 @@ struct TraceCounters { uint32_t nodes, tris; };
 
 PT_DEV float byte_of(uint32_t w, int k) { return (float)((w >> (8 * k)) & 0xFFu); }
 
+#ifndef PTMI_OCTANT_SLAB
+#define PTMI_OCTANT_SLAB 1
+#endif
+#ifndef PTMI_BRANCHLESS_CHILDREN
+#define PTMI_BRANCHLESS_CHILDREN 1
+#endif
+
 // Per-lane traversal state of the wide-BVH walk, so a persistent kernel can interleave
 // rays: a lane that finishes picks up the next ray while its wave-mates keep walking.
 struct TravState {
     f3 o, d;
     float ix, iy, iz, tmax;
     uint32_t oct_inv, g_base, g_bits;
+    uint32_t t_base, t_bits;    // pending leaf-triangle group (24-bit mask relative to t_base)
     int prim, sp;
 };
 
@@ -100,17 +108,28 @@ PT_DEV void trace_init(TravState& st, f3 o, f3 d, float tmax, uint32_t root = 0)
     st.oct_inv = oct ^ 7u;
     st.g_base = root;
     st.g_bits = (1u << 8) | (1u << (0u ^ st.oct_inv));   // root = slot 0 of a virtual parent
+    st.t_base = 0; st.t_bits = 0;
 }
+
+// Stack entries with this bit set are postponed triangle groups, not node groups.
+constexpr uint32_t TRI_GROUP = 0x80000000u;
 
 // One step: visit the nearest pending node (if any) and test the triangles it exposes.
 // Returns true when the ray is finished: the stack is empty, or (ANY) a hit with t < tmax
 // was found, in which case st.prim is set to its id.
-template <bool ANY, int BLOCK, bool COUNT>
+//
+// POSTPONE > 0 (with the lane stack's capacity CAP and the tree's depth): triangle
+// postponing (Ylitie et al. 2017, sec. 4): when fewer than POSTPONE lanes of the wave are
+// testing triangles and this lane still has nodes of its group to visit, the triangle group
+// goes onto the stack and the lane keeps traversing with the others. The closest hit is the
+// (t, id) minimum over every triangle whose box the ray enters, so testing order does not
+// change it; a group is postponed only while the stack keeps room for every node group the
+// remaining levels can push.
+template <bool ANY, int BLOCK, bool COUNT, int POSTPONE = 0, int CAP = 0>
 PT_DEV bool trace_step(const uint4* __restrict__ nodes, const float4* __restrict__ tris, TravState& st, uint2* stk,
-                       TraceCounters& cnt) {
+                       TraceCounters& cnt, int depth = 0) {
     const int lane_slot = threadIdx.x;
-    uint32_t tri_base = 0, tri_hits = 0;
-    if ((st.g_bits & 0xFFu) != 0u) {
+    if (st.t_bits == 0u && (st.g_bits & 0xFFu) != 0u) {
         uint32_t k = __builtin_ctz(st.g_bits & 0xFFu);
         st.g_bits &= ~(1u << k);
         uint32_t slot = k ^ st.oct_inv;
@@ -129,12 +148,31 @@ PT_DEV bool trace_step(const uint4* __restrict__ nodes, const float4* __restrict
         const float sz = __uint_as_float(((w0.w >> 16) & 0xFFu) << 23);
         const uint32_t nimask = w0.w >> 24;
         const f3 o = st.o;
-        uint32_t nhits = 0;
+        uint32_t nhits = 0, tri_hits = 0;
+#if PTMI_OCTANT_SLAB
+        // Near / far planes chosen once per node by the ray's octant. Decoding and the slab
+        // distance (x - o) * inv are monotone in the quantized plane (lo <= hi), so for inv > 0
+        // min(t_lo, t_hi) is t_lo exactly (t_hi for inv < 0): the same tn and tf, bit for bit,
+        // as the per-axis min / max of the reference's slab test, with 48 fewer VALU ops.
+        const bool nxs = st.ix < 0.0f, nys = st.iy < 0.0f, nzs = st.iz < 0.0f;
+        const uint32_t nx0 = nxs ? w3.z : w2.x, nx1 = nxs ? w3.w : w2.y, fx0 = nxs ? w2.x : w3.z, fx1 = nxs ? w2.y : w3.w;
+        const uint32_t ny0 = nys ? w4.x : w2.z, ny1 = nys ? w4.y : w2.w, fy0 = nys ? w2.z : w4.x, fy1 = nys ? w2.w : w4.y;
+        const uint32_t nz0 = nzs ? w4.z : w3.x, nz1 = nzs ? w4.w : w3.y, fz0 = nzs ? w3.x : w4.z, fz1 = nzs ? w3.y : w4.w;
+#endif
 #pragma unroll
         for (int s = 0; s < 8; s++) {
             const uint32_t mw = s < 4 ? w1.z : w1.w;
             const uint32_t meta = (mw >> (8 * (s & 3))) & 0xFFu;
             const int b = s & 3;
+#if PTMI_OCTANT_SLAB
+            // decode: p + q * 2^e. q * 2^e is exact, so one fused multiply-add rounds exactly
+            // like the builder's separate multiply and add.
+            const float nx = __builtin_fmaf(byte_of(s < 4 ? nx0 : nx1, b), sx, px), fx = __builtin_fmaf(byte_of(s < 4 ? fx0 : fx1, b), sx, px);
+            const float ny = __builtin_fmaf(byte_of(s < 4 ? ny0 : ny1, b), sy, py), fy = __builtin_fmaf(byte_of(s < 4 ? fy0 : fy1, b), sy, py);
+            const float nz = __builtin_fmaf(byte_of(s < 4 ? nz0 : nz1, b), sz, pz), fz = __builtin_fmaf(byte_of(s < 4 ? fz0 : fz1, b), sz, pz);
+            const float tn = SMAX(SMAX((nx - o.x) * st.ix, (ny - o.y) * st.iy), (nz - o.z) * st.iz);
+            const float tf = SMIN(SMIN((fx - o.x) * st.ix, (fy - o.y) * st.iy), (fz - o.z) * st.iz) * BVH_TFAR_PAD;
+#else
             const uint32_t lxw = s < 4 ? w2.x : w2.y, lyw = s < 4 ? w2.z : w2.w;
             const uint32_t lzw = s < 4 ? w3.x : w3.y, hxw = s < 4 ? w3.z : w3.w;
             const uint32_t hyw = s < 4 ? w4.x : w4.y, hzw = s < 4 ? w4.z : w4.w;
@@ -150,20 +188,37 @@ PT_DEV bool trace_step(const uint4* __restrict__ nodes, const float4* __restrict
             // and every comparison below sees the same values
             const float tn = SMAX(SMAX(SMIN(t0x, t1x), SMIN(t0y, t1y)), SMIN(t0z, t1z));
             const float tf = SMIN(SMIN(SMAX(t0x, t1x), SMAX(t0y, t1y)), SMAX(t0z, t1z)) * BVH_TFAR_PAD;
+#endif
             const float te = SMAX(tn, 0.0f);
-            if (meta != 0u && tf >= te && te <= st.tmax) {
+            if (PTMI_BRANCHLESS_CHILDREN && !ANY) {
+                // closest hit: branch-free classification (selects, no exec-mask region per
+                // child; measured faster). Any-hit walks keep the branches, which measured
+                // faster there. An empty slot (meta 0) or a missed box contributes nothing.
+                const uint32_t ok = (uint32_t)(meta != 0u) & (uint32_t)(tf >= te) & (uint32_t)(te <= st.tmax);
+                const uint32_t inner = (nimask >> s) & 1u;
+                nhits |= (ok & inner) << (s ^ st.oct_inv);
+                tri_hits |= (ok & (inner ^ 1u)) ? ((meta >> 5) & 7u) << (meta & 31u) : 0u;
+            } else if (meta != 0u && tf >= te && te <= st.tmax) {
                 if (nimask & (1u << s)) nhits |= 1u << (s ^ st.oct_inv);
                 else tri_hits |= ((meta >> 5) & 7u) << (meta & 31u);
             }
         }
         st.g_base = w1.x;
         st.g_bits = (nimask << 8) | nhits;
-        tri_base = w1.y;
+        st.t_base = w1.y;
+        st.t_bits = tri_hits;
     }
-    while (tri_hits != 0u) {
-        uint32_t k = __builtin_ctz(tri_hits);
-        tri_hits &= tri_hits - 1u;
-        const float4* tp = tris + 3 * (size_t)(tri_base + k);
+    while (st.t_bits != 0u) {
+        if (POSTPONE > 0 && (st.g_bits & 0xFFu) != 0u && st.sp + depth + 1 < CAP &&
+            (int)__popcll(__ballot(1)) < POSTPONE) {
+            stk[st.sp * BLOCK + lane_slot] = make_uint2(st.t_base, st.t_bits | TRI_GROUP);
+            st.sp++;
+            st.t_bits = 0u;
+            break;
+        }
+        uint32_t k = __builtin_ctz(st.t_bits);
+        st.t_bits &= st.t_bits - 1u;
+        const float4* tp = tris + 3 * (size_t)(st.t_base + k);
         float4 ta = tp[0], te1 = tp[1], te2 = tp[2];
         if (COUNT) cnt.tris++;
         float t;
@@ -176,23 +231,24 @@ PT_DEV bool trace_step(const uint4* __restrict__ nodes, const float4* __restrict
             }
         }
     }
-    if ((st.g_bits & 0xFFu) == 0u) {
+    if ((st.g_bits & 0xFFu) == 0u && st.t_bits == 0u) {
         if (st.sp == 0) return true;
         st.sp--;
         uint2 e = stk[st.sp * BLOCK + lane_slot];
-        st.g_base = e.x; st.g_bits = e.y;
+        if (POSTPONE > 0 && (e.y & TRI_GROUP)) { st.t_base = e.x; st.t_bits = e.y & ~TRI_GROUP; }
+        else { st.g_base = e.x; st.g_bits = e.y; }
     }
     return false;
 }
 
 // Whole-ray wide-BVH walk (one ray per lane, no interleaving).
-template <bool ANY, int STACK, int BLOCK, bool COUNT>
+template <bool ANY, int STACK, int BLOCK, bool COUNT, int POSTPONE = 0>
 PT_DEV bool trace_wide(const DevScene& S, f3 o, f3 d, float& tmax, int& prim, uint2* stk, TraceCounters& cnt) {
     if (S.n_cw_nodes == 0) return false;
     TravState st;
     trace_init(st, o, d, tmax);
     st.prim = prim;
-    while (!trace_step<ANY, BLOCK, COUNT>(S.cw_nodes, S.cw_tris, st, stk, cnt)) {}
+    while (!trace_step<ANY, BLOCK, COUNT, POSTPONE, STACK>(S.cw_nodes, S.cw_tris, st, stk, cnt, S.cw_depth)) {}
     (void)STACK;
     if (ANY) return st.prim >= 0 && st.prim != prim;
     bool found = st.prim != prim;
