@@ -280,6 +280,17 @@ pt_status pt_scene_upload(pt_ctx* ctx, const pt_scene_desc* scene);
 
 /* Engine.loadRenderer (Engine.ts:153-164): select the kernel instantiation for program. */
 pt_status pt_renderer_load(pt_ctx* ctx, const pt_program* program, pt_renderer** out);
+
+/* The same from the reference's own plan record: `program_json` is a ProgramDescription
+ * (src/compiler/plan/types.ts:39-249) as JSON.stringify writes it; width, height and
+ * frames_in_flight are the Engine state it does not carry (Engine.setImageSize). Fields this
+ * backend implements are lowered to a pt_program; a field selecting a component it does not
+ * implement fails with PT_E_UNSUPPORTED and pt_last_error() names the field and value, as the
+ * reference's Validator names a rejected option. pt_program_from_description needs no device. */
+pt_status pt_program_from_description(const char* program_json, size_t len, int32_t width, int32_t height,
+                                      int32_t frames_in_flight, pt_program* out);
+pt_status pt_renderer_load_description(pt_ctx* ctx, const char* program_json, size_t len, int32_t width, int32_t height,
+                                       int32_t frames_in_flight, pt_renderer** out);
 void      pt_renderer_unload(pt_renderer* r);
 
 /* Engine.renderFrame x nframes (Engine.ts:190-204): frame k traces 1 spp per pixel with

@@ -458,3 +458,74 @@ def compile_scene(scene, strategy=None, width=256, height=256, reset_salt=1234):
     return CompiledRenderer(id=f"{strategy.color}-{strategy.direct_lighting}-{scene.get('id', 'scene')}", scene=arrays,
                             program=prog, width=width, height=height, camera=scene["camera"],
                             reset_salt=reset_salt, warnings=warnings, material_names=list(mat_names))
+
+
+# The proc-sky environment body of the reference's procedural-sky scenes, the one PT_ENV_SKY bakes
+# (tests/compiler/__snapshots__/generated-glsl.snapshot.test.ts.snap:63758).
+PROC_SKY_GLSL = ("mix(vec3(0.06, 0.08, 0.12), vec3(0.35, 0.45, 0.65), 0.5 * (dir.y + 1.0)) + vec3(18.0, 16.0, 13.0) * "
+                 "smoothstep(0.9994, 0.9999, dot(dir, normalize(vec3(0.4, 0.35, 0.2))))")
+
+
+def program_from_description(description, width, height, frames_in_flight=1):
+    """The library's lowering of a reference ProgramDescription (a dict, or its JSON text) to a
+    pt_program (pt_program_from_description); raises _lib.PtError naming the unsupported field."""
+    import json as _json
+    from . import _lib
+    text = description if isinstance(description, str) else _json.dumps(description)
+    raw = text.encode()
+    prog = abi.PtProgram()
+    lib = _lib.load()
+    _lib.check(lib.pt_program_from_description(raw, len(raw), int(width), int(height), int(frames_in_flight), C_byref(prog)))
+    return prog
+
+
+def C_byref(x):
+    import ctypes
+    return ctypes.byref(x)
+
+
+def describe(scene, strategy):
+    """The ProgramDescription fields (src/compiler/plan/types.ts:39-249) this backend's plan
+    decides for (scene, strategy), in the reference's vocabulary: what the reference's Planner
+    would record for the same choices. Round trip: program_from_description(describe(...))
+    equals compile_scene(...).program for the lowered fields."""
+    lights = scene.get("lights", [])
+    env = scene.get("environment", {"type": "none"})
+    models = []
+    for m in scene.get("materials", {}).values():
+        if m["model"] not in models:
+            models.append(m["model"])
+    media_models = ["hg"] if any(m.get("medium") is not None and medium_arm(m["medium"], strategy) != abi.MED_ABSORB
+                                 for m in scene.get("materials", {}).values()) else []
+    d = {
+        "dispersion": None,
+        "environment": ({"type": "none"} if env.get("type", "none") == "none" else
+                        {"type": "procedural", "glsl": {"kind": "glsl", "source": PROC_SKY_GLSL}}),
+        "estimator": {
+            "accumulation": {"type": "average", **({"variance": True} if strategy.variance else {})},
+            "envSampler": {"chart": "equirect", "compensation": bool(env.get("compensated", False))},
+            "lighting": None if strategy.direct_lighting == "none" else {"method": strategy.direct_lighting,
+                                                                          "selection": "power"},
+            "mediumLightSampling": strategy.medium_light_sampling,
+            "russianRoulette": None if strategy.rr_start < 0 else {"startDepth": strategy.rr_start,
+                                                                    "maxSurvival": strategy.rr_max_survival},
+            "volumeSampling": strategy.volume_sampling if media_models else "none",
+            "wavelengths": strategy.wavelengths if strategy.color == "spectral" else None,
+        },
+        "intersection": {"drivenPlacement": False, "instanceAccel": "tlas", "meshTraversal": "bvh",
+                         "objectDispatch": "unrolled"},
+        "materials": {"models": models, "twoSidedShading": False, "materialsReadUv": False},
+        "measurement": {
+            "ambient": "euclidean",
+            "camera": {"type": "pinhole", **{k: v for k, v in scene["camera"].items() if k != "type"}},
+            "color": strategy.color, "maxBounces": strategy.max_bounces,
+            "maxNullCrossings": strategy.max_null_crossings, "response": "radiance",
+            "scattering": strategy.scattering, "sensor": "cie1931" if strategy.color == "spectral" else "ideal",
+            "shadows": "opaque-dielectrics",
+        },
+        "media": {"deflecting": False, "heterogeneousArms": False, "models": media_models},
+        "spectrumKinds": [],
+        "view": {"tonemap": {"type": "reinhard"}},
+    }
+    _ = lights
+    return d

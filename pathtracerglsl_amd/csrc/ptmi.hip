@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "builder.h"
+#include "description.h"
 #include "dev_common.h"
 #include "shading.h"
 #include "traversal.h"
@@ -1121,6 +1122,21 @@ struct pt_renderer {
 extern "C" {
 
 const char* pt_last_error(void) { return g_err.c_str(); }
+
+pt_status pt_program_from_description(const char* json, size_t len, int32_t width, int32_t height, int32_t frames_in_flight,
+                                      pt_program* out) {
+    if (!json || !out) return fail(PT_E_INVALID, "pt_program_from_description: null argument");
+    std::string err;
+    pt_status st = ptmi_lower_description(json, len, width, height, frames_in_flight, out, err);
+    return st == PT_OK ? PT_OK : fail(st, err);
+}
+
+pt_status pt_renderer_load_description(pt_ctx* ctx, const char* json, size_t len, int32_t width, int32_t height,
+                                       int32_t frames_in_flight, pt_renderer** out) {
+    pt_program pg;
+    pt_status st = pt_program_from_description(json, len, width, height, frames_in_flight, &pg);
+    return st == PT_OK ? pt_renderer_load(ctx, &pg, out) : st;
+}
 
 pt_status pt_finish(pt_ctx* ctx) {
     if (!ctx) return fail(PT_E_INVALID, "pt_finish: null ctx");

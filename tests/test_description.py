@@ -1,0 +1,102 @@
+"""The reference's ProgramDescription (the Planner's link map, src/compiler/plan/types.ts:39-249)
+lowered by the library (pt_program_from_description), over the 505 frozen link maps of
+tests/compiler/__snapshots__/program-description.snapshot.test.ts.snap (tests/golden/
+program_descriptions.json; fields the lowering reads, plus five complete records).
+
+Pins: every accepted record lowers to the program its fields name (estimator, bounce budget,
+Russian roulette, colour model, wavelength sampler, medium options, variance); every rejected
+record is rejected with PT_E_UNSUPPORTED naming a field whose value selects a component outside
+this backend; complete records (with the fields the lowering ignores) lower like their pruned
+form; and this backend's own plans round-trip: the lowering of compiler.describe(scene,
+strategy) is compile_scene's program."""
+import json
+import os
+
+import pytest
+
+from pathtracerglsl_amd import _lib, abi, compiler, scenes
+
+FIX = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "program_descriptions.json")))
+SUPPORTED_MODELS = {"lambert", "dielectric", "ggx", "none"}
+
+
+def _lower(d, w=64, h=48, fif=1):
+    return compiler.program_from_description(d, w, h, fif)
+
+
+def _expected_estimator(d):
+    lighting = d["estimator"].get("lighting")
+    return abi.EST_PT if lighting is None else {"nee": abi.EST_NEE, "mis": abi.EST_MIS}[lighting["method"]]
+
+
+def test_library_lowers_or_names_every_snapshot_case():
+    accepted, rejected = 0, 0
+    for name, d in FIX["cases"].items():
+        try:
+            p = _lower(d)
+        except _lib.PtError as e:
+            assert e.status == 5, (name, str(e))                      # PT_E_UNSUPPORTED
+            msg = str(e)
+            assert "is not supported by this backend" in msg, (name, msg)
+            rejected += 1
+            continue
+        accepted += 1
+        m, e = d["measurement"], d["estimator"]
+        assert p.max_bounces == m["maxBounces"], name
+        assert p.max_null_crossings == m.get("maxNullCrossings", 32), name
+        assert p.color == (abi.COLOR_SPECTRAL if m["color"] == "spectral" else abi.COLOR_RGB), name
+        assert p.estimator == _expected_estimator(d), name
+        rr = e.get("russianRoulette")
+        assert p.rr_start == (-1 if rr is None else rr["startDepth"]), name
+        assert abs(p.rr_max_survival - ((rr or {}).get("maxSurvival") or 0.95)) < 1e-7, name
+        assert p.wavelengths == (abi.WL_UNIFORM if e.get("wavelengths") == "uniform" else abi.WL_VISIBLE), name
+        assert p.medium_light_sampling == (abi.MLS_EQUIANGULAR if e.get("mediumLightSampling") == "equiangular"
+                                           else abi.MLS_VERTEX), name
+        assert p.volume_sampling == (abi.VS_DELTA_TRACKING if e.get("volumeSampling") == "delta-tracking"
+                                     else abi.VS_ANALYTIC), name
+        assert p.variance == int(bool(e.get("accumulation", {}).get("variance"))), name
+        assert set(d["materials"]["models"]) <= SUPPORTED_MODELS, name
+        assert (p.width, p.height, p.frames_in_flight, p.abi_version) == (64, 48, 1, abi.ABI_VERSION)
+    assert accepted >= 100 and accepted + rejected == 505
+
+
+@pytest.mark.parametrize("name", sorted(FIX["full"]))
+def test_complete_records_lower_like_their_pruned_form(name):
+    full, pruned = FIX["full"][name], FIX["cases"][name]
+    try:
+        a = _lower(full)
+    except _lib.PtError as e:
+        with pytest.raises(_lib.PtError) as e2:
+            _lower(pruned)
+        assert str(e2.value) == str(e)
+        return
+    b = _lower(pruned)
+    assert bytes(a) == bytes(b)
+
+
+def test_rejection_names_the_field():
+    d = json.loads(json.dumps(FIX["cases"]["analytic-glass + pathtracer"]))
+    d["measurement"]["camera"]["type"] = "fisheye"
+    with pytest.raises(_lib.PtError, match="measurement.camera.type 'fisheye'"):
+        _lower(d)
+    d = json.loads(json.dumps(FIX["cases"]["analytic-glass + pathtracer"]))
+    d["materials"]["models"].append("rough_dielectric")
+    with pytest.raises(_lib.PtError, match="material model 'rough_dielectric'"):
+        _lower(d)
+    with pytest.raises(_lib.PtError, match="malformed JSON"):
+        _lower('{"measurement": {')
+
+
+@pytest.mark.parametrize("make,strategy", [
+    (scenes.cornell_box, compiler.RenderStrategy("mis", 8, 3)),
+    (scenes.cornell_box, compiler.RenderStrategy("none", 5, -1, variance=True)),
+    (scenes.haze, compiler.RenderStrategy("nee", 24, 4, medium_light_sampling="equiangular")),
+    (scenes.emit_sat, compiler.RenderStrategy("nee", 32, -1, volume_sampling="delta-tracking")),
+    (scenes.cornell_spectral_medium, compiler.RenderStrategy("mis", 8, 3, color="spectral", wavelengths="uniform")),
+    (scenes.icosphere_cloud, compiler.RenderStrategy("mis", 12, 3, 0.95, 4)),
+])
+def test_own_plans_round_trip(make, strategy):
+    sc = make()
+    comp = compiler.compile_scene(sc, strategy, 40, 30)
+    p = _lower(compiler.describe(sc, strategy), 40, 30, strategy.frames_in_flight)
+    assert bytes(p) == bytes(comp.program)
