@@ -157,7 +157,7 @@ typedef struct pt_light {
     pt_spectrum s_radiance;  /* spectral color model, role 'emission' */
 } pt_light;
 
-enum { PT_ENV_NONE = 0, PT_ENV_SKY = 1 };
+enum { PT_ENV_NONE = 0, PT_ENV_SKY = 1, PT_ENV_CONSTANT = 2 };
 
 /* Environment. PT_ENV_SKY is the procedural sky of the reference's proc-sky scenes
  * (tests/compiler/__snapshots__/generated-glsl.snapshot.test.ts.snap:63758), sampled
@@ -170,7 +170,11 @@ typedef struct pt_env {
     float   intensity;
     float   rotation;           /* u_envRotation (radians) */
     float   select_prob;        /* two-stage env-vs-finite draw; ignored without finite lights */
-    float   pad;
+    int32_t samplable;          /* PT_ENV_CONSTANT: sampled as a light (sampleAsLight); the sky always is */
+    /* PT_ENV_CONSTANT (src/compiler/generate/features/environment.ts:65-133): radiance
+     * color * intensity in every direction; sampled uniformly on the sphere, pdf 1/(4 pi) */
+    float   color[4];
+    pt_spectrum s_color;        /* spectral, role 'emission' (SETTING_ROLES.sky) */
 } pt_env;
 
 typedef struct pt_scene_desc {
@@ -247,10 +251,16 @@ typedef struct pt_params {
     float    tan_fov;         /* tan(fov/2), host f64 -> f32 */
     float    aspect;          /* width / height */
     uint32_t reset_salt;
-    uint32_t pad0;
+    int32_t  camera;          /* PT_CAM_PINHOLE or PT_CAM_ORTHOGRAPHIC */
     int32_t  pixel_offset[2];
-    int32_t  pad1[2];
+    float    ortho_scale;     /* orthographic: world half-height of the view (u_orthoScale) */
+    int32_t  pad1;
 } pt_params;
+
+/* Cameras (src/components/camera/pinhole/pinhole.glsl:4-18, orthographic/orthographic.glsl:6-21):
+ * the pinhole spreads directions over tan(fov/2); the orthographic camera shares the forward
+ * direction and slides the origin over the film plane. */
+enum { PT_CAM_PINHOLE = 0, PT_CAM_ORTHOGRAPHIC = 1 };
 
 /* ---- rays for the traversal entry points (SoA) ---- */
 typedef struct pt_rays_soa {

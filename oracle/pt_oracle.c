@@ -1077,7 +1077,9 @@ static v3 sky_radiance_raw(v3 dir) {
     float k = ptm_smoothstep(0.9994f, 0.9999f, dot(dir, sun));
     return add(base, muls(V(18.0f, 16.0f, 13.0f), k));
 }
-static sp env_radiance(const or_scene* S, v3 dir) {
+static sp env_radiance(const or_scene* S, v3 dir, const wl_t* wl) {
+    if (S->env.type == PT_ENV_CONSTANT)   /* features/environment.ts:119-133: color * u_envIntensity */
+        return smuls(lift(S, S->env.color, &S->env.s_color, wl), S->env.intensity);
     if (S->env.type != PT_ENV_SKY) return sgrey(0.0f);
     v3 d = normalize(dir);
     float cx = S->env_rot_cs[0], sy = S->env_rot_cs[1];
@@ -1148,6 +1150,7 @@ static float env_uniform_fraction(const or_scene* S) {
     return env_marg(S, S->env_h - 1) > 0.0f ? 0.0625f : 1.0f;
 }
 static float environment_pdf(const or_scene* S, v3 dir) {
+    if (S->env.type == PT_ENV_CONSTANT) return 1.0f / (4.0f * PTM_PI);   /* CONSTANT_PDF */
     float a = env_uniform_fraction(S);
     return (1.0f - a) * env_table_pdf(S, dir) + a / (4.0f * PTM_PI);
 }
@@ -1258,10 +1261,21 @@ static lsample_t lighting_sample_finite(const or_scene* S, v3 p, float x0, float
     ls.pdf *= sel;
     return ls;
 }
-static int env_samplable(const or_scene* S) { return S->env.type == PT_ENV_SKY; }
+static int env_samplable(const or_scene* S) { return S->env.type == PT_ENV_SKY || (S->env.type == PT_ENV_CONSTANT && S->env.samplable); }
 static int env_select_live(const or_scene* S) { return env_samplable(S) && S->n_lights > 0; }
-static lsample_t environment_sample(const or_scene* S, float x0, float x1) {
+static lsample_t environment_sample(const or_scene* S, float x0, float x1, const wl_t* wl) {
     lsample_t ls;
+    if (S->env.type == PT_ENV_CONSTANT) {   /* the constant sampler (features/environment.ts:74-87) */
+        float z = 1.0f - 2.0f * x0;
+        float r = sqrtf(fmaxf_(0.0f, 1.0f - z * z));
+        float phi = PTM_TWO_PI * x1;
+        ls.wi = V(r * COS(phi), z, r * SIN(phi));
+        ls.distance = MAX_DIST;
+        ls.radiance = env_radiance(S, ls.wi, wl);
+        ls.pdf = 1.0f / (4.0f * PTM_PI);
+        ls.flags = 0; ls.light_id = -1;
+        return ls;
+    }
     float a = env_uniform_fraction(S);
     if (x0 < a) {
         float y = 1.0f - 2.0f * (x0 / a);
@@ -1272,7 +1286,7 @@ static lsample_t environment_sample(const or_scene* S, float x0, float x1) {
         ls.wi = env_sample_table(S, (x0 - a) / (1.0f - a), x1);
     }
     ls.distance = MAX_DIST;
-    ls.radiance = env_radiance(S, ls.wi);
+    ls.radiance = env_radiance(S, ls.wi, wl);
     ls.pdf = environment_pdf(S, ls.wi);
     ls.flags = 0; ls.light_id = -1;
     return ls;
@@ -1280,13 +1294,13 @@ static lsample_t environment_sample(const or_scene* S, float x0, float x1) {
 static lsample_t lighting_sample(const or_scene* S, v3 p, float x0, float x1, const wl_t* wl) {
     lsample_t ls;
     if (S->n_lights == 0) {
-        if (env_samplable(S)) return environment_sample(S, x0, x1);
+        if (env_samplable(S)) return environment_sample(S, x0, x1, wl);
         ls.pdf = 0.0f; return ls;
     }
     if (!env_select_live(S)) return lighting_sample_finite(S, p, x0, x1, wl);
     float ps = S->env.select_prob;
     if (x0 < ps) {
-        ls = environment_sample(S, clampf(x0 / ps, 0.0f, 0.9999999f), x1);
+        ls = environment_sample(S, clampf(x0 / ps, 0.0f, 0.9999999f), x1, wl);
         ls.pdf *= ps;
         return ls;
     }
@@ -1652,7 +1666,7 @@ static sp transport_trace(const or_scene* S, const or_program* P, v3 ro, v3 rd, 
                 else if (mis) w = power_heuristic(prev_bsdf_pdf, env_select_live(S) ? S->env.select_prob * environment_pdf(S, rd) : environment_pdf(S, rd));
                 else w = 0.0f;
             }
-            sp Le = env_radiance(S, rd);
+            sp Le = env_radiance(S, rd, wl);
             L = sadd(L, smul(smuls(T, w), Le));
             if (A) adj_scatter(A, smul(smuls(T, w), Le), A->D);
             break;
@@ -1960,6 +1974,19 @@ void or_trace_closest_brute(const or_scene* S, const float* rays, int n, float* 
 }
 int or_region_at(const or_scene* S, const float* p) { return region_at(S, V(p[0], p[1], p[2])); }
 
+/* camera_generateRay: pinhole (camera/pinhole/pinhole.glsl:4-18) or orthographic
+ * (camera/orthographic/orthographic.glsl:6-21: the origin slides over the film plane of
+ * half-height u_orthoScale, every ray looks along forward). ndcx already carries the aspect. */
+static void camera_ray(const pt_params* par, v3 O, v3 F, v3 R, v3 U, float ndcx, float ndcy, v3* org, v3* dir) {
+    if (par->camera == PT_CAM_ORTHOGRAPHIC) {
+        *org = add(add(O, muls(R, ndcx * par->ortho_scale)), muls(U, ndcy * par->ortho_scale));
+        *dir = F;
+    } else {
+        *org = O;
+        *dir = normalize(add(add(F, muls(R, ndcx * par->tan_fov)), muls(U, ndcy * par->tan_fov)));
+    }
+}
+
 /* Render nframes frames starting at sample_count into film (RGBA32F, row 0 = bottom),
  * following features/accumulation.ts:14-51 and accumulator/average/average.glsl:6-10.
  * Optional per-sample radiance output `samples` [nframes][H][W][3] (may be NULL).
@@ -1998,8 +2025,9 @@ void or_render(const or_scene* S, const pt_program* prog, const pt_params* par, 
                 float ndcx = (2.0f * filmx / (float)W) - 1.0f;
                 float ndcy = (2.0f * filmy / (float)H) - 1.0f;
                 ndcx *= par->aspect;
-                v3 dir = normalize(add(add(F, muls(R, ndcx * par->tan_fov)), muls(U, ndcy * par->tan_fov)));
-                sp Ls = transport_trace(S, &P, O, dir, &rng, &c1, &c2, &wl, NULL);
+                v3 org, dir;
+                camera_ray(par, O, F, R, U, ndcx, ndcy, &org, &dir);
+                sp Ls = transport_trace(S, &P, org, dir, &rng, &c1, &c2, &wl, NULL);
                 v3 Lr = V(Ls.x, Ls.y, Ls.z);
                 if (P.spectral) {                   /* color_to_pixel (color/spectral/spectral.glsl:27-33) */
                     v3 pix = V(0.0f, 0.0f, 0.0f);
@@ -2086,7 +2114,8 @@ void or_env_pdf(const or_scene* S, const float* dirs, int n, float* out) {
 }
 void or_env_sample(const or_scene* S, const float* xi, int n, float* dirs, float* pdf) {
     for (int i = 0; i < n; i++) {
-        lsample_t ls = environment_sample(S, xi[2 * i], xi[2 * i + 1]);
+        wl_t wl0; memset(&wl0, 0, sizeof wl0);
+        lsample_t ls = environment_sample(S, xi[2 * i], xi[2 * i + 1], &wl0);
         dirs[3 * i] = ls.wi.x; dirs[3 * i + 1] = ls.wi.y; dirs[3 * i + 2] = ls.wi.z; pdf[i] = ls.pdf;
     }
 }
@@ -2121,10 +2150,11 @@ void or_adjoint(const or_scene* S, const pt_program* prog, const pt_params* par,
                 float ndcx = (2.0f * filmx / (float)W) - 1.0f;
                 float ndcy = (2.0f * filmy / (float)H) - 1.0f;
                 ndcx *= par->aspect;
-                v3 dir = normalize(add(add(F, muls(R, ndcx * par->tan_fov)), muls(U, ndcy * par->tan_fov)));
+                v3 org, dir;
+                camera_ray(par, O, F, R, U, ndcx, ndcy, &org, &dir);
                 wl_t wl; memset(&wl, 0, sizeof wl);
                 memset(A.D, 0, sizeof A.D);
-                (void)transport_trace(S, &P, O, dir, &rng, &c1, &c2, &wl, &A);
+                (void)transport_trace(S, &P, org, dir, &rng, &c1, &c2, &wl, &A);
             }
             for (int k = 0; k < ADJ_N; k++) gr[k] += A.G[k];
         }

@@ -15,7 +15,8 @@ MED_ANALYTIC, MED_ABSORB, MED_DELTA = 0, 1, 2
 VS_ANALYTIC, VS_DELTA_TRACKING = 0, 1
 IOR_CONSTANT, IOR_CAUCHY, IOR_SELLMEIER = 0, 1, 2
 LIGHT_QUAD, LIGHT_POINT = 0, 1
-ENV_NONE, ENV_SKY = 0, 1
+ENV_NONE, ENV_SKY, ENV_CONSTANT = 0, 1, 2
+CAM_PINHOLE, CAM_ORTHOGRAPHIC = 0, 1
 EST_PT, EST_NEE, EST_MIS = 0, 1, 2
 COLOR_RGB, COLOR_SPECTRAL = 0, 1
 WL_VISIBLE, WL_UNIFORM = 0, 1
@@ -52,10 +53,25 @@ assert SPECTRUM_DTYPE.itemsize == 32 and INSTANCE_DTYPE.itemsize == 32
 assert OBJECT_DTYPE.itemsize == 112 and MATERIAL_DTYPE.itemsize == 416 and LIGHT_DTYPE.itemsize == 144
 
 
+class PtSpectrum(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("scaled", C.c_int16), ("d65", C.c_int16), ("value", C.c_float),
+                ("scale", C.c_float), ("c", C.c_float * 3), ("pad", C.c_float)]
+
+    @classmethod
+    def of(cls, rec):
+        s = cls()
+        s.kind, s.scaled, s.d65 = int(rec["kind"]), int(rec["scaled"]), int(rec["d65"])
+        s.value, s.scale = float(rec["value"]), float(rec["scale"])
+        for i in range(3):
+            s.c[i] = float(rec["c"][i])
+        return s
+
+
 class PtEnv(C.Structure):
     _fields_ = [("type", C.c_int32), ("table_w", C.c_int32), ("table_h", C.c_int32),
                 ("compensated", C.c_int32), ("intensity", C.c_float), ("rotation", C.c_float),
-                ("select_prob", C.c_float), ("pad", C.c_float)]
+                ("select_prob", C.c_float), ("samplable", C.c_int32), ("color", C.c_float * 4),
+                ("s_color", PtSpectrum)]
 
 
 class PtSceneDesc(C.Structure):
@@ -84,7 +100,8 @@ class PtParams(C.Structure):
     _fields_ = [("cam_position", C.c_float * 4), ("cam_forward", C.c_float * 4),
                 ("cam_right", C.c_float * 4), ("cam_up", C.c_float * 4),
                 ("tan_fov", C.c_float), ("aspect", C.c_float), ("reset_salt", C.c_uint32),
-                ("pad0", C.c_uint32), ("pixel_offset", C.c_int32 * 2), ("pad1", C.c_int32 * 2)]
+                ("camera", C.c_int32), ("pixel_offset", C.c_int32 * 2), ("ortho_scale", C.c_float),
+                ("pad1", C.c_int32)]
 
 
 ADJ_MAX_ALBEDO = 16
@@ -140,7 +157,7 @@ class SceneArrays:
         self.objects = np.ascontiguousarray(objects, dtype=OBJECT_DTYPE)
         self.materials = np.ascontiguousarray(materials, dtype=MATERIAL_DTYPE)
         self.lights = np.ascontiguousarray(lights, dtype=LIGHT_DTYPE)
-        self.env = env if env is not None else PtEnv(ENV_NONE, 512, 256, 0, 1.0, 0.0, 0.5, 0.0)
+        self.env = env if env is not None else PtEnv(ENV_NONE, 512, 256, 0, 1.0, 0.0, 0.5, 0)
         self.ambient_material = int(ambient_material)
         self.spectral_curves = (None if spectral_curves is None
                                 else np.ascontiguousarray(spectral_curves, dtype=np.float32).reshape(95, 4))

@@ -77,7 +77,12 @@ def camera_params(camera, width, height, reset_salt=1234):
                      (prm.cam_right, r), (prm.cam_up, u)):
         for i in range(3):
             dst[i] = float(np.float32(src[i]))
-    prm.tan_fov = float(np.float32(math.tan(camera["fov"] / 2.0)))   # features/camera.ts:54-66
+    if camera.get("type", "pinhole") == "orthographic":
+        prm.camera = abi.CAM_ORTHOGRAPHIC
+        prm.ortho_scale = float(np.float32(camera.get("scale", 1.0)))     # u_orthoScale
+    else:
+        prm.camera = abi.CAM_PINHOLE
+        prm.tan_fov = float(np.float32(math.tan(camera["fov"] / 2.0)))   # features/camera.ts:54-66
     prm.aspect = float(np.float32(width / height))
     prm.reset_salt = reset_salt
     return prm
@@ -189,8 +194,10 @@ def _analyze_validate(scene, strategy):
                            f"{', '.join(tracked)}) but volumeSampling is 'analytic' — set volumeSampling 'delta-tracking'")
     if tracked and strategy.medium_light_sampling == "equiangular":
         raise CompileError("mediumLightSampling 'equiangular' does not support media on the null-collision arms; use 'vertex'")
-    if strategy.color == "spectral" and scene.get("environment", {"type": "none"}).get("type", "none") != "none":
-        raise CompileError("spectral programs with an environment light are not supported by this backend")
+    if strategy.color == "spectral" and scene.get("environment", {"type": "none"}).get("type", "none") == "procedural":
+        raise CompileError("spectral programs with a procedural environment are not supported by this backend")
+    if scene.get("camera", {}).get("type", "pinhole") not in ("pinhole", "orthographic"):
+        raise CompileError(f"camera '{scene['camera'].get('type')}' is not supported by this backend")
     mats = scene.get("materials", {})
     for name, m in mats.items():
         if m.get("model") not in MODELS:
@@ -201,7 +208,7 @@ def _analyze_validate(scene, strategy):
         if o["type"] not in ("mesh", "sphere", "quad", "instances"):
             raise CompileError(f"object type '{o['type']}' is not supported by this backend")
     env = scene.get("environment", {"type": "none"})
-    if env.get("type") not in ("none", "procedural"):
+    if env.get("type") not in ("none", "procedural", "constant"):
         raise CompileError(f"environment '{env.get('type')}' is not supported by this backend")
     for l in scene.get("lights", []):
         if l["type"] == "quad":
@@ -413,12 +420,20 @@ def compile_scene(scene, strategy=None, width=256, height=256, reset_salt=1234):
         rec["cdf"] = acc / total
         light_recs.append(rec)
 
-    env = abi.PtEnv(abi.ENV_NONE, 512, 256, 0, 1.0, 0.0, 0.5, 0.0)
+    env = abi.PtEnv(abi.ENV_NONE, 512, 256, 0, 1.0, 0.0, 0.5, 0)
     e = scene.get("environment", {"type": "none"})
     if e.get("type") == "procedural":
         env = abi.PtEnv(abi.ENV_SKY, 512, 256, int(e.get("compensated", False)),
                         float(e.get("intensity", 1.0)), float(e.get("rotation", 0.0)),
-                        float(e.get("select_prob", 0.5)), 0.0)
+                        float(e.get("select_prob", 0.5)), 1)
+    elif e.get("type") == "constant":
+        # features/environment.ts:119-133: a constant sky, sampled as a light only when asked
+        col = e.get("color", [1.0, 1.0, 1.0])
+        env = abi.PtEnv(abi.ENV_CONSTANT, 0, 0, 0, float(e.get("intensity", 1.0)), 0.0,
+                        float(e.get("select_prob", 0.5)), int(bool(e.get("sample_as_light", False))))
+        for i, c in enumerate(spectral.rgb_of(col)):
+            env.color[i] = float(c)
+        env.s_color = abi.PtSpectrum.of(srec(col, spectral.ROLE_EMISSION))
 
     tv = np.concatenate(tri_chunks) if tri_chunks else np.zeros((0, 9), np.float32)
     to = np.concatenate(tri_obj_chunks) if tri_obj_chunks else np.zeros(0, np.int32)

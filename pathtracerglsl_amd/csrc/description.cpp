@@ -185,7 +185,9 @@ bool lower(const JVal& d, int32_t width, int32_t height, int32_t fif, pt_program
     if (const JVal* aov = m->get("aovs"); aov && aov->kind == JVal::ARR && !aov->arr.empty())
         return L.unsupported("measurement.aovs", "(first-hit guides)");
     const JVal* cam = m->get("camera");
-    if (!cam || !is_str(cam->get("type"), "pinhole")) return L.unsupported("measurement.camera.type", quoted(cam ? cam->get("type") : nullptr));
+    // pinhole and orthographic (the pt_params camera); the lens draws are consumed either way
+    if (!cam || !(is_str(cam->get("type"), "pinhole") || is_str(cam->get("type"), "orthographic")))
+        return L.unsupported("measurement.camera.type", quoted(cam ? cam->get("type") : nullptr));
 
     // estimator (compiler/types.ts:642-721)
     const JVal* e = d.get("estimator");
@@ -262,6 +264,8 @@ bool lower(const JVal& d, int32_t width, int32_t height, int32_t fif, pt_program
         const JVal* src = path(*env, {"glsl", "source"});
         if (!src || src->kind != JVal::STR || src->str != kProcSky) return L.unsupported("environment", "'procedural' with this GLSL body");
         if (pg.color == PT_COLOR_SPECTRAL) return L.unsupported("environment", "'procedural' under the spectral colour model");
+    } else if (is_str(et, "constant")) {
+        // color * intensity in every direction; the colour itself travels with the scene (pt_env)
     } else {
         return L.unsupported("environment.type", quoted(et));
     }

@@ -102,6 +102,9 @@ struct DevScene {
     const pt_light* lights;
     int n_lights;
     int env_type, env_w, env_h, env_compensated;
+    int env_samplable;          // PT_ENV_CONSTANT sampled as a light
+    float4 env_color;           // PT_ENV_CONSTANT radiance (RGB)
+    pt_spectrum env_s_color;    // PT_ENV_CONSTANT radiance (spectral)
     float env_intensity, env_rotation, env_rot_c, env_rot_s, env_select_prob;
     const float* env_cond;
     const float* env_marg;

@@ -170,6 +170,8 @@ struct Camera {
     float ox, oy, oz, fx, fy, fz, rx, ry, rz, ux, uy, uz, tan_fov, aspect;
     uint32_t salt;
     int off_x, off_y;
+    int ortho;          // PT_CAM_ORTHOGRAPHIC
+    float ortho_scale;
 };
 
 // path_state_init's current_medium = scene_region_at(camera origin), once per wave.
@@ -177,6 +179,15 @@ template <int STACK>
 __global__ void __launch_bounds__(64) k_camera_medium(DevScene S, Paths P, Camera C) {
     __shared__ uint2 stk[STACK * 64];
     if (threadIdx.x == 0) P.counts[C_CAM_MEDIUM] = region_at<64>(S, mk3(C.ox, C.oy, C.oz), stk);
+}
+// The same per path when camera rays do not share an origin (orthographic), after raygen.
+template <int STACK>
+__global__ void __launch_bounds__(SBLOCK) k_ray_medium(DevScene S, Paths P, int n) {
+    __shared__ uint2 stk[STACK * SBLOCK];
+    for (int base = blockIdx.x * SBLOCK; base < n; base += gridDim.x * SBLOCK) {
+        const int i = base + threadIdx.x;
+        if (i < n) P.medium[i] = region_at<SBLOCK>(S, xyz(P.ray_o[i]), stk);
+    }
 }
 
 // One camera sample of pixel (x, y) at sampleCount `sample` into path slot i.
@@ -198,18 +209,25 @@ PT_DEV void raygen_path(const Paths& P, const Camera& C, int W, int H, int i, in
         P.wl[i] = make_float4(l[0], l[1], l[2], l[3]);
         P.eta[i] = 1.0f;
     }
-    // camera/pinhole/pinhole.glsl:4-18
+    // camera/pinhole/pinhole.glsl:4-18; orthographic/orthographic.glsl:6-21 (the origin slides
+    // over the film plane, every ray looks along forward)
     float ndcx = (2.0f * filmx / (float)W) - 1.0f;
     float ndcy = (2.0f * filmy / (float)H) - 1.0f;
     ndcx *= C.aspect;
     f3 F = mk3(C.fx, C.fy, C.fz), R = mk3(C.rx, C.ry, C.rz), U = mk3(C.ux, C.uy, C.uz);
-    f3 d = normalize((F + R * (ndcx * C.tan_fov)) + U * (ndcy * C.tan_fov));
-    P.ray_o[i] = make_float4(C.ox, C.oy, C.oz, 0.0f);
+    f3 org = mk3(C.ox, C.oy, C.oz), d;
+    if (C.ortho) {
+        org = (org + R * (ndcx * C.ortho_scale)) + U * (ndcy * C.ortho_scale);
+        d = F;
+    } else {
+        d = normalize((F + R * (ndcx * C.tan_fov)) + U * (ndcy * C.tan_fov));
+    }
+    P.ray_o[i] = f4(org, 0.0f);
     P.ray_d[i] = f4(d, 0.0f);
     P.thr[i] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
     P.rad[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     P.rng[i] = make_uint2(r.base, r.dim);
-    P.prev[i] = make_float4(C.ox, C.oy, C.oz, 0.0f);
+    P.prev[i] = f4(org, 0.0f);
     P.flags[i] = mk_flags(true, 0, 0);
     if (media) P.medium[i] = P.counts[C_CAM_MEDIUM];
     P.q[0][i] = i;
@@ -669,7 +687,7 @@ __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene Sk, Paths P, int qsel
                     else w = 0.0f;
                 }
                 if (S.env_type != PT_ENV_NONE) {
-                    const sp4 c = (T * w) * env_radiance(S, rd);
+                    const sp4 c = (T * w) * env_radiance<SPEC>(S, rd, wl);
                     L = L + c;
                     if (ADJ) adj_add_G(P, i, A.D, adj_e(P, i, c));
                 }
@@ -1509,6 +1527,12 @@ static pt_status build_scene(const pt_scene_desc* sd, hipStream_t stream, SceneS
         D.env_select_prob = sd->env.select_prob;
         if ((st = SS.upload(cond.data(), cond.size(), &D.env_cond)) != PT_OK) return st;
         if ((st = SS.upload(marg.data(), marg.size(), &D.env_marg)) != PT_OK) return st;
+    } else if (sd->env.type == PT_ENV_CONSTANT) {
+        D.env_intensity = sd->env.intensity;
+        D.env_color = make_float4(sd->env.color[0], sd->env.color[1], sd->env.color[2], 0.0f);
+        D.env_s_color = sd->env.s_color;
+        D.env_samplable = sd->env.samplable ? 1 : 0;
+        D.env_select_prob = sd->env.select_prob;
     } else if (sd->env.type != PT_ENV_NONE) {
         return fail(PT_E_UNSUPPORTED, "environment type not provided by this build");
     }
@@ -1549,8 +1573,8 @@ pt_status pt_renderer_load(pt_ctx* ctx, const pt_program* prog, pt_renderer** ou
         return fail(PT_E_UNSUPPORTED, "unknown volume sampling");
     if (prog->medium_light_sampling == PT_MLS_EQUIANGULAR && prog->estimator != PT_EST_NEE)
         return fail(PT_E_UNSUPPORTED, "equiangular medium NEE runs under the nee estimator");
-    if (prog->color == PT_COLOR_SPECTRAL && ctx->has_scene && ctx->dev.env_type != PT_ENV_NONE)
-        return fail(PT_E_UNSUPPORTED, "spectral programs with an environment light are not provided by this build");
+    if (prog->color == PT_COLOR_SPECTRAL && ctx->has_scene && ctx->dev.env_type == PT_ENV_SKY)
+        return fail(PT_E_UNSUPPORTED, "spectral programs with the procedural sky are not provided by this build");
     if (prog->width <= 0 || prog->height <= 0 || prog->max_bounces < 0) return fail(PT_E_INVALID, "bad film size or bounce budget");
     HIP_TRY(hipSetDevice(ctx->device));
     auto* r = new pt_renderer();
@@ -1643,6 +1667,7 @@ static void launch_wave(pt_renderer* r, const Camera& cam, int frames, uint32_t 
     hipStream_t st = ctx->stream;
     if (MEDIA) hipLaunchKernelGGL((k_camera_medium<STACK>), dim3(1), dim3(64), 0, st, S, P, cam);
     TIMED(ctx, "raygen", n,  hipLaunchKernelGGL((k_raygen<SPEC, ADJ>), dim3((unsigned)((n + SBLOCK - 1) / SBLOCK)), dim3(SBLOCK), 0, st, P, cam, W, H, (int)n, sample0, MEDIA ? 1 : 0, pg.wavelengths));
+    if (MEDIA && cam.ortho) hipLaunchKernelGGL((k_ray_medium<STACK>), dim3(ctx->shade_blocks), dim3(SBLOCK), 0, st, S, P, (int)n);
     int cur = 0;
     const int tgrid = ctx->persist_blocks;
     // a null crossing consumes an iteration but no bounce (pt.ts null-interface branch)
@@ -1701,6 +1726,8 @@ static Camera make_camera(const pt_params* prm) {
     cam.ux = prm->cam_up[0]; cam.uy = prm->cam_up[1]; cam.uz = prm->cam_up[2];
     cam.tan_fov = prm->tan_fov; cam.aspect = prm->aspect; cam.salt = prm->reset_salt;
     cam.off_x = prm->pixel_offset[0]; cam.off_y = prm->pixel_offset[1];
+    cam.ortho = prm->camera == PT_CAM_ORTHOGRAPHIC ? 1 : 0;
+    cam.ortho_scale = prm->ortho_scale;
     return cam;
 }
 
@@ -1751,6 +1778,7 @@ pt_status pt_render_frames(pt_renderer* r, const pt_params* prm, uint32_t nframe
     if (!r || !prm) return fail(PT_E_INVALID, "pt_render_frames: null argument");
     pt_ctx* ctx = r->ctx;
     if (!ctx->has_scene) return fail(PT_E_MISSING_INPUT, "missing input: no scene uploaded (pt_scene_upload)");
+    if (prm->camera != PT_CAM_PINHOLE && prm->camera != PT_CAM_ORTHOGRAPHIC) return fail(PT_E_UNSUPPORTED, "unknown camera");
     HIP_TRY(hipSetDevice(ctx->device));
     const pt_program& pg = r->prog;
     const int F = std::max(1, std::min<int>(pg.frames_in_flight, (int)std::max<uint32_t>(nframes, 1)));

@@ -514,8 +514,12 @@ PT_DEV f3 sky_raw(f3 dir) {
     float k = ptm_smoothstep(0.9994f, 0.9999f, dot(dir, sun));
     return base + mk3(18.0f, 16.0f, 13.0f) * k;
 }
-// Environment radiance: RGB programs only (the spectral compile rejects environments).
-PT_DEV sp4 env_radiance(const DevScene& S, f3 dir) {
+// Environment radiance. The constant sky is color * u_envIntensity in either colour model
+// (features/environment.ts:119-133); the procedural sky runs RGB programs only.
+template <bool SPEC>
+PT_DEV sp4 env_radiance(const DevScene& S, f3 dir, const Wl& wl) {
+    if (S.env_type == PT_ENV_CONSTANT)
+        return lift<SPEC>(S, &S.env_color.x, S.env_s_color, wl) * S.env_intensity;
     if (S.env_type != PT_ENV_SKY) return sgrey(0.0f);
     f3 d = normalize(dir);
     float cx = S.env_rot_c, sy = S.env_rot_s;
@@ -544,6 +548,7 @@ PT_DEV float env_uniform_fraction(const DevScene& S) {
     return S.env_marg[S.env_h - 1] > 0.0f ? 0.0625f : 1.0f;
 }
 PT_DEV float environment_pdf(const DevScene& S, f3 dir) {
+    if (S.env_type == PT_ENV_CONSTANT) return 1.0f / (4.0f * PTM_PI);   // CONSTANT_PDF
     float a = env_uniform_fraction(S);
     return (1.0f - a) * env_table_pdf(S, dir) + a / (4.0f * PTM_PI);
 }
@@ -611,8 +616,8 @@ PT_DEV float light_pdf_one(const pt_light& l, f3 p, f3 light_p, f3 wi) {
     f3 d = light_p - p;
     return dot(d, d) / (l.area * cos_l);
 }
-PT_DEV bool env_samplable(const DevScene& S) { return S.env_type == PT_ENV_SKY; }
-PT_DEV bool env_select_live(const DevScene& S) { return S.env_type == PT_ENV_SKY && S.n_lights > 0; }
+PT_DEV bool env_samplable(const DevScene& S) { return S.env_type == PT_ENV_SKY || (S.env_type == PT_ENV_CONSTANT && S.env_samplable); }
+PT_DEV bool env_select_live(const DevScene& S) { return env_samplable(S) && S.n_lights > 0; }
 template <bool SPEC>
 PT_DEV LSample lighting_sample_finite(const DevScene& S, f3 p, float x0, float x1, const Wl& wl) {
     const int n = S.n_lights;
@@ -627,8 +632,21 @@ PT_DEV LSample lighting_sample_finite(const DevScene& S, f3 p, float x0, float x
     ls.pdf *= sel;
     return ls;
 }
-PT_DEV LSample environment_sample(const DevScene& S, float x0, float x1) {   // sampler_cdf.glsl:61-80
+template <bool SPEC>
+PT_DEV LSample environment_sample(const DevScene& S, float x0, float x1, const Wl& wl) {   // sampler_cdf.glsl:61-80
     LSample ls;
+    if (S.env_type == PT_ENV_CONSTANT) {   // the constant sampler (features/environment.ts:74-87)
+        const float z = 1.0f - 2.0f * x0;
+        const float r = sqrtf(gmax(0.0f, 1.0f - z * z));
+        const float phi = PTM_TWO_PI * x1;
+        float sn, cs; ptm_sincos(phi, &sn, &cs);
+        ls.wi = mk3(r * cs, z, r * sn);
+        ls.distance = MAX_DIST;
+        ls.radiance = env_radiance<SPEC>(S, ls.wi, wl);
+        ls.pdf = 1.0f / (4.0f * PTM_PI);
+        ls.flags = 0; ls.light_id = -1;
+        return ls;
+    }
     float a = env_uniform_fraction(S);
     if (x0 < a) {
         float y = 1.0f - 2.0f * (x0 / a);
@@ -640,7 +658,7 @@ PT_DEV LSample environment_sample(const DevScene& S, float x0, float x1) {   // 
         ls.wi = env_sample_table(S, (x0 - a) / (1.0f - a), x1);
     }
     ls.distance = MAX_DIST;
-    ls.radiance = env_radiance(S, ls.wi);
+    ls.radiance = env_radiance<SPEC>(S, ls.wi, wl);
     ls.pdf = environment_pdf(S, ls.wi);
     ls.flags = 0; ls.light_id = -1;
     return ls;
@@ -649,13 +667,13 @@ PT_DEV LSample environment_sample(const DevScene& S, float x0, float x1) {   // 
 template <bool SPEC>
 PT_DEV LSample lighting_sample(const DevScene& S, f3 p, float x0, float x1, const Wl& wl) {
     if (S.n_lights == 0) {
-        if (env_samplable(S)) return environment_sample(S, x0, x1);
+        if (env_samplable(S)) return environment_sample<SPEC>(S, x0, x1, wl);
         LSample ls; ls.pdf = 0.0f; return ls;
     }
     if (!env_select_live(S)) return lighting_sample_finite<SPEC>(S, p, x0, x1, wl);
     float ps = S.env_select_prob;
     if (x0 < ps) {
-        LSample ls = environment_sample(S, gclamp(x0 / ps, 0.0f, 0.9999999f), x1);
+        LSample ls = environment_sample<SPEC>(S, gclamp(x0 / ps, 0.0f, 0.9999999f), x1, wl);
         ls.pdf *= ps;
         return ls;
     }

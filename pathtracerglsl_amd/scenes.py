@@ -644,3 +644,30 @@ def cornell_smooth_instances(n=4, seed=11, detail=2):
                           "material": "glass", "closed": True})
     sc["id"] = "cornell-smooth-instances"
     return sc
+
+
+def furnace_sky(sample_as_light=True, color=(1.0, 1.0, 1.0)):
+    """furnace-sky (tests/witnesses/scenes/envScenes.ts:69-120): a Lambert sphere (albedo 0.4)
+    under a uniform constant sky L = 1, sampled as a light. A convex body never re-illuminates
+    itself, so the sphere's pixels are rho L = 0.4 and the sky's pixels are L = 1 exactly, for
+    every estimator."""
+    return {
+        "id": "furnace-sky",
+        "materials": {"gray": {"model": "lambert", "albedo": [0.4, 0.4, 0.4]}},
+        "objects": [{"type": "sphere", "center": [0.0, 0.0, 0.0], "radius": 1.0, "material": "gray"}],
+        "lights": [],
+        "environment": {"type": "constant", "color": list(color), "intensity": 1.0, "sample_as_light": sample_as_light},
+        "camera": {"type": "pinhole", "position": [0.0, 0.0, 3.5], "target": [0.0, 0.0, 0.0], "fov": 0.8},
+    }
+
+
+def orthographic(sc, scale=0.5):
+    """The scene seen through the reference's orthographic camera (u_orthoScale = scale), as the
+    media witnesses pose it (tests/witnesses/scenes/spectralMedia.ts:10-16)."""
+    sc = dict(sc)
+    cam = dict(sc["camera"])
+    cam.pop("fov", None)
+    cam.update({"type": "orthographic", "scale": scale})
+    sc["camera"] = cam
+    sc["id"] = sc["id"] + "-ortho"
+    return sc

@@ -133,3 +133,22 @@ def test_delta_majorant_follows_material_updates(engine, oracle):
     o, _, _ = oracle_lib.OracleScene(oracle, comp2.scene).render(comp2.program, comp2.params(), 8)
     rel = np.abs(g[..., :3] - o[..., :3]) / np.maximum(np.abs(o[..., :3]), 1e-6)
     assert (rel <= REL_TOL).all()
+
+
+def test_orthographic_media_camera_inside_and_outside(engine, oracle):
+    """Orthographic rays do not share an origin: each camera ray's medium is its own
+    region_at (the glass-medium sphere straddles part of the film plane)."""
+    sc = scenes.cornell_glass_medium()
+    sc["camera"] = {"type": "orthographic", "scale": 0.9, "position": [0.0, 0.0, 0.3], "target": [0.0, 0.0, -2.0]}
+    sc["id"] = "cornell-glass-medium-ortho-inside"
+    _, o, bad = _render_both(engine, oracle, "glass-medium-ortho", sc, compiler.RenderStrategy("mis", 12, 3), 32, 32, 8)
+    assert bad == 0.0
+
+
+@pytest.mark.parametrize("color", ["rgb", "spectral"])
+def test_orthographic_slab_and_glow_match_oracle(engine, oracle, color):
+    sc = scenes.orthographic(scenes.emit_slab() if color == "rgb" else
+                             scenes.emit_slab(sigma_a=[0.3, 0.7, 1.2], emission=[0.4, 0.8, 1.2]))
+    _, o, bad = _render_both(engine, oracle, f"ortho-glow-{color}", sc, compiler.RenderStrategy("none", 4, -1, color=color),
+                             24, 24, 8)
+    assert bad == 0.0

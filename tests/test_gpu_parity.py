@@ -212,3 +212,32 @@ def test_smooth_instances_and_glass_mesh_match_oracle(engine, oracle, est):
     assert o[..., :3].mean() > 0.01
     assert bad == 0.0
     assert engine.ray_counts() == (int(counts[0]), int(counts[1]))
+
+
+@pytest.mark.parametrize("est,samplable", [("mis", True), ("nee", True), ("none", True), ("mis", False)])
+def test_constant_sky_matches_oracle(engine, oracle, est, samplable):
+    comp = scene_cache.compiler.compile_scene(scenes.furnace_sky(samplable), scene_cache.compiler.RenderStrategy(est, 12, 3),
+                                              32, 32)
+    engine.load_renderer(comp)
+    engine.clear_accumulation()
+    engine.render_frames(8)
+    g = engine.read_export()
+    o, _, counts = oracle_lib.OracleScene(oracle, comp.scene).render(comp.program, comp.params(), 8)
+    exact, bad = _compare_film(f"furnace-sky-{est}-{samplable}", g, o)
+    assert bad == 0.0
+    assert engine.ray_counts() == (int(counts[0]), int(counts[1]))
+
+
+def test_orthographic_cornell_with_constant_sky_matches_oracle(engine, oracle):
+    sc = scenes.cornell_box()
+    sc["environment"] = {"type": "constant", "color": [0.2, 0.3, 0.5], "intensity": 2.0, "sample_as_light": True}
+    sc = scenes.orthographic(sc, 1.3)
+    comp = scene_cache.compiler.compile_scene(sc, scene_cache.compiler.RenderStrategy("mis", 8, 3), 40, 32)
+    engine.load_renderer(comp)
+    engine.clear_accumulation()
+    engine.render_frames(8)
+    g = engine.read_export()
+    o, _, counts = oracle_lib.OracleScene(oracle, comp.scene).render(comp.program, comp.params(), 8)
+    exact, bad = _compare_film("cornell-ortho-sky", g, o)
+    assert o[..., :3].mean() > 0.05
+    assert bad == 0.0

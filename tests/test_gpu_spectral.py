@@ -79,3 +79,11 @@ def test_dispersive_prism_matches_oracle(engine, oracle, color, ior):
     bad, _ = _compare(engine, oracle, f"prism-{color}-{ior}", sc,
                       compiler.RenderStrategy("none", 12, 3, color=color), 32, 24, 8)
     assert bad == 0.0
+
+
+def test_spectral_constant_sky_glass_medium_matches_oracle(engine, oracle):
+    sc = scenes.cornell_spectral_medium()
+    sc["environment"] = {"type": "constant", "color": [0.4, 0.8, 1.2], "intensity": 1.5, "sample_as_light": True}
+    bad, _ = _compare(engine, oracle, "spectral-constant-sky", sc, compiler.RenderStrategy("mis", 8, 3, color="spectral"),
+                      24, 24, 8)
+    assert bad == 0.0
