@@ -297,6 +297,13 @@ pt_status pt_renderer_load(pt_ctx* ctx, const pt_program* program, pt_renderer**
  * backend implements are lowered to a pt_program; a field selecting a component it does not
  * implement fails with PT_E_UNSUPPORTED and pt_last_error() names the field and value, as the
  * reference's Validator names a rejected option. pt_program_from_description needs no device. */
+/* The importance table's luminance mass of a tabulated environment: sum of the blurred
+ * (and, when compensated, compensated) luminance times sin(theta) over the W x H cells, the
+ * `totalWeight` of src/components/env/importance.ts:100-127 the reference ships as the
+ * env.totalWeight parameter (the derived env selection probability reads it). 0 for
+ * untabulated environments. Needs no device. */
+double pt_env_total_weight(const pt_env* env);
+
 pt_status pt_program_from_description(const char* program_json, size_t len, int32_t width, int32_t height,
                                       int32_t frames_in_flight, pt_program* out);
 pt_status pt_renderer_load_description(pt_ctx* ctx, const char* program_json, size_t len, int32_t width, int32_t height,

@@ -16,7 +16,7 @@ DECLARED = ["pt_last_error", "pt_ctx_create", "pt_ctx_destroy", "pt_scene_upload
             "pt_scene_update_materials", "pt_forward", "pt_backward",
             "pt_batch_create", "pt_batch_destroy", "pt_batch_set_config", "pt_batch_get_config",
             "pt_batch_render", "pt_batch_read_export", "pt_batch_autotune",
-            "pt_program_from_description", "pt_renderer_load_description"]
+            "pt_program_from_description", "pt_renderer_load_description", "pt_env_total_weight"]
 
 _lib = None
 
@@ -74,6 +74,8 @@ def load():
     lib.pt_batch_autotune.argtypes = [vp, vp, u32, C.POINTER(abi.PtBatchConfig), vp, vp, i32, C.POINTER(i32)]
     lib.pt_program_from_description.argtypes = [C.c_char_p, C.c_size_t, i32, i32, i32, C.POINTER(abi.PtProgram)]
     lib.pt_renderer_load_description.argtypes = [vp, C.c_char_p, C.c_size_t, i32, i32, i32, C.POINTER(vp)]
+    lib.pt_env_total_weight.argtypes = [C.POINTER(abi.PtEnv)]
+    lib.pt_env_total_weight.restype = C.c_double
     lib.pt_abi_sizeof.argtypes = [C.c_int]
     lib.pt_abi_sizeof.restype = C.c_size_t
     _ = (i32, st)

@@ -141,6 +141,38 @@ def index_record(value, spectral_program, label):
     return float(value), abi.IOR_CONSTANT, coef
 
 
+def scene_radius_estimate(scene):
+    """sceneRadiusEstimate (features/lighting.ts:459-474): max(1, |point| + sum of lengths) over
+    the primitive objects (sphere: centre + radius; quad: corner) and the lights (quad corner,
+    point position); meshes are not primitive objects and do not count."""
+    r = 1.0
+    for o in scene.get("objects", []):
+        if o["type"] == "sphere":
+            r = max(r, math.hypot(*o["center"]) + float(o["radius"]))
+        elif o["type"] == "quad":
+            r = max(r, math.hypot(*o["corner"]))
+    for l in scene.get("lights", []):
+        r = max(r, math.hypot(*(l["corner"] if l["type"] == "quad" else l["position"])))
+    return r
+
+
+def env_select_probability(scene, env, light_powers):
+    """envSelectionProbability, 'power' selection (features/lighting.ts:488-518): P(env) =
+    Phi_env / (Phi_env + sum Phi_light), Phi_env = 4 pi^2 r^2 mean(L) intensity, clamped to
+    [0.01, 0.99]. mean(L): the constant colour's channel mean, or the importance table's mass
+    totalWeight pi / (2 W H) for the tabulated sky (equirect chart)."""
+    if env.type == abi.ENV_CONSTANT:
+        mean_l = (env.color[0] + env.color[1] + env.color[2]) / 3.0
+    else:
+        from . import _lib
+        total = _lib.load().pt_env_total_weight(env)
+        mean_l = total * math.pi / (2.0 * env.table_w * env.table_h) if total > 0 else 1.0
+    r = scene_radius_estimate(scene)
+    phi_env = 4.0 * math.pi * math.pi * r * r * mean_l * env.intensity
+    phi_fin = sum(light_powers)
+    return min(0.99, max(0.01, phi_env / max(phi_env + phi_fin, 1e-20)))
+
+
 def _rgb3(v):
     return [float(v)] * 3 if not hasattr(v, "__len__") else [float(x) for x in v]
 
@@ -409,7 +441,7 @@ def compile_scene(scene, strategy=None, width=256, height=256, reset_salt=1234):
             rec["kind"] = abi.LIGHT_POINT
             rec["region"] = -1
             rec["corner"][:3] = l["position"]
-            power = 4.0 * math.pi * float(rad.mean())
+            power = max(1e-8, 4.0 * math.pi * float(rad.mean()))
         lights.append((rec, power))
     total = sum(p for _, p in lights)
     acc = 0.0
@@ -434,6 +466,9 @@ def compile_scene(scene, strategy=None, width=256, height=256, reset_salt=1234):
         for i, c in enumerate(spectral.rgb_of(col)):
             env.color[i] = float(c)
         env.s_color = abi.PtSpectrum.of(srec(col, spectral.ROLE_EMISSION))
+
+    if env.type != abi.ENV_NONE and (env.type == abi.ENV_SKY or env.samplable) and light_recs and "select_prob" not in e:
+        env.select_prob = env_select_probability(scene, env, [p for _, p in lights])
 
     tv = np.concatenate(tri_chunks) if tri_chunks else np.zeros((0, 9), np.float32)
     to = np.concatenate(tri_obj_chunks) if tri_obj_chunks else np.zeros(0, np.int32)

@@ -1141,6 +1141,14 @@ extern "C" {
 
 const char* pt_last_error(void) { return g_err.c_str(); }
 
+double pt_env_total_weight(const pt_env* env) {
+    if (!env || env->type != PT_ENV_SKY) return 0.0;
+    std::vector<float> cond, marg;
+    double total = 0.0;
+    build_env_table(*env, cond, marg, total);
+    return total;
+}
+
 pt_status pt_program_from_description(const char* json, size_t len, int32_t width, int32_t height, int32_t frames_in_flight,
                                       pt_program* out) {
     if (!json || !out) return fail(PT_E_INVALID, "pt_program_from_description: null argument");

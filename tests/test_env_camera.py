@@ -66,3 +66,33 @@ def test_spectral_constant_sky_is_white(oracle):
     sc["objects"] = []
     _, film, _ = _render(oracle, sc, compiler.RenderStrategy("none", 2, -1, color="spectral"), 16, 16, 256)
     np.testing.assert_allclose(film[..., :3].reshape(-1, 3).mean(axis=0), [1.0, 1.0, 1.0], atol=0.005)
+
+
+def test_env_selection_probability_is_the_power_partition():
+    """envSelectionProbability (features/lighting.ts:476-518), worked by hand: unit sphere at the
+    origin, quad light at corner (0, 2, 0) with unit area and Le = 3 (Phi = 3 pi), constant sky
+    L = 1: r = 2, Phi_env = 4 pi^2 r^2 = 16 pi^2, P = 16 pi / (16 pi + 3)."""
+    sc = scenes.furnace_sky(True)
+    sc["lights"] = [{"type": "quad", "corner": [0.0, 2.0, 0.0], "edge1": [1.0, 0.0, 0.0], "edge2": [0.0, 0.0, 1.0],
+                     "radiance": [3.0, 3.0, 3.0]}]
+    comp = compiler.compile_scene(sc, compiler.RenderStrategy("mis", 4, -1), 8, 8)
+    assert abs(comp.scene.env.select_prob - 16 * np.pi / (16 * np.pi + 3)) < 1e-6
+    sc["environment"]["select_prob"] = 0.3                          # an authored envSelectWeight wins
+    assert abs(compiler.compile_scene(sc, compiler.RenderStrategy("mis", 4, -1), 8, 8).scene.env.select_prob - 0.3) < 1e-7
+
+
+@pytest.mark.parametrize("est", ["nee", "mis"])
+def test_sky_and_lamp_estimators_agree(oracle, est):
+    """sky-lamp (index.ts:1432-1446): the two-stage env / finite draw with the derived P changes
+    noise, never brightness: pt-nee and pt-mis converge to pt."""
+    sc = scenes.cornell_box()
+    sc["objects"] = sc["objects"][3:]                               # the two boxes, open to the sky
+    sc["environment"] = {"type": "constant", "color": [0.3, 0.4, 0.6], "intensity": 1.0, "sample_as_light": True}
+    means = {}
+    for e in ("none", est):
+        comp = compiler.compile_scene(sc, compiler.RenderStrategy(e, 6, -1), 24, 24)
+        _, s, _ = oracle_lib.OracleScene(oracle, comp.scene).render(comp.program, comp.params(), 64, keep_samples=True)
+        px = s.mean(axis=(1, 2))
+        means[e] = (px.mean(axis=0), px.std(axis=0) / np.sqrt(len(px)))
+    z = np.abs(means["none"][0] - means[est][0]) / np.sqrt(means["none"][1] ** 2 + means[est][1] ** 2)
+    assert (z < 4.5).all(), (means, z)
