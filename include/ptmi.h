@@ -410,6 +410,47 @@ pt_status pt_forward(pt_renderer* r, const pt_params* params, uint32_t nframes);
 pt_status pt_backward(pt_renderer* r, const pt_params* params, uint32_t nframes, const pt_adjoint_params* adj,
                       const float* dL_dimage, float* dparams);
 
+/* ---------------------------------------------------------------- on-disk inputs (host only)
+ * Readers for the two file formats the reference's authoring layer loads, so a scene can be
+ * assembled without the TypeScript App. Neither needs a device or a context. */
+
+/* Wavefront OBJ text -> indexed mesh (authoring/loadOBJ.ts:62-163, parseOBJ): v / vt / vn / f
+ * records; polygons fan-triangulated; 1-based and negative (relative) indices resolved
+ * against the records read so far; every distinct resolved (v, vt, vn) corner is one output
+ * vertex. Normals and uvs are present only when the file has them. PT_OBJ_SMOOTH_NORMALS is
+ * MeshAuthoring.smoothNormals: area-weighted vertex normals for a file without normals
+ * (authoring/loadOBJ.ts:35-60); corners the file leaves without a normal in a file that has
+ * normals always get them. A file without faces fails with PT_E_INVALID ("parseOBJ: no faces
+ * found ..."). The result is a library-owned pt_mesh_data: query its sizes with
+ * pt_mesh_data_info, copy it out with pt_mesh_data_copy (any output may be NULL), free it with
+ * pt_mesh_data_free. */
+enum { PT_OBJ_SMOOTH_NORMALS = 1 };
+typedef struct pt_mesh_data pt_mesh_data;
+pt_status pt_obj_parse(const char* text, size_t len, uint32_t flags, pt_mesh_data** out);
+pt_status pt_mesh_data_info(const pt_mesh_data* m, uint32_t* n_vertices, uint32_t* n_indices, int32_t* has_normals,
+                            int32_t* has_uvs);
+pt_status pt_mesh_data_copy(const pt_mesh_data* m, float* positions, uint32_t* indices, float* normals, float* uvs);
+void      pt_mesh_data_free(pt_mesh_data* m);
+
+/* A parsed `.inst` instance cloud (authoring/loadInstances.ts:9-22, InstanceTable; layout
+ * tools/inst-format.mjs:8-19, version 1, little-endian). Offsets are byte offsets into the
+ * caller's buffer of 4-byte-aligned f32 blocks: positions 3N, sizes N, colors 3N (linear RGB),
+ * orientations 4N (unit quaternions x, y, z, w), then n_scalars columns of N each; 0 = block
+ * absent (flags bit0 sizes, bit1 colors, bit2 orientations). Column c's name is the
+ * zero-padded 32 bytes at names_offset + 32 c; the provenance is provenance_len UTF-8 bytes.
+ * aabb is the header's positions-only box (min xyz, max xyz). */
+typedef struct pt_inst_table {
+    uint32_t count, flags, n_scalars, pad;
+    float    aabb[6];
+    uint64_t names_offset, provenance_offset, provenance_len;
+    uint64_t positions_offset, sizes_offset, colors_offset, orientations_offset, scalars_offset;
+} pt_inst_table;
+/* parseInstances (authoring/loadInstances.ts:30-98): validates the header (magic, version,
+ * nonzero count, no unknown flag bits, finite non-empty box, column names present and
+ * distinct, byte length equal to the header arithmetic) and fills `out`. Failures are
+ * PT_E_INVALID with the reference's "inst: ..." message. Nothing is copied. */
+pt_status pt_inst_parse(const void* buffer, size_t len, pt_inst_table* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -16,7 +16,8 @@ DECLARED = ["pt_last_error", "pt_ctx_create", "pt_ctx_destroy", "pt_scene_upload
             "pt_scene_update_materials", "pt_forward", "pt_backward",
             "pt_batch_create", "pt_batch_destroy", "pt_batch_set_config", "pt_batch_get_config",
             "pt_batch_render", "pt_batch_read_export", "pt_batch_autotune",
-            "pt_program_from_description", "pt_renderer_load_description", "pt_env_total_weight"]
+            "pt_program_from_description", "pt_renderer_load_description", "pt_env_total_weight",
+            "pt_obj_parse", "pt_mesh_data_info", "pt_mesh_data_copy", "pt_mesh_data_free", "pt_inst_parse"]
 
 _lib = None
 
@@ -76,6 +77,12 @@ def load():
     lib.pt_renderer_load_description.argtypes = [vp, C.c_char_p, C.c_size_t, i32, i32, i32, C.POINTER(vp)]
     lib.pt_env_total_weight.argtypes = [C.POINTER(abi.PtEnv)]
     lib.pt_env_total_weight.restype = C.c_double
+    lib.pt_obj_parse.argtypes = [C.c_char_p, C.c_size_t, u32, C.POINTER(vp)]
+    lib.pt_mesh_data_info.argtypes = [vp, C.POINTER(u32), C.POINTER(u32), C.POINTER(i32), C.POINTER(i32)]
+    lib.pt_mesh_data_copy.argtypes = [vp, vp, vp, vp, vp]
+    lib.pt_mesh_data_free.argtypes = [vp]
+    lib.pt_mesh_data_free.restype = None
+    lib.pt_inst_parse.argtypes = [C.c_char_p, C.c_size_t, C.POINTER(abi.PtInstTable)]
     lib.pt_abi_sizeof.argtypes = [C.c_int]
     lib.pt_abi_sizeof.restype = C.c_size_t
     _ = (i32, st)

@@ -88,6 +88,17 @@ class PtSceneDesc(C.Structure):
                 ("tri_normals", C.c_void_p), ("proto_normals", C.c_void_p)]
 
 
+class PtInstTable(C.Structure):
+    """pt_inst_table: a parsed `.inst` header, byte offsets into the caller's buffer."""
+    _fields_ = [("count", C.c_uint32), ("flags", C.c_uint32), ("n_scalars", C.c_uint32), ("pad", C.c_uint32),
+                ("aabb", C.c_float * 6), ("names_offset", C.c_uint64), ("provenance_offset", C.c_uint64),
+                ("provenance_len", C.c_uint64), ("positions_offset", C.c_uint64), ("sizes_offset", C.c_uint64),
+                ("colors_offset", C.c_uint64), ("orientations_offset", C.c_uint64), ("scalars_offset", C.c_uint64)]
+
+
+OBJ_SMOOTH_NORMALS = 1
+
+
 class PtProgram(C.Structure):
     _fields_ = [("abi_version", C.c_uint32), ("estimator", C.c_int32), ("max_bounces", C.c_int32),
                 ("rr_start", C.c_int32), ("rr_max_survival", C.c_float), ("color", C.c_int32),

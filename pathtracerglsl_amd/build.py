@@ -36,11 +36,13 @@ def build(verbose=False):
     b_obj = os.path.join(OUT_DIR, "builder.o")
     k_obj = os.path.join(OUT_DIR, "ptmi.o")
     d_obj = os.path.join(OUT_DIR, "description.o")
+    f_obj = os.path.join(OUT_DIR, "formats.o")
     _run(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, "builder.cpp"), "-o", b_obj])
     _run(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, "description.cpp"), "-o", d_obj])
+    _run(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, "formats.cpp"), "-o", f_obj])
     _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "ptmi.hip"), "-o", k_obj])
     tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", tmp, k_obj, b_obj, d_obj])
+    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", tmp, k_obj, b_obj, d_obj, f_obj])
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)

@@ -251,6 +251,70 @@ def _analyze_validate(scene, strategy):
             raise CompileError(f"light '{l['type']}' is not supported by this backend")
 
 
+def placement_arrays(placements):
+    """(q, t, s) as f64 arrays [N, 4], [N, 3], [N] of an instance batch's placements: a list of
+    Transform-like records {rotation: unit quaternion x y z w or {axis, angle}, translation or
+    position, scale}, or the packed form {count, positions, orientations?, sizes?} an instance
+    cloud carries (compiler/types.ts PackedPlacements; identity rotation and scale 1 where a
+    column is absent)."""
+    if isinstance(placements, dict):
+        n = int(placements["count"])
+        t = np.asarray(placements["positions"], np.float64).reshape(n, 3)
+        q = (np.asarray(placements["orientations"], np.float64).reshape(n, 4) if placements.get("orientations") is not None
+             else np.tile([0.0, 0.0, 0.0, 1.0], (n, 1)))
+        s = (np.asarray(placements["sizes"], np.float64).reshape(n) if placements.get("sizes") is not None
+             else np.ones(n))
+        return q, t, s
+    n = len(placements)
+    q, t, s = np.zeros((n, 4)), np.zeros((n, 3)), np.ones(n)
+    for i, pl in enumerate(placements):
+        q[i] = rotation_quaternion(pl.get("rotation"))
+        t[i] = pl.get("translation", pl.get("position", [0.0, 0.0, 0.0]))
+        s[i] = float(pl.get("scale", 1.0))
+    return q, t, s
+
+
+def rotation_quaternion(r):
+    """A Transform rotation as a quaternion x y z w (components/geometry/similarity.ts:59-64,
+    218-228): a quaternion as given (normalised by the placement lowering), or axis-angle
+    {axis, angle} as (axis sin(angle/2) / |axis|, cos(angle/2)); identity when absent."""
+    if r is None:
+        return [0.0, 0.0, 0.0, 1.0]
+    if isinstance(r, dict):
+        ax = [float(a) for a in r["axis"]]
+        ln = math.hypot(*ax)
+        h = float(r["angle"]) / 2.0
+        sn = math.sin(h) / ln
+        return [ax[0] * sn, ax[1] * sn, ax[2] * sn, math.cos(h)]
+    return [float(c) for c in r]
+
+
+def indexed_mesh_object(o):
+    """A MeshObject in the reference's indexed form (compiler/types.ts:134-158: object-local
+    positions, indices, optional per-vertex normals, optional transform) as this compiler's
+    triangle form. Without a transform it is a world-space mesh; with one it is walked as one
+    placement of a local-space prototype, as the reference walks a transformed mesh
+    (app/sceneData.ts:72-80). Other objects pass through."""
+    if o.get("type") != "mesh" or "positions" not in o:
+        return o
+    P = np.asarray(o["positions"], np.float32).reshape(-1, 3)
+    I = np.asarray(o["indices"], np.int64).reshape(-1, 3)
+    if I.size and (I.min() < 0 or I.max() >= P.shape[0]):
+        raise CompileError("mesh indices must name vertices of the mesh")
+    out = {"type": "mesh", "triangles": P[I].reshape(-1, 9).astype(np.float64), "material": o["material"],
+           "closed": bool(o.get("closed", False))}
+    if o.get("normals") is not None:
+        N = np.asarray(o["normals"], np.float32).reshape(-1, 3)
+        out["normals"] = N[I].reshape(-1, 9).astype(np.float64)
+    tr = o.get("transform")
+    if tr is not None:
+        out["type"] = "instances"
+        out["placements"] = [{"rotation": rotation_quaternion(tr.get("rotation")),
+                              "translation": list(tr.get("position", [0.0, 0.0, 0.0])),
+                              "scale": float(tr.get("scale", 1.0))}]
+    return out
+
+
 def compile_scene(scene, strategy=None, width=256, height=256, reset_salt=1234):
     """Compile one (scene, strategy) pair into a CompiledRenderer."""
     strategy = strategy or RenderStrategy()
@@ -369,31 +433,30 @@ def compile_scene(scene, strategy=None, width=256, height=256, reset_salt=1234):
         rec["box_min"][:3] = v.min(axis=0)
         rec["box_max"][:3] = v.max(axis=0)
         rec["first_instance"] = len(inst_recs)
-        rec["instance_count"] = len(o["placements"])
         proto_chunks.append(t32)
         pn = corner_normals(o, t32.shape[0])
         proto_nrm_chunks.append(pn if pn is not None else np.zeros_like(t32))
         rec["smooth"] = int(pn is not None)
         n_proto += t32.shape[0]
-        for pl in o["placements"]:
-            q = np.asarray(pl.get("rotation", [0.0, 0.0, 0.0, 1.0]), np.float64)
-            q = q / math.sqrt(float(q @ q))
-            qi = np.array([-q[0], -q[1], -q[2], q[3]])                    # world -> local
-            t = np.asarray(pl.get("translation", [0.0, 0.0, 0.0]), np.float64)
-            s_ = float(pl.get("scale", 1.0))
-            if not s_ > 0.0:
-                raise CompileError("instance scale must be > 0")
-            u, w = qi[:3], qi[3]
-            rt = t + 2.0 * np.cross(u, np.cross(u, t) + w * t)             # R^-1 t
-            ir = np.zeros(1, abi.INSTANCE_DTYPE)[0]
-            ir["q"][:] = qi
-            ir["ts"][:3] = -rt
-            ir["ts"][3] = s_
-            inst_recs.append(ir)
+        q, t, sc = placement_arrays(o["placements"])
+        if not np.all(sc > 0.0):
+            raise CompileError("instance scale must be > 0")
+        qq = q[:, 0] * q[:, 0] + q[:, 1] * q[:, 1] + q[:, 2] * q[:, 2] + q[:, 3] * q[:, 3]
+        q = q / np.sqrt(qq)[:, None]
+        u, w = -q[:, :3], q[:, 3:4]                                    # world -> local: conj(q)
+        rt = t + 2.0 * np.cross(u, np.cross(u, t) + w * t)             # R^-1 t
+        ir = np.zeros(q.shape[0], abi.INSTANCE_DTYPE)
+        ir["q"][:, :3] = u
+        ir["q"][:, 3] = q[:, 3]
+        ir["ts"][:, :3] = -rt
+        ir["ts"][:, 3] = sc
+        inst_recs.extend(ir)
+        rec["instance_count"] = q.shape[0]
         objects.append(rec)
 
     for o in scene.get("objects", []):
         mi = mat_index[o["material"]]
+        o = indexed_mesh_object(o)
         if o["type"] == "instances":
             add_batch(o, mi)
             continue

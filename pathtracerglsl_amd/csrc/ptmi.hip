@@ -25,6 +25,7 @@
 
 #include "builder.h"
 #include "description.h"
+#include "formats.h"
 #include "dev_common.h"
 #include "shading.h"
 #include "traversal.h"
@@ -1162,6 +1163,46 @@ pt_status pt_renderer_load_description(pt_ctx* ctx, const char* json, size_t len
     pt_program pg;
     pt_status st = pt_program_from_description(json, len, width, height, frames_in_flight, &pg);
     return st == PT_OK ? pt_renderer_load(ctx, &pg, out) : st;
+}
+
+pt_status pt_obj_parse(const char* text, size_t len, uint32_t flags, pt_mesh_data** out) {
+    if (!out || (!text && len)) return fail(PT_E_INVALID, "pt_obj_parse: null argument");
+    *out = nullptr;
+    auto* m = new pt_mesh_data();
+    std::string err;
+    pt_status st = ptmi_parse_obj(text ? text : "", len, flags, *m, err);
+    if (st != PT_OK) { delete m; return fail(st, err); }
+    *out = m;
+    return PT_OK;
+}
+
+pt_status pt_mesh_data_info(const pt_mesh_data* m, uint32_t* n_vertices, uint32_t* n_indices, int32_t* has_normals,
+                            int32_t* has_uvs) {
+    if (!m) return fail(PT_E_INVALID, "pt_mesh_data_info: null mesh");
+    if (n_vertices) *n_vertices = (uint32_t)(m->positions.size() / 3);
+    if (n_indices) *n_indices = (uint32_t)m->indices.size();
+    if (has_normals) *has_normals = m->has_normals;
+    if (has_uvs) *has_uvs = m->has_uvs;
+    return PT_OK;
+}
+
+pt_status pt_mesh_data_copy(const pt_mesh_data* m, float* positions, uint32_t* indices, float* normals, float* uvs) {
+    if (!m) return fail(PT_E_INVALID, "pt_mesh_data_copy: null mesh");
+    if (positions) std::memcpy(positions, m->positions.data(), m->positions.size() * 4);
+    if (indices) std::memcpy(indices, m->indices.data(), m->indices.size() * 4);
+    if (normals && m->has_normals) std::memcpy(normals, m->normals.data(), m->normals.size() * 4);
+    if (uvs && m->has_uvs) std::memcpy(uvs, m->uvs.data(), m->uvs.size() * 4);
+    return PT_OK;
+}
+
+void pt_mesh_data_free(pt_mesh_data* m) { delete m; }
+
+pt_status pt_inst_parse(const void* buffer, size_t len, pt_inst_table* out) {
+    if (!out || (!buffer && len)) return fail(PT_E_INVALID, "pt_inst_parse: null argument");
+    std::string err;
+    static const uint8_t empty = 0;
+    pt_status st = ptmi_parse_inst(buffer ? (const uint8_t*)buffer : &empty, len, *out, err);
+    return st == PT_OK ? PT_OK : fail(st, err);
 }
 
 pt_status pt_finish(pt_ctx* ctx) {
@@ -2441,6 +2482,7 @@ size_t pt_abi_sizeof(int which) {
         case 4: return sizeof(pt_scene_desc);
         case 5: return sizeof(pt_program);
         case 6: return sizeof(pt_params);
+        case 7: return sizeof(pt_inst_table);
         default: return 0;
     }
 }

@@ -29,7 +29,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_record_sizes_match_mirrors():
     lib = _lib.load()
-    sizes = [lib.pt_abi_sizeof(i) for i in range(7)]
+    sizes = [lib.pt_abi_sizeof(i) for i in range(8)]
     assert sizes[0] == abi.OBJECT_DTYPE.itemsize
     assert sizes[1] == abi.MATERIAL_DTYPE.itemsize
     assert sizes[2] == abi.LIGHT_DTYPE.itemsize
@@ -37,6 +37,7 @@ def test_record_sizes_match_mirrors():
     assert sizes[4] == C.sizeof(abi.PtSceneDesc)
     assert sizes[5] == C.sizeof(abi.PtProgram)
     assert sizes[6] == C.sizeof(abi.PtParams)
+    assert sizes[7] == C.sizeof(abi.PtInstTable)
 
 
 def test_null_arguments_are_errors_not_crashes():
