@@ -108,6 +108,7 @@ struct DevScene {
     float env_intensity, env_rotation, env_rot_c, env_rot_s, env_select_prob;
     const float* env_cond;
     const float* env_marg;
+    const double* env_total;   // the table's mass (totalWeight), on the device
     // containment (closed meshes)
     const float4* obj_nodes;    // binary nodes, 2 float4 each: (min, A) (max, B)
     const int* obj_node_base;   // per object

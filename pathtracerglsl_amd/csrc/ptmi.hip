@@ -1031,6 +1031,123 @@ pt_status fail(pt_status s, const std::string& msg) { g_err = msg; return s; }
                         std::string(#expr) + ": " + hipGetErrorString(e_));                 \
     } while (0)
 
+// ---------------------------------------------------------------- environment bake
+// The procedural sky's importance table on the device. The reference renders the environment
+// into a W x H equirect target (compiler/EnvironmentBake.ts:29-80) and builds the CDFs on the CPU
+// (components/env/importance.ts:16-127). Here every step runs on the GPU, in the order and
+// precision of the host builder (builder.cpp build_env_table), so the tables are bit-identical:
+// texels and blur taps are independent (one thread each), and the sums the reference
+// accumulates sequentially in f64 stay sequential (one thread per row for the conditional CDFs,
+// one thread for the marginal CDF and for the compensation mean).
+// Per-texel kernels are grid-stride (the launch grid is capped at MAX_GRID blocks).
+__global__ void __launch_bounds__(256) k_env_luma(float* Y, int W, int H, float sx, float sy, float sz) {
+    for (int k = blockIdx.x * 256 + threadIdx.x; k < W * H; k += gridDim.x * 256) {
+        int i = k % W, j = k / W;
+        float u = ((float)i + 0.5f) / (float)W, v = ((float)j + 0.5f) / (float)H;
+        float phi = (u - 0.5f) * PTM_TWO_PI - 0.0f;       // env_chart_dir, the bake's rotation 0
+        float theta = v * PTM_PI;
+        float st = ptm_sin(theta);
+        float dx = ptm_cos(phi) * st, dy = ptm_cos(theta), dz = ptm_sin(phi) * st;
+        float s = 0.5f * (dy + 1.0f);
+        float kk = ptm_smoothstep(0.9994f, 0.9999f, dx * sx + dy * sy + dz * sz);
+        float cx = 0.06f * (1.0f - s) + 0.35f * s + 18.0f * kk;
+        float cy = 0.08f * (1.0f - s) + 0.45f * s + 16.0f * kk;
+        float cz = 0.12f * (1.0f - s) + 0.65f * s + 13.0f * kk;
+        Y[k] = (float)(0.2126 * (double)cx + 0.7152 * (double)cy + 0.0722 * (double)cz);
+    }
+}
+
+// 3 x 3 tent, u wraps (the phi seam), v clamps (the poles).
+__global__ void __launch_bounds__(256) k_env_blur(const float* __restrict__ Y, float* __restrict__ B, int W, int H) {
+    for (int k = blockIdx.x * 256 + threadIdx.x; k < W * H; k += gridDim.x * 256) {
+        int i = k % W, j = k / W;
+        double sum = 0.0, wsum = 0.0;
+        for (int dj = -1; dj <= 1; dj++) {
+            int jj = min(H - 1, max(0, j + dj));
+            for (int di = -1; di <= 1; di++) {
+                int ii = (i + di + W) % W;
+                double w = (dj == 0 ? 2 : 1) * (di == 0 ? 2 : 1);
+                sum += w * (double)Y[(size_t)jj * W + ii];
+                wsum += w;
+            }
+        }
+        B[k] = (float)(sum / wsum);
+    }
+}
+
+// MIS compensation mean: sum_j sum_i B dOmega_j / sum dOmega, accumulated by thread 0 in texel
+// order; the block stages whole rows through LDS, 8192 texels (32 KB) at a time. One block of
+// 256 threads; W <= PT_ENV_MAX_W.
+constexpr int PT_ENV_MAX_W = 4096;
+constexpr int PT_ENV_STAGE = 8192;
+__global__ void __launch_bounds__(256) k_env_mean(const float* __restrict__ B, const double* __restrict__ dO, int W, int H,
+                                                  double* mean) {
+    __shared__ __align__(16) float stage[PT_ENV_STAGE];
+    const int rows = PT_ENV_STAGE / W;                   // >= 2 rows per stage
+    double num = 0.0, den = 0.0;
+    for (int j0 = 0; j0 < H; j0 += rows) {
+        const int nr = min(rows, H - j0);
+        for (int i = threadIdx.x; i < nr * W; i += 256) stage[i] = B[(size_t)j0 * W + i];
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int r = 0; r < nr; r++) {
+                double d = dO[j0 + r];
+                const float* row = stage + r * W;
+                int i = 0;
+                // 16 texels per step: four 16-byte LDS reads issued ahead of the dependent adds
+                // (rows start 16-byte aligned when W is a multiple of 4)
+                for (; (W & 3) == 0 && i + 16 <= W; i += 16) {
+                    float4 q[4];
+                    for (int m = 0; m < 4; m++) q[m] = *(const float4*)(row + i + 4 * m);
+                    for (int m = 0; m < 4; m++) {
+                        num += (double)q[m].x * d; den += d;
+                        num += (double)q[m].y * d; den += d;
+                        num += (double)q[m].z * d; den += d;
+                        num += (double)q[m].w * d; den += d;
+                    }
+                }
+                for (; i < W; i++) { num += (double)row[i] * d; den += d; }
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *mean = den > 0.0 ? num / den : 0.0;
+}
+
+__global__ void __launch_bounds__(256) k_env_compensate(float* B, int n, const double* mean) {
+    for (int k = blockIdx.x * 256 + threadIdx.x; k < n; k += gridDim.x * 256) {
+        double t = (double)B[k] - *mean;
+        B[k] = (float)(t > 0 ? t : 0);
+    }
+}
+
+// Conditional CDF of row j (thread j) and the row's mass.
+__global__ void __launch_bounds__(64) k_env_rows(const float* __restrict__ B, const double* __restrict__ sinT, int W, int H,
+                                                 float* __restrict__ cond, double* __restrict__ rows) {
+    for (int j = blockIdx.x * 64 + threadIdx.x; j < H; j += gridDim.x * 64) {
+        const float* r = B + (size_t)j * W;
+        double st = sinT[j], sum = 0.0;
+        for (int i = 0; i < W; i++) { double y = (double)r[i]; sum += (0.0 < y ? y : 0.0) * st; }
+        rows[j] = sum;
+        double c = 0.0;
+        for (int i = 0; i < W; i++) {
+            double y = (double)r[i];
+            c += (0.0 < y ? y : 0.0) * st;
+            cond[(size_t)j * W + i] = (float)(sum > 0 ? c / sum : 0);
+        }
+    }
+}
+
+// Marginal CDF over rows and the table's total mass (the env.totalWeight parameter).
+__global__ void k_env_marginal(const double* __restrict__ rows, int H, float* __restrict__ marg, double* total_out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double total = 0.0;
+    for (int j = 0; j < H; j++) total += rows[j];
+    double acc = 0.0;
+    for (int j = 0; j < H; j++) { acc += rows[j]; marg[j] = (float)(total > 0 ? acc / total : 0); }
+    *total_out = total;
+}
+
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
@@ -1061,6 +1178,14 @@ struct SceneStore {
         HIP_TRY(hipMalloc(&b->p, b->bytes));
         if (n) HIP_TRY(hipMemcpyAsync(b->p, host, n * sizeof(T), hipMemcpyHostToDevice, stream));
         *out = (const T*)b->p;
+        bufs.push_back(std::move(b));
+        return PT_OK;
+    }
+    template <class T> pt_status alloc(size_t n, T** out) {
+        auto b = std::make_unique<DevBuf>();
+        b->bytes = std::max<size_t>(n, 1) * sizeof(T);
+        HIP_TRY(hipMalloc(&b->p, b->bytes));
+        *out = (T*)b->p;
         bufs.push_back(std::move(b));
         return PT_OK;
     }
@@ -1324,6 +1449,51 @@ static pt_status validate_scene(const pt_scene_desc* sd) {
 }
 
 // Build every device structure of one scene (wide BVH, containment, env table) into SS.
+// The procedural sky's importance table, baked on the device into SS (k_env_* above). The f64
+// row weights sin(theta_j) and texel solid angles are the host builder's expressions, evaluated
+// on the host (H values each). ev0 / ev1, when given, bracket the kernels on SS.stream.
+static pt_status bake_env_table(const pt_env& env, SceneStore& SS, DevScene& D, hipEvent_t ev0 = nullptr,
+                                hipEvent_t ev1 = nullptr) {
+    const int W = env.table_w > 0 ? env.table_w : 512, H = env.table_h > 0 ? env.table_h : 256;
+    if (W > PT_ENV_MAX_W) return fail(PT_E_UNSUPPORTED, "environment table wider than 4096 texels");
+    const long long n = (long long)W * H;
+    std::vector<double> sin_t((size_t)H), d_omega((size_t)H);
+    for (int j = 0; j < H; j++) {
+        sin_t[j] = std::sin(M_PI * (j + 0.5) / H);
+        d_omega[j] = (2 * M_PI / W) * (M_PI / H) * std::sin(M_PI * (j + 0.5) / H);
+    }
+    float sx = 0.4f, sy = 0.35f, sz = 0.2f;               // the sun direction, normalised in f32
+    float d = sx * sx + sy * sy + sz * sz;
+    float inv = 1.0f / sqrtf(d);
+    sx *= inv; sy *= inv; sz *= inv;
+    pt_status st;
+    float *Y, *B, *cond, *marg;
+    double *rows, *aux;
+    const double *d_sin, *d_dom;
+    if ((st = SS.alloc((size_t)n, &Y)) != PT_OK || (st = SS.alloc((size_t)n, &B)) != PT_OK ||
+        (st = SS.alloc((size_t)n, &cond)) != PT_OK || (st = SS.alloc((size_t)H, &marg)) != PT_OK ||
+        (st = SS.alloc((size_t)H, &rows)) != PT_OK || (st = SS.alloc(2, &aux)) != PT_OK ||
+        (st = SS.upload(sin_t.data(), sin_t.size(), &d_sin)) != PT_OK ||
+        (st = SS.upload(d_omega.data(), d_omega.size(), &d_dom)) != PT_OK)
+        return st;
+    if (ev0) HIP_TRY(hipEventRecord(ev0, SS.stream));
+    hipLaunchKernelGGL(k_env_luma, dim3(grid_for(n, 256)), dim3(256), 0, SS.stream, Y, W, H, sx, sy, sz);
+    hipLaunchKernelGGL(k_env_blur, dim3(grid_for(n, 256)), dim3(256), 0, SS.stream, (const float*)Y, B, W, H);
+    if (env.compensated) {
+        hipLaunchKernelGGL(k_env_mean, dim3(1), dim3(256), 0, SS.stream, (const float*)B, d_dom, W, H, aux);
+        hipLaunchKernelGGL(k_env_compensate, dim3(grid_for(n, 256)), dim3(256), 0, SS.stream, B, (int)n,
+                           (const double*)aux);
+    }
+    hipLaunchKernelGGL(k_env_rows, dim3(grid_for(H, 64)), dim3(64), 0, SS.stream, (const float*)B, d_sin, W, H, cond, rows);
+    hipLaunchKernelGGL(k_env_marginal, dim3(1), dim3(64), 0, SS.stream, (const double*)rows, H, marg, aux + 1);
+    if (ev1) HIP_TRY(hipEventRecord(ev1, SS.stream));
+    HIP_TRY(hipGetLastError());
+    D.env_cond = cond;
+    D.env_marg = marg;
+    D.env_total = aux + 1;
+    return PT_OK;
+}
+
 static pt_status build_scene(const pt_scene_desc* sd, hipStream_t stream, SceneStore& SS) {
     SS.stream = stream;
     DevScene& D = SS.dev;
@@ -1564,8 +1734,7 @@ static pt_status build_scene(const pt_scene_desc* sd, hipStream_t stream, SceneS
     // environment
     D.env_type = sd->env.type;
     if (sd->env.type == PT_ENV_SKY) {
-        std::vector<float> cond, marg; double total;
-        build_env_table(sd->env, cond, marg, total);
+        if ((st = bake_env_table(sd->env, SS, D)) != PT_OK) return st;
         D.env_w = sd->env.table_w > 0 ? sd->env.table_w : 512;
         D.env_h = sd->env.table_h > 0 ? sd->env.table_h : 256;
         D.env_compensated = sd->env.compensated;
@@ -1574,8 +1743,6 @@ static pt_status build_scene(const pt_scene_desc* sd, hipStream_t stream, SceneS
         D.env_rot_c = (float)std::cos((double)sd->env.rotation);
         D.env_rot_s = (float)std::sin((double)sd->env.rotation);
         D.env_select_prob = sd->env.select_prob;
-        if ((st = SS.upload(cond.data(), cond.size(), &D.env_cond)) != PT_OK) return st;
-        if ((st = SS.upload(marg.data(), marg.size(), &D.env_marg)) != PT_OK) return st;
     } else if (sd->env.type == PT_ENV_CONSTANT) {
         D.env_intensity = sd->env.intensity;
         D.env_color = make_float4(sd->env.color[0], sd->env.color[1], sd->env.color[2], 0.0f);
@@ -2577,6 +2744,38 @@ pt_status pt_calib_copy(pt_ctx* ctx, size_t bytes, int iters) {
 
 extern "C" {
 // Host-only: the env importance table libptmi builds for `env` (cond W*H, marg H floats).
+// Device bake of `env`'s importance table on `device` (the path pt_scene_upload takes): copies
+// the tables and total out and writes the kernels' time (HIP events) to *ms. Returns a pt_status.
+int pt_debug_env_bake(int device, const pt_env* env, float* cond, float* marg, double* total, float* ms) {
+    if (!env || env->type != PT_ENV_SKY) return fail(PT_E_INVALID, "pt_debug_env_bake: not a tabulated environment");
+    HIP_TRY(hipSetDevice(device));
+    hipStream_t stream;
+    HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    pt_status st;
+    {
+        SceneStore SS;
+        SS.stream = stream;
+        DevScene D{};
+        st = bake_env_table(*env, SS, D, e0, e1);
+        if (st == PT_OK) {
+            const size_t W = env->table_w > 0 ? env->table_w : 512, H = env->table_h > 0 ? env->table_h : 256;
+            HIP_TRY(hipMemcpyAsync(cond, D.env_cond, W * H * 4, hipMemcpyDeviceToHost, stream));
+            HIP_TRY(hipMemcpyAsync(marg, D.env_marg, H * 4, hipMemcpyDeviceToHost, stream));
+            HIP_TRY(hipMemcpyAsync(total, D.env_total, 8, hipMemcpyDeviceToHost, stream));
+            HIP_TRY(hipStreamSynchronize(stream));
+            HIP_TRY(hipEventElapsedTime(ms, e0, e1));
+        }
+        (void)hipStreamSynchronize(stream);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(stream);
+    return st;
+}
+
 int pt_debug_env_table(const pt_env* env, float* cond, float* marg, double* total) {
     std::vector<float> c, m; double t = 0;
     build_env_table(*env, c, m, t);
