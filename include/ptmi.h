@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define PT_ABI_VERSION 4u
+#define PT_ABI_VERSION 5u
 
 typedef enum pt_status {
     PT_OK = 0,
@@ -251,16 +251,20 @@ typedef struct pt_params {
     float    tan_fov;         /* tan(fov/2), host f64 -> f32 */
     float    aspect;          /* width / height */
     uint32_t reset_salt;
-    int32_t  camera;          /* PT_CAM_PINHOLE or PT_CAM_ORTHOGRAPHIC */
+    int32_t  camera;          /* PT_CAM_PINHOLE, PT_CAM_ORTHOGRAPHIC or PT_CAM_THINLENS */
     int32_t  pixel_offset[2];
     float    ortho_scale;     /* orthographic: world half-height of the view (u_orthoScale) */
+    float    aperture;        /* thin lens: lens radius (u_aperture), >= 0; 0 images like the pinhole */
+    float    focus_distance;  /* thin lens: distance of the sharp plane along forward (u_focusDistance), > 0 */
     int32_t  pad1;
 } pt_params;
 
-/* Cameras (src/components/camera/pinhole/pinhole.glsl:4-18, orthographic/orthographic.glsl:6-21):
- * the pinhole spreads directions over tan(fov/2); the orthographic camera shares the forward
- * direction and slides the origin over the film plane. */
-enum { PT_CAM_PINHOLE = 0, PT_CAM_ORTHOGRAPHIC = 1 };
+/* Cameras (src/components/camera/pinhole/pinhole.glsl:4-18, orthographic/orthographic.glsl:6-21,
+ * thinlens/thinlens.glsl:6-33): the pinhole spreads directions over tan(fov/2); the orthographic
+ * camera shares the forward direction and slides the origin over the film plane; the thin lens
+ * moves the pinhole ray's origin over a disk of radius `aperture` (the concentric map of the
+ * lens sample) and re-aims it at the pinhole ray's point on the focus plane. */
+enum { PT_CAM_PINHOLE = 0, PT_CAM_ORTHOGRAPHIC = 1, PT_CAM_THINLENS = 2 };
 
 /* ---- rays for the traversal entry points (SoA) ---- */
 typedef struct pt_rays_soa {

@@ -1973,17 +1973,40 @@ void or_trace_closest_brute(const or_scene* S, const float* rays, int n, float* 
     }
 }
 int or_region_at(const or_scene* S, const float* p) { return region_at(S, V(p[0], p[1], p[2])); }
+/* concentric_disk over n square samples u[2n] -> out[2n] (pins the lens map). */
+static void concentric_disk(float u0, float u1, float* dx, float* dy);
+void or_concentric_disk(const float* u, int n, float* out) {
+    for (int i = 0; i < n; i++) concentric_disk(u[2 * i], u[2 * i + 1], &out[2 * i], &out[2 * i + 1]);
+}
 
 /* camera_generateRay: pinhole (camera/pinhole/pinhole.glsl:4-18) or orthographic
  * (camera/orthographic/orthographic.glsl:6-21: the origin slides over the film plane of
  * half-height u_orthoScale, every ray looks along forward). ndcx already carries the aspect. */
-static void camera_ray(const pt_params* par, v3 O, v3 F, v3 R, v3 U, float ndcx, float ndcy, v3* org, v3* dir) {
+/* concentric_disk (glsl/core/math.glsl:71-78): Shirley-Chiu square-to-disk. */
+static void concentric_disk(float u0, float u1, float* dx, float* dy) {
+    float ox = 2.0f * u0 - 1.0f, oy = 2.0f * u1 - 1.0f;
+    if (ox == 0.0f && oy == 0.0f) { *dx = 0.0f; *dy = 0.0f; return; }
+    float r, th;
+    if (fabsf(ox) > fabsf(oy)) { r = ox; th = (PTM_PI / 4.0f) * (oy / ox); }
+    else { r = oy; th = PTM_PI / 2.0f - (PTM_PI / 4.0f) * (ox / oy); }
+    *dx = r * ptm_cos(th); *dy = r * ptm_sin(th);
+}
+static void camera_ray(const pt_params* par, v3 O, v3 F, v3 R, v3 U, float ndcx, float ndcy, float l0, float l1,
+                       v3* org, v3* dir) {
     if (par->camera == PT_CAM_ORTHOGRAPHIC) {
         *org = add(add(O, muls(R, ndcx * par->ortho_scale)), muls(U, ndcy * par->ortho_scale));
         *dir = F;
     } else {
         *org = O;
         *dir = normalize(add(add(F, muls(R, ndcx * par->tan_fov)), muls(U, ndcy * par->tan_fov)));
+        if (par->camera == PT_CAM_THINLENS) {
+            /* thinlens/thinlens.glsl:6-33: the pinhole ray's point on the focus plane, seen from
+             * a point of the lens disk */
+            v3 focus = add(O, muls(*dir, par->focus_distance / dot(*dir, F)));
+            float lx, ly; concentric_disk(l0, l1, &lx, &ly);
+            *org = add(add(O, muls(R, lx * par->aperture)), muls(U, ly * par->aperture));
+            *dir = normalize(sub(focus, *org));
+        }
     }
 }
 
@@ -2015,7 +2038,7 @@ void or_render(const or_scene* S, const pt_program* prog, const pt_params* par, 
                 rng_t rng; rng_init(&rng, (uint32_t)fx, (uint32_t)fy, n, par->reset_salt);
                 float j0 = rnd(&rng), j1 = rnd(&rng);
                 float filmx = fx + (j0 - 0.5f), filmy = fy + (j1 - 0.5f);
-                (void)rnd(&rng); (void)rnd(&rng);   /* lens sample, consumed by every camera */
+                float l0 = rnd(&rng), l1 = rnd(&rng);   /* lens sample, consumed by every camera */
                 wl_t wl; wl.spec = P.spectral; wl.uniform = P.wl_uniform;
                 wl.lam[0] = wl.lam[1] = wl.lam[2] = wl.lam[3] = 0.0f;
                 if (P.spectral) {                   /* wavelengths_sample (color/spectral/spectral.glsl:1-10) */
@@ -2026,7 +2049,7 @@ void or_render(const or_scene* S, const pt_program* prog, const pt_params* par, 
                 float ndcy = (2.0f * filmy / (float)H) - 1.0f;
                 ndcx *= par->aspect;
                 v3 org, dir;
-                camera_ray(par, O, F, R, U, ndcx, ndcy, &org, &dir);
+                camera_ray(par, O, F, R, U, ndcx, ndcy, l0, l1, &org, &dir);
                 sp Ls = transport_trace(S, &P, org, dir, &rng, &c1, &c2, &wl, NULL);
                 v3 Lr = V(Ls.x, Ls.y, Ls.z);
                 if (P.spectral) {                   /* color_to_pixel (color/spectral/spectral.glsl:27-33) */
@@ -2146,12 +2169,12 @@ void or_adjoint(const or_scene* S, const pt_program* prog, const pt_params* par,
                 rng_t rng; rng_init(&rng, (uint32_t)fx, (uint32_t)fy, (uint32_t)f, par->reset_salt);
                 float j0 = rnd(&rng), j1 = rnd(&rng);
                 float filmx = fx + (j0 - 0.5f), filmy = fy + (j1 - 0.5f);
-                (void)rnd(&rng); (void)rnd(&rng);
+                float l0 = rnd(&rng), l1 = rnd(&rng);
                 float ndcx = (2.0f * filmx / (float)W) - 1.0f;
                 float ndcy = (2.0f * filmy / (float)H) - 1.0f;
                 ndcx *= par->aspect;
                 v3 org, dir;
-                camera_ray(par, O, F, R, U, ndcx, ndcy, &org, &dir);
+                camera_ray(par, O, F, R, U, ndcx, ndcy, l0, l1, &org, &dir);
                 wl_t wl; memset(&wl, 0, sizeof wl);
                 memset(A.D, 0, sizeof A.D);
                 (void)transport_trace(S, &P, org, dir, &rng, &c1, &c2, &wl, &A);

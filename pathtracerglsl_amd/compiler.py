@@ -83,6 +83,10 @@ def camera_params(camera, width, height, reset_salt=1234):
     else:
         prm.camera = abi.CAM_PINHOLE
         prm.tan_fov = float(np.float32(math.tan(camera["fov"] / 2.0)))   # features/camera.ts:54-66
+        if camera.get("type") == "thinlens":                             # u_aperture, u_focusDistance
+            prm.camera = abi.CAM_THINLENS
+            prm.aperture = float(np.float32(camera["aperture"]))
+            prm.focus_distance = float(np.float32(camera.get("focusDistance", camera.get("focus_distance"))))
     prm.aspect = float(np.float32(width / height))
     prm.reset_salt = reset_salt
     return prm
@@ -228,8 +232,14 @@ def _analyze_validate(scene, strategy):
         raise CompileError("mediumLightSampling 'equiangular' does not support media on the null-collision arms; use 'vertex'")
     if strategy.color == "spectral" and scene.get("environment", {"type": "none"}).get("type", "none") == "procedural":
         raise CompileError("spectral programs with a procedural environment are not supported by this backend")
-    if scene.get("camera", {}).get("type", "pinhole") not in ("pinhole", "orthographic"):
+    cam = scene.get("camera", {})
+    if cam.get("type", "pinhole") not in ("pinhole", "orthographic", "thinlens"):
         raise CompileError(f"camera '{scene['camera'].get('type')}' is not supported by this backend")
+    if cam.get("type") == "thinlens":     # camera/thinlens/thinlens.ts:15-16
+        if not float(cam.get("aperture", -1.0)) >= 0.0:
+            raise CompileError("camera 'thinlens': aperture must be nonnegative")
+        if not float(cam.get("focusDistance", cam.get("focus_distance", 0.0))) > 0.0:
+            raise CompileError("camera 'thinlens': focusDistance must be positive")
     mats = scene.get("materials", {})
     for name, m in mats.items():
         if m.get("model") not in MODELS:
@@ -630,7 +640,8 @@ def describe(scene, strategy):
         "materials": {"models": models, "twoSidedShading": False, "materialsReadUv": False},
         "measurement": {
             "ambient": "euclidean",
-            "camera": {"type": "pinhole", **{k: v for k, v in scene["camera"].items() if k != "type"}},
+            "camera": {"type": scene["camera"].get("type", "pinhole"),
+                       **{k: v for k, v in scene["camera"].items() if k != "type"}},
             "color": strategy.color, "maxBounces": strategy.max_bounces,
             "maxNullCrossings": strategy.max_null_crossings, "response": "radiance",
             "scattering": strategy.scattering, "sensor": "cie1931" if strategy.color == "spectral" else "ideal",

@@ -185,8 +185,9 @@ bool lower(const JVal& d, int32_t width, int32_t height, int32_t fif, pt_program
     if (const JVal* aov = m->get("aovs"); aov && aov->kind == JVal::ARR && !aov->arr.empty())
         return L.unsupported("measurement.aovs", "(first-hit guides)");
     const JVal* cam = m->get("camera");
-    // pinhole and orthographic (the pt_params camera); the lens draws are consumed either way
-    if (!cam || !(is_str(cam->get("type"), "pinhole") || is_str(cam->get("type"), "orthographic")))
+    // pinhole, orthographic and thin lens (the pt_params camera); the lens draws are consumed either way
+    if (!cam || !(is_str(cam->get("type"), "pinhole") || is_str(cam->get("type"), "orthographic") ||
+                  is_str(cam->get("type"), "thinlens")))
         return L.unsupported("measurement.camera.type", quoted(cam ? cam->get("type") : nullptr));
 
     // estimator (compiler/types.ts:642-721)

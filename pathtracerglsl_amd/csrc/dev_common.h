@@ -31,6 +31,15 @@ PT_DEV float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 PT_DEV f3 cross(f3 a, f3 b) { return mk3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
 PT_DEV float length(f3 a) { return sqrtf(dot(a, a)); }
 PT_DEV f3 normalize(f3 a) { return a * (1.0f / sqrtf(dot(a, a))); }
+// concentric_disk (glsl/core/math.glsl:71-78): the Shirley-Chiu concentric square-to-disk map.
+PT_DEV float2 concentric_disk(float u0, float u1) {
+    const float ox = 2.0f * u0 - 1.0f, oy = 2.0f * u1 - 1.0f;
+    if (ox == 0.0f && oy == 0.0f) return make_float2(0.0f, 0.0f);
+    float r, th;
+    if (fabsf(ox) > fabsf(oy)) { r = ox; th = (PTM_PI / 4.0f) * (oy / ox); }
+    else { r = oy; th = PTM_PI / 2.0f - (PTM_PI / 4.0f) * (ox / oy); }
+    return make_float2(r * ptm_cos(th), r * ptm_sin(th));
+}
 PT_DEV float gmax(float a, float b) { return a > b ? a : b; }   // GLSL max on non-NaN input
 PT_DEV float gmin(float a, float b) { return a < b ? a : b; }
 PT_DEV float gclamp(float x, float lo, float hi) { return gmin(gmax(x, lo), hi); }

@@ -173,6 +173,8 @@ struct Camera {
     int off_x, off_y;
     int ortho;          // PT_CAM_ORTHOGRAPHIC
     float ortho_scale;
+    int thin;           // PT_CAM_THINLENS
+    float aperture, focus;
 };
 
 // path_state_init's current_medium = scene_region_at(camera origin), once per wave.
@@ -202,7 +204,7 @@ PT_DEV void raygen_path(const Paths& P, const Camera& C, int W, int H, int i, in
     r.dim = 0;
     float j0 = r.next(), j1 = r.next();
     float filmx = fx + (j0 - 0.5f), filmy = fy + (j1 - 0.5f);   // pixel/box/box.glsl:5-7
-    r.dim += 2;                                                   // lens sample (unused by a pinhole)
+    float l0 = r.next(), l1 = r.next();                           // lens sample (read by the thin lens)
     if (SPEC) {                                                   // wavelengths_sample (spectral.glsl:1-10)
         float u = r.next();
         float l[4];
@@ -222,6 +224,12 @@ PT_DEV void raygen_path(const Paths& P, const Camera& C, int W, int H, int i, in
         d = F;
     } else {
         d = normalize((F + R * (ndcx * C.tan_fov)) + U * (ndcy * C.tan_fov));
+        if (C.thin) {                                              // camera/thinlens/thinlens.glsl:6-33
+            f3 focus = org + d * (C.focus / dot(d, F));
+            float2 lens = concentric_disk(l0, l1);
+            org = (org + R * (lens.x * C.aperture)) + U * (lens.y * C.aperture);
+            d = normalize(focus - org);
+        }
     }
     P.ray_o[i] = f4(org, 0.0f);
     P.ray_d[i] = f4(d, 0.0f);
@@ -1883,7 +1891,7 @@ static void launch_wave(pt_renderer* r, const Camera& cam, int frames, uint32_t 
     hipStream_t st = ctx->stream;
     if (MEDIA) hipLaunchKernelGGL((k_camera_medium<STACK>), dim3(1), dim3(64), 0, st, S, P, cam);
     TIMED(ctx, "raygen", n,  hipLaunchKernelGGL((k_raygen<SPEC, ADJ>), dim3((unsigned)((n + SBLOCK - 1) / SBLOCK)), dim3(SBLOCK), 0, st, P, cam, W, H, (int)n, sample0, MEDIA ? 1 : 0, pg.wavelengths));
-    if (MEDIA && cam.ortho) hipLaunchKernelGGL((k_ray_medium<STACK>), dim3(ctx->shade_blocks), dim3(SBLOCK), 0, st, S, P, (int)n);
+    if (MEDIA && (cam.ortho || cam.thin)) hipLaunchKernelGGL((k_ray_medium<STACK>), dim3(ctx->shade_blocks), dim3(SBLOCK), 0, st, S, P, (int)n);
     int cur = 0;
     const int tgrid = ctx->persist_blocks;
     // a null crossing consumes an iteration but no bounce (pt.ts null-interface branch)
@@ -1934,6 +1942,16 @@ extern "C" {
 
 }  // extern "C"
 
+// The camera models pt_params can select, with the thin lens's parameter domain
+// (camera/thinlens/thinlens.ts:15-16: aperture nonnegative, focusDistance positive).
+static pt_status check_camera(const pt_params* prm) {
+    if (prm->camera == PT_CAM_PINHOLE || prm->camera == PT_CAM_ORTHOGRAPHIC) return PT_OK;
+    if (prm->camera != PT_CAM_THINLENS) return fail(PT_E_UNSUPPORTED, "unknown camera");
+    if (!(prm->aperture >= 0.0f) || !(prm->focus_distance > 0.0f) || !std::isfinite(prm->aperture) || !std::isfinite(prm->focus_distance))
+        return fail(PT_E_INVALID, "thin lens: aperture must be >= 0 and focus_distance > 0");
+    return PT_OK;
+}
+
 static Camera make_camera(const pt_params* prm) {
     Camera cam;
     cam.ox = prm->cam_position[0]; cam.oy = prm->cam_position[1]; cam.oz = prm->cam_position[2];
@@ -1944,6 +1962,9 @@ static Camera make_camera(const pt_params* prm) {
     cam.off_x = prm->pixel_offset[0]; cam.off_y = prm->pixel_offset[1];
     cam.ortho = prm->camera == PT_CAM_ORTHOGRAPHIC ? 1 : 0;
     cam.ortho_scale = prm->ortho_scale;
+    cam.thin = prm->camera == PT_CAM_THINLENS ? 1 : 0;
+    cam.aperture = prm->aperture;
+    cam.focus = prm->focus_distance;
     return cam;
 }
 
@@ -1994,7 +2015,8 @@ pt_status pt_render_frames(pt_renderer* r, const pt_params* prm, uint32_t nframe
     if (!r || !prm) return fail(PT_E_INVALID, "pt_render_frames: null argument");
     pt_ctx* ctx = r->ctx;
     if (!ctx->has_scene) return fail(PT_E_MISSING_INPUT, "missing input: no scene uploaded (pt_scene_upload)");
-    if (prm->camera != PT_CAM_PINHOLE && prm->camera != PT_CAM_ORTHOGRAPHIC) return fail(PT_E_UNSUPPORTED, "unknown camera");
+    pt_status st0;
+    if ((st0 = check_camera(prm)) != PT_OK) return st0;
     HIP_TRY(hipSetDevice(ctx->device));
     const pt_program& pg = r->prog;
     const int F = std::max(1, std::min<int>(pg.frames_in_flight, (int)std::max<uint32_t>(nframes, 1)));
@@ -2020,6 +2042,7 @@ pt_status pt_backward(pt_renderer* r, const pt_params* prm, uint32_t nframes, co
     if (!ctx->has_scene) return fail(PT_E_MISSING_INPUT, "missing input: no scene uploaded (pt_scene_upload)");
     const pt_program& pg = r->prog;
     if (pg.color != PT_COLOR_RGB) return fail(PT_E_UNSUPPORTED, "pt_backward: the adjoint runs RGB programs only");
+    if (pt_status sc = check_camera(prm); sc != PT_OK) return sc;
     if (pg.medium_light_sampling != PT_MLS_VERTEX) return fail(PT_E_UNSUPPORTED, "pt_backward: the adjoint runs vertex medium NEE only");
     for (const pt_material& m : ctx->host_materials)
         if (m.has_medium && (m.medium_arm == PT_MED_DELTA || m.medium_emission[0] != 0.0f || m.medium_emission[1] != 0.0f ||
@@ -2240,6 +2263,8 @@ pt_status pt_batch_create(pt_ctx* ctx, int32_t n, const pt_scene_desc* scenes, c
                           const pt_params* params, pt_batch** out) {
     if (!ctx || !scenes || !programs || !params || !out) return fail(PT_E_INVALID, "pt_batch_create: null argument");
     if (n <= 0) return fail(PT_E_INVALID, "pt_batch_create: no scenes");
+    for (int s = 0; s < n; s++)
+        if (pt_status sc = check_camera(&params[s]); sc != PT_OK) return sc;
     const pt_program& p0 = programs[0];
     for (int s = 0; s < n; s++) {
         const pt_program& p = programs[s];

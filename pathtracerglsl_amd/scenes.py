@@ -661,6 +661,17 @@ def furnace_sky(sample_as_light=True, color=(1.0, 1.0, 1.0)):
     }
 
 
+def thinlens(sc, aperture=0.05, focus_distance=3.0):
+    """The scene seen through the reference's thin-lens camera (camera/thinlens/thinlens.ts:
+    aperture = lens radius, focusDistance along forward), with the scene camera's pose and fov."""
+    sc = dict(sc)
+    cam = dict(sc["camera"])
+    cam.update({"type": "thinlens", "aperture": aperture, "focusDistance": focus_distance})
+    sc["camera"] = cam
+    sc["id"] = sc["id"] + "-thinlens"
+    return sc
+
+
 def orthographic(sc, scale=0.5):
     """The scene seen through the reference's orthographic camera (u_orthoScale = scale), as the
     media witnesses pose it (tests/witnesses/scenes/spectralMedia.ts:10-16)."""

@@ -36,6 +36,8 @@ def load(variant=""):
     lib.or_env_total.restype = C.c_double
     lib.or_env_total.argtypes = [vp]
     lib.or_region_at.argtypes = [vp, vp]
+    lib.or_concentric_disk.argtypes = [vp, C.c_int, vp]
+    lib.or_concentric_disk.restype = None
     lib.or_bsdf_probe.argtypes = [vp, vp, vp, C.c_float, C.c_float, C.c_float, vp]
     lib.or_env_pdf.argtypes = [vp, vp, C.c_int, vp]
     lib.or_env_sample.argtypes = [vp, vp, C.c_int, vp, vp]

@@ -94,6 +94,7 @@ def test_rejection_names_the_field():
     (scenes.emit_sat, compiler.RenderStrategy("nee", 32, -1, volume_sampling="delta-tracking")),
     (scenes.cornell_spectral_medium, compiler.RenderStrategy("mis", 8, 3, color="spectral", wavelengths="uniform")),
     (scenes.icosphere_cloud, compiler.RenderStrategy("mis", 12, 3, 0.95, 4)),
+    (lambda: scenes.thinlens(scenes.cornell_box(), 0.05, 3.0), compiler.RenderStrategy("nee", 6, 2)),
 ])
 def test_own_plans_round_trip(make, strategy):
     sc = make()

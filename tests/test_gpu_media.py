@@ -145,6 +145,19 @@ def test_orthographic_media_camera_inside_and_outside(engine, oracle):
     assert bad == 0.0
 
 
+def test_thinlens_media_camera_straddles_the_medium(engine, oracle):
+    """Thin-lens rays start over the lens disk (radius 0.3 at z = 0.45): the inner part of the
+    disk lies inside the glass-medium sphere, the rim outside, so each camera ray takes its own
+    region_at as its starting medium."""
+    sc = scenes.cornell_glass_medium()
+    sc["camera"] = {"type": "thinlens", "fov": 1.0, "aperture": 0.3, "focusDistance": 1.2,
+                    "position": [0.0, 0.0, 0.45], "target": [0.0, 0.0, -2.0]}
+    sc["id"] = "cornell-glass-medium-thinlens-inside"
+    _, o, bad = _render_both(engine, oracle, "glass-medium-thinlens", sc, compiler.RenderStrategy("mis", 12, 3), 32, 32, 8)
+    assert o[..., :3].mean() > 0.01
+    assert bad == 0.0
+
+
 @pytest.mark.parametrize("color", ["rgb", "spectral"])
 def test_orthographic_slab_and_glow_match_oracle(engine, oracle, color):
     sc = scenes.orthographic(scenes.emit_slab() if color == "rgb" else

@@ -241,3 +241,18 @@ def test_orthographic_cornell_with_constant_sky_matches_oracle(engine, oracle):
     exact, bad = _compare_film("cornell-ortho-sky", g, o)
     assert o[..., :3].mean() > 0.05
     assert bad == 0.0
+
+
+@pytest.mark.parametrize("aperture,focus", [(0.08, 3.2), (0.25, 1.5)])
+def test_thinlens_cornell_matches_oracle(engine, oracle, aperture, focus):
+    sc = scenes.thinlens(scenes.cornell_box(), aperture, focus)
+    comp = scene_cache.compiler.compile_scene(sc, scene_cache.compiler.RenderStrategy("mis", 8, 3), 40, 32)
+    engine.load_renderer(comp)
+    engine.clear_accumulation()
+    engine.render_frames(8)
+    g = engine.read_export()
+    o, _, counts = oracle_lib.OracleScene(oracle, comp.scene).render(comp.program, comp.params(), 8)
+    exact, bad = _compare_film(f"cornell-thinlens-{aperture}-{focus}", g, o)
+    assert o[..., :3].mean() > 0.05
+    assert bad == 0.0
+    assert engine.ray_counts() == (int(counts[0]), int(counts[1]))
