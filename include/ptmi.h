@@ -68,7 +68,9 @@ typedef struct pt_object {
     float   box_max[4];
 } pt_object;
 
-enum { PT_MAT_LAMBERT = 0, PT_MAT_DIELECTRIC = 1, PT_MAT_GGX = 2, PT_MAT_NULL = 3 };
+/* Material models (src/components/materials/): PT_MAT_MIRROR is the perfect specular conductor
+ * (mirror/mirror.glsl): a delta reflection weighted by Schlick's Fresnel of f0, no emission. */
+enum { PT_MAT_LAMBERT = 0, PT_MAT_DIELECTRIC = 1, PT_MAT_GGX = 2, PT_MAT_NULL = 3, PT_MAT_MIRROR = 4 };
 
 /* The arm a homogeneous medium is sampled by (src/compiler/generate/features/materials.ts:675-728,
  * routing predicates src/compiler/types.ts:351-417):

@@ -1017,6 +1017,13 @@ static isample_t surface_sample(const or_scene* S, const pt_material* m, v3 wo, 
             s.pdf = 0.0f;
             s.flags = LOBE_TRANSMISSION | LOBE_DELTA;
         }
+    } else if (m->model == PT_MAT_MIRROR) {        /* materials/mirror/mirror.glsl:21-33 */
+        v3 n = f.n;
+        float cos_i = clampf(dot(wo, n), 1e-6f, 1.0f);
+        s.wi = normalize(sub(muls(n, 2.0f * cos_i), wo));
+        s.weight = schlick(lift(S, m->f0, &m->s_f0, wl), cos_i);
+        s.pdf = 0.0f;
+        s.flags = LOBE_REFLECTION | LOBE_DELTA;
     } else {                                        /* materials/ggx/ggx.glsl:27-46 */
         v3 wol = to_local(f, wo);
         float side = wol.z < 0.0f ? -1.0f : 1.0f;
@@ -1066,7 +1073,7 @@ static float surface_pdf(const pt_material* m, v3 wi, v3 wo, const hit_t* hit) {
     }
     return 0.0f;
 }
-static inline int has_nondelta(const pt_material* m) { return m->model != PT_MAT_DIELECTRIC; }
+static inline int has_nondelta(const pt_material* m) { return m->model != PT_MAT_DIELECTRIC && m->model != PT_MAT_MIRROR; }
 
 /* ---------------- environment (components/env, proc-sky snapshot) ---------------- */
 static v3 sky_radiance_raw(v3 dir) {

@@ -16,7 +16,8 @@ import numpy as np
 from . import abi, spectral
 
 ESTIMATORS = {"none": abi.EST_PT, "pt": abi.EST_PT, "nee": abi.EST_NEE, "mis": abi.EST_MIS}
-MODELS = {"lambert": abi.MAT_LAMBERT, "dielectric": abi.MAT_DIELECTRIC, "ggx": abi.MAT_GGX, "none": abi.MAT_NULL}
+MODELS = {"lambert": abi.MAT_LAMBERT, "dielectric": abi.MAT_DIELECTRIC, "ggx": abi.MAT_GGX, "none": abi.MAT_NULL,
+          "mirror": abi.MAT_MIRROR}
 
 
 class CompileError(ValueError):
@@ -355,7 +356,7 @@ def compile_scene(scene, strategy=None, width=256, height=256, reset_salt=1234):
         albedo = m.get("albedo", [0.8, 0.8, 0.8])
         rec["albedo"][:3] = spectral.rgb_of(albedo)
         rec["s_albedo"] = srec(albedo, spectral.ROLE_REFLECTANCE)
-        em = m.get("emission", [0.0, 0.0, 0.0])
+        em = m.get("emission", [0.0, 0.0, 0.0]) if m["model"] != "mirror" else [0.0, 0.0, 0.0]   # mirror_emission = 0
         rec["emission"][:3] = spectral.rgb_of(em)
         rec["s_emission"] = srec(em, spectral.ROLE_EMISSION)
         rec["emissive"] = int(emits(em))

@@ -42,9 +42,9 @@ constexpr int MAX_GRID = 8192;
 
 // Shading categories: the bin kernel sorts every path of a bounce by what its vertex needs,
 // so each shade launch runs one code path (no material divergence inside a wave).
-enum { CAT_MISS = 0, CAT_LAMBERT = 1, CAT_GGX = 2, CAT_DIELECTRIC = 3, CAT_MEDIUM = 4, CAT_NULL = 5, N_CAT = 6 };
+enum { CAT_MISS = 0, CAT_LAMBERT = 1, CAT_GGX = 2, CAT_DIELECTRIC = 3, CAT_MEDIUM = 4, CAT_NULL = 5, CAT_MIRROR = 6, N_CAT = 7 };
 // counts[] slots
-enum { C_RAYQ0 = 0, C_RAYQ1 = 1, C_SHADOW = 2, C_CAT0 = 3, C_EXT_FETCH = 9, C_SHD_FETCH = 10, C_CAM_MEDIUM = 11,
+enum { C_RAYQ0 = 0, C_RAYQ1 = 1, C_SHADOW = 2, C_CAT0 = 3, C_EXT_FETCH = C_CAT0 + N_CAT, C_SHD_FETCH, C_CAM_MEDIUM,
        N_COUNTS = 16 };
 
 struct Paths {
@@ -600,7 +600,8 @@ __global__ void __launch_bounds__(SBLOCK) k_bin(DevScene Sk, Paths P, int qsel, 
                     int model = S.materials[S.objects[owner].material].model;
                     cat = model == PT_MAT_LAMBERT ? CAT_LAMBERT
                         : model == PT_MAT_GGX ? CAT_GGX
-                        : model == PT_MAT_DIELECTRIC ? CAT_DIELECTRIC : CAT_NULL;
+                        : model == PT_MAT_DIELECTRIC ? CAT_DIELECTRIC
+                        : model == PT_MAT_MIRROR ? CAT_MIRROR : CAT_NULL;
                 }
             }
         }
@@ -649,7 +650,13 @@ template <int EST, int CAT, int STACK, bool MEDIA, bool SPEC, bool ADJ, bool BAT
 __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene Sk, Paths P, int qsel, int max_bounces,
                                                   int rr_start, float rr_max, int max_null, int equiangular = 0) {
     constexpr bool PROBE = CAT == CAT_DIELECTRIC || MEDIA;
-    __shared__ uint2 stk[PROBE && CAT != CAT_MISS && CAT != CAT_MEDIUM ? STACK * SBLOCK : 1];
+    // The region probe's LDS stack: static in the probing categories; the other surface
+    // categories probe only at a back-face hit in a scene with an emissive solid, and are then
+    // launched with the same stack as dynamic LDS (shade_dynamic_lds).
+    constexpr bool STATIC_STK = PROBE && CAT != CAT_MISS && CAT != CAT_MEDIUM;
+    __shared__ uint2 stk_static[STATIC_STK ? STACK * SBLOCK : 1];
+    extern __shared__ uint2 stk_dynamic[];
+    uint2* stk = STATIC_STK ? stk_static : stk_dynamic;
     __shared__ int s_cnt[2], s_base[2];
     int* ctr[2] = {&P.counts[qsel ^ 1], &P.counts[C_SHADOW]};
     const int n = P.counts[C_CAT0 + CAT];
@@ -657,7 +664,8 @@ __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene Sk, Paths P, int qsel
     int* qn = P.q[qsel ^ 1];
     constexpr bool NEE = EST != PT_EST_PT;
     constexpr bool MIS = EST == PT_EST_MIS;
-    constexpr int MODEL = CAT == CAT_LAMBERT ? PT_MAT_LAMBERT : (CAT == CAT_GGX ? PT_MAT_GGX : PT_MAT_DIELECTRIC);
+    constexpr int MODEL = CAT == CAT_LAMBERT ? PT_MAT_LAMBERT : CAT == CAT_GGX ? PT_MAT_GGX
+                        : CAT == CAT_MIRROR ? PT_MAT_MIRROR : PT_MAT_DIELECTRIC;
     for (int base = blockIdx.x * SBLOCK; base < n; base += gridDim.x * SBLOCK) {
         const int k = base + threadIdx.x;
         const bool valid = k < n;
@@ -812,7 +820,7 @@ __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene Sk, Paths P, int qsel
                             collapsed = true;
                         }
                         // light_sample_direct (skipped for delta-only materials)
-                        if (NEE && CAT != CAT_DIELECTRIC) {
+                        if (NEE && CAT != CAT_DIELECTRIC && CAT != CAT_MIRROR) {   // delta lobes take no NEE
                             float x0 = rng.next(), x1 = rng.next();
                             LSample ls = lighting_sample<SPEC>(S, h.p, x0, x1, wl);
                             if (ls.pdf > 0.0f) {
@@ -1178,6 +1186,7 @@ struct SceneStore {
     int stack = 16;
     uint32_t wide_depth = 0;
     bool has_ggx = false;
+    bool has_mirror = false;
     std::vector<pt_material> host_materials;
     hipStream_t stream = nullptr;
     template <class T> pt_status upload(const T* host, size_t n, const T** out) {
@@ -1211,6 +1220,7 @@ struct pt_ctx {
     int cu_count = 256;
     int shade_blocks = 2048;
     bool has_ggx = false;
+    bool has_mirror = false;
     std::vector<pt_material> host_materials;   // the uploaded records (adjoint checks, updates)
     bool timing = false;
     std::map<std::string, TimingSlot> timings;
@@ -1555,9 +1565,11 @@ static pt_status build_scene(const pt_scene_desc* sd, hipStream_t stream, SceneS
     if ((st = SS.upload(sd->lights, (size_t)sd->n_lights, &D.lights)) != PT_OK) return st;
     D.n_lights = sd->n_lights;
     SS.has_ggx = false;
+    SS.has_mirror = false;
     for (int m = 0; m < sd->n_materials; m++) {
         if (sd->materials[m].model == PT_MAT_DIELECTRIC) D.has_transmission = 1;
         if (sd->materials[m].model == PT_MAT_GGX) SS.has_ggx = true;
+        if (sd->materials[m].model == PT_MAT_MIRROR) SS.has_mirror = true;
         if (sd->materials[m].has_medium) D.has_media = 1;
         if (sd->materials[m].model == PT_MAT_NULL) D.has_null = 1;
     }
@@ -1780,6 +1792,7 @@ pt_status pt_scene_upload(pt_ctx* ctx, const pt_scene_desc* sd) {
     ctx->stack = SS.stack;
     ctx->wide_depth = SS.wide_depth;
     ctx->has_ggx = SS.has_ggx;
+    ctx->has_mirror = SS.has_mirror;
     ctx->host_materials = std::move(SS.host_materials);
     ctx->has_scene = true;
     return PT_OK;
@@ -1827,12 +1840,12 @@ void pt_renderer_unload(pt_renderer* r) {
 
 static pt_status ensure_paths(pt_renderer* r, long long n) {
     if (r->cap >= n) return PT_OK;
-    struct { DevBuf* b; size_t elem; } list[] = {
+    struct Entry { DevBuf* b; size_t elem; };
+    std::vector<Entry> list = {
         {&r->b_ray_o, 16}, {&r->b_ray_d, 16}, {&r->b_thr, 16}, {&r->b_rad, 16}, {&r->b_rng, 8}, {&r->b_prev, 16},
         {&r->b_flags, 4}, {&r->b_hit, 8}, {&r->b_q0, 4}, {&r->b_q1, 4}, {&r->b_spath, 4}, {&r->b_so, 16},
-        {&r->b_sd, 16}, {&r->b_sc, 16}, {&r->b_sl, 16}, {&r->b_medium, 4}, {&r->b_wl, 16}, {&r->b_eta, 4}, {&r->b_pscene, 4}, {&r->b_cat[0], 4}, {&r->b_cat[1], 4},
-        {&r->b_cat[2], 4}, {&r->b_cat[3], 4}, {&r->b_cat[4], 4}, {&r->b_cat[5], 4}};
-    static_assert(N_CAT == 6, "one queue buffer per shading category");
+        {&r->b_sd, 16}, {&r->b_sc, 16}, {&r->b_sl, 16}, {&r->b_medium, 4}, {&r->b_wl, 16}, {&r->b_eta, 4}, {&r->b_pscene, 4}};
+    for (int c = 0; c < N_CAT; c++) list.push_back({&r->b_cat[c], 4});   // one queue per shading category
     for (auto& e : list) {
         if (e.b->p) { HIP_TRY(hipFree(e.b->p)); e.b->p = nullptr; }
         e.b->bytes = e.elem * (size_t)n;
@@ -1854,11 +1867,23 @@ static pt_status ensure_paths(pt_renderer* r, long long n) {
     P.medium = (int*)r->b_medium.p; P.wl = (float4*)r->b_wl.p; P.eta = (float*)r->b_eta.p;
     P.path_scene = (int*)r->b_pscene.p; P.hist = (unsigned long long*)r->b_hist.p;
     P.trav = (unsigned long long*)r->b_trav.p;
-    for (int c = 0; c < N_CAT; c++) P.catq[c] = (int*)r->b_cat[c].p;
+    for (int c = 0; c < N_CAT; c++) {
+        P.catq[c] = (int*)r->b_cat[c].p;
+        if (!P.catq[c]) return fail(PT_E_OOM, "ensure_paths: a shading queue was not allocated");
+    }
     P.cap = (int)n;
     P.npix = r->prog.width * r->prog.height;
     r->cap = n;
     return PT_OK;
+}
+
+// Dynamic LDS for a k_shade launch: the region probe's stack for the surface categories
+// without a static one, in scenes whose emissive solids make them probe at back-face hits.
+template <int CAT, bool MEDIA, int STACK>
+static size_t shade_dynamic_lds(bool emissive_solid) {
+    constexpr bool STATIC_STK = (CAT == CAT_DIELECTRIC || MEDIA) && CAT != CAT_MISS && CAT != CAT_MEDIUM;
+    constexpr bool SURFACE = CAT != CAT_MISS && CAT != CAT_MEDIUM;
+    return (!STATIC_STK && SURFACE && emissive_solid) ? (size_t)STACK * SBLOCK * sizeof(uint2) : 0;
 }
 
 template <int EST, int STACK, bool MEDIA, bool SPEC, bool ADJ>
@@ -1869,10 +1894,11 @@ static void launch_shade(pt_renderer* r, int cur) {
     hipStream_t st = ctx->stream;
     const int g = ctx->shade_blocks;
     const int mn = pg.max_null_crossings;
-#define PT_SHADE(name, CATV) TIMED(ctx, name, 0, hipLaunchKernelGGL((k_shade<EST, CATV, STACK, MEDIA, SPEC, ADJ>), dim3(g), dim3(SBLOCK), 0, st, S, r->paths, cur, pg.max_bounces, pg.rr_start, pg.rr_max_survival, mn, pg.medium_light_sampling))
+#define PT_SHADE(name, CATV) TIMED(ctx, name, 0, hipLaunchKernelGGL((k_shade<EST, CATV, STACK, MEDIA, SPEC, ADJ>), dim3(g), dim3(SBLOCK), (shade_dynamic_lds<CATV, MEDIA, STACK>(S.emissive_solid != 0)), st, S, r->paths, cur, pg.max_bounces, pg.rr_start, pg.rr_max_survival, mn, pg.medium_light_sampling))
     PT_SHADE("shade_miss", CAT_MISS);
     PT_SHADE("shade_lambert", CAT_LAMBERT);
     if (ctx->has_ggx) PT_SHADE("shade_ggx", CAT_GGX);
+    if (ctx->has_mirror) PT_SHADE("shade_mirror", CAT_MIRROR);
     if (S.has_transmission) PT_SHADE("shade_dielectric", CAT_DIELECTRIC);
     if (MEDIA) PT_SHADE("shade_medium", CAT_MEDIUM);
     if (S.has_null) PT_SHADE("shade_null", CAT_NULL);
@@ -2125,7 +2151,7 @@ struct pt_batch {
     std::vector<long long> film_off;   // first pixel of each scene's film
     long long film_pixels = 0;
     int max_npix = 0, min_stack = 16;
-    bool any_ggx = false, any_dielectric = false;
+    bool any_ggx = false, any_dielectric = false, any_mirror = false, any_emissive_solid = false;
     DevBuf d_scenes, d_cams, d_entries, films;
     pt_renderer* r = nullptr;          // path slots
     pt_batch_config cfg{};
@@ -2189,10 +2215,11 @@ static void batch_shade(pt_batch* b, int cur) {
     const int g = ctx->cu_count * b->cfg.shade_blocks_per_cu;
     Paths& P = b->r->paths;
     hipStream_t st = ctx->stream;
-#define PT_BSHADE(CATV) hipLaunchKernelGGL((k_shade<EST, CATV, STACK, false, false, false, true>), dim3(g), dim3(SBLOCK), 0, st, S0, P, cur, pg.max_bounces, pg.rr_start, pg.rr_max_survival, pg.max_null_crossings, 0)
+#define PT_BSHADE(CATV) hipLaunchKernelGGL((k_shade<EST, CATV, STACK, false, false, false, true>), dim3(g), dim3(SBLOCK), (shade_dynamic_lds<CATV, false, STACK>(b->any_emissive_solid)), st, S0, P, cur, pg.max_bounces, pg.rr_start, pg.rr_max_survival, pg.max_null_crossings, 0)
     PT_BSHADE(CAT_MISS);
     PT_BSHADE(CAT_LAMBERT);
     if (b->any_ggx) PT_BSHADE(CAT_GGX);
+    if (b->any_mirror) PT_BSHADE(CAT_MIRROR);
     if (b->any_dielectric) PT_BSHADE(CAT_DIELECTRIC);
 #undef PT_BSHADE
 }
@@ -2291,6 +2318,8 @@ pt_status pt_batch_create(pt_ctx* ctx, int32_t n, const pt_scene_desc* scenes, c
         if (st != PT_OK) { delete b; return st; }
         b->min_stack = std::max(b->min_stack, store->stack);
         b->any_ggx = b->any_ggx || store->has_ggx;
+        b->any_mirror = b->any_mirror || store->has_mirror;
+        b->any_emissive_solid = b->any_emissive_solid || store->dev.emissive_solid != 0;
         b->any_dielectric = b->any_dielectric || store->dev.has_transmission;
         devs.push_back(store->dev);
         b->stores.push_back(std::move(store));

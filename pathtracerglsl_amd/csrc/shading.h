@@ -453,6 +453,13 @@ PT_DEV ISample surface_sample(const DevScene& S, const pt_material& m, f3 wo, co
             s.pdf = 0.0f;
             s.flags = LOBE_TRANSMISSION | LOBE_DELTA;
         }
+    } else if (is_model<MODEL>(m, PT_MAT_MIRROR)) {      // materials/mirror/mirror.glsl:21-33
+        f3 n = f.n;
+        float cos_i = gclamp(dot(wo, n), 1e-6f, 1.0f);
+        s.wi = normalize(n * (2.0f * cos_i) - wo);
+        s.weight = schlick(lift<SPEC>(S, m.f0, m.s_f0, wl), cos_i);
+        s.pdf = 0.0f;
+        s.flags = LOBE_REFLECTION | LOBE_DELTA;
     } else {                                    // materials/ggx/ggx.glsl:27-46
         f3 wol = to_local(f, wo);
         float side = wol.z < 0.0f ? -1.0f : 1.0f;

@@ -661,6 +661,28 @@ def furnace_sky(sample_as_light=True, color=(1.0, 1.0, 1.0)):
     }
 
 
+def mirror_furnace(f0=(1.0, 1.0, 1.0)):
+    """furnace-sky with the reference's mirror material on the sphere: one delta reflection per
+    camera ray (a convex mirror never sees itself), weight Schlick(f0, cos), then the L = 1 sky;
+    f0 = 1 renders the sphere as L = 1 exactly."""
+    sc = furnace_sky(True)
+    sc["id"] = "mirror-furnace"
+    sc["materials"] = {"chrome": {"model": "mirror", "f0": list(f0)}}
+    sc["objects"][0]["material"] = "chrome"
+    return sc
+
+
+def cornell_mirror(f0=(0.9, 0.9, 0.9)):
+    """The Cornell box with the tall inner box turned into a mirror mesh and a mirror sphere
+    (radius 0.3) on the floor: delta reflections that see the light and each other."""
+    sc = cornell_box()
+    sc["id"] = "cornell-mirror"
+    sc["materials"]["chrome"] = {"model": "mirror", "f0": list(f0)}
+    sc["objects"][4]["material"] = "chrome"
+    sc["objects"].append({"type": "sphere", "center": [0.45, -0.7, 0.35], "radius": 0.3, "material": "chrome"})
+    return sc
+
+
 def thinlens(sc, aperture=0.05, focus_distance=3.0):
     """The scene seen through the reference's thin-lens camera (camera/thinlens/thinlens.ts:
     aperture = lens radius, focusDistance along forward), with the scene camera's pose and fov."""

@@ -17,7 +17,7 @@ import pytest
 from pathtracerglsl_amd import _lib, abi, compiler, scenes
 
 FIX = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "program_descriptions.json")))
-SUPPORTED_MODELS = {"lambert", "dielectric", "ggx", "none"}
+SUPPORTED_MODELS = {"lambert", "dielectric", "ggx", "none", "mirror"}
 
 
 def _lower(d, w=64, h=48, fif=1):

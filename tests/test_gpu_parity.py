@@ -256,3 +256,40 @@ def test_thinlens_cornell_matches_oracle(engine, oracle, aperture, focus):
     assert o[..., :3].mean() > 0.05
     assert bad == 0.0
     assert engine.ray_counts() == (int(counts[0]), int(counts[1]))
+
+
+@pytest.mark.parametrize("est", ["mis", "nee", "none"])
+def test_mirror_cornell_matches_oracle(engine, oracle, est):
+    comp = scene_cache.compiler.compile_scene(scenes.cornell_mirror(), scene_cache.compiler.RenderStrategy(est, 8, 3), 40, 40)
+    engine.load_renderer(comp)
+    engine.clear_accumulation()
+    engine.render_frames(8)
+    g = engine.read_export()
+    o, _, counts = oracle_lib.OracleScene(oracle, comp.scene).render(comp.program, comp.params(), 8)
+    exact, bad = _compare_film(f"cornell-mirror-{est}", g, o)
+    assert o[..., :3].mean() > 0.05
+    assert bad == 0.0
+    assert engine.ray_counts() == (int(counts[0]), int(counts[1]))
+
+
+@pytest.mark.parametrize("est", ["mis", "none"])
+def test_back_face_probes_with_an_emissive_solid_match_oracle(engine, oracle, est):
+    """The camera sits inside the closed tall box, which holds an emissive Lambert sphere: every
+    camera hit is a back-face hit on a Lambert surface in a scene with an emissive solid, so the
+    Lambert category classifies the region behind it (region_at through the closed meshes' BVHs
+    on its dynamic LDS stack)."""
+    sc = scenes.cornell_box()
+    sc["materials"]["glow"] = {"model": "lambert", "albedo": [0.5, 0.5, 0.5], "emission": [3.0, 2.5, 2.0]}
+    sc["objects"].append({"type": "sphere", "center": [-0.35, -0.75, -0.45], "radius": 0.12, "material": "glow"})
+    sc["camera"] = {"type": "pinhole", "position": [-0.35, -0.35, -0.08], "target": [-0.35, -0.6, -0.6], "fov": 1.4}
+    sc["id"] = "cornell-inside-box-glow"
+    comp = scene_cache.compiler.compile_scene(sc, scene_cache.compiler.RenderStrategy(est, 8, 3), 32, 32)
+    engine.load_renderer(comp)
+    engine.clear_accumulation()
+    engine.render_frames(8)
+    g = engine.read_export()
+    o, _, counts = oracle_lib.OracleScene(oracle, comp.scene).render(comp.program, comp.params(), 8)
+    exact, bad = _compare_film(f"inside-box-glow-{est}", g, o)
+    assert o[..., :3].mean() > 0.01
+    assert bad == 0.0
+    assert engine.ray_counts() == (int(counts[0]), int(counts[1]))

@@ -87,3 +87,11 @@ def test_spectral_constant_sky_glass_medium_matches_oracle(engine, oracle):
     bad, _ = _compare(engine, oracle, "spectral-constant-sky", sc, compiler.RenderStrategy("mis", 8, 3, color="spectral"),
                       24, 24, 8)
     assert bad == 0.0
+
+
+@pytest.mark.parametrize("color", ["rgb", "spectral"])
+def test_coloured_mirror_cornell_matches_oracle(engine, oracle, color):
+    sc = scenes.cornell_mirror(f0=(0.5, 0.25, 0.125))
+    bad, _ = _compare(engine, oracle, f"cornell-mirror-{color}", sc, compiler.RenderStrategy("mis", 8, 3, color=color),
+                      24, 24, 8)
+    assert bad == 0.0
