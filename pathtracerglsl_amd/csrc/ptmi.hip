@@ -334,9 +334,6 @@ struct LaneFeeder {
 #ifndef PTMI_FEED_MIN_IDLE
 #define PTMI_FEED_MIN_IDLE 8
 #endif
-#ifndef PTMI_SHADOW_PERSISTENT
-#define PTMI_SHADOW_PERSISTENT 0
-#endif
 #ifndef PTMI_POSTPONE_EXT
 #define PTMI_POSTPONE_EXT 0
 #endif
