@@ -69,8 +69,12 @@ typedef struct pt_object {
 } pt_object;
 
 /* Material models (src/components/materials/): PT_MAT_MIRROR is the perfect specular conductor
- * (mirror/mirror.glsl): a delta reflection weighted by Schlick's Fresnel of f0, no emission. */
-enum { PT_MAT_LAMBERT = 0, PT_MAT_DIELECTRIC = 1, PT_MAT_GGX = 2, PT_MAT_NULL = 3, PT_MAT_MIRROR = 4 };
+ * (mirror/mirror.glsl): a delta reflection weighted by Schlick's Fresnel of f0, no emission.
+ * PT_MAT_ROUGH_DIELECTRIC is the two-lobe microfacet dielectric (rough_dielectric/
+ * rough_dielectric.glsl, Walter et al. 2007): GGX reflection and transmission with the exact
+ * Fresnel, alpha = max(1e-3, roughness^2), the region's index and the transmittance tint. */
+enum { PT_MAT_LAMBERT = 0, PT_MAT_DIELECTRIC = 1, PT_MAT_GGX = 2, PT_MAT_NULL = 3, PT_MAT_MIRROR = 4,
+       PT_MAT_ROUGH_DIELECTRIC = 5 };
 
 /* The arm a homogeneous medium is sampled by (src/compiler/generate/features/materials.ts:675-728,
  * routing predicates src/compiler/types.ts:351-417):

@@ -838,14 +838,17 @@ static inline float sellmeier_index(const float* c, float lam) {
 static inline int region_has_index_row(const pt_object* ob) {
     return !((ob->kind == PT_OBJ_MESH || ob->kind == PT_OBJ_INSTANCES) && !ob->closed);
 }
-/* ior_of: the dielectric's index, at the hero wavelength wavelengths_hero (spectral.glsl:12-15)
- * for a dispersive formula under the spectral colour model. */
+/* The models that cross their interface and read the region's index (the descriptors'
+ * 'transmission' capability: dielectric, rough_dielectric). */
+static inline int transmissive_model(int model) { return model == PT_MAT_DIELECTRIC || model == PT_MAT_ROUGH_DIELECTRIC; }
+/* ior_of: the transmissive material's index, at the hero wavelength wavelengths_hero
+ * (spectral.glsl:12-15) for a dispersive formula under the spectral colour model. */
 static inline float ior_of(const or_scene* S, int region, const wl_t* wl) {
     if (region < 0) return 1.0f;
     const pt_object* ob = &S->objects[region];
     if (!region_has_index_row(ob)) return 1.0f;
     const pt_material* m = &S->materials[ob->material];
-    if (m->model != PT_MAT_DIELECTRIC) return 1.0f;
+    if (!transmissive_model(m->model)) return 1.0f;
     if (wl->spec && m->ior_kind == PT_IOR_CAUCHY) return cauchy_index(m->ior_coef, wl->lam[0]);
     if (wl->spec && m->ior_kind == PT_IOR_SELLMEIER) return sellmeier_index(m->ior_coef, wl->lam[0]);
     return m->ior;
@@ -855,7 +858,7 @@ static inline int dispersive_region(const or_scene* S, int region) {
     if (region < 0) return 0;
     const pt_object* ob = &S->objects[region];
     const pt_material* m = &S->materials[ob->material];
-    return region_has_index_row(ob) && m->model == PT_MAT_DIELECTRIC && m->ior_kind != PT_IOR_CONSTANT;
+    return region_has_index_row(ob) && transmissive_model(m->model) && m->ior_kind != PT_IOR_CONSTANT;
 }
 
 static inline v3 lv(const float* a) { return V(a[0], a[1], a[2]); }
@@ -983,6 +986,17 @@ static inline float dielectric_fresnel(float cos_i, float eta) {
     float r_perp = (eta * cos_i - cos_t) / (eta * cos_i + cos_t);
     return 0.5f * (r_par * r_par + r_perp * r_perp);
 }
+/* rough_dielectric.glsl:37-46: the generalized half vector of (wi, wo), faced up; 0 for a
+ * degenerate pair or a back-facing microfacet. */
+static int rough_half(v3 wil, v3 wol, float eta, v3* m) {
+    float etap = wil.z > 0.0f ? 1.0f : 1.0f / eta;
+    v3 h = add(muls(wil, etap), wol);
+    if (dot(h, h) <= 0.0f) return 0;
+    h = normalize(h);
+    if (h.z < 0.0f) h = neg(h);
+    *m = h;
+    return dot(h, wil) * wil.z >= 0.0f && dot(h, wol) * wol.z >= 0.0f;
+}
 
 static isample_t surface_sample(const or_scene* S, const pt_material* m, v3 wo, const hit_t* hit, float uc, float ux, float uy, const wl_t* wl) {
     isample_t s;
@@ -1024,6 +1038,42 @@ static isample_t surface_sample(const or_scene* S, const pt_material* m, v3 wo, 
         s.weight = schlick(lift(S, m->f0, &m->s_f0, wl), cos_i);
         s.pdf = 0.0f;
         s.flags = LOBE_REFLECTION | LOBE_DELTA;
+    } else if (m->model == PT_MAT_ROUGH_DIELECTRIC) {   /* rough_dielectric.glsl:74-133 */
+        v3 wol = to_local(f, wo);
+        s.wi = wo; s.weight = sgrey(0.0f); s.pdf = 0.0f; s.flags = LOBE_REFLECTION;
+        if (wol.z <= 0.0f) return s;
+        float a = m->alpha;
+        float n_i = ior_of(S, hit->region_from, wl), n_t = ior_of(S, hit->region_to, wl);
+        float eta = n_i / n_t;
+        if (n_i == n_t) {                          /* index-matched: the delta pass-through */
+            s.wi = neg(wo);
+            s.weight = lift(S, m->transmittance, &m->s_transmittance, wl);
+            s.flags = LOBE_TRANSMISSION | LOBE_DELTA;
+            return s;
+        }
+        v3 mh = mf_vndf(wol, a, ux, uy);
+        float cos_om = clampf(dot(wol, mh), 1e-6f, 1.0f);
+        float F = dielectric_fresnel(cos_om, eta);
+        v3 wil;
+        if (uc < F) {
+            v3 I = neg(wol);
+            wil = sub(I, muls(mh, 2.0f * dot(mh, I)));
+            if (wil.z <= 0.0f) return s;
+            s.weight = sgrey(mf_G1(wil, a));
+            s.pdf = F * mf_G1(wol, a) * mf_D(mh, a) / (4.0f * wol.z);
+            s.flags = LOBE_REFLECTION;
+        } else {
+            float sin2_t = eta * eta * (1.0f - cos_om * cos_om);
+            float cos_t = sqrtf(fmaxf_(0.0f, 1.0f - sin2_t));
+            wil = normalize(add(muls(wol, -eta), muls(mh, eta * cos_om - cos_t)));
+            if (wil.z >= 0.0f) return s;
+            float denom = dot(wil, mh) + dot(wol, mh) * eta;
+            denom *= denom;
+            s.weight = smuls(lift(S, m->transmittance, &m->s_transmittance, wl), mf_G1(wil, a) * eta * eta);
+            s.pdf = (1.0f - F) * mf_G1(wol, a) * mf_D(mh, a) * cos_om / wol.z * fabsf(dot(wil, mh)) / denom;
+            s.flags = LOBE_TRANSMISSION;
+        }
+        s.wi = from_local(f, wil);
     } else {                                        /* materials/ggx/ggx.glsl:27-46 */
         v3 wol = to_local(f, wo);
         float side = wol.z < 0.0f ? -1.0f : 1.0f;
@@ -1056,10 +1106,27 @@ static sp surface_eval(const or_scene* S, const pt_material* m, v3 wi, v3 wo, co
         v3 h = normalize(add(wil, wol));
         sp F = schlick(lift(S, m->f0, &m->s_f0, wl), fabsf(dot(wol, h)));
         return smuls(F, mf_D(h, a) * mf_G1(wil, a) * mf_G1(wol, a) / (4.0f * fabsf(wil.z) * fabsf(wol.z)));
+    } else if (m->model == PT_MAT_ROUGH_DIELECTRIC) {   /* rough_dielectric.glsl:48-72 */
+        v3 wil = to_local(f, wi), wol = to_local(f, wo);
+        if (wol.z <= 0.0f || wil.z == 0.0f) return sgrey(0.0f);
+        float n_i = ior_of(S, hit->region_from, wl), n_t = ior_of(S, hit->region_to, wl);
+        if (n_i == n_t) return sgrey(0.0f);
+        float eta = n_i / n_t;
+        v3 mh;
+        if (!rough_half(wil, wol, eta, &mh)) return sgrey(0.0f);
+        float a = m->alpha;
+        float cos_om = clampf(fabsf(dot(wol, mh)), 1e-6f, 1.0f);
+        float F = dielectric_fresnel(cos_om, eta);
+        float DG = mf_D(mh, a) * mf_G1(wol, a) * mf_G1(wil, a);
+        if (wil.z > 0.0f) return sgrey(DG * F / (4.0f * wil.z * wol.z));
+        float denom = dot(wil, mh) + dot(wol, mh) * eta;
+        denom *= denom;
+        float ft = DG * (1.0f - F) * fabsf(dot(wil, mh) * dot(wol, mh) / (wil.z * wol.z * denom));
+        return smuls(lift(S, m->transmittance, &m->s_transmittance, wl), ft * eta * eta);
     }
     return sgrey(0.0f);
 }
-static float surface_pdf(const pt_material* m, v3 wi, v3 wo, const hit_t* hit) {
+static float surface_pdf(const or_scene* S, const pt_material* m, v3 wi, v3 wo, const hit_t* hit, const wl_t* wl) {
     frame_t f = hit->frame;
     if (m->model == PT_MAT_LAMBERT) {
         float c = dot(wi, f.n) * signf(dot(wo, f.n));
@@ -1070,6 +1137,21 @@ static float surface_pdf(const pt_material* m, v3 wi, v3 wo, const hit_t* hit) {
         float a = m->alpha;
         v3 h = normalize(add(wil, wol));
         return mf_G1(wol, a) * mf_D(h, a) / (4.0f * fabsf(wol.z));
+    } else if (m->model == PT_MAT_ROUGH_DIELECTRIC) {   /* rough_dielectric.glsl:135-155 */
+        v3 wil = to_local(f, wi), wol = to_local(f, wo);
+        if (wol.z <= 0.0f || wil.z == 0.0f) return 0.0f;
+        float n_i = ior_of(S, hit->region_from, wl), n_t = ior_of(S, hit->region_to, wl);
+        if (n_i == n_t) return 0.0f;
+        float eta = n_i / n_t;
+        v3 mh;
+        if (!rough_half(wil, wol, eta, &mh)) return 0.0f;
+        float a = m->alpha;
+        float cos_om = clampf(fabsf(dot(wol, mh)), 1e-6f, 1.0f);
+        float F = dielectric_fresnel(cos_om, eta);
+        if (wil.z > 0.0f) return F * mf_G1(wol, a) * mf_D(mh, a) / (4.0f * wol.z);
+        float denom = dot(wil, mh) + dot(wol, mh) * eta;
+        denom *= denom;
+        return (1.0f - F) * mf_G1(wol, a) * mf_D(mh, a) * cos_om / wol.z * fabsf(dot(wil, mh)) / denom;
     }
     return 0.0f;
 }
@@ -1708,7 +1790,7 @@ static sp transport_trace(const or_scene* S, const or_program* P, v3 ro, v3 rd, 
         if (bounce == P->max_bounces) break;
         /* collapse_wavelengths (pt.ts:440,466-475): a dielectric bounding a dispersive region sends
          * the path along the hero wavelength; the hero lane carries all four (once per path) */
-        if (wl->spec && !collapsed && M->model == PT_MAT_DIELECTRIC &&
+        if (wl->spec && !collapsed && transmissive_model(M->model) &&
             (dispersive_region(S, hit.region_from) || dispersive_region(S, hit.region_to))) {
             T = smul(T, SP(4.0f, 0.0f, 0.0f, 0.0f));
             collapsed = 1;
@@ -1734,7 +1816,7 @@ static sp transport_trace(const or_scene* S, const or_program* P, v3 ro, v3 rd, 
                     sp f = surface_eval(S, M, ls.wi, wo, &hit, wl);
                     float cos_i = fabsf(dot(ls.wi, hit.frame.n));
                     float w = 1.0f;
-                    if (mis && !(ls.flags & 1)) w = power_heuristic(ls.pdf, surface_pdf(M, ls.wi, wo, &hit));
+                    if (mis && !(ls.flags & 1)) w = power_heuristic(ls.pdf, surface_pdf(S, M, ls.wi, wo, &hit, wl));
                     sp c = smul(smul(T, ls.radiance), f);
                     c = smuls(c, cos_i);
                     c = smul(c, vis);
@@ -1823,7 +1905,7 @@ or_scene* or_scene_create(const pt_scene_desc* d) {
     }
     S->env_rot_cs[0] = (float)cos((double)d->env.rotation);
     S->env_rot_cs[1] = (float)sin((double)d->env.rotation);
-    for (int m = 0; m < S->n_materials; m++) if (S->materials[m].model == PT_MAT_DIELECTRIC) S->has_transmission = 1;
+    for (int m = 0; m < S->n_materials; m++) if (transmissive_model(S->materials[m].model)) S->has_transmission = 1;
     /* per-mesh BLAS; instance batches get theirs over the prototype range */
     S->blas = (bvh_t*)calloc((size_t)(S->n_objects > 0 ? S->n_objects : 1), sizeof(bvh_t));
     S->obj_first = (int*)calloc((size_t)(S->n_objects > 0 ? S->n_objects : 1), sizeof(int));
@@ -2134,7 +2216,7 @@ void or_bsdf_probe(const pt_material* m, const float* n, const float* wo, float 
     wl_t wl; memset(&wl, 0, sizeof wl);
     isample_t s = surface_sample(&S, m, w, &h, uc, ux, uy, &wl);
     sp f = surface_eval(&S, m, s.wi, w, &h, &wl);
-    float p = surface_pdf(m, s.wi, w, &h);
+    float p = surface_pdf(&S, m, s.wi, w, &h, &wl);
     float r[12] = {s.wi.x, s.wi.y, s.wi.z, s.weight.x, s.weight.y, s.weight.z, s.pdf, (float)s.flags, f.x, f.y, f.z, p};
     memcpy(out, r, sizeof r);
 }

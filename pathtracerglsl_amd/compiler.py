@@ -17,7 +17,9 @@ from . import abi, spectral
 
 ESTIMATORS = {"none": abi.EST_PT, "pt": abi.EST_PT, "nee": abi.EST_NEE, "mis": abi.EST_MIS}
 MODELS = {"lambert": abi.MAT_LAMBERT, "dielectric": abi.MAT_DIELECTRIC, "ggx": abi.MAT_GGX, "none": abi.MAT_NULL,
-          "mirror": abi.MAT_MIRROR}
+          "mirror": abi.MAT_MIRROR, "rough_dielectric": abi.MAT_ROUGH_DIELECTRIC}
+# models whose emission term is identically zero (their descriptors' emissive capability is false)
+NON_EMISSIVE = ("mirror", "rough_dielectric")
 
 
 class CompileError(ValueError):
@@ -356,14 +358,15 @@ def compile_scene(scene, strategy=None, width=256, height=256, reset_salt=1234):
         albedo = m.get("albedo", [0.8, 0.8, 0.8])
         rec["albedo"][:3] = spectral.rgb_of(albedo)
         rec["s_albedo"] = srec(albedo, spectral.ROLE_REFLECTANCE)
-        em = m.get("emission", [0.0, 0.0, 0.0]) if m["model"] != "mirror" else [0.0, 0.0, 0.0]   # mirror_emission = 0
+        em = m.get("emission", [0.0, 0.0, 0.0]) if m["model"] not in NON_EMISSIVE else [0.0, 0.0, 0.0]
         rec["emission"][:3] = spectral.rgb_of(em)
         rec["s_emission"] = srec(em, spectral.ROLE_EMISSION)
         rec["emissive"] = int(emits(em))
         f0 = m.get("f0", [0.9, 0.9, 0.9])
         rec["f0"][:3] = spectral.rgb_of(f0)
         rec["s_f0"] = srec(f0, spectral.ROLE_REFLECTANCE)
-        rough = float(m.get("roughness", 0.5))
+        # roughness defaults: ggx 0.5, rough_dielectric 0.2 (rough_dielectric.ts:17)
+        rough = float(m.get("roughness", 0.2 if m["model"] == "rough_dielectric" else 0.5))
         rec["alpha"] = max(1e-3, rough * rough)                  # ggx.ts derived alpha
         tr = m.get("transmittance", [1.0, 1.0, 1.0])
         rec["transmittance"][:3] = spectral.rgb_of(tr)

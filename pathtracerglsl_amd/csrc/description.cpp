@@ -237,7 +237,7 @@ bool lower(const JVal& d, int32_t width, int32_t height, int32_t fif, pt_program
         if (const JVal* models = mats->get("models"); models && models->kind == JVal::ARR)
             for (const JVal& md : models->arr)
                 if (!(is_str(&md, "lambert") || is_str(&md, "dielectric") || is_str(&md, "ggx") || is_str(&md, "none") ||
-                      is_str(&md, "mirror")))
+                      is_str(&md, "mirror") || is_str(&md, "rough_dielectric")))
                     return L.unsupported("material model", quoted(&md));
         if (truthy(mats->get("twoSidedShading"))) return L.unsupported("materials.twoSidedShading", "true");
         if (truthy(mats->get("materialsReadUv"))) return L.unsupported("materials.materialsReadUv", "true (textures)");

@@ -42,7 +42,11 @@ constexpr int MAX_GRID = 8192;
 
 // Shading categories: the bin kernel sorts every path of a bounce by what its vertex needs,
 // so each shade launch runs one code path (no material divergence inside a wave).
-enum { CAT_MISS = 0, CAT_LAMBERT = 1, CAT_GGX = 2, CAT_DIELECTRIC = 3, CAT_MEDIUM = 4, CAT_NULL = 5, CAT_MIRROR = 6, N_CAT = 7 };
+enum { CAT_MISS = 0, CAT_LAMBERT = 1, CAT_GGX = 2, CAT_DIELECTRIC = 3, CAT_MEDIUM = 4, CAT_NULL = 5, CAT_MIRROR = 6,
+       CAT_ROUGH = 7, N_CAT = 8 };
+// Categories whose shading classifies every hit's regions (the transmissive models read both
+// sides' indices; media programs track the current medium).
+constexpr bool cat_probes(int cat, bool media) { return cat == CAT_DIELECTRIC || cat == CAT_ROUGH || media; }
 // counts[] slots
 enum { C_RAYQ0 = 0, C_RAYQ1 = 1, C_SHADOW = 2, C_CAT0 = 3, C_EXT_FETCH = C_CAT0 + N_CAT, C_SHD_FETCH, C_CAM_MEDIUM,
        N_COUNTS = 16 };
@@ -598,7 +602,8 @@ __global__ void __launch_bounds__(SBLOCK) k_bin(DevScene Sk, Paths P, int qsel, 
                     cat = model == PT_MAT_LAMBERT ? CAT_LAMBERT
                         : model == PT_MAT_GGX ? CAT_GGX
                         : model == PT_MAT_DIELECTRIC ? CAT_DIELECTRIC
-                        : model == PT_MAT_MIRROR ? CAT_MIRROR : CAT_NULL;
+                        : model == PT_MAT_MIRROR ? CAT_MIRROR
+                        : model == PT_MAT_ROUGH_DIELECTRIC ? CAT_ROUGH : CAT_NULL;
                 }
             }
         }
@@ -646,7 +651,7 @@ __global__ void k_bookkeep(Paths P, int cur, int next, int first) {
 template <int EST, int CAT, int STACK, bool MEDIA, bool SPEC, bool ADJ, bool BATCH = false>
 __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene Sk, Paths P, int qsel, int max_bounces,
                                                   int rr_start, float rr_max, int max_null, int equiangular = 0) {
-    constexpr bool PROBE = CAT == CAT_DIELECTRIC || MEDIA;
+    constexpr bool PROBE = cat_probes(CAT, MEDIA);
     // The region probe's LDS stack: static in the probing categories; the other surface
     // categories probe only at a back-face hit in a scene with an emissive solid, and are then
     // launched with the same stack as dynamic LDS (shade_dynamic_lds).
@@ -662,7 +667,8 @@ __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene Sk, Paths P, int qsel
     constexpr bool NEE = EST != PT_EST_PT;
     constexpr bool MIS = EST == PT_EST_MIS;
     constexpr int MODEL = CAT == CAT_LAMBERT ? PT_MAT_LAMBERT : CAT == CAT_GGX ? PT_MAT_GGX
-                        : CAT == CAT_MIRROR ? PT_MAT_MIRROR : PT_MAT_DIELECTRIC;
+                        : CAT == CAT_MIRROR ? PT_MAT_MIRROR : CAT == CAT_ROUGH ? PT_MAT_ROUGH_DIELECTRIC
+                        : PT_MAT_DIELECTRIC;
     for (int base = blockIdx.x * SBLOCK; base < n; base += gridDim.x * SBLOCK) {
         const int k = base + threadIdx.x;
         const bool valid = k < n;
@@ -811,7 +817,7 @@ __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene Sk, Paths P, int qsel
                     if (bounce < max_bounces) {
                         // collapse_wavelengths (pt.ts:440,466-475): a dielectric bounding a dispersive
                         // region sends the path along the hero wavelength, lane 0 carrying all four
-                        if (SPEC && CAT == CAT_DIELECTRIC && !collapsed &&
+                        if (SPEC && (CAT == CAT_DIELECTRIC || CAT == CAT_ROUGH) && !collapsed &&
                             (dispersive_region(S, h.region_from) || dispersive_region(S, h.region_to))) {
                             T = T * mk4(4.0f, 0.0f, 0.0f, 0.0f);
                             collapsed = true;
@@ -830,7 +836,7 @@ __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene Sk, Paths P, int qsel
                                 sp4 f = surface_eval<MODEL, SPEC>(S, M, ls.wi, wo, h, wl);
                                 float cos_i = fabsf(dot(ls.wi, h.frame.n));
                                 s_w = 1.0f;
-                                if (MIS && !(ls.flags & 1)) s_w = power_heuristic(ls.pdf, surface_pdf<MODEL>(M, ls.wi, wo, h));
+                                if (MIS && !(ls.flags & 1)) s_w = power_heuristic(ls.pdf, surface_pdf<MODEL, SPEC>(S, M, ls.wi, wo, h, wl));
                                 s_a = ((T * ls.radiance) * f) * cos_i;
                                 s_pdf = ls.pdf;
                                 shadow = true;
@@ -1184,6 +1190,7 @@ struct SceneStore {
     uint32_t wide_depth = 0;
     bool has_ggx = false;
     bool has_mirror = false;
+    bool has_rough = false;
     std::vector<pt_material> host_materials;
     hipStream_t stream = nullptr;
     template <class T> pt_status upload(const T* host, size_t n, const T** out) {
@@ -1218,6 +1225,7 @@ struct pt_ctx {
     int shade_blocks = 2048;
     bool has_ggx = false;
     bool has_mirror = false;
+    bool has_rough = false;
     std::vector<pt_material> host_materials;   // the uploaded records (adjoint checks, updates)
     bool timing = false;
     std::map<std::string, TimingSlot> timings;
@@ -1563,8 +1571,10 @@ static pt_status build_scene(const pt_scene_desc* sd, hipStream_t stream, SceneS
     D.n_lights = sd->n_lights;
     SS.has_ggx = false;
     SS.has_mirror = false;
+    SS.has_rough = false;
     for (int m = 0; m < sd->n_materials; m++) {
-        if (sd->materials[m].model == PT_MAT_DIELECTRIC) D.has_transmission = 1;
+        if (transmissive_model(sd->materials[m].model)) D.has_transmission = 1;
+        if (sd->materials[m].model == PT_MAT_ROUGH_DIELECTRIC) SS.has_rough = true;
         if (sd->materials[m].model == PT_MAT_GGX) SS.has_ggx = true;
         if (sd->materials[m].model == PT_MAT_MIRROR) SS.has_mirror = true;
         if (sd->materials[m].has_medium) D.has_media = 1;
@@ -1790,6 +1800,7 @@ pt_status pt_scene_upload(pt_ctx* ctx, const pt_scene_desc* sd) {
     ctx->wide_depth = SS.wide_depth;
     ctx->has_ggx = SS.has_ggx;
     ctx->has_mirror = SS.has_mirror;
+    ctx->has_rough = SS.has_rough;
     ctx->host_materials = std::move(SS.host_materials);
     ctx->has_scene = true;
     return PT_OK;
@@ -1878,7 +1889,7 @@ static pt_status ensure_paths(pt_renderer* r, long long n) {
 // without a static one, in scenes whose emissive solids make them probe at back-face hits.
 template <int CAT, bool MEDIA, int STACK>
 static size_t shade_dynamic_lds(bool emissive_solid) {
-    constexpr bool STATIC_STK = (CAT == CAT_DIELECTRIC || MEDIA) && CAT != CAT_MISS && CAT != CAT_MEDIUM;
+    constexpr bool STATIC_STK = cat_probes(CAT, MEDIA) && CAT != CAT_MISS && CAT != CAT_MEDIUM;
     constexpr bool SURFACE = CAT != CAT_MISS && CAT != CAT_MEDIUM;
     return (!STATIC_STK && SURFACE && emissive_solid) ? (size_t)STACK * SBLOCK * sizeof(uint2) : 0;
 }
@@ -1896,6 +1907,7 @@ static void launch_shade(pt_renderer* r, int cur) {
     PT_SHADE("shade_lambert", CAT_LAMBERT);
     if (ctx->has_ggx) PT_SHADE("shade_ggx", CAT_GGX);
     if (ctx->has_mirror) PT_SHADE("shade_mirror", CAT_MIRROR);
+    if (ctx->has_rough) PT_SHADE("shade_rough_dielectric", CAT_ROUGH);
     if (S.has_transmission) PT_SHADE("shade_dielectric", CAT_DIELECTRIC);
     if (MEDIA) PT_SHADE("shade_medium", CAT_MEDIUM);
     if (S.has_null) PT_SHADE("shade_null", CAT_NULL);
@@ -2148,7 +2160,7 @@ struct pt_batch {
     std::vector<long long> film_off;   // first pixel of each scene's film
     long long film_pixels = 0;
     int max_npix = 0, min_stack = 16;
-    bool any_ggx = false, any_dielectric = false, any_mirror = false, any_emissive_solid = false;
+    bool any_ggx = false, any_dielectric = false, any_mirror = false, any_rough = false, any_emissive_solid = false;
     DevBuf d_scenes, d_cams, d_entries, films;
     pt_renderer* r = nullptr;          // path slots
     pt_batch_config cfg{};
@@ -2217,6 +2229,7 @@ static void batch_shade(pt_batch* b, int cur) {
     PT_BSHADE(CAT_LAMBERT);
     if (b->any_ggx) PT_BSHADE(CAT_GGX);
     if (b->any_mirror) PT_BSHADE(CAT_MIRROR);
+    if (b->any_rough) PT_BSHADE(CAT_ROUGH);
     if (b->any_dielectric) PT_BSHADE(CAT_DIELECTRIC);
 #undef PT_BSHADE
 }
@@ -2316,6 +2329,7 @@ pt_status pt_batch_create(pt_ctx* ctx, int32_t n, const pt_scene_desc* scenes, c
         b->min_stack = std::max(b->min_stack, store->stack);
         b->any_ggx = b->any_ggx || store->has_ggx;
         b->any_mirror = b->any_mirror || store->has_mirror;
+        b->any_rough = b->any_rough || store->has_rough;
         b->any_emissive_solid = b->any_emissive_solid || store->dev.emissive_solid != 0;
         b->any_dielectric = b->any_dielectric || store->dev.has_transmission;
         devs.push_back(store->dev);

@@ -174,6 +174,9 @@ PT_DEV bool object_thin(const DevScene& S, int o) {
 struct Wl { float4 lam; };
 // Regions with a row in the generated ior_of table (features/intersection.ts:1609-1649): every
 // primitive object, closed meshes and closed instance batches; open meshes are thin (1.0).
+// The models that cross their interface and read the region's index (the descriptors'
+// 'transmission' capability: dielectric and rough_dielectric).
+__host__ __device__ inline bool transmissive_model(int model) { return model == PT_MAT_DIELECTRIC || model == PT_MAT_ROUGH_DIELECTRIC; }
 PT_DEV bool region_has_index_row(const pt_object& ob) {
     return !((ob.kind == PT_OBJ_MESH || ob.kind == PT_OBJ_INSTANCES) && !ob.closed);
 }
@@ -185,7 +188,7 @@ PT_DEV float ior_of(const DevScene& S, int region, const Wl& wl) {
     const pt_object& ob = S.objects[region];
     if (!region_has_index_row(ob)) return 1.0f;
     const pt_material& m = S.materials[ob.material];
-    if (m.model != PT_MAT_DIELECTRIC) return 1.0f;
+    if (!transmissive_model(m.model)) return 1.0f;
     if (SPEC && m.ior_kind == PT_IOR_CAUCHY) {
         const float lam = wl.lam.x;
         const float inv2 = 1e6f / (lam * lam);
@@ -205,7 +208,7 @@ PT_DEV bool dispersive_region(const DevScene& S, int region) {
     if (region < 0) return false;
     const pt_object& ob = S.objects[region];
     const pt_material& m = S.materials[ob.material];
-    return region_has_index_row(ob) && m.model == PT_MAT_DIELECTRIC && m.ior_kind != PT_IOR_CONSTANT;
+    return region_has_index_row(ob) && transmissive_model(m.model) && m.ior_kind != PT_IOR_CONSTANT;
 }
 
 // The placement owning an instance primitive id: the last instance whose first id <= prim.
@@ -418,6 +421,17 @@ PT_DEV float dielectric_fresnel(float cos_i, float eta) {   // math.glsl:95-102
 // (category kernels shade one model per launch).
 template <int MODEL> PT_DEV bool is_model(const pt_material& m, int x) { return MODEL < 0 ? m.model == x : MODEL == x; }
 
+// materials/rough_dielectric/rough_dielectric.glsl:37-46: the generalized half vector of a
+// (wi, wo) pair, faced up; false for a degenerate pair or a back-facing microfacet.
+PT_DEV bool rough_half(f3 wil, f3 wol, float eta, f3& m) {
+    float etap = wil.z > 0.0f ? 1.0f : 1.0f / eta;
+    m = wil * etap + wol;
+    if (dot(m, m) <= 0.0f) return false;
+    m = normalize(m);
+    if (m.z < 0.0f) m = -m;
+    return dot(m, wil) * wil.z >= 0.0f && dot(m, wol) * wol.z >= 0.0f;
+}
+
 template <int MODEL, bool SPEC>
 PT_DEV ISample surface_sample(const DevScene& S, const pt_material& m, f3 wo, const Hit& h, float uc, float ux, float uy, const Wl& wl) {
     ISample s;
@@ -460,6 +474,44 @@ PT_DEV ISample surface_sample(const DevScene& S, const pt_material& m, f3 wo, co
         s.weight = schlick(lift<SPEC>(S, m.f0, m.s_f0, wl), cos_i);
         s.pdf = 0.0f;
         s.flags = LOBE_REFLECTION | LOBE_DELTA;
+    } else if (is_model<MODEL>(m, PT_MAT_ROUGH_DIELECTRIC)) {   // rough_dielectric.glsl:74-133
+        f3 wol = to_local(f, wo);
+        s.wi = wo; s.weight = sgrey(0.0f); s.pdf = 0.0f; s.flags = LOBE_REFLECTION;
+        if (wol.z <= 0.0f) return s;
+        float a = m.alpha;
+        float n_i = ior_of<SPEC>(S, h.region_from, wl);
+        float n_t = ior_of<SPEC>(S, h.region_to, wl);
+        float eta = n_i / n_t;
+        if (n_i == n_t) {                           // index-matched: the delta pass-through
+            s.wi = -wo;
+            s.weight = lift<SPEC>(S, m.transmittance, m.s_transmittance, wl);
+            s.pdf = 0.0f;
+            s.flags = LOBE_TRANSMISSION | LOBE_DELTA;
+            return s;
+        }
+        f3 mh = mf_vndf(wol, a, ux, uy);
+        float cos_om = gclamp(dot(wol, mh), 1e-6f, 1.0f);
+        float F = dielectric_fresnel(cos_om, eta);
+        f3 wil;
+        if (uc < F) {
+            f3 I = -wol;
+            wil = I - mh * (2.0f * dot(mh, I));
+            if (wil.z <= 0.0f) return s;
+            s.weight = sgrey(mf_G1(wil, a));
+            s.pdf = F * mf_G1(wol, a) * mf_D(mh, a) / (4.0f * wol.z);
+            s.flags = LOBE_REFLECTION;
+        } else {
+            float sin2_t = eta * eta * (1.0f - cos_om * cos_om);
+            float cos_t = sqrtf(gmax(0.0f, 1.0f - sin2_t));
+            wil = normalize(wol * (-eta) + mh * (eta * cos_om - cos_t));
+            if (wil.z >= 0.0f) return s;
+            float denom = dot(wil, mh) + dot(wol, mh) * eta;
+            denom *= denom;
+            s.weight = lift<SPEC>(S, m.transmittance, m.s_transmittance, wl) * (mf_G1(wil, a) * eta * eta);
+            s.pdf = (1.0f - F) * mf_G1(wol, a) * mf_D(mh, a) * cos_om / wol.z * fabsf(dot(wil, mh)) / denom;
+            s.flags = LOBE_TRANSMISSION;
+        }
+        s.wi = from_local(f, wil);
     } else {                                    // materials/ggx/ggx.glsl:27-46
         f3 wol = to_local(f, wo);
         float side = wol.z < 0.0f ? -1.0f : 1.0f;
@@ -493,11 +545,29 @@ PT_DEV sp4 surface_eval(const DevScene& S, const pt_material& m, f3 wi, f3 wo, c
         f3 hh = normalize(wil + wol);
         sp4 F = schlick(lift<SPEC>(S, m.f0, m.s_f0, wl), fabsf(dot(wol, hh)));
         return F * (mf_D(hh, a) * mf_G1(wil, a) * mf_G1(wol, a) / (4.0f * fabsf(wil.z) * fabsf(wol.z)));
+    } else if (is_model<MODEL>(m, PT_MAT_ROUGH_DIELECTRIC)) {   // rough_dielectric.glsl:48-72
+        f3 wil = to_local(f, wi), wol = to_local(f, wo);
+        if (wol.z <= 0.0f || wil.z == 0.0f) return sgrey(0.0f);
+        float n_i = ior_of<SPEC>(S, h.region_from, wl);
+        float n_t = ior_of<SPEC>(S, h.region_to, wl);
+        if (n_i == n_t) return sgrey(0.0f);
+        float eta = n_i / n_t;
+        f3 mh;
+        if (!rough_half(wil, wol, eta, mh)) return sgrey(0.0f);
+        float a = m.alpha;
+        float cos_om = gclamp(fabsf(dot(wol, mh)), 1e-6f, 1.0f);
+        float F = dielectric_fresnel(cos_om, eta);
+        float DG = mf_D(mh, a) * mf_G1(wol, a) * mf_G1(wil, a);
+        if (wil.z > 0.0f) return sgrey(DG * F / (4.0f * wil.z * wol.z));
+        float denom = dot(wil, mh) + dot(wol, mh) * eta;
+        denom *= denom;
+        float ft = DG * (1.0f - F) * fabsf(dot(wil, mh) * dot(wol, mh) / (wil.z * wol.z * denom));
+        return lift<SPEC>(S, m.transmittance, m.s_transmittance, wl) * (ft * eta * eta);
     }
     return sgrey(0.0f);
 }
-template <int MODEL = -1>
-PT_DEV float surface_pdf(const pt_material& m, f3 wi, f3 wo, const Hit& h) {
+template <int MODEL, bool SPEC>
+PT_DEV float surface_pdf(const DevScene& S, const pt_material& m, f3 wi, f3 wo, const Hit& h, const Wl& wl) {
     const Frame& f = h.frame;
     if (is_model<MODEL>(m, PT_MAT_LAMBERT)) {
         float c = dot(wi, f.n) * gsign(dot(wo, f.n));
@@ -508,7 +578,24 @@ PT_DEV float surface_pdf(const pt_material& m, f3 wi, f3 wo, const Hit& h) {
         float a = m.alpha;
         f3 hh = normalize(wil + wol);
         return mf_G1(wol, a) * mf_D(hh, a) / (4.0f * fabsf(wol.z));
+    } else if (is_model<MODEL>(m, PT_MAT_ROUGH_DIELECTRIC)) {   // rough_dielectric.glsl:135-155
+        f3 wil = to_local(f, wi), wol = to_local(f, wo);
+        if (wol.z <= 0.0f || wil.z == 0.0f) return 0.0f;
+        float n_i = ior_of<SPEC>(S, h.region_from, wl);
+        float n_t = ior_of<SPEC>(S, h.region_to, wl);
+        if (n_i == n_t) return 0.0f;
+        float eta = n_i / n_t;
+        f3 mh;
+        if (!rough_half(wil, wol, eta, mh)) return 0.0f;
+        float a = m.alpha;
+        float cos_om = gclamp(fabsf(dot(wol, mh)), 1e-6f, 1.0f);
+        float F = dielectric_fresnel(cos_om, eta);
+        if (wil.z > 0.0f) return F * mf_G1(wol, a) * mf_D(mh, a) / (4.0f * wol.z);
+        float denom = dot(wil, mh) + dot(wol, mh) * eta;
+        denom *= denom;
+        return (1.0f - F) * mf_G1(wol, a) * mf_D(mh, a) * cos_om / wol.z * fabsf(dot(wil, mh)) / denom;
     }
+    (void)S; (void)wl;
     return 0.0f;
 }
 // ---------------------------------------------------------------- environment

@@ -661,6 +661,31 @@ def furnace_sky(sample_as_light=True, color=(1.0, 1.0, 1.0)):
     }
 
 
+def rough_glass_furnace(roughness=0.03):
+    """furnace-sky with a rough dielectric sphere (index 1.5, clear): every camera path enters
+    and leaves the sphere, so eta^2 cancels; near-smooth (alpha at the 1e-3 floor) the sphere
+    renders L = 1 up to the microfacet model's single-scattering loss."""
+    sc = furnace_sky(True)
+    sc["id"] = "rough-glass-furnace"
+    sc["materials"] = {"frost": {"model": "rough_dielectric", "ior": 1.5, "roughness": roughness}}
+    sc["objects"][0]["material"] = "frost"
+    return sc
+
+
+def cornell_rough_glass(roughness=0.3, medium=None):
+    """The Cornell box with a rough glass sphere (radius 0.3) and the short inner box as a rough
+    glass mesh, optionally filled with a homogeneous medium."""
+    sc = cornell_box()
+    sc["id"] = "cornell-rough-glass"
+    mat = {"model": "rough_dielectric", "ior": 1.5, "roughness": roughness, "transmittance": [1.0, 0.95, 0.9]}
+    if medium is not None:
+        mat["medium"] = medium
+    sc["materials"]["frost"] = mat
+    sc["objects"][3]["material"] = "frost"
+    sc["objects"].append({"type": "sphere", "center": [-0.45, 0.35, 0.35], "radius": 0.3, "material": "frost"})
+    return sc
+
+
 def mirror_furnace(f0=(1.0, 1.0, 1.0)):
     """furnace-sky with the reference's mirror material on the sphere: one delta reflection per
     camera ray (a convex mirror never sees itself), weight Schlick(f0, cos), then the L = 1 sky;

@@ -17,7 +17,7 @@ import pytest
 from pathtracerglsl_amd import _lib, abi, compiler, scenes
 
 FIX = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "program_descriptions.json")))
-SUPPORTED_MODELS = {"lambert", "dielectric", "ggx", "none", "mirror"}
+SUPPORTED_MODELS = {"lambert", "dielectric", "ggx", "none", "mirror", "rough_dielectric"}
 
 
 def _lower(d, w=64, h=48, fif=1):
@@ -80,8 +80,8 @@ def test_rejection_names_the_field():
     with pytest.raises(_lib.PtError, match="measurement.camera.type 'fisheye'"):
         _lower(d)
     d = json.loads(json.dumps(FIX["cases"]["analytic-glass + pathtracer"]))
-    d["materials"]["models"].append("rough_dielectric")
-    with pytest.raises(_lib.PtError, match="material model 'rough_dielectric'"):
+    d["materials"]["models"].append("checker")
+    with pytest.raises(_lib.PtError, match="material model 'checker'"):
         _lower(d)
     with pytest.raises(_lib.PtError, match="malformed JSON"):
         _lower('{"measurement": {')

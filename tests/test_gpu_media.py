@@ -145,6 +145,16 @@ def test_orthographic_media_camera_inside_and_outside(engine, oracle):
     assert bad == 0.0
 
 
+@pytest.mark.parametrize("est", ["mis", "none"])
+def test_rough_glass_medium_matches_oracle(engine, oracle, est):
+    """Rough glass bounding a scattering medium: the medium switch and eta_scale on the
+    non-delta transmission lobe, NEE at the rough interface."""
+    sc = scenes.cornell_rough_glass(0.25, medium={"sigma_s": [0.6, 0.8, 1.0], "sigma_a": [0.1, 0.1, 0.2], "phase_g": 0.3})
+    _, o, bad = _render_both(engine, oracle, f"rough-glass-medium-{est}", sc, compiler.RenderStrategy(est, 12, 3), 32, 32, 8)
+    assert o[..., :3].mean() > 0.01
+    assert bad == 0.0
+
+
 def test_thinlens_media_camera_straddles_the_medium(engine, oracle):
     """Thin-lens rays start over the lens disk (radius 0.3 at z = 0.45): the inner part of the
     disk lies inside the glass-medium sphere, the rim outside, so each camera ray takes its own

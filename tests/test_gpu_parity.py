@@ -293,3 +293,18 @@ def test_back_face_probes_with_an_emissive_solid_match_oracle(engine, oracle, es
     assert o[..., :3].mean() > 0.01
     assert bad == 0.0
     assert engine.ray_counts() == (int(counts[0]), int(counts[1]))
+
+
+@pytest.mark.parametrize("est", ["mis", "nee", "none"])
+def test_rough_glass_cornell_matches_oracle(engine, oracle, est):
+    comp = scene_cache.compiler.compile_scene(scenes.cornell_rough_glass(), scene_cache.compiler.RenderStrategy(est, 10, 3),
+                                              40, 40)
+    engine.load_renderer(comp)
+    engine.clear_accumulation()
+    engine.render_frames(8)
+    g = engine.read_export()
+    o, _, counts = oracle_lib.OracleScene(oracle, comp.scene).render(comp.program, comp.params(), 8)
+    exact, bad = _compare_film(f"cornell-rough-glass-{est}", g, o)
+    assert o[..., :3].mean() > 0.05
+    assert bad == 0.0
+    assert engine.ray_counts() == (int(counts[0]), int(counts[1]))

@@ -95,3 +95,14 @@ def test_coloured_mirror_cornell_matches_oracle(engine, oracle, color):
     bad, _ = _compare(engine, oracle, f"cornell-mirror-{color}", sc, compiler.RenderStrategy("mis", 8, 3, color=color),
                       24, 24, 8)
     assert bad == 0.0
+
+
+@pytest.mark.parametrize("color", ["spectral", "rgb"])
+def test_rough_dispersive_prism_matches_oracle(engine, oracle, color):
+    """The prism as a rough dielectric (roughness 0.15): the hero collapse at a transmissive
+    surface bounding a dispersive region, through the microfacet lobes."""
+    sc = scenes.dispersion_prism(scenes.N_BK7)
+    sc["materials"]["glass"] = dict(sc["materials"]["glass"], model="rough_dielectric", roughness=0.15)
+    bad, _ = _compare(engine, oracle, f"rough-prism-{color}", sc, compiler.RenderStrategy("none", 12, 3, color=color),
+                      32, 24, 8)
+    assert bad == 0.0

@@ -82,3 +82,48 @@ def test_dielectric_fresnel_split_and_eta2(oracle):
     assert int(refl[7]) & 1 and np.allclose(refl[0:3], [0, 1, 0], atol=1e-6)
     assert int(trans[7]) & 2 and np.allclose(trans[0:3], [0, -1, 0], atol=1e-6)
     np.testing.assert_allclose(trans[3:6], 1.0 / 1.5 ** 2, rtol=1e-6)   # (n_i / n_t)^2
+
+
+def test_rough_dielectric_weight_pdf_identity_both_lobes(oracle):
+    """rough_dielectric.test.ts's triple: weight * pdf == f * |cos| and pdf(sampled) == sample.pdf
+    on the reflection and the transmission lobe (the probe's interface: air outside, index 1.5
+    inside); the transmission weight carries eta^2 = (1 / 1.5)^2 times G1."""
+    rng = np.random.default_rng(7)
+    for alpha in (0.05, 0.2, 0.6):
+        m = mat(abi.MAT_ROUGH_DIELECTRIC, transmittance=[1.0, 0.8, 0.6], ior=1.5, alpha=alpha)
+        seen = {1: 0, 2: 0}
+        for _ in range(4000):
+            n = rand_dir(rng, 1)[0]
+            wo = rand_dir(rng, 1)[0]
+            if np.dot(n, wo) < 0.05:
+                continue
+            r = probe(oracle, m, n, wo, rng.uniform(size=3))
+            wi, weight, pdf, flags, f, pdf2 = r[0:3], r[3:6], r[6], int(r[7]), r[8:11], r[11]
+            if weight.max() <= 0:
+                continue
+            lobe = 2 if flags & 2 else 1
+            assert (np.dot(wi, n) < 0) == (lobe == 2)
+            cos = abs(np.dot(wi, n))
+            np.testing.assert_allclose(weight * pdf, f * cos, rtol=2e-3, atol=1e-7)
+            np.testing.assert_allclose(pdf2, pdf, rtol=2e-3)
+            if lobe == 2:
+                np.testing.assert_allclose(weight[1:] / weight[0], [0.8, 0.6], rtol=1e-5)
+                assert weight[0] <= (1 / 1.5) ** 2 * 1.0001
+            seen[lobe] += 1
+        assert seen[1] > 100 and seen[2] > 1000, seen
+
+
+def test_rough_dielectric_lobe_split_and_index_match(oracle):
+    # near-smooth at normal incidence: the reflection lobe is chosen with probability F0 = 0.04
+    m = mat(abi.MAT_ROUGH_DIELECTRIC, transmittance=[1, 1, 1], ior=1.5, alpha=1e-3)
+    n = np.array([0.0, 0.0, 1.0], np.float32)
+    rng = np.random.default_rng(3)
+    refl = [int(probe(oracle, m, n, n, rng.uniform(size=3))[7]) & 1 for _ in range(20000)]
+    assert abs(np.mean(refl) - 0.04) < 0.005
+    # index-matched interface: the delta pass-through wi = -wo with the tint as weight
+    m1 = mat(abi.MAT_ROUGH_DIELECTRIC, transmittance=[0.9, 0.7, 0.5], ior=1.0, alpha=0.3)
+    wo = np.array([0.3, 0.4, 0.866], np.float32)
+    r = probe(oracle, m1, n, wo / np.linalg.norm(wo), (0.5, 0.5, 0.5))
+    assert int(r[7]) == (2 | 4) and np.allclose(r[0:3], -wo / np.linalg.norm(wo), atol=1e-7)
+    np.testing.assert_allclose(r[3:6], [0.9, 0.7, 0.5], rtol=1e-7)
+    assert r[6] == 0.0 and r[11] == 0.0 and np.all(r[8:11] == 0.0)
