@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define PT_ABI_VERSION 5u
+#define PT_ABI_VERSION 6u
 
 typedef enum pt_status {
     PT_OK = 0,
@@ -262,7 +262,9 @@ typedef struct pt_params {
     float    ortho_scale;     /* orthographic: world half-height of the view (u_orthoScale) */
     float    aperture;        /* thin lens: lens radius (u_aperture), >= 0; 0 images like the pinhole */
     float    focus_distance;  /* thin lens: distance of the sharp plane along forward (u_focusDistance), > 0 */
-    int32_t  pad1;
+    int32_t  projection;      /* fisheye: PT_FISHEYE_* radial map (the FISHEYE_THETA selector) */
+    float    fisheye_k;       /* fisheye: the map's radial constant u_fisheyeK (fisheye.ts fisheyeK) */
+    float    cyl_focal;       /* cylindrical: cylinder radius in pixels u_cylFocal = width / hfov(rad) */
 } pt_params;
 
 /* Cameras (src/components/camera/pinhole/pinhole.glsl:4-18, orthographic/orthographic.glsl:6-21,
@@ -270,7 +272,12 @@ typedef struct pt_params {
  * camera shares the forward direction and slides the origin over the film plane; the thin lens
  * moves the pinhole ray's origin over a disk of radius `aperture` (the concentric map of the
  * lens sample) and re-aims it at the pinhole ray's point on the focus plane. */
-enum { PT_CAM_PINHOLE = 0, PT_CAM_ORTHOGRAPHIC = 1, PT_CAM_THINLENS = 2 };
+enum { PT_CAM_PINHOLE = 0, PT_CAM_ORTHOGRAPHIC = 1, PT_CAM_THINLENS = 2, PT_CAM_FISHEYE = 3, PT_CAM_CYLINDRICAL = 4,
+       PT_CAM_EQUIRECT = 5 };
+/* Fisheye radial maps (camera/fisheye/fisheye.glsl): theta = rho K, 2 asin(rho K), 2 atan(rho K),
+ * asin(rho K). The cylindrical camera sweeps azimuth = arc length over a cylinder of radius
+ * cyl_focal pixels; the equirect camera maps the film to azimuth x elevation. */
+enum { PT_FISHEYE_EQUIDISTANT = 0, PT_FISHEYE_EQUISOLID = 1, PT_FISHEYE_STEREOGRAPHIC = 2, PT_FISHEYE_ORTHOGRAPHIC = 3 };
 
 /* ---- rays for the traversal entry points (SoA) ---- */
 typedef struct pt_rays_soa {

@@ -149,6 +149,10 @@ PTM_FN float ptm_atan2(float y, float x) {
 PTM_FN float ptm_acos(float x) {
     return ptm_atan2(sqrtf((1.0f - x) * (1.0f + x)), x);
 }
+/* asin(x) = atan(x, sqrt(1 - x^2)) on [-1, 1] (exactly +-pi/2 at +-1). */
+PTM_FN float ptm_asin(float x) {
+    return ptm_atan2(x, sqrtf((1.0f - x) * (1.0f + x)));
+}
 /* pow(x, y) = exp(y log x) for x > 0; pow(0, y>0) = 0. GLSL leaves negative x undefined;
  * declared here: an integer exponent gives the real power (sign of x^y), any other NaN.
  * Schlick's pow(1 - cos, 5) meets x = -1 ulp when a rounded |dot| exceeds 1. */

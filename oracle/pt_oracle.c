@@ -2080,11 +2080,36 @@ static void concentric_disk(float u0, float u1, float* dx, float* dy) {
     else { r = oy; th = PTM_PI / 2.0f - (PTM_PI / 4.0f) * (ox / oy); }
     *dx = r * ptm_cos(th); *dy = r * ptm_sin(th);
 }
+/* fisheye.glsl FISHEYE_THETA */
+static float fisheye_theta(int projection, float rho, float K) {
+    if (projection == PT_FISHEYE_EQUISOLID) return 2.0f * ptm_asin(clampf(rho * K, 0.0f, 1.0f));
+    if (projection == PT_FISHEYE_STEREOGRAPHIC) return 2.0f * ptm_atan(rho * K);
+    if (projection == PT_FISHEYE_ORTHOGRAPHIC) return ptm_asin(clampf(rho * K, 0.0f, 1.0f));
+    return rho * K;
+}
 static void camera_ray(const pt_params* par, v3 O, v3 F, v3 R, v3 U, float ndcx, float ndcy, float l0, float l1,
-                       v3* org, v3* dir) {
+                       float filmx, float filmy, int W, int H, v3* org, v3* dir) {
     if (par->camera == PT_CAM_ORTHOGRAPHIC) {
         *org = add(add(O, muls(R, ndcx * par->ortho_scale)), muls(U, ndcy * par->ortho_scale));
         *dir = F;
+    } else if (par->camera == PT_CAM_FISHEYE) {     /* camera/fisheye/fisheye.glsl:23-38 */
+        *org = O;
+        float rho = sqrtf(ndcx * ndcx + ndcy * ndcy);
+        if (rho < 1e-6f) { *dir = F; return; }
+        float th = fisheye_theta(par->projection, rho, par->fisheye_k);
+        float tx = ndcx / rho, ty = ndcy / rho;
+        *dir = normalize(add(muls(F, ptm_cos(th)), muls(add(muls(R, tx), muls(U, ty)), ptm_sin(th))));
+    } else if (par->camera == PT_CAM_CYLINDRICAL) {   /* camera/cylindrical/cylindrical.glsl:9-21 */
+        *org = O;
+        float cx = filmx - 0.5f * (float)W, cy = filmy - 0.5f * (float)H;
+        float phi = cx / par->cyl_focal, hh = cy / par->cyl_focal;
+        *dir = normalize(add(add(muls(F, ptm_cos(phi)), muls(R, ptm_sin(phi))), muls(U, hh)));
+    } else if (par->camera == PT_CAM_EQUIRECT) {      /* camera/equirect/equirect.glsl:9-22 */
+        *org = O;
+        float u = filmx / (float)W, v = filmy / (float)H;
+        float phi = (u - 0.5f) * PTM_TWO_PI, th = (v - 0.5f) * PTM_PI;
+        float ct = ptm_cos(th);
+        *dir = normalize(add(muls(add(muls(F, ptm_cos(phi)), muls(R, ptm_sin(phi))), ct), muls(U, ptm_sin(th))));
     } else {
         *org = O;
         *dir = normalize(add(add(F, muls(R, ndcx * par->tan_fov)), muls(U, ndcy * par->tan_fov)));
@@ -2138,7 +2163,7 @@ void or_render(const or_scene* S, const pt_program* prog, const pt_params* par, 
                 float ndcy = (2.0f * filmy / (float)H) - 1.0f;
                 ndcx *= par->aspect;
                 v3 org, dir;
-                camera_ray(par, O, F, R, U, ndcx, ndcy, l0, l1, &org, &dir);
+                camera_ray(par, O, F, R, U, ndcx, ndcy, l0, l1, filmx, filmy, W, H, &org, &dir);
                 sp Ls = transport_trace(S, &P, org, dir, &rng, &c1, &c2, &wl, NULL);
                 v3 Lr = V(Ls.x, Ls.y, Ls.z);
                 if (P.spectral) {                   /* color_to_pixel (color/spectral/spectral.glsl:27-33) */
@@ -2181,6 +2206,7 @@ void or_math_eval(int fn, const float* x, const float* y, float* out, int n) {
             case 8: out[i] = ptm_acos(x[i]); break;
             case 9: out[i] = ptm_tanh(x[i]); break;
             case 10: out[i] = ptm_atanh(x[i]); break;
+            case 11: out[i] = ptm_asin(x[i]); break;
             default: out[i] = NAN;
         }
     }
@@ -2263,7 +2289,7 @@ void or_adjoint(const or_scene* S, const pt_program* prog, const pt_params* par,
                 float ndcy = (2.0f * filmy / (float)H) - 1.0f;
                 ndcx *= par->aspect;
                 v3 org, dir;
-                camera_ray(par, O, F, R, U, ndcx, ndcy, l0, l1, &org, &dir);
+                camera_ray(par, O, F, R, U, ndcx, ndcy, l0, l1, filmx, filmy, W, H, &org, &dir);
                 wl_t wl; memset(&wl, 0, sizeof wl);
                 memset(A.D, 0, sizeof A.D);
                 (void)transport_trace(S, &P, org, dir, &rng, &c1, &c2, &wl, &A);

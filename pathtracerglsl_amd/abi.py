@@ -7,7 +7,7 @@ import ctypes as C
 
 import numpy as np
 
-ABI_VERSION = 5
+ABI_VERSION = 6
 
 OBJ_MESH, OBJ_QUAD, OBJ_SPHERE, OBJ_INSTANCES = 0, 1, 2, 3
 MAT_LAMBERT, MAT_DIELECTRIC, MAT_GGX, MAT_NULL, MAT_MIRROR, MAT_ROUGH_DIELECTRIC = 0, 1, 2, 3, 4, 5
@@ -16,7 +16,8 @@ VS_ANALYTIC, VS_DELTA_TRACKING = 0, 1
 IOR_CONSTANT, IOR_CAUCHY, IOR_SELLMEIER = 0, 1, 2
 LIGHT_QUAD, LIGHT_POINT = 0, 1
 ENV_NONE, ENV_SKY, ENV_CONSTANT = 0, 1, 2
-CAM_PINHOLE, CAM_ORTHOGRAPHIC, CAM_THINLENS = 0, 1, 2
+CAM_PINHOLE, CAM_ORTHOGRAPHIC, CAM_THINLENS, CAM_FISHEYE, CAM_CYLINDRICAL, CAM_EQUIRECT = 0, 1, 2, 3, 4, 5
+FISHEYE_PROJECTIONS = {"equidistant": 0, "equisolid": 1, "stereographic": 2, "orthographic": 3}
 EST_PT, EST_NEE, EST_MIS = 0, 1, 2
 COLOR_RGB, COLOR_SPECTRAL = 0, 1
 WL_VISIBLE, WL_UNIFORM = 0, 1
@@ -112,7 +113,8 @@ class PtParams(C.Structure):
                 ("cam_right", C.c_float * 4), ("cam_up", C.c_float * 4),
                 ("tan_fov", C.c_float), ("aspect", C.c_float), ("reset_salt", C.c_uint32),
                 ("camera", C.c_int32), ("pixel_offset", C.c_int32 * 2), ("ortho_scale", C.c_float),
-                ("aperture", C.c_float), ("focus_distance", C.c_float), ("pad1", C.c_int32)]
+                ("aperture", C.c_float), ("focus_distance", C.c_float), ("projection", C.c_int32),
+                ("fisheye_k", C.c_float), ("cyl_focal", C.c_float)]
 
 
 ADJ_MAX_ALBEDO = 16

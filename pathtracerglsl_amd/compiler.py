@@ -83,6 +83,19 @@ def camera_params(camera, width, height, reset_salt=1234):
     if camera.get("type", "pinhole") == "orthographic":
         prm.camera = abi.CAM_ORTHOGRAPHIC
         prm.ortho_scale = float(np.float32(camera.get("scale", 1.0)))     # u_orthoScale
+    elif camera.get("type") == "fisheye":                                # fisheye.ts fisheyeK, in f64
+        prm.camera = abi.CAM_FISHEYE
+        proj = camera["projection"]
+        tmax = float(camera["fov"]) * 0.5
+        k = {"equisolid": math.sin(tmax * 0.5), "stereographic": math.tan(tmax * 0.5),
+             "orthographic": math.sin(tmax)}.get(proj, tmax)
+        prm.projection = abi.FISHEYE_PROJECTIONS[proj]
+        prm.fisheye_k = float(np.float32(k))
+    elif camera.get("type") == "cylindrical":                            # cylindrical.ts u_cylFocal
+        prm.camera = abi.CAM_CYLINDRICAL
+        prm.cyl_focal = float(np.float32(width / max(float(camera["hfov"]) * math.pi / 180.0, 1e-6)))
+    elif camera.get("type") == "equirect":
+        prm.camera = abi.CAM_EQUIRECT
     else:
         prm.camera = abi.CAM_PINHOLE
         prm.tan_fov = float(np.float32(math.tan(camera["fov"] / 2.0)))   # features/camera.ts:54-66
@@ -236,13 +249,24 @@ def _analyze_validate(scene, strategy):
     if strategy.color == "spectral" and scene.get("environment", {"type": "none"}).get("type", "none") == "procedural":
         raise CompileError("spectral programs with a procedural environment are not supported by this backend")
     cam = scene.get("camera", {})
-    if cam.get("type", "pinhole") not in ("pinhole", "orthographic", "thinlens"):
+    if cam.get("type", "pinhole") not in ("pinhole", "orthographic", "thinlens", "fisheye", "cylindrical", "equirect"):
         raise CompileError(f"camera '{scene['camera'].get('type')}' is not supported by this backend")
     if cam.get("type") == "thinlens":     # camera/thinlens/thinlens.ts:15-16
         if not float(cam.get("aperture", -1.0)) >= 0.0:
             raise CompileError("camera 'thinlens': aperture must be nonnegative")
         if not float(cam.get("focusDistance", cam.get("focus_distance", 0.0))) > 0.0:
             raise CompileError("camera 'thinlens': focusDistance must be positive")
+    if cam.get("type") == "fisheye":      # camera/fisheye/fisheye.ts:31-53
+        proj = cam.get("projection")
+        if proj not in abi.FISHEYE_PROJECTIONS:
+            raise CompileError(f"camera 'fisheye': projection must be one of {sorted(abi.FISHEYE_PROJECTIONS)}")
+        fov = float(cam.get("fov", 0.0))
+        top, inclusive = (math.pi, True) if proj == "orthographic" else (2 * math.pi, proj != "stereographic")
+        if not (fov > 0.0 and (fov <= top if inclusive else fov < top)):
+            raise CompileError(f"fisheye '{proj}' fov must be {'<=' if inclusive else '<'} "
+                               f"{'pi' if top == math.pi else '2 pi'} radians (got {fov})")
+    if cam.get("type") == "cylindrical" and not float(cam.get("hfov", 0.0)) > 0.0:
+        raise CompileError("camera 'cylindrical': hfov (degrees) must be positive")
     mats = scene.get("materials", {})
     for name, m in mats.items():
         if m.get("model") not in MODELS:

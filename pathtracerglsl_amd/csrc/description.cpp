@@ -187,7 +187,8 @@ bool lower(const JVal& d, int32_t width, int32_t height, int32_t fif, pt_program
     const JVal* cam = m->get("camera");
     // pinhole, orthographic and thin lens (the pt_params camera); the lens draws are consumed either way
     if (!cam || !(is_str(cam->get("type"), "pinhole") || is_str(cam->get("type"), "orthographic") ||
-                  is_str(cam->get("type"), "thinlens")))
+                  is_str(cam->get("type"), "thinlens") || is_str(cam->get("type"), "fisheye") ||
+                  is_str(cam->get("type"), "cylindrical") || is_str(cam->get("type"), "equirect")))
         return L.unsupported("measurement.camera.type", quoted(cam ? cam->get("type") : nullptr));
 
     // estimator (compiler/types.ts:642-721)

@@ -179,7 +179,18 @@ struct Camera {
     float ortho_scale;
     int thin;           // PT_CAM_THINLENS
     float aperture, focus;
+    int model;          // the PT_CAM_* value (fisheye / cylindrical / equirect branches)
+    int projection;     // fisheye radial map
+    float fk, cyl_f;
 };
+
+// fisheye.glsl FISHEYE_THETA: the radial map of each projection.
+PT_DEV float fisheye_theta(int projection, float rho, float K) {
+    if (projection == PT_FISHEYE_EQUISOLID) return 2.0f * ptm_asin(gclamp(rho * K, 0.0f, 1.0f));
+    if (projection == PT_FISHEYE_STEREOGRAPHIC) return 2.0f * ptm_atan(rho * K);
+    if (projection == PT_FISHEYE_ORTHOGRAPHIC) return ptm_asin(gclamp(rho * K, 0.0f, 1.0f));
+    return rho * K;
+}
 
 // path_state_init's current_medium = scene_region_at(camera origin), once per wave.
 template <int STACK>
@@ -226,6 +237,23 @@ PT_DEV void raygen_path(const Paths& P, const Camera& C, int W, int H, int i, in
     if (C.ortho) {
         org = (org + R * (ndcx * C.ortho_scale)) + U * (ndcy * C.ortho_scale);
         d = F;
+    } else if (C.model == PT_CAM_FISHEYE) {                    // camera/fisheye/fisheye.glsl:23-38
+        const float rho = sqrtf(ndcx * ndcx + ndcy * ndcy);
+        if (rho < 1e-6f) d = F;                                // centre pixel: on axis
+        else {
+            const float th = fisheye_theta(C.projection, rho, C.fk);
+            const float tx = ndcx / rho, ty = ndcy / rho;
+            d = normalize(F * ptm_cos(th) + (R * tx + U * ty) * ptm_sin(th));
+        }
+    } else if (C.model == PT_CAM_CYLINDRICAL) {                // camera/cylindrical/cylindrical.glsl:9-21
+        const float cx = filmx - 0.5f * (float)W, cy = filmy - 0.5f * (float)H;
+        const float phi = cx / C.cyl_f, hh = cy / C.cyl_f;
+        d = normalize((F * ptm_cos(phi) + R * ptm_sin(phi)) + U * hh);
+    } else if (C.model == PT_CAM_EQUIRECT) {                   // camera/equirect/equirect.glsl:9-22
+        const float u = filmx / (float)W, v = filmy / (float)H;
+        const float phi = (u - 0.5f) * PTM_TWO_PI, th = (v - 0.5f) * PTM_PI;
+        const float ct = ptm_cos(th);
+        d = normalize((F * ptm_cos(phi) + R * ptm_sin(phi)) * ct + U * ptm_sin(th));
     } else {
         d = normalize((F + R * (ndcx * C.tan_fov)) + U * (ndcy * C.tan_fov));
         if (C.thin) {                                              // camera/thinlens/thinlens.glsl:6-33
@@ -1980,7 +2008,17 @@ extern "C" {
 // The camera models pt_params can select, with the thin lens's parameter domain
 // (camera/thinlens/thinlens.ts:15-16: aperture nonnegative, focusDistance positive).
 static pt_status check_camera(const pt_params* prm) {
-    if (prm->camera == PT_CAM_PINHOLE || prm->camera == PT_CAM_ORTHOGRAPHIC) return PT_OK;
+    if (prm->camera == PT_CAM_PINHOLE || prm->camera == PT_CAM_ORTHOGRAPHIC || prm->camera == PT_CAM_EQUIRECT) return PT_OK;
+    if (prm->camera == PT_CAM_FISHEYE) {
+        if (prm->projection < PT_FISHEYE_EQUIDISTANT || prm->projection > PT_FISHEYE_ORTHOGRAPHIC)
+            return fail(PT_E_INVALID, "fisheye: unknown projection");
+        if (!(prm->fisheye_k > 0.0f) || !std::isfinite(prm->fisheye_k)) return fail(PT_E_INVALID, "fisheye: fisheye_k must be finite and > 0");
+        return PT_OK;
+    }
+    if (prm->camera == PT_CAM_CYLINDRICAL) {
+        if (!(prm->cyl_focal > 0.0f) || !std::isfinite(prm->cyl_focal)) return fail(PT_E_INVALID, "cylindrical: cyl_focal must be finite and > 0");
+        return PT_OK;
+    }
     if (prm->camera != PT_CAM_THINLENS) return fail(PT_E_UNSUPPORTED, "unknown camera");
     if (!(prm->aperture >= 0.0f) || !(prm->focus_distance > 0.0f) || !std::isfinite(prm->aperture) || !std::isfinite(prm->focus_distance))
         return fail(PT_E_INVALID, "thin lens: aperture must be >= 0 and focus_distance > 0");
@@ -2000,6 +2038,10 @@ static Camera make_camera(const pt_params* prm) {
     cam.thin = prm->camera == PT_CAM_THINLENS ? 1 : 0;
     cam.aperture = prm->aperture;
     cam.focus = prm->focus_distance;
+    cam.model = prm->camera;
+    cam.projection = prm->projection;
+    cam.fk = prm->fisheye_k;
+    cam.cyl_f = prm->cyl_focal;
     return cam;
 }
 
@@ -2755,6 +2797,7 @@ __global__ void k_debug_math(int fn, const float* x, const float* y, float* out,
         case 8: r = ptm_acos(a); break;
         case 9: r = ptm_tanh(a); break;
         case 10: r = ptm_atanh(a); break;
+        case 11: r = ptm_asin(a); break;
         case 20: r = a / b; break;
         case 21: r = sqrtf(a); break;
         case 22: r = 1.0f / sqrtf(a); break;

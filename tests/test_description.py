@@ -76,8 +76,8 @@ def test_complete_records_lower_like_their_pruned_form(name):
 
 def test_rejection_names_the_field():
     d = json.loads(json.dumps(FIX["cases"]["analytic-glass + pathtracer"]))
-    d["measurement"]["camera"]["type"] = "fisheye"
-    with pytest.raises(_lib.PtError, match="measurement.camera.type 'fisheye'"):
+    d["measurement"]["camera"]["type"] = "panoramic"
+    with pytest.raises(_lib.PtError, match="measurement.camera.type 'panoramic'"):
         _lower(d)
     d = json.loads(json.dumps(FIX["cases"]["analytic-glass + pathtracer"]))
     d["materials"]["models"].append("checker")

@@ -27,7 +27,7 @@ def cpu(oracle, fn, x, y):
 
 
 CASES = [(0, -30, 30), (1, -30, 30), (2, -80, 80), (3, 1e-6, 1e4), (4, -5, 5), (5, -1.5, 1.5), (6, 0, 1), (6, -1, 0),
-         (7, -40, 40), (8, -1, 1), (9, -6, 6), (10, -0.99, 0.99), (20, -5, 5), (21, 0, 100), (22, 1e-3, 100),
+         (7, -40, 40), (8, -1, 1), (9, -6, 6), (10, -0.99, 0.99), (11, -1, 1), (20, -5, 5), (21, 0, 100), (22, 1e-3, 100),
          (23, 0, 100), (24, 0.01, 10), (25, -3, 3), (26, -3, 3)]
 
 

@@ -308,3 +308,23 @@ def test_rough_glass_cornell_matches_oracle(engine, oracle, est):
     assert o[..., :3].mean() > 0.05
     assert bad == 0.0
     assert engine.ray_counts() == (int(counts[0]), int(counts[1]))
+
+
+@pytest.mark.parametrize("camera", [{"type": "fisheye", "projection": "equisolid", "fov": 3.4},
+                                    {"type": "fisheye", "projection": "stereographic", "fov": 2.2},
+                                    {"type": "cylindrical", "hfov": 220.0},
+                                    {"type": "equirect"}])
+def test_wide_cameras_cornell_matches_oracle(engine, oracle, camera):
+    sc = scenes.cornell_box()
+    sc["environment"] = {"type": "procedural", "intensity": 1.0}
+    sc["camera"] = dict(camera, position=[0.0, 0.0, 0.6], target=[0.0, 0.0, -1.0])
+    comp = scene_cache.compiler.compile_scene(sc, scene_cache.compiler.RenderStrategy("mis", 8, 3), 40, 32)
+    engine.load_renderer(comp)
+    engine.clear_accumulation()
+    engine.render_frames(8)
+    g = engine.read_export()
+    o, _, counts = oracle_lib.OracleScene(oracle, comp.scene).render(comp.program, comp.params(), 8)
+    exact, bad = _compare_film(f"cornell-{camera['type']}-{camera.get('projection', '')}", g, o)
+    assert o[..., :3].mean() > 0.05
+    assert bad == 0.0
+    assert engine.ray_counts() == (int(counts[0]), int(counts[1]))

@@ -81,3 +81,7 @@ def test_tan_pow_tanh_atanh(oracle):
     assert np.abs(run(oracle, 9, t) - np.tanh(t.astype(np.float64))).max() < 3e-7
     a = rng.uniform(-0.99, 0.99, 50000).astype(np.float32)
     assert np.abs(run(oracle, 10, a) - np.arctanh(a.astype(np.float64))).max() < 2e-6
+    s = np.concatenate([rng.uniform(-1.0, 1.0, 50000), [-1.0, -0.5, 0.0, 0.5, 1.0]]).astype(np.float32)
+    got = run(oracle, 11, s)
+    assert np.abs(got - np.arcsin(s.astype(np.float64))).max() < 2e-6
+    np.testing.assert_allclose(got[-5:], [-np.pi / 2, -np.pi / 6, 0.0, np.pi / 6, np.pi / 2], atol=3e-7)
