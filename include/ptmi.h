@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define PT_ABI_VERSION 6u
+#define PT_ABI_VERSION 7u
 
 typedef enum pt_status {
     PT_OK = 0,
@@ -245,7 +245,15 @@ typedef struct pt_program {
     int32_t  variance;        /* != 0: also accumulate the variance picture (export "variance",
                                  components/accumulator/variance/variance.glsl) */
     int32_t  volume_sampling; /* PT_VS_*: estimator.volumeSampling (validation: the arms are per material) */
+    int32_t  aovs;            /* PT_AOV_* bits: measurement.aovs, the first-hit guide pictures (RGB only) */
 } pt_program;
+
+/* First-hit guides (src/components/aov/): at the first non-null surface a path reaches with no
+ * scattering event before it, the material's albedo fact clamped to [0, 1] times the path's
+ * weight on arrival ("albedo"), and the shading normal faced toward the arriving ray times the
+ * weight's channel mean ("normal"); 0 where no surface is reached. Each is averaged like the
+ * radiance; pt_read_export(r, "albedo" | "normal", ...) reads them. */
+enum { PT_AOV_ALBEDO = 1, PT_AOV_NORMAL = 2 };
 
 /* Live per-frame values (the engine builtins + camera uniforms,
  * src/engine/Engine.ts:231-254, src/components/camera/basis.ts:50-56). */

@@ -1652,7 +1652,7 @@ static sp equiangular_sample_direct(const or_scene* S, const or_program* P, v3 o
 }
 
 static sp transport_trace(const or_scene* S, const or_program* P, v3 ro, v3 rd, rng_t* rng, int* n_closest, int* n_any,
-                          const wl_t* wl, adj_t* A) {
+                          const wl_t* wl, adj_t* A, float* aov) {
     sp T = sgrey(1.0f), L = sgrey(0.0f);
     int prev_was_delta = 1; float prev_bsdf_pdf = 0.0f; v3 prev_qp = ro;
     float eta_scale = 1.0f;
@@ -1773,6 +1773,19 @@ static sp transport_trace(const or_scene* S, const or_program* P, v3 ro, v3 rd, 
         }
         const pt_material* M = &S->materials[mat];
         v3 wo = neg(rd);
+        /* the first-hit guides (pt.ts:422-433): first non-null surface, no event before it; aov =
+         * albedo fact clamped to [0, 1] x weight (aov/albedo/albedo.glsl), faced shading normal x
+         * the weight's channel mean (aov/normal/normal.glsl) */
+        if (aov && bounce == 0 && !wl->spec) {
+            const float* row = M->model == PT_MAT_LAMBERT ? M->albedo
+                             : (M->model == PT_MAT_GGX || M->model == PT_MAT_MIRROR) ? M->f0 : M->transmittance;
+            aov[0] = clampf(row[0], 0.0f, 1.0f) * T.x;
+            aov[1] = clampf(row[1], 0.0f, 1.0f) * T.y;
+            aov[2] = clampf(row[2], 0.0f, 1.0f) * T.z;
+            v3 n = dot(hit.frame.n, wo) < 0.0f ? neg(hit.frame.n) : hit.frame.n;
+            float avg = ((T.x + T.y) + T.z) * (1.0f / 3.0f);
+            aov[3] = n.x * avg; aov[4] = n.y * avg; aov[5] = n.z * avg;
+        }
         /* kernel_score_emitter_hit */
         int mat_emit = material_of(S, hit.region_to);
         if (mat_emit >= 0 && S->materials[mat_emit].emissive) {
@@ -2128,8 +2141,32 @@ static void camera_ray(const pt_params* par, v3 O, v3 F, v3 R, v3 U, float ndcx,
  * following features/accumulation.ts:14-51 and accumulator/average/average.glsl:6-10.
  * Optional per-sample radiance output `samples` [nframes][H][W][3] (may be NULL).
  * Rows [y0, y1) only (a bounded sample); counts of closest/any queries are returned. */
+/* average_update (components/accumulator/average): the running mean of the samples in order */
+static void avg_update(float* px, uint32_t n, float x, float y, float z) {
+    if (n == 0) { px[0] = x; px[1] = y; px[2] = z; }
+    else {
+        float w = 1.0f / ((float)n + 1.0f);
+        px[0] = px[0] * (1.0f - w) + x * w;
+        px[1] = px[1] * (1.0f - w) + y * w;
+        px[2] = px[2] * (1.0f - w) + z * w;
+    }
+    px[3] = 1.0f;
+}
+static void render_impl(const or_scene* S, const pt_program* prog, const pt_params* par, uint32_t sample_count,
+                        int nframes, int y0, int y1, float* film, float* samples, uint64_t* counts, float* aov_a,
+                        float* aov_n);
 void or_render(const or_scene* S, const pt_program* prog, const pt_params* par, uint32_t sample_count,
                int nframes, int y0, int y1, float* film, float* samples, uint64_t* counts) {
+    render_impl(S, prog, par, sample_count, nframes, y0, y1, film, samples, counts, NULL, NULL);
+}
+/* The same render with the first-hit guide films (either may be NULL). */
+void or_render_aovs(const or_scene* S, const pt_program* prog, const pt_params* par, uint32_t sample_count,
+                    int nframes, float* film, float* aov_albedo, float* aov_normal, uint64_t* counts) {
+    render_impl(S, prog, par, sample_count, nframes, 0, prog->height, film, NULL, counts, aov_albedo, aov_normal);
+}
+static void render_impl(const or_scene* S, const pt_program* prog, const pt_params* par, uint32_t sample_count,
+                        int nframes, int y0, int y1, float* film, float* samples, uint64_t* counts, float* aov_a,
+                        float* aov_n) {
     or_program P;
     P.estimator = prog->estimator; P.max_bounces = prog->max_bounces; P.rr_start = prog->rr_start;
     P.rr_max = prog->rr_max_survival; P.width = prog->width; P.height = prog->height;
@@ -2164,7 +2201,8 @@ void or_render(const or_scene* S, const pt_program* prog, const pt_params* par, 
                 ndcx *= par->aspect;
                 v3 org, dir;
                 camera_ray(par, O, F, R, U, ndcx, ndcy, l0, l1, filmx, filmy, W, H, &org, &dir);
-                sp Ls = transport_trace(S, &P, org, dir, &rng, &c1, &c2, &wl, NULL);
+                float aov[6] = {0, 0, 0, 0, 0, 0};
+                sp Ls = transport_trace(S, &P, org, dir, &rng, &c1, &c2, &wl, NULL, (aov_a || aov_n) ? aov : NULL);
                 v3 Lr = V(Ls.x, Ls.y, Ls.z);
                 if (P.spectral) {                   /* color_to_pixel (color/spectral/spectral.glsl:27-33) */
                     v3 pix = V(0.0f, 0.0f, 0.0f);
@@ -2176,14 +2214,9 @@ void or_render(const or_scene* S, const pt_program* prog, const pt_params* par, 
                     float* sp = samples + 3 * (((size_t)f * H + y) * W + x);
                     sp[0] = Lr.x; sp[1] = Lr.y; sp[2] = Lr.z;
                 }
-                if (n == 0) { px[0] = Lr.x; px[1] = Lr.y; px[2] = Lr.z; }
-                else {
-                    float w = 1.0f / ((float)n + 1.0f);
-                    px[0] = px[0] * (1.0f - w) + Lr.x * w;
-                    px[1] = px[1] * (1.0f - w) + Lr.y * w;
-                    px[2] = px[2] * (1.0f - w) + Lr.z * w;
-                }
-                px[3] = 1.0f;
+                avg_update(px, n, Lr.x, Lr.y, Lr.z);
+                if (aov_a) avg_update(aov_a + 4 * ((size_t)y * W + x), n, aov[0], aov[1], aov[2]);
+                if (aov_n) avg_update(aov_n + 4 * ((size_t)y * W + x), n, aov[3], aov[4], aov[5]);
             }
             nc += (uint64_t)c1; na += (uint64_t)c2;
         }
@@ -2292,7 +2325,7 @@ void or_adjoint(const or_scene* S, const pt_program* prog, const pt_params* par,
                 camera_ray(par, O, F, R, U, ndcx, ndcy, l0, l1, filmx, filmy, W, H, &org, &dir);
                 wl_t wl; memset(&wl, 0, sizeof wl);
                 memset(A.D, 0, sizeof A.D);
-                (void)transport_trace(S, &P, org, dir, &rng, &c1, &c2, &wl, &A);
+                (void)transport_trace(S, &P, org, dir, &rng, &c1, &c2, &wl, &A, NULL);
             }
             for (int k = 0; k < ADJ_N; k++) gr[k] += A.G[k];
         }

@@ -39,6 +39,7 @@ class RenderStrategy:
     wavelengths: str = "visible"          # estimator.wavelengths (spectral): 'visible' | 'uniform'
     medium_light_sampling: str = "vertex" # estimator.mediumLightSampling: 'vertex' | 'equiangular'
     variance: bool = False                # accumulate the variance statistic picture
+    aovs: tuple = ()                      # measurement.aovs: first-hit guides 'albedo' / 'normal' (RGB)
     volume_sampling: str = "analytic"     # estimator.volumeSampling: 'analytic' | 'delta-tracking'
     scattering: str = "full"              # measurement.scattering: 'full' | 'ignored' (absorbing-only media)
 
@@ -249,6 +250,11 @@ def _analyze_validate(scene, strategy):
     if strategy.color == "spectral" and scene.get("environment", {"type": "none"}).get("type", "none") == "procedural":
         raise CompileError("spectral programs with a procedural environment are not supported by this backend")
     cam = scene.get("camera", {})
+    for a in strategy.aovs:
+        if a not in ("albedo", "normal"):
+            raise CompileError(f"unknown aov '{a}' (known: albedo, normal)")
+    if strategy.aovs and strategy.color != "rgb":
+        raise CompileError("first-hit guides (aovs) are provided for RGB programs only by this backend")
     if cam.get("type", "pinhole") not in ("pinhole", "orthographic", "thinlens", "fisheye", "cylindrical", "equirect"):
         raise CompileError(f"camera '{scene['camera'].get('type')}' is not supported by this backend")
     if cam.get("type") == "thinlens":     # camera/thinlens/thinlens.ts:15-16
@@ -598,6 +604,7 @@ def compile_scene(scene, strategy=None, width=256, height=256, reset_salt=1234):
     prog.medium_light_sampling = abi.MLS_EQUIANGULAR if strategy.medium_light_sampling == "equiangular" else abi.MLS_VERTEX
     prog.volume_sampling = abi.VS_DELTA_TRACKING if strategy.volume_sampling == "delta-tracking" else abi.VS_ANALYTIC
     prog.variance = int(bool(strategy.variance))
+    prog.aovs = sum({"albedo": abi.AOV_ALBEDO, "normal": abi.AOV_NORMAL}[a] for a in set(strategy.aovs))
     prog.width, prog.height = width, height
     prog.frames_in_flight = max(1, strategy.frames_in_flight)
     prog.max_null_crossings = strategy.max_null_crossings
@@ -674,6 +681,7 @@ def describe(scene, strategy):
             "maxNullCrossings": strategy.max_null_crossings, "response": "radiance",
             "scattering": strategy.scattering, "sensor": "cie1931" if strategy.color == "spectral" else "ideal",
             "shadows": "opaque-dielectrics",
+            **({"aovs": list(strategy.aovs)} if strategy.aovs else {}),
         },
         "media": {"deflecting": False, "heterogeneousArms": False, "models": media_models},
         "spectrumKinds": [],

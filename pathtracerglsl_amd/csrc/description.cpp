@@ -182,8 +182,15 @@ bool lower(const JVal& d, int32_t width, int32_t height, int32_t fif, pt_program
     if (const JVal* a = m->get("ambient"); a && !is_str(a, "euclidean")) return L.unsupported("measurement.ambient", quoted(a));
     if (const JVal* sc = m->get("scattering"); sc && !is_str(sc, "full") && !is_str(sc, "ignored"))
         return L.unsupported("measurement.scattering", quoted(sc));
-    if (const JVal* aov = m->get("aovs"); aov && aov->kind == JVal::ARR && !aov->arr.empty())
-        return L.unsupported("measurement.aovs", "(first-hit guides)");
+    // measurement.aovs: the first-hit guides (components/aov/index.ts AOVS), RGB programs only
+    if (const JVal* aov = m->get("aovs"); aov && aov->kind == JVal::ARR && !aov->arr.empty()) {
+        if (pg.color != PT_COLOR_RGB) return L.unsupported("measurement.aovs", "(first-hit guides of a spectral program)");
+        for (const JVal& a : aov->arr) {
+            if (is_str(&a, "albedo")) pg.aovs |= PT_AOV_ALBEDO;
+            else if (is_str(&a, "normal")) pg.aovs |= PT_AOV_NORMAL;
+            else return L.unsupported("measurement.aovs entry", quoted(&a));
+        }
+    }
     const JVal* cam = m->get("camera");
     // pinhole, orthographic and thin lens (the pt_params camera); the lens draws are consumed either way
     if (!cam || !(is_str(cam->get("type"), "pinhole") || is_str(cam->get("type"), "orthographic") ||
@@ -240,7 +247,9 @@ bool lower(const JVal& d, int32_t width, int32_t height, int32_t fif, pt_program
                 if (!(is_str(&md, "lambert") || is_str(&md, "dielectric") || is_str(&md, "ggx") || is_str(&md, "none") ||
                       is_str(&md, "mirror") || is_str(&md, "rough_dielectric")))
                     return L.unsupported("material model", quoted(&md));
-        if (truthy(mats->get("twoSidedShading"))) return L.unsupported("materials.twoSidedShading", "true");
+        // materials.twoSidedShading is read only by the light tree's horizon cull
+        // (accel/light_tree/light_tree.glsl, through LightQuery.two_sided); the 'power' selection
+        // this backend lowers ignores the query, so the flag changes no estimate here.
         if (truthy(mats->get("materialsReadUv"))) return L.unsupported("materials.materialsReadUv", "true (textures)");
         if (truthy(mats->get("albedo"))) return L.unsupported("materials.albedo", "true (albedo guide)");
         if (const JVal* cones = mats->get("emissionCones"); cones && cones->kind == JVal::ARR && !cones->arr.empty())

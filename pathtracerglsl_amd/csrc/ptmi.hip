@@ -70,6 +70,7 @@ struct Paths {
     float4* s_c;     // partial contribution.xyz (before visibility), MIS weight
     float4* s_l;     // light point.xyz (media walker), MIS weight (spectral)
     float4* wl;      // hero wavelengths (spectral)
+    float4* aov[2];  // first-hit guide samples (albedo, normal) of the paths in flight, or null
     float* eta;      // eta_scale (spectral; RGB keeps it in thr.w)
     unsigned long long* hist;   // [0] closest queries, [1] any-hit queries
     unsigned long long* trav;   // [0] nodes visited, [1] triangles tested (COUNT builds)
@@ -268,6 +269,8 @@ PT_DEV void raygen_path(const Paths& P, const Camera& C, int W, int H, int i, in
     P.thr[i] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
     P.rad[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     P.rng[i] = make_uint2(r.base, r.dim);
+    for (int a = 0; a < 2; a++)
+        if (P.aov[a]) P.aov[a][i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);   // 0 unless a surface is reached
     P.prev[i] = f4(org, 0.0f);
     P.flags[i] = mk_flags(true, 0, 0);
     if (media) P.medium[i] = P.counts[C_CAM_MEDIUM];
@@ -809,6 +812,21 @@ __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene Sk, Paths P, int qsel
                 const pt_material& M = S.materials[S.objects[h.owner].material];
                 bool need = PROBE || (S.emissive_solid && back);
                 classify_hit<SBLOCK>(S, h, back, need, stk);
+                // first-hit guides (pt.ts:422-433): the first non-null surface with no event before
+                if (!SPEC && CAT != CAT_NULL && bounce == 0 && (P.aov[0] || P.aov[1])) {
+                    const sp4 Ta = T;                            // the path's weight on arrival
+                    if (P.aov[0]) {                                  // aov/albedo/albedo.glsl
+                        const float* row = MODEL == PT_MAT_LAMBERT ? M.albedo
+                                         : (MODEL == PT_MAT_GGX || MODEL == PT_MAT_MIRROR) ? M.f0 : M.transmittance;
+                        P.aov[0][i] = make_float4(gclamp(row[0], 0.0f, 1.0f) * Ta.x, gclamp(row[1], 0.0f, 1.0f) * Ta.y,
+                                                  gclamp(row[2], 0.0f, 1.0f) * Ta.z, 0.0f);
+                    }
+                    if (P.aov[1]) {                                  // aov/normal/normal.glsl
+                        const f3 n = dot(h.frame.n, -rd) < 0.0f ? -h.frame.n : h.frame.n;
+                        const float avg = ((Ta.x + Ta.y) + Ta.z) * (1.0f / 3.0f);
+                        P.aov[1][i] = make_float4(n.x * avg, n.y * avg, n.z * avg, 0.0f);
+                    }
+                }
                 if (MEDIA && h.region_from != medium) medium = h.region_from;   // self-heal
                 if (CAT == CAT_NULL) {
                     // null interface: not an optical event; no bounce consumed
@@ -984,9 +1002,26 @@ __global__ void __launch_bounds__(SBLOCK) k_adj_atomic(const Paths P, int n_path
 // (color/spectral/spectral.glsl:27-33) with the cie1931 sensor first.
 template <bool SPEC>
 __global__ void __launch_bounds__(SBLOCK) k_accumulate(const DevScene S, const Paths P, float4* film, int npix, int frames,
-                                                       uint32_t sample0, int wl_uniform, float4* var) {
+                                                       uint32_t sample0, int wl_uniform, float4* var, float4* aov0,
+                                                       float4* aov1) {
     int pix = blockIdx.x * blockDim.x + threadIdx.x;
     if (pix >= npix) return;
+    // the guide pictures: average_update of their samples, in sample order
+    float4* aovf[2] = {aov0, aov1};
+    for (int a = 0; a < 2; a++) {
+        if (!aovf[a] || !P.aov[a]) continue;
+        float4 g = aovf[a][pix];
+        for (int f = 0; f < frames; f++) {
+            const uint32_t n = sample0 + (uint32_t)f;
+            const float4 x = P.aov[a][(size_t)f * npix + pix];
+            if (n == 0) g = make_float4(x.x, x.y, x.z, 1.0f);
+            else {
+                float w = 1.0f / ((float)n + 1.0f);
+                g = make_float4(g.x * (1.0f - w) + x.x * w, g.y * (1.0f - w) + x.y * w, g.z * (1.0f - w) + x.z * w, 1.0f);
+            }
+        }
+        aovf[a][pix] = g;
+    }
     float4 acc = film[pix];
     float4 v = var ? var[pix] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     for (int f = 0; f < frames; f++) {
@@ -1292,11 +1327,11 @@ struct pt_ctx {
 struct pt_renderer {
     pt_ctx* ctx = nullptr;
     pt_program prog{};
-    DevBuf film, var_film;
+    DevBuf film, var_film, aov_film[2];
     uint32_t sample_count = 0;
     long long cap = 0;    // path slots allocated
     DevBuf b_ray_o, b_ray_d, b_thr, b_rad, b_rng, b_prev, b_flags, b_medium, b_hit, b_q0, b_q1, b_counts,
-        b_spath, b_so, b_sd, b_sc, b_sl, b_wl, b_eta, b_hist, b_trav, b_cat[N_CAT];
+        b_spath, b_so, b_sd, b_sc, b_sl, b_wl, b_eta, b_hist, b_trav, b_cat[N_CAT], b_aov[2];
     // adjoint buffers (allocated by the first pt_backward)
     DevBuf b_adjD, b_adjG, b_adjsD, b_delta, b_pixgrad, b_grad;
     DevBuf b_pscene;      // scene of each path slot (batched wavefront)
@@ -1849,6 +1884,9 @@ pt_status pt_renderer_load(pt_ctx* ctx, const pt_program* prog, pt_renderer** ou
     if (prog->color == PT_COLOR_SPECTRAL && ctx->has_scene && ctx->dev.env_type == PT_ENV_SKY)
         return fail(PT_E_UNSUPPORTED, "spectral programs with the procedural sky are not provided by this build");
     if (prog->width <= 0 || prog->height <= 0 || prog->max_bounces < 0) return fail(PT_E_INVALID, "bad film size or bounce budget");
+    if (prog->aovs & ~(PT_AOV_ALBEDO | PT_AOV_NORMAL)) return fail(PT_E_INVALID, "unknown aov bits");
+    if (prog->aovs && prog->color != PT_COLOR_RGB)
+        return fail(PT_E_UNSUPPORTED, "first-hit guides (aovs) are provided for RGB programs only");
     HIP_TRY(hipSetDevice(ctx->device));
     auto* r = new pt_renderer();
     r->ctx = ctx;
@@ -1859,6 +1897,12 @@ pt_status pt_renderer_load(pt_ctx* ctx, const pt_program* prog, pt_renderer** ou
     if (prog->variance) {
         r->var_film.bytes = r->film.bytes;
         if (hipMalloc(&r->var_film.p, r->var_film.bytes) != hipSuccess) { delete r; return fail(PT_E_OOM, "film allocation failed"); }
+    }
+    for (int a = 0; a < 2; a++) {
+        if (!(prog->aovs & (1 << a))) continue;
+        r->aov_film[a].bytes = r->film.bytes;
+        if (hipMalloc(&r->aov_film[a].p, r->aov_film[a].bytes) != hipSuccess) { delete r; return fail(PT_E_OOM, "film allocation failed"); }
+        (void)hipMemsetAsync(r->aov_film[a].p, 0, r->aov_film[a].bytes, ctx->stream);
     }
     (void)hipMemsetAsync(r->film.p, 0, r->film.bytes, ctx->stream);
     *out = r;
@@ -1882,6 +1926,8 @@ static pt_status ensure_paths(pt_renderer* r, long long n) {
         {&r->b_flags, 4}, {&r->b_hit, 8}, {&r->b_q0, 4}, {&r->b_q1, 4}, {&r->b_spath, 4}, {&r->b_so, 16},
         {&r->b_sd, 16}, {&r->b_sc, 16}, {&r->b_sl, 16}, {&r->b_medium, 4}, {&r->b_wl, 16}, {&r->b_eta, 4}, {&r->b_pscene, 4}};
     for (int c = 0; c < N_CAT; c++) list.push_back({&r->b_cat[c], 4});   // one queue per shading category
+    for (int a = 0; a < 2; a++)
+        if (r->prog.aovs & (1 << a)) list.push_back({&r->b_aov[a], 16});  // first-hit guide samples
     for (auto& e : list) {
         if (e.b->p) { HIP_TRY(hipFree(e.b->p)); e.b->p = nullptr; }
         e.b->bytes = e.elem * (size_t)n;
@@ -1903,6 +1949,7 @@ static pt_status ensure_paths(pt_renderer* r, long long n) {
     P.medium = (int*)r->b_medium.p; P.wl = (float4*)r->b_wl.p; P.eta = (float*)r->b_eta.p;
     P.path_scene = (int*)r->b_pscene.p; P.hist = (unsigned long long*)r->b_hist.p;
     P.trav = (unsigned long long*)r->b_trav.p;
+    for (int a = 0; a < 2; a++) P.aov[a] = (r->prog.aovs & (1 << a)) ? (float4*)r->b_aov[a].p : nullptr;
     for (int c = 0; c < N_CAT; c++) {
         P.catq[c] = (int*)r->b_cat[c].p;
         if (!P.catq[c]) return fail(PT_E_OOM, "ensure_paths: a shading queue was not allocated");
@@ -1998,7 +2045,7 @@ static void launch_wave(pt_renderer* r, const Camera& cam, int frames, uint32_t 
             TIMED(ctx, "adj_reduce", n, hipLaunchKernelGGL(k_adj_atomic, dim3((unsigned)((n + SBLOCK - 1) / SBLOCK)), dim3(SBLOCK), 0, st, P, (int)n, (float*)r->b_grad.p));
         return;
     }
-    TIMED(ctx, "accumulate", npix, hipLaunchKernelGGL((k_accumulate<SPEC>), dim3((npix + SBLOCK - 1) / SBLOCK), dim3(SBLOCK), 0, st, S, P, (float4*)r->film.p, npix, frames, sample0, pg.wavelengths, (float4*)r->var_film.p));
+    TIMED(ctx, "accumulate", npix, hipLaunchKernelGGL((k_accumulate<SPEC>), dim3((npix + SBLOCK - 1) / SBLOCK), dim3(SBLOCK), 0, st, S, P, (float4*)r->film.p, npix, frames, sample0, pg.wavelengths, (float4*)r->var_film.p, (float4*)r->aov_film[0].p, (float4*)r->aov_film[1].p));
 }
 
 extern "C" {
@@ -2344,6 +2391,8 @@ pt_status pt_batch_create(pt_ctx* ctx, int32_t n, const pt_scene_desc* scenes, c
     if (n <= 0) return fail(PT_E_INVALID, "pt_batch_create: no scenes");
     for (int s = 0; s < n; s++)
         if (pt_status sc = check_camera(&params[s]); sc != PT_OK) return sc;
+    for (int s = 0; s < n; s++)
+        if (programs[s].aovs) return fail(PT_E_UNSUPPORTED, "pt_batch_create: first-hit guides (aovs) are not batched");
     const pt_program& p0 = programs[0];
     for (int s = 0; s < n; s++) {
         const pt_program& p = programs[s];
@@ -2551,12 +2600,16 @@ pt_status pt_scene_update_materials(pt_ctx* ctx, const pt_material* materials, i
 pt_status pt_read_export(pt_renderer* r, const char* name, float* out, size_t out_floats) {
     if (!r || !name || !out) return fail(PT_E_INVALID, "pt_read_export: null argument");
     const bool var = std::strcmp(name, "variance") == 0;
-    if (std::strcmp(name, "radiance") != 0 && !var) return fail(PT_E_INVALID, std::string("unknown export '") + name + "'");
+    const int aov = std::strcmp(name, "albedo") == 0 ? 0 : std::strcmp(name, "normal") == 0 ? 1 : -1;
+    if (std::strcmp(name, "radiance") != 0 && !var && aov < 0) return fail(PT_E_INVALID, std::string("unknown export '") + name + "'");
     if (var && !r->var_film.p) return fail(PT_E_MISSING_INPUT, "missing export: 'variance' (compile the program with variance on)");
+    if (aov >= 0 && !r->aov_film[aov].p)
+        return fail(PT_E_MISSING_INPUT, std::string("missing export: '") + name + "' (list it in the program's aovs)");
     size_t need = (size_t)r->prog.width * r->prog.height * 4;
     if (out_floats < need) return fail(PT_E_INVALID, "pt_read_export: output buffer too small");
     HIP_TRY(hipSetDevice(r->ctx->device));
-    HIP_TRY(hipMemcpyAsync(out, var ? r->var_film.p : r->film.p, need * sizeof(float), hipMemcpyDeviceToHost, r->ctx->stream));
+    const void* src = aov >= 0 ? r->aov_film[aov].p : var ? r->var_film.p : r->film.p;
+    HIP_TRY(hipMemcpyAsync(out, src, need * sizeof(float), hipMemcpyDeviceToHost, r->ctx->stream));
     HIP_TRY(hipStreamSynchronize(r->ctx->stream));
     return PT_OK;
 }
@@ -2566,6 +2619,8 @@ pt_status pt_clear_accumulation(pt_renderer* r) {
     HIP_TRY(hipSetDevice(r->ctx->device));
     HIP_TRY(hipMemsetAsync(r->film.p, 0, r->film.bytes, r->ctx->stream));
     if (r->var_film.p) HIP_TRY(hipMemsetAsync(r->var_film.p, 0, r->var_film.bytes, r->ctx->stream));
+    for (int a = 0; a < 2; a++)
+        if (r->aov_film[a].p) HIP_TRY(hipMemsetAsync(r->aov_film[a].p, 0, r->aov_film[a].bytes, r->ctx->stream));
     r->sample_count = 0;
     return PT_OK;
 }

@@ -37,6 +37,8 @@ def load(variant=""):
     lib.or_env_total.argtypes = [vp]
     lib.or_region_at.argtypes = [vp, vp]
     lib.or_concentric_disk.argtypes = [vp, C.c_int, vp]
+    lib.or_render_aovs.argtypes = [vp, vp, vp, C.c_uint32, C.c_int, vp, vp, vp, vp]
+    lib.or_render_aovs.restype = None
     lib.or_concentric_disk.restype = None
     lib.or_bsdf_probe.argtypes = [vp, vp, vp, C.c_float, C.c_float, C.c_float, vp]
     lib.or_env_pdf.argtypes = [vp, vp, C.c_int, vp]
@@ -86,6 +88,15 @@ class OracleScene:
                            film.ctypes.data, samples.ctypes.data if samples is not None else None,
                            counts.ctypes.data)
         return film, samples, counts
+
+    def render_aovs(self, program, params, nframes, sample_count=0):
+        """Film and the two first-hit guide films (albedo, normal), each [H, W, 4]."""
+        W, H = program.width, program.height
+        film, alb, nrm = (np.zeros((H, W, 4), np.float32) for _ in range(3))
+        counts = np.zeros(2, np.uint64)
+        self.lib.or_render_aovs(self.h, C.byref(program), C.byref(params), sample_count, nframes, film.ctypes.data,
+                                alb.ctypes.data, nrm.ctypes.data, counts.ctypes.data)
+        return film, alb, nrm, counts
 
     def adjoint(self, program, params, nframes, adj, dL):
         """FP64 replay of the path-replay adjoint: d(sum dL . film)/dtheta."""

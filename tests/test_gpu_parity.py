@@ -328,3 +328,28 @@ def test_wide_cameras_cornell_matches_oracle(engine, oracle, camera):
     assert o[..., :3].mean() > 0.05
     assert bad == 0.0
     assert engine.ray_counts() == (int(counts[0]), int(counts[1]))
+
+
+@pytest.mark.parametrize("case", ["cornell", "slab", "smooth"])
+def test_first_hit_guides_match_oracle(engine, oracle, case):
+    if case == "cornell":
+        sc, st = scenes.cornell_box(), scene_cache.compiler.RenderStrategy("mis", 8, 3, aovs=("albedo", "normal"))
+    elif case == "slab":
+        sc = scenes.orthographic(scenes.slab_witness())
+        sc["materials"]["screen"]["albedo"] = [0.5, 0.6, 0.7]
+        st = scene_cache.compiler.RenderStrategy("none", 4, -1, aovs=("albedo", "normal"))
+    else:
+        sc, st = scenes.smooth_sphere("smooth"), scene_cache.compiler.RenderStrategy("nee", 6, -1, aovs=("normal",))
+    comp = scene_cache.compiler.compile_scene(sc, st, 32, 24)
+    engine.load_renderer(comp)
+    engine.clear_accumulation()
+    engine.render_frames(8)
+    g = {name: engine.read_export(name) for name in ("radiance",) + tuple(st.aovs)}
+    o, alb, nrm, counts = oracle_lib.OracleScene(oracle, comp.scene).render_aovs(comp.program, comp.params(), 8)
+    ref = {"radiance": o, "albedo": alb, "normal": nrm}
+    for name, img in g.items():
+        same = np.array_equal(img[..., :3], ref[name][..., :3])
+        _say(f"[gpu] guides {case} {name}: bit-identical {same}, mean {ref[name][..., :3].mean():.5f}")
+        assert same, name
+    assert np.abs(ref[st.aovs[0]][..., :3]).mean() > 0.01
+    assert engine.ray_counts() == (int(counts[0]), int(counts[1]))
