@@ -167,8 +167,8 @@ def main():
                     help="run a plain device copy of this size first (counter calibration runs)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "the path would shard by image tiles, see DESIGN.md", file=sys.stderr)
+        print(f"bench.py: runs on one GPU only (asked for --gpus {args.gpus}); a multi-GPU run would "
+              "shard the frame by image tiles, one process per GPU", file=sys.stderr)
         sys.exit(2)
 
     from pathtracerglsl_amd import compiler, scenes
