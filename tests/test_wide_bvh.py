@@ -58,3 +58,39 @@ def test_icosphere_cloud_structure():
     tv = np.concatenate([o["triangles"].reshape(-1, 9) for o in sc["objects"]]).astype(np.float32)
     depth = check(tv)
     assert depth <= 16
+
+
+def test_leaf_caps_and_node_budget():
+    """cwbvh.test.ts's caps: a leaf slot holds 1..3 triangles, a node's leaf triangles fit the
+    24 addressable offsets, every node has a child (the SAH collapse never emits an empty node)."""
+    for n, seed in ((9, 7), (300, 8), (5000, 9)):
+        nodes, tris, _ = wide_bvh_ref.build(_soup(n, seed))
+        for k in range(len(nodes)):
+            ch = wide_bvh_ref.decode(nodes[k])
+            assert len(ch) > 0
+            for c in ch:
+                if not c["internal"]:
+                    cnt = len(c["tris"])
+                    assert 1 <= cnt <= 3 and (c["meta"] & 31) + cnt <= 24
+
+
+def test_adversarial_extents():
+    """cwbvh.test.ts 'adversarial extents': zero-width axes, tiny and huge triangles; decoded
+    boxes still contain every triangle and every triangle appears once."""
+    rng = np.random.default_rng(11)
+    flat = _soup(600, 12).reshape(-1, 3, 3)
+    flat[:, :, 2] = 0.0                                           # every triangle in z = 0
+    check(flat.reshape(-1, 9))
+    line = np.zeros((200, 3, 3), np.float32)
+    line[:, :, 0] = rng.uniform(-5, 5, size=(200, 3))             # y = z = 0: a segment cloud
+    check(line.reshape(-1, 9))
+    tiny = (rng.normal(size=(300, 3, 3)) * 1e-12 + rng.uniform(-1, 1, size=(300, 1, 3))).astype(np.float32)
+    check(tiny.reshape(-1, 9))
+    huge = (rng.normal(size=(300, 3, 3)) * 1e18).astype(np.float32)
+    check(huge.reshape(-1, 9))
+
+
+def test_depth_stays_far_below_the_stack():
+    """cwbvh.test.ts: a 50k-item cloud sits far below the walk's stack (depth < 12)."""
+    depth = check(_soup(50000, 13))
+    assert depth < 12
