@@ -353,3 +353,21 @@ def test_first_hit_guides_match_oracle(engine, oracle, case):
         assert same, name
     assert np.abs(ref[st.aovs[0]][..., :3]).mean() > 0.01
     assert engine.ray_counts() == (int(counts[0]), int(counts[1]))
+
+
+def test_full_size_cloud_is_wave_and_call_invariant(engine):
+    """At the benchmark's shape (1M-triangle cloud, 1920 x 1080, depth 12, pt-mis, sky), where the
+    oracle is too slow to render the frame: the film does not depend on how many frames share a
+    wave or on how the samples are split over calls (the sample order is the parity contract),
+    and every pixel is finite."""
+    films = []
+    for fif, calls in ((1, (2,)), (2, (2,)), (2, (1, 1))):
+        comp = scene_cache.cloud("mis", 1920, 1080, 12, 3, fif)
+        engine.load_renderer(comp)
+        engine.clear_accumulation()
+        for n in calls:
+            engine.render_frames(n)
+        films.append(engine.read_export())
+        _say(f"[gpu] full-size cloud fif={fif} calls={calls}: mean {films[-1][..., :3].mean():.5f}")
+    assert all(np.array_equal(films[0], f) for f in films[1:])
+    assert np.isfinite(films[0]).all() and films[0][..., :3].mean() > 0.05
