@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define PT_ABI_VERSION 7u
+#define PT_ABI_VERSION 8u
 
 typedef enum pt_status {
     PT_OK = 0,
@@ -181,7 +181,12 @@ typedef struct pt_env {
      * color * intensity in every direction; sampled uniformly on the sphere, pdf 1/(4 pi) */
     float   color[4];
     pt_spectrum s_color;        /* spectral, role 'emission' (SETTING_ROLES.sky) */
+    int32_t chart;              /* PT_ENV_SKY: the importance table's chart (estimator.envSampler.chart) */
 } pt_env;
+/* Environment charts (src/components/env/): equirect W x H with sin(theta) row weights, or the
+ * equal-area octahedral map (Clarberg 2008; octahedral.glsl) on a table_h x table_h grid, every
+ * texel 4 pi / N^2 sr, rotated through (cos rotation, sin rotation). */
+enum { PT_CHART_EQUIRECT = 0, PT_CHART_OCTAHEDRAL = 1 };
 
 typedef struct pt_scene_desc {
     uint32_t abi_version;

@@ -149,6 +149,44 @@ PTM_FN float ptm_atan2(float y, float x) {
 PTM_FN float ptm_acos(float x) {
     return ptm_atan2(sqrtf((1.0f - x) * (1.0f + x)), x);
 }
+/* The equal-area octahedral environment chart (src/components/env/octahedral/octahedral.glsl;
+ * Clarberg 2008, pbrt-v4's ImageInfiniteLight mapping, y up). ptm_oct_dir maps table uv to a
+ * direction and applies the inverse of the rotation (c, s); ptm_oct_uv maps a direction, rotated
+ * by (c, s), to table uv. */
+PTM_FN float ptm_copysign_abs(float m, float s) { return s < 0.0f ? -fabsf(m) : fabsf(m); }
+PTM_FN void ptm_oct_dir(float u, float v, float c, float s, float* dx, float* dy, float* dz) {
+    float up = 2.0f * u - 1.0f, vp = 2.0f * v - 1.0f;
+    float upAbs = fabsf(up), vpAbs = fabsf(vp);
+    float sd = 1.0f - (upAbs + vpAbs);
+    float d = fabsf(sd);
+    float r = 1.0f - d;
+    float phi = ((r == 0.0f ? 1.0f : (vpAbs - upAbs) / r) + 1.0f) * PTM_PI * 0.25f;
+    float y = ptm_copysign_abs(1.0f - r * r, sd);
+    float cosPhi = ptm_copysign_abs(ptm_cos(phi), up);
+    float sinPhi = ptm_copysign_abs(ptm_sin(phi), vp);
+    float t = 2.0f - r * r;
+    float ss = r * sqrtf(t > 0.0f ? t : 0.0f);
+    float x = cosPhi * ss, z = sinPhi * ss, ns = -s;           /* env_rotate_cs(d, (c, -s)) */
+    *dx = c * x - ns * z; *dy = y; *dz = ns * x + c * z;
+}
+PTM_FN void ptm_oct_uv(float dx, float dy, float dz, float c, float s, float* uo, float* vo) {
+    float il = 1.0f / sqrtf(dx * dx + dy * dy + dz * dz);
+    float nx0 = dx * il, ny = dy * il, nz0 = dz * il;
+    float nx = c * nx0 - s * nz0, nz = s * nx0 + c * nz0;      /* env_rotate_cs(n, (c, s)) */
+    float x = fabsf(nx), y = fabsf(ny), z = fabsf(nz);
+    float t = 1.0f - y;
+    float r = sqrtf(t > 0.0f ? t : 0.0f);
+    float a = x > z ? x : z;
+    float b = (a == 0.0f) ? 0.0f : (x < z ? x : z) / a;
+    float phi = ptm_atan(b) * (2.0f / PTM_PI);
+    if (x < z) phi = 1.0f - phi;
+    float vv = phi * r;
+    float uu = r - vv;
+    if (ny < 0.0f) { float w = uu; uu = vv; vv = w; uu = 1.0f - uu; vv = 1.0f - vv; }
+    uu = ptm_copysign_abs(uu, nx);
+    vv = ptm_copysign_abs(vv, nz);
+    *uo = (uu + 1.0f) * 0.5f; *vo = (vv + 1.0f) * 0.5f;
+}
 /* asin(x) = atan(x, sqrt(1 - x^2)) on [-1, 1] (exactly +-pi/2 at +-1). */
 PTM_FN float ptm_asin(float x) {
     return ptm_atan2(x, sqrtf((1.0f - x) * (1.0f + x)));

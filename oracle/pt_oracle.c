@@ -1189,12 +1189,18 @@ static v3 env_chart_dir(float rot, float u, float v) {
     float s = SIN(theta);
     return V(COS(phi) * s, COS(theta), SIN(phi) * s);
 }
-static float env_texel_dOmega(int j, int W, int H) {
+static float env_texel_dOmega(const or_scene* S, int j, int W, int H) {
+    if (S->env.chart == PT_CHART_OCTAHEDRAL) return (4.0f * PTM_PI) / ((float)W * (float)H);   /* octahedral.glsl:21-23 */
     float theta = PTM_PI * ((float)j + 0.5f) / (float)H;
     return (PTM_TWO_PI / (float)W) * 2.0f * SIN(theta) * SIN(0.5f * PTM_PI / (float)H);
 }
 static v3 env_chart_sample(const or_scene* S, int ci, int cj, float ux, float uy) {
     int W = S->env_w, H = S->env_h;
+    if (S->env.chart == PT_CHART_OCTAHEDRAL) {   /* octahedral.glsl:67-69 */
+        v3 d;
+        ptm_oct_dir(((float)ci + ux) / (float)W, ((float)cj + uy) / (float)H, S->env_rot_cs[0], S->env_rot_cs[1], &d.x, &d.y, &d.z);
+        return d;
+    }
     float phi = (((float)ci + ux) / (float)W - 0.5f) * PTM_TWO_PI - S->env.rotation;
     int north = 2 * cj < H;
     float row = north ? (float)cj : (float)(H - cj - 1);
@@ -1211,12 +1217,14 @@ static inline float env_cond(const or_scene* S, int j, int i) { return S->env_co
 /* sampler_cdf.glsl:13-84 */
 static float env_table_pdf(const or_scene* S, v3 dir) {
     int W = S->env_w, H = S->env_h;
-    float u, v; equirect_uv(S, dir, &u, &v);
+    float u, v;
+    if (S->env.chart == PT_CHART_OCTAHEDRAL) ptm_oct_uv(dir.x, dir.y, dir.z, S->env_rot_cs[0], S->env_rot_cs[1], &u, &v);   /* octahedral.glsl:44-65 */
+    else equirect_uv(S, dir, &u, &v);
     int i = (int)(u * (float)W); i = i < 0 ? 0 : (i > W - 1 ? W - 1 : i);
     int j = (int)(v * (float)H); j = j < 0 ? 0 : (j > H - 1 ? H - 1 : j);
     float dMarg = env_marg(S, j) - (j > 0 ? env_marg(S, j - 1) : 0.0f);
     float dCond = env_cond(S, j, i) - (i > 0 ? env_cond(S, j, i - 1) : 0.0f);
-    return (dCond * dMarg) / env_texel_dOmega(j, W, H);
+    return (dCond * dMarg) / env_texel_dOmega(S, j, W, H);
 }
 static v3 env_sample_table(const or_scene* S, float x0, float x1) {
     int W = S->env_w, H = S->env_h;
@@ -1246,13 +1254,17 @@ static float environment_pdf(const or_scene* S, v3 dir) {
 
 /* Env bake (compiler/EnvironmentBake.ts:62-67) + importance table (env/importance.ts:16-140). */
 static void env_build(or_scene* S) {
-    int W = S->env.table_w > 0 ? S->env.table_w : 512, H = S->env.table_h > 0 ? S->env.table_h : 256;
+    /* equirect W x H; octahedral N x N with N = table_h (EnvironmentBake.ts:21-27) */
+    int oct = S->env.chart == PT_CHART_OCTAHEDRAL;
+    int H = S->env.table_h > 0 ? S->env.table_h : 256, W = oct ? H : (S->env.table_w > 0 ? S->env.table_w : 512);
     S->env_w = W; S->env_h = H;
     float* Y = (float*)malloc(sizeof(float) * (size_t)W * H);
     for (int j = 0; j < H; j++)
         for (int i = 0; i < W; i++) {
             float u = ((float)i + 0.5f) / (float)W, v = ((float)j + 0.5f) / (float)H;
-            v3 d = env_chart_dir(0.0f, u, v);
+            v3 d;
+            if (oct) ptm_oct_dir(u, v, 1.0f, 0.0f, &d.x, &d.y, &d.z);   /* octahedral.glsl:25-42, rotation 0 */
+            else d = env_chart_dir(0.0f, u, v);
             v3 c = sky_radiance_raw(d);
             Y[(size_t)j * W + i] = (float)(0.2126 * (double)c.x + 0.7152 * (double)c.y + 0.0722 * (double)c.z);
         }
@@ -1263,7 +1275,8 @@ static void env_build(or_scene* S) {
             for (int dj = -1; dj <= 1; dj++) {
                 int jj = j + dj; jj = jj < 0 ? 0 : (jj > H - 1 ? H - 1 : jj);
                 for (int di = -1; di <= 1; di++) {
-                    int ii = (i + di + W) % W;
+                    /* importance.ts:53-63: equirect wraps u, octahedral clamps both */
+                    int ii = oct ? (i + di < 0 ? 0 : (i + di > W - 1 ? W - 1 : i + di)) : (i + di + W) % W;
                     double w = (dj == 0 ? 2 : 1) * (di == 0 ? 2 : 1);
                     sum += w * (double)Y[(size_t)jj * W + ii];
                     wsum += w;
@@ -1274,7 +1287,7 @@ static void env_build(or_scene* S) {
     if (S->env.compensated) {
         double num = 0, den = 0;
         for (int j = 0; j < H; j++) {
-            double dO = (2 * M_PI / W) * (M_PI / H) * sin(M_PI * (j + 0.5) / H);
+            double dO = oct ? (4 * M_PI) / (W * H) : (2 * M_PI / W) * (M_PI / H) * sin(M_PI * (j + 0.5) / H);   /* importance.ts:77-80 */
             for (int i = 0; i < W; i++) { num += (double)Bl[(size_t)j * W + i] * dO; den += dO; }
         }
         double mean = den > 0 ? num / den : 0;
@@ -1285,7 +1298,7 @@ static void env_build(or_scene* S) {
     double* rowSum = (double*)malloc(sizeof(double) * (size_t)H);
     double total = 0;
     for (int j = 0; j < H; j++) {
-        double sinT = sin(M_PI * (j + 0.5) / H);
+        double sinT = oct ? 1.0 : sin(M_PI * (j + 0.5) / H);   /* importance.ts:107-109 */
         double sum = 0;
         for (int i = 0; i < W; i++) { double y = Bl[(size_t)j * W + i]; sum += (y > 0 ? y : 0) * sinT; }
         rowSum[j] = sum;
@@ -2278,6 +2291,15 @@ void or_bsdf_probe(const pt_material* m, const float* n, const float* wo, float 
     float p = surface_pdf(&S, m, s.wi, w, &h, &wl);
     float r[12] = {s.wi.x, s.wi.y, s.wi.z, s.weight.x, s.weight.y, s.weight.z, s.pdf, (float)s.flags, f.x, f.y, f.z, p};
     memcpy(out, r, sizeof r);
+}
+/* The octahedral chart's two maps (ptm_oct_dir / ptm_oct_uv) over n points: inverse = 0 maps
+ * uv [n][2] to directions [n][3]; inverse = 1 maps directions [n][3] to uv [n][2]. (c, s) is the
+ * rotation's cosine and sine. */
+void or_oct_map(const float* in, int n, int inverse, float c, float s, float* out) {
+    for (int i = 0; i < n; i++) {
+        if (inverse) ptm_oct_uv(in[3 * i], in[3 * i + 1], in[3 * i + 2], c, s, &out[2 * i], &out[2 * i + 1]);
+        else ptm_oct_dir(in[2 * i], in[2 * i + 1], c, s, &out[3 * i], &out[3 * i + 1], &out[3 * i + 2]);
+    }
 }
 /* Environment probes: pdf of directions, and samples with their pdfs. */
 void or_env_pdf(const or_scene* S, const float* dirs, int n, float* out) {

@@ -7,7 +7,7 @@ import ctypes as C
 
 import numpy as np
 
-ABI_VERSION = 7
+ABI_VERSION = 8
 
 OBJ_MESH, OBJ_QUAD, OBJ_SPHERE, OBJ_INSTANCES = 0, 1, 2, 3
 MAT_LAMBERT, MAT_DIELECTRIC, MAT_GGX, MAT_NULL, MAT_MIRROR, MAT_ROUGH_DIELECTRIC = 0, 1, 2, 3, 4, 5
@@ -17,6 +17,7 @@ IOR_CONSTANT, IOR_CAUCHY, IOR_SELLMEIER = 0, 1, 2
 LIGHT_QUAD, LIGHT_POINT = 0, 1
 ENV_NONE, ENV_SKY, ENV_CONSTANT = 0, 1, 2
 AOV_ALBEDO, AOV_NORMAL = 1, 2
+CHART_EQUIRECT, CHART_OCTAHEDRAL = 0, 1
 CAM_PINHOLE, CAM_ORTHOGRAPHIC, CAM_THINLENS, CAM_FISHEYE, CAM_CYLINDRICAL, CAM_EQUIRECT = 0, 1, 2, 3, 4, 5
 FISHEYE_PROJECTIONS = {"equidistant": 0, "equisolid": 1, "stereographic": 2, "orthographic": 3}
 EST_PT, EST_NEE, EST_MIS = 0, 1, 2
@@ -73,7 +74,7 @@ class PtEnv(C.Structure):
     _fields_ = [("type", C.c_int32), ("table_w", C.c_int32), ("table_h", C.c_int32),
                 ("compensated", C.c_int32), ("intensity", C.c_float), ("rotation", C.c_float),
                 ("select_prob", C.c_float), ("samplable", C.c_int32), ("color", C.c_float * 4),
-                ("s_color", PtSpectrum)]
+                ("s_color", PtSpectrum), ("chart", C.c_int32)]
 
 
 class PtSceneDesc(C.Structure):

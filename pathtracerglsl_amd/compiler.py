@@ -177,17 +177,24 @@ def scene_radius_estimate(scene):
     return r
 
 
+ENV_CHARTS = {"equirect": abi.CHART_EQUIRECT, "octahedral": abi.CHART_OCTAHEDRAL}
+
+
 def env_select_probability(scene, env, light_powers):
     """envSelectionProbability, 'power' selection (features/lighting.ts:488-518): P(env) =
     Phi_env / (Phi_env + sum Phi_light), Phi_env = 4 pi^2 r^2 mean(L) intensity, clamped to
     [0.01, 0.99]. mean(L): the constant colour's channel mean, or the importance table's mass
-    totalWeight pi / (2 W H) for the tabulated sky (equirect chart)."""
+    totalWeight pi / (2 W H) for the tabulated sky on the equirect chart, totalWeight / N^2 on the
+    N x N octahedral chart (lighting.ts:502-512)."""
     if env.type == abi.ENV_CONSTANT:
         mean_l = (env.color[0] + env.color[1] + env.color[2]) / 3.0
     else:
         from . import _lib
         total = _lib.load().pt_env_total_weight(env)
-        mean_l = total * math.pi / (2.0 * env.table_w * env.table_h) if total > 0 else 1.0
+        if env.chart == abi.CHART_OCTAHEDRAL:
+            mean_l = total / float(env.table_h * env.table_h) if total > 0 else 1.0
+        else:
+            mean_l = total * math.pi / (2.0 * env.table_w * env.table_h) if total > 0 else 1.0
     r = scene_radius_estimate(scene)
     phi_env = 4.0 * math.pi * math.pi * r * r * mean_l * env.intensity
     phi_fin = sum(light_powers)
@@ -565,6 +572,11 @@ def compile_scene(scene, strategy=None, width=256, height=256, reset_salt=1234):
         env = abi.PtEnv(abi.ENV_SKY, 512, 256, int(e.get("compensated", False)),
                         float(e.get("intensity", 1.0)), float(e.get("rotation", 0.0)),
                         float(e.get("select_prob", 0.5)), 1)
+        # estimator.envSampler.chart (components/env/index.ts ENV_CHARTS): the table's chart
+        chart = e.get("chart", "equirect")
+        if chart not in ENV_CHARTS:
+            raise CompileError(f"environment.chart {chart!r}: expected one of {sorted(ENV_CHARTS)}")
+        env.chart = ENV_CHARTS[chart]
     elif e.get("type") == "constant":
         # features/environment.ts:119-133: a constant sky, sampled as a light only when asked
         col = e.get("color", [1.0, 1.0, 1.0])
@@ -661,7 +673,7 @@ def describe(scene, strategy):
                         {"type": "procedural", "glsl": {"kind": "glsl", "source": PROC_SKY_GLSL}}),
         "estimator": {
             "accumulation": {"type": "average", **({"variance": True} if strategy.variance else {})},
-            "envSampler": {"chart": "equirect", "compensation": bool(env.get("compensated", False))},
+            "envSampler": {"chart": env.get("chart", "equirect"), "compensation": bool(env.get("compensated", False))},
             "lighting": None if strategy.direct_lighting == "none" else {"method": strategy.direct_lighting,
                                                                           "selection": "power"},
             "mediumLightSampling": strategy.medium_light_sampling,

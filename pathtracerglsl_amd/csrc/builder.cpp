@@ -499,17 +499,22 @@ inline F3 f3n(F3 a) {
 }  // namespace
 
 void build_env_table(const pt_env& env, std::vector<float>& cond, std::vector<float>& marg, double& total) {
-    const int W = env.table_w > 0 ? env.table_w : 512, H = env.table_h > 0 ? env.table_h : 256;
+    const bool oct = env.chart == PT_CHART_OCTAHEDRAL;          // N x N, N = table_h (EnvironmentBake.ts:20-27)
+    const int H = env.table_h > 0 ? env.table_h : 256, W = oct ? H : (env.table_w > 0 ? env.table_w : 512);
     std::vector<float> Y((size_t)W * H), Bl((size_t)W * H);
     const F3 sun = f3n({0.4f, 0.35f, 0.2f});
     for (int j = 0; j < H; j++) {
         for (int i = 0; i < W; i++) {
             float u = ((float)i + 0.5f) / (float)W, v = ((float)j + 0.5f) / (float)H;
-            // env_chart_dir with the bake's rotation 0 (EnvironmentBake.ts: const u_envRotation)
-            float phi = (u - 0.5f) * PTM_TWO_PI - 0.0f;
-            float theta = v * PTM_PI;
-            float st = ptm_sin(theta);
-            F3 d = {ptm_cos(phi) * st, ptm_cos(theta), ptm_sin(phi) * st};
+            // env_chart_dir with the bake's rotation 0 (EnvironmentBake.ts: const u_envRotation / u_envRotCS)
+            F3 d;
+            if (oct) ptm_oct_dir(u, v, 1.0f, 0.0f, &d.x, &d.y, &d.z);
+            else {
+                float phi = (u - 0.5f) * PTM_TWO_PI - 0.0f;
+                float theta = v * PTM_PI;
+                float st = ptm_sin(theta);
+                d = {ptm_cos(phi) * st, ptm_cos(theta), ptm_sin(phi) * st};
+            }
             float s = 0.5f * (d.y + 1.0f);
             float k = ptm_smoothstep(0.9994f, 0.9999f, d.x * sun.x + d.y * sun.y + d.z * sun.z);
             F3 c = {0.06f * (1.0f - s) + 0.35f * s + 18.0f * k,
@@ -524,7 +529,7 @@ void build_env_table(const pt_env& env, std::vector<float>& cond, std::vector<fl
             for (int dj = -1; dj <= 1; dj++) {
                 int jj = std::min(H - 1, std::max(0, j + dj));
                 for (int di = -1; di <= 1; di++) {
-                    int ii = (i + di + W) % W;
+                    int ii = oct ? std::min(W - 1, std::max(0, i + di)) : (i + di + W) % W;   // u wraps on equirect
                     double w = (dj == 0 ? 2 : 1) * (di == 0 ? 2 : 1);
                     sum += w * (double)Y[(size_t)jj * W + ii];
                     wsum += w;
@@ -535,7 +540,7 @@ void build_env_table(const pt_env& env, std::vector<float>& cond, std::vector<fl
     if (env.compensated) {
         double num = 0, den = 0;
         for (int j = 0; j < H; j++) {
-            double dO = (2 * M_PI / W) * (M_PI / H) * std::sin(M_PI * (j + 0.5) / H);
+            double dO = oct ? (4 * M_PI) / (W * H) : (2 * M_PI / W) * (M_PI / H) * std::sin(M_PI * (j + 0.5) / H);
             for (int i = 0; i < W; i++) { num += (double)Bl[(size_t)j * W + i] * dO; den += dO; }
         }
         double mean = den > 0 ? num / den : 0;
@@ -546,7 +551,7 @@ void build_env_table(const pt_env& env, std::vector<float>& cond, std::vector<fl
     std::vector<double> row((size_t)H);
     total = 0.0;
     for (int j = 0; j < H; j++) {
-        double sinT = std::sin(M_PI * (j + 0.5) / H);
+        double sinT = oct ? 1.0 : std::sin(M_PI * (j + 0.5) / H);   // octahedral texels are equal-area
         double sum = 0;
         for (int i = 0; i < W; i++) sum += std::max(0.0, (double)Bl[(size_t)j * W + i]) * sinT;
         row[j] = sum;

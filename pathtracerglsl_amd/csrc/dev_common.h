@@ -110,7 +110,7 @@ struct DevScene {
     const pt_material* materials;
     const pt_light* lights;
     int n_lights;
-    int env_type, env_w, env_h, env_compensated;
+    int env_type, env_w, env_h, env_compensated, env_chart;
     int env_samplable;          // PT_ENV_CONSTANT sampled as a light
     float4 env_color;           // PT_ENV_CONSTANT radiance (RGB)
     pt_spectrum env_s_color;    // PT_ENV_CONSTANT radiance (spectral)

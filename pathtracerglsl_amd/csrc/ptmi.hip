@@ -1122,14 +1122,18 @@ pt_status fail(pt_status s, const std::string& msg) { g_err = msg; return s; }
 // accumulates sequentially in f64 stay sequential (one thread per row for the conditional CDFs,
 // one thread for the marginal CDF and for the compensation mean).
 // Per-texel kernels are grid-stride (the launch grid is capped at MAX_GRID blocks).
-__global__ void __launch_bounds__(256) k_env_luma(float* Y, int W, int H, float sx, float sy, float sz) {
+__global__ void __launch_bounds__(256) k_env_luma(float* Y, int W, int H, float sx, float sy, float sz, int oct) {
     for (int k = blockIdx.x * 256 + threadIdx.x; k < W * H; k += gridDim.x * 256) {
         int i = k % W, j = k / W;
         float u = ((float)i + 0.5f) / (float)W, v = ((float)j + 0.5f) / (float)H;
-        float phi = (u - 0.5f) * PTM_TWO_PI - 0.0f;       // env_chart_dir, the bake's rotation 0
-        float theta = v * PTM_PI;
-        float st = ptm_sin(theta);
-        float dx = ptm_cos(phi) * st, dy = ptm_cos(theta), dz = ptm_sin(phi) * st;
+        float dx, dy, dz;                                   // env_chart_dir, the bake's rotation 0
+        if (oct) ptm_oct_dir(u, v, 1.0f, 0.0f, &dx, &dy, &dz);
+        else {
+            float phi = (u - 0.5f) * PTM_TWO_PI - 0.0f;
+            float theta = v * PTM_PI;
+            float st = ptm_sin(theta);
+            dx = ptm_cos(phi) * st; dy = ptm_cos(theta); dz = ptm_sin(phi) * st;
+        }
         float s = 0.5f * (dy + 1.0f);
         float kk = ptm_smoothstep(0.9994f, 0.9999f, dx * sx + dy * sy + dz * sz);
         float cx = 0.06f * (1.0f - s) + 0.35f * s + 18.0f * kk;
@@ -1140,14 +1144,14 @@ __global__ void __launch_bounds__(256) k_env_luma(float* Y, int W, int H, float 
 }
 
 // 3 x 3 tent, u wraps (the phi seam), v clamps (the poles).
-__global__ void __launch_bounds__(256) k_env_blur(const float* __restrict__ Y, float* __restrict__ B, int W, int H) {
+__global__ void __launch_bounds__(256) k_env_blur(const float* __restrict__ Y, float* __restrict__ B, int W, int H, int oct) {
     for (int k = blockIdx.x * 256 + threadIdx.x; k < W * H; k += gridDim.x * 256) {
         int i = k % W, j = k / W;
         double sum = 0.0, wsum = 0.0;
         for (int dj = -1; dj <= 1; dj++) {
             int jj = min(H - 1, max(0, j + dj));
             for (int di = -1; di <= 1; di++) {
-                int ii = (i + di + W) % W;
+                int ii = oct ? min(W - 1, max(0, i + di)) : (i + di + W) % W;
                 double w = (dj == 0 ? 2 : 1) * (di == 0 ? 2 : 1);
                 sum += w * (double)Y[(size_t)jj * W + ii];
                 wsum += w;
@@ -1540,13 +1544,14 @@ static pt_status validate_scene(const pt_scene_desc* sd) {
 // on the host (H values each). ev0 / ev1, when given, bracket the kernels on SS.stream.
 static pt_status bake_env_table(const pt_env& env, SceneStore& SS, DevScene& D, hipEvent_t ev0 = nullptr,
                                 hipEvent_t ev1 = nullptr) {
-    const int W = env.table_w > 0 ? env.table_w : 512, H = env.table_h > 0 ? env.table_h : 256;
+    const bool oct = env.chart == PT_CHART_OCTAHEDRAL;
+    const int H = env.table_h > 0 ? env.table_h : 256, W = oct ? H : (env.table_w > 0 ? env.table_w : 512);
     if (W > PT_ENV_MAX_W) return fail(PT_E_UNSUPPORTED, "environment table wider than 4096 texels");
     const long long n = (long long)W * H;
     std::vector<double> sin_t((size_t)H), d_omega((size_t)H);
     for (int j = 0; j < H; j++) {
-        sin_t[j] = std::sin(M_PI * (j + 0.5) / H);
-        d_omega[j] = (2 * M_PI / W) * (M_PI / H) * std::sin(M_PI * (j + 0.5) / H);
+        sin_t[j] = oct ? 1.0 : std::sin(M_PI * (j + 0.5) / H);
+        d_omega[j] = oct ? (4 * M_PI) / (W * H) : (2 * M_PI / W) * (M_PI / H) * std::sin(M_PI * (j + 0.5) / H);
     }
     float sx = 0.4f, sy = 0.35f, sz = 0.2f;               // the sun direction, normalised in f32
     float d = sx * sx + sy * sy + sz * sz;
@@ -1563,8 +1568,8 @@ static pt_status bake_env_table(const pt_env& env, SceneStore& SS, DevScene& D, 
         (st = SS.upload(d_omega.data(), d_omega.size(), &d_dom)) != PT_OK)
         return st;
     if (ev0) HIP_TRY(hipEventRecord(ev0, SS.stream));
-    hipLaunchKernelGGL(k_env_luma, dim3(grid_for(n, 256)), dim3(256), 0, SS.stream, Y, W, H, sx, sy, sz);
-    hipLaunchKernelGGL(k_env_blur, dim3(grid_for(n, 256)), dim3(256), 0, SS.stream, (const float*)Y, B, W, H);
+    hipLaunchKernelGGL(k_env_luma, dim3(grid_for(n, 256)), dim3(256), 0, SS.stream, Y, W, H, sx, sy, sz, oct ? 1 : 0);
+    hipLaunchKernelGGL(k_env_blur, dim3(grid_for(n, 256)), dim3(256), 0, SS.stream, (const float*)Y, B, W, H, oct ? 1 : 0);
     if (env.compensated) {
         hipLaunchKernelGGL(k_env_mean, dim3(1), dim3(256), 0, SS.stream, (const float*)B, d_dom, W, H, aux);
         hipLaunchKernelGGL(k_env_compensate, dim3(grid_for(n, 256)), dim3(256), 0, SS.stream, B, (int)n,
@@ -1825,8 +1830,9 @@ static pt_status build_scene(const pt_scene_desc* sd, hipStream_t stream, SceneS
     D.env_type = sd->env.type;
     if (sd->env.type == PT_ENV_SKY) {
         if ((st = bake_env_table(sd->env, SS, D)) != PT_OK) return st;
-        D.env_w = sd->env.table_w > 0 ? sd->env.table_w : 512;
+        D.env_chart = sd->env.chart == PT_CHART_OCTAHEDRAL ? PT_CHART_OCTAHEDRAL : PT_CHART_EQUIRECT;
         D.env_h = sd->env.table_h > 0 ? sd->env.table_h : 256;
+        D.env_w = D.env_chart == PT_CHART_OCTAHEDRAL ? D.env_h : (sd->env.table_w > 0 ? sd->env.table_w : 512);
         D.env_compensated = sd->env.compensated;
         D.env_intensity = sd->env.intensity;
         D.env_rotation = sd->env.rotation;
@@ -2924,7 +2930,8 @@ int pt_debug_env_bake(int device, const pt_env* env, float* cond, float* marg, d
         DevScene D{};
         st = bake_env_table(*env, SS, D, e0, e1);
         if (st == PT_OK) {
-            const size_t W = env->table_w > 0 ? env->table_w : 512, H = env->table_h > 0 ? env->table_h : 256;
+            const size_t H = env->table_h > 0 ? env->table_h : 256,
+                         W = env->chart == PT_CHART_OCTAHEDRAL ? H : (env->table_w > 0 ? env->table_w : 512);
             HIP_TRY(hipMemcpyAsync(cond, D.env_cond, W * H * 4, hipMemcpyDeviceToHost, stream));
             HIP_TRY(hipMemcpyAsync(marg, D.env_marg, H * 4, hipMemcpyDeviceToHost, stream));
             HIP_TRY(hipMemcpyAsync(total, D.env_total, 8, hipMemcpyDeviceToHost, stream));

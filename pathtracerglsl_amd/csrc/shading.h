@@ -620,22 +620,27 @@ PT_DEV sp4 env_radiance(const DevScene& S, f3 dir, const Wl& wl) {
     d = mk3(cx * d.x - sy * d.z, d.y, sy * d.x + cx * d.z);
     return rgb4(sky_raw(d) * S.env_intensity);
 }
-PT_DEV float env_texel_dOmega(int j, int W, int H) {
+PT_DEV float env_texel_dOmega(const DevScene& S, int j, int W, int H) {
+    if (S.env_chart == PT_CHART_OCTAHEDRAL) return (4.0f * PTM_PI) / ((float)W * (float)H);   // octahedral.glsl
     float theta = PTM_PI * ((float)j + 0.5f) / (float)H;
     return (PTM_TWO_PI / (float)W) * 2.0f * ptm_sin(theta) * ptm_sin(0.5f * PTM_PI / (float)H);
 }
 PT_DEV float env_table_pdf(const DevScene& S, f3 dir) {
     const int W = S.env_w, H = S.env_h;
-    f3 n = normalize(dir);
-    float phi = (n.x == 0.0f && n.z == 0.0f ? 0.0f : ptm_atan2(n.z, n.x)) + S.env_rotation;
-    float theta = ptm_atan2(sqrtf(n.x * n.x + n.z * n.z), n.y);
-    float u = ptm_fract(phi * (1.0f / PTM_TWO_PI) + 0.5f);
-    float v = theta * (1.0f / PTM_PI);
+    float u, v;
+    if (S.env_chart == PT_CHART_OCTAHEDRAL) ptm_oct_uv(dir.x, dir.y, dir.z, S.env_rot_c, S.env_rot_s, &u, &v);
+    else {
+        f3 n = normalize(dir);
+        float phi = (n.x == 0.0f && n.z == 0.0f ? 0.0f : ptm_atan2(n.z, n.x)) + S.env_rotation;
+        float theta = ptm_atan2(sqrtf(n.x * n.x + n.z * n.z), n.y);
+        u = ptm_fract(phi * (1.0f / PTM_TWO_PI) + 0.5f);
+        v = theta * (1.0f / PTM_PI);
+    }
     int i = (int)(u * (float)W); i = i < 0 ? 0 : (i > W - 1 ? W - 1 : i);
     int j = (int)(v * (float)H); j = j < 0 ? 0 : (j > H - 1 ? H - 1 : j);
     float dMarg = S.env_marg[j] - (j > 0 ? S.env_marg[j - 1] : 0.0f);
     float dCond = S.env_cond[(size_t)j * W + i] - (i > 0 ? S.env_cond[(size_t)j * W + i - 1] : 0.0f);
-    return (dCond * dMarg) / env_texel_dOmega(j, W, H);
+    return (dCond * dMarg) / env_texel_dOmega(S, j, W, H);
 }
 PT_DEV float env_uniform_fraction(const DevScene& S) {
     if (S.env_compensated) return 0.0f;
@@ -661,6 +666,11 @@ PT_DEV f3 env_sample_table(const DevScene& S, float x0, float x1) {
     float cLo = (i > 0) ? row[i - 1] : 0.0f;
     float dCond = row[i] - cLo;
     float uFrac = (dCond > 0.0f) ? gclamp((x1 - cLo) / dCond, 0.0f, 0.9999999f) : 0.5f;
+    if (S.env_chart == PT_CHART_OCTAHEDRAL) {         // octahedral.glsl env_chart_sample
+        f3 d;
+        ptm_oct_dir(((float)i + uFrac) / (float)W, ((float)j + vFrac) / (float)H, S.env_rot_c, S.env_rot_s, &d.x, &d.y, &d.z);
+        return d;
+    }
     // env_chart_sample (equirect.glsl:36-48)
     float phi = (((float)i + uFrac) / (float)W - 0.5f) * PTM_TWO_PI - S.env_rotation;
     bool north = 2 * j < H;

@@ -40,6 +40,8 @@ def load(variant=""):
     lib.or_render_aovs.argtypes = [vp, vp, vp, C.c_uint32, C.c_int, vp, vp, vp, vp]
     lib.or_render_aovs.restype = None
     lib.or_concentric_disk.restype = None
+    lib.or_oct_map.argtypes = [vp, C.c_int, C.c_int, C.c_float, C.c_float, vp]
+    lib.or_oct_map.restype = None
     lib.or_bsdf_probe.argtypes = [vp, vp, vp, C.c_float, C.c_float, C.c_float, vp]
     lib.or_env_pdf.argtypes = [vp, vp, C.c_int, vp]
     lib.or_env_sample.argtypes = [vp, vp, C.c_int, vp, vp]

@@ -23,7 +23,8 @@ def _env(w, h, compensated):
 def _host(env):
     lib = _lib.load()
     lib.pt_debug_env_table.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_double)]
-    cond = np.zeros((env.table_h, env.table_w), np.float32)
+    w = env.table_h if env.chart == abi.CHART_OCTAHEDRAL else env.table_w
+    cond = np.zeros((env.table_h, w), np.float32)
     marg = np.zeros(env.table_h, np.float32)
     tot = C.c_double()
     t0 = time.perf_counter()
@@ -35,7 +36,8 @@ def _device(env):
     lib = _lib.load()
     lib.pt_debug_env_bake.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_double),
                                       C.POINTER(C.c_float)]
-    cond = np.zeros((env.table_h, env.table_w), np.float32)
+    w = env.table_h if env.chart == abi.CHART_OCTAHEDRAL else env.table_w
+    cond = np.zeros((env.table_h, w), np.float32)
     marg = np.zeros(env.table_h, np.float32)
     tot, ms = C.c_double(), C.c_float()
     _lib.check(lib.pt_debug_env_bake(0, C.byref(env), cond.ctypes.data, marg.ctypes.data, C.byref(tot), C.byref(ms)))
@@ -69,3 +71,26 @@ def test_device_bake_equals_oracle(engine, oracle, w, h, comp):
     gc, gm, gt, _ = _device(cs.scene.env)
     assert np.array_equal(gc.view(np.uint32), oc.view(np.uint32)) and np.array_equal(gm.view(np.uint32), om.view(np.uint32))
     assert gt == oracle.or_env_total(ors.h)
+
+
+@pytest.mark.parametrize("n,comp", [(256, 0), (256, 1), (512, 0), (1024, 1)])
+def test_device_bake_octahedral_equals_host_and_oracle(engine, oracle, n, comp):
+    """The N x N equal-area octahedral table (N = tableSize[1]): device bake == host builder bit
+    for bit, and == the oracle's table at the default N = 256."""
+    env = abi.PtEnv(abi.ENV_SKY, 512, n, comp, 1.0, 0.0, 0.5, 1)
+    env.chart = abi.CHART_OCTAHEDRAL
+    print(f"[gpu] octahedral env bake {n}x{n} compensated={comp}", flush=True)
+    gc, gm, gt, gms = _device(env)
+    hc, hm, ht, hms = _host(env)
+    assert gc.shape == (n, n)
+    print(f"[gpu] octahedral env bake {n}x{n} comp={comp}: device {gms:.3f} ms, host {hms:.1f} ms", flush=True)
+    assert np.array_equal(gc.view(np.uint32), hc.view(np.uint32))
+    assert np.array_equal(gm.view(np.uint32), hm.view(np.uint32))
+    assert gt == ht and gm[-1] == np.float32(1.0)
+    if n == 256:
+        sc = scenes.cornell_box()
+        sc["environment"] = {"type": "procedural", "intensity": 1.0, "compensated": bool(comp), "chart": "octahedral"}
+        cs = compiler.compile_scene(sc, compiler.RenderStrategy("mis", 4, -1), 16, 16)
+        ors = oracle_lib.OracleScene(oracle, cs.scene)
+        oc, om = ors.env_table(n, n)
+        assert np.array_equal(gc.view(np.uint32), oc.view(np.uint32)) and np.array_equal(gm.view(np.uint32), om.view(np.uint32))

@@ -330,6 +330,27 @@ def test_wide_cameras_cornell_matches_oracle(engine, oracle, camera):
     assert engine.ray_counts() == (int(counts[0]), int(counts[1]))
 
 
+@pytest.mark.parametrize("comp,rotation", [(False, 0.0), (False, 0.7), (True, 0.7)])
+def test_octahedral_sky_cornell_matches_oracle(engine, oracle, comp, rotation):
+    """The sky importance-sampled through the equal-area octahedral chart (rotated, and with MIS
+    compensation): the GPU film equals the oracle's."""
+    sc = scenes.cornell_box()
+    sc["environment"] = {"type": "procedural", "intensity": 1.0, "chart": "octahedral", "compensated": comp,
+                         "rotation": rotation}
+    sc["camera"]["position"] = [0.0, 0.0, 0.6]
+    comp_ = scene_cache.compiler.compile_scene(sc, scene_cache.compiler.RenderStrategy("mis", 8, 3), 40, 32)
+    print(f"[gpu] octahedral sky cornell comp={comp} rotation={rotation}", flush=True)
+    engine.load_renderer(comp_)
+    engine.clear_accumulation()
+    engine.render_frames(8)
+    g = engine.read_export()
+    o, _, counts = oracle_lib.OracleScene(oracle, comp_.scene).render(comp_.program, comp_.params(), 8)
+    exact, bad = _compare_film(f"cornell-octahedral-{comp}-{rotation}", g, o)
+    assert o[..., :3].mean() > 0.05
+    assert bad == 0.0
+    assert engine.ray_counts() == (int(counts[0]), int(counts[1]))
+
+
 @pytest.mark.parametrize("case", ["cornell", "slab", "smooth"])
 def test_first_hit_guides_match_oracle(engine, oracle, case):
     if case == "cornell":
