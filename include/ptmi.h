@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define PT_ABI_VERSION 8u
+#define PT_ABI_VERSION 9u
 
 typedef enum pt_status {
     PT_OK = 0,
@@ -251,6 +251,8 @@ typedef struct pt_program {
                                  components/accumulator/variance/variance.glsl) */
     int32_t  volume_sampling; /* PT_VS_*: estimator.volumeSampling (validation: the arms are per material) */
     int32_t  aovs;            /* PT_AOV_* bits: measurement.aovs, the first-hit guide pictures (RGB only) */
+    int32_t  env_chart;       /* estimator.envSampler.chart as PT_CHART_* + 1, 0 = not stated; when stated,
+                               * a procedural sky's importance table (pt_env.chart) must be on that chart */
 } pt_program;
 
 /* First-hit guides (src/components/aov/): at the first non-null surface a path reaches with no

@@ -7,7 +7,7 @@ import ctypes as C
 
 import numpy as np
 
-ABI_VERSION = 8
+ABI_VERSION = 9
 
 OBJ_MESH, OBJ_QUAD, OBJ_SPHERE, OBJ_INSTANCES = 0, 1, 2, 3
 MAT_LAMBERT, MAT_DIELECTRIC, MAT_GGX, MAT_NULL, MAT_MIRROR, MAT_ROUGH_DIELECTRIC = 0, 1, 2, 3, 4, 5
@@ -107,7 +107,8 @@ class PtProgram(C.Structure):
                 ("rr_start", C.c_int32), ("rr_max_survival", C.c_float), ("color", C.c_int32),
                 ("width", C.c_int32), ("height", C.c_int32), ("frames_in_flight", C.c_int32),
                 ("max_null_crossings", C.c_int32), ("wavelengths", C.c_int32), ("medium_light_sampling", C.c_int32),
-                ("variance", C.c_int32), ("volume_sampling", C.c_int32), ("aovs", C.c_int32)]
+                ("variance", C.c_int32), ("volume_sampling", C.c_int32), ("aovs", C.c_int32),
+                ("env_chart", C.c_int32)]
 
 
 class PtParams(C.Structure):

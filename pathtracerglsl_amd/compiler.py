@@ -617,6 +617,7 @@ def compile_scene(scene, strategy=None, width=256, height=256, reset_salt=1234):
     prog.volume_sampling = abi.VS_DELTA_TRACKING if strategy.volume_sampling == "delta-tracking" else abi.VS_ANALYTIC
     prog.variance = int(bool(strategy.variance))
     prog.aovs = sum({"albedo": abi.AOV_ALBEDO, "normal": abi.AOV_NORMAL}[a] for a in set(strategy.aovs))
+    prog.env_chart = ENV_CHARTS[e.get("chart", "equirect")] + 1 if e.get("type") == "procedural" else abi.CHART_EQUIRECT + 1
     prog.width, prog.height = width, height
     prog.frames_in_flight = max(1, strategy.frames_in_flight)
     prog.max_null_crossings = strategy.max_null_crossings

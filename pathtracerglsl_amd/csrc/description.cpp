@@ -225,7 +225,13 @@ bool lower(const JVal& d, int32_t width, int32_t height, int32_t fif, pt_program
         pg.variance = truthy(acc->get("variance")) ? 1 : 0;
     }
     if (const JVal* es = e->get("envSampler")) {
-        if (const JVal* ch = es->get("chart"); ch && !is_str(ch, "equirect")) return L.unsupported("estimator.envSampler.chart", quoted(ch));
+        if (const JVal* ch = es->get("chart"); ch && ch->kind != JVal::NUL) {
+            // the table itself is built at scene upload on pt_env.chart; the program records the chart
+            // so that loading it over a scene whose table is on the other chart fails
+            if (is_str(ch, "equirect")) pg.env_chart = PT_CHART_EQUIRECT + 1;
+            else if (is_str(ch, "octahedral")) pg.env_chart = PT_CHART_OCTAHEDRAL + 1;
+            else return L.unsupported("estimator.envSampler.chart", quoted(ch));
+        }
     }
     const JVal* vs = e->get("volumeSampling");
     if (!vs || vs->kind == JVal::NUL || is_str(vs, "none") || is_str(vs, "analytic")) pg.volume_sampling = PT_VS_ANALYTIC;

@@ -1893,6 +1893,11 @@ pt_status pt_renderer_load(pt_ctx* ctx, const pt_program* prog, pt_renderer** ou
     if (prog->aovs & ~(PT_AOV_ALBEDO | PT_AOV_NORMAL)) return fail(PT_E_INVALID, "unknown aov bits");
     if (prog->aovs && prog->color != PT_COLOR_RGB)
         return fail(PT_E_UNSUPPORTED, "first-hit guides (aovs) are provided for RGB programs only");
+    if (prog->env_chart < 0 || prog->env_chart > PT_CHART_OCTAHEDRAL + 1) return fail(PT_E_INVALID, "unknown env chart");
+    if (prog->env_chart && ctx->has_scene && ctx->dev.env_type == PT_ENV_SKY && prog->env_chart - 1 != ctx->dev.env_chart)
+        return fail(PT_E_INVALID, std::string("the program's envSampler.chart is '") +
+                                      (prog->env_chart - 1 == PT_CHART_OCTAHEDRAL ? "octahedral" : "equirect") +
+                                      "' but the scene's sky table was built on the other chart (pt_env.chart)");
     HIP_TRY(hipSetDevice(ctx->device));
     auto* r = new pt_renderer();
     r->ctx = ctx;
@@ -2399,6 +2404,10 @@ pt_status pt_batch_create(pt_ctx* ctx, int32_t n, const pt_scene_desc* scenes, c
         if (pt_status sc = check_camera(&params[s]); sc != PT_OK) return sc;
     for (int s = 0; s < n; s++)
         if (programs[s].aovs) return fail(PT_E_UNSUPPORTED, "pt_batch_create: first-hit guides (aovs) are not batched");
+    for (int s = 0; s < n; s++)
+        if (programs[s].env_chart && scenes[s].env.type == PT_ENV_SKY &&
+            programs[s].env_chart - 1 != (scenes[s].env.chart == PT_CHART_OCTAHEDRAL ? PT_CHART_OCTAHEDRAL : PT_CHART_EQUIRECT))
+            return fail(PT_E_INVALID, "pt_batch_create: a program's envSampler.chart differs from its scene's sky table chart");
     const pt_program& p0 = programs[0];
     for (int s = 0; s < n; s++) {
         const pt_program& p = programs[s];

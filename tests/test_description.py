@@ -56,7 +56,10 @@ def test_library_lowers_or_names_every_snapshot_case():
                                      else abi.VS_ANALYTIC), name
         assert p.variance == int(bool(e.get("accumulation", {}).get("variance"))), name
         assert set(d["materials"]["models"]) <= SUPPORTED_MODELS, name
+        chart = (e.get("envSampler") or {}).get("chart")
+        assert p.env_chart == {None: 0, "equirect": abi.CHART_EQUIRECT + 1, "octahedral": abi.CHART_OCTAHEDRAL + 1}[chart], name
         assert (p.width, p.height, p.frames_in_flight, p.abi_version) == (64, 48, 1, abi.ABI_VERSION)
+    print(f"lowered {accepted} of {accepted + rejected}")
     assert accepted >= 280 and accepted + rejected == 505
 
 

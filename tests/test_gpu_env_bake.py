@@ -94,3 +94,21 @@ def test_device_bake_octahedral_equals_host_and_oracle(engine, oracle, n, comp):
         ors = oracle_lib.OracleScene(oracle, cs.scene)
         oc, om = ors.env_table(n, n)
         assert np.array_equal(gc.view(np.uint32), oc.view(np.uint32)) and np.array_equal(gm.view(np.uint32), om.view(np.uint32))
+
+
+def test_program_chart_must_match_the_scene_table(engine):
+    """A program that names an envSampler.chart (a lowered ProgramDescription does) loads only
+    over a sky whose table is on that chart; the matching chart loads and renders."""
+    sc = scenes.cornell_box()
+    sc["environment"] = {"type": "procedural", "intensity": 1.0}
+    cs = compiler.compile_scene(sc, compiler.RenderStrategy("mis", 4, -1), 16, 16)
+    print("[gpu] chart mismatch at renderer load", flush=True)
+    cs.program.env_chart = abi.CHART_OCTAHEDRAL + 1
+    with pytest.raises(_lib.PtError) as e:
+        engine.load_renderer(cs)
+    assert e.value.status == 1 and "envSampler.chart" in str(e.value)
+    cs.program.env_chart = abi.CHART_EQUIRECT + 1
+    engine.load_renderer(cs)
+    engine.clear_accumulation()
+    engine.render_frames(1)
+    assert np.isfinite(engine.read_export()).all()
