@@ -257,7 +257,8 @@ bool lower(const JVal& d, int32_t width, int32_t height, int32_t fif, pt_program
         // (accel/light_tree/light_tree.glsl, through LightQuery.two_sided); the 'power' selection
         // this backend lowers ignores the query, so the flag changes no estimate here.
         if (truthy(mats->get("materialsReadUv"))) return L.unsupported("materials.materialsReadUv", "true (textures)");
-        if (truthy(mats->get("albedo"))) return L.unsupported("materials.albedo", "true (albedo guide)");
+        // materials.albedo marks the material_albedo dispatch an albedo AOV needs (plan/types.ts:246-248);
+        // the albedo guide itself is lowered from measurement.aovs, so the flag adds nothing here.
         if (const JVal* cones = mats->get("emissionCones"); cones && cones->kind == JVal::ARR && !cones->arr.empty())
             return L.unsupported("materials.emissionCones", "(emitter profiles)");
     }
