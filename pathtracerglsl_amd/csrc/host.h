@@ -1,0 +1,195 @@
+// host.h — host-side state shared by the library's translation units: device buffers, the
+// context / renderer / batch records, error reporting, launch timing and the per-variant
+// wave launchers (each compiled in a unit of its own).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "dev_common.h"
+#include "pipeline.h"
+
+namespace ptmi {
+
+// Records msg as the calling thread's last error (pt_last_error) and returns s.
+pt_status fail(pt_status s, const std::string& msg);
+
+#define HIP_TRY(expr)                                                                       \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return ptmi::fail(e_ == hipErrorOutOfMemory ? PT_E_OOM : PT_E_DEVICE,            \
+                              std::string(#expr) + ": " + hipGetErrorString(e_));           \
+    } while (0)
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+};
+
+inline int grid_for(long long n, int block) {
+    long long g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    return (int)std::min<long long>(g, MAX_GRID);
+}
+
+}  // namespace ptmi
+
+using namespace ptmi;
+
+struct TimingSlot { std::vector<std::pair<hipEvent_t, hipEvent_t>> ev; float ms = 0; uint32_t launches = 0; uint64_t items = 0; };
+
+// One uploaded scene: its device buffers and the DevScene view of them.
+struct SceneStore {
+    std::vector<std::unique_ptr<DevBuf>> bufs;
+    DevScene dev{};
+    int stack = 16;
+    uint32_t wide_depth = 0;
+    bool has_ggx = false;
+    bool has_mirror = false;
+    bool has_rough = false;
+    std::vector<pt_material> host_materials;
+    hipStream_t stream = nullptr;
+    template <class T> pt_status upload(const T* host, size_t n, const T** out) {
+        auto b = std::make_unique<DevBuf>();
+        b->bytes = std::max<size_t>(n, 1) * sizeof(T);
+        HIP_TRY(hipMalloc(&b->p, b->bytes));
+        if (n) HIP_TRY(hipMemcpyAsync(b->p, host, n * sizeof(T), hipMemcpyHostToDevice, stream));
+        *out = (const T*)b->p;
+        bufs.push_back(std::move(b));
+        return PT_OK;
+    }
+    template <class T> pt_status alloc(size_t n, T** out) {
+        auto b = std::make_unique<DevBuf>();
+        b->bytes = std::max<size_t>(n, 1) * sizeof(T);
+        HIP_TRY(hipMalloc(&b->p, b->bytes));
+        *out = (T*)b->p;
+        bufs.push_back(std::move(b));
+        return PT_OK;
+    }
+};
+
+struct pt_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<std::unique_ptr<DevBuf>> scene_bufs;
+    DevScene dev{};
+    bool has_scene = false;
+    int stack = 16;
+    uint32_t wide_depth = 0;
+    int persist_blocks = 2048;   // persistent traversal grid (set from the device's CU count)
+    int cu_count = 256;
+    int shade_blocks = 2048;
+    bool has_ggx = false;
+    bool has_mirror = false;
+    bool has_rough = false;
+    std::vector<pt_material> host_materials;   // the uploaded records (adjoint checks, updates)
+    bool timing = false;
+    std::map<std::string, TimingSlot> timings;
+    std::vector<std::string> timing_order;
+    DevBuf trav;   // 2 x u64 counters for instrumented traces
+    DevBuf rays_tmp, t_tmp, p_tmp;
+
+    template <class T> pt_status upload(const T* host, size_t n, const T** out) {
+        auto b = std::make_unique<DevBuf>();
+        b->bytes = std::max<size_t>(n, 1) * sizeof(T);
+        HIP_TRY(hipMalloc(&b->p, b->bytes));
+        if (n) HIP_TRY(hipMemcpyAsync(b->p, host, n * sizeof(T), hipMemcpyHostToDevice, stream));
+        *out = (const T*)b->p;
+        scene_bufs.push_back(std::move(b));
+        return PT_OK;
+    }
+    void ev_begin(const char* name, hipEvent_t* b) {
+        if (!timing) return;
+        (void)hipEventCreate(b);
+        (void)hipEventRecord(*b, stream);
+        (void)name;
+    }
+    void ev_end(const char* name, hipEvent_t b, uint64_t items) {
+        if (!timing) return;
+        hipEvent_t e;
+        (void)hipEventCreate(&e);
+        (void)hipEventRecord(e, stream);
+        auto it = timings.find(name);
+        if (it == timings.end()) { timing_order.push_back(name); it = timings.emplace(name, TimingSlot()).first; }
+        it->second.ev.emplace_back(b, e);
+        it->second.launches++;
+        it->second.items += items;
+    }
+};
+
+struct pt_renderer {
+    pt_ctx* ctx = nullptr;
+    pt_program prog{};
+    DevBuf film, var_film, aov_film[2];
+    uint32_t sample_count = 0;
+    long long cap = 0;    // path slots allocated
+    DevBuf b_ray_o, b_ray_d, b_thr, b_rad, b_rng, b_prev, b_flags, b_medium, b_hit, b_q0, b_q1, b_counts,
+        b_spath, b_so, b_sd, b_sc, b_sl, b_wl, b_eta, b_hist, b_trav, b_cat[N_CAT], b_aov[2];
+    // adjoint buffers (allocated by the first pt_backward)
+    DevBuf b_adjD, b_adjG, b_adjsD, b_delta, b_pixgrad, b_grad;
+    DevBuf b_pscene;      // scene of each path slot (batched wavefront)
+    long long adj_cap = 0;
+    int adj_mode = PT_REDUCE_DETERMINISTIC;
+    Paths paths{};
+    bool count_traversal = false;
+};
+
+#define TIMED(ctx, name, items, launch)             \
+    do {                                            \
+        hipEvent_t ev_b_ = nullptr;                 \
+        (ctx)->ev_begin(name, &ev_b_);              \
+        launch;                                     \
+        (ctx)->ev_end(name, ev_b_, items);          \
+    } while (0)
+
+// ---------------------------------------------------------------- batched wavefront
+struct pt_batch {
+    pt_ctx* ctx = nullptr;
+    std::vector<std::unique_ptr<SceneStore>> stores;
+    std::vector<pt_program> progs;
+    std::vector<Camera> cams;
+    std::vector<long long> film_off;   // first pixel of each scene's film
+    long long film_pixels = 0;
+    int max_npix = 0, min_stack = 16;
+    bool any_ggx = false, any_dielectric = false, any_mirror = false, any_rough = false, any_emissive_solid = false;
+    DevBuf d_scenes, d_cams, d_entries, films;
+    pt_renderer* r = nullptr;          // path slots
+    pt_batch_config cfg{};
+    std::vector<BatchEntry> entries;
+    std::vector<std::pair<int, int>> waves;   // (first entry, entry count)
+    std::vector<long long> wave_paths;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<uint32_t> graph_spp;
+    pt_batch_config graph_cfg{};
+    ~pt_batch() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        delete r;
+    }
+    void drop_graph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr; graph = nullptr; graph_spp.clear();
+    }
+};
+
+namespace ptmi {
+// One frame wave of a renderer's program (wave_*.hip): frames samples per pixel starting at
+// sampleCount sample0, traced and accumulated (forward) or replayed and reduced (adjoint).
+void wave_rgb(pt_renderer* r, const Camera& cam, int frames, uint32_t sample0, int stack);
+void wave_spectral(pt_renderer* r, const Camera& cam, int frames, uint32_t sample0);
+void wave_media(pt_renderer* r, const Camera& cam, int frames, uint32_t sample0);
+void wave_media_spectral(pt_renderer* r, const Camera& cam, int frames, uint32_t sample0);
+void wave_adjoint(pt_renderer* r, const Camera& cam, int frames, uint32_t sample0);
+void wave_adjoint_media(pt_renderer* r, const Camera& cam, int frames, uint32_t sample0);
+// Every wave of a batch's schedule, then the film updates (batch_kernels.hip).
+void batch_launch(pt_batch* b);
+}  // namespace ptmi

@@ -1,4 +1,5 @@
-// ptmi.hip — libptmi: the C ABI and the wavefront path-tracing pipeline for gfx950.
+// ptmi.hip — libptmi: the C ABI of the wavefront path-tracing pipeline for gfx950 (the
+// kernels are in kernels.h, the per-variant launch sequences in wave.h / wave_*.hip).
 //
 // One frame wave traces frames_in_flight samples for every pixel as independent paths in
 // fixed slots of an SoA path state in HBM. Per bounce, four kernels run over compacted
@@ -26,1092 +27,15 @@
 #include "builder.h"
 #include "description.h"
 #include "formats.h"
-#include "dev_common.h"
-#include "shading.h"
-#include "traversal.h"
+#include "kernels.h"
+#include "host.h"
 
-using namespace ptmi;
-
-// ============================================================================ kernels
-
-namespace {
-
-constexpr int TBLOCK = 128;     // traversal blocks (2 waves)
-constexpr int SBLOCK = 256;     // shading blocks
-constexpr int MAX_GRID = 8192;
-
-// Shading categories: the bin kernel sorts every path of a bounce by what its vertex needs,
-// so each shade launch runs one code path (no material divergence inside a wave).
-enum { CAT_MISS = 0, CAT_LAMBERT = 1, CAT_GGX = 2, CAT_DIELECTRIC = 3, CAT_MEDIUM = 4, CAT_NULL = 5, CAT_MIRROR = 6,
-       CAT_ROUGH = 7, N_CAT = 8 };
-// Categories whose shading classifies every hit's regions (the transmissive models read both
-// sides' indices; media programs track the current medium).
-constexpr bool cat_probes(int cat, bool media) { return cat == CAT_DIELECTRIC || cat == CAT_ROUGH || media; }
-// counts[] slots
-enum { C_RAYQ0 = 0, C_RAYQ1 = 1, C_SHADOW = 2, C_CAT0 = 3, C_EXT_FETCH = C_CAT0 + N_CAT, C_SHD_FETCH, C_CAM_MEDIUM,
-       N_COUNTS = 16 };
-
-struct Paths {
-    float4* ray_o;   // origin.xyz
-    float4* ray_d;   // direction.xyz
-    float4* thr;     // throughput.xyz, eta_scale (RGB) | throughput, 4 lanes (spectral)
-    float4* rad;     // radiance.xyz (RGB) | 4 lanes (spectral)
-    uint2* rng;      // rng_base, rng_dim
-    float4* prev;    // prev_query.p.xyz, prev_bsdf_pdf
-    int* flags;      // bit 0 prev_was_delta, bits 8..15 null crossings, bits 16..31 bounce
-    int* medium;     // current_medium region (media scenes)
-    float2* hit;     // t (medium event t after binning), primitive id (bits)
-    int* q[2];       // ray queues (current / next iteration)
-    int* catq[N_CAT];// shading queues
-    int* counts;
-    int* s_path;
-    float4* s_o;     // shadow origin.xyz, maxDist (opaque) | crossings left (media, int bits)
-    float4* s_d;     // shadow direction.xyz, light pdf
-    float4* s_c;     // partial contribution.xyz (before visibility), MIS weight
-    float4* s_l;     // light point.xyz (media walker), MIS weight (spectral)
-    float4* wl;      // hero wavelengths (spectral)
-    float4* aov[2];  // first-hit guide samples (albedo, normal) of the paths in flight, or null
-    float* eta;      // eta_scale (spectral; RGB keeps it in thr.w)
-    unsigned long long* hist;   // [0] closest queries, [1] any-hit queries
-    unsigned long long* trav;   // [0] nodes visited, [1] triangles tested (COUNT builds)
-    // path-replay adjoint (ADJ kernels): SoA per parameter, [k * cap + path]
-    float* adj_D;               // dlog f / dtheta of the path so far (sampling detached)
-    float* adj_G;               // the path's gradient
-    float* adj_sD;              // [k * cap + shadow entry]: D at the NEE vertex incl. local terms
-    const float4* adj_delta;    // dL/dI per pixel (times 1/nframes)
-    int cap, npix;
-    int adj_medium;             // material whose medium sigma_s / sigma_a / g are parameters
-    int adj_n_albedo;
-    int adj_albedo[PT_ADJ_MAX_ALBEDO];
-    // batched wavefront (BATCH kernels): one DevScene per scene, the scene of each path slot
-    const DevScene* scenes;
-    int* path_scene;
-};
-// The scene a path belongs to: the kernel argument, or its entry in the batch.
-template <bool BATCH> PT_DEV const DevScene& scene_of(const DevScene& Sk, const Paths& P, int path) {
-    return BATCH ? P.scenes[P.path_scene[path]] : Sk;
-}
-constexpr int ADJ_N = PT_ADJ_N_PARAMS;
-
-// Adjoint helpers. Parameter layout: [0..2] sigma_s (channel c), [3..5] sigma_a (channel c),
-// [6] g, [7 + k] albedo k; a contribution c scatters e = dL/dI * c into G with the channel
-// structure of D (sigma terms act on one channel, g and albedos on all three).
-// D is loaded once per vertex and written back only when a parameter factor changed it; G is
-// read-modified-written only where a contribution lands (most vertices have none).
-struct AdjState { float D[ADJ_N]; bool dirty; };
-PT_DEV void adj_load(const Paths& P, int i, AdjState& A) {
-#pragma unroll
-    for (int k = 0; k < ADJ_N; k++) A.D[k] = P.adj_D[(size_t)k * P.cap + i];
-    A.dirty = false;
-}
-PT_DEV void adj_store(const Paths& P, int i, const AdjState& A) {
-    if (!A.dirty) return;
-#pragma unroll
-    for (int k = 0; k < ADJ_N; k++) P.adj_D[(size_t)k * P.cap + i] = A.D[k];
-}
-PT_DEV void adj_scatter(float* G, const float* D, f3 e);
-PT_DEV void adj_add_G(const Paths& P, int i, const float* D, f3 e) {
-    float G[ADJ_N];
-#pragma unroll
-    for (int k = 0; k < ADJ_N; k++) G[k] = P.adj_G[(size_t)k * P.cap + i];
-    adj_scatter(G, D, e);
-#pragma unroll
-    for (int k = 0; k < ADJ_N; k++) P.adj_G[(size_t)k * P.cap + i] = G[k];
-}
-PT_DEV void adj_scatter(float* G, const float* D, f3 e) {
-    G[0] += e.x * D[0]; G[1] += e.y * D[1]; G[2] += e.z * D[2];
-    G[3] += e.x * D[3]; G[4] += e.y * D[4]; G[5] += e.z * D[5];
-    const float es = (e.x + e.y) + e.z;
-#pragma unroll
-    for (int k = 6; k < ADJ_N; k++) G[k] += es * D[k];
-}
-PT_DEV f3 adj_e(const Paths& P, int path, sp4 c) {
-    float4 d = P.adj_delta[path % P.npix];
-    return mk3(d.x * c.x, d.y * c.y, d.z * c.z);
-}
-PT_DEV int adj_albedo_index(const Paths& P, int mat) {
-    int r = -1;
-    for (int k = 0; k < P.adj_n_albedo; k++) if (P.adj_albedo[k] == mat) r = k;
-    return r;
-}
-// add v to D[7 + k] without a run-time register index
-PT_DEV void adj_add_albedo(float* D, int k, float v) {
-#pragma unroll
-    for (int kk = 0; kk < PT_ADJ_MAX_ALBEDO; kk++) if (kk == k) D[7 + kk] += v;
-}
-// d log hg / dg at c = dot(wi, -wo) (volume_scattering/hg/hg.glsl:11-19)
-PT_DEV float dlog_hg(float g, float c) {
-    float d = 1.0f + g * g - 2.0f * g * c;
-    return -2.0f * g / (1.0f - g * g) - 3.0f * (g - c) / d;
-}
-
-PT_DEV int fl_bounce(int f) { return (f >> 16) & 0xFFFF; }
-PT_DEV int fl_nulls(int f) { return (f >> 8) & 0xFF; }
-// bit 1: the path's wavelengths collapsed to the hero (pt.ts:466-475, once per path)
-PT_DEV int mk_flags(bool delta, int nulls, int bounce, bool collapsed = false) {
-    return (delta ? 1 : 0) | (collapsed ? 2 : 0) | (nulls << 8) | (bounce << 16);
-}
-
-template <bool SPEC> PT_DEV void load_thr(const Paths& P, int i, sp4& T, float& eta) {
-    float4 t = P.thr[i];
-    if (SPEC) { T = ld4(t); eta = P.eta[i]; } else { T = mk4(t.x, t.y, t.z, 0.0f); eta = t.w; }
-}
-template <bool SPEC> PT_DEV void store_thr(const Paths& P, int i, sp4 T, float eta) {
-    if (SPEC) { P.thr[i] = st4(T); P.eta[i] = eta; } else P.thr[i] = make_float4(T.x, T.y, T.z, eta);
-}
-template <bool SPEC> PT_DEV sp4 load_rad(const Paths& P, int i) {
-    float4 v = P.rad[i];
-    return SPEC ? ld4(v) : mk4(v.x, v.y, v.z, 0.0f);
-}
-template <bool SPEC> PT_DEV void store_rad(const Paths& P, int i, sp4 L) {
-    P.rad[i] = SPEC ? st4(L) : make_float4(L.x, L.y, L.z, 0.0f);
-}
-template <bool SPEC> PT_DEV Wl load_wl(const Paths& P, int i) {
-    Wl w;
-    w.lam = SPEC ? P.wl[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    return w;
-}
-
-struct Camera {
-    float ox, oy, oz, fx, fy, fz, rx, ry, rz, ux, uy, uz, tan_fov, aspect;
-    uint32_t salt;
-    int off_x, off_y;
-    int ortho;          // PT_CAM_ORTHOGRAPHIC
-    float ortho_scale;
-    int thin;           // PT_CAM_THINLENS
-    float aperture, focus;
-    int model;          // the PT_CAM_* value (fisheye / cylindrical / equirect branches)
-    int projection;     // fisheye radial map
-    float fk, cyl_f;
-};
-
-// fisheye.glsl FISHEYE_THETA: the radial map of each projection.
-PT_DEV float fisheye_theta(int projection, float rho, float K) {
-    if (projection == PT_FISHEYE_EQUISOLID) return 2.0f * ptm_asin(gclamp(rho * K, 0.0f, 1.0f));
-    if (projection == PT_FISHEYE_STEREOGRAPHIC) return 2.0f * ptm_atan(rho * K);
-    if (projection == PT_FISHEYE_ORTHOGRAPHIC) return ptm_asin(gclamp(rho * K, 0.0f, 1.0f));
-    return rho * K;
-}
-
-// path_state_init's current_medium = scene_region_at(camera origin), once per wave.
-template <int STACK>
-__global__ void __launch_bounds__(64) k_camera_medium(DevScene S, Paths P, Camera C) {
-    __shared__ uint2 stk[STACK * 64];
-    if (threadIdx.x == 0) P.counts[C_CAM_MEDIUM] = region_at<64>(S, mk3(C.ox, C.oy, C.oz), stk);
-}
-// The same per path when camera rays do not share an origin (orthographic), after raygen.
-template <int STACK>
-__global__ void __launch_bounds__(SBLOCK) k_ray_medium(DevScene S, Paths P, int n) {
-    __shared__ uint2 stk[STACK * SBLOCK];
-    for (int base = blockIdx.x * SBLOCK; base < n; base += gridDim.x * SBLOCK) {
-        const int i = base + threadIdx.x;
-        if (i < n) P.medium[i] = region_at<SBLOCK>(S, xyz(P.ray_o[i]), stk);
-    }
-}
-
-// One camera sample of pixel (x, y) at sampleCount `sample` into path slot i.
-template <bool SPEC, bool ADJ>
-PT_DEV void raygen_path(const Paths& P, const Camera& C, int W, int H, int i, int x, int y, uint32_t sample, int media, int wl_uniform) {
-    // main() (features/accumulation.ts:14-26): pixel = gl_FragCoord.xy + u_pixelOffset
-    float fx = (float)x + 0.5f + (float)C.off_x;
-    float fy = (float)y + 0.5f + (float)C.off_y;
-    Rng r;
-    r.base = pcg4d_x((uint32_t)fx, (uint32_t)fy, sample, C.salt);
-    r.dim = 0;
-    float j0 = r.next(), j1 = r.next();
-    float filmx = fx + (j0 - 0.5f), filmy = fy + (j1 - 0.5f);   // pixel/box/box.glsl:5-7
-    float l0 = r.next(), l1 = r.next();                           // lens sample (read by the thin lens)
-    if (SPEC) {                                                   // wavelengths_sample (spectral.glsl:1-10)
-        float u = r.next();
-        float l[4];
-        for (int k = 0; k < 4; k++) l[k] = wavelength_from_cdf(wl_uniform != 0, ptm_fract(u + 0.25f * (float)k));
-        P.wl[i] = make_float4(l[0], l[1], l[2], l[3]);
-        P.eta[i] = 1.0f;
-    }
-    // camera/pinhole/pinhole.glsl:4-18; orthographic/orthographic.glsl:6-21 (the origin slides
-    // over the film plane, every ray looks along forward)
-    float ndcx = (2.0f * filmx / (float)W) - 1.0f;
-    float ndcy = (2.0f * filmy / (float)H) - 1.0f;
-    ndcx *= C.aspect;
-    f3 F = mk3(C.fx, C.fy, C.fz), R = mk3(C.rx, C.ry, C.rz), U = mk3(C.ux, C.uy, C.uz);
-    f3 org = mk3(C.ox, C.oy, C.oz), d;
-    if (C.ortho) {
-        org = (org + R * (ndcx * C.ortho_scale)) + U * (ndcy * C.ortho_scale);
-        d = F;
-    } else if (C.model == PT_CAM_FISHEYE) {                    // camera/fisheye/fisheye.glsl:23-38
-        const float rho = sqrtf(ndcx * ndcx + ndcy * ndcy);
-        if (rho < 1e-6f) d = F;                                // centre pixel: on axis
-        else {
-            const float th = fisheye_theta(C.projection, rho, C.fk);
-            const float tx = ndcx / rho, ty = ndcy / rho;
-            d = normalize(F * ptm_cos(th) + (R * tx + U * ty) * ptm_sin(th));
-        }
-    } else if (C.model == PT_CAM_CYLINDRICAL) {                // camera/cylindrical/cylindrical.glsl:9-21
-        const float cx = filmx - 0.5f * (float)W, cy = filmy - 0.5f * (float)H;
-        const float phi = cx / C.cyl_f, hh = cy / C.cyl_f;
-        d = normalize((F * ptm_cos(phi) + R * ptm_sin(phi)) + U * hh);
-    } else if (C.model == PT_CAM_EQUIRECT) {                   // camera/equirect/equirect.glsl:9-22
-        const float u = filmx / (float)W, v = filmy / (float)H;
-        const float phi = (u - 0.5f) * PTM_TWO_PI, th = (v - 0.5f) * PTM_PI;
-        const float ct = ptm_cos(th);
-        d = normalize((F * ptm_cos(phi) + R * ptm_sin(phi)) * ct + U * ptm_sin(th));
-    } else {
-        d = normalize((F + R * (ndcx * C.tan_fov)) + U * (ndcy * C.tan_fov));
-        if (C.thin) {                                              // camera/thinlens/thinlens.glsl:6-33
-            f3 focus = org + d * (C.focus / dot(d, F));
-            float2 lens = concentric_disk(l0, l1);
-            org = (org + R * (lens.x * C.aperture)) + U * (lens.y * C.aperture);
-            d = normalize(focus - org);
-        }
-    }
-    P.ray_o[i] = f4(org, 0.0f);
-    P.ray_d[i] = f4(d, 0.0f);
-    P.thr[i] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
-    P.rad[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    P.rng[i] = make_uint2(r.base, r.dim);
-    for (int a = 0; a < 2; a++)
-        if (P.aov[a]) P.aov[a][i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);   // 0 unless a surface is reached
-    P.prev[i] = f4(org, 0.0f);
-    P.flags[i] = mk_flags(true, 0, 0);
-    if (media) P.medium[i] = P.counts[C_CAM_MEDIUM];
-    P.q[0][i] = i;
-    if (ADJ) {
-#pragma unroll
-        for (int k = 0; k < ADJ_N; k++) { P.adj_D[(size_t)k * P.cap + i] = 0.0f; P.adj_G[(size_t)k * P.cap + i] = 0.0f; }
-    }
-}
-
-template <bool SPEC, bool ADJ>
-__global__ void __launch_bounds__(SBLOCK) k_raygen(Paths P, Camera C, int W, int H, int n_paths, uint32_t sample0, int media, int wl_uniform) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) P.counts[C_RAYQ0] = n_paths;
-    if (i >= n_paths) return;
-    int pix = i % (W * H), frame = i / (W * H);
-    raygen_path<SPEC, ADJ>(P, C, W, H, i, pix % W, pix / W, sample0 + (uint32_t)frame, media, wl_uniform);
-}
-
-// Batched wavefront: a wave is a list of (scene, first frame, frames) chunks laid out back to
-// back in the path slots; every scene keeps its own camera, film and sample order.
-struct BatchEntry {
-    int scene, W, H, npix;
-    int f0, nf;
-    long long base;       // first path slot of the chunk
-    long long film;       // first pixel of the scene's film
-};
-__global__ void __launch_bounds__(SBLOCK) k_raygen_batch(Paths P, const BatchEntry* E, int ne, const Camera* cams, int n_paths) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) P.counts[C_RAYQ0] = n_paths;
-    if (i >= n_paths) return;
-    int lo = 0, hi = ne - 1;                      // last entry with base <= i
-    while (lo < hi) { int mid = (lo + hi + 1) >> 1; if (E[mid].base <= (long long)i) lo = mid; else hi = mid - 1; }
-    const BatchEntry& e = E[lo];
-    const int local = (int)((long long)i - e.base);
-    const int pix = local % e.npix, frame = e.f0 + local / e.npix;
-    P.path_scene[i] = e.scene;
-    raygen_path<false, false>(P, cams[e.scene], e.W, e.H, i, pix % e.W, pix / e.W, (uint32_t)frame, 0, 0);
-}
-__global__ void __launch_bounds__(SBLOCK) k_accumulate_batch(const Paths P, const BatchEntry* E, float4* films) {
-    const BatchEntry& e = E[blockIdx.y];
-    const int pix = blockIdx.x * blockDim.x + threadIdx.x;
-    if (pix >= e.npix) return;
-    float4* film = films + e.film;
-    float4 acc = film[pix];
-    for (int f = 0; f < e.nf; f++) {
-        const uint32_t n = (uint32_t)(e.f0 + f);
-        float4 x = P.rad[e.base + (long long)f * e.npix + pix];
-        if (n == 0) acc = make_float4(x.x, x.y, x.z, 1.0f);
-        else {
-            float w = 1.0f / ((float)n + 1.0f);
-            acc = make_float4(acc.x * (1.0f - w) + x.x * w, acc.y * (1.0f - w) + x.y * w, acc.z * (1.0f - w) + x.z * w, 1.0f);
-        }
-    }
-    film[pix] = acc;
-}
-__global__ void __launch_bounds__(SBLOCK) k_fill_f4(float4* p, long long n) {
-    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    for (; i < n; i += (long long)gridDim.x * blockDim.x) p[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-}
-
-// Persistent-lane work distribution: each wave owns a chunk of queue entries and hands them
-// to lanes as they go idle (no per-ray atomics; one atomic per CHUNK entries per wave).
-struct LaneFeeder {
-    uint32_t next, end;
-    bool exhausted;
-    PT_DEV void init() { next = end = 0; exhausted = false; }
-    // Idle lanes get a queue index (or -1). Convergent: every lane of the wave calls it.
-    template <int CHUNK>
-    PT_DEV int feed(int* counter, uint32_t n, bool idle, int min_idle) {
-        uint64_t im = __ballot(idle);
-        int nidle = (int)__popcll(im);
-        if (nidle == 0 || exhausted) return -1;
-        uint64_t am = __ballot(!idle);
-        if (am != 0 && nidle < min_idle) return -1;
-        int lane = (int)__lane_id();
-        if (end == next) {
-            uint32_t base = 0;
-            if (lane == 0) base = (uint32_t)atomicAdd(counter, CHUNK);
-            base = __shfl(base, 0);
-            if (base >= n) { exhausted = true; return -1; }
-            next = base;
-            end = min(base + (uint32_t)CHUNK, n);
-        }
-        uint32_t avail = end - next;
-        uint32_t pre = (uint32_t)__popcll(im & ((1ull << lane) - 1ull));
-        int idx = (idle && pre < avail) ? (int)(next + pre) : -1;
-        next += min((uint32_t)nidle, avail);
-        return idx;
-    }
-};
-
-#ifndef PTMI_FEED_CHUNK
-#define PTMI_FEED_CHUNK 64
-#endif
-#ifndef PTMI_FEED_MIN_IDLE
-#define PTMI_FEED_MIN_IDLE 8
-#endif
-#ifndef PTMI_POSTPONE_EXT
-#define PTMI_POSTPONE_EXT 0
-#endif
-#ifndef PTMI_POSTPONE_SHD
-#define PTMI_POSTPONE_SHD 0
-#endif
-constexpr int POSTPONE_EXT = PTMI_POSTPONE_EXT;    // triangle postponing below this many lanes (0: off)
-constexpr int POSTPONE_SHD = PTMI_POSTPONE_SHD;    // the same for opaque shadow rays
-constexpr int FEED_CHUNK = PTMI_FEED_CHUNK;        // queue entries a wave claims per atomic
-constexpr int FEED_MIN_IDLE = PTMI_FEED_MIN_IDLE;  // idle lanes before a wave refills
-
-// extend: closest hit for every ray of the current queue, then bin the path by shading category.
-// INST: the scene has instance batches. The TLAS walk keeps a private stack, so only scenes
-// with placements compile it in (the plain kernels then need no scratch memory).
-template <int STACK, bool COUNT, bool BATCH = false, int TB = TBLOCK, bool INST = false>
-__global__ void __launch_bounds__(TB) k_extend(DevScene Sk, Paths P, int qsel) {
-    __shared__ uint2 stk[STACK * TB];
-    const uint32_t n = (uint32_t)P.counts[qsel];
-    const int* q = P.q[qsel];
-    TraceCounters cnt = {0, 0};
-    LaneFeeder feed; feed.init();
-    TravState st;
-    bool active = false;
-    int path = -1;
-    const DevScene* Sp = &Sk;
-    for (;;) {
-        int k = feed.feed<FEED_CHUNK>(&P.counts[C_EXT_FETCH], n, !active, FEED_MIN_IDLE);
-        if (k >= 0) {
-            path = q[k];
-            Sp = &scene_of<BATCH>(Sk, P, path);
-            float4 o4 = P.ray_o[path], d4 = P.ray_d[path];
-            trace_init(st, xyz(o4), xyz(d4), MAX_DIST);
-            active = true;
-        }
-        if (__ballot(active) == 0) {
-            if (feed.exhausted) break;
-            continue;
-        }
-        if (active && (Sp->n_cw_nodes == 0 ||
-                       trace_step<false, TB, COUNT, POSTPONE_EXT, STACK>(Sp->cw_nodes, Sp->cw_tris, st, stk, cnt, Sp->cw_depth))) {
-            float tmax = st.tmax; int prim = st.prim;
-            trace_analytic<false>(*Sp, st.o, st.d, tmax, prim);
-            if (INST) trace_instances<false, TB, COUNT>(*Sp, st.o, st.d, tmax, prim, stk, cnt);
-            P.hit[path] = make_float2(tmax, __int_as_float(prim));
-            active = false;
-        }
-    }
-    if (COUNT) {
-        atomicAdd(&P.trav[0], (unsigned long long)cnt.nodes);
-        atomicAdd(&P.trav[1], (unsigned long long)cnt.tris);
-    }
-}
-
-// shadow (opaque scenes): any hit for every NEE ray; unoccluded contributions are added to
-// their path as ((a * vis) * w) / pdf, the reference's operation order with vis = 1.
-template <int STACK, bool SPEC, bool ADJ, bool BATCH = false, int TB = TBLOCK, bool INST = false>
-__global__ void __launch_bounds__(TB) k_shadow(DevScene Sk, Paths P) {
-    __shared__ uint2 stk[STACK * TB];
-    const int n = P.counts[C_SHADOW];
-    TraceCounters cnt = {0, 0};
-    for (int base = blockIdx.x * TB; base < n; base += gridDim.x * TB) {
-        int k = base + threadIdx.x;
-        if (k >= n) break;
-        const DevScene& S = scene_of<BATCH>(Sk, P, P.s_path[k]);
-        float4 o4 = P.s_o[k], d4 = P.s_d[k];
-        f3 o = xyz(o4), d = xyz(d4);
-        float maxd = o4.w; int prim = -1;
-        bool blocked = trace_analytic<true>(S, o, d, maxd, prim);
-        if (!blocked) blocked = trace_wide<true, STACK, TB, false, POSTPONE_SHD>(S, o, d, maxd, prim, stk, cnt);
-        if (INST && !blocked) blocked = trace_instances<true, TB, false>(S, o, d, maxd, prim, stk, cnt);
-        if (!blocked) {
-            int i = P.s_path[k];
-            float4 c = P.s_c[k];
-            const float w = SPEC ? P.s_l[k].w : c.w;
-            sp4 a = SPEC ? ld4(c) : mk4(c.x, c.y, c.z, 0.0f);
-            const sp4 v = (a * w) / d4.w;
-            store_rad<SPEC>(P, i, load_rad<SPEC>(P, i) + v);
-            if (ADJ) {
-                float D[ADJ_N];
-#pragma unroll
-                for (int kk = 0; kk < ADJ_N; kk++) D[kk] = P.adj_sD[(size_t)kk * P.cap + k];
-                adj_add_G(P, i, D, adj_e(P, i, v));
-            }
-        }
-    }
-}
-
-// shadow (media scenes): the segment walker of transport/shadow/media/media.glsl:29-62 —
-// closest hits up to the light point, Beer-Lambert per segment in the current medium, pass
-// through null interfaces, anything else blocks (dielectrics are opaque to shadow rays).
-template <int STACK, bool SPEC, bool ADJ, bool INST = false>
-__global__ void __launch_bounds__(TBLOCK) k_shadow_media(DevScene S, Paths P) {
-    __shared__ uint2 stk[STACK * TBLOCK];
-    const int n = P.counts[C_SHADOW];
-    TraceCounters cnt = {0, 0};
-    for (int base = blockIdx.x * TBLOCK; base < n; base += gridDim.x * TBLOCK) {
-        int k = base + threadIdx.x;
-        if (k >= n) break;
-        float4 o4 = P.s_o[k], d4 = P.s_d[k], l4 = P.s_l[k];
-        f3 o = xyz(o4), d = xyz(d4), light_p = xyz(l4);
-        const int crossings_left = __float_as_int(o4.w);
-        const Wl wl = load_wl<SPEC>(P, P.s_path[k]);
-        sp4 T = sgrey(1.0f);
-        float len_param = 0.0f;     // length travelled inside the parameter medium
-        bool blocked = true;
-        int medium = region_at<TBLOCK>(S, o, stk);
-        for (int crossed = 0; crossed <= crossings_left; crossed++) {
-            float remaining = length(light_p - o) - SHADOW_BACKOFF;
-            if (remaining <= 0.0f) { blocked = false; break; }
-            float tmax = MAX_DIST; int prim = -1;
-            if (S.n_cw_nodes > 0) trace_wide<false, STACK, TBLOCK, false>(S, o, d, tmax, prim, stk, cnt);
-            trace_analytic<false>(S, o, d, tmax, prim);
-            if (INST) trace_instances<false, TBLOCK, false>(S, o, d, tmax, prim, stk, cnt);
-            bool hit_boundary = prim >= 0 && tmax < remaining;
-            float seg_len = hit_boundary ? tmax : remaining;
-            int med_mat = material_of(S, medium);
-            if (material_has_medium(S, med_mat)) T = T * medium_transmittance<SPEC>(S, S.materials[med_mat], seg_len, wl);
-            if (ADJ && med_mat == P.adj_medium && material_has_medium(S, med_mat)) len_param += seg_len;
-            if (!hit_boundary) { blocked = false; break; }
-            Hit h; bool back;
-            fill_hit(S, o, d, tmax, prim, h, back);
-            if (!is_null_interface(S, material_of(S, h.owner))) break;
-            classify_hit<TBLOCK>(S, h, back, true, stk);
-            medium = h.region_to;
-            float side = dot(d, h.ng) >= 0.0f ? 1.0f : -1.0f;
-            o = h.p + (h.ng * side) * h.eps;
-            d = normalize(light_p - o);
-        }
-        if (!blocked && !is_black<SPEC>(T)) {
-            int i = P.s_path[k];
-            float4 c = P.s_c[k];
-            const float w = SPEC ? l4.w : c.w;
-            sp4 a = SPEC ? ld4(c) : mk4(c.x, c.y, c.z, 0.0f);
-            const sp4 v = ((a * T) * w) / d4.w;
-            store_rad<SPEC>(P, i, load_rad<SPEC>(P, i) + v);
-            if (ADJ) {
-                float D[ADJ_N];
-#pragma unroll
-                for (int kk = 0; kk < ADJ_N; kk++) D[kk] = P.adj_sD[(size_t)kk * P.cap + k];
-#pragma unroll
-                for (int kk = 0; kk < 6; kk++) D[kk] -= len_param;     // e^{-sigma_t len} along the shadow ray
-                adj_add_G(P, i, D, adj_e(P, i, v));
-            }
-        }
-    }
-}
-
-// Block-aggregated queue appends: LDS atomics give each lane its slot inside the block's
-// share, then one global atomic per queue per block. Every thread of the block must call it.
-template <int K>
-PT_DEV int block_append(int* const* counters, int which, int* s_cnt, int* s_base) {
-    if (threadIdx.x < K) s_cnt[threadIdx.x] = 0;
-    __syncthreads();
-    int local = which >= 0 ? atomicAdd(&s_cnt[which], 1) : -1;
-    __syncthreads();
-    if (threadIdx.x < K) s_base[threadIdx.x] = s_cnt[threadIdx.x] ? atomicAdd(counters[threadIdx.x], s_cnt[threadIdx.x]) : 0;
-    __syncthreads();
-    return which >= 0 ? s_base[which] + local : -1;
-}
-
-// bin: per path of the iteration, the medium segment first (media scenes: the analytic
-// channel-MIS distance sample of pt.ts's media branch, which may end the segment in a
-// scattering event), then the shading category of the vertex.
-template <bool MEDIA, bool SPEC, bool ADJ, bool BATCH = false>
-__global__ void __launch_bounds__(SBLOCK) k_bin(DevScene Sk, Paths P, int qsel, int equiangular, int max_bounces, int max_null) {
-    __shared__ int s_cnt[N_CAT], s_base[N_CAT];
-    int* ctr[N_CAT];
-    for (int c = 0; c < N_CAT; c++) ctr[c] = &P.counts[C_CAT0 + c];
-    int* sctr[1] = {&P.counts[C_SHADOW]};
-    const int n = P.counts[qsel];
-    const int* q = P.q[qsel];
-    for (int base = blockIdx.x * SBLOCK; base < n; base += gridDim.x * SBLOCK) {
-        int k = base + threadIdx.x;
-        int cat = -1, path = 0;
-        bool eq_shadow = false;
-        f3 eq_o = mk3(0, 0, 0), eq_wi = mk3(0, 0, 0), eq_pos = mk3(0, 0, 0);
-        sp4 eq_a = sgrey(0.0f);
-        float eq_pdf = 1.0f;
-        int eq_cross = 0;
-        if (k < n) {
-            path = q[k];
-            const DevScene& S = scene_of<BATCH>(Sk, P, path);
-            float2 hv = P.hit[path];
-            int prim = __float_as_int(hv.y);
-            if (MEDIA) {
-                int med_mat = material_of(S, P.medium[path]);
-                if (material_has_medium(S, med_mat)) {
-                    uint2 rs = P.rng[path];
-                    Rng rng; rng.base = rs.x; rng.dim = rs.y;
-                    const float t_max = prim >= 0 ? hv.x : MAX_DIST;
-                    const int fl = P.flags[path];
-                    if (!ADJ && !BATCH && equiangular && fl_bounce(fl) < max_bounces) {
-                        // equiangular_sample_direct (techniques/equiangular/equiangular.glsl:18-57): one
-                        // direct-light vertex per medium segment, from segment-start throughput; the
-                        // shadow walker runs before this iteration's shading, in the reference's order.
-                        const pt_material& Mm = S.materials[med_mat];
-                        const Wl wl = load_wl<SPEC>(P, path);
-                        const float uc = rng.next();
-                        int id = S.n_lights - 1;
-                        for (int li = 0; li < S.n_lights - 1; li++) if (uc < S.lights[li].cdf) { id = li; break; }
-                        const pt_light& l = S.lights[id];
-                        const float select_pdf = l.select_pdf;
-                        const f3 pos = ld3(l.corner);
-                        const sp4 intensity = lift<SPEC>(S, l.radiance, l.s_radiance, wl);
-                        const f3 o = xyz(P.ray_o[path]), d = xyz(P.ray_d[path]);
-                        const f3 D = pos - o;
-                        const float t_c = dot(D, d);
-                        const float h2 = gmax(dot(D, D) - t_c * t_c, 1e-8f);
-                        const float h = sqrtf(h2);
-                        const float theta_a = ptm_atan2(0.0f - t_c, h);
-                        const float theta_b = ptm_atan2(t_max - t_c, h);
-                        const float u = rng.next();
-                        const float theta = glsl_mix(theta_a, theta_b, u);
-                        const float t = t_c + h * ptm_tan(theta);
-                        const float pdf_t = h / ((theta_b - theta_a) * (h2 + (t - t_c) * (t - t_c)));
-                        const f3 p_evt = o + d * t;
-                        sp4 Tp; float etap;
-                        load_thr<SPEC>(P, path, Tp, etap);
-                        const sp4 T_seg = spectrum_exp((-(sigma_a_of<SPEC>(S, Mm, wl) + sigma_s_of<SPEC>(S, Mm, wl))) * t);
-                        const f3 to_light = pos - p_evt;
-                        const float d2 = gmax(dot(to_light, to_light), 1e-8f);
-                        const float dist = sqrtf(d2);
-                        eq_wi = mk3(to_light.x / dist, to_light.y / dist, to_light.z / dist);
-                        const float f = hg_value(eq_wi, -d, Mm.hg_g);
-                        eq_a = (((Tp * T_seg) * sigma_s_of<SPEC>(S, Mm, wl)) * sgrey(f)) * (intensity / d2);
-                        eq_pdf = pdf_t * select_pdf;
-                        eq_o = p_evt; eq_pos = pos;
-                        eq_cross = S.has_null ? max_null - fl_nulls(fl) : 0;
-                        eq_shadow = true;
-                    }
-                    float x0 = rng.next(), x1 = rng.next();
-                    const pt_material& Mm = S.materials[med_mat];
-                    const Wl wl = load_wl<SPEC>(P, path);
-                    // medium_sample dispatch by the material's arm (features/materials.ts:675-728)
-                    MSample ms = Mm.medium_arm == PT_MED_DELTA ? medium_sample_delta<SPEC>(S, Mm, t_max, x0, x1, rng, wl)
-                               : Mm.medium_arm == PT_MED_ABSORB ? medium_sample_absorb<SPEC>(S, Mm, t_max, wl)
-                               : medium_sample_analytic<SPEC>(S, Mm, t_max, x0, x1, wl);
-                    sp4 T; float eta;
-                    load_thr<SPEC>(P, path, T, eta);
-                    // the source term, weighted by segment-start throughput (pt.ts:333-339)
-                    if (medium_emits(Mm)) store_rad<SPEC>(P, path, load_rad<SPEC>(P, path) + T * ms.radiance);
-                    store_thr<SPEC>(P, path, T * ms.weight, eta);
-                    if (ADJ && med_mat == P.adj_medium) {   // f = sigma_s e^{-sigma_t t} | e^{-sigma_t t_max}
-                        const float* ssv = S.materials[med_mat].sigma_s;
-#pragma unroll
-                        for (int ch = 0; ch < 3; ch++) {
-                            const size_t ks = (size_t)ch * P.cap + path, ka = (size_t)(3 + ch) * P.cap + path;
-                            if (Mm.medium_arm != PT_MED_ABSORB)
-                                P.adj_D[ks] += (ms.scattered && ssv[ch] > 0.0f ? 1.0f / ssv[ch] : 0.0f) - ms.t;
-                            P.adj_D[ka] -= ms.t;
-                        }
-                    }
-                    P.rng[path] = make_uint2(rng.base, rng.dim);
-                    if (ms.scattered) { cat = CAT_MEDIUM; P.hit[path] = make_float2(ms.t, hv.y); }
-                }
-            }
-            if (cat < 0) {
-                if (prim < 0) cat = CAT_MISS;
-                else {
-                    int owner = prim_owner(S, prim);
-                    int model = S.materials[S.objects[owner].material].model;
-                    cat = model == PT_MAT_LAMBERT ? CAT_LAMBERT
-                        : model == PT_MAT_GGX ? CAT_GGX
-                        : model == PT_MAT_DIELECTRIC ? CAT_DIELECTRIC
-                        : model == PT_MAT_MIRROR ? CAT_MIRROR
-                        : model == PT_MAT_ROUGH_DIELECTRIC ? CAT_ROUGH : CAT_NULL;
-                }
-            }
-        }
-        int slot = block_append<N_CAT>(ctr, cat, s_cnt, s_base);
-        if (cat >= 0) P.catq[cat][slot] = path;
-        if (MEDIA && !ADJ && !BATCH && equiangular) {
-            int ss = block_append<1>(sctr, eq_shadow ? 0 : -1, s_cnt, s_base);
-            if (eq_shadow) {   // the shadow walker adds ((a * vis) * 1) / pdf
-                P.s_path[ss] = path;
-                P.s_o[ss] = f4(eq_o, __int_as_float(eq_cross));
-                P.s_d[ss] = f4(eq_wi, eq_pdf);
-                P.s_c[ss] = SPEC ? st4(eq_a) : make_float4(eq_a.x, eq_a.y, eq_a.z, 1.0f);
-                P.s_l[ss] = f4(eq_pos, 1.0f);
-            }
-        }
-    }
-}
-
-// After the equiangular shadow rays of an iteration: count them and empty the shadow queue
-// for the surface NEE of the same iteration.
-__global__ void k_shadow_reset(Paths P) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        P.hist[1] += (unsigned long long)P.counts[C_SHADOW];
-        P.counts[C_SHADOW] = 0;
-    }
-}
-
-// Queue bookkeeping before an iteration (single thread).
-__global__ void k_bookkeep(Paths P, int cur, int next, int first) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        if (!first) P.hist[1] += (unsigned long long)P.counts[C_SHADOW];
-        if (cur >= 0) P.hist[0] += (unsigned long long)P.counts[cur];
-        if (next >= 0) P.counts[next] = 0;
-        P.counts[C_SHADOW] = 0;
-        for (int c = 0; c < N_CAT; c++) P.counts[C_CAT0 + c] = 0;
-        P.counts[C_EXT_FETCH] = 0;
-        P.counts[C_SHD_FETCH] = 0;
-    }
-}
-
-// The per-vertex body of transport_trace (integrators/pt/pt.ts:299-464) with the combiner
-// (transport/combiner.ts:23-102) and the kernel/light techniques (techniques/kernel/kernel.glsl,
-// kernel_phase.glsl, techniques/light/light.glsl, light_medium.glsl), for the paths of one
-// category. MEDIA selects the media program (region tracking, medium NEE shadow walker).
-template <int EST, int CAT, int STACK, bool MEDIA, bool SPEC, bool ADJ, bool BATCH = false>
-__global__ void __launch_bounds__(SBLOCK) k_shade(DevScene Sk, Paths P, int qsel, int max_bounces,
-                                                  int rr_start, float rr_max, int max_null, int equiangular = 0) {
-    constexpr bool PROBE = cat_probes(CAT, MEDIA);
-    // The region probe's LDS stack: static in the probing categories; the other surface
-    // categories probe only at a back-face hit in a scene with an emissive solid, and are then
-    // launched with the same stack as dynamic LDS (shade_dynamic_lds).
-    constexpr bool STATIC_STK = PROBE && CAT != CAT_MISS && CAT != CAT_MEDIUM;
-    __shared__ uint2 stk_static[STATIC_STK ? STACK * SBLOCK : 1];
-    extern __shared__ uint2 stk_dynamic[];
-    uint2* stk = STATIC_STK ? stk_static : stk_dynamic;
-    __shared__ int s_cnt[2], s_base[2];
-    int* ctr[2] = {&P.counts[qsel ^ 1], &P.counts[C_SHADOW]};
-    const int n = P.counts[C_CAT0 + CAT];
-    const int* q = P.catq[CAT];
-    int* qn = P.q[qsel ^ 1];
-    constexpr bool NEE = EST != PT_EST_PT;
-    constexpr bool MIS = EST == PT_EST_MIS;
-    constexpr int MODEL = CAT == CAT_LAMBERT ? PT_MAT_LAMBERT : CAT == CAT_GGX ? PT_MAT_GGX
-                        : CAT == CAT_MIRROR ? PT_MAT_MIRROR : CAT == CAT_ROUGH ? PT_MAT_ROUGH_DIELECTRIC
-                        : PT_MAT_DIELECTRIC;
-    for (int base = blockIdx.x * SBLOCK; base < n; base += gridDim.x * SBLOCK) {
-        const int k = base + threadIdx.x;
-        const bool valid = k < n;
-        const int i = valid ? q[k] : 0;
-        bool cont = false, shadow = false;
-        f3 s_o = mk3(0, 0, 0), s_d = mk3(0, 0, 0), s_l = mk3(0, 0, 0);
-        sp4 s_a = sgrey(0.0f);
-        float s_max = 0.0f, s_w = 0.0f, s_pdf = 1.0f;
-        int s_cross = 0;
-        AdjState A;
-        float sD[ADJ_N];
-        if (valid) {
-            const DevScene& S = scene_of<BATCH>(Sk, P, i);
-            if (ADJ && (CAT != CAT_MISS || S.env_type != PT_ENV_NONE)) adj_load(P, i, A);
-            else A.dirty = false;
-            f3 rd = xyz(P.ray_d[i]);
-            sp4 T; float eta_scale;
-            load_thr<SPEC>(P, i, T, eta_scale);
-            sp4 L = load_rad<SPEC>(P, i);
-            const Wl wl = load_wl<SPEC>(P, i);
-            float4 pv = P.prev[i];
-            f3 prev_qp = xyz(pv); float prev_pdf = pv.w;
-            const int fl = P.flags[i];
-            bool prev_delta = (fl & 1) != 0;
-            bool collapsed = SPEC && (fl & 2) != 0;
-            int nulls = fl_nulls(fl);
-            const int bounce = fl_bounce(fl);
-            int medium = MEDIA ? P.medium[i] : -1;
-            const int crossings_left = S.has_null ? max_null - nulls : 0;
-            if (CAT == CAT_MISS) {
-                // kernel_score_miss + combiner_w_env
-                float w = 1.0f;
-                if (NEE && env_samplable(S)) {
-                    if (prev_delta) w = 1.0f;
-                    else if (MIS) w = power_heuristic(prev_pdf, env_select_live(S) ? S.env_select_prob * environment_pdf(S, rd) : environment_pdf(S, rd));
-                    else w = 0.0f;
-                }
-                if (S.env_type != PT_ENV_NONE) {
-                    const sp4 c = (T * w) * env_radiance<SPEC>(S, rd, wl);
-                    L = L + c;
-                    if (ADJ) adj_add_G(P, i, A.D, adj_e(P, i, c));
-                }
-            } else if (CAT == CAT_MEDIUM) {
-                // ---- medium event (pt.ts media branch: scattered) ----
-                if (bounce < max_bounces) {
-                    const int mmat = material_of(S, medium);
-                    const pt_material& Mm = S.materials[mmat];
-                    const float g = Mm.hg_g;
-                    const bool param_medium = ADJ && mmat == P.adj_medium;
-                    f3 ro = xyz(P.ray_o[i]);
-                    const float t = P.hit[i].x;
-                    f3 p_evt = ro + rd * t;
-                    f3 wo_med = -rd;
-                    uint2 rs = P.rng[i];
-                    Rng rng; rng.base = rs.x; rng.dim = rs.y;
-                    if (NEE && !equiangular) {   // light_sample_direct_medium
-                        float y0 = rng.next(), y1 = rng.next();
-                        LSample ls = lighting_sample<SPEC>(S, p_evt, y0, y1, wl);
-                        if (ls.pdf > 0.0f) {
-                            s_l = p_evt + ls.wi * ls.distance;
-                            s_o = p_evt; s_d = ls.wi; s_cross = crossings_left;
-                            float f = hg_value(ls.wi, wo_med, g);
-                            s_w = 1.0f;
-                            if (MIS && !(ls.flags & 1)) s_w = power_heuristic(ls.pdf, hg_pdf<SPEC>(ls.wi, wo_med, g));
-                            s_a = (T * ls.radiance) * sgrey(f);
-                            s_pdf = ls.pdf;
-                            shadow = true;
-                            if (ADJ) {
-#pragma unroll
-                                for (int kk = 0; kk < ADJ_N; kk++) sD[kk] = A.D[kk];
-                                if (param_medium) sD[6] += dlog_hg(g, dot(ls.wi, -wo_med));
-                            }
-                        }
-                    }
-                    // kernel_sample_phase
-                    float z0 = rng.next(), z1 = rng.next();
-                    float ppdf;
-                    f3 wi = hg_sample<SPEC>(wo_med, g, z0, z1, ppdf);
-                    if (param_medium) { A.D[6] += dlog_hg(g, dot(wi, -wo_med)); A.dirty = true; }
-                    cont = true;
-                    // roulette_interior, present when some medium weights absorption; medium_survival =
-                    // sigma_s / max(sigma_t, 1e-9) there, 1 on the tracking arm (features/materials.ts:731-772)
-                    if (rr_start >= 0 && bounce >= rr_start && S.weighted_absorption) {
-                        sp4 surv = sgrey(1.0f);
-                        if (Mm.medium_arm == PT_MED_ANALYTIC) {
-                            sp4 ss = sigma_s_of<SPEC>(S, Mm, wl), st = ss + sigma_a_of<SPEC>(S, Mm, wl);
-                            surv = mk4(ss.x / gmax(st.x, 1e-9f), ss.y / gmax(st.y, 1e-9f), ss.z / gmax(st.z, 1e-9f), ss.w / gmax(st.w, 1e-9f));
-                        }
-                        float p = spectrum_max<SPEC>(surv);
-                        if (p <= 0.0f) cont = false;
-                        else if (rng.next() > p) cont = false;
-                        else T = T / p;
-                    }
-                    if (cont) {
-                        P.ray_o[i] = f4(p_evt, 0.0f);
-                        P.ray_d[i] = f4(wi, 0.0f);
-                        store_thr<SPEC>(P, i, T, eta_scale);
-                        P.prev[i] = f4(p_evt, ppdf);
-                        P.flags[i] = mk_flags(false, nulls, bounce + 1, collapsed);
-                    }
-                    P.rng[i] = make_uint2(rng.base, rng.dim);
-                }
-            } else {
-                float2 hv = P.hit[i];
-                const int prim = __float_as_int(hv.y);
-                f3 ro = xyz(P.ray_o[i]);
-                const bool emitters = S.n_lights > 0;
-                Hit h; bool back;
-                fill_hit(S, ro, rd, hv.x, prim, h, back);
-                const pt_material& M = S.materials[S.objects[h.owner].material];
-                bool need = PROBE || (S.emissive_solid && back);
-                classify_hit<SBLOCK>(S, h, back, need, stk);
-                // first-hit guides (pt.ts:422-433): the first non-null surface with no event before
-                if (!SPEC && CAT != CAT_NULL && bounce == 0 && (P.aov[0] || P.aov[1])) {
-                    const sp4 Ta = T;                            // the path's weight on arrival
-                    if (P.aov[0]) {                                  // aov/albedo/albedo.glsl
-                        const float* row = MODEL == PT_MAT_LAMBERT ? M.albedo
-                                         : (MODEL == PT_MAT_GGX || MODEL == PT_MAT_MIRROR) ? M.f0 : M.transmittance;
-                        P.aov[0][i] = make_float4(gclamp(row[0], 0.0f, 1.0f) * Ta.x, gclamp(row[1], 0.0f, 1.0f) * Ta.y,
-                                                  gclamp(row[2], 0.0f, 1.0f) * Ta.z, 0.0f);
-                    }
-                    if (P.aov[1]) {                                  // aov/normal/normal.glsl
-                        const f3 n = dot(h.frame.n, -rd) < 0.0f ? -h.frame.n : h.frame.n;
-                        const float avg = ((Ta.x + Ta.y) + Ta.z) * (1.0f / 3.0f);
-                        P.aov[1][i] = make_float4(n.x * avg, n.y * avg, n.z * avg, 0.0f);
-                    }
-                }
-                if (MEDIA && h.region_from != medium) medium = h.region_from;   // self-heal
-                if (CAT == CAT_NULL) {
-                    // null interface: not an optical event; no bounce consumed
-                    medium = h.region_to;
-                    float side = dot(rd, h.ng) >= 0.0f ? 1.0f : -1.0f;
-                    nulls++;
-                    cont = nulls <= max_null;
-                    if (cont) {
-                        P.ray_o[i] = f4(h.p + (h.ng * side) * h.eps, 0.0f);
-                        P.flags[i] = mk_flags(prev_delta, nulls, bounce, collapsed);
-                        if (MEDIA) P.medium[i] = medium;
-                    }
-                } else {
-                    uint2 rs = P.rng[i];
-                    Rng rng; rng.base = rs.x; rng.dim = rs.y;
-                    f3 wo = -rd;
-                    // kernel_score_emitter_hit
-                    int mat_emit = h.region_to >= -1 ? material_of(S, h.region_to) : -1;
-                    if (mat_emit >= 0) {
-                        const pt_material& E = S.materials[mat_emit];
-                        if (E.emissive) {
-                            float w = 1.0f;
-                            if (NEE && emitters) {
-                                int lid = light_of(S, h.region_to);
-                                if (lid < 0 || prev_delta) w = 1.0f;
-                                else if (MIS) w = power_heuristic(prev_pdf, lighting_pdf(S, prev_qp, rd, lid, h.p));
-                                else w = 0.0f;
-                            }
-                            const sp4 c = (T * w) * lift<SPEC>(S, E.emission, E.s_emission, wl);
-                            L = L + c;
-                            if (ADJ) adj_add_G(P, i, A.D, adj_e(P, i, c));
-                        }
-                    }
-                    if (bounce < max_bounces) {
-                        // collapse_wavelengths (pt.ts:440,466-475): a dielectric bounding a dispersive
-                        // region sends the path along the hero wavelength, lane 0 carrying all four
-                        if (SPEC && (CAT == CAT_DIELECTRIC || CAT == CAT_ROUGH) && !collapsed &&
-                            (dispersive_region(S, h.region_from) || dispersive_region(S, h.region_to))) {
-                            T = T * mk4(4.0f, 0.0f, 0.0f, 0.0f);
-                            collapsed = true;
-                        }
-                        // light_sample_direct (skipped for delta-only materials)
-                        if (NEE && CAT != CAT_DIELECTRIC && CAT != CAT_MIRROR) {   // delta lobes take no NEE
-                            float x0 = rng.next(), x1 = rng.next();
-                            LSample ls = lighting_sample<SPEC>(S, h.p, x0, x1, wl);
-                            if (ls.pdf > 0.0f) {
-                                f3 light_p = h.p + ls.wi * ls.distance;
-                                float side = dot(ls.wi, h.ng) >= 0.0f ? 1.0f : -1.0f;
-                                s_o = h.p + (h.ng * side) * h.eps;
-                                s_d = normalize(light_p - s_o);
-                                s_max = length(light_p - s_o) - SHADOW_BACKOFF;
-                                s_l = light_p; s_cross = crossings_left;
-                                sp4 f = surface_eval<MODEL, SPEC>(S, M, ls.wi, wo, h, wl);
-                                float cos_i = fabsf(dot(ls.wi, h.frame.n));
-                                s_w = 1.0f;
-                                if (MIS && !(ls.flags & 1)) s_w = power_heuristic(ls.pdf, surface_pdf<MODEL, SPEC>(S, M, ls.wi, wo, h, wl));
-                                s_a = ((T * ls.radiance) * f) * cos_i;
-                                s_pdf = ls.pdf;
-                                shadow = true;
-                                if (ADJ) {
-#pragma unroll
-                                    for (int kk = 0; kk < ADJ_N; kk++) sD[kk] = A.D[kk];
-                                    if (CAT == CAT_LAMBERT) {
-                                        const int ak = adj_albedo_index(P, S.objects[h.owner].material);
-                                        if (ak >= 0) adj_add_albedo(sD, ak, 1.0f / M.albedo[0]);
-                                    }
-                                }
-                            }
-                        }
-                        // kernel_sample_continuation
-                        float uc = rng.next(), ux = rng.next(), uy = rng.next();
-                        ISample bs = surface_sample<MODEL, SPEC>(S, M, wo, h, uc, ux, uy, wl);
-                        if (!is_black<SPEC>(bs.weight)) {
-                            T = T * bs.weight;
-                            if (ADJ && CAT == CAT_LAMBERT) {
-                                const int ak = adj_albedo_index(P, S.objects[h.owner].material);
-                                if (ak >= 0) { adj_add_albedo(A.D, ak, 1.0f / M.albedo[0]); A.dirty = true; }
-                            }
-                            prev_delta = (bs.flags & LOBE_DELTA) != 0;
-                            prev_pdf = bs.pdf;
-                            prev_qp = h.p;
-                            if (MEDIA && (bs.flags & LOBE_TRANSMISSION)) medium = h.region_to;
-                            if (S.has_transmission && (bs.flags & LOBE_TRANSMISSION)) {
-                                float r = ior_of<SPEC>(S, h.region_to, wl) / ior_of<SPEC>(S, h.region_from, wl);
-                                eta_scale *= r * r;
-                            }
-                            cont = true;
-                            // roulette (pt.ts:204-222)
-                            if (rr_start >= 0 && bounce >= rr_start) {
-                                float m = S.has_transmission ? spectrum_max<SPEC>(T) * eta_scale : spectrum_max<SPEC>(T);
-                                float p = gmin(rr_max, m);
-                                if (p <= 0.0f) cont = false;
-                                else if (rng.next() > p) cont = false;
-                                else T = T / p;
-                            }
-                            if (cont) {
-                                float side = dot(bs.wi, h.ng) >= 0.0f ? 1.0f : -1.0f;
-                                P.ray_o[i] = f4(h.p + (h.ng * side) * h.eps, 0.0f);
-                                P.ray_d[i] = f4(bs.wi, 0.0f);
-                                store_thr<SPEC>(P, i, T, eta_scale);
-                                P.prev[i] = f4(prev_qp, prev_pdf);
-                                P.flags[i] = mk_flags(prev_delta, nulls, bounce + 1, collapsed);
-                                P.rng[i] = make_uint2(rng.base, rng.dim);
-                                if (MEDIA) P.medium[i] = medium;
-                            }
-                        }
-                    }
-                }
-            }
-            store_rad<SPEC>(P, i, L);
-            if (ADJ) adj_store(P, i, A);
-        }
-        int slot = block_append<2>(ctr, cont ? 0 : -1, s_cnt, s_base);
-        if (cont) qn[slot] = i;
-        int sslot = block_append<2>(ctr, shadow ? 1 : -1, s_cnt, s_base);
-        if (shadow) {
-            P.s_path[sslot] = i;
-            P.s_o[sslot] = f4(s_o, MEDIA ? __int_as_float(s_cross) : s_max);
-            P.s_d[sslot] = f4(s_d, s_pdf);
-            P.s_c[sslot] = SPEC ? st4(s_a) : make_float4(s_a.x, s_a.y, s_a.z, s_w);
-            if (MEDIA || SPEC) P.s_l[sslot] = f4(s_l, s_w);
-            if (ADJ) {
-#pragma unroll
-                for (int kk = 0; kk < ADJ_N; kk++) P.adj_sD[(size_t)kk * P.cap + sslot] = sD[kk];
-            }
-        }
-    }
-}
-
-// Adjoint reductions. Deterministic: per pixel, the wave's paths are added in sample order
-// into a per-pixel partial; after the last wave one block per parameter sums the partials
-// in a fixed order. Atomic: block partials (fixed-order LDS tree) are added to the gradient
-// with fp32 atomics.
-__global__ void __launch_bounds__(SBLOCK) k_adj_pixel(const Paths P, float* pix_grad, int frames) {
-    const int pix = blockIdx.x * blockDim.x + threadIdx.x;
-    if (pix >= P.npix) return;
-    for (int k = 0; k < ADJ_N; k++) {
-        float acc = pix_grad[(size_t)k * P.npix + pix];
-        for (int f = 0; f < frames; f++) acc += P.adj_G[(size_t)k * P.cap + (size_t)f * P.npix + pix];
-        pix_grad[(size_t)k * P.npix + pix] = acc;
-    }
-}
-__global__ void __launch_bounds__(SBLOCK) k_adj_final(const float* pix_grad, int npix, float* grad) {
-    __shared__ float red[SBLOCK];
-    const int k = blockIdx.x;
-    float acc = 0.0f;
-    for (int p = threadIdx.x; p < npix; p += SBLOCK) acc += pix_grad[(size_t)k * npix + p];
-    red[threadIdx.x] = acc;
-    __syncthreads();
-    for (int w = SBLOCK / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) grad[k] = red[0];
-}
-__global__ void __launch_bounds__(SBLOCK) k_adj_atomic(const Paths P, int n_paths, float* grad) {
-    __shared__ float red[SBLOCK];
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    for (int k = 0; k < ADJ_N; k++) {
-        red[threadIdx.x] = i < n_paths ? P.adj_G[(size_t)k * P.cap + i] : 0.0f;
-        __syncthreads();
-        for (int w = SBLOCK / 2; w > 0; w >>= 1) {
-            if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) atomicAdd(&grad[k], red[0]);
-        __syncthreads();
-    }
-}
-
-// film: average_update (components/accumulator/average/average.glsl:6-10) of each frame's
-// sample in sample order; spectral samples go through color_to_pixel
-// (color/spectral/spectral.glsl:27-33) with the cie1931 sensor first.
-template <bool SPEC>
-__global__ void __launch_bounds__(SBLOCK) k_accumulate(const DevScene S, const Paths P, float4* film, int npix, int frames,
-                                                       uint32_t sample0, int wl_uniform, float4* var, float4* aov0,
-                                                       float4* aov1) {
-    int pix = blockIdx.x * blockDim.x + threadIdx.x;
-    if (pix >= npix) return;
-    // the guide pictures: average_update of their samples, in sample order
-    float4* aovf[2] = {aov0, aov1};
-    for (int a = 0; a < 2; a++) {
-        if (!aovf[a] || !P.aov[a]) continue;
-        float4 g = aovf[a][pix];
-        for (int f = 0; f < frames; f++) {
-            const uint32_t n = sample0 + (uint32_t)f;
-            const float4 x = P.aov[a][(size_t)f * npix + pix];
-            if (n == 0) g = make_float4(x.x, x.y, x.z, 1.0f);
-            else {
-                float w = 1.0f / ((float)n + 1.0f);
-                g = make_float4(g.x * (1.0f - w) + x.x * w, g.y * (1.0f - w) + x.y * w, g.z * (1.0f - w) + x.z * w, 1.0f);
-            }
-        }
-        aovf[a][pix] = g;
-    }
-    float4 acc = film[pix];
-    float4 v = var ? var[pix] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    for (int f = 0; f < frames; f++) {
-        uint32_t n = sample0 + (uint32_t)f;
-        const size_t i = (size_t)f * npix + pix;
-        float4 x = P.rad[i];
-        if (SPEC) {
-            float4 lam = P.wl[i];
-            f3 px = mk3(0.0f, 0.0f, 0.0f);
-            for (int k = 0; k < 4; k++) {
-                float l = lane(lam, k);
-                px = px + sensor_response(S, l) * (lane(x, k) / wavelength_pdf(wl_uniform != 0, l));
-            }
-            px = px * 0.25f;
-            x = make_float4(px.x, px.y, px.z, 0.0f);
-        }
-        const float4 prev = acc;
-        if (n == 0) acc = make_float4(x.x, x.y, x.z, 1.0f);
-        else {
-            float w = 1.0f / ((float)n + 1.0f);
-            acc = make_float4(acc.x * (1.0f - w) + x.x * w, acc.y * (1.0f - w) + x.y * w, acc.z * (1.0f - w) + x.z * w, 1.0f);
-        }
-        if (var) {   // variance_update (components/accumulator/variance/variance.glsl:10-16)
-            if (n == 0) v = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
-            else {
-                float w = 1.0f / ((float)n + 1.0f);
-                float dx = (x.x - prev.x) * (x.x - acc.x), dy = (x.y - prev.y) * (x.y - acc.y), dz = (x.z - prev.z) * (x.z - acc.z);
-                v = make_float4(v.x * (1.0f - w) + dx * w, v.y * (1.0f - w) + dy * w, v.z * (1.0f - w) + dz * w, 1.0f);
-            }
-        }
-    }
-    film[pix] = acc;
-    if (var) var[pix] = v;
-}
-
-template <int STACK, bool COUNT>
-__global__ void __launch_bounds__(TBLOCK) k_trace_closest(DevScene S, const float* rays, int n, float* t_out, int* prim_out,
-                                                          unsigned long long* trav) {
-    __shared__ uint2 stk[STACK * TBLOCK];
-    TraceCounters cnt = {0, 0};
-    for (int base = blockIdx.x * TBLOCK; base < n; base += gridDim.x * TBLOCK) {
-        int i = base + threadIdx.x;
-        if (i >= n) break;
-        f3 o = mk3(rays[i], rays[n + i], rays[2 * (size_t)n + i]);
-        f3 d = mk3(rays[3 * (size_t)n + i], rays[4 * (size_t)n + i], rays[5 * (size_t)n + i]);
-        float t0 = rays[6 * (size_t)n + i], tmax = t0; int prim = -1;
-        trace_wide<false, STACK, TBLOCK, COUNT>(S, o, d, tmax, prim, stk, cnt);
-        trace_analytic<false>(S, o, d, tmax, prim);
-        trace_instances<false, TBLOCK, COUNT>(S, o, d, tmax, prim, stk, cnt);
-        t_out[i] = tmax;
-        prim_out[i] = prim;
-    }
-    if (COUNT) {
-        atomicAdd(&trav[0], (unsigned long long)cnt.nodes);
-        atomicAdd(&trav[1], (unsigned long long)cnt.tris);
-    }
-}
-
-template <int STACK>
-__global__ void __launch_bounds__(TBLOCK) k_trace_any(DevScene S, const float* rays, int n, uint8_t* hit_out) {
-    __shared__ uint2 stk[STACK * TBLOCK];
-    TraceCounters cnt = {0, 0};
-    for (int base = blockIdx.x * TBLOCK; base < n; base += gridDim.x * TBLOCK) {
-        int i = base + threadIdx.x;
-        if (i >= n) break;
-        f3 o = mk3(rays[i], rays[n + i], rays[2 * (size_t)n + i]);
-        f3 d = mk3(rays[3 * (size_t)n + i], rays[4 * (size_t)n + i], rays[5 * (size_t)n + i]);
-        float tmax = rays[6 * (size_t)n + i]; int prim = -1;
-        bool h = trace_analytic<true>(S, o, d, tmax, prim);
-        if (!h) h = trace_wide<true, STACK, TBLOCK, false>(S, o, d, tmax, prim, stk, cnt);
-        if (!h) h = trace_instances<true, TBLOCK, false>(S, o, d, tmax, prim, stk, cnt);
-        hit_out[i] = h ? 1 : 0;
-    }
-}
-
-}  // namespace
-
-// ============================================================================ host side
-
-namespace {
+namespace ptmi {
 thread_local std::string g_err;
 pt_status fail(pt_status s, const std::string& msg) { g_err = msg; return s; }
+}  // namespace ptmi
 
-#define HIP_TRY(expr)                                                                       \
-    do {                                                                                    \
-        hipError_t e_ = (expr);                                                             \
-        if (e_ != hipSuccess)                                                               \
-            return fail(e_ == hipErrorOutOfMemory ? PT_E_OOM : PT_E_DEVICE,                  \
-                        std::string(#expr) + ": " + hipGetErrorString(e_));                 \
-    } while (0)
+namespace {
 
 // ---------------------------------------------------------------- environment bake
 // The procedural sky's importance table on the device. The reference renders the environment
@@ -1233,125 +157,7 @@ __global__ void k_env_marginal(const double* __restrict__ rows, int H, float* __
     for (int j = 0; j < H; j++) { acc += rows[j]; marg[j] = (float)(total > 0 ? acc / total : 0); }
     *total_out = total;
 }
-
-struct DevBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    ~DevBuf() { if (p) (void)hipFree(p); }
-};
-
-int grid_for(long long n, int block) {
-    long long g = (n + block - 1) / block;
-    if (g < 1) g = 1;
-    return (int)std::min<long long>(g, MAX_GRID);
-}
 }  // namespace
-
-struct TimingSlot { std::vector<std::pair<hipEvent_t, hipEvent_t>> ev; float ms = 0; uint32_t launches = 0; uint64_t items = 0; };
-
-// One uploaded scene: its device buffers and the DevScene view of them.
-struct SceneStore {
-    std::vector<std::unique_ptr<DevBuf>> bufs;
-    DevScene dev{};
-    int stack = 16;
-    uint32_t wide_depth = 0;
-    bool has_ggx = false;
-    bool has_mirror = false;
-    bool has_rough = false;
-    std::vector<pt_material> host_materials;
-    hipStream_t stream = nullptr;
-    template <class T> pt_status upload(const T* host, size_t n, const T** out) {
-        auto b = std::make_unique<DevBuf>();
-        b->bytes = std::max<size_t>(n, 1) * sizeof(T);
-        HIP_TRY(hipMalloc(&b->p, b->bytes));
-        if (n) HIP_TRY(hipMemcpyAsync(b->p, host, n * sizeof(T), hipMemcpyHostToDevice, stream));
-        *out = (const T*)b->p;
-        bufs.push_back(std::move(b));
-        return PT_OK;
-    }
-    template <class T> pt_status alloc(size_t n, T** out) {
-        auto b = std::make_unique<DevBuf>();
-        b->bytes = std::max<size_t>(n, 1) * sizeof(T);
-        HIP_TRY(hipMalloc(&b->p, b->bytes));
-        *out = (T*)b->p;
-        bufs.push_back(std::move(b));
-        return PT_OK;
-    }
-};
-
-struct pt_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    std::vector<std::unique_ptr<DevBuf>> scene_bufs;
-    DevScene dev{};
-    bool has_scene = false;
-    int stack = 16;
-    uint32_t wide_depth = 0;
-    int persist_blocks = 2048;   // persistent traversal grid (set from the device's CU count)
-    int cu_count = 256;
-    int shade_blocks = 2048;
-    bool has_ggx = false;
-    bool has_mirror = false;
-    bool has_rough = false;
-    std::vector<pt_material> host_materials;   // the uploaded records (adjoint checks, updates)
-    bool timing = false;
-    std::map<std::string, TimingSlot> timings;
-    std::vector<std::string> timing_order;
-    DevBuf trav;   // 2 x u64 counters for instrumented traces
-    DevBuf rays_tmp, t_tmp, p_tmp;
-
-    template <class T> pt_status upload(const T* host, size_t n, const T** out) {
-        auto b = std::make_unique<DevBuf>();
-        b->bytes = std::max<size_t>(n, 1) * sizeof(T);
-        HIP_TRY(hipMalloc(&b->p, b->bytes));
-        if (n) HIP_TRY(hipMemcpyAsync(b->p, host, n * sizeof(T), hipMemcpyHostToDevice, stream));
-        *out = (const T*)b->p;
-        scene_bufs.push_back(std::move(b));
-        return PT_OK;
-    }
-    void ev_begin(const char* name, hipEvent_t* b) {
-        if (!timing) return;
-        (void)hipEventCreate(b);
-        (void)hipEventRecord(*b, stream);
-        (void)name;
-    }
-    void ev_end(const char* name, hipEvent_t b, uint64_t items) {
-        if (!timing) return;
-        hipEvent_t e;
-        (void)hipEventCreate(&e);
-        (void)hipEventRecord(e, stream);
-        auto it = timings.find(name);
-        if (it == timings.end()) { timing_order.push_back(name); it = timings.emplace(name, TimingSlot()).first; }
-        it->second.ev.emplace_back(b, e);
-        it->second.launches++;
-        it->second.items += items;
-    }
-};
-
-struct pt_renderer {
-    pt_ctx* ctx = nullptr;
-    pt_program prog{};
-    DevBuf film, var_film, aov_film[2];
-    uint32_t sample_count = 0;
-    long long cap = 0;    // path slots allocated
-    DevBuf b_ray_o, b_ray_d, b_thr, b_rad, b_rng, b_prev, b_flags, b_medium, b_hit, b_q0, b_q1, b_counts,
-        b_spath, b_so, b_sd, b_sc, b_sl, b_wl, b_eta, b_hist, b_trav, b_cat[N_CAT], b_aov[2];
-    // adjoint buffers (allocated by the first pt_backward)
-    DevBuf b_adjD, b_adjG, b_adjsD, b_delta, b_pixgrad, b_grad;
-    DevBuf b_pscene;      // scene of each path slot (batched wavefront)
-    long long adj_cap = 0;
-    int adj_mode = PT_REDUCE_DETERMINISTIC;
-    Paths paths{};
-    bool count_traversal = false;
-};
-
-#define TIMED(ctx, name, items, launch)             \
-    do {                                            \
-        hipEvent_t ev_b_ = nullptr;                 \
-        (ctx)->ev_begin(name, &ev_b_);              \
-        launch;                                     \
-        (ctx)->ev_end(name, ev_b_, items);          \
-    } while (0)
 
 extern "C" {
 
@@ -1971,98 +777,6 @@ static pt_status ensure_paths(pt_renderer* r, long long n) {
     return PT_OK;
 }
 
-// Dynamic LDS for a k_shade launch: the region probe's stack for the surface categories
-// without a static one, in scenes whose emissive solids make them probe at back-face hits.
-template <int CAT, bool MEDIA, int STACK>
-static size_t shade_dynamic_lds(bool emissive_solid) {
-    constexpr bool STATIC_STK = cat_probes(CAT, MEDIA) && CAT != CAT_MISS && CAT != CAT_MEDIUM;
-    constexpr bool SURFACE = CAT != CAT_MISS && CAT != CAT_MEDIUM;
-    return (!STATIC_STK && SURFACE && emissive_solid) ? (size_t)STACK * SBLOCK * sizeof(uint2) : 0;
-}
-
-template <int EST, int STACK, bool MEDIA, bool SPEC, bool ADJ>
-static void launch_shade(pt_renderer* r, int cur) {
-    pt_ctx* ctx = r->ctx;
-    const pt_program& pg = r->prog;
-    const DevScene& S = ctx->dev;
-    hipStream_t st = ctx->stream;
-    const int g = ctx->shade_blocks;
-    const int mn = pg.max_null_crossings;
-#define PT_SHADE(name, CATV) TIMED(ctx, name, 0, hipLaunchKernelGGL((k_shade<EST, CATV, STACK, MEDIA, SPEC, ADJ>), dim3(g), dim3(SBLOCK), (shade_dynamic_lds<CATV, MEDIA, STACK>(S.emissive_solid != 0)), st, S, r->paths, cur, pg.max_bounces, pg.rr_start, pg.rr_max_survival, mn, pg.medium_light_sampling))
-    PT_SHADE("shade_miss", CAT_MISS);
-    PT_SHADE("shade_lambert", CAT_LAMBERT);
-    if (ctx->has_ggx) PT_SHADE("shade_ggx", CAT_GGX);
-    if (ctx->has_mirror) PT_SHADE("shade_mirror", CAT_MIRROR);
-    if (ctx->has_rough) PT_SHADE("shade_rough_dielectric", CAT_ROUGH);
-    if (S.has_transmission) PT_SHADE("shade_dielectric", CAT_DIELECTRIC);
-    if (MEDIA) PT_SHADE("shade_medium", CAT_MEDIUM);
-    if (S.has_null) PT_SHADE("shade_null", CAT_NULL);
-#undef PT_SHADE
-}
-
-template <int STACK, bool MEDIA, bool SPEC, bool ADJ>
-static void launch_wave(pt_renderer* r, const Camera& cam, int frames, uint32_t sample0) {
-    static_assert(!(SPEC && ADJ), "the adjoint runs RGB programs");
-    pt_ctx* ctx = r->ctx;
-    const DevScene& S = ctx->dev;
-    Paths& P = r->paths;
-    const pt_program& pg = r->prog;
-    const int W = pg.width, H = pg.height;
-    const long long n = (long long)W * H * frames;
-    hipStream_t st = ctx->stream;
-    if (MEDIA) hipLaunchKernelGGL((k_camera_medium<STACK>), dim3(1), dim3(64), 0, st, S, P, cam);
-    TIMED(ctx, "raygen", n,  hipLaunchKernelGGL((k_raygen<SPEC, ADJ>), dim3((unsigned)((n + SBLOCK - 1) / SBLOCK)), dim3(SBLOCK), 0, st, P, cam, W, H, (int)n, sample0, MEDIA ? 1 : 0, pg.wavelengths));
-    if (MEDIA && (cam.ortho || cam.thin)) hipLaunchKernelGGL((k_ray_medium<STACK>), dim3(ctx->shade_blocks), dim3(SBLOCK), 0, st, S, P, (int)n);
-    int cur = 0;
-    const int tgrid = ctx->persist_blocks;
-    // a null crossing consumes an iteration but no bounce (pt.ts null-interface branch)
-    const int iterations = pg.max_bounces + 1 + (S.has_null ? std::max(pg.max_null_crossings, 0) : 0);
-    for (int it = 0; it < iterations; it++) {
-        hipLaunchKernelGGL(k_bookkeep, dim3(1), dim3(64), 0, st, P, cur, cur ^ 1, it == 0 ? 1 : 0);
-        const bool inst = S.n_inst_nodes > 0;
-        if (r->count_traversal) {
-            if (inst) TIMED(ctx, "extend_count", 0, hipLaunchKernelGGL((k_extend<STACK, true, false, TBLOCK, true>), dim3(tgrid), dim3(TBLOCK), 0, st, S, P, cur));
-            else TIMED(ctx, "extend_count", 0, hipLaunchKernelGGL((k_extend<STACK, true>), dim3(tgrid), dim3(TBLOCK), 0, st, S, P, cur));
-        } else {
-            if (inst) TIMED(ctx, "extend", 0, hipLaunchKernelGGL((k_extend<STACK, false, false, TBLOCK, true>), dim3(tgrid), dim3(TBLOCK), 0, st, S, P, cur));
-            else TIMED(ctx, "extend", 0, hipLaunchKernelGGL((k_extend<STACK, false>), dim3(tgrid), dim3(TBLOCK), 0, st, S, P, cur));
-        }
-        const int eq = MEDIA && !ADJ && pg.medium_light_sampling == PT_MLS_EQUIANGULAR ? 1 : 0;
-        TIMED(ctx, "bin", 0, hipLaunchKernelGGL((k_bin<MEDIA, SPEC, ADJ>), dim3(ctx->shade_blocks), dim3(SBLOCK), 0, st, S, P, cur, eq, pg.max_bounces, pg.max_null_crossings));
-        if (eq) {
-            if (inst) TIMED(ctx, "shadow_media", 0, hipLaunchKernelGGL((k_shadow_media<STACK, SPEC, ADJ, true>), dim3(tgrid), dim3(TBLOCK), 0, st, S, P));
-            else TIMED(ctx, "shadow_media", 0, hipLaunchKernelGGL((k_shadow_media<STACK, SPEC, ADJ>), dim3(tgrid), dim3(TBLOCK), 0, st, S, P));
-            hipLaunchKernelGGL(k_shadow_reset, dim3(1), dim3(64), 0, st, P);
-        }
-        switch (pg.estimator) {
-            case PT_EST_PT: launch_shade<PT_EST_PT, STACK, MEDIA, SPEC, ADJ>(r, cur); break;
-            case PT_EST_NEE: launch_shade<PT_EST_NEE, STACK, MEDIA, SPEC, ADJ>(r, cur); break;
-            default: launch_shade<PT_EST_MIS, STACK, MEDIA, SPEC, ADJ>(r, cur);
-        }
-        if (pg.estimator != PT_EST_PT) {
-            if (MEDIA && inst) TIMED(ctx, "shadow_media", 0, hipLaunchKernelGGL((k_shadow_media<STACK, SPEC, ADJ, true>), dim3(tgrid), dim3(TBLOCK), 0, st, S, P));
-            else if (MEDIA) TIMED(ctx, "shadow_media", 0, hipLaunchKernelGGL((k_shadow_media<STACK, SPEC, ADJ>), dim3(tgrid), dim3(TBLOCK), 0, st, S, P));
-            else if (inst) TIMED(ctx, "shadow", 0, hipLaunchKernelGGL((k_shadow<STACK, SPEC, ADJ, false, TBLOCK, true>), dim3(tgrid), dim3(TBLOCK), 0, st, S, P));
-            else TIMED(ctx, "shadow", 0, hipLaunchKernelGGL((k_shadow<STACK, SPEC, ADJ>), dim3(tgrid), dim3(TBLOCK), 0, st, S, P));
-        }
-        cur ^= 1;
-    }
-    hipLaunchKernelGGL(k_bookkeep, dim3(1), dim3(64), 0, st, P, -1, -1, 0);
-    const int npix = W * H;
-    if (ADJ) {
-        if (r->adj_mode == PT_REDUCE_DETERMINISTIC)
-            TIMED(ctx, "adj_reduce", npix, hipLaunchKernelGGL(k_adj_pixel, dim3((npix + SBLOCK - 1) / SBLOCK), dim3(SBLOCK), 0, st, P, (float*)r->b_pixgrad.p, frames));
-        else
-            TIMED(ctx, "adj_reduce", n, hipLaunchKernelGGL(k_adj_atomic, dim3((unsigned)((n + SBLOCK - 1) / SBLOCK)), dim3(SBLOCK), 0, st, P, (int)n, (float*)r->b_grad.p));
-        return;
-    }
-    TIMED(ctx, "accumulate", npix, hipLaunchKernelGGL((k_accumulate<SPEC>), dim3((npix + SBLOCK - 1) / SBLOCK), dim3(SBLOCK), 0, st, S, P, (float4*)r->film.p, npix, frames, sample0, pg.wavelengths, (float4*)r->var_film.p, (float4*)r->aov_film[0].p, (float4*)r->aov_film[1].p));
-}
-
-extern "C" {
-
-}  // extern "C"
-
 // The camera models pt_params can select, with the thin lens's parameter domain
 // (camera/thinlens/thinlens.ts:15-16: aperture nonnegative, focusDistance positive).
 static pt_status check_camera(const pt_params* prm) {
@@ -2122,20 +836,20 @@ static pt_status run_waves(pt_renderer* r, const pt_params* prm, uint32_t nframe
         const uint32_t s0 = sample0 + done;
         const int v = (ctx->stack == 16 ? 0 : 4) | (ctx->dev.has_media ? 2 : 0) | (spectral ? 1 : 0);
         if (adj) {
+            // the media, spectral and adjoint programs run with the 32-entry stack, which holds
+            // every accepted BVH (depth <= 31); only the plain RGB path keeps two depths
             switch (v) {
-                // the media, spectral and adjoint programs run with the 32-entry stack, which
-                // holds every accepted BVH (depth <= 31); only the plain RGB path keeps two depths
-                case 0: case 4: launch_wave<32, false, false, true>(r, cam, f, s0); break;
-                case 2: case 6: launch_wave<32, true, false, true>(r, cam, f, s0); break;
+                case 0: case 4: wave_adjoint(r, cam, f, s0); break;
+                case 2: case 6: wave_adjoint_media(r, cam, f, s0); break;
                 default: return fail(PT_E_UNSUPPORTED, "the adjoint runs RGB programs only");
             }
         } else {
             switch (v) {
-                case 0: launch_wave<16, false, false, false>(r, cam, f, s0); break;
-                case 4: launch_wave<32, false, false, false>(r, cam, f, s0); break;
-                case 1: case 5: launch_wave<32, false, true, false>(r, cam, f, s0); break;
-                case 2: case 6: launch_wave<32, true, false, false>(r, cam, f, s0); break;
-                default: launch_wave<32, true, true, false>(r, cam, f, s0);
+                case 0: wave_rgb(r, cam, f, s0, 16); break;
+                case 4: wave_rgb(r, cam, f, s0, 32); break;
+                case 1: case 5: wave_spectral(r, cam, f, s0); break;
+                case 2: case 6: wave_media(r, cam, f, s0); break;
+                default: wave_media_spectral(r, cam, f, s0);
             }
         }
         HIP_TRY(hipGetLastError());
@@ -2251,38 +965,6 @@ pt_status pt_backward(pt_renderer* r, const pt_params* prm, uint32_t nframes, co
 
 }  // extern "C"
 
-// ---------------------------------------------------------------- batched wavefront
-struct pt_batch {
-    pt_ctx* ctx = nullptr;
-    std::vector<std::unique_ptr<SceneStore>> stores;
-    std::vector<pt_program> progs;
-    std::vector<Camera> cams;
-    std::vector<long long> film_off;   // first pixel of each scene's film
-    long long film_pixels = 0;
-    int max_npix = 0, min_stack = 16;
-    bool any_ggx = false, any_dielectric = false, any_mirror = false, any_rough = false, any_emissive_solid = false;
-    DevBuf d_scenes, d_cams, d_entries, films;
-    pt_renderer* r = nullptr;          // path slots
-    pt_batch_config cfg{};
-    std::vector<BatchEntry> entries;
-    std::vector<std::pair<int, int>> waves;   // (first entry, entry count)
-    std::vector<long long> wave_paths;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    std::vector<uint32_t> graph_spp;
-    pt_batch_config graph_cfg{};
-    ~pt_batch() {
-        if (exec) (void)hipGraphExecDestroy(exec);
-        if (graph) (void)hipGraphDestroy(graph);
-        delete r;
-    }
-    void drop_graph() {
-        if (exec) (void)hipGraphExecDestroy(exec);
-        if (graph) (void)hipGraphDestroy(graph);
-        exec = nullptr; graph = nullptr; graph_spp.clear();
-    }
-};
-
 static bool same_cfg(const pt_batch_config& a, const pt_batch_config& b) {
     return a.wave_paths == b.wave_paths && a.trace_block == b.trace_block && a.trace_blocks_per_cu == b.trace_blocks_per_cu &&
            a.shade_blocks_per_cu == b.shade_blocks_per_cu && a.stack == b.stack && a.use_graph == b.use_graph;
@@ -2314,74 +996,6 @@ static pt_status batch_schedule(pt_batch* b, const uint32_t* spp) {
     }
     if (used > 0) { b->waves.emplace_back(first, (int)b->entries.size() - first); b->wave_paths.push_back(used); }
     return PT_OK;
-}
-
-template <int EST, int STACK>
-static void batch_shade(pt_batch* b, int cur) {
-    pt_ctx* ctx = b->ctx;
-    const pt_program& pg = b->progs[0];
-    const DevScene& S0 = b->stores[0]->dev;
-    const int g = ctx->cu_count * b->cfg.shade_blocks_per_cu;
-    Paths& P = b->r->paths;
-    hipStream_t st = ctx->stream;
-#define PT_BSHADE(CATV) hipLaunchKernelGGL((k_shade<EST, CATV, STACK, false, false, false, true>), dim3(g), dim3(SBLOCK), (shade_dynamic_lds<CATV, false, STACK>(b->any_emissive_solid)), st, S0, P, cur, pg.max_bounces, pg.rr_start, pg.rr_max_survival, pg.max_null_crossings, 0)
-    PT_BSHADE(CAT_MISS);
-    PT_BSHADE(CAT_LAMBERT);
-    if (b->any_ggx) PT_BSHADE(CAT_GGX);
-    if (b->any_mirror) PT_BSHADE(CAT_MIRROR);
-    if (b->any_rough) PT_BSHADE(CAT_ROUGH);
-    if (b->any_dielectric) PT_BSHADE(CAT_DIELECTRIC);
-#undef PT_BSHADE
-}
-
-template <int STACK, int TB>
-static void batch_waves(pt_batch* b) {
-    pt_ctx* ctx = b->ctx;
-    const pt_program& pg = b->progs[0];
-    const DevScene& S0 = b->stores[0]->dev;
-    Paths& P = b->r->paths;
-    hipStream_t st = ctx->stream;
-    const int tgrid = ctx->cu_count * b->cfg.trace_blocks_per_cu;
-    const int sgrid = ctx->cu_count * b->cfg.shade_blocks_per_cu;
-    const BatchEntry* dE = (const BatchEntry*)b->d_entries.p;
-    for (size_t w = 0; w < b->waves.size(); w++) {
-        const int e0 = b->waves[w].first, ne = b->waves[w].second;
-        const long long n = b->wave_paths[w];
-        hipLaunchKernelGGL(k_raygen_batch, dim3((unsigned)((n + SBLOCK - 1) / SBLOCK)), dim3(SBLOCK), 0, st, P, dE + e0, ne,
-                           (const Camera*)b->d_cams.p, (int)n);
-        int cur = 0;
-        for (int it = 0; it <= pg.max_bounces; it++) {
-            hipLaunchKernelGGL(k_bookkeep, dim3(1), dim3(64), 0, st, P, cur, cur ^ 1, it == 0 ? 1 : 0);
-            hipLaunchKernelGGL((k_extend<STACK, false, true, TB>), dim3(tgrid), dim3(TB), 0, st, S0, P, cur);
-            hipLaunchKernelGGL((k_bin<false, false, false, true>), dim3(sgrid), dim3(SBLOCK), 0, st, S0, P, cur, 0, pg.max_bounces, pg.max_null_crossings);
-            switch (pg.estimator) {
-                case PT_EST_PT: batch_shade<PT_EST_PT, STACK>(b, cur); break;
-                case PT_EST_NEE: batch_shade<PT_EST_NEE, STACK>(b, cur); break;
-                default: batch_shade<PT_EST_MIS, STACK>(b, cur);
-            }
-            if (pg.estimator != PT_EST_PT)
-                hipLaunchKernelGGL((k_shadow<STACK, false, false, true, TB>), dim3(tgrid), dim3(TB), 0, st, S0, P);
-            cur ^= 1;
-        }
-        hipLaunchKernelGGL(k_bookkeep, dim3(1), dim3(64), 0, st, P, -1, -1, 0);
-        int maxpix = 0;
-        for (int e = e0; e < e0 + ne; e++) maxpix = std::max(maxpix, b->entries[e].npix);
-        hipLaunchKernelGGL(k_accumulate_batch, dim3((maxpix + SBLOCK - 1) / SBLOCK, ne), dim3(SBLOCK), 0, st, P, dE + e0,
-                           (float4*)b->films.p);
-    }
-}
-
-static void batch_launch(pt_batch* b) {
-    hipLaunchKernelGGL(k_fill_f4, dim3(1024), dim3(SBLOCK), 0, b->ctx->stream, (float4*)b->films.p, b->film_pixels);
-    const int v = (b->cfg.stack == 16 ? 0 : 3) + (b->cfg.trace_block == 64 ? 0 : (b->cfg.trace_block == 128 ? 1 : 2));
-    switch (v) {
-        case 0: batch_waves<16, 64>(b); break;
-        case 1: batch_waves<16, 128>(b); break;
-        case 2: batch_waves<16, 256>(b); break;
-        case 3: batch_waves<32, 64>(b); break;
-        case 4: batch_waves<32, 128>(b); break;
-        default: batch_waves<32, 256>(b);
-    }
 }
 
 static pt_status batch_check_config(const pt_batch* b, const pt_batch_config& c) {

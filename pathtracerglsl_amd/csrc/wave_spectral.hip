@@ -1,0 +1,10 @@
+// wave_spectral.hip — the frame wave of spectral programs without media.
+#include "wave.h"
+
+namespace ptmi {
+
+void wave_spectral(pt_renderer* r, const Camera& cam, int frames, uint32_t sample0) {
+    launch_wave<32, false, true, false>(r, cam, frames, sample0);
+}
+
+}  // namespace ptmi
