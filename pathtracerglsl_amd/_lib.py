@@ -8,7 +8,7 @@ from . import abi
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libptmi.so")
 
-# every symbol declared in include/ptmi.h
+# every symbol declared in include/ptmi.h (the boundary)
 DECLARED = ["pt_last_error", "pt_ctx_create", "pt_ctx_destroy", "pt_scene_upload", "pt_renderer_load",
             "pt_renderer_unload", "pt_render_frames", "pt_read_export", "pt_clear_accumulation",
             "pt_sample_count", "pt_trace_closest", "pt_trace_any", "pt_trace_closest_device",
@@ -18,6 +18,10 @@ DECLARED = ["pt_last_error", "pt_ctx_create", "pt_ctx_destroy", "pt_scene_upload
             "pt_batch_render", "pt_batch_read_export", "pt_batch_autotune",
             "pt_program_from_description", "pt_renderer_load_description", "pt_env_total_weight",
             "pt_obj_parse", "pt_mesh_data_info", "pt_mesh_data_copy", "pt_mesh_data_free", "pt_inst_parse"]
+# every symbol declared in include/ptmi_debug.h (test and bench instrumentation)
+DECLARED_DEBUG = ["pt_renderer_set_traversal_counting", "pt_last_traversal_counts", "pt_trace_closest_device_counted",
+                  "pt_scene_stats", "pt_abi_sizeof", "pt_debug_wide_bvh", "pt_debug_math", "pt_calib_copy",
+                  "pt_debug_env_bake", "pt_debug_env_table"]
 
 _lib = None
 

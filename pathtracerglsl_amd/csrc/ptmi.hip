@@ -341,6 +341,8 @@ static pt_status validate_scene(const pt_scene_desc* sd) {
         if (ob.kind == PT_OBJ_MESH && (ob.first_tri < 0 || ob.tri_count < 0 || ob.first_tri + ob.tri_count > sd->n_tris))
             return fail(PT_E_INVALID, "object " + std::to_string(o) + ": triangle range out of bounds");
     }
+    if (sd->env.type == PT_ENV_SKY && sd->env.chart != PT_CHART_EQUIRECT && sd->env.chart != PT_CHART_OCTAHEDRAL)
+        return fail(PT_E_INVALID, "pt_env.chart must be PT_CHART_EQUIRECT or PT_CHART_OCTAHEDRAL");
     return PT_OK;
 }
 
@@ -681,6 +683,24 @@ pt_status pt_scene_upload(pt_ctx* ctx, const pt_scene_desc* sd) {
     return PT_OK;
 }
 
+}  // extern "C"
+
+// The program-vs-scene rules, checked at renderer load when a scene is resident and again by
+// every render, forward and backward call (a later pt_scene_upload may have replaced the
+// scene): the spectral path has no procedural-sky radiance, and a program that names its
+// envSampler.chart must sample the chart the sky table was built on.
+static pt_status check_program_scene(const pt_program& prog, const DevScene& dev) {
+    if (prog.color == PT_COLOR_SPECTRAL && dev.env_type == PT_ENV_SKY)
+        return fail(PT_E_UNSUPPORTED, "spectral programs with the procedural sky are not provided by this build");
+    if (prog.env_chart && dev.env_type == PT_ENV_SKY && prog.env_chart - 1 != dev.env_chart)
+        return fail(PT_E_INVALID, std::string("the program's envSampler.chart is '") +
+                                      (prog.env_chart - 1 == PT_CHART_OCTAHEDRAL ? "octahedral" : "equirect") +
+                                      "' but the scene's sky table was built on the other chart (pt_env.chart)");
+    return PT_OK;
+}
+
+extern "C" {
+
 pt_status pt_renderer_load(pt_ctx* ctx, const pt_program* prog, pt_renderer** out) {
     if (!ctx || !prog || !out) return fail(PT_E_INVALID, "pt_renderer_load: null argument");
     if (prog->abi_version != PT_ABI_VERSION) return fail(PT_E_INVALID, "pt_renderer_load: ABI version mismatch");
@@ -693,17 +713,13 @@ pt_status pt_renderer_load(pt_ctx* ctx, const pt_program* prog, pt_renderer** ou
         return fail(PT_E_UNSUPPORTED, "unknown volume sampling");
     if (prog->medium_light_sampling == PT_MLS_EQUIANGULAR && prog->estimator != PT_EST_NEE)
         return fail(PT_E_UNSUPPORTED, "equiangular medium NEE runs under the nee estimator");
-    if (prog->color == PT_COLOR_SPECTRAL && ctx->has_scene && ctx->dev.env_type == PT_ENV_SKY)
-        return fail(PT_E_UNSUPPORTED, "spectral programs with the procedural sky are not provided by this build");
     if (prog->width <= 0 || prog->height <= 0 || prog->max_bounces < 0) return fail(PT_E_INVALID, "bad film size or bounce budget");
     if (prog->aovs & ~(PT_AOV_ALBEDO | PT_AOV_NORMAL)) return fail(PT_E_INVALID, "unknown aov bits");
     if (prog->aovs && prog->color != PT_COLOR_RGB)
         return fail(PT_E_UNSUPPORTED, "first-hit guides (aovs) are provided for RGB programs only");
     if (prog->env_chart < 0 || prog->env_chart > PT_CHART_OCTAHEDRAL + 1) return fail(PT_E_INVALID, "unknown env chart");
-    if (prog->env_chart && ctx->has_scene && ctx->dev.env_type == PT_ENV_SKY && prog->env_chart - 1 != ctx->dev.env_chart)
-        return fail(PT_E_INVALID, std::string("the program's envSampler.chart is '") +
-                                      (prog->env_chart - 1 == PT_CHART_OCTAHEDRAL ? "octahedral" : "equirect") +
-                                      "' but the scene's sky table was built on the other chart (pt_env.chart)");
+    if (ctx->has_scene)
+        if (pt_status sc = check_program_scene(*prog, ctx->dev); sc != PT_OK) return sc;
     HIP_TRY(hipSetDevice(ctx->device));
     auto* r = new pt_renderer();
     r->ctx = ctx;
@@ -825,6 +841,7 @@ static pt_status run_waves(pt_renderer* r, const pt_params* prm, uint32_t nframe
     const pt_program& pg = r->prog;
     const int F = std::max(1, std::min<int>(pg.frames_in_flight, (int)std::max<uint32_t>(nframes, 1)));
     const bool spectral = pg.color == PT_COLOR_SPECTRAL;
+    if (pt_status sc = check_program_scene(pg, ctx->dev); sc != PT_OK) return sc;
     if (spectral && !ctx->dev.spec_curves)
         return fail(PT_E_MISSING_INPUT, "missing input: spectral_curves (the spectral program needs the CIE/D65 curves in pt_scene_desc)");
     HIP_TRY(hipMemsetAsync(r->b_hist.p, 0, r->b_hist.bytes, ctx->stream));
@@ -1016,16 +1033,20 @@ pt_status pt_batch_create(pt_ctx* ctx, int32_t n, const pt_scene_desc* scenes, c
     if (n <= 0) return fail(PT_E_INVALID, "pt_batch_create: no scenes");
     for (int s = 0; s < n; s++)
         if (pt_status sc = check_camera(&params[s]); sc != PT_OK) return sc;
+    // every record's ABI version before any field that a newer ABI added is read
     for (int s = 0; s < n; s++)
+        if (programs[s].abi_version != PT_ABI_VERSION || scenes[s].abi_version != PT_ABI_VERSION)
+            return fail(PT_E_INVALID, "pt_batch_create: ABI version mismatch");
+    for (int s = 0; s < n; s++) {
         if (programs[s].aovs) return fail(PT_E_UNSUPPORTED, "pt_batch_create: first-hit guides (aovs) are not batched");
-    for (int s = 0; s < n; s++)
-        if (programs[s].env_chart && scenes[s].env.type == PT_ENV_SKY &&
-            programs[s].env_chart - 1 != (scenes[s].env.chart == PT_CHART_OCTAHEDRAL ? PT_CHART_OCTAHEDRAL : PT_CHART_EQUIRECT))
+        if (programs[s].env_chart < 0 || programs[s].env_chart > PT_CHART_OCTAHEDRAL + 1)
+            return fail(PT_E_INVALID, "pt_batch_create: unknown env chart");
+        if (programs[s].env_chart && scenes[s].env.type == PT_ENV_SKY && programs[s].env_chart - 1 != scenes[s].env.chart)
             return fail(PT_E_INVALID, "pt_batch_create: a program's envSampler.chart differs from its scene's sky table chart");
+    }
     const pt_program& p0 = programs[0];
     for (int s = 0; s < n; s++) {
         const pt_program& p = programs[s];
-        if (p.abi_version != PT_ABI_VERSION || scenes[s].abi_version != PT_ABI_VERSION) return fail(PT_E_INVALID, "pt_batch_create: ABI version mismatch");
         if (p.estimator != p0.estimator || p.max_bounces != p0.max_bounces || p.rr_start != p0.rr_start || p.rr_max_survival != p0.rr_max_survival)
             return fail(PT_E_UNSUPPORTED, "pt_batch_create: every program must share estimator, maxBounces and Russian roulette");
         if (p.color != PT_COLOR_RGB) return fail(PT_E_UNSUPPORTED, "pt_batch_create: batches run RGB programs");

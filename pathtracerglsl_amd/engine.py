@@ -37,9 +37,32 @@ class Engine:
         self._selected = compiled.id
         return compiled.id
 
+    def load_renderer_description(self, description, compiled):
+        """Engine.loadRenderer from the reference's plan record: the ProgramDescription (a dict or
+        its JSON text) is lowered by the library (pt_renderer_load_description) over compiled's
+        scene, film size and frames in flight; compiled also supplies the per-frame params."""
+        import json
+        if self._scene is not compiled.scene:
+            self.upload_scene(compiled.scene)
+        text = (description if isinstance(description, str) else json.dumps(description)).encode()
+        h = C.c_void_p()
+        _lib.check(self.lib.pt_renderer_load_description(self.ctx, text, len(text), compiled.width, compiled.height,
+                                                         compiled.program.frames_in_flight, C.byref(h)))
+        rid = f"{compiled.id}#description"
+        if rid in self._renderers:
+            self.lib.pt_renderer_unload(self._renderers[rid][0])
+        self._renderers[rid] = (h, compiled)
+        self._selected = rid
+        return rid
+
     def select_renderer(self, rid):
+        """Engine.selectRenderer: the renderer's own scene becomes resident again when another
+        load replaced it (every render call checks the program against the resident scene)."""
         if rid not in self._renderers:
             raise KeyError(f"unknown renderer '{rid}'")
+        _, compiled = self._renderers[rid]
+        if self._scene is not compiled.scene:
+            self.upload_scene(compiled.scene)
         self._selected = rid
 
     def _cur(self):

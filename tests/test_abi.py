@@ -1,18 +1,21 @@
-"""The C ABI library loads without a GPU, exports every function include/ptmi.h declares,
-and its record sizes match the Python mirrors (no compute calls here)."""
+"""The C ABI library loads without a GPU, exports every function include/ptmi.h and
+include/ptmi_debug.h declare and nothing else, and its record sizes match the Python mirrors (no
+compute calls here)."""
 import ctypes as C
 import os
 import re
 
 from pathtracerglsl_amd import _lib, abi
 
-HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "ptmi.h")
+INCLUDE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+HEADER = os.path.join(INCLUDE, "ptmi.h")
+DEBUG_HEADER = os.path.join(INCLUDE, "ptmi_debug.h")
 
 
-def declared_functions():
-    text = open(HEADER).read()
+def declared_functions(header=HEADER):
+    text = open(header).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"^\s*(?:const\s+)?[a-z_0-9]+\*?\s+\**(pt_[a-z_0-9]+)\s*\(", text, flags=re.M)))
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?(?:[a-z_0-9]+\s+)*[a-z_0-9]+\*?\s+\**(pt_[a-z_0-9]+)\s*\(", text, flags=re.M)))
 
 
 def test_header_parse_finds_entry_points():
@@ -23,8 +26,22 @@ def test_header_parse_finds_entry_points():
 
 def test_library_exports_every_declared_symbol():
     lib = C.CDLL(_lib.LIB_PATH)
-    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    missing = [n for n in declared_functions() + declared_functions(DEBUG_HEADER) if not hasattr(lib, n)]
     assert not missing, missing
+
+
+def test_debug_header_lists_the_instrumentation():
+    debug = declared_functions(DEBUG_HEADER)
+    assert sorted(_lib.DECLARED_DEBUG) == debug
+    assert not set(debug) & set(declared_functions())
+
+
+def test_library_exports_nothing_undeclared():
+    """Every exported pt_ symbol is declared in one of the two headers."""
+    import subprocess
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True, text=True, check=True).stdout
+    exported = {ln.split()[-1] for ln in out.splitlines() if ln.split()[-2:-1] == ["T"] and ln.split()[-1].startswith("pt_")}
+    assert exported == set(declared_functions()) | set(declared_functions(DEBUG_HEADER))
 
 
 def test_record_sizes_match_mirrors():

@@ -29,8 +29,13 @@ def _expected_estimator(d):
     return abi.EST_PT if lighting is None else {"nee": abi.EST_NEE, "mis": abi.EST_MIS}[lighting["method"]]
 
 
+# The number of the 505 frozen link maps this backend lowers: a lowering that starts rejecting
+# a case (or accepting one it cannot render) changes it.
+EXPECTED_ACCEPTED = 338
+
+
 def test_library_lowers_or_names_every_snapshot_case():
-    accepted, rejected = 0, 0
+    accepted, rejected = [], 0
     for name, d in FIX["cases"].items():
         try:
             p = _lower(d)
@@ -40,7 +45,7 @@ def test_library_lowers_or_names_every_snapshot_case():
             assert "is not supported by this backend" in msg, (name, msg)
             rejected += 1
             continue
-        accepted += 1
+        accepted.append(name)
         m, e = d["measurement"], d["estimator"]
         assert p.max_bounces == m["maxBounces"], name
         assert p.max_null_crossings == m.get("maxNullCrossings", 32), name
@@ -59,8 +64,31 @@ def test_library_lowers_or_names_every_snapshot_case():
         chart = (e.get("envSampler") or {}).get("chart")
         assert p.env_chart == {None: 0, "equirect": abi.CHART_EQUIRECT + 1, "octahedral": abi.CHART_OCTAHEDRAL + 1}[chart], name
         assert (p.width, p.height, p.frames_in_flight, p.abi_version) == (64, 48, 1, abi.ABI_VERSION)
-    print(f"lowered {accepted} of {accepted + rejected}")
-    assert accepted >= 280 and accepted + rejected == 505
+    print(f"lowered {len(accepted)} of {len(accepted) + rejected}")
+    assert len(accepted) + rejected == 505
+    assert len(accepted) == EXPECTED_ACCEPTED
+    # the one frozen octahedral-chart case on a sky this backend renders (the others name image
+    # environments, expression media or spectral procedural skies)
+    assert "spectral-image-constant-ref + oct" in accepted
+
+
+def test_chart_and_albedo_flag_lowering():
+    """envSampler.chart lowers to env_chart (0 = unnamed, 1 + PT_CHART_*); materials.albedo (the
+    albedo guide's dispatch flag) is accepted and leaves the program as measurement.aovs sets it."""
+    base = FIX["cases"]["proc-sky + pt-mis"]
+    assert _lower(base).env_chart == abi.CHART_EQUIRECT + 1
+    d = json.loads(json.dumps(base))
+    del d["estimator"]["envSampler"]
+    assert _lower(d).env_chart == 0
+    for chart, want in (("equirect", abi.CHART_EQUIRECT + 1), ("octahedral", abi.CHART_OCTAHEDRAL + 1)):
+        d = json.loads(json.dumps(base))
+        d["estimator"]["envSampler"] = {"chart": chart, "compensation": False}
+        assert _lower(d).env_chart == want
+    d = json.loads(json.dumps(base))
+    d["materials"]["albedo"] = True
+    assert bytes(_lower(d)) == bytes(_lower(base))
+    d["measurement"]["aovs"] = ["albedo", "normal"]
+    assert _lower(d).aovs == abi.AOV_ALBEDO | abi.AOV_NORMAL
 
 
 @pytest.mark.parametrize("name", sorted(FIX["full"]))
