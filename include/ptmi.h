@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define PT_ABI_VERSION 9u
+#define PT_ABI_VERSION 10u
 
 typedef enum pt_status {
     PT_OK = 0,
@@ -253,7 +253,17 @@ typedef struct pt_program {
     int32_t  aovs;            /* PT_AOV_* bits: measurement.aovs, the first-hit guide pictures (RGB only) */
     int32_t  env_chart;       /* estimator.envSampler.chart as PT_CHART_* + 1, 0 = not stated; when stated,
                                * a procedural sky's importance table (pt_env.chart) must be on that chart */
+    int32_t  light_selection; /* PT_SEL_*: estimator.lightSelection (computeSelectPdf,
+                               * compiler/generate/features/lighting.ts:537-542). The selection table itself
+                               * is host-derived scene data (pt_light.select_pdf / cdf, pt_env.select_prob);
+                               * under PT_SEL_UNIFORM the library checks that the resident scene holds the
+                               * uniform table */
 } pt_program;
+
+/* Light selection (components/lights/index.ts LIGHT_SELECTIONS): the power-weighted CDF (the
+ * reference's default) or one of N lights with equal probability, the environment then taking
+ * 1/(N+1) of the NEE draws (envSelectionProbability, lighting.ts:488-518). */
+enum { PT_SEL_POWER = 0, PT_SEL_UNIFORM = 1 };
 
 /* First-hit guides (src/components/aov/): at the first non-null surface a path reaches with no
  * scattering event before it, the material's albedo fact clamped to [0, 1] times the path's

@@ -30,6 +30,21 @@
 #include "../include/ptmi.h"
 #include "../include/ptmi_math.h"
 
+/* Planted mistakes (the liboracle_plants variant only): a test switches one on with
+ * or_set_plant(k) to show that its check fails when the oracle computes a plausible wrong
+ * thing. The default builds have no such code path. k = 1: the MIS light pdf takes light 0's
+ * selection pdf for every light; 2: the light-selection random is reused for the light's own
+ * sample without the cdf_rescale; 3: d log hg / dg with the sign of its (g - c) term flipped;
+ * 4: the adjoint's in-scattering term -1/sigma_s instead of +1/sigma_s; 5: the adjoint's
+ * sigma_a transmittance term dropped. */
+#ifdef ORACLE_PLANTS
+static int g_plant = 0;
+void or_set_plant(int k) { g_plant = k; }
+#define PLANT(k) (g_plant == (k))
+#else
+#define PLANT(k) 0
+#endif
+
 #ifdef ORACLE_LIBM
 /* libm variant: GLSL builtins mapped to the C library instead of the declared arithmetic.
  * Used only to show the declared arithmetic does not move the estimate. */
@@ -1357,7 +1372,7 @@ static lsample_t lighting_sample_finite(const or_scene* S, v3 p, float x0, float
     for (int i = 0; i < n - 1; i++) {
         if (x0 < S->lights[i].cdf) { id = i; sel = S->lights[i].select_pdf; lo = i == 0 ? 0.0f : S->lights[i - 1].cdf; break; }
     }
-    float xr = (x0 - lo) / sel;
+    float xr = PLANT(2) ? x0 : (x0 - lo) / sel;
     ls = light_sample_one(S, &S->lights[id], p, clampf(xr, 0.0f, 0.9999999f), x1, wl);
     ls.light_id = id;
     ls.pdf *= sel;
@@ -1418,7 +1433,7 @@ static int light_of(const or_scene* S, int region) {
 }
 static float lighting_pdf(const or_scene* S, v3 qp, v3 wi, int lid, v3 light_p) {
     const pt_light* l = &S->lights[lid];
-    float sel = S->n_lights == 1 ? 1.0f : l->select_pdf;
+    float sel = S->n_lights == 1 ? 1.0f : (PLANT(1) ? S->lights[0].select_pdf : l->select_pdf);
     if (env_select_live(S)) sel = sel * (1.0f - S->env.select_prob);
     return sel * light_pdf_one(l, qp, light_p, wi);
 }
@@ -1615,7 +1630,7 @@ static inline int adj_albedo_index(const adj_t* A, int mat) {
 /* d log hg / dg at c = dot(wi, -wo) (volume_scattering/hg/hg.glsl:11-19) */
 static inline double dlog_hg(double g, double c) {
     double d = 1.0 + g * g - 2.0 * g * c;
-    return -2.0 * g / (1.0 - g * g) - 3.0 * (g - c) / d;
+    return -2.0 * g / (1.0 - g * g) - (PLANT(3) ? -3.0 : 3.0) * (g - c) / d;
 }
 static inline void adj_scatter(adj_t* A, sp c, const double* D) {
     double e[3] = {A->delta[0] * (double)c.x, A->delta[1] * (double)c.y, A->delta[2] * (double)c.z};
@@ -1699,8 +1714,8 @@ static sp transport_trace(const or_scene* S, const or_program* P, v3 ro, v3 rd, 
                     for (int ch = 0; ch < 3; ch++) {
                         /* a channel with sigma_s = 0 scatters nothing (f = 0): no finite dlog f */
                         if (Mm->medium_arm != PT_MED_ABSORB)
-                            A->D[ch] += (ms.scattered && ssv[ch] > 0.0f ? 1.0 / (double)ssv[ch] : 0.0) - (double)ms.t;
-                        A->D[3 + ch] -= (double)ms.t;
+                            A->D[ch] += (ms.scattered && ssv[ch] > 0.0f ? (PLANT(4) ? -1.0 : 1.0) / (double)ssv[ch] : 0.0) - (double)ms.t;
+                        if (!PLANT(5)) A->D[3 + ch] -= (double)ms.t;
                     }
                 }
                 if (ms.scattered) {

@@ -7,7 +7,7 @@ import ctypes as C
 
 import numpy as np
 
-ABI_VERSION = 9
+ABI_VERSION = 10
 
 OBJ_MESH, OBJ_QUAD, OBJ_SPHERE, OBJ_INSTANCES = 0, 1, 2, 3
 MAT_LAMBERT, MAT_DIELECTRIC, MAT_GGX, MAT_NULL, MAT_MIRROR, MAT_ROUGH_DIELECTRIC = 0, 1, 2, 3, 4, 5
@@ -18,6 +18,7 @@ LIGHT_QUAD, LIGHT_POINT = 0, 1
 ENV_NONE, ENV_SKY, ENV_CONSTANT = 0, 1, 2
 AOV_ALBEDO, AOV_NORMAL = 1, 2
 CHART_EQUIRECT, CHART_OCTAHEDRAL = 0, 1
+SEL_POWER, SEL_UNIFORM = 0, 1
 CAM_PINHOLE, CAM_ORTHOGRAPHIC, CAM_THINLENS, CAM_FISHEYE, CAM_CYLINDRICAL, CAM_EQUIRECT = 0, 1, 2, 3, 4, 5
 FISHEYE_PROJECTIONS = {"equidistant": 0, "equisolid": 1, "stereographic": 2, "orthographic": 3}
 EST_PT, EST_NEE, EST_MIS = 0, 1, 2
@@ -108,7 +109,7 @@ class PtProgram(C.Structure):
                 ("width", C.c_int32), ("height", C.c_int32), ("frames_in_flight", C.c_int32),
                 ("max_null_crossings", C.c_int32), ("wavelengths", C.c_int32), ("medium_light_sampling", C.c_int32),
                 ("variance", C.c_int32), ("volume_sampling", C.c_int32), ("aovs", C.c_int32),
-                ("env_chart", C.c_int32)]
+                ("env_chart", C.c_int32), ("light_selection", C.c_int32)]
 
 
 class PtParams(C.Structure):

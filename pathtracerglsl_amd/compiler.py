@@ -42,6 +42,7 @@ class RenderStrategy:
     aovs: tuple = ()                      # measurement.aovs: first-hit guides 'albedo' / 'normal' (RGB)
     volume_sampling: str = "analytic"     # estimator.volumeSampling: 'analytic' | 'delta-tracking'
     scattering: str = "full"              # measurement.scattering: 'full' | 'ignored' (absorbing-only media)
+    light_selection: str = "power"        # estimator.lightSelection: 'power' | 'uniform'
 
 
 @dataclass
@@ -180,12 +181,31 @@ def scene_radius_estimate(scene):
 ENV_CHARTS = {"equirect": abi.CHART_EQUIRECT, "octahedral": abi.CHART_OCTAHEDRAL}
 
 
-def env_select_probability(scene, env, light_powers):
-    """envSelectionProbability, 'power' selection (features/lighting.ts:488-518): P(env) =
-    Phi_env / (Phi_env + sum Phi_light), Phi_env = 4 pi^2 r^2 mean(L) intensity, clamped to
+def select_pdfs(light_powers, selection):
+    """computeSelectPdf + cumulative (features/lighting.ts:375-380, 537-542): weight 1 per light
+    under 'uniform', its power under 'power'; pdf_i = w_i / sum(w) and cdf_i the running sum
+    of those pdfs, in f64 (the reference bakes both as literals, so the device sees them rounded
+    to fp32)."""
+    weights = [1.0 if selection == "uniform" else float(p) for p in light_powers]
+    total = 0.0
+    for w in weights:
+        total += w
+    pdfs = [w / total for w in weights]
+    cdf, acc = [], 0.0
+    for p in pdfs:
+        acc += p
+        cdf.append(acc)
+    return pdfs, cdf
+
+
+def env_select_probability(scene, env, light_powers, selection="power"):
+    """envSelectionProbability (features/lighting.ts:488-518). 'uniform': 1 / (N + 1). 'power':
+    Phi_env / (Phi_env + sum Phi_light), Phi_env = 4 pi^2 r^2 mean(L) intensity. Both clamped to
     [0.01, 0.99]. mean(L): the constant colour's channel mean, or the importance table's mass
     totalWeight pi / (2 W H) for the tabulated sky on the equirect chart, totalWeight / N^2 on the
     N x N octahedral chart (lighting.ts:502-512)."""
+    if selection == "uniform":
+        return min(0.99, max(0.01, 1.0 / (len(light_powers) + 1)))
     if env.type == abi.ENV_CONSTANT:
         mean_l = (env.color[0] + env.color[1] + env.color[2]) / 3.0
     else:
@@ -238,6 +258,9 @@ def _analyze_validate(scene, strategy):
             raise CompileError("mediumLightSampling 'equiangular' does not combine with an environment light")
     # volumeSampling axis (Validator.ts:497-615): a constant-emission SCATTERING medium routes to
     # the null-collision (delta-tracking) arm; everything else stays on the analytic arms
+    if strategy.light_selection not in ("power", "uniform"):
+        raise CompileError(f"estimator.lightSelection '{strategy.light_selection}' not supported — use 'power' or "
+                           "'uniform'")
     if strategy.scattering not in ("full", "ignored"):
         raise CompileError(f"unknown measurement.scattering '{strategy.scattering}'")
     if strategy.volume_sampling not in ("analytic", "delta-tracking"):
@@ -550,21 +573,20 @@ def compile_scene(scene, strategy=None, width=256, height=256, reset_salt=1234):
             rec["edge2"][:3] = e2
             rec["normal"][:3] = nrm
             rec["area"] = area
-            power = max(1e-8, math.pi * area * float(rad.mean()))
+            power = max(1e-8, math.pi * area * ((float(rad[0]) + float(rad[1])) + float(rad[2])) / 3.0)
         else:
             rec["kind"] = abi.LIGHT_POINT
             rec["region"] = -1
             rec["corner"][:3] = l["position"]
-            power = max(1e-8, 4.0 * math.pi * float(rad.mean()))
+            power = max(1e-8, 4.0 * math.pi * ((float(rad[0]) + float(rad[1])) + float(rad[2])) / 3.0)
         lights.append((rec, power))
-    total = sum(p for _, p in lights)
-    acc = 0.0
     light_recs = []
-    for rec, p in lights:
-        acc += p
-        rec["select_pdf"] = p / total
-        rec["cdf"] = acc / total
-        light_recs.append(rec)
+    if lights:
+        pdfs, cdf = select_pdfs([p for _, p in lights], strategy.light_selection)
+        for (rec, _), p, c in zip(lights, pdfs, cdf):
+            rec["select_pdf"] = p
+            rec["cdf"] = c
+            light_recs.append(rec)
 
     env = abi.PtEnv(abi.ENV_NONE, 512, 256, 0, 1.0, 0.0, 0.5, 0)
     e = scene.get("environment", {"type": "none"})
@@ -587,7 +609,7 @@ def compile_scene(scene, strategy=None, width=256, height=256, reset_salt=1234):
         env.s_color = abi.PtSpectrum.of(srec(col, spectral.ROLE_EMISSION))
 
     if env.type != abi.ENV_NONE and (env.type == abi.ENV_SKY or env.samplable) and light_recs and "select_prob" not in e:
-        env.select_prob = env_select_probability(scene, env, [p for _, p in lights])
+        env.select_prob = env_select_probability(scene, env, [p for _, p in lights], strategy.light_selection)
 
     tv = np.concatenate(tri_chunks) if tri_chunks else np.zeros((0, 9), np.float32)
     to = np.concatenate(tri_obj_chunks) if tri_obj_chunks else np.zeros(0, np.int32)
@@ -621,12 +643,14 @@ def compile_scene(scene, strategy=None, width=256, height=256, reset_salt=1234):
     prog.width, prog.height = width, height
     prog.frames_in_flight = max(1, strategy.frames_in_flight)
     prog.max_null_crossings = strategy.max_null_crossings
+    prog.light_selection = abi.SEL_UNIFORM if strategy.light_selection == "uniform" else abi.SEL_POWER
     warnings = []
     if strategy.volume_sampling == "delta-tracking" and not any(int(r["medium_arm"]) == abi.MED_DELTA and r["has_medium"]
                                                                  for r in materials):
         warnings.append("volumeSampling 'delta-tracking' controls nothing here — no medium is emissive and "
                         "scattering; constant media stay on the analytic arms")
-    return CompiledRenderer(id=f"{strategy.color}-{strategy.direct_lighting}-{scene.get('id', 'scene')}", scene=arrays,
+    sel = "-uniform" if strategy.light_selection == "uniform" else ""
+    return CompiledRenderer(id=f"{strategy.color}-{strategy.direct_lighting}{sel}-{scene.get('id', 'scene')}", scene=arrays,
                             program=prog, width=width, height=height, camera=scene["camera"],
                             reset_salt=reset_salt, warnings=warnings, material_names=list(mat_names))
 
@@ -676,7 +700,7 @@ def describe(scene, strategy):
             "accumulation": {"type": "average", **({"variance": True} if strategy.variance else {})},
             "envSampler": {"chart": env.get("chart", "equirect"), "compensation": bool(env.get("compensated", False))},
             "lighting": None if strategy.direct_lighting == "none" else {"method": strategy.direct_lighting,
-                                                                          "selection": "power"},
+                                                                          "selection": strategy.light_selection},
             "mediumLightSampling": strategy.medium_light_sampling,
             "russianRoulette": None if strategy.rr_start < 0 else {"startDepth": strategy.rr_start,
                                                                     "maxSurvival": strategy.rr_max_survival},

@@ -209,7 +209,8 @@ bool lower(const JVal& d, int32_t width, int32_t height, int32_t fif, pt_program
         else if (is_str(method, "mis")) pg.estimator = PT_EST_MIS;
         else return L.unsupported("estimator.lighting.method", quoted(method));
         const JVal* sel = lighting->get("selection");
-        if (sel && !is_str(sel, "power")) return L.unsupported("estimator.lighting.selection", quoted(sel));
+        if (sel && is_str(sel, "uniform")) pg.light_selection = PT_SEL_UNIFORM;
+        else if (sel && !is_str(sel, "power")) return L.unsupported("estimator.lighting.selection", quoted(sel));
     }
     const JVal* rr = e->get("russianRoulette");
     pg.rr_max_survival = 0.95f;

@@ -55,6 +55,7 @@ struct SceneStore {
     bool has_mirror = false;
     bool has_rough = false;
     std::vector<pt_material> host_materials;
+    std::vector<pt_light> host_lights;
     hipStream_t stream = nullptr;
     template <class T> pt_status upload(const T* host, size_t n, const T** out) {
         auto b = std::make_unique<DevBuf>();
@@ -90,6 +91,7 @@ struct pt_ctx {
     bool has_mirror = false;
     bool has_rough = false;
     std::vector<pt_material> host_materials;   // the uploaded records (adjoint checks, updates)
+    std::vector<pt_light> host_lights;         // the uploaded light records (selection-table check)
     bool timing = false;
     std::map<std::string, TimingSlot> timings;
     std::vector<std::string> timing_order;

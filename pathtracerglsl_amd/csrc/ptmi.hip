@@ -445,6 +445,7 @@ static pt_status build_scene(const pt_scene_desc* sd, hipStream_t stream, SceneS
     if ((st = SS.upload(SS.host_materials.data(), SS.host_materials.size(), &D.materials)) != PT_OK) return st;
     if ((st = SS.upload(sd->lights, (size_t)sd->n_lights, &D.lights)) != PT_OK) return st;
     D.n_lights = sd->n_lights;
+    SS.host_lights.assign(sd->lights, sd->lights + sd->n_lights);
     SS.has_ggx = false;
     SS.has_mirror = false;
     SS.has_rough = false;
@@ -679,17 +680,41 @@ pt_status pt_scene_upload(pt_ctx* ctx, const pt_scene_desc* sd) {
     ctx->has_mirror = SS.has_mirror;
     ctx->has_rough = SS.has_rough;
     ctx->host_materials = std::move(SS.host_materials);
+    ctx->host_lights = std::move(SS.host_lights);
     ctx->has_scene = true;
     return PT_OK;
 }
 
 }  // extern "C"
 
+// The uniform selection table (computeSelectPdf with weight 1 per light + cumulative,
+// features/lighting.ts:375-380, 537-542; envSelectionProbability's 1 / (N + 1), :488-518),
+// each value the f64 expression rounded to fp32 as the reference's baked literal is.
+static bool holds_uniform_selection(const DevScene& dev, const std::vector<pt_light>& lights) {
+    const int n = (int)lights.size();
+    if (n >= 2) {
+        double acc = 0.0;
+        for (int i = 0; i < n; i++) {
+            const double p = 1.0 / (double)n;
+            acc += p;
+            if (lights[i].select_pdf != (float)p || lights[i].cdf != (float)acc) return false;
+        }
+    }
+    const bool env_live = (dev.env_type == PT_ENV_SKY || (dev.env_type == PT_ENV_CONSTANT && dev.env_samplable)) && n > 0;
+    return !env_live || dev.env_select_prob == (float)std::min(0.99, std::max(0.01, 1.0 / (double)(n + 1)));
+}
+
 // The program-vs-scene rules, checked at renderer load when a scene is resident and again by
 // every render, forward and backward call (a later pt_scene_upload may have replaced the
-// scene): the spectral path has no procedural-sky radiance, and a program that names its
-// envSampler.chart must sample the chart the sky table was built on.
-static pt_status check_program_scene(const pt_program& prog, const DevScene& dev) {
+// scene): the spectral path has no procedural-sky radiance, a program that names its
+// envSampler.chart must sample the chart the sky table was built on, and a program with
+// uniform light selection needs the scene's selection table to be the uniform one.
+static pt_status check_program_scene(const pt_program& prog, const DevScene& dev, const std::vector<pt_light>& lights) {
+    if (prog.light_selection != PT_SEL_POWER && prog.light_selection != PT_SEL_UNIFORM)
+        return fail(PT_E_INVALID, "unknown light selection");
+    if (prog.light_selection == PT_SEL_UNIFORM && !holds_uniform_selection(dev, lights))
+        return fail(PT_E_INVALID, "the program selects lights uniformly but the scene's selection table (pt_light.select_pdf / "
+                                  "cdf, pt_env.select_prob) is not the uniform one");
     if (prog.color == PT_COLOR_SPECTRAL && dev.env_type == PT_ENV_SKY)
         return fail(PT_E_UNSUPPORTED, "spectral programs with the procedural sky are not provided by this build");
     if (prog.env_chart && dev.env_type == PT_ENV_SKY && prog.env_chart - 1 != dev.env_chart)
@@ -719,7 +744,7 @@ pt_status pt_renderer_load(pt_ctx* ctx, const pt_program* prog, pt_renderer** ou
         return fail(PT_E_UNSUPPORTED, "first-hit guides (aovs) are provided for RGB programs only");
     if (prog->env_chart < 0 || prog->env_chart > PT_CHART_OCTAHEDRAL + 1) return fail(PT_E_INVALID, "unknown env chart");
     if (ctx->has_scene)
-        if (pt_status sc = check_program_scene(*prog, ctx->dev); sc != PT_OK) return sc;
+        if (pt_status sc = check_program_scene(*prog, ctx->dev, ctx->host_lights); sc != PT_OK) return sc;
     HIP_TRY(hipSetDevice(ctx->device));
     auto* r = new pt_renderer();
     r->ctx = ctx;
@@ -841,7 +866,7 @@ static pt_status run_waves(pt_renderer* r, const pt_params* prm, uint32_t nframe
     const pt_program& pg = r->prog;
     const int F = std::max(1, std::min<int>(pg.frames_in_flight, (int)std::max<uint32_t>(nframes, 1)));
     const bool spectral = pg.color == PT_COLOR_SPECTRAL;
-    if (pt_status sc = check_program_scene(pg, ctx->dev); sc != PT_OK) return sc;
+    if (pt_status sc = check_program_scene(pg, ctx->dev, ctx->host_lights); sc != PT_OK) return sc;
     if (spectral && !ctx->dev.spec_curves)
         return fail(PT_E_MISSING_INPUT, "missing input: spectral_curves (the spectral program needs the CIE/D65 curves in pt_scene_desc)");
     HIP_TRY(hipMemsetAsync(r->b_hist.p, 0, r->b_hist.bytes, ctx->stream));
@@ -1066,6 +1091,7 @@ pt_status pt_batch_create(pt_ctx* ctx, int32_t n, const pt_scene_desc* scenes, c
     for (int s = 0; s < n; s++) {
         auto store = std::make_unique<SceneStore>();
         pt_status st = build_scene(&scenes[s], ctx->stream, *store);
+        if (st == PT_OK) st = check_program_scene(programs[s], store->dev, store->host_lights);
         if (st != PT_OK) { delete b; return st; }
         b->min_stack = std::max(b->min_stack, store->stack);
         b->any_ggx = b->any_ggx || store->has_ggx;

@@ -66,6 +66,52 @@ def cornell_box(light_radiance=(17.0, 12.0, 4.0)):
     }
 
 
+def two_light():
+    """The reference's two-light witness (tests/witnesses/scenes/twoLightScene.ts): two diffuse
+    spheres on a floor before a back wall, lit only by a warm key and a cool fill point light of
+    unequal power, posed at (0, 1.2, 4) -> (0, 0.5, 0) with fov 0.9 (tests/witnesses/index.ts:
+    178-191). The reference's planes (floor y = 0, wall z = -3) are 40 x 40 quads here."""
+    return {
+        "id": "two-light",
+        "materials": {
+            "floor": {"model": "lambert", "albedo": [0.6, 0.6, 0.6]},
+            "ivory": {"model": "lambert", "albedo": [0.82, 0.78, 0.68]},
+        },
+        "objects": [
+            {"type": "quad", "corner": [-20.0, 0.0, 20.0], "edge1": [40.0, 0.0, 0.0], "edge2": [0.0, 0.0, -40.0],
+             "material": "floor"},
+            {"type": "quad", "corner": [-20.0, 0.0, -3.0], "edge1": [40.0, 0.0, 0.0], "edge2": [0.0, 40.0, 0.0],
+             "material": "floor"},
+            {"type": "sphere", "center": [-0.75, 0.5, 0.0], "radius": 0.5, "material": "ivory"},
+            {"type": "sphere", "center": [0.75, 0.5, 0.0], "radius": 0.5, "material": "ivory"},
+        ],
+        "lights": [
+            {"type": "point", "position": [-2.5, 3.0, 1.5], "intensity": [26.0, 14.3, 6.5]},
+            {"type": "point", "position": [2.5, 2.5, 1.5], "intensity": [4.9, 7.7, 14.0]},
+        ],
+        "environment": {"type": "none"},
+        "camera": {"type": "pinhole", "position": [0.0, 1.2, 4.0], "target": [0.0, 0.5, 0.0], "fov": 0.9},
+    }
+
+
+def cornell_multi_light(point=True):
+    """The Cornell box lit by several finite lights of unequal power: the ceiling quad (17, 12, 4),
+    a smaller, cooler quad on the back wall facing +z, a dim warm quad low on the left wall
+    facing +x and (point=True) a point light under the ceiling: the N-light selection CDF, its
+    cdf_rescale and the per-light selection pdf in the MIS weight all act."""
+    sc = cornell_box()
+    sc["id"] = "cornell-multi-light" + ("" if point else "-quads")
+    sc["lights"] = sc["lights"] + [
+        {"type": "quad", "corner": [0.3, 0.3, -0.999], "edge1": [0.3, 0.0, 0.0], "edge2": [0.0, 0.3, 0.0],
+         "radiance": [3.0, 6.0, 12.0]},
+        {"type": "quad", "corner": [-0.999, -0.9, 0.2], "edge1": [0.0, 0.2, 0.0], "edge2": [0.0, 0.0, 0.4],
+         "radiance": [6.0, 3.0, 1.0]},
+    ]
+    if point:
+        sc["lights"].append({"type": "point", "position": [0.4, 0.7, 0.5], "intensity": [0.6, 0.5, 0.4]})
+    return sc
+
+
 # ---------------------------------------------------------- icosphere cloud (workload C1)
 
 _ICO_T = (1.0 + math.sqrt(5.0)) / 2.0

@@ -31,7 +31,7 @@ def _expected_estimator(d):
 
 # The number of the 505 frozen link maps this backend lowers: a lowering that starts rejecting
 # a case (or accepting one it cannot render) changes it.
-EXPECTED_ACCEPTED = 338
+EXPECTED_ACCEPTED = 339
 
 
 def test_library_lowers_or_names_every_snapshot_case():
@@ -51,6 +51,8 @@ def test_library_lowers_or_names_every_snapshot_case():
         assert p.max_null_crossings == m.get("maxNullCrossings", 32), name
         assert p.color == (abi.COLOR_SPECTRAL if m["color"] == "spectral" else abi.COLOR_RGB), name
         assert p.estimator == _expected_estimator(d), name
+        sel = (d["estimator"].get("lighting") or {}).get("selection", "power")
+        assert p.light_selection == (abi.SEL_UNIFORM if sel == "uniform" else abi.SEL_POWER), name
         rr = e.get("russianRoulette")
         assert p.rr_start == (-1 if rr is None else rr["startDepth"]), name
         assert abs(p.rr_max_survival - ((rr or {}).get("maxSurvival") or 0.95)) < 1e-7, name
@@ -70,6 +72,7 @@ def test_library_lowers_or_names_every_snapshot_case():
     # the one frozen octahedral-chart case on a sky this backend renders (the others name image
     # environments, expression media or spectral procedural skies)
     assert "spectral-image-constant-ref + oct" in accepted
+    assert "two-light + uniform" in accepted                  # estimator.lighting.selection 'uniform'
 
 
 def test_chart_and_albedo_flag_lowering():
@@ -120,6 +123,7 @@ def test_rejection_names_the_field():
 
 @pytest.mark.parametrize("make,strategy", [
     (scenes.cornell_box, compiler.RenderStrategy("mis", 8, 3)),
+    (scenes.two_light, compiler.RenderStrategy("nee", 6, -1, light_selection="uniform")),
     (scenes.cornell_box, compiler.RenderStrategy("none", 5, -1, variance=True)),
     (scenes.haze, compiler.RenderStrategy("nee", 24, 4, medium_light_sampling="equiangular")),
     (scenes.emit_sat, compiler.RenderStrategy("nee", 32, -1, volume_sampling="delta-tracking")),

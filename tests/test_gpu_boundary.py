@@ -143,3 +143,19 @@ def test_scene_upload_rejects_unknown_chart(engine):
     assert lib.pt_scene_upload(engine.ctx, C.byref(desc)) == 1
     assert b"pt_env.chart" in lib.pt_last_error()
     engine._scene = None
+
+
+def test_uniform_program_needs_the_uniform_table(engine):
+    """A program that selects lights uniformly over a scene whose table was built for power
+    selection is rejected at load; over the uniform table it renders."""
+    lib = _lib.load()
+    power = compiler.compile_scene(scenes.cornell_multi_light(), compiler.RenderStrategy("mis", 4, 3), 16, 16)
+    uni = compiler.compile_scene(scenes.cornell_multi_light(), compiler.RenderStrategy("mis", 4, 3, light_selection="uniform"),
+                                 16, 16)
+    engine.upload_scene(power.scene)
+    h = C.c_void_p()
+    assert lib.pt_renderer_load(engine.ctx, C.byref(uni.program), C.byref(h)) == 1
+    assert b"uniform" in lib.pt_last_error()
+    engine.load_renderer(uni)
+    engine.render_frames(1)
+    assert np.isfinite(engine.read_export()).all()

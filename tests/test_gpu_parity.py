@@ -392,3 +392,26 @@ def test_full_size_cloud_is_wave_and_call_invariant(engine):
         _say(f"[gpu] full-size cloud fif={fif} calls={calls}: mean {films[-1][..., :3].mean():.5f}")
     assert all(np.array_equal(films[0], f) for f in films[1:])
     assert np.isfinite(films[0]).all() and films[0][..., :3].mean() > 0.05
+
+
+@pytest.mark.parametrize("case,est,sel", [("two-light", "nee", "power"), ("two-light", "nee", "uniform"),
+                                          ("multi", "mis", "power"), ("multi", "mis", "uniform"),
+                                          ("multi", "nee", "uniform"), ("multi-quads", "none", "power")])
+def test_multi_light_selection_matches_oracle(engine, oracle, case, est, sel):
+    """N > 1 finite lights (the selection CDF, cdf_rescale and per-light selection pdfs in the MIS
+    weight) under power and uniform selection: GPU film = oracle film."""
+    sc = scenes.two_light() if case == "two-light" else scenes.cornell_multi_light(point=case == "multi")
+    st = scene_cache.compiler.RenderStrategy(est, 6 if case == "two-light" else 8, -1 if case == "two-light" else 3,
+                                             light_selection=sel)
+    comp = scene_cache.compiler.compile_scene(sc, st, 40, 32)
+    assert len(comp.scene.lights) >= 2
+    _say(f"[gpu] multi-light {case} {est} {sel}")
+    engine.load_renderer(comp)
+    engine.clear_accumulation()
+    engine.render_frames(8)
+    g = engine.read_export()
+    o, _, counts = oracle_lib.OracleScene(oracle, comp.scene).render(comp.program, comp.params(), 8)
+    exact, bad = _compare_film(f"multi-light-{case}-{est}-{sel}", g, o)
+    assert o[..., :3].mean() > 0.02
+    assert bad == 0.0
+    assert engine.ray_counts() == (int(counts[0]), int(counts[1]))
