@@ -1462,22 +1462,46 @@ static void derive_majorants(pt_material* m) {
     uint32_t u; memcpy(&u, &r, 4); u++; memcpy(&r, &u, 4);
     m->majorant[1] = r;
 }
+/* Frozen sampling proposal (test infrastructure: off unless a test calls or_set_proposal; the
+ * reference has no such mode). For one material, the analytic medium arm draws its channel and
+ * distance, and the HG phase its direction, from fixed proposal coefficients (RGB), and the
+ * path is weighted by the live ones: importance sampling with a proposal that does not depend
+ * on the live coefficients, unbiased for every value of them. With Russian roulette off, a
+ * path's decisions then do not move when the live coefficients do, so the same-seed central
+ * difference of a render is the derivative of one smooth estimator; with the proposal equal to
+ * the live coefficients every value is bit-identical to the plain walk. */
+static int g_prop_mat = -1;
+static float g_prop_ss[3], g_prop_sa[3], g_prop_g;
+void or_set_proposal(int material, const float* sigma_s, const float* sigma_a, float g) {
+    g_prop_mat = material;
+    for (int c = 0; c < 3 && material >= 0; c++) { g_prop_ss[c] = sigma_s[c]; g_prop_sa[c] = sigma_a[c]; }
+    g_prop_g = g;
+}
+static inline int proposal_of(const or_scene* S, const pt_material* m) {
+    return g_prop_mat >= 0 && g_prop_mat < S->n_materials && m == &S->materials[g_prop_mat];
+}
+
 static msample_t medium_sample_analytic(const or_scene* S, const pt_material* m, float t_max, float x0, float x1, const wl_t* wl) {
     msample_t ms;
     sp sa = sigma_a_of(S, m, wl), ss = sigma_s_of(S, m, wl);
     sp st = sadd(sa, ss);
+    sp sq = st;                                   /* the sampling extinction */
+    if (!wl->spec && proposal_of(S, m))
+        sq = sadd(SP(g_prop_sa[0], g_prop_sa[1], g_prop_sa[2], 0.0f), SP(g_prop_ss[0], g_prop_ss[1], g_prop_ss[2], 0.0f));
     const int N = wl->spec ? 4 : 3;
     int c = (int)(x0 * (float)N); if (c > N - 1) c = N - 1;     /* spectrum_sample_channel */
-    float sc = s_lane(st, c);
+    float sc = s_lane(sq, c);
     float t = sc == 0.0f ? t_max : -LOG(1.0f - x1) / sc;
     if (t < t_max) {
         sp tr = s_exp(smuls(sneg(st), t));
-        float pdf = s_avg(wl, smul(st, tr));
+        sp trq = s_exp(smuls(sneg(sq), t));
+        float pdf = s_avg(wl, smul(sq, trq));
         ms.scattered = 1; ms.t = t;
         ms.weight = sdivs(smul(ss, tr), fmaxf_(pdf, 1e-20f));
     } else {
         sp tr = s_exp(smuls(sneg(st), t_max));
-        float pdf = s_avg(wl, tr);
+        sp trq = s_exp(smuls(sneg(sq), t_max));
+        float pdf = s_avg(wl, trq);
         ms.scattered = 0; ms.t = t_max;
         ms.weight = sdivs(tr, fmaxf_(pdf, 1e-20f));
     }
@@ -1723,6 +1747,8 @@ static sp transport_trace(const or_scene* S, const or_program* P, v3 ro, v3 rd, 
                     v3 p_evt = add(ro, muls(rd, ms.t));
                     v3 wo_med = neg(rd);
                     float g = Mm->hg_g;
+                    const int frozen = !wl->spec && proposal_of(S, Mm);
+                    const float gq = frozen ? g_prop_g : g;     /* the phase sampling's g */
                     if (nee && !P->equiangular) {   /* light_sample_direct_medium (techniques/light/light_medium.glsl:7-18) */
                         float y0 = rnd(rng), y1 = rnd(rng);
                         lsample_t ls = lighting_sample(S, p_evt, y0, y1, wl);
@@ -1735,7 +1761,7 @@ static sp transport_trace(const or_scene* S, const or_program* P, v3 ro, v3 rd, 
                             if (!s_black(wl, vis)) {
                                 float f = hg_value(ls.wi, wo_med, g);
                                 float w = 1.0f;
-                                if (mis && !(ls.flags & 1)) w = power_heuristic(ls.pdf, hg_pdf(ls.wi, wo_med, g, wl));
+                                if (mis && !(ls.flags & 1)) w = power_heuristic(ls.pdf, hg_pdf(ls.wi, wo_med, gq, wl));
                                 sp c = smul(smul(T, ls.radiance), sgrey(f));
                                 c = smul(c, vis);
                                 c = smuls(c, w);
@@ -1753,7 +1779,8 @@ static sp transport_trace(const or_scene* S, const or_program* P, v3 ro, v3 rd, 
                     /* kernel_sample_phase (techniques/kernel/kernel_phase.glsl:7-15) */
                     float z0 = rnd(rng), z1 = rnd(rng);
                     float ppdf;
-                    v3 wi = hg_sample(wo_med, g, z0, z1, &ppdf, wl);
+                    v3 wi = hg_sample(wo_med, gq, z0, z1, &ppdf, wl);
+                    if (frozen) T = smuls(T, hg_value(wi, wo_med, g) / hg_value(wi, wo_med, gq));
                     if (A && med_mat == A->p->medium_material) A->D[6] += dlog_hg(g, dot(wi, neg(wo_med)));
                     prev_was_delta = 0; prev_bsdf_pdf = ppdf; prev_qp = p_evt;
                     ro = p_evt; rd = wi;

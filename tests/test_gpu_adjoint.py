@@ -26,11 +26,13 @@ def _setup(engine, theta, strategy, w, h, spp, mode=abi.REDUCE_DETERMINISTIC):
     return comp, inverse.InverseRender(engine, comp, med, alb, spp, mode)
 
 
-def test_backward_matches_fp64_replay(engine, oracle):
+@pytest.mark.parametrize("depth", [12, 32])
+def test_backward_matches_fp64_replay(engine, oracle, depth):
+    """At the differentiable media workload's bounce depth 32 (and 12), on a small film."""
     _, theta = scenes.inverse_media_params()
-    comp, inv = _setup(engine, theta, compiler.RenderStrategy("mis", 12, 3), 32, 32, 16)
+    comp, inv = _setup(engine, theta, compiler.RenderStrategy("mis", depth, 3), 32, 32, 16)
     dL = np.random.default_rng(5).uniform(-1, 1, size=(32, 32, 4)).astype(np.float32)
-    _say("[gpu] backward 32x32x16 inverse-media vs FP64 replay")
+    _say(f"[gpu] backward 32x32x16 depth {depth} inverse-media vs FP64 replay")
     g = inv.gradient(theta, dL, 7)
     comp.reset_salt = 7
     ors = oracle_lib.OracleScene(oracle, comp.scene)
@@ -94,3 +96,17 @@ def test_inverse_render_recovers_parameters(engine):
     e0, e1 = inverse.param_error(init, target), inverse.param_error(theta, target)
     _say(f"[gpu] inverse: mean relative parameter error {e0:.3f} -> {e1:.3f} in 60 Adam steps")
     assert e1 < 0.7 * e0
+
+
+def test_backward_matches_central_differences_all_parameters(engine, oracle):
+    """pt_backward at the differentiable media workload's depth 32 against central differences
+    (h = 1e-3) of the L2 loss for all 23 parameters, on the oracle's frozen-proposal estimator
+    (test_adjoint_fd.py: its paths are the ones pt_backward replays): within 1 % per parameter
+    (1e-4 of the largest gradient for the smallest ones)."""
+    import test_adjoint_fd as F
+    theta, comp, med, alb, dL, fd = F.fd_problem(oracle)
+    _say("[gpu] backward vs central differences, 23 parameters, depth 32")
+    engine.load_renderer(comp)
+    g = engine.backward(F.SPP, abi.PtAdjointParams.make(med, alb), dL, comp.params(F.SALT)).astype(np.float64)
+    err = F.report("[gpu] pt_backward vs FD", g, fd)
+    assert err.max() < 0.01, err

@@ -37,11 +37,14 @@ def _compare(engine, oracle, name, sc, strategy, w, h, spp):
     return bad, counts
 
 
-@pytest.mark.parametrize("est", ["mis", "nee", "none"])
-def test_spectral_glass_medium_cornell_matches_oracle(engine, oracle, est):
-    bad, _ = _compare(engine, oracle, "spectral-glass-medium", scenes.cornell_spectral_medium(),
-                      compiler.RenderStrategy(est, 16, 3, color="spectral"), 32, 32, 8)
+@pytest.mark.parametrize("est,depth", [("mis", 16), ("nee", 16), ("none", 16), ("mis", 32), ("nee", 32), ("none", 32)])
+def test_spectral_glass_medium_cornell_matches_oracle(engine, oracle, est, depth):
+    """The spectral glass-medium Cornell box of the spectral media workload, at its own bounce
+    depth 32 (and 16), on a small film."""
+    bad, counts = _compare(engine, oracle, f"spectral-glass-medium-{est}-d{depth}", scenes.cornell_spectral_medium(),
+                           compiler.RenderStrategy(est, depth, 3, color="spectral"), 32, 32, 8)
     assert bad == 0.0
+    assert engine.ray_counts() == (int(counts[0]), int(counts[1]))
 
 
 def test_spectral_cornell_uniform_wavelengths(engine, oracle):
