@@ -356,6 +356,19 @@ void      pt_renderer_unload(pt_renderer* r);
  * sampleCount = current count + k and averages into the film. Asynchronous. */
 pt_status pt_render_frames(pt_renderer* r, const pt_params* params, uint32_t nframes);
 
+/* pt_render_frames without any device allocation. It has no counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). The path state lives in one static device
+ * array of the library per GPU (about 68 MiB: PT_NOALLOC_MAX_PATHS slots, held until the
+ * process ends) instead of the renderer's own buffers; the film is the renderer's, allocated
+ * at load. The film may have at most PT_NOALLOC_MAX_PATHS pixels (PT_E_INVALID otherwise),
+ * and frames in flight shrink so that width * height * frames fits; the film equals
+ * pt_render_frames' bit for bit. Every call on a GPU uses that GPU's one array, which keeps
+ * the last call's contents: calls that overlap on one GPU (two contexts' streams) race.
+ * pt_last_ray_counts reports pt_render_frames calls only. Asynchronous. */
+#define PT_NOALLOC_MAX_PATHS 262144
+pt_status pt_render_frames_noalloc(pt_renderer* r, const pt_params* params, uint32_t nframes);
+
 /* Engine.readExport('radiance') (Engine.ts:351-398): RGBA32F film, row 0 = bottom
  * (gl_FragCoord convention), width*height*4 floats. Synchronises the stream. "variance"
  * reads the Welford statistic picture (per-channel population variance of the samples) of a

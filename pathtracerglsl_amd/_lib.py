@@ -17,7 +17,8 @@ DECLARED = ["pt_last_error", "pt_ctx_create", "pt_ctx_destroy", "pt_scene_upload
             "pt_batch_create", "pt_batch_destroy", "pt_batch_set_config", "pt_batch_get_config",
             "pt_batch_render", "pt_batch_read_export", "pt_batch_autotune",
             "pt_program_from_description", "pt_renderer_load_description", "pt_env_total_weight",
-            "pt_obj_parse", "pt_mesh_data_info", "pt_mesh_data_copy", "pt_mesh_data_free", "pt_inst_parse"]
+            "pt_obj_parse", "pt_mesh_data_info", "pt_mesh_data_copy", "pt_mesh_data_free", "pt_inst_parse",
+            "pt_render_frames_noalloc"]
 # every symbol declared in include/ptmi_debug.h (test and bench instrumentation)
 DECLARED_DEBUG = ["pt_renderer_set_traversal_counting", "pt_last_traversal_counts", "pt_trace_closest_device_counted",
                   "pt_scene_stats", "pt_abi_sizeof", "pt_debug_wide_bvh", "pt_debug_math", "pt_calib_copy",
@@ -50,6 +51,7 @@ def load():
     lib.pt_renderer_unload.argtypes = [vp]
     lib.pt_renderer_unload.restype = None
     lib.pt_render_frames.argtypes = [vp, C.POINTER(abi.PtParams), u32]
+    lib.pt_render_frames_noalloc.argtypes = [vp, C.POINTER(abi.PtParams), u32]
     lib.pt_read_export.argtypes = [vp, C.c_char_p, vp, C.c_size_t]
     lib.pt_clear_accumulation.argtypes = [vp]
     lib.pt_sample_count.argtypes = [vp]

@@ -79,6 +79,7 @@ struct SceneStore {
 struct pt_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    void* noalloc_state = nullptr;   // device address of this GPU's allocation-free path state
     std::vector<std::unique_ptr<DevBuf>> scene_bufs;
     DevScene dev{};
     bool has_scene = false;
@@ -132,11 +133,9 @@ struct pt_renderer {
     DevBuf film, var_film, aov_film[2];
     uint32_t sample_count = 0;
     long long cap = 0;    // path slots allocated
-    DevBuf b_ray_o, b_ray_d, b_thr, b_rad, b_rng, b_prev, b_flags, b_medium, b_hit, b_q0, b_q1, b_counts,
-        b_spath, b_so, b_sd, b_sc, b_sl, b_wl, b_eta, b_hist, b_trav, b_cat[N_CAT], b_aov[2];
+    DevBuf b_state;       // the path state of cap slots, one block carved by carve_paths
     // adjoint buffers (allocated by the first pt_backward)
     DevBuf b_adjD, b_adjG, b_adjsD, b_delta, b_pixgrad, b_grad;
-    DevBuf b_pscene;      // scene of each path slot (batched wavefront)
     long long adj_cap = 0;
     int adj_mode = PT_REDUCE_DETERMINISTIC;
     Paths paths{};

@@ -21,6 +21,9 @@ constexpr bool cat_probes(int cat, bool media) { return cat == CAT_DIELECTRIC ||
 enum { C_RAYQ0 = 0, C_RAYQ1 = 1, C_SHADOW = 2, C_CAT0 = 3, C_EXT_FETCH = C_CAT0 + N_CAT, C_SHD_FETCH, C_CAM_MEDIUM,
        N_COUNTS = 16 };
 
+// Bytes ahead of a path state's arrays: the queue counters, ray and traversal statistics.
+constexpr size_t STATE_HEAD_BYTES = 768;
+
 struct Paths {
     float4* ray_o;   // origin.xyz
     float4* ray_d;   // direction.xyz

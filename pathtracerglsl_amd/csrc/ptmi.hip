@@ -21,6 +21,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -233,6 +234,37 @@ pt_status pt_finish(pt_ctx* ctx) {
     return PT_OK;
 }
 
+}  // extern "C"
+
+// ---------------------------------------------------------------- allocation-free path state
+// The path state of pt_render_frames_noalloc: one array per GPU, the library's own, held until
+// the process ends (PT_NOALLOC_MAX_PATHS slots with both guide pictures, plus the counter
+// head). Host code never passes the array's name to a kernel or a copy: its device address on
+// each GPU comes from hipGetSymbolAddress, taken once per device with that device current (at
+// the first pt_ctx_create on it) and kept in noalloc_addr.
+constexpr size_t NOALLOC_BYTES_PER_PATH = 272;   // the carve of one slot: 208 B of state, 8 queues, 2 guides
+constexpr size_t NOALLOC_BYTES = STATE_HEAD_BYTES + NOALLOC_BYTES_PER_PATH * (size_t)PT_NOALLOC_MAX_PATHS;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static void* noalloc_addr[NOALLOC_MAX_DEVICES];
+static std::mutex noalloc_mutex;
+
+static pt_status noalloc_setup(int device, void** out) {
+    if (device >= NOALLOC_MAX_DEVICES) return fail(PT_E_UNSUPPORTED, "allocation-free path state: device index too large");
+    std::lock_guard<std::mutex> lock(noalloc_mutex);
+    if (!noalloc_addr[device]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(PT_E_DEVICE, std::string("allocation-free path state: hipGetSymbolAddress: ") + hipGetErrorString(e));
+        noalloc_addr[device] = p;
+    }
+    *out = noalloc_addr[device];
+    return PT_OK;
+}
+
+extern "C" {
+
 pt_status pt_ctx_create(int device_index, pt_ctx** out) {
     if (!out) return fail(PT_E_INVALID, "pt_ctx_create: out is null");
     int n = 0;
@@ -250,6 +282,7 @@ pt_status pt_ctx_create(int device_index, pt_ctx** out) {
     }
     c->trav.bytes = 2 * sizeof(unsigned long long);
     if (hipMalloc(&c->trav.p, c->trav.bytes) != hipSuccess) { delete c; return fail(PT_E_OOM, "counter allocation failed"); }
+    if (pt_status st = noalloc_setup(device_index, &c->noalloc_state); st != PT_OK) { delete c; return st; }
     *out = c;
     return PT_OK;
 }
@@ -776,44 +809,46 @@ void pt_renderer_unload(pt_renderer* r) {
 
 }  // extern "C"
 
+// The path state of n slots carved from one device block, each piece 256-byte aligned: the
+// queue counters and the ray / traversal statistics first, then the SoA arrays (ray, throughput,
+// radiance, rng, MIS state, flags, hit, two ray queues, the shadow queue, medium, wavelengths,
+// eta, the batch's scene ids, one queue per shading category and the guide samples the program
+// asks for). Returns the bytes needed; with a null base only the size is computed.
+constexpr size_t STATE_HEAD = STATE_HEAD_BYTES;
+static size_t carve_paths(const pt_program& prog, char* base, long long n, Paths& P) {
+    size_t off = 0;
+    auto take = [&](size_t bytes) -> void* {
+        void* p = base ? base + off : nullptr;
+        off += (bytes + 255) & ~(size_t)255;
+        return p;
+    };
+    const size_t m = (size_t)n;
+    P.counts = (int*)take(N_COUNTS * sizeof(int));
+    P.hist = (unsigned long long*)take(2 * sizeof(unsigned long long));
+    P.trav = (unsigned long long*)take(2 * sizeof(unsigned long long));
+    P.ray_o = (float4*)take(16 * m); P.ray_d = (float4*)take(16 * m); P.thr = (float4*)take(16 * m);
+    P.rad = (float4*)take(16 * m); P.rng = (uint2*)take(8 * m); P.prev = (float4*)take(16 * m);
+    P.flags = (int*)take(4 * m); P.hit = (float2*)take(8 * m); P.q[0] = (int*)take(4 * m); P.q[1] = (int*)take(4 * m);
+    P.s_path = (int*)take(4 * m); P.s_o = (float4*)take(16 * m); P.s_d = (float4*)take(16 * m);
+    P.s_c = (float4*)take(16 * m); P.s_l = (float4*)take(16 * m); P.medium = (int*)take(4 * m);
+    P.wl = (float4*)take(16 * m); P.eta = (float*)take(4 * m); P.path_scene = (int*)take(4 * m);
+    for (int c = 0; c < N_CAT; c++) P.catq[c] = (int*)take(4 * m);       // one queue per shading category
+    for (int a = 0; a < 2; a++) P.aov[a] = (prog.aovs & (1 << a)) ? (float4*)take(16 * m) : nullptr;
+    P.cap = (int)n;
+    P.npix = prog.width * prog.height;
+    return off;
+}
+
 static pt_status ensure_paths(pt_renderer* r, long long n) {
     if (r->cap >= n) return PT_OK;
-    struct Entry { DevBuf* b; size_t elem; };
-    std::vector<Entry> list = {
-        {&r->b_ray_o, 16}, {&r->b_ray_d, 16}, {&r->b_thr, 16}, {&r->b_rad, 16}, {&r->b_rng, 8}, {&r->b_prev, 16},
-        {&r->b_flags, 4}, {&r->b_hit, 8}, {&r->b_q0, 4}, {&r->b_q1, 4}, {&r->b_spath, 4}, {&r->b_so, 16},
-        {&r->b_sd, 16}, {&r->b_sc, 16}, {&r->b_sl, 16}, {&r->b_medium, 4}, {&r->b_wl, 16}, {&r->b_eta, 4}, {&r->b_pscene, 4}};
-    for (int c = 0; c < N_CAT; c++) list.push_back({&r->b_cat[c], 4});   // one queue per shading category
-    for (int a = 0; a < 2; a++)
-        if (r->prog.aovs & (1 << a)) list.push_back({&r->b_aov[a], 16});  // first-hit guide samples
-    for (auto& e : list) {
-        if (e.b->p) { HIP_TRY(hipFree(e.b->p)); e.b->p = nullptr; }
-        e.b->bytes = e.elem * (size_t)n;
-        HIP_TRY(hipMalloc(&e.b->p, e.b->bytes));
-    }
-    if (!r->b_counts.p) {
-        r->b_counts.bytes = N_COUNTS * sizeof(int);
-        HIP_TRY(hipMalloc(&r->b_counts.p, r->b_counts.bytes));
-        HIP_TRY(hipMemset(r->b_counts.p, 0, r->b_counts.bytes));
-    }
-    if (!r->b_hist.p) { r->b_hist.bytes = 2 * sizeof(unsigned long long); HIP_TRY(hipMalloc(&r->b_hist.p, r->b_hist.bytes)); }
-    if (!r->b_trav.p) { r->b_trav.bytes = 2 * sizeof(unsigned long long); HIP_TRY(hipMalloc(&r->b_trav.p, r->b_trav.bytes)); }
-    Paths& P = r->paths;
-    P.ray_o = (float4*)r->b_ray_o.p; P.ray_d = (float4*)r->b_ray_d.p; P.thr = (float4*)r->b_thr.p;
-    P.rad = (float4*)r->b_rad.p; P.rng = (uint2*)r->b_rng.p; P.prev = (float4*)r->b_prev.p;
-    P.flags = (int*)r->b_flags.p; P.hit = (float2*)r->b_hit.p; P.q[0] = (int*)r->b_q0.p; P.q[1] = (int*)r->b_q1.p;
-    P.counts = (int*)r->b_counts.p; P.s_path = (int*)r->b_spath.p; P.s_o = (float4*)r->b_so.p;
-    P.s_d = (float4*)r->b_sd.p; P.s_c = (float4*)r->b_sc.p; P.s_l = (float4*)r->b_sl.p;
-    P.medium = (int*)r->b_medium.p; P.wl = (float4*)r->b_wl.p; P.eta = (float*)r->b_eta.p;
-    P.path_scene = (int*)r->b_pscene.p; P.hist = (unsigned long long*)r->b_hist.p;
-    P.trav = (unsigned long long*)r->b_trav.p;
-    for (int a = 0; a < 2; a++) P.aov[a] = (r->prog.aovs & (1 << a)) ? (float4*)r->b_aov[a].p : nullptr;
-    for (int c = 0; c < N_CAT; c++) {
-        P.catq[c] = (int*)r->b_cat[c].p;
-        if (!P.catq[c]) return fail(PT_E_OOM, "ensure_paths: a shading queue was not allocated");
-    }
-    P.cap = (int)n;
-    P.npix = r->prog.width * r->prog.height;
+    Paths P = r->paths;
+    const size_t bytes = carve_paths(r->prog, nullptr, n, P);
+    if (r->b_state.p) { HIP_TRY(hipFree(r->b_state.p)); r->b_state.p = nullptr; r->b_state.bytes = 0; }
+    HIP_TRY(hipMalloc(&r->b_state.p, bytes));
+    r->b_state.bytes = bytes;
+    carve_paths(r->prog, (char*)r->b_state.p, n, P);
+    HIP_TRY(hipMemset(r->b_state.p, 0, STATE_HEAD));
+    r->paths = P;
     r->cap = n;
     return PT_OK;
 }
@@ -861,16 +896,20 @@ static Camera make_camera(const pt_params* prm) {
 // Trace nframes samples (sampleCount sample0, sample0 + 1, ...) in waves of frames_in_flight.
 // The forward program accumulates into the film; the adjoint program (ADJ) replays the same
 // paths and reduces their gradients instead.
-static pt_status run_waves(pt_renderer* r, const pt_params* prm, uint32_t nframes, uint32_t sample0, bool adj) {
+static pt_status run_waves(pt_renderer* r, const pt_params* prm, uint32_t nframes, uint32_t sample0, bool adj,
+                           int max_frames = 0, bool clear_stats = true) {
     pt_ctx* ctx = r->ctx;
     const pt_program& pg = r->prog;
-    const int F = std::max(1, std::min<int>(pg.frames_in_flight, (int)std::max<uint32_t>(nframes, 1)));
+    int F = std::max(1, std::min<int>(pg.frames_in_flight, (int)std::max<uint32_t>(nframes, 1)));
+    if (max_frames > 0) F = std::min(F, max_frames);
     const bool spectral = pg.color == PT_COLOR_SPECTRAL;
     if (pt_status sc = check_program_scene(pg, ctx->dev, ctx->host_lights); sc != PT_OK) return sc;
     if (spectral && !ctx->dev.spec_curves)
         return fail(PT_E_MISSING_INPUT, "missing input: spectral_curves (the spectral program needs the CIE/D65 curves in pt_scene_desc)");
-    HIP_TRY(hipMemsetAsync(r->b_hist.p, 0, r->b_hist.bytes, ctx->stream));
-    HIP_TRY(hipMemsetAsync(r->b_trav.p, 0, r->b_trav.bytes, ctx->stream));
+    if (clear_stats) {
+        HIP_TRY(hipMemsetAsync(r->paths.hist, 0, 2 * sizeof(unsigned long long), ctx->stream));
+        HIP_TRY(hipMemsetAsync(r->paths.trav, 0, 2 * sizeof(unsigned long long), ctx->stream));
+    }
     const Camera cam = make_camera(prm);
     uint32_t done = 0;
     while (done < nframes) {
@@ -914,6 +953,41 @@ pt_status pt_render_frames(pt_renderer* r, const pt_params* prm, uint32_t nframe
     pt_status st = ensure_paths(r, (long long)pg.width * pg.height * F);
     if (st != PT_OK) return st;
     st = run_waves(r, prm, nframes, r->sample_count, false);
+    if (st != PT_OK) return st;
+    r->sample_count += nframes;
+    return PT_OK;
+}
+
+pt_status pt_render_frames_noalloc(pt_renderer* r, const pt_params* prm, uint32_t nframes) {
+    if (!r || !prm) return fail(PT_E_INVALID, "pt_render_frames_noalloc: null argument");
+    pt_ctx* ctx = r->ctx;
+    if (!ctx->has_scene) return fail(PT_E_MISSING_INPUT, "missing input: no scene uploaded (pt_scene_upload)");
+    if (pt_status sc = check_camera(prm); sc != PT_OK) return sc;
+    const pt_program& pg = r->prog;
+    const long long npix = (long long)pg.width * pg.height;
+    if (npix > PT_NOALLOC_MAX_PATHS)
+        return fail(PT_E_INVALID, "pt_render_frames_noalloc: the film has " + std::to_string(npix) +
+                                      " pixels; the allocation-free path state holds " + std::to_string(PT_NOALLOC_MAX_PATHS));
+    if (!ctx->noalloc_state) return fail(PT_E_DEVICE, "pt_render_frames_noalloc: no allocation-free path state on this device");
+    const int F = (int)std::min<long long>(std::max(1, pg.frames_in_flight), PT_NOALLOC_MAX_PATHS / npix);
+    Paths P = r->paths;
+    if (carve_paths(pg, nullptr, npix * F, P) > NOALLOC_BYTES)
+        return fail(PT_E_INVALID, "pt_render_frames_noalloc: the path state exceeds the allocation-free array");
+    carve_paths(pg, (char*)ctx->noalloc_state, npix * F, P);
+    HIP_TRY(hipSetDevice(ctx->device));
+    // counters and ray statistics start at zero: a fill kernel, so the call may be captured
+    hipLaunchKernelGGL(k_fill_f4, dim3(1), dim3(SBLOCK), 0, ctx->stream, (float4*)ctx->noalloc_state,
+                       (long long)(STATE_HEAD / sizeof(float4)));
+    const Paths own = r->paths;
+    const long long own_cap = r->cap;
+    const bool timing = ctx->timing;
+    r->paths = P;
+    r->cap = npix * F;
+    ctx->timing = false;                  // no event objects inside the call
+    pt_status st = run_waves(r, prm, nframes, r->sample_count, false, F, false);
+    ctx->timing = timing;
+    r->paths = own;
+    r->cap = own_cap;
     if (st != PT_OK) return st;
     r->sample_count += nframes;
     return PT_OK;
@@ -1425,8 +1499,8 @@ int pt_get_timings(pt_ctx* ctx, pt_kernel_timing* out, int n_slots) {
 pt_status pt_last_ray_counts(pt_renderer* r, uint64_t* closest, uint64_t* any) {
     if (!r) return fail(PT_E_INVALID, "pt_last_ray_counts: null renderer");
     unsigned long long h[2] = {0, 0};
-    if (r->b_hist.p) {
-        HIP_TRY(hipMemcpyAsync(h, r->b_hist.p, sizeof h, hipMemcpyDeviceToHost, r->ctx->stream));
+    if (r->b_state.p) {
+        HIP_TRY(hipMemcpyAsync(h, r->paths.hist, sizeof h, hipMemcpyDeviceToHost, r->ctx->stream));
         HIP_TRY(hipStreamSynchronize(r->ctx->stream));
     }
     if (closest) *closest = h[0];
@@ -1445,8 +1519,8 @@ pt_status pt_renderer_set_traversal_counting(pt_renderer* r, int enabled) {
 pt_status pt_last_traversal_counts(pt_renderer* r, uint64_t* nodes, uint64_t* tris) {
     if (!r) return fail(PT_E_INVALID, "null renderer");
     unsigned long long h[2] = {0, 0};
-    if (r->b_trav.p) {
-        HIP_TRY(hipMemcpyAsync(h, r->b_trav.p, sizeof h, hipMemcpyDeviceToHost, r->ctx->stream));
+    if (r->b_state.p) {
+        HIP_TRY(hipMemcpyAsync(h, r->paths.trav, sizeof h, hipMemcpyDeviceToHost, r->ctx->stream));
         HIP_TRY(hipStreamSynchronize(r->ctx->stream));
     }
     *nodes = h[0]; *tris = h[1];
