@@ -80,12 +80,11 @@ struct TraceCounters { uint32_t nodes, tris; };
 
 PT_DEV float byte_of(uint32_t w, int k) { return (float)((w >> (8 * k)) & 0xFFu); }
 
-#ifndef PTMI_OCTANT_SLAB
-#define PTMI_OCTANT_SLAB 1
-#endif
-#ifndef PTMI_BRANCHLESS_CHILDREN
-#define PTMI_BRANCHLESS_CHILDREN 1
-#endif
+// The slab bound: 6 u relative to |oq| + 255 |dq| (u = 2^-24), and an absolute floor for the
+// subnormal range of dq.
+constexpr float SLAB_REL = 6.0f * 5.9604644775390625e-8f;
+constexpr float SLAB_ABS = 1e-30f;
+
 
 // Per-lane traversal state of the wide-BVH walk, so a persistent kernel can interleave
 // rays: a lane that finishes picks up the next ray while its wave-mates keep walking.
@@ -148,61 +147,60 @@ PT_DEV bool trace_step(const uint4* __restrict__ nodes, const float4* __restrict
         const float sz = __uint_as_float(((w0.w >> 16) & 0xFFu) << 23);
         const uint32_t nimask = w0.w >> 24;
         const f3 o = st.o;
-        uint32_t nhits = 0, tri_hits = 0;
-#if PTMI_OCTANT_SLAB
-        // Near / far planes chosen once per node by the ray's octant. Decoding and the slab
-        // distance (x - o) * inv are monotone in the quantized plane (lo <= hi), so for inv > 0
-        // min(t_lo, t_hi) is t_lo exactly (t_hi for inv < 0): the same tn and tf, bit for bit,
-        // as the per-axis min / max of the reference's slab test, with 48 fewer VALU ops.
+        // The ray in the node's quantized frame, once per node: a child plane q (0..255) lies at
+        // t = q * dq + oq with dq = 2^e * inv (exact: a power-of-two scale) and oq = (p - o) * inv,
+        // one fused multiply-add per plane. Against the exact slab distance (p + q 2^e - o) / d
+        // these roundings err by at most 5 u (|oq| + 255 |dq|) (u = 2^-24: the subtraction, the
+        // two products, the rounded reciprocal, the final rounding); near planes move down and far
+        // planes up by slab_err >= 6 u (|oq| + 255 |dq|), so a child is never culled where the
+        // exact slab test of its quantized box would keep it. The quantized boxes contain the
+        // triangles' boxes padded by the Moller-Trumbore tolerance, so every walk returns the
+        // brute-force closest hit; the margin also covers the hit distance's own rounding
+        // against the current tmax.
+        const float dqx = sx * st.ix, dqy = sy * st.iy, dqz = sz * st.iz;
+        const float oqx = (px - o.x) * st.ix, oqy = (py - o.y) * st.iy, oqz = (pz - o.z) * st.iz;
+        const float ex = __builtin_fmaf(255.0f, fabsf(dqx), fabsf(oqx)) * SLAB_REL + SLAB_ABS;
+        const float ey = __builtin_fmaf(255.0f, fabsf(dqy), fabsf(oqy)) * SLAB_REL + SLAB_ABS;
+        const float ez = __builtin_fmaf(255.0f, fabsf(dqz), fabsf(oqz)) * SLAB_REL + SLAB_ABS;
+        const float onx = oqx - ex, ofx = oqx + ex, ony = oqy - ey, ofy = oqy + ey, onz = oqz - ez, ofz = oqz + ez;
+        // Near / far planes chosen once per node by the ray's octant: for inv > 0 the near plane of
+        // an axis is the low one (the high one for inv < 0).
         const bool nxs = st.ix < 0.0f, nys = st.iy < 0.0f, nzs = st.iz < 0.0f;
         const uint32_t nx0 = nxs ? w3.z : w2.x, nx1 = nxs ? w3.w : w2.y, fx0 = nxs ? w2.x : w3.z, fx1 = nxs ? w2.y : w3.w;
         const uint32_t ny0 = nys ? w4.x : w2.z, ny1 = nys ? w4.y : w2.w, fy0 = nys ? w2.z : w4.x, fy1 = nys ? w2.w : w4.y;
         const uint32_t nz0 = nzs ? w4.z : w3.x, nz1 = nzs ? w4.w : w3.y, fz0 = nzs ? w3.x : w4.z, fz1 = nzs ? w3.y : w4.w;
-#endif
+        // The children whose boxes the ray enters before tmax, as a slot mask. An empty slot needs
+        // no test of its own: its imask bit is 0 and its meta byte 0 (no triangles), so it adds
+        // nothing below.
+        uint32_t hit = 0, tri_hits = 0;
 #pragma unroll
         for (int s = 0; s < 8; s++) {
-            const uint32_t mw = s < 4 ? w1.z : w1.w;
-            const uint32_t meta = (mw >> (8 * (s & 3))) & 0xFFu;
             const int b = s & 3;
-#if PTMI_OCTANT_SLAB
-            // decode: p + q * 2^e. q * 2^e is exact, so one fused multiply-add rounds exactly
-            // like the builder's separate multiply and add.
-            const float nx = __builtin_fmaf(byte_of(s < 4 ? nx0 : nx1, b), sx, px), fx = __builtin_fmaf(byte_of(s < 4 ? fx0 : fx1, b), sx, px);
-            const float ny = __builtin_fmaf(byte_of(s < 4 ? ny0 : ny1, b), sy, py), fy = __builtin_fmaf(byte_of(s < 4 ? fy0 : fy1, b), sy, py);
-            const float nz = __builtin_fmaf(byte_of(s < 4 ? nz0 : nz1, b), sz, pz), fz = __builtin_fmaf(byte_of(s < 4 ? fz0 : fz1, b), sz, pz);
-            const float tn = SMAX(SMAX((nx - o.x) * st.ix, (ny - o.y) * st.iy), (nz - o.z) * st.iz);
-            const float tf = SMIN(SMIN((fx - o.x) * st.ix, (fy - o.y) * st.iy), (fz - o.z) * st.iz) * BVH_TFAR_PAD;
-#else
-            const uint32_t lxw = s < 4 ? w2.x : w2.y, lyw = s < 4 ? w2.z : w2.w;
-            const uint32_t lzw = s < 4 ? w3.x : w3.y, hxw = s < 4 ? w3.z : w3.w;
-            const uint32_t hyw = s < 4 ? w4.x : w4.y, hzw = s < 4 ? w4.z : w4.w;
-            // decode: p + q * 2^e. q * 2^e is exact, so one fused multiply-add rounds exactly
-            // like the builder's separate multiply and add.
-            const float lox = __builtin_fmaf(byte_of(lxw, b), sx, px), hix = __builtin_fmaf(byte_of(hxw, b), sx, px);
-            const float loy = __builtin_fmaf(byte_of(lyw, b), sy, py), hiy = __builtin_fmaf(byte_of(hyw, b), sy, py);
-            const float loz = __builtin_fmaf(byte_of(lzw, b), sz, pz), hiz = __builtin_fmaf(byte_of(hzw, b), sz, pz);
-            const float t0x = (lox - o.x) * st.ix, t1x = (hix - o.x) * st.ix;
-            const float t0y = (loy - o.y) * st.iy, t1y = (hiy - o.y) * st.iy;
-            const float t0z = (loz - o.z) * st.iz, t1z = (hiz - o.z) * st.iz;
-            // slab test: the t values are finite (guarded inverse), so IEEE min/max equal GLSL's
-            // and every comparison below sees the same values
-            const float tn = SMAX(SMAX(SMIN(t0x, t1x), SMIN(t0y, t1y)), SMIN(t0z, t1z));
-            const float tf = SMIN(SMIN(SMAX(t0x, t1x), SMAX(t0y, t1y)), SMAX(t0z, t1z)) * BVH_TFAR_PAD;
-#endif
-            const float te = SMAX(tn, 0.0f);
-            if (PTMI_BRANCHLESS_CHILDREN && !ANY) {
-                // closest hit: branch-free classification (selects, no exec-mask region per
-                // child; measured faster). Any-hit walks keep the branches, which measured
-                // faster there. An empty slot (meta 0) or a missed box contributes nothing.
-                const uint32_t ok = (uint32_t)(meta != 0u) & (uint32_t)(tf >= te) & (uint32_t)(te <= st.tmax);
-                const uint32_t inner = (nimask >> s) & 1u;
-                nhits |= (ok & inner) << (s ^ st.oct_inv);
-                tri_hits |= (ok & (inner ^ 1u)) ? ((meta >> 5) & 7u) << (meta & 31u) : 0u;
-            } else if (meta != 0u && tf >= te && te <= st.tmax) {
-                if (nimask & (1u << s)) nhits |= 1u << (s ^ st.oct_inv);
-                else tri_hits |= ((meta >> 5) & 7u) << (meta & 31u);
-            }
+            const float tnx = __builtin_fmaf(byte_of(s < 4 ? nx0 : nx1, b), dqx, onx);
+            const float tny = __builtin_fmaf(byte_of(s < 4 ? ny0 : ny1, b), dqy, ony);
+            const float tnz = __builtin_fmaf(byte_of(s < 4 ? nz0 : nz1, b), dqz, onz);
+            const float tfx = __builtin_fmaf(byte_of(s < 4 ? fx0 : fx1, b), dqx, ofx);
+            const float tfy = __builtin_fmaf(byte_of(s < 4 ? fy0 : fy1, b), dqy, ofy);
+            const float tfz = __builtin_fmaf(byte_of(s < 4 ? fz0 : fz1, b), dqz, ofz);
+            const float te = SMAX(SMAX(SMAX(tnx, tny), tnz), 0.0f);
+            const float tf = SMIN(SMIN(SMIN(tfx, tfy), tfz), st.tmax);
+            hit |= te <= tf ? (1u << s) : 0u;
         }
+        // leaf children: their triangles as bits relative to the node's triangle base
+        const uint32_t leaf = hit & ~nimask;
+#pragma unroll
+        for (int s = 0; s < 8; s++) {
+            const uint32_t meta = ((s < 4 ? w1.z : w1.w) >> (8 * (s & 3))) & 0xFFu;
+            tri_hits |= ((leaf >> s) & 1u) ? ((meta >> 5) & 7u) << (meta & 31u) : 0u;
+        }
+        // internal children, reordered front to back: slot s moves to bit s ^ (octant ^ 7)
+        uint32_t inner = hit & nimask;
+        const uint32_t c = st.oct_inv;
+        if (c & 1u) inner = ((inner & 0x55u) << 1) | ((inner >> 1) & 0x55u);
+        if (c & 2u) inner = ((inner & 0x33u) << 2) | ((inner >> 2) & 0x33u);
+        if (c & 4u) inner = ((inner & 0x0Fu) << 4) | ((inner >> 4) & 0x0Fu);
+        const uint32_t nhits = inner;
+        (void)ANY;
         st.g_base = w1.x;
         st.g_bits = (nimask << 8) | nhits;
         st.t_base = w1.y;
