@@ -2280,6 +2280,36 @@ static void render_impl(const or_scene* S, const pt_program* prog, const pt_para
 }
 
 /* Declared-arithmetic exports, so tests can pin them against correctly rounded math. */
+/* The device's wide-BVH slab test (pathtracerglsl_amd/csrc/traversal.h trace_step) restated in
+ * fp32 for the bound check (test infrastructure; the oracle's own walks use the exact binary
+ * boxes): for ray k and a quantized child box (origin p, exponents e, planes qlo / qhi per
+ * axis), out[2k] = the entry distance max(near planes, 0) and out[2k+1] = the exit distance
+ * min(far planes), each plane q * dq + (oq -+ err) with dq = 2^e / d, oq = (p - o) / d and
+ * err = 6u (|oq| + 255 |dq|) + 1e-30. rays: o.xyz, d.xyz per ray; boxes: p.xyz, e.xyz (as floats),
+ * qlo.xyz, qhi.xyz per ray. */
+void or_slab_probe(const float* rays, const float* boxes, int n, float* out) {
+    const float REL = 6.0f * 5.9604644775390625e-8f, ABS = 1e-30f;
+    for (int k = 0; k < n; k++) {
+        const float* r = rays + 6 * (size_t)k;
+        const float* b = boxes + 12 * (size_t)k;
+        float te = 0.0f, tf = 3.0e38f;
+        for (int a = 0; a < 3; a++) {
+            float d = r[3 + a];
+            d = fabsf(d) < 1e-20f ? copysignf(1e-20f, d) : d;
+            const float inv = 1.0f / d;
+            const float s = ldexpf(1.0f, (int)b[3 + a]);
+            const float dq = s * inv, oq = (b[a] - r[a]) * inv;
+            const float e = fmaf(255.0f, fabsf(dq), fabsf(oq)) * REL + ABS;
+            const float qn = inv < 0.0f ? b[9 + a] : b[6 + a], qf = inv < 0.0f ? b[6 + a] : b[9 + a];
+            const float tn = fmaf(qn, dq, oq - e), tfa = fmaf(qf, dq, oq + e);
+            te = fmaxf(te, tn);
+            tf = fminf(tf, tfa);
+        }
+        out[2 * (size_t)k] = te;
+        out[2 * (size_t)k + 1] = tf;
+    }
+}
+
 void or_math_eval(int fn, const float* x, const float* y, float* out, int n) {
     for (int i = 0; i < n; i++) {
         switch (fn) {
