@@ -506,6 +506,77 @@ typedef struct pt_inst_table {
  * PT_E_INVALID with the reference's "inst: ..." message. Nothing is copied. */
 pt_status pt_inst_parse(const void* buffer, size_t len, pt_inst_table* out);
 
+/* ---------------------------------------------------------------- scene-data rail
+ * The App's data packing run natively (app/sceneData.ts over components/accel/bvh/bvh.ts and
+ * components/data/ledger.ts): the reference's BVH builder byte for byte, its data ledger and
+ * the mesh channels written at the ledger's bases. Host-only: no device and no context. */
+
+/* buildBVHNodesFlat (components/accel/bvh/bvh.ts:323-360 over buildBVHCore :116-320): the binned
+ * SAH binary BVH over n boxes (6 f64 each: min.xyz, max.xyz), leaves of <= leaf_size items,
+ * byte for byte (nodes as 8 f32: min.xyz, A, max.xyz, B; A >= 0 leaf of A items at offset B of
+ * order, A < 0 internal with axis -A-1, right child B, left child next). PT_BVH_ALWAYS_SPLIT
+ * splits above leaf_size even where SAH would stop; PT_BVH_SORTED_MEDIAN takes buildBVHCore's
+ * sorted-median fallback instead of the median. nodes_out holds 8 (2n - 1) floats, order_out n. */
+enum { PT_BVH_ALWAYS_SPLIT = 1, PT_BVH_SORTED_MEDIAN = 2 };
+pt_status pt_bvh_build(const double* boxes, int32_t n, int32_t leaf_size, uint32_t flags, float* nodes_out,
+                       uint32_t* order_out, int32_t* node_count, int32_t* max_depth);
+/* buildBVH (bvh.ts:444-470): a mesh BLAS over n_tris triangles (3 indices each into n_vertices
+ * positions of 3 f32), leaf size 2; nodes as pt_bvh_build, reindexed_out the 3 n_tris indices
+ * in leaf order. */
+pt_status pt_mesh_bvh_build(const float* positions, int32_t n_vertices, const uint32_t* indices, int32_t n_tris,
+                            float* nodes_out, uint32_t* reindexed_out, int32_t* node_count, int32_t* max_depth);
+
+/* DataTenants and DataLayout (components/data/ledger.ts:47-114). A region base is a texel index
+ * in its channel; -1 where the structure is absent. */
+typedef struct pt_mesh_tenant { int32_t vertex_count, tri_count; } pt_mesh_tenant;
+typedef struct pt_batch_tenant {
+    int32_t instance_count, placement_texels;   /* placement_texels: 2 rigid frame, 1 folded params */
+    int32_t attr_texels, cwbvh_node_texels, cwbvh_record_texels;
+} pt_batch_tenant;
+typedef struct pt_mesh_light_tenant { int32_t mesh_ordinal, vertex_count, tri_count; } pt_mesh_light_tenant;
+typedef struct pt_data_tenants {
+    int32_t n_meshes; const pt_mesh_tenant* meshes;
+    int32_t n_batches; const pt_batch_tenant* batches;
+    int32_t n_mesh_lights; const pt_mesh_light_tenant* mesh_lights;
+    int32_t has_scene_table, table_leaf_count, table_analytic_texels;
+    int32_t has_light_tree, light_count, light_table_texels;
+    int32_t has_region_materials, region_count;
+} pt_data_tenants;
+typedef struct pt_mesh_slot { int32_t vbase, tbase, nbase; } pt_mesh_slot;
+typedef struct pt_batch_slot { int32_t placements_base, tlas_base, attrs_base, cwbvh_nodes_base, cwbvh_records_base; } pt_batch_slot;
+typedef struct pt_mesh_light_slot { int32_t mesh_ordinal, wpos_base, cdf_base; } pt_mesh_light_slot;
+typedef struct pt_data_layout {
+    int32_t total_vertices, total_normals, total_uvs, total_indices, total_nodes, total_records, total_nodesq;
+    pt_mesh_slot* meshes;             /* caller arrays the planner fills: n_meshes, */
+    pt_batch_slot* batches;           /* n_batches and */
+    pt_mesh_light_slot* mesh_lights;  /* n_mesh_lights entries (tenant order) */
+    int32_t table_leaf_list_base, table_analytic_base, table_tlas_base;
+    int32_t light_table_base, light_trails_base, light_tree_base;
+    int32_t region_materials_base;
+} pt_data_layout;
+/* planDataLayout (ledger.ts:122-184): every region from counts alone, always-built regions
+ * before optional ones. */
+pt_status pt_plan_data_layout(const pt_data_tenants* tenants, pt_data_layout* layout);
+
+/* One mesh of the packing (MeshObject): positions (3 f32 per vertex), triangle indices, and
+ * per-vertex normals (3 f32) / uvs (2 f32) or NULL. */
+typedef struct pt_mesh_view {
+    const float* positions; int32_t n_vertices;
+    const uint32_t* indices; int32_t n_tris;
+    const float* normals;
+    const float* uvs;
+} pt_mesh_view;
+/* packSceneData's geometry pass (app/sceneData.ts:36-47, components/data/pack.ts): for mesh slot
+ * s, its positions at layout.meshes[s].vbase of the vertices channel (xyz, w 0), normals and uvs
+ * likewise when given, its BLAS's reindexed triangles at tbase of the indices channel (exact
+ * floats, w 0) and its nodes at nbase of the nodes channel. Channels are caller RGBA32F arrays
+ * of 4 x the layout's totals (zero-filled by the caller); a mesh shared by several slots builds
+ * once; a BLAS that outgrows its declared region fails with PT_E_INVALID (assertFits). root_boxes
+ * (6 f32 per mesh, or NULL) receive each BLAS's root box (the prototype box of an instance batch). */
+pt_status pt_pack_mesh_channels(const pt_data_layout* layout, int32_t n_meshes, const pt_mesh_view* meshes,
+                                float* vertices, float* normals, float* uvs, float* indices, float* nodes,
+                                float* root_boxes);
+
 #ifdef __cplusplus
 }
 #endif

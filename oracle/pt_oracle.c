@@ -253,6 +253,19 @@ static void bvh_build(bvh_t* out, const double* boxes, int n, int leaf_size) {
 }
 static void bvh_free(bvh_t* b) { free(b->nodes); free(b->order); memset(b, 0, sizeof *b); }
 
+/* The restated builder on its own (test infrastructure: the library's byte-faithful builder is
+ * pinned against it): nodes (8 f32 each, capacity 8 (2n - 1)) and the item order. Returns the
+ * node count. */
+int or_bvh_build(const double* boxes, int n, int leaf_size, float* nodes, uint32_t* order) {
+    bvh_t b;
+    bvh_build(&b, boxes, n, leaf_size);
+    if (b.n_nodes > 0) memcpy(nodes, b.nodes, sizeof(float) * 8 * (size_t)b.n_nodes);
+    if (n > 0) memcpy(order, b.order, sizeof(uint32_t) * (size_t)n);
+    int nn = b.n_nodes;
+    bvh_free(&b);
+    return nn;
+}
+
 /* Inverse ray direction for the slab tests. The BLAS walk (mesh.glsl:167) takes 1/rd, which
  * for an exactly axis-aligned ray gives 0*inf = NaN in the slab test (undefined in GLSL);
  * declared rule: the CWBVH walk's guard |d| < 1e-20 -> +-1e-20 (accel/cwbvh, cwbvh.ts). */

@@ -26,7 +26,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contr
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall",
              "-Wno-unused-function"]
 
-CXX_SOURCES = ["builder.cpp", "description.cpp", "formats.cpp"]
+CXX_SOURCES = ["builder.cpp", "description.cpp", "formats.cpp", "rail.cpp"]
 HIP_SOURCES = ["ptmi.hip", "wave_rgb.hip", "wave_spectral.hip", "wave_media.hip", "wave_media_spectral.hip",
                "wave_adjoint.hip", "wave_adjoint_media.hip", "batch_kernels.hip"]
 

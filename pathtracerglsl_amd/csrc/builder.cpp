@@ -1,5 +1,6 @@
 // builder.cpp — wide BVH, binary BVH and env-table builders (host, C++17).
 #include "builder.h"
+#include "rail.h"
 
 #include <algorithm>
 #include <cmath>
@@ -383,109 +384,12 @@ void build_wide_bvh(const float* tv, int n_tris, WideBVH& out) {
 }
 
 // ---------------------------------------------------------------- binary BVH (containment)
-namespace {
-struct BinBuild {
-    const double* boxes;
-    std::vector<float> cx, cy, cz;
-    BinaryBVH* out;
-    int leaf;
-
-    int emit(int s, int e) {
-        int node = (int)(out->nodes.size() / 8);
-        out->nodes.resize(out->nodes.size() + 8);
-        double b[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
-        double c[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
-        for (int i = s; i < e; i++) {
-            uint32_t t = out->order[i];
-            for (int a = 0; a < 3; a++) {
-                b[a] = std::min(b[a], boxes[6 * (size_t)t + a]);
-                b[3 + a] = std::max(b[3 + a], boxes[6 * (size_t)t + 3 + a]);
-            }
-            double cc[3] = {cx[t], cy[t], cz[t]};
-            for (int a = 0; a < 3; a++) { c[a] = std::min(c[a], cc[a]); c[3 + a] = std::max(c[3 + a], cc[a]); }
-        }
-        auto set = [&](float A, float B) {
-            float* nd = &out->nodes[8 * (size_t)node];
-            nd[0] = (float)b[0]; nd[1] = (float)b[1]; nd[2] = (float)b[2]; nd[3] = A;
-            nd[4] = (float)b[3]; nd[5] = (float)b[4]; nd[6] = (float)b[5]; nd[7] = B;
-        };
-        int count = e - s;
-        if (count <= leaf) { set((float)count, (float)s); return node; }
-        constexpr int NB = 12;
-        int ba = -1, bs = -1; double bc = INFINITY;
-        for (int a = 0; a < 3; a++) {
-            double ext = c[3 + a] - c[a];
-            if (!(ext >= 1e-12)) continue;
-            double scale = NB / ext;
-            double bb[NB][6]; int cnt[NB];
-            for (int k = 0; k < NB; k++) { bb[k][0] = bb[k][1] = bb[k][2] = INFINITY; bb[k][3] = bb[k][4] = bb[k][5] = -INFINITY; cnt[k] = 0; }
-            const std::vector<float>& cen = a == 0 ? cx : a == 1 ? cy : cz;
-            for (int i = s; i < e; i++) {
-                uint32_t t = out->order[i];
-                int k = (int)std::floor((cen[t] - c[a]) * scale);
-                k = std::min(std::max(k, 0), NB - 1);
-                cnt[k]++;
-                for (int x = 0; x < 3; x++) {
-                    bb[k][x] = std::min(bb[k][x], boxes[6 * (size_t)t + x]);
-                    bb[k][3 + x] = std::max(bb[k][3 + x], boxes[6 * (size_t)t + 3 + x]);
-                }
-            }
-            auto area = [](const double* q) {
-                double dx = q[3] - q[0], dy = q[4] - q[1], dz = q[5] - q[2];
-                return (dx < 0 || dy < 0 || dz < 0) ? 0.0 : 2 * (dx * dy + dy * dz + dz * dx);
-            };
-            double acc[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
-            float la[NB - 1]; int lc[NB - 1]; int n = 0;
-            for (int k = 0; k < NB - 1; k++) {
-                for (int x = 0; x < 3; x++) { acc[x] = std::min(acc[x], bb[k][x]); acc[3 + x] = std::max(acc[3 + x], bb[k][3 + x]); }
-                n += cnt[k]; la[k] = (float)area(acc); lc[k] = n;
-            }
-            double acr[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY}; n = 0;
-            for (int k = NB - 1; k > 0; k--) {
-                for (int x = 0; x < 3; x++) { acr[x] = std::min(acr[x], bb[k][x]); acr[3 + x] = std::max(acr[3 + x], bb[k][3 + x]); }
-                n += cnt[k];
-                if (lc[k - 1] == 0 || n == 0) continue;
-                double cost = (double)la[k - 1] * lc[k - 1] + (double)(float)area(acr) * n;
-                if (cost < bc) { bc = cost; ba = a; bs = k - 1; }
-            }
-        }
-        double bxa[6] = {b[0], b[1], b[2], b[3], b[4], b[5]};
-        double dx = bxa[3] - bxa[0], dy = bxa[4] - bxa[1], dz = bxa[5] - bxa[2];
-        double leaf_cost = 2 * (dx * dy + dy * dz + dz * dx) * count;
-        if (ba < 0 || bc >= leaf_cost) { set((float)count, (float)s); return node; }
-        const std::vector<float>& cen = ba == 0 ? cx : ba == 1 ? cy : cz;
-        double scale = NB / (c[3 + ba] - c[ba]);
-        auto it = std::partition(out->order.begin() + s, out->order.begin() + e, [&](uint32_t t) {
-            int k = (int)std::floor((cen[t] - c[ba]) * scale);
-            k = std::min(std::max(k, 0), NB - 1);
-            return k <= bs;
-        });
-        int mid = (int)(it - out->order.begin());
-        if (mid <= s || mid >= e) mid = (s + e) / 2;
-        emit(s, mid);
-        int right = emit(mid, e);
-        float* nd = &out->nodes[8 * (size_t)node];
-        nd[0] = (float)b[0]; nd[1] = (float)b[1]; nd[2] = (float)b[2]; nd[3] = (float)(-(ba + 1));
-        nd[4] = (float)b[3]; nd[5] = (float)b[4]; nd[6] = (float)b[5]; nd[7] = (float)right;
-        return node;
-    }
-};
-}  // namespace
-
+// The reference's own builder (buildBVHNodesFlat, median fallback; rail.cpp), byte for byte.
 void build_binary_bvh(const double* boxes, int n, int leaf_size, BinaryBVH& out) {
-    out.nodes.clear();
-    out.order.resize((size_t)std::max(n, 0));
-    if (n <= 0) return;
-    BinBuild B;
-    B.boxes = boxes; B.out = &out; B.leaf = leaf_size;
-    B.cx.resize(n); B.cy.resize(n); B.cz.resize(n);
-    for (int t = 0; t < n; t++) {
-        B.cx[t] = (float)((boxes[6 * (size_t)t] + boxes[6 * (size_t)t + 3]) * 0.5);
-        B.cy[t] = (float)((boxes[6 * (size_t)t + 1] + boxes[6 * (size_t)t + 4]) * 0.5);
-        B.cz[t] = (float)((boxes[6 * (size_t)t + 2] + boxes[6 * (size_t)t + 5]) * 0.5);
-        out.order[t] = (uint32_t)t;
-    }
-    B.emit(0, n);
+    FlatBVH b;
+    build_bvh_flat(boxes, n, leaf_size, false, false, b);
+    out.nodes = std::move(b.nodes);
+    out.order = std::move(b.order);
 }
 
 // ---------------------------------------------------------------- env importance table

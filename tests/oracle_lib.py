@@ -48,6 +48,8 @@ def load(variant=""):
     lib.or_basic_eval.argtypes = [C.c_int, vp, vp, vp, C.c_int]
     lib.or_adjoint.argtypes = [vp, C.POINTER(abi.PtProgram), C.POINTER(abi.PtParams), C.c_int,
                                C.POINTER(abi.PtAdjointParams), vp, vp]
+    lib.or_bvh_build.argtypes = [vp, C.c_int, C.c_int, vp, vp]
+    lib.or_bvh_build.restype = C.c_int
     lib.or_slab_probe.argtypes = [vp, vp, C.c_int, vp]
     lib.or_slab_probe.restype = None
     lib.or_set_proposal.argtypes = [C.c_int, vp, vp, C.c_float]
