@@ -4,35 +4,24 @@
 - spectral_curves.f32: the 128 x 2 RGBA32F curves texture (CIE 1931 X, Y, Z and D65 at 5 nm from
   360 nm; z nodes of the coefficient atlas), byte-identical to
   src/components/color/spectral/data/curves.bin (SHA-256 pinned in spectral.py).
-- spectral_lift_cache.json: sigmoid-quadratic coefficients, looked up in
-  src/components/color/spectral/data/coefficients.bin by spectral.spectral_coefficients, for the
-  bounded RGB inputs of the colours the bundled scenes and witnesses use.
+- spectral_atlas.zlib: the Jakob-Hanika coefficient atlas src/components/color/spectral/data/
+  coefficients.bin (3 x 64^3 RGBA32F texels whose w is always 0) stored as its three used
+  float32 channels, byte planes shuffled (plane k holds byte k of every float) and deflated;
+  spectral.atlas() inflates it and checks the reference's SHA-256 of the RGBA bytes
+  (src/components/color/spectral/data.ts).
 
 Usage: python3 pathtracerglsl_amd/data/make_spectral_data.py [/root/reference]
 """
-import json
 import os
 import shutil
 import sys
+
+import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 
 from pathtracerglsl_amd import spectral  # noqa: E402
-
-# (value, role) of every authored colour the bundled scenes / witness checks lift
-COLOURS = [
-    ((0.63, 0.065, 0.05), "reflectance"), ((0.14, 0.45, 0.09), "reflectance"),      # Cornell red / green
-    ((17.0, 12.0, 4.0), "emission"), ((12.0, 12.0, 12.0), "emission"),
-    ((0.72, 1.08, 1.8), "coefficient"), ((0.08, 0.12, 0.2), "coefficient"),         # glass-medium sigma_s / sigma_a
-    ((0.2, 0.5, 0.8), "reflectance"), ((0.1, 0.5, 0.8), "reflectance"),             # spectral witnesses
-    ((0.3, 0.7, 1.2), "coefficient"), ((0.1, 0.3, 0.8), "coefficient"),
-    ((0.1, 0.2, 0.4), "coefficient"), ((0.5, 1.0, 2.0), "coefficient"),
-    ((1.0, 2.0, 4.0), "coefficient"), ((0.9, 0.9, 0.9), "reflectance"),
-    ((0.4, 0.8, 1.2), "emission"), ((1.0, 2.0, 4.0), "emission"),                   # medium emission witnesses
-    ((0.8, 0.4, 0.2), "emission"), ((0.02, 0.05, 0.1), "coefficient"), ((0.2, 0.4, 0.7), "coefficient"),
-]
-
 
 def main():
     ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
@@ -41,16 +30,10 @@ def main():
     spectral._curves = None
     spectral.curves_texture()                       # verifies the SHA-256
     atlas = spectral.load_atlas(os.path.join(d, "coefficients.bin"))
-    cache = {}
-    for value, role in COLOURS:
-        _, b = spectral.bounded_input(value, role)
-        if b is None or (b[0] == b[1] and b[1] == b[2]):
-            continue
-        c = spectral.spectral_coefficients(b, atlas)
-        cache[spectral._key(b)] = [f"{spectral.np.float32(v).view(spectral.np.uint32):08x}" for v in c]
-    json.dump(cache, open(spectral.CACHE_PATH, "w"), indent=1, sort_keys=True)
-    print(f"wrote {spectral.CURVES_PATH} and {len(cache)} lifts to {spectral.CACHE_PATH}")
-
+    spectral.write_atlas(atlas, spectral.ATLAS_PATH)
+    spectral._atlas = None
+    assert np.array_equal(spectral.atlas(), atlas)
+    print(f"wrote {spectral.CURVES_PATH} and {spectral.ATLAS_PATH}")
 
 if __name__ == "__main__":
     main()

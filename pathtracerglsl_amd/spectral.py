@@ -14,16 +14,15 @@ its lifts for a constant authored value:
 spectral_bounded's sigmoid-quadratic coefficients come from the Jakob-Hanika coefficient atlas
 through the trilinear lookup of lookup.glsl:16-38 (spectral_coefficients), restated here in
 float32 operations in the same order. The atlas (3 x 64^3 RGBA32F texels, 12.6 MB) is the
-reference's shipped asset `coefficients.bin`; it is read from the path in PTMI_SPECTRAL_ATLAS
-when set. Coefficients of the colours used by the bundled scenes are kept in a small table
-(data/spectral_lift_cache.json) built from that atlas by data/make_spectral_data.py, so those
-scenes compile without the atlas. The CIE 1931 / D65 curves (curves.bin, 4 KB) ship as
-data/spectral_curves.f32.
+reference's shipped asset `coefficients.bin`; it ships as data/spectral_atlas.zlib (its three
+used channels, byte planes shuffled, deflated: 7 MB) and is checked against the reference's
+SHA-256 when it is inflated, so any authored colour lifts. The CIE 1931 / D65 curves (curves.bin,
+4 KB) ship as data/spectral_curves.f32.
 """
 import hashlib
-import json
 import math
 import os
+import zlib
 
 import numpy as np
 
@@ -31,7 +30,7 @@ from . import abi
 
 DATA_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
 CURVES_PATH = os.path.join(DATA_DIR, "spectral_curves.f32")
-CACHE_PATH = os.path.join(DATA_DIR, "spectral_lift_cache.json")
+ATLAS_PATH = os.path.join(DATA_DIR, "spectral_atlas.zlib")
 CURVES_SHA256 = "86f1739e08f078a4665a7bc64a8b9fb0bf261c592f2696dd73af0a32ee52650e"
 ATLAS_SHA256 = "a753491b81547d3f033a5fc91c523a21079ac43a7d2360c9258eb0912823748b"
 
@@ -79,14 +78,51 @@ def z_nodes():
     return curves_texture()[1, :64, 0]
 
 
-def load_atlas(path):
-    raw = np.fromfile(path, np.float32)
-    if raw.size != 3 * 64 ** 3 * 4:
-        raise SpectralDataError(f"{path}: expected 3 x 64^3 RGBA32F texels")
-    h = hashlib.sha256(raw.tobytes()).hexdigest()
+ATLAS_TEXELS = 3 * 64 ** 3
+
+
+def _check_atlas(rgba, where):
+    h = hashlib.sha256(rgba.tobytes()).hexdigest()
     if h != ATLAS_SHA256:
-        raise SpectralDataError(f"{path}: SHA-256 {h} does not match the shipped coefficient atlas")
-    return raw.reshape(-1, 4)
+        raise SpectralDataError(f"{where}: SHA-256 {h} does not match the reference's coefficient atlas")
+    return rgba.reshape(-1, 4)
+
+
+def load_atlas(path):
+    """The reference's coefficients.bin (3 x 64^3 RGBA32F texels), SHA-256 checked."""
+    raw = np.fromfile(path, np.float32)
+    if raw.size != ATLAS_TEXELS * 4:
+        raise SpectralDataError(f"{path}: expected 3 x 64^3 RGBA32F texels")
+    return _check_atlas(raw, path)
+
+
+def write_atlas(rgba, path):
+    """The shipped form of the atlas: channels x, y, z as float32 (w is 0 in every texel), the
+    four byte planes one after another, deflated."""
+    rgba = np.asarray(rgba, np.float32).reshape(-1, 4)
+    if rgba.shape[0] != ATLAS_TEXELS or rgba[:, 3].any():
+        raise SpectralDataError("write_atlas: expected 3 x 64^3 texels with w = 0")
+    planes = np.ascontiguousarray(rgba[:, :3]).view(np.uint8).reshape(-1, 4).T
+    with open(path, "wb") as f:
+        f.write(zlib.compress(np.ascontiguousarray(planes).tobytes(), 9))
+
+
+_atlas = None
+
+
+def atlas():
+    """The coefficient atlas as [3 * 64^3][4] float32 (RGBA, w = 0), inflated from the shipped
+    file and checked against the reference's SHA-256."""
+    global _atlas
+    if _atlas is None:
+        raw = zlib.decompress(open(ATLAS_PATH, "rb").read())
+        if len(raw) != ATLAS_TEXELS * 12:
+            raise SpectralDataError(f"{ATLAS_PATH}: wrong inflated size")
+        xyz = np.frombuffer(raw, np.uint8).reshape(4, -1).T.copy().view(np.float32).reshape(-1, 3)
+        rgba = np.zeros((ATLAS_TEXELS, 4), np.float32)
+        rgba[:, :3] = xyz
+        _atlas = _check_atlas(rgba, ATLAS_PATH)
+    return _atlas
 
 
 def _mix(a, b, t):
@@ -126,30 +162,9 @@ def spectral_coefficients(rgb, atlas):
     return out
 
 
-def _key(rgb):
-    return ",".join(f"{np.float32(v).view(np.uint32):08x}" for v in rgb)
-
-
-_cache = None
-_atlas = None
-
-
 def _lookup(bounded):
-    """Coefficients of a non-grey bounded rgb: the lift table first, then the atlas."""
-    global _cache, _atlas
-    if _cache is None:
-        _cache = json.load(open(CACHE_PATH)) if os.path.exists(CACHE_PATH) else {}
-    k = _key(bounded)
-    if k in _cache:
-        return [np.uint32(int(h, 16)).view(np.float32) for h in _cache[k]]
-    path = os.environ.get("PTMI_SPECTRAL_ATLAS")
-    if not path:
-        raise SpectralDataError(
-            f"spectral lift of rgb {tuple(float(v) for v in bounded)} needs the Jakob-Hanika coefficient "
-            "atlas: set PTMI_SPECTRAL_ATLAS to the path of coefficients.bin")
-    if _atlas is None:
-        _atlas = load_atlas(path)
-    return spectral_coefficients(bounded, _atlas)
+    """Coefficients of a non-grey bounded rgb from the atlas."""
+    return spectral_coefficients(bounded, atlas())
 
 
 def bounded_input(rgb, role):

@@ -109,3 +109,16 @@ def test_rough_dispersive_prism_matches_oracle(engine, oracle, color):
     bad, _ = _compare(engine, oracle, f"rough-prism-{color}", sc, compiler.RenderStrategy("none", 12, 3, color=color),
                       32, 24, 8)
     assert bad == 0.0
+
+
+def test_spectral_arbitrary_colours_match_oracle(engine, oracle):
+    """Authored colours outside any precomputed table (lifted through the shipped atlas)."""
+    sc = scenes.cornell_box()
+    rng = np.random.default_rng(11)
+    for name in ("white", "red", "green"):
+        sc["materials"][name]["albedo"] = [float(v) for v in rng.uniform(0.05, 0.95, size=3)]
+    sc["lights"][0]["radiance"] = [float(v) for v in rng.uniform(2.0, 20.0, size=3)]
+    bad, counts = _compare(engine, oracle, "spectral-arbitrary-colours", sc,
+                           compiler.RenderStrategy("mis", 8, 3, color="spectral"), 32, 32, 8)
+    assert bad == 0.0
+    assert engine.ray_counts() == (int(counts[0]), int(counts[1]))

@@ -70,11 +70,18 @@ def test_blackbody_projection_matches_reference_ratios():
     assert min(spectral.blackbody_color(6500.0)[0]) > 0.85
 
 
-def test_unknown_colour_without_atlas_is_a_compile_error(monkeypatch):
-    monkeypatch.delenv("PTMI_SPECTRAL_ATLAS", raising=False)
+def test_any_colour_lifts_and_a_tampered_atlas_is_refused(monkeypatch, tmp_path):
     sc = scenes.spectral_furnace((0.31, 0.17, 0.66))
-    with pytest.raises(compiler.CompileError, match="PTMI_SPECTRAL_ATLAS"):
-        compiler.compile_scene(sc, compiler.RenderStrategy("none", 2, -1, color="spectral"), 8, 8)
+    comp = compiler.compile_scene(sc, compiler.RenderStrategy("none", 2, -1, color="spectral"), 8, 8)
+    assert any(int(m["s_albedo"]["kind"]) == abi.SPEC_SIGMOID for m in comp.scene.materials)
+    a = spectral.atlas().copy()
+    a[12345, 1] = np.nextafter(a[12345, 1], np.float32(np.inf))
+    bad = tmp_path / "atlas.zlib"
+    spectral.write_atlas(a, str(bad))
+    monkeypatch.setattr(spectral, "ATLAS_PATH", str(bad))
+    monkeypatch.setattr(spectral, "_atlas", None)
+    with pytest.raises(spectral.SpectralDataError, match="SHA-256"):
+        spectral.atlas()
 
 
 def test_spectral_rejects_environment():
