@@ -20,6 +20,10 @@ as one hipGraph-captured batch after the autotuner's sweep, against 64 sequentia
 launches (pt_render_frames per scene; uploads untimed); reported with the per-pixel identity
 of all 64 films.
 
+Configuration legs (same JSON line): the plain Cornell box (256x256, 16 spp, depth 8, pt-mis, RGB)
+and the spectral glass-medium Cornell box (1024x1024, 256 spp, depth 32, pt-mis, 4 hero
+wavelengths), each with Mpaths/s, Mrays/s and a bounded CPU sample on the oracle.
+
 Also reported: the roofline of the dominant kernel (the extend / closest-hit traversal),
 whose algorithmic bytes per query are 32 (ray) + 16 (hit) + 80 per wide node visited + 36 per
 triangle tested, counted by an instrumented, untimed replay of the identical render; and a
@@ -42,7 +46,7 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(comp, spp):
+def cpu_baseline(comp, spp, sample=None):
     """CPU restatement of the reference walk (the oracle), all host threads, bounded sample."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib
@@ -53,9 +57,42 @@ def cpu_baseline(comp, spp):
     dt = time.perf_counter() - t0
     paths = comp.width * comp.height * spp
     cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    flags_path = os.path.join(ROOT, "oracle", "_build", "liboracle.flags")
+    flags = open(flags_path).read().strip() if os.path.exists(flags_path) else None
     return {"value": round(paths / dt / 1e6, 4), "unit": "Mpaths/s", "cores": cores, "kind": "port",
-            "sample": f"{comp.width}x{comp.height} frame, {spp} spp of the 64 (same scene, streams, depth)",
-            "seconds": round(dt, 2), "mrays_per_s": round(float(counts.sum()) / dt / 1e6, 4)}
+            "sample": sample or f"{comp.width}x{comp.height} frame, {spp} spp of the 64 (same scene, streams, depth)",
+            "seconds": round(dt, 2), "mrays_per_s": round(float(counts.sum()) / dt / 1e6, 4), "build": flags}
+
+
+def config_leg(Engine, comp, spp, reps=3, cpu=None):
+    """Throughput of one more configuration at its own size: Mpaths/s and Mrays/s of full
+    renders with the scene resident (best of reps after one warm render), and a bounded CPU
+    sample of the same scene on the oracle."""
+    eng = Engine(0)
+    eng.load_renderer(comp)
+    eng.clear_accumulation()
+    eng.render_frames(spp)
+    eng.finish()
+    best = None
+    for _ in range(reps):
+        eng.clear_accumulation()
+        eng.finish()
+        t0 = time.perf_counter()
+        eng.render_frames(spp)
+        eng.finish()
+        dt = time.perf_counter() - t0
+        best = dt if best is None else min(best, dt)
+    c, a = eng.ray_counts()
+    film = eng.read_export()
+    eng.close()
+    paths = comp.width * comp.height * spp
+    out = {"paths": paths, "ms": round(best * 1e3, 3), "mpaths_per_s": round(paths / best / 1e6, 3),
+           "mrays_per_s": round((c + a) / best / 1e6, 3), "closest_rays": int(c), "any_rays": int(a),
+           "film_mean": float(film[..., :3].mean()), "finite": bool(__import__("numpy").isfinite(film).all())}
+    if cpu is not None:
+        ccomp, cspp, label = cpu
+        out["cpu_baseline"] = cpu_baseline(ccomp, cspp, label)
+    return out
 
 
 def adjoint_leg(Engine, compiler, scenes, abi, fp64=True, res=512, spp=64):
@@ -163,6 +200,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-configs", action="store_true", help="skip the Cornell and spectral-media legs")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="run a plain device copy of this size first (counter calibration runs)")
     args = ap.parse_args()
@@ -264,6 +302,19 @@ def main():
         log("[bench] CPU baseline")
         result["cpu_baseline"] = cpu_baseline(comp, args.cpu_spp)
     eng.close()
+    if not args.no_configs:
+        log("[bench] configuration legs")
+        st0 = compiler.RenderStrategy("mis", 8, 3, 0.95, 16)
+        c0 = compiler.compile_scene(scenes.cornell_box(), st0, 256, 256)
+        result["cornell"] = dict(config_leg(Engine, c0, 16, cpu=None if args.no_cpu else (c0, 16, "the whole workload")),
+                                 workload="cornell 256x256 16spp depth8 pt-mis rgb")
+        log(f"[bench] cornell {result['cornell']}")
+        st2 = compiler.RenderStrategy("mis", 32, 3, 0.95, 4, color="spectral")
+        c2 = compiler.compile_scene(scenes.cornell_spectral_medium(), st2, 1024, 1024)
+        c2s = compiler.compile_scene(scenes.cornell_spectral_medium(), st2, 128, 128)
+        result["spectral_media"] = dict(config_leg(Engine, c2, 256, cpu=None if args.no_cpu else (c2s, 16, "128x128 16spp of the same scene")),
+                                        workload="spectral glass-medium cornell 1024x1024 256spp depth32 pt-mis")
+        log(f"[bench] spectral media {result['spectral_media']}")
     if not args.no_adjoint:
         log("[bench] adjoint leg")
         from pathtracerglsl_amd import abi
