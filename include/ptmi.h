@@ -31,7 +31,8 @@ typedef enum pt_status {
     PT_E_MISSING_INPUT = 2,  /* a required scene channel or table was not uploaded */
     PT_E_DEVICE = 3,         /* HIP runtime error */
     PT_E_OOM = 4,            /* device allocation failed */
-    PT_E_UNSUPPORTED = 5     /* program asks for a feature this build does not provide */
+    PT_E_UNSUPPORTED = 5,    /* program asks for a feature this build does not provide */
+    PT_E_CONTEXT_LOST = 6    /* the renderer or batch lost its device resources to pt_ctx_reset */
 } pt_status;
 
 /* ---- scene description (the data rail: src/components/data/channels.ts:11-20) ---- */
@@ -325,6 +326,15 @@ const char* pt_last_error(void);
 pt_status pt_ctx_create(int device_index, pt_ctx** out);
 void      pt_ctx_destroy(pt_ctx* ctx);
 
+/* Context loss (Engine._handleContextLoss, src/engine/Engine.ts:476-491): every device resource
+ * the context holds is released (the resident scene and its tables, each loaded renderer's film
+ * and path state, each batch's scenes, films and graph). Renderer and batch handles stay valid
+ * but lost: their calls fail with PT_E_CONTEXT_LOST (pt_sample_count reads 0) and only
+ * pt_renderer_unload / pt_batch_destroy remain. The caller restores as the reference's
+ * _handleContextRestored does: upload the scene again and load each renderer again from its
+ * retained program (a fresh, zeroed accumulation). The context itself (device, stream) stays. */
+pt_status pt_ctx_reset(pt_ctx* ctx);
+
 /* Upload and build the scene: triangle wide BVH, containment structures, env table.
  * Replaces registerDataTexture[U32]/registerEnvironmentTable (Engine.ts:438-460) plus
  * the App-side packing (app/App.ts:301-345). */
@@ -403,6 +413,38 @@ pt_status pt_last_ray_counts(pt_renderer* r, uint64_t* closest, uint64_t* any);
  * src/engine/Engine.ts:206-236): replace the n material records of the uploaded scene
  * without rebuilding geometry. model, has_medium and emissive must not change. */
 pt_status pt_scene_update_materials(pt_ctx* ctx, const pt_material* materials, int32_t n);
+
+/* Live light values (setParameter on driven emission, src/engine/Engine.ts:258, with the
+ * CPU-recomputed u_light_selpdf / u_light_cdf arrays of src/compiler/generate/features/
+ * lighting.ts:166-183): replace the n light records of the uploaded scene. Only the emission
+ * (radiance, s_radiance) and the selection table (select_pdf, cdf) may change: kind, region and
+ * the geometry rows are constant, as the reference's contract holds them (resolveLightValues,
+ * lighting.ts:337-348). Emission must be finite and >= 0, select_pdf in (0, 1], and cdf must
+ * rise to 1 (within 1e-5) across the records. A hittable light's emission also lives in its
+ * quad's material record (pt_scene_update_materials). Renderers read the new records from their
+ * next call. */
+pt_status pt_scene_update_lights(pt_ctx* ctx, const pt_light* lights, int32_t n);
+
+/* Live environment values (setParameter on env.intensity, env.rotation, env.selectProb and a
+ * constant sky's colour; src/compiler/generate/features/environment.ts:119-160, lighting.ts
+ * envSelectionDeps :520-531): type, samplable, chart, compensated and the table size must
+ * equal the resident environment's (the table is rebuilt by pt_env_table_upload or a scene
+ * upload); intensity, rotation, select_prob, color and s_color take effect from the next call.
+ * The procedural sky's table is baked at rotation 0, so rotation never invalidates it. */
+pt_status pt_scene_update_env(pt_ctx* ctx, const pt_env* env);
+
+/* Engine.registerEnvironmentTable (src/engine/Engine.ts:450-460, build-environment-sampler.ts
+ * :20-43): the importance table of the resident tabulated environment (PT_ENV_SKY) rebuilt from
+ * the caller's W x H linear RGB picture on the chart (rgb: W*H*3 floats, texel (i, j) at chart
+ * coordinates ((i + 0.5) / W, (j + 0.5) / H); equirect row 0 is the +Y pole) by
+ * buildEnvImportanceTable (src/components/env/importance.ts:16-127): luminance, 3 x 3 tent blur,
+ * optional MIS compensation, sin(theta)-weighted (equirect) or equal-area (octahedral, W == H)
+ * conditional and marginal CDFs, on the device. It replaces the table the scene upload baked;
+ * chart and compensated become the resident table's. Every sample must be finite and >= 0
+ * (PT_E_INVALID otherwise, as the reference throws); 1 <= W <= 4096, H >= 1. total_weight
+ * (may be NULL) receives the table's mass, the reference's returned totalWeight. Synchronous. */
+pt_status pt_env_table_upload(pt_ctx* ctx, const float* rgb, int32_t width, int32_t height, int32_t chart,
+                              int32_t compensated, double* total_weight);
 
 /* ---- differentiation: path-replay adjoint (RGB programs) ----
  * Parameters: the homogeneous medium of material `medium_material` (sigma_s RGB, sigma_a RGB,

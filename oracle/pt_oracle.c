@@ -1281,21 +1281,11 @@ static float environment_pdf(const or_scene* S, v3 dir) {
 }
 
 /* Env bake (compiler/EnvironmentBake.ts:62-67) + importance table (env/importance.ts:16-140). */
-static void env_build(or_scene* S) {
-    /* equirect W x H; octahedral N x N with N = table_h (EnvironmentBake.ts:21-27) */
-    int oct = S->env.chart == PT_CHART_OCTAHEDRAL;
-    int H = S->env.table_h > 0 ? S->env.table_h : 256, W = oct ? H : (S->env.table_w > 0 ? S->env.table_w : 512);
+/* The importance-table chain after the luminance pass (components/env/importance.ts:47-127): 3 x 3
+ * tent blur, MIS compensation when asked, conditional CDF per row and the marginal CDF over rows.
+ * Y (W x H luminance) is consumed; the table replaces S's. */
+static void env_chain(or_scene* S, float* Y, int W, int H, int oct, int compensated) {
     S->env_w = W; S->env_h = H;
-    float* Y = (float*)malloc(sizeof(float) * (size_t)W * H);
-    for (int j = 0; j < H; j++)
-        for (int i = 0; i < W; i++) {
-            float u = ((float)i + 0.5f) / (float)W, v = ((float)j + 0.5f) / (float)H;
-            v3 d;
-            if (oct) ptm_oct_dir(u, v, 1.0f, 0.0f, &d.x, &d.y, &d.z);   /* octahedral.glsl:25-42, rotation 0 */
-            else d = env_chart_dir(0.0f, u, v);
-            v3 c = sky_radiance_raw(d);
-            Y[(size_t)j * W + i] = (float)(0.2126 * (double)c.x + 0.7152 * (double)c.y + 0.0722 * (double)c.z);
-        }
     float* Bl = (float*)malloc(sizeof(float) * (size_t)W * H);
     for (int j = 0; j < H; j++)
         for (int i = 0; i < W; i++) {
@@ -1312,7 +1302,7 @@ static void env_build(or_scene* S) {
             }
             Bl[(size_t)j * W + i] = (float)(sum / wsum);
         }
-    if (S->env.compensated) {
+    if (compensated) {
         double num = 0, den = 0;
         for (int j = 0; j < H; j++) {
             double dO = oct ? (4 * M_PI) / (W * H) : (2 * M_PI / W) * (M_PI / H) * sin(M_PI * (j + 0.5) / H);   /* importance.ts:77-80 */
@@ -1321,6 +1311,7 @@ static void env_build(or_scene* S) {
         double mean = den > 0 ? num / den : 0;
         for (size_t k = 0; k < (size_t)W * H; k++) { double y = (double)Bl[k] - mean; Bl[k] = (float)(y > 0 ? y : 0); }
     }
+    free(S->env_cond); free(S->env_marg);
     S->env_cond = (float*)malloc(sizeof(float) * (size_t)W * H);
     S->env_marg = (float*)malloc(sizeof(float) * (size_t)H);
     double* rowSum = (double*)malloc(sizeof(double) * (size_t)H);
@@ -1341,6 +1332,44 @@ static void env_build(or_scene* S) {
     for (int j = 0; j < H; j++) { acc += rowSum[j]; S->env_marg[j] = (float)(total > 0 ? acc / total : 0); }
     S->env_total = total;
     free(Y); free(Bl); free(rowSum);
+}
+
+/* The procedural sky's table: its luminance at every texel centre of the chart (the bake pass,
+ * compiler/EnvironmentBake.ts:29-80, at rotation 0), then the chain. */
+static void env_build(or_scene* S) {
+    /* equirect W x H; octahedral N x N with N = table_h (EnvironmentBake.ts:21-27) */
+    int oct = S->env.chart == PT_CHART_OCTAHEDRAL;
+    int H = S->env.table_h > 0 ? S->env.table_h : 256, W = oct ? H : (S->env.table_w > 0 ? S->env.table_w : 512);
+    float* Y = (float*)malloc(sizeof(float) * (size_t)W * H);
+    for (int j = 0; j < H; j++)
+        for (int i = 0; i < W; i++) {
+            float u = ((float)i + 0.5f) / (float)W, v = ((float)j + 0.5f) / (float)H;
+            v3 d;
+            if (oct) ptm_oct_dir(u, v, 1.0f, 0.0f, &d.x, &d.y, &d.z);   /* octahedral.glsl:25-42, rotation 0 */
+            else d = env_chart_dir(0.0f, u, v);
+            v3 c = sky_radiance_raw(d);
+            Y[(size_t)j * W + i] = (float)(0.2126 * (double)c.x + 0.7152 * (double)c.y + 0.0722 * (double)c.z);
+        }
+    env_chain(S, Y, W, H, oct, S->env.compensated);
+}
+
+/* registerEnvironmentTable (engine/Engine.ts:450-460 -> importance.ts:16-127): the table rebuilt
+ * from a caller's W x H RGB picture on `chart`; the luminance is the f64 weighted sum of the f32
+ * channels (importance.ts:38-45). Returns 0, or -1 for a bad size or a negative / non-finite
+ * sample (the reference throws). */
+int or_scene_env_table_rgb(or_scene* S, const float* rgb, int W, int H, int chart, int compensated, double* total) {
+    int oct = chart == PT_CHART_OCTAHEDRAL;
+    if (W < 1 || H < 1 || (oct && W != H) || S->env.type != PT_ENV_SKY) return -1;
+    for (size_t k = 0; k < (size_t)3 * W * H; k++)
+        if (!isfinite(rgb[k]) || rgb[k] < 0.0f) return -1;
+    float* Y = (float*)malloc(sizeof(float) * (size_t)W * H);
+    for (size_t k = 0; k < (size_t)W * H; k++)
+        Y[k] = (float)(0.2126 * (double)rgb[3 * k] + 0.7152 * (double)rgb[3 * k + 1] + 0.0722 * (double)rgb[3 * k + 2]);
+    env_chain(S, Y, W, H, oct, compensated);
+    S->env.chart = oct ? PT_CHART_OCTAHEDRAL : PT_CHART_EQUIRECT;
+    S->env.table_w = W; S->env.table_h = H; S->env.compensated = compensated ? 1 : 0;
+    if (total) *total = S->env_total;
+    return 0;
 }
 
 /* ---------------- lights (lights/quad/quad.glsl:14-41, features/lighting.ts:546-673) ---- */

@@ -19,13 +19,18 @@ DECLARED = ["pt_last_error", "pt_ctx_create", "pt_ctx_destroy", "pt_scene_upload
             "pt_program_from_description", "pt_renderer_load_description", "pt_env_total_weight",
             "pt_obj_parse", "pt_mesh_data_info", "pt_mesh_data_copy", "pt_mesh_data_free", "pt_inst_parse",
             "pt_render_frames_noalloc", "pt_bvh_build", "pt_mesh_bvh_build", "pt_plan_data_layout",
-            "pt_pack_mesh_channels"]
+            "pt_pack_mesh_channels", "pt_ctx_reset", "pt_scene_update_lights", "pt_scene_update_env",
+            "pt_env_table_upload"]
 # every symbol declared in include/ptmi_debug.h (test and bench instrumentation)
 DECLARED_DEBUG = ["pt_renderer_set_traversal_counting", "pt_last_traversal_counts", "pt_trace_closest_device_counted",
                   "pt_scene_stats", "pt_abi_sizeof", "pt_debug_wide_bvh", "pt_debug_math", "pt_calib_copy",
                   "pt_debug_env_bake", "pt_debug_env_table"]
 
 _lib = None
+
+
+# pt_status codes (include/ptmi.h)
+E_INVALID, E_MISSING_INPUT, E_DEVICE, E_OOM, E_UNSUPPORTED, E_CONTEXT_LOST = 1, 2, 3, 4, 5, 6
 
 
 class PtError(RuntimeError):
@@ -70,6 +75,10 @@ def load():
     lib.pt_trace_closest_device_counted.argtypes = [vp, vp, u32, vp, vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
     lib.pt_scene_stats.argtypes = [vp, C.POINTER(u32), C.POINTER(u32), C.POINTER(u32)]
     lib.pt_scene_update_materials.argtypes = [vp, vp, i32]
+    lib.pt_scene_update_lights.argtypes = [vp, vp, i32]
+    lib.pt_scene_update_env.argtypes = [vp, C.POINTER(abi.PtEnv)]
+    lib.pt_env_table_upload.argtypes = [vp, vp, i32, i32, i32, i32, C.POINTER(C.c_double)]
+    lib.pt_ctx_reset.argtypes = [vp]
     lib.pt_forward.argtypes = [vp, C.POINTER(abi.PtParams), u32]
     lib.pt_backward.argtypes = [vp, C.POINTER(abi.PtParams), u32, C.POINTER(abi.PtAdjointParams), vp, vp]
     lib.pt_batch_create.argtypes = [vp, i32, vp, vp, vp, C.POINTER(vp)]

@@ -56,6 +56,8 @@ class CompiledRenderer:
     reset_salt: int = 1234
     warnings: list = field(default_factory=list)
     material_names: list = field(default_factory=list)   # authored material of each record index
+    source: dict = None                  # the authored scene (live parameters recompile from it)
+    strategy: "RenderStrategy" = None
 
     def params(self, reset_salt=None):
         return camera_params(self.camera, self.width, self.height,
@@ -198,19 +200,20 @@ def select_pdfs(light_powers, selection):
     return pdfs, cdf
 
 
-def env_select_probability(scene, env, light_powers, selection="power"):
+def env_select_probability(scene, env, light_powers, selection="power", total_weight=None):
     """envSelectionProbability (features/lighting.ts:488-518). 'uniform': 1 / (N + 1). 'power':
     Phi_env / (Phi_env + sum Phi_light), Phi_env = 4 pi^2 r^2 mean(L) intensity. Both clamped to
     [0.01, 0.99]. mean(L): the constant colour's channel mean, or the importance table's mass
     totalWeight pi / (2 W H) for the tabulated sky on the equirect chart, totalWeight / N^2 on the
-    N x N octahedral chart (lighting.ts:502-512)."""
+    N x N octahedral chart (lighting.ts:502-512). totalWeight is the env.totalWeight parameter:
+    the mass of the table the library bakes, unless a registered table's is given."""
     if selection == "uniform":
         return min(0.99, max(0.01, 1.0 / (len(light_powers) + 1)))
     if env.type == abi.ENV_CONSTANT:
         mean_l = (env.color[0] + env.color[1] + env.color[2]) / 3.0
     else:
         from . import _lib
-        total = _lib.load().pt_env_total_weight(env)
+        total = float(total_weight) if total_weight is not None else _lib.load().pt_env_total_weight(env)
         if env.chart == abi.CHART_OCTAHEDRAL:
             mean_l = total / float(env.table_h * env.table_h) if total > 0 else 1.0
         else:
@@ -591,7 +594,9 @@ def compile_scene(scene, strategy=None, width=256, height=256, reset_salt=1234):
     env = abi.PtEnv(abi.ENV_NONE, 512, 256, 0, 1.0, 0.0, 0.5, 0)
     e = scene.get("environment", {"type": "none"})
     if e.get("type") == "procedural":
-        env = abi.PtEnv(abi.ENV_SKY, 512, 256, int(e.get("compensated", False)),
+        # env.size (u_envSize): the importance table's W x H, 512 x 256 unless a registered table's
+        tw, th = e.get("table_size", (512, 256))
+        env = abi.PtEnv(abi.ENV_SKY, int(tw), int(th), int(e.get("compensated", False)),
                         float(e.get("intensity", 1.0)), float(e.get("rotation", 0.0)),
                         float(e.get("select_prob", 0.5)), 1)
         # estimator.envSampler.chart (components/env/index.ts ENV_CHARTS): the table's chart
@@ -609,7 +614,8 @@ def compile_scene(scene, strategy=None, width=256, height=256, reset_salt=1234):
         env.s_color = abi.PtSpectrum.of(srec(col, spectral.ROLE_EMISSION))
 
     if env.type != abi.ENV_NONE and (env.type == abi.ENV_SKY or env.samplable) and light_recs and "select_prob" not in e:
-        env.select_prob = env_select_probability(scene, env, [p for _, p in lights], strategy.light_selection)
+        env.select_prob = env_select_probability(scene, env, [p for _, p in lights], strategy.light_selection,
+                                                 e.get("total_weight"))
 
     tv = np.concatenate(tri_chunks) if tri_chunks else np.zeros((0, 9), np.float32)
     to = np.concatenate(tri_obj_chunks) if tri_obj_chunks else np.zeros(0, np.int32)
@@ -652,7 +658,8 @@ def compile_scene(scene, strategy=None, width=256, height=256, reset_salt=1234):
     sel = "-uniform" if strategy.light_selection == "uniform" else ""
     return CompiledRenderer(id=f"{strategy.color}-{strategy.direct_lighting}{sel}-{scene.get('id', 'scene')}", scene=arrays,
                             program=prog, width=width, height=height, camera=scene["camera"],
-                            reset_salt=reset_salt, warnings=warnings, material_names=list(mat_names))
+                            reset_salt=reset_salt, warnings=warnings, material_names=list(mat_names),
+                            source=scene, strategy=strategy)
 
 
 # The proc-sky environment body of the reference's procedural-sky scenes, the one PT_ENV_SKY bakes

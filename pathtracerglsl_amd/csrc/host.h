@@ -30,7 +30,11 @@ pt_status fail(pt_status s, const std::string& msg);
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
-    ~DevBuf() { if (p) (void)hipFree(p); }
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() { if (p) (void)hipFree(p); p = nullptr; bytes = 0; }
 };
 
 inline int grid_for(long long n, int block) {
@@ -56,6 +60,7 @@ struct SceneStore {
     bool has_rough = false;
     std::vector<pt_material> host_materials;
     std::vector<pt_light> host_lights;
+    size_t env_first = 0, env_end = 0;   // bufs[env_first, env_end): the environment table
     hipStream_t stream = nullptr;
     template <class T> pt_status upload(const T* host, size_t n, const T** out) {
         auto b = std::make_unique<DevBuf>();
@@ -93,6 +98,10 @@ struct pt_ctx {
     bool has_rough = false;
     std::vector<pt_material> host_materials;   // the uploaded records (adjoint checks, updates)
     std::vector<pt_light> host_lights;         // the uploaded light records (selection-table check)
+    pt_env host_env{};                         // the resident environment (live updates, table uploads)
+    std::vector<std::unique_ptr<DevBuf>> env_bufs;   // the resident environment table
+    std::vector<pt_renderer*> renderers;       // loaded renderers (released by pt_ctx_reset)
+    std::vector<pt_batch*> batches;            // live batches (released by pt_ctx_reset)
     bool timing = false;
     std::map<std::string, TimingSlot> timings;
     std::vector<std::string> timing_order;
@@ -140,6 +149,7 @@ struct pt_renderer {
     int adj_mode = PT_REDUCE_DETERMINISTIC;
     Paths paths{};
     bool count_traversal = false;
+    bool lost = false;    // its device resources went with pt_ctx_reset
 };
 
 #define TIMED(ctx, name, items, launch)             \
@@ -170,6 +180,7 @@ struct pt_batch {
     hipGraphExec_t exec = nullptr;
     std::vector<uint32_t> graph_spp;
     pt_batch_config graph_cfg{};
+    bool lost = false;    // its device resources went with pt_ctx_reset
     ~pt_batch() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);

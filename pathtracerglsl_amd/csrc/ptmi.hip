@@ -68,6 +68,15 @@ __global__ void __launch_bounds__(256) k_env_luma(float* Y, int W, int H, float 
     }
 }
 
+// Luminance of a caller's RGB picture (importance.ts:38-45): the f64 weighted sum of the f32
+// channels, stored as f32.
+__global__ void __launch_bounds__(256) k_env_luma_rgb(const float* __restrict__ rgb, float* __restrict__ Y, long long n) {
+    for (long long k = blockIdx.x * 256LL + threadIdx.x; k < n; k += (long long)gridDim.x * 256) {
+        const float r = rgb[3 * k], g = rgb[3 * k + 1], b = rgb[3 * k + 2];
+        Y[k] = (float)(0.2126 * (double)r + 0.7152 * (double)g + 0.0722 * (double)b);
+    }
+}
+
 // 3 x 3 tent, u wraps (the phi seam), v clamps (the poles).
 __global__ void __launch_bounds__(256) k_env_blur(const float* __restrict__ Y, float* __restrict__ B, int W, int H, int oct) {
     for (int k = blockIdx.x * 256 + threadIdx.x; k < W * H; k += gridDim.x * 256) {
@@ -380,50 +389,84 @@ static pt_status validate_scene(const pt_scene_desc* sd) {
 }
 
 // Build every device structure of one scene (wide BVH, containment, env table) into SS.
-// The procedural sky's importance table, baked on the device into SS (k_env_* above). The f64
-// row weights sin(theta_j) and texel solid angles are the host builder's expressions, evaluated
-// on the host (H values each). ev0 / ev1, when given, bracket the kernels on SS.stream.
-static pt_status bake_env_table(const pt_env& env, SceneStore& SS, DevScene& D, hipEvent_t ev0 = nullptr,
-                                hipEvent_t ev1 = nullptr) {
-    const bool oct = env.chart == PT_CHART_OCTAHEDRAL;
-    const int H = env.table_h > 0 ? env.table_h : 256, W = oct ? H : (env.table_w > 0 ? env.table_w : 512);
-    if (W > PT_ENV_MAX_W) return fail(PT_E_UNSUPPORTED, "environment table wider than 4096 texels");
+// The importance-table chain after the luminance pass (components/env/importance.ts:44-127):
+// tent blur, MIS compensation when asked, row and marginal CDFs. The table (cond, marg and the
+// mean / total pair) is allocated in SS; the passes' scratch (blurred luminance, row masses,
+// the f64 row weights sin(theta_j) and texel solid angles, the host builder's expressions
+// evaluated on the host, H values each) in tmp, which the caller keeps until the stream has
+// run the chain.
+static pt_status env_table_chain(float* Y, int W, int H, bool oct, bool compensated, SceneStore& SS, SceneStore& tmp,
+                                 DevScene& D) {
     const long long n = (long long)W * H;
     std::vector<double> sin_t((size_t)H), d_omega((size_t)H);
     for (int j = 0; j < H; j++) {
         sin_t[j] = oct ? 1.0 : std::sin(M_PI * (j + 0.5) / H);
         d_omega[j] = oct ? (4 * M_PI) / (W * H) : (2 * M_PI / W) * (M_PI / H) * std::sin(M_PI * (j + 0.5) / H);
     }
-    float sx = 0.4f, sy = 0.35f, sz = 0.2f;               // the sun direction, normalised in f32
-    float d = sx * sx + sy * sy + sz * sz;
-    float inv = 1.0f / sqrtf(d);
-    sx *= inv; sy *= inv; sz *= inv;
     pt_status st;
-    float *Y, *B, *cond, *marg;
+    float *B, *cond, *marg;
     double *rows, *aux;
     const double *d_sin, *d_dom;
-    if ((st = SS.alloc((size_t)n, &Y)) != PT_OK || (st = SS.alloc((size_t)n, &B)) != PT_OK ||
-        (st = SS.alloc((size_t)n, &cond)) != PT_OK || (st = SS.alloc((size_t)H, &marg)) != PT_OK ||
-        (st = SS.alloc((size_t)H, &rows)) != PT_OK || (st = SS.alloc(2, &aux)) != PT_OK ||
-        (st = SS.upload(sin_t.data(), sin_t.size(), &d_sin)) != PT_OK ||
-        (st = SS.upload(d_omega.data(), d_omega.size(), &d_dom)) != PT_OK)
+    if ((st = tmp.alloc((size_t)n, &B)) != PT_OK || (st = SS.alloc((size_t)n, &cond)) != PT_OK ||
+        (st = SS.alloc((size_t)H, &marg)) != PT_OK || (st = tmp.alloc((size_t)H, &rows)) != PT_OK ||
+        (st = SS.alloc(2, &aux)) != PT_OK || (st = tmp.upload(sin_t.data(), sin_t.size(), &d_sin)) != PT_OK ||
+        (st = tmp.upload(d_omega.data(), d_omega.size(), &d_dom)) != PT_OK)
         return st;
-    if (ev0) HIP_TRY(hipEventRecord(ev0, SS.stream));
-    hipLaunchKernelGGL(k_env_luma, dim3(grid_for(n, 256)), dim3(256), 0, SS.stream, Y, W, H, sx, sy, sz, oct ? 1 : 0);
     hipLaunchKernelGGL(k_env_blur, dim3(grid_for(n, 256)), dim3(256), 0, SS.stream, (const float*)Y, B, W, H, oct ? 1 : 0);
-    if (env.compensated) {
+    if (compensated) {
         hipLaunchKernelGGL(k_env_mean, dim3(1), dim3(256), 0, SS.stream, (const float*)B, d_dom, W, H, aux);
         hipLaunchKernelGGL(k_env_compensate, dim3(grid_for(n, 256)), dim3(256), 0, SS.stream, B, (int)n,
                            (const double*)aux);
     }
     hipLaunchKernelGGL(k_env_rows, dim3(grid_for(H, 64)), dim3(64), 0, SS.stream, (const float*)B, d_sin, W, H, cond, rows);
     hipLaunchKernelGGL(k_env_marginal, dim3(1), dim3(64), 0, SS.stream, (const double*)rows, H, marg, aux + 1);
-    if (ev1) HIP_TRY(hipEventRecord(ev1, SS.stream));
     HIP_TRY(hipGetLastError());
     D.env_cond = cond;
     D.env_marg = marg;
     D.env_total = aux + 1;
     return PT_OK;
+}
+
+// The procedural sky's importance table, baked on the device into SS (k_env_* above): the
+// sky's luminance at every texel centre, then the chain. ev0 / ev1, when given, bracket the
+// kernels on SS.stream. Synchronises the stream before the scratch is released.
+static pt_status bake_env_table(const pt_env& env, SceneStore& SS, DevScene& D, hipEvent_t ev0 = nullptr,
+                                hipEvent_t ev1 = nullptr) {
+    const bool oct = env.chart == PT_CHART_OCTAHEDRAL;
+    const int H = env.table_h > 0 ? env.table_h : 256, W = oct ? H : (env.table_w > 0 ? env.table_w : 512);
+    if (W > PT_ENV_MAX_W) return fail(PT_E_UNSUPPORTED, "environment table wider than 4096 texels");
+    const long long n = (long long)W * H;
+    float sx = 0.4f, sy = 0.35f, sz = 0.2f;               // the sun direction, normalised in f32
+    float d = sx * sx + sy * sy + sz * sz;
+    float inv = 1.0f / sqrtf(d);
+    sx *= inv; sy *= inv; sz *= inv;
+    SceneStore tmp;
+    tmp.stream = SS.stream;
+    pt_status st;
+    float* Y;
+    if ((st = tmp.alloc((size_t)n, &Y)) != PT_OK) return st;
+    if (ev0) HIP_TRY(hipEventRecord(ev0, SS.stream));
+    hipLaunchKernelGGL(k_env_luma, dim3(grid_for(n, 256)), dim3(256), 0, SS.stream, Y, W, H, sx, sy, sz, oct ? 1 : 0);
+    if ((st = env_table_chain(Y, W, H, oct, env.compensated != 0, SS, tmp, D)) != PT_OK) return st;
+    if (ev1) HIP_TRY(hipEventRecord(ev1, SS.stream));
+    HIP_TRY(hipStreamSynchronize(SS.stream));
+    return PT_OK;
+}
+
+// The environment's live values on the device record: the uniforms env.intensity, env.selectProb,
+// the sky's env.rotation with its derived (cos, sin) pair (features/environment.ts:36-45) and a
+// constant sky's colour.
+static void set_env_values(DevScene& D, const pt_env& e) {
+    D.env_intensity = e.intensity;
+    D.env_select_prob = e.select_prob;
+    if (e.type == PT_ENV_SKY) {
+        D.env_rotation = e.rotation;
+        D.env_rot_c = (float)std::cos((double)e.rotation);
+        D.env_rot_s = (float)std::sin((double)e.rotation);
+    } else if (e.type == PT_ENV_CONSTANT) {
+        D.env_color = make_float4(e.color[0], e.color[1], e.color[2], 0.0f);
+        D.env_s_color = e.s_color;
+    }
 }
 
 static pt_status build_scene(const pt_scene_desc* sd, hipStream_t stream, SceneStore& SS) {
@@ -671,22 +714,17 @@ static pt_status build_scene(const pt_scene_desc* sd, hipStream_t stream, SceneS
     // environment
     D.env_type = sd->env.type;
     if (sd->env.type == PT_ENV_SKY) {
+        SS.env_first = SS.bufs.size();
         if ((st = bake_env_table(sd->env, SS, D)) != PT_OK) return st;
+        SS.env_end = SS.bufs.size();
         D.env_chart = sd->env.chart == PT_CHART_OCTAHEDRAL ? PT_CHART_OCTAHEDRAL : PT_CHART_EQUIRECT;
         D.env_h = sd->env.table_h > 0 ? sd->env.table_h : 256;
         D.env_w = D.env_chart == PT_CHART_OCTAHEDRAL ? D.env_h : (sd->env.table_w > 0 ? sd->env.table_w : 512);
         D.env_compensated = sd->env.compensated;
-        D.env_intensity = sd->env.intensity;
-        D.env_rotation = sd->env.rotation;
-        D.env_rot_c = (float)std::cos((double)sd->env.rotation);
-        D.env_rot_s = (float)std::sin((double)sd->env.rotation);
-        D.env_select_prob = sd->env.select_prob;
+        set_env_values(D, sd->env);
     } else if (sd->env.type == PT_ENV_CONSTANT) {
-        D.env_intensity = sd->env.intensity;
-        D.env_color = make_float4(sd->env.color[0], sd->env.color[1], sd->env.color[2], 0.0f);
-        D.env_s_color = sd->env.s_color;
         D.env_samplable = sd->env.samplable ? 1 : 0;
-        D.env_select_prob = sd->env.select_prob;
+        set_env_values(D, sd->env);
     } else if (sd->env.type != PT_ENV_NONE) {
         return fail(PT_E_UNSUPPORTED, "environment type not provided by this build");
     }
@@ -701,12 +739,15 @@ pt_status pt_scene_upload(pt_ctx* ctx, const pt_scene_desc* sd) {
     HIP_TRY(hipSetDevice(ctx->device));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     ctx->scene_bufs.clear();
+    ctx->env_bufs.clear();
     ctx->has_scene = false;
     ctx->dev = DevScene{};
     SceneStore SS;
     if ((st = build_scene(sd, ctx->stream, SS)) != PT_OK) return st;
-    ctx->scene_bufs = std::move(SS.bufs);
+    for (size_t k = 0; k < SS.bufs.size(); k++)   // the env table apart: pt_env_table_upload replaces it
+        (k >= SS.env_first && k < SS.env_end ? ctx->env_bufs : ctx->scene_bufs).push_back(std::move(SS.bufs[k]));
     ctx->dev = SS.dev;
+    ctx->host_env = sd->env;
     ctx->stack = SS.stack;
     ctx->wide_depth = SS.wide_depth;
     ctx->has_ggx = SS.has_ggx;
@@ -796,14 +837,17 @@ pt_status pt_renderer_load(pt_ctx* ctx, const pt_program* prog, pt_renderer** ou
         (void)hipMemsetAsync(r->aov_film[a].p, 0, r->aov_film[a].bytes, ctx->stream);
     }
     (void)hipMemsetAsync(r->film.p, 0, r->film.bytes, ctx->stream);
+    ctx->renderers.push_back(r);
     *out = r;
     return PT_OK;
 }
 
 void pt_renderer_unload(pt_renderer* r) {
     if (!r) return;
-    (void)hipSetDevice(r->ctx->device);
-    (void)hipStreamSynchronize(r->ctx->stream);
+    pt_ctx* ctx = r->ctx;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    ctx->renderers.erase(std::remove(ctx->renderers.begin(), ctx->renderers.end(), r), ctx->renderers.end());
     delete r;
 }
 
@@ -939,10 +983,16 @@ static pt_status run_waves(pt_renderer* r, const pt_params* prm, uint32_t nframe
     return PT_OK;
 }
 
+// Calls on a renderer or batch whose device resources went with pt_ctx_reset.
+static pt_status lost_error(const char* fn) {
+    return fail(PT_E_CONTEXT_LOST, std::string(fn) + ": the context was reset (pt_ctx_reset); unload the handle and load it again");
+}
+
 extern "C" {
 
 pt_status pt_render_frames(pt_renderer* r, const pt_params* prm, uint32_t nframes) {
     if (!r || !prm) return fail(PT_E_INVALID, "pt_render_frames: null argument");
+    if (r->lost) return lost_error("pt_render_frames");
     pt_ctx* ctx = r->ctx;
     if (!ctx->has_scene) return fail(PT_E_MISSING_INPUT, "missing input: no scene uploaded (pt_scene_upload)");
     pt_status st0;
@@ -960,6 +1010,7 @@ pt_status pt_render_frames(pt_renderer* r, const pt_params* prm, uint32_t nframe
 
 pt_status pt_render_frames_noalloc(pt_renderer* r, const pt_params* prm, uint32_t nframes) {
     if (!r || !prm) return fail(PT_E_INVALID, "pt_render_frames_noalloc: null argument");
+    if (r->lost) return lost_error("pt_render_frames_noalloc");
     pt_ctx* ctx = r->ctx;
     if (!ctx->has_scene) return fail(PT_E_MISSING_INPUT, "missing input: no scene uploaded (pt_scene_upload)");
     if (pt_status sc = check_camera(prm); sc != PT_OK) return sc;
@@ -1003,6 +1054,7 @@ pt_status pt_forward(pt_renderer* r, const pt_params* prm, uint32_t nframes) {
 pt_status pt_backward(pt_renderer* r, const pt_params* prm, uint32_t nframes, const pt_adjoint_params* adj,
                       const float* dL_dimage, float* dparams) {
     if (!r || !prm || !adj || !dL_dimage || !dparams) return fail(PT_E_INVALID, "pt_backward: null argument");
+    if (r->lost) return lost_error("pt_backward");
     pt_ctx* ctx = r->ctx;
     if (!ctx->has_scene) return fail(PT_E_MISSING_INPUT, "missing input: no scene uploaded (pt_scene_upload)");
     const pt_program& pg = r->prog;
@@ -1200,19 +1252,23 @@ pt_status pt_batch_create(pt_ctx* ctx, int32_t n, const pt_scene_desc* scenes, c
     b->cfg.shade_blocks_per_cu = 8;
     b->cfg.stack = b->min_stack;
     b->cfg.use_graph = 1;
+    ctx->batches.push_back(b);
     *out = b;
     return PT_OK;
 }
 
 void pt_batch_destroy(pt_batch* b) {
     if (!b) return;
-    (void)hipSetDevice(b->ctx->device);
-    (void)hipStreamSynchronize(b->ctx->stream);
+    pt_ctx* ctx = b->ctx;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    ctx->batches.erase(std::remove(ctx->batches.begin(), ctx->batches.end(), b), ctx->batches.end());
     delete b;
 }
 
 pt_status pt_batch_set_config(pt_batch* b, const pt_batch_config* cfg) {
     if (!b || !cfg) return fail(PT_E_INVALID, "pt_batch_set_config: null argument");
+    if (b->lost) return lost_error("pt_batch_set_config");
     pt_status st = batch_check_config(b, *cfg);
     if (st != PT_OK) return st;
     b->cfg = *cfg;
@@ -1221,12 +1277,14 @@ pt_status pt_batch_set_config(pt_batch* b, const pt_batch_config* cfg) {
 
 pt_status pt_batch_get_config(const pt_batch* b, pt_batch_config* cfg) {
     if (!b || !cfg) return fail(PT_E_INVALID, "pt_batch_get_config: null argument");
+    if (b->lost) return lost_error("pt_batch_get_config");
     *cfg = b->cfg;
     return PT_OK;
 }
 
 pt_status pt_batch_render(pt_batch* b, const uint32_t* spp) {
     if (!b || !spp) return fail(PT_E_INVALID, "pt_batch_render: null argument");
+    if (b->lost) return lost_error("pt_batch_render");
     pt_ctx* ctx = b->ctx;
     HIP_TRY(hipSetDevice(ctx->device));
     std::vector<uint32_t> sv(spp, spp + b->stores.size());
@@ -1267,6 +1325,7 @@ pt_status pt_batch_render(pt_batch* b, const uint32_t* spp) {
 
 pt_status pt_batch_read_export(pt_batch* b, int32_t scene, float* out, size_t out_floats) {
     if (!b || !out) return fail(PT_E_INVALID, "pt_batch_read_export: null argument");
+    if (b->lost) return lost_error("pt_batch_read_export");
     if (scene < 0 || scene >= (int32_t)b->stores.size()) return fail(PT_E_INVALID, "pt_batch_read_export: no such scene");
     const size_t need = (size_t)b->progs[scene].width * b->progs[scene].height * 4;
     if (out_floats < need) return fail(PT_E_INVALID, "pt_batch_read_export: output buffer too small");
@@ -1279,6 +1338,7 @@ pt_status pt_batch_read_export(pt_batch* b, int32_t scene, float* out, size_t ou
 pt_status pt_batch_autotune(pt_batch* b, const uint32_t* spp, uint32_t probe_div, pt_batch_config* best,
                             pt_batch_config* results, float* ms, int32_t max_results, int32_t* n_results) {
     if (!b || !spp) return fail(PT_E_INVALID, "pt_batch_autotune: null argument");
+    if (b->lost) return lost_error("pt_batch_autotune");
     pt_ctx* ctx = b->ctx;
     HIP_TRY(hipSetDevice(ctx->device));
     std::vector<uint32_t> probe(b->stores.size());
@@ -1347,8 +1407,153 @@ pt_status pt_scene_update_materials(pt_ctx* ctx, const pt_material* materials, i
     return PT_OK;
 }
 
+pt_status pt_scene_update_lights(pt_ctx* ctx, const pt_light* lights, int32_t n) {
+    if (!ctx || (!lights && n)) return fail(PT_E_INVALID, "pt_scene_update_lights: null argument");
+    if (!ctx->has_scene) return fail(PT_E_MISSING_INPUT, "missing input: no scene uploaded (pt_scene_upload)");
+    if (n != (int32_t)ctx->host_lights.size()) return fail(PT_E_INVALID, "pt_scene_update_lights: light count differs from the scene");
+    float prev_cdf = 0.0f;
+    for (int i = 0; i < n; i++) {
+        const pt_light& a = ctx->host_lights[i];
+        const pt_light& b = lights[i];
+        if (a.kind != b.kind || a.region != b.region)
+            return fail(PT_E_INVALID, "pt_scene_update_lights: kind and region must not change");
+        // geometry rows are constant by contract (resolveLightValues, lighting.ts:337-348)
+        if (std::memcmp(a.corner, b.corner, sizeof a.corner) || std::memcmp(a.edge1, b.edge1, sizeof a.edge1) ||
+            std::memcmp(a.edge2, b.edge2, sizeof a.edge2) || std::memcmp(a.normal, b.normal, sizeof a.normal) ||
+            std::memcmp(&a.area, &b.area, sizeof a.area))
+            return fail(PT_E_INVALID, "pt_scene_update_lights: a light's geometry must not change (upload the scene again)");
+        for (int c = 0; c < 3; c++)
+            if (!std::isfinite(b.radiance[c]) || b.radiance[c] < 0.0f)
+                return fail(PT_E_INVALID, "pt_scene_update_lights: emission must be finite and >= 0");
+        if (!(b.select_pdf > 0.0f && b.select_pdf <= 1.0f) || !(b.cdf >= prev_cdf && b.cdf <= 1.0f + 1e-5f))
+            return fail(PT_E_INVALID, "pt_scene_update_lights: select_pdf must lie in (0, 1] and cdf must rise");
+        prev_cdf = b.cdf;
+    }
+    if (n > 0 && std::fabs(prev_cdf - 1.0f) > 1e-5f) return fail(PT_E_INVALID, "pt_scene_update_lights: the selection cdf must end at 1");
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (n) HIP_TRY(hipMemcpy((void*)ctx->dev.lights, lights, sizeof(pt_light) * (size_t)n, hipMemcpyHostToDevice));
+    ctx->host_lights.assign(lights, lights + n);
+    return PT_OK;
+}
+
+pt_status pt_scene_update_env(pt_ctx* ctx, const pt_env* env) {
+    if (!ctx || !env) return fail(PT_E_INVALID, "pt_scene_update_env: null argument");
+    if (!ctx->has_scene) return fail(PT_E_MISSING_INPUT, "missing input: no scene uploaded (pt_scene_upload)");
+    const pt_env& a = ctx->host_env;
+    if (env->type != a.type || env->samplable != a.samplable)
+        return fail(PT_E_INVALID, "pt_scene_update_env: type and samplable must not change");
+    if (a.type == PT_ENV_SKY && (env->chart != a.chart || env->compensated != a.compensated ||
+                                 env->table_h != a.table_h || (a.chart != PT_CHART_OCTAHEDRAL && env->table_w != a.table_w)))
+        return fail(PT_E_INVALID, "pt_scene_update_env: the table's chart, size and compensation must not change "
+                                  "(pt_env_table_upload rebuilds the table)");
+    if (!std::isfinite(env->intensity) || env->intensity < 0.0f || !std::isfinite(env->rotation) ||
+        !(env->select_prob >= 0.0f && env->select_prob <= 1.0f))
+        return fail(PT_E_INVALID, "pt_scene_update_env: intensity >= 0, a finite rotation and select_prob in [0, 1]");
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));   // kernels take the scene record by value at launch
+    set_env_values(ctx->dev, *env);
+    const int32_t chart = a.chart, tw = a.table_w, th = a.table_h, comp = a.compensated;
+    ctx->host_env = *env;
+    ctx->host_env.chart = chart;    // the resident table's (a table upload may have changed them)
+    ctx->host_env.table_w = tw;
+    ctx->host_env.table_h = th;
+    ctx->host_env.compensated = comp;
+    return PT_OK;
+}
+
+pt_status pt_env_table_upload(pt_ctx* ctx, const float* rgb, int32_t W, int32_t H, int32_t chart, int32_t compensated,
+                              double* total_weight) {
+    if (!ctx || !rgb) return fail(PT_E_INVALID, "pt_env_table_upload: null argument");
+    if (!ctx->has_scene) return fail(PT_E_MISSING_INPUT, "missing input: no scene uploaded (pt_scene_upload)");
+    if (ctx->host_env.type != PT_ENV_SKY)
+        return fail(PT_E_INVALID, "pt_env_table_upload: the resident environment has no importance table (PT_ENV_SKY)");
+    if (chart != PT_CHART_EQUIRECT && chart != PT_CHART_OCTAHEDRAL)
+        return fail(PT_E_INVALID, "pt_env_table_upload: chart must be PT_CHART_EQUIRECT or PT_CHART_OCTAHEDRAL");
+    if (W < 1 || H < 1 || W > PT_ENV_MAX_W) return fail(PT_E_INVALID, "pt_env_table_upload: 1 <= width <= 4096 and height >= 1");
+    if (chart == PT_CHART_OCTAHEDRAL && W != H) return fail(PT_E_INVALID, "pt_env_table_upload: the octahedral chart is square");
+    const long long n = (long long)W * H;
+    for (long long k = 0; k < 3 * n; k++)
+        if (!std::isfinite(rgb[k]) || rgb[k] < 0.0f)
+            return fail(PT_E_INVALID, "Environment RGB samples must be finite and nonnegative");
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    SceneStore SS, tmp;
+    SS.stream = tmp.stream = ctx->stream;
+    DevScene D = ctx->dev;
+    pt_status st;
+    const float* d_rgb;
+    float* Y;
+    if ((st = tmp.upload(rgb, (size_t)(3 * n), &d_rgb)) != PT_OK || (st = tmp.alloc((size_t)n, &Y)) != PT_OK) return st;
+    hipLaunchKernelGGL(k_env_luma_rgb, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, d_rgb, Y, n);
+    if ((st = env_table_chain(Y, W, H, chart == PT_CHART_OCTAHEDRAL, compensated != 0, SS, tmp, D)) != PT_OK) return st;
+    double total = 0.0;
+    HIP_TRY(hipMemcpyAsync(&total, D.env_total, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    // every kernel that read the previous table has finished (the stream synchronised above)
+    ctx->env_bufs = std::move(SS.bufs);
+    ctx->dev.env_cond = D.env_cond;
+    ctx->dev.env_marg = D.env_marg;
+    ctx->dev.env_total = D.env_total;
+    ctx->dev.env_chart = chart;
+    ctx->dev.env_w = W;
+    ctx->dev.env_h = H;
+    ctx->dev.env_compensated = compensated ? 1 : 0;
+    ctx->host_env.chart = chart;
+    ctx->host_env.table_w = W;
+    ctx->host_env.table_h = H;
+    ctx->host_env.compensated = compensated ? 1 : 0;
+    if (total_weight) *total_weight = total;
+    return PT_OK;
+}
+
+// A renderer's device resources, released by a context reset.
+static void release_renderer(pt_renderer* r) {
+    r->film.release();
+    r->var_film.release();
+    for (DevBuf& b : r->aov_film) b.release();
+    r->b_state.release();
+    for (DevBuf* b : {&r->b_adjD, &r->b_adjG, &r->b_adjsD, &r->b_delta, &r->b_pixgrad, &r->b_grad}) b->release();
+    r->cap = 0;
+    r->adj_cap = 0;
+    r->paths = Paths{};
+    r->sample_count = 0;
+    r->lost = true;
+}
+
+pt_status pt_ctx_reset(pt_ctx* ctx) {
+    if (!ctx) return fail(PT_E_INVALID, "pt_ctx_reset: null context");
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    for (pt_renderer* r : ctx->renderers) release_renderer(r);
+    for (pt_batch* b : ctx->batches) {
+        b->drop_graph();
+        b->stores.clear();
+        b->d_scenes.release();
+        b->d_cams.release();
+        b->d_entries.release();
+        b->films.release();
+        if (b->r) release_renderer(b->r);
+        b->lost = true;
+    }
+    ctx->scene_bufs.clear();
+    ctx->env_bufs.clear();
+    ctx->dev = DevScene{};
+    ctx->has_scene = false;
+    ctx->host_materials.clear();
+    ctx->host_lights.clear();
+    ctx->host_env = pt_env{};
+    for (DevBuf* b : {&ctx->trav, &ctx->rays_tmp, &ctx->t_tmp, &ctx->p_tmp}) b->release();
+    for (auto& kv : ctx->timings)
+        for (auto& pr : kv.second.ev) { (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second); }
+    ctx->timings.clear();
+    ctx->timing_order.clear();
+    return PT_OK;
+}
+
 pt_status pt_read_export(pt_renderer* r, const char* name, float* out, size_t out_floats) {
     if (!r || !name || !out) return fail(PT_E_INVALID, "pt_read_export: null argument");
+    if (r->lost) return lost_error("pt_read_export");
     const bool var = std::strcmp(name, "variance") == 0;
     const int aov = std::strcmp(name, "albedo") == 0 ? 0 : std::strcmp(name, "normal") == 0 ? 1 : -1;
     if (std::strcmp(name, "radiance") != 0 && !var && aov < 0) return fail(PT_E_INVALID, std::string("unknown export '") + name + "'");
@@ -1366,6 +1571,7 @@ pt_status pt_read_export(pt_renderer* r, const char* name, float* out, size_t ou
 
 pt_status pt_clear_accumulation(pt_renderer* r) {
     if (!r) return fail(PT_E_INVALID, "pt_clear_accumulation: null renderer");
+    if (r->lost) return lost_error("pt_clear_accumulation");
     HIP_TRY(hipSetDevice(r->ctx->device));
     HIP_TRY(hipMemsetAsync(r->film.p, 0, r->film.bytes, r->ctx->stream));
     if (r->var_film.p) HIP_TRY(hipMemsetAsync(r->var_film.p, 0, r->var_film.bytes, r->ctx->stream));
@@ -1375,7 +1581,7 @@ pt_status pt_clear_accumulation(pt_renderer* r) {
     return PT_OK;
 }
 
-uint32_t pt_sample_count(const pt_renderer* r) { return r ? r->sample_count : 0; }
+uint32_t pt_sample_count(const pt_renderer* r) { return r && !r->lost ? r->sample_count : 0; }
 
 pt_status pt_trace_closest_device(pt_ctx* ctx, const float* d_rays, uint32_t n, float* d_t, int32_t* d_prim) {
     if (!ctx || (!d_rays && n)) return fail(PT_E_INVALID, "pt_trace_closest_device: null argument");
@@ -1498,6 +1704,7 @@ int pt_get_timings(pt_ctx* ctx, pt_kernel_timing* out, int n_slots) {
 
 pt_status pt_last_ray_counts(pt_renderer* r, uint64_t* closest, uint64_t* any) {
     if (!r) return fail(PT_E_INVALID, "pt_last_ray_counts: null renderer");
+    if (r->lost) return lost_error("pt_last_ray_counts");
     unsigned long long h[2] = {0, 0};
     if (r->b_state.p) {
         HIP_TRY(hipMemcpyAsync(h, r->paths.hist, sizeof h, hipMemcpyDeviceToHost, r->ctx->stream));

@@ -7,6 +7,54 @@ import numpy as np
 
 from . import _lib, abi
 
+# Live parameter roots (the dotted paths of Engine.setParameter, Engine.ts:258): the authored
+# field each one drives. Geometry is not a parameter: light geometry rows are constant by the
+# reference's contract (resolveLightValues, features/lighting.ts:337-348).
+ENV_PARAMS = {"intensity": "intensity", "rotation": "rotation", "selectProb": "select_prob", "color": "color",
+              "totalWeight": "total_weight"}
+LIGHT_EMISSION_ROWS = ("radiance", "intensity")
+
+
+def with_parameter(source, name, value):
+    """The authored scene with parameter `name` set to value: 'camera.<field>', 'env.intensity' |
+    'env.rotation' | 'env.selectProb' | 'env.color' | 'env.totalWeight', 'lights.<i>.radiance' |
+    'lights.<i>.intensity' (the emission row) and 'materials.<name>.<field>' (a medium's field as
+    'materials.<name>.medium.<field>'). Only the branch the path names is copied."""
+    parts = name.split(".")
+    out = dict(source)
+    if parts[0] == "camera" and len(parts) == 2:
+        out["camera"] = dict(source["camera"], **{parts[1]: value})
+    elif parts[0] == "env" and len(parts) == 2 and parts[1] in ENV_PARAMS:
+        out["environment"] = dict(source.get("environment", {"type": "none"}), **{ENV_PARAMS[parts[1]]: value})
+    elif parts[0] == "lights" and len(parts) == 3:
+        i = int(parts[1])
+        lights = list(source.get("lights", []))
+        if not 0 <= i < len(lights):
+            raise KeyError(f"parameter '{name}': no light {i}")
+        if parts[2] not in LIGHT_EMISSION_ROWS or parts[2] not in lights[i]:
+            raise KeyError(f"parameter '{name}': a light's live row is its emission ({lights[i].get('type')} "
+                           f"light: '{'radiance' if 'radiance' in lights[i] else 'intensity'}')")
+        lights[i] = dict(lights[i], **{parts[2]: value})
+        out["lights"] = lights
+    elif parts[0] == "materials" and len(parts) in (3, 4):
+        mats = dict(source.get("materials", {}))
+        if parts[1] not in mats:
+            raise KeyError(f"parameter '{name}': no material '{parts[1]}'")
+        m = dict(mats[parts[1]])
+        if len(parts) == 4:
+            if parts[2] != "medium" or m.get("medium") is None:
+                raise KeyError(f"parameter '{name}': unknown material field")
+            m["medium"] = dict(m["medium"], **{parts[3]: value})
+        elif parts[2] == "model":
+            raise KeyError(f"parameter '{name}': the material model selects a kernel; compile a new renderer")
+        else:
+            m[parts[2]] = value
+        mats[parts[1]] = m
+        out["materials"] = mats
+    else:
+        raise KeyError(f"unknown parameter '{name}'")
+    return out
+
 
 class Engine:
     def __init__(self, device=0):
@@ -18,6 +66,11 @@ class Engine:
         self._renderers = {}
         self._selected = None
         self._salt = None
+        self._params = {}
+        self._env_table = None
+        self._descriptions = {}   # renderers loaded from a ProgramDescription: its JSON text
+        self.state = "ready"
+        self._lost_selected = None
 
     # -- scene data (registerDataTexture / registerEnvironmentTable) --
     def upload_scene(self, arrays):
@@ -52,6 +105,7 @@ class Engine:
         if rid in self._renderers:
             self.lib.pt_renderer_unload(self._renderers[rid][0])
         self._renderers[rid] = (h, compiled)
+        self._descriptions[rid] = text
         self._selected = rid
         return rid
 
@@ -70,10 +124,115 @@ class Engine:
             raise RuntimeError("no renderer loaded")
         return self._renderers[self._selected]
 
+    # -- live parameters (setParameter / getParameter / getAllParameters, Engine.ts:258-268) --
+    def set_parameter(self, name, value):
+        """Engine.setParameter: the selected renderer's scene with one parameter changed. The
+        host recompiles the authored scene with the new value (the reference's CPU-side compute
+        of derived uniforms: light selection table, env selection probability, majorants) and
+        sends the changed records to the resident scene (pt_scene_update_materials / _lights /
+        _env); the camera rides the per-frame params. Geometry never changes. Accumulation is
+        not cleared (the caller clears, as the reference's triggersReset does)."""
+        from . import compiler
+        _, comp = self._cur()
+        src = with_parameter(comp.source, name, value)
+        new = compiler.compile_scene(src, comp.strategy, comp.width, comp.height, comp.reset_salt)
+        if self._scene is not comp.scene:
+            self.upload_scene(comp.scene)
+        self._apply_records(comp.scene, new.scene)
+        comp.source = src
+        comp.camera = new.camera
+        self._params[name] = value
+
+    def get_parameter(self, name):
+        return self._params.get(name)
+
+    def get_all_parameters(self):
+        return dict(self._params)
+
+    def _apply_records(self, sc, ns):
+        if not (np.array_equal(sc.tri_vertices, ns.tri_vertices) and sc.objects.tobytes() == ns.objects.tobytes()):
+            raise ValueError("a live parameter may not change the scene's geometry")
+        if sc.materials.tobytes() != ns.materials.tobytes():
+            self.update_materials(ns.materials)
+        if sc.lights.tobytes() != ns.lights.tobytes():
+            _lib.check(self.lib.pt_scene_update_lights(self.ctx, ns.lights.ctypes.data, len(ns.lights)))
+            sc.lights[...] = ns.lights
+        if bytes(sc.env) != bytes(ns.env):
+            _lib.check(self.lib.pt_scene_update_env(self.ctx, C.byref(ns.env)))
+            C.memmove(C.byref(sc.env), C.byref(ns.env), C.sizeof(abi.PtEnv))
+
+    # -- environment table (registerEnvironmentTable, Engine.ts:450-460) --
+    def register_environment_table(self, rgb, width, height, chart="equirect", compensation=False):
+        """The resident sky's importance table rebuilt from an RGB picture [height][width][3] on
+        `chart` (pt_env_table_upload); returns {'totalWeight': mass}. The table's size, chart and
+        mass become the env.size / env.totalWeight the derived env selection probability reads,
+        and the picture is kept so a context restore registers it again."""
+        from .compiler import ENV_CHARTS
+        a = np.ascontiguousarray(rgb, np.float32)
+        if a.size != width * height * 3:
+            raise ValueError(f"rgb must hold {height} x {width} x 3 floats")
+        if chart not in ENV_CHARTS:
+            raise ValueError(f"chart {chart!r}: expected one of {sorted(ENV_CHARTS)}")
+        tw = C.c_double()
+        _lib.check(self.lib.pt_env_table_upload(self.ctx, a.ctypes.data, int(width), int(height), ENV_CHARTS[chart],
+                                                int(bool(compensation)), C.byref(tw)))
+        self._env_table = (a.copy(), int(width), int(height), chart, bool(compensation))
+        if self._selected is not None:
+            _, comp = self._cur()
+            env = dict(comp.source.get("environment", {}), table_size=(int(width), int(height)), chart=chart,
+                       compensated=bool(compensation), total_weight=tw.value)
+            comp.source = dict(comp.source, environment=env)
+            from . import compiler
+            new = compiler.compile_scene(comp.source, comp.strategy, comp.width, comp.height, comp.reset_salt)
+            self._apply_records(comp.scene, new.scene)
+        return {"totalWeight": tw.value}
+
+    # -- context loss and restore (Engine._handleContextLoss / _handleContextRestored, Engine.ts:476-520) --
+    def lose_context(self):
+        """Every device resource goes (pt_ctx_reset); the compiled renderers are kept as data.
+        Rendering no-ops while lost, as renderFrame does (Engine.ts:193)."""
+        _lib.check(self.lib.pt_ctx_reset(self.ctx))
+        for h, _ in self._renderers.values():
+            self.lib.pt_renderer_unload(h)
+        self._renderers = {rid: (None, comp) for rid, (_, comp) in self._renderers.items()}
+        self._lost_selected = self._selected
+        self._selected = None
+        self._scene = None
+        self.state = "context-lost"
+
+    def restore_context(self):
+        """Rebuild from the retained compiled renderers: upload the scene, register the last
+        environment table again, load every renderer (fresh, zeroed accumulation) and select the
+        one that was active."""
+        kept = self._renderers
+        self._renderers = {}
+        self.state = "ready"
+        for rid, (_, compiled) in kept.items():
+            if self._scene is not compiled.scene:
+                self.upload_scene(compiled.scene)
+            h = C.c_void_p()
+            text = self._descriptions.get(rid)
+            if text is not None:
+                _lib.check(self.lib.pt_renderer_load_description(self.ctx, text, len(text), compiled.width, compiled.height,
+                                                                 compiled.program.frames_in_flight, C.byref(h)))
+            else:
+                _lib.check(self.lib.pt_renderer_load(self.ctx, C.byref(compiled.program), C.byref(h)))
+            self._renderers[rid] = (h, compiled)
+        if self._lost_selected in self._renderers:
+            self.select_renderer(self._lost_selected)
+        if self._env_table is not None:
+            rgb, w, h2, chart, comp = self._env_table
+            tw = C.c_double()
+            _lib.check(self.lib.pt_env_table_upload(self.ctx, rgb.ctypes.data, w, h2, abi.CHART_OCTAHEDRAL
+                                                    if chart == "octahedral" else abi.CHART_EQUIRECT, int(comp),
+                                                    C.byref(tw)))
+
     def pin_reset_salt(self, salt):
         self._salt = int(salt)
 
     def render_frames(self, n, params=None):
+        if self.state == "context-lost":
+            return
         h, comp = self._cur()
         prm = params if params is not None else comp.params(self._salt)
         _lib.check(self.lib.pt_render_frames(h, C.byref(prm), int(n)))
@@ -176,7 +335,8 @@ class Engine:
     def close(self):
         if self.ctx:
             for h, _ in self._renderers.values():
-                self.lib.pt_renderer_unload(h)
+                if h:
+                    self.lib.pt_renderer_unload(h)
             self._renderers.clear()
             self.lib.pt_ctx_destroy(self.ctx)
             self.ctx = None

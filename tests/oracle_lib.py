@@ -33,6 +33,8 @@ def load(variant=""):
     lib.or_random_stream.argtypes = [C.c_uint32] * 4 + [vp, C.c_int]
     lib.or_math_eval.argtypes = [C.c_int, vp, vp, vp, C.c_int]
     lib.or_env_table.argtypes = [vp, vp, vp]
+    lib.or_scene_env_table_rgb.argtypes = [vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]
+    lib.or_scene_env_table_rgb.restype = C.c_int
     lib.or_env_total.restype = C.c_double
     lib.or_env_total.argtypes = [vp]
     lib.or_region_at.argtypes = [vp, vp]
@@ -74,6 +76,16 @@ class OracleScene:
 
     def __del__(self):
         self.close()
+
+    def register_env_table(self, rgb, width, height, chart=0, compensated=False):
+        """The scene's sky table rebuilt from an RGB picture (registerEnvironmentTable); returns
+        the table's totalWeight."""
+        a = np.ascontiguousarray(rgb, np.float32)
+        tw = C.c_double()
+        if self.lib.or_scene_env_table_rgb(self.h, a.ctypes.data, width, height, chart, int(bool(compensated)),
+                                           C.byref(tw)) != 0:
+            raise ValueError("or_scene_env_table_rgb rejected the picture")
+        return tw.value
 
     def trace_closest(self, rays, brute=False):
         n = rays.shape[1]
