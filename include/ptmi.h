@@ -369,7 +369,7 @@ pt_status pt_render_frames(pt_renderer* r, const pt_params* params, uint32_t nfr
 /* pt_render_frames without any device allocation. It has no counterpart in the reference: it
  * is for callers that cannot allocate device memory (code under HIP graph capture, tight
  * serving loops, harnesses that forbid hipMalloc). The path state lives in one static device
- * array of the library per GPU (about 68 MiB: PT_NOALLOC_MAX_PATHS slots, held until the
+ * array of the library per GPU (about 70 MiB: PT_NOALLOC_MAX_PATHS slots, held until the
  * process ends) instead of the renderer's own buffers; the film is the renderer's, allocated
  * at load. The film may have at most PT_NOALLOC_MAX_PATHS pixels (PT_E_INVALID otherwise),
  * and frames in flight shrink so that width * height * frames fits; the film equals

@@ -40,6 +40,9 @@ long long pt_debug_wide_bvh(const float* tri_vertices, int n_tris, uint32_t* nod
 /* One function of the declared fp32 arithmetic (include/ptmi_math.h) evaluated on the GPU
  * over n host values: out[i] = f(x[i], y[i]). */
 pt_status pt_debug_math(int fn, const float* x, const float* y, float* out, int n);
+/* scene_region_at (features/intersection.ts:1121-1265) on the GPU at n points (xyz, host
+ * memory) of the resident scene: out[i] = the innermost region holding point i, -1 outside all. */
+pt_status pt_debug_region_at(pt_ctx* ctx, const float* pts, int n, int32_t* out);
 /* A plain device copy of `bytes` (a multiple of 16), repeated `iters` times: the kernel of
  * known traffic that scales the FETCH_SIZE / WRITE_SIZE counters of profiled kernels. */
 pt_status pt_calib_copy(pt_ctx* ctx, size_t bytes, int iters);

@@ -2,7 +2,7 @@
 // batch's path slots, in the traversal block size and stack depth of its configuration.
 #include "wave.h"
 
-template <int EST, int STACK>
+template <int EST>
 static void batch_shade(pt_batch* b, int cur) {
     pt_ctx* ctx = b->ctx;
     const pt_program& pg = b->progs[0];
@@ -10,7 +10,8 @@ static void batch_shade(pt_batch* b, int cur) {
     const int g = ctx->cu_count * b->cfg.shade_blocks_per_cu;
     Paths& P = b->r->paths;
     hipStream_t st = ctx->stream;
-#define PT_BSHADE(CATV) hipLaunchKernelGGL((k_shade<EST, CATV, STACK, false, false, false, true>), dim3(g), dim3(SBLOCK), (shade_dynamic_lds<CATV, false, STACK>(b->any_emissive_solid)), st, S0, P, cur, pg.max_bounces, pg.rr_start, pg.rr_max_survival, pg.max_null_crossings, 0)
+#define PT_BSHADE(CATV) hipLaunchKernelGGL((k_shade<EST, CATV, false, false, false, true>), dim3(g), dim3(SBLOCK), 0, st, S0, P, cur, pg.max_bounces, pg.rr_start, pg.rr_max_survival, pg.max_null_crossings, 0)
+    if (b->any_dielectric || b->any_rough) hipLaunchKernelGGL(k_probe<true>, dim3(g), dim3(SBLOCK), 0, st, S0, P);
     PT_BSHADE(CAT_MISS);
     PT_BSHADE(CAT_LAMBERT);
     if (b->any_ggx) PT_BSHADE(CAT_GGX);
@@ -41,9 +42,9 @@ static void batch_waves(pt_batch* b) {
             hipLaunchKernelGGL((k_extend<STACK, false, true, TB>), dim3(tgrid), dim3(TB), 0, st, S0, P, cur);
             hipLaunchKernelGGL((k_bin<false, false, false, true>), dim3(sgrid), dim3(SBLOCK), 0, st, S0, P, cur, 0, pg.max_bounces, pg.max_null_crossings);
             switch (pg.estimator) {
-                case PT_EST_PT: batch_shade<PT_EST_PT, STACK>(b, cur); break;
-                case PT_EST_NEE: batch_shade<PT_EST_NEE, STACK>(b, cur); break;
-                default: batch_shade<PT_EST_MIS, STACK>(b, cur);
+                case PT_EST_PT: batch_shade<PT_EST_PT>(b, cur); break;
+                case PT_EST_NEE: batch_shade<PT_EST_NEE>(b, cur); break;
+                default: batch_shade<PT_EST_MIS>(b, cur);
             }
             if (pg.estimator != PT_EST_PT)
                 hipLaunchKernelGGL((k_shadow<STACK, false, false, true, TB>), dim3(tgrid), dim3(TB), 0, st, S0, P);

@@ -470,4 +470,104 @@ void build_env_table(const pt_env& env, std::vector<float>& cond, std::vector<fl
     for (int j = 0; j < H; j++) { acc += row[j]; marg[j] = (float)(total > 0 ? acc / total : 0); }
 }
 
+// ---------------------------------------------------------------- containment probe map
+void probe_basis(float basis[6]) {
+    const double d[3] = {PROBE_DIR[0], PROBE_DIR[1], PROBE_DIR[2]};
+    const double dl = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    const double n[3] = {d[0] / dl, d[1] / dl, d[2] / dl};
+    // a = normalize(n x z), b = n x a: both across the direction (its z component is the smallest)
+    double a[3] = {n[1], -n[0], 0.0};
+    const double al = std::sqrt(a[0] * a[0] + a[1] * a[1]);
+    a[0] /= al; a[1] /= al;
+    const double b[3] = {n[1] * a[2] - n[2] * a[1], n[2] * a[0] - n[0] * a[2], n[0] * a[1] - n[1] * a[0]};
+    for (int k = 0; k < 3; k++) { basis[k] = (float)a[k]; basis[3 + k] = (float)b[k]; }
+}
+
+// fp32 neighbours of x on the outside: the largest float <= x, the smallest float >= x.
+static float round_down(double x) { float f = (float)x; return (double)f > x ? std::nextafter(f, -INFINITY) : f; }
+static float round_up(double x) { float f = (float)x; return (double)f < x ? std::nextafter(f, INFINITY) : f; }
+
+void build_probe_map(const float* tris12, int n, int first, ProbeMap& m) {
+    probe_basis(m.basis);
+    const double A[3] = {m.basis[0], m.basis[1], m.basis[2]}, B[3] = {m.basis[3], m.basis[4], m.basis[5]};
+    // the triangles' projected boxes and the object's 3D box (f64; b = a + e1, c = a + e2)
+    std::vector<double> fb(4 * (size_t)n);
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int k = 0; k < n; k++) {
+        const float* t = tris12 + 12 * (size_t)k;
+        double v[3][3];
+        for (int a = 0; a < 3; a++) {
+            v[0][a] = t[a];
+            v[1][a] = (double)t[a] + (double)t[4 + a];
+            v[2][a] = (double)t[a] + (double)t[8 + a];
+        }
+        double umin = INFINITY, umax = -INFINITY, vmin = INFINITY, vmax = -INFINITY;
+        for (int c = 0; c < 3; c++) {
+            const double u = A[0] * v[c][0] + A[1] * v[c][1] + A[2] * v[c][2];
+            const double w = B[0] * v[c][0] + B[1] * v[c][1] + B[2] * v[c][2];
+            umin = std::min(umin, u); umax = std::max(umax, u);
+            vmin = std::min(vmin, w); vmax = std::max(vmax, w);
+            for (int a = 0; a < 3; a++) { lo[a] = std::min(lo[a], v[c][a]); hi[a] = std::max(hi[a], v[c][a]); }
+        }
+        fb[4 * k] = umin; fb[4 * k + 1] = umax; fb[4 * k + 2] = vmin; fb[4 * k + 3] = vmax;
+    }
+    // Margin: the Moller-Trumbore test accepts rays that miss the exact triangle by its 1e-6
+    // barycentric tolerance plus rounding (both below 1e-6 of the distance from the ray origin
+    // to the triangle, the origin lying in the object's box), and the device's fp32 projection
+    // and cell index err by under 1e-6 of the point's magnitude: 1e-4 of the box diagonal plus
+    // 1e-5 of the box's largest corner norm covers both a hundredfold.
+    double diag = 0.0, rmax = 0.0;
+    for (int a = 0; a < 3; a++) diag += (hi[a] - lo[a]) * (hi[a] - lo[a]);
+    diag = std::sqrt(diag);
+    for (int c = 0; c < 8; c++) {
+        double r = 0.0;
+        for (int a = 0; a < 3; a++) { const double x = (c >> a) & 1 ? hi[a] : lo[a]; r += x * x; }
+        rmax = std::max(rmax, std::sqrt(r));
+    }
+    const double margin = 1e-4 * diag + 1e-5 * rmax + 1e-30;
+    double U0 = INFINITY, U1 = -INFINITY, V0 = INFINITY, V1 = -INFINITY;
+    for (int k = 0; k < n; k++) {
+        U0 = std::min(U0, fb[4 * k] - margin); U1 = std::max(U1, fb[4 * k + 1] + margin);
+        V0 = std::min(V0, fb[4 * k + 2] - margin); V1 = std::max(V1, fb[4 * k + 3] + margin);
+    }
+    // about two cells per triangle, square cells, at most 2048 a side
+    const double W = U1 - U0, H = V1 - V0;
+    const double cell = std::sqrt(std::max(W * H, 1e-60) / std::max(2.0 * n, 1.0));
+    m.gu = (int)std::min(2048.0, std::max(1.0, std::ceil(W / cell)));
+    m.gv = (int)std::min(2048.0, std::max(1.0, std::ceil(H / cell)));
+    // the device indexes with fp32: keep the grid's origin and scale as the fp32 values it uses
+    m.u0 = (float)U0; m.v0 = (float)V0;
+    m.inv_u = (float)(m.gu / std::max(W, 1e-30));
+    m.inv_v = (float)(m.gv / std::max(H, 1e-30));
+    auto cell_u = [&](double x) { return (int)std::min<double>(m.gu - 1, std::max(0.0, std::floor((x - (double)m.u0) * (double)m.inv_u))); };
+    auto cell_v = [&](double x) { return (int)std::min<double>(m.gv - 1, std::max(0.0, std::floor((x - (double)m.v0) * (double)m.inv_v))); };
+    std::vector<int> count((size_t)m.gu * m.gv + 1, 0);
+    for (int pass = 0; pass < 2; pass++) {
+        if (pass == 1) {
+            m.cell_start.assign((size_t)m.gu * m.gv + 1, 0);
+            for (size_t c = 0; c < (size_t)m.gu * m.gv; c++) m.cell_start[c + 1] = m.cell_start[c] + count[c];
+            m.items.assign((size_t)m.cell_start.back(), 0);
+            m.boxes.assign(4 * (size_t)m.cell_start.back(), 0.0f);
+            std::fill(count.begin(), count.end(), 0);
+        }
+        for (int k = 0; k < n; k++) {
+            const int cu0 = cell_u(fb[4 * k] - margin), cu1 = cell_u(fb[4 * k + 1] + margin);
+            const int cv0 = cell_v(fb[4 * k + 2] - margin), cv1 = cell_v(fb[4 * k + 3] + margin);
+            for (int cv = cv0; cv <= cv1; cv++)
+                for (int cu = cu0; cu <= cu1; cu++) {
+                    const size_t c = (size_t)cv * m.gu + cu;
+                    if (pass == 1) {
+                        const size_t it = (size_t)m.cell_start[c] + count[c];
+                        m.items[it] = first + k;
+                        m.boxes[4 * it] = round_down(fb[4 * k] - margin);
+                        m.boxes[4 * it + 1] = round_up(fb[4 * k + 1] + margin);
+                        m.boxes[4 * it + 2] = round_down(fb[4 * k + 2] - margin);
+                        m.boxes[4 * it + 3] = round_up(fb[4 * k + 3] + margin);
+                    }
+                    count[c]++;
+                }
+        }
+    }
+}
+
 }  // namespace ptmi

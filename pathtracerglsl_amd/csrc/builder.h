@@ -45,6 +45,28 @@ void build_wide_bvh(const float* tri_vertices, int n_tris, WideBVH& out);
 // Binned-SAH binary BVH over boxes [n][6] (f64 min.xyz, max.xyz), leaf size <= leaf_size.
 void build_binary_bvh(const double* boxes, int n, int leaf_size, BinaryBVH& out);
 
+// The containment probe (mesh_inside_bvh, intersection/mesh/mesh.glsl:202-296) casts every ray
+// along one fixed direction, so the triangles a probe ray can hit are those whose projections
+// onto the plane across that direction cover the ray's point. The probe map is a uniform grid
+// over that plane per object: each cell lists the triangles whose projected footprint, widened
+// by a margin far above the rounding of the projection and of the Moller-Trumbore test,
+// overlaps it. The cell of a probe's point therefore holds every triangle the probe ray can
+// hit, and the closest of the cell's hits under the (t, id) rule is the walk's closest hit.
+constexpr float PROBE_DIR[3] = {0.5320544f, 0.7396997f, 0.4114743f};
+struct ProbeMap {
+    float basis[6];                 // u = basis[0..2] . p, v = basis[3..5] . p (fp32 on the device)
+    float u0 = 0, v0 = 0, inv_u = 0, inv_v = 0;
+    int gu = 0, gv = 0;
+    std::vector<int> cell_start;    // gu * gv + 1 offsets into items
+    std::vector<int> items;         // triangle indices (first + k) into the concatenated leaf triangles
+    std::vector<float> boxes;       // per item its widened footprint (umin, umax, vmin, vmax), rounded outward
+};
+// The basis of the plane across PROBE_DIR (f64, rounded once to fp32).
+void probe_basis(float basis[6]);
+// The map of n triangles in the wide-BVH leaf layout (12 floats each: a.xyz,id | e1 | e2),
+// which sit at index `first` of the concatenated triangle array.
+void build_probe_map(const float* tris12, int n, int first, ProbeMap& m);
+
 // Env importance table for the procedural sky: fp32 bake on the equirect grid, then the
 // f64 luminance / blur / CDF build, stored f32 (EnvironmentBake.ts:62-67, env/importance.ts).
 void build_env_table(const pt_env& env, std::vector<float>& cond, std::vector<float>& marg, double& total);

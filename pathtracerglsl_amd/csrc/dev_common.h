@@ -126,6 +126,16 @@ struct DevScene {
     const uint4* ow_nodes;      // per closed mesh wide BVH (same format as cw_nodes), concatenated
     const float4* ow_tris;      // their leaf triangles (ids are global triangle ids)
     const int* ow_root;         // per object: root node in ow_nodes, -1 if none
+    // containment probe maps (per closed mesh / instance batch): the plane basis across the probe
+    // direction, per object (first cell, cells across u, cells across v, 0) and (u0, v0, 1/du, 1/dv),
+    // cell offsets into the items (gu * gv + 1 per object), the items (indices into ow_tris) and
+    // their footprints
+    float pm_basis[6];
+    const int4* pm_head;
+    const float4* pm_grid;
+    const int* pm_cells;
+    const int* pm_items;
+    const float4* pm_boxes;     // per item its widened footprint (umin, umax, vmin, vmax)
     // uniform grid over closed-mesh boxes: candidates of scene_region_at's containment test
     float grid_lo[3], grid_inv[3];
     int grid_n[3];

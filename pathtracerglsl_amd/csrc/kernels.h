@@ -106,18 +106,14 @@ PT_DEV float fisheye_theta(int projection, float rho, float K) {
 }
 
 // path_state_init's current_medium = scene_region_at(camera origin), once per wave.
-template <int STACK>
 __global__ void __launch_bounds__(64) k_camera_medium(DevScene S, Paths P, Camera C) {
-    __shared__ uint2 stk[STACK * 64];
-    if (threadIdx.x == 0) P.counts[C_CAM_MEDIUM] = region_at<64>(S, mk3(C.ox, C.oy, C.oz), stk);
+    if (threadIdx.x == 0) P.counts[C_CAM_MEDIUM] = region_at(S, mk3(C.ox, C.oy, C.oz));
 }
 // The same per path when camera rays do not share an origin (orthographic), after raygen.
-template <int STACK>
 __global__ void __launch_bounds__(SBLOCK) k_ray_medium(DevScene S, Paths P, int n) {
-    __shared__ uint2 stk[STACK * SBLOCK];
     for (int base = blockIdx.x * SBLOCK; base < n; base += gridDim.x * SBLOCK) {
         const int i = base + threadIdx.x;
-        if (i < n) P.medium[i] = region_at<SBLOCK>(S, xyz(P.ray_o[i]), stk);
+        if (i < n) P.medium[i] = region_at(S, xyz(P.ray_o[i]));
     }
 }
 
@@ -378,7 +374,7 @@ __global__ void __launch_bounds__(TBLOCK) k_shadow_media(DevScene S, Paths P) {
         sp4 T = sgrey(1.0f);
         float len_param = 0.0f;     // length travelled inside the parameter medium
         bool blocked = true;
-        int medium = region_at<TBLOCK>(S, o, stk);
+        int medium = region_at(S, o);
         for (int crossed = 0; crossed <= crossings_left; crossed++) {
             float remaining = length(light_p - o) - SHADOW_BACKOFF;
             if (remaining <= 0.0f) { blocked = false; break; }
@@ -395,7 +391,7 @@ __global__ void __launch_bounds__(TBLOCK) k_shadow_media(DevScene S, Paths P) {
             Hit h; bool back;
             fill_hit(S, o, d, tmax, prim, h, back);
             if (!is_null_interface(S, material_of(S, h.owner))) break;
-            classify_hit<TBLOCK>(S, h, back, true, stk);
+            classify_hit(S, h, back, true);
             medium = h.region_to;
             float side = dot(d, h.ng) >= 0.0f ? 1.0f : -1.0f;
             o = h.p + (h.ng * side) * h.eps;
@@ -579,21 +575,42 @@ __global__ void k_bookkeep(Paths P, int cur, int next, int first) {
     }
 }
 
+// probe: the regions on both sides of every transmissive hit of the iteration (the dielectric
+// and rough-dielectric queues; scene_region_at, features/intersection.ts:1121-1265), ahead of
+// their shading. A probe is a chain of dependent loads (the hit, its object, the containment
+// grid cell, the probe-map cell and its candidate triangles) with little arithmetic, and an
+// iteration's queue is too short to fill the GPU with one probe per lane; so each probe runs on
+// a group of PROBE_GROUP lanes that share its candidates, and a small kernel of its own keeps
+// its waves resident where the shading kernel's registers would not.
+constexpr int PROBE_GROUP = 4;
+template <bool BATCH = false>
+__global__ void __launch_bounds__(SBLOCK) k_probe(DevScene Sk, Paths P) {
+    constexpr int G = PROBE_GROUP;
+    const int n0 = P.counts[C_CAT0 + CAT_DIELECTRIC];
+    const int n = n0 + P.counts[C_CAT0 + CAT_ROUGH];
+    const int g = threadIdx.x % G;
+    // the stride is a multiple of G, so a group's lanes share k on every pass (and leave together)
+    for (int w = blockIdx.x * SBLOCK + threadIdx.x; w / G < n; w += gridDim.x * SBLOCK) {
+        const int k = w / G;
+        const int i = k < n0 ? P.catq[CAT_DIELECTRIC][k] : P.catq[CAT_ROUGH][k - n0];
+        const DevScene& S = scene_of<BATCH>(Sk, P, i);
+        const float2 hv = P.hit[i];
+        const f3 d = xyz(P.ray_d[i]);
+        Hit h;
+        hit_geometry(S, xyz(P.ray_o[i]), d, hv.x, __float_as_int(hv.y), h.p, h.ng, h.owner);
+        classify_regions<G>(S, h, !(dot(d, h.ng) < 0.0f), true, g);
+        if (g == 0) P.region[i] = make_int2(h.region_from, h.region_to);
+    }
+}
+
 // The per-vertex body of transport_trace (integrators/pt/pt.ts:299-464) with the combiner
 // (transport/combiner.ts:23-102) and the kernel/light techniques (techniques/kernel/kernel.glsl,
 // kernel_phase.glsl, techniques/light/light.glsl, light_medium.glsl), for the paths of one
 // category. MEDIA selects the media program (region tracking, medium NEE shadow walker).
-template <int EST, int CAT, int STACK, bool MEDIA, bool SPEC, bool ADJ, bool BATCH = false>
+template <int EST, int CAT, bool MEDIA, bool SPEC, bool ADJ, bool BATCH = false>
 __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene Sk, Paths P, int qsel, int max_bounces,
                                                   int rr_start, float rr_max, int max_null, int equiangular = 0) {
     constexpr bool PROBE = cat_probes(CAT, MEDIA);
-    // The region probe's LDS stack: static in the probing categories; the other surface
-    // categories probe only at a back-face hit in a scene with an emissive solid, and are then
-    // launched with the same stack as dynamic LDS (shade_dynamic_lds).
-    constexpr bool STATIC_STK = PROBE && CAT != CAT_MISS && CAT != CAT_MEDIUM;
-    __shared__ uint2 stk_static[STATIC_STK ? STACK * SBLOCK : 1];
-    extern __shared__ uint2 stk_dynamic[];
-    uint2* stk = STATIC_STK ? stk_static : stk_dynamic;
     __shared__ int s_cnt[2], s_base[2];
     int* ctr[2] = {&P.counts[qsel ^ 1], &P.counts[C_SHADOW]};
     const int n = P.counts[C_CAT0 + CAT];
@@ -714,8 +731,14 @@ __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene Sk, Paths P, int qsel
                 Hit h; bool back;
                 fill_hit(S, ro, rd, hv.x, prim, h, back);
                 const pt_material& M = S.materials[S.objects[h.owner].material];
-                bool need = PROBE || (S.emissive_solid && back);
-                classify_hit<SBLOCK>(S, h, back, need, stk);
+                if (cat_preprobed(CAT, MEDIA)) {
+                    const int2 rg = P.region[i];
+                    h.region_from = rg.x;
+                    h.region_to = rg.y;
+                    orient_hit(h, back);
+                } else {
+                    classify_hit(S, h, back, PROBE || (S.emissive_solid && back));
+                }
                 // first-hit guides (pt.ts:422-433): the first non-null surface with no event before
                 if (!SPEC && CAT != CAT_NULL && bounce == 0 && (P.aov[0] || P.aov[1])) {
                     const sp4 Ta = T;                            // the path's weight on arrival

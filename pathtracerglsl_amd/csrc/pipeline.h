@@ -17,6 +17,9 @@ enum { CAT_MISS = 0, CAT_LAMBERT = 1, CAT_GGX = 2, CAT_DIELECTRIC = 3, CAT_MEDIU
 // Categories whose shading classifies every hit's regions (the transmissive models read both
 // sides' indices; media programs track the current medium).
 constexpr bool cat_probes(int cat, bool media) { return cat == CAT_DIELECTRIC || cat == CAT_ROUGH || media; }
+// Categories whose regions k_probe classifies ahead of their shading (opaque-medium programs: the
+// transmissive models); their shade kernels read them from the path state.
+constexpr bool cat_preprobed(int cat, bool media) { return !media && (cat == CAT_DIELECTRIC || cat == CAT_ROUGH); }
 // counts[] slots
 enum { C_RAYQ0 = 0, C_RAYQ1 = 1, C_SHADOW = 2, C_CAT0 = 3, C_EXT_FETCH = C_CAT0 + N_CAT, C_SHD_FETCH, C_CAM_MEDIUM,
        N_COUNTS = 16 };
@@ -34,6 +37,7 @@ struct Paths {
     int* flags;      // bit 0 prev_was_delta, bits 8..15 null crossings, bits 16..31 bounce
     int* medium;     // current_medium region (media scenes)
     float2* hit;     // t (medium event t after binning), primitive id (bits)
+    int2* region;    // (region_from, region_to) of a transmissive hit, from k_probe
     int* q[2];       // ray queues (current / next iteration)
     int* catq[N_CAT];// shading queues
     int* counts;

@@ -251,7 +251,7 @@ pt_status pt_finish(pt_ctx* ctx) {
 // head). Host code never passes the array's name to a kernel or a copy: its device address on
 // each GPU comes from hipGetSymbolAddress, taken once per device with that device current (at
 // the first pt_ctx_create on it) and kept in noalloc_addr.
-constexpr size_t NOALLOC_BYTES_PER_PATH = 272;   // the carve of one slot: 208 B of state, 8 queues, 2 guides
+constexpr size_t NOALLOC_BYTES_PER_PATH = 280;   // the carve of one slot: 212 B of state, 8 queues, 2 guides
 constexpr size_t NOALLOC_BYTES = STATE_HEAD_BYTES + NOALLOC_BYTES_PER_PATH * (size_t)PT_NOALLOC_MAX_PATHS;
 static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
 constexpr int NOALLOC_MAX_DEVICES = 64;
@@ -584,6 +584,10 @@ static pt_status build_scene(const pt_scene_desc* sd, hipStream_t stream, SceneS
     if ((st = SS.upload(oorder.data(), oorder.size(), &D.obj_order)) != PT_OK) return st;
     // per closed mesh / instance batch wide BVH (inside probe, prototype walk), concatenated
     std::vector<uint32_t> own; std::vector<float> owt; std::vector<int> oroot((size_t)std::max(sd->n_objects, 1), -1);
+    std::vector<int4> pm_head((size_t)std::max(sd->n_objects, 1), make_int4(0, 0, 0, 0));
+    std::vector<float4> pm_grid((size_t)std::max(sd->n_objects, 1), make_float4(0, 0, 0, 0));
+    std::vector<int> pm_cells, pm_items;
+    std::vector<float> pm_boxes;
     uint32_t obj_depth = 0;
     for (int o : wide_items) {
         const pt_object& ob = sd->objects[o];
@@ -595,6 +599,15 @@ static pt_status build_scene(const pt_scene_desc* sd, hipStream_t stream, SceneS
         const uint32_t s0 = (uint32_t)obj_shade0(ob);
         for (size_t k = 0; k < w.tris.size() / 12; k++)
             w.tris[12 * k + 3] = ptm_u2f(s0 + ptm_f2u(w.tris[12 * k + 3]));
+        // the containment probe map over the same leaf triangles
+        ProbeMap pm;
+        build_probe_map(w.tris.data(), (int)(w.tris.size() / 12), (int)toff, pm);
+        pm_head[o] = make_int4((int)pm_cells.size(), pm.gu, pm.gv, 0);
+        pm_grid[o] = make_float4(pm.u0, pm.v0, pm.inv_u, pm.inv_v);
+        const int item0 = (int)pm_items.size();
+        for (int c : pm.cell_start) pm_cells.push_back(item0 + c);
+        pm_items.insert(pm_items.end(), pm.items.begin(), pm.items.end());
+        pm_boxes.insert(pm_boxes.end(), pm.boxes.begin(), pm.boxes.end());
         own.insert(own.end(), w.nodes.begin(), w.nodes.end());
         owt.insert(owt.end(), w.tris.begin(), w.tris.end());
         oroot[o] = (int)noff;
@@ -607,6 +620,13 @@ static pt_status build_scene(const pt_scene_desc* sd, hipStream_t stream, SceneS
     if ((st = SS.upload((const uint4*)own.data(), own.size() / 4, &D.ow_nodes)) != PT_OK) return st;
     if ((st = SS.upload((const float4*)owt.data(), owt.size() / 4, &D.ow_tris)) != PT_OK) return st;
     if ((st = SS.upload(oroot.data(), oroot.size(), &D.ow_root)) != PT_OK) return st;
+    probe_basis(D.pm_basis);
+    if ((st = SS.upload(pm_head.data(), pm_head.size(), &D.pm_head)) != PT_OK ||
+        (st = SS.upload(pm_grid.data(), pm_grid.size(), &D.pm_grid)) != PT_OK ||
+        (st = SS.upload(pm_cells.data(), pm_cells.size(), &D.pm_cells)) != PT_OK ||
+        (st = SS.upload(pm_items.data(), pm_items.size(), &D.pm_items)) != PT_OK ||
+        (st = SS.upload((const float4*)pm_boxes.data(), pm_boxes.size() / 4, &D.pm_boxes)) != PT_OK)
+        return st;
     // placements (features/intersection.ts:981-1066): per instance the world->local rotation and
     // (translation, scale), owning batch, first primitive offset; a binary TLAS over the world
     // boxes of the prototype's local box (f64, padded so float ray transforms stay inside)
@@ -872,7 +892,8 @@ static size_t carve_paths(const pt_program& prog, char* base, long long n, Paths
     P.trav = (unsigned long long*)take(2 * sizeof(unsigned long long));
     P.ray_o = (float4*)take(16 * m); P.ray_d = (float4*)take(16 * m); P.thr = (float4*)take(16 * m);
     P.rad = (float4*)take(16 * m); P.rng = (uint2*)take(8 * m); P.prev = (float4*)take(16 * m);
-    P.flags = (int*)take(4 * m); P.hit = (float2*)take(8 * m); P.q[0] = (int*)take(4 * m); P.q[1] = (int*)take(4 * m);
+    P.flags = (int*)take(4 * m); P.hit = (float2*)take(8 * m); P.region = (int2*)take(8 * m);
+    P.q[0] = (int*)take(4 * m); P.q[1] = (int*)take(4 * m);
     P.s_path = (int*)take(4 * m); P.s_o = (float4*)take(16 * m); P.s_d = (float4*)take(16 * m);
     P.s_c = (float4*)take(16 * m); P.s_l = (float4*)take(16 * m); P.medium = (int*)take(4 * m);
     P.wl = (float4*)take(16 * m); P.eta = (float*)take(4 * m); P.path_scene = (int*)take(4 * m);
@@ -1834,6 +1855,33 @@ pt_status pt_debug_math(int fn, const float* x, const float* y, float* out, int 
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpy(out, dout, n * 4, hipMemcpyDeviceToHost));
     HIP_TRY(hipFree(dx)); HIP_TRY(hipFree(dy)); HIP_TRY(hipFree(dout));
+    return PT_OK;
+}
+}  // extern "C"
+
+namespace {
+// scene_region_at over n points of the resident scene (the containment probe's parity hook).
+__global__ void __launch_bounds__(256) k_debug_region(DevScene S, const float* __restrict__ pts, int n, int* __restrict__ out) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) out[i] = region_at(S, mk3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]));
+}
+}  // namespace
+
+extern "C" {
+pt_status pt_debug_region_at(pt_ctx* ctx, const float* pts, int n, int32_t* out) {
+    if (!ctx || !pts || !out || n < 0) return fail(PT_E_INVALID, "pt_debug_region_at: bad argument");
+    if (!ctx->has_scene) return fail(PT_E_MISSING_INPUT, "missing input: no scene uploaded (pt_scene_upload)");
+    if (n == 0) return PT_OK;
+    HIP_TRY(hipSetDevice(ctx->device));
+    DevBuf dp, dout;
+    HIP_TRY(hipMalloc(&dp.p, (size_t)n * 12));
+    HIP_TRY(hipMalloc(&dout.p, (size_t)n * 4));
+    HIP_TRY(hipMemcpyAsync(dp.p, pts, (size_t)n * 12, hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_debug_region, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, ctx->dev, (const float*)dp.p, n,
+                       (int*)dout.p);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(out, dout.p, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
     return PT_OK;
 }
 }  // extern "C"

@@ -30,23 +30,13 @@ PT_DEV Frame ambient_frame(f3 n) {
 
 // ---------------------------------------------------------------- containment
 // mesh_inside_bvh / mesh_closest_bvh (intersection/mesh/mesh.glsl:202-296) and
-// scene_region_at (features/intersection.ts:1121-1265), innermost wins. The first-hit-facing
-// probe walks the object's own wide BVH with the LDS-stack traversal; like every walk here it
-// equals the brute-force nearest triangle under the (t, id) rule.
-template <int BLOCK>
-PT_DEV bool mesh_inside(const DevScene& S, int obj, f3 p, uint2* stk) {
-    const int root = S.ow_root[obj];
-    if (root < 0) return false;
-    const f3 dir = mk3(0.5320544f, 0.7396997f, 0.4114743f);
-    TravState st;
-    trace_init(st, p, dir, 1.0e20f, (uint32_t)root);
-    TraceCounters cnt = {0, 0};
-    while (!trace_step<false, BLOCK, false>(S.ow_nodes, S.ow_tris, st, stk, cnt)) {}
-    if (st.prim < 0) return false;
-    const float4* tp = S.tri_shade + 3 * (size_t)st.prim;
-    f3 a = xyz(tp[0]);
-    return dot(dir, cross(xyz(tp[1]) - a, xyz(tp[2]) - a)) > 0.0f;
-}
+// scene_region_at (features/intersection.ts:1121-1265), innermost wins.
+// The inside probe casts a ray along one fixed direction and asks whether the first triangle it
+// hits faces away. Every probe ray is parallel, so the triangles it can hit are those whose
+// projections onto the plane across the direction cover the ray's point: the probe map's cell
+// of that point lists them with their conservatively widened footprints (builder.h); the ones
+// whose footprint holds the point are tested, and the closest of their Moller-Trumbore hits under
+// the (t, id) rule is the walk's closest hit, with no traversal.
 PT_DEV float point_tri_dist2(f3 p, f3 a, f3 b, f3 c) {
     f3 ab = b - a, ac = c - a, ap = p - a;
     float d1 = dot(ab, ap), d2 = dot(ac, ap);
@@ -70,6 +60,72 @@ PT_DEV float point_tri_dist2(f3 p, f3 a, f3 b, f3 c) {
     f3 closest = (a + ab * (vb * denom)) + ac * (vc * denom);
     f3 q = p - closest;
     return dot(q, q);
+}
+// The diagonal of an object's box (bounds its points' distances to its triangles).
+PT_DEV float box_diagonal(const pt_object& ob) {
+    return length(mk3(ob.box_max[0] - ob.box_min[0], ob.box_max[1] - ob.box_min[1], ob.box_max[2] - ob.box_min[2]));
+}
+// With G > 1 a group of G consecutive lanes answers one probe together (every lane of the group
+// calls with the same object and point, lane g of the group testing candidates g, g + G, ...), and
+// the group's closest hit is reduced across its lanes: the probe's candidates load in parallel.
+template <int G>
+PT_DEV void group_closest(float& t, int& id) {
+#pragma unroll
+    for (int m = 1; m < G; m <<= 1) {
+        const float t2 = __shfl_xor(t, m, G);
+        const int id2 = __shfl_xor(id, m, G);
+        if (t2 < t || (t2 == t && id2 < id)) { t = t2; id = id2; }
+    }
+}
+// on_surface (when given) is set when the probe reports inside and p lies on one of the
+// footprint candidates (point_tri_dist2 == 0): a triangle at distance 0 is one of them, since its
+// footprint holds p's projection.
+template <int G = 1>
+PT_DEV bool mesh_inside(const DevScene& S, int obj, f3 p, int g = 0, bool* on_surface = nullptr) {
+    const int4 h = S.pm_head[obj];
+    if (h.y == 0) return false;
+    const float4 gr = S.pm_grid[obj];
+    const float u = (S.pm_basis[0] * p.x + S.pm_basis[1] * p.y) + S.pm_basis[2] * p.z;
+    const float v = (S.pm_basis[3] * p.x + S.pm_basis[4] * p.y) + S.pm_basis[5] * p.z;
+    const float fu = floorf((u - gr.x) * gr.z), fv = floorf((v - gr.y) * gr.w);
+    if (!(fu >= 0.0f && fu < (float)h.y && fv >= 0.0f && fv < (float)h.z)) return false;   // no footprint here
+    const int c = h.x + (int)fv * h.y + (int)fu;
+    const int beg = S.pm_cells[c], end = S.pm_cells[c + 1];
+    const f3 dir = mk3(0.5320544f, 0.7396997f, 0.4114743f);
+    float tb = 1.0e20f;
+    int best = -1;
+    for (int k = beg + g; k < end; k += G) {
+        const float4 fb = S.pm_boxes[k];                     // contiguous per cell: cheap rejection
+        if (u < fb.x || u > fb.y || v < fb.z || v > fb.w) continue;
+        const float4* tp = S.ow_tris + 3 * (size_t)S.pm_items[k];
+        const float4 ta = tp[0], te1 = tp[1], te2 = tp[2];
+        float t;
+        if (tri_test(p, dir, xyz(ta), xyz(te1), xyz(te2), t) && t > 0.0f) {
+            const int id = (int)__float_as_uint(ta.w);
+            if (t < tb || (t == tb && id < best)) { tb = t; best = id; }
+        }
+    }
+    if (G > 1) group_closest<G>(tb, best);
+    if (best < 0) return false;
+    const float4* tp = S.tri_shade + 3 * (size_t)best;
+    f3 a = xyz(tp[0]);
+    const bool inside = dot(dir, cross(xyz(tp[1]) - a, xyz(tp[2]) - a)) > 0.0f;
+    if (inside && on_surface) {
+        bool on = false;
+        for (int k = beg + g; k < end; k += G) {
+            const float4 fb = S.pm_boxes[k];
+            if (u < fb.x || u > fb.y || v < fb.z || v > fb.w) continue;
+            const int id = (int)__float_as_uint(S.ow_tris[3 * (size_t)S.pm_items[k]].w);
+            const float4* sp = S.tri_shade + 3 * (size_t)id;
+            on = on || point_tri_dist2(p, xyz(sp[0]), xyz(sp[1]), xyz(sp[2])) == 0.0f;
+        }
+        if (G > 1) {
+#pragma unroll
+            for (int m = 1; m < G; m <<= 1) on = on || __shfl_xor((int)on, m, G) != 0;
+        }
+        *on_surface = on;
+    }
+    return inside;
 }
 PT_DEV float box_dist2(float4 n0, float4 n1, f3 p) {
     f3 d = mk3(gmax(gmax(n0.x - p.x, 0.0f), p.x - n1.x), gmax(gmax(n0.y - p.y, 0.0f), p.y - n1.y),
@@ -103,15 +159,30 @@ PT_DEV float mesh_closest(const DevScene& S, int obj, f3 p) {
     }
     return sqrtf(best2);
 }
-template <int BLOCK>
-PT_DEV int region_at(const DevScene& S, f3 p, uint2* stk) {
+// scene_region_at: every closed object holding p has depth d = -(distance to its nearest
+// triangle) (sphere: |p - c| - r), and the region is the deepest one, the first in the
+// reference's order on ties (spheres, then the grid's meshes, then the instance batches). It runs
+// in two passes. The first only asks which objects hold p (sphere test, box test and inside
+// probe) and, when exactly one does, returns it: the depths only order several holders. That
+// holder's depth is strictly negative and above the -1e20 floor the comparison starts from, which
+// the first pass makes sure of: a mesh's distance is at most its box's diagonal times the
+// placement scale (checked against 1e19), and it is 0 only if p lies on one of the triangles,
+// which then is one of the probe's footprint candidates (mesh_inside reports it). Otherwise, and
+// with two or more holders, the second pass computes the depths (the closest-triangle search).
+template <int G = 1, bool DEPTHS>
+PT_DEV int region_pass(const DevScene& S, f3 p, int g, int& holders) {
     int region = -1; float best = -1.0e20f;
+    holders = 0;
     for (int k = 0; k < S.n_analytic; k++) {
         int o = S.analytic[k];
         const pt_object& ob = S.objects[o];
         if (ob.kind != PT_OBJ_SPHERE) continue;
         float d = length(p - ld3(ob.p)) - ob.p[3];
-        if (d < 0.0f && d > best) { best = d; region = o; }
+        if (d < 0.0f && d > best) {
+            if (DEPTHS) best = d;
+            region = o;
+            holders++;
+        }
     }
     if (S.has_grid) {
         int c[3];
@@ -127,11 +198,18 @@ PT_DEV int region_at(const DevScene& S, f3 p, uint2* stk) {
             for (int j = 0; j < cell.y; j++) {
                 int o = S.grid_items[cell.x + j];
                 const pt_object& ob = S.objects[o];
-                float d = 1.0e20f;
+                bool on_surface = false;
                 if (p.x >= ob.box_min[0] && p.y >= ob.box_min[1] && p.z >= ob.box_min[2] &&
-                    p.x <= ob.box_max[0] && p.y <= ob.box_max[1] && p.z <= ob.box_max[2] && mesh_inside<BLOCK>(S, o, p, stk))
-                    d = -1.0f * mesh_closest(S, o, p);
-                if (d < 0.0f && d > best) { best = d; region = o; }
+                    p.x <= ob.box_max[0] && p.y <= ob.box_max[1] && p.z <= ob.box_max[2] &&
+                    mesh_inside<G>(S, o, p, g, &on_surface)) {
+                    if (DEPTHS) {
+                        const float d = -1.0f * mesh_closest(S, o, p);
+                        if (d < 0.0f && d > best) { best = d; region = o; }
+                    } else {
+                        region = o;
+                        holders += on_surface || !(box_diagonal(ob) < 1e19f) ? 2 : 1;
+                    }
+                }
             }
         }
     }
@@ -153,11 +231,18 @@ PT_DEV int region_at(const DevScene& S, f3 p, uint2* stk) {
                     const float4 q = S.inst_q[i], ts = S.inst_ts[i];
                     const f3 r = qrot(q, p) + xyz(ts);
                     const f3 lp = mk3(r.x / ts.w, r.y / ts.w, r.z / ts.w);
-                    float d = 1.0e20f;
+                    bool on_surface = false;
                     if (lp.x >= ob.box_min[0] && lp.y >= ob.box_min[1] && lp.z >= ob.box_min[2] &&
-                        lp.x <= ob.box_max[0] && lp.y <= ob.box_max[1] && lp.z <= ob.box_max[2] && mesh_inside<BLOCK>(S, o, lp, stk))
-                        d = -ts.w * mesh_closest(S, o, lp);
-                    if (d < 0.0f && d > best) { best = d; region = o; }
+                        lp.x <= ob.box_max[0] && lp.y <= ob.box_max[1] && lp.z <= ob.box_max[2] &&
+                        mesh_inside<G>(S, o, lp, g, &on_surface)) {
+                        if (DEPTHS) {
+                            const float d = -ts.w * mesh_closest(S, o, lp);
+                            if (d < 0.0f && d > best) { best = d; region = o; }
+                        } else {
+                            region = o;
+                            holders += on_surface || !(ts.w * box_diagonal(ob) < 1e19f) ? 2 : 1;
+                        }
+                    }
                 }
             } else if (sp + 2 < 64) {
                 stack[++sp] = ni + 1; stack[++sp] = (int)n1.w;
@@ -165,6 +250,13 @@ PT_DEV int region_at(const DevScene& S, f3 p, uint2* stk) {
         }
     }
     return region;
+}
+template <int G = 1>
+PT_DEV int region_at(const DevScene& S, f3 p, int g = 0) {
+    int holders;
+    const int region = region_pass<G, false>(S, p, g, holders);
+    if (holders <= 1) return region;
+    return region_pass<G, true>(S, p, g, holders);
 }
 PT_DEV bool object_thin(const DevScene& S, int o) {
     const pt_object& ob = S.objects[o];
@@ -246,67 +338,90 @@ PT_DEV f3 smooth_normal(const DevScene& S, int tri, f3 o, f3 d, f3 a, f3 b, f3 c
     if (dot(d, ns) * dot(d, g) < 0.0f) ns = -ns;
     return ns;
 }
-PT_DEV void fill_hit(const DevScene& S, f3 o, f3 d, float t, int prim, Hit& h, bool& back) {
-    h.t = t;
-    h.p = o + d * t;
-    h.eps = fp_uncertainty(h.p);
+// The hit point, the unflipped geometric normal and the owning region of a closest hit: the part
+// of the hit fill the region probes read.
+PT_DEV void hit_geometry(const DevScene& S, f3 o, f3 d, float t, int prim, f3& p, f3& ng, int& owner) {
+    p = o + d * t;
     if (prim < S.n_tris) {
         const float4* tp = S.tri_shade + 3 * (size_t)prim;
-        f3 a = xyz(tp[0]);
-        f3 g = cross(xyz(tp[1]) - a, xyz(tp[2]) - a);
-        h.owner = S.tri_object[prim];
-        h.frame = ambient_frame(normalize(S.objects[h.owner].smooth ? smooth_normal(S, prim, o, d, a, xyz(tp[1]), xyz(tp[2])) : g));
-        h.ng = normalize(g);
+        const f3 a = xyz(tp[0]);
+        ng = normalize(cross(xyz(tp[1]) - a, xyz(tp[2]) - a));
+        owner = S.tri_object[prim];
     } else if (prim >= S.inst_prim0 && S.n_instances > 0) {
-        // instanceLeafItem mesh arm: the prototype triangle, normals by the forward rotation
+        // instanceLeafItem mesh arm: the prototype triangle's normal by the forward rotation
         const int rel = prim - S.inst_prim0;
         const int lo = instance_of(S, prim);
         const int batch = S.inst_batch[lo];
         const int tri = S.n_tris + S.objects[batch].first_tri + (rel - S.inst_base[lo]);
         const float4* tp = S.tri_shade + 3 * (size_t)tri;
-        f3 a = xyz(tp[0]);
-        f3 gl = cross(xyz(tp[1]) - a, xyz(tp[2]) - a);
+        const f3 a = xyz(tp[0]);
         const float4 q = S.inst_q[lo];
-        const float4 qc = make_float4(-q.x, -q.y, -q.z, q.w);
-        const f3 ng = normalize(qrot(qc, gl));
-        f3 ns = ng;
-        if (S.objects[batch].smooth) {   // the placement's local ray, as the prototype walk saw it
+        ng = normalize(qrot(make_float4(-q.x, -q.y, -q.z, q.w), cross(xyz(tp[1]) - a, xyz(tp[2]) - a)));
+        owner = batch;
+    } else {
+        const int o2 = prim - S.n_tris;
+        const pt_object& ob = S.objects[o2];
+        ng = normalize(ob.kind == PT_OBJ_QUAD ? ld3(ob.p + 9) : normalize(p - ld3(ob.p)));
+        owner = o2;
+    }
+}
+PT_DEV void fill_hit(const DevScene& S, f3 o, f3 d, float t, int prim, Hit& h, bool& back) {
+    h.t = t;
+    hit_geometry(S, o, d, t, prim, h.p, h.ng, h.owner);
+    h.eps = fp_uncertainty(h.p);
+    if (prim < S.n_tris) {
+        const float4* tp = S.tri_shade + 3 * (size_t)prim;
+        f3 a = xyz(tp[0]);
+        f3 g = cross(xyz(tp[1]) - a, xyz(tp[2]) - a);
+        h.frame = ambient_frame(normalize(S.objects[h.owner].smooth ? smooth_normal(S, prim, o, d, a, xyz(tp[1]), xyz(tp[2])) : g));
+    } else if (prim >= S.inst_prim0 && S.n_instances > 0) {
+        f3 ns = h.ng;
+        if (S.objects[h.owner].smooth) {   // the placement's local ray, as the prototype walk saw it
+            const int rel = prim - S.inst_prim0;
+            const int lo = instance_of(S, prim);
+            const int tri = S.n_tris + S.objects[h.owner].first_tri + (rel - S.inst_base[lo]);
+            const float4* tp = S.tri_shade + 3 * (size_t)tri;
+            const f3 a = xyz(tp[0]);
+            const float4 q = S.inst_q[lo];
+            const float4 qc = make_float4(-q.x, -q.y, -q.z, q.w);
             const float4 ts = S.inst_ts[lo];
             const f3 r = qrot(q, o) + xyz(ts), rd = qrot(q, d);
             const f3 lo3 = mk3(r.x / ts.w, r.y / ts.w, r.z / ts.w), ld = mk3(rd.x / ts.w, rd.y / ts.w, rd.z / ts.w);
             ns = normalize(qrot(qc, smooth_normal(S, tri, lo3, ld, a, xyz(tp[1]), xyz(tp[2]))));
         }
         h.frame = ambient_frame(ns);
-        h.ng = ng;
-        h.owner = batch;
     } else {
-        int o2 = prim - S.n_tris;
-        const pt_object& ob = S.objects[o2];
-        f3 n = ob.kind == PT_OBJ_QUAD ? ld3(ob.p + 9) : normalize(h.p - ld3(ob.p));
-        h.frame = ambient_frame(n);
-        h.ng = h.frame.n;
-        h.owner = o2;
+        const pt_object& ob = S.objects[h.owner];
+        // ambient_frame normalises its argument, so frame.n is the normal hit_geometry returned
+        h.frame = ambient_frame(ob.kind == PT_OBJ_QUAD ? ld3(ob.p + 9) : normalize(h.p - ld3(ob.p)));
     }
     back = !(dot(d, h.ng) < 0.0f);
 }
-template <int BLOCK>
-PT_DEV void classify_hit(const DevScene& S, Hit& h, bool back, bool need, uint2* stk) {
+template <int G = 1>
+PT_DEV void classify_regions(const DevScene& S, Hit& h, bool back, bool need, int g = 0) {
     // h.ng is still the unflipped geometric normal here
     if (need) {
-        int outside = region_at<BLOCK>(S, h.p + h.ng * EPS_INTERFACE, stk);
+        int outside = region_at<G>(S, h.p + h.ng * EPS_INTERFACE, g);
         if (!back) { h.region_from = outside; h.region_to = h.owner; }
         else {
-            h.region_from = object_thin(S, h.owner) ? region_at<BLOCK>(S, h.p + h.ng * (-EPS_INTERFACE), stk) : h.owner;
+            h.region_from = object_thin(S, h.owner) ? region_at<G>(S, h.p + h.ng * (-EPS_INTERFACE), g) : h.owner;
             h.region_to = outside;
         }
     } else {
         h.region_from = -2;
         h.region_to = back ? -2 : h.owner;   // -2: not read (no emissive solid can be behind)
     }
+}
+// The dispatcher's orientation step: a back-face hit turns the frame and normal to the ray's side.
+PT_DEV void orient_hit(Hit& h, bool back) {
     if (back) {
         h.frame = ambient_frame(-h.frame.n);
         h.ng = -h.ng;
     }
+}
+PT_DEV void classify_hit(const DevScene& S, Hit& h, bool back, bool need) {
+    classify_regions(S, h, back, need);
+    orient_hit(h, back);
 }
 
 // ---------------------------------------------------------------- spectra

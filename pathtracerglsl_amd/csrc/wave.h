@@ -5,16 +5,7 @@
 #include "kernels.h"
 #include "host.h"
 
-// Dynamic LDS for a k_shade launch: the region probe's stack for the surface categories
-// without a static one, in scenes whose emissive solids make them probe at back-face hits.
-template <int CAT, bool MEDIA, int STACK>
-static size_t shade_dynamic_lds(bool emissive_solid) {
-    constexpr bool STATIC_STK = cat_probes(CAT, MEDIA) && CAT != CAT_MISS && CAT != CAT_MEDIUM;
-    constexpr bool SURFACE = CAT != CAT_MISS && CAT != CAT_MEDIUM;
-    return (!STATIC_STK && SURFACE && emissive_solid) ? (size_t)STACK * SBLOCK * sizeof(uint2) : 0;
-}
-
-template <int EST, int STACK, bool MEDIA, bool SPEC, bool ADJ>
+template <int EST, bool MEDIA, bool SPEC, bool ADJ>
 static void launch_shade(pt_renderer* r, int cur) {
     pt_ctx* ctx = r->ctx;
     const pt_program& pg = r->prog;
@@ -22,7 +13,9 @@ static void launch_shade(pt_renderer* r, int cur) {
     hipStream_t st = ctx->stream;
     const int g = ctx->shade_blocks;
     const int mn = pg.max_null_crossings;
-#define PT_SHADE(name, CATV) TIMED(ctx, name, 0, hipLaunchKernelGGL((k_shade<EST, CATV, STACK, MEDIA, SPEC, ADJ>), dim3(g), dim3(SBLOCK), (shade_dynamic_lds<CATV, MEDIA, STACK>(S.emissive_solid != 0)), st, S, r->paths, cur, pg.max_bounces, pg.rr_start, pg.rr_max_survival, mn, pg.medium_light_sampling))
+#define PT_SHADE(name, CATV) TIMED(ctx, name, 0, hipLaunchKernelGGL((k_shade<EST, CATV, MEDIA, SPEC, ADJ>), dim3(g), dim3(SBLOCK), 0, st, S, r->paths, cur, pg.max_bounces, pg.rr_start, pg.rr_max_survival, mn, pg.medium_light_sampling))
+    if (!MEDIA && (S.has_transmission || ctx->has_rough))
+        TIMED(ctx, "probe", 0, hipLaunchKernelGGL(k_probe<false>, dim3(g), dim3(SBLOCK), 0, st, S, r->paths));
     PT_SHADE("shade_miss", CAT_MISS);
     PT_SHADE("shade_lambert", CAT_LAMBERT);
     if (ctx->has_ggx) PT_SHADE("shade_ggx", CAT_GGX);
@@ -44,9 +37,9 @@ static void launch_wave(pt_renderer* r, const Camera& cam, int frames, uint32_t 
     const int W = pg.width, H = pg.height;
     const long long n = (long long)W * H * frames;
     hipStream_t st = ctx->stream;
-    if (MEDIA) hipLaunchKernelGGL((k_camera_medium<STACK>), dim3(1), dim3(64), 0, st, S, P, cam);
+    if (MEDIA) hipLaunchKernelGGL(k_camera_medium, dim3(1), dim3(64), 0, st, S, P, cam);
     TIMED(ctx, "raygen", n,  hipLaunchKernelGGL((k_raygen<SPEC, ADJ>), dim3((unsigned)((n + SBLOCK - 1) / SBLOCK)), dim3(SBLOCK), 0, st, P, cam, W, H, (int)n, sample0, MEDIA ? 1 : 0, pg.wavelengths));
-    if (MEDIA && (cam.ortho || cam.thin)) hipLaunchKernelGGL((k_ray_medium<STACK>), dim3(ctx->shade_blocks), dim3(SBLOCK), 0, st, S, P, (int)n);
+    if (MEDIA && (cam.ortho || cam.thin)) hipLaunchKernelGGL(k_ray_medium, dim3(ctx->shade_blocks), dim3(SBLOCK), 0, st, S, P, (int)n);
     int cur = 0;
     const int tgrid = ctx->persist_blocks;
     // a null crossing consumes an iteration but no bounce (pt.ts null-interface branch)
@@ -69,9 +62,9 @@ static void launch_wave(pt_renderer* r, const Camera& cam, int frames, uint32_t 
             hipLaunchKernelGGL(k_shadow_reset, dim3(1), dim3(64), 0, st, P);
         }
         switch (pg.estimator) {
-            case PT_EST_PT: launch_shade<PT_EST_PT, STACK, MEDIA, SPEC, ADJ>(r, cur); break;
-            case PT_EST_NEE: launch_shade<PT_EST_NEE, STACK, MEDIA, SPEC, ADJ>(r, cur); break;
-            default: launch_shade<PT_EST_MIS, STACK, MEDIA, SPEC, ADJ>(r, cur);
+            case PT_EST_PT: launch_shade<PT_EST_PT, MEDIA, SPEC, ADJ>(r, cur); break;
+            case PT_EST_NEE: launch_shade<PT_EST_NEE, MEDIA, SPEC, ADJ>(r, cur); break;
+            default: launch_shade<PT_EST_MIS, MEDIA, SPEC, ADJ>(r, cur);
         }
         if (pg.estimator != PT_EST_PT) {
             if (MEDIA && inst) TIMED(ctx, "shadow_media", 0, hipLaunchKernelGGL((k_shadow_media<STACK, SPEC, ADJ, true>), dim3(tgrid), dim3(TBLOCK), 0, st, S, P));
