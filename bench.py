@@ -6,7 +6,9 @@ noise-displaced detail-4 icospheres (60 % GGX, 20 % dielectric, 20 % Lambert) un
 procedural sky, pt-mis with env NEE, 1920x1080, 64 spp, depth 12, Russian roulette from
 bounce 3, through the 8-wide compressed BVH. One step = one full 64-spp render of the frame
 (pt_render_frames(64) + pt_finish); inputs (scene, BVH, env table) are resident in HBM
-before the timed region.
+before the timed region. All 64 frames are in flight at once: one wave of 132.7 M paths
+(about 37 GB of path state), so the late, short bounces launch 13 times per step rather than
+208 times (each launch of a short bounce costs the latency of its longest traversal).
 
 Adjoint leg (the differentiable media inverse render): the glass-medium Cornell box with 16 grey
 Lambert patches at 512x512, 64 spp, depth 32, pt-mis; the adjoint image is the L2 loss gradient
@@ -99,7 +101,7 @@ def adjoint_leg(Engine, compiler, scenes, abi, fp64=True, res=512, spp=64):
     """Forward / backward cost of the path-replay adjoint and its error vs the FP64 replay."""
     import numpy as np
     target, init = scenes.inverse_media_params()
-    strategy = compiler.RenderStrategy("mis", 32, 3, 0.95, 4)
+    strategy = compiler.RenderStrategy("mis", 32, 3, 0.95, 64)
     eng = Engine(0)
     ct = compiler.compile_scene(scenes.inverse_media_scene(target), strategy, res, res)
     eng.load_renderer(ct)
@@ -145,7 +147,7 @@ def batch_leg(Engine, compiler, scenes, abi):
     import numpy as np
     from pathtracerglsl_amd.batch import BatchRenderer
     items = scenes.batch_workload()
-    strategy = compiler.RenderStrategy("mis", 12, 3, 0.95, 4)
+    strategy = compiler.RenderStrategy("mis", 12, 3, 0.95, 64)    # sequential launches: all frames in flight
     comps = [compiler.compile_scene(sc, strategy, w, h) for sc, w, h, _ in items]
     spp = [n for _, _, _, n in items]
     paths = sum(c.width * c.height * n for c, n in zip(comps, spp))
@@ -195,7 +197,7 @@ def main():
     ap.add_argument("--spp", type=int, default=64)
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
-    ap.add_argument("--frames-in-flight", type=int, default=4)
+    ap.add_argument("--frames-in-flight", type=int, default=64)
     ap.add_argument("--cpu-spp", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
@@ -261,7 +263,9 @@ def main():
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "extend_traffic.json")
     if os.path.exists(tpath) and (args.width, args.height, args.spp) == (1920, 1080, 64):
-        traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
+        rec = json.load(open(tpath))
+        if rec.get("frames_in_flight") == args.frames_in_flight:
+            traffic = rec.get("hbm_bytes_per_launch")
 
     paths = args.width * args.height * args.spp * args.steps
     ms_per_step = elapsed * 1e3 / args.steps
@@ -309,7 +313,7 @@ def main():
         result["cornell"] = dict(config_leg(Engine, c0, 16, cpu=None if args.no_cpu else (c0, 16, "the whole workload")),
                                  workload="cornell 256x256 16spp depth8 pt-mis rgb")
         log(f"[bench] cornell {result['cornell']}")
-        st2 = compiler.RenderStrategy("mis", 32, 3, 0.95, 4, color="spectral")
+        st2 = compiler.RenderStrategy("mis", 32, 3, 0.95, 64, color="spectral")
         c2 = compiler.compile_scene(scenes.cornell_spectral_medium(), st2, 1024, 1024)
         c2s = compiler.compile_scene(scenes.cornell_spectral_medium(), st2, 128, 128)
         result["spectral_media"] = dict(config_leg(Engine, c2, 256, cpu=None if args.no_cpu else (c2s, 16, "128x128 16spp of the same scene")),

@@ -1371,35 +1371,47 @@ pt_status pt_batch_autotune(pt_batch* b, const uint32_t* spp, uint32_t probe_div
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
-    const int waves[] = {1 << 21, 1 << 22, 1 << 23};
+    // two stages: the wave size at the starting launch shape, then the launch shape (trace block,
+    // blocks per CU, stack depth) at the best wave size
+    auto trial = [&](const pt_batch_config& c) -> pt_status {
+        if (batch_check_config(b, c) != PT_OK) return PT_OK;
+        b->cfg = c;
+        pt_status st = pt_batch_render(b, probe.data());       // build + warm
+        if (st != PT_OK) return st;
+        HIP_TRY(hipEventRecord(e0, ctx->stream));
+        if ((st = pt_batch_render(b, probe.data())) != PT_OK) return st;
+        HIP_TRY(hipEventRecord(e1, ctx->stream));
+        HIP_TRY(hipEventSynchronize(e1));
+        float t = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&t, e0, e1));
+        if (results && ms && count < max_results) { results[count] = c; ms[count] = t; }
+        count++;
+        if (t < best_ms) { best_ms = t; winner = c; }
+        return PT_OK;
+    };
+    pt_status st = PT_OK;
+    const int waves[] = {1 << 22, 1 << 23, 1 << 24, 1 << 25, 1 << 26};
+    for (int wp : waves) {
+        pt_batch_config c = start;
+        c.wave_paths = std::max(wp, b->max_npix);
+        c.use_graph = 1;
+        if ((st = trial(c)) != PT_OK) break;
+    }
+    const pt_batch_config stage1 = winner;
     const int tbs[] = {64, 128, 256};
-    const int bpcs[] = {4, 8};
+    const int bpcs[] = {4, 8, 16};
     const int stacks[] = {16, 32};
-    for (int wp : waves)
-        for (int tb : tbs)
-            for (int bpc : bpcs)
-                for (int sk : stacks) {
-                    pt_batch_config c = start;
-                    c.wave_paths = std::max(wp, b->max_npix);
-                    c.trace_block = tb;
-                    c.trace_blocks_per_cu = std::max(1, bpc * 128 / tb);   // bpc x 128 lanes per CU
-                    c.stack = sk;
-                    c.use_graph = 1;
-                    if (batch_check_config(b, c) != PT_OK) continue;
-                    b->cfg = c;
-                    pt_status st = pt_batch_render(b, probe.data());       // build + warm
-                    if (st != PT_OK) { (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); return st; }
-                    HIP_TRY(hipEventRecord(e0, ctx->stream));
-                    st = pt_batch_render(b, probe.data());
-                    if (st != PT_OK) { (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); return st; }
-                    HIP_TRY(hipEventRecord(e1, ctx->stream));
-                    HIP_TRY(hipEventSynchronize(e1));
-                    float t = 0.0f;
-                    HIP_TRY(hipEventElapsedTime(&t, e0, e1));
-                    if (results && ms && count < max_results) { results[count] = c; ms[count] = t; }
-                    count++;
-                    if (t < best_ms) { best_ms = t; winner = c; }
-                }
+    for (int tb : tbs)
+        for (int bpc : bpcs)
+            for (int sk : stacks) {
+                if (st != PT_OK) break;
+                pt_batch_config c = stage1;
+                c.trace_block = tb;
+                c.trace_blocks_per_cu = std::max(1, bpc * 128 / tb);   // bpc x 128 lanes per CU
+                c.stack = sk;
+                st = trial(c);
+            }
+    if (st != PT_OK) { (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); return st; }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     b->cfg = winner;
