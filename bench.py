@@ -154,7 +154,8 @@ def batch_leg(Engine, compiler, scenes, abi):
     eng = Engine(0)
     seq, films = 0.0, []
     for c, n in zip(comps, spp):
-        eng.load_renderer(c)
+        rid = eng.load_renderer(c)
+        eng.render_frames(min(n, c.program.frames_in_flight))   # path state allocated outside the clock
         eng.finish()
         t0 = time.perf_counter()
         eng.clear_accumulation()
@@ -162,6 +163,7 @@ def batch_leg(Engine, compiler, scenes, abi):
         eng.finish()
         seq += time.perf_counter() - t0
         films.append(eng.read_export())
+        eng.unload_renderer(rid)           # its path state goes before the next scene's
     log(f"[bench] batch: sequential {seq:.3f}s for {paths / 1e6:.1f} Mpaths")
     br = BatchRenderer(eng, comps)
     best, sweep = br.autotune(spp, probe_div=8)

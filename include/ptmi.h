@@ -490,8 +490,8 @@ pt_status pt_batch_get_config(const pt_batch* b, pt_batch_config* cfg);
 /* Clear every film and render spp[s] frames of scene s from sampleCount 0. Asynchronous. */
 pt_status pt_batch_render(pt_batch* b, const uint32_t* spp);
 pt_status pt_batch_read_export(pt_batch* b, int32_t scene, float* out, size_t out_floats);
-/* Autotuner: times the batch at spp[s] / probe_div (at least 1), first over wave sizes of 2^22
- * to 2^26 path slots at the current launch shape, then over traversal block size, blocks per CU
+/* Autotuner: times the batch at spp[s] / probe_div (at least 1), first over wave sizes of 2^23
+ * to 2^28 path slots at the current launch shape, then over traversal block size, blocks per CU
  * and stack depth at the best wave size; sets and returns the fastest. The sweep (configs, ms)
  * is written to the first max_results entries of the optional arrays. */
 pt_status pt_batch_autotune(pt_batch* b, const uint32_t* spp, uint32_t probe_div, pt_batch_config* best,

@@ -1191,7 +1191,7 @@ static pt_status batch_check_config(const pt_batch* b, const pt_batch_config& c)
     if (c.trace_block != 64 && c.trace_block != 128 && c.trace_block != 256) return fail(PT_E_INVALID, "pt_batch: trace_block must be 64, 128 or 256");
     if (c.stack != 16 && c.stack != 32) return fail(PT_E_INVALID, "pt_batch: stack must be 16 or 32");
     if (c.stack < b->min_stack) return fail(PT_E_INVALID, "pt_batch: stack shallower than a scene's BVH needs");
-    if (c.wave_paths < b->max_npix || c.wave_paths > (1 << 26)) return fail(PT_E_INVALID, "pt_batch: wave_paths must hold the largest film and be <= 2^26");
+    if (c.wave_paths < b->max_npix || c.wave_paths > (1 << 28)) return fail(PT_E_INVALID, "pt_batch: wave_paths must hold the largest film and be <= 2^28");
     if (c.trace_blocks_per_cu < 1 || c.trace_blocks_per_cu > 32 || c.shade_blocks_per_cu < 1 || c.shade_blocks_per_cu > 32)
         return fail(PT_E_INVALID, "pt_batch: blocks per CU must be in [1, 32]");
     return PT_OK;
@@ -1267,7 +1267,7 @@ pt_status pt_batch_create(pt_ctx* ctx, int32_t n, const pt_scene_desc* scenes, c
     b->r = new pt_renderer();
     b->r->ctx = ctx;
     b->r->prog = p0;
-    b->cfg.wave_paths = std::max(b->max_npix, 1 << 23);
+    b->cfg.wave_paths = std::max(b->max_npix, 1 << 26);
     b->cfg.trace_block = 128;
     b->cfg.trace_blocks_per_cu = 8;
     b->cfg.shade_blocks_per_cu = 8;
@@ -1390,7 +1390,7 @@ pt_status pt_batch_autotune(pt_batch* b, const uint32_t* spp, uint32_t probe_div
         return PT_OK;
     };
     pt_status st = PT_OK;
-    const int waves[] = {1 << 22, 1 << 23, 1 << 24, 1 << 25, 1 << 26};
+    const int waves[] = {1 << 23, 1 << 24, 1 << 25, 1 << 26, 1 << 27, 1 << 28};
     for (int wp : waves) {
         pt_batch_config c = start;
         c.wave_paths = std::max(wp, b->max_npix);

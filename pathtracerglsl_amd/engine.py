@@ -86,9 +86,20 @@ class Engine:
         h = C.c_void_p()
         prog = compiled.program
         _lib.check(self.lib.pt_renderer_load(self.ctx, C.byref(prog), C.byref(h)))
+        if compiled.id in self._renderers:           # a reload replaces the renderer of that id
+            self.unload_renderer(compiled.id)
         self._renderers[compiled.id] = (h, compiled)
         self._selected = compiled.id
         return compiled.id
+
+    def unload_renderer(self, rid):
+        """Release a loaded renderer (its film and path state)."""
+        h, _ = self._renderers.pop(rid)
+        self._descriptions.pop(rid, None)
+        if h:
+            self.lib.pt_renderer_unload(h)
+        if self._selected == rid:
+            self._selected = None
 
     def load_renderer_description(self, description, compiled):
         """Engine.loadRenderer from the reference's plan record: the ProgramDescription (a dict or
@@ -103,7 +114,7 @@ class Engine:
                                                          compiled.program.frames_in_flight, C.byref(h)))
         rid = f"{compiled.id}#description"
         if rid in self._renderers:
-            self.lib.pt_renderer_unload(self._renderers[rid][0])
+            self.unload_renderer(rid)
         self._renderers[rid] = (h, compiled)
         self._descriptions[rid] = text
         self._selected = rid
