@@ -530,9 +530,10 @@ void build_probe_map(const float* tris12, int n, int first, ProbeMap& m) {
         U0 = std::min(U0, fb[4 * k] - margin); U1 = std::max(U1, fb[4 * k + 1] + margin);
         V0 = std::min(V0, fb[4 * k + 2] - margin); V1 = std::max(V1, fb[4 * k + 3] + margin);
     }
-    // about two cells per triangle, square cells, at most 2048 a side
+    // about one cell per triangle (two per triangle test fewer candidates in the same time, with
+    // 60 % more items), square cells, at most 2048 a side
     const double W = U1 - U0, H = V1 - V0;
-    const double cell = std::sqrt(std::max(W * H, 1e-60) / std::max(2.0 * n, 1.0));
+    const double cell = std::sqrt(std::max(W * H, 1e-60) / std::max(1.0 * n, 1.0));
     m.gu = (int)std::min(2048.0, std::max(1.0, std::ceil(W / cell)));
     m.gv = (int)std::min(2048.0, std::max(1.0, std::ceil(H / cell)));
     // the device indexes with fp32: keep the grid's origin and scale as the fp32 values it uses

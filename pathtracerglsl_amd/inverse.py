@@ -125,7 +125,7 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=2024)
     a = ap.parse_args(argv)
     target, init = scenes.inverse_media_params(a.seed)
-    strategy = compiler.RenderStrategy("mis", a.depth, 3, 0.95, 4)
+    strategy = compiler.RenderStrategy("mis", a.depth, 3, 0.95, 64)
     comp = compiler.compile_scene(scenes.inverse_media_scene(target), strategy, a.res, a.res)
     med, alb = scenes.inverse_adjoint_materials(comp)
     with Engine(0) as eng:
