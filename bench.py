@@ -165,6 +165,19 @@ def batch_leg(Engine, compiler, scenes, abi):
         films.append(eng.read_export())
         eng.unload_renderer(rid)           # its path state goes before the next scene's
     log(f"[bench] batch: sequential {seq:.3f}s for {paths / 1e6:.1f} Mpaths")
+    # the same sequential launches with 4 frames per wave (the library's earlier default)
+    comps4 = [compiler.compile_scene(sc, compiler.RenderStrategy("mis", 12, 3, 0.95, 4), w, h) for sc, w, h, _ in items]
+    seq4 = 0.0
+    for c, n in zip(comps4, spp):
+        rid = eng.load_renderer(c)
+        eng.render_frames(min(n, 4))
+        eng.finish()
+        t0 = time.perf_counter()
+        eng.clear_accumulation()
+        eng.render_frames(n)
+        eng.finish()
+        seq4 += time.perf_counter() - t0
+        eng.unload_renderer(rid)
     br = BatchRenderer(eng, comps)
     best, sweep = br.autotune(spp, probe_div=8)
     br.render(spp)                       # capture + first replay
@@ -183,6 +196,7 @@ def batch_leg(Engine, compiler, scenes, abi):
     out = {"workload": "64 icosphere-cloud scenes, log-uniform 1k-500k tris, 128^2-1024^2, 4-256 spp, pt-mis depth12",
            "triangles": int(sum(c.scene.n_tris for c in comps)), "paths": int(paths),
            "sequential_s": round(seq, 4), "batched_s": round(bt, 4), "speedup": round(seq / bt, 3),
+           "sequential_4_frames_per_wave_s": round(seq4, 4), "speedup_vs_4_frames_per_wave": round(seq4 / bt, 3),
            "batched_mpaths_per_s": round(paths / bt / 1e6, 2), "sequential_mpaths_per_s": round(paths / seq / 1e6, 2),
            "identical_films": bool(identical), "nonfinite_pixels": nonfinite, "autotuned": best.as_dict(),
            "sweep_top": [{"config": c, "ms": round(m, 2)} for c, m in sweep[:5]], "sweep_size": len(sweep)}
