@@ -90,8 +90,8 @@ struct pt_ctx {
     bool has_scene = false;
     int stack = 16;
     uint32_t wide_depth = 0;
-    int persist_blocks = 2048;   // persistent traversal grid (set from the device's CU count)
     int cu_count = 256;
+    int lds_per_cu = 160 * 1024;   // LDS bytes per CU (device property)
     int shade_blocks = 2048;
     bool has_ggx = false;
     bool has_mirror = false;
@@ -135,6 +135,14 @@ struct pt_ctx {
         it->second.items += items;
     }
 };
+
+// Persistent traversal grid: as many 2-wave blocks per CU as the LDS stacks leave room for, at
+// most 10 (a 16-entry stack takes 16 KB per block, so 10 blocks fill a 160 KB CU).
+inline int trace_grid(const pt_ctx* c, int stack) {
+    const int per_block = stack * TBLOCK * (int)sizeof(uint2);
+    const int per_cu = std::max(1, std::min(10, c->lds_per_cu / per_block));
+    return c->cu_count * per_cu;
+}
 
 struct pt_renderer {
     pt_ctx* ctx = nullptr;

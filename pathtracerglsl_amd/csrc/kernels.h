@@ -232,6 +232,11 @@ __global__ void __launch_bounds__(SBLOCK) k_fill_f4(float4* p, long long n) {
     for (; i < n; i += (long long)gridDim.x * blockDim.x) p[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
+// A queue of at most SMALL_QUEUE entries per lane of the grid is walked grid-stride, one ray per
+// lane at a time: the persistent lanes' queue claims are same-address atomics, and on a short
+// queue a grid's worth of them costs more than the imbalance they remove (the 256x256 Cornell
+// box: 2.0 ms per render at 2 entries per lane, 1.26 ms at 4 and at 16).
+constexpr uint32_t SMALL_QUEUE = 4;
 // Persistent-lane work distribution: each wave owns a chunk of queue entries and hands them
 // to lanes as they go idle (no per-ray atomics; one atomic per CHUNK entries per wave).
 struct LaneFeeder {
@@ -269,14 +274,6 @@ struct LaneFeeder {
 #ifndef PTMI_FEED_MIN_IDLE
 #define PTMI_FEED_MIN_IDLE 8
 #endif
-#ifndef PTMI_POSTPONE_EXT
-#define PTMI_POSTPONE_EXT 0
-#endif
-#ifndef PTMI_POSTPONE_SHD
-#define PTMI_POSTPONE_SHD 0
-#endif
-constexpr int POSTPONE_EXT = PTMI_POSTPONE_EXT;    // triangle postponing below this many lanes (0: off)
-constexpr int POSTPONE_SHD = PTMI_POSTPONE_SHD;    // the same for opaque shadow rays
 constexpr int FEED_CHUNK = PTMI_FEED_CHUNK;        // queue entries a wave claims per atomic
 constexpr int FEED_MIN_IDLE = PTMI_FEED_MIN_IDLE;  // idle lanes before a wave refills
 
@@ -289,6 +286,23 @@ __global__ void __launch_bounds__(TB) k_extend(DevScene Sk, Paths P, int qsel) {
     const uint32_t n = (uint32_t)P.counts[qsel];
     const int* q = P.q[qsel];
     TraceCounters cnt = {0, 0};
+    if (n <= SMALL_QUEUE * gridDim.x * TB) {
+        for (uint32_t k = blockIdx.x * TB + threadIdx.x; k < n; k += gridDim.x * TB) {
+            const int path = q[k];
+            const DevScene& S = scene_of<BATCH>(Sk, P, path);
+            const float4 o4 = P.ray_o[path], d4 = P.ray_d[path];
+            float tmax = MAX_DIST; int prim = -1;
+            trace_wide<false, STACK, TB, COUNT>(S, xyz(o4), xyz(d4), tmax, prim, stk, cnt);
+            trace_analytic<false>(S, xyz(o4), xyz(d4), tmax, prim);
+            if (INST) trace_instances<false, TB, COUNT>(S, xyz(o4), xyz(d4), tmax, prim, stk, cnt);
+            P.hit[path] = make_float2(tmax, __int_as_float(prim));
+        }
+        if (COUNT) {
+            atomicAdd(&P.trav[0], (unsigned long long)cnt.nodes);
+            atomicAdd(&P.trav[1], (unsigned long long)cnt.tris);
+        }
+        return;
+    }
     LaneFeeder feed; feed.init();
     TravState st;
     bool active = false;
@@ -308,7 +322,7 @@ __global__ void __launch_bounds__(TB) k_extend(DevScene Sk, Paths P, int qsel) {
             continue;
         }
         if (active && (Sp->n_cw_nodes == 0 ||
-                       trace_step<false, TB, COUNT, POSTPONE_EXT, STACK>(Sp->cw_nodes, Sp->cw_tris, st, stk, cnt, Sp->cw_depth))) {
+                       trace_step<false, TB, COUNT>(Sp->cw_nodes, Sp->cw_tris, st, stk, cnt))) {
             float tmax = st.tmax; int prim = st.prim;
             trace_analytic<false>(*Sp, st.o, st.d, tmax, prim);
             if (INST) trace_instances<false, TB, COUNT>(*Sp, st.o, st.d, tmax, prim, stk, cnt);
@@ -324,34 +338,75 @@ __global__ void __launch_bounds__(TB) k_extend(DevScene Sk, Paths P, int qsel) {
 
 // shadow (opaque scenes): any hit for every NEE ray; unoccluded contributions are added to
 // their path as ((a * vis) * w) / pdf, the reference's operation order with vis = 1.
+template <bool SPEC, bool ADJ>
+PT_DEV void shadow_add(Paths& P, int k, float4 d4) {
+    const int i = P.s_path[k];
+    const float4 c = P.s_c[k];
+    const float w = SPEC ? P.s_l[k].w : c.w;
+    const sp4 a = SPEC ? ld4(c) : mk4(c.x, c.y, c.z, 0.0f);
+    const sp4 v = (a * w) / d4.w;
+    store_rad<SPEC>(P, i, load_rad<SPEC>(P, i) + v);
+    if (ADJ) {
+        float D[ADJ_N];
+#pragma unroll
+        for (int kk = 0; kk < ADJ_N; kk++) D[kk] = P.adj_sD[(size_t)kk * P.cap + k];
+        adj_add_G(P, i, D, adj_e(P, i, v));
+    }
+}
+// Persistent lanes, as k_extend (a short queue is walked grid-stride): a lane whose ray is done
+// takes the next one while its wave-mates keep walking (any-hit walks end at very different
+// depths). The analytic objects come first (a
+// hit there needs no walk), then the wide BVH step by step, then the instance batches.
 template <int STACK, bool SPEC, bool ADJ, bool BATCH = false, int TB = TBLOCK, bool INST = false>
 __global__ void __launch_bounds__(TB) k_shadow(DevScene Sk, Paths P) {
     __shared__ uint2 stk[STACK * TB];
-    const int n = P.counts[C_SHADOW];
+    const uint32_t n = (uint32_t)P.counts[C_SHADOW];
     TraceCounters cnt = {0, 0};
-    for (int base = blockIdx.x * TB; base < n; base += gridDim.x * TB) {
-        int k = base + threadIdx.x;
-        if (k >= n) break;
-        const DevScene& S = scene_of<BATCH>(Sk, P, P.s_path[k]);
-        float4 o4 = P.s_o[k], d4 = P.s_d[k];
-        f3 o = xyz(o4), d = xyz(d4);
-        float maxd = o4.w; int prim = -1;
-        bool blocked = trace_analytic<true>(S, o, d, maxd, prim);
-        if (!blocked) blocked = trace_wide<true, STACK, TB, false, POSTPONE_SHD>(S, o, d, maxd, prim, stk, cnt);
-        if (INST && !blocked) blocked = trace_instances<true, TB, false>(S, o, d, maxd, prim, stk, cnt);
-        if (!blocked) {
-            int i = P.s_path[k];
-            float4 c = P.s_c[k];
-            const float w = SPEC ? P.s_l[k].w : c.w;
-            sp4 a = SPEC ? ld4(c) : mk4(c.x, c.y, c.z, 0.0f);
-            const sp4 v = (a * w) / d4.w;
-            store_rad<SPEC>(P, i, load_rad<SPEC>(P, i) + v);
-            if (ADJ) {
-                float D[ADJ_N];
-#pragma unroll
-                for (int kk = 0; kk < ADJ_N; kk++) D[kk] = P.adj_sD[(size_t)kk * P.cap + k];
-                adj_add_G(P, i, D, adj_e(P, i, v));
+    if (n <= SMALL_QUEUE * gridDim.x * TB) {
+        for (uint32_t k = blockIdx.x * TB + threadIdx.x; k < n; k += gridDim.x * TB) {
+            const DevScene& S = scene_of<BATCH>(Sk, P, P.s_path[k]);
+            const float4 o4 = P.s_o[k], d4 = P.s_d[k];
+            float maxd = o4.w; int prim = -1;
+            bool blocked = trace_analytic<true>(S, xyz(o4), xyz(d4), maxd, prim);
+            if (!blocked) blocked = trace_wide<true, STACK, TB, false>(S, xyz(o4), xyz(d4), maxd, prim, stk, cnt);
+            if (INST && !blocked) blocked = trace_instances<true, TB, false>(S, xyz(o4), xyz(d4), maxd, prim, stk, cnt);
+            if (!blocked) shadow_add<SPEC, ADJ>(P, (int)k, d4);
+        }
+        return;
+    }
+    LaneFeeder feed; feed.init();
+    TravState st;
+    bool active = false;
+    int k = -1;
+    float4 d4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const DevScene* Sp = &Sk;
+    for (;;) {
+        const int kk = feed.feed<FEED_CHUNK>(&P.counts[C_SHD_FETCH], n, !active, FEED_MIN_IDLE);
+        if (kk >= 0) {
+            k = kk;
+            Sp = &scene_of<BATCH>(Sk, P, P.s_path[k]);
+            const float4 o4 = P.s_o[k];
+            d4 = P.s_d[k];
+            float maxd = o4.w; int prim = -1;
+            if (!trace_analytic<true>(*Sp, xyz(o4), xyz(d4), maxd, prim)) {
+                if (Sp->n_cw_nodes > 0) {
+                    trace_init(st, xyz(o4), xyz(d4), o4.w);
+                    active = true;
+                } else if (!(INST && trace_instances<true, TB, false>(*Sp, xyz(o4), xyz(d4), maxd, prim, stk, cnt))) {
+                    shadow_add<SPEC, ADJ>(P, k, d4);
+                }
             }
+        }
+        if (__ballot(active) == 0) {
+            if (feed.exhausted) break;
+            continue;
+        }
+        if (active && trace_step<true, TB, false>(Sp->cw_nodes, Sp->cw_tris, st, stk, cnt)) {
+            active = false;
+            bool blocked = st.prim >= 0;
+            float maxd = st.tmax; int prim = -1;
+            if (INST && !blocked) blocked = trace_instances<true, TB, false>(*Sp, st.o, st.d, maxd, prim, stk, cnt);
+            if (!blocked) shadow_add<SPEC, ADJ>(P, k, d4);
         }
     }
 }

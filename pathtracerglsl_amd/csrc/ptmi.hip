@@ -285,8 +285,8 @@ pt_status pt_ctx_create(int device_index, pt_ctx** out) {
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return fail(PT_E_DEVICE, "stream creation failed"); }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device_index) == hipSuccess && prop.multiProcessorCount > 0) {
-        c->persist_blocks = prop.multiProcessorCount * 8;   // 16 waves per CU of 2-wave blocks
         c->cu_count = prop.multiProcessorCount;
+        if (prop.maxSharedMemoryPerMultiProcessor > 0) c->lds_per_cu = (int)prop.maxSharedMemoryPerMultiProcessor;
         c->shade_blocks = prop.multiProcessorCount * 8;
     }
     c->trav.bytes = 2 * sizeof(unsigned long long);

@@ -110,36 +110,34 @@ PT_DEV void trace_init(TravState& st, f3 o, f3 d, float tmax, uint32_t root = 0)
     st.t_base = 0; st.t_bits = 0;
 }
 
-// Stack entries with this bit set are postponed triangle groups, not node groups.
-constexpr uint32_t TRI_GROUP = 0x80000000u;
+// The node at the head of the pending group (the nearest child not yet visited).
+PT_DEV uint32_t head_node(const TravState& st) {
+    const uint32_t k = __builtin_ctz(st.g_bits & 0xFFu);
+    const uint32_t slot = k ^ st.oct_inv;
+    const uint32_t imask = st.g_bits >> 8;
+    return st.g_base + __popc(imask & ((1u << slot) - 1u));
+}
 
-// One step: visit the nearest pending node (if any) and test the triangles it exposes.
-// Returns true when the ray is finished: the stack is empty, or (ANY) a hit with t < tmax
-// was found, in which case st.prim is set to its id.
-//
-// POSTPONE > 0 (with the lane stack's capacity CAP and the tree's depth): triangle
-// postponing (Ylitie et al. 2017, sec. 4): when fewer than POSTPONE lanes of the wave are
-// testing triangles and this lane still has nodes of its group to visit, the triangle group
-// goes onto the stack and the lane keeps traversing with the others. The closest hit is the
-// (t, id) minimum over every triangle whose box the ray enters, so testing order does not
-// change it; a group is postponed only while the stack keeps room for every node group the
-// remaining levels can push.
-template <bool ANY, int BLOCK, bool COUNT, int POSTPONE = 0, int CAP = 0>
+// One step of a lane's walk: with no triangle pending, visit the nearest pending node; then
+// test one pending triangle. At most one triangle per lane per step keeps a wave's lanes in
+// step: a node that exposes several triangles does not hold the wave in a triangle loop while
+// the other lanes wait (a wave's walk costs the sum of its steps, each step the slowest lane's
+// work). The closest hit is the (t, id) minimum over every triangle whose box the ray enters,
+// so the order of tests does not change it.
+// Returns true when the ray is finished: nothing is pending, or (ANY) a hit with t < tmax was
+// found, in which case st.prim is set to its id.
+template <bool ANY, int BLOCK, bool COUNT>
 PT_DEV bool trace_step(const uint4* __restrict__ nodes, const float4* __restrict__ tris, TravState& st, uint2* stk,
-                       TraceCounters& cnt, int depth = 0) {
+                       TraceCounters& cnt) {
     const int lane_slot = threadIdx.x;
     if (st.t_bits == 0u && (st.g_bits & 0xFFu) != 0u) {
-        uint32_t k = __builtin_ctz(st.g_bits & 0xFFu);
-        st.g_bits &= ~(1u << k);
-        uint32_t slot = k ^ st.oct_inv;
-        uint32_t imask = st.g_bits >> 8;
-        uint32_t node = st.g_base + __popc(imask & ((1u << slot) - 1u));
+        const uint4* np = nodes + 5 * (size_t)head_node(st);
+        const uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3], w4 = np[4];
+        st.g_bits &= ~(1u << __builtin_ctz(st.g_bits & 0xFFu));
         if ((st.g_bits & 0xFFu) != 0u) {
             stk[st.sp * BLOCK + lane_slot] = make_uint2(st.g_base, st.g_bits);
             st.sp++;
         }
-        const uint4* np = nodes + 5 * (size_t)node;
-        uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3], w4 = np[4];
         if (COUNT) cnt.nodes++;
         const float px = __uint_as_float(w0.x), py = __uint_as_float(w0.y), pz = __uint_as_float(w0.z);
         const float sx = __uint_as_float((w0.w & 0xFFu) << 23);
@@ -206,22 +204,31 @@ PT_DEV bool trace_step(const uint4* __restrict__ nodes, const float4* __restrict
         st.t_base = w1.y;
         st.t_bits = tri_hits;
     }
-    while (st.t_bits != 0u) {
-        if (POSTPONE > 0 && (st.g_bits & 0xFFu) != 0u && st.sp + depth + 1 < CAP &&
-            (int)__popcll(__ballot(1)) < POSTPONE) {
-            stk[st.sp * BLOCK + lane_slot] = make_uint2(st.t_base, st.t_bits | TRI_GROUP);
-            st.sp++;
-            st.t_bits = 0u;
-            break;
+    // this step's triangle: its loads go out first, then the next group is settled (a pop when
+    // nothing else is pending), so the stack read overlaps the triangle fetch
+    const bool tri = st.t_bits != 0u;
+    uint32_t tk = 0;
+    if (tri) { tk = __builtin_ctz(st.t_bits); st.t_bits &= st.t_bits - 1u; }
+    float4 ta, te1, te2;
+    if (tri) {
+        const float4* tp = tris + 3 * (size_t)(st.t_base + tk);
+        ta = tp[0]; te1 = tp[1]; te2 = tp[2];
+    }
+    bool finished = false;
+    if ((st.g_bits & 0xFFu) == 0u && st.t_bits == 0u) {
+        if (st.sp == 0) {
+            finished = true;
+        } else {
+            st.sp--;
+            const uint2 e = stk[st.sp * BLOCK + lane_slot];
+            st.g_base = e.x; st.g_bits = e.y;
         }
-        uint32_t k = __builtin_ctz(st.t_bits);
-        st.t_bits &= st.t_bits - 1u;
-        const float4* tp = tris + 3 * (size_t)(st.t_base + k);
-        float4 ta = tp[0], te1 = tp[1], te2 = tp[2];
+    }
+    if (tri) {
         if (COUNT) cnt.tris++;
         float t;
         if (tri_test(st.o, st.d, xyz(ta), xyz(te1), xyz(te2), t) && t > 0.0f) {
-            int id = (int)__float_as_uint(ta.w);
+            const int id = (int)__float_as_uint(ta.w);
             if (ANY) {
                 if (t < st.tmax) { st.prim = id; return true; }
             } else if (t < st.tmax || (t == st.tmax && id < st.prim)) {
@@ -229,24 +236,17 @@ PT_DEV bool trace_step(const uint4* __restrict__ nodes, const float4* __restrict
             }
         }
     }
-    if ((st.g_bits & 0xFFu) == 0u && st.t_bits == 0u) {
-        if (st.sp == 0) return true;
-        st.sp--;
-        uint2 e = stk[st.sp * BLOCK + lane_slot];
-        if (POSTPONE > 0 && (e.y & TRI_GROUP)) { st.t_base = e.x; st.t_bits = e.y & ~TRI_GROUP; }
-        else { st.g_base = e.x; st.g_bits = e.y; }
-    }
-    return false;
+    return finished;
 }
 
 // Whole-ray wide-BVH walk (one ray per lane, no interleaving).
-template <bool ANY, int STACK, int BLOCK, bool COUNT, int POSTPONE = 0>
+template <bool ANY, int STACK, int BLOCK, bool COUNT>
 PT_DEV bool trace_wide(const DevScene& S, f3 o, f3 d, float& tmax, int& prim, uint2* stk, TraceCounters& cnt) {
     if (S.n_cw_nodes == 0) return false;
     TravState st;
     trace_init(st, o, d, tmax);
     st.prim = prim;
-    while (!trace_step<ANY, BLOCK, COUNT, POSTPONE, STACK>(S.cw_nodes, S.cw_tris, st, stk, cnt, S.cw_depth)) {}
+    while (!trace_step<ANY, BLOCK, COUNT>(S.cw_nodes, S.cw_tris, st, stk, cnt)) {}
     (void)STACK;
     if (ANY) return st.prim >= 0 && st.prim != prim;
     bool found = st.prim != prim;

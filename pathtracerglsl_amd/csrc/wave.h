@@ -41,7 +41,8 @@ static void launch_wave(pt_renderer* r, const Camera& cam, int frames, uint32_t 
     TIMED(ctx, "raygen", n,  hipLaunchKernelGGL((k_raygen<SPEC, ADJ>), dim3((unsigned)((n + SBLOCK - 1) / SBLOCK)), dim3(SBLOCK), 0, st, P, cam, W, H, (int)n, sample0, MEDIA ? 1 : 0, pg.wavelengths));
     if (MEDIA && (cam.ortho || cam.thin)) hipLaunchKernelGGL(k_ray_medium, dim3(ctx->shade_blocks), dim3(SBLOCK), 0, st, S, P, (int)n);
     int cur = 0;
-    const int tgrid = ctx->persist_blocks;
+    const int tgrid = trace_grid(ctx, STACK);
+    const int mgrid = ctx->cu_count * 8;   // the media walker is grid-stride: more blocks than fit balance it
     // a null crossing consumes an iteration but no bounce (pt.ts null-interface branch)
     const int iterations = pg.max_bounces + 1 + (S.has_null ? std::max(pg.max_null_crossings, 0) : 0);
     for (int it = 0; it < iterations; it++) {
@@ -57,8 +58,8 @@ static void launch_wave(pt_renderer* r, const Camera& cam, int frames, uint32_t 
         const int eq = MEDIA && !ADJ && pg.medium_light_sampling == PT_MLS_EQUIANGULAR ? 1 : 0;
         TIMED(ctx, "bin", 0, hipLaunchKernelGGL((k_bin<MEDIA, SPEC, ADJ>), dim3(ctx->shade_blocks), dim3(SBLOCK), 0, st, S, P, cur, eq, pg.max_bounces, pg.max_null_crossings));
         if (eq) {
-            if (inst) TIMED(ctx, "shadow_media", 0, hipLaunchKernelGGL((k_shadow_media<STACK, SPEC, ADJ, true>), dim3(tgrid), dim3(TBLOCK), 0, st, S, P));
-            else TIMED(ctx, "shadow_media", 0, hipLaunchKernelGGL((k_shadow_media<STACK, SPEC, ADJ>), dim3(tgrid), dim3(TBLOCK), 0, st, S, P));
+            if (inst) TIMED(ctx, "shadow_media", 0, hipLaunchKernelGGL((k_shadow_media<STACK, SPEC, ADJ, true>), dim3(mgrid), dim3(TBLOCK), 0, st, S, P));
+            else TIMED(ctx, "shadow_media", 0, hipLaunchKernelGGL((k_shadow_media<STACK, SPEC, ADJ>), dim3(mgrid), dim3(TBLOCK), 0, st, S, P));
             hipLaunchKernelGGL(k_shadow_reset, dim3(1), dim3(64), 0, st, P);
         }
         switch (pg.estimator) {
@@ -67,8 +68,8 @@ static void launch_wave(pt_renderer* r, const Camera& cam, int frames, uint32_t 
             default: launch_shade<PT_EST_MIS, MEDIA, SPEC, ADJ>(r, cur);
         }
         if (pg.estimator != PT_EST_PT) {
-            if (MEDIA && inst) TIMED(ctx, "shadow_media", 0, hipLaunchKernelGGL((k_shadow_media<STACK, SPEC, ADJ, true>), dim3(tgrid), dim3(TBLOCK), 0, st, S, P));
-            else if (MEDIA) TIMED(ctx, "shadow_media", 0, hipLaunchKernelGGL((k_shadow_media<STACK, SPEC, ADJ>), dim3(tgrid), dim3(TBLOCK), 0, st, S, P));
+            if (MEDIA && inst) TIMED(ctx, "shadow_media", 0, hipLaunchKernelGGL((k_shadow_media<STACK, SPEC, ADJ, true>), dim3(mgrid), dim3(TBLOCK), 0, st, S, P));
+            else if (MEDIA) TIMED(ctx, "shadow_media", 0, hipLaunchKernelGGL((k_shadow_media<STACK, SPEC, ADJ>), dim3(mgrid), dim3(TBLOCK), 0, st, S, P));
             else if (inst) TIMED(ctx, "shadow", 0, hipLaunchKernelGGL((k_shadow<STACK, SPEC, ADJ, false, TBLOCK, true>), dim3(tgrid), dim3(TBLOCK), 0, st, S, P));
             else TIMED(ctx, "shadow", 0, hipLaunchKernelGGL((k_shadow<STACK, SPEC, ADJ>), dim3(tgrid), dim3(TBLOCK), 0, st, S, P));
         }
