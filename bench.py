@@ -314,7 +314,13 @@ def main():
                      "alg_bytes_per_launch": round(alg_bytes_per_render / max(launches_per_render, 1)),
                      "avg_launch_ms": round(ext_ms_per_render / max(launches_per_render, 1), 4),
                      "nodes_per_query": round(nodes / max(rc_closest, 1), 2),
-                     "tris_per_query": round(tris / max(rc_closest, 1), 2)},
+                     "tris_per_query": round(tris / max(rc_closest, 1), 2),
+                     # the counted HBM bytes over the same launch time: the algorithmic bytes are
+                     # mostly served by L2 and the Infinity Cache, so this is the DRAM share
+                     "traffic_gbs": round(traffic / (ext_ms_per_render / max(launches_per_render, 1) * 1e-3) / 1e9, 1)
+                     if traffic and ext_ms_per_render > 0 else None,
+                     "traffic_frac": round(traffic / (ext_ms_per_render / max(launches_per_render, 1) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+                     if traffic and ext_ms_per_render > 0 else None},
         "kernel_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in timings.items()},
         "shade_ms_per_step": round(sum(v[0] for k, v in timings.items() if k.startswith("shade")) / args.steps, 3),
     }

@@ -24,8 +24,11 @@ constexpr bool cat_preprobed(int cat, bool media) { return !media && (cat == CAT
 enum { C_RAYQ0 = 0, C_RAYQ1 = 1, C_SHADOW = 2, C_CAT0 = 3, C_EXT_FETCH = C_CAT0 + N_CAT, C_SHD_FETCH, C_CAM_MEDIUM,
        N_COUNTS = 16 };
 
+// Queue counters sit CNT_STRIDE ints apart, each in its own 128-byte line, so the atomics of
+// different queues do not queue behind each other.
+constexpr int CNT_STRIDE = 32;
 // Bytes ahead of a path state's arrays: the queue counters, ray and traversal statistics.
-constexpr size_t STATE_HEAD_BYTES = 768;
+constexpr size_t STATE_HEAD_BYTES = ((N_COUNTS * CNT_STRIDE * 4 + 255) & ~(size_t)255) + 512;
 
 struct Paths {
     float4* ray_o;   // origin.xyz
@@ -65,6 +68,7 @@ struct Paths {
     int* path_scene;
 };
 constexpr int ADJ_N = PT_ADJ_N_PARAMS;
+__host__ __device__ inline int& qcount(const Paths& P, int slot) { return P.counts[slot * CNT_STRIDE]; }
 
 struct Camera {
     float ox, oy, oz, fx, fy, fz, rx, ry, rz, ux, uy, uz, tan_fov, aspect;

@@ -887,7 +887,7 @@ static size_t carve_paths(const pt_program& prog, char* base, long long n, Paths
         return p;
     };
     const size_t m = (size_t)n;
-    P.counts = (int*)take(N_COUNTS * sizeof(int));
+    P.counts = (int*)take(N_COUNTS * CNT_STRIDE * sizeof(int));
     P.hist = (unsigned long long*)take(2 * sizeof(unsigned long long));
     P.trav = (unsigned long long*)take(2 * sizeof(unsigned long long));
     P.ray_o = (float4*)take(16 * m); P.ray_d = (float4*)take(16 * m); P.thr = (float4*)take(16 * m);
