@@ -43,6 +43,10 @@ pt_status pt_debug_math(int fn, const float* x, const float* y, float* out, int 
 /* scene_region_at (features/intersection.ts:1121-1265) on the GPU at n points (xyz, host
  * memory) of the resident scene: out[i] = the innermost region holding point i, -1 outside all. */
 pt_status pt_debug_region_at(pt_ctx* ctx, const float* pts, int n, int32_t* out);
+/* environment_sample (components/env/sampler_cdf.glsl:61-80) of the resident scene's environment
+ * on the GPU at n (x0, x1) pairs (host memory, 2 floats each): out[4i..4i+3] = the sampled
+ * direction and its pdf. */
+pt_status pt_debug_env_sample(pt_ctx* ctx, const float* xi, int n, float* out);
 /* A plain device copy of `bytes` (a multiple of 16), repeated `iters` times: the kernel of
  * known traffic that scales the FETCH_SIZE / WRITE_SIZE counters of profiled kernels. */
 pt_status pt_calib_copy(pt_ctx* ctx, size_t bytes, int iters);

@@ -24,7 +24,7 @@ DECLARED = ["pt_last_error", "pt_ctx_create", "pt_ctx_destroy", "pt_scene_upload
 # every symbol declared in include/ptmi_debug.h (test and bench instrumentation)
 DECLARED_DEBUG = ["pt_renderer_set_traversal_counting", "pt_last_traversal_counts", "pt_trace_closest_device_counted",
                   "pt_scene_stats", "pt_abi_sizeof", "pt_debug_wide_bvh", "pt_debug_math", "pt_calib_copy",
-                  "pt_debug_env_bake", "pt_debug_env_table", "pt_debug_region_at"]
+                  "pt_debug_env_bake", "pt_debug_env_table", "pt_debug_region_at", "pt_debug_env_sample"]
 
 _lib = None
 
@@ -80,6 +80,7 @@ def load():
     lib.pt_env_table_upload.argtypes = [vp, vp, i32, i32, i32, i32, C.POINTER(C.c_double)]
     lib.pt_ctx_reset.argtypes = [vp]
     lib.pt_debug_region_at.argtypes = [vp, vp, C.c_int, vp]
+    lib.pt_debug_env_sample.argtypes = [vp, vp, C.c_int, vp]
     lib.pt_forward.argtypes = [vp, C.POINTER(abi.PtParams), u32]
     lib.pt_backward.argtypes = [vp, C.POINTER(abi.PtParams), u32, C.POINTER(abi.PtAdjointParams), vp, vp]
     lib.pt_batch_create.argtypes = [vp, i32, vp, vp, vp, C.POINTER(vp)]

@@ -118,6 +118,12 @@ struct DevScene {
     const float* env_cond;
     const float* env_marg;
     const double* env_total;   // the table's mass (totalWeight), on the device
+    // cut-point guides of the table's CDFs (Chen and Asau 1974): for bucket b of K equal parts of
+    // [0, 1), the index the sampler's search returns at u = b / K; the marginal's K_m + 1 entries,
+    // then K_c + 1 per row. K is a power of two, so floor(u K) is exact.
+    const int* env_guide_marg;
+    const int* env_guide_cond;
+    int env_km, env_kc;
     // containment (closed meshes)
     const float4* obj_nodes;    // binary nodes, 2 float4 each: (min, A) (max, B)
     const int* obj_node_base;   // per object

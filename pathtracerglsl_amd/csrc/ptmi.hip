@@ -395,6 +395,23 @@ static pt_status validate_scene(const pt_scene_desc* sd) {
 // the f64 row weights sin(theta_j) and texel solid angles, the host builder's expressions
 // evaluated on the host, H values each) in tmp, which the caller keeps until the stream has
 // run the chain.
+// The guides of the table's CDFs: block 0 the marginal, block 1 + j row j. Entry b of a guide is
+// the first index whose CDF value exceeds b / K (n when none does), by the sampler's own search.
+__global__ void k_env_guide(const float* __restrict__ cond, const float* __restrict__ marg, int W, int H, int km, int kc,
+                            int* __restrict__ gm, int* __restrict__ gc) {
+    const int r = blockIdx.x;
+    const float* c = r == 0 ? marg : cond + (size_t)(r - 1) * W;
+    const int n = r == 0 ? H : W, k = r == 0 ? km : kc;
+    int* g = r == 0 ? gm : gc + (size_t)(r - 1) * (kc + 1);
+    for (int b = threadIdx.x; b <= k; b += blockDim.x) {
+        const float x = (float)b / (float)k;
+        int lo = 0, hi = n;
+        while (lo < hi) { const int mid = (lo + hi) >> 1; if (x >= c[mid]) lo = mid + 1; else hi = mid; }
+        g[b] = lo;
+    }
+}
+static int guide_buckets(int n) { int k = 1; while (k < n) k <<= 1; return k; }
+
 static pt_status env_table_chain(float* Y, int W, int H, bool oct, bool compensated, SceneStore& SS, SceneStore& tmp,
                                  DevScene& D) {
     const long long n = (long long)W * H;
@@ -420,10 +437,18 @@ static pt_status env_table_chain(float* Y, int W, int H, bool oct, bool compensa
     }
     hipLaunchKernelGGL(k_env_rows, dim3(grid_for(H, 64)), dim3(64), 0, SS.stream, (const float*)B, d_sin, W, H, cond, rows);
     hipLaunchKernelGGL(k_env_marginal, dim3(1), dim3(64), 0, SS.stream, (const double*)rows, H, marg, aux + 1);
+    const int km = guide_buckets(H), kc = guide_buckets(W);
+    int *gm, *gc;
+    if ((st = SS.alloc((size_t)km + 1, &gm)) != PT_OK || (st = SS.alloc((size_t)H * (kc + 1), &gc)) != PT_OK) return st;
+    hipLaunchKernelGGL(k_env_guide, dim3(H + 1), dim3(256), 0, SS.stream, (const float*)cond, (const float*)marg, W, H, km, kc, gm, gc);
     HIP_TRY(hipGetLastError());
     D.env_cond = cond;
     D.env_marg = marg;
     D.env_total = aux + 1;
+    D.env_guide_marg = gm;
+    D.env_guide_cond = gc;
+    D.env_km = km;
+    D.env_kc = kc;
     return PT_OK;
 }
 
@@ -1528,6 +1553,10 @@ pt_status pt_env_table_upload(pt_ctx* ctx, const float* rgb, int32_t W, int32_t 
     ctx->dev.env_cond = D.env_cond;
     ctx->dev.env_marg = D.env_marg;
     ctx->dev.env_total = D.env_total;
+    ctx->dev.env_guide_marg = D.env_guide_marg;
+    ctx->dev.env_guide_cond = D.env_guide_cond;
+    ctx->dev.env_km = D.env_km;
+    ctx->dev.env_kc = D.env_kc;
     ctx->dev.env_chart = chart;
     ctx->dev.env_w = W;
     ctx->dev.env_h = H;
@@ -1877,6 +1906,15 @@ __global__ void __launch_bounds__(256) k_debug_region(DevScene S, const float* _
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i < n) out[i] = region_at(S, mk3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]));
 }
+// environment_sample (sampler_cdf.glsl:61-80) at n (x0, x1) pairs of the resident scene's
+// environment: the sampled direction and its pdf (the guided CDF search's parity hook).
+__global__ void __launch_bounds__(256) k_debug_env_sample(DevScene S, const float* __restrict__ xi, int n, float* __restrict__ out) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    Wl wl{};
+    const LSample ls = environment_sample<false>(S, xi[2 * i], xi[2 * i + 1], wl);
+    out[4 * i] = ls.wi.x; out[4 * i + 1] = ls.wi.y; out[4 * i + 2] = ls.wi.z; out[4 * i + 3] = ls.pdf;
+}
 }  // namespace
 
 extern "C" {
@@ -1893,6 +1931,25 @@ pt_status pt_debug_region_at(pt_ctx* ctx, const float* pts, int n, int32_t* out)
                        (int*)dout.p);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(out, dout.p, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return PT_OK;
+}
+
+pt_status pt_debug_env_sample(pt_ctx* ctx, const float* xi, int n, float* out) {
+    if (!ctx || !xi || !out || n < 0) return fail(PT_E_INVALID, "pt_debug_env_sample: bad argument");
+    if (!ctx->has_scene) return fail(PT_E_MISSING_INPUT, "missing input: no scene uploaded (pt_scene_upload)");
+    if (ctx->dev.env_type != PT_ENV_SKY && ctx->dev.env_type != PT_ENV_CONSTANT)
+        return fail(PT_E_INVALID, "pt_debug_env_sample: the scene has no environment to sample");
+    if (n == 0) return PT_OK;
+    HIP_TRY(hipSetDevice(ctx->device));
+    DevBuf dx, dout;
+    HIP_TRY(hipMalloc(&dx.p, (size_t)n * 8));
+    HIP_TRY(hipMalloc(&dout.p, (size_t)n * 16));
+    HIP_TRY(hipMemcpyAsync(dx.p, xi, (size_t)n * 8, hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_debug_env_sample, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, ctx->dev, (const float*)dx.p, n,
+                       (float*)dout.p);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(out, dout.p, (size_t)n * 16, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     return PT_OK;
 }

@@ -766,18 +766,28 @@ PT_DEV float environment_pdf(const DevScene& S, f3 dir) {
     float a = env_uniform_fraction(S);
     return (1.0f - a) * env_table_pdf(S, dir) + a / (4.0f * PTM_PI);
 }
+// The sampler's search (sampler_cdf.glsl): the first index in [0, n - 1] whose CDF value exceeds
+// u, n - 1 when none does. The guide narrows it to [g[b], g[b + 1]] for b = floor(u K): the first
+// index above u lies between those for u in [b / K, (b + 1) / K), and the search over the narrowed
+// range returns it (about one step on average rather than log2 n dependent loads).
+PT_DEV int cdf_search(const float* c, int n, float u, const int* g, int k) {
+    int lo = 0, hi = n - 1;
+    if (u >= 0.0f && u < 1.0f) {
+        const int b = (int)(u * (float)k);
+        lo = min(g[b], n - 1);
+        hi = min(g[b + 1], n - 1);
+    }
+    while (lo < hi) { const int mid = (lo + hi) >> 1; if (u >= c[mid]) lo = mid + 1; else hi = mid; }
+    return lo;
+}
 PT_DEV f3 env_sample_table(const DevScene& S, float x0, float x1) {
     const int W = S.env_w, H = S.env_h;
-    int lo = 0, hi = H - 1;
-    while (lo < hi) { int mid = (lo + hi) >> 1; if (x0 >= S.env_marg[mid]) lo = mid + 1; else hi = mid; }
-    int j = lo;
+    const int j = cdf_search(S.env_marg, H, x0, S.env_guide_marg, S.env_km);
     float mLo = (j > 0) ? S.env_marg[j - 1] : 0.0f;
     float dMarg = S.env_marg[j] - mLo;
     float vFrac = (dMarg > 0.0f) ? gclamp((x0 - mLo) / dMarg, 0.0f, 0.9999999f) : 0.5f;
     const float* row = S.env_cond + (size_t)j * W;
-    lo = 0; hi = W - 1;
-    while (lo < hi) { int mid = (lo + hi) >> 1; if (x1 >= row[mid]) lo = mid + 1; else hi = mid; }
-    int i = lo;
+    const int i = cdf_search(row, W, x1, S.env_guide_cond + (size_t)j * (S.env_kc + 1), S.env_kc);
     float cLo = (i > 0) ? row[i - 1] : 0.0f;
     float dCond = row[i] - cLo;
     float uFrac = (dCond > 0.0f) ? gclamp((x1 - cLo) / dCond, 0.0f, 0.9999999f) : 0.5f;
