@@ -302,11 +302,16 @@ __global__ void __launch_bounds__(TB) k_extend(DevScene Sk, Paths P, int qsel) {
     bool active = false;
     int path = -1;
     const DevScene* Sp = &Sk;
+    // the lane's scene's BVH, read once per ray (a batch's scenes live in device memory)
+    const uint4* nodes = Sk.cw_nodes;
+    const float4* tris = Sk.cw_tris;
+    bool walk = Sk.n_cw_nodes > 0;
     for (;;) {
         int k = feed.feed<FEED_CHUNK>(&qcount(P, C_EXT_FETCH), n, !active, FEED_MIN_IDLE);
         if (k >= 0) {
             path = q[k];
             Sp = &scene_of<BATCH>(Sk, P, path);
+            if (BATCH) { nodes = Sp->cw_nodes; tris = Sp->cw_tris; walk = Sp->n_cw_nodes > 0; }
             float4 o4 = P.ray_o[path], d4 = P.ray_d[path];
             trace_init(st, xyz(o4), xyz(d4), MAX_DIST);
             active = true;
@@ -315,8 +320,7 @@ __global__ void __launch_bounds__(TB) k_extend(DevScene Sk, Paths P, int qsel) {
             if (feed.exhausted) break;
             continue;
         }
-        if (active && (Sp->n_cw_nodes == 0 ||
-                       trace_step<false, TB, COUNT>(Sp->cw_nodes, Sp->cw_tris, st, stk, cnt))) {
+        if (active && (!walk || trace_step<false, TB, COUNT>(nodes, tris, st, stk, cnt))) {
             float tmax = st.tmax; int prim = st.prim;
             trace_analytic<false>(*Sp, st.o, st.d, tmax, prim);
             if (INST) trace_instances<false, TB, COUNT>(*Sp, st.o, st.d, tmax, prim, stk, cnt);
@@ -374,11 +378,14 @@ __global__ void __launch_bounds__(TB) k_shadow(DevScene Sk, Paths P) {
     int k = -1;
     float4 d4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     const DevScene* Sp = &Sk;
+    const uint4* nodes = Sk.cw_nodes;   // the lane's scene's BVH, read once per ray
+    const float4* tris = Sk.cw_tris;
     for (;;) {
         const int kk = feed.feed<FEED_CHUNK>(&qcount(P, C_SHD_FETCH), n, !active, FEED_MIN_IDLE);
         if (kk >= 0) {
             k = kk;
             Sp = &scene_of<BATCH>(Sk, P, P.s_path[k]);
+            if (BATCH) { nodes = Sp->cw_nodes; tris = Sp->cw_tris; }
             const float4 o4 = P.s_o[k];
             d4 = P.s_d[k];
             float maxd = o4.w; int prim = -1;
@@ -395,7 +402,7 @@ __global__ void __launch_bounds__(TB) k_shadow(DevScene Sk, Paths P) {
             if (feed.exhausted) break;
             continue;
         }
-        if (active && trace_step<true, TB, false>(Sp->cw_nodes, Sp->cw_tris, st, stk, cnt)) {
+        if (active && trace_step<true, TB, false>(nodes, tris, st, stk, cnt)) {
             active = false;
             bool blocked = st.prim >= 0;
             float maxd = st.tmax; int prim = -1;
