@@ -68,3 +68,46 @@ class BatchRenderer:
             self.close()
         except Exception:
             pass
+
+
+def main():
+    """Render the 64-scene batch workload (scenes.batch_workload) at a given launch shape, several
+    times, printing each render's wall time: the command the batch's kernel-trace profile runs."""
+    import argparse
+    import time
+
+    from . import compiler, scenes
+    from .engine import Engine
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--renders", type=int, default=3)
+    ap.add_argument("--wave-paths", type=int, default=1 << 28)
+    ap.add_argument("--trace-block", type=int, default=64)
+    ap.add_argument("--trace-blocks-per-cu", type=int, default=32)
+    ap.add_argument("--shade-blocks-per-cu", type=int, default=8)
+    ap.add_argument("--stack", type=int, default=16)
+    ap.add_argument("--no-graph", action="store_true", help="launch each kernel (so a kernel trace sees every one)")
+    a = ap.parse_args()
+    items = scenes.batch_workload()
+    strategy = compiler.RenderStrategy("mis", 12, 3, 0.95, 64)
+    comps = [compiler.compile_scene(sc, strategy, w, h) for sc, w, h, _ in items]
+    spp = [n for _, _, _, n in items]
+    eng = Engine(0)
+    br = BatchRenderer(eng, comps)
+    c = abi.PtBatchConfig()
+    c.wave_paths, c.trace_block, c.trace_blocks_per_cu = a.wave_paths, a.trace_block, a.trace_blocks_per_cu
+    c.shade_blocks_per_cu, c.stack, c.use_graph = a.shade_blocks_per_cu, a.stack, 0 if a.no_graph else 1
+    br.config = c
+    paths = sum(cc.width * cc.height * n for cc, n in zip(comps, spp))
+    for r in range(a.renders):
+        eng.finish()
+        t0 = time.perf_counter()
+        br.render(spp)
+        eng.finish()
+        dt = time.perf_counter() - t0
+        print(f"batch render {r}: {dt:.4f} s, {paths / dt / 1e6:.1f} Mpaths/s", flush=True)
+    br.close()
+    eng.close()
+
+
+if __name__ == "__main__":
+    main()
