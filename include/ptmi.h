@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define PT_ABI_VERSION 10u
+#define PT_ABI_VERSION 11u
 
 typedef enum pt_status {
     PT_OK = 0,
@@ -87,6 +87,9 @@ enum { PT_MAT_LAMBERT = 0, PT_MAT_DIELECTRIC = 1, PT_MAT_GGX = 2, PT_MAT_NULL = 
  * The host decides the arm from the authored values, as the reference's Planner does. */
 enum { PT_MED_ANALYTIC = 0, PT_MED_ABSORB = 1, PT_MED_DELTA = 2 };
 enum { PT_IOR_CONSTANT = 0, PT_IOR_CAUCHY = 1, PT_IOR_SELLMEIER = 2 };
+/* Volume scattering models (components/volume_scattering/): Henyey-Greenstein with hg_g
+ * (hg/hg.glsl) or Rayleigh, (3 / 16 pi)(1 + cos^2), parameter-free (rayleigh/rayleigh.glsl). */
+enum { PT_PHASE_HG = 0, PT_PHASE_RAYLEIGH = 1 };
 
 /* A spectral value in one radiometric role (src/components/descriptors.ts:43), as the
  * spectral color model evaluates it at a path's four wavelengths
@@ -96,10 +99,13 @@ enum { PT_IOR_CONSTANT = 0, PT_IOR_CAUCHY = 1, PT_IOR_SELLMEIER = 2 };
  *   PT_SPEC_SIGMOID    clamp(0.5 + 0.5 p / sqrt(1 + p^2), 0, 1),  p = (c0 t + c1) t + c2,
  *                      t = (lambda - 360) / 470              (Jakob-Hanika coefficients c)
  *   PT_SPEC_BLACKBODY  scale * Planck(c0 = kelvin, c1 = normalization)
+ *   PT_SPEC_GAUSSIAN   value + c2 exp((c1 (lambda - c0)) (lambda - c0)): a reflectance band
+ *                      (color/spectra/gaussian/gaussian.ts emit): value = floor, c0 = center (nm),
+ *                      c1 = -0.5 / width^2, c2 = peak - floor (the host's f64 values, in f32)
  * GREY and SIGMOID are multiplied by `scale` when scaled != 0 (role 'coefficient':
  * scale = 2 max(rgb)) and then by the D65 curve when d65 != 0 (role 'emission').
  * The host computes c from the authored RGB (spectral_coefficients, lookup.glsl:16-38). */
-enum { PT_SPEC_ZERO = 0, PT_SPEC_GREY = 1, PT_SPEC_SIGMOID = 2, PT_SPEC_BLACKBODY = 3 };
+enum { PT_SPEC_ZERO = 0, PT_SPEC_GREY = 1, PT_SPEC_SIGMOID = 2, PT_SPEC_BLACKBODY = 3, PT_SPEC_GAUSSIAN = 4 };
 typedef struct pt_spectrum {
     int32_t kind;
     int16_t scaled, d65;
@@ -112,8 +118,8 @@ typedef struct pt_spectrum {
 /* Material record: the MaterialProperties the generator bakes per material id
  * (src/compiler/generate/features/materials.ts, components/materials/{lambert,dielectric,ggx}).
  * PT_MAT_NULL is the reference's model 'none': a null interface that only bounds a medium
- * (components/transport/integrators/pt/pt.ts:378-388). Interior medium (homogeneous, HG
- * phase) when has_medium != 0 (features/materials.ts scene_medium_properties). */
+ * (components/transport/integrators/pt/pt.ts:378-388). Interior medium (homogeneous, HG or
+ * Rayleigh phase) when has_medium != 0 (features/materials.ts scene_medium_properties). */
 typedef struct pt_material {
     int32_t model;
     int32_t emissive;        /* material_is_emissive */
@@ -144,7 +150,7 @@ typedef struct pt_material {
      *   PT_IOR_CAUCHY     ior_coef = A, B, C (micrometres): A + B / l^2 + C / l^4
      *   PT_IOR_SELLMEIER  ior_coef = B1 B2 B3 C1 C2 C3: n^2 = 1 + sum B_i l^2 / (l^2 - C_i) */
     int32_t ior_kind;
-    float   pad2;
+    int32_t phase;           /* PT_PHASE_*: the medium's volume scattering model (medium.model) */
     float   ior_coef[8];
 } pt_material;
 

@@ -36,7 +36,8 @@
  * selection pdf for every light; 2: the light-selection random is reused for the light's own
  * sample without the cdf_rescale; 3: d log hg / dg with the sign of its (g - c) term flipped;
  * 4: the adjoint's in-scattering term -1/sigma_s instead of +1/sigma_s; 5: the adjoint's
- * sigma_a transmittance term dropped. */
+ * sigma_a transmittance term dropped; 6: Rayleigh's Cardano root with its second cube root added
+ * instead of subtracted; 7: a Gaussian band's exponent with one factor of (lambda - center). */
 #ifdef ORACLE_PLANTS
 static int g_plant = 0;
 void or_set_plant(int k) { g_plant = k; }
@@ -939,13 +940,17 @@ static inline float blackbody_lane(float kelvin, float normalization, float lam)
     float ratio2 = ratio * ratio;
     return normalization * (EXP(q / 550.0f) - 1.0f) * ratio2 * ratio2 * ratio / (EXP(q / lam) - 1.0f);
 }
-/* color_lift_{reflectance,coefficient,emission} (lookup.glsl:50-60) or scale * blackbody */
+/* color_lift_{reflectance,coefficient,emission} (lookup.glsl:50-60), scale * blackbody, or a
+ * Gaussian reflectance band (color/spectra/gaussian/gaussian.ts emit) */
 static sp spectrum_eval(const or_scene* S, const pt_spectrum* r, const wl_t* wl) {
     float v[4];
     for (int i = 0; i < 4; i++) {
         float lam = wl->lam[i], x;
         if (r->kind == PT_SPEC_BLACKBODY) x = r->scale * blackbody_lane(r->c[0], r->c[1], lam);
-        else {
+        else if (r->kind == PT_SPEC_GAUSSIAN) {   /* gaussian.ts emit: floor + (peak - floor) exp(B s s) */
+            float sh = lam - r->c[0];
+            x = r->value + r->c[2] * EXP(PLANT(7) ? r->c[1] * sh : (r->c[1] * sh) * sh);
+        } else {
             x = r->kind == PT_SPEC_ZERO ? 0.0f : (r->kind == PT_SPEC_GREY ? r->value : spectral_bounded_lane(r, lam));
             if (r->scaled) x = r->scale * x;
             if (r->d65) { float c[4]; spectral_curves(S, lam, c); x = x * c[3]; }
@@ -1617,15 +1622,33 @@ static inline float hg_value(v3 wi, v3 wo, float g) {
     float d = 1.0f + g * g - 2.0f * g * c;
     return (1.0f - g * g) / (4.0f * PTM_PI * d * sqrtf(fmaxf_(d, 1e-8f)));
 }
-static inline float hg_pdf(v3 wi, v3 wo, float g, const wl_t* wl) { float h = hg_value(wi, wo, g); return s_avg(wl, sgrey(h)); }
+/* Rayleigh phase (volume_scattering/rayleigh/rayleigh.glsl): (3 / 16 pi)(1 + c^2), c from the
+ * propagation direction, the constant folded in fp32; parameter-free and grayscale. */
+static inline float rayleigh_value(v3 wi, v3 wo) {
+    float c = dot(wi, neg(wo));
+    return (3.0f / (16.0f * PTM_PI)) * (1.0f + c * c);
+}
+/* The medium's phase (kernel_phase.glsl dispatch over the volume scattering models). */
+static inline float phase_value(int model, v3 wi, v3 wo, float g) {
+    return model == PT_PHASE_RAYLEIGH ? rayleigh_value(wi, wo) : hg_value(wi, wo, g);
+}
+static inline float phase_pdf(int model, v3 wi, v3 wo, float g, const wl_t* wl) { return s_avg(wl, sgrey(phase_value(model, wi, wo, g))); }
 static inline void build_basis(v3 n, v3* t, v3* b) {   /* glsl/core/math.glsl:63-67 */
     v3 up = fabsf(n.y) < 0.999f ? V(0.0f, 1.0f, 0.0f) : V(1.0f, 0.0f, 0.0f);
     *t = normalize(cross(up, n));
     *b = cross(n, *t);
 }
-static v3 hg_sample(v3 wo, float g, float x0, float x1, float* pdf, const wl_t* wl) {
+/* hg_sample (hg.glsl) and rayleigh_sample (rayleigh.glsl:15-33: the marginal (3/8)(1 + mu^2)
+ * inverted by Cardano's formula for mu^3 + 3 mu + (4 - 8 xi) = 0, u = 4 xi - 2, D = sqrt(u^2 + 1),
+ * mu = cbrt(u + D) - cbrt(D - u), clamped); both weight 1, the same frame around -wo. */
+static v3 phase_sample(int model, v3 wo, float g, float x0, float x1, float* pdf, const wl_t* wl) {
     float cos_theta;
-    if (fabsf(g) < 1e-3f) cos_theta = 1.0f - 2.0f * x0;
+    if (model == PT_PHASE_RAYLEIGH) {
+        float u = 4.0f * x0 - 2.0f;
+        float D = sqrtf(u * u + 1.0f);
+        cos_theta = POW(u + D, 1.0f / 3.0f) + (PLANT(6) ? 1.0f : -1.0f) * POW(D - u, 1.0f / 3.0f);
+        cos_theta = cos_theta < -1.0f ? -1.0f : (cos_theta > 1.0f ? 1.0f : cos_theta);
+    } else if (fabsf(g) < 1e-3f) cos_theta = 1.0f - 2.0f * x0;
     else {
         float sq = (1.0f - g * g) / (1.0f - g + 2.0f * g * x0);
         cos_theta = (1.0f + g * g - sq * sq) / (2.0f * g);
@@ -1636,7 +1659,7 @@ static v3 hg_sample(v3 wo, float g, float x0, float x1, float* pdf, const wl_t* 
     v3 nw = neg(wo);
     build_basis(nw, &t, &b);
     v3 wi = normalize(add(add(muls(t, sin_theta * COS(phi)), muls(b, sin_theta * SIN(phi))), muls(nw, cos_theta)));
-    *pdf = hg_pdf(wi, wo, g, wl);
+    *pdf = phase_pdf(model, wi, wo, g, wl);
     return wi;
 }
 /* shadow_transmittance, media walker (transport/shadow/media/media.glsl:29-62). */
@@ -1739,7 +1762,7 @@ static sp equiangular_sample_direct(const or_scene* S, const or_program* P, v3 o
     v3 wi = V(to_light.x / dist, to_light.y / dist, to_light.z / dist);
     sp vis = shadow_transmittance_media(S, p_evt, wi, pos, crossings_left, n_any, wl, -2, NULL);
     if (s_black(wl, vis)) return sgrey(0.0f);
-    float f = hg_value(wi, neg(d), Mm->hg_g);
+    float f = phase_value(Mm->phase, wi, neg(d), Mm->hg_g);
     sp c = smul(smul(smul(smul(T, T_seg), sigma_s_of(S, Mm, wl)), sgrey(f)), sdivs(intensity, d2));
     c = smul(c, vis);
     return sdivs(c, pdf_t * select_pdf);
@@ -1789,6 +1812,9 @@ static sp transport_trace(const or_scene* S, const or_program* P, v3 ro, v3 rd, 
                     v3 p_evt = add(ro, muls(rd, ms.t));
                     v3 wo_med = neg(rd);
                     float g = Mm->hg_g;
+                    const int phase = Mm->phase;
+                    /* only a Henyey-Greenstein medium has a phase parameter (g) */
+                    const int g_param = A && med_mat == A->p->medium_material && phase == PT_PHASE_HG;
                     const int frozen = !wl->spec && proposal_of(S, Mm);
                     const float gq = frozen ? g_prop_g : g;     /* the phase sampling's g */
                     if (nee && !P->equiangular) {   /* light_sample_direct_medium (techniques/light/light_medium.glsl:7-18) */
@@ -1801,9 +1827,9 @@ static sp transport_trace(const or_scene* S, const or_program* P, v3 ro, v3 rd, 
                                                                 S->has_null ? P->max_null - null_crossings : 0, n_any, wl,
                                                                 A ? A->p->medium_material : -2, A ? &sh_len : NULL);
                             if (!s_black(wl, vis)) {
-                                float f = hg_value(ls.wi, wo_med, g);
+                                float f = phase_value(phase, ls.wi, wo_med, g);
                                 float w = 1.0f;
-                                if (mis && !(ls.flags & 1)) w = power_heuristic(ls.pdf, hg_pdf(ls.wi, wo_med, gq, wl));
+                                if (mis && !(ls.flags & 1)) w = power_heuristic(ls.pdf, phase_pdf(phase, ls.wi, wo_med, gq, wl));
                                 sp c = smul(smul(T, ls.radiance), sgrey(f));
                                 c = smul(c, vis);
                                 c = smuls(c, w);
@@ -1811,7 +1837,7 @@ static sp transport_trace(const or_scene* S, const or_program* P, v3 ro, v3 rd, 
                                 if (A) {
                                     double Dl[ADJ_N];
                                     memcpy(Dl, A->D, sizeof Dl);
-                                    if (med_mat == A->p->medium_material) Dl[6] += dlog_hg(g, dot(ls.wi, neg(wo_med)));
+                                    if (g_param) Dl[6] += dlog_hg(g, dot(ls.wi, neg(wo_med)));
                                     for (int ch = 0; ch < 6; ch++) Dl[ch] -= sh_len;
                                     adj_scatter(A, sdivs(c, ls.pdf), Dl);
                                 }
@@ -1821,9 +1847,9 @@ static sp transport_trace(const or_scene* S, const or_program* P, v3 ro, v3 rd, 
                     /* kernel_sample_phase (techniques/kernel/kernel_phase.glsl:7-15) */
                     float z0 = rnd(rng), z1 = rnd(rng);
                     float ppdf;
-                    v3 wi = hg_sample(wo_med, gq, z0, z1, &ppdf, wl);
-                    if (frozen) T = smuls(T, hg_value(wi, wo_med, g) / hg_value(wi, wo_med, gq));
-                    if (A && med_mat == A->p->medium_material) A->D[6] += dlog_hg(g, dot(wi, neg(wo_med)));
+                    v3 wi = phase_sample(phase, wo_med, gq, z0, z1, &ppdf, wl);
+                    if (frozen) T = smuls(T, phase_value(phase, wi, wo_med, g) / phase_value(phase, wi, wo_med, gq));
+                    if (g_param) A->D[6] += dlog_hg(g, dot(wi, neg(wo_med)));
                     prev_was_delta = 0; prev_bsdf_pdf = ppdf; prev_qp = p_evt;
                     ro = p_evt; rd = wi;
                     /* roulette_interior (pt.ts:230-283), present when some medium weights absorption;
@@ -2415,6 +2441,32 @@ void or_oct_map(const float* in, int n, int inverse, float c, float s, float* ou
         else ptm_oct_dir(in[2 * i], in[2 * i + 1], c, s, &out[3 * i], &out[3 * i + 1], &out[3 * i + 2]);
     }
 }
+/* Phase-function probes (RGB wavelengths): values at direction pairs (wi, wo: 6 floats each),
+ * and samples about wo at xi (2 floats each) with their pdfs. model is a PT_PHASE_* value. */
+void or_phase_value(int model, float g, const float* dirs, int n, float* out) {
+    for (int i = 0; i < n; i++) {
+        const float* d = dirs + 6 * (size_t)i;
+        out[i] = phase_value(model, V(d[0], d[1], d[2]), V(d[3], d[4], d[5]), g);
+    }
+}
+void or_phase_sample(int model, float g, const float* wo, const float* xi, int n, float* wi, float* pdf) {
+    wl_t wl0; memset(&wl0, 0, sizeof wl0);
+    for (int i = 0; i < n; i++) {
+        v3 w = phase_sample(model, V(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]), g, xi[2 * i], xi[2 * i + 1], &pdf[i], &wl0);
+        wi[3 * i] = w.x; wi[3 * i + 1] = w.y; wi[3 * i + 2] = w.z;
+    }
+}
+/* Spectral records at n wavelengths (one lane each), through the kernels' evaluation. */
+void or_spectrum_eval(const or_scene* S, const pt_spectrum* r, const float* lam, int n, float* out) {
+    for (int i = 0; i < n; i++) {
+        wl_t wl; memset(&wl, 0, sizeof wl);
+        wl.spec = 1;
+        for (int k = 0; k < 4; k++) wl.lam[k] = lam[i];
+        sp v = spectrum_eval(S, r, &wl);
+        out[i] = v.x;
+    }
+}
+
 /* Environment probes: pdf of directions, and samples with their pdfs. */
 void or_env_pdf(const or_scene* S, const float* dirs, int n, float* out) {
     for (int i = 0; i < n; i++) out[i] = environment_pdf(S, V(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]));

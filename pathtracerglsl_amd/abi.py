@@ -7,7 +7,7 @@ import ctypes as C
 
 import numpy as np
 
-ABI_VERSION = 10
+ABI_VERSION = 11
 
 OBJ_MESH, OBJ_QUAD, OBJ_SPHERE, OBJ_INSTANCES = 0, 1, 2, 3
 MAT_LAMBERT, MAT_DIELECTRIC, MAT_GGX, MAT_NULL, MAT_MIRROR, MAT_ROUGH_DIELECTRIC = 0, 1, 2, 3, 4, 5
@@ -25,7 +25,8 @@ EST_PT, EST_NEE, EST_MIS = 0, 1, 2
 COLOR_RGB, COLOR_SPECTRAL = 0, 1
 WL_VISIBLE, WL_UNIFORM = 0, 1
 MLS_VERTEX, MLS_EQUIANGULAR = 0, 1
-SPEC_ZERO, SPEC_GREY, SPEC_SIGMOID, SPEC_BLACKBODY = 0, 1, 2, 3
+SPEC_ZERO, SPEC_GREY, SPEC_SIGMOID, SPEC_BLACKBODY, SPEC_GAUSSIAN = 0, 1, 2, 3, 4
+PHASE_HG, PHASE_RAYLEIGH = 0, 1
 
 SPECTRUM_DTYPE = np.dtype([
     ("kind", "<i4"), ("scaled", "<i2"), ("d65", "<i2"), ("value", "<f4"), ("scale", "<f4"),
@@ -45,7 +46,7 @@ MATERIAL_DTYPE = np.dtype([
     ("s_albedo", SPECTRUM_DTYPE), ("s_emission", SPECTRUM_DTYPE), ("s_f0", SPECTRUM_DTYPE),
     ("s_transmittance", SPECTRUM_DTYPE), ("s_sigma_s", SPECTRUM_DTYPE), ("s_sigma_a", SPECTRUM_DTYPE),
     ("medium_emission", "<f4", 4), ("s_medium_emission", SPECTRUM_DTYPE), ("majorant", "<f4", 2),
-    ("ior_kind", "<i4"), ("pad2", "<f4"), ("ior_coef", "<f4", 8),
+    ("ior_kind", "<i4"), ("phase", "<i4"), ("ior_coef", "<f4", 8),
 ])
 LIGHT_DTYPE = np.dtype([
     ("kind", "<i4"), ("region", "<i4"), ("select_pdf", "<f4"), ("cdf", "<f4"),

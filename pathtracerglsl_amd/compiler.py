@@ -20,6 +20,8 @@ MODELS = {"lambert": abi.MAT_LAMBERT, "dielectric": abi.MAT_DIELECTRIC, "ggx": a
           "mirror": abi.MAT_MIRROR, "rough_dielectric": abi.MAT_ROUGH_DIELECTRIC}
 # models whose emission term is identically zero (their descriptors' emissive capability is false)
 NON_EMISSIVE = ("mirror", "rough_dielectric")
+# volume scattering models (components/volume_scattering/index.ts registry): medium.model
+PHASE_MODELS = {"hg": abi.PHASE_HG, "rayleigh": abi.PHASE_RAYLEIGH}
 
 
 class CompileError(ValueError):
@@ -310,6 +312,22 @@ def _analyze_validate(scene, strategy):
     for name, m in mats.items():
         if m.get("model") not in MODELS:
             raise CompileError(f"material '{name}': model '{m.get('model')}' is not supported by this backend")
+        if spectral.is_gaussian(m.get("emission")):
+            raise CompileError(f"material '{name}': emission: gaussian values are reflectances only")
+        for k in ("albedo", "f0", "transmittance"):
+            if spectral.is_gaussian(m.get(k)):
+                try:
+                    spectral.gaussian_params(m[k])
+                except spectral.SpectralDataError as e:
+                    raise CompileError(f"material '{name}': {k}: {e}") from None
+        med = m.get("medium")
+        if med is not None:
+            if med.get("model", "hg") not in PHASE_MODELS:
+                raise CompileError(f"material '{name}': medium model '{med.get('model')}' is not a volume scattering "
+                                   f"model (known: {', '.join(PHASE_MODELS)})")
+            for k in ("sigma_a", "sigma_s", "emission"):
+                if spectral.is_gaussian(med.get(k)):
+                    raise CompileError(f"material '{name}': medium.{k}: gaussian values are reflectances only")
     for o in scene.get("objects", []):
         if o.get("material") not in mats:
             raise CompileError(f"object references unknown material '{o.get('material')}'")
@@ -318,7 +336,11 @@ def _analyze_validate(scene, strategy):
     env = scene.get("environment", {"type": "none"})
     if env.get("type") not in ("none", "procedural", "constant"):
         raise CompileError(f"environment '{env.get('type')}' is not supported by this backend")
+    if spectral.is_gaussian(env.get("color")):
+        raise CompileError("environment.color: gaussian values are reflectances only")
     for l in scene.get("lights", []):
+        if spectral.is_gaussian(l.get("radiance", l.get("intensity"))):
+            raise CompileError("light emission: gaussian values are reflectances only")
         if l["type"] == "quad":
             area = np.linalg.norm(np.cross(l["edge1"], l["edge2"]))
             if area < 1e-8:
@@ -449,6 +471,7 @@ def compile_scene(scene, strategy=None, width=256, height=256, reset_salt=1234):
             rec["s_sigma_a"] = srec(_rgb3(med.get("sigma_a", 0.0)), spectral.ROLE_COEFFICIENT)
             rec["s_sigma_s"] = srec(sig_s, spectral.ROLE_COEFFICIENT)
             rec["hg_g"] = float(med.get("phase_g", 0.0))
+            rec["phase"] = PHASE_MODELS[med.get("model", "hg")]
             eps = med.get("emission", 0.0)
             rec["medium_emission"][:3] = spectral.rgb_of(eps)
             rec["s_medium_emission"] = srec(eps, spectral.ROLE_EMISSION)
@@ -697,8 +720,18 @@ def describe(scene, strategy):
     for m in scene.get("materials", {}).values():
         if m["model"] not in models:
             models.append(m["model"])
-    media_models = ["hg"] if any(m.get("medium") is not None and medium_arm(m["medium"], strategy) != abi.MED_ABSORB
-                                 for m in scene.get("materials", {}).values()) else []
+    media_models = []
+    for m in scene.get("materials", {}).values():
+        med = m.get("medium")
+        if med is not None and medium_arm(med, strategy) != abi.MED_ABSORB and med.get("model", "hg") not in media_models:
+            media_models.append(med.get("model", "hg"))
+    # the spectral kinds the program evaluates (a Gaussian band is a reflectance of a material)
+    kinds = []
+    if strategy.color == "spectral":
+        for m in scene.get("materials", {}).values():
+            for k in ("albedo", "f0", "transmittance"):
+                if spectral.is_gaussian(m.get(k)) and "gaussian" not in kinds:
+                    kinds.append("gaussian")
     d = {
         "dispersion": None,
         "environment": ({"type": "none"} if env.get("type", "none") == "none" else
@@ -728,7 +761,7 @@ def describe(scene, strategy):
             **({"aovs": list(strategy.aovs)} if strategy.aovs else {}),
         },
         "media": {"deflecting": False, "heterogeneousArms": False, "models": media_models},
-        "spectrumKinds": [],
+        "spectrumKinds": kinds,
         "view": {"tonemap": {"type": "reinhard"}},
     }
     _ = lights

@@ -268,14 +268,14 @@ bool lower(const JVal& d, int32_t width, int32_t height, int32_t fif, pt_program
         if (truthy(media->get("heterogeneousArms"))) return L.unsupported("media.heterogeneousArms", "true (expression media)");
         if (const JVal* models = media->get("models"); models && models->kind == JVal::ARR)
             for (const JVal& md : models->arr)
-                if (!is_str(&md, "hg")) return L.unsupported("phase model", quoted(&md));
+                if (!(is_str(&md, "hg") || is_str(&md, "rayleigh"))) return L.unsupported("phase model", quoted(&md));
     }
     if (const JVal* isec = d.get("intersection")) {
         if (truthy(isec->get("drivenPlacement"))) return L.unsupported("intersection.drivenPlacement", "true");
     }
     if (const JVal* kinds = d.get("spectrumKinds"); kinds && kinds->kind == JVal::ARR)
         for (const JVal& k : kinds->arr)
-            if (!(is_str(&k, "blackbody") || is_str(&k, "cauchy") || is_str(&k, "sellmeier")))
+            if (!(is_str(&k, "blackbody") || is_str(&k, "cauchy") || is_str(&k, "sellmeier") || is_str(&k, "gaussian")))
                 return L.unsupported("spectrum kind", quoted(&k));
     const JVal* env = d.get("environment");
     const JVal* et = env ? env->get("type") : nullptr;

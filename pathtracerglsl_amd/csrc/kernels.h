@@ -596,7 +596,7 @@ __global__ void __launch_bounds__(SBLOCK) k_bin(DevScene Sk, Paths P, int qsel, 
                         const float d2 = gmax(dot(to_light, to_light), 1e-8f);
                         const float dist = sqrtf(d2);
                         eq_wi = mk3(to_light.x / dist, to_light.y / dist, to_light.z / dist);
-                        const float f = hg_value(eq_wi, -d, Mm.hg_g);
+                        const float f = phase_value(Mm.phase, eq_wi, -d, Mm.hg_g);
                         eq_a = (((Tp * T_seg) * sigma_s_of<SPEC>(S, Mm, wl)) * sgrey(f)) * (intensity / d2);
                         eq_pdf = pdf_t * select_pdf;
                         eq_o = p_evt; eq_pos = pos;
@@ -762,7 +762,10 @@ __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene Sk, Paths P, int qsel
                     const int mmat = material_of(S, medium);
                     const pt_material& Mm = S.materials[mmat];
                     const float g = Mm.hg_g;
+                    const int phase = Mm.phase;
+                    // the phase's own parameter: g of a Henyey-Greenstein medium (Rayleigh has none)
                     const bool param_medium = ADJ && mmat == P.adj_medium;
+                    const bool param_g = param_medium && phase == PT_PHASE_HG;
                     f3 ro = xyz(P.ray_o[i]);
                     const float t = P.hit[i].x;
                     f3 p_evt = ro + rd * t;
@@ -775,24 +778,24 @@ __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene Sk, Paths P, int qsel
                         if (ls.pdf > 0.0f) {
                             s_l = p_evt + ls.wi * ls.distance;
                             s_o = p_evt; s_d = ls.wi; s_cross = crossings_left;
-                            float f = hg_value(ls.wi, wo_med, g);
+                            float f = phase_value(phase, ls.wi, wo_med, g);
                             s_w = 1.0f;
-                            if (MIS && !(ls.flags & 1)) s_w = power_heuristic(ls.pdf, hg_pdf<SPEC>(ls.wi, wo_med, g));
+                            if (MIS && !(ls.flags & 1)) s_w = power_heuristic(ls.pdf, phase_pdf<SPEC>(phase, ls.wi, wo_med, g));
                             s_a = (T * ls.radiance) * sgrey(f);
                             s_pdf = ls.pdf;
                             shadow = true;
                             if (ADJ) {
 #pragma unroll
                                 for (int kk = 0; kk < ADJ_N; kk++) sD[kk] = A.D[kk];
-                                if (param_medium) sD[6] += dlog_hg(g, dot(ls.wi, -wo_med));
+                                if (param_g) sD[6] += dlog_hg(g, dot(ls.wi, -wo_med));
                             }
                         }
                     }
                     // kernel_sample_phase
                     float z0 = rng.next(), z1 = rng.next();
                     float ppdf;
-                    f3 wi = hg_sample<SPEC>(wo_med, g, z0, z1, ppdf);
-                    if (param_medium) { A.D[6] += dlog_hg(g, dot(wi, -wo_med)); A.dirty = true; }
+                    f3 wi = phase_sample<SPEC>(phase, wo_med, g, z0, z1, ppdf);
+                    if (param_g) { A.D[6] += dlog_hg(g, dot(wi, -wo_med)); A.dirty = true; }
                     cont = true;
                     // roulette_interior, present when some medium weights absorption; medium_survival =
                     // sigma_s / max(sigma_t, 1e-9) there, 1 on the tracking arm (features/materials.ts:731-772)

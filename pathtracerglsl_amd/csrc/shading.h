@@ -451,7 +451,7 @@ PT_DEV float blackbody_lane(float kelvin, float normalization, float lam) {
     return normalization * (ptm_exp(q / 550.0f) - 1.0f) * ratio2 * ratio2 * ratio / (ptm_exp(q / lam) - 1.0f);
 }
 // A pt_spectrum at the path's four wavelengths: color_lift_{reflectance,coefficient,emission}
-// (lookup.glsl:50-60) for an authored RGB, or scale * blackbody_emission.
+// (lookup.glsl:50-60) for an authored RGB, scale * blackbody_emission, or a Gaussian band.
 PT_DEV sp4 spectrum_eval(const DevScene& S, const pt_spectrum& r, const Wl& wl) {
     float v[4];
     for (int i = 0; i < 4; i++) {
@@ -459,6 +459,9 @@ PT_DEV sp4 spectrum_eval(const DevScene& S, const pt_spectrum& r, const Wl& wl) 
         float x;
         if (r.kind == PT_SPEC_BLACKBODY) {
             x = r.scale * blackbody_lane(r.c[0], r.c[1], lam);
+        } else if (r.kind == PT_SPEC_GAUSSIAN) {          // gaussian.ts emit: floor + A exp(B s s)
+            const float sh = lam - r.c[0];
+            x = r.value + r.c[2] * ptm_exp((r.c[1] * sh) * sh);
         } else {
             x = r.kind == PT_SPEC_ZERO ? 0.0f : (r.kind == PT_SPEC_GREY ? r.value : spectral_bounded_lane(r, lam));
             if (r.scaled) x = r.scale * x;
@@ -1029,13 +1032,32 @@ PT_DEV float hg_value(f3 wi, f3 wo, float g) {
     float d = 1.0f + g * g - 2.0f * g * c;
     return (1.0f - g * g) / (4.0f * PTM_PI * d * sqrtf(gmax(d, 1e-8f)));
 }
+// rayleigh_eval (volume_scattering/rayleigh/rayleigh.glsl): (3 / 16 pi)(1 + c^2), c measured from
+// the propagation direction; 3 / (16 PI) folded in fp32.
+PT_DEV float rayleigh_value(f3 wi, f3 wo) {
+    const float c = dot(wi, -wo);
+    return (3.0f / (16.0f * PTM_PI)) * (1.0f + c * c);
+}
+// The medium's phase function (kernel_phase.glsl dispatch): value and pdf (grayscale: equal).
+PT_DEV float phase_value(int model, f3 wi, f3 wo, float g) {
+    return model == PT_PHASE_RAYLEIGH ? rayleigh_value(wi, wo) : hg_value(wi, wo, g);
+}
 template <bool SPEC>
-PT_DEV float hg_pdf(f3 wi, f3 wo, float g) { float h = hg_value(wi, wo, g); return spectrum_average<SPEC>(sgrey(h)); }
+PT_DEV float phase_pdf(int model, f3 wi, f3 wo, float g) { return spectrum_average<SPEC>(sgrey(phase_value(model, wi, wo, g))); }
+// hg_sample (hg.glsl) and rayleigh_sample (rayleigh.glsl: the marginal (3/8)(1 + mu^2) inverted in
+// closed form, Cardano's root of mu^3 + 3 mu + (4 - 8 xi) = 0), then the same frame around the
+// propagation direction; the weight is 1 for both.
 template <bool SPEC>
-PT_DEV f3 hg_sample(f3 wo, float g, float x0, float x1, float& pdf) {
+PT_DEV f3 phase_sample(int model, f3 wo, float g, float x0, float x1, float& pdf) {
     float cos_theta;
-    if (fabsf(g) < 1e-3f) cos_theta = 1.0f - 2.0f * x0;
-    else {
+    if (model == PT_PHASE_RAYLEIGH) {
+        const float u = 4.0f * x0 - 2.0f;
+        const float D = sqrtf(u * u + 1.0f);
+        cos_theta = ptm_pow(u + D, 1.0f / 3.0f) - ptm_pow(D - u, 1.0f / 3.0f);
+        cos_theta = gclamp(cos_theta, -1.0f, 1.0f);
+    } else if (fabsf(g) < 1e-3f) {
+        cos_theta = 1.0f - 2.0f * x0;
+    } else {
         float sq = (1.0f - g * g) / (1.0f - g + 2.0f * g * x0);
         cos_theta = (1.0f + g * g - sq * sq) / (2.0f * g);
     }
@@ -1048,7 +1070,7 @@ PT_DEV f3 hg_sample(f3 wo, float g, float x0, float x1, float& pdf) {
     f3 b = cross(nw, t);
     float sn, cs; ptm_sincos(phi, &sn, &cs);
     f3 wi = normalize((t * (sin_theta * cs) + b * (sin_theta * sn)) + nw * cos_theta);
-    pdf = hg_pdf<SPEC>(wi, wo, g);
+    pdf = phase_pdf<SPEC>(model, wi, wo, g);
     return wi;
 }
 

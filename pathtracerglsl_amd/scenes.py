@@ -295,6 +295,35 @@ def furnace_scatter():
     return sc
 
 
+def rayleigh_furnace(sigma_s=(0.1, 0.2, 0.4)):
+    """spectral-rayleigh-furnace (tests/witnesses/scenes/spectralMedia.ts:65-73): the F-BOX-M haze
+    with the Rayleigh phase (parameter-free, volume_scattering/rayleigh). The phase law changes
+    without changing the furnace's equilibrium: every channel is 0.4 in expectation."""
+    sc = furnace_scatter()
+    sc["id"] = "rayleigh-furnace"
+    sc["materials"]["haze"] = {"model": "none", "medium": {"model": "rayleigh", "sigma_a": 0.0, "sigma_s": list(sigma_s)}}
+    return sc
+
+
+GAUSSIAN_BAND = {"gaussian": {"center": 610.0, "width": 25.0, "peak": 0.85, "floor": 0.04}}
+
+
+def gaussian_plane(band=None):
+    """spectral-gaussian-plane (tests/witnesses/scenes/spectralGaussian.ts): a Lambert plane with a
+    Gaussian reflectance band under a unit white constant sky sampled as a light, seen head-on. The
+    one-bounce pixel is the band's RGB stand-in: in a spectral program the integral of sensor x band
+    x D65, in an RGB program the stand-in itself."""
+    return {
+        "id": "spectral-gaussian-plane",
+        "materials": {"plane": {"model": "lambert", "albedo": band or GAUSSIAN_BAND}},
+        "objects": [{"type": "quad", "corner": [-50.0, -50.0, -1.0], "edge1": [100.0, 0.0, 0.0],
+                     "edge2": [0.0, 100.0, 0.0], "material": "plane"}],
+        "lights": [],
+        "environment": {"type": "constant", "color": [1.0, 1.0, 1.0], "intensity": 1.0, "sample_as_light": True},
+        "camera": {"type": "pinhole", "position": [0.0, 0.0, 0.0], "target": [0.0, 0.0, -1.0], "fov": 0.8},
+    }
+
+
 def cornell_glass_medium(light_radiance=(12.0, 12.0, 12.0)):
     """The spectral-media workload's geometry in RGB: the Cornell box with a radius-0.5 smooth
     dielectric sphere (eta 1.5) at the centre, filled with a chromatic homogeneous medium

@@ -178,9 +178,69 @@ def bounded_input(rgb, role):
     return scale, [F32(v / scale) for v in r]
 
 
+def is_gaussian(value):
+    return isinstance(value, dict) and "gaussian" in value
+
+
+def gaussian_params(value):
+    """(center, width, peak, floor) of {'gaussian': {...}} (gaussian.ts fields), validated."""
+    g = value["gaussian"]
+    try:
+        center, width, peak = float(g["center"]), float(g["width"]), float(g["peak"])
+        floor = float(g.get("floor", 0.0))
+    except (KeyError, TypeError, ValueError):
+        raise SpectralDataError("gaussian values need numeric center, width and peak") from None
+    if not math.isfinite(center):
+        raise SpectralDataError("gaussian center must be finite")
+    if not (width > 0.0 and math.isfinite(width)):
+        raise SpectralDataError("gaussian width must be positive")
+    if not (0.0 <= peak <= 1.0 and 0.0 <= floor <= 1.0):
+        raise SpectralDataError("gaussian peak and floor must lie in [0, 1]")
+    return center, width, peak, floor
+
+
+def gaussian_reflectance(value, lam):
+    """gaussianReflectance (gaussian.ts): the band's reflectance at lam nm, float64."""
+    center, width, peak, floor = gaussian_params(value)
+    d = (lam - center) / width
+    return floor + (peak - floor) * math.exp(-0.5 * d * d)
+
+
+def gaussian_stand_in(value):
+    """gaussianStandIn (gaussian.ts): D65 reflected by the band in linear sRGB, each channel
+    divided by the same projection of a perfect white reflector, Simpson's rule at 0.5 nm in
+    each 5 nm interval of the tabulated observer and D65 (linear between knots), float64."""
+    rows = curve_rows().astype(np.float64)
+    num, den = [0.0, 0.0, 0.0], [0.0, 0.0, 0.0]
+    for interval in range(rows.shape[0] - 1):
+        for sample in range(11):
+            fraction = sample / 10
+            lam = 360 + 5 * (interval + fraction)
+            weight = 0.5 / 3 * (1 if sample in (0, 10) else (4 if sample % 2 else 2))
+            cmf = [rows[interval][j] * (1 - fraction) + rows[interval + 1][j] * fraction for j in range(3)]
+            light = weight * (rows[interval][3] * (1 - fraction) + rows[interval + 1][3] * fraction)
+            g = gaussian_reflectance(value, lam)
+            for c in range(3):
+                response = XYZ_TO_RGB[c][0] * cmf[0] + XYZ_TO_RGB[c][1] * cmf[1] + XYZ_TO_RGB[c][2] * cmf[2]
+                num[c] += g * light * response
+                den[c] += light * response
+    return [num[c] / den[c] for c in range(3)]
+
+
 def spectrum_record(value, role):
-    """pt_spectrum for an authored value (rgb triple, scalar, or {'blackbody': {...}})."""
+    """pt_spectrum for an authored value (rgb triple, scalar, {'blackbody': {...}} or
+    {'gaussian': {...}})."""
     rec = np.zeros(1, abi.SPECTRUM_DTYPE)[0]
+    if is_gaussian(value):
+        if role != ROLE_REFLECTANCE:
+            raise SpectralDataError("gaussian values are reflectances only")
+        center, width, peak, floor = gaussian_params(value)
+        # gaussian.ts emit: Spectrum(floor) + (peak - floor) * exp(-0.5 / width^2 * s * s), each
+        # constant a GLSL literal of the host's float64 value
+        rec["kind"] = abi.SPEC_GAUSSIAN
+        rec["value"] = floor
+        rec["c"][:3] = [center, -0.5 / (width * width), peak - floor]
+        return rec
     if isinstance(value, dict) and "blackbody" in value:
         if role != ROLE_EMISSION:
             raise SpectralDataError("blackbody values are emission only")
@@ -210,11 +270,14 @@ def spectrum_record(value, role):
 
 
 def rgb_of(value):
-    """RGB stand-in of an authored value in an RGB program (blackbody: blackbodyRGB)."""
+    """RGB stand-in of an authored value in an RGB program (blackbody: blackbodyRGB; a Gaussian
+    band: gaussianStandIn clipped to [0, 1])."""
     if isinstance(value, dict) and "blackbody" in value:
         bb = value["blackbody"]
         rgb, _ = blackbody_color(float(bb["kelvin"]))
         return [v * float(bb.get("scale", 1.0)) for v in rgb]
+    if is_gaussian(value):          # a reflectance: its stand-in clipped to [0, 1] (rgbStandIn)
+        return [min(1.0, max(0.0, c)) for c in gaussian_stand_in(value)]
     return [float(value)] * 3 if not hasattr(value, "__len__") else [float(v) for v in value]
 
 

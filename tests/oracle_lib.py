@@ -47,6 +47,12 @@ def load(variant=""):
     lib.or_bsdf_probe.argtypes = [vp, vp, vp, C.c_float, C.c_float, C.c_float, vp]
     lib.or_env_pdf.argtypes = [vp, vp, C.c_int, vp]
     lib.or_env_sample.argtypes = [vp, vp, C.c_int, vp, vp]
+    lib.or_phase_value.argtypes = [C.c_int, C.c_float, vp, C.c_int, vp]
+    lib.or_phase_value.restype = None
+    lib.or_phase_sample.argtypes = [C.c_int, C.c_float, vp, vp, C.c_int, vp, vp]
+    lib.or_phase_sample.restype = None
+    lib.or_spectrum_eval.argtypes = [vp, vp, vp, C.c_int, vp]
+    lib.or_spectrum_eval.restype = None
     lib.or_basic_eval.argtypes = [C.c_int, vp, vp, vp, C.c_int]
     lib.or_adjoint.argtypes = [vp, C.POINTER(abi.PtProgram), C.POINTER(abi.PtParams), C.c_int,
                                C.POINTER(abi.PtAdjointParams), vp, vp]

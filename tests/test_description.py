@@ -31,7 +31,7 @@ def _expected_estimator(d):
 
 # The number of the 505 frozen link maps this backend lowers: a lowering that starts rejecting
 # a case (or accepting one it cannot render) changes it.
-EXPECTED_ACCEPTED = 339
+EXPECTED_ACCEPTED = 342
 
 
 def test_library_lowers_or_names_every_snapshot_case():
@@ -73,6 +73,8 @@ def test_library_lowers_or_names_every_snapshot_case():
     # environments, expression media or spectral procedural skies)
     assert "spectral-image-constant-ref + oct" in accepted
     assert "two-light + uniform" in accepted                  # estimator.lighting.selection 'uniform'
+    assert "spectral-rayleigh-furnace + visible" in accepted  # media.models ['rayleigh']
+    assert {"spectral-gaussian-plane + spectral-mis", "spectral-gaussian-plane + spectral-none"} <= set(accepted)
 
 
 def test_chart_and_albedo_flag_lowering():

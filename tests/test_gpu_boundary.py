@@ -159,3 +159,20 @@ def test_uniform_program_needs_the_uniform_table(engine):
     engine.load_renderer(uni)
     engine.render_frames(1)
     assert np.isfinite(engine.read_export()).all()
+
+
+@pytest.mark.parametrize("record", ["spectral-gaussian-plane + spectral-mis", "spectral-rayleigh-furnace + visible"])
+def test_frozen_rayleigh_and_gaussian_records_render_like_the_oracle(engine, oracle, record):
+    # link maps naming media.models ['rayleigh'] and spectrumKinds ['gaussian'], lowered through
+    # pt_renderer_load_description over the witness scenes they were planned for
+    d = FIX["cases"][record]
+    if record.startswith("spectral-gaussian-plane"):
+        comp = compiler.compile_scene(scenes.gaussian_plane(), compiler.RenderStrategy("mis", 1, -1, color="spectral"), 24, 24)
+    else:
+        comp = compiler.compile_scene(scenes.rayleigh_furnace(), compiler.RenderStrategy("none", 48, -1, color="spectral"), 24, 24)
+    _say(f"[gpu] description '{record}' through pt_renderer_load_description")
+    g, o, counts, prog = _render_description(engine, oracle, d, comp)
+    assert prog.color == abi.COLOR_SPECTRAL and prog.max_bounces == d["measurement"]["maxBounces"]
+    assert o[..., :3].mean() > 0.02
+    assert np.array_equal(g[..., :3], o[..., :3])
+    assert engine.ray_counts() == (int(counts[0]), int(counts[1]))

@@ -175,3 +175,27 @@ def test_orthographic_slab_and_glow_match_oracle(engine, oracle, color):
     _, o, bad = _render_both(engine, oracle, f"ortho-glow-{color}", sc, compiler.RenderStrategy("none", 4, -1, color=color),
                              24, 24, 8)
     assert bad == 0.0
+
+
+def _cornell_rayleigh_medium():
+    sc = scenes.cornell_glass_medium()
+    med = sc["materials"]["glass"]["medium"]
+    sc["materials"]["glass"]["medium"] = {"model": "rayleigh", "sigma_s": med["sigma_s"], "sigma_a": med["sigma_a"]}
+    return sc
+
+
+@pytest.mark.parametrize("est", ["mis", "none"])
+def test_rayleigh_glass_medium_matches_oracle(engine, oracle, est):
+    # the Rayleigh phase (volume_scattering/rayleigh): medium NEE through its value and pdf, phase
+    # sampling by its closed-form inversion
+    _, o, bad = _render_both(engine, oracle, "rayleigh-glass-medium", _cornell_rayleigh_medium(),
+                             compiler.RenderStrategy(est, 16, 3), 32, 32, 8)
+    assert o[..., :3].max() > 0.0
+    assert bad == 0.0
+
+
+def test_rayleigh_furnace_matches_oracle(engine, oracle):
+    _, o, bad = _render_both(engine, oracle, "rayleigh-furnace", scenes.rayleigh_furnace(),
+                             compiler.RenderStrategy("none", 48, -1), 24, 24, 8)
+    assert abs(o[..., :3].mean() - 0.4) < 0.03
+    assert bad == 0.0

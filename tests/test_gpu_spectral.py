@@ -122,3 +122,27 @@ def test_spectral_arbitrary_colours_match_oracle(engine, oracle):
                            compiler.RenderStrategy("mis", 8, 3, color="spectral"), 32, 32, 8)
     assert bad == 0.0
     assert engine.ray_counts() == (int(counts[0]), int(counts[1]))
+
+
+def test_spectral_rayleigh_furnace_matches_oracle(engine, oracle):
+    bad, _ = _compare(engine, oracle, "spectral-rayleigh-furnace", scenes.rayleigh_furnace(),
+                      compiler.RenderStrategy("none", 48, -1, color="spectral"), 24, 24, 8)
+    assert bad == 0.0
+
+
+@pytest.mark.parametrize("est,color", [("none", "spectral"), ("mis", "spectral"), ("mis", "rgb")])
+def test_gaussian_plane_matches_oracle(engine, oracle, est, color):
+    # a Gaussian reflectance band: evaluated at the wavelengths in a spectral program, its clipped
+    # RGB stand-in in an RGB one
+    bad, _ = _compare(engine, oracle, f"gaussian-plane-{color}", scenes.gaussian_plane(),
+                      compiler.RenderStrategy(est, 1, -1, color=color), 24, 24, 16)
+    assert bad == 0.0
+
+
+def test_spectral_cornell_with_gaussian_walls_matches_oracle(engine, oracle):
+    sc = scenes.cornell_box()
+    sc["materials"]["red"]["albedo"] = {"gaussian": {"center": 620.0, "width": 30.0, "peak": 0.8, "floor": 0.05}}
+    sc["materials"]["green"]["albedo"] = {"gaussian": {"center": 530.0, "width": 25.0, "peak": 0.7}}
+    bad, _ = _compare(engine, oracle, "cornell-gaussian-walls", sc,
+                      compiler.RenderStrategy("mis", 8, 3, color="spectral"), 32, 32, 8)
+    assert bad == 0.0
