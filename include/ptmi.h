@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define PT_ABI_VERSION 11u
+#define PT_ABI_VERSION 12u
 
 typedef enum pt_status {
     PT_OK = 0,
@@ -265,7 +265,14 @@ typedef struct pt_program {
                                * is host-derived scene data (pt_light.select_pdf / cdf, pt_env.select_prob);
                                * under PT_SEL_UNIFORM the library checks that the resident scene holds the
                                * uniform table */
+    int32_t  accumulation;    /* PT_ACC_*: estimator.accumulation.type (components/accumulator/) */
 } pt_program;
+
+/* Accumulators: the running average of every sample (average/average.glsl), or the latest sample
+ * alone (oneshot/oneshot.glsl: the picture updates live, the sample count still advances, and it
+ * keeps no statistic, so a oneshot program with variance != 0 is PT_E_INVALID). The guide
+ * pictures follow the same rule. */
+enum { PT_ACC_AVERAGE = 0, PT_ACC_ONESHOT = 1 };
 
 /* Light selection (components/lights/index.ts LIGHT_SELECTIONS): the power-weighted CDF (the
  * reference's default) or one of N lights with equal probability, the environment then taking

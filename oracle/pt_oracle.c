@@ -2337,9 +2337,11 @@ static void render_impl(const or_scene* S, const pt_program* prog, const pt_para
                     float* sp = samples + 3 * (((size_t)f * H + y) * W + x);
                     sp[0] = Lr.x; sp[1] = Lr.y; sp[2] = Lr.z;
                 }
-                avg_update(px, n, Lr.x, Lr.y, Lr.z);
-                if (aov_a) avg_update(aov_a + 4 * ((size_t)y * W + x), n, aov[0], aov[1], aov[2]);
-                if (aov_n) avg_update(aov_n + 4 * ((size_t)y * W + x), n, aov[3], aov[4], aov[5]);
+                /* average_update, or oneshot_update (accumulator/oneshot/oneshot.glsl): the latest sample */
+                const uint32_t na_ = prog->accumulation == PT_ACC_ONESHOT ? 0u : n;
+                avg_update(px, na_, Lr.x, Lr.y, Lr.z);
+                if (aov_a) avg_update(aov_a + 4 * ((size_t)y * W + x), na_, aov[0], aov[1], aov[2]);
+                if (aov_n) avg_update(aov_n + 4 * ((size_t)y * W + x), na_, aov[3], aov[4], aov[5]);
             }
             nc += (uint64_t)c1; na += (uint64_t)c2;
         }

@@ -7,7 +7,7 @@ import ctypes as C
 
 import numpy as np
 
-ABI_VERSION = 11
+ABI_VERSION = 12
 
 OBJ_MESH, OBJ_QUAD, OBJ_SPHERE, OBJ_INSTANCES = 0, 1, 2, 3
 MAT_LAMBERT, MAT_DIELECTRIC, MAT_GGX, MAT_NULL, MAT_MIRROR, MAT_ROUGH_DIELECTRIC = 0, 1, 2, 3, 4, 5
@@ -27,6 +27,7 @@ WL_VISIBLE, WL_UNIFORM = 0, 1
 MLS_VERTEX, MLS_EQUIANGULAR = 0, 1
 SPEC_ZERO, SPEC_GREY, SPEC_SIGMOID, SPEC_BLACKBODY, SPEC_GAUSSIAN = 0, 1, 2, 3, 4
 PHASE_HG, PHASE_RAYLEIGH = 0, 1
+ACC_AVERAGE, ACC_ONESHOT = 0, 1
 
 SPECTRUM_DTYPE = np.dtype([
     ("kind", "<i4"), ("scaled", "<i2"), ("d65", "<i2"), ("value", "<f4"), ("scale", "<f4"),
@@ -110,7 +111,7 @@ class PtProgram(C.Structure):
                 ("width", C.c_int32), ("height", C.c_int32), ("frames_in_flight", C.c_int32),
                 ("max_null_crossings", C.c_int32), ("wavelengths", C.c_int32), ("medium_light_sampling", C.c_int32),
                 ("variance", C.c_int32), ("volume_sampling", C.c_int32), ("aovs", C.c_int32),
-                ("env_chart", C.c_int32), ("light_selection", C.c_int32)]
+                ("env_chart", C.c_int32), ("light_selection", C.c_int32), ("accumulation", C.c_int32)]
 
 
 class PtParams(C.Structure):

@@ -20,6 +20,8 @@ MODELS = {"lambert": abi.MAT_LAMBERT, "dielectric": abi.MAT_DIELECTRIC, "ggx": a
           "mirror": abi.MAT_MIRROR, "rough_dielectric": abi.MAT_ROUGH_DIELECTRIC}
 # models whose emission term is identically zero (their descriptors' emissive capability is false)
 NON_EMISSIVE = ("mirror", "rough_dielectric")
+# accumulators (components/accumulator/index.ts registry): estimator.accumulation.type
+ACCUMULATIONS = {"average": abi.ACC_AVERAGE, "oneshot": abi.ACC_ONESHOT}
 # volume scattering models (components/volume_scattering/index.ts registry): medium.model
 PHASE_MODELS = {"hg": abi.PHASE_HG, "rayleigh": abi.PHASE_RAYLEIGH}
 
@@ -45,6 +47,7 @@ class RenderStrategy:
     volume_sampling: str = "analytic"     # estimator.volumeSampling: 'analytic' | 'delta-tracking'
     scattering: str = "full"              # measurement.scattering: 'full' | 'ignored' (absorbing-only media)
     light_selection: str = "power"        # estimator.lightSelection: 'power' | 'uniform'
+    accumulation: str = "average"         # estimator.accumulation.type: 'average' | 'oneshot'
 
 
 @dataclass
@@ -249,6 +252,10 @@ def _analyze_validate(scene, strategy):
         raise CompileError("russianRoulette.maxSurvival must be in (0, 1]")
     if strategy.color not in ("rgb", "spectral"):
         raise CompileError(f"unknown color model '{strategy.color}'")
+    if strategy.accumulation not in ACCUMULATIONS:
+        raise CompileError(f"Accumulation type '{strategy.accumulation}' not yet supported")
+    if strategy.accumulation == "oneshot" and strategy.variance:
+        raise CompileError("Accumulation 'oneshot' does not support variance (it keeps no statistic)")
     if strategy.wavelengths not in ("visible", "uniform"):
         raise CompileError(f"unknown wavelength sampler '{strategy.wavelengths}'")
     if strategy.medium_light_sampling not in ("vertex", "equiangular"):
@@ -667,6 +674,7 @@ def compile_scene(scene, strategy=None, width=256, height=256, reset_salt=1234):
     prog.medium_light_sampling = abi.MLS_EQUIANGULAR if strategy.medium_light_sampling == "equiangular" else abi.MLS_VERTEX
     prog.volume_sampling = abi.VS_DELTA_TRACKING if strategy.volume_sampling == "delta-tracking" else abi.VS_ANALYTIC
     prog.variance = int(bool(strategy.variance))
+    prog.accumulation = ACCUMULATIONS[strategy.accumulation]
     prog.aovs = sum({"albedo": abi.AOV_ALBEDO, "normal": abi.AOV_NORMAL}[a] for a in set(strategy.aovs))
     prog.env_chart = ENV_CHARTS[e.get("chart", "equirect")] + 1 if e.get("type") == "procedural" else abi.CHART_EQUIRECT + 1
     prog.width, prog.height = width, height
@@ -737,7 +745,7 @@ def describe(scene, strategy):
         "environment": ({"type": "none"} if env.get("type", "none") == "none" else
                         {"type": "procedural", "glsl": {"kind": "glsl", "source": PROC_SKY_GLSL}}),
         "estimator": {
-            "accumulation": {"type": "average", **({"variance": True} if strategy.variance else {})},
+            "accumulation": {"type": strategy.accumulation, **({"variance": True} if strategy.variance else {})},
             "envSampler": {"chart": env.get("chart", "equirect"), "compensation": bool(env.get("compensated", False))},
             "lighting": None if strategy.direct_lighting == "none" else {"method": strategy.direct_lighting,
                                                                           "selection": strategy.light_selection},

@@ -222,8 +222,12 @@ bool lower(const JVal& d, int32_t width, int32_t height, int32_t fif, pt_program
         if (const JVal* ms = rr->get("maxSurvival"); ms && ms->kind == JVal::NUM) pg.rr_max_survival = (float)ms->num;
     }
     if (const JVal* acc = e->get("accumulation")) {
-        if (!is_str(acc->get("type"), "average")) return L.unsupported("estimator.accumulation.type", quoted(acc->get("type")));
+        if (is_str(acc->get("type"), "average")) pg.accumulation = PT_ACC_AVERAGE;
+        else if (is_str(acc->get("type"), "oneshot")) pg.accumulation = PT_ACC_ONESHOT;
+        else return L.unsupported("estimator.accumulation.type", quoted(acc->get("type")));
         pg.variance = truthy(acc->get("variance")) ? 1 : 0;
+        if (pg.variance && pg.accumulation == PT_ACC_ONESHOT)
+            return L.unsupported("estimator.accumulation", "'oneshot' with variance (oneshot keeps no statistic)");
     }
     if (const JVal* es = e->get("envSampler")) {
         if (const JVal* ch = es->get("chart"); ch && ch->kind != JVal::NUL) {

@@ -1026,7 +1026,7 @@ __global__ void __launch_bounds__(SBLOCK) k_adj_atomic(const Paths P, int n_path
 template <bool SPEC>
 __global__ void __launch_bounds__(SBLOCK) k_accumulate(const DevScene S, const Paths P, float4* film, int npix, int frames,
                                                        uint32_t sample0, int wl_uniform, float4* var, float4* aov0,
-                                                       float4* aov1) {
+                                                       float4* aov1, int oneshot) {
     int pix = blockIdx.x * blockDim.x + threadIdx.x;
     if (pix >= npix) return;
     // the guide pictures: average_update of their samples, in sample order
@@ -1037,7 +1037,7 @@ __global__ void __launch_bounds__(SBLOCK) k_accumulate(const DevScene S, const P
         for (int f = 0; f < frames; f++) {
             const uint32_t n = sample0 + (uint32_t)f;
             const float4 x = P.aov[a][(size_t)f * npix + pix];
-            if (n == 0) g = make_float4(x.x, x.y, x.z, 1.0f);
+            if (n == 0 || oneshot) g = make_float4(x.x, x.y, x.z, 1.0f);
             else {
                 float w = 1.0f / ((float)n + 1.0f);
                 g = make_float4(g.x * (1.0f - w) + x.x * w, g.y * (1.0f - w) + x.y * w, g.z * (1.0f - w) + x.z * w, 1.0f);
@@ -1062,7 +1062,7 @@ __global__ void __launch_bounds__(SBLOCK) k_accumulate(const DevScene S, const P
             x = make_float4(px.x, px.y, px.z, 0.0f);
         }
         const float4 prev = acc;
-        if (n == 0) acc = make_float4(x.x, x.y, x.z, 1.0f);
+        if (n == 0 || oneshot) acc = make_float4(x.x, x.y, x.z, 1.0f);   // oneshot_update: the latest sample
         else {
             float w = 1.0f / ((float)n + 1.0f);
             acc = make_float4(acc.x * (1.0f - w) + x.x * w, acc.y * (1.0f - w) + x.y * w, acc.z * (1.0f - w) + x.z * w, 1.0f);

@@ -862,6 +862,9 @@ pt_status pt_renderer_load(pt_ctx* ctx, const pt_program* prog, pt_renderer** ou
     if (prog->aovs && prog->color != PT_COLOR_RGB)
         return fail(PT_E_UNSUPPORTED, "first-hit guides (aovs) are provided for RGB programs only");
     if (prog->env_chart < 0 || prog->env_chart > PT_CHART_OCTAHEDRAL + 1) return fail(PT_E_INVALID, "unknown env chart");
+    if (prog->accumulation != PT_ACC_AVERAGE && prog->accumulation != PT_ACC_ONESHOT) return fail(PT_E_INVALID, "unknown accumulation");
+    if (prog->accumulation == PT_ACC_ONESHOT && prog->variance)
+        return fail(PT_E_INVALID, "accumulation 'oneshot' does not support variance (it keeps no statistic)");
     if (ctx->has_scene)
         if (pt_status sc = check_program_scene(*prog, ctx->dev, ctx->host_lights); sc != PT_OK) return sc;
     HIP_TRY(hipSetDevice(ctx->device));
@@ -1236,6 +1239,8 @@ pt_status pt_batch_create(pt_ctx* ctx, int32_t n, const pt_scene_desc* scenes, c
             return fail(PT_E_INVALID, "pt_batch_create: ABI version mismatch");
     for (int s = 0; s < n; s++) {
         if (programs[s].aovs) return fail(PT_E_UNSUPPORTED, "pt_batch_create: first-hit guides (aovs) are not batched");
+        if (programs[s].accumulation != PT_ACC_AVERAGE)
+            return fail(PT_E_UNSUPPORTED, "pt_batch_create: batches average their samples (accumulation 'average')");
         if (programs[s].env_chart < 0 || programs[s].env_chart > PT_CHART_OCTAHEDRAL + 1)
             return fail(PT_E_INVALID, "pt_batch_create: unknown env chart");
         if (programs[s].env_chart && scenes[s].env.type == PT_ENV_SKY && programs[s].env_chart - 1 != scenes[s].env.chart)

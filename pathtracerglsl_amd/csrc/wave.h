@@ -84,5 +84,6 @@ static void launch_wave(pt_renderer* r, const Camera& cam, int frames, uint32_t 
             TIMED(ctx, "adj_reduce", n, hipLaunchKernelGGL(k_adj_atomic, dim3((unsigned)((n + SBLOCK - 1) / SBLOCK)), dim3(SBLOCK), 0, st, P, (int)n, (float*)r->b_grad.p));
         return;
     }
-    TIMED(ctx, "accumulate", npix, hipLaunchKernelGGL((k_accumulate<SPEC>), dim3((npix + SBLOCK - 1) / SBLOCK), dim3(SBLOCK), 0, st, S, P, (float4*)r->film.p, npix, frames, sample0, pg.wavelengths, (float4*)r->var_film.p, (float4*)r->aov_film[0].p, (float4*)r->aov_film[1].p));
+    TIMED(ctx, "accumulate", npix, hipLaunchKernelGGL((k_accumulate<SPEC>), dim3((npix + SBLOCK - 1) / SBLOCK), dim3(SBLOCK), 0, st, S, P, (float4*)r->film.p, npix, frames, sample0, pg.wavelengths, (float4*)r->var_film.p, (float4*)r->aov_film[0].p, (float4*)r->aov_film[1].p,
+                                                        pg.accumulation == PT_ACC_ONESHOT ? 1 : 0));
 }

@@ -31,7 +31,7 @@ def _expected_estimator(d):
 
 # The number of the 505 frozen link maps this backend lowers: a lowering that starts rejecting
 # a case (or accepting one it cannot render) changes it.
-EXPECTED_ACCEPTED = 342
+EXPECTED_ACCEPTED = 343
 
 
 def test_library_lowers_or_names_every_snapshot_case():
@@ -62,6 +62,8 @@ def test_library_lowers_or_names_every_snapshot_case():
         assert p.volume_sampling == (abi.VS_DELTA_TRACKING if e.get("volumeSampling") == "delta-tracking"
                                      else abi.VS_ANALYTIC), name
         assert p.variance == int(bool(e.get("accumulation", {}).get("variance"))), name
+        assert p.accumulation == (abi.ACC_ONESHOT if e.get("accumulation", {}).get("type") == "oneshot"
+                                  else abi.ACC_AVERAGE), name
         assert set(d["materials"]["models"]) <= SUPPORTED_MODELS, name
         chart = (e.get("envSampler") or {}).get("chart")
         assert p.env_chart == {None: 0, "equirect": abi.CHART_EQUIRECT + 1, "octahedral": abi.CHART_OCTAHEDRAL + 1}[chart], name
@@ -75,6 +77,7 @@ def test_library_lowers_or_names_every_snapshot_case():
     assert "two-light + uniform" in accepted                  # estimator.lighting.selection 'uniform'
     assert "spectral-rayleigh-furnace + visible" in accepted  # media.models ['rayleigh']
     assert {"spectral-gaussian-plane + spectral-mis", "spectral-gaussian-plane + spectral-none"} <= set(accepted)
+    assert "cornell-oneshot + oneshot" in accepted                 # estimator.accumulation.type 'oneshot'
 
 
 def test_chart_and_albedo_flag_lowering():
