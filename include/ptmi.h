@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define PT_ABI_VERSION 12u
+#define PT_ABI_VERSION 13u
 
 typedef enum pt_status {
     PT_OK = 0,
@@ -172,10 +172,22 @@ typedef struct pt_light {
 
 enum { PT_ENV_NONE = 0, PT_ENV_SKY = 1, PT_ENV_CONSTANT = 2 };
 
-/* Environment. PT_ENV_SKY is the procedural sky of the reference's proc-sky scenes
- * (tests/compiler/__snapshots__/generated-glsl.snapshot.test.ts.snap:63758), sampled
- * through an equirect importance table the library bakes itself
- * (src/compiler/EnvironmentBake.ts:29, src/components/env/importance.ts:16). */
+/* Environment. PT_ENV_SKY is a procedural sky (features/environment.ts:191-231): one of the
+ * formula bodies below evaluated at the direction turned into the table's azimuth, sampled
+ * through an importance table the library bakes itself at texel centres
+ * (src/compiler/EnvironmentBake.ts:29-80, src/components/env/importance.ts:16). */
+/* Procedural sky bodies (environment.glsl.source), the formulas of the reference's scenes:
+ *   PT_SKY_SUN      mix(vec3(0.06, 0.08, 0.12), vec3(0.35, 0.45, 0.65), 0.5 * (dir.y + 1.0))
+ *                   + vec3(18, 16, 13) * smoothstep(0.9994, 0.9999, dot(dir, normalize(vec3(0.4, 0.35, 0.2))))
+ *                   (proc-sky, generated-glsl.snapshot.test.ts.snap:63758)
+ *   PT_SKY_BIG_SUN  vec3(0.02) + vec3(40.0) * smoothstep(0.985, 0.99, dot(dir, normalize(vec3(0.4, 0.35, 0.2))))
+ *                   (proc-sky-rotated, tests/witnesses/scenes/estimatorAgreementWitness.ts:61-72)
+ *   PT_SKY_CAP      vec3(step(0.9, dir.y))           (sky-missed-cap, witnesses/scenes/spectralSkies.ts:39-41)
+ *   PT_SKY_WHITE    vec3(1.0)                        (spectral-comp-empty, spectralSkies.ts:48)
+ * A spectral program lifts the body's RGB per direction as an emission
+ * (color_lift_emission, components/color/spectral/lookup.glsl:50-60): an exact grey c is
+ * c * D65, any other colour goes through the coefficient atlas of pt_scene_desc. */
+enum { PT_SKY_SUN = 0, PT_SKY_BIG_SUN = 1, PT_SKY_CAP = 2, PT_SKY_WHITE = 3 };
 typedef struct pt_env {
     int32_t type;
     int32_t table_w, table_h;   /* 512 x 256 default */
@@ -189,6 +201,7 @@ typedef struct pt_env {
     float   color[4];
     pt_spectrum s_color;        /* spectral, role 'emission' (SETTING_ROLES.sky) */
     int32_t chart;              /* PT_ENV_SKY: the importance table's chart (estimator.envSampler.chart) */
+    int32_t body;               /* PT_ENV_SKY: PT_SKY_* */
 } pt_env;
 /* Environment charts (src/components/env/): equirect W x H with sin(theta) row weights, or the
  * equal-area octahedral map (Clarberg 2008; octahedral.glsl) on a table_h x table_h grid, every
@@ -226,6 +239,11 @@ typedef struct pt_scene_desc {
      * vertices ([n][9], object-local for prototypes); may be null when no object is smooth */
     const float* tri_normals;
     const float* proto_normals;
+    /* the Jakob-Hanika coefficient atlas (u_spectralCoefficients, lookup.glsl:9-12: [3 * 64^3][4]
+     * texels, channel-major, then z, y, x; w unused) and its 64 z nodes (u_spectralCurves row 1):
+     * needed only by a spectral program whose procedural sky body is not grey; may be null */
+    const float* spectral_atlas;
+    const float* spectral_z_nodes;
 } pt_scene_desc;
 
 /* ---- program (the Plan stage's decisions) ---- */

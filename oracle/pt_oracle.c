@@ -319,6 +319,8 @@ typedef struct {
     float spec_curves[95 * 4];/* CIE X, Y, Z, D65 at 360 + 5 i nm (spectral programs) */
     float xyz_rgb[9], cie_y;
     int has_spectral;
+    float* atlas;             /* the coefficient atlas [3 * 64^3][4] (null: none) and its z nodes */
+    float atlas_z[64];
 } or_scene;
 
 /* Möller–Trumbore (intersection/mesh/mesh.glsl:41-59). */
@@ -1190,14 +1192,73 @@ static float surface_pdf(const or_scene* S, const pt_material* m, v3 wi, v3 wo, 
 }
 static inline int has_nondelta(const pt_material* m) { return m->model != PT_MAT_DIELECTRIC && m->model != PT_MAT_MIRROR; }
 
-/* ---------------- environment (components/env, proc-sky snapshot) ---------------- */
-static v3 sky_radiance_raw(v3 dir) {
+/* ---------------- environment (components/env, features/environment.ts:191-231) ---------------- */
+/* The procedural sky bodies (environment.glsl.source) of the reference's scenes, at a unit
+ * direction in the table's azimuth frame:
+ *   PT_SKY_SUN      proc-sky (generated-glsl.snapshot.test.ts.snap:63758)
+ *   PT_SKY_BIG_SUN  proc-sky-rotated (tests/witnesses/scenes/estimatorAgreementWitness.ts:61-72)
+ *   PT_SKY_CAP      sky-missed-cap, vec3(step(0.9, dir.y)) (witnesses/scenes/spectralSkies.ts:39-41)
+ *   PT_SKY_WHITE    spectral-comp-empty, vec3(1.0) (spectralSkies.ts:48) */
+static v3 sky_radiance_raw(int body, v3 dir) {
     v3 sun = normalize(V(0.4f, 0.35f, 0.2f));
-    float s = 0.5f * (dir.y + 1.0f);
-    v3 lo = V(0.06f, 0.08f, 0.12f), hi = V(0.35f, 0.45f, 0.65f);
-    v3 base = add(muls(lo, 1.0f - s), muls(hi, s));
-    float k = ptm_smoothstep(0.9994f, 0.9999f, dot(dir, sun));
-    return add(base, muls(V(18.0f, 16.0f, 13.0f), k));
+    if (body == PT_SKY_SUN) {
+        float s = 0.5f * (dir.y + 1.0f);
+        v3 lo = V(0.06f, 0.08f, 0.12f), hi = V(0.35f, 0.45f, 0.65f);
+        v3 base = add(muls(lo, 1.0f - s), muls(hi, s));
+        float k = ptm_smoothstep(0.9994f, 0.9999f, dot(dir, sun));
+        return add(base, muls(V(18.0f, 16.0f, 13.0f), k));
+    }
+    if (body == PT_SKY_BIG_SUN) {
+        float k = ptm_smoothstep(0.985f, 0.99f, dot(dir, sun));
+        return add(V(0.02f, 0.02f, 0.02f), muls(V(40.0f, 40.0f, 40.0f), k));
+    }
+    if (body == PT_SKY_CAP) {   /* GLSL step(edge, x) = x < edge ? 0 : 1 */
+        float v = PLANT(9) ? (0.9f < dir.y ? 0.0f : 1.0f) : (dir.y < 0.9f ? 0.0f : 1.0f);
+        return V(v, v, v);
+    }
+    if (body == PT_SKY_WHITE) return V(1.0f, 1.0f, 1.0f);
+    return V(0.0f, 0.0f, 0.0f);
+}
+/* spectral_coefficients (lookup.glsl:16-38): trilinear lookup of the coefficient atlas for a
+ * bounded rgb, channel = the largest (first on ties), x and y = the other two (channel + 1,
+ * channel + 2) over it, the z cell by six halvings over the 64 z nodes. */
+static void atlas_coefficients(const or_scene* S, v3 rgb, float out[3]) {
+    float v[3] = {rgb.x, rgb.y, rgb.z};
+    int ch = rgb.x >= rgb.y && rgb.x >= rgb.z ? 0 : (rgb.y >= rgb.z ? 1 : 2);
+    float z = v[ch];
+    int ax = PLANT(8) ? (ch + 2) % 3 : (ch + 1) % 3, ay = PLANT(8) ? (ch + 1) % 3 : (ch + 2) % 3;
+    float x = (v[ax] / z) * 63.0f, y = (v[ay] / z) * 63.0f;
+    int ix = (int)x < 62 ? (int)x : 62, iy = (int)y < 62 ? (int)y : 62;
+    float fx = x - (float)ix, fy = y - (float)iy;
+    int lo = 0, hi = 63;
+    for (int k = 0; k < 6; k++) {
+        int mid = (lo + hi) / 2;
+        if (S->atlas_z[mid] <= z) lo = mid; else hi = mid;
+    }
+    float fz = (z - S->atlas_z[lo]) / (S->atlas_z[lo + 1] - S->atlas_z[lo]);
+    for (int c = 0; c < 3; c++) {
+        float p[2];
+        for (int dz = 0; dz < 2; dz++) {
+#define NODE(xx, yy) S->atlas[4 * ((((size_t)ch * 64 + (size_t)(lo + dz)) * 64 + (size_t)(yy)) * 64 + (size_t)(xx)) + c]
+            p[dz] = glsl_mix(glsl_mix(NODE(ix, iy), NODE(ix + 1, iy), fx), glsl_mix(NODE(ix, iy + 1), NODE(ix + 1, iy + 1), fx), fy);
+#undef NODE
+        }
+        out[c] = glsl_mix(p[0], p[1], fz);
+    }
+}
+/* color_lift_emission of an RGB computed at run time (lookup.glsl:50-60) */
+static sp lift_emission(const or_scene* S, v3 rgb, const wl_t* wl) {
+    pt_spectrum r;
+    memset(&r, 0, sizeof r);
+    r.scaled = 1; r.d65 = 1;
+    r.scale = 2.0f * fmaxf_(rgb.x, fmaxf_(rgb.y, rgb.z));
+    if (r.scale == 0.0f) r.kind = PT_SPEC_ZERO;
+    else {
+        v3 b = V(rgb.x / r.scale, rgb.y / r.scale, rgb.z / r.scale);
+        if (b.x == b.y && b.y == b.z) { r.kind = PT_SPEC_GREY; r.value = b.x; }
+        else { r.kind = PT_SPEC_SIGMOID; atlas_coefficients(S, b, r.c); }
+    }
+    return spectrum_eval(S, &r, wl);
 }
 static sp env_radiance(const or_scene* S, v3 dir, const wl_t* wl) {
     if (S->env.type == PT_ENV_CONSTANT)   /* features/environment.ts:119-133: color * u_envIntensity */
@@ -1206,14 +1267,16 @@ static sp env_radiance(const or_scene* S, v3 dir, const wl_t* wl) {
     v3 d = normalize(dir);
     float cx = S->env_rot_cs[0], sy = S->env_rot_cs[1];
     d = V(cx * d.x - sy * d.z, d.y, sy * d.x + cx * d.z);
-    return srgbv(muls(sky_radiance_raw(d), S->env.intensity));
+    v3 c = sky_radiance_raw(S->env.body, d);
+    if (!wl->spec) return srgbv(muls(c, S->env.intensity));
+    return smuls(lift_emission(S, c, wl), S->env.intensity);   /* skyRadiance: lift, then * u_envIntensity */
 }
 /* equirect chart (env/equirect/equirect_map.glsl, equirect.glsl:19-48) */
 static void equirect_uv(const or_scene* S, v3 dir, float* u, float* v) {
     v3 n = normalize(dir);
     float phi = (n.x == 0.0f && n.z == 0.0f ? 0.0f : ATAN2(n.z, n.x)) + S->env.rotation;
     float theta = ATAN2(sqrtf(n.x * n.x + n.z * n.z), n.y);
-    *u = ptm_fract(phi * (1.0f / PTM_TWO_PI) + 0.5f);
+    *u = PLANT(10) ? clampf(phi * (1.0f / PTM_TWO_PI) + 0.5f, 0.0f, 1.0f) : ptm_fract(phi * (1.0f / PTM_TWO_PI) + 0.5f);
     *v = theta * (1.0f / PTM_PI);
 }
 static v3 env_chart_dir(float rot, float u, float v) {
@@ -1352,7 +1415,7 @@ static void env_build(or_scene* S) {
             v3 d;
             if (oct) ptm_oct_dir(u, v, 1.0f, 0.0f, &d.x, &d.y, &d.z);   /* octahedral.glsl:25-42, rotation 0 */
             else d = env_chart_dir(0.0f, u, v);
-            v3 c = sky_radiance_raw(d);
+            v3 c = sky_radiance_raw(S->env.body, d);
             Y[(size_t)j * W + i] = (float)(0.2126 * (double)c.x + 0.7152 * (double)c.y + 0.0722 * (double)c.z);
         }
     env_chain(S, Y, W, H, oct, S->env.compensated);
@@ -2033,6 +2096,11 @@ or_scene* or_scene_create(const pt_scene_desc* d) {
         S->cie_y = d->cie_y_integral;
         S->has_spectral = 1;
     }
+    if (d->spectral_atlas && d->spectral_z_nodes) {
+        S->atlas = (float*)malloc(sizeof(float) * 4 * 3 * 64 * 64 * 64);
+        memcpy(S->atlas, d->spectral_atlas, sizeof(float) * 4 * 3 * 64 * 64 * 64);
+        memcpy(S->atlas_z, d->spectral_z_nodes, sizeof S->atlas_z);
+    }
     for (int m = 0; m < S->n_materials; m++) {
         if (S->materials[m].has_medium) S->has_media = 1;
         if (S->materials[m].model == PT_MAT_NULL) S->has_null = 1;
@@ -2138,7 +2206,7 @@ void or_scene_destroy(or_scene* S) {
     free(S->blas); bvh_free(&S->tlas); free(S->tlas_obj);
     free(S->obj_first); bvh_free(&S->itlas); free(S->inst); free(S->inst_batch); free(S->inst_base);
     free(S->tri); free(S->nrm); free(S->tri_object); free(S->objects); free(S->materials); free(S->lights);
-    free(S->env_cond); free(S->env_marg);
+    free(S->env_cond); free(S->env_marg); free(S->atlas);
     free(S);
 }
 void or_env_table(const or_scene* S, float* cond, float* marg) {
@@ -2469,6 +2537,18 @@ void or_spectrum_eval(const or_scene* S, const pt_spectrum* r, const float* lam,
     }
 }
 
+/* The environment's radiance at directions: RGB (lam null; out [n][3]) or the spectral lift at
+ * one wavelength per direction (out [n]). */
+void or_env_radiance(const or_scene* S, const float* dirs, const float* lam, int n, float* out) {
+    for (int i = 0; i < n; i++) {
+        wl_t wl; memset(&wl, 0, sizeof wl);
+        wl.spec = lam != NULL;
+        for (int k = 0; k < 4; k++) wl.lam[k] = lam ? lam[i] : 0.0f;
+        sp v = env_radiance(S, V(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]), &wl);
+        if (lam) out[i] = v.x;
+        else { out[3 * i] = v.x; out[3 * i + 1] = v.y; out[3 * i + 2] = v.z; }
+    }
+}
 /* Environment probes: pdf of directions, and samples with their pdfs. */
 void or_env_pdf(const or_scene* S, const float* dirs, int n, float* out) {
     for (int i = 0; i < n; i++) out[i] = environment_pdf(S, V(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]));

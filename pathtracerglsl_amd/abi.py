@@ -7,7 +7,7 @@ import ctypes as C
 
 import numpy as np
 
-ABI_VERSION = 12
+ABI_VERSION = 13
 
 OBJ_MESH, OBJ_QUAD, OBJ_SPHERE, OBJ_INSTANCES = 0, 1, 2, 3
 MAT_LAMBERT, MAT_DIELECTRIC, MAT_GGX, MAT_NULL, MAT_MIRROR, MAT_ROUGH_DIELECTRIC = 0, 1, 2, 3, 4, 5
@@ -16,6 +16,7 @@ VS_ANALYTIC, VS_DELTA_TRACKING = 0, 1
 IOR_CONSTANT, IOR_CAUCHY, IOR_SELLMEIER = 0, 1, 2
 LIGHT_QUAD, LIGHT_POINT = 0, 1
 ENV_NONE, ENV_SKY, ENV_CONSTANT = 0, 1, 2
+SKY_SUN, SKY_BIG_SUN, SKY_CAP, SKY_WHITE = 0, 1, 2, 3
 AOV_ALBEDO, AOV_NORMAL = 1, 2
 CHART_EQUIRECT, CHART_OCTAHEDRAL = 0, 1
 SEL_POWER, SEL_UNIFORM = 0, 1
@@ -77,7 +78,7 @@ class PtEnv(C.Structure):
     _fields_ = [("type", C.c_int32), ("table_w", C.c_int32), ("table_h", C.c_int32),
                 ("compensated", C.c_int32), ("intensity", C.c_float), ("rotation", C.c_float),
                 ("select_prob", C.c_float), ("samplable", C.c_int32), ("color", C.c_float * 4),
-                ("s_color", PtSpectrum), ("chart", C.c_int32)]
+                ("s_color", PtSpectrum), ("chart", C.c_int32), ("body", C.c_int32)]
 
 
 class PtSceneDesc(C.Structure):
@@ -91,7 +92,8 @@ class PtSceneDesc(C.Structure):
                 ("cie_y_integral", C.c_float), ("pad1", C.c_int32 * 2),
                 ("n_proto_tris", C.c_int32), ("n_instances", C.c_int32),
                 ("proto_vertices", C.c_void_p), ("instances", C.c_void_p),
-                ("tri_normals", C.c_void_p), ("proto_normals", C.c_void_p)]
+                ("tri_normals", C.c_void_p), ("proto_normals", C.c_void_p),
+                ("spectral_atlas", C.c_void_p), ("spectral_z_nodes", C.c_void_p)]
 
 
 class PtInstTable(C.Structure):
@@ -170,7 +172,7 @@ class SceneArrays:
 
     def __init__(self, tri_vertices, tri_object, objects, materials, lights, env=None, ambient_material=-1,
                  spectral_curves=None, xyz_to_rgb=None, cie_y_integral=0.0, proto_vertices=None, instances=None,
-                 tri_normals=None, proto_normals=None):
+                 tri_normals=None, proto_normals=None, spectral_atlas=None, spectral_z_nodes=None):
         self.tri_vertices = np.ascontiguousarray(tri_vertices, dtype=np.float32).reshape(-1, 9)
         self.tri_object = np.ascontiguousarray(tri_object, dtype=np.int32).reshape(-1)
         self.objects = np.ascontiguousarray(objects, dtype=OBJECT_DTYPE)
@@ -188,6 +190,10 @@ class SceneArrays:
                           else np.ascontiguousarray(instances, dtype=INSTANCE_DTYPE))
         self.tri_normals = None if tri_normals is None else np.ascontiguousarray(tri_normals, np.float32).reshape(-1, 9)
         self.proto_normals = None if proto_normals is None else np.ascontiguousarray(proto_normals, np.float32).reshape(-1, 9)
+        self.spectral_atlas = (None if spectral_atlas is None
+                               else np.ascontiguousarray(spectral_atlas, np.float32).reshape(3 * 64 ** 3, 4))
+        self.spectral_z_nodes = (None if spectral_z_nodes is None
+                                 else np.ascontiguousarray(spectral_z_nodes, np.float32).reshape(64))
         assert self.tri_object.shape[0] == self.tri_vertices.shape[0]
         assert self.tri_normals is None or self.tri_normals.shape == self.tri_vertices.shape
         assert self.proto_normals is None or self.proto_normals.shape == self.proto_vertices.shape
@@ -220,6 +226,8 @@ class SceneArrays:
         d.instances = _ptr(self.instances)
         d.tri_normals = _ptr(self.tri_normals)
         d.proto_normals = _ptr(self.proto_normals)
+        d.spectral_atlas = _ptr(self.spectral_atlas)
+        d.spectral_z_nodes = _ptr(self.spectral_z_nodes)
         return d
 
 

@@ -289,8 +289,6 @@ def _analyze_validate(scene, strategy):
                            f"{', '.join(tracked)}) but volumeSampling is 'analytic' — set volumeSampling 'delta-tracking'")
     if tracked and strategy.medium_light_sampling == "equiangular":
         raise CompileError("mediumLightSampling 'equiangular' does not support media on the null-collision arms; use 'vertex'")
-    if strategy.color == "spectral" and scene.get("environment", {"type": "none"}).get("type", "none") == "procedural":
-        raise CompileError("spectral programs with a procedural environment are not supported by this backend")
     cam = scene.get("camera", {})
     for a in strategy.aovs:
         if a not in ("albedo", "normal"):
@@ -343,6 +341,8 @@ def _analyze_validate(scene, strategy):
     env = scene.get("environment", {"type": "none"})
     if env.get("type") not in ("none", "procedural", "constant"):
         raise CompileError(f"environment '{env.get('type')}' is not supported by this backend")
+    if env.get("type") == "procedural":
+        sky_body(env)
     if spectral.is_gaussian(env.get("color")):
         raise CompileError("environment.color: gaussian values are reflectances only")
     for l in scene.get("lights", []):
@@ -629,6 +629,7 @@ def compile_scene(scene, strategy=None, width=256, height=256, reset_salt=1234):
         env = abi.PtEnv(abi.ENV_SKY, int(tw), int(th), int(e.get("compensated", False)),
                         float(e.get("intensity", 1.0)), float(e.get("rotation", 0.0)),
                         float(e.get("select_prob", 0.5)), 1)
+        env.body = sky_body(e)
         # estimator.envSampler.chart (components/env/index.ts ENV_CHARTS): the table's chart
         chart = e.get("chart", "equirect")
         if chart not in ENV_CHARTS:
@@ -652,6 +653,8 @@ def compile_scene(scene, strategy=None, width=256, height=256, reset_salt=1234):
     amb = scene.get("ambient_medium")
     if amb is not None and amb not in mat_index:
         raise CompileError(f"ambientMedium references unknown material '{amb}'")
+    # a spectral program lifts a coloured sky body per direction through the coefficient atlas
+    atlas = spec and env.type == abi.ENV_SKY and env.body not in GREY_SKY_BODIES
     arrays = abi.SceneArrays(tv, to, np.array(objects, abi.OBJECT_DTYPE),
                              np.array(materials, abi.MATERIAL_DTYPE),
                              np.array(light_recs, abi.LIGHT_DTYPE), env,
@@ -661,7 +664,8 @@ def compile_scene(scene, strategy=None, width=256, height=256, reset_salt=1234):
                              np.array(inst_recs, abi.INSTANCE_DTYPE) if inst_recs else None,
                              np.concatenate(nrm_chunks) if any(r["smooth"] and r["kind"] == abi.OBJ_MESH for r in objects) else None,
                              np.concatenate(proto_nrm_chunks) if any(r["smooth"] and r["kind"] == abi.OBJ_INSTANCES
-                                                                     for r in objects) else None)
+                                                                     for r in objects) else None,
+                             spectral.atlas() if atlas else None, spectral.z_nodes() if atlas else None)
 
     prog = abi.PtProgram()
     prog.abi_version = abi.ABI_VERSION
@@ -693,10 +697,39 @@ def compile_scene(scene, strategy=None, width=256, height=256, reset_salt=1234):
                             source=scene, strategy=strategy)
 
 
-# The proc-sky environment body of the reference's procedural-sky scenes, the one PT_ENV_SKY bakes
-# (tests/compiler/__snapshots__/generated-glsl.snapshot.test.ts.snap:63758).
+# The procedural sky bodies (environment.glsl.source) the library evaluates: the reference's
+# proc-sky (tests/compiler/__snapshots__/generated-glsl.snapshot.test.ts.snap:63758),
+# proc-sky-rotated (tests/witnesses/scenes/estimatorAgreementWitness.ts:61-72), sky-missed-cap and
+# spectral-comp-empty (tests/witnesses/scenes/spectralSkies.ts:39-48).
 PROC_SKY_GLSL = ("mix(vec3(0.06, 0.08, 0.12), vec3(0.35, 0.45, 0.65), 0.5 * (dir.y + 1.0)) + vec3(18.0, 16.0, 13.0) * "
                  "smoothstep(0.9994, 0.9999, dot(dir, normalize(vec3(0.4, 0.35, 0.2))))")
+BIG_SUN_GLSL = "vec3(0.02) + vec3(40.0) * smoothstep(0.985, 0.99, dot(dir, normalize(vec3(0.4, 0.35, 0.2))))"
+CAP_GLSL = "vec3(step(0.9, dir.y))"
+WHITE_GLSL = "vec3(1.0)"
+SKY_BODY_GLSL = {abi.SKY_SUN: PROC_SKY_GLSL, abi.SKY_BIG_SUN: BIG_SUN_GLSL, abi.SKY_CAP: CAP_GLSL,
+                 abi.SKY_WHITE: WHITE_GLSL}
+GREY_SKY_BODIES = (abi.SKY_BIG_SUN, abi.SKY_CAP, abi.SKY_WHITE)
+
+
+def _squash(src):
+    return "".join(src.split())
+
+
+def sky_body(env):
+    """The PT_SKY_* body of a procedural environment: its 'glsl' source (a string or the
+    reference's {'kind': 'glsl', 'source': ...}), whitespace aside, must be one of the bodies
+    above; no 'glsl' is the proc-sky body."""
+    g = env.get("glsl")
+    if g is None:
+        return abi.SKY_SUN
+    src = g.get("source") if isinstance(g, dict) else g
+    if not isinstance(src, str):
+        raise CompileError("environment.glsl: expected a GLSL source string")
+    for body, known in SKY_BODY_GLSL.items():
+        if _squash(src) == _squash(known):
+            return body
+    raise CompileError(f"environment 'procedural' with the GLSL body {src!r} is not supported by this backend "
+                       f"(known bodies: {', '.join(repr(v) for v in SKY_BODY_GLSL.values())})")
 
 
 def program_from_description(description, width, height, frames_in_flight=1):
@@ -742,8 +775,8 @@ def describe(scene, strategy):
                     kinds.append("gaussian")
     d = {
         "dispersion": None,
-        "environment": ({"type": "none"} if env.get("type", "none") == "none" else
-                        {"type": "procedural", "glsl": {"kind": "glsl", "source": PROC_SKY_GLSL}}),
+        "environment": ({"type": "procedural", "glsl": {"kind": "glsl", "source": SKY_BODY_GLSL[sky_body(env)]}}
+                        if env.get("type") == "procedural" else {"type": env.get("type", "none")}),
         "estimator": {
             "accumulation": {"type": strategy.accumulation, **({"variance": True} if strategy.variance else {})},
             "envSampler": {"chart": env.get("chart", "equirect"), "compensation": bool(env.get("compensated", False))},

@@ -8,6 +8,7 @@
 #include <stdexcept>
 
 #include "../../include/ptmi_math.h"
+#include "sky.h"
 
 // Collapse of the binary SAH tree into 8-wide nodes: 2 = the SAH-optimal dynamic programme
 // (Ylitie, Karras, Laine 2017, section 3.2; the reference's accel/cwbvh/cwbvh.ts builds the same
@@ -395,18 +396,12 @@ void build_binary_bvh(const double* boxes, int n, int leaf_size, BinaryBVH& out)
 // ---------------------------------------------------------------- env importance table
 namespace {
 struct F3 { float x, y, z; };
-inline F3 f3n(F3 a) {
-    float d = a.x * a.x + a.y * a.y + a.z * a.z;
-    float s = 1.0f / sqrtf(d);
-    return {a.x * s, a.y * s, a.z * s};
-}
 }  // namespace
 
 void build_env_table(const pt_env& env, std::vector<float>& cond, std::vector<float>& marg, double& total) {
     const bool oct = env.chart == PT_CHART_OCTAHEDRAL;          // N x N, N = table_h (EnvironmentBake.ts:20-27)
     const int H = env.table_h > 0 ? env.table_h : 256, W = oct ? H : (env.table_w > 0 ? env.table_w : 512);
     std::vector<float> Y((size_t)W * H), Bl((size_t)W * H);
-    const F3 sun = f3n({0.4f, 0.35f, 0.2f});
     for (int j = 0; j < H; j++) {
         for (int i = 0; i < W; i++) {
             float u = ((float)i + 0.5f) / (float)W, v = ((float)j + 0.5f) / (float)H;
@@ -419,12 +414,9 @@ void build_env_table(const pt_env& env, std::vector<float>& cond, std::vector<fl
                 float st = ptm_sin(theta);
                 d = {ptm_cos(phi) * st, ptm_cos(theta), ptm_sin(phi) * st};
             }
-            float s = 0.5f * (d.y + 1.0f);
-            float k = ptm_smoothstep(0.9994f, 0.9999f, d.x * sun.x + d.y * sun.y + d.z * sun.z);
-            F3 c = {0.06f * (1.0f - s) + 0.35f * s + 18.0f * k,
-                    0.08f * (1.0f - s) + 0.45f * s + 16.0f * k,
-                    0.12f * (1.0f - s) + 0.65f * s + 13.0f * k};
-            Y[(size_t)j * W + i] = (float)(0.2126 * (double)c.x + 0.7152 * (double)c.y + 0.0722 * (double)c.z);
+            float c[3];
+            pt_sky_body(env.body, d.x, d.y, d.z, c);
+            Y[(size_t)j * W + i] = (float)(0.2126 * (double)c[0] + 0.7152 * (double)c[1] + 0.0722 * (double)c[2]);
         }
     }
     for (int j = 0; j < H; j++)

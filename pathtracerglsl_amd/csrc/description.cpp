@@ -8,6 +8,7 @@
 // option (src/compiler/analyze/Validator.ts). Fields that only choose between equivalent
 // implementations (objectDispatch, meshTraversal, instanceAccel, regionLookup) or that belong
 // to the display (view) are accepted as they are.
+#include <cctype>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -149,10 +150,27 @@ std::string quoted(const JVal* v) {
 bool is_str(const JVal* v, const char* s) { return v && v->kind == JVal::STR && v->str == s; }
 bool truthy(const JVal* v) { return v && v->kind == JVal::BOOL && v->b; }
 
-// The procedural sky the library bakes for PT_ENV_SKY (the reference's proc-sky scenes).
-const char* const kProcSky =
+// The procedural sky bodies the library evaluates for PT_ENV_SKY (PT_SKY_* in ptmi.h): proc-sky,
+// proc-sky-rotated, sky-missed-cap and spectral-comp-empty.
+const char* const kSkyBodies[] = {
     "mix(vec3(0.06, 0.08, 0.12), vec3(0.35, 0.45, 0.65), 0.5 * (dir.y + 1.0)) + vec3(18.0, 16.0, 13.0) * "
-    "smoothstep(0.9994, 0.9999, dot(dir, normalize(vec3(0.4, 0.35, 0.2))))";
+    "smoothstep(0.9994, 0.9999, dot(dir, normalize(vec3(0.4, 0.35, 0.2))))",
+    "vec3(0.02) + vec3(40.0) * smoothstep(0.985, 0.99, dot(dir, normalize(vec3(0.4, 0.35, 0.2))))",
+    "vec3(step(0.9, dir.y))",
+    "vec3(1.0)",
+};
+// a GLSL source without its whitespace
+std::string squash(const std::string& src) {
+    std::string out;
+    for (char c : src)
+        if (!std::isspace((unsigned char)c)) out += c;
+    return out;
+}
+bool known_sky_body(const std::string& src) {
+    for (const char* b : kSkyBodies)
+        if (squash(src) == squash(b)) return true;
+    return false;
+}
 
 bool lower(const JVal& d, int32_t width, int32_t height, int32_t fif, pt_program& pg, Lower& L) {
     if (d.kind != JVal::OBJ) return L.invalid("not a JSON object");
@@ -286,8 +304,7 @@ bool lower(const JVal& d, int32_t width, int32_t height, int32_t fif, pt_program
     if (!et || is_str(et, "none")) {
     } else if (is_str(et, "procedural")) {
         const JVal* src = path(*env, {"glsl", "source"});
-        if (!src || src->kind != JVal::STR || src->str != kProcSky) return L.unsupported("environment", "'procedural' with this GLSL body");
-        if (pg.color == PT_COLOR_SPECTRAL) return L.unsupported("environment", "'procedural' under the spectral colour model");
+        if (!src || src->kind != JVal::STR || !known_sky_body(src->str)) return L.unsupported("environment", "'procedural' with this GLSL body");
     } else if (is_str(et, "constant")) {
         // color * intensity in every direction; the colour itself travels with the scene (pt_env)
     } else {

@@ -111,6 +111,7 @@ struct DevScene {
     const pt_light* lights;
     int n_lights;
     int env_type, env_w, env_h, env_compensated, env_chart;
+    int env_body;               // PT_ENV_SKY: PT_SKY_* formula body
     int env_samplable;          // PT_ENV_CONSTANT sampled as a light
     float4 env_color;           // PT_ENV_CONSTANT radiance (RGB)
     pt_spectrum env_s_color;    // PT_ENV_CONSTANT radiance (spectral)
@@ -169,6 +170,10 @@ struct DevScene {
     const float4* spec_curves;
     float xyz_rgb[9];
     float cie_y;
+    // the Jakob-Hanika coefficient atlas ([3 * 64^3] texels) and its 64 z nodes: resident only for
+    // a spectral program whose procedural sky body has colour (null otherwise)
+    const float4* spec_atlas;
+    const float* spec_znodes;
 };
 
 // Wave-aggregated queue append: every active lane calls it (convergent), lanes with

@@ -47,7 +47,7 @@ namespace {
 // accumulates sequentially in f64 stay sequential (one thread per row for the conditional CDFs,
 // one thread for the marginal CDF and for the compensation mean).
 // Per-texel kernels are grid-stride (the launch grid is capped at MAX_GRID blocks).
-__global__ void __launch_bounds__(256) k_env_luma(float* Y, int W, int H, float sx, float sy, float sz, int oct) {
+__global__ void __launch_bounds__(256) k_env_luma(float* Y, int W, int H, int body, int oct) {
     for (int k = blockIdx.x * 256 + threadIdx.x; k < W * H; k += gridDim.x * 256) {
         int i = k % W, j = k / W;
         float u = ((float)i + 0.5f) / (float)W, v = ((float)j + 0.5f) / (float)H;
@@ -59,12 +59,9 @@ __global__ void __launch_bounds__(256) k_env_luma(float* Y, int W, int H, float 
             float st = ptm_sin(theta);
             dx = ptm_cos(phi) * st; dy = ptm_cos(theta); dz = ptm_sin(phi) * st;
         }
-        float s = 0.5f * (dy + 1.0f);
-        float kk = ptm_smoothstep(0.9994f, 0.9999f, dx * sx + dy * sy + dz * sz);
-        float cx = 0.06f * (1.0f - s) + 0.35f * s + 18.0f * kk;
-        float cy = 0.08f * (1.0f - s) + 0.45f * s + 16.0f * kk;
-        float cz = 0.12f * (1.0f - s) + 0.65f * s + 13.0f * kk;
-        Y[k] = (float)(0.2126 * (double)cx + 0.7152 * (double)cy + 0.0722 * (double)cz);
+        float c[3];
+        pt_sky_body(body, dx, dy, dz, c);
+        Y[k] = (float)(0.2126 * (double)c[0] + 0.7152 * (double)c[1] + 0.0722 * (double)c[2]);
     }
 }
 
@@ -385,6 +382,10 @@ static pt_status validate_scene(const pt_scene_desc* sd) {
     }
     if (sd->env.type == PT_ENV_SKY && sd->env.chart != PT_CHART_EQUIRECT && sd->env.chart != PT_CHART_OCTAHEDRAL)
         return fail(PT_E_INVALID, "pt_env.chart must be PT_CHART_EQUIRECT or PT_CHART_OCTAHEDRAL");
+    if (sd->env.type == PT_ENV_SKY && (sd->env.body < PT_SKY_SUN || sd->env.body > PT_SKY_WHITE))
+        return fail(PT_E_INVALID, "pt_env.body must be one of PT_SKY_*");
+    if ((sd->spectral_atlas == nullptr) != (sd->spectral_z_nodes == nullptr))
+        return fail(PT_E_INVALID, "pt_scene_desc: spectral_atlas and spectral_z_nodes come together");
     return PT_OK;
 }
 
@@ -461,17 +462,13 @@ static pt_status bake_env_table(const pt_env& env, SceneStore& SS, DevScene& D, 
     const int H = env.table_h > 0 ? env.table_h : 256, W = oct ? H : (env.table_w > 0 ? env.table_w : 512);
     if (W > PT_ENV_MAX_W) return fail(PT_E_UNSUPPORTED, "environment table wider than 4096 texels");
     const long long n = (long long)W * H;
-    float sx = 0.4f, sy = 0.35f, sz = 0.2f;               // the sun direction, normalised in f32
-    float d = sx * sx + sy * sy + sz * sz;
-    float inv = 1.0f / sqrtf(d);
-    sx *= inv; sy *= inv; sz *= inv;
     SceneStore tmp;
     tmp.stream = SS.stream;
     pt_status st;
     float* Y;
     if ((st = tmp.alloc((size_t)n, &Y)) != PT_OK) return st;
     if (ev0) HIP_TRY(hipEventRecord(ev0, SS.stream));
-    hipLaunchKernelGGL(k_env_luma, dim3(grid_for(n, 256)), dim3(256), 0, SS.stream, Y, W, H, sx, sy, sz, oct ? 1 : 0);
+    hipLaunchKernelGGL(k_env_luma, dim3(grid_for(n, 256)), dim3(256), 0, SS.stream, Y, W, H, env.body, oct ? 1 : 0);
     if ((st = env_table_chain(Y, W, H, oct, env.compensated != 0, SS, tmp, D)) != PT_OK) return st;
     if (ev1) HIP_TRY(hipEventRecord(ev1, SS.stream));
     HIP_TRY(hipStreamSynchronize(SS.stream));
@@ -564,6 +561,10 @@ static pt_status build_scene(const pt_scene_desc* sd, hipStream_t stream, SceneS
         if ((st = SS.upload((const float4*)sd->spectral_curves, 95, &D.spec_curves)) != PT_OK) return st;
         for (int k = 0; k < 9; k++) D.xyz_rgb[k] = sd->xyz_to_rgb[k];
         D.cie_y = sd->cie_y_integral;
+    }
+    if (sd->spectral_atlas) {
+        if ((st = SS.upload((const float4*)sd->spectral_atlas, (size_t)3 * 64 * 64 * 64, &D.spec_atlas)) != PT_OK) return st;
+        if ((st = SS.upload(sd->spectral_z_nodes, 64, &D.spec_znodes)) != PT_OK) return st;
     }
     for (int o = 0; o < sd->n_objects; o++) {
         const pt_object& ob = sd->objects[o];
@@ -766,6 +767,7 @@ static pt_status build_scene(const pt_scene_desc* sd, hipStream_t stream, SceneS
         D.env_h = sd->env.table_h > 0 ? sd->env.table_h : 256;
         D.env_w = D.env_chart == PT_CHART_OCTAHEDRAL ? D.env_h : (sd->env.table_w > 0 ? sd->env.table_w : 512);
         D.env_compensated = sd->env.compensated;
+        D.env_body = sd->env.body;
         set_env_values(D, sd->env);
     } else if (sd->env.type == PT_ENV_CONSTANT) {
         D.env_samplable = sd->env.samplable ? 1 : 0;
@@ -825,7 +827,8 @@ static bool holds_uniform_selection(const DevScene& dev, const std::vector<pt_li
 
 // The program-vs-scene rules, checked at renderer load when a scene is resident and again by
 // every render, forward and backward call (a later pt_scene_upload may have replaced the
-// scene): the spectral path has no procedural-sky radiance, a program that names its
+// scene): a spectral program lifts a coloured sky body through the coefficient atlas, which the
+// scene must then carry, a program that names its
 // envSampler.chart must sample the chart the sky table was built on, and a program with
 // uniform light selection needs the scene's selection table to be the uniform one.
 static pt_status check_program_scene(const pt_program& prog, const DevScene& dev, const std::vector<pt_light>& lights) {
@@ -834,8 +837,9 @@ static pt_status check_program_scene(const pt_program& prog, const DevScene& dev
     if (prog.light_selection == PT_SEL_UNIFORM && !holds_uniform_selection(dev, lights))
         return fail(PT_E_INVALID, "the program selects lights uniformly but the scene's selection table (pt_light.select_pdf / "
                                   "cdf, pt_env.select_prob) is not the uniform one");
-    if (prog.color == PT_COLOR_SPECTRAL && dev.env_type == PT_ENV_SKY)
-        return fail(PT_E_UNSUPPORTED, "spectral programs with the procedural sky are not provided by this build");
+    if (prog.color == PT_COLOR_SPECTRAL && dev.env_type == PT_ENV_SKY && !pt_sky_body_grey(dev.env_body) && !dev.spec_atlas)
+        return fail(PT_E_MISSING_INPUT, "missing input: spectral_atlas (a spectral program lifts the procedural sky's "
+                                        "colours through the coefficient atlas in pt_scene_desc)");
     if (prog.env_chart && dev.env_type == PT_ENV_SKY && prog.env_chart - 1 != dev.env_chart)
         return fail(PT_E_INVALID, std::string("the program's envSampler.chart is '") +
                                       (prog.env_chart - 1 == PT_CHART_OCTAHEDRAL ? "octahedral" : "equirect") +
@@ -1506,9 +1510,9 @@ pt_status pt_scene_update_env(pt_ctx* ctx, const pt_env* env) {
     const pt_env& a = ctx->host_env;
     if (env->type != a.type || env->samplable != a.samplable)
         return fail(PT_E_INVALID, "pt_scene_update_env: type and samplable must not change");
-    if (a.type == PT_ENV_SKY && (env->chart != a.chart || env->compensated != a.compensated ||
+    if (a.type == PT_ENV_SKY && (env->chart != a.chart || env->compensated != a.compensated || env->body != a.body ||
                                  env->table_h != a.table_h || (a.chart != PT_CHART_OCTAHEDRAL && env->table_w != a.table_w)))
-        return fail(PT_E_INVALID, "pt_scene_update_env: the table's chart, size and compensation must not change "
+        return fail(PT_E_INVALID, "pt_scene_update_env: the sky's body and its table's chart, size and compensation must not change "
                                   "(pt_env_table_upload rebuilds the table)");
     if (!std::isfinite(env->intensity) || env->intensity < 0.0f || !std::isfinite(env->rotation) ||
         !(env->select_prob >= 0.0f && env->select_prob <= 1.0f))

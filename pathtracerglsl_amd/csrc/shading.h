@@ -5,6 +5,7 @@
 #pragma once
 #include "dev_common.h"
 #include "traversal.h"
+#include "sky.h"
 
 namespace ptmi {
 
@@ -717,17 +718,64 @@ PT_DEV float surface_pdf(const DevScene& S, const pt_material& m, f3 wi, f3 wo, 
     return 0.0f;
 }
 // ---------------------------------------------------------------- environment
-// Procedural sky (proc-sky snapshot, generated-glsl.snapshot.test.ts.snap:63758) and the
-// equirect chart + table sampler (components/env/sampler_cdf.glsl, equirect/*.glsl).
-PT_DEV f3 sky_raw(f3 dir) {
-    f3 sun = normalize(mk3(0.4f, 0.35f, 0.2f));
-    float s = 0.5f * (dir.y + 1.0f);
-    f3 base = mk3(0.06f, 0.08f, 0.12f) * (1.0f - s) + mk3(0.35f, 0.45f, 0.65f) * s;
-    float k = ptm_smoothstep(0.9994f, 0.9999f, dot(dir, sun));
-    return base + mk3(18.0f, 16.0f, 13.0f) * k;
+// The equirect chart + table sampler (components/env/sampler_cdf.glsl, equirect/*.glsl); the sky
+// bodies are pt_sky_body (sky.h).
+// spectral_coefficients (lookup.glsl:16-38): the atlas's trilinear lookup for a bounded rgb whose
+// largest channel is z; x, y the other two over z (the division before the scale), the z node
+// by a six-step search of the 64 nodes.
+PT_DEV void atlas_coefficients(const DevScene& S, f3 rgb, float* out) {
+    const float v[3] = {rgb.x, rgb.y, rgb.z};
+    const int ch = rgb.x >= rgb.y && rgb.x >= rgb.z ? 0 : (rgb.y >= rgb.z ? 1 : 2);
+    const float z = v[ch];
+    const float x = (v[(ch + 1) % 3] / z) * 63.0f, y = (v[(ch + 2) % 3] / z) * 63.0f;
+    int ix = (int)x, iy = (int)y;
+    ix = ix < 62 ? ix : 62; iy = iy < 62 ? iy : 62;
+    const float fx = x - (float)ix, fy = y - (float)iy;
+    int lo = 0, hi = 63;
+    for (int step = 0; step < 6; ++step) {
+        const int mid = (lo + hi) / 2;
+        if (S.spec_znodes[mid] <= z) lo = mid; else hi = mid;
+    }
+    const float z0 = S.spec_znodes[lo], z1 = S.spec_znodes[lo + 1];
+    const float fz = (z - z0) / (z1 - z0);
+    const float4* A = S.spec_atlas + (size_t)ch * 64 * 64 * 64;
+    float4 n[2][2][2];
+    for (int dz = 0; dz < 2; dz++)
+        for (int dy = 0; dy < 2; dy++)
+            for (int dx = 0; dx < 2; dx++) n[dz][dy][dx] = A[((size_t)(lo + dz) * 64 + (iy + dy)) * 64 + (ix + dx)];
+    for (int c = 0; c < 3; c++) {
+        float p[2];
+        for (int dz = 0; dz < 2; dz++)
+            p[dz] = glsl_mix(glsl_mix(lane(n[dz][0][0], c), lane(n[dz][0][1], c), fx),
+                             glsl_mix(lane(n[dz][1][0], c), lane(n[dz][1][1], c), fx), fy);
+        out[c] = glsl_mix(p[0], p[1], fz);
+    }
+}
+// color_lift_emission of a runtime RGB (lookup.glsl:50-60): scale = 2 max(rgb); an exact grey
+// rgb / scale is Spectrum(value), any other goes through the atlas; times scale, times D65.
+PT_DEV sp4 lift_emission_rgb(const DevScene& S, f3 c, const Wl& wl) {
+    pt_spectrum r{};
+    r.scaled = 1;
+    r.d65 = 1;
+    r.scale = 2.0f * fmaxf(c.x, fmaxf(c.y, c.z));
+    if (r.scale == 0.0f) {
+        r.kind = PT_SPEC_ZERO;
+    } else {
+        const f3 b = mk3(c.x / r.scale, c.y / r.scale, c.z / r.scale);
+        if (b.x == b.y && b.y == b.z) {
+            r.kind = PT_SPEC_GREY;
+            r.value = b.x;
+        } else {
+            r.kind = PT_SPEC_SIGMOID;
+            atlas_coefficients(S, b, r.c);
+        }
+    }
+    return spectrum_eval(S, r, wl);
 }
 // Environment radiance. The constant sky is color * u_envIntensity in either colour model
-// (features/environment.ts:119-133); the procedural sky runs RGB programs only.
+// (features/environment.ts:119-133); the procedural sky is its body at the direction turned
+// into the table's azimuth (env_rotate_cs), lifted as an emission in a spectral program, times
+// u_envIntensity (features/environment.ts:207-213).
 template <bool SPEC>
 PT_DEV sp4 env_radiance(const DevScene& S, f3 dir, const Wl& wl) {
     if (S.env_type == PT_ENV_CONSTANT)
@@ -736,7 +784,10 @@ PT_DEV sp4 env_radiance(const DevScene& S, f3 dir, const Wl& wl) {
     f3 d = normalize(dir);
     float cx = S.env_rot_c, sy = S.env_rot_s;
     d = mk3(cx * d.x - sy * d.z, d.y, sy * d.x + cx * d.z);
-    return rgb4(sky_raw(d) * S.env_intensity);
+    float c[3];
+    pt_sky_body(S.env_body, d.x, d.y, d.z, c);
+    if (!SPEC) return rgb4(mk3(c[0], c[1], c[2]) * S.env_intensity);
+    return lift_emission_rgb(S, mk3(c[0], c[1], c[2]), wl) * S.env_intensity;
 }
 PT_DEV float env_texel_dOmega(const DevScene& S, int j, int W, int H) {
     if (S.env_chart == PT_CHART_OCTAHEDRAL) return (4.0f * PTM_PI) / ((float)W * (float)H);   // octahedral.glsl

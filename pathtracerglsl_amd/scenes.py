@@ -324,6 +324,78 @@ def gaussian_plane(band=None):
     }
 
 
+def _ground(y=0.0, half=200.0):
+    """The infinite plane y = const of the reference's scenes, as a 2 half x 2 half quad facing +y."""
+    return {"type": "quad", "corner": [-half, y, half], "edge1": [2.0 * half, 0.0, 0.0],
+            "edge2": [0.0, 0.0, -2.0 * half]}
+
+
+def proc_sky(glsl=None, rotation=0.0, chart="equirect", compensated=False):
+    """proc-sky (tests/witnesses/scenes/envScenes.ts:10-40, 172-187): a Lambert ground (albedo
+    0.4), a clay sphere and a glass sphere under a procedural sky (the proc-sky body unless
+    `glsl` names another), posed from (0, 1.4, 5) at (0, 0.9, 0), fov 0.9."""
+    env = {"type": "procedural", "intensity": 1.0, "rotation": rotation, "chart": chart, "compensated": compensated}
+    if glsl is not None:
+        env["glsl"] = {"kind": "glsl", "source": glsl}
+    return {
+        "id": "proc-sky",
+        "materials": {"ground": {"model": "lambert", "albedo": [0.4, 0.4, 0.4]},
+                      "clay": {"model": "lambert", "albedo": [0.7, 0.45, 0.3]},
+                      "glass": {"model": "dielectric", "ior": 1.5}},
+        "objects": [{**_ground(), "material": "ground"},
+                    {"type": "sphere", "center": [0.0, 1.0, 0.0], "radius": 1.0, "material": "clay"},
+                    {"type": "sphere", "center": [2.2, 0.7, -0.5], "radius": 0.7, "material": "glass"}],
+        "lights": [],
+        "environment": env,
+        "camera": {"type": "pinhole", "position": [0.0, 1.4, 5.0], "target": [0.0, 0.9, 0.0], "fov": 0.9},
+    }
+
+
+# proc-sky-rotated's body (tests/witnesses/scenes/estimatorAgreementWitness.ts:61-72): a 10-degree
+# sun of radiance 40 over a sky of 0.02
+BIG_SUN_SKY = "vec3(0.02) + vec3(40.0) * smoothstep(0.985, 0.99, dot(dir, normalize(vec3(0.4, 0.35, 0.2))))"
+
+
+def proc_sky_rotated(chart="equirect"):
+    """proc-sky-rotated (estimatorAgreementWitness.ts:61-72): proc-sky under the big-sun body
+    turned by -3 rad, which puts the sun in the band of table columns whose u = phi / 2 pi + 1/2 +
+    r / 2 pi leaves [0, 1): the environment pdf must wrap u as the sampler does, so pt-nee and
+    pt-mis converge to one image."""
+    sc = proc_sky(BIG_SUN_SKY, rotation=-3.0, chart=chart)
+    sc["id"] = "proc-sky-rotated"
+    return sc
+
+
+def sky_plane(glsl, scene_id, chart="equirect", compensated=False, table_size=(4, 2)):
+    """The spectral-skies plane (tests/witnesses/scenes/spectralSkies.ts:12-16): a Lambert plane
+    y = 0 (albedo 0.5) seen from (0, 1, 0) toward (0, 0, -0.1), fov 0.6, under a procedural sky
+    with a tiny importance table."""
+    return {
+        "id": scene_id,
+        "materials": {"floor": {"model": "lambert", "albedo": [0.5, 0.5, 0.5]}},
+        "objects": [{**_ground(), "material": "floor"}],
+        "lights": [],
+        "environment": {"type": "procedural", "intensity": 1.0, "chart": chart, "compensated": compensated,
+                        "table_size": list(table_size), "glsl": {"kind": "glsl", "source": glsl}},
+        "camera": {"type": "pinhole", "position": [0.0, 1.0, 0.0], "target": [0.0, 0.0, -0.1], "fov": 0.6},
+    }
+
+
+def sky_missed_cap(chart="equirect"):
+    """sky-missed-cap (spectralSkies.ts:38-46): the sky is 1 inside the polar cap dir.y >= 0.9 and
+    0 elsewhere, and a 4 x 2 table (2 x 2 octahedral) has no texel centre in the cap, so the table
+    holds no mass and the sampler draws the uniform sphere alone. The plane's radiance is
+    rho (1 - cos^2 theta_cap) = 0.5 (1 - 0.81) = 0.095 under NEE."""
+    return sky_plane("vec3(step(0.9, dir.y))", "sky-missed-cap", chart)
+
+
+def spectral_comp_empty(chart="equirect"):
+    """spectral-comp-empty (spectralSkies.ts:47-52): a white sky's MIS-compensated table is empty,
+    so environment NEE draws nothing and the BSDF's misses take full MIS weight: the plane's
+    radiance is rho L = 0.5."""
+    return sky_plane("vec3(1.0)", "spectral-comp-empty", chart, compensated=True)
+
+
 def cornell_glass_medium(light_radiance=(12.0, 12.0, 12.0)):
     """The spectral-media workload's geometry in RGB: the Cornell box with a radius-0.5 smooth
     dielectric sphere (eta 1.5) at the centre, filled with a chromatic homogeneous medium

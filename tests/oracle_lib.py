@@ -53,6 +53,8 @@ def load(variant=""):
     lib.or_phase_sample.restype = None
     lib.or_spectrum_eval.argtypes = [vp, vp, vp, C.c_int, vp]
     lib.or_spectrum_eval.restype = None
+    lib.or_env_radiance.argtypes = [vp, vp, vp, C.c_int, vp]
+    lib.or_env_radiance.restype = None
     lib.or_basic_eval.argtypes = [C.c_int, vp, vp, vp, C.c_int]
     lib.or_adjoint.argtypes = [vp, C.POINTER(abi.PtProgram), C.POINTER(abi.PtParams), C.c_int,
                                C.POINTER(abi.PtAdjointParams), vp, vp]

@@ -31,7 +31,7 @@ def _expected_estimator(d):
 
 # The number of the 505 frozen link maps this backend lowers: a lowering that starts rejecting
 # a case (or accepting one it cannot render) changes it.
-EXPECTED_ACCEPTED = 343
+EXPECTED_ACCEPTED = 354
 
 
 def test_library_lowers_or_names_every_snapshot_case():
@@ -71,13 +71,16 @@ def test_library_lowers_or_names_every_snapshot_case():
     print(f"lowered {len(accepted)} of {len(accepted) + rejected}")
     assert len(accepted) + rejected == 505
     assert len(accepted) == EXPECTED_ACCEPTED
-    # the one frozen octahedral-chart case on a sky this backend renders (the others name image
-    # environments, expression media or spectral procedural skies)
-    assert "spectral-image-constant-ref + oct" in accepted
+    # frozen octahedral-chart cases on skies this backend renders (the others name image
+    # environments)
+    assert {"spectral-image-constant-ref + oct", "sky-missed-cap + spectral-octahedral", "spectral-comp-empty + octahedral",
+            "spectral-proc-sky + oct"} <= set(accepted)
     assert "two-light + uniform" in accepted                  # estimator.lighting.selection 'uniform'
     assert "spectral-rayleigh-furnace + visible" in accepted  # media.models ['rayleigh']
     assert {"spectral-gaussian-plane + spectral-mis", "spectral-gaussian-plane + spectral-none"} <= set(accepted)
     assert "cornell-oneshot + oneshot" in accepted                 # estimator.accumulation.type 'oneshot'
+    # procedural sky bodies other than proc-sky's, and proc-sky under the spectral colour model
+    assert {"proc-sky-rotated + pt-mis", "spectral-proc-sky + mis"} <= set(accepted)
 
 
 def test_chart_and_albedo_flag_lowering():

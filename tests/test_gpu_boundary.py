@@ -92,7 +92,8 @@ def test_octahedral_description_renders_like_the_oracle_and_batch_rejects_the_ot
 
 def test_scene_replaced_after_load_is_checked_at_render(engine):
     lib = _lib.load()
-    # a spectral program, loaded over a scene without a sky; then a procedural-sky scene arrives
+    # a spectral program, loaded over a scene without a sky; then a procedural-sky scene compiled
+    # for an RGB program arrives, without the coefficient atlas the spectral lift of its body needs
     spec = compiler.compile_scene(scenes.spectral_direct(), compiler.RenderStrategy("nee", 4, -1, color="spectral"), 16, 16)
     engine.load_renderer(spec)
     engine.render_frames(1)
@@ -100,9 +101,9 @@ def test_scene_replaced_after_load_is_checked_at_render(engine):
     engine.upload_scene(sky.scene)
     h, _ = engine._renderers[engine._selected]
     prm = spec.params()
-    assert lib.pt_render_frames(h, C.byref(prm), 1) == 5
-    assert b"spectral" in lib.pt_last_error()
-    assert lib.pt_forward(h, C.byref(prm), 1) == 5
+    assert lib.pt_render_frames(h, C.byref(prm), 1) == 2              # PT_E_MISSING_INPUT
+    assert b"spectral_atlas" in lib.pt_last_error()
+    assert lib.pt_forward(h, C.byref(prm), 1) == 2
     # a program naming the octahedral chart, loaded over an octahedral table; then an equirect table
     octa = compiler.compile_scene(_sky_cornell("octahedral"), compiler.RenderStrategy("mis", 4, -1), 16, 16)
     octa.program.env_chart = abi.CHART_OCTAHEDRAL + 1

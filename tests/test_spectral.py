@@ -84,11 +84,15 @@ def test_any_colour_lifts_and_a_tampered_atlas_is_refused(monkeypatch, tmp_path)
         spectral.atlas()
 
 
-def test_spectral_rejects_environment():
+def test_spectral_procedural_sky_carries_the_atlas():
     sc = scenes.cornell_box()
     sc["environment"] = {"type": "procedural"}
-    with pytest.raises(compiler.CompileError, match="environment"):
-        compiler.compile_scene(sc, compiler.RenderStrategy("mis", 2, -1, color="spectral"), 8, 8)
+    comp = compiler.compile_scene(sc, compiler.RenderStrategy("mis", 2, -1, color="spectral"), 8, 8)
+    assert comp.scene.spectral_atlas.shape == (3 * 64 ** 3, 4) and comp.scene.spectral_z_nodes.shape == (64,)
+    d = comp.scene.desc()
+    assert d.spectral_atlas and d.spectral_z_nodes
+    rgb = compiler.compile_scene(sc, compiler.RenderStrategy("mis", 2, -1), 8, 8)
+    assert rgb.scene.spectral_atlas is None and not rgb.scene.desc().spectral_atlas
 
 
 def _render(oracle, sc, strategy, w, h, spp):
