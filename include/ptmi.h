@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define PT_ABI_VERSION 13u
+#define PT_ABI_VERSION 14u
 
 typedef enum pt_status {
     PT_OK = 0,
@@ -244,6 +244,12 @@ typedef struct pt_scene_desc {
      * needed only by a spectral program whose procedural sky body is not grey; may be null */
     const float* spectral_atlas;
     const float* spectral_z_nodes;
+    /* the light tree's inputs (sceneData.ts:99-143): per light its box (6 f64: min.xyz, max.xyz,
+     * the kind's treeBounds: a quad's four corners, a point's position) and its selection power
+     * (f64, > 0: pi A mean(Le) for a quad, 4 pi mean(I) for a point, floored at 1e-8); needed
+     * only by programs with PT_SEL_BVH; may be null */
+    const double* light_boxes;
+    const double* light_powers;
 } pt_scene_desc;
 
 /* ---- program (the Plan stage's decisions) ---- */
@@ -293,9 +299,16 @@ typedef struct pt_program {
 enum { PT_ACC_AVERAGE = 0, PT_ACC_ONESHOT = 1 };
 
 /* Light selection (components/lights/index.ts LIGHT_SELECTIONS): the power-weighted CDF (the
- * reference's default) or one of N lights with equal probability, the environment then taking
- * 1/(N+1) of the NEE draws (envSelectionProbability, lighting.ts:488-518). */
-enum { PT_SEL_POWER = 0, PT_SEL_UNIFORM = 1 };
+ * reference's default), one of N lights with equal probability, the environment then taking
+ * 1/(N+1) of the NEE draws (envSelectionProbability, lighting.ts:488-518), or the light tree
+ * (lighting.ts:676-933): a stochastic descent of the tree pt_scene_upload builds over the lights
+ * (pt_scene_desc.light_boxes / light_powers, pt_light_tree_build), each level choosing a child
+ * in proportion to its importance at the shading point (accel/light_tree/light_tree.glsl:28-77:
+ * power over clamped squared distance, the receiver's horizon cull and best-case cosine),
+ * drawing a fresh random number per level and one more for the light's own sample; MIS
+ * replays the chosen light's bit trail from the stored query, and the environment takes 1/2
+ * (pt_env.select_prob). */
+enum { PT_SEL_POWER = 0, PT_SEL_UNIFORM = 1, PT_SEL_BVH = 2 };
 
 /* First-hit guides (src/components/aov/): at the first non-null surface a path reaches with no
  * scattering event before it, the material's albedo fact clamped to [0, 1] times the path's
@@ -594,6 +607,15 @@ pt_status pt_inst_parse(const void* buffer, size_t len, pt_inst_table* out);
 enum { PT_BVH_ALWAYS_SPLIT = 1, PT_BVH_SORTED_MEDIAN = 2 };
 pt_status pt_bvh_build(const double* boxes, int32_t n, int32_t leaf_size, uint32_t flags, float* nodes_out,
                        uint32_t* order_out, int32_t* node_count, int32_t* max_depth);
+/* buildLightTree (src/components/accel/light_tree/light_tree.ts:46-77): the light tree over n
+ * lights (boxes: 6 f64 each, min.xyz, max.xyz, degenerate allowed; powers: n f64, finite and
+ * > 0), the binned-SAH core weighted by power, one light per leaf, always split, depth <= 48
+ * (the balanced sorted-median split where the cap bites). nodes_out: 8 (2n - 1) floats, node k
+ * = (min.xyz, phi) (max.xyz, link), phi the subtree power, link >= 0 the right child (the left
+ * is k + 1), link < 0 a leaf of light -link - 1; trails_out: 4 n floats, per light (lo, hi, 0, 0),
+ * its root-to-leaf left/right bits LSB first, levels 0-23 in lo and 24-47 in hi. */
+pt_status pt_light_tree_build(const double* boxes, const double* powers, int32_t n, float* nodes_out, float* trails_out,
+                              int32_t* node_count, int32_t* max_depth);
 /* buildBVH (bvh.ts:444-470): a mesh BLAS over n_tris triangles (3 indices each into n_vertices
  * positions of 3 f32), leaf size 2; nodes as pt_bvh_build, reindexed_out the 3 n_tris indices
  * in leaf order. */

@@ -267,6 +267,135 @@ int or_bvh_build(const double* boxes, int n, int leaf_size, float* nodes, uint32
     return nn;
 }
 
+/* ---------------- the light tree (accel/light_tree/light_tree.ts:46-77) ----------------
+ * buildBVHCore (accel/bvh/bvh.ts:132-320) with the light tree's parameters: the split cost
+ * A_L Phi_L + A_R Phi_R, one light per leaf, a split always (the sorted-median fallback where
+ * SAH has no plane), balanced splits once depth + ceil(log2 count) reaches 48, and per light
+ * the root-to-leaf bits (0 left, 1 right) LSB first, levels 0-23 in lo and 24-47 in hi.
+ * Node = (min.xyz, Phi) (max.xyz, link): link >= 0 right child (left = node + 1), link < 0
+ * the leaf of light -link - 1. */
+#define LT_MAX_DEPTH 48
+typedef struct {
+    const double* boxes; const double* pw; float *cx, *cy, *cz; uint32_t* order;
+    float* nodes; float* trails; int n_nodes; int max_depth;
+} lt_build_t;
+
+static int lt_ceil_log2(int c) { int k = 0; while ((1 << k) < c) k++; return k; }
+
+static int lt_emit(lt_build_t* B, int start, int end, int depth, uint32_t lo, uint32_t hi) {
+    if (depth > B->max_depth) B->max_depth = depth;
+    int node = B->n_nodes++;
+    double bounds[6], cb[6], w = 0.0;
+    box_reset(bounds); box_reset(cb);
+    for (int i = start; i < end; i++) {
+        uint32_t t = B->order[i]; const double* bx = B->boxes + 6 * (size_t)t;
+        for (int a = 0; a < 3; a++) { if (bx[a] < bounds[a]) bounds[a] = bx[a]; if (bx[3 + a] > bounds[3 + a]) bounds[3 + a] = bx[3 + a]; }
+        double c[3] = {B->cx[t], B->cy[t], B->cz[t]};
+        for (int a = 0; a < 3; a++) { if (c[a] < cb[a]) cb[a] = c[a]; if (c[a] > cb[3 + a]) cb[3 + a] = c[a]; }
+        w += B->pw[t];
+    }
+    float* nd = B->nodes + 8 * (size_t)node;
+    nd[0] = (float)bounds[0]; nd[1] = (float)bounds[1]; nd[2] = (float)bounds[2]; nd[3] = (float)w;
+    nd[4] = (float)bounds[3]; nd[5] = (float)bounds[4]; nd[6] = (float)bounds[5];
+    int count = end - start;
+    if (count == 1) {
+        uint32_t light = B->order[start];
+        nd[7] = -(float)light - 1.0f;
+        B->trails[4 * (size_t)light] = (float)lo; B->trails[4 * (size_t)light + 1] = (float)hi;
+        return node;
+    }
+    int balanced = depth + lt_ceil_log2(count) >= LT_MAX_DEPTH;
+    int bestAxis = -1, bestSplit = -1; double bestCost = INFINITY;
+    int v[3] = {cb[3] - cb[0] >= 1e-12, cb[4] - cb[1] >= 1e-12, cb[5] - cb[2] >= 1e-12};
+    if (!balanced && (v[0] || v[1] || v[2])) {
+        double s[3];
+        for (int a = 0; a < 3; a++) s[a] = v[a] ? BVH_BINS / (cb[3 + a] - cb[a]) : 0;
+        double binBox[3 * BVH_BINS][6], binW[3 * BVH_BINS]; int binCnt[3 * BVH_BINS];
+        for (int b = 0; b < 3 * BVH_BINS; b++) { box_reset(binBox[b]); binCnt[b] = 0; binW[b] = 0.0; }
+        for (int i = start; i < end; i++) {
+            uint32_t t = B->order[i]; const double* bx = B->boxes + 6 * (size_t)t;
+            float c[3] = {B->cx[t], B->cy[t], B->cz[t]};
+            for (int a = 0; a < 3; a++) {
+                if (!v[a]) continue;
+                int b = (int)floor(((double)c[a] - cb[a]) * s[a]);
+                if (b < 0) b = 0; if (b >= BVH_BINS) b = BVH_BINS - 1;
+                binCnt[a * BVH_BINS + b]++; binW[a * BVH_BINS + b] += B->pw[t];
+                box_grow(binBox[a * BVH_BINS + b], bx);
+            }
+        }
+        for (int a = 0; a < 3; a++) {
+            if (!v[a]) continue;
+            float la[BVH_BINS - 1], ra[BVH_BINS - 1]; int lc[BVH_BINS - 1], rc[BVH_BINS - 1];
+            double lw[BVH_BINS - 1], rw[BVH_BINS - 1], acc[6], aw = 0.0; int an = 0;
+            box_reset(acc);
+            for (int b = 0; b < BVH_BINS - 1; b++) {
+                box_grow(acc, binBox[a * BVH_BINS + b]); an += binCnt[a * BVH_BINS + b]; aw += binW[a * BVH_BINS + b];
+                la[b] = (float)box_area(acc); lc[b] = an; lw[b] = aw;
+            }
+            box_reset(acc); an = 0; aw = 0.0;
+            for (int b = BVH_BINS - 1; b > 0; b--) {
+                box_grow(acc, binBox[a * BVH_BINS + b]); an += binCnt[a * BVH_BINS + b]; aw += binW[a * BVH_BINS + b];
+                ra[b - 1] = (float)box_area(acc); rc[b - 1] = an; rw[b - 1] = aw;
+            }
+            for (int b = 0; b < BVH_BINS - 1; b++) {
+                if (lc[b] == 0 || rc[b] == 0) continue;
+                double cost = (double)la[b] * lw[b] + (double)ra[b] * rw[b];
+                if (cost < bestCost) { bestCost = cost; bestAxis = a; bestSplit = b; }
+            }
+        }
+    }
+    int mid = -1;
+    if (bestAxis != -1) {
+        double plo = cb[bestAxis], scale = BVH_BINS / (cb[3 + bestAxis] - cb[bestAxis]);
+        const float* cen = bestAxis == 0 ? B->cx : bestAxis == 1 ? B->cy : B->cz;
+        int m = start;
+        for (int i = start; i < end; i++) {
+            uint32_t t = B->order[i];
+            int b = (int)floor(((double)cen[t] - plo) * scale);
+            if (b < 0) b = 0; if (b >= BVH_BINS) b = BVH_BINS - 1;
+            if (b <= bestSplit) { uint32_t tmp = B->order[i]; B->order[i] = B->order[m]; B->order[m] = tmp; m++; }
+        }
+        if (m > start && m < end) mid = m;
+    }
+    if (mid == -1) {   /* sorted median: a stable insertion sort by centroid on the widest axis, halved */
+        double e0 = cb[3] - cb[0], e1 = cb[4] - cb[1], e2 = cb[5] - cb[2];
+        const float* cen = (e0 >= e1 && e0 >= e2) ? B->cx : (e1 >= e2 ? B->cy : B->cz);
+        for (int i = start + 1; i < end; i++) {
+            uint32_t t = B->order[i]; int j = i - 1;
+            while (j >= start && cen[B->order[j]] > cen[t]) { B->order[j + 1] = B->order[j]; j--; }
+            B->order[j + 1] = t;
+        }
+        mid = (start + end) >> 1;
+    }
+    uint32_t rlo = depth < 24 ? lo | (1u << depth) : lo;
+    uint32_t rhi = depth < 24 ? hi : hi | (1u << (depth - 24));
+    lt_emit(B, start, mid, depth + 1, lo, hi);
+    int right = lt_emit(B, mid, end, depth + 1, rlo, rhi);
+    B->nodes[8 * (size_t)node + 7] = (float)right;
+    return node;
+}
+/* nodes: 8 (2n - 1) floats, trails 4 n. Returns the node count; *max_depth the deepest leaf. */
+int or_light_tree_build(const double* boxes, const double* powers, int n, float* nodes, float* trails, int* max_depth) {
+    if (n <= 0) { if (max_depth) *max_depth = 0; return 0; }
+    lt_build_t B;
+    memset(&B, 0, sizeof B);
+    B.boxes = boxes; B.pw = powers; B.nodes = nodes; B.trails = trails;
+    B.order = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)n);
+    B.cx = (float*)malloc(sizeof(float) * (size_t)n); B.cy = (float*)malloc(sizeof(float) * (size_t)n);
+    B.cz = (float*)malloc(sizeof(float) * (size_t)n);
+    for (int t = 0; t < n; t++) {
+        B.cx[t] = (float)((boxes[6 * t] + boxes[6 * t + 3]) * 0.5);
+        B.cy[t] = (float)((boxes[6 * t + 1] + boxes[6 * t + 4]) * 0.5);
+        B.cz[t] = (float)((boxes[6 * t + 2] + boxes[6 * t + 5]) * 0.5);
+        B.order[t] = (uint32_t)t;
+    }
+    memset(trails, 0, sizeof(float) * 4 * (size_t)n);
+    lt_emit(&B, 0, n, 0, 0u, 0u);
+    free(B.order); free(B.cx); free(B.cy); free(B.cz);
+    if (max_depth) *max_depth = B.max_depth;
+    return B.n_nodes;
+}
+
 /* Inverse ray direction for the slab tests. The BLAS walk (mesh.glsl:167) takes 1/rd, which
  * for an exactly axis-aligned ray gives 0*inf = NaN in the slab test (undefined in GLSL);
  * declared rule: the CWBVH walk's guard |d| < 1e-20 -> +-1e-20 (accel/cwbvh, cwbvh.ts). */
@@ -321,6 +450,8 @@ typedef struct {
     int has_spectral;
     float* atlas;             /* the coefficient atlas [3 * 64^3][4] (null: none) and its z nodes */
     float atlas_z[64];
+    float* lt_nodes;          /* the light tree (8 floats per node) and the lights' trails (4 each), or null */
+    float* lt_trails;
 } or_scene;
 
 /* Möller–Trumbore (intersection/mesh/mesh.glsl:41-59). */
@@ -1518,13 +1649,102 @@ static lsample_t environment_sample(const or_scene* S, float x0, float x1, const
     ls.flags = 0; ls.light_id = -1;
     return ls;
 }
-static lsample_t lighting_sample(const or_scene* S, v3 p, float x0, float x1, const wl_t* wl) {
+/* LightQuery (glsl/core/structs.glsl:83-87; pt.ts:125-140): the receiver point, its shading
+ * normal (0 at a medium event) and whether the receiver's BSDF support is the sphere and NEE
+ * evaluates it (modelTwoSidedShading, materials/index.ts:56-69: the rough dielectric here). */
+typedef struct { v3 p, n; int two_sided; } lquery_t;
+
+/* light_tree_importance (accel/light_tree/light_tree.glsl:28-77) */
+static float lt_importance(const float* n0, const float* n1, const lquery_t* q) {
+    v3 bmin = V(n0[0], n0[1], n0[2]), bmax = V(n1[0], n1[1], n1[2]);
+    v3 c = muls(add(bmin, bmax), 0.5f);
+    v3 e = sub(bmax, bmin);
+    v3 d = sub(c, q->p);
+    float d2raw = dot(d, d);
+    float diag2 = 0.25f * dot(e, e);
+    float base = n0[3] / fmaxf_(d2raw, fmaxf_(diag2, 1e-8f));
+    if (dot(q->n, q->n) < 0.5f) return base;                       /* no orientation */
+    /* the box corner farthest along n: mix(bmin, bmax, step(0, n)) */
+    v3 far_c = V(q->n.x < 0.0f ? bmin.x : bmax.x, q->n.y < 0.0f ? bmin.y : bmax.y, q->n.z < 0.0f ? bmin.z : bmax.z);
+    float h = dot(q->n, sub(far_c, q->p));
+    const int two = q->two_sided && !PLANT(11);   /* plant 11: the query's sphere support ignored */
+    if (two) {
+        v3 near_c = sub(add(bmin, bmax), far_c);
+        h = fmaxf_(h, -dot(q->n, sub(near_c, q->p)));
+    } else if (h <= 0.0f) {
+        return 0.0f;
+    }
+    float cos_c = dot(q->n, d) * (1.0f / sqrtf(fmaxf_(d2raw, 1e-12f)));
+    if (two) cos_c = fabsf(cos_c);
+    v3 dmx = V(fmaxf_(q->p.x - bmin.x, bmax.x - q->p.x), fmaxf_(q->p.y - bmin.y, bmax.y - q->p.y),
+               fmaxf_(q->p.z - bmin.z, bmax.z - q->p.z));
+    float lo = h * (1.0f / sqrtf(fmaxf_(dot(dmx, dmx), 1e-12f)));
+    return base * fminf_(1.0f, fmaxf_(cos_c, lo));
+}
+/* light_tree_pick (lighting.ts:786-800): the stochastic descent, importance-proportional child
+ * choice, a fresh random number per level; the product pmf out; -1 on a dead descent. */
+static int lt_pick(const or_scene* S, const lquery_t* q, float xi, rng_t* rng, float* pmf) {
+    *pmf = 1.0f;
+    int ni = 0;
+    const float* n1 = S->lt_nodes + 4;
+    for (int lvl = 0; lvl < LT_MAX_DEPTH; lvl++) {
+        if (n1[3] < 0.0f) break;
+        int L = ni + 1, R = (int)n1[3];
+        const float *l0 = S->lt_nodes + 8 * (size_t)L, *r0 = S->lt_nodes + 8 * (size_t)R;
+        float il = lt_importance(l0, l0 + 4, q), ir = lt_importance(r0, r0 + 4, q);
+        float isum = il + ir;
+        if (isum <= 0.0f) { *pmf = 0.0f; return -1; }
+        float p_l = il / isum;
+        if (xi < p_l) { *pmf *= p_l; ni = L; n1 = l0 + 4; }
+        else { *pmf *= (1.0f - p_l); ni = R; n1 = r0 + 4; }
+        xi = rnd(rng);
+    }
+    return (int)(-n1[3] - 1.0f);
+}
+/* light_tree_pmf (lighting.ts:802-822): the light's trail replayed with the stored query */
+static float lt_pmf(const or_scene* S, const lquery_t* q, int light) {
+    uint32_t tw = (uint32_t)S->lt_trails[4 * (size_t)light];
+    float pmf = 1.0f;
+    int ni = 0;
+    const float* n1 = S->lt_nodes + 4;
+    for (int lvl = 0; lvl < LT_MAX_DEPTH; lvl++) {
+        if (n1[3] < 0.0f) break;
+        if (lvl == 24) tw = (uint32_t)S->lt_trails[4 * (size_t)light + 1];
+        int L = ni + 1, R = (int)n1[3];
+        const float *l0 = S->lt_nodes + 8 * (size_t)L, *r0 = S->lt_nodes + 8 * (size_t)R;
+        float il = lt_importance(l0, l0 + 4, q), ir = lt_importance(r0, r0 + 4, q);
+        float isum = il + ir;
+        if (isum <= 0.0f) return 0.0f;
+        float p_l = il / isum;
+        if (((tw & 1u) == 0u) != PLANT(12)) { pmf *= p_l; ni = L; n1 = l0 + 4; }   /* plant 12: the bits read inverted */
+        else { pmf *= (1.0f - p_l); ni = R; n1 = r0 + 4; }
+        tw >>= 1u;
+    }
+    return pmf;
+}
+/* the tree's finite stage (generateLightSamplingBvh, lighting.ts:834-870): pick, then the
+ * light's sample at (random(), x1), its pdf times the selection pmf */
+static lsample_t lighting_sample_tree(const or_scene* S, const lquery_t* q, float x0, float x1, rng_t* rng, const wl_t* wl) {
+    lsample_t ls;
+    float sel;
+    int id = lt_pick(S, q, x0, rng, &sel);
+    if (id < 0) { ls.pdf = 0.0f; return ls; }
+    float lx = rnd(rng);
+    ls = light_sample_one(S, &S->lights[id], q->p, lx, x1, wl);
+    ls.light_id = id;
+    ls.pdf *= sel;
+    return ls;
+}
+/* lighting_sample: env-only, finite-only or two-stage; the finite stage by the selection CDF or,
+ * with tree != 0, the light tree (then rng feeds its descent) */
+static lsample_t lighting_sample(const or_scene* S, const lquery_t* q, float x0, float x1, const wl_t* wl, rng_t* rng, int tree) {
     lsample_t ls;
     if (S->n_lights == 0) {
         if (env_samplable(S)) return environment_sample(S, x0, x1, wl);
         ls.pdf = 0.0f; return ls;
     }
-    if (!env_select_live(S)) return lighting_sample_finite(S, p, x0, x1, wl);
+    if (!env_select_live(S))
+        return tree ? lighting_sample_tree(S, q, x0, x1, rng, wl) : lighting_sample_finite(S, q->p, x0, x1, wl);
     float ps = S->env.select_prob;
     if (x0 < ps) {
         ls = environment_sample(S, clampf(x0 / ps, 0.0f, 0.9999999f), x1, wl);
@@ -1532,7 +1752,7 @@ static lsample_t lighting_sample(const or_scene* S, v3 p, float x0, float x1, co
         return ls;
     }
     float fx = clampf((x0 - ps) / (1.0f - ps), 0.0f, 0.9999999f);
-    ls = lighting_sample_finite(S, p, fx, x1, wl);
+    ls = tree ? lighting_sample_tree(S, q, fx, x1, rng, wl) : lighting_sample_finite(S, q->p, fx, x1, wl);
     ls.pdf *= (1.0f - ps);
     return ls;
 }
@@ -1541,11 +1761,13 @@ static int light_of(const or_scene* S, int region) {
     for (int i = 0; i < S->n_lights; i++) if (S->lights[i].region == region) return i;
     return -1;
 }
-static float lighting_pdf(const or_scene* S, v3 qp, v3 wi, int lid, v3 light_p) {
+/* lighting_pdf (lighting.ts:636-673; the tree's, :909-933, replays the trail) */
+static float lighting_pdf(const or_scene* S, const lquery_t* q, v3 wi, int lid, v3 light_p, int tree) {
     const pt_light* l = &S->lights[lid];
-    float sel = S->n_lights == 1 ? 1.0f : (PLANT(1) ? S->lights[0].select_pdf : l->select_pdf);
+    float sel = tree ? lt_pmf(S, q, lid)
+                     : (S->n_lights == 1 ? 1.0f : (PLANT(1) ? S->lights[0].select_pdf : l->select_pdf));
     if (env_select_live(S)) sel = sel * (1.0f - S->env.select_prob);
-    return sel * light_pdf_one(l, qp, light_p, wi);
+    return sel * light_pdf_one(l, q->p, light_p, wi);
 }
 
 /* ---------------- homogeneous media (transport/volume/analytic/analytic.glsl:9-51) ---- */
@@ -1756,6 +1978,7 @@ typedef struct {
     int width, height, max_null;
     int spectral, wl_uniform;
     int equiangular;          /* estimator.mediumLightSampling == 'equiangular' */
+    int light_tree;           /* estimator.lighting.selection == 'bvh' */
 } or_program;
 
 static inline float power_heuristic(float pf, float pg) {
@@ -1834,7 +2057,9 @@ static sp equiangular_sample_direct(const or_scene* S, const or_program* P, v3 o
 static sp transport_trace(const or_scene* S, const or_program* P, v3 ro, v3 rd, rng_t* rng, int* n_closest, int* n_any,
                           const wl_t* wl, adj_t* A, float* aov) {
     sp T = sgrey(1.0f), L = sgrey(0.0f);
-    int prev_was_delta = 1; float prev_bsdf_pdf = 0.0f; v3 prev_qp = ro;
+    int prev_was_delta = 1; float prev_bsdf_pdf = 0.0f;
+    lquery_t prev_q = {ro, V(0.0f, 0.0f, 0.0f), 0};   /* kernel_init's prev_query (kernel.ts:35) */
+    const int tree = P->light_tree;
     float eta_scale = 1.0f;
     int nee = P->estimator != PT_EST_PT, mis = P->estimator == PT_EST_MIS;
     int emitters = S->n_lights > 0;
@@ -1882,7 +2107,8 @@ static sp transport_trace(const or_scene* S, const or_program* P, v3 ro, v3 rd, 
                     const float gq = frozen ? g_prop_g : g;     /* the phase sampling's g */
                     if (nee && !P->equiangular) {   /* light_sample_direct_medium (techniques/light/light_medium.glsl:7-18) */
                         float y0 = rnd(rng), y1 = rnd(rng);
-                        lsample_t ls = lighting_sample(S, p_evt, y0, y1, wl);
+                        lquery_t qm = {p_evt, V(0.0f, 0.0f, 0.0f), 0};   /* light_query_medium */
+                        lsample_t ls = lighting_sample(S, &qm, y0, y1, wl, rng, tree);
                         if (ls.pdf > 0.0f) {
                             v3 light_p = add(p_evt, muls(ls.wi, ls.distance));
                             double sh_len = 0.0;
@@ -1913,7 +2139,7 @@ static sp transport_trace(const or_scene* S, const or_program* P, v3 ro, v3 rd, 
                     v3 wi = phase_sample(phase, wo_med, gq, z0, z1, &ppdf, wl);
                     if (frozen) T = smuls(T, phase_value(phase, wi, wo_med, g) / phase_value(phase, wi, wo_med, gq));
                     if (g_param) A->D[6] += dlog_hg(g, dot(wi, neg(wo_med)));
-                    prev_was_delta = 0; prev_bsdf_pdf = ppdf; prev_qp = p_evt;
+                    prev_was_delta = 0; prev_bsdf_pdf = ppdf; prev_q.p = p_evt; prev_q.n = V(0.0f, 0.0f, 0.0f); prev_q.two_sided = 0;
                     ro = p_evt; rd = wi;
                     /* roulette_interior (pt.ts:230-283), present when some medium weights absorption;
                      * medium_survival = sigma_s / sigma_t there, 1 on the tracking arm (materials.ts:731-772) */
@@ -1979,7 +2205,7 @@ static sp transport_trace(const or_scene* S, const or_program* P, v3 ro, v3 rd, 
             if (nee && emitters) {
                 int lid = light_of(S, hit.region_to);
                 if (lid < 0 || prev_was_delta) w = 1.0f;
-                else if (mis) w = power_heuristic(prev_bsdf_pdf, lighting_pdf(S, prev_qp, rd, lid, hit.p));
+                else if (mis) w = power_heuristic(prev_bsdf_pdf, lighting_pdf(S, &prev_q, rd, lid, hit.p, tree));
                 else w = 0.0f;
             }
             const pt_material* E = &S->materials[mat_emit];
@@ -1997,7 +2223,8 @@ static sp transport_trace(const or_scene* S, const or_program* P, v3 ro, v3 rd, 
         /* light_sample_direct (techniques/light/light.glsl:8-28) */
         if (nee && has_nondelta(M)) {
             float x0 = rnd(rng), x1 = rnd(rng);
-            lsample_t ls = lighting_sample(S, hit.p, x0, x1, wl);
+            lquery_t qs = {hit.p, hit.frame.n, M->model == PT_MAT_ROUGH_DIELECTRIC};   /* light_query_surface */
+            lsample_t ls = lighting_sample(S, &qs, x0, x1, wl, rng, tree);
             if (ls.pdf > 0.0f) {
                 v3 light_p = add(hit.p, muls(ls.wi, ls.distance));
                 v3 so = ray_spawn_origin(&hit, ls.wi);
@@ -2042,7 +2269,8 @@ static sp transport_trace(const or_scene* S, const or_program* P, v3 ro, v3 rd, 
             int ak = adj_albedo_index(A, mat);
             if (ak >= 0) A->D[7 + ak] += 1.0 / (double)M->albedo[0];
         }
-        prev_was_delta = (bs.flags & LOBE_DELTA) != 0; prev_bsdf_pdf = bs.pdf; prev_qp = hit.p;
+        prev_was_delta = (bs.flags & LOBE_DELTA) != 0; prev_bsdf_pdf = bs.pdf;
+        prev_q.p = hit.p; prev_q.n = hit.frame.n; prev_q.two_sided = M->model == PT_MAT_ROUGH_DIELECTRIC;
         if (media && (bs.flags & LOBE_TRANSMISSION)) current_medium = hit.region_to;
         if (S->has_transmission && (bs.flags & LOBE_TRANSMISSION)) {
             float r = ior_of(S, hit.region_to, wl) / ior_of(S, hit.region_from, wl);
@@ -2095,6 +2323,11 @@ or_scene* or_scene_create(const pt_scene_desc* d) {
         memcpy(S->xyz_rgb, d->xyz_to_rgb, sizeof S->xyz_rgb);
         S->cie_y = d->cie_y_integral;
         S->has_spectral = 1;
+    }
+    if (d->light_boxes && d->light_powers && d->n_lights > 0) {
+        S->lt_nodes = (float*)malloc(sizeof(float) * 8 * (size_t)(2 * d->n_lights - 1));
+        S->lt_trails = (float*)malloc(sizeof(float) * 4 * (size_t)d->n_lights);
+        or_light_tree_build(d->light_boxes, d->light_powers, d->n_lights, S->lt_nodes, S->lt_trails, NULL);
     }
     if (d->spectral_atlas && d->spectral_z_nodes) {
         S->atlas = (float*)malloc(sizeof(float) * 4 * 3 * 64 * 64 * 64);
@@ -2206,7 +2439,7 @@ void or_scene_destroy(or_scene* S) {
     free(S->blas); bvh_free(&S->tlas); free(S->tlas_obj);
     free(S->obj_first); bvh_free(&S->itlas); free(S->inst); free(S->inst_batch); free(S->inst_base);
     free(S->tri); free(S->nrm); free(S->tri_object); free(S->objects); free(S->materials); free(S->lights);
-    free(S->env_cond); free(S->env_marg); free(S->atlas);
+    free(S->env_cond); free(S->env_marg); free(S->atlas); free(S->lt_nodes); free(S->lt_trails);
     free(S);
 }
 void or_env_table(const or_scene* S, float* cond, float* marg) {
@@ -2364,6 +2597,7 @@ static void render_impl(const or_scene* S, const pt_program* prog, const pt_para
     P.max_null = prog->max_null_crossings;
     P.spectral = prog->color == PT_COLOR_SPECTRAL; P.wl_uniform = prog->wavelengths == PT_WL_UNIFORM;
     P.equiangular = prog->medium_light_sampling == PT_MLS_EQUIANGULAR;
+    P.light_tree = prog->light_selection == PT_SEL_BVH && S->lt_nodes != NULL;
     if (P.spectral && !S->has_spectral) { if (counts) { counts[0] = counts[1] = 0; } return; }
     int W = P.width, H = P.height;
     uint64_t nc = 0, na = 0;
@@ -2549,6 +2783,30 @@ void or_env_radiance(const or_scene* S, const float* dirs, const float* lam, int
         else { out[3 * i] = v.x; out[3 * i + 1] = v.y; out[3 * i + 2] = v.z; }
     }
 }
+/* Light-tree probes over a built tree (nodes, trails as or_light_tree_build writes them). */
+static or_scene* lt_probe_scene(const float* nodes, const float* trails) {
+    static or_scene T;
+    memset(&T, 0, sizeof T);
+    T.lt_nodes = (float*)nodes; T.lt_trails = (float*)trails;
+    return &T;
+}
+float or_light_tree_importance(const float* n0, const float* n1, const float* p, const float* n, int two_sided) {
+    lquery_t q = {V(p[0], p[1], p[2]), V(n[0], n[1], n[2]), two_sided};
+    return lt_importance(n0, n1, &q);
+}
+float or_light_tree_pmf(const float* nodes, const float* trails, int light, const float* p, const float* n, int two_sided) {
+    lquery_t q = {V(p[0], p[1], p[2]), V(n[0], n[1], n[2]), two_sided};
+    return lt_pmf(lt_probe_scene(nodes, trails), &q, light);
+}
+/* the descent from xi with the pixel stream (px, py, sample, salt) feeding the later levels;
+ * returns the light (-1 dead) and its product pmf in *pmf */
+int or_light_tree_pick(const float* nodes, float xi, const float* p, const float* n, int two_sided, uint32_t px, uint32_t py,
+                       uint32_t sample, uint32_t salt, float* pmf) {
+    lquery_t q = {V(p[0], p[1], p[2]), V(n[0], n[1], n[2]), two_sided};
+    rng_t r;
+    rng_init(&r, px, py, sample, salt);
+    return lt_pick(lt_probe_scene(nodes, NULL), &q, xi, &r, pmf);
+}
 /* Environment probes: pdf of directions, and samples with their pdfs. */
 void or_env_pdf(const or_scene* S, const float* dirs, int n, float* out) {
     for (int i = 0; i < n; i++) out[i] = environment_pdf(S, V(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]));
@@ -2569,7 +2827,7 @@ void or_adjoint(const or_scene* S, const pt_program* prog, const pt_params* par,
     or_program P;
     P.estimator = prog->estimator; P.max_bounces = prog->max_bounces; P.rr_start = prog->rr_start;
     P.rr_max = prog->rr_max_survival; P.width = prog->width; P.height = prog->height;
-    P.max_null = prog->max_null_crossings; P.spectral = 0; P.wl_uniform = 0; P.equiangular = 0;
+    P.max_null = prog->max_null_crossings; P.spectral = 0; P.wl_uniform = 0; P.equiangular = 0; P.light_tree = 0;
     const int W = P.width, H = P.height;
     double* rows = (double*)calloc((size_t)H * ADJ_N, sizeof(double));
     v3 F = lv(par->cam_forward), R = lv(par->cam_right), U = lv(par->cam_up), O = lv(par->cam_position);

@@ -18,7 +18,7 @@ DECLARED = ["pt_last_error", "pt_ctx_create", "pt_ctx_destroy", "pt_scene_upload
             "pt_batch_render", "pt_batch_read_export", "pt_batch_autotune",
             "pt_program_from_description", "pt_renderer_load_description", "pt_env_total_weight",
             "pt_obj_parse", "pt_mesh_data_info", "pt_mesh_data_copy", "pt_mesh_data_free", "pt_inst_parse",
-            "pt_render_frames_noalloc", "pt_bvh_build", "pt_mesh_bvh_build", "pt_plan_data_layout",
+            "pt_render_frames_noalloc", "pt_bvh_build", "pt_light_tree_build", "pt_mesh_bvh_build", "pt_plan_data_layout",
             "pt_pack_mesh_channels", "pt_ctx_reset", "pt_scene_update_lights", "pt_scene_update_env",
             "pt_env_table_upload"]
 # every symbol declared in include/ptmi_debug.h (test and bench instrumentation)
@@ -102,6 +102,7 @@ def load():
     lib.pt_mesh_data_free.restype = None
     lib.pt_inst_parse.argtypes = [C.c_char_p, C.c_size_t, C.POINTER(abi.PtInstTable)]
     lib.pt_bvh_build.argtypes = [vp, i32, i32, u32, vp, vp, C.POINTER(i32), C.POINTER(i32)]
+    lib.pt_light_tree_build.argtypes = [vp, vp, i32, vp, vp, C.POINTER(i32), C.POINTER(i32)]
     lib.pt_mesh_bvh_build.argtypes = [vp, i32, vp, i32, vp, vp, C.POINTER(i32), C.POINTER(i32)]
     lib.pt_plan_data_layout.argtypes = [vp, vp]
     lib.pt_pack_mesh_channels.argtypes = [vp, i32, vp, vp, vp, vp, vp, vp, vp]

@@ -7,7 +7,7 @@ import ctypes as C
 
 import numpy as np
 
-ABI_VERSION = 13
+ABI_VERSION = 14
 
 OBJ_MESH, OBJ_QUAD, OBJ_SPHERE, OBJ_INSTANCES = 0, 1, 2, 3
 MAT_LAMBERT, MAT_DIELECTRIC, MAT_GGX, MAT_NULL, MAT_MIRROR, MAT_ROUGH_DIELECTRIC = 0, 1, 2, 3, 4, 5
@@ -19,7 +19,7 @@ ENV_NONE, ENV_SKY, ENV_CONSTANT = 0, 1, 2
 SKY_SUN, SKY_BIG_SUN, SKY_CAP, SKY_WHITE = 0, 1, 2, 3
 AOV_ALBEDO, AOV_NORMAL = 1, 2
 CHART_EQUIRECT, CHART_OCTAHEDRAL = 0, 1
-SEL_POWER, SEL_UNIFORM = 0, 1
+SEL_POWER, SEL_UNIFORM, SEL_BVH = 0, 1, 2
 CAM_PINHOLE, CAM_ORTHOGRAPHIC, CAM_THINLENS, CAM_FISHEYE, CAM_CYLINDRICAL, CAM_EQUIRECT = 0, 1, 2, 3, 4, 5
 FISHEYE_PROJECTIONS = {"equidistant": 0, "equisolid": 1, "stereographic": 2, "orthographic": 3}
 EST_PT, EST_NEE, EST_MIS = 0, 1, 2
@@ -93,7 +93,8 @@ class PtSceneDesc(C.Structure):
                 ("n_proto_tris", C.c_int32), ("n_instances", C.c_int32),
                 ("proto_vertices", C.c_void_p), ("instances", C.c_void_p),
                 ("tri_normals", C.c_void_p), ("proto_normals", C.c_void_p),
-                ("spectral_atlas", C.c_void_p), ("spectral_z_nodes", C.c_void_p)]
+                ("spectral_atlas", C.c_void_p), ("spectral_z_nodes", C.c_void_p),
+                ("light_boxes", C.c_void_p), ("light_powers", C.c_void_p)]
 
 
 class PtInstTable(C.Structure):
@@ -172,7 +173,8 @@ class SceneArrays:
 
     def __init__(self, tri_vertices, tri_object, objects, materials, lights, env=None, ambient_material=-1,
                  spectral_curves=None, xyz_to_rgb=None, cie_y_integral=0.0, proto_vertices=None, instances=None,
-                 tri_normals=None, proto_normals=None, spectral_atlas=None, spectral_z_nodes=None):
+                 tri_normals=None, proto_normals=None, spectral_atlas=None, spectral_z_nodes=None,
+                 light_boxes=None, light_powers=None):
         self.tri_vertices = np.ascontiguousarray(tri_vertices, dtype=np.float32).reshape(-1, 9)
         self.tri_object = np.ascontiguousarray(tri_object, dtype=np.int32).reshape(-1)
         self.objects = np.ascontiguousarray(objects, dtype=OBJECT_DTYPE)
@@ -194,6 +196,8 @@ class SceneArrays:
                                else np.ascontiguousarray(spectral_atlas, np.float32).reshape(3 * 64 ** 3, 4))
         self.spectral_z_nodes = (None if spectral_z_nodes is None
                                  else np.ascontiguousarray(spectral_z_nodes, np.float32).reshape(64))
+        self.light_boxes = None if light_boxes is None else np.ascontiguousarray(light_boxes, np.float64).reshape(-1, 6)
+        self.light_powers = None if light_powers is None else np.ascontiguousarray(light_powers, np.float64).reshape(-1)
         assert self.tri_object.shape[0] == self.tri_vertices.shape[0]
         assert self.tri_normals is None or self.tri_normals.shape == self.tri_vertices.shape
         assert self.proto_normals is None or self.proto_normals.shape == self.proto_vertices.shape
@@ -228,6 +232,8 @@ class SceneArrays:
         d.proto_normals = _ptr(self.proto_normals)
         d.spectral_atlas = _ptr(self.spectral_atlas)
         d.spectral_z_nodes = _ptr(self.spectral_z_nodes)
+        d.light_boxes = _ptr(self.light_boxes)
+        d.light_powers = _ptr(self.light_powers)
         return d
 
 

@@ -46,7 +46,7 @@ class RenderStrategy:
     aovs: tuple = ()                      # measurement.aovs: first-hit guides 'albedo' / 'normal' (RGB)
     volume_sampling: str = "analytic"     # estimator.volumeSampling: 'analytic' | 'delta-tracking'
     scattering: str = "full"              # measurement.scattering: 'full' | 'ignored' (absorbing-only media)
-    light_selection: str = "power"        # estimator.lightSelection: 'power' | 'uniform'
+    light_selection: str = "power"        # estimator.lightSelection: 'power' | 'uniform' | 'bvh'
     accumulation: str = "average"         # estimator.accumulation.type: 'average' | 'oneshot'
 
 
@@ -214,6 +214,8 @@ def env_select_probability(scene, env, light_powers, selection="power", total_we
     the mass of the table the library bakes, unless a registered table's is given."""
     if selection == "uniform":
         return min(0.99, max(0.01, 1.0 / (len(light_powers) + 1)))
+    if selection == "bvh":          # the environment against the whole tree (lighting.ts:477-493)
+        return 0.5
     if env.type == abi.ENV_CONSTANT:
         mean_l = (env.color[0] + env.color[1] + env.color[2]) / 3.0
     else:
@@ -270,9 +272,11 @@ def _analyze_validate(scene, strategy):
             raise CompileError("mediumLightSampling 'equiangular' does not combine with an environment light")
     # volumeSampling axis (Validator.ts:497-615): a constant-emission SCATTERING medium routes to
     # the null-collision (delta-tracking) arm; everything else stays on the analytic arms
-    if strategy.light_selection not in ("power", "uniform"):
-        raise CompileError(f"estimator.lightSelection '{strategy.light_selection}' not supported — use 'power' or "
-                           "'uniform'")
+    if strategy.light_selection not in ("power", "uniform", "bvh"):
+        raise CompileError(f"estimator.lightSelection '{strategy.light_selection}' not supported — use 'power', "
+                           "'uniform' or 'bvh'")
+    if strategy.light_selection == "bvh" and strategy.medium_light_sampling == "equiangular":
+        raise CompileError("mediumLightSampling 'equiangular' with lightSelection 'bvh' is not supported by this backend")
     if strategy.scattering not in ("full", "ignored"):
         raise CompileError(f"unknown measurement.scattering '{strategy.scattering}'")
     if strategy.volume_sampling not in ("analytic", "delta-tracking"):
@@ -580,6 +584,7 @@ def compile_scene(scene, strategy=None, width=256, height=256, reset_salt=1234):
     # Lights: an area light is a thin emissive quad region plus a light record
     # (lights/quad/quad.ts: region + derived normal/area; lighting.ts power CDF).
     lights = []
+    light_tree_in = []     # per light: its tree box (6 f64) and selection power (sceneData.ts:99-143)
     for l in scene.get("lights", []):
         rec = np.zeros(1, abi.LIGHT_DTYPE)[0]
         rad_value = l.get("radiance", l.get("intensity", [1, 1, 1]))
@@ -607,12 +612,18 @@ def compile_scene(scene, strategy=None, width=256, height=256, reset_salt=1234):
             rec["normal"][:3] = nrm
             rec["area"] = area
             power = max(1e-8, math.pi * area * ((float(rad[0]) + float(rad[1])) + float(rad[2])) / 3.0)
+            # treeBounds (lights/quad/quad.ts:66-78): the four corners
+            c0 = np.asarray(l["corner"], np.float64)
+            corners = np.array([c0 + s1 * e1 + s2 * e2 for s1, s2 in ((0, 0), (1, 0), (0, 1), (1, 1))])
+            tree_box = list(corners.min(0)) + list(corners.max(0))
         else:
             rec["kind"] = abi.LIGHT_POINT
             rec["region"] = -1
             rec["corner"][:3] = l["position"]
             power = max(1e-8, 4.0 * math.pi * ((float(rad[0]) + float(rad[1])) + float(rad[2])) / 3.0)
+            tree_box = [float(v) for v in l["position"]] * 2       # treeBounds: the position (point.ts:30-33)
         lights.append((rec, power))
+        light_tree_in.append((tree_box, power))
     light_recs = []
     if lights:
         pdfs, cdf = select_pdfs([p for _, p in lights], strategy.light_selection)
@@ -665,7 +676,9 @@ def compile_scene(scene, strategy=None, width=256, height=256, reset_salt=1234):
                              np.concatenate(nrm_chunks) if any(r["smooth"] and r["kind"] == abi.OBJ_MESH for r in objects) else None,
                              np.concatenate(proto_nrm_chunks) if any(r["smooth"] and r["kind"] == abi.OBJ_INSTANCES
                                                                      for r in objects) else None,
-                             spectral.atlas() if atlas else None, spectral.z_nodes() if atlas else None)
+                             spectral.atlas() if atlas else None, spectral.z_nodes() if atlas else None,
+                             np.array([b for b, _ in light_tree_in], np.float64) if light_tree_in else None,
+                             np.array([p for _, p in light_tree_in], np.float64) if light_tree_in else None)
 
     prog = abi.PtProgram()
     prog.abi_version = abi.ABI_VERSION
@@ -684,13 +697,13 @@ def compile_scene(scene, strategy=None, width=256, height=256, reset_salt=1234):
     prog.width, prog.height = width, height
     prog.frames_in_flight = max(1, strategy.frames_in_flight)
     prog.max_null_crossings = strategy.max_null_crossings
-    prog.light_selection = abi.SEL_UNIFORM if strategy.light_selection == "uniform" else abi.SEL_POWER
+    prog.light_selection = {"power": abi.SEL_POWER, "uniform": abi.SEL_UNIFORM, "bvh": abi.SEL_BVH}[strategy.light_selection]
     warnings = []
     if strategy.volume_sampling == "delta-tracking" and not any(int(r["medium_arm"]) == abi.MED_DELTA and r["has_medium"]
                                                                  for r in materials):
         warnings.append("volumeSampling 'delta-tracking' controls nothing here — no medium is emissive and "
                         "scattering; constant media stay on the analytic arms")
-    sel = "-uniform" if strategy.light_selection == "uniform" else ""
+    sel = "" if strategy.light_selection == "power" else "-" + strategy.light_selection
     return CompiledRenderer(id=f"{strategy.color}-{strategy.direct_lighting}{sel}-{scene.get('id', 'scene')}", scene=arrays,
                             program=prog, width=width, height=height, camera=scene["camera"],
                             reset_salt=reset_salt, warnings=warnings, material_names=list(mat_names),

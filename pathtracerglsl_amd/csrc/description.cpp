@@ -228,6 +228,7 @@ bool lower(const JVal& d, int32_t width, int32_t height, int32_t fif, pt_program
         else return L.unsupported("estimator.lighting.method", quoted(method));
         const JVal* sel = lighting->get("selection");
         if (sel && is_str(sel, "uniform")) pg.light_selection = PT_SEL_UNIFORM;
+        else if (sel && is_str(sel, "bvh")) pg.light_selection = PT_SEL_BVH;
         else if (sel && !is_str(sel, "power")) return L.unsupported("estimator.lighting.selection", quoted(sel));
     }
     const JVal* rr = e->get("russianRoulette");
@@ -277,8 +278,9 @@ bool lower(const JVal& d, int32_t width, int32_t height, int32_t fif, pt_program
                       is_str(&md, "mirror") || is_str(&md, "rough_dielectric")))
                     return L.unsupported("material model", quoted(&md));
         // materials.twoSidedShading is read only by the light tree's horizon cull
-        // (accel/light_tree/light_tree.glsl, through LightQuery.two_sided); the 'power' selection
-        // this backend lowers ignores the query, so the flag changes no estimate here.
+        // (accel/light_tree/light_tree.glsl, through LightQuery.two_sided); the kernels answer
+        // material_two_sided per material (the rough dielectric: sphere support, non-delta), which
+        // is what the flag folds away for scenes without such a material.
         if (truthy(mats->get("materialsReadUv"))) return L.unsupported("materials.materialsReadUv", "true (textures)");
         // materials.albedo marks the material_albedo dispatch an albedo AOV needs (plan/types.ts:246-248);
         // the albedo guide itself is lowered from measurement.aovs, so the flag adds nothing here.

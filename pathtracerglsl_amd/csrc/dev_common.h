@@ -174,6 +174,10 @@ struct DevScene {
     // a spectral program whose procedural sky body has colour (null otherwise)
     const float4* spec_atlas;
     const float* spec_znodes;
+    // the light tree over the lights (2 float4 per node: (min, phi) (max, link)) and each light's
+    // trail (low, high 24-bit halves); null without the tree inputs
+    const float4* lt_nodes;
+    const uint2* lt_trails;
 };
 
 // Wave-aggregated queue append: every active lane calls it (convergent), lanes with

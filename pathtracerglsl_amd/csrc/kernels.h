@@ -180,6 +180,7 @@ PT_DEV void raygen_path(const Paths& P, const Camera& C, int W, int H, int i, in
     for (int a = 0; a < 2; a++)
         if (P.aov[a]) P.aov[a][i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);   // 0 unless a surface is reached
     P.prev[i] = f4(org, 0.0f);
+    if (P.prev_qn) P.prev_qn[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);   // kernel_init's prev_query: no normal
     P.flags[i] = mk_flags(true, 0, 0);
     if (media) P.medium[i] = qcount(P, C_CAM_MEDIUM);
     P.q[0][i] = i;
@@ -736,6 +737,7 @@ __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene Sk, Paths P, int qsel
             const Wl wl = load_wl<SPEC>(P, i);
             float4 pv = P.prev[i];
             f3 prev_qp = xyz(pv); float prev_pdf = pv.w;
+            const bool tree = P.prev_qn != nullptr;     // light-tree selection
             const int fl = P.flags[i];
             bool prev_delta = (fl & 1) != 0;
             bool collapsed = SPEC && (fl & 2) != 0;
@@ -774,7 +776,8 @@ __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene Sk, Paths P, int qsel
                     Rng rng; rng.base = rs.x; rng.dim = rs.y;
                     if (NEE && !equiangular) {   // light_sample_direct_medium
                         float y0 = rng.next(), y1 = rng.next();
-                        LSample ls = lighting_sample<SPEC>(S, p_evt, y0, y1, wl);
+                        const LQuery qm{p_evt, mk3(0.0f, 0.0f, 0.0f), false};   // light_query_medium
+                        LSample ls = lighting_sample<SPEC>(S, qm, y0, y1, wl, rng, tree);
                         if (ls.pdf > 0.0f) {
                             s_l = p_evt + ls.wi * ls.distance;
                             s_o = p_evt; s_d = ls.wi; s_cross = crossings_left;
@@ -815,6 +818,7 @@ __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene Sk, Paths P, int qsel
                         P.ray_d[i] = f4(wi, 0.0f);
                         store_thr<SPEC>(P, i, T, eta_scale);
                         P.prev[i] = f4(p_evt, ppdf);
+                        if (tree) P.prev_qn[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                         P.flags[i] = mk_flags(false, nulls, bounce + 1, collapsed);
                     }
                     P.rng[i] = make_uint2(rng.base, rng.dim);
@@ -875,7 +879,11 @@ __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene Sk, Paths P, int qsel
                             if (NEE && emitters) {
                                 int lid = light_of(S, h.region_to);
                                 if (lid < 0 || prev_delta) w = 1.0f;
-                                else if (MIS) w = power_heuristic(prev_pdf, lighting_pdf(S, prev_qp, rd, lid, h.p));
+                                else if (MIS) {
+                                    LQuery pq{prev_qp, mk3(0.0f, 0.0f, 0.0f), false};
+                                    if (tree) { const float4 qn = P.prev_qn[i]; pq.n = xyz(qn); pq.two_sided = qn.w != 0.0f; }
+                                    w = power_heuristic(prev_pdf, lighting_pdf(S, pq, rd, lid, h.p, tree));
+                                }
                                 else w = 0.0f;
                             }
                             const sp4 c = (T * w) * lift<SPEC>(S, E.emission, E.s_emission, wl);
@@ -894,7 +902,8 @@ __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene Sk, Paths P, int qsel
                         // light_sample_direct (skipped for delta-only materials)
                         if (NEE && CAT != CAT_DIELECTRIC && CAT != CAT_MIRROR) {   // delta lobes take no NEE
                             float x0 = rng.next(), x1 = rng.next();
-                            LSample ls = lighting_sample<SPEC>(S, h.p, x0, x1, wl);
+                            const LQuery qs{h.p, h.frame.n, MODEL == PT_MAT_ROUGH_DIELECTRIC};   // light_query_surface
+                            LSample ls = lighting_sample<SPEC>(S, qs, x0, x1, wl, rng, tree);
                             if (ls.pdf > 0.0f) {
                                 f3 light_p = h.p + ls.wi * ls.distance;
                                 float side = dot(ls.wi, h.ng) >= 0.0f ? 1.0f : -1.0f;
@@ -951,6 +960,7 @@ __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene Sk, Paths P, int qsel
                                 P.ray_d[i] = f4(bs.wi, 0.0f);
                                 store_thr<SPEC>(P, i, T, eta_scale);
                                 P.prev[i] = f4(prev_qp, prev_pdf);
+                                if (tree) P.prev_qn[i] = f4(h.frame.n, MODEL == PT_MAT_ROUGH_DIELECTRIC ? 1.0f : 0.0f);
                                 P.flags[i] = mk_flags(prev_delta, nulls, bounce + 1, collapsed);
                                 P.rng[i] = make_uint2(rng.base, rng.dim);
                                 if (MEDIA) P.medium[i] = medium;

@@ -37,6 +37,7 @@ struct Paths {
     float4* rad;     // radiance.xyz (RGB) | 4 lanes (spectral)
     uint2* rng;      // rng_base, rng_dim
     float4* prev;    // prev_query.p.xyz, prev_bsdf_pdf
+    float4* prev_qn; // prev_query.n.xyz, two_sided (1 / 0): light-tree programs only, null otherwise
     int* flags;      // bit 0 prev_was_delta, bits 8..15 null crossings, bits 16..31 bounce
     int* medium;     // current_medium region (media scenes)
     float2* hit;     // t (medium event t after binning), primitive id (bits)

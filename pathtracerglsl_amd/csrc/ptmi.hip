@@ -386,6 +386,8 @@ static pt_status validate_scene(const pt_scene_desc* sd) {
         return fail(PT_E_INVALID, "pt_env.body must be one of PT_SKY_*");
     if ((sd->spectral_atlas == nullptr) != (sd->spectral_z_nodes == nullptr))
         return fail(PT_E_INVALID, "pt_scene_desc: spectral_atlas and spectral_z_nodes come together");
+    if ((sd->light_boxes == nullptr) != (sd->light_powers == nullptr))
+        return fail(PT_E_INVALID, "pt_scene_desc: light_boxes and light_powers come together");
     return PT_OK;
 }
 
@@ -561,6 +563,17 @@ static pt_status build_scene(const pt_scene_desc* sd, hipStream_t stream, SceneS
         if ((st = SS.upload((const float4*)sd->spectral_curves, 95, &D.spec_curves)) != PT_OK) return st;
         for (int k = 0; k < 9; k++) D.xyz_rgb[k] = sd->xyz_to_rgb[k];
         D.cie_y = sd->cie_y_integral;
+    }
+    if (sd->light_boxes && sd->n_lights > 0) {
+        // the light tree (sceneData.ts:99-143 -> buildLightTree), built on the host
+        std::vector<float> lt_nodes((size_t)8 * (2 * sd->n_lights - 1)), lt_trails((size_t)4 * sd->n_lights);
+        int32_t nc = 0;
+        if ((st = pt_light_tree_build(sd->light_boxes, sd->light_powers, sd->n_lights, lt_nodes.data(), lt_trails.data(),
+                                      &nc, nullptr)) != PT_OK) return st;
+        std::vector<uint2> trails((size_t)sd->n_lights);
+        for (int l = 0; l < sd->n_lights; l++) trails[l] = make_uint2((uint32_t)lt_trails[4 * (size_t)l], (uint32_t)lt_trails[4 * (size_t)l + 1]);
+        if ((st = SS.upload((const float4*)lt_nodes.data(), (size_t)2 * nc, &D.lt_nodes)) != PT_OK) return st;
+        if ((st = SS.upload(trails.data(), trails.size(), &D.lt_trails)) != PT_OK) return st;
     }
     if (sd->spectral_atlas) {
         if ((st = SS.upload((const float4*)sd->spectral_atlas, (size_t)3 * 64 * 64 * 64, &D.spec_atlas)) != PT_OK) return st;
@@ -832,8 +845,18 @@ static bool holds_uniform_selection(const DevScene& dev, const std::vector<pt_li
 // envSampler.chart must sample the chart the sky table was built on, and a program with
 // uniform light selection needs the scene's selection table to be the uniform one.
 static pt_status check_program_scene(const pt_program& prog, const DevScene& dev, const std::vector<pt_light>& lights) {
-    if (prog.light_selection != PT_SEL_POWER && prog.light_selection != PT_SEL_UNIFORM)
+    if (prog.light_selection != PT_SEL_POWER && prog.light_selection != PT_SEL_UNIFORM && prog.light_selection != PT_SEL_BVH)
         return fail(PT_E_INVALID, "unknown light selection");
+    if (prog.light_selection == PT_SEL_BVH) {
+        if (!lights.empty() && !dev.lt_nodes)
+            return fail(PT_E_MISSING_INPUT, "missing input: light_boxes / light_powers (the light tree's inputs in pt_scene_desc)");
+        const bool env_live = (dev.env_type == PT_ENV_SKY || (dev.env_type == PT_ENV_CONSTANT && dev.env_samplable)) && !lights.empty();
+        if (env_live && dev.env_select_prob != 0.5f)
+            return fail(PT_E_INVALID, "the program selects lights through the light tree but the scene's pt_env.select_prob "
+                                      "is not the tree's 1/2");
+        if (prog.medium_light_sampling == PT_MLS_EQUIANGULAR)
+            return fail(PT_E_UNSUPPORTED, "mediumLightSampling 'equiangular' with the light tree is not provided by this build");
+    }
     if (prog.light_selection == PT_SEL_UNIFORM && !holds_uniform_selection(dev, lights))
         return fail(PT_E_INVALID, "the program selects lights uniformly but the scene's selection table (pt_light.select_pdf / "
                                   "cdf, pt_env.select_prob) is not the uniform one");
@@ -931,6 +954,7 @@ static size_t carve_paths(const pt_program& prog, char* base, long long n, Paths
     P.wl = (float4*)take(16 * m); P.eta = (float*)take(4 * m); P.path_scene = (int*)take(4 * m);
     for (int c = 0; c < N_CAT; c++) P.catq[c] = (int*)take(4 * m);       // one queue per shading category
     for (int a = 0; a < 2; a++) P.aov[a] = (prog.aovs & (1 << a)) ? (float4*)take(16 * m) : nullptr;
+    P.prev_qn = prog.light_selection == PT_SEL_BVH ? (float4*)take(16 * m) : nullptr;   // the stored query's normal
     P.cap = (int)n;
     P.npix = prog.width * prog.height;
     return off;
@@ -1001,6 +1025,8 @@ static pt_status run_waves(pt_renderer* r, const pt_params* prm, uint32_t nframe
     if (max_frames > 0) F = std::min(F, max_frames);
     const bool spectral = pg.color == PT_COLOR_SPECTRAL;
     if (pt_status sc = check_program_scene(pg, ctx->dev, ctx->host_lights); sc != PT_OK) return sc;
+    if (adj && pg.light_selection == PT_SEL_BVH)
+        return fail(PT_E_UNSUPPORTED, "the adjoint replays power or uniform light selection only (not the light tree)");
     if (spectral && !ctx->dev.spec_curves)
         return fail(PT_E_MISSING_INPUT, "missing input: spectral_curves (the spectral program needs the CIE/D65 curves in pt_scene_desc)");
     if (clear_stats) {
@@ -1112,6 +1138,8 @@ pt_status pt_backward(pt_renderer* r, const pt_params* prm, uint32_t nframes, co
     if (!ctx->has_scene) return fail(PT_E_MISSING_INPUT, "missing input: no scene uploaded (pt_scene_upload)");
     const pt_program& pg = r->prog;
     if (pg.color != PT_COLOR_RGB) return fail(PT_E_UNSUPPORTED, "pt_backward: the adjoint runs RGB programs only");
+    if (pg.light_selection == PT_SEL_BVH)
+        return fail(PT_E_UNSUPPORTED, "pt_backward: the adjoint replays power or uniform light selection only (not the light tree)");
     if (pt_status sc = check_camera(prm); sc != PT_OK) return sc;
     if (pg.medium_light_sampling != PT_MLS_VERTEX) return fail(PT_E_UNSUPPORTED, "pt_backward: the adjoint runs vertex medium NEE only");
     for (const pt_material& m : ctx->host_materials)
@@ -1256,6 +1284,7 @@ pt_status pt_batch_create(pt_ctx* ctx, int32_t n, const pt_scene_desc* scenes, c
         if (p.estimator != p0.estimator || p.max_bounces != p0.max_bounces || p.rr_start != p0.rr_start || p.rr_max_survival != p0.rr_max_survival)
             return fail(PT_E_UNSUPPORTED, "pt_batch_create: every program must share estimator, maxBounces and Russian roulette");
         if (p.color != PT_COLOR_RGB) return fail(PT_E_UNSUPPORTED, "pt_batch_create: batches run RGB programs");
+        if (p.light_selection == PT_SEL_BVH) return fail(PT_E_UNSUPPORTED, "pt_batch_create: batches select lights by power or uniformly");
         if (p.width <= 0 || p.height <= 0) return fail(PT_E_INVALID, "pt_batch_create: bad film size");
         pt_status st = validate_scene(&scenes[s]);
         if (st != PT_OK) return st;

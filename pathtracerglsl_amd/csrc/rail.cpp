@@ -18,17 +18,23 @@ namespace {
 
 constexpr int BINS = 12;   // BVH_BINS (accel/bvh/bvh.ts:37)
 
-// buildBVHCore (accel/bvh/bvh.ts:116-320), count-weighted (Sigma 1 is exact in f64, so the
-// weighted cost is the area x count cost), emitting the flat node format of buildBVHNodesFlat
-// (:323-360).
+// buildBVHCore (accel/bvh/bvh.ts:132-320): count-weighted for the ray trees (Sigma 1 is exact in
+// f64, so the weighted cost is the area x count cost), emitting the flat node format of
+// buildBVHNodesFlat (:344-383); power-weighted with leaf size 1, always split, the 48-level
+// depth cap, the sorted-median fallback and the bit trails for the light tree
+// (accel/light_tree/light_tree.ts:46-77), emitting its (min, phi) (max, link) nodes.
 struct Core {
     const double* boxes;
+    const double* weights = nullptr;  // per-item split-cost weight (null: the count)
     int leaf_size;
     bool always_split, sorted_median;
+    int depth_cap = -1;               // < 0: none
+    float* trails = nullptr;          // light tree: per item (lo 24 bits, hi 24 bits, 0, 0)
     std::vector<float> c[3];          // f32 centroid stores
     std::vector<uint32_t> order;
     std::vector<float>* nodes;
     int node_count = 0, max_depth = 0;
+    static int ceil_log2(int c) { return c <= 1 ? 0 : 32 - __builtin_clz((unsigned)(c - 1)); }
 
     static void reset(double* b) { b[0] = b[1] = b[2] = INFINITY; b[3] = b[4] = b[5] = -INFINITY; }
     static void grow(double* b, const double* s) {
@@ -48,11 +54,24 @@ struct Core {
         nd[4] = (float)b[3]; nd[5] = (float)b[4]; nd[6] = (float)b[5]; nd[7] = (float)B;
     }
 
-    int emit(int start, int end, int depth) {
+    // The leaf over order[start, end): the ray trees' (count, start), or the light tree's
+    // (phi, -(light + 1)) with the light's trail.
+    int leaf(int ni, const double* bounds, int start, int count, double w, uint32_t lo, uint32_t hi) {
+        if (!trails) { set(ni, bounds, count, start); return ni; }
+        const uint32_t light = order[start];
+        set(ni, bounds, w, -(double)light - 1.0);
+        trails[4 * (size_t)light] = (float)lo;
+        trails[4 * (size_t)light + 1] = (float)hi;
+        return ni;
+    }
+
+    // lo / hi: the trail down to this node, one bit per level, LSB first, 24 per half
+    int emit(int start, int end, int depth, uint32_t lo = 0, uint32_t hi = 0) {
         if (depth > max_depth) max_depth = depth;
         const int ni = node_count++;
         double bounds[6], cb[6];
         reset(bounds); reset(cb);
+        double w = 0;
         for (int i = start; i < end; i++) {
             const uint32_t t = order[i];
             const double* bx = boxes + 6 * (size_t)t;
@@ -63,23 +82,28 @@ struct Core {
                 if (cc < cb[a]) cb[a] = cc;
                 if (cc > cb[3 + a]) cb[3 + a] = cc;
             }
+            if (weights) w += weights[t];
         }
         const int count = end - start;
-        if (count <= leaf_size) { set(ni, bounds, count, start); return ni; }
+        if (count <= leaf_size) return leaf(ni, bounds, start, count, w, lo, hi);
+        // the depth cap: once depth + ceil(log2 count) reaches it, only balanced splits
+        const bool balanced_only = depth_cap >= 0 && depth + ceil_log2(count) >= depth_cap;
 
         // binned SAH over every axis with a centroid extent >= 1e-12: per-axis prefix and
-        // suffix sweeps, f32 area stores, axes then bins ascending, strict < (the tie order)
+        // suffix sweeps, f32 area stores, axes then bins ascending, strict < (the tie order);
+        // cost = A_L w_L + A_R w_R with w the item count or the summed weights
         int best_axis = -1, best_split = -1;
         double best_cost = INFINITY;
         bool valid[3];
         for (int a = 0; a < 3; a++) valid[a] = cb[3 + a] - cb[a] >= 1e-12;
-        if (valid[0] || valid[1] || valid[2]) {
+        if (!balanced_only && (valid[0] || valid[1] || valid[2])) {
             double bin_box[3][BINS][6];
-            double bin_cnt[3][BINS];
+            int bin_cnt[3][BINS];
+            double bin_w[3][BINS];
             double scale[3];
             for (int a = 0; a < 3; a++) {
                 scale[a] = valid[a] ? BINS / (cb[3 + a] - cb[a]) : 0;
-                for (int b = 0; b < BINS; b++) { reset(bin_box[a][b]); bin_cnt[a][b] = 0; }
+                for (int b = 0; b < BINS; b++) { reset(bin_box[a][b]); bin_cnt[a][b] = 0; bin_w[a][b] = 0; }
             }
             for (int i = start; i < end; i++) {
                 const uint32_t t = order[i];
@@ -90,35 +114,38 @@ struct Core {
                     if (b < 0) b = 0;
                     if (b >= BINS) b = BINS - 1;
                     bin_cnt[a][b] += 1;
+                    bin_w[a][b] += weights ? weights[t] : 1;
                     grow(bin_box[a][b], bx);
                 }
             }
             for (int a = 0; a < 3; a++) {
                 if (!valid[a]) continue;
                 float left_area[BINS - 1], right_area[BINS - 1];
-                double left_cnt[BINS - 1], right_cnt[BINS - 1];
+                int left_cnt[BINS - 1], right_cnt[BINS - 1];
+                double left_w[BINS - 1], right_w[BINS - 1];
                 double acc[6];
-                double n = 0;
+                int n = 0;
+                double wa = 0;
                 reset(acc);
                 for (int b = 0; b < BINS - 1; b++) {
-                    grow(acc, bin_box[a][b]); n += bin_cnt[a][b];
-                    left_area[b] = (float)area(acc); left_cnt[b] = n;
+                    grow(acc, bin_box[a][b]); n += bin_cnt[a][b]; wa += bin_w[a][b];
+                    left_area[b] = (float)area(acc); left_cnt[b] = n; left_w[b] = wa;
                 }
-                reset(acc); n = 0;
+                reset(acc); n = 0; wa = 0;
                 for (int b = BINS - 1; b > 0; b--) {
-                    grow(acc, bin_box[a][b]); n += bin_cnt[a][b];
-                    right_area[b - 1] = (float)area(acc); right_cnt[b - 1] = n;
+                    grow(acc, bin_box[a][b]); n += bin_cnt[a][b]; wa += bin_w[a][b];
+                    right_area[b - 1] = (float)area(acc); right_cnt[b - 1] = n; right_w[b - 1] = wa;
                 }
                 for (int b = 0; b < BINS - 1; b++) {
                     if (left_cnt[b] == 0 || right_cnt[b] == 0) continue;
-                    const double cost = (double)left_area[b] * left_cnt[b] + (double)right_area[b] * right_cnt[b];
+                    const double cost = (double)left_area[b] * left_w[b] + (double)right_area[b] * right_w[b];
                     if (cost < best_cost) { best_cost = cost; best_axis = a; best_split = b; }
                 }
             }
         }
         if (!always_split) {
-            const double leaf_cost = area(bounds) * count;
-            if (best_axis == -1 || best_cost >= leaf_cost) { set(ni, bounds, count, start); return ni; }
+            const double leaf_cost = area(bounds) * (weights ? w : count);
+            if (best_axis == -1 || best_cost >= leaf_cost) return leaf(ni, bounds, start, count, w, lo, hi);
         }
         int mid = -1;
         if (best_axis != -1) {
@@ -149,10 +176,12 @@ struct Core {
         }
         double keep[6];
         std::memcpy(keep, bounds, sizeof keep);
-        emit(start, mid, depth + 1);                 // left child at ni + 1
-        const int right = emit(mid, end, depth + 1);
-        // a fallback split has no SAH axis: the node stores axis 0 (-A - 1 = -1)
-        set(ni, keep, -1 - std::max(best_axis, 0), right);
+        const uint32_t rlo = depth < 24 ? lo | (1u << depth) : lo;
+        const uint32_t rhi = depth < 24 ? hi : hi | (1u << (depth - 24));
+        emit(start, mid, depth + 1, lo, hi);                 // left child at ni + 1: trail bit 0
+        const int right = emit(mid, end, depth + 1, rlo, rhi);   // right: bit 1 at this level
+        if (trails) set(ni, keep, w, right);                 // the light tree's (phi, right child)
+        else set(ni, keep, -1 - std::max(best_axis, 0), right);   // a fallback split stores axis 0
         return ni;
     }
 };
@@ -181,6 +210,30 @@ void build_bvh_flat(const double* boxes, int n, int leaf_size, bool always_split
     out.order = std::move(C.order);
     out.node_count = C.node_count;
     out.max_depth = C.max_depth;
+}
+
+void build_light_tree(const double* boxes, const double* powers, int n, std::vector<float>& nodes,
+                      std::vector<float>& trails, int& node_count, int& max_depth) {
+    nodes.assign(n > 0 ? 8 * (size_t)(2 * n - 1) : 0, 0.0f);
+    trails.assign(4 * (size_t)std::max(n, 0), 0.0f);
+    node_count = max_depth = 0;
+    if (n <= 0) return;
+    Core C;
+    C.boxes = boxes; C.weights = powers; C.leaf_size = 1; C.always_split = true; C.sorted_median = true;
+    C.depth_cap = LIGHT_TREE_MAX_DEPTH; C.trails = trails.data();
+    for (int a = 0; a < 3; a++) C.c[a].resize((size_t)n);
+    for (int t = 0; t < n; t++) {
+        const double* b = boxes + 6 * (size_t)t;
+        C.c[0][t] = (float)((b[0] + b[3]) * 0.5);
+        C.c[1][t] = (float)((b[1] + b[4]) * 0.5);
+        C.c[2][t] = (float)((b[2] + b[5]) * 0.5);
+    }
+    C.order.resize((size_t)n);
+    for (int i = 0; i < n; i++) C.order[i] = (uint32_t)i;
+    C.nodes = &nodes;
+    C.emit(0, n, 0);
+    node_count = C.node_count;
+    max_depth = C.max_depth;
 }
 
 void build_mesh_bvh(const float* positions, const uint32_t* indices, int n_tris, FlatBVH& out, std::vector<uint32_t>& reindexed) {
@@ -225,6 +278,31 @@ pt_status pt_bvh_build(const double* boxes, int32_t n, int32_t leaf_size, uint32
     if (n > 0) std::memcpy(order_out, b.order.data(), (size_t)n * sizeof(uint32_t));
     *node_count = b.node_count;
     if (max_depth) *max_depth = b.max_depth;
+    return PT_OK;
+}
+
+pt_status pt_light_tree_build(const double* boxes, const double* powers, int32_t n, float* nodes_out, float* trails_out,
+                              int32_t* node_count, int32_t* max_depth) {
+    if ((n > 0 && (!boxes || !powers || !nodes_out || !trails_out)) || !node_count)
+        return fail(PT_E_INVALID, "pt_light_tree_build: null argument");
+    if (n < 0 || n > (1 << 22)) return fail(PT_E_INVALID, "pt_light_tree_build: n must be in [0, 2^22]");
+    for (int32_t i = 0; i < n; i++) {
+        if (!(powers[i] > 0.0) || !std::isfinite(powers[i]))
+            return fail(PT_E_INVALID, "pt_light_tree_build: light " + std::to_string(i) + " has a power that is not finite and > 0");
+        for (int k = 0; k < 3; k++)
+            if (!std::isfinite(boxes[6 * (size_t)i + k]) || !std::isfinite(boxes[6 * (size_t)i + 3 + k]) ||
+                boxes[6 * (size_t)i + k] > boxes[6 * (size_t)i + 3 + k])
+                return fail(PT_E_INVALID, "pt_light_tree_build: light " + std::to_string(i) + " has a bad box");
+    }
+    std::vector<float> nodes, trails;
+    int nc = 0, md = 0;
+    build_light_tree(boxes, powers, n, nodes, trails, nc, md);
+    if (n > 0) {
+        std::memcpy(nodes_out, nodes.data(), nodes.size() * sizeof(float));
+        std::memcpy(trails_out, trails.data(), trails.size() * sizeof(float));
+    }
+    *node_count = nc;
+    if (max_depth) *max_depth = md;
     return PT_OK;
 }
 

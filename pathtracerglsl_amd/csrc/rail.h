@@ -26,6 +26,12 @@ struct FlatBVH {
 // buildBVHNodesFlat(boxes, n, leafSize, alwaysSplit) with the 'median' fallback, or
 // buildBVHCore's 'sorted-median' fallback when sorted_median is set. boxes: n x 6 f64.
 void build_bvh_flat(const double* boxes, int n, int leaf_size, bool always_split, bool sorted_median, FlatBVH& out);
+// buildLightTree (accel/light_tree/light_tree.ts:46-77): 2n - 1 nodes of 8 floats, (min, phi)
+// (max, link: right child, or -(light + 1) at a leaf), left child implicit at node + 1; trails
+// 4 floats per light (the 24-bit low and high halves of its root-to-leaf bits, LSB first).
+constexpr int LIGHT_TREE_MAX_DEPTH = 48;
+void build_light_tree(const double* boxes, const double* powers, int n, std::vector<float>& nodes,
+                      std::vector<float>& trails, int& node_count, int& max_depth);
 
 // buildBVH(positions, indices): per-triangle f64 boxes, the flat build at leaf size 2, and the
 // triangle indices re-emitted in leaf order (3 per triangle).

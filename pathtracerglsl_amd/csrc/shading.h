@@ -946,14 +946,104 @@ PT_DEV LSample environment_sample(const DevScene& S, float x0, float x1, const W
     ls.flags = 0; ls.light_id = -1;
     return ls;
 }
-// lighting_sample (features/lighting.ts:546-628): env-only, finite-only or two-stage.
+// LightQuery (glsl/core/structs.glsl:83-87; pt.ts:125-140): the receiver point, its shading
+// normal (0 at a medium event) and whether its BSDF support is the sphere and NEE evaluates it
+// (modelTwoSidedShading, materials/index.ts:56-69: the rough dielectric).
+struct LQuery { f3 p, n; bool two_sided; };
+// light_tree_importance (accel/light_tree/light_tree.glsl:28-77): power over the clamped squared
+// distance; with a normal, the box's exact below-horizon cull and the best-case cosine.
+PT_DEV float lt_importance(float4 n0, float4 n1, const LQuery& q) {
+    const f3 bmin = xyz(n0), bmax = xyz(n1);
+    const f3 c = (bmin + bmax) * 0.5f;
+    const f3 e = bmax - bmin;
+    const f3 d = c - q.p;
+    const float d2raw = dot(d, d);
+    const float diag2 = 0.25f * dot(e, e);
+    const float base = n0.w / gmax(d2raw, gmax(diag2, 1e-8f));
+    if (dot(q.n, q.n) < 0.5f) return base;
+    const f3 far_c = mk3(q.n.x < 0.0f ? bmin.x : bmax.x, q.n.y < 0.0f ? bmin.y : bmax.y, q.n.z < 0.0f ? bmin.z : bmax.z);
+    float h = dot(q.n, far_c - q.p);
+    if (q.two_sided) {
+        const f3 near_c = (bmin + bmax) - far_c;
+        h = gmax(h, -dot(q.n, near_c - q.p));
+    } else if (h <= 0.0f) {
+        return 0.0f;
+    }
+    float cos_c = dot(q.n, d) * (1.0f / sqrtf(gmax(d2raw, 1e-12f)));
+    if (q.two_sided) cos_c = fabsf(cos_c);
+    const f3 dmx = mk3(gmax(q.p.x - bmin.x, bmax.x - q.p.x), gmax(q.p.y - bmin.y, bmax.y - q.p.y),
+                       gmax(q.p.z - bmin.z, bmax.z - q.p.z));
+    const float lo = h * (1.0f / sqrtf(gmax(dot(dmx, dmx), 1e-12f)));
+    return base * gmin(1.0f, gmax(cos_c, lo));
+}
+constexpr int LT_MAX_DEPTH = 48;
+// light_tree_pick (lighting.ts:786-800): importance-proportional descent, a fresh random number
+// per level, the product pmf; -1 on a dead descent (both children culled).
+PT_DEV int lt_pick(const DevScene& S, const LQuery& q, float xi, Rng& rng, float& pmf) {
+    pmf = 1.0f;
+    int ni = 0;
+    float4 n1 = S.lt_nodes[1];
+    for (int lvl = 0; lvl < LT_MAX_DEPTH; lvl++) {
+        if (n1.w < 0.0f) break;
+        const int L = ni + 1, R = (int)n1.w;
+        const float4 l0 = S.lt_nodes[2 * L], l1 = S.lt_nodes[2 * L + 1];
+        const float4 r0 = S.lt_nodes[2 * R], r1 = S.lt_nodes[2 * R + 1];
+        const float il = lt_importance(l0, l1, q), ir = lt_importance(r0, r1, q);
+        const float isum = il + ir;
+        if (isum <= 0.0f) { pmf = 0.0f; return -1; }
+        const float p_l = il / isum;
+        if (xi < p_l) { pmf *= p_l; ni = L; n1 = l1; }
+        else { pmf *= (1.0f - p_l); ni = R; n1 = r1; }
+        xi = rng.next();
+    }
+    return (int)(-n1.w - 1.0f);
+}
+// light_tree_pmf (lighting.ts:802-822): the light's bit trail replayed with the stored query.
+PT_DEV float lt_pmf(const DevScene& S, const LQuery& q, int light) {
+    const uint2 trail = S.lt_trails[light];
+    uint32_t tw = trail.x;
+    float pmf = 1.0f;
+    int ni = 0;
+    float4 n1 = S.lt_nodes[1];
+    for (int lvl = 0; lvl < LT_MAX_DEPTH; lvl++) {
+        if (n1.w < 0.0f) break;
+        if (lvl == 24) tw = trail.y;
+        const int L = ni + 1, R = (int)n1.w;
+        const float4 l0 = S.lt_nodes[2 * L], l1 = S.lt_nodes[2 * L + 1];
+        const float4 r0 = S.lt_nodes[2 * R], r1 = S.lt_nodes[2 * R + 1];
+        const float il = lt_importance(l0, l1, q), ir = lt_importance(r0, r1, q);
+        const float isum = il + ir;
+        if (isum <= 0.0f) return 0.0f;
+        const float p_l = il / isum;
+        if ((tw & 1u) == 0u) { pmf *= p_l; ni = L; n1 = l1; }
+        else { pmf *= (1.0f - p_l); ni = R; n1 = r1; }
+        tw >>= 1u;
+    }
+    return pmf;
+}
+// The tree's finite stage (generateLightSamplingBvh, lighting.ts:834-870): pick, the light's sample
+// at (random(), x1), its pdf times the selection pmf.
 template <bool SPEC>
-PT_DEV LSample lighting_sample(const DevScene& S, f3 p, float x0, float x1, const Wl& wl) {
+PT_DEV LSample lighting_sample_tree(const DevScene& S, const LQuery& q, float x0, float x1, Rng& rng, const Wl& wl) {
+    float sel;
+    const int id = lt_pick(S, q, x0, rng, sel);
+    if (id < 0) { LSample ls; ls.pdf = 0.0f; return ls; }
+    const float lx = rng.next();
+    LSample ls = light_sample_one<SPEC>(S, S.lights[id], q.p, lx, x1, wl);
+    ls.light_id = id;
+    ls.pdf *= sel;
+    return ls;
+}
+// lighting_sample (features/lighting.ts:546-628, 834-895): env-only, finite-only or two-stage; the
+// finite stage by the selection CDF or, with tree, the light tree (rng feeds its descent).
+template <bool SPEC>
+PT_DEV LSample lighting_sample(const DevScene& S, const LQuery& q, float x0, float x1, const Wl& wl, Rng& rng, bool tree) {
     if (S.n_lights == 0) {
         if (env_samplable(S)) return environment_sample<SPEC>(S, x0, x1, wl);
         LSample ls; ls.pdf = 0.0f; return ls;
     }
-    if (!env_select_live(S)) return lighting_sample_finite<SPEC>(S, p, x0, x1, wl);
+    if (!env_select_live(S))
+        return tree ? lighting_sample_tree<SPEC>(S, q, x0, x1, rng, wl) : lighting_sample_finite<SPEC>(S, q.p, x0, x1, wl);
     float ps = S.env_select_prob;
     if (x0 < ps) {
         LSample ls = environment_sample<SPEC>(S, gclamp(x0 / ps, 0.0f, 0.9999999f), x1, wl);
@@ -961,7 +1051,7 @@ PT_DEV LSample lighting_sample(const DevScene& S, f3 p, float x0, float x1, cons
         return ls;
     }
     float fx = gclamp((x0 - ps) / (1.0f - ps), 0.0f, 0.9999999f);
-    LSample ls = lighting_sample_finite<SPEC>(S, p, fx, x1, wl);
+    LSample ls = tree ? lighting_sample_tree<SPEC>(S, q, fx, x1, rng, wl) : lighting_sample_finite<SPEC>(S, q.p, fx, x1, wl);
     ls.pdf *= (1.0f - ps);
     return ls;
 }
@@ -970,11 +1060,12 @@ PT_DEV int light_of(const DevScene& S, int region) {
     for (int i = 0; i < S.n_lights; i++) if (S.lights[i].region == region) return i;
     return -1;
 }
-PT_DEV float lighting_pdf(const DevScene& S, f3 qp, f3 wi, int lid, f3 light_p) {   // lighting.ts:636-673
+// lighting_pdf (lighting.ts:636-673; the tree's, :909-933, replays the light's trail)
+PT_DEV float lighting_pdf(const DevScene& S, const LQuery& q, f3 wi, int lid, f3 light_p, bool tree) {
     const pt_light& l = S.lights[lid];
-    float sel = S.n_lights == 1 ? 1.0f : l.select_pdf;
+    float sel = tree ? lt_pmf(S, q, lid) : (S.n_lights == 1 ? 1.0f : l.select_pdf);
     if (env_select_live(S)) sel = sel * (1.0f - S.env_select_prob);
-    return sel * light_pdf_one(l, qp, light_p, wi);
+    return sel * light_pdf_one(l, q.p, light_p, wi);
 }
 // ---------------------------------------------------------------- media
 // material_of for regions (the ambient region -1 maps to the ambientMedium material),

@@ -94,6 +94,43 @@ def two_light():
     }
 
 
+def many_lights(n=64, seed=3, rough_glass=False):
+    """A many-lights scene in the spirit of the reference's hundred-spheres witness
+    (tests/witnesses/index.ts:192-211), with n small one-sided quad lights of assorted colours
+    and powers scattered over a 12 x 12 floor at heights 0.4-2.5, facing down or tilted, some
+    below a raised slab, around two diffuse spheres (a rough glass sphere instead of one with
+    rough_glass, whose two-sided light queries disarm the horizon cull). Posed from a corner,
+    as the witness is, so that selection by distance matters."""
+    rng = np.random.default_rng(seed)
+    lights = []
+    for k in range(n):
+        x, z = rng.uniform(-6.0, 6.0, 2)
+        y = rng.uniform(0.4, 2.5)
+        size = rng.uniform(0.08, 0.3)
+        tilt = rng.uniform(-0.6, 0.6)
+        e1 = [size, 0.0, 0.0]
+        e2 = [0.0, size * tilt, -size] if k % 3 else [0.0, -size * tilt, size]
+        col = rng.uniform(0.2, 1.0, 3) * rng.uniform(5.0, 60.0)
+        lights.append({"type": "quad", "corner": [float(x), float(y), float(z)], "edge1": e1,
+                       "edge2": [float(v) for v in e2], "radiance": [float(c) for c in col]})
+    mats = {"floor": {"model": "lambert", "albedo": [0.55, 0.55, 0.55]},
+            "clay": {"model": "lambert", "albedo": [0.7, 0.45, 0.3]},
+            "slab": {"model": "lambert", "albedo": [0.3, 0.5, 0.7]}}
+    objs = [{"type": "quad", "corner": [-8.0, 0.0, 8.0], "edge1": [16.0, 0.0, 0.0], "edge2": [0.0, 0.0, -16.0],
+             "material": "floor"},
+            {"type": "quad", "corner": [-2.0, 1.2, 3.0], "edge1": [4.0, 0.0, 0.0], "edge2": [0.0, 0.0, -2.0],
+             "material": "slab"},
+            {"type": "sphere", "center": [-1.0, 0.6, 0.0], "radius": 0.6, "material": "clay"}]
+    if rough_glass:
+        mats["glass"] = {"model": "rough_dielectric", "ior": 1.5, "roughness": 0.3}
+        objs.append({"type": "sphere", "center": [1.2, 0.5, -0.5], "radius": 0.5, "material": "glass"})
+    else:
+        objs.append({"type": "sphere", "center": [1.2, 0.5, -0.5], "radius": 0.5, "material": "clay"})
+    return {"id": "many-lights", "materials": mats, "objects": objs, "lights": lights,
+            "environment": {"type": "none"},
+            "camera": {"type": "pinhole", "position": [-7.5, 1.8, -7.5], "target": [-4.0, 0.6, -4.0], "fov": 0.9}}
+
+
 def cornell_multi_light(point=True):
     """The Cornell box lit by several finite lights of unequal power: the ceiling quad (17, 12, 4),
     a smaller, cooler quad on the back wall facing +z, a dim warm quad low on the left wall

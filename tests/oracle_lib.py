@@ -55,6 +55,15 @@ def load(variant=""):
     lib.or_spectrum_eval.restype = None
     lib.or_env_radiance.argtypes = [vp, vp, vp, C.c_int, vp]
     lib.or_env_radiance.restype = None
+    lib.or_light_tree_build.argtypes = [vp, vp, C.c_int, vp, vp, C.POINTER(C.c_int)]
+    lib.or_light_tree_build.restype = C.c_int
+    lib.or_light_tree_importance.argtypes = [vp, vp, vp, vp, C.c_int]
+    lib.or_light_tree_importance.restype = C.c_float
+    lib.or_light_tree_pmf.argtypes = [vp, vp, C.c_int, vp, vp, C.c_int]
+    lib.or_light_tree_pmf.restype = C.c_float
+    lib.or_light_tree_pick.argtypes = [vp, C.c_float, vp, vp, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                       C.POINTER(C.c_float)]
+    lib.or_light_tree_pick.restype = C.c_int
     lib.or_basic_eval.argtypes = [C.c_int, vp, vp, vp, C.c_int]
     lib.or_adjoint.argtypes = [vp, C.POINTER(abi.PtProgram), C.POINTER(abi.PtParams), C.c_int,
                                C.POINTER(abi.PtAdjointParams), vp, vp]

@@ -31,7 +31,7 @@ def _expected_estimator(d):
 
 # The number of the 505 frozen link maps this backend lowers: a lowering that starts rejecting
 # a case (or accepting one it cannot render) changes it.
-EXPECTED_ACCEPTED = 354
+EXPECTED_ACCEPTED = 382
 
 
 def test_library_lowers_or_names_every_snapshot_case():
@@ -52,7 +52,7 @@ def test_library_lowers_or_names_every_snapshot_case():
         assert p.color == (abi.COLOR_SPECTRAL if m["color"] == "spectral" else abi.COLOR_RGB), name
         assert p.estimator == _expected_estimator(d), name
         sel = (d["estimator"].get("lighting") or {}).get("selection", "power")
-        assert p.light_selection == (abi.SEL_UNIFORM if sel == "uniform" else abi.SEL_POWER), name
+        assert p.light_selection == {"uniform": abi.SEL_UNIFORM, "bvh": abi.SEL_BVH}.get(sel, abi.SEL_POWER), name
         rr = e.get("russianRoulette")
         assert p.rr_start == (-1 if rr is None else rr["startDepth"]), name
         assert abs(p.rr_max_survival - ((rr or {}).get("maxSurvival") or 0.95)) < 1e-7, name
@@ -81,6 +81,7 @@ def test_library_lowers_or_names_every_snapshot_case():
     assert "cornell-oneshot + oneshot" in accepted                 # estimator.accumulation.type 'oneshot'
     # procedural sky bodies other than proc-sky's, and proc-sky under the spectral colour model
     assert {"proc-sky-rotated + pt-mis", "spectral-proc-sky + mis"} <= set(accepted)
+    assert {"two-light + bvh", "hundred-spheres + mis-bvh", "spectral-packed + nee-bvh"} <= set(accepted)   # the light tree
 
 
 def test_chart_and_albedo_flag_lowering():
