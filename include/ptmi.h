@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define PT_ABI_VERSION 14u
+#define PT_ABI_VERSION 15u
 
 typedef enum pt_status {
     PT_OK = 0,
@@ -170,7 +170,13 @@ typedef struct pt_light {
     pt_spectrum s_radiance;  /* spectral color model, role 'emission' */
 } pt_light;
 
-enum { PT_ENV_NONE = 0, PT_ENV_SKY = 1, PT_ENV_CONSTANT = 2 };
+enum { PT_ENV_NONE = 0, PT_ENV_SKY = 1, PT_ENV_CONSTANT = 2, PT_ENV_IMAGE = 3 };
+/* PT_ENV_IMAGE (features/environment.ts:144-190): an equirect picture (pt_scene_desc.env_image,
+ * the reference's env_map: RGB32F, NEAREST, clamped edges) read at equirect_uv(dir) (the additive
+ * rotation) times intensity, lifted as an emission in a spectral program; its importance table is
+ * built from the picture (components/env/importance.ts:16-127) on the equirect chart (the
+ * picture's own W x H) or, on the octahedral chart, from the picture resampled to N x N, N =
+ * table_h (256 default; resampleEquirectToOctahedral, env/octahedral/octahedral.ts:72-100, f64). */
 
 /* Environment. PT_ENV_SKY is a procedural sky (features/environment.ts:191-231): one of the
  * formula bodies below evaluated at the direction turned into the table's azimuth, sampled
@@ -250,6 +256,10 @@ typedef struct pt_scene_desc {
      * only by programs with PT_SEL_BVH; may be null */
     const double* light_boxes;
     const double* light_powers;
+    /* PT_ENV_IMAGE: the picture, env_image_w x env_image_h RGB f32 texels, row 0 at theta = 0
+     * (finite and >= 0; width <= 4096) */
+    const float* env_image;
+    int32_t  env_image_w, env_image_h;
 } pt_scene_desc;
 
 /* ---- program (the Plan stage's decisions) ---- */
@@ -399,6 +409,9 @@ pt_status pt_renderer_load(pt_ctx* ctx, const pt_program* program, pt_renderer**
  * env.totalWeight parameter (the derived env selection probability reads it). 0 for
  * untabulated environments. Needs no device. */
 double pt_env_total_weight(const pt_env* env);
+/* The same for an image environment: the mass of the table `env` (its chart, table_h, compensation)
+ * builds from the w x h picture rgb; 0 on a bad argument (see pt_last_error). */
+double pt_env_image_total_weight(const pt_env* env, const float* rgb, int32_t w, int32_t h);
 
 pt_status pt_program_from_description(const char* program_json, size_t len, int32_t width, int32_t height,
                                       int32_t frames_in_flight, pt_program* out);

@@ -452,6 +452,7 @@ typedef struct {
     float atlas_z[64];
     float* lt_nodes;          /* the light tree (8 floats per node) and the lights' trails (4 each), or null */
     float* lt_trails;
+    float* img; int img_w, img_h;   /* PT_ENV_IMAGE: the equirect picture, RGB */
 } or_scene;
 
 /* Möller–Trumbore (intersection/mesh/mesh.glsl:41-59). */
@@ -1330,6 +1331,7 @@ static inline int has_nondelta(const pt_material* m) { return m->model != PT_MAT
  *   PT_SKY_BIG_SUN  proc-sky-rotated (tests/witnesses/scenes/estimatorAgreementWitness.ts:61-72)
  *   PT_SKY_CAP      sky-missed-cap, vec3(step(0.9, dir.y)) (witnesses/scenes/spectralSkies.ts:39-41)
  *   PT_SKY_WHITE    spectral-comp-empty, vec3(1.0) (spectralSkies.ts:48) */
+static void equirect_uv(const or_scene* S, v3 dir, float* u, float* v);
 static v3 sky_radiance_raw(int body, v3 dir) {
     v3 sun = normalize(V(0.4f, 0.35f, 0.2f));
     if (body == PT_SKY_SUN) {
@@ -1394,6 +1396,17 @@ static sp lift_emission(const or_scene* S, v3 rgb, const wl_t* wl) {
 static sp env_radiance(const or_scene* S, v3 dir, const wl_t* wl) {
     if (S->env.type == PT_ENV_CONSTANT)   /* features/environment.ts:119-133: color * u_envIntensity */
         return smuls(lift(S, S->env.color, &S->env.s_color, wl), S->env.intensity);
+    if (S->env.type == PT_ENV_IMAGE) {   /* texture(u_envMap, equirect_uv(dir)): NEAREST, clamped edges */
+        float u, v;
+        equirect_uv(S, dir, &u, &v);
+        int i = (int)(u * (float)S->img_w), j = (int)(v * (float)S->img_h);
+        if (i < 0) i = 0; if (i > S->img_w - 1) i = S->img_w - 1;
+        if (j < 0) j = 0; if (j > S->img_h - 1) j = S->img_h - 1;
+        const float* t = S->img + 3 * ((size_t)j * S->img_w + i);
+        v3 c = V(t[0], t[1], t[2]);
+        if (!wl->spec) return srgbv(muls(c, S->env.intensity));
+        return smuls(lift_emission(S, c, wl), S->env.intensity);
+    }
     if (S->env.type != PT_ENV_SKY) return sgrey(0.0f);
     v3 d = normalize(dir);
     float cx = S->env_rot_cs[0], sy = S->env_rot_cs[1];
@@ -1552,13 +1565,59 @@ static void env_build(or_scene* S) {
     env_chain(S, Y, W, H, oct, S->env.compensated);
 }
 
+/* resampleEquirectToOctahedral (env/octahedral/octahedral.ts:72-100) at the equalAreaSquareToSphere
+ * (:10-26) texel centres, in f64 */
+static void or_resample_oct(const float* rgb, int W, int H, int N, float* out) {
+    for (int j = 0; j < N; j++)
+        for (int i = 0; i < N; i++) {
+            double u0 = (i + 0.5) / N, v0 = (j + 0.5) / N;
+            double up = 2 * u0 - 1, vp = 2 * v0 - 1, ua = fabs(up), va = fabs(vp);
+            double sd = 1 - (ua + va), r = 1 - fabs(sd);
+            double ph = ((r == 0 ? 1 : (va - ua) / r) + 1) * M_PI / 4;
+            double y = copysign(1 - r * r, sd);
+            double s = r * sqrt(2 - r * r > 0 ? 2 - r * r : 0);
+            double x = copysign(cos(ph), up) * s, z = copysign(sin(ph), vp) * s;
+            double phi = atan2(z, x), theta = acos(y > 1 ? 1 : (y < -1 ? -1 : y));
+            double fx = (phi / (2 * M_PI) + 0.5) * W - 0.5, fy = (theta / M_PI) * H - 0.5;
+            double x0 = floor(fx), y0 = floor(fy), tx = fx - x0, ty = fy - y0;
+            int xa = (int)fmod(fmod(x0, W) + W, W), xb = (int)fmod(fmod(x0 + 1, W) + W, W);
+            int ya = (int)(y0 < 0 ? 0 : (y0 > H - 1 ? H - 1 : y0)), yb = (int)(y0 + 1 < 0 ? 0 : (y0 + 1 > H - 1 ? H - 1 : y0 + 1));
+            for (int c = 0; c < 3; c++) {
+                double s00 = rgb[3 * ((size_t)ya * W + xa) + c], s10 = rgb[3 * ((size_t)ya * W + xb) + c];
+                double s01 = rgb[3 * ((size_t)yb * W + xa) + c], s11 = rgb[3 * ((size_t)yb * W + xb) + c];
+                out[3 * ((size_t)j * N + i) + c] = (float)((1 - tx) * (1 - ty) * s00 + tx * (1 - ty) * s10 +
+                                                           (1 - tx) * ty * s01 + tx * ty * s11);
+            }
+        }
+}
+/* An image environment's table: the picture's luminance (equirect) or its octahedral resample's */
+static void env_build_image(or_scene* S) {
+    int oct = S->env.chart == PT_CHART_OCTAHEDRAL;
+    int W = S->img_w, H = S->img_h;
+    const float* rgb = S->img;
+    float* rs = NULL;
+    if (oct) {
+        W = H = S->env.table_h > 0 ? S->env.table_h : 256;
+        rs = (float*)malloc(sizeof(float) * 3 * (size_t)W * H);
+        or_resample_oct(S->img, S->img_w, S->img_h, W, rs);
+        rgb = rs;
+    }
+    float* Y = (float*)malloc(sizeof(float) * (size_t)W * H);
+    for (size_t k = 0; k < (size_t)W * H; k++)
+        Y[k] = (float)(0.2126 * (double)rgb[3 * k] + 0.7152 * (double)rgb[3 * k + 1] + 0.0722 * (double)rgb[3 * k + 2]);
+    env_chain(S, Y, W, H, oct, S->env.compensated);
+    free(rs);
+}
+/* the resample on its own (test probe) */
+void or_resample_equirect_to_octahedral(const float* rgb, int W, int H, int N, float* out) { or_resample_oct(rgb, W, H, N, out); }
+
 /* registerEnvironmentTable (engine/Engine.ts:450-460 -> importance.ts:16-127): the table rebuilt
  * from a caller's W x H RGB picture on `chart`; the luminance is the f64 weighted sum of the f32
  * channels (importance.ts:38-45). Returns 0, or -1 for a bad size or a negative / non-finite
  * sample (the reference throws). */
 int or_scene_env_table_rgb(or_scene* S, const float* rgb, int W, int H, int chart, int compensated, double* total) {
     int oct = chart == PT_CHART_OCTAHEDRAL;
-    if (W < 1 || H < 1 || (oct && W != H) || S->env.type != PT_ENV_SKY) return -1;
+    if (W < 1 || H < 1 || (oct && W != H) || (S->env.type != PT_ENV_SKY && S->env.type != PT_ENV_IMAGE)) return -1;
     for (size_t k = 0; k < (size_t)3 * W * H; k++)
         if (!isfinite(rgb[k]) || rgb[k] < 0.0f) return -1;
     float* Y = (float*)malloc(sizeof(float) * (size_t)W * H);
@@ -1619,7 +1678,9 @@ static lsample_t lighting_sample_finite(const or_scene* S, v3 p, float x0, float
     ls.pdf *= sel;
     return ls;
 }
-static int env_samplable(const or_scene* S) { return S->env.type == PT_ENV_SKY || (S->env.type == PT_ENV_CONSTANT && S->env.samplable); }
+static int env_samplable(const or_scene* S) {
+    return S->env.type == PT_ENV_SKY || S->env.type == PT_ENV_IMAGE || (S->env.type == PT_ENV_CONSTANT && S->env.samplable);
+}
 static int env_select_live(const or_scene* S) { return env_samplable(S) && S->n_lights > 0; }
 static lsample_t environment_sample(const or_scene* S, float x0, float x1, const wl_t* wl) {
     lsample_t ls;
@@ -2324,6 +2385,11 @@ or_scene* or_scene_create(const pt_scene_desc* d) {
         S->cie_y = d->cie_y_integral;
         S->has_spectral = 1;
     }
+    if (d->env.type == PT_ENV_IMAGE && d->env_image) {
+        S->img_w = d->env_image_w; S->img_h = d->env_image_h;
+        S->img = (float*)malloc(sizeof(float) * 3 * (size_t)S->img_w * (size_t)S->img_h);
+        memcpy(S->img, d->env_image, sizeof(float) * 3 * (size_t)S->img_w * (size_t)S->img_h);
+    }
     if (d->light_boxes && d->light_powers && d->n_lights > 0) {
         S->lt_nodes = (float*)malloc(sizeof(float) * 8 * (size_t)(2 * d->n_lights - 1));
         S->lt_trails = (float*)malloc(sizeof(float) * 4 * (size_t)d->n_lights);
@@ -2431,6 +2497,7 @@ or_scene* or_scene_create(const pt_scene_desc* d) {
         free(ib);
     }
     if (S->env.type == PT_ENV_SKY) env_build(S);
+    if (S->env.type == PT_ENV_IMAGE) env_build_image(S);
     return S;
 }
 void or_scene_destroy(or_scene* S) {
@@ -2439,7 +2506,7 @@ void or_scene_destroy(or_scene* S) {
     free(S->blas); bvh_free(&S->tlas); free(S->tlas_obj);
     free(S->obj_first); bvh_free(&S->itlas); free(S->inst); free(S->inst_batch); free(S->inst_base);
     free(S->tri); free(S->nrm); free(S->tri_object); free(S->objects); free(S->materials); free(S->lights);
-    free(S->env_cond); free(S->env_marg); free(S->atlas); free(S->lt_nodes); free(S->lt_trails);
+    free(S->env_cond); free(S->env_marg); free(S->atlas); free(S->lt_nodes); free(S->lt_trails); free(S->img);
     free(S);
 }
 void or_env_table(const or_scene* S, float* cond, float* marg) {

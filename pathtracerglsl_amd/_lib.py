@@ -16,7 +16,7 @@ DECLARED = ["pt_last_error", "pt_ctx_create", "pt_ctx_destroy", "pt_scene_upload
             "pt_scene_update_materials", "pt_forward", "pt_backward",
             "pt_batch_create", "pt_batch_destroy", "pt_batch_set_config", "pt_batch_get_config",
             "pt_batch_render", "pt_batch_read_export", "pt_batch_autotune",
-            "pt_program_from_description", "pt_renderer_load_description", "pt_env_total_weight",
+            "pt_program_from_description", "pt_renderer_load_description", "pt_env_total_weight", "pt_env_image_total_weight",
             "pt_obj_parse", "pt_mesh_data_info", "pt_mesh_data_copy", "pt_mesh_data_free", "pt_inst_parse",
             "pt_render_frames_noalloc", "pt_bvh_build", "pt_light_tree_build", "pt_mesh_bvh_build", "pt_plan_data_layout",
             "pt_pack_mesh_channels", "pt_ctx_reset", "pt_scene_update_lights", "pt_scene_update_env",
@@ -95,6 +95,8 @@ def load():
     lib.pt_renderer_load_description.argtypes = [vp, C.c_char_p, C.c_size_t, i32, i32, i32, C.POINTER(vp)]
     lib.pt_env_total_weight.argtypes = [C.POINTER(abi.PtEnv)]
     lib.pt_env_total_weight.restype = C.c_double
+    lib.pt_env_image_total_weight.argtypes = [C.POINTER(abi.PtEnv), vp, i32, i32]
+    lib.pt_env_image_total_weight.restype = C.c_double
     lib.pt_obj_parse.argtypes = [C.c_char_p, C.c_size_t, u32, C.POINTER(vp)]
     lib.pt_mesh_data_info.argtypes = [vp, C.POINTER(u32), C.POINTER(u32), C.POINTER(i32), C.POINTER(i32)]
     lib.pt_mesh_data_copy.argtypes = [vp, vp, vp, vp, vp]

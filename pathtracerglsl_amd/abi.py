@@ -7,7 +7,7 @@ import ctypes as C
 
 import numpy as np
 
-ABI_VERSION = 14
+ABI_VERSION = 15
 
 OBJ_MESH, OBJ_QUAD, OBJ_SPHERE, OBJ_INSTANCES = 0, 1, 2, 3
 MAT_LAMBERT, MAT_DIELECTRIC, MAT_GGX, MAT_NULL, MAT_MIRROR, MAT_ROUGH_DIELECTRIC = 0, 1, 2, 3, 4, 5
@@ -15,7 +15,7 @@ MED_ANALYTIC, MED_ABSORB, MED_DELTA = 0, 1, 2
 VS_ANALYTIC, VS_DELTA_TRACKING = 0, 1
 IOR_CONSTANT, IOR_CAUCHY, IOR_SELLMEIER = 0, 1, 2
 LIGHT_QUAD, LIGHT_POINT = 0, 1
-ENV_NONE, ENV_SKY, ENV_CONSTANT = 0, 1, 2
+ENV_NONE, ENV_SKY, ENV_CONSTANT, ENV_IMAGE = 0, 1, 2, 3
 SKY_SUN, SKY_BIG_SUN, SKY_CAP, SKY_WHITE = 0, 1, 2, 3
 AOV_ALBEDO, AOV_NORMAL = 1, 2
 CHART_EQUIRECT, CHART_OCTAHEDRAL = 0, 1
@@ -94,7 +94,8 @@ class PtSceneDesc(C.Structure):
                 ("proto_vertices", C.c_void_p), ("instances", C.c_void_p),
                 ("tri_normals", C.c_void_p), ("proto_normals", C.c_void_p),
                 ("spectral_atlas", C.c_void_p), ("spectral_z_nodes", C.c_void_p),
-                ("light_boxes", C.c_void_p), ("light_powers", C.c_void_p)]
+                ("light_boxes", C.c_void_p), ("light_powers", C.c_void_p),
+                ("env_image", C.c_void_p), ("env_image_w", C.c_int32), ("env_image_h", C.c_int32)]
 
 
 class PtInstTable(C.Structure):
@@ -174,7 +175,7 @@ class SceneArrays:
     def __init__(self, tri_vertices, tri_object, objects, materials, lights, env=None, ambient_material=-1,
                  spectral_curves=None, xyz_to_rgb=None, cie_y_integral=0.0, proto_vertices=None, instances=None,
                  tri_normals=None, proto_normals=None, spectral_atlas=None, spectral_z_nodes=None,
-                 light_boxes=None, light_powers=None):
+                 light_boxes=None, light_powers=None, env_image=None):
         self.tri_vertices = np.ascontiguousarray(tri_vertices, dtype=np.float32).reshape(-1, 9)
         self.tri_object = np.ascontiguousarray(tri_object, dtype=np.int32).reshape(-1)
         self.objects = np.ascontiguousarray(objects, dtype=OBJECT_DTYPE)
@@ -198,6 +199,8 @@ class SceneArrays:
                                  else np.ascontiguousarray(spectral_z_nodes, np.float32).reshape(64))
         self.light_boxes = None if light_boxes is None else np.ascontiguousarray(light_boxes, np.float64).reshape(-1, 6)
         self.light_powers = None if light_powers is None else np.ascontiguousarray(light_powers, np.float64).reshape(-1)
+        self.env_image = None if env_image is None else np.ascontiguousarray(env_image, np.float32)
+        assert self.env_image is None or (self.env_image.ndim == 3 and self.env_image.shape[2] == 3)
         assert self.tri_object.shape[0] == self.tri_vertices.shape[0]
         assert self.tri_normals is None or self.tri_normals.shape == self.tri_vertices.shape
         assert self.proto_normals is None or self.proto_normals.shape == self.proto_vertices.shape
@@ -234,6 +237,9 @@ class SceneArrays:
         d.spectral_z_nodes = _ptr(self.spectral_z_nodes)
         d.light_boxes = _ptr(self.light_boxes)
         d.light_powers = _ptr(self.light_powers)
+        d.env_image = _ptr(self.env_image)
+        if self.env_image is not None:
+            d.env_image_h, d.env_image_w = int(self.env_image.shape[0]), int(self.env_image.shape[1])
         return d
 
 

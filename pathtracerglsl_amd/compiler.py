@@ -343,8 +343,15 @@ def _analyze_validate(scene, strategy):
         if o["type"] not in ("mesh", "sphere", "quad", "instances"):
             raise CompileError(f"object type '{o['type']}' is not supported by this backend")
     env = scene.get("environment", {"type": "none"})
-    if env.get("type") not in ("none", "procedural", "constant"):
+    if env.get("type") not in ("none", "procedural", "constant", "image"):
         raise CompileError(f"environment '{env.get('type')}' is not supported by this backend")
+    if env.get("type") == "image":
+        img = env.get("image")
+        if not (isinstance(img, np.ndarray) and img.ndim == 3 and img.shape[2] == 3 and img.shape[0] >= 1
+                and 1 <= img.shape[1] <= 4096):
+            raise CompileError("environment 'image': 'image' must be an H x W x 3 array of linear RGB (1 <= W <= 4096)")
+        if not (np.all(np.isfinite(img)) and np.all(img >= 0)):
+            raise CompileError("Environment RGB samples must be finite and nonnegative")
     if env.get("type") == "procedural":
         sky_body(env)
     if spectral.is_gaussian(env.get("color")):
@@ -646,6 +653,20 @@ def compile_scene(scene, strategy=None, width=256, height=256, reset_salt=1234):
         if chart not in ENV_CHARTS:
             raise CompileError(f"environment.chart {chart!r}: expected one of {sorted(ENV_CHARTS)}")
         env.chart = ENV_CHARTS[chart]
+    elif e.get("type") == "image":
+        # features/environment.ts:144-190: the equirect picture; its table on the sampler's chart
+        # (the picture's own W x H, or N x N octahedral, N = table_size[1], 256 default)
+        img = np.ascontiguousarray(e["image"], np.float32)
+        tw, th = e.get("table_size", (512, 256))
+        env = abi.PtEnv(abi.ENV_IMAGE, int(img.shape[1]), int(th), int(e.get("compensated", False)),
+                        float(e.get("intensity", 1.0)), float(e.get("rotation", 0.0)),
+                        float(e.get("select_prob", 0.5)), 1)
+        chart = e.get("chart", "equirect")
+        if chart not in ENV_CHARTS:
+            raise CompileError(f"environment.chart {chart!r}: expected one of {sorted(ENV_CHARTS)}")
+        env.chart = ENV_CHARTS[chart]
+        if env.chart == abi.CHART_EQUIRECT:
+            env.table_w, env.table_h = int(img.shape[1]), int(img.shape[0])
     elif e.get("type") == "constant":
         # features/environment.ts:119-133: a constant sky, sampled as a light only when asked
         col = e.get("color", [1.0, 1.0, 1.0])
@@ -655,9 +676,14 @@ def compile_scene(scene, strategy=None, width=256, height=256, reset_salt=1234):
             env.color[i] = float(c)
         env.s_color = abi.PtSpectrum.of(srec(col, spectral.ROLE_EMISSION))
 
-    if env.type != abi.ENV_NONE and (env.type == abi.ENV_SKY or env.samplable) and light_recs and "select_prob" not in e:
-        env.select_prob = env_select_probability(scene, env, [p for _, p in lights], strategy.light_selection,
-                                                 e.get("total_weight"))
+    if env.type != abi.ENV_NONE and (env.type in (abi.ENV_SKY, abi.ENV_IMAGE) or env.samplable) and light_recs \
+            and "select_prob" not in e:
+        total = e.get("total_weight")
+        if env.type == abi.ENV_IMAGE and total is None:
+            from . import _lib
+            img = np.ascontiguousarray(e["image"], np.float32)
+            total = _lib.load().pt_env_image_total_weight(env, img.ctypes.data, img.shape[1], img.shape[0])
+        env.select_prob = env_select_probability(scene, env, [p for _, p in lights], strategy.light_selection, total)
 
     tv = np.concatenate(tri_chunks) if tri_chunks else np.zeros((0, 9), np.float32)
     to = np.concatenate(tri_obj_chunks) if tri_obj_chunks else np.zeros(0, np.int32)
@@ -665,7 +691,7 @@ def compile_scene(scene, strategy=None, width=256, height=256, reset_salt=1234):
     if amb is not None and amb not in mat_index:
         raise CompileError(f"ambientMedium references unknown material '{amb}'")
     # a spectral program lifts a coloured sky body per direction through the coefficient atlas
-    atlas = spec and env.type == abi.ENV_SKY and env.body not in GREY_SKY_BODIES
+    atlas = spec and ((env.type == abi.ENV_SKY and env.body not in GREY_SKY_BODIES) or env.type == abi.ENV_IMAGE)
     arrays = abi.SceneArrays(tv, to, np.array(objects, abi.OBJECT_DTYPE),
                              np.array(materials, abi.MATERIAL_DTYPE),
                              np.array(light_recs, abi.LIGHT_DTYPE), env,
@@ -678,7 +704,8 @@ def compile_scene(scene, strategy=None, width=256, height=256, reset_salt=1234):
                                                                      for r in objects) else None,
                              spectral.atlas() if atlas else None, spectral.z_nodes() if atlas else None,
                              np.array([b for b, _ in light_tree_in], np.float64) if light_tree_in else None,
-                             np.array([p for _, p in light_tree_in], np.float64) if light_tree_in else None)
+                             np.array([p for _, p in light_tree_in], np.float64) if light_tree_in else None,
+                             np.ascontiguousarray(e["image"], np.float32) if e.get("type") == "image" else None)
 
     prog = abi.PtProgram()
     prog.abi_version = abi.ABI_VERSION
@@ -693,7 +720,8 @@ def compile_scene(scene, strategy=None, width=256, height=256, reset_salt=1234):
     prog.variance = int(bool(strategy.variance))
     prog.accumulation = ACCUMULATIONS[strategy.accumulation]
     prog.aovs = sum({"albedo": abi.AOV_ALBEDO, "normal": abi.AOV_NORMAL}[a] for a in set(strategy.aovs))
-    prog.env_chart = ENV_CHARTS[e.get("chart", "equirect")] + 1 if e.get("type") == "procedural" else abi.CHART_EQUIRECT + 1
+    prog.env_chart = (ENV_CHARTS[e.get("chart", "equirect")] + 1 if e.get("type") in ("procedural", "image")
+                      else abi.CHART_EQUIRECT + 1)
     prog.width, prog.height = width, height
     prog.frames_in_flight = max(1, strategy.frames_in_flight)
     prog.max_null_crossings = strategy.max_null_crossings

@@ -401,7 +401,7 @@ struct F3 { float x, y, z; };
 void build_env_table(const pt_env& env, std::vector<float>& cond, std::vector<float>& marg, double& total) {
     const bool oct = env.chart == PT_CHART_OCTAHEDRAL;          // N x N, N = table_h (EnvironmentBake.ts:20-27)
     const int H = env.table_h > 0 ? env.table_h : 256, W = oct ? H : (env.table_w > 0 ? env.table_w : 512);
-    std::vector<float> Y((size_t)W * H), Bl((size_t)W * H);
+    std::vector<float> Y((size_t)W * H);
     for (int j = 0; j < H; j++) {
         for (int i = 0; i < W; i++) {
             float u = ((float)i + 0.5f) / (float)W, v = ((float)j + 0.5f) / (float)H;
@@ -419,6 +419,14 @@ void build_env_table(const pt_env& env, std::vector<float>& cond, std::vector<fl
             Y[(size_t)j * W + i] = (float)(0.2126 * (double)c[0] + 0.7152 * (double)c[1] + 0.0722 * (double)c[2]);
         }
     }
+    env_table_from_luma(Y, W, H, oct, env.compensated != 0, cond, marg, total);
+}
+
+// The importance chain of components/env/importance.ts:44-127 over a W x H luminance table:
+// 3 x 3 tent blur (u wraps on the equirect chart), MIS compensation, row and marginal CDFs.
+void env_table_from_luma(const std::vector<float>& Y, int W, int H, bool oct, bool compensated, std::vector<float>& cond,
+                         std::vector<float>& marg, double& total) {
+    std::vector<float> Bl((size_t)W * H);
     for (int j = 0; j < H; j++)
         for (int i = 0; i < W; i++) {
             double sum = 0, wsum = 0;
@@ -433,7 +441,7 @@ void build_env_table(const pt_env& env, std::vector<float>& cond, std::vector<fl
             }
             Bl[(size_t)j * W + i] = (float)(sum / wsum);
         }
-    if (env.compensated) {
+    if (compensated) {
         double num = 0, den = 0;
         for (int j = 0; j < H; j++) {
             double dO = oct ? (4 * M_PI) / (W * H) : (2 * M_PI / W) * (M_PI / H) * std::sin(M_PI * (j + 0.5) / H);
@@ -460,6 +468,57 @@ void build_env_table(const pt_env& env, std::vector<float>& cond, std::vector<fl
     }
     double acc = 0;
     for (int j = 0; j < H; j++) { acc += row[j]; marg[j] = (float)(total > 0 ? acc / total : 0); }
+}
+
+// equalAreaSquareToSphere (env/octahedral/octahedral.ts:10-26), f64
+static void eq_area_dir(double u, double v, double* d) {
+    const double up = 2 * u - 1, vp = 2 * v - 1;
+    const double ua = std::fabs(up), va = std::fabs(vp);
+    const double sd = 1 - (ua + va);
+    const double r = 1 - std::fabs(sd);
+    const double phi = ((r == 0 ? 1 : (va - ua) / r) + 1) * M_PI / 4;
+    const double y = std::copysign(1 - r * r, sd);
+    const double cp = std::copysign(std::cos(phi), up), sp = std::copysign(std::sin(phi), vp);
+    const double s = r * std::sqrt(std::max(0.0, 2 - r * r));
+    d[0] = cp * s; d[1] = y; d[2] = sp * s;
+}
+
+void resample_equirect_to_octahedral(const float* rgb, int W, int H, int N, std::vector<float>& out) {
+    out.assign((size_t)3 * N * N, 0.0f);
+    for (int j = 0; j < N; j++)
+        for (int i = 0; i < N; i++) {
+            double d[3];
+            eq_area_dir((i + 0.5) / N, (j + 0.5) / N, d);
+            const double phi = std::atan2(d[2], d[0]);
+            const double theta = std::acos(std::min(1.0, std::max(-1.0, d[1])));
+            const double u = phi / (2 * M_PI) + 0.5, v = theta / M_PI;
+            const double fx = u * W - 0.5, fy = v * H - 0.5;
+            const double x0 = std::floor(fx), y0 = std::floor(fy);
+            const double tx = fx - x0, ty = fy - y0;
+            auto wrap = [&](double x) { return (int)std::fmod(std::fmod(x, (double)W) + W, (double)W); };
+            auto cy = [&](double y) { return (int)std::min((double)(H - 1), std::max(0.0, y)); };
+            for (int c = 0; c < 3; c++) {
+                auto at = [&](double x, double y) { return (double)rgb[3 * ((size_t)cy(y) * W + wrap(x)) + c]; };
+                out[3 * ((size_t)j * N + i) + c] = (float)((1 - tx) * (1 - ty) * at(x0, y0) + tx * (1 - ty) * at(x0 + 1, y0) +
+                                                           (1 - tx) * ty * at(x0, y0 + 1) + tx * ty * at(x0 + 1, y0 + 1));
+            }
+        }
+}
+
+void build_env_table_image(const pt_env& env, const float* rgb, int w, int h, std::vector<float>& cond,
+                           std::vector<float>& marg, double& total) {
+    const bool oct = env.chart == PT_CHART_OCTAHEDRAL;
+    std::vector<float> resampled;
+    int W = w, H = h;
+    if (oct) {
+        W = H = env.table_h > 0 ? env.table_h : 256;
+        resample_equirect_to_octahedral(rgb, w, h, W, resampled);
+        rgb = resampled.data();
+    }
+    std::vector<float> Y((size_t)W * H);
+    for (size_t k = 0; k < Y.size(); k++)
+        Y[k] = (float)(0.2126 * (double)rgb[3 * k] + 0.7152 * (double)rgb[3 * k + 1] + 0.0722 * (double)rgb[3 * k + 2]);
+    env_table_from_luma(Y, W, H, oct, env.compensated != 0, cond, marg, total);
 }
 
 // ---------------------------------------------------------------- containment probe map

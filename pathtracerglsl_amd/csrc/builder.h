@@ -70,5 +70,14 @@ void build_probe_map(const float* tris12, int n, int first, ProbeMap& m);
 // Env importance table for the procedural sky: fp32 bake on the equirect grid, then the
 // f64 luminance / blur / CDF build, stored f32 (EnvironmentBake.ts:62-67, env/importance.ts).
 void build_env_table(const pt_env& env, std::vector<float>& cond, std::vector<float>& marg, double& total);
+// The chain after the luminance pass (importance.ts:44-127), f64, stored f32.
+void env_table_from_luma(const std::vector<float>& Y, int W, int H, bool oct, bool compensated, std::vector<float>& cond,
+                         std::vector<float>& marg, double& total);
+// resampleEquirectToOctahedral (env/octahedral/octahedral.ts:72-100): the W x H equirect RGB picture
+// at the N x N equal-area texel centres, bilinear, u wrapping, v clamped, f64, stored f32.
+void resample_equirect_to_octahedral(const float* rgb, int W, int H, int N, std::vector<float>& out);
+// An image environment's table on env.chart (the picture, or its octahedral resample at N = table_h).
+void build_env_table_image(const pt_env& env, const float* rgb, int w, int h, std::vector<float>& cond,
+                           std::vector<float>& marg, double& total);
 
 }  // namespace ptmi

@@ -309,6 +309,8 @@ bool lower(const JVal& d, int32_t width, int32_t height, int32_t fif, pt_program
         if (!src || src->kind != JVal::STR || !known_sky_body(src->str)) return L.unsupported("environment", "'procedural' with this GLSL body");
     } else if (is_str(et, "constant")) {
         // color * intensity in every direction; the colour itself travels with the scene (pt_env)
+    } else if (is_str(et, "image")) {
+        // the equirect picture travels with the scene (pt_scene_desc.env_image, PT_ENV_IMAGE)
     } else {
         return L.unsupported("environment.type", quoted(et));
     }

@@ -112,6 +112,8 @@ struct DevScene {
     int n_lights;
     int env_type, env_w, env_h, env_compensated, env_chart;
     int env_body;               // PT_ENV_SKY: PT_SKY_* formula body
+    const float* env_img;       // PT_ENV_IMAGE: the equirect picture, RGB f32, env_img_w x env_img_h
+    int env_img_w, env_img_h;
     int env_samplable;          // PT_ENV_CONSTANT sampled as a light
     float4 env_color;           // PT_ENV_CONSTANT radiance (RGB)
     pt_spectrum env_s_color;    // PT_ENV_CONSTANT radiance (spectral)

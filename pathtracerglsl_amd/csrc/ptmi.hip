@@ -170,6 +170,14 @@ extern "C" {
 
 const char* pt_last_error(void) { return g_err.c_str(); }
 
+double pt_env_image_total_weight(const pt_env* env, const float* rgb, int32_t w, int32_t h) {
+    if (!env || !rgb || w < 1 || h < 1 || w > PT_ENV_MAX_W) { fail(PT_E_INVALID, "pt_env_image_total_weight: bad argument"); return 0.0; }
+    std::vector<float> cond, marg;
+    double total = 0.0;
+    build_env_table_image(*env, rgb, w, h, cond, marg, total);
+    return total;
+}
+
 double pt_env_total_weight(const pt_env* env) {
     if (!env || env->type != PT_ENV_SKY) return 0.0;
     std::vector<float> cond, marg;
@@ -380,7 +388,15 @@ static pt_status validate_scene(const pt_scene_desc* sd) {
         if (ob.kind == PT_OBJ_MESH && (ob.first_tri < 0 || ob.tri_count < 0 || ob.first_tri + ob.tri_count > sd->n_tris))
             return fail(PT_E_INVALID, "object " + std::to_string(o) + ": triangle range out of bounds");
     }
-    if (sd->env.type == PT_ENV_SKY && sd->env.chart != PT_CHART_EQUIRECT && sd->env.chart != PT_CHART_OCTAHEDRAL)
+    if (sd->env.type == PT_ENV_IMAGE) {
+        if (!sd->env_image || sd->env_image_w < 1 || sd->env_image_h < 1 || sd->env_image_w > PT_ENV_MAX_W)
+            return fail(PT_E_MISSING_INPUT, "missing input: env_image (PT_ENV_IMAGE needs its picture, 1 <= width <= 4096)");
+        for (long long k = 0; k < 3LL * sd->env_image_w * sd->env_image_h; k++)
+            if (!std::isfinite(sd->env_image[k]) || sd->env_image[k] < 0.0f)
+                return fail(PT_E_INVALID, "Environment RGB samples must be finite and nonnegative");
+    }
+    if ((sd->env.type == PT_ENV_SKY || sd->env.type == PT_ENV_IMAGE) && sd->env.chart != PT_CHART_EQUIRECT &&
+        sd->env.chart != PT_CHART_OCTAHEDRAL)
         return fail(PT_E_INVALID, "pt_env.chart must be PT_CHART_EQUIRECT or PT_CHART_OCTAHEDRAL");
     if (sd->env.type == PT_ENV_SKY && (sd->env.body < PT_SKY_SUN || sd->env.body > PT_SKY_WHITE))
         return fail(PT_E_INVALID, "pt_env.body must be one of PT_SKY_*");
@@ -390,6 +406,9 @@ static pt_status validate_scene(const pt_scene_desc* sd) {
         return fail(PT_E_INVALID, "pt_scene_desc: light_boxes and light_powers come together");
     return PT_OK;
 }
+
+// Environments sampled through an importance table: the procedural sky and the picture.
+static bool env_tabulated(int type) { return type == PT_ENV_SKY || type == PT_ENV_IMAGE; }
 
 // Build every device structure of one scene (wide BVH, containment, env table) into SS.
 // The importance-table chain after the luminance pass (components/env/importance.ts:44-127):
@@ -483,7 +502,7 @@ static pt_status bake_env_table(const pt_env& env, SceneStore& SS, DevScene& D, 
 static void set_env_values(DevScene& D, const pt_env& e) {
     D.env_intensity = e.intensity;
     D.env_select_prob = e.select_prob;
-    if (e.type == PT_ENV_SKY) {
+    if (e.type == PT_ENV_SKY || e.type == PT_ENV_IMAGE) {
         D.env_rotation = e.rotation;
         D.env_rot_c = (float)std::cos((double)e.rotation);
         D.env_rot_s = (float)std::sin((double)e.rotation);
@@ -782,6 +801,37 @@ static pt_status build_scene(const pt_scene_desc* sd, hipStream_t stream, SceneS
         D.env_compensated = sd->env.compensated;
         D.env_body = sd->env.body;
         set_env_values(D, sd->env);
+    } else if (sd->env.type == PT_ENV_IMAGE) {
+        // the picture (NEAREST lookups) and its importance table on the chart
+        if ((st = SS.upload(sd->env_image, (size_t)3 * sd->env_image_w * sd->env_image_h, &D.env_img)) != PT_OK) return st;
+        D.env_img_w = sd->env_image_w;
+        D.env_img_h = sd->env_image_h;
+        const bool oct = sd->env.chart == PT_CHART_OCTAHEDRAL;
+        std::vector<float> resampled;
+        const float* tex = sd->env_image;
+        int W = sd->env_image_w, H = sd->env_image_h;
+        if (oct) {
+            W = H = sd->env.table_h > 0 ? sd->env.table_h : 256;
+            if (W > PT_ENV_MAX_W) return fail(PT_E_UNSUPPORTED, "environment table wider than 4096 texels");
+            resample_equirect_to_octahedral(sd->env_image, sd->env_image_w, sd->env_image_h, W, resampled);
+            tex = resampled.data();
+        }
+        SS.env_first = SS.bufs.size();
+        SceneStore tmp;
+        tmp.stream = SS.stream;
+        const long long n = (long long)W * H;
+        const float* d_rgb;
+        float* Y;
+        if ((st = tmp.upload(tex, (size_t)(3 * n), &d_rgb)) != PT_OK || (st = tmp.alloc((size_t)n, &Y)) != PT_OK) return st;
+        hipLaunchKernelGGL(k_env_luma_rgb, dim3(grid_for(n, 256)), dim3(256), 0, SS.stream, d_rgb, Y, n);
+        if ((st = env_table_chain(Y, W, H, oct, sd->env.compensated != 0, SS, tmp, D)) != PT_OK) return st;
+        HIP_TRY(hipStreamSynchronize(SS.stream));
+        SS.env_end = SS.bufs.size();
+        D.env_chart = oct ? PT_CHART_OCTAHEDRAL : PT_CHART_EQUIRECT;
+        D.env_w = W;
+        D.env_h = H;
+        D.env_compensated = sd->env.compensated;
+        set_env_values(D, sd->env);
     } else if (sd->env.type == PT_ENV_CONSTANT) {
         D.env_samplable = sd->env.samplable ? 1 : 0;
         set_env_values(D, sd->env);
@@ -834,7 +884,7 @@ static bool holds_uniform_selection(const DevScene& dev, const std::vector<pt_li
             if (lights[i].select_pdf != (float)p || lights[i].cdf != (float)acc) return false;
         }
     }
-    const bool env_live = (dev.env_type == PT_ENV_SKY || (dev.env_type == PT_ENV_CONSTANT && dev.env_samplable)) && n > 0;
+    const bool env_live = (env_tabulated(dev.env_type) || (dev.env_type == PT_ENV_CONSTANT && dev.env_samplable)) && n > 0;
     return !env_live || dev.env_select_prob == (float)std::min(0.99, std::max(0.01, 1.0 / (double)(n + 1)));
 }
 
@@ -850,7 +900,7 @@ static pt_status check_program_scene(const pt_program& prog, const DevScene& dev
     if (prog.light_selection == PT_SEL_BVH) {
         if (!lights.empty() && !dev.lt_nodes)
             return fail(PT_E_MISSING_INPUT, "missing input: light_boxes / light_powers (the light tree's inputs in pt_scene_desc)");
-        const bool env_live = (dev.env_type == PT_ENV_SKY || (dev.env_type == PT_ENV_CONSTANT && dev.env_samplable)) && !lights.empty();
+        const bool env_live = (env_tabulated(dev.env_type) || (dev.env_type == PT_ENV_CONSTANT && dev.env_samplable)) && !lights.empty();
         if (env_live && dev.env_select_prob != 0.5f)
             return fail(PT_E_INVALID, "the program selects lights through the light tree but the scene's pt_env.select_prob "
                                       "is not the tree's 1/2");
@@ -860,10 +910,11 @@ static pt_status check_program_scene(const pt_program& prog, const DevScene& dev
     if (prog.light_selection == PT_SEL_UNIFORM && !holds_uniform_selection(dev, lights))
         return fail(PT_E_INVALID, "the program selects lights uniformly but the scene's selection table (pt_light.select_pdf / "
                                   "cdf, pt_env.select_prob) is not the uniform one");
-    if (prog.color == PT_COLOR_SPECTRAL && dev.env_type == PT_ENV_SKY && !pt_sky_body_grey(dev.env_body) && !dev.spec_atlas)
+    if (prog.color == PT_COLOR_SPECTRAL && ((dev.env_type == PT_ENV_SKY && !pt_sky_body_grey(dev.env_body)) ||
+                                            dev.env_type == PT_ENV_IMAGE) && !dev.spec_atlas)
         return fail(PT_E_MISSING_INPUT, "missing input: spectral_atlas (a spectral program lifts the procedural sky's "
                                         "colours through the coefficient atlas in pt_scene_desc)");
-    if (prog.env_chart && dev.env_type == PT_ENV_SKY && prog.env_chart - 1 != dev.env_chart)
+    if (prog.env_chart && env_tabulated(dev.env_type) && prog.env_chart - 1 != dev.env_chart)
         return fail(PT_E_INVALID, std::string("the program's envSampler.chart is '") +
                                       (prog.env_chart - 1 == PT_CHART_OCTAHEDRAL ? "octahedral" : "equirect") +
                                       "' but the scene's sky table was built on the other chart (pt_env.chart)");
@@ -1275,7 +1326,7 @@ pt_status pt_batch_create(pt_ctx* ctx, int32_t n, const pt_scene_desc* scenes, c
             return fail(PT_E_UNSUPPORTED, "pt_batch_create: batches average their samples (accumulation 'average')");
         if (programs[s].env_chart < 0 || programs[s].env_chart > PT_CHART_OCTAHEDRAL + 1)
             return fail(PT_E_INVALID, "pt_batch_create: unknown env chart");
-        if (programs[s].env_chart && scenes[s].env.type == PT_ENV_SKY && programs[s].env_chart - 1 != scenes[s].env.chart)
+        if (programs[s].env_chart && env_tabulated(scenes[s].env.type) && programs[s].env_chart - 1 != scenes[s].env.chart)
             return fail(PT_E_INVALID, "pt_batch_create: a program's envSampler.chart differs from its scene's sky table chart");
     }
     const pt_program& p0 = programs[0];
@@ -1539,7 +1590,7 @@ pt_status pt_scene_update_env(pt_ctx* ctx, const pt_env* env) {
     const pt_env& a = ctx->host_env;
     if (env->type != a.type || env->samplable != a.samplable)
         return fail(PT_E_INVALID, "pt_scene_update_env: type and samplable must not change");
-    if (a.type == PT_ENV_SKY && (env->chart != a.chart || env->compensated != a.compensated || env->body != a.body ||
+    if (env_tabulated(a.type) && (env->chart != a.chart || env->compensated != a.compensated || env->body != a.body ||
                                  env->table_h != a.table_h || (a.chart != PT_CHART_OCTAHEDRAL && env->table_w != a.table_w)))
         return fail(PT_E_INVALID, "pt_scene_update_env: the sky's body and its table's chart, size and compensation must not change "
                                   "(pt_env_table_upload rebuilds the table)");
@@ -1562,8 +1613,8 @@ pt_status pt_env_table_upload(pt_ctx* ctx, const float* rgb, int32_t W, int32_t 
                               double* total_weight) {
     if (!ctx || !rgb) return fail(PT_E_INVALID, "pt_env_table_upload: null argument");
     if (!ctx->has_scene) return fail(PT_E_MISSING_INPUT, "missing input: no scene uploaded (pt_scene_upload)");
-    if (ctx->host_env.type != PT_ENV_SKY)
-        return fail(PT_E_INVALID, "pt_env_table_upload: the resident environment has no importance table (PT_ENV_SKY)");
+    if (!env_tabulated(ctx->host_env.type))
+        return fail(PT_E_INVALID, "pt_env_table_upload: the resident environment has no importance table (PT_ENV_SKY or PT_ENV_IMAGE)");
     if (chart != PT_CHART_EQUIRECT && chart != PT_CHART_OCTAHEDRAL)
         return fail(PT_E_INVALID, "pt_env_table_upload: chart must be PT_CHART_EQUIRECT or PT_CHART_OCTAHEDRAL");
     if (W < 1 || H < 1 || W > PT_ENV_MAX_W) return fail(PT_E_INVALID, "pt_env_table_upload: 1 <= width <= 4096 and height >= 1");
@@ -1976,7 +2027,7 @@ pt_status pt_debug_region_at(pt_ctx* ctx, const float* pts, int n, int32_t* out)
 pt_status pt_debug_env_sample(pt_ctx* ctx, const float* xi, int n, float* out) {
     if (!ctx || !xi || !out || n < 0) return fail(PT_E_INVALID, "pt_debug_env_sample: bad argument");
     if (!ctx->has_scene) return fail(PT_E_MISSING_INPUT, "missing input: no scene uploaded (pt_scene_upload)");
-    if (ctx->dev.env_type != PT_ENV_SKY && ctx->dev.env_type != PT_ENV_CONSTANT)
+    if (!env_tabulated(ctx->dev.env_type) && ctx->dev.env_type != PT_ENV_CONSTANT)
         return fail(PT_E_INVALID, "pt_debug_env_sample: the scene has no environment to sample");
     if (n == 0) return PT_OK;
     HIP_TRY(hipSetDevice(ctx->device));

@@ -780,6 +780,20 @@ template <bool SPEC>
 PT_DEV sp4 env_radiance(const DevScene& S, f3 dir, const Wl& wl) {
     if (S.env_type == PT_ENV_CONSTANT)
         return lift<SPEC>(S, &S.env_color.x, S.env_s_color, wl) * S.env_intensity;
+    if (S.env_type == PT_ENV_IMAGE) {
+        // texture(u_envMap, equirect_uv(dir)) with NEAREST filtering and clamped edges
+        const f3 n = normalize(dir);
+        const float phi = (n.x == 0.0f && n.z == 0.0f ? 0.0f : ptm_atan2(n.z, n.x)) + S.env_rotation;
+        const float theta = ptm_atan2(sqrtf(n.x * n.x + n.z * n.z), n.y);
+        const float u = ptm_fract(phi * (1.0f / PTM_TWO_PI) + 0.5f), v = theta * (1.0f / PTM_PI);
+        const int W = S.env_img_w, H = S.env_img_h;
+        int i = (int)(u * (float)W); i = i < 0 ? 0 : (i > W - 1 ? W - 1 : i);
+        int j = (int)(v * (float)H); j = j < 0 ? 0 : (j > H - 1 ? H - 1 : j);
+        const float* t = S.env_img + 3 * ((size_t)j * W + i);
+        const f3 c = mk3(t[0], t[1], t[2]);
+        if (!SPEC) return rgb4(c * S.env_intensity);
+        return lift_emission_rgb(S, c, wl) * S.env_intensity;
+    }
     if (S.env_type != PT_ENV_SKY) return sgrey(0.0f);
     f3 d = normalize(dir);
     float cx = S.env_rot_c, sy = S.env_rot_s;
@@ -899,7 +913,9 @@ PT_DEV float light_pdf_one(const pt_light& l, f3 p, f3 light_p, f3 wi) {
     f3 d = light_p - p;
     return dot(d, d) / (l.area * cos_l);
 }
-PT_DEV bool env_samplable(const DevScene& S) { return S.env_type == PT_ENV_SKY || (S.env_type == PT_ENV_CONSTANT && S.env_samplable); }
+PT_DEV bool env_samplable(const DevScene& S) {
+    return S.env_type == PT_ENV_SKY || S.env_type == PT_ENV_IMAGE || (S.env_type == PT_ENV_CONSTANT && S.env_samplable);
+}
 PT_DEV bool env_select_live(const DevScene& S) { return env_samplable(S) && S.n_lights > 0; }
 template <bool SPEC>
 PT_DEV LSample lighting_sample_finite(const DevScene& S, f3 p, float x0, float x1, const Wl& wl) {

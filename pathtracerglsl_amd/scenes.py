@@ -418,6 +418,34 @@ def sky_plane(glsl, scene_id, chart="equirect", compensated=False, table_size=(4
     }
 
 
+# spectral-image-constant's picture (tests/witnesses/scenes/spectralSkies.ts:18-23): 16 x 16 texels of
+# exactly this linear RGB (the RGBE bytes 32, 64, 128, 129)
+IMAGE_CONSTANT_RGB = (0.25, 0.5, 1.0)
+
+
+def sun_picture(w=64, h=32, seed=5):
+    """A synthetic equirect sky picture: a vertical gradient, noise, and a hot 3 x 2 sun patch."""
+    rng = np.random.default_rng(seed)
+    v = (np.arange(h) + 0.5) / h
+    pic = np.empty((h, w, 3), np.float32)
+    pic[:] = (0.1 + 0.5 * (1 - v))[:, None, None] * np.array([0.6, 0.8, 1.0], np.float32)
+    pic *= rng.uniform(0.8, 1.2, size=(h, w, 1)).astype(np.float32)
+    pic[h // 4: h // 4 + 2, (2 * w) // 3: (2 * w) // 3 + 3] = [60.0, 50.0, 35.0]
+    return pic
+
+
+def image_plane(image=None, intensity=2.0, rotation=0.73, chart="equirect", compensated=False, table_n=256):
+    """spectral-image-constant (spectralSkies.ts:53-66): the Lambert plane (albedo 0.5) of the
+    spectral-skies suite under an image environment, by default the constant picture of
+    IMAGE_CONSTANT_RGB at intensity 2 and rotation 0.73, so the plane's radiance is that RGB."""
+    if image is None:
+        image = np.broadcast_to(np.array(IMAGE_CONSTANT_RGB, np.float32), (16, 16, 3)).copy()
+    sc = sky_plane("vec3(1.0)", "spectral-image-constant", chart, compensated)
+    sc["environment"] = {"type": "image", "image": image, "intensity": intensity, "rotation": rotation,
+                         "chart": chart, "compensated": compensated, "table_size": [2 * table_n, table_n]}
+    return sc
+
+
 def sky_missed_cap(chart="equirect"):
     """sky-missed-cap (spectralSkies.ts:38-46): the sky is 1 inside the polar cap dir.y >= 0.9 and
     0 elsewhere, and a 4 x 2 table (2 x 2 octahedral) has no texel centre in the cap, so the table

@@ -55,6 +55,8 @@ def load(variant=""):
     lib.or_spectrum_eval.restype = None
     lib.or_env_radiance.argtypes = [vp, vp, vp, C.c_int, vp]
     lib.or_env_radiance.restype = None
+    lib.or_resample_equirect_to_octahedral.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp]
+    lib.or_resample_equirect_to_octahedral.restype = None
     lib.or_light_tree_build.argtypes = [vp, vp, C.c_int, vp, vp, C.POINTER(C.c_int)]
     lib.or_light_tree_build.restype = C.c_int
     lib.or_light_tree_importance.argtypes = [vp, vp, vp, vp, C.c_int]

@@ -31,7 +31,7 @@ def _expected_estimator(d):
 
 # The number of the 505 frozen link maps this backend lowers: a lowering that starts rejecting
 # a case (or accepting one it cannot render) changes it.
-EXPECTED_ACCEPTED = 382
+EXPECTED_ACCEPTED = 403
 
 
 def test_library_lowers_or_names_every_snapshot_case():
@@ -71,8 +71,7 @@ def test_library_lowers_or_names_every_snapshot_case():
     print(f"lowered {len(accepted)} of {len(accepted) + rejected}")
     assert len(accepted) + rejected == 505
     assert len(accepted) == EXPECTED_ACCEPTED
-    # frozen octahedral-chart cases on skies this backend renders (the others name image
-    # environments)
+    # frozen octahedral-chart cases on procedural skies and image environments
     assert {"spectral-image-constant-ref + oct", "sky-missed-cap + spectral-octahedral", "spectral-comp-empty + octahedral",
             "spectral-proc-sky + oct"} <= set(accepted)
     assert "two-light + uniform" in accepted                  # estimator.lighting.selection 'uniform'
