@@ -75,7 +75,12 @@ typedef struct pt_object {
  * rough_dielectric.glsl, Walter et al. 2007): GGX reflection and transmission with the exact
  * Fresnel, alpha = max(1e-3, roughness^2), the region's index and the transmittance tint. */
 enum { PT_MAT_LAMBERT = 0, PT_MAT_DIELECTRIC = 1, PT_MAT_GGX = 2, PT_MAT_NULL = 3, PT_MAT_MIRROR = 4,
-       PT_MAT_ROUGH_DIELECTRIC = 5 };
+       PT_MAT_ROUGH_DIELECTRIC = 5, PT_MAT_CHECKER = 6 };
+/* PT_MAT_CHECKER (materials/checker/checker.glsl): Lambert transport whose albedo is
+ * mix(albedo, f0, mod(floor(u s) + floor(v s), 2)) at the hit's surface coordinates (Hit.uv,
+ * glsl/core/structs.glsl:40-43): quad_uv (quad.glsl:33-41) on a quad, sphere_uv (sphere.glsl:84-89)
+ * on a sphere, the planar placeholder (x, z) * 0.1 on meshes and instances; s = pad1 (uv_scale).
+ * The record reuses albedo / s_albedo for albedo_a, f0 / s_f0 for albedo_b. */
 
 /* The arm a homogeneous medium is sampled by (src/compiler/generate/features/materials.ts:675-728,
  * routing predicates src/compiler/types.ts:351-417):

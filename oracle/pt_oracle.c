@@ -1165,16 +1165,46 @@ static int rough_half(v3 wil, v3 wol, float eta, v3* m) {
     return dot(h, wil) * wil.z >= 0.0f && dot(h, wol) * wol.z >= 0.0f;
 }
 
+/* Hit.uv (glsl/core/structs.glsl:40-43): quad_uv (quad.glsl:33-41), sphere_uv (sphere.glsl:84-89),
+ * else the planar placeholder (x, z) * 0.1 */
+static void hit_uv(const or_scene* S, const hit_t* h, float* u, float* v) {
+    const pt_object* o = &S->objects[h->region_owner];
+    if (o->kind == PT_OBJ_QUAD) {
+        v3 local = sub(h->p, V(o->p[0], o->p[1], o->p[2]));
+        v3 e1 = V(o->p[3], o->p[4], o->p[5]), e2 = V(o->p[6], o->p[7], o->p[8]);
+        float e11 = dot(e1, e1), e22 = dot(e2, e2), e12 = dot(e1, e2);
+        float d1 = dot(local, e1), d2 = dot(local, e2);
+        float det = e11 * e22 - e12 * e12;
+        *u = (d1 * e22 - d2 * e12) / det; *v = (d2 * e11 - d1 * e12) / det;
+    } else if (o->kind == PT_OBJ_SPHERE) {
+        v3 d = normalize(sub(h->p, V(o->p[0], o->p[1], o->p[2])));
+        *u = ATAN2(d.z, d.x) / PTM_TWO_PI + 0.5f;
+        *v = ptm_acos(clampf(d.y, -1.0f, 1.0f)) / PTM_PI;
+    } else {
+        *u = h->p.x * 0.1f; *v = h->p.z * 0.1f;
+    }
+}
+/* the Lambert lobe's albedo: the material's, or the checker cell's (checker.glsl:13-17) */
+static sp lambert_albedo(const or_scene* S, const pt_material* m, const hit_t* hit, const wl_t* wl) {
+    sp a = lift(S, m->albedo, &m->s_albedo, wl);
+    if (m->model != PT_MAT_CHECKER) return a;
+    float u, v;
+    hit_uv(S, hit, &u, &v);
+    float x = floorf(u * m->pad1) + floorf(v * m->pad1);
+    float k = x - 2.0f * floorf(x / 2.0f);
+    if (PLANT(13)) k = 1.0f - k;
+    return sadd(smuls(a, 1.0f - k), smuls(lift(S, m->f0, &m->s_f0, wl), k));
+}
 static isample_t surface_sample(const or_scene* S, const pt_material* m, v3 wo, const hit_t* hit, float uc, float ux, float uy, const wl_t* wl) {
     isample_t s;
     frame_t f = hit->frame;
-    if (m->model == PT_MAT_LAMBERT) {               /* materials/lambert/lambert.glsl:17-28 */
+    if (m->model == PT_MAT_LAMBERT || m->model == PT_MAT_CHECKER) {   /* materials/lambert/lambert.glsl:17-28 */
         v3 n = dot(wo, f.n) < 0.0f ? neg(f.n) : f.n;
         float cos_theta = sqrtf(uy);
         float sin_theta = sqrtf(fmaxf_(0.0f, 1.0f - uy));
         float phi = PTM_TWO_PI * ux;
         s.wi = normalize(add(add(muls(f.t, sin_theta * COS(phi)), muls(f.b, sin_theta * SIN(phi))), muls(n, cos_theta)));
-        s.weight = lift(S, m->albedo, &m->s_albedo, wl);
+        s.weight = lambert_albedo(S, m, hit, wl);
         s.pdf = cos_theta * INV_PI;
         s.flags = LOBE_REFLECTION;
     } else if (m->model == PT_MAT_DIELECTRIC) {     /* materials/dielectric/dielectric.glsl:14-52 */
@@ -1263,9 +1293,9 @@ static isample_t surface_sample(const or_scene* S, const pt_material* m, v3 wo, 
 }
 static sp surface_eval(const or_scene* S, const pt_material* m, v3 wi, v3 wo, const hit_t* hit, const wl_t* wl) {
     frame_t f = hit->frame;
-    if (m->model == PT_MAT_LAMBERT) {
+    if (m->model == PT_MAT_LAMBERT || m->model == PT_MAT_CHECKER) {
         if (dot(wi, f.n) * dot(wo, f.n) <= 0.0f) return sgrey(0.0f);
-        return smuls(lift(S, m->albedo, &m->s_albedo, wl), INV_PI);
+        return smuls(lambert_albedo(S, m, hit, wl), INV_PI);
     } else if (m->model == PT_MAT_GGX) {
         v3 wil = to_local(f, wi), wol = to_local(f, wo);
         if (wil.z * wol.z <= 0.0f) return sgrey(0.0f);
@@ -1295,7 +1325,7 @@ static sp surface_eval(const or_scene* S, const pt_material* m, v3 wi, v3 wo, co
 }
 static float surface_pdf(const or_scene* S, const pt_material* m, v3 wi, v3 wo, const hit_t* hit, const wl_t* wl) {
     frame_t f = hit->frame;
-    if (m->model == PT_MAT_LAMBERT) {
+    if (m->model == PT_MAT_LAMBERT || m->model == PT_MAT_CHECKER) {
         float c = dot(wi, f.n) * signf(dot(wo, f.n));
         return fmaxf_(0.0f, c) * INV_PI;
     } else if (m->model == PT_MAT_GGX) {
@@ -2252,6 +2282,12 @@ static sp transport_trace(const or_scene* S, const or_program* P, v3 ro, v3 rd, 
         if (aov && bounce == 0 && !wl->spec) {
             const float* row = M->model == PT_MAT_LAMBERT ? M->albedo
                              : (M->model == PT_MAT_GGX || M->model == PT_MAT_MIRROR) ? M->f0 : M->transmittance;
+            float cell[3];
+            if (M->model == PT_MAT_CHECKER) {   /* checker_albedo at the hit */
+                sp c = lambert_albedo(S, M, &hit, wl);
+                cell[0] = c.x; cell[1] = c.y; cell[2] = c.z;
+                row = cell;
+            }
             aov[0] = clampf(row[0], 0.0f, 1.0f) * T.x;
             aov[1] = clampf(row[1], 0.0f, 1.0f) * T.y;
             aov[2] = clampf(row[2], 0.0f, 1.0f) * T.z;

@@ -10,7 +10,7 @@ import numpy as np
 ABI_VERSION = 15
 
 OBJ_MESH, OBJ_QUAD, OBJ_SPHERE, OBJ_INSTANCES = 0, 1, 2, 3
-MAT_LAMBERT, MAT_DIELECTRIC, MAT_GGX, MAT_NULL, MAT_MIRROR, MAT_ROUGH_DIELECTRIC = 0, 1, 2, 3, 4, 5
+MAT_LAMBERT, MAT_DIELECTRIC, MAT_GGX, MAT_NULL, MAT_MIRROR, MAT_ROUGH_DIELECTRIC, MAT_CHECKER = 0, 1, 2, 3, 4, 5, 6
 MED_ANALYTIC, MED_ABSORB, MED_DELTA = 0, 1, 2
 VS_ANALYTIC, VS_DELTA_TRACKING = 0, 1
 IOR_CONSTANT, IOR_CAUCHY, IOR_SELLMEIER = 0, 1, 2

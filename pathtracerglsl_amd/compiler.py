@@ -17,9 +17,9 @@ from . import abi, spectral
 
 ESTIMATORS = {"none": abi.EST_PT, "pt": abi.EST_PT, "nee": abi.EST_NEE, "mis": abi.EST_MIS}
 MODELS = {"lambert": abi.MAT_LAMBERT, "dielectric": abi.MAT_DIELECTRIC, "ggx": abi.MAT_GGX, "none": abi.MAT_NULL,
-          "mirror": abi.MAT_MIRROR, "rough_dielectric": abi.MAT_ROUGH_DIELECTRIC}
+          "mirror": abi.MAT_MIRROR, "rough_dielectric": abi.MAT_ROUGH_DIELECTRIC, "checker": abi.MAT_CHECKER}
 # models whose emission term is identically zero (their descriptors' emissive capability is false)
-NON_EMISSIVE = ("mirror", "rough_dielectric")
+NON_EMISSIVE = ("mirror", "rough_dielectric", "checker")
 # accumulators (components/accumulator/index.ts registry): estimator.accumulation.type
 ACCUMULATIONS = {"average": abi.ACC_AVERAGE, "oneshot": abi.ACC_ONESHOT}
 # volume scattering models (components/volume_scattering/index.ts registry): medium.model
@@ -458,15 +458,20 @@ def compile_scene(scene, strategy=None, width=256, height=256, reset_salt=1234):
         m = scene["materials"][n]
         rec = np.zeros(1, abi.MATERIAL_DTYPE)[0]
         rec["model"] = MODELS[m["model"]]
-        albedo = m.get("albedo", [0.8, 0.8, 0.8])
+        albedo = m.get("albedo", [0.8, 0.8, 0.8]) if m["model"] != "checker" else m.get("albedo_a", 0.9)
         rec["albedo"][:3] = spectral.rgb_of(albedo)
         rec["s_albedo"] = srec(albedo, spectral.ROLE_REFLECTANCE)
         em = m.get("emission", [0.0, 0.0, 0.0]) if m["model"] not in NON_EMISSIVE else [0.0, 0.0, 0.0]
         rec["emission"][:3] = spectral.rgb_of(em)
         rec["s_emission"] = srec(em, spectral.ROLE_EMISSION)
         rec["emissive"] = int(emits(em))
-        f0 = m.get("f0", [0.9, 0.9, 0.9])
+        f0 = m.get("f0", [0.9, 0.9, 0.9]) if m["model"] != "checker" else m.get("albedo_b", 0.1)
         rec["f0"][:3] = spectral.rgb_of(f0)
+        if m["model"] == "checker":           # checker.ts: albedo_a 0.9, albedo_b 0.1, uv_scale 8 (positive)
+            scale = float(m.get("uv_scale", 8.0))
+            if not (scale > 0 and math.isfinite(scale)):
+                raise CompileError(f"material '{n}': uv_scale must be positive")
+            rec["pad1"] = scale
         rec["s_f0"] = srec(f0, spectral.ROLE_REFLECTANCE)
         # roughness defaults: ggx 0.5, rough_dielectric 0.2 (rough_dielectric.ts:17)
         rough = float(m.get("roughness", 0.2 if m["model"] == "rough_dielectric" else 0.5))

@@ -275,13 +275,18 @@ bool lower(const JVal& d, int32_t width, int32_t height, int32_t fif, pt_program
         if (const JVal* models = mats->get("models"); models && models->kind == JVal::ARR)
             for (const JVal& md : models->arr)
                 if (!(is_str(&md, "lambert") || is_str(&md, "dielectric") || is_str(&md, "ggx") || is_str(&md, "none") ||
-                      is_str(&md, "mirror") || is_str(&md, "rough_dielectric")))
+                      is_str(&md, "mirror") || is_str(&md, "rough_dielectric") || is_str(&md, "checker")))
                     return L.unsupported("material model", quoted(&md));
         // materials.twoSidedShading is read only by the light tree's horizon cull
         // (accel/light_tree/light_tree.glsl, through LightQuery.two_sided); the kernels answer
         // material_two_sided per material (the rough dielectric: sphere support, non-delta), which
         // is what the flag folds away for scenes without such a material.
-        if (truthy(mats->get("materialsReadUv"))) return L.unsupported("materials.materialsReadUv", "true (textures)");
+        // Hit.uv readers: the checker material reads the shape charts; any other reader is an image
+        // texture or an expression material
+        bool checker = false;
+        if (const JVal* models = mats->get("models"); models && models->kind == JVal::ARR)
+            for (const JVal& md : models->arr) checker = checker || is_str(&md, "checker");
+        if (truthy(mats->get("materialsReadUv")) && !checker) return L.unsupported("materials.materialsReadUv", "true (textures)");
         // materials.albedo marks the material_albedo dispatch an albedo AOV needs (plan/types.ts:246-248);
         // the albedo guide itself is lowered from measurement.aovs, so the flag adds nothing here.
         if (const JVal* cones = mats->get("emissionCones"); cones && cones->kind == JVal::ARR && !cones->arr.empty())

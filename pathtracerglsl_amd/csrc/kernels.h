@@ -504,7 +504,7 @@ PT_DEV void block_append_n(int* const* counters, const int* which, int* slot, in
 PT_DEV int surface_category(const DevScene& S, int prim) {
     if (prim < 0) return CAT_MISS;
     const int model = S.materials[S.objects[prim_owner(S, prim)].material].model;
-    return model == PT_MAT_LAMBERT ? CAT_LAMBERT
+    return model == PT_MAT_LAMBERT || model == PT_MAT_CHECKER ? CAT_LAMBERT
          : model == PT_MAT_GGX ? CAT_GGX
          : model == PT_MAT_DIELECTRIC ? CAT_DIELECTRIC
          : model == PT_MAT_MIRROR ? CAT_MIRROR
@@ -843,8 +843,14 @@ __global__ void __launch_bounds__(SBLOCK) k_shade(DevScene Sk, Paths P, int qsel
                 if (!SPEC && CAT != CAT_NULL && bounce == 0 && (P.aov[0] || P.aov[1])) {
                     const sp4 Ta = T;                            // the path's weight on arrival
                     if (P.aov[0]) {                                  // aov/albedo/albedo.glsl
+                        float cell[3];
                         const float* row = MODEL == PT_MAT_LAMBERT ? M.albedo
                                          : (MODEL == PT_MAT_GGX || MODEL == PT_MAT_MIRROR) ? M.f0 : M.transmittance;
+                        if (MODEL == PT_MAT_LAMBERT && M.model == PT_MAT_CHECKER) {   // checker_albedo at the hit
+                            const sp4 c = lambert_albedo<false>(S, M, h, wl);
+                            cell[0] = c.x; cell[1] = c.y; cell[2] = c.z;
+                            row = cell;
+                        }
                         P.aov[0][i] = make_float4(gclamp(row[0], 0.0f, 1.0f) * Ta.x, gclamp(row[1], 0.0f, 1.0f) * Ta.y,
                                                   gclamp(row[2], 0.0f, 1.0f) * Ta.z, 0.0f);
                     }

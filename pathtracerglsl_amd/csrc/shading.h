@@ -538,7 +538,10 @@ PT_DEV float dielectric_fresnel(float cos_i, float eta) {   // math.glsl:95-102
 }
 // MODEL < 0: dispatch on m.model at run time; otherwise the material is known to be MODEL
 // (category kernels shade one model per launch).
-template <int MODEL> PT_DEV bool is_model(const pt_material& m, int x) { return MODEL < 0 ? m.model == x : MODEL == x; }
+// (the checker is the Lambert lobe with a per-hit albedo)
+template <int MODEL> PT_DEV bool is_model(const pt_material& m, int x) {
+    return MODEL < 0 ? (m.model == x || (x == PT_MAT_LAMBERT && m.model == PT_MAT_CHECKER)) : MODEL == x;
+}
 
 // materials/rough_dielectric/rough_dielectric.glsl:37-46: the generalized half vector of a
 // (wi, wo) pair, faced up; false for a degenerate pair or a back-facing microfacet.
@@ -549,6 +552,36 @@ PT_DEV bool rough_half(f3 wil, f3 wol, float eta, f3& m) {
     m = normalize(m);
     if (m.z < 0.0f) m = -m;
     return dot(m, wil) * wil.z >= 0.0f && dot(m, wol) * wol.z >= 0.0f;
+}
+
+// Hit.uv (glsl/core/structs.glsl:40-43): the owning shape's chart, quad_uv (quad.glsl:33-41) or
+// sphere_uv (sphere.glsl:84-89), else the planar placeholder (x, z) * UV_PLANAR_SCALE (0.1).
+PT_DEV float2 hit_uv(const DevScene& S, const Hit& h) {
+    const pt_object& o = S.objects[h.owner];
+    if (o.kind == PT_OBJ_QUAD) {
+        const f3 local = h.p - mk3(o.p[0], o.p[1], o.p[2]);
+        const f3 e1 = mk3(o.p[3], o.p[4], o.p[5]), e2 = mk3(o.p[6], o.p[7], o.p[8]);
+        const float e11 = dot(e1, e1), e22 = dot(e2, e2), e12 = dot(e1, e2);
+        const float d1 = dot(local, e1), d2 = dot(local, e2);
+        const float det = e11 * e22 - e12 * e12;
+        return make_float2((d1 * e22 - d2 * e12) / det, (d2 * e11 - d1 * e12) / det);
+    }
+    if (o.kind == PT_OBJ_SPHERE) {
+        const f3 d = normalize(h.p - mk3(o.p[0], o.p[1], o.p[2]));
+        return make_float2(ptm_atan2(d.z, d.x) / PTM_TWO_PI + 0.5f, ptm_acos(gclamp(d.y, -1.0f, 1.0f)) / PTM_PI);
+    }
+    return make_float2(h.p.x * 0.1f, h.p.z * 0.1f);
+}
+// The Lambert lobe's albedo: the material's, or the checker's cell (checker.glsl:13-17:
+// mix(albedo_a, albedo_b, mod(floor(uv.x s) + floor(uv.y s), 2))).
+template <bool SPEC>
+PT_DEV sp4 lambert_albedo(const DevScene& S, const pt_material& m, const Hit& h, const Wl& wl) {
+    const sp4 a = lift<SPEC>(S, m.albedo, m.s_albedo, wl);
+    if (m.model != PT_MAT_CHECKER) return a;
+    const float2 uv = hit_uv(S, h);
+    const float x = floorf(uv.x * m.pad1) + floorf(uv.y * m.pad1);
+    const float k = x - 2.0f * floorf(x / 2.0f);
+    return a * (1.0f - k) + lift<SPEC>(S, m.f0, m.s_f0, wl) * k;
 }
 
 template <int MODEL, bool SPEC>
@@ -562,7 +595,7 @@ PT_DEV ISample surface_sample(const DevScene& S, const pt_material& m, f3 wo, co
         float phi = PTM_TWO_PI * ux;
         float sn, cs; ptm_sincos(phi, &sn, &cs);
         s.wi = normalize((f.t * (sin_theta * cs) + f.b * (sin_theta * sn)) + n * cos_theta);
-        s.weight = lift<SPEC>(S, m.albedo, m.s_albedo, wl);
+        s.weight = lambert_albedo<SPEC>(S, m, h, wl);
         s.pdf = cos_theta * INV_PI;
         s.flags = LOBE_REFLECTION;
     } else if (is_model<MODEL>(m, PT_MAT_DIELECTRIC)) {  // materials/dielectric/dielectric.glsl:14-52
@@ -656,7 +689,7 @@ PT_DEV sp4 surface_eval(const DevScene& S, const pt_material& m, f3 wi, f3 wo, c
     const Frame& f = h.frame;
     if (is_model<MODEL>(m, PT_MAT_LAMBERT)) {
         if (dot(wi, f.n) * dot(wo, f.n) <= 0.0f) return sgrey(0.0f);
-        return lift<SPEC>(S, m.albedo, m.s_albedo, wl) * INV_PI;
+        return lambert_albedo<SPEC>(S, m, h, wl) * INV_PI;
     } else if (is_model<MODEL>(m, PT_MAT_GGX)) {
         f3 wil = to_local(f, wi), wol = to_local(f, wo);
         if (wil.z * wol.z <= 0.0f) return sgrey(0.0f);

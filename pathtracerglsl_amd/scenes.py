@@ -446,6 +446,35 @@ def image_plane(image=None, intensity=2.0, rotation=0.73, chart="equirect", comp
     return sc
 
 
+def checker_plane(corner=(-0.5, 0.0, 0.5), size=3.0, uv_scale=3.0, albedo_a=(0.9, 0.6, 0.3), albedo_b=(0.1, 0.2, 0.4)):
+    """The spectral-skies plane as a size x size checker quad (materials/checker) under a white sky: in
+    pt with one bounce each pixel is the albedo of the cell its hit lies in, exactly. The default
+    corner puts every visible hit in cell (0, 0) (albedo_a); corner (-1.5, 0, 0.5) in cell (1, 0)
+    (albedo_b)."""
+    sc = sky_plane("vec3(1.0)", "checker-plane")
+    sc["materials"] = {"floor": {"model": "checker", "albedo_a": list(albedo_a), "albedo_b": list(albedo_b),
+                                 "uv_scale": uv_scale}}
+    sc["objects"] = [{"type": "quad", "corner": list(corner), "edge1": [size, 0.0, 0.0], "edge2": [0.0, 0.0, -size],
+                      "material": "floor"}]
+    sc["environment"] = {"type": "constant", "color": [1.0, 1.0, 1.0], "intensity": 1.0, "sample_as_light": True}
+    return sc
+
+
+def checker_room(detail=2):
+    """A checker floor quad, a checker sphere and a checker mesh (the planar placeholder chart) in
+    the Cornell box's light."""
+    sc = cornell_box()
+    sc["materials"]["check"] = {"model": "checker", "albedo_a": [0.8, 0.8, 0.2], "albedo_b": [0.1, 0.3, 0.7],
+                                "uv_scale": 6.0}
+    sc["objects"].append({"type": "sphere", "center": [-0.45, -0.7, -0.2], "radius": 0.3, "material": "check"})
+    v, f = unit_icosphere(detail)
+    tris = (v[f] * 0.25 + np.array([0.4, -0.7, 0.2])).reshape(-1, 9)
+    sc["objects"].append({"type": "mesh", "triangles": tris.tolist(), "material": "check"})
+    sc["objects"].append({"type": "quad", "corner": [-0.9, -0.99, 0.9], "edge1": [1.8, 0.0, 0.0], "edge2": [0.0, 0.0, -1.8],
+                          "material": "check"})
+    return sc
+
+
 def sky_missed_cap(chart="equirect"):
     """sky-missed-cap (spectralSkies.ts:38-46): the sky is 1 inside the polar cap dir.y >= 0.9 and
     0 elsewhere, and a 4 x 2 table (2 x 2 octahedral) has no texel centre in the cap, so the table
